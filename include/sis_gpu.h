@@ -1,0 +1,183 @@
+/*
+ * sis_gpu.h - C ABI of the MI355X engine for the SIS molecular-dynamics step.
+ *
+ * The reference SIS program (Fortran) has no plugin API; its hot path is a
+ * set of external subroutines that share module globals.  Each entry point
+ * below replaces one of them; the Fortran host binds them with iso_c_binding
+ * (sis_amd/fortran/sis_gpu_iface.F90).  All arrays are plain host pointers;
+ * the library copies what it needs and never keeps a host pointer.
+ *
+ * Conventions
+ *   - every call returns int: 0 = success, < 0 = error code below;
+ *     sis_gpu_last_error() gives the message of the last failure (the
+ *     Fortran side maps a failure to sis_abort, src/sis_abort.F90:1-20);
+ *   - replica indices irep are 1-based as in the reference (0 = all);
+ *   - coordinate arrays use the reference layout xyz(3, nmp, nrep),
+ *     Fortran column-major (src/var_state.F90:26-28);
+ *   - pair and base-pair lists use the reference layout wca_mp(2, n),
+ *     ele_mp(2, n), bp_mp(7, n) with 1-based bead ids
+ *     (src/var_potential.F90:86-134);
+ *   - one host thread drives a context; calls on one context do not overlap.
+ */
+#ifndef SIS_GPU_H
+#define SIS_GPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    SIS_OK = 0,
+    SIS_ERR_ARG = -1,         /* invalid argument or unsupported option   */
+    SIS_ERR_HIP = -2,         /* HIP runtime failure                      */
+    SIS_ERR_UNSUPPORTED = -3, /* feature the GPU backend does not cover   */
+    SIS_ERR_CAPACITY = -4,    /* a list outgrew its allocation            */
+    SIS_ERR_STATE = -5        /* call order (e.g. force before rebuild)   */
+};
+
+/* One base-pair type; field order of basepair_parameters
+ * (src/var_potential.F90:63-80).  cutoff_ddist is derived by the library
+ * as init_bp does (src/init_bp.F90:321-346). */
+typedef struct sis_bp_para {
+    double cutoff_ddist, U0, bond_k, bond_r;
+    double angl_k1, angl_k2, angl_k3, angl_k4;
+    double angl_theta1, angl_theta2, angl_theta3, angl_theta4;
+    double dihd_k1, dihd_k2, dihd_phi1, dihd_phi2;
+} sis_bp_para;
+
+/* Force field and run parameters: what read_input / read_force_field
+ * store in var_potential / var_state (src/read_force_field.F90:74-281,
+ * src/read_input.F90:280-790).  bp[] holds GC, AU, GU; CG, UA, UG are the
+ * mirrored types (angl_k1<->angl_k2, angl_k3<->angl_k4,
+ * src/read_force_field.F90:203-207). */
+typedef struct sis_params {
+    double bond_k, bond_r0;
+    double angl_k, angl_t0;
+    double dih_k, dih_w, dih_p0;
+    double wca_sigma, wca_eps;
+    double ele_cutoff, length_per_charge, ionic_strength;
+    double bp_cutoff_energy;
+    double bp_dH0, bp_dS0, bp_dHend0, bp_dSend0, bp_coef_dG;
+    double nn_dH[21], nn_dS[21];     /* NNT order, src/const_idx.F90:101-124 */
+    double nnend_dH[6], nnend_dS[6]; /* NNENDT order */
+    sis_bp_para bp[3];
+    double tempK;
+    double dt, viscosity_pas, nl_margin;
+    double box[3];
+    int32_t angl_reb;          /* 1: restricted bending (angle_ReB), 0: harmonic */
+    int32_t dih_type;          /* 0: none, 1: cosine, 2: exp (Gaussian) */
+    int32_t ele_on, ele_cutoff_type, ele_excl_bond, ele_excl_angl, dummy_has_charge;
+    int32_t bp_model, bp_min_loop, max_bp_per_nt, nstep_bp_mc, nnend;
+    int32_t pbc, pad;
+} sis_params;
+
+/* Topology: what read_fasta / init_sequence build (src/var_top.F90). */
+typedef struct sis_topo {
+    int32_t nchains, nmp;
+    const int32_t *nmp_chain;  /* [nchains] */
+    const int32_t *moltype;    /* [nchains] MOLT: 1 linear, 2 circular */
+    const int32_t *seq;        /* [nmp] SEQT codes A0 U1 G2 C3 D4, chain after chain */
+    const int32_t *no_charge;  /* [n_no_charge] 1-based beads without charge, or NULL */
+    int32_t n_no_charge, pad;
+} sis_topo;
+
+/* Engine options (no counterpart in the reference). */
+typedef struct sis_options {
+    int32_t nrep;            /* replicas (temperature ladder set by set_rep_conditions) */
+    int32_t precision;       /* 0: FP64 everywhere; 1: FP32 pair arithmetic, FP64 state */
+    int32_t parity;          /* 1: consume injected MT19937 words as the reference does */
+    int32_t mode;            /* 0 auto, 1 one launch per phase, 2 persistent, 3 batched */
+    int32_t block;           /* threads per workgroup, 0 = tuned default */
+    int32_t max_nbr;         /* neighbour slots per bead, 0 = grow on demand */
+    int32_t rebuild_every;   /* 0: displacement trigger (src/job_md.F90:376-395) */
+    int32_t exchange_every;  /* replica exchange cadence in steps, 0 = none */
+    uint64_t seed;           /* counter-RNG seed for the performance mode */
+} sis_options;
+
+typedef struct sis_gpu_ctx sis_gpu_ctx;
+
+/* Set up a system; derives what init_bp, init_ele, list_local and
+ * set_md_coef derive (src/init_bp.F90, src/set_ele.F90:45-99,
+ * src/list_local.F90, src/job_md.F90:94-142,700-767). */
+int sis_gpu_create(const sis_topo *topo, const sis_params *prm,
+                   const sis_options *opt, sis_gpu_ctx **ctx);
+int sis_gpu_destroy(sis_gpu_ctx *ctx);
+const char *sis_gpu_last_error(void);
+
+/* Copy bead state in / out; xyz, velos, accels are (3, nmp, nrep);
+ * velos / accels may be NULL. */
+int sis_gpu_set_state(sis_gpu_ctx *ctx, const double *xyz, const double *velos,
+                      const double *accels);
+int sis_gpu_get_state(sis_gpu_ctx *ctx, double *xyz, double *velos, double *accels);
+
+/* Per-replica temperature [nrep] and ionic strength [nrep] (NULL keeps the
+ * current value); re-derives set_ele and set_md_coef per replica
+ * (src/job_md.F90:536-567). */
+int sis_gpu_set_rep_conditions(sis_gpu_ctx *ctx, const double *tempK, const double *ionic);
+
+/* neighbor_list(irep): pbc_wrap + WCA / DH / BP lists
+ * (src/neighbor_list.F90:1-233, called from src/job_md.F90:197,389,584). */
+int sis_gpu_rebuild(sis_gpu_ctx *ctx, int32_t irep);
+
+/* force(irep, forces) (src/force.F90:1-77); bp_mc = flg_bp_MC.
+ * forces is (3, nmp) or NULL (keep on device). */
+int sis_gpu_force(sis_gpu_ctx *ctx, int32_t irep, int32_t bp_mc, double *forces);
+
+/* energy_sumup(irep, tempK, energies) (src/energy_sumup.F90:1-56);
+ * flags bit0 = flg_bp_MC, bit1 = flg_bp_energy (reuse the BP state of the
+ * last force call).  energies[11] in ENE order (src/const_idx.F90:21). */
+int sis_gpu_energy(sis_gpu_ctx *ctx, int32_t irep, double tempK, int32_t flags,
+                   double *energies);
+
+/* Parity dumps of the interaction lists.  kind 0 = wca_mp, 1 = ele_mp
+ * (2 ints per pair, i < j), 2 = bp_mp (7 ints per candidate, list order
+ * of src/neighbor_list.F90:99-114). */
+int sis_gpu_get_pair_count(sis_gpu_ctx *ctx, int32_t irep, int32_t kind, int64_t *n);
+int sis_gpu_get_pairs(sis_gpu_ctx *ctx, int32_t irep, int32_t kind, int32_t *out,
+                      int64_t cap, int64_t *n);
+/* BP candidates with bp_coef(2, n), bp_status and ene_bp of the last force
+ * or energy call (src/force_bp_limit_triplet.F90:243-250). */
+int sis_gpu_get_bp(sis_gpu_ctx *ctx, int32_t irep, int32_t *bp_mp, double *bp_coef,
+                   uint8_t *status, double *ene, int64_t cap, int64_t *n);
+
+/* Random numbers.  Performance mode: counter-based normals from `seed`.
+ * Parity mode: replica irep consumes `words` (tempered MT19937 output of
+ * its mts(irep) stream) exactly as rnd_boxmuller / genrand_double1 /
+ * genrand_double3 would (src/rnd_boxmuller.F90:1-49,
+ * mt_stream.F90:708-775); irep = -1 gives the exchange stream mts_rep. */
+int sis_gpu_set_rng(sis_gpu_ctx *ctx, uint64_t seed);
+int sis_gpu_set_rng_tape(sis_gpu_ctx *ctx, int32_t irep, const uint32_t *words, int64_t n);
+
+/* Maxwell-Boltzmann velocities and initial GJF noise terms
+ * (src/job_md.F90:157-176). */
+int sis_gpu_init_velocities(sis_gpu_ctx *ctx);
+
+/* nsteps of the job_md loop body for every replica: skin test and
+ * rebuild, force (with the BP limit MC every nstep_bp_mc steps), noise,
+ * GJF-2GJ update (src/job_md.F90:360-460), and replica exchange every
+ * exchange_every steps (src/job_md.F90:462-570).  *rebuilds (may be NULL)
+ * returns the number of neighbour-list rebuilds done. */
+int sis_gpu_step(sis_gpu_ctx *ctx, int64_t nsteps, int64_t *rebuilds);
+
+/* energy_replica + replica_exchange now (src/energy_replica.F90,
+ * src/replica_exchange.F90); replica_energies(2, nrep) out (may be NULL),
+ * accepted[nrep/2] out (may be NULL). */
+int sis_gpu_exchange(sis_gpu_ctx *ctx, double *replica_energies, int32_t *accepted);
+/* rep2lab(1:nrep) (src/var_replica.F90:49). */
+int sis_gpu_get_labels(sis_gpu_ctx *ctx, int32_t *rep2lab);
+
+/* Counters: out[0] nmp, [1] nrep, [2] istep, [3] rebuilds, [4] nwca,
+ * [5] nele, [6] nbp (replica 1), [7] neighbour slots per bead,
+ * [8] launch mode used, [9] workgroup size. */
+int sis_gpu_get_info(sis_gpu_ctx *ctx, int64_t *out, int32_t n);
+/* Device time (ms, HIP events on the context stream) of the last call of
+ * a kind: 0 = step, 1 = rebuild, 2 = force. */
+double sis_gpu_last_kernel_ms(sis_gpu_ctx *ctx, int32_t which);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SIS_GPU_H */

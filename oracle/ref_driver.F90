@@ -1,0 +1,513 @@
+! Test-only driver around the reference SIS Fortran (compiled from its own
+! sources by oracle/Makefile into oracle/_ref/).  It is part of the test
+! oracle: only tests and the CPU-baseline leg of bench.py load it.
+!
+! The reference reads its configuration with read_input/read_force_field,
+! which need the un-vendored toml-f library; those two files are not
+! compiled.  This driver instead fills the same module variables that
+! read_input.F90 and read_force_field.F90 would fill, then runs the
+! reference's own initialisation chain in the order of src/main.F90:28-140
+! (init_replica, init_mts, init_pbc, init_out_files, init_bp, init_ele,
+! list_local) and exposes the hot-path routines through bind(C) entry
+! points:
+!   neighbor_list  (src/neighbor_list.F90)
+!   force          (src/force.F90)
+!   energy_sumup   (src/energy_sumup.F90)
+!   rnd_boxmuller  (src/rnd_boxmuller.F90)
+!   job_md         (src/job_md.F90)
+! Nothing here re-implements physics.
+
+module sisref_driver
+
+   use, intrinsic :: iso_c_binding
+   implicit none
+
+   ! One base-pair parameter block, field order of basepair_parameters
+   ! (src/var_potential.F90:63-80).
+   type, bind(C) :: c_bp_para
+      real(c_double) :: cutoff_ddist, U0, bond_k, bond_r
+      real(c_double) :: angl_k1, angl_k2, angl_k3, angl_k4
+      real(c_double) :: angl_theta1, angl_theta2, angl_theta3, angl_theta4
+      real(c_double) :: dihd_k1, dihd_k2, dihd_phi1, dihd_phi2
+   end type c_bp_para
+
+   ! Same layout as struct sis_params in include/sis_gpu.h.
+   type, bind(C) :: c_params
+      real(c_double) :: bond_k, bond_r0
+      real(c_double) :: angl_k, angl_t0
+      real(c_double) :: dih_k, dih_w, dih_p0
+      real(c_double) :: wca_sigma, wca_eps
+      real(c_double) :: ele_cutoff, length_per_charge, ionic_strength
+      real(c_double) :: bp_cutoff_energy
+      real(c_double) :: bp_dH0, bp_dS0, bp_dHend0, bp_dSend0, bp_coef_dG
+      real(c_double) :: nn_dH(21), nn_dS(21)
+      real(c_double) :: nnend_dH(6), nnend_dS(6)
+      type(c_bp_para) :: bp(3)
+      real(c_double) :: tempK
+      real(c_double) :: dt, viscosity_pas, nl_margin
+      real(c_double) :: box(3)
+      integer(c_int32_t) :: angl_reb, dih_type
+      integer(c_int32_t) :: ele_on, ele_cutoff_type, ele_excl_bond, ele_excl_angl, dummy_has_charge
+      integer(c_int32_t) :: bp_model, bp_min_loop, max_bp_per_nt, nstep_bp_mc, nnend
+      integer(c_int32_t) :: pbc, pad
+   end type c_params
+
+contains
+
+   subroutine copy_bp(src, dst)
+      use var_potential, only : basepair_parameters
+      type(c_bp_para), intent(in) :: src
+      type(basepair_parameters), intent(out) :: dst
+      dst%cutoff_ddist = src%cutoff_ddist
+      dst%U0 = src%U0
+      dst%bond_k = src%bond_k
+      dst%bond_r = src%bond_r
+      dst%angl_k1 = src%angl_k1
+      dst%angl_k2 = src%angl_k2
+      dst%angl_k3 = src%angl_k3
+      dst%angl_k4 = src%angl_k4
+      dst%angl_theta1 = src%angl_theta1
+      dst%angl_theta2 = src%angl_theta2
+      dst%angl_theta3 = src%angl_theta3
+      dst%angl_theta4 = src%angl_theta4
+      dst%dihd_k1 = src%dihd_k1
+      dst%dihd_k2 = src%dihd_k2
+      dst%dihd_phi1 = src%dihd_phi1
+      dst%dihd_phi2 = src%dihd_phi2
+   end subroutine copy_bp
+
+   ! Mirror of the CG/UA/UG derivation done by read_force_field
+   ! (src/read_force_field.F90:203-207 and the AU/GU analogues): the
+   ! mirrored type swaps angl_k1<->angl_k2 and angl_k3<->angl_k4.
+   subroutine mirror_bp(src, dst)
+      use var_potential, only : basepair_parameters
+      type(basepair_parameters), intent(in) :: src
+      type(basepair_parameters), intent(out) :: dst
+      dst = src
+      dst%angl_k1 = src%angl_k2
+      dst%angl_k2 = src%angl_k1
+      dst%angl_k3 = src%angl_k4
+      dst%angl_k4 = src%angl_k3
+   end subroutine mirror_bp
+
+   ! --------------------------------------------------------------------
+   ! Set up a system.  seq_flat holds SEQT codes chain after chain;
+   ! xyz is (3, nmp, nrep) column-major.  rep_temps(nrep) gives the
+   ! temperature ladder when nrep > 1 (temperature REMD), else ignored.
+   ! prefix is a NUL-terminated output prefix for the .out/.dcd files
+   ! that the reference's job_md writes.
+   ! --------------------------------------------------------------------
+   subroutine sisref_setup(p, nchains_in, nmp_chain_in, moltype_in, seq_flat, nrep_in, &
+                           rep_temps, rng_seed_in, xyz_in, prefix, nstep_rep_exchange_in) &
+                           bind(C, name="sisref_setup")
+      use const
+      use const_phys, only : BOLTZ_KCAL_MOL
+      use const_idx, only : JOBT, INTGRT, REPT, MOLT
+      use var_parallel, only : init_parallel
+      use var_state
+      use var_top
+      use var_potential
+      use var_io
+      use var_replica
+      use pbc, only : flg_pbc, pbc_box_input, flg_pbc_resize, flg_pbc_ignore_rst, init_pbc
+
+      type(c_params), intent(in) :: p
+      integer(c_int), value :: nchains_in, nrep_in, rng_seed_in
+      integer(c_int), intent(in) :: nmp_chain_in(nchains_in), moltype_in(nchains_in)
+      integer(c_int), intent(in) :: seq_flat(*)
+      real(c_double), intent(in) :: rep_temps(*)
+      real(c_double), intent(in) :: xyz_in(*)
+      character(kind=c_char), intent(in) :: prefix(*)
+      integer(c_int64_t), value :: nstep_rep_exchange_in
+
+      integer :: ichain, i, imp, maxnmp, n, irep
+      character(len=:), allocatable :: cpre
+
+      call init_const()
+      call init_parallel()
+
+      ! ---- what read_input.F90 would set ([Job], [Condition], [MD]) ----
+      job = JOBT%MD
+      restarted = .False.
+      reset_step = .False.
+      tempK = p%tempK
+      kT = BOLTZ_KCAL_MOL * tempK
+      rng_seed = rng_seed_in
+      integrator = INTGRT%LD_GJF2GJ
+      viscosity_Pas = p%viscosity_pas
+      dt = p%dt
+      nstep = 0
+      nstep_save = huge(nstep_save)
+      nstep_save_rst = huge(nstep_save_rst)
+      nstep_check_stop = huge(nstep_check_stop)
+      stop_wall_time_sec = 0
+      fix_com_origin = 0
+      nl_margin = p%nl_margin
+      opt_anneal = 0
+      temp_independent = 0
+      ionic_strength = p%ionic_strength
+      length_per_charge = p%length_per_charge
+      nstep_bp_MC = p%nstep_bp_mc
+      flg_progress = .False.
+      flg_freeze = .False.
+      dummy_has_charge = (p%dummy_has_charge /= 0)
+
+      flg_in_ct = .False.
+      flg_in_bpseq = .False.
+      flg_in_bpl = .False.
+      flg_in_fasta = .True.
+      flg_out_bp = .False.
+      flg_out_bpall = .False.
+      flg_out_bpe = .False.
+      flg_out_bpcoef = .False.
+
+      n = 0
+      do while (prefix(n+1) /= c_null_char)
+         n = n + 1
+      enddo
+      allocate(character(len=n) :: cpre)
+      do i = 1, n
+         cpre(i:i) = prefix(i)
+      enddo
+      cfile_prefix = cpre
+
+      ! ---- [Replica] ----
+      flg_repvar(:) = .False.
+      nrep(:) = 1
+      if (nrep_in > 1) then
+         flg_replica = .True.
+         flg_exchange = .True.
+         flg_repvar(REPT%TEMP) = .True.
+         nrep(REPT%TEMP) = nrep_in
+         replica_values(:,:) = 0.0_PREC
+         do irep = 1, nrep_in
+            replica_values(irep, REPT%TEMP) = rep_temps(irep)
+         enddo
+         nstep_rep_exchange = nstep_rep_exchange_in
+         nstep_rep_save = nstep_rep_exchange_in
+      else
+         flg_replica = .False.
+         flg_exchange = .False.
+      endif
+
+      ! ---- [PBC_box] ----
+      flg_pbc = (p%pbc /= 0)
+      pbc_box_input(:) = p%box(:)
+      flg_pbc_resize = .False.
+      flg_pbc_ignore_rst = .True.
+
+      ! ---- [Basepair], [Electrostatic] sections of read_input ----
+      bp_model = p%bp_model
+      bp_min_loop = p%bp_min_loop
+      max_bp_per_nt = p%max_bp_per_nt
+      flg_ele = (p%ele_on /= 0)
+      ele_cutoff_type = p%ele_cutoff_type
+      ele_cutoff_inp = p%ele_cutoff
+      ele_exclude_covalent_bond_pairs = (p%ele_excl_bond /= 0)
+      ele_exclude_covalent_angle_pairs = (p%ele_excl_angl /= 0)
+      flg_stage = .False.
+      flg_twz = .False.
+      flg_bias_ss = .False.
+      flg_bias_rg = .False.
+      flg_timed_bias_rg = .False.
+      flg_restraint = .False.
+      ntwz_DCF = 0
+      ntwz_FR = 0
+
+      call init_replica()
+      call init_mts()
+
+      ! ---- what read_force_field.F90 would set ----
+      bond_k = p%bond_k
+      bond_r0 = p%bond_r0
+      flg_angl_ReB = (p%angl_reb /= 0)
+      angl_k = p%angl_k
+      angl_t0 = p%angl_t0
+      flg_dih_cos = (p%dih_type == 1)
+      flg_dih_exp = (p%dih_type == 2)
+      dih_k = p%dih_k
+      dih_w = p%dih_w
+      dih_p0 = p%dih_p0
+      wca_sigma = p%wca_sigma
+      wca_eps = p%wca_eps
+      bp_cutoff_energy = p%bp_cutoff_energy
+      coef_dG = p%bp_coef_dG
+      dH0 = p%bp_dH0
+      dS0 = p%bp_dS0
+      dHend0 = p%bp_dHend0
+      dSend0 = p%bp_dSend0
+      flg_NNend = (p%nnend /= 0)
+      allocate(NN_dH(21), NN_dS(21), NNend_dH(6), NNend_dS(6))
+      NN_dH(:) = p%nn_dH(:)
+      NN_dS(:) = p%nn_dS(:)
+      NNend_dH(:) = p%nnend_dH(:)
+      NNend_dS(:) = p%nnend_dS(:)
+      call copy_bp(p%bp(1), bp_paras(1))   ! GC
+      call copy_bp(p%bp(2), bp_paras(2))   ! AU
+      call copy_bp(p%bp(3), bp_paras(3))   ! GU
+      call mirror_bp(bp_paras(1), bp_paras(4))   ! CG
+      call mirror_bp(bp_paras(2), bp_paras(5))   ! UA
+      call mirror_bp(bp_paras(3), bp_paras(6))   ! UG
+
+      if (flg_pbc) call init_pbc(restarted)
+
+      call init_out_files()
+
+      ! ---- what read_fasta.F90 would set (init_sequence) ----
+      nchains = nchains_in
+      allocate(nmp_chain(nchains), moltypes(nchains))
+      nmp_chain(:) = nmp_chain_in(:)
+      moltypes(:) = moltype_in(:)
+      maxnmp = maxval(nmp_chain)
+      nmp = sum(nmp_chain)
+      allocate(seq(maxnmp, nchains), imp_chain(maxnmp, nchains), ichain_mp(nmp), lmp_mp(nmp))
+      seq(:,:) = -1
+      imp_chain(:,:) = 0
+      imp = 0
+      do ichain = 1, nchains
+         do i = 1, nmp_chain(ichain)
+            imp = imp + 1
+            seq(i, ichain) = seq_flat(imp)
+            imp_chain(i, ichain) = imp
+            ichain_mp(imp) = ichain
+            lmp_mp(imp) = i
+         enddo
+      enddo
+
+      allocate(xyz(3, nmp, nrep_proc))
+      do irep = 1, nrep_proc
+         do imp = 1, nmp
+            do i = 1, 3
+               xyz(i, imp, irep) = xyz_in(i + 3*(imp-1) + 3*nmp*(irep-1))
+            enddo
+         enddo
+      enddo
+
+      call init_bp()
+      if (flg_ele) call init_ele()
+      call list_local()
+
+   end subroutine sisref_setup
+
+   subroutine sisref_set_xyz(xyz_in) bind(C, name="sisref_set_xyz")
+      use var_state, only : xyz
+      use var_top, only : nmp
+      use var_replica, only : nrep_proc
+      real(c_double), intent(in) :: xyz_in(3, nmp, nrep_proc)
+      xyz(:,:,:) = xyz_in(:,:,:)
+   end subroutine sisref_set_xyz
+
+   subroutine sisref_get_xyz(xyz_out) bind(C, name="sisref_get_xyz")
+      use var_state, only : xyz
+      use var_top, only : nmp
+      use var_replica, only : nrep_proc
+      real(c_double), intent(out) :: xyz_out(3, nmp, nrep_proc)
+      xyz_out(:,:,:) = xyz(:,:,:)
+   end subroutine sisref_get_xyz
+
+   subroutine sisref_get_state(velos_out, accels_out) bind(C, name="sisref_get_state")
+      use var_state, only : velos, accels
+      use var_top, only : nmp
+      use var_replica, only : nrep_proc
+      real(c_double), intent(out) :: velos_out(3, nmp, nrep_proc), accels_out(3, nmp, nrep_proc)
+      velos_out(:,:,:) = velos(:,:,:)
+      accels_out(:,:,:) = accels(:,:,:)
+   end subroutine sisref_get_state
+
+   subroutine sisref_neighbor_list(irep) bind(C, name="sisref_neighbor_list")
+      integer(c_int), value :: irep
+      call neighbor_list(irep)
+   end subroutine sisref_neighbor_list
+
+   subroutine sisref_counts(irep, counts) bind(C, name="sisref_counts")
+      use var_potential, only : nwca, nele, nbp, flg_ele
+      integer(c_int), value :: irep
+      integer(c_int), intent(out) :: counts(3)
+      counts(1) = nwca(irep)
+      counts(2) = 0
+      if (flg_ele) counts(2) = nele(irep)
+      counts(3) = nbp(irep)
+   end subroutine sisref_counts
+
+   ! wca(2,nwca), ele(2,nele), bp(7,nbp), bpcoef(2,nbp), 1-based bead ids
+   subroutine sisref_lists(irep, wca, ele, bp, bpcoef) bind(C, name="sisref_lists")
+      use var_potential, only : nwca, nele, nbp, wca_mp, ele_mp, bp_mp, bp_coef, flg_ele
+      integer(c_int), value :: irep
+      integer(c_int), intent(out) :: wca(2,*), ele(2,*), bp(7,*)
+      real(c_double), intent(out) :: bpcoef(2,*)
+      integer :: k
+      do k = 1, nwca(irep)
+         wca(:,k) = wca_mp(:,k,irep)
+      enddo
+      if (flg_ele) then
+         do k = 1, nele(irep)
+            ele(:,k) = ele_mp(:,k,irep)
+         enddo
+      endif
+      do k = 1, nbp(irep)
+         bp(:,k) = bp_mp(:,k,irep)
+         bpcoef(:,k) = bp_coef(:,k,irep)
+      enddo
+   end subroutine sisref_lists
+
+   ! The BP MC flag is a module variable set per step by job_md
+   ! (src/job_md.F90:364-367); here the caller chooses it.
+   subroutine sisref_force(irep, bp_mc, forces) bind(C, name="sisref_force")
+      use var_state, only : flg_bp_MC, bp_status_MC, nstep_bp_MC
+      use var_top, only : nmp
+      integer(c_int), value :: irep, bp_mc
+      real(c_double), intent(out) :: forces(3, nmp)
+      flg_bp_MC = (bp_mc /= 0)
+      call force(irep, forces)
+   end subroutine sisref_force
+
+   subroutine sisref_set_bp_status_mc(irep, st) bind(C, name="sisref_set_bp_status_mc")
+      use var_state, only : bp_status_MC
+      use var_potential, only : nbp
+      integer(c_int), value :: irep
+      integer(c_int8_t), intent(in) :: st(*)
+      integer :: k
+      do k = 1, nbp(irep)
+         bp_status_MC(k, irep) = (st(k) /= 0)
+      enddo
+   end subroutine sisref_set_bp_status_mc
+
+   subroutine sisref_bp_state(irep, st, ene) bind(C, name="sisref_bp_state")
+      use var_state, only : bp_status, ene_bp
+      use var_potential, only : nbp
+      integer(c_int), value :: irep
+      integer(c_int8_t), intent(out) :: st(*)
+      real(c_double), intent(out) :: ene(*)
+      integer :: k
+      do k = 1, nbp(irep)
+         st(k) = 0
+         if (bp_status(k, irep)) st(k) = 1
+         ene(k) = ene_bp(k, irep)
+      enddo
+   end subroutine sisref_bp_state
+
+   ! energies(0:10) as ENE indices (src/const_idx.F90:21)
+   subroutine sisref_energy(irep, tempk_in, bp_mc, bp_energy_valid, energies) bind(C, name="sisref_energy")
+      use var_state, only : flg_bp_MC, flg_bp_energy
+      integer(c_int), value :: irep, bp_mc, bp_energy_valid
+      real(c_double), value :: tempk_in
+      real(c_double), intent(out) :: energies(0:10)
+      interface
+         subroutine energy_sumup(irep, tempK_in, energies)
+            use const, only : PREC
+            use const_idx, only : ENE
+            integer,    intent(in)  :: irep
+            real(PREC), intent(in)  :: tempK_in
+            real(PREC), intent(out) :: energies(0:ENE%MAX)
+         end subroutine energy_sumup
+      end interface
+      flg_bp_MC = (bp_mc /= 0)
+      flg_bp_energy = (bp_energy_valid /= 0)
+      call energy_sumup(irep, tempk_in, energies)
+   end subroutine sisref_energy
+
+   subroutine sisref_ele_params(irep, out) bind(C, name="sisref_ele_params")
+      use var_state, only : lambdaD, diele
+      use var_potential, only : ele_coef, ele_cutoff
+      integer(c_int), value :: irep
+      real(c_double), intent(out) :: out(4)
+      out(1) = lambdaD(irep)
+      out(2) = ele_coef(irep)
+      out(3) = ele_cutoff(irep)
+      out(4) = diele(irep)
+   end subroutine sisref_ele_params
+
+   ! bp3 triplet table (src/init_bp.F90:39-172) and the BP cutoffs
+   subroutine sisref_bp_tables(dh, ds, cut) bind(C, name="sisref_bp_tables")
+      use var_potential, only : bp3_dH, bp3_dS, bp_cutoff_dist, bp_paras
+      real(c_double), intent(out) :: dh(468), ds(468), cut(7)
+      integer :: k
+      dh(:) = bp3_dH(:)
+      ds(:) = bp3_dS(:)
+      cut(1) = bp_cutoff_dist
+      do k = 1, 6
+         cut(k+1) = bp_paras(k)%cutoff_ddist
+      enddo
+   end subroutine sisref_bp_tables
+
+   ! dense bp_map / bp3_map (src/init_bp.F90:28-29), (nmp, nmp) column-major
+   subroutine sisref_bp_maps(bpm, bp3m) bind(C, name="sisref_bp_maps")
+      use var_potential, only : bp_map, bp3_map
+      use var_top, only : nmp
+      integer(c_int), intent(out) :: bpm(nmp, nmp), bp3m(nmp, nmp)
+      bpm(:,:) = bp_map(:,:)
+      bp3m(:,:) = bp3_map(:,:)
+   end subroutine sisref_bp_maps
+
+   ! n raw tempered MT19937 words from a deep copy of mts(irep)
+   ! (irep = -1 selects mts_rep); the stream itself does not advance.
+   subroutine sisref_mt_tape(irep, n, words) bind(C, name="sisref_mt_tape")
+      use mt_stream, only : mt_state, genrand_int32
+      use var_state, only : mts, mts_rep
+      integer(c_int), value :: irep
+      integer(c_int64_t), value :: n
+      integer(c_int32_t), intent(out) :: words(n)
+      type(mt_state) :: tmp
+      integer(c_int64_t) :: k
+      if (irep < 0) then
+         tmp = mts_rep
+         allocate(tmp%state(0:tmp%nn-1))
+         tmp%state(:) = mts_rep%state(:)
+      else
+         tmp = mts(irep)
+         allocate(tmp%state(0:tmp%nn-1))
+         tmp%state(:) = mts(irep)%state(:)
+      endif
+      do k = 1, n
+         words(k) = genrand_int32(tmp)
+      enddo
+      deallocate(tmp%state)
+   end subroutine sisref_mt_tape
+
+   subroutine sisref_genrand(irep, kind, n, out) bind(C, name="sisref_genrand")
+      use mt_stream, only : genrand_double1, genrand_double3
+      use var_state, only : mts
+      integer(c_int), value :: irep, kind
+      integer(c_int64_t), value :: n
+      real(c_double), intent(out) :: out(n)
+      integer(c_int64_t) :: k
+      do k = 1, n
+         if (kind == 1) then
+            out(k) = genrand_double1(mts(irep))
+         else
+            out(k) = genrand_double3(mts(irep))
+         endif
+      enddo
+   end subroutine sisref_genrand
+
+   subroutine sisref_boxmuller(irep, n, out) bind(C, name="sisref_boxmuller")
+      integer(c_int), value :: irep
+      integer(c_int64_t), value :: n
+      real(c_double), intent(out) :: out(n)
+      real(c_double) :: rnd_boxmuller
+      integer(c_int64_t) :: k
+      do k = 1, n
+         out(k) = rnd_boxmuller(irep)
+      enddo
+   end subroutine sisref_boxmuller
+
+   ! Run the reference MD loop for nstep steps (single use per process:
+   ! job_md allocates its arrays and keeps first-call state).
+   subroutine sisref_job_md(nstep_in, energies_out) bind(C, name="sisref_job_md")
+      use var_state, only : nstep, nstep_save, energies
+      use var_replica, only : nrep_proc
+      integer(c_int64_t), value :: nstep_in
+      real(c_double), intent(out) :: energies_out(0:10, *)
+      nstep = nstep_in
+      nstep_save = max(nstep_in, 1_c_int64_t)
+      call job_md()
+      energies_out(:, 1:nrep_proc) = energies(:, 1:nrep_proc)
+   end subroutine sisref_job_md
+
+   subroutine sisref_replica_labels(rep2lab_out) bind(C, name="sisref_replica_labels")
+      use var_replica, only : rep2lab, nrep_all
+      integer(c_int), intent(out) :: rep2lab_out(*)
+      rep2lab_out(1:nrep_all) = rep2lab(1:nrep_all)
+   end subroutine sisref_replica_labels
+
+end module sisref_driver

@@ -1,0 +1,193 @@
+"""Python access to the reference SIS Fortran built in oracle/_ref/ (test
+oracle only: imported by tests/ and by bench.py's CPU-baseline leg).
+
+The reference keeps its state in module globals and job_md is single-use
+per process, so every call here runs in a fresh spawned process.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import multiprocessing as mp
+import os
+import tempfile
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+REF_SO = HERE / "_ref" / "libsisref.so"
+REF_OMP_SO = HERE / "_ref" / "libsisref_omp.so"
+
+
+def available(omp: bool = False) -> bool:
+    return (REF_OMP_SO if omp else REF_SO).exists()
+
+
+def _dp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _ip(a):
+    return a.ctypes.data_as(C.POINTER(C.c_int32))
+
+
+def _setup(lib, sysd, prefix, seed, nstep_exchange=0):
+    import sys
+    sys.path.insert(0, str(HERE.parent))
+    from sis_amd.abi import params_to_struct
+    p = params_to_struct(sysd["params"])
+    nmp_chain = np.ascontiguousarray(sysd["nmp_chain"], np.int32)
+    molt = np.ascontiguousarray(sysd["moltype"], np.int32)
+    seq = np.ascontiguousarray(sysd["seq"], np.int32)
+    temps = np.ascontiguousarray(sysd["temps"], np.float64)
+    xyz = np.ascontiguousarray(sysd["xyz"], np.float64)
+    lib.sisref_setup(C.byref(p), C.c_int(int(sysd["nchains"])), _ip(nmp_chain), _ip(molt),
+                     _ip(seq), C.c_int(int(sysd["nrep"])), _dp(temps), C.c_int(int(seed)),
+                     _dp(xyz), prefix.encode() + b"\0", C.c_int64(int(nstep_exchange)))
+
+
+def _worker(q, so, sysd, ops, seed, nstep_exchange):
+    try:
+        # quiet the reference's stdout banner
+        devnull = os.open(os.devnull, os.O_WRONLY)
+        os.dup2(devnull, 1)
+        lib = C.CDLL(str(so))
+        tmp = tempfile.mkdtemp(prefix="sisref_")
+        _setup(lib, sysd, os.path.join(tmp, "ref"), seed, nstep_exchange)
+        nmp = int(np.sum(sysd["nmp_chain"]))
+        nrep = int(sysd["nrep"])
+        out = {}
+        for op in ops:
+            kind = op[0]
+            if kind == "nl":
+                lib.sisref_neighbor_list(C.c_int(op[1]))
+            elif kind == "lists":
+                irep = op[1]
+                cnt = np.zeros(3, np.int32)
+                lib.sisref_counts(C.c_int(irep), _ip(cnt))
+                wca = np.zeros((max(cnt[0], 1), 2), np.int32)
+                ele = np.zeros((max(cnt[1], 1), 2), np.int32)
+                bp = np.zeros((max(cnt[2], 1), 7), np.int32)
+                bpc = np.zeros((max(cnt[2], 1), 2), np.float64)
+                lib.sisref_lists(C.c_int(irep), _ip(wca), _ip(ele), _ip(bp), _dp(bpc))
+                out[op[-1]] = dict(wca=wca[: cnt[0]].copy(), ele=ele[: cnt[1]].copy(),
+                                   bp=bp[: cnt[2]].copy(), bpcoef=bpc[: cnt[2]].copy())
+            elif kind == "force":
+                f = np.zeros((nmp, 3), np.float64)
+                lib.sisref_force(C.c_int(op[1]), C.c_int(op[2]), _dp(f))
+                out[op[-1]] = f
+            elif kind == "bp_state":
+                cnt = np.zeros(3, np.int32)
+                lib.sisref_counts(C.c_int(op[1]), _ip(cnt))
+                st = np.zeros(max(cnt[2], 1), np.int8)
+                en = np.zeros(max(cnt[2], 1), np.float64)
+                lib.sisref_bp_state(C.c_int(op[1]), st.ctypes.data_as(C.POINTER(C.c_int8)), _dp(en))
+                out[op[-1]] = (st[: cnt[2]].copy(), en[: cnt[2]].copy())
+            elif kind == "energy":
+                e = np.zeros(11, np.float64)
+                lib.sisref_energy(C.c_int(op[1]), C.c_double(op[2]), C.c_int(op[3]),
+                                  C.c_int(op[4]), _dp(e))
+                out[op[-1]] = e
+            elif kind == "ele":
+                e = np.zeros(4, np.float64)
+                lib.sisref_ele_params(C.c_int(op[1]), _dp(e))
+                out[op[-1]] = e
+            elif kind == "bp_tables":
+                dh = np.zeros(468)
+                ds = np.zeros(468)
+                cut = np.zeros(7)
+                lib.sisref_bp_tables(_dp(dh), _dp(ds), _dp(cut))
+                out[op[-1]] = (dh, ds, cut)
+            elif kind == "bp_maps":
+                a = np.zeros((nmp, nmp), np.int32)
+                b = np.zeros((nmp, nmp), np.int32)
+                lib.sisref_bp_maps(_ip(a), _ip(b))
+                out[op[-1]] = (a.T.copy(), b.T.copy())   # -> [imp-1, jmp-1]
+            elif kind == "tape":
+                n = int(op[2])
+                w = np.zeros(n, np.uint32)
+                lib.sisref_mt_tape(C.c_int(op[1]), C.c_int64(n),
+                                   w.ctypes.data_as(C.POINTER(C.c_int32)))
+                out[op[-1]] = w
+            elif kind == "genrand":
+                n = int(op[3])
+                r = np.zeros(n)
+                lib.sisref_genrand(C.c_int(op[1]), C.c_int(op[2]), C.c_int64(n), _dp(r))
+                out[op[-1]] = r
+            elif kind == "boxmuller":
+                n = int(op[2])
+                r = np.zeros(n)
+                lib.sisref_boxmuller(C.c_int(op[1]), C.c_int64(n), _dp(r))
+                out[op[-1]] = r
+            elif kind == "set_xyz":
+                x = np.ascontiguousarray(op[1], np.float64)
+                lib.sisref_set_xyz(_dp(x))
+            elif kind == "get_xyz":
+                x = np.zeros((nrep, nmp, 3))
+                lib.sisref_get_xyz(_dp(x))
+                out[op[-1]] = x
+            elif kind == "job_md":
+                e = np.zeros((nrep, 11))
+                lib.sisref_job_md(C.c_int64(int(op[1])), _dp(e))
+                x = np.zeros((nrep, nmp, 3))
+                v = np.zeros((nrep, nmp, 3))
+                a = np.zeros((nrep, nmp, 3))
+                lib.sisref_get_xyz(_dp(x))
+                lib.sisref_get_state(_dp(v), _dp(a))
+                lab = np.zeros(max(nrep, 1), np.int32)
+                lib.sisref_replica_labels(_ip(lab))
+                out[op[-1]] = dict(xyz=x, velos=v, accels=a, energies=e, rep2lab=lab)
+            else:
+                raise ValueError(kind)
+        q.put(("ok", out))
+    except BaseException as exc:  # report to the parent
+        import traceback
+        q.put(("err", traceback.format_exc() + repr(exc)))
+
+
+def run(sysd: dict, ops: list, seed: int = 12345, omp: bool = False, nstep_exchange: int = 0,
+        timeout: float = 600.0, threads: int = 1) -> dict:
+    """Run `ops` against a fresh reference process set up with `sysd`.
+
+    ops entries (last element is the result key where one is produced):
+      ("nl", irep) ; ("lists", irep, key) ; ("force", irep, bp_mc, key)
+      ("bp_state", irep, key) ; ("energy", irep, tempK, bp_mc, bp_energy_valid, key)
+      ("ele", irep, key) ; ("bp_tables", key) ; ("bp_maps", key)
+      ("tape", irep, n, key) ; ("genrand", irep, kind, n, key)
+      ("boxmuller", irep, n, key) ; ("set_xyz", xyz) ; ("get_xyz", key)
+      ("job_md", nstep, key)
+    """
+    so = REF_OMP_SO if omp else REF_SO
+    if not so.exists():
+        raise RuntimeError(f"{so} not built (make -C oracle ref)")
+    ctx = mp.get_context("spawn")
+    q = ctx.Queue()
+    env_old = os.environ.get("OMP_NUM_THREADS")
+    os.environ["OMP_NUM_THREADS"] = str(threads)
+    try:
+        pr = ctx.Process(target=_worker, args=(q, so, sysd, ops, seed, nstep_exchange))
+        pr.start()
+        import queue
+        import time
+        t0 = time.time()
+        while True:
+            try:
+                status, payload = q.get(timeout=1.0)
+                break
+            except queue.Empty:
+                if not pr.is_alive():
+                    status, payload = "err", f"reference process exited with code {pr.exitcode}"
+                    break
+                if time.time() - t0 > timeout:
+                    pr.kill()
+                    status, payload = "err", "reference process timed out"
+                    break
+        pr.join(timeout=30)
+    finally:
+        if env_old is None:
+            os.environ.pop("OMP_NUM_THREADS", None)
+        else:
+            os.environ["OMP_NUM_THREADS"] = env_old
+    if status != "ok":
+        raise RuntimeError("reference run failed:\n" + payload)
+    return payload

@@ -1,0 +1,1199 @@
+/*
+ * sis_oracle.c - CPU restatement of the SIS MD hot path.  TEST ORACLE ONLY:
+ * loaded by tests/, __graft_entry__.smoke() and the CPU-baseline leg of
+ * bench.py; never by the product library.
+ *
+ * Restated from the reference Fortran (file:line in each comment).  The
+ * expressions keep the reference's operation order so that, built with
+ * -ffp-contract=off, results agree with the reference to the last few ulps
+ * (libm is glibc on both sides).  Floating point is FP64 throughout
+ * (PREC = REAL64, src/const.F90:10).
+ */
+#include "sis_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+/* ---- physical constants (src/const_phys.F90:5-14) ---- */
+#define KCAL2JOUL 4184.0
+static const double PI_ = 3.141592653589793115997963468544185161590576171875;
+static const double EPS0 = 8.8541878128e-12, ELE = 1.602176634e-19;
+static const double BOLTZ_J = 1.380649e-23, N_AVO = 6.02214076e23;
+static double JOUL2KCAL_MOL, BOLTZ_KCAL_MOL;
+
+enum { A_ = 0, U_ = 1, G_ = 2, C_ = 3, D_ = 4 };
+enum { GC = 1, AU = 2, GU = 3, CG = 4, UA = 5, UG = 6 };
+
+typedef struct { int32_t *a; int64_t n, cap; } ivec;
+typedef struct { double *a; int64_t n, cap; } dvec;
+
+static void ipush(ivec *v, int32_t x) {
+    if (v->n == v->cap) { v->cap = v->cap ? 2 * v->cap : 1024; v->a = realloc(v->a, v->cap * sizeof(int32_t)); }
+    v->a[v->n++] = x;
+}
+static void dpush(dvec *v, double x) {
+    if (v->n == v->cap) { v->cap = v->cap ? 2 * v->cap : 1024; v->a = realloc(v->a, v->cap * sizeof(double)); }
+    v->a[v->n++] = x;
+}
+
+typedef struct {
+    const uint32_t *w; int64_t n, pos;
+} tape_t;
+
+typedef struct {
+    ivec wca, ele, bp;       /* wca/ele: 2 per pair; bp: 7 per candidate */
+    dvec bpcoef;             /* 2 per candidate */
+    uint8_t *bp_status, *bp_status_mc;
+    double *ene_bp;
+    int have_status_mc;
+    double lambdaD, ele_coef, ele_cutoff, diele;
+    double tK;               /* temperature of this replica */
+    tape_t tape;
+} rep_t;
+
+struct orc_sys {
+    sis_params p;
+    sis_bp_para bpp[7];      /* 1..6 */
+    int nmp, nchains, nrep;
+    int *nmp_chain, *moltype, *seq, *chain_of, *lmp, *chain_start;
+    uint8_t *has_charge;
+    double *xyz, *velos, *accels, *move;  /* [nrep][nmp][3] */
+    double *mass, *fric;
+    double bp_cutoff_dist;
+    double bp3_dH[469], bp3_dS[469];
+    int bp3_hash[4097];
+    rep_t *rep;
+    /* rnd_boxmuller process-global cache (src/rnd_boxmuller.F90:13-15) */
+    int bm_stored; double bm_value;
+    /* replica exchange */
+    int *rep2lab, *lab2rep; double *lab2val; int idx_pair; int *pair_tb; /* [2][nrep+1] */
+    tape_t tape_rep;
+    int64_t istep;
+    int flg_bp_energy;
+};
+
+/* ---------------- PBC (src/pbc.F90:140-165, 196-226) ---------------- */
+static inline void pbc_vec_d(const orc_sys *s, const double *a, const double *b, double *v) {
+    for (int k = 0; k < 3; k++) v[k] = a[k] - b[k];
+    if (!s->p.pbc) return;
+    for (int k = 0; k < 3; k++) {
+        double half = 0.5 * s->p.box[k];
+        if (v[k] > half) v[k] = v[k] - s->p.box[k];
+        else if (v[k] < -half) v[k] = v[k] + s->p.box[k];
+    }
+}
+static inline double dot3(const double *a, const double *b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
+
+static void pbc_wrap(orc_sys *s, int irep) {
+    double *x = s->xyz + (size_t)irep * s->nmp * 3;
+    for (int i = 0; i < s->nmp; i++)
+        for (int k = 0; k < 3; k++) {
+            double v = x[3 * i + k], L = s->p.box[k];
+            if (v > L) x[3 * i + k] = v - L * ((double)(int)((v - L) / L) + 1);
+            else if (v < 0.0) x[3 * i + k] = v + L * ((double)(int)((0.0 - v) / L) + 1);
+        }
+}
+
+/* ---------------- base-pair tables (src/init_bp.F90:39-172) ---------------- */
+static int is_complement(int a, int b) {   /* src/const_idx.F90:339-369 */
+    if (a == A_) return b == U_;
+    if (a == U_) return b == A_ || b == G_;
+    if (a == G_) return b == C_ || b == U_;
+    if (a == C_) return b == G_;
+    return 0;
+}
+/* NN stack index for 5'-w x-3' / 3'-z y-5' (src/const_idx.F90:230-330) */
+static int seqt2nnt(int w, int x, int z, int y) {
+    static const char ch[5] = {'A', 'U', 'G', 'C', 'D'};
+    static const char *names[21][2] = {
+        {"GC_CG", 0}, {"CC_GG", "GG_CC"}, {"GA_CU", "UC_AG"}, {"CG_GC", 0}, {"AC_UG", "GU_CA"},
+        {"CA_GU", "UG_AC"}, {"AG_UC", "CU_GA"}, {"UA_AU", 0}, {"AU_UA", 0}, {"AA_UU", "UU_AA"},
+        {"GC_UG", "GU_CG"}, {"CU_GG", "GG_UC"}, {"GG_CU", "UC_GG"}, {"CG_GU", "UG_GC"},
+        {"AU_UG", "GU_UA"}, {"GA_UU", "UU_AG"}, {"UG_GU", 0}, {"UA_GU", "UG_AU"},
+        {"GG_UU", "UU_GG"}, {"GU_UG", 0}, {"AG_UU", "UU_GA"}};
+    char key[6] = {ch[w], ch[x], '_', ch[z], ch[y], 0};
+    for (int i = 0; i < 21; i++)
+        for (int k = 0; k < 2; k++)
+            if (names[i][k] && strcmp(names[i][k], key) == 0) return i + 1;
+    return 0;
+}
+static int bp3_hash_key(int w, int x, int u, int z, int y, int v) {   /* src/init_bp.F90:352-378 */
+    int a, b, c, d, e, f;
+    if (x < y) { a = w; b = x; c = u; d = z; e = y; f = v; }
+    else { f = w; e = x; d = u; c = z; b = y; a = v; }
+    return 1 + a + b * 4 + c * 16 + d * 64 + e * 256 + f * 1024;
+}
+
+static void nnend_add(orc_sys *s, int h, int cx, int cy, int ox, int oy) {
+    /* end correction for centre pair (cx,cy) stacked on outer pair (ox,oy)
+     * (src/init_bp.F90:98-167) */
+    const sis_params *p = &s->p;
+    int au = (cx == A_ && cy == U_) || (cx == U_ && cy == A_);
+    int gu = (cx == G_ && cy == U_) || (cx == U_ && cy == G_);
+    int oau = (ox == A_ && oy == U_) || (ox == U_ && oy == A_);
+    int ogc = (ox == G_ && oy == C_) || (ox == C_ && oy == G_);
+    int ogu = (ox == G_ && oy == U_) || (ox == U_ && oy == G_);
+    int idx = 0;
+    if (au) idx = oau ? 1 : ogc ? 2 : ogu ? 3 : 0;
+    else if (gu) idx = oau ? 5 : ogc ? 4 : ogu ? 6 : 0;
+    if (idx) {
+        s->bp3_dH[h] = s->bp3_dH[h] + p->nnend_dH[idx - 1] - p->bp_dHend0;
+        s->bp3_dS[h] = s->bp3_dS[h] + (p->nnend_dS[idx - 1] - p->bp_dSend0) * 1.0e-3;
+    }
+}
+
+static void init_bp_tables(orc_sys *s) {
+    const sis_params *p = &s->p;
+    memset(s->bp3_hash, 0, sizeof s->bp3_hash);
+    memset(s->bp3_dH, 0, sizeof s->bp3_dH);
+    memset(s->bp3_dS, 0, sizeof s->bp3_dS);
+    int h = 0;
+    for (int x = 0; x <= 2; x++)
+        for (int y = x + 1; y <= 3; y++) {
+            if (!is_complement(x, y)) continue;
+            for (int w = 0; w <= 3; w++)
+                for (int z = 0; z <= 3; z++) {
+                    int comp_wz = is_complement(w, z);
+                    double dH = 0.0, dS = 0.0;
+                    if (comp_wz) {
+                        dH = 0.5 * (p->nn_dH[seqt2nnt(w, x, z, y) - 1] - p->bp_dH0);
+                        dS = 0.5 * (p->nn_dS[seqt2nnt(w, x, z, y) - 1] - p->bp_dS0);
+                    }
+                    for (int u = 0; u <= 3; u++)
+                        for (int v = 0; v <= 3; v++) {
+                            int comp_uv = is_complement(u, v);
+                            if (!comp_wz && !comp_uv) continue;
+                            h++;
+                            s->bp3_hash[bp3_hash_key(w, x, u, z, y, v)] = h;
+                            if (comp_uv) {
+                                s->bp3_dH[h] = dH + 0.5 * (p->nn_dH[seqt2nnt(x, u, y, v) - 1] - p->bp_dH0);
+                                s->bp3_dS[h] = (dS + 0.5 * (p->nn_dS[seqt2nnt(x, u, y, v) - 1] - p->bp_dS0)) * 1.0e-3;
+                            } else {
+                                s->bp3_dH[h] = dH;
+                                s->bp3_dS[h] = dS * 1.0e-3;
+                            }
+                            if (p->nnend) {
+                                if (comp_wz && !comp_uv) nnend_add(s, h, x, y, w, z);
+                                else if (!comp_wz && comp_uv) nnend_add(s, h, x, y, u, v);
+                            }
+                        }
+                }
+        }
+    /* BP cutoffs (src/init_bp.F90:321-346) */
+    if (fabs(p->bp_cutoff_energy) <= 2.220446049250313e-16) {
+        s->bp_cutoff_dist = 18.0;
+        for (int t = 1; t <= 6; t++) s->bpp[t].cutoff_ddist = s->bp_cutoff_dist - 13.8;
+    } else {
+        s->bp_cutoff_dist = 0.0;
+        for (int t = 1; t <= 6; t++) {
+            s->bpp[t].cutoff_ddist = sqrt(log(fabs(10.0 / p->bp_cutoff_energy)) / s->bpp[t].bond_k);
+            double c = s->bpp[t].bond_r + s->bpp[t].cutoff_ddist;
+            if (c > s->bp_cutoff_dist) s->bp_cutoff_dist = c;
+        }
+    }
+}
+
+/* Base-pair eligibility of beads imp < jmp (0-based) and its triplet index:
+ * the body of the pair loop of src/init_bp.F90:184-313 for models 3/4/5
+ * without secondary-structure files.  Returns 0 when not eligible. */
+void orc_bp_pair(orc_sys *s, int32_t imp, int32_t jmp, int32_t *bptype, int32_t *bp3) {
+    *bptype = 0; *bp3 = 0;
+    if (imp >= jmp) return;
+    int ich = s->chain_of[imp], jch = s->chain_of[jmp];
+    int i = s->lmp[imp], j = s->lmp[jmp];                 /* 1-based local */
+    int ni = s->nmp_chain[ich], nj = s->nmp_chain[jch];
+    int icirc = s->moltype[ich] == 2, jcirc = s->moltype[jch] == 2;
+    int i_pre = i - 1, i_nxt = i + 1, j_pre = j - 1, j_nxt = j + 1;
+    if (icirc) { if (i == 1) i_pre = ni; if (i == ni) i_nxt = 1; }
+    else { if (i == 1 || i == ni) return; }
+    if (jcirc) { if (j == 1) j_pre = nj; if (j == nj) j_nxt = 1; }
+    else { if (j == 1 || j == nj) return; }
+    const int *si = s->seq + s->chain_start[ich] - 1;   /* si[k], k 1-based */
+    const int *sj = s->seq + s->chain_start[jch] - 1;
+    int p_ipre_jnxt = is_complement(si[i_pre], sj[j_nxt]);
+    int p_inxt_jpre = is_complement(si[i_nxt], sj[j_pre]);
+    if (!icirc && i_pre == 1) p_ipre_jnxt = 0;
+    if (!jcirc && j_nxt == nj) p_ipre_jnxt = 0;
+    if (!icirc && i_nxt == ni) p_inxt_jpre = 0;
+    if (!jcirc && j_pre == 1) p_inxt_jpre = 0;
+    int L = s->p.bp_min_loop;
+    if (ich == jch) {
+        if (i + L >= j) return;
+        if (i + 1 + L >= j - 1) p_inxt_jpre = 0;
+        if (icirc) {
+            if (j + L >= i + ni) return;
+            if (j + 1 + L >= i - 1 + ni) p_ipre_jnxt = 0;
+        }
+    }
+    int model = s->p.bp_model;
+    if ((model == 3 || model == 4 || model == 5) && !p_ipre_jnxt && !p_inxt_jpre) return;
+    int a = si[i], b = sj[j], t = 0;
+    if (a == G_ && b == C_) t = GC; else if (a == C_ && b == G_) t = CG;
+    else if (a == A_ && b == U_) t = AU; else if (a == U_ && b == A_) t = UA;
+    else if (a == G_ && b == U_) t = GU; else if (a == U_ && b == G_) t = UG;
+    *bptype = t;
+    if (model == 5) {
+        int key;
+        if (!p_ipre_jnxt) key = bp3_hash_key(U_, si[i], si[i_nxt], U_, sj[j], sj[j_pre]);
+        else if (!p_inxt_jpre) key = bp3_hash_key(si[i_pre], si[i], U_, sj[j_nxt], sj[j], U_);
+        else key = bp3_hash_key(si[i_pre], si[i], si[i_nxt], sj[j_nxt], sj[j], sj[j_pre]);
+        *bp3 = s->bp3_hash[key];
+    } else {
+        *bp3 = t;
+    }
+}
+
+/* ---------------- set_ele (src/set_ele.F90:45-99) ---------------- */
+static void set_ele(orc_sys *s, int irep, double tempk, double ionstr) {
+    const double MM_A = 87.740, MM_B = -0.40008, MM_C = 9.398e-4, MM_D = -1.410e-6;
+    rep_t *r = &s->rep[irep];
+    double Tc = tempk - 273.15;
+    r->diele = MM_A + MM_B * Tc + MM_C * Tc * Tc + MM_D * Tc * Tc * Tc;
+    double lb = ELE * ELE / (4.0 * PI_ * EPS0 * r->diele * BOLTZ_J * tempk) * 1.0e10;
+    double Zp = s->p.length_per_charge / lb;
+    r->ele_coef = JOUL2KCAL_MOL * 1.0e10 * (ELE * ELE) / (4.0 * PI_ * EPS0 * r->diele);
+    r->ele_coef = r->ele_coef * (Zp * Zp);
+    r->lambdaD = 1.0e10 * sqrt((1.0e-3 * EPS0 * r->diele * BOLTZ_J) / (2.0 * N_AVO * (ELE * ELE))) *
+                 sqrt(tempk / ionstr);
+    if (s->p.ele_cutoff_type == 1) r->ele_cutoff = s->p.ele_cutoff;
+    else r->ele_cutoff = s->p.ele_cutoff * r->lambdaD;   /* src/init_ele.F90:99-101 */
+}
+
+/* ---------------- MT19937 words and conversions ---------------- */
+/* init_genrand (mt_stream.F90:419-455) and genrand_int32 with tempering
+ * (mt_stream.F90:510-549). */
+void orc_mt_words(uint32_t seed, int64_t n, uint32_t *out) {
+    uint32_t mt[624];
+    mt[0] = seed;
+    for (int i = 1; i < 624; i++) mt[i] = 1812433253u * (mt[i - 1] ^ (mt[i - 1] >> 30)) + (uint32_t)i;
+    int idx = 624;
+    for (int64_t k = 0; k < n; k++) {
+        if (idx >= 624) {
+            for (int i = 0; i < 624; i++) {
+                uint32_t y = (mt[i] & 0x80000000u) | (mt[(i + 1) % 624] & 0x7fffffffu);
+                mt[i] = mt[(i + 397) % 624] ^ (y >> 1) ^ ((y & 1u) ? 0x9908b0dfu : 0u);
+            }
+            idx = 0;
+        }
+        uint32_t y = mt[idx++];
+        y ^= y >> 11;
+        y ^= (y << 7) & 0x9d2c5680u;
+        y ^= (y << 15) & 0xefc60000u;
+        y ^= y >> 18;
+        out[k] = y;
+    }
+}
+
+static tape_t *tape_of(orc_sys *s, int irep) { return irep < 0 ? &s->tape_rep : &s->rep[irep].tape; }
+static uint32_t next_word(tape_t *t) {
+    if (t->pos >= t->n) { abort(); }
+    return t->w[t->pos++];
+}
+/* genrand_double1: [0,1], 53-bit (mt_stream.F90:708-729) */
+static double gen_d1(tape_t *t) {
+    uint32_t ia = next_word(t) >> 5, ib = next_word(t) >> 6;
+    double a = (double)ia, b = (double)ib;
+    return (a * 67108864.0 + b) * (1.0 / 9007199254740991.0);
+}
+/* genrand_double3: (0,1), 52-bit (mt_stream.F90:754-775) */
+static double gen_d3(tape_t *t) {
+    uint32_t ia = next_word(t) >> 6, ib = next_word(t) >> 6;
+    double a = (double)ia, b = (double)ib;
+    return (a * 67108864.0 + b + 0.5) * (1.0 / 4503599627370496.0);
+}
+/* rnd_boxmuller (src/rnd_boxmuller.F90:1-49): polar method, one cached
+ * value shared by all replicas. */
+static double boxmuller(orc_sys *s, int irep) {
+    if (!s->bm_stored) {
+        s->bm_stored = 1;
+        tape_t *t = tape_of(s, irep);
+        double vx, vy, r2;
+        for (;;) {
+            vx = 2.0 * gen_d1(t) - 1.0;
+            vy = 2.0 * gen_d1(t) - 1.0;
+            r2 = vx * vx + vy * vy;
+            if (r2 < 1.0 && r2 > 0.0) break;
+        }
+        double rf = sqrt(-2.0 * log(r2) / r2);
+        s->bm_value = vx * rf;
+        return vy * rf;
+    }
+    s->bm_stored = 0;
+    return s->bm_value;
+}
+
+/* ---------------- neighbour lists (src/neighbor_list.F90:84-233) ---------------- */
+void orc_neighbor_list(orc_sys *s, int32_t irep1) {
+    int irep = irep1 - 1;
+    rep_t *r = &s->rep[irep];
+    if (s->p.pbc) pbc_wrap(s, irep);
+    r->wca.n = r->ele.n = r->bp.n = r->bpcoef.n = 0;
+    const double margin = s->p.nl_margin;
+    double ele_nl_cut2 = 0.0;
+    if (s->p.ele_on) ele_nl_cut2 = (r->ele_cutoff + margin) * (r->ele_cutoff + margin);
+    double wca_nl_cut2 = (s->p.wca_sigma + margin) * (s->p.wca_sigma + margin);
+    double bp_nl_cut2 = (s->bp_cutoff_dist + margin) * (s->bp_cutoff_dist + margin);
+    const double *x = s->xyz + (size_t)irep * s->nmp * 3;
+    for (int ich = 0; ich < s->nchains; ich++)
+        for (int jch = ich; jch < s->nchains; jch++)
+            for (int i = 1; i <= s->nmp_chain[ich]; i++) {
+                int imp = s->chain_start[ich] + i - 1;
+                int iseq = s->seq[imp];
+                int j_start = (ich == jch) ? i + 1 : 1;
+                for (int j = j_start; j <= s->nmp_chain[jch]; j++) {
+                    int jmp = s->chain_start[jch] + j - 1;
+                    int jseq = s->seq[jmp];
+                    double v[3];
+                    pbc_vec_d(s, x + 3 * imp, x + 3 * jmp, v);
+                    double d2 = dot3(v, v);
+                    if (iseq != D_ && jseq != D_ && d2 <= wca_nl_cut2) {
+                        int add = 1;
+                        if (ich == jch) {
+                            if (i + 2 >= j) add = 0;
+                            else if (s->moltype[ich] == 2 && j + 2 >= i + s->nmp_chain[ich]) add = 0;
+                        }
+                        if (add) { ipush(&r->wca, imp + 1); ipush(&r->wca, jmp + 1); }
+                    }
+                    if (s->p.ele_on && d2 <= ele_nl_cut2 && s->has_charge[imp] && s->has_charge[jmp]) {
+                        int add = 1;
+                        if (ich == jch) {
+                            int n = s->nmp_chain[ich];
+                            if (j == i + 1 && s->p.ele_excl_bond) add = 0;
+                            if (j == i + 2 && s->p.ele_excl_angl) add = 0;
+                            if (s->moltype[ich] == 2) {
+                                if (i == 1) {
+                                    if (j == n && s->p.ele_excl_bond) add = 0;
+                                    if (j == n - 1 && s->p.ele_excl_angl) add = 0;
+                                } else if (i == 2) {
+                                    if (j == n && s->p.ele_excl_angl) add = 0;
+                                }
+                            }
+                        }
+                        if (add) { ipush(&r->ele, imp + 1); ipush(&r->ele, jmp + 1); }
+                    }
+                    int bt, b3;
+                    orc_bp_pair(s, imp, jmp, &bt, &b3);
+                    if (b3 > 0 && d2 <= bp_nl_cut2) {
+                        int ni = s->nmp_chain[ich], nj = s->nmp_chain[jch];
+                        ipush(&r->bp, imp + 1);
+                        ipush(&r->bp, jmp + 1);
+                        ipush(&r->bp, (i == 1) ? imp + 1 + ni - 1 : imp + 1 - 1);
+                        ipush(&r->bp, (j == 1) ? jmp + 1 + nj - 1 : jmp + 1 - 1);
+                        ipush(&r->bp, (i == ni) ? imp + 1 + 1 - ni : imp + 1 + 1);
+                        /* the reference compares j with nmp_chain(ichain) here
+                         * (src/neighbor_list.F90:212) */
+                        ipush(&r->bp, (j == ni) ? jmp + 1 + 1 - nj : jmp + 1 + 1);
+                        ipush(&r->bp, bt);
+                        dpush(&r->bpcoef, s->bp3_dH[b3]);
+                        dpush(&r->bpcoef, s->bp3_dS[b3]);
+                    }
+                }
+            }
+    int64_t nbp = r->bp.n / 7;
+    r->bp_status = realloc(r->bp_status, nbp + 1);
+    r->ene_bp = realloc(r->ene_bp, (nbp + 1) * sizeof(double));
+    memset(r->bp_status, 0, nbp + 1);
+    memset(r->ene_bp, 0, (nbp + 1) * sizeof(double));
+    r->bp_status_mc = realloc(r->bp_status_mc, nbp + 1);
+    memset(r->bp_status_mc, 0, nbp + 1);
+    r->have_status_mc = 0;
+}
+
+int64_t orc_count(orc_sys *s, int32_t irep, int32_t kind) {
+    rep_t *r = &s->rep[irep - 1];
+    return kind == 0 ? r->wca.n / 2 : kind == 1 ? r->ele.n / 2 : r->bp.n / 7;
+}
+void orc_lists(orc_sys *s, int32_t irep, int32_t kind, int32_t *out, double *bpcoef) {
+    rep_t *r = &s->rep[irep - 1];
+    ivec *v = kind == 0 ? &r->wca : kind == 1 ? &r->ele : &r->bp;
+    memcpy(out, v->a, v->n * sizeof(int32_t));
+    if (kind == 2 && bpcoef) memcpy(bpcoef, r->bpcoef.a, r->bpcoef.n * sizeof(double));
+}
+
+/* ---------------- forces ---------------- */
+#define F3(f, i) (f + 3 * (size_t)(i))
+static void force_bond(orc_sys *s, const double *x, double *f) {    /* src/force_bond.F90:33-57 */
+    for (int c = 0; c < s->nchains; c++) {
+        int n = s->nmp_chain[c], st = s->chain_start[c];
+        int nb = n - 1 + (s->moltype[c] == 2);
+        for (int b = 0; b < nb; b++) {
+            int i1 = (b < n - 1) ? st + b : st + n - 1, i2 = (b < n - 1) ? st + b + 1 : st;
+            double v[3], fv[3];
+            pbc_vec_d(s, F3(x, i1), F3(x, i2), v);
+            double d = sqrt(dot3(v, v));
+            double delta = d - s->p.bond_r0;
+            for (int k = 0; k < 3; k++) fv[k] = s->p.bond_k * delta / d * v[k];
+            for (int k = 0; k < 3; k++) { F3(f, i1)[k] -= fv[k]; F3(f, i2)[k] += fv[k]; }
+        }
+    }
+}
+/* angle / dihedral index lists in list_local order (src/list_local.F90:24-104) */
+static int angle_list(orc_sys *s, int (*out)[3]) {
+    int n = 0;
+    for (int c = 0; c < s->nchains; c++) {
+        int m = s->nmp_chain[c], st = s->chain_start[c];
+        for (int i = 1; i <= m - 1; i++)
+            if (i != m - 1) { out[n][0] = st + i - 1; out[n][1] = st + i; out[n][2] = st + i + 1; n++; }
+        if (s->moltype[c] == 2) {
+            int ini = st, end = st + m - 1;
+            out[n][0] = end - 1; out[n][1] = end; out[n][2] = ini; n++;
+            out[n][0] = end; out[n][1] = ini; out[n][2] = ini + 1; n++;
+        }
+    }
+    return n;
+}
+static int dih_list(orc_sys *s, int (*out)[4]) {
+    int n = 0;
+    for (int c = 0; c < s->nchains; c++) {
+        int m = s->nmp_chain[c], st = s->chain_start[c];
+        for (int i = 1; i <= m - 1; i++)
+            if (i < m - 2) { for (int k = 0; k < 4; k++) out[n][k] = st + i - 1 + k; n++; }
+        if (s->moltype[c] == 2) {
+            int ini = st, end = st + m - 1;
+            int q[3][4] = {{end - 2, end - 1, end, ini}, {end - 1, end, ini, ini + 1}, {end, ini, ini + 1, ini + 2}};
+            for (int a = 0; a < 3; a++) { for (int k = 0; k < 4; k++) out[n][k] = q[a][k]; n++; }
+        }
+    }
+    return n;
+}
+static void force_angl(orc_sys *s, const double *x, double *f) {
+    int (*al)[3] = malloc(sizeof(int[3]) * (s->nmp + 2 * s->nchains));
+    int na = angle_list(s, al);
+    const double k = s->p.angl_k, t0 = s->p.angl_t0;
+    for (int a = 0; a < na; a++) {
+        int i1 = al[a][0], i2 = al[a][1], i3 = al[a][2];
+        if (s->p.angl_reb) {   /* src/force_angl_ReB.F90:41-74 */
+            double co0 = cos(t0), v12[3], v32[3], f1[3], f3[3];
+            pbc_vec_d(s, F3(x, i1), F3(x, i2), v12);
+            pbc_vec_d(s, F3(x, i3), F3(x, i2), v32);
+            double d12 = dot3(v12, v12), d32 = dot3(v32, v32), a1232 = sqrt(d12 * d32);
+            double cosine = dot3(v12, v32) / a1232;
+            double den = (1.0 - cosine * cosine);
+            double ff = -k * (cosine - co0) * (1.0 - cosine * co0) / (den * den);
+            for (int c = 0; c < 3; c++) {
+                f1[c] = ff * (v32[c] / a1232 - cosine * v12[c] / d12);
+                f3[c] = ff * (v12[c] / a1232 - cosine * v32[c] / d32);
+            }
+            for (int c = 0; c < 3; c++) {
+                F3(f, i1)[c] += f1[c]; F3(f, i2)[c] -= (f1[c] + f3[c]); F3(f, i3)[c] += f3[c];
+            }
+        } else {               /* src/force_angl.F90:35-83 */
+            double v21[3], v32[3], f21[3], f32[3];
+            pbc_vec_d(s, F3(x, i2), F3(x, i1), v21);
+            pbc_vec_d(s, F3(x, i3), F3(x, i2), v32);
+            double d21 = dot3(v21, v21), d32 = dot3(v32, v32), d2132 = dot3(v32, v21);
+            double cosine = -d2132 / sqrt(d21 * d32);
+            if (fabs(cosine) > 1.0) cosine = copysign(1.0, cosine);
+            double delta = acos(cosine) - t0;
+            double t = d21 * d32 - d2132 * d2132;
+            if (t <= 1.0) t = 1.0;
+            double ff = k * delta / sqrt(t);
+            for (int c = 0; c < 3; c++) {
+                f21[c] = ff * (v21[c] * (d2132 / d21) - v32[c]);
+                f32[c] = ff * (v32[c] * (d2132 / d32) - v21[c]);
+            }
+            for (int c = 0; c < 3; c++) {
+                F3(f, i1)[c] -= f21[c]; F3(f, i2)[c] += f21[c] - f32[c]; F3(f, i3)[c] += f32[c];
+            }
+        }
+    }
+    free(al);
+}
+static void cross(const double *a, const double *b, double *c) {
+    c[0] = a[1] * b[2] - a[2] * b[1];
+    c[1] = a[2] * b[0] - a[0] * b[2];
+    c[2] = a[0] * b[1] - a[1] * b[0];
+}
+static void force_dih(orc_sys *s, const double *x, double *f) {
+    /* src/force_dih_exp.F90:36-95 and src/force_dih_cos.F90:21-60 */
+    if (s->p.dih_type == 0) return;
+    int (*dl)[4] = malloc(sizeof(int[4]) * (s->nmp + 3 * s->nchains));
+    int nd = dih_list(s, dl);
+    double coef1 = s->p.dih_k * s->p.dih_w, coef2 = -0.5 * s->p.dih_w;
+    for (int a = 0; a < nd; a++) {
+        int i1 = dl[a][0], i2 = dl[a][1], i3 = dl[a][2], i4 = dl[a][3];
+        double vij[3], vkj[3], vkl[3], m[3], n[3], fi[3], fj[3], fk[3], fl[3];
+        pbc_vec_d(s, F3(x, i1), F3(x, i2), vij);
+        pbc_vec_d(s, F3(x, i3), F3(x, i2), vkj);
+        pbc_vec_d(s, F3(x, i3), F3(x, i4), vkl);
+        cross(vij, vkj, m);
+        cross(vkj, vkl, n);
+        double akj2 = dot3(vkj, vkj), akj = sqrt(akj2);
+        double dih = atan2(akj * dot3(vij, n), dot3(m, n));
+        double pre;
+        if (s->p.dih_type == 2) {
+            double dd = dih - s->p.dih_p0;
+            pre = coef1 * dd * exp(coef2 * (dd * dd)) * akj;
+        } else {
+            pre = -s->p.dih_k * sin(dih + s->p.dih_p0) * akj;
+        }
+        double mm = dot3(m, m), nn = dot3(n, n);
+        for (int c = 0; c < 3; c++) { fi[c] = -pre / mm * m[c]; fl[c] = pre / nn * n[c]; }
+        double dvij = dot3(vij, vkj) / akj2, dvkl = dot3(vkl, vkj) / akj2;
+        for (int c = 0; c < 3; c++) {
+            fj[c] = (-1.0 + dvij) * fi[c] + (-dvkl) * fl[c];
+            fk[c] = (-dvij) * fi[c] + (-1.0 + dvkl) * fl[c];
+        }
+        for (int c = 0; c < 3; c++) {
+            F3(f, i1)[c] += fi[c]; F3(f, i2)[c] += fj[c]; F3(f, i3)[c] += fk[c]; F3(f, i4)[c] += fl[c];
+        }
+    }
+    free(dl);
+}
+static void force_wca(orc_sys *s, int irep, const double *x, double *f) {   /* src/force_wca.F90:34-67 */
+    rep_t *r = &s->rep[irep];
+    double sig2 = s->p.wca_sigma * s->p.wca_sigma, coef = 12.0 * s->p.wca_eps / sig2;
+    for (int64_t k = 0; k < r->wca.n / 2; k++) {
+        int i1 = r->wca.a[2 * k] - 1, i2 = r->wca.a[2 * k + 1] - 1;
+        double v[3];
+        pbc_vec_d(s, F3(x, i1), F3(x, i2), v);
+        double d2 = dot3(v, v);
+        if (d2 >= sig2) continue;
+        d2 = sig2 / d2;
+        double d4 = d2 * d2 * d2 * d2, d7 = d4 * d2 * d2 * d2;
+        for (int c = 0; c < 3; c++) { double fv = coef * (d7 - d4) * v[c]; F3(f, i1)[c] += fv; F3(f, i2)[c] -= fv; }
+    }
+}
+static void force_ele(orc_sys *s, int irep, const double *x, double *f) {  /* src/force_ele_DH.F90:39-82 */
+    rep_t *r = &s->rep[irep];
+    double cutoff2 = r->ele_cutoff * r->ele_cutoff, rcdist = 1.0 / r->lambdaD;
+    for (int64_t k = 0; k < r->ele.n / 2; k++) {
+        int i1 = r->ele.a[2 * k] - 1, i2 = r->ele.a[2 * k + 1] - 1;
+        double v[3];
+        pbc_vec_d(s, F3(x, i2), F3(x, i1), v);
+        double dist2 = dot3(v, v);
+        if (dist2 > cutoff2) continue;
+        double dist1 = sqrt(dist2), rdist1 = 1.0 / dist1;
+        double dv = r->ele_coef * rdist1 * rdist1 * (rdist1 + rcdist) * exp(-dist1 * rcdist);
+        for (int c = 0; c < 3; c++) { double fv = dv * v[c]; F3(f, i1)[c] -= fv; F3(f, i2)[c] += fv; }
+    }
+}
+
+/* 6-bead base-pair geometry: energy factor u and unscaled forces fb[6][3]
+ * (src/force_bp_limit_triplet.F90:100-240).  Returns 0 when the pair is
+ * outside cutoff_ddist. */
+static int bp_geom(orc_sys *s, const double *x, const int32_t *mp, double *u_out, double fb[6][3]) {
+    const sis_bp_para *bpp = &s->bpp[mp[6]];
+    int i1 = mp[0] - 1, i2 = mp[1] - 1, i3 = mp[2] - 1, i4 = mp[3] - 1, i5 = mp[4] - 1, i6 = mp[5] - 1;
+    double v12[3], v13[3], v15[3], v42[3], v62[3], fi[3], fk[3], m[3], n[3];
+    pbc_vec_d(s, F3(x, i1), F3(x, i2), v12);
+    double d1212 = dot3(v12, v12), a12 = sqrt(d1212), d = a12 - bpp->bond_r;
+    if (fabs(d) > bpp->cutoff_ddist) return 0;
+    double u = bpp->bond_k * (d * d);
+    for (int c = 0; c < 3; c++) { fi[c] = (2.0 * bpp->bond_k * d / a12) * v12[c]; fb[0][c] = fi[c]; fb[1][c] = -fi[c]; }
+    pbc_vec_d(s, F3(x, i1), F3(x, i3), v13);
+    pbc_vec_d(s, F3(x, i1), F3(x, i5), v15);
+    pbc_vec_d(s, F3(x, i4), F3(x, i2), v42);
+    pbc_vec_d(s, F3(x, i6), F3(x, i2), v62);
+    double d1313 = dot3(v13, v13), d4242 = dot3(v42, v42), d1213 = dot3(v12, v13);
+    double d1242 = dot3(v12, v42), d1215 = dot3(v12, v15), d1515 = dot3(v15, v15);
+    double d1262 = dot3(v12, v62), d6262 = dot3(v62, v62);
+    double d1213o = d1213 / d1212, d1242o = d1242 / d1212, d1215o = d1215 / d1212, d1262o = d1262 / d1212;
+    double cosine, pre;
+    /* angle 3-1=2 */
+    cosine = d1213 / (sqrt(d1313) * a12);
+    d = acos(cosine) - bpp->angl_theta2;
+    pre = 2.0 * bpp->angl_k2 * d / sqrt(d1313 * d1212 - d1213 * d1213);
+    u = u + bpp->angl_k2 * (d * d);
+    for (int c = 0; c < 3; c++) {
+        fi[c] = pre * (v12[c] - (d1213 / d1313 * v13[c]));
+        fk[c] = pre * (v13[c] - (d1213o * v12[c]));
+        fb[0][c] = fb[0][c] - fi[c] - fk[c]; fb[1][c] = fb[1][c] + fk[c]; fb[2][c] = fi[c];
+    }
+    /* angle 1=2-4 */
+    cosine = d1242 / (a12 * sqrt(d4242));
+    d = acos(cosine) - bpp->angl_theta1;
+    u = u + bpp->angl_k1 * (d * d);
+    pre = 2.0 * bpp->angl_k1 * d / sqrt(d1212 * d4242 - d1242 * d1242);
+    for (int c = 0; c < 3; c++) {
+        fi[c] = -pre * (v42[c] - (d1242o * v12[c]));
+        fk[c] = -pre * (v12[c] - (d1242 / d4242 * v42[c]));
+        fb[0][c] = fb[0][c] + fi[c]; fb[1][c] = fb[1][c] - fi[c] - fk[c]; fb[3][c] = fk[c];
+    }
+    /* angle 5-1=2 */
+    cosine = d1215 / (sqrt(d1515) * a12);
+    d = acos(cosine) - bpp->angl_theta4;
+    u = u + bpp->angl_k4 * (d * d);
+    pre = 2.0 * bpp->angl_k4 * d / sqrt(d1515 * d1212 - d1215 * d1215);
+    for (int c = 0; c < 3; c++) {
+        fi[c] = pre * (v12[c] - (d1215 / d1515 * v15[c]));
+        fk[c] = pre * (v15[c] - (d1215o * v12[c]));
+        fb[0][c] = fb[0][c] - fi[c] - fk[c]; fb[1][c] = fb[1][c] + fk[c]; fb[4][c] = fi[c];
+    }
+    /* angle 1=2-6 */
+    cosine = d1262 / (a12 * sqrt(d6262));
+    d = acos(cosine) - bpp->angl_theta3;
+    u = u + bpp->angl_k3 * (d * d);
+    pre = 2.0 * bpp->angl_k3 * d / sqrt(d1212 * d6262 - d1262 * d1262);
+    for (int c = 0; c < 3; c++) {
+        fi[c] = -pre * (v62[c] - (d1262o * v12[c]));
+        fk[c] = -pre * (v12[c] - (d1262 / d6262 * v62[c]));
+        fb[0][c] = fb[0][c] + fi[c]; fb[1][c] = fb[1][c] - fi[c] - fk[c]; fb[5][c] = fk[c];
+    }
+    /* dihedral 4-2=1=3 */
+    cross(v42, v12, m);
+    cross(v12, v13, n);
+    double dih = atan2(dot3(v42, n) * a12, dot3(m, n));
+    d = dih + bpp->dihd_phi1;
+    u = u + bpp->dihd_k1 * (1.0 + cos(d));
+    pre = -bpp->dihd_k1 * sin(d) * a12;
+    double mm = dot3(m, m), nn = dot3(n, n);
+    double fl[3];
+    for (int c = 0; c < 3; c++) {
+        fi[c] = +pre / mm * m[c];
+        fl[c] = -pre / nn * n[c];
+        fb[3][c] = fb[3][c] + fi[c];
+        fb[1][c] = fb[1][c] + (-1.0 + d1242o) * fi[c] - (d1213o) * fl[c];
+        fb[0][c] = fb[0][c] + (-1.0 + d1213o) * fl[c] - (d1242o) * fi[c];
+        fb[2][c] = fb[2][c] + fl[c];
+    }
+    /* dihedral 6-2=1=5 */
+    cross(v62, v12, m);
+    cross(v12, v15, n);
+    dih = atan2(dot3(v62, n) * a12, dot3(m, n));
+    d = dih + bpp->dihd_phi2;
+    u = u + bpp->dihd_k2 * (1.0 + cos(d));
+    pre = -bpp->dihd_k2 * sin(d) * a12;
+    mm = dot3(m, m); nn = dot3(n, n);
+    for (int c = 0; c < 3; c++) {
+        fi[c] = +pre / mm * m[c];
+        fl[c] = -pre / nn * n[c];
+        fb[5][c] = fb[5][c] + fi[c];
+        fb[1][c] = fb[1][c] + (-1.0 + d1262o) * fi[c] - (d1215o) * fl[c];
+        fb[0][c] = fb[0][c] + (-1.0 + d1215o) * fl[c] - (d1262o) * fi[c];
+        fb[4][c] = fb[4][c] + fl[c];
+    }
+    *u_out = u;
+    return 1;
+}
+
+/* The limit Monte Carlo that removes excess base pairs per nucleotide,
+ * shared by force_bp_limit_triplet (src/force_bp_limit_triplet.F90:259-330)
+ * and energy_bp_limit_triplet (src/energy_bp_limit_triplet.F90:108-190). */
+static void bp_limit_mc(orc_sys *s, int irep, int *nt_excess, double beta) {
+    rep_t *r = &s->rep[irep];
+    int64_t nbp = r->bp.n / 7;
+    int *ntlist = malloc(sizeof(int) * (s->nmp + 1));
+    int *bp_seq = malloc(sizeof(int) * (nbp + 1));
+    tape_t *t = &r->tape;
+    for (;;) {
+        int nnt = 0;
+        for (int imp = 0; imp < s->nmp; imp++) if (nt_excess[imp] > 0) ntlist[++nnt] = imp + 1;
+        if (nnt == 0) break;
+        double rnd = gen_d3(t);
+        int nt_delete = ntlist[(int)ceil(rnd * nnt)];
+        int nseq = 0;
+        for (int64_t b = 0; b < nbp; b++)
+            if (r->bp_status[b]) {
+                int i1 = r->bp.a[7 * b], i2 = r->bp.a[7 * b + 1];
+                if (i1 == nt_delete || i2 == nt_delete) bp_seq[++nseq] = (int)b;
+            }
+        for (int i = 1; i <= nseq; i++) {
+            rnd = gen_d3(t);
+            int isw = (int)ceil(rnd * nseq);
+            int sv = bp_seq[i]; bp_seq[i] = bp_seq[isw]; bp_seq[isw] = sv;
+        }
+        int del = bp_seq[1];
+        for (int i = 2; i <= nseq; i++) {
+            int jb = bp_seq[i];
+            double ratio = exp((r->ene_bp[jb] - r->ene_bp[del]) * beta);
+            rnd = gen_d1(t);
+            if (rnd < ratio) del = jb;
+        }
+        r->bp_status[del] = 0;
+        nt_excess[r->bp.a[7 * del] - 1] -= 1;
+        nt_excess[r->bp.a[7 * del + 1] - 1] -= 1;
+    }
+    free(ntlist);
+    free(bp_seq);
+}
+
+static void force_bp(orc_sys *s, int irep, int bp_mc, const double *x, double *f) {
+    /* models 4/5: src/force_bp_limit_triplet.F90 (bp_mc) and
+     * src/force_bp_triplet.F90 (no MC, skips candidates not in bp_status_MC) */
+    rep_t *r = &s->rep[irep];
+    int64_t nbp = r->bp.n / 7;
+    double tK = r->tK, beta = 1.0 / (BOLTZ_KCAL_MOL * tK);
+    double (*fbp)[6][3] = calloc(nbp + 1, sizeof(double[6][3]));
+    int *nt_excess = malloc(sizeof(int) * s->nmp);
+    for (int i = 0; i < s->nmp; i++) nt_excess[i] = -s->p.max_bp_per_nt;
+    for (int64_t b = 0; b < nbp; b++) { r->bp_status[b] = 0; r->ene_bp[b] = 0.0; }
+    for (int64_t b = 0; b < nbp; b++) {
+        if (!bp_mc && s->p.nstep_bp_mc > 0 && !r->bp_status_mc[b]) continue;
+        const int32_t *mp = r->bp.a + 7 * b;
+        double dG = r->bpcoef.a[2 * b] - tK * r->bpcoef.a[2 * b + 1];
+        if (dG >= 0.0) continue;
+        double u, fb[6][3];
+        if (!bp_geom(s, x, mp, &u, fb)) continue;
+        double e = s->bpp[mp[6]].U0 * dG * exp(-u);
+        if (e <= s->p.bp_cutoff_energy) {
+            r->bp_status[b] = 1;
+            r->ene_bp[b] = e;
+            for (int q = 0; q < 6; q++) for (int c = 0; c < 3; c++) fbp[b][q][c] = e * fb[q][c];
+            nt_excess[mp[0] - 1] += 1;
+            nt_excess[mp[1] - 1] += 1;
+        }
+    }
+    if (bp_mc) bp_limit_mc(s, irep, nt_excess, beta);
+    for (int64_t b = 0; b < nbp; b++) {
+        if (!r->bp_status[b]) continue;
+        const int32_t *mp = r->bp.a + 7 * b;
+        for (int q = 0; q < 6; q++) for (int c = 0; c < 3; c++) F3(f, mp[q] - 1)[c] += fbp[b][q][c];
+    }
+    s->flg_bp_energy = 1;
+    free(fbp);
+    free(nt_excess);
+}
+
+/* force (src/force.F90:27-69), single thread */
+void orc_force(orc_sys *s, int32_t irep1, int32_t bp_mc, double *forces) {
+    int irep = irep1 - 1;
+    const double *x = s->xyz + (size_t)irep * s->nmp * 3;
+    memset(forces, 0, sizeof(double) * 3 * s->nmp);
+    force_bond(s, x, forces);
+    force_angl(s, x, forces);
+    force_dih(s, x, forces);
+    force_bp(s, irep, bp_mc, x, forces);
+    force_wca(s, irep, x, forces);
+    if (s->p.ele_on) force_ele(s, irep, x, forces);
+}
+
+/* ---------------- energies (src/energy_*.F90) ---------------- */
+static double mp_angle(orc_sys *s, const double *x, int a, int b, int c) {   /* energy_bp_limit_triplet.F90:199-218 */
+    double v12[3], v32[3];
+    pbc_vec_d(s, F3(x, a), F3(x, b), v12);
+    pbc_vec_d(s, F3(x, c), F3(x, b), v32);
+    double co = dot3(v32, v12) / sqrt(dot3(v12, v12) * dot3(v32, v32));
+    if (co > 1.0) co = 1.0; else if (co < -1.0) co = -1.0;
+    return acos(co);
+}
+static double mp_dihedral(orc_sys *s, const double *x, int a, int b, int c, int d) {   /* :220-240 */
+    double v12[3], v32[3], v34[3], m[3], n[3];
+    pbc_vec_d(s, F3(x, a), F3(x, b), v12);
+    pbc_vec_d(s, F3(x, c), F3(x, b), v32);
+    pbc_vec_d(s, F3(x, c), F3(x, d), v34);
+    cross(v12, v32, m);
+    cross(v32, v34, n);
+    return atan2(dot3(v12, n) * sqrt(dot3(v32, v32)), dot3(m, n));
+}
+static double energy_bp(orc_sys *s, int irep, double tK, int bp_mc, const double *x) {
+    /* src/energy_bp_limit_triplet.F90:35-195 and src/energy_bp_triplet.F90 */
+    rep_t *r = &s->rep[irep];
+    int64_t nbp = r->bp.n / 7;
+    if (!s->flg_bp_energy) {
+        double beta = 1.0 / (BOLTZ_KCAL_MOL * tK);
+        int *nt_excess = malloc(sizeof(int) * s->nmp);
+        for (int i = 0; i < s->nmp; i++) nt_excess[i] = -s->p.max_bp_per_nt;
+        for (int64_t b = 0; b < nbp; b++) { r->bp_status[b] = 0; r->ene_bp[b] = 0.0; }
+        for (int64_t b = 0; b < nbp; b++) {
+            if (!bp_mc && s->p.nstep_bp_mc > 0 && !r->bp_status_mc[b]) continue;
+            const int32_t *mp = r->bp.a + 7 * b;
+            const sis_bp_para *bpp = &s->bpp[mp[6]];
+            double dG = r->bpcoef.a[2 * b] - tK * r->bpcoef.a[2 * b + 1];
+            if (dG >= 0.0) continue;
+            double v[3];
+            pbc_vec_d(s, F3(x, mp[0] - 1), F3(x, mp[1] - 1), v);
+            double d = sqrt(dot3(v, v)) - bpp->bond_r;
+            if (fabs(d) > bpp->cutoff_ddist) continue;
+            double u = bpp->bond_k * (d * d), th, ph;
+            th = mp_angle(s, x, mp[0] - 1, mp[1] - 1, mp[3] - 1); u = u + bpp->angl_k1 * ((th - bpp->angl_theta1) * (th - bpp->angl_theta1));
+            th = mp_angle(s, x, mp[2] - 1, mp[0] - 1, mp[1] - 1); u = u + bpp->angl_k2 * ((th - bpp->angl_theta2) * (th - bpp->angl_theta2));
+            th = mp_angle(s, x, mp[0] - 1, mp[1] - 1, mp[5] - 1); u = u + bpp->angl_k3 * ((th - bpp->angl_theta3) * (th - bpp->angl_theta3));
+            th = mp_angle(s, x, mp[4] - 1, mp[0] - 1, mp[1] - 1); u = u + bpp->angl_k4 * ((th - bpp->angl_theta4) * (th - bpp->angl_theta4));
+            ph = mp_dihedral(s, x, mp[2] - 1, mp[0] - 1, mp[1] - 1, mp[3] - 1); u = u + bpp->dihd_k1 * (1.0 + cos(ph + bpp->dihd_phi1));
+            ph = mp_dihedral(s, x, mp[4] - 1, mp[0] - 1, mp[1] - 1, mp[5] - 1); u = u + bpp->dihd_k2 * (1.0 + cos(ph + bpp->dihd_phi2));
+            u = bpp->U0 * dG * exp(-u);
+            if (u <= s->p.bp_cutoff_energy) {
+                r->ene_bp[b] = u;
+                r->bp_status[b] = 1;
+                nt_excess[mp[0] - 1] += 1;
+                nt_excess[mp[1] - 1] += 1;
+            }
+        }
+        if (bp_mc) bp_limit_mc(s, irep, nt_excess, beta);
+        free(nt_excess);
+    }
+    double e = 0.0;
+    for (int64_t b = 0; b < nbp; b++) if (r->bp_status[b]) e += r->ene_bp[b];
+    return e;
+}
+
+void orc_energy(orc_sys *s, int32_t irep1, double tK, int32_t bp_mc, int32_t bp_energy_valid, double *E) {
+    int irep = irep1 - 1;
+    rep_t *r = &s->rep[irep];
+    const double *x = s->xyz + (size_t)irep * s->nmp * 3;
+    for (int k = 0; k < 11; k++) E[k] = 0.0;
+    s->flg_bp_energy = bp_energy_valid;
+    /* bond (src/energy_bond.F90:16-24) */
+    for (int c = 0; c < s->nchains; c++) {
+        int n = s->nmp_chain[c], st = s->chain_start[c];
+        int nb = n - 1 + (s->moltype[c] == 2);
+        for (int b = 0; b < nb; b++) {
+            int i1 = (b < n - 1) ? st + b : st + n - 1, i2 = (b < n - 1) ? st + b + 1 : st;
+            double v[3];
+            pbc_vec_d(s, F3(x, i1), F3(x, i2), v);
+            double d = sqrt(dot3(v, v));
+            E[1] = E[1] + 0.5 * s->p.bond_k * ((d - s->p.bond_r0) * (d - s->p.bond_r0));
+        }
+    }
+    /* angle (src/energy_angl_ReB.F90:17-30, src/energy_angl.F90) */
+    {
+        int (*al)[3] = malloc(sizeof(int[3]) * (s->nmp + 2 * s->nchains));
+        int na = angle_list(s, al);
+        double co0 = cos(s->p.angl_t0);
+        for (int a = 0; a < na; a++) {
+            if (s->p.angl_reb) {
+                double v12[3], v32[3];
+                pbc_vec_d(s, F3(x, al[a][0]), F3(x, al[a][1]), v12);
+                pbc_vec_d(s, F3(x, al[a][2]), F3(x, al[a][1]), v32);
+                double co = dot3(v32, v12) / sqrt(dot3(v12, v12) * dot3(v32, v32));
+                E[2] = E[2] + 0.5 * s->p.angl_k * ((co - co0) * (co - co0)) / (1.0 - co * co);
+            } else {
+                double th = mp_angle(s, x, al[a][0], al[a][1], al[a][2]);
+                E[2] = E[2] + 0.5 * s->p.angl_k * ((th - s->p.angl_t0) * (th - s->p.angl_t0));
+            }
+        }
+        free(al);
+    }
+    /* dihedral (src/energy_dih_exp.F90:18-40, src/energy_dih_cos.F90) */
+    if (s->p.dih_type) {
+        int (*dl)[4] = malloc(sizeof(int[4]) * (s->nmp + 3 * s->nchains));
+        int nd = dih_list(s, dl);
+        double coef = -0.5 * s->p.dih_w;
+        for (int a = 0; a < nd; a++) {
+            double v12[3], v23[3], v34[3], c1[3], c2[3];
+            pbc_vec_d(s, F3(x, dl[a][1]), F3(x, dl[a][0]), v12);
+            pbc_vec_d(s, F3(x, dl[a][2]), F3(x, dl[a][1]), v23);
+            pbc_vec_d(s, F3(x, dl[a][3]), F3(x, dl[a][2]), v34);
+            cross(v12, v23, c1);
+            cross(v23, v34, c2);
+            double dih = atan2(sqrt(dot3(v23, v23)) * dot3(v12, c2), dot3(c1, c2));
+            if (s->p.dih_type == 2) E[3] = E[3] - s->p.dih_k * exp(coef * ((dih - s->p.dih_p0) * (dih - s->p.dih_p0)));
+            else E[3] = E[3] + s->p.dih_k * (1.0 + cos(dih + s->p.dih_p0));
+        }
+        free(dl);
+    }
+    E[4] = energy_bp(s, irep, tK, bp_mc, x);
+    /* WCA (src/energy_wca.F90:19-33) */
+    {
+        double e = 0.0;
+        for (int64_t k = 0; k < r->wca.n / 2; k++) {
+            double v[3];
+            pbc_vec_d(s, F3(x, r->wca.a[2 * k] - 1), F3(x, r->wca.a[2 * k + 1] - 1), v);
+            double d = sqrt(dot3(v, v));
+            if (d >= s->p.wca_sigma) continue;
+            d = s->p.wca_sigma / d;
+            double d6 = d * d * d * d * d * d;
+            e = e + s->p.wca_eps * (d6 * d6 - 2 * d6 + 1.0);
+        }
+        E[5] = e;
+    }
+    /* DH (src/energy_ele_DH.F90:19-42) */
+    if (s->p.ele_on) {
+        double e = 0.0, rc = 1.0 / r->lambdaD;
+        for (int64_t k = 0; k < r->ele.n / 2; k++) {
+            double v[3];
+            pbc_vec_d(s, F3(x, r->ele.a[2 * k] - 1), F3(x, r->ele.a[2 * k + 1] - 1), v);
+            double dist = sqrt(dot3(v, v));
+            if (dist > r->ele_cutoff) continue;
+            e = e + r->ele_coef / dist * exp(-dist * rc);
+        }
+        E[6] = e;
+    }
+    double tot = 0.0;
+    for (int k = 1; k <= 10; k++) tot += E[k];
+    E[0] = tot;
+}
+
+void orc_bp_state(orc_sys *s, int32_t irep, uint8_t *status, double *ene) {
+    rep_t *r = &s->rep[irep - 1];
+    int64_t nbp = r->bp.n / 7;
+    memcpy(status, r->bp_status, nbp);
+    memcpy(ene, r->ene_bp, nbp * sizeof(double));
+}
+
+/* ---------------- set up ---------------- */
+static double md_mass(int sq) {   /* src/job_md.F90:116-133 */
+    switch (sq) { case A_: return 328.212; case G_: return 344.212; case C_: return 304.182;
+                  case U_: return 305.164; default: return 320.443; }
+}
+
+orc_sys *orc_create(const sis_topo *topo, const sis_params *prm, int32_t nrep, const double *temps) {
+    JOUL2KCAL_MOL = (1.0 / KCAL2JOUL) * N_AVO;
+    BOLTZ_KCAL_MOL = BOLTZ_J * JOUL2KCAL_MOL;
+    orc_sys *s = calloc(1, sizeof *s);
+    s->p = *prm;
+    s->nmp = topo->nmp; s->nchains = topo->nchains; s->nrep = nrep;
+    s->nmp_chain = malloc(sizeof(int) * s->nchains);
+    s->moltype = malloc(sizeof(int) * s->nchains);
+    s->chain_start = malloc(sizeof(int) * s->nchains);
+    s->seq = malloc(sizeof(int) * s->nmp);
+    s->chain_of = malloc(sizeof(int) * s->nmp);
+    s->lmp = malloc(sizeof(int) * s->nmp);
+    s->has_charge = malloc(s->nmp);
+    int imp = 0;
+    for (int c = 0; c < s->nchains; c++) {
+        s->nmp_chain[c] = topo->nmp_chain[c];
+        s->moltype[c] = topo->moltype[c];
+        s->chain_start[c] = imp;
+        for (int i = 1; i <= s->nmp_chain[c]; i++, imp++) { s->chain_of[imp] = c; s->lmp[imp] = i; }
+    }
+    for (int i = 0; i < s->nmp; i++) {
+        s->seq[i] = topo->seq[i];
+        s->has_charge[i] = prm->dummy_has_charge ? 1 : (s->seq[i] != D_);   /* src/init_ele.F90:58-68 */
+    }
+    for (int k = 0; k < topo->n_no_charge; k++) s->has_charge[topo->no_charge[k] - 1] = 0;
+    /* base-pair parameter blocks and their mirrors (src/read_force_field.F90:203-207) */
+    for (int t = 0; t < 3; t++) {
+        sis_bp_para b = prm->bp[t], m = b;
+        m.angl_k1 = b.angl_k2; m.angl_k2 = b.angl_k1; m.angl_k3 = b.angl_k4; m.angl_k4 = b.angl_k3;
+        s->bpp[t + 1] = b; s->bpp[t + 4] = m;
+    }
+    init_bp_tables(s);
+    size_t n3 = (size_t)nrep * s->nmp * 3;
+    s->xyz = calloc(n3, sizeof(double)); s->velos = calloc(n3, sizeof(double));
+    s->accels = calloc(n3, sizeof(double)); s->move = calloc(n3, sizeof(double));
+    s->rep = calloc(nrep, sizeof(rep_t));
+    /* friction and masses (src/job_md.F90:94-133) */
+    s->mass = malloc(sizeof(double) * s->nmp);
+    s->fric = malloc(sizeof(double) * s->nmp);
+    double v = prm->viscosity_pas * sqrt(1.0e3 / KCAL2JOUL) * N_AVO * 1.0e-20;
+    for (int i = 0; i < s->nmp; i++) { s->mass[i] = md_mass(s->seq[i]); s->fric[i] = 6.0 * PI_ * v * 10.0; }
+    /* replicas: labels, values and the odd/even pair table of a
+     * one-dimensional temperature ladder (src/init_replica.F90:70-125,
+     * src/var_replica.F90:162-259) */
+    s->rep2lab = malloc(sizeof(int) * (nrep + 1));
+    s->lab2rep = malloc(sizeof(int) * (nrep + 1));
+    s->lab2val = malloc(sizeof(double) * (nrep + 1));
+    s->pair_tb = calloc(2 * (nrep + 2), sizeof(int));
+    for (int k = 1; k <= nrep; k++) { s->rep2lab[k] = k; s->lab2rep[k] = k; s->lab2val[k] = temps ? temps[k - 1] : prm->tempK; }
+    for (int par = 0; par < 2; par++) {
+        int *tb = s->pair_tb + par * (nrep + 2);
+        int iset = 0;
+        if (par == 1) tb[++iset] = 0;
+        for (int ir = (par == 0 ? 1 : 2); ir <= nrep - 1; ir += 2) { iset++; tb[iset] = iset + 1; iset++; tb[iset] = iset - 1; }
+        while (iset < nrep) tb[++iset] = 0;
+    }
+    s->idx_pair = 0;
+    for (int r = 0; r < nrep; r++) {
+        s->rep[r].tK = (nrep > 1) ? s->lab2val[s->rep2lab[r + 1]] : prm->tempK;
+        set_ele(s, r, s->rep[r].tK, prm->ionic_strength);
+    }
+    return s;
+}
+
+void orc_set_temps(orc_sys *s, const double *temps) {
+    for (int r = 0; r < s->nrep; r++) { s->rep[r].tK = temps[r]; set_ele(s, r, temps[r], s->p.ionic_strength); }
+}
+
+void orc_destroy(orc_sys *s) {
+    if (!s) return;
+    for (int r = 0; r < s->nrep; r++) {
+        free(s->rep[r].wca.a); free(s->rep[r].ele.a); free(s->rep[r].bp.a); free(s->rep[r].bpcoef.a);
+        free(s->rep[r].bp_status); free(s->rep[r].bp_status_mc); free(s->rep[r].ene_bp);
+    }
+    free(s->rep); free(s->nmp_chain); free(s->moltype); free(s->chain_start); free(s->seq);
+    free(s->chain_of); free(s->lmp); free(s->has_charge); free(s->xyz); free(s->velos);
+    free(s->accels); free(s->move); free(s->mass); free(s->fric); free(s->rep2lab);
+    free(s->lab2rep); free(s->lab2val); free(s->pair_tb);
+    free(s);
+}
+
+void orc_set_xyz(orc_sys *s, const double *xyz) { memcpy(s->xyz, xyz, sizeof(double) * 3 * s->nmp * s->nrep); }
+void orc_set_state(orc_sys *s, const double *xyz, const double *velos, const double *accels) {
+    size_t n = sizeof(double) * 3 * s->nmp * s->nrep;
+    if (xyz) memcpy(s->xyz, xyz, n);
+    if (velos) memcpy(s->velos, velos, n);
+    if (accels) memcpy(s->accels, accels, n);
+}
+void orc_get_state(orc_sys *s, double *xyz, double *velos, double *accels) {
+    size_t n = sizeof(double) * 3 * s->nmp * s->nrep;
+    if (xyz) memcpy(xyz, s->xyz, n);
+    if (velos) memcpy(velos, s->velos, n);
+    if (accels) memcpy(accels, s->accels, n);
+}
+void orc_tables(orc_sys *s, double *dh, double *ds, double *cut) {
+    memcpy(dh, s->bp3_dH + 1, 468 * sizeof(double));
+    memcpy(ds, s->bp3_dS + 1, 468 * sizeof(double));
+    cut[0] = s->bp_cutoff_dist;
+    for (int t = 1; t <= 6; t++) cut[t] = s->bpp[t].cutoff_ddist;
+}
+void orc_ele_params(orc_sys *s, int32_t irep, double *out) {
+    rep_t *r = &s->rep[irep - 1];
+    out[0] = r->lambdaD; out[1] = r->ele_coef; out[2] = r->ele_cutoff; out[3] = r->diele;
+}
+void orc_set_tape(orc_sys *s, int32_t irep, const uint32_t *words, int64_t n) {
+    tape_t *t = irep < 0 ? &s->tape_rep : &s->rep[irep - 1].tape;
+    t->w = words; t->n = n; t->pos = 0;
+}
+int64_t orc_tape_pos(orc_sys *s, int32_t irep) { return irep < 0 ? s->tape_rep.pos : s->rep[irep - 1].tape.pos; }
+void orc_boxmuller(orc_sys *s, int32_t irep, int64_t n, double *out) {
+    for (int64_t k = 0; k < n; k++) out[k] = boxmuller(s, irep - 1);
+}
+void orc_genrand(orc_sys *s, int32_t irep, int32_t kind, int64_t n, double *out) {
+    tape_t *t = irep < 0 ? &s->tape_rep : &s->rep[irep - 1].tape;
+    for (int64_t k = 0; k < n; k++) out[k] = kind == 1 ? gen_d1(t) : gen_d3(t);
+}
+void orc_labels(orc_sys *s, int32_t *rep2lab) { for (int k = 1; k <= s->nrep; k++) rep2lab[k - 1] = s->rep2lab[k]; }
+
+/* set_md_coef (src/job_md.F90:700-767): a, sqrt(b)h/m, sqrt(b)h, and the
+ * per-replica noise amplitude sqrt(b)/2m*sqrt(2 gamma kT h) */
+static void md_coef(orc_sys *s, int i, double tK, double *a, double *cf, double *cx, double *cr) {
+    double c1 = 0.5 * s->p.dt * s->fric[i] / s->mass[i];
+    double c2 = sqrt(1.0 / (1.0 + c1));
+    *a = (1.0 - c1) / (1.0 + c1);
+    *cf = c2 * s->p.dt / s->mass[i];
+    *cx = c2 * s->p.dt;
+    *cr = 0.5 * c2 / s->mass[i] * sqrt(2.0 * s->fric[i] * BOLTZ_KCAL_MOL * tK * s->p.dt);
+}
+void orc_md_coef(orc_sys *s, int32_t irep, double *coef4) {
+    for (int i = 0; i < s->nmp; i++)
+        md_coef(s, i, s->rep[irep - 1].tK, coef4 + 4 * i, coef4 + 4 * i + 1, coef4 + 4 * i + 2, coef4 + 4 * i + 3);
+}
+
+/* initial velocities and accelerations (src/job_md.F90:157-176) */
+void orc_init_velocities(orc_sys *s) {
+    for (int r = 0; r < s->nrep; r++) {
+        double tK = s->rep[r].tK;
+        double *v = s->velos + (size_t)r * s->nmp * 3, *ac = s->accels + (size_t)r * s->nmp * 3;
+        for (int i = 0; i < s->nmp; i++) {
+            double c1 = sqrt(tK * BOLTZ_KCAL_MOL / s->mass[i]);
+            for (int k = 0; k < 3; k++) v[3 * i + k] = c1 * boxmuller(s, r);
+        }
+        for (int i = 0; i < s->nmp; i++) {
+            double a, cf, cx, cr;
+            md_coef(s, i, tK, &a, &cf, &cx, &cr);
+            for (int k = 0; k < 3; k++) ac[3 * i + k] = cr * boxmuller(s, r);
+        }
+    }
+}
+
+static int get_pair(orc_sys *s, int lab) { return s->pair_tb[s->idx_pair * (s->nrep + 2) + lab]; }
+
+/* energy_replica + replica_exchange for a 1-D temperature ladder
+ * (src/energy_replica.F90:47-156, src/replica_exchange.F90:35-154,
+ * src/job_md.F90:536-567) */
+static void exchange(orc_sys *s, double *energies_out) {
+    int n = s->nrep;
+    double *E = calloc(2 * (n + 1), sizeof(double));
+    double e[11];
+    for (int r = 0; r < n; r++) {
+        orc_energy(s, r + 1, s->rep[r].tK, s->p.nstep_bp_mc > 0, 0, e);
+        if (energies_out) memcpy(energies_out + 11 * r, e, sizeof e);
+        E[2 * (r + 1)] = e[0];
+    }
+    /* energies at the partner's temperature; BP state saved/restored */
+    for (int r = 0; r < n; r++) {
+        rep_t *rr = &s->rep[r];
+        int64_t nbp = rr->bp.n / 7;
+        uint8_t *st = malloc(nbp + 1); double *en = malloc((nbp + 1) * sizeof(double));
+        memcpy(st, rr->bp_status, nbp + 1); memcpy(en, rr->ene_bp, (nbp + 1) * sizeof(double));
+        int lab = s->rep2lab[r + 1], opp = get_pair(s, lab);
+        if (opp == 0 || opp > n) E[2 * (r + 1) + 1] = 0.0;
+        else {
+            double tk_own = rr->tK, t_opp = s->lab2val[opp];
+            set_ele(s, r, t_opp, s->p.ionic_strength);
+            orc_energy(s, r + 1, t_opp, s->p.nstep_bp_mc > 0, 0, e);
+            E[2 * (r + 1) + 1] = e[0];
+            set_ele(s, r, tk_own, s->p.ionic_strength);
+        }
+        memcpy(rr->bp_status, st, nbp + 1); memcpy(rr->ene_bp, en, (nbp + 1) * sizeof(double));
+        free(st); free(en);
+    }
+    double *rnd = malloc(sizeof(double) * (n + 1));
+    int nr = 0;
+    for (int i = 1; i <= n; i += 2) rnd[++nr] = gen_d1(&s->tape_rep);
+    nr = 0;
+    for (int li = 1; li <= n; li++) {
+        int lj = get_pair(s, li);
+        if (lj < li) continue;
+        nr++;
+        int ri = s->lab2rep[li], rj = s->lab2rep[lj];
+        double tm = s->lab2val[li], tn = s->lab2val[lj];
+        double pim = E[2 * ri], pin = E[2 * ri + 1], pjn = E[2 * rj], pjm = E[2 * rj + 1];
+        double delta = ((pjm - pim) / tm - (pjn - pin) / tn) / BOLTZ_KCAL_MOL;
+        int acc = (delta < 0.0) || (rnd[nr] <= exp(-delta));
+        if (acc) {
+            s->rep2lab[ri] = lj; s->rep2lab[rj] = li;
+            s->lab2rep[li] = rj; s->lab2rep[lj] = ri;
+            double sf = sqrt(tn / tm);
+            double *vi = s->velos + (size_t)(ri - 1) * s->nmp * 3, *vj = s->velos + (size_t)(rj - 1) * s->nmp * 3;
+            for (int k = 0; k < 3 * s->nmp; k++) { vi[k] = sf * vi[k]; vj[k] = vj[k] / sf; }
+        }
+    }
+    s->idx_pair = (s->idx_pair + 1) % 2;
+    for (int r = 0; r < n; r++) { s->rep[r].tK = s->lab2val[s->rep2lab[r + 1]]; set_ele(s, r, s->rep[r].tK, s->p.ionic_strength); }
+    free(rnd); free(E);
+}
+
+/* job_md: initial lists + energies, then the time loop
+ * (src/job_md.F90:196-198,276-291,360-460).  Returns the number of list
+ * rebuilds; energies_out (nrep*11) gets the last energies. */
+int64_t orc_md(orc_sys *s, int64_t nsteps, int32_t nstep_exchange, double *energies_out) {
+    int n = s->nmp, nr = s->nrep;
+    int64_t rebuilds = 0;
+    for (int r = 1; r <= nr; r++) orc_neighbor_list(s, r);
+    memset(s->move, 0, sizeof(double) * 3 * n * nr);
+    double e[11];
+    for (int r = 1; r <= nr; r++) {
+        s->flg_bp_energy = 0;
+        orc_energy(s, r, s->rep[r - 1].tK, s->p.nstep_bp_mc > 0, 0, e);
+        if (energies_out) memcpy(energies_out + 11 * (r - 1), e, sizeof e);
+    }
+    for (int r = 0; r < nr; r++) {
+        rep_t *rr = &s->rep[r];
+        int64_t nbp = rr->bp.n / 7;
+        for (int64_t b = 0; b < nbp; b++) rr->bp_status_mc[b] = s->p.nstep_bp_mc > 0 ? rr->bp_status[b] : 1;
+    }
+    double *forces = malloc(sizeof(double) * 3 * n), *rnd = malloc(sizeof(double) * 3 * n);
+    int last_exchange = 0;
+    for (int64_t st = 0; st < nsteps; st++) {
+        s->istep++;
+        int bp_mc = 0;
+        if (s->p.nstep_bp_mc > 0 && s->istep % s->p.nstep_bp_mc == 0) bp_mc = 1;
+        for (int r = 0; r < nr; r++) {
+            double *x = s->xyz + (size_t)r * n * 3, *v = s->velos + (size_t)r * n * 3;
+            double *ac = s->accels + (size_t)r * n * 3, *mv = s->move + (size_t)r * n * 3;
+            double d2max = 0.0, d2max2 = 0.0;
+            for (int i = 0; i < n; i++) {
+                double d2 = dot3(mv + 3 * i, mv + 3 * i);
+                if (d2 > d2max) { d2max2 = d2max; d2max = d2; }
+                else if (d2 > d2max2) d2max2 = d2;
+            }
+            int mc = bp_mc;
+            if (sqrt(d2max) + sqrt(d2max2) > s->p.nl_margin) {
+                orc_neighbor_list(s, r + 1);
+                memset(mv, 0, sizeof(double) * 3 * n);
+                rebuilds++;
+                if (s->p.nstep_bp_mc > 0) mc = 1;
+            }
+            orc_force(s, r + 1, mc, forces);
+            if (mc) memcpy(s->rep[r].bp_status_mc, s->rep[r].bp_status, s->rep[r].bp.n / 7 + 1);
+            for (int i = 0; i < 3 * n; i++) rnd[i] = boxmuller(s, r);
+            for (int i = 0; i < n; i++) {
+                double a, cf, cx, cr;
+                md_coef(s, i, s->rep[r].tK, &a, &cf, &cx, &cr);
+                for (int k = 0; k < 3; k++) {
+                    double apre = cr * rnd[3 * i + k];
+                    v[3 * i + k] = a * v[3 * i + k] + cf * forces[3 * i + k] + (ac[3 * i + k] + apre);
+                    ac[3 * i + k] = apre;
+                    double dx = cx * v[3 * i + k];
+                    x[3 * i + k] = x[3 * i + k] + dx;
+                    mv[3 * i + k] = mv[3 * i + k] + dx;
+                }
+            }
+        }
+        last_exchange = 0;
+        if (nr > 1 && nstep_exchange > 0 && s->istep % nstep_exchange == 0) {
+            exchange(s, energies_out);
+            last_exchange = 1;
+        }
+    }
+    /* energies of the last step as written at a save step: the BP state of
+     * the last force call is reused (src/job_md.F90:474-481) */
+    if (nsteps > 0 && !last_exchange && energies_out)
+        for (int r = 1; r <= nr; r++) orc_energy(s, r, s->rep[r - 1].tK, 0, 1, energies_out + 11 * (r - 1));
+    free(forces); free(rnd);
+    return rebuilds;
+}
