@@ -671,6 +671,8 @@ static int bp_geom(orc_sys *s, const double *x, const int32_t *mp, double *u_out
 /* The limit Monte Carlo that removes excess base pairs per nucleotide,
  * shared by force_bp_limit_triplet (src/force_bp_limit_triplet.F90:259-330)
  * and energy_bp_limit_triplet (src/energy_bp_limit_triplet.F90:108-190). */
+static int64_t mc_iterations = 0;
+int64_t orc_mc_iterations(void) { return mc_iterations; }
 static void bp_limit_mc(orc_sys *s, int irep, int *nt_excess, double beta) {
     rep_t *r = &s->rep[irep];
     int64_t nbp = r->bp.n / 7;
@@ -681,6 +683,7 @@ static void bp_limit_mc(orc_sys *s, int irep, int *nt_excess, double beta) {
         int nnt = 0;
         for (int imp = 0; imp < s->nmp; imp++) if (nt_excess[imp] > 0) ntlist[++nnt] = imp + 1;
         if (nnt == 0) break;
+        mc_iterations++;
         double rnd = gen_d3(t);
         int nt_delete = ntlist[(int)ceil(rnd * nnt)];
         int nseq = 0;
@@ -814,8 +817,12 @@ static double energy_bp(orc_sys *s, int irep, double tK, int bp_mc, const double
         if (bp_mc) bp_limit_mc(s, irep, nt_excess, beta);
         free(nt_excess);
     }
+    /* energy_bp_limit_triplet sums formed pairs only
+     * (src/energy_bp_limit_triplet.F90:195); energy_bp_triplet sums ene_bp
+     * of every candidate (src/energy_bp_triplet.F90:88), which still holds
+     * the energies of pairs removed by an earlier limit MC */
     double e = 0.0;
-    for (int64_t b = 0; b < nbp; b++) if (r->bp_status[b]) e += r->ene_bp[b];
+    for (int64_t b = 0; b < nbp; b++) if (r->bp_status[b] || !bp_mc) e += r->ene_bp[b];
     return e;
 }
 
@@ -1192,8 +1199,10 @@ int64_t orc_md(orc_sys *s, int64_t nsteps, int32_t nstep_exchange, double *energ
     }
     /* energies of the last step as written at a save step: the BP state of
      * the last force call is reused (src/job_md.F90:474-481) */
-    if (nsteps > 0 && !last_exchange && energies_out)
-        for (int r = 1; r <= nr; r++) orc_energy(s, r, s->rep[r - 1].tK, 0, 1, energies_out + 11 * (r - 1));
+    if (nsteps > 0 && !last_exchange && energies_out) {
+        int mc_last = s->p.nstep_bp_mc > 0 && s->istep % s->p.nstep_bp_mc == 0;
+        for (int r = 1; r <= nr; r++) orc_energy(s, r, s->rep[r - 1].tK, mc_last, 1, energies_out + 11 * (r - 1));
+    }
     free(forces); free(rnd);
     return rebuilds;
 }

@@ -36,4 +36,5 @@ void orc_init_velocities(orc_sys *s);
 int64_t orc_md(orc_sys *s, int64_t nsteps, int32_t nstep_exchange, double *energies_out);
 void orc_labels(orc_sys *s, int32_t *rep2lab);
 void orc_set_temps(orc_sys *s, const double *temps);
+int64_t orc_mc_iterations(void);
 #endif

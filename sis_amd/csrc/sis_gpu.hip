@@ -1,0 +1,1192 @@
+// sis_gpu.hip - host side of the MI355X engine: context set-up, list
+// management, the per-phase step loop and the C ABI of include/sis_gpu.h.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/sis_gpu.h"
+#include "sis_device.h"
+#include "sis_kernels.h"
+#include "sis_persistent.h"
+
+using namespace sis;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string &msg) {
+  g_err = msg;
+  return code;
+}
+
+#define HIPCHK(expr)                                                                        \
+  do {                                                                                      \
+    hipError_t e_ = (expr);                                                                 \
+    if (e_ != hipSuccess)                                                                   \
+      throw HipError(std::string(#expr) + ": " + hipGetErrorString(e_));                    \
+  } while (0)
+
+struct HipError {
+  std::string msg;
+  explicit HipError(std::string m) : msg(std::move(m)) {}
+};
+struct ApiError {
+  int code;
+  std::string msg;
+};
+
+const double KB_KCAL = 1.380649e-23 * ((1.0 / 4184.0) * 6.02214076e23);  // src/const_phys.F90:10-14
+const double PI_D = 3.141592653589793115997963468544185161590576171875;
+
+template <typename T>
+struct DevBuf {
+  T *p = nullptr;
+  size_t n = 0;
+  void alloc(size_t count) {
+    if (p) HIPCHK(hipFree(p));
+    p = nullptr;
+    n = count;
+    if (count) {
+      HIPCHK(hipMalloc(&p, count * sizeof(T)));
+      HIPCHK(hipMemset(p, 0, count * sizeof(T)));
+    }
+  }
+  void free_() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+};
+
+}  // namespace
+
+struct sis_gpu_ctx {
+  Params P{};
+  Ptrs D{};
+  sis_options opt{};
+  sis_params prm{};
+  int N = 0, R = 0;
+  int parity = 0, precision = 0, mode = 1;
+  uint64_t seed = 0;
+  int64_t istep = 0, rebuilds = 0;
+  bool lists_valid = false;
+  bool use_cells = false;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  double last_ms[4] = {0, 0, 0, 0};
+  // host copies
+  std::vector<Topo> topo;
+  std::vector<double> tK, ionic, mass, fric;
+  std::vector<int> rep2lab, lab2rep;
+  std::vector<double> lab2val;
+  int idx_pair = 0;
+  std::vector<int> pair_tb;  // [2][R+2]
+  std::vector<uint32_t> rep_tape;
+  int64_t rep_tape_pos = 0;
+  std::vector<int64_t> tape_off_h;
+  // device buffers
+  DevBuf<Topo> topo_d;
+  DevBuf<int> bp3_hash_d;
+  DevBuf<double> bp3_dH_d, bp3_dS_d, md_a_d, md_cf_d, md_cx_d, mass_d, md_cr_d;
+  DevBuf<double> tK_d, ele_coef_d, lambdaD_d, ele_cut_d, box_d;
+  DevBuf<double4> pos_d, vel_d, acc_d, move_d, frc_d;
+  DevBuf<int> nb_ele_d, n_ele_d, nb_wca_d, n_wca_d, bp_tmp_d, n_bp_tmp_d, bp_off_d, bp_mp_d,
+      n_bp_d, inc_d, n_inc_d, nt_cnt_d, comp_d, flags_d;
+  DevBuf<double> bp_coef_d, ene_bp_d, for_bp_d;
+  DevBuf<uint8_t> bp_status_d, bp_status_mc_d;
+  DevBuf<int> cell_of_d, cell_cnt_d, cell_start_d, cell_beads_d, cell_fill_d;
+  DevBuf<double> blk_top2_d;
+  DevBuf<int> rebuild_flag_d, mask_d, mc_d, scan_bsum_d, scan_total_d;
+  DevBuf<int> exc_cnt_d, exc_off_d, exc_fill_d, exc_list_d, ntlist_d, bpseq_d;
+  DevBuf<uint32_t> tape_d;
+  DevBuf<int64_t> tape_off_d, tape_pos_d;
+  DevBuf<double> noise_d, bm_cache_d, ene_part_d, ene_out_d, ecT_d, eCoef_d, eLam_d;
+  DevBuf<int> pers_sync_d;
+  int nbp_total = 0;
+  int nblk_force = 0;
+  int block = 256;
+
+  Ptrs ptrs() {
+    Ptrs d{};
+    d.topo = topo_d.p;
+    d.bp3_hash = bp3_hash_d.p;
+    d.bp3_dH = bp3_dH_d.p;
+    d.bp3_dS = bp3_dS_d.p;
+    d.md_a = md_a_d.p; d.md_cf = md_cf_d.p; d.md_cx = md_cx_d.p; d.mass = mass_d.p;
+    d.md_cr = md_cr_d.p;
+    d.tK = tK_d.p; d.ele_coef = ele_coef_d.p; d.lambdaD = lambdaD_d.p; d.ele_cut = ele_cut_d.p;
+    d.box = box_d.p;
+    d.pos = pos_d.p; d.vel = vel_d.p; d.acc = acc_d.p; d.move = move_d.p; d.frc = frc_d.p;
+    d.maxnb = D.maxnb; d.maxw = D.maxw; d.maxbp = D.maxbp; d.maxinc = D.maxinc; d.maxbpb = D.maxbpb;
+    d.nb_ele = nb_ele_d.p; d.n_ele = n_ele_d.p; d.nb_wca = nb_wca_d.p; d.n_wca = n_wca_d.p;
+    d.bp_tmp = bp_tmp_d.p; d.n_bp_tmp = n_bp_tmp_d.p; d.bp_off = bp_off_d.p;
+    d.bp_mp = bp_mp_d.p; d.bp_coef = bp_coef_d.p; d.n_bp = n_bp_d.p;
+    d.inc = inc_d.p; d.n_inc = n_inc_d.p;
+    d.bp_status = bp_status_d.p; d.bp_status_mc = bp_status_mc_d.p;
+    d.ene_bp = ene_bp_d.p; d.for_bp = for_bp_d.p; d.nt_cnt = nt_cnt_d.p; d.comp = comp_d.p;
+    d.flags = flags_d.p;
+    d.cell_of = cell_of_d.p; d.cell_cnt = cell_cnt_d.p; d.cell_start = cell_start_d.p;
+    d.cell_beads = cell_beads_d.p;
+    d.ncell_x = D.ncell_x; d.ncell_y = D.ncell_y; d.ncell_z = D.ncell_z; d.ncell = D.ncell;
+    d.blk_top2 = blk_top2_d.p; d.rebuild_flag = rebuild_flag_d.p;
+    d.tape = tape_d.p; d.tape_off = tape_off_d.p; d.tape_pos = tape_pos_d.p;
+    d.noise = noise_d.p; d.bm_cache = bm_cache_d.p;
+    d.ene_acc = ene_out_d.p;
+    return d;
+  }
+  ~sis_gpu_ctx() {
+    if (stream) (void)hipStreamSynchronize(stream);
+    DevBuf<int> *ib[] = {&bp3_hash_d, &nb_ele_d, &n_ele_d, &nb_wca_d, &n_wca_d, &bp_tmp_d,
+                         &n_bp_tmp_d, &bp_off_d, &bp_mp_d, &n_bp_d, &inc_d, &n_inc_d, &nt_cnt_d,
+                         &comp_d, &flags_d, &cell_of_d, &cell_cnt_d, &cell_start_d,
+                         &cell_beads_d, &cell_fill_d, &rebuild_flag_d, &mask_d, &mc_d,
+                         &scan_bsum_d, &scan_total_d, &exc_cnt_d, &exc_off_d, &exc_fill_d,
+                         &exc_list_d, &ntlist_d, &bpseq_d, &pers_sync_d};
+    for (auto *b : ib) b->free_();
+    DevBuf<double> *db[] = {&bp3_dH_d, &bp3_dS_d, &md_a_d, &md_cf_d, &md_cx_d, &mass_d,
+                            &md_cr_d, &tK_d, &ele_coef_d, &lambdaD_d, &ele_cut_d, &box_d,
+                            &bp_coef_d, &ene_bp_d, &for_bp_d, &blk_top2_d, &noise_d,
+                            &bm_cache_d, &ene_part_d, &ene_out_d, &ecT_d, &eCoef_d, &eLam_d};
+    for (auto *b : db) b->free_();
+    DevBuf<double4> *vb[] = {&pos_d, &vel_d, &acc_d, &move_d, &frc_d};
+    for (auto *b : vb) b->free_();
+    topo_d.free_();
+    bp_status_d.free_();
+    bp_status_mc_d.free_();
+    tape_d.free_();
+    tape_off_d.free_();
+    tape_pos_d.free_();
+    if (ev0) (void)hipEventDestroy(ev0);
+    if (ev1) (void)hipEventDestroy(ev1);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+};
+
+namespace {
+
+// ---------------------------------------------------------------------
+// Host-side set-up restating init_bp's triplet table
+// (src/init_bp.F90:39-172) and BP cutoffs (src/init_bp.F90:321-346).
+bool host_comp(int a, int b) {
+  return (a == SQ_A && b == SQ_U) || (a == SQ_U && (b == SQ_A || b == SQ_G)) ||
+         (a == SQ_G && (b == SQ_C || b == SQ_U)) || (a == SQ_C && b == SQ_G);
+}
+int host_hkey(int w, int x, int u, int z, int y, int v) {
+  if (x < y) return 1 + w + x * 4 + u * 16 + z * 64 + y * 256 + v * 1024;
+  return 1 + v + y * 4 + z * 16 + u * 64 + x * 256 + w * 1024;
+}
+// NN stack index (1..21) of 5'-w x-3' / 3'-z y-5' (src/const_idx.F90:230-330)
+int nn_index(int w, int x, int z, int y) {
+  static const char ch[5] = {'A', 'U', 'G', 'C', 'D'};
+  static const char *tbl[21][2] = {
+      {"GC_CG", nullptr}, {"CC_GG", "GG_CC"}, {"GA_CU", "UC_AG"}, {"CG_GC", nullptr},
+      {"AC_UG", "GU_CA"}, {"CA_GU", "UG_AC"}, {"AG_UC", "CU_GA"}, {"UA_AU", nullptr},
+      {"AU_UA", nullptr}, {"AA_UU", "UU_AA"}, {"GC_UG", "GU_CG"}, {"CU_GG", "GG_UC"},
+      {"GG_CU", "UC_GG"}, {"CG_GU", "UG_GC"}, {"AU_UG", "GU_UA"}, {"GA_UU", "UU_AG"},
+      {"UG_GU", nullptr}, {"UA_GU", "UG_AU"}, {"GG_UU", "UU_GG"}, {"GU_UG", nullptr},
+      {"AG_UU", "UU_GA"}};
+  char k[6] = {ch[w], ch[x], '_', ch[z], ch[y], 0};
+  for (int i = 0; i < 21; i++)
+    for (int a = 0; a < 2; a++)
+      if (tbl[i][a] && std::strcmp(tbl[i][a], k) == 0) return i + 1;
+  return 0;
+}
+void build_triplet_table(const sis_params &p, std::vector<int> &hash, std::vector<double> &dH,
+                         std::vector<double> &dS) {
+  hash.assign(4097, 0);
+  dH.assign(469, 0.0);
+  dS.assign(469, 0.0);
+  auto endc = [&](int h, int cx, int cy, int ox, int oy) {
+    bool au = (cx == SQ_A && cy == SQ_U) || (cx == SQ_U && cy == SQ_A);
+    bool gu = (cx == SQ_G && cy == SQ_U) || (cx == SQ_U && cy == SQ_G);
+    bool oau = (ox == SQ_A && oy == SQ_U) || (ox == SQ_U && oy == SQ_A);
+    bool ogc = (ox == SQ_G && oy == SQ_C) || (ox == SQ_C && oy == SQ_G);
+    bool ogu = (ox == SQ_G && oy == SQ_U) || (ox == SQ_U && oy == SQ_G);
+    int idx = 0;
+    if (au) idx = oau ? 1 : ogc ? 2 : ogu ? 3 : 0;
+    else if (gu) idx = oau ? 5 : ogc ? 4 : ogu ? 6 : 0;
+    if (idx) {
+      dH[h] = dH[h] + p.nnend_dH[idx - 1] - p.bp_dHend0;
+      dS[h] = dS[h] + (p.nnend_dS[idx - 1] - p.bp_dSend0) * 1.0e-3;
+    }
+  };
+  int h = 0;
+  for (int x = 0; x <= 2; x++)
+    for (int y = x + 1; y <= 3; y++) {
+      if (!host_comp(x, y)) continue;
+      for (int w = 0; w <= 3; w++)
+        for (int z = 0; z <= 3; z++) {
+          bool cwz = host_comp(w, z);
+          double h1 = 0.0, s1 = 0.0;
+          if (cwz) {
+            int q = nn_index(w, x, z, y);
+            h1 = 0.5 * (p.nn_dH[q - 1] - p.bp_dH0);
+            s1 = 0.5 * (p.nn_dS[q - 1] - p.bp_dS0);
+          }
+          for (int u = 0; u <= 3; u++)
+            for (int v = 0; v <= 3; v++) {
+              bool cuv = host_comp(u, v);
+              if (!cwz && !cuv) continue;
+              h++;
+              hash[host_hkey(w, x, u, z, y, v)] = h;
+              if (cuv) {
+                int q = nn_index(x, u, y, v);
+                dH[h] = h1 + 0.5 * (p.nn_dH[q - 1] - p.bp_dH0);
+                dS[h] = (s1 + 0.5 * (p.nn_dS[q - 1] - p.bp_dS0)) * 1.0e-3;
+              } else {
+                dH[h] = h1;
+                dS[h] = s1 * 1.0e-3;
+              }
+              if (p.nnend) {
+                if (cwz && !cuv) endc(h, x, y, w, z);
+                else if (!cwz && cuv) endc(h, x, y, u, v);
+              }
+            }
+        }
+    }
+}
+
+// set_ele (src/set_ele.F90:45-99)
+void host_set_ele(const sis_params &p, double T, double I, double &coef, double &lam, double &cut) {
+  const double MM_A = 87.740, MM_B = -0.40008, MM_C = 9.398e-4, MM_D = -1.410e-6;
+  const double EPS0 = 8.8541878128e-12, ELE = 1.602176634e-19, BOLTZ_J = 1.380649e-23,
+               N_AVO = 6.02214076e23;
+  const double J2K = (1.0 / 4184.0) * N_AVO;
+  double Tc = T - 273.15;
+  double diele = MM_A + MM_B * Tc + MM_C * Tc * Tc + MM_D * Tc * Tc * Tc;
+  double lb = ELE * ELE / (4.0 * PI_D * EPS0 * diele * BOLTZ_J * T) * 1.0e10;
+  double Zp = p.length_per_charge / lb;
+  coef = J2K * 1.0e10 * (ELE * ELE) / (4.0 * PI_D * EPS0 * diele);
+  coef = coef * (Zp * Zp);
+  lam = 1.0e10 * std::sqrt((1.0e-3 * EPS0 * diele * BOLTZ_J) / (2.0 * N_AVO * (ELE * ELE))) *
+        std::sqrt(T / I);
+  cut = (p.ele_cutoff_type == 1) ? p.ele_cutoff : p.ele_cutoff * lam;
+}
+
+double md_mass(int sq) {  // src/job_md.F90:116-133
+  switch (sq) {
+    case SQ_A: return 328.212;
+    case SQ_G: return 344.212;
+    case SQ_C: return 304.182;
+    case SQ_U: return 305.164;
+    default: return 320.443;
+  }
+}
+
+inline dim3 grid_beads(int N, int R, int B) { return dim3((N + B - 1) / B, R); }
+
+void scan_exclusive(sis_gpu_ctx *c, const int *in, int *out, int64_t n, int *total_dev) {
+  const int B = 1024;
+  int64_t nb = (n + B - 1) / B;
+  if ((int64_t)c->scan_bsum_d.n < nb + 1) c->scan_bsum_d.alloc(nb + 1);
+  hipLaunchKernelGGL(k_scan_blocks<B>, dim3(nb), dim3(B), 0, c->stream, in, out, c->scan_bsum_d.p, n);
+  hipLaunchKernelGGL(k_scan_bsum, dim3(1), dim3(1), 0, c->stream, c->scan_bsum_d.p, (int)nb,
+                     c->scan_total_d.p, out + n);
+  hipLaunchKernelGGL(k_scan_add<B>, dim3(nb), dim3(B), 0, c->stream, out, c->scan_bsum_d.p, n);
+  (void)total_dev;
+}
+
+void alloc_lists(sis_gpu_ctx *c) {
+  size_t RN = (size_t)c->R * c->N;
+  c->nb_ele_d.alloc(RN * c->D.maxnb);
+  c->nb_wca_d.alloc(RN * c->D.maxw);
+  c->bp_tmp_d.alloc(RN * c->D.maxbpb);
+  c->inc_d.alloc(RN * c->D.maxinc);
+  c->bp_mp_d.alloc((size_t)c->D.maxbp * 8);
+  c->bp_coef_d.alloc((size_t)c->D.maxbp * 2);
+  c->bp_status_d.alloc(c->D.maxbp);
+  c->bp_status_mc_d.alloc(c->D.maxbp);
+  c->ene_bp_d.alloc(c->D.maxbp);
+  c->for_bp_d.alloc((size_t)c->D.maxbp * 18);
+  c->bpseq_d.alloc((size_t)c->D.maxbp + c->R + 2);
+}
+
+void setup_cells(sis_gpu_ctx *c) {
+  c->use_cells = false;
+  if (!c->prm.pbc) return;
+  double rcut = std::max({c->P.wca_sigma, c->P.bp_cutoff_dist, 0.0}) + c->P.nl_margin;
+  for (int r = 0; r < c->R; r++) {
+    double ec = 0.0, lam, coef;
+    host_set_ele(c->prm, c->tK[r], c->ionic[r], coef, lam, ec);
+    if (c->prm.ele_on) rcut = std::max(rcut, ec + c->P.nl_margin);
+  }
+  int nc[3];
+  for (int k = 0; k < 3; k++) nc[k] = (int)std::floor(c->prm.box[k] / rcut);
+  if (nc[0] < 3 || nc[1] < 3 || nc[2] < 3 || c->N < 1024) return;
+  c->use_cells = true;
+  c->D.ncell_x = nc[0];
+  c->D.ncell_y = nc[1];
+  c->D.ncell_z = nc[2];
+  c->D.ncell = nc[0] * nc[1] * nc[2];
+  size_t RC = (size_t)c->R * c->D.ncell;
+  c->cell_of_d.alloc((size_t)c->R * c->N);
+  c->cell_cnt_d.alloc(RC);
+  c->cell_start_d.alloc(RC + 1);
+  c->cell_fill_d.alloc(RC);
+  c->cell_beads_d.alloc((size_t)c->R * c->N);
+}
+
+// Per-replica conditions: set_ele and set_md_coef (src/job_md.F90:700-767).
+void upload_conditions(sis_gpu_ctx *c) {
+  int R = c->R, N = c->N;
+  std::vector<double> coef(R), lam(R), cut(R), cr((size_t)R * N), box(3 * R);
+  for (int r = 0; r < R; r++) {
+    host_set_ele(c->prm, c->tK[r], c->ionic[r], coef[r], lam[r], cut[r]);
+    for (int i = 0; i < N; i++) {
+      double c1 = 0.5 * c->prm.dt * c->fric[i] / c->mass[i];
+      double c2 = std::sqrt(1.0 / (1.0 + c1));
+      cr[(size_t)r * N + i] =
+          0.5 * c2 / c->mass[i] * std::sqrt(2.0 * c->fric[i] * KB_KCAL * c->tK[r] * c->prm.dt);
+    }
+    for (int k = 0; k < 3; k++) box[3 * r + k] = c->prm.box[k];
+  }
+  HIPCHK(hipMemcpyAsync(c->tK_d.p, c->tK.data(), R * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(c->ele_coef_d.p, coef.data(), R * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(c->lambdaD_d.p, lam.data(), R * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(c->ele_cut_d.p, cut.data(), R * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(c->md_cr_d.p, cr.data(), cr.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(c->box_d.p, box.data(), box.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+}
+
+void set_mask(sis_gpu_ctx *c, const std::vector<int> &m) {
+  HIPCHK(hipMemcpyAsync(c->mask_d.p, m.data(), m.size() * sizeof(int), hipMemcpyHostToDevice, c->stream));
+}
+void set_mc(sis_gpu_ctx *c, const std::vector<int> &m) {
+  HIPCHK(hipMemcpyAsync(c->mc_d.p, m.data(), m.size() * sizeof(int), hipMemcpyHostToDevice, c->stream));
+}
+
+// neighbor_list for the replicas in `mask` (src/neighbor_list.F90).
+void rebuild(sis_gpu_ctx *c, const std::vector<int> &mask) {
+  const int N = c->N, R = c->R, B = 256;
+  set_mask(c, mask);
+  for (int attempt = 0; attempt < 8; attempt++) {
+    HIPCHK(hipMemsetAsync(c->flags_d.p, 0, 8 * sizeof(int), c->stream));
+    Ptrs D = c->ptrs();
+    hipLaunchKernelGGL(k_wrap, grid_beads(N, R, B), dim3(B), 0, c->stream, c->P, D, c->mask_d.p);
+    if (c->use_cells) {
+      size_t RC = (size_t)R * D.ncell;
+      HIPCHK(hipMemsetAsync(c->cell_cnt_d.p, 0, RC * sizeof(int), c->stream));
+      HIPCHK(hipMemsetAsync(c->cell_fill_d.p, 0, RC * sizeof(int), c->stream));
+      hipLaunchKernelGGL(k_cell_count, grid_beads(N, R, B), dim3(B), 0, c->stream, c->P, D, c->mask_d.p);
+      scan_exclusive(c, c->cell_cnt_d.p, c->cell_start_d.p, (int64_t)RC, nullptr);
+      hipLaunchKernelGGL(k_cell_fill, grid_beads(N, R, B), dim3(B), 0, c->stream, c->P, D, c->mask_d.p,
+                         c->cell_fill_d.p);
+      hipLaunchKernelGGL(k_cell_sort, dim3((RC + B - 1) / B), dim3(B), 0, c->stream, c->P, D, c->mask_d.p);
+      hipLaunchKernelGGL(k_search_cell, grid_beads(N, R, 128), dim3(128), 0, c->stream, c->P, D, c->mask_d.p);
+    } else {
+      hipLaunchKernelGGL(k_search_brute<128>, grid_beads(N, R, 128), dim3(128), 0, c->stream, c->P, D,
+                         c->mask_d.p);
+    }
+    // BP candidate list in reference order over all replicas
+    scan_exclusive(c, c->n_bp_tmp_d.p, c->bp_off_d.p, (int64_t)R * N, nullptr);
+    HIPCHK(hipMemsetAsync(c->n_inc_d.p, 0, (size_t)R * N * sizeof(int), c->stream));
+    hipLaunchKernelGGL(k_bp_fill, grid_beads(N, R, B), dim3(B), 0, c->stream, c->P, D, nullptr);
+    hipLaunchKernelGGL(k_inc_sort, grid_beads(N, R, B), dim3(B), 0, c->stream, c->P, D, nullptr);
+    int hflags[8];
+    int total = 0;
+    HIPCHK(hipMemcpyAsync(hflags, c->flags_d.p, sizeof hflags, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpyAsync(&total, c->bp_off_d.p + (size_t)R * N, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    bool grow = false;
+    if (hflags[2] > c->D.maxnb) { c->D.maxnb = hflags[2] + hflags[2] / 4 + 8; grow = true; }
+    if (hflags[3] > c->D.maxw) { c->D.maxw = hflags[3] + hflags[3] / 4 + 4; grow = true; }
+    if (hflags[4] > c->D.maxbpb) { c->D.maxbpb = hflags[4] + hflags[4] / 4 + 4; grow = true; }
+    if (hflags[6] > c->D.maxinc) { c->D.maxinc = hflags[6] + hflags[6] / 4 + 8; grow = true; }
+    if (total > c->D.maxbp) { c->D.maxbp = total + total / 4 + 64; grow = true; }
+    if (!grow && hflags[0]) {  // overflow reported without sizes: double everything
+      c->D.maxnb *= 2; c->D.maxw *= 2; c->D.maxbpb *= 2; c->D.maxinc *= 2; c->D.maxbp *= 2;
+      grow = true;
+    }
+    if (!grow) {
+      c->nbp_total = total;
+      hipLaunchKernelGGL(k_clear_move, grid_beads(N, R, B), dim3(B), 0, c->stream, c->P, c->ptrs(),
+                         c->mask_d.p);
+      c->lists_valid = true;
+      return;
+    }
+    // lists of every replica are rebuilt after a resize
+    std::vector<int> all(R, 1);
+    set_mask(c, all);
+    alloc_lists(c);
+  }
+  throw ApiError{SIS_ERR_CAPACITY, "neighbour lists did not converge in size"};
+}
+
+// Base-pair evaluation + limit MC (force_bp_limit_triplet / force_bp_triplet).
+void bp_phase(sis_gpu_ctx *c, const std::vector<int> &mc, const double *ecT_dev, const int *mask_dev) {
+  const int N = c->N, R = c->R, B = 256;
+  set_mc(c, mc);
+  Ptrs D = c->ptrs();
+  hipLaunchKernelGGL(k_reset_cnt, grid_beads(N, R, B), dim3(B), 0, c->stream, c->P, D, mask_dev);
+  if (c->nbp_total > 0) {
+    if (c->precision == 1)
+      hipLaunchKernelGGL(k_bp_eval<float>, dim3((c->nbp_total + B - 1) / B), dim3(B), 0, c->stream, c->P, D,
+                         c->nbp_total, c->mc_d.p, ecT_dev, mask_dev);
+    else
+      hipLaunchKernelGGL(k_bp_eval<double>, dim3((c->nbp_total + B - 1) / B), dim3(B), 0, c->stream, c->P, D,
+                         c->nbp_total, c->mc_d.p, ecT_dev, mask_dev);
+  }
+  bool any_mc = false;
+  for (int r = 0; r < R; r++) any_mc |= mc[r] != 0;
+  if (!any_mc || c->P.nstep_bp_mc <= 0) return;
+  if (c->parity) {
+    hipLaunchKernelGGL(k_mc_serial, dim3(R), dim3(64), 0, c->stream, c->P, D, c->mc_d.p, ecT_dev,
+                       c->ntlist_d.p, c->bpseq_d.p, mask_dev);
+  } else {
+    if (c->nbp_total == 0) return;
+    hipLaunchKernelGGL(k_uf_init, grid_beads(N, R, B), dim3(B), 0, c->stream, c->P, D, c->exc_cnt_d.p,
+                       c->exc_fill_d.p, mask_dev);
+    hipLaunchKernelGGL(k_uf_union, dim3((c->nbp_total + B - 1) / B), dim3(B), 0, c->stream, c->P, D,
+                       c->nbp_total, c->mc_d.p, mask_dev);
+    hipLaunchKernelGGL(k_uf_count, grid_beads(N, R, B), dim3(B), 0, c->stream, c->P, D, c->exc_cnt_d.p,
+                       c->mc_d.p, mask_dev);
+    scan_exclusive(c, c->exc_cnt_d.p, c->exc_off_d.p, (int64_t)R * N, nullptr);
+    hipLaunchKernelGGL(k_uf_fill, grid_beads(N, R, B), dim3(B), 0, c->stream, c->P, D, c->exc_off_d.p,
+                       c->exc_fill_d.p, c->exc_list_d.p, c->mc_d.p, mask_dev);
+    hipLaunchKernelGGL(k_mc_comp, grid_beads(N, R, 64), dim3(64), 0, c->stream, c->P, D, c->exc_off_d.p,
+                       c->exc_cnt_d.p, c->exc_list_d.p, c->mc_d.p, ecT_dev, c->seed, c->istep, mask_dev);
+  }
+}
+
+void force_kernel(sis_gpu_ctx *c, int integrate, const int *mask_dev) {
+  const int N = c->N, R = c->R;
+  const int B = 256;
+  dim3 g = grid_beads(N, R, B);
+  c->nblk_force = g.x;
+  if (c->precision == 1)
+    hipLaunchKernelGGL((k_force_int<float, B>), g, dim3(B), 0, c->stream, c->P, c->ptrs(), integrate,
+                       c->parity, c->seed, c->istep, mask_dev);
+  else
+    hipLaunchKernelGGL((k_force_int<double, B>), g, dim3(B), 0, c->stream, c->P, c->ptrs(), integrate,
+                       c->parity, c->seed, c->istep, mask_dev);
+}
+
+void check_flags(sis_gpu_ctx *c) {
+  int f[8];
+  HIPCHK(hipMemcpyAsync(f, c->flags_d.p, sizeof f, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  if (f[1] == 2) throw ApiError{SIS_ERR_STATE, "limit MC found an excess nucleotide without pairs"};
+  if (f[1]) throw ApiError{SIS_ERR_STATE, "parity random-number tape exhausted"};
+}
+
+// energy_sumup for replica mask at conditions (ecT, coef, lam per replica)
+void energy_phase(sis_gpu_ctx *c, const std::vector<int> &mask, const std::vector<double> &T,
+                  int flags, double *out /* R*16 */) {
+  const int N = c->N, R = c->R;
+  const int B = 256;
+  std::vector<double> coef(R), lam(R);
+  for (int r = 0; r < R; r++) {
+    double cut;
+    host_set_ele(c->prm, T[r], c->ionic[r], coef[r], lam[r], cut);
+  }
+  HIPCHK(hipMemcpyAsync(c->ecT_d.p, T.data(), R * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(c->eCoef_d.p, coef.data(), R * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(c->eLam_d.p, lam.data(), R * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  set_mask(c, mask);
+  if (!(flags & 2)) {
+    std::vector<int> mc(R, 0);
+    for (int r = 0; r < R; r++) mc[r] = (flags & 1) && mask[r];
+    bp_phase(c, mc, c->ecT_d.p, c->mask_d.p);
+  }
+  dim3 g = grid_beads(N, R, B);
+  if ((int64_t)c->ene_part_d.n < (int64_t)R * g.x * 8) c->ene_part_d.alloc((size_t)R * g.x * 8);
+  hipLaunchKernelGGL(k_energy<B>, g, dim3(B), 0, c->stream, c->P, c->ptrs(), c->ecT_d.p, c->eCoef_d.p,
+                     c->eLam_d.p, c->ene_part_d.p, c->mask_d.p, flags & 1);
+  hipLaunchKernelGGL(k_energy_reduce, dim3(R), dim3(1), 0, c->stream, c->ene_part_d.p, (int)g.x, c->ene_out_d.p);
+  HIPCHK(hipMemcpyAsync(out, c->ene_out_d.p, (size_t)R * 16 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  check_flags(c);
+}
+
+int get_pair(sis_gpu_ctx *c, int lab) { return c->pair_tb[c->idx_pair * (c->R + 2) + lab]; }
+
+double rep_uniform(sis_gpu_ctx *c, uint64_t k) {
+  if (c->parity) {
+    if (c->rep_tape_pos + 2 > (int64_t)c->rep_tape.size())
+      throw ApiError{SIS_ERR_STATE, "parity exchange tape exhausted"};
+    uint32_t w1 = c->rep_tape[c->rep_tape_pos], w2 = c->rep_tape[c->rep_tape_pos + 1];
+    c->rep_tape_pos += 2;
+    double a = (double)(w1 >> 5), b = (double)(w2 >> 6);
+    return (a * 67108864.0 + b) * (1.0 / 9007199254740991.0);
+  }
+  U4 o = philox(U4{(uint32_t)c->istep, (uint32_t)(c->istep >> 32), (uint32_t)k, 0x72657863u},
+                (uint32_t)c->seed, (uint32_t)(c->seed >> 32) ^ 0x1234567u);
+  uint64_t m = (((uint64_t)o.x << 21) ^ (uint64_t)(o.y >> 11)) & ((1ull << 53) - 1);
+  return ((double)m + 0.5) * (1.0 / 9007199254740992.0);
+}
+
+// energy_replica + replica_exchange (src/energy_replica.F90:47-156,
+// src/replica_exchange.F90:35-154) for a one-dimensional temperature ladder.
+void do_exchange(sis_gpu_ctx *c, double *replica_energies, int32_t *accepted) {
+  const int R = c->R;
+  std::vector<int> all(R, 1);
+  std::vector<double> e1((size_t)R * 16), e2((size_t)R * 16);
+  int mcflag = c->P.nstep_bp_mc > 0 ? 1 : 0;
+  energy_phase(c, all, c->tK, mcflag, e1.data());
+  // counterpart conditions; the BP state of the own-condition pass is kept
+  std::vector<double> Topp(R);
+  std::vector<int> has(R, 0);
+  for (int r = 0; r < R; r++) {
+    int lab = c->rep2lab[r + 1], opp = get_pair(c, lab);
+    if (opp > 0 && opp <= R) { Topp[r] = c->lab2val[opp]; has[r] = 1; }
+    else Topp[r] = c->tK[r];
+  }
+  size_t nb = c->nbp_total;
+  std::vector<uint8_t> st(nb);
+  std::vector<double> en(nb);
+  if (nb) {
+    HIPCHK(hipMemcpyAsync(st.data(), c->bp_status_d.p, nb, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpyAsync(en.data(), c->ene_bp_d.p, nb * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  }
+  energy_phase(c, has, Topp, mcflag, e2.data());
+  if (nb) {
+    HIPCHK(hipMemcpyAsync(c->bp_status_d.p, st.data(), nb, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(c->ene_bp_d.p, en.data(), nb * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  }
+  std::vector<double> E(2 * (R + 1), 0.0);
+  for (int r = 0; r < R; r++) {
+    E[2 * (r + 1)] = e1[(size_t)r * 16];
+    E[2 * (r + 1) + 1] = has[r] ? e2[(size_t)r * 16] : 0.0;
+  }
+  if (replica_energies)
+    for (int r = 0; r < R; r++) { replica_energies[2 * r] = E[2 * (r + 1)]; replica_energies[2 * r + 1] = E[2 * (r + 1) + 1]; }
+  std::vector<double> rnd(R + 2);
+  int nr = 0;
+  for (int i = 1; i <= R; i += 2) rnd[++nr] = rep_uniform(c, nr);
+  nr = 0;
+  int nacc = 0;
+  for (int li = 1; li <= R; li++) {
+    int lj = get_pair(c, li);
+    if (lj < li) continue;
+    nr++;
+    int ri = c->lab2rep[li], rj = c->lab2rep[lj];
+    double tm = c->lab2val[li], tn = c->lab2val[lj];
+    double pim = E[2 * ri], pin = E[2 * ri + 1], pjn = E[2 * rj], pjm = E[2 * rj + 1];
+    double delta = ((pjm - pim) / tm - (pjn - pin) / tn) / KB_KCAL;
+    bool acc = (delta < 0.0) || (rnd[nr] <= std::exp(-delta));
+    if (accepted) accepted[nr - 1] = acc ? 1 : 0;
+    if (acc) {
+      nacc++;
+      c->rep2lab[ri] = lj; c->rep2lab[rj] = li;
+      c->lab2rep[li] = rj; c->lab2rep[lj] = ri;
+      double sf = std::sqrt(tn / tm);
+      const int B = 256;
+      hipLaunchKernelGGL(k_scale_vel, dim3((c->N + B - 1) / B), dim3(B), 0, c->stream, c->P, c->ptrs(), ri - 1, sf, 0);
+      hipLaunchKernelGGL(k_scale_vel, dim3((c->N + B - 1) / B), dim3(B), 0, c->stream, c->P, c->ptrs(), rj - 1, sf, 1);
+    }
+  }
+  c->idx_pair = (c->idx_pair + 1) % 2;
+  for (int r = 0; r < R; r++) c->tK[r] = c->lab2val[c->rep2lab[r + 1]];
+  upload_conditions(c);
+  (void)nacc;
+}
+
+// One job_md step for every replica (src/job_md.F90:362-460), one launch
+// per phase.
+void step_phases(sis_gpu_ctx *c) {
+  const int N = c->N, R = c->R;
+  c->istep++;
+  int mc_all = (c->P.nstep_bp_mc > 0 && c->istep % c->P.nstep_bp_mc == 0) ? 1 : 0;
+  std::vector<int> flag(R, 0), mc(R, mc_all);
+  if (!c->lists_valid) {
+    std::fill(flag.begin(), flag.end(), 1);
+  } else if (c->opt.rebuild_every > 0) {
+    if ((c->istep - 1) % c->opt.rebuild_every == 0 && c->istep > 1) std::fill(flag.begin(), flag.end(), 1);
+  } else {
+    HIPCHK(hipMemcpyAsync(flag.data(), c->rebuild_flag_d.p, R * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+  }
+  bool any = false;
+  for (int r = 0; r < R; r++) if (flag[r]) { any = true; if (c->P.nstep_bp_mc > 0) mc[r] = 1; }
+  if (any) { rebuild(c, flag); c->rebuilds++; }
+  std::vector<int> all(R, 1);
+  set_mask(c, all);
+  bp_phase(c, mc, nullptr, nullptr);
+  if (c->P.nstep_bp_mc > 1) {
+    // bp_status_MC = bp_status after an MC step (src/job_md.F90:399-401)
+    for (int r = 0; r < R; r++) (void)r;
+    if (mc_all || any) HIPCHK(hipMemcpyAsync(c->bp_status_mc_d.p, c->bp_status_d.p, c->nbp_total, hipMemcpyDeviceToDevice, c->stream));
+  }
+  if (c->parity)
+    hipLaunchKernelGGL(k_noise_parity, dim3(1), dim3(1), 0, c->stream, c->P, c->ptrs(), 3 * N, 0, nullptr);
+  force_kernel(c, 1, nullptr);
+  hipLaunchKernelGGL(k_skin, dim3(R), dim3(64), 0, c->stream, c->P, c->ptrs(), c->nblk_force);
+  if (c->opt.exchange_every > 0 && R > 1 && c->istep % c->opt.exchange_every == 0) do_exchange(c, nullptr, nullptr);
+}
+
+}  // namespace
+
+#include "sis_persistent_host.h"
+
+// ======================================================================
+// C ABI
+// ======================================================================
+#define API_BEGIN try {
+#define API_END                                                      \
+  }                                                                  \
+  catch (const HipError &e) { return fail(SIS_ERR_HIP, e.msg); }     \
+  catch (const ApiError &e) { return fail(e.code, e.msg); }          \
+  catch (const std::exception &e) { return fail(SIS_ERR_ARG, e.what()); }
+
+extern "C" {
+
+const char *sis_gpu_last_error(void) { return g_err.c_str(); }
+
+int sis_gpu_create(const sis_topo *topo, const sis_params *prm, const sis_options *opt,
+                   sis_gpu_ctx **out) {
+  API_BEGIN
+  if (!topo || !prm || !out) return fail(SIS_ERR_ARG, "null argument");
+  *out = nullptr;
+  if (topo->nchains <= 0 || topo->nmp <= 0) return fail(SIS_ERR_ARG, "empty topology");
+  if (prm->bp_model != 5 && prm->bp_model != 0)
+    return fail(SIS_ERR_UNSUPPORTED, "only base-pair model 5 (triplet) is implemented on the GPU");
+  if (prm->max_bp_per_nt < 1) return fail(SIS_ERR_ARG, "max_bp_per_nt must be >= 1");
+  sis_options o{};
+  if (opt) o = *opt;
+  if (o.nrep <= 0) o.nrep = 1;
+  int sum = 0;
+  for (int c = 0; c < topo->nchains; c++) {
+    if (topo->nmp_chain[c] < 1) return fail(SIS_ERR_ARG, "chain of zero length");
+    sum += topo->nmp_chain[c];
+  }
+  if (sum != topo->nmp) return fail(SIS_ERR_ARG, "sum of chain lengths differs from nmp");
+  int dev_count = 0;
+  HIPCHK(hipGetDeviceCount(&dev_count));
+  if (dev_count < 1) return fail(SIS_ERR_HIP, "no HIP device");
+  auto *c = new sis_gpu_ctx();
+  try {
+    c->opt = o;
+    c->prm = *prm;
+    c->N = topo->nmp;
+    c->R = o.nrep;
+    c->parity = o.parity;
+    c->precision = o.precision;
+    c->mode = o.mode;
+    c->seed = o.seed ? o.seed : 0x5151u;
+    const int N = c->N, R = c->R;
+    HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    HIPCHK(hipEventCreate(&c->ev0));
+    HIPCHK(hipEventCreate(&c->ev1));
+    // topology
+    c->topo.resize(N);
+    std::vector<uint8_t> charged(N, 1);
+    for (int k = 0; k < topo->n_no_charge; k++) {
+      int b = topo->no_charge[k] - 1;
+      if (b < 0 || b >= N) throw ApiError{SIS_ERR_ARG, "no_charge index out of range"};
+      charged[b] = 0;
+    }
+    int imp = 0;
+    for (int ch = 0; ch < topo->nchains; ch++) {
+      int n = topo->nmp_chain[ch];
+      for (int i = 1; i <= n; i++, imp++) {
+        int sq = topo->seq[imp];
+        if (sq < 0 || sq > 4) throw ApiError{SIS_ERR_ARG, "sequence code out of range"};
+        bool q = prm->dummy_has_charge ? true : sq != SQ_D;  // src/init_ele.F90:58-68
+        if (!charged[imp]) q = false;
+        c->topo[imp] = Topo{imp - i + 1, n, i, (topo->moltype[ch] == 2 ? 1 : 0) | (sq << 4) | ((q ? 1 : 0) << 8)};
+      }
+    }
+    c->topo_d.alloc(N);
+    HIPCHK(hipMemcpy(c->topo_d.p, c->topo.data(), N * sizeof(Topo), hipMemcpyHostToDevice));
+    // parameters
+    Params &P = c->P;
+    P.bond_k = prm->bond_k; P.bond_r0 = prm->bond_r0;
+    P.angl_k = prm->angl_k; P.angl_t0 = prm->angl_t0; P.angl_co0 = std::cos(prm->angl_t0);
+    P.dih_k = prm->dih_k; P.dih_w = prm->dih_w; P.dih_p0 = prm->dih_p0;
+    P.wca_sigma = prm->wca_sigma; P.wca_eps = prm->wca_eps;
+    P.wca_sig2 = prm->wca_sigma * prm->wca_sigma;
+    P.wca_coef = 12.0 * prm->wca_eps / P.wca_sig2;
+    P.bp_cutoff_energy = prm->bp_cutoff_energy;
+    P.nl_margin = prm->nl_margin;
+    P.angl_reb = prm->angl_reb; P.dih_type = prm->dih_type; P.ele_on = prm->ele_on;
+    P.ele_excl_bond = prm->ele_excl_bond; P.ele_excl_angl = prm->ele_excl_angl;
+    P.bp_model = prm->bp_model; P.bp_min_loop = prm->bp_min_loop;
+    P.max_bp_per_nt = prm->max_bp_per_nt; P.nstep_bp_mc = prm->nstep_bp_mc; P.pbc = prm->pbc;
+    P.N = N; P.R = R; P.fix_h4 = 0;
+    for (int t = 0; t < 3; t++) {
+      const sis_bp_para &b = prm->bp[t];
+      BpPara m;
+      std::memcpy(&m, &b, sizeof m);
+      std::memcpy(&P.bpp[t + 1], &b, sizeof m);
+      BpPara mr = m;
+      mr.angl_k1 = m.angl_k2; mr.angl_k2 = m.angl_k1; mr.angl_k3 = m.angl_k4; mr.angl_k4 = m.angl_k3;
+      P.bpp[t + 4] = mr;
+    }
+    if (std::fabs(prm->bp_cutoff_energy) <= 2.220446049250313e-16) {
+      P.bp_cutoff_dist = 18.0;
+      for (int t = 1; t <= 6; t++) P.bpp[t].cutoff_ddist = P.bp_cutoff_dist - 13.8;
+    } else {
+      P.bp_cutoff_dist = 0.0;
+      for (int t = 1; t <= 6; t++) {
+        P.bpp[t].cutoff_ddist = std::sqrt(std::log(std::fabs(10.0 / prm->bp_cutoff_energy)) / P.bpp[t].bond_k);
+        P.bp_cutoff_dist = std::max(P.bp_cutoff_dist, P.bpp[t].bond_r + P.bpp[t].cutoff_ddist);
+      }
+    }
+    P.wca_nl_cut2 = (P.wca_sigma + P.nl_margin) * (P.wca_sigma + P.nl_margin);
+    P.bp_nl_cut2 = (P.bp_cutoff_dist + P.nl_margin) * (P.bp_cutoff_dist + P.nl_margin);
+    if (P.bp_model == 0) P.bp_nl_cut2 = -1.0;
+    std::vector<int> hash;
+    std::vector<double> dH, dS;
+    build_triplet_table(*prm, hash, dH, dS);
+    c->bp3_hash_d.alloc(hash.size());
+    c->bp3_dH_d.alloc(dH.size());
+    c->bp3_dS_d.alloc(dS.size());
+    HIPCHK(hipMemcpy(c->bp3_hash_d.p, hash.data(), hash.size() * sizeof(int), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(c->bp3_dH_d.p, dH.data(), dH.size() * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(c->bp3_dS_d.p, dS.data(), dS.size() * sizeof(double), hipMemcpyHostToDevice));
+    // dynamics coefficients (src/job_md.F90:94-133, 700-767)
+    c->mass.resize(N);
+    c->fric.resize(N);
+    double v = prm->viscosity_pas * std::sqrt(1.0e3 / 4184.0) * 6.02214076e23 * 1.0e-20;
+    std::vector<double> ma(N), mf(N), mx(N);
+    for (int i = 0; i < N; i++) {
+      c->mass[i] = md_mass(topo->seq[i]);
+      c->fric[i] = 6.0 * PI_D * v * 10.0;
+      double c1 = 0.5 * prm->dt * c->fric[i] / c->mass[i];
+      double c2 = std::sqrt(1.0 / (1.0 + c1));
+      ma[i] = (1.0 - c1) / (1.0 + c1);
+      mf[i] = c2 * prm->dt / c->mass[i];
+      mx[i] = c2 * prm->dt;
+    }
+    c->md_a_d.alloc(N); c->md_cf_d.alloc(N); c->md_cx_d.alloc(N); c->mass_d.alloc(N);
+    HIPCHK(hipMemcpy(c->md_a_d.p, ma.data(), N * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(c->md_cf_d.p, mf.data(), N * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(c->md_cx_d.p, mx.data(), N * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(c->mass_d.p, c->mass.data(), N * sizeof(double), hipMemcpyHostToDevice));
+    // replicas (src/init_replica.F90:70-125, src/var_replica.F90:162-259)
+    c->tK.assign(R, prm->tempK);
+    c->ionic.assign(R, prm->ionic_strength);
+    c->rep2lab.resize(R + 1); c->lab2rep.resize(R + 1); c->lab2val.resize(R + 1);
+    for (int k = 1; k <= R; k++) { c->rep2lab[k] = k; c->lab2rep[k] = k; c->lab2val[k] = prm->tempK; }
+    c->pair_tb.assign(2 * (R + 2), 0);
+    for (int par = 0; par < 2; par++) {
+      int *tb = c->pair_tb.data() + par * (R + 2);
+      int iset = 0;
+      if (par == 1) tb[++iset] = 0;
+      for (int ir = (par == 0 ? 1 : 2); ir <= R - 1; ir += 2) { iset++; tb[iset] = iset + 1; iset++; tb[iset] = iset - 1; }
+      while (iset < R) tb[++iset] = 0;
+    }
+    size_t RN = (size_t)R * N;
+    c->md_cr_d.alloc(RN);
+    c->tK_d.alloc(R); c->ele_coef_d.alloc(R); c->lambdaD_d.alloc(R); c->ele_cut_d.alloc(R);
+    c->box_d.alloc(3 * R);
+    c->ecT_d.alloc(R); c->eCoef_d.alloc(R); c->eLam_d.alloc(R);
+    c->pos_d.alloc(RN); c->vel_d.alloc(RN); c->acc_d.alloc(RN); c->move_d.alloc(RN); c->frc_d.alloc(RN);
+    c->n_ele_d.alloc(RN); c->n_wca_d.alloc(RN); c->n_bp_tmp_d.alloc(RN); c->bp_off_d.alloc(RN + 1);
+    c->n_bp_d.alloc(R); c->n_inc_d.alloc(RN); c->nt_cnt_d.alloc(RN); c->comp_d.alloc(RN);
+    c->flags_d.alloc(8);
+    c->blk_top2_d.alloc((size_t)R * ((N + 255) / 256 + 1) * 2);
+    c->rebuild_flag_d.alloc(R); c->mask_d.alloc(R); c->mc_d.alloc(R);
+    c->scan_total_d.alloc(1);
+    c->exc_cnt_d.alloc(RN); c->exc_off_d.alloc(RN + 1); c->exc_fill_d.alloc(RN); c->exc_list_d.alloc(RN);
+    c->ntlist_d.alloc((size_t)R * (N + 1));
+    c->noise_d.alloc(RN * 3);
+    c->bm_cache_d.alloc(2);
+    c->ene_out_d.alloc((size_t)R * 16);
+    c->tape_off_d.alloc(R + 1);
+    c->tape_pos_d.alloc(R);
+    c->tape_off_h.assign(R + 1, 0);
+    // list capacities (grown on demand)
+    c->D.maxnb = o.max_nbr > 0 ? o.max_nbr : 64;
+    c->D.maxw = 16;
+    c->D.maxbpb = 16;
+    c->D.maxinc = 32;
+    c->D.maxbp = std::max(64, N);
+    alloc_lists(c);
+    upload_conditions(c);
+    setup_cells(c);
+    *out = c;
+  } catch (...) {
+    delete c;
+    throw;
+  }
+  return SIS_OK;
+  API_END
+}
+
+int sis_gpu_destroy(sis_gpu_ctx *c) {
+  API_BEGIN
+  delete c;
+  return SIS_OK;
+  API_END
+}
+
+int sis_gpu_set_state(sis_gpu_ctx *c, const double *xyz, const double *velos, const double *accels) {
+  API_BEGIN
+  if (!c) return fail(SIS_ERR_ARG, "null context");
+  size_t RN = (size_t)c->R * c->N;
+  std::vector<double4> h(RN);
+  auto up = [&](const double *src, DevBuf<double4> &dst) {
+    for (size_t g = 0; g < RN; g++) h[g] = double4{src[3 * g], src[3 * g + 1], src[3 * g + 2], 0.0};
+    HIPCHK(hipMemcpy(dst.p, h.data(), RN * sizeof(double4), hipMemcpyHostToDevice));
+  };
+  if (xyz) { up(xyz, c->pos_d); c->lists_valid = false; }
+  if (velos) up(velos, c->vel_d);
+  if (accels) up(accels, c->acc_d);
+  return SIS_OK;
+  API_END
+}
+
+int sis_gpu_get_state(sis_gpu_ctx *c, double *xyz, double *velos, double *accels) {
+  API_BEGIN
+  if (!c) return fail(SIS_ERR_ARG, "null context");
+  HIPCHK(hipStreamSynchronize(c->stream));
+  size_t RN = (size_t)c->R * c->N;
+  std::vector<double4> h(RN);
+  auto down = [&](double *dst, DevBuf<double4> &src) {
+    HIPCHK(hipMemcpy(h.data(), src.p, RN * sizeof(double4), hipMemcpyDeviceToHost));
+    for (size_t g = 0; g < RN; g++) { dst[3 * g] = h[g].x; dst[3 * g + 1] = h[g].y; dst[3 * g + 2] = h[g].z; }
+  };
+  if (xyz) down(xyz, c->pos_d);
+  if (velos) down(velos, c->vel_d);
+  if (accels) down(accels, c->acc_d);
+  return SIS_OK;
+  API_END
+}
+
+int sis_gpu_set_rep_conditions(sis_gpu_ctx *c, const double *tempK, const double *ionic) {
+  API_BEGIN
+  if (!c) return fail(SIS_ERR_ARG, "null context");
+  for (int r = 0; r < c->R; r++) {
+    if (tempK) {
+      if (!(tempK[r] > 0.0)) return fail(SIS_ERR_ARG, "temperature must be positive");
+      c->tK[r] = tempK[r];
+      c->lab2val[r + 1] = tempK[r];
+      c->rep2lab[r + 1] = r + 1;
+      c->lab2rep[r + 1] = r + 1;
+    }
+    if (ionic) {
+      if (!(ionic[r] > 0.0)) return fail(SIS_ERR_ARG, "ionic strength must be positive");
+      c->ionic[r] = ionic[r];
+    }
+  }
+  upload_conditions(c);
+  setup_cells(c);
+  c->lists_valid = false;
+  return SIS_OK;
+  API_END
+}
+
+int sis_gpu_rebuild(sis_gpu_ctx *c, int32_t irep) {
+  API_BEGIN
+  if (!c) return fail(SIS_ERR_ARG, "null context");
+  if (irep < 0 || irep > c->R) return fail(SIS_ERR_ARG, "irep out of range");
+  std::vector<int> m(c->R, irep == 0 ? 1 : 0);
+  if (irep > 0) m[irep - 1] = 1;
+  if (!c->lists_valid) std::fill(m.begin(), m.end(), 1);
+  HIPCHK(hipEventRecord(c->ev0, c->stream));
+  rebuild(c, m);
+  HIPCHK(hipEventRecord(c->ev1, c->stream));
+  HIPCHK(hipEventSynchronize(c->ev1));
+  float ms = 0;
+  HIPCHK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+  c->last_ms[1] = ms;
+  return SIS_OK;
+  API_END
+}
+
+int sis_gpu_force(sis_gpu_ctx *c, int32_t irep, int32_t bp_mc, double *forces) {
+  API_BEGIN
+  if (!c) return fail(SIS_ERR_ARG, "null context");
+  if (irep < 1 || irep > c->R) return fail(SIS_ERR_ARG, "irep out of range");
+  if (!c->lists_valid) return fail(SIS_ERR_STATE, "force called before rebuild");
+  std::vector<int> m(c->R, 0), mc(c->R, 0);
+  m[irep - 1] = 1;
+  mc[irep - 1] = bp_mc ? 1 : 0;
+  set_mask(c, m);
+  HIPCHK(hipEventRecord(c->ev0, c->stream));
+  bp_phase(c, mc, nullptr, c->mask_d.p);
+  force_kernel(c, 0, c->mask_d.p);
+  HIPCHK(hipEventRecord(c->ev1, c->stream));
+  if (forces) {
+    std::vector<double4> h(c->N);
+    HIPCHK(hipMemcpyAsync(h.data(), c->frc_d.p + (size_t)(irep - 1) * c->N, c->N * sizeof(double4),
+                          hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    for (int i = 0; i < c->N; i++) { forces[3 * i] = h[i].x; forces[3 * i + 1] = h[i].y; forces[3 * i + 2] = h[i].z; }
+  }
+  HIPCHK(hipEventSynchronize(c->ev1));
+  float ms = 0;
+  HIPCHK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+  c->last_ms[2] = ms;
+  check_flags(c);
+  return SIS_OK;
+  API_END
+}
+
+int sis_gpu_energy(sis_gpu_ctx *c, int32_t irep, double tempK, int32_t flags, double *energies) {
+  API_BEGIN
+  if (!c || !energies) return fail(SIS_ERR_ARG, "null argument");
+  if (irep < 1 || irep > c->R) return fail(SIS_ERR_ARG, "irep out of range");
+  if (!c->lists_valid) return fail(SIS_ERR_STATE, "energy called before rebuild");
+  std::vector<int> m(c->R, 0);
+  m[irep - 1] = 1;
+  std::vector<double> T = c->tK;
+  T[irep - 1] = tempK > 0 ? tempK : c->tK[irep - 1];
+  std::vector<double> out((size_t)c->R * 16);
+  energy_phase(c, m, T, flags, out.data());
+  for (int k = 0; k < 11; k++) energies[k] = out[(size_t)(irep - 1) * 16 + k];
+  return SIS_OK;
+  API_END
+}
+
+int sis_gpu_get_pair_count(sis_gpu_ctx *c, int32_t irep, int32_t kind, int64_t *n) {
+  API_BEGIN
+  if (!c || !n) return fail(SIS_ERR_ARG, "null argument");
+  if (irep < 1 || irep > c->R || kind < 0 || kind > 2) return fail(SIS_ERR_ARG, "bad irep/kind");
+  if (!c->lists_valid) return fail(SIS_ERR_STATE, "lists not built");
+  HIPCHK(hipStreamSynchronize(c->stream));
+  const int N = c->N;
+  if (kind == 2) {
+    int a = 0, b = 0;
+    HIPCHK(hipMemcpy(&a, c->bp_off_d.p + (size_t)(irep - 1) * N, sizeof(int), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(&b, c->bp_off_d.p + (size_t)irep * N, sizeof(int), hipMemcpyDeviceToHost));
+    *n = b - a;
+    return SIS_OK;
+  }
+  int maxs = kind == 0 ? c->D.maxw : c->D.maxnb;
+  std::vector<int> cnt(N), nb((size_t)maxs * N);
+  HIPCHK(hipMemcpy(cnt.data(), (kind == 0 ? c->n_wca_d.p : c->n_ele_d.p) + (size_t)(irep - 1) * N,
+                   N * sizeof(int), hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(nb.data(), (kind == 0 ? c->nb_wca_d.p : c->nb_ele_d.p) + (size_t)(irep - 1) * maxs * N,
+                   nb.size() * sizeof(int), hipMemcpyDeviceToHost));
+  int64_t t = 0;
+  for (int a = 0; a < N; a++)
+    for (int k = 0; k < std::min(cnt[a], maxs); k++) t += nb[(size_t)k * N + a] > a;
+  *n = t;
+  return SIS_OK;
+  API_END
+}
+
+int sis_gpu_get_pairs(sis_gpu_ctx *c, int32_t irep, int32_t kind, int32_t *out, int64_t cap, int64_t *n) {
+  API_BEGIN
+  if (!c || !out) return fail(SIS_ERR_ARG, "null argument");
+  if (irep < 1 || irep > c->R || kind < 0 || kind > 2) return fail(SIS_ERR_ARG, "bad irep/kind");
+  if (!c->lists_valid) return fail(SIS_ERR_STATE, "lists not built");
+  HIPCHK(hipStreamSynchronize(c->stream));
+  const int N = c->N;
+  int64_t t = 0;
+  if (kind == 2) {
+    int a = 0, b = 0;
+    HIPCHK(hipMemcpy(&a, c->bp_off_d.p + (size_t)(irep - 1) * N, sizeof(int), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(&b, c->bp_off_d.p + (size_t)irep * N, sizeof(int), hipMemcpyDeviceToHost));
+    std::vector<int> mp((size_t)(b - a) * 8 + 8);
+    if (b > a) HIPCHK(hipMemcpy(mp.data(), c->bp_mp_d.p + (size_t)a * 8, (size_t)(b - a) * 8 * sizeof(int), hipMemcpyDeviceToHost));
+    for (int k = 0; k < b - a; k++) {
+      if (t < cap) for (int q = 0; q < 7; q++) out[7 * t + q] = mp[8 * (size_t)k + q];
+      t++;
+    }
+    if (n) *n = t;
+    if (t > cap) return fail(SIS_ERR_CAPACITY, "output capacity too small");
+    return SIS_OK;
+  }
+  int maxs = kind == 0 ? c->D.maxw : c->D.maxnb;
+  std::vector<int> cnt(N), nb((size_t)maxs * N);
+  HIPCHK(hipMemcpy(cnt.data(), (kind == 0 ? c->n_wca_d.p : c->n_ele_d.p) + (size_t)(irep - 1) * N,
+                   N * sizeof(int), hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(nb.data(), (kind == 0 ? c->nb_wca_d.p : c->nb_ele_d.p) + (size_t)(irep - 1) * maxs * N,
+                   nb.size() * sizeof(int), hipMemcpyDeviceToHost));
+  for (int a = 0; a < N; a++) {
+    std::vector<int> js;
+    for (int k = 0; k < std::min(cnt[a], maxs); k++) {
+      int b = nb[(size_t)k * N + a];
+      if (b > a) js.push_back(b);
+    }
+    std::sort(js.begin(), js.end());
+    for (int b : js) {
+      if (t < cap) { out[2 * t] = a + 1; out[2 * t + 1] = b + 1; }
+      t++;
+    }
+  }
+  if (n) *n = t;
+  if (t > cap) return fail(SIS_ERR_CAPACITY, "output capacity too small");
+  return SIS_OK;
+  API_END
+}
+
+int sis_gpu_get_bp(sis_gpu_ctx *c, int32_t irep, int32_t *bp_mp, double *bp_coef, uint8_t *status,
+                   double *ene, int64_t cap, int64_t *n) {
+  API_BEGIN
+  if (!c) return fail(SIS_ERR_ARG, "null context");
+  if (irep < 1 || irep > c->R) return fail(SIS_ERR_ARG, "irep out of range");
+  if (!c->lists_valid) return fail(SIS_ERR_STATE, "lists not built");
+  HIPCHK(hipStreamSynchronize(c->stream));
+  const int N = c->N;
+  int a = 0, b = 0;
+  HIPCHK(hipMemcpy(&a, c->bp_off_d.p + (size_t)(irep - 1) * N, sizeof(int), hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(&b, c->bp_off_d.p + (size_t)irep * N, sizeof(int), hipMemcpyDeviceToHost));
+  int64_t m = b - a;
+  if (n) *n = m;
+  if (m > cap) return fail(SIS_ERR_CAPACITY, "output capacity too small");
+  if (m == 0) return SIS_OK;
+  std::vector<int> mp((size_t)m * 8);
+  std::vector<double> cf((size_t)m * 2), en(m);
+  std::vector<uint8_t> st(m);
+  HIPCHK(hipMemcpy(mp.data(), c->bp_mp_d.p + (size_t)a * 8, mp.size() * sizeof(int), hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(cf.data(), c->bp_coef_d.p + (size_t)a * 2, cf.size() * sizeof(double), hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(st.data(), c->bp_status_d.p + a, m, hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(en.data(), c->ene_bp_d.p + a, m * sizeof(double), hipMemcpyDeviceToHost));
+  for (int64_t k = 0; k < m; k++) {
+    if (bp_mp) for (int q = 0; q < 7; q++) bp_mp[7 * k + q] = mp[8 * k + q];
+    if (bp_coef) { bp_coef[2 * k] = cf[2 * k]; bp_coef[2 * k + 1] = cf[2 * k + 1]; }
+    if (status) status[k] = st[k];
+    if (ene) ene[k] = en[k];
+  }
+  return SIS_OK;
+  API_END
+}
+
+int sis_gpu_set_rng(sis_gpu_ctx *c, uint64_t seed) {
+  API_BEGIN
+  if (!c) return fail(SIS_ERR_ARG, "null context");
+  c->seed = seed;
+  return SIS_OK;
+  API_END
+}
+
+int sis_gpu_set_rng_tape(sis_gpu_ctx *c, int32_t irep, const uint32_t *words, int64_t n) {
+  API_BEGIN
+  if (!c || (!words && n > 0)) return fail(SIS_ERR_ARG, "null argument");
+  if (!c->parity) return fail(SIS_ERR_STATE, "context was not created in parity mode");
+  if (irep == -1) {
+    c->rep_tape.assign(words, words + n);
+    c->rep_tape_pos = 0;
+    return SIS_OK;
+  }
+  if (irep < 1 || irep > c->R) return fail(SIS_ERR_ARG, "irep out of range");
+  // rebuild the concatenated tape with this replica's words replaced
+  const int R = c->R;
+  std::vector<uint32_t> all;
+  std::vector<int64_t> off(R + 1, 0);
+  std::vector<uint32_t> old;
+  if (c->tape_d.n) {
+    old.resize(c->tape_d.n);
+    HIPCHK(hipMemcpy(old.data(), c->tape_d.p, old.size() * sizeof(uint32_t), hipMemcpyDeviceToHost));
+  }
+  std::vector<int64_t> pos(R, 0);
+  HIPCHK(hipMemcpy(pos.data(), c->tape_pos_d.p, R * sizeof(int64_t), hipMemcpyDeviceToHost));
+  for (int r = 0; r < R; r++) {
+    off[r] = (int64_t)all.size();
+    if (r == irep - 1) {
+      all.insert(all.end(), words, words + n);
+      pos[r] = 0;
+    } else if (!old.empty()) {
+      all.insert(all.end(), old.begin() + c->tape_off_h[r], old.begin() + c->tape_off_h[r + 1]);
+    }
+  }
+  off[R] = (int64_t)all.size();
+  c->tape_d.alloc(std::max<size_t>(all.size(), 1));
+  if (!all.empty()) HIPCHK(hipMemcpy(c->tape_d.p, all.data(), all.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(c->tape_off_d.p, off.data(), off.size() * sizeof(int64_t), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(c->tape_pos_d.p, pos.data(), R * sizeof(int64_t), hipMemcpyHostToDevice));
+  c->tape_off_h = off;
+  return SIS_OK;
+  API_END
+}
+
+int sis_gpu_init_velocities(sis_gpu_ctx *c) {
+  API_BEGIN
+  if (!c) return fail(SIS_ERR_ARG, "null context");
+  const int N = c->N, R = c->R, B = 256;
+  if (c->parity) {
+    // per replica: 3N velocity normals then 3N acceleration normals
+    // (src/job_md.F90:157-176); replicas in order share the Box-Muller cache
+    for (int r = 0; r < R; r++) {
+      std::vector<int> m(R, 0);
+      m[r] = 1;
+      set_mask(c, m);
+      hipLaunchKernelGGL(k_noise_parity, dim3(1), dim3(1), 0, c->stream, c->P, c->ptrs(), 3 * N, 0, c->mask_d.p);
+      hipLaunchKernelGGL(k_init_vel, dim3((N + B - 1) / B, R), dim3(B), 0, c->stream, c->P, c->ptrs(), 0, 1, c->seed,
+                         c->mask_d.p);
+      hipLaunchKernelGGL(k_noise_parity, dim3(1), dim3(1), 0, c->stream, c->P, c->ptrs(), 3 * N, 0, c->mask_d.p);
+      hipLaunchKernelGGL(k_init_vel, dim3((N + B - 1) / B, R), dim3(B), 0, c->stream, c->P, c->ptrs(), 1, 1, c->seed,
+                         c->mask_d.p);
+      HIPCHK(hipStreamSynchronize(c->stream));
+    }
+    check_flags(c);
+  } else {
+    hipLaunchKernelGGL(k_init_vel, dim3((N + B - 1) / B, R), dim3(B), 0, c->stream, c->P, c->ptrs(), 0, 0, c->seed,
+                       nullptr);
+    hipLaunchKernelGGL(k_init_vel, dim3((N + B - 1) / B, R), dim3(B), 0, c->stream, c->P, c->ptrs(), 1, 0, c->seed,
+                       nullptr);
+    HIPCHK(hipStreamSynchronize(c->stream));
+  }
+  return SIS_OK;
+  API_END
+}
+
+int sis_gpu_step(sis_gpu_ctx *c, int64_t nsteps, int64_t *rebuilds) {
+  API_BEGIN
+  if (!c) return fail(SIS_ERR_ARG, "null context");
+  if (nsteps < 0) return fail(SIS_ERR_ARG, "nsteps < 0");
+  int64_t r0 = c->rebuilds;
+  HIPCHK(hipEventRecord(c->ev0, c->stream));
+  if (!c->lists_valid) {
+    std::vector<int> all(c->R, 1);
+    rebuild(c, all);
+  }
+  int mode = c->mode;
+  if (mode == 0) mode = choose_mode(c);
+  if (mode == 2 && !c->parity) {
+    persistent_steps(c, nsteps);
+  } else if (mode == 3 && !c->parity) {
+    batched_steps(c, nsteps);
+  } else {
+    for (int64_t s = 0; s < nsteps; s++) step_phases(c);
+  }
+  HIPCHK(hipEventRecord(c->ev1, c->stream));
+  HIPCHK(hipEventSynchronize(c->ev1));
+  float ms = 0;
+  HIPCHK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+  c->last_ms[0] = ms;
+  check_flags(c);
+  if (rebuilds) *rebuilds = c->rebuilds - r0;
+  return SIS_OK;
+  API_END
+}
+
+int sis_gpu_exchange(sis_gpu_ctx *c, double *replica_energies, int32_t *accepted) {
+  API_BEGIN
+  if (!c) return fail(SIS_ERR_ARG, "null context");
+  if (!c->lists_valid) return fail(SIS_ERR_STATE, "exchange called before rebuild");
+  do_exchange(c, replica_energies, accepted);
+  return SIS_OK;
+  API_END
+}
+
+int sis_gpu_get_labels(sis_gpu_ctx *c, int32_t *rep2lab) {
+  API_BEGIN
+  if (!c || !rep2lab) return fail(SIS_ERR_ARG, "null argument");
+  for (int r = 0; r < c->R; r++) rep2lab[r] = c->rep2lab[r + 1];
+  return SIS_OK;
+  API_END
+}
+
+int sis_gpu_get_info(sis_gpu_ctx *c, int64_t *out, int32_t n) {
+  API_BEGIN
+  if (!c || !out) return fail(SIS_ERR_ARG, "null argument");
+  int64_t v[10] = {c->N, c->R, c->istep, c->rebuilds, 0, 0, 0, c->D.maxnb, c->mode, c->block};
+  if (c->lists_valid) {
+    int64_t k;
+    if (sis_gpu_get_pair_count(c, 1, 0, &k) == SIS_OK) v[4] = k;
+    if (sis_gpu_get_pair_count(c, 1, 1, &k) == SIS_OK) v[5] = k;
+    if (sis_gpu_get_pair_count(c, 1, 2, &k) == SIS_OK) v[6] = k;
+  }
+  for (int k = 0; k < n && k < 10; k++) out[k] = v[k];
+  return SIS_OK;
+  API_END
+}
+
+double sis_gpu_last_kernel_ms(sis_gpu_ctx *c, int32_t which) {
+  if (!c || which < 0 || which > 3) return -1.0;
+  return c->last_ms[which];
+}
+
+}  // extern "C"

@@ -1,0 +1,951 @@
+// Phase kernels of the SIS MD step (one thread per bead, per candidate or
+// per component).  Included by sis_gpu.hip.
+#pragma once
+#include "sis_device.h"
+
+namespace sis {
+
+// ---------------------------------------------------------------------
+// pbc_wrap (src/pbc.F90:196-226) for replicas with mask[r] != 0.
+__global__ void k_wrap(Params P, Ptrs D, const int *mask) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x, r = blockIdx.y;
+  if (i >= P.N || !P.pbc || (mask && !mask[r])) return;
+  const double *box = D.box + 3 * r;
+  double4 &p = D.pos[(size_t)r * P.N + i];
+  double v[3] = {p.x, p.y, p.z};
+  for (int k = 0; k < 3; k++) {
+    double L = box[k];
+    if (v[k] > L) v[k] = v[k] - L * ((double)(int)((v[k] - L) / L) + 1);
+    else if (v[k] < 0.0) v[k] = v[k] + L * ((double)(int)((0.0 - v[k]) / L) + 1);
+  }
+  p.x = v[0]; p.y = v[1]; p.z = v[2];
+}
+
+// ---------------------------------------------------------------------
+// Base-pair eligibility of replica-local beads i < j (0-based) and the
+// triplet-table index: the pair rules of src/init_bp.F90:184-313 (models
+// 3/4/5, no secondary-structure input).  Returns table index (0 = not
+// eligible); *bpt receives the BPT type.
+__device__ __forceinline__ int is_comp(int a, int b) {
+  return (a == SQ_A && b == SQ_U) || (a == SQ_U && (b == SQ_A || b == SQ_G)) ||
+         (a == SQ_G && (b == SQ_C || b == SQ_U)) || (a == SQ_C && b == SQ_G);
+}
+__device__ __forceinline__ int hkey(int w, int x, int u, int z, int y, int v) {
+  if (x < y) return 1 + w + x * 4 + u * 16 + z * 64 + y * 256 + v * 1024;
+  return 1 + v + y * 4 + z * 16 + u * 64 + x * 256 + w * 1024;
+}
+__device__ __forceinline__ int bp_eligible(const Params &P, const Ptrs &D, int i, int j, int *bpt) {
+  *bpt = 0;
+  Topo ti = D.topo[i], tj = D.topo[j];
+  int li = ti.loc, lj = tj.loc, ni = ti.len, nj = tj.len;
+  bool ic = topo_circ(ti), jc = topo_circ(tj);
+  int ipre = li - 1, inxt = li + 1, jpre = lj - 1, jnxt = lj + 1;
+  if (ic) { if (li == 1) ipre = ni; if (li == ni) inxt = 1; }
+  else if (li == 1 || li == ni) return 0;
+  if (jc) { if (lj == 1) jpre = nj; if (lj == nj) jnxt = 1; }
+  else if (lj == 1 || lj == nj) return 0;
+  auto sq = [&](const Topo &t, int l) { return topo_seq(D.topo[t.start + l - 1]); };
+  int s_i = topo_seq(ti), s_j = topo_seq(tj);
+  int s_ipre = sq(ti, ipre), s_inxt = sq(ti, inxt), s_jpre = sq(tj, jpre), s_jnxt = sq(tj, jnxt);
+  int p1 = is_comp(s_ipre, s_jnxt), p2 = is_comp(s_inxt, s_jpre);
+  if (!ic && ipre == 1) p1 = 0;
+  if (!jc && jnxt == nj) p1 = 0;
+  if (!ic && inxt == ni) p2 = 0;
+  if (!jc && jpre == 1) p2 = 0;
+  int L = P.bp_min_loop;
+  if (ti.start == tj.start) {
+    if (li + L >= lj) return 0;
+    if (li + 1 + L >= lj - 1) p2 = 0;
+    if (ic) {
+      if (lj + L >= li + ni) return 0;
+      if (lj + 1 + L >= li - 1 + ni) p1 = 0;
+    }
+  }
+  if ((P.bp_model >= 3) && !p1 && !p2) return 0;
+  int t = 0;
+  if (s_i == SQ_G && s_j == SQ_C) t = 1; else if (s_i == SQ_C && s_j == SQ_G) t = 4;
+  else if (s_i == SQ_A && s_j == SQ_U) t = 2; else if (s_i == SQ_U && s_j == SQ_A) t = 5;
+  else if (s_i == SQ_G && s_j == SQ_U) t = 3; else if (s_i == SQ_U && s_j == SQ_G) t = 6;
+  *bpt = t;
+  if (P.bp_model != 5) return t;
+  int key;
+  if (!p1) key = hkey(SQ_U, s_i, s_inxt, SQ_U, s_j, s_jpre);
+  else if (!p2) key = hkey(s_ipre, s_i, SQ_U, s_jnxt, s_j, SQ_U);
+  else key = hkey(s_ipre, s_i, s_inxt, s_jnxt, s_j, s_jpre);
+  return D.bp3_hash[key];
+}
+
+// Pair criteria of src/neighbor_list.F90:123-223 for replica-local beads
+// a != b of replica r; appends to the owner's (a's) full lists.
+struct SearchOut {
+  int ne, nw, nb;
+};
+__device__ __forceinline__ void pair_test(const Params &P, const Ptrs &D, int r, int a, int b,
+                                          const double4 &pa, const double4 &pb, double ele_cut2,
+                                          SearchOut &o) {
+  const int N = P.N;
+  int lo = a < b ? a : b, hi = a < b ? b : a;
+  const double4 &plo = a < b ? pa : pb;
+  const double4 &phi = a < b ? pb : pa;
+  double vx, vy, vz;
+  pbc_vec(D.box + 3 * r, P.pbc, plo.x, plo.y, plo.z, phi.x, phi.y, phi.z, vx, vy, vz);
+  double d2 = dot_nc(vx, vy, vz, vx, vy, vz);
+  if (d2 > ele_cut2 && d2 > P.wca_nl_cut2 && d2 > P.bp_nl_cut2) return;
+  Topo tl = D.topo[lo], th = D.topo[hi];
+  bool same = tl.start == th.start;
+  int il = tl.loc, jl = th.loc, n = tl.len;
+  bool circ = topo_circ(tl);
+  // WCA
+  if (topo_seq(tl) != SQ_D && topo_seq(th) != SQ_D && d2 <= P.wca_nl_cut2) {
+    bool add = true;
+    if (same) {
+      if (il + 2 >= jl) add = false;
+      else if (circ && jl + 2 >= il + n) add = false;
+    }
+    if (add) {
+      if (o.nw < D.maxw) D.nb_wca[((size_t)r * D.maxw + o.nw) * N + a] = b;
+      o.nw++;
+    }
+  }
+  // Debye-Hueckel
+  if (P.ele_on && d2 <= ele_cut2 && topo_charged(tl) && topo_charged(th)) {
+    bool add = true;
+    if (same) {
+      if (jl == il + 1 && P.ele_excl_bond) add = false;
+      if (jl == il + 2 && P.ele_excl_angl) add = false;
+      if (circ) {
+        if (il == 1) {
+          if (jl == n && P.ele_excl_bond) add = false;
+          if (jl == n - 1 && P.ele_excl_angl) add = false;
+        } else if (il == 2) {
+          if (jl == n && P.ele_excl_angl) add = false;
+        }
+      }
+    }
+    if (add) {
+      if (o.ne < D.maxnb) D.nb_ele[((size_t)r * D.maxnb + o.ne) * N + a] = b;
+      o.ne++;
+    }
+  }
+  // base-pair candidates, owned by the lower bead
+  if (a < b && P.bp_model > 0 && d2 <= P.bp_nl_cut2) {
+    int bt;
+    if (bp_eligible(P, D, a, b, &bt) > 0) {
+      if (o.nb < D.maxbpb) D.bp_tmp[((size_t)r * N + a) * D.maxbpb + o.nb] = b;
+      o.nb++;
+    }
+  }
+}
+
+__device__ __forceinline__ void search_finish(const Params &P, const Ptrs &D, int r, int a,
+                                              SearchOut &o) {
+  size_t g = (size_t)r * P.N + a;
+  D.n_ele[g] = o.ne;
+  D.n_wca[g] = o.nw;
+  int nb = o.nb < D.maxbpb ? o.nb : D.maxbpb;
+  D.n_bp_tmp[g] = nb;
+  // sort this bead's BP partners so the global list follows the
+  // reference order (chain_i, chain_j, i, j)
+  int *t = D.bp_tmp + g * D.maxbpb;
+  for (int x = 1; x < nb; x++) {
+    int v = t[x], y = x - 1;
+    while (y >= 0 && t[y] > v) { t[y + 1] = t[y]; y--; }
+    t[y + 1] = v;
+  }
+  if (o.ne > D.maxnb || o.nw > D.maxw || o.nb > D.maxbpb) {
+    D.flags[0] = 1;
+    atomicMax(&D.flags[2], o.ne);
+    atomicMax(&D.flags[3], o.nw);
+    atomicMax(&D.flags[4], o.nb);
+  }
+}
+
+// All-pairs search through LDS tiles (small systems or small boxes).
+template <int B>
+__global__ void __launch_bounds__(B) k_search_brute(Params P, Ptrs D, const int *mask) {
+  __shared__ double4 sp[B];
+  int r = blockIdx.y;
+  if (mask && !mask[r]) return;
+  int a = blockIdx.x * B + threadIdx.x;
+  const int N = P.N;
+  bool act = a < N;
+  double4 pa = act ? D.pos[(size_t)r * N + a] : double4{0, 0, 0, 0};
+  double ec = D.ele_cut[r] + P.nl_margin;
+  double ele_cut2 = P.ele_on ? ec * ec : 0.0;
+  SearchOut o{0, 0, 0};
+  for (int t0 = 0; t0 < N; t0 += B) {
+    __syncthreads();
+    if (t0 + threadIdx.x < N) sp[threadIdx.x] = D.pos[(size_t)r * N + t0 + threadIdx.x];
+    __syncthreads();
+    if (act) {
+      int lim = min(B, N - t0);
+      for (int k = 0; k < lim; k++) {
+        int b = t0 + k;
+        if (b == a) continue;
+        pair_test(P, D, r, a, b, pa, sp[k], ele_cut2, o);
+      }
+    }
+  }
+  if (act) search_finish(P, D, r, a, o);
+}
+
+// Cell list: assignment / fill / per-cell sort / search.
+__device__ __forceinline__ int cell_index(const Ptrs &D, int r, const double4 &p) {
+  const double *box = D.box + 3 * r;
+  int cx = (int)(p.x * D.ncell_x / box[0]), cy = (int)(p.y * D.ncell_y / box[1]),
+      cz = (int)(p.z * D.ncell_z / box[2]);
+  cx = min(max(cx, 0), D.ncell_x - 1);
+  cy = min(max(cy, 0), D.ncell_y - 1);
+  cz = min(max(cz, 0), D.ncell_z - 1);
+  return (cz * D.ncell_y + cy) * D.ncell_x + cx;
+}
+__global__ void k_cell_count(Params P, Ptrs D, const int *mask) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x, r = blockIdx.y;
+  if (i >= P.N || (mask && !mask[r])) return;
+  int c = cell_index(D, r, D.pos[(size_t)r * P.N + i]);
+  D.cell_of[(size_t)r * P.N + i] = c;
+  atomicAdd(&D.cell_cnt[(size_t)r * D.ncell + c], 1);
+}
+__global__ void k_cell_fill(Params P, Ptrs D, const int *mask, int *fill) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x, r = blockIdx.y;
+  if (i >= P.N || (mask && !mask[r])) return;
+  size_t cg = (size_t)r * D.ncell + D.cell_of[(size_t)r * P.N + i];
+  int slot = atomicAdd(&fill[cg], 1);
+  D.cell_beads[D.cell_start[cg] + slot] = i;
+}
+__global__ void k_cell_sort(Params P, Ptrs D, const int *mask) {
+  size_t cg = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (cg >= (size_t)P.R * D.ncell) return;
+  int r = (int)(cg / D.ncell);
+  if (mask && !mask[r]) return;
+  int s = D.cell_start[cg], e = D.cell_start[cg + 1];
+  int *t = D.cell_beads;
+  for (int x = s + 1; x < e; x++) {
+    int v = t[x], y = x - 1;
+    while (y >= s && t[y] > v) { t[y + 1] = t[y]; y--; }
+    t[y + 1] = v;
+  }
+}
+__global__ void k_search_cell(Params P, Ptrs D, const int *mask) {
+  int a = blockIdx.x * blockDim.x + threadIdx.x, r = blockIdx.y;
+  if (a >= P.N || (mask && !mask[r])) return;
+  const int N = P.N;
+  double4 pa = D.pos[(size_t)r * N + a];
+  double ec = D.ele_cut[r] + P.nl_margin;
+  double ele_cut2 = P.ele_on ? ec * ec : 0.0;
+  int c = D.cell_of[(size_t)r * N + a];
+  int cx = c % D.ncell_x, cy = (c / D.ncell_x) % D.ncell_y, cz = c / (D.ncell_x * D.ncell_y);
+  SearchOut o{0, 0, 0};
+  for (int dz = -1; dz <= 1; dz++)
+    for (int dy = -1; dy <= 1; dy++)
+      for (int dx = -1; dx <= 1; dx++) {
+        int x = (cx + dx + D.ncell_x) % D.ncell_x, y = (cy + dy + D.ncell_y) % D.ncell_y,
+            z = (cz + dz + D.ncell_z) % D.ncell_z;
+        size_t cg = (size_t)r * D.ncell + (z * D.ncell_y + y) * D.ncell_x + x;
+        int s = D.cell_start[cg], e = D.cell_start[cg + 1];
+        for (int q = s; q < e; q++) {
+          int b = D.cell_beads[q];
+          if (b == a) continue;
+          pair_test(P, D, r, a, b, pa, D.pos[(size_t)r * N + b], ele_cut2, o);
+        }
+      }
+  search_finish(P, D, r, a, o);
+}
+
+// ---------------------------------------------------------------------
+// Exclusive scan (three passes) over n ints.
+template <int B>
+__global__ void k_scan_blocks(const int *in, int *out, int *bsum, int64_t n) {
+  __shared__ int s[B];
+  int64_t i = (int64_t)blockIdx.x * B + threadIdx.x;
+  int v = i < n ? in[i] : 0;
+  s[threadIdx.x] = v;
+  __syncthreads();
+  for (int off = 1; off < B; off <<= 1) {
+    int t = threadIdx.x >= off ? s[threadIdx.x - off] : 0;
+    __syncthreads();
+    s[threadIdx.x] += t;
+    __syncthreads();
+  }
+  if (i < n) out[i] = s[threadIdx.x] - v;
+  if (threadIdx.x == B - 1) bsum[blockIdx.x] = s[B - 1];
+}
+__global__ void k_scan_bsum(int *bsum, int nb, int *total, int *out_end) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  int acc = 0;
+  for (int k = 0; k < nb; k++) { int v = bsum[k]; bsum[k] = acc; acc += v; }
+  *total = acc;
+  if (out_end) *out_end = acc;
+}
+template <int B>
+__global__ void k_scan_add(int *out, const int *bsum, int64_t n) {
+  int64_t i = (int64_t)blockIdx.x * B + threadIdx.x;
+  if (i < n) out[i] += bsum[blockIdx.x];
+}
+
+// ---------------------------------------------------------------------
+// Fill the BP candidate list (src/neighbor_list.F90:183-223) and scatter
+// incidences.  bp_mp holds replica-local 1-based ids as the reference.
+__global__ void k_bp_fill(Params P, Ptrs D, const int *mask) {
+  int a = blockIdx.x * blockDim.x + threadIdx.x, r = blockIdx.y;
+  if (a >= P.N || (mask && !mask[r])) return;
+  const int N = P.N;
+  size_t g = (size_t)r * N + a;
+  int nb = D.n_bp_tmp[g], off = D.bp_off[g];
+  Topo ti = D.topo[a];
+  for (int k = 0; k < nb; k++) {
+    int b = D.bp_tmp[g * D.maxbpb + k];
+    int gb = off + k;
+    if (gb >= D.maxbp) { D.flags[0] = 1; continue; }
+    Topo tj = D.topo[b];
+    int i = ti.loc, j = tj.loc, imp = a + 1, jmp = b + 1;
+    int mp[7];
+    mp[0] = imp;
+    mp[1] = jmp;
+    mp[2] = (i == 1) ? imp + ti.len - 1 : imp - 1;
+    mp[3] = (j == 1) ? jmp + tj.len - 1 : jmp - 1;
+    mp[4] = (i == ti.len) ? imp + 1 - ti.len : imp + 1;
+    // the reference compares j with the length of chain i here
+    // (src/neighbor_list.F90:212); kept unless it leaves the system
+    int m6 = (j == ti.len) ? jmp + 1 - tj.len : jmp + 1;
+    if (P.fix_h4 || m6 < 1 || m6 > N) m6 = (j == tj.len) ? jmp + 1 - tj.len : jmp + 1;
+    mp[5] = m6;
+    int bt;
+    int h = bp_eligible(P, D, a, b, &bt);
+    mp[6] = bt;
+    int *dst = D.bp_mp + (size_t)gb * 8;
+    for (int q = 0; q < 7; q++) dst[q] = mp[q];
+    dst[7] = r;
+    D.bp_coef[2 * (size_t)gb] = D.bp3_dH[h];
+    D.bp_coef[2 * (size_t)gb + 1] = D.bp3_dS[h];
+    for (int q = 0; q < 6; q++) {
+      size_t gq = (size_t)r * N + (mp[q] - 1);
+      int slot = atomicAdd(&D.n_inc[gq], 1);
+      if (slot < D.maxinc) D.inc[((size_t)r * D.maxinc + slot) * N + (mp[q] - 1)] = gb * 8 + q;
+      else { D.flags[0] = 1; atomicMax(&D.flags[6], slot + 1); }
+    }
+  }
+}
+// deterministic incidence order: sort each bead's list by (candidate, role)
+__global__ void k_inc_sort(Params P, Ptrs D, const int *mask) {
+  int a = blockIdx.x * blockDim.x + threadIdx.x, r = blockIdx.y;
+  if (a >= P.N || (mask && !mask[r])) return;
+  size_t g = (size_t)r * P.N + a;
+  int n = min(D.n_inc[g], D.maxinc);
+  size_t base = (size_t)r * D.maxinc * P.N + a, st = P.N;
+  for (int x = 1; x < n; x++) {
+    int v = D.inc[base + x * st], y = x - 1;
+    while (y >= 0 && D.inc[base + y * st] > v) { D.inc[base + (y + 1) * st] = D.inc[base + y * st]; y--; }
+    D.inc[base + (y + 1) * st] = v;
+  }
+  D.nt_cnt[g] = -P.max_bp_per_nt;
+}
+
+// per-nucleotide formed-pair counter, reset before each BP evaluation
+__global__ void k_reset_cnt(Params P, Ptrs D, const int *mask) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x, r = blockIdx.y;
+  if (i >= P.N || (mask && !mask[r])) return;
+  D.nt_cnt[(size_t)r * P.N + i] = -P.max_bp_per_nt;
+}
+
+// ---------------------------------------------------------------------
+// BP evaluation per candidate (src/force_bp_limit_triplet.F90:85-256 and
+// src/force_bp_triplet.F90).  mc[r]: MC step for replica r.  ecT (may be
+// null): temperature override per replica (energy at another condition).
+template <typename T>
+__global__ void k_bp_eval(Params P, Ptrs D, int nbp_total, const int *mc, const double *ecT,
+                          const int *mask) {
+  int gb = blockIdx.x * blockDim.x + threadIdx.x;
+  if (gb >= nbp_total) return;
+  const int *mp = D.bp_mp + (size_t)gb * 8;
+  int r = mp[7];
+  if (mask && !mask[r]) return;
+  const int N = P.N;
+  D.bp_status[gb] = 0;
+  D.ene_bp[gb] = 0.0;
+  if (!mc[r] && P.nstep_bp_mc > 0 && !D.bp_status_mc[gb]) return;
+  double tK = ecT ? ecT[r] : D.tK[r];
+  double dG = D.bp_coef[2 * (size_t)gb] - tK * D.bp_coef[2 * (size_t)gb + 1];
+  if (dG >= 0.0) return;
+  const double *box = D.box + 3 * r;
+  const double4 *x = D.pos + (size_t)r * N;
+  double4 p1 = x[mp[0] - 1], p2 = x[mp[1] - 1];
+  V3<double> v12 = pvec(P, box, p1, p2), v13 = pvec(P, box, p1, x[mp[2] - 1]),
+             v15 = pvec(P, box, p1, x[mp[4] - 1]), v42 = pvec(P, box, x[mp[3] - 1], p2),
+             v62 = pvec(P, box, x[mp[5] - 1], p2);
+  const BpPara &bp = P.bpp[mp[6]];
+  T u;
+  V3<T> fb[6];
+  if (!bp_geom<T>(bp, v12, v13, v15, v42, v62, u, fb)) return;
+  double e = bp.U0 * dG * exp(-(double)u);
+  if (e <= P.bp_cutoff_energy) {
+    D.bp_status[gb] = 1;
+    D.ene_bp[gb] = e;
+    double *fo = D.for_bp + (size_t)gb * 18;
+    for (int q = 0; q < 6; q++) {
+      fo[3 * q] = e * (double)fb[q].x;
+      fo[3 * q + 1] = e * (double)fb[q].y;
+      fo[3 * q + 2] = e * (double)fb[q].z;
+    }
+    atomicAdd(&D.nt_cnt[(size_t)r * N + mp[0] - 1], 1);
+    atomicAdd(&D.nt_cnt[(size_t)r * N + mp[1] - 1], 1);
+  }
+}
+
+// ---------------------------------------------------------------------
+// Limit MC, parity form: the serial algorithm of
+// src/force_bp_limit_triplet.F90:259-330 on the injected MT words of the
+// replica, one thread.  Consumes words exactly as genrand_double3 /
+// genrand_double1 would.
+__global__ void k_mc_serial(Params P, Ptrs D, const int *mc, const double *ecT, int *ntlist,
+                            int *bpseq, const int *mask) {
+  if (threadIdx.x != 0) return;
+  int r = blockIdx.x;
+  if (mask && !mask[r]) return;
+  if (!mc[r]) return;
+  const int N = P.N;
+  double tK = ecT ? ecT[r] : D.tK[r];
+  const double BOLTZ = 1.380649e-23 * ((1.0 / 4184.0) * 6.02214076e23);
+  double beta = 1.0 / (BOLTZ * tK);
+  int b0 = D.bp_off[(size_t)r * N], nb = D.bp_off[(size_t)(r + 1) * N] - b0;
+  const uint32_t *w = D.tape + D.tape_off[r];
+  int64_t pos = D.tape_pos[r], lim = D.tape_off[r + 1] - D.tape_off[r];
+  int *cnt = D.nt_cnt + (size_t)r * N;
+  int *nl = ntlist + (size_t)r * (N + 1);
+  int *bs = bpseq + b0 + r;
+  for (;;) {
+    int nnt = 0;
+    for (int k = 0; k < N; k++) if (cnt[k] > 0) nl[++nnt] = k + 1;
+    if (nnt == 0) break;
+    if (pos + 2 > lim) { D.flags[1] = 1; break; }
+    double rnd = mt_d3(w[pos], w[pos + 1]); pos += 2;
+    int ntd = nl[(int)ceil(rnd * nnt)];
+    int ns = 0;
+    for (int k = 0; k < nb; k++) {
+      int gb = b0 + k;
+      if (D.bp_status[gb]) {
+        const int *mp = D.bp_mp + (size_t)gb * 8;
+        if (mp[0] == ntd || mp[1] == ntd) bs[++ns] = gb;
+      }
+    }
+    for (int k = 1; k <= ns; k++) {
+      if (pos + 2 > lim) { D.flags[1] = 1; break; }
+      rnd = mt_d3(w[pos], w[pos + 1]); pos += 2;
+      int sw = (int)ceil(rnd * ns);
+      int sv = bs[k]; bs[k] = bs[sw]; bs[sw] = sv;
+    }
+    if (ns == 0) { D.flags[1] = 2; break; }
+    int del = bs[1];
+    for (int k = 2; k <= ns; k++) {
+      int jb = bs[k];
+      double ratio = exp((D.ene_bp[jb] - D.ene_bp[del]) * beta);
+      if (pos + 2 > lim) { D.flags[1] = 1; break; }
+      rnd = mt_d1(w[pos], w[pos + 1]); pos += 2;
+      if (rnd < ratio) del = jb;
+    }
+    D.bp_status[del] = 0;
+    const int *mp = D.bp_mp + (size_t)del * 8;
+    cnt[mp[0] - 1] -= 1;
+    cnt[mp[1] - 1] -= 1;
+  }
+  D.tape_pos[r] = pos;
+}
+
+// Limit MC, performance form.  The excess-removal process of the reference
+// decomposes over connected components of the formed-pair graph: a step
+// only changes counts inside the component of the chosen nucleotide, and
+// the uniform choice among all excess nucleotides restricted to one
+// component is uniform within it.  Each component is therefore resolved
+// independently by one thread with its own counter-RNG stream.
+__device__ __forceinline__ int uf_find(int *p, int x) {
+  while (true) {
+    int q = p[x];
+    if (q == x) return x;
+    int qq = p[q];
+    if (qq != q) atomicCAS(&p[x], q, qq);
+    x = q;
+  }
+}
+__global__ void k_uf_init(Params P, Ptrs D, int *cnt_exc, int *fill, const int *mask) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x, r = blockIdx.y;
+  if (i >= P.N || (mask && !mask[r])) return;
+  size_t g = (size_t)r * P.N + i;
+  D.comp[g] = i;
+  cnt_exc[g] = 0;
+  fill[g] = 0;
+}
+__global__ void k_uf_union(Params P, Ptrs D, int nbp_total, const int *mc, const int *mask) {
+  int gb = blockIdx.x * blockDim.x + threadIdx.x;
+  if (gb >= nbp_total || !D.bp_status[gb]) return;
+  const int *mp = D.bp_mp + (size_t)gb * 8;
+  int r = mp[7];
+  if ((mask && !mask[r]) || !mc[r]) return;
+  int *p = D.comp + (size_t)r * P.N;
+  int a = mp[0] - 1, b = mp[1] - 1;
+  while (true) {
+    a = uf_find(p, a);
+    b = uf_find(p, b);
+    if (a == b) return;
+    if (a > b) { int t = a; a = b; b = t; }
+    if (atomicCAS(&p[b], b, a) == b) return;  // link larger root under smaller
+  }
+}
+__global__ void k_uf_count(Params P, Ptrs D, int *cnt_exc, const int *mc, const int *mask) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x, r = blockIdx.y;
+  if (i >= P.N || (mask && !mask[r]) || !mc[r]) return;
+  size_t g = (size_t)r * P.N + i;
+  if (D.nt_cnt[g] > 0) {
+    int root = uf_find(D.comp + (size_t)r * P.N, i);
+    atomicAdd(&cnt_exc[(size_t)r * P.N + root], 1);
+  }
+}
+__global__ void k_uf_fill(Params P, Ptrs D, const int *exc_off, int *fill, int *exc_list,
+                          const int *mc, const int *mask) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x, r = blockIdx.y;
+  if (i >= P.N || (mask && !mask[r]) || !mc[r]) return;
+  size_t g = (size_t)r * P.N + i;
+  if (D.nt_cnt[g] > 0) {
+    int root = uf_find(D.comp + (size_t)r * P.N, i);
+    size_t gr = (size_t)r * P.N + root;
+    int slot = atomicAdd(&fill[gr], 1);
+    exc_list[exc_off[gr] + slot] = i;
+  }
+}
+__global__ void k_mc_comp(Params P, Ptrs D, const int *exc_off, const int *cnt_exc,
+                          int *exc_list, const int *mc, const double *ecT, uint64_t seed,
+                          int64_t step, const int *mask) {
+  int root = blockIdx.x * blockDim.x + threadIdx.x, r = blockIdx.y;
+  if (root >= P.N || (mask && !mask[r]) || !mc[r]) return;
+  const int N = P.N;
+  size_t gr = (size_t)r * N + root;
+  int ne = cnt_exc[gr];
+  if (ne == 0) return;
+  int *E = exc_list + exc_off[gr];
+  for (int x = 1; x < ne; x++) {
+    int v = E[x], y = x - 1;
+    while (y >= 0 && E[y] > v) { E[y + 1] = E[y]; y--; }
+    E[y + 1] = v;
+  }
+  double tK = ecT ? ecT[r] : D.tK[r];
+  const double BOLTZ = 1.380649e-23 * ((1.0 / 4184.0) * 6.02214076e23);
+  double beta = 1.0 / (BOLTZ * tK);
+  int *cnt = D.nt_cnt + (size_t)r * N;
+  uint32_t k0 = (uint32_t)seed, k1 = (uint32_t)(seed >> 32) ^ (0x5bd1e995u * (uint32_t)(r + 1));
+  uint32_t draw = 0;
+  auto uni = [&]() {
+    U4 o = philox(U4{(uint32_t)step, (uint32_t)(step >> 32) ^ 0x4d43u, (uint32_t)root, draw++}, k0, k1);
+    return u01(o.x, o.y);
+  };
+  const int CAP = 64;
+  int seqv[CAP];
+  size_t st = N;
+  for (;;) {
+    int nact = 0;
+    for (int k = 0; k < ne; k++) nact += cnt[E[k]] > 0;
+    if (nact == 0) break;
+    int pick = (int)ceil(uni() * nact), ntd = -1;
+    for (int k = 0, c = 0; k < ne; k++)
+      if (cnt[E[k]] > 0 && ++c == pick) { ntd = E[k]; break; }
+    if (ntd < 0) break;
+    int ni = min(D.n_inc[(size_t)r * N + ntd], D.maxinc), ns = 0;
+    const int *inc = D.inc + (size_t)r * D.maxinc * N + ntd;
+    for (int k = 0; k < ni && ns < CAP; k++) {
+      int e = inc[k * st], gb = e >> 3, role = e & 7;
+      if (role <= 1 && D.bp_status[gb]) seqv[ns++] = gb;
+    }
+    for (int k = 0; k < ns; k++) {
+      int sw = (int)ceil(uni() * ns) - 1;
+      int sv = seqv[k]; seqv[k] = seqv[sw]; seqv[sw] = sv;
+    }
+    if (ns == 0) { D.flags[1] = 2; break; }
+    int del = seqv[0];
+    for (int k = 1; k < ns; k++) {
+      int jb = seqv[k];
+      double ratio = exp((D.ene_bp[jb] - D.ene_bp[del]) * beta);
+      if (uni() < ratio) del = jb;
+    }
+    D.bp_status[del] = 0;
+    const int *mp = D.bp_mp + (size_t)del * 8;
+    cnt[mp[0] - 1] -= 1;
+    cnt[mp[1] - 1] -= 1;
+  }
+}
+
+// ---------------------------------------------------------------------
+// Parity noise: 3N (or `count`) normals per replica with the polar
+// Box-Muller of src/rnd_boxmuller.F90 on the replica's MT words, replicas
+// in order, sharing one cached value as the reference does.
+__global__ void k_noise_parity(Params P, Ptrs D, int count, int dest, const int *mask) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  for (int r = 0; r < P.R; r++) {
+    if (mask && !mask[r]) continue;
+    const uint32_t *w = D.tape + D.tape_off[r];
+    int64_t pos = D.tape_pos[r], lim = D.tape_off[r + 1] - D.tape_off[r];
+    double *out = D.noise + (size_t)r * P.N * 3;
+    for (int k = 0; k < count; k++) {
+      double val;
+      if (D.bm_cache[0] == 0.0) {
+        D.bm_cache[0] = 1.0;
+        double vx, vy, r2;
+        for (;;) {
+          if (pos + 4 > lim) { D.flags[1] = 1; D.tape_pos[r] = pos; return; }
+          vx = 2.0 * mt_d1(w[pos], w[pos + 1]) - 1.0;
+          vy = 2.0 * mt_d1(w[pos + 2], w[pos + 3]) - 1.0;
+          pos += 4;
+          r2 = vx * vx + vy * vy;
+          if (r2 < 1.0 && r2 > 0.0) break;
+        }
+        double rf = sqrt(-2.0 * log(r2) / r2);
+        D.bm_cache[1] = vx * rf;
+        val = vy * rf;
+      } else {
+        D.bm_cache[0] = 0.0;
+        val = D.bm_cache[1];
+      }
+      out[k] = val;
+    }
+    D.tape_pos[r] = pos;
+  }
+  (void)dest;
+}
+
+// ---------------------------------------------------------------------
+// Owner-computes force on bead a of replica r: bonded terms recomputed
+// per owner, WCA / DH over the full neighbour lists, 6-bead BP forces
+// gathered through the incidence list (no atomics on forces).
+template <typename T>
+__device__ __forceinline__ V3<double> bead_force(const Params &P, const Ptrs &D, int r, int a) {
+  const int N = P.N;
+  const double *box = D.box + 3 * r;
+  const double4 *x = D.pos + (size_t)r * N;
+  double4 pa = x[a];
+  Topo ta = D.topo[a];
+  int loc = ta.loc, n = ta.len, st = ta.start;
+  bool circ = topo_circ(ta);
+  V3<double> F{0.0, 0.0, 0.0};
+  auto at = [&](int s, int k) -> int {  // bead of term starting at local s, position k
+    int l = s + k;
+    if (l > n) l -= n;
+    return st + l - 1;
+  };
+  auto valid = [&](int &s, int m) -> bool {  // term of m beads starting at local s
+    if (circ) {
+      if (n < m) return false;
+      if (s < 1) s += n;
+      return true;
+    }
+    return s >= 1 && s + m - 1 <= n;
+  };
+  // bonds (src/list_local.F90:24-58)
+  for (int p = 0; p < 2; p++) {
+    int s = loc - p;
+    if (!valid(s, 2)) continue;
+    V3<T> v = cvt<T>(pvec(P, box, x[at(s, 0)], x[at(s, 1)]));
+    V3<T> f[2];
+    bond_force<T>(P, v, f);
+    F = F + cvt<double>(f[p]);
+  }
+  for (int p = 0; p < 3; p++) {
+    int s = loc - p;
+    if (!valid(s, 3)) continue;
+    double4 p0 = x[at(s, 0)], p1 = x[at(s, 1)], p2 = x[at(s, 2)];
+    V3<T> v12 = cvt<T>(pvec(P, box, p0, p1)), v32 = cvt<T>(pvec(P, box, p2, p1));
+    V3<T> f[3];
+    angle_force<T>(P, v12, v32, f);
+    F = F + cvt<double>(f[p]);
+  }
+  if (P.dih_type) {
+    for (int p = 0; p < 4; p++) {
+      int s = loc - p;
+      if (!valid(s, 4)) continue;
+      double4 p0 = x[at(s, 0)], p1 = x[at(s, 1)], p2 = x[at(s, 2)], p3 = x[at(s, 3)];
+      V3<T> vij = cvt<T>(pvec(P, box, p0, p1)), vkj = cvt<T>(pvec(P, box, p2, p1)),
+            vkl = cvt<T>(pvec(P, box, p2, p3));
+      V3<T> f[4];
+      dih_force<T>(P, vij, vkj, vkl, f);
+      F = F + cvt<double>(f[p]);
+    }
+  }
+  size_t g = (size_t)r * N + a;
+  // WCA
+  {
+    int nw = min(D.n_wca[g], D.maxw);
+    const int *nb = D.nb_wca + (size_t)r * D.maxw * N + a;
+    for (int k = 0; k < nw; k++) {
+      int b = nb[(size_t)k * N];
+      V3<T> v = cvt<T>(pvec(P, box, pa, x[b]));
+      F = F + cvt<double>(wca_force<T>(P, v));
+    }
+  }
+  // Debye-Hueckel
+  if (P.ele_on) {
+    int ne = min(D.n_ele[g], D.maxnb);
+    const int *nb = D.nb_ele + (size_t)r * D.maxnb * N + a;
+    T coef = (T)D.ele_coef[r], rc2 = (T)(D.ele_cut[r] * D.ele_cut[r]), rl = (T)(1.0 / D.lambdaD[r]);
+    V3<T> acc{0, 0, 0};
+    V3<double> accd{0.0, 0.0, 0.0};
+    for (int k = 0; k < ne; k++) {
+      int b = nb[(size_t)k * N];
+      V3<T> v = cvt<T>(pvec(P, box, pa, x[b]));
+      V3<T> f = dh_force<T>(coef, rc2, rl, v);
+      accd = accd + cvt<double>(f);
+    }
+    (void)acc;
+    F = F + accd;
+  }
+  // base pairs
+  {
+    int ni = min(D.n_inc[g], D.maxinc);
+    const int *inc = D.inc + (size_t)r * D.maxinc * N + a;
+    for (int k = 0; k < ni; k++) {
+      int e = inc[(size_t)k * N], gb = e >> 3, role = e & 7;
+      if (!D.bp_status[gb]) continue;
+      const double *fo = D.for_bp + (size_t)gb * 18 + 3 * role;
+      F.x += fo[0]; F.y += fo[1]; F.z += fo[2];
+    }
+  }
+  return F;
+}
+
+// Force (and optionally the GJF-2GJ update, src/job_md.F90:415-444) per
+// bead; block-level top-2 of the accumulated displacement for the skin
+// test (src/job_md.F90:376-386).
+template <typename T, int B>
+__global__ void __launch_bounds__(B) k_force_int(Params P, Ptrs D, int integrate, int parity,
+                                                uint64_t seed, int64_t step, const int *mask) {
+  __shared__ double s1[B], s2[B];
+  int a = blockIdx.x * B + threadIdx.x, r = blockIdx.y;
+  const int N = P.N;
+  double d2 = 0.0;
+  bool act = a < N && !(mask && !mask[r]);
+  if (act) {
+    size_t g = (size_t)r * N + a;
+    V3<double> F = bead_force<T>(P, D, r, a);
+    D.nt_cnt[g] = -P.max_bp_per_nt;
+    if (!integrate) {
+      D.frc[g] = double4{F.x, F.y, F.z, 0.0};
+    } else {
+      double n0, n1, n2;
+      if (parity) {
+        const double *nz = D.noise + g * 3;
+        n0 = nz[0]; n1 = nz[1]; n2 = nz[2];
+      } else {
+        U4 o = philox(U4{(uint32_t)step, (uint32_t)(step >> 32), (uint32_t)a, 0x6e6f6973u},
+                      (uint32_t)seed, (uint32_t)(seed >> 32) ^ (0x9e3779b9u * (uint32_t)(r + 1)));
+        U4 o2 = philox(U4{(uint32_t)step, (uint32_t)(step >> 32), (uint32_t)a, 0x6e6f6974u},
+                       (uint32_t)seed, (uint32_t)(seed >> 32) ^ (0x9e3779b9u * (uint32_t)(r + 1)));
+        double u1 = u01(o.x, o.y), u2 = u01(o.z, o.w), u3 = u01(o2.x, o2.y), u4 = u01(o2.z, o2.w);
+        double rr = sqrt(-2.0 * log(u1)), r2 = sqrt(-2.0 * log(u3));
+        n0 = rr * cos(6.283185307179586 * u2);
+        n1 = rr * sin(6.283185307179586 * u2);
+        n2 = r2 * cos(6.283185307179586 * u4);
+      }
+      double cr = D.md_cr[g], ca = D.md_a[a], cf = D.md_cf[a], cx = D.md_cx[a];
+      double4 v = D.vel[g], ac = D.acc[g], p = D.pos[g], mv = D.move[g];
+      double b0 = cr * n0, b1 = cr * n1, b2 = cr * n2;
+      v.x = ca * v.x + cf * F.x + (ac.x + b0);
+      v.y = ca * v.y + cf * F.y + (ac.y + b1);
+      v.z = ca * v.z + cf * F.z + (ac.z + b2);
+      ac.x = b0; ac.y = b1; ac.z = b2;
+      double dx = cx * v.x, dy = cx * v.y, dz = cx * v.z;
+      D.vel[g] = v;
+      D.acc[g] = ac;
+      D.frc[g] = double4{F.x, F.y, F.z, 0.0};
+      mv.x += dx; mv.y += dy; mv.z += dz;
+      D.move[g] = mv;
+      D.pos[g] = double4{p.x + dx, p.y + dy, p.z + dz, p.w};
+      d2 = dot_nc(mv.x, mv.y, mv.z, mv.x, mv.y, mv.z);
+    }
+  }
+  if (!integrate) return;
+  // block top-2 of d2
+  s1[threadIdx.x] = d2;
+  s2[threadIdx.x] = 0.0;
+  __syncthreads();
+  for (int off = B / 2; off > 0; off >>= 1) {
+    if (threadIdx.x < off) {
+      double a1 = s1[threadIdx.x], a2 = s2[threadIdx.x], b1 = s1[threadIdx.x + off], b2 = s2[threadIdx.x + off];
+      double m1 = a1 > b1 ? a1 : b1;
+      double m2 = a1 > b1 ? (a2 > b1 ? a2 : b1) : (a1 > b2 ? a1 : b2);
+      s1[threadIdx.x] = m1;
+      s2[threadIdx.x] = m2;
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    D.blk_top2[((size_t)r * gridDim.x + blockIdx.x) * 2] = s1[0];
+    D.blk_top2[((size_t)r * gridDim.x + blockIdx.x) * 2 + 1] = s2[0];
+  }
+}
+
+// skin test per replica: sqrt(d2max) + sqrt(d2max_2nd) > nl_margin
+__global__ void k_skin(Params P, Ptrs D, int nblk) {
+  int r = blockIdx.x;
+  if (threadIdx.x != 0) return;
+  double m1 = 0.0, m2 = 0.0;
+  for (int b = 0; b < nblk; b++) {
+    double a1 = D.blk_top2[((size_t)r * nblk + b) * 2], a2 = D.blk_top2[((size_t)r * nblk + b) * 2 + 1];
+    double n1 = m1 > a1 ? m1 : a1;
+    double n2 = m1 > a1 ? (m2 > a1 ? m2 : a1) : (m1 > a2 ? m1 : a2);
+    m1 = n1;
+    m2 = n2;
+  }
+  D.rebuild_flag[r] = (sqrt(m1) + sqrt(m2) > P.nl_margin) ? 1 : 0;
+}
+
+__global__ void k_clear_move(Params P, Ptrs D, const int *mask) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x, r = blockIdx.y;
+  if (i >= P.N || (mask && !mask[r])) return;
+  D.move[(size_t)r * P.N + i] = double4{0, 0, 0, 0};
+}
+
+// velocities / initial noise terms (src/job_md.F90:157-176); parity uses
+// the noise buffer filled twice (velocities then accelerations).
+__global__ void k_init_vel(Params P, Ptrs D, int which, int parity, uint64_t seed,
+                           const int *mask) {
+  int a = blockIdx.x * blockDim.x + threadIdx.x, r = blockIdx.y;
+  if (a >= P.N || (mask && !mask[r])) return;
+  size_t g = (size_t)r * P.N + a;
+  double n0, n1, n2;
+  if (parity) {
+    n0 = D.noise[3 * g]; n1 = D.noise[3 * g + 1]; n2 = D.noise[3 * g + 2];
+  } else {
+    U4 o = philox(U4{0xffffffffu, (uint32_t)which, (uint32_t)a, 0x696e6974u}, (uint32_t)seed,
+                  (uint32_t)(seed >> 32) ^ (0x9e3779b9u * (uint32_t)(r + 1)));
+    U4 o2 = philox(U4{0xfffffffeu, (uint32_t)which, (uint32_t)a, 0x696e6974u}, (uint32_t)seed,
+                   (uint32_t)(seed >> 32) ^ (0x9e3779b9u * (uint32_t)(r + 1)));
+    double rr = sqrt(-2.0 * log(u01(o.x, o.y))), r2 = sqrt(-2.0 * log(u01(o2.x, o2.y)));
+    n0 = rr * cos(6.283185307179586 * u01(o.z, o.w));
+    n1 = rr * sin(6.283185307179586 * u01(o.z, o.w));
+    n2 = r2 * cos(6.283185307179586 * u01(o2.z, o2.w));
+  }
+  if (which == 0) {
+    const double BOLTZ = 1.380649e-23 * ((1.0 / 4184.0) * 6.02214076e23);
+    double c = sqrt(D.tK[r] * BOLTZ / D.mass[a]);
+    D.vel[g] = double4{c * n0, c * n1, c * n2, 0.0};
+  } else {
+    double cr = D.md_cr[g];
+    D.acc[g] = double4{cr * n0, cr * n1, cr * n2, 0.0};
+  }
+}
+
+// rescale velocities of replica r by f (replica exchange,
+// src/replica_exchange.F90:175-213)
+__global__ void k_scale_vel(Params P, Ptrs D, int r, double f, int divide) {
+  int a = blockIdx.x * blockDim.x + threadIdx.x;
+  if (a >= P.N) return;
+  double4 &v = D.vel[(size_t)r * P.N + a];
+  if (divide) { v.x = v.x / f; v.y = v.y / f; v.z = v.z / f; }
+  else { v.x = f * v.x; v.y = f * v.y; v.z = f * v.z; }
+}
+
+// ---------------------------------------------------------------------
+// Energies (src/energy_sumup.F90 and the terms): per-block partial sums,
+// reduced in a fixed order.  Terms: 1 bond, 2 angle, 3 dihedral, 4 BP,
+// 5 WCA, 6 DH.  Each term is counted once (first bead of a bonded term,
+// lower bead of a pair, candidate owner for BP).
+template <int B>
+__global__ void __launch_bounds__(B) k_energy(Params P, Ptrs D, const double *ecT,
+                                              const double *eCoef, const double *eLam,
+                                              double *partial, const int *mask, int bp_masked) {
+  __shared__ double s[7][B];
+  int a = blockIdx.x * B + threadIdx.x, r = blockIdx.y;
+  const int N = P.N;
+  double e[7] = {0, 0, 0, 0, 0, 0, 0};
+  if (a < N && !(mask && !mask[r])) {
+    const double *box = D.box + 3 * r;
+    const double4 *x = D.pos + (size_t)r * N;
+    Topo ta = D.topo[a];
+    int loc = ta.loc, n = ta.len, st = ta.start;
+    bool circ = topo_circ(ta);
+    auto at = [&](int k) { int l = loc + k; if (l > n) l -= n; return st + l - 1; };
+    // bond starting at this bead
+    if (circ || loc + 1 <= n) {
+      V3<double> v = pvec(P, box, x[at(0)], x[at(1)]);
+      double d = sqrt(dot(v, v));
+      e[1] += 0.5 * P.bond_k * ((d - P.bond_r0) * (d - P.bond_r0));
+    }
+    if ((circ && n >= 3) || loc + 2 <= n) {
+      V3<double> v12 = pvec(P, box, x[at(0)], x[at(1)]), v32 = pvec(P, box, x[at(2)], x[at(1)]);
+      double co = dot(v32, v12) / sqrt(dot(v12, v12) * dot(v32, v32));
+      if (P.angl_reb) {
+        e[2] += 0.5 * P.angl_k * ((co - P.angl_co0) * (co - P.angl_co0)) / (1.0 - co * co);
+      } else {
+        if (co > 1.0) co = 1.0; else if (co < -1.0) co = -1.0;
+        double t = acos(co);
+        e[2] += 0.5 * P.angl_k * ((t - P.angl_t0) * (t - P.angl_t0));
+      }
+    }
+    if (P.dih_type && ((circ && n >= 4) || loc + 3 <= n)) {
+      V3<double> v12 = pvec(P, box, x[at(1)], x[at(0)]), v23 = pvec(P, box, x[at(2)], x[at(1)]),
+                 v34 = pvec(P, box, x[at(3)], x[at(2)]);
+      V3<double> c1 = cross(v12, v23), c2 = cross(v23, v34);
+      double dih = atan2(sqrt(dot(v23, v23)) * dot(v12, c2), dot(c1, c2));
+      if (P.dih_type == 2) e[3] -= P.dih_k * exp(-0.5 * P.dih_w * ((dih - P.dih_p0) * (dih - P.dih_p0)));
+      else e[3] += P.dih_k * (1.0 + cos(dih + P.dih_p0));
+    }
+    size_t g = (size_t)r * N + a;
+    // BP: candidates owned by this bead (role 0)
+    {
+      int ni = min(D.n_inc[g], D.maxinc);
+      const int *inc = D.inc + (size_t)r * D.maxinc * N + a;
+      for (int k = 0; k < ni; k++) {
+        int q = inc[(size_t)k * N];
+        // masked sum after the limit MC (src/energy_bp_limit_triplet.F90:195),
+        // plain sum otherwise (src/energy_bp_triplet.F90:88)
+        if ((q & 7) == 0 && (D.bp_status[q >> 3] || !bp_masked)) e[4] += D.ene_bp[q >> 3];
+      }
+    }
+    {
+      int nw = min(D.n_wca[g], D.maxw);
+      const int *nb = D.nb_wca + (size_t)r * D.maxw * N + a;
+      for (int k = 0; k < nw; k++) {
+        int b = nb[(size_t)k * N];
+        if (b < a) continue;
+        V3<double> v = pvec(P, box, x[a], x[b]);
+        double d = sqrt(dot(v, v));
+        if (d >= P.wca_sigma) continue;
+        d = P.wca_sigma / d;
+        double d6 = d * d * d * d * d * d;
+        e[5] += P.wca_eps * (d6 * d6 - 2 * d6 + 1.0);
+      }
+    }
+    if (P.ele_on) {
+      int ne = min(D.n_ele[g], D.maxnb);
+      const int *nb = D.nb_ele + (size_t)r * D.maxnb * N + a;
+      double coef = eCoef ? eCoef[r] : D.ele_coef[r];
+      double rc = 1.0 / (eLam ? eLam[r] : D.lambdaD[r]);
+      for (int k = 0; k < ne; k++) {
+        int b = nb[(size_t)k * N];
+        if (b < a) continue;
+        V3<double> v = pvec(P, box, x[a], x[b]);
+        double dist = sqrt(dot(v, v));
+        if (dist > D.ele_cut[r]) continue;
+        e[6] += coef / dist * exp(-dist * rc);
+      }
+    }
+  }
+  (void)ecT;
+  for (int k = 1; k < 7; k++) s[k][threadIdx.x] = e[k];
+  __syncthreads();
+  for (int off = B / 2; off > 0; off >>= 1) {
+    if (threadIdx.x < off)
+      for (int k = 1; k < 7; k++) s[k][threadIdx.x] += s[k][threadIdx.x + off];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0)
+    for (int k = 1; k < 7; k++) partial[((size_t)r * gridDim.x + blockIdx.x) * 8 + k] = s[k][0];
+}
+__global__ void k_energy_reduce(const double *partial, int nblk, double *out) {
+  int r = blockIdx.x;
+  if (threadIdx.x != 0) return;
+  double e[11] = {0};
+  for (int b = 0; b < nblk; b++)
+    for (int k = 1; k < 7; k++) e[k] += partial[((size_t)r * nblk + b) * 8 + k];
+  double t = 0.0;
+  for (int k = 1; k <= 10; k++) t += e[k];
+  e[0] = t;
+  for (int k = 0; k < 11; k++) out[(size_t)r * 16 + k] = e[k];
+}
+
+}  // namespace sis

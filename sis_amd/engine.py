@@ -1,0 +1,192 @@
+"""Host-side mirror of the reference's hot-path interface over the C ABI.
+
+Method names follow the reference subroutines they replace:
+  neighbor_list(irep)             src/neighbor_list.F90:1
+  force(irep, bp_mc)              src/force.F90:1
+  energy_sumup(irep, tempK, ...)  src/energy_sumup.F90:1
+  init_velocities()               src/job_md.F90:157-176
+  step(nsteps)                    src/job_md.F90:360-460 (loop body)
+  exchange()                      src/energy_replica.F90 + src/replica_exchange.F90
+Every call goes through libsis_gpu.so; there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import abi
+from .abi import SisOptions, SisTopo, params_to_struct
+
+MODES = {"auto": 0, "phases": 1, "persistent": 2, "batched": 3}
+
+
+class SisGpuError(RuntimeError):
+    pass
+
+
+_LIB = None
+
+
+def lib() -> C.CDLL:
+    global _LIB
+    if _LIB is None:
+        _LIB = abi.load_library()
+    return _LIB
+
+
+def _check(rc: int):
+    if rc != 0:
+        msg = lib().sis_gpu_last_error().decode(errors="replace")
+        raise SisGpuError(f"sis_gpu error {rc}: {msg}")
+
+
+class Engine:
+    def __init__(self, sysd: dict, precision: int = 0, parity: bool = False, mode: str | int = "phases",
+                 seed: int = 20241, exchange_every: int = 0, rebuild_every: int = 0, block: int = 0,
+                 max_nbr: int = 0):
+        self.sysd = sysd
+        self.nmp = int(np.sum(sysd["nmp_chain"]))
+        self.nrep = int(sysd["nrep"])
+        self._keep = dict(
+            nmp_chain=np.ascontiguousarray(sysd["nmp_chain"], np.int32),
+            moltype=np.ascontiguousarray(sysd["moltype"], np.int32),
+            seq=np.ascontiguousarray(sysd["seq"], np.int32),
+        )
+        t = SisTopo()
+        t.nchains = int(sysd["nchains"])
+        t.nmp = self.nmp
+        t.nmp_chain = abi.iptr(self._keep["nmp_chain"])
+        t.moltype = abi.iptr(self._keep["moltype"])
+        t.seq = abi.iptr(self._keep["seq"])
+        t.no_charge = None
+        t.n_no_charge = 0
+        self.topo = t
+        self.prm = params_to_struct(sysd["params"])
+        o = SisOptions()
+        o.nrep = self.nrep
+        o.precision = precision
+        o.parity = 1 if parity else 0
+        o.mode = MODES.get(mode, mode) if isinstance(mode, str) else int(mode)
+        o.block = block
+        o.max_nbr = max_nbr
+        o.rebuild_every = rebuild_every
+        o.exchange_every = exchange_every
+        o.seed = seed
+        self.opt = o
+        h = C.c_void_p()
+        _check(lib().sis_gpu_create(C.byref(t), C.byref(self.prm), C.byref(o), C.byref(h)))
+        self.h = h
+        if self.nrep > 1 or np.any(np.asarray(sysd["temps"]) != sysd["params"]["tempK"]):
+            self.set_rep_conditions(np.asarray(sysd["temps"], float), None)
+        self.set_state(sysd["xyz"])
+
+    def close(self):
+        if getattr(self, "h", None) is not None and self.h.value:
+            lib().sis_gpu_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ---- state ----
+    def set_state(self, xyz=None, velos=None, accels=None):
+        arrs = [None if a is None else np.ascontiguousarray(np.asarray(a, np.float64).reshape(self.nrep, self.nmp, 3))
+                for a in (xyz, velos, accels)]
+        _check(lib().sis_gpu_set_state(self.h, *[abi.dptr(a) if a is not None else None for a in arrs]))
+
+    def get_state(self):
+        x = np.zeros((self.nrep, self.nmp, 3))
+        v = np.zeros_like(x)
+        a = np.zeros_like(x)
+        _check(lib().sis_gpu_get_state(self.h, abi.dptr(x), abi.dptr(v), abi.dptr(a)))
+        return x, v, a
+
+    def set_rep_conditions(self, temps=None, ionic=None):
+        t = None if temps is None else np.ascontiguousarray(temps, np.float64)
+        i = None if ionic is None else np.ascontiguousarray(ionic, np.float64)
+        _check(lib().sis_gpu_set_rep_conditions(self.h, abi.dptr(t) if t is not None else None,
+                                                abi.dptr(i) if i is not None else None))
+
+    # ---- hot path ----
+    def neighbor_list(self, irep: int = 0):
+        _check(lib().sis_gpu_rebuild(self.h, irep))
+
+    def force(self, irep: int = 1, bp_mc: int = 1) -> np.ndarray:
+        f = np.zeros((self.nmp, 3))
+        _check(lib().sis_gpu_force(self.h, irep, bp_mc, abi.dptr(f)))
+        return f
+
+    def energy_sumup(self, irep: int = 1, tempK: float = -1.0, bp_mc: int = 1,
+                     bp_energy_valid: int = 0) -> np.ndarray:
+        e = np.zeros(11)
+        flags = (1 if bp_mc else 0) | (2 if bp_energy_valid else 0)
+        _check(lib().sis_gpu_energy(self.h, irep, float(tempK), flags, abi.dptr(e)))
+        return e
+
+    def pairs(self, irep: int = 1, kind: int = 0) -> np.ndarray:
+        n = C.c_int64()
+        _check(lib().sis_gpu_get_pair_count(self.h, irep, kind, C.byref(n)))
+        w = 7 if kind == 2 else 2
+        out = np.zeros((max(n.value, 1), w), np.int32)
+        m = C.c_int64()
+        _check(lib().sis_gpu_get_pairs(self.h, irep, kind, abi.iptr(out), n.value, C.byref(m)))
+        return out[: m.value].copy()
+
+    def bp(self, irep: int = 1) -> dict:
+        n = C.c_int64()
+        _check(lib().sis_gpu_get_pair_count(self.h, irep, 2, C.byref(n)))
+        k = max(n.value, 1)
+        mp = np.zeros((k, 7), np.int32)
+        cf = np.zeros((k, 2))
+        st = np.zeros(k, np.uint8)
+        en = np.zeros(k)
+        m = C.c_int64()
+        _check(lib().sis_gpu_get_bp(self.h, irep, abi.iptr(mp), abi.dptr(cf), abi.u8ptr(st), abi.dptr(en),
+                                    k, C.byref(m)))
+        n = m.value
+        return dict(bp=mp[:n].copy(), bpcoef=cf[:n].copy(), status=st[:n].copy(), ene=en[:n].copy())
+
+    def lists(self, irep: int = 1) -> dict:
+        b = self.bp(irep)
+        return dict(wca=self.pairs(irep, 0), ele=self.pairs(irep, 1), bp=b["bp"], bpcoef=b["bpcoef"])
+
+    # ---- random numbers ----
+    def set_rng(self, seed: int):
+        _check(lib().sis_gpu_set_rng(self.h, C.c_uint64(seed)))
+
+    def set_rng_tape(self, irep: int, words):
+        w = np.ascontiguousarray(words, np.uint32)
+        _check(lib().sis_gpu_set_rng_tape(self.h, irep, abi.u32ptr(w), w.size))
+
+    # ---- dynamics ----
+    def init_velocities(self):
+        _check(lib().sis_gpu_init_velocities(self.h))
+
+    def step(self, nsteps: int) -> int:
+        n = C.c_int64()
+        _check(lib().sis_gpu_step(self.h, int(nsteps), C.byref(n)))
+        return n.value
+
+    def exchange(self):
+        e = np.zeros((self.nrep, 2))
+        acc = np.zeros(max(self.nrep // 2 + 1, 1), np.int32)
+        _check(lib().sis_gpu_exchange(self.h, abi.dptr(e), abi.iptr(acc)))
+        return e, acc
+
+    def labels(self) -> np.ndarray:
+        lab = np.zeros(self.nrep, np.int32)
+        _check(lib().sis_gpu_get_labels(self.h, abi.iptr(lab)))
+        return lab
+
+    def info(self) -> dict:
+        v = np.zeros(10, np.int64)
+        _check(lib().sis_gpu_get_info(self.h, abi.i64ptr(v), 10))
+        keys = ["nmp", "nrep", "istep", "rebuilds", "nwca", "nele", "nbp", "max_nbr", "mode", "block"]
+        return dict(zip(keys, [int(x) for x in v]))
+
+    def last_ms(self, which: int = 0) -> float:
+        return float(lib().sis_gpu_last_kernel_ms(self.h, which))

@@ -1,0 +1,208 @@
+! iso_c_binding interface of libsis_gpu.so (include/sis_gpu.h) for the
+! Fortran host.  A host that keeps the reference's module state calls
+! sis_gpu_set_state / sis_gpu_get_state around the calls that replace
+! neighbor_list (src/neighbor_list.F90:1), force (src/force.F90:1),
+! energy_sumup (src/energy_sumup.F90:1) and the job_md loop body
+! (src/job_md.F90:360-460).  Struct layouts match include/sis_gpu.h.
+
+module sis_gpu_iface
+
+   use, intrinsic :: iso_c_binding
+   implicit none
+
+   integer(c_int), parameter :: SIS_OK = 0
+
+   type, bind(C) :: sis_bp_para
+      real(c_double) :: cutoff_ddist, U0, bond_k, bond_r
+      real(c_double) :: angl_k1, angl_k2, angl_k3, angl_k4
+      real(c_double) :: angl_theta1, angl_theta2, angl_theta3, angl_theta4
+      real(c_double) :: dihd_k1, dihd_k2, dihd_phi1, dihd_phi2
+   end type sis_bp_para
+
+   type, bind(C) :: sis_params
+      real(c_double) :: bond_k, bond_r0
+      real(c_double) :: angl_k, angl_t0
+      real(c_double) :: dih_k, dih_w, dih_p0
+      real(c_double) :: wca_sigma, wca_eps
+      real(c_double) :: ele_cutoff, length_per_charge, ionic_strength
+      real(c_double) :: bp_cutoff_energy
+      real(c_double) :: bp_dH0, bp_dS0, bp_dHend0, bp_dSend0, bp_coef_dG
+      real(c_double) :: nn_dH(21), nn_dS(21)
+      real(c_double) :: nnend_dH(6), nnend_dS(6)
+      type(sis_bp_para) :: bp(3)
+      real(c_double) :: tempK
+      real(c_double) :: dt, viscosity_pas, nl_margin
+      real(c_double) :: box(3)
+      integer(c_int32_t) :: angl_reb, dih_type
+      integer(c_int32_t) :: ele_on, ele_cutoff_type, ele_excl_bond, ele_excl_angl, dummy_has_charge
+      integer(c_int32_t) :: bp_model, bp_min_loop, max_bp_per_nt, nstep_bp_mc, nnend
+      integer(c_int32_t) :: pbc, pad
+   end type sis_params
+
+   type, bind(C) :: sis_topo
+      integer(c_int32_t) :: nchains, nmp
+      type(c_ptr) :: nmp_chain, moltype, seq, no_charge
+      integer(c_int32_t) :: n_no_charge, pad
+   end type sis_topo
+
+   type, bind(C) :: sis_options
+      integer(c_int32_t) :: nrep, precision, parity, mode, block, max_nbr
+      integer(c_int32_t) :: rebuild_every, exchange_every
+      integer(c_int64_t) :: seed
+   end type sis_options
+
+   interface
+      integer(c_int) function sis_gpu_create(topo, prm, opt, ctx) bind(C, name="sis_gpu_create")
+         import :: c_int, c_ptr, sis_topo, sis_params, sis_options
+         type(sis_topo), intent(in) :: topo
+         type(sis_params), intent(in) :: prm
+         type(sis_options), intent(in) :: opt
+         type(c_ptr), intent(out) :: ctx
+      end function
+
+      integer(c_int) function sis_gpu_destroy(ctx) bind(C, name="sis_gpu_destroy")
+         import :: c_int, c_ptr
+         type(c_ptr), value :: ctx
+      end function
+
+      type(c_ptr) function sis_gpu_last_error() bind(C, name="sis_gpu_last_error")
+         import :: c_ptr
+      end function
+
+      ! xyz, velos, accels: real(8) (3, nmp, nrep) as var_state (src/var_state.F90:26-28)
+      integer(c_int) function sis_gpu_set_state(ctx, xyz, velos, accels) bind(C, name="sis_gpu_set_state")
+         import :: c_int, c_ptr
+         type(c_ptr), value :: ctx, xyz, velos, accels
+      end function
+
+      integer(c_int) function sis_gpu_get_state(ctx, xyz, velos, accels) bind(C, name="sis_gpu_get_state")
+         import :: c_int, c_ptr
+         type(c_ptr), value :: ctx, xyz, velos, accels
+      end function
+
+      integer(c_int) function sis_gpu_set_rep_conditions(ctx, tempK, ionic) &
+            bind(C, name="sis_gpu_set_rep_conditions")
+         import :: c_int, c_ptr
+         type(c_ptr), value :: ctx, tempK, ionic
+      end function
+
+      ! replaces: call neighbor_list(irep)
+      integer(c_int) function sis_gpu_rebuild(ctx, irep) bind(C, name="sis_gpu_rebuild")
+         import :: c_int, c_ptr, c_int32_t
+         type(c_ptr), value :: ctx
+         integer(c_int32_t), value :: irep
+      end function
+
+      ! replaces: call force(irep, forces)
+      integer(c_int) function sis_gpu_force(ctx, irep, bp_mc, forces) bind(C, name="sis_gpu_force")
+         import :: c_int, c_ptr, c_int32_t
+         type(c_ptr), value :: ctx
+         integer(c_int32_t), value :: irep, bp_mc
+         type(c_ptr), value :: forces
+      end function
+
+      ! replaces: call energy_sumup(irep, tempK, energies)
+      integer(c_int) function sis_gpu_energy(ctx, irep, tempK, flags, energies) bind(C, name="sis_gpu_energy")
+         import :: c_int, c_ptr, c_int32_t, c_double
+         type(c_ptr), value :: ctx
+         integer(c_int32_t), value :: irep, flags
+         real(c_double), value :: tempK
+         real(c_double), intent(out) :: energies(0:10)
+      end function
+
+      integer(c_int) function sis_gpu_get_pair_count(ctx, irep, kind, n) bind(C, name="sis_gpu_get_pair_count")
+         import :: c_int, c_ptr, c_int32_t, c_int64_t
+         type(c_ptr), value :: ctx
+         integer(c_int32_t), value :: irep, kind
+         integer(c_int64_t), intent(out) :: n
+      end function
+
+      integer(c_int) function sis_gpu_get_pairs(ctx, irep, kind, out, cap, n) bind(C, name="sis_gpu_get_pairs")
+         import :: c_int, c_ptr, c_int32_t, c_int64_t
+         type(c_ptr), value :: ctx
+         integer(c_int32_t), value :: irep, kind
+         type(c_ptr), value :: out
+         integer(c_int64_t), value :: cap
+         integer(c_int64_t), intent(out) :: n
+      end function
+
+      integer(c_int) function sis_gpu_get_bp(ctx, irep, bp_mp, bp_coef, status, ene, cap, n) &
+            bind(C, name="sis_gpu_get_bp")
+         import :: c_int, c_ptr, c_int32_t, c_int64_t
+         type(c_ptr), value :: ctx
+         integer(c_int32_t), value :: irep
+         type(c_ptr), value :: bp_mp, bp_coef, status, ene
+         integer(c_int64_t), value :: cap
+         integer(c_int64_t), intent(out) :: n
+      end function
+
+      integer(c_int) function sis_gpu_set_rng(ctx, seed) bind(C, name="sis_gpu_set_rng")
+         import :: c_int, c_ptr, c_int64_t
+         type(c_ptr), value :: ctx
+         integer(c_int64_t), value :: seed
+      end function
+
+      integer(c_int) function sis_gpu_set_rng_tape(ctx, irep, words, n) bind(C, name="sis_gpu_set_rng_tape")
+         import :: c_int, c_ptr, c_int32_t, c_int64_t
+         type(c_ptr), value :: ctx
+         integer(c_int32_t), value :: irep
+         type(c_ptr), value :: words
+         integer(c_int64_t), value :: n
+      end function
+
+      integer(c_int) function sis_gpu_init_velocities(ctx) bind(C, name="sis_gpu_init_velocities")
+         import :: c_int, c_ptr
+         type(c_ptr), value :: ctx
+      end function
+
+      ! replaces the body of the job_md time loop for nsteps steps
+      integer(c_int) function sis_gpu_step(ctx, nsteps, rebuilds) bind(C, name="sis_gpu_step")
+         import :: c_int, c_ptr, c_int64_t
+         type(c_ptr), value :: ctx
+         integer(c_int64_t), value :: nsteps
+         integer(c_int64_t), intent(out) :: rebuilds
+      end function
+
+      integer(c_int) function sis_gpu_exchange(ctx, replica_energies, accepted) bind(C, name="sis_gpu_exchange")
+         import :: c_int, c_ptr
+         type(c_ptr), value :: ctx, replica_energies, accepted
+      end function
+
+      integer(c_int) function sis_gpu_get_labels(ctx, rep2lab) bind(C, name="sis_gpu_get_labels")
+         import :: c_int, c_ptr
+         type(c_ptr), value :: ctx, rep2lab
+      end function
+
+      integer(c_int) function sis_gpu_get_info(ctx, out, n) bind(C, name="sis_gpu_get_info")
+         import :: c_int, c_ptr, c_int32_t
+         type(c_ptr), value :: ctx, out
+         integer(c_int32_t), value :: n
+      end function
+
+      real(c_double) function sis_gpu_last_kernel_ms(ctx, which) bind(C, name="sis_gpu_last_kernel_ms")
+         import :: c_double, c_ptr, c_int32_t
+         type(c_ptr), value :: ctx
+         integer(c_int32_t), value :: which
+      end function
+   end interface
+
+contains
+
+   ! Message of the last failed call as a Fortran string (for sis_abort).
+   function sis_gpu_error_message() result(msg)
+      character(len=:), allocatable :: msg
+      type(c_ptr) :: p
+      character(kind=c_char), pointer :: s(:)
+      integer :: n
+      p = sis_gpu_last_error()
+      n = 0
+      call c_f_pointer(p, s, [4096])
+      do while (n < 4096)
+         if (s(n+1) == c_null_char) exit
+         n = n + 1
+      enddo
+      allocate(character(len=n) :: msg)
+      msg = transfer(s(1:n), msg)
+   end function sis_gpu_error_message
+
+end module sis_gpu_iface

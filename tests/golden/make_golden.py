@@ -1,0 +1,139 @@
+"""Write the golden fixtures in tests/golden/ from the reference SIS Fortran
+(oracle/_ref, built from /root/reference by oracle/Makefile).
+
+Each fixture holds the inputs (system, MT word tapes of the reference's own
+mt_stream streams) and the reference's outputs: interaction lists, forces,
+energies, base-pair state after the limit MC, and the state after a short
+job_md run.  Only the tape prefix that the run consumes is stored (counted
+by replaying the run in the C restatement).
+
+Run:  python tests/golden/make_golden.py
+"""
+from __future__ import annotations
+
+import sys
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+ROOT = HERE.parent.parent
+sys.path.insert(0, str(ROOT))
+
+from sis_amd import configs  # noqa: E402
+from sis_amd.abi import MOLT_CIRCRNA  # noqa: E402
+from oracle import corc, refdrv  # noqa: E402
+
+
+def mixed_chains(seed=11):
+    """A circular 16-nt chain, then linear chains of 24 and 40 nt packed in
+    a 64 A region (unequal lengths exercise the bp_mp(6) index rule of
+    src/neighbor_list.F90:212), non-periodic.  The first seed from `seed`
+    whose base-pair candidates include that rule is used."""
+    for sd in range(seed, seed + 200):
+        rng = np.random.default_rng(sd)
+        seqs = [configs.random_seq(rng, 16, 0.6), configs.random_seq(rng, 24, 0.6),
+                configs.random_seq(rng, 40, 0.6)]
+        lin = configs._place_chains(rng, seqs[1:], 64.0, clash=9.0)
+        n = len(seqs[0])
+        R = 5.9 / (2 * np.sin(np.pi / n))
+        th = 2 * np.pi * np.arange(n) / n
+        ring = np.stack([R * np.cos(th), R * np.sin(th), np.zeros(n)], axis=1)
+        ring = ring + np.array([-40.0, 32.0, 32.0])
+        p = configs.default_params(tempK=300.0, nl_margin=6.0)
+        sysd = configs._mk("mixed", seqs, np.vstack([ring, lin]), p)
+        sysd["moltype"][0] = MOLT_CIRCRNA
+        o = corc.Oracle(sysd)
+        o.neighbor_list(1)
+        bp = o.lists(1)["bp"]
+        if np.any(bp[:, 5] != bp[:, 1] + 1):
+            return sysd
+    raise RuntimeError("no seed exercises the bp_mp(6) rule")
+
+
+def remd_small(nrep=4, seed=5):
+    sysd = configs.remd_c3(nrep=nrep, n=24, seed=seed)
+    sysd["name"] = "remd4"
+    return sysd
+
+
+SYSTEMS = {
+    "hairpin64": (lambda: configs.hairpin64(1), 60, 0),
+    "small_pbc": (lambda: configs.small_pbc(4, 30, 100.0, seed=5), 60, 0),
+    "mixed": (mixed_chains, 40, 0),
+    "remd4": (remd_small, 40, 10),
+}
+
+
+def make(name, builder, nstep, nex, seed=4321):
+    s = builder()
+    nrep = int(s["nrep"])
+    big = 3_000_000
+    ops = [("tape", r, big, f"tape{r}") for r in range(1, nrep + 1)]
+    if nrep > 1:
+        ops.append(("tape", -1, 4096, "tape_rep"))
+    ops += [("nl", r) for r in range(1, nrep + 1)]
+    ops += [("lists", 1, "L"), ("force", 1, 1, "F"), ("bp_state", 1, "B"),
+            ("energy", 1, float(s["temps"][0]), 0, 1, "E"), ("ele", 1, "ele"), ("bp_tables", "T")]
+    ref0 = refdrv.run(s, ops, seed=seed, nstep_exchange=nex)
+    ref1 = refdrv.run(s, [("job_md", nstep, "R")], seed=seed, nstep_exchange=nex)
+    # count the words the static part + the run consume, using the oracle
+    o = corc.Oracle(s)
+    for r in range(1, nrep + 1):
+        o.set_tape(r, ref0[f"tape{r}"])
+    if nrep > 1:
+        o.set_tape(-1, ref0["tape_rep"])
+    for r in range(1, nrep + 1):
+        o.neighbor_list(r)
+    o.force(1, 1)
+    used_static = [o.tape_pos(r) for r in range(1, nrep + 1)]
+    o2 = corc.Oracle(s)
+    for r in range(1, nrep + 1):
+        o2.set_tape(r, ref0[f"tape{r}"])
+    if nrep > 1:
+        o2.set_tape(-1, ref0["tape_rep"])
+    o2.init_velocities()
+    o2.md(nstep, nex)
+    used_run = [o2.tape_pos(r) for r in range(1, nrep + 1)]
+    keep = max(max(used_static), max(used_run)) + 64
+    out = dict(
+        nchains=s["nchains"], nmp_chain=s["nmp_chain"], moltype=s["moltype"], seq=s["seq"],
+        xyz=s["xyz"], temps=s["temps"], nrep=nrep, nstep=nstep, nstep_exchange=nex,
+        tapes=np.stack([ref0[f"tape{r}"][:keep] for r in range(1, nrep + 1)]),
+        tape_rep=ref0.get("tape_rep", np.zeros(1, np.uint32))[:256],
+        wca=ref0["L"]["wca"], ele=ref0["L"]["ele"], bp=ref0["L"]["bp"], bpcoef=ref0["L"]["bpcoef"],
+        force=ref0["F"], bp_status=ref0["B"][0], ene_bp=ref0["B"][1], energies=ref0["E"],
+        ele_params=ref0["ele"], bp3_dH=ref0["T"][0], bp3_dS=ref0["T"][1], bp_cut=ref0["T"][2],
+        md_xyz=ref1["R"]["xyz"], md_velos=ref1["R"]["velos"], md_accels=ref1["R"]["accels"],
+        md_energies=ref1["R"]["energies"], md_rep2lab=ref1["R"]["rep2lab"],
+    )
+    # parameters as a flat float/int record
+    p = s["params"]
+    scal = {k: v for k, v in p.items() if isinstance(v, (int, float))}
+    out["param_names"] = np.array(sorted(scal))
+    out["param_values"] = np.array([float(scal[k]) for k in sorted(scal)])
+    out["box"] = np.array(p["box"], float)
+    np.savez_compressed(HERE / f"{name}.npz", **out)
+    print(name, "tape words kept", keep, "lists", {k: len(ref0["L"][k]) for k in ("wca", "ele", "bp")})
+
+
+def load(name):
+    """Rebuild the system dict of a fixture (params from the defaults plus
+    the stored scalar overrides)."""
+    d = dict(np.load(HERE / f"{name}.npz", allow_pickle=False))
+    pn = [str(x) for x in d["param_names"]]
+    over = {k: float(v) for k, v in zip(pn, d["param_values"])}
+    p = configs.default_params()
+    for k, v in over.items():
+        p[k] = int(v) if isinstance(p.get(k), int) else v
+    seqs = None
+    sysd = dict(name=name, nchains=int(d["nchains"]), nmp_chain=d["nmp_chain"], moltype=d["moltype"],
+                seq=d["seq"], xyz=d["xyz"], params=p, nrep=int(d["nrep"]), temps=d["temps"],
+                nstep=int(d["nstep"]), seqs=seqs)
+    p["box"] = [float(b) for b in d["box"]]
+    return sysd, d
+
+
+if __name__ == "__main__":
+    for nm, (b, n, nex) in SYSTEMS.items():
+        make(nm, b, n, nex)

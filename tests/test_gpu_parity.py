@@ -1,0 +1,171 @@
+"""GPU engine vs the oracle and the reference's golden fixtures, through the
+C ABI (sis_amd/libsis_gpu.so).
+
+Bars: interaction lists bit-exact (same pair sets; BP candidates in the
+reference's order); FP64 forces / energies within 1e-10 relative of the
+oracle; base-pair status after the limit MC identical on the same MT words;
+trajectories driven by the same MT words within 1e-8 A of the reference
+after the fixture's step count; FP32 pair arithmetic within 1e-5 per-bead
+relative force error (|dF| / max(|F_ref|, 1e-3 kcal/mol/A))."""
+import numpy as np
+import pytest
+
+from sis_amd import configs
+from sis_amd.engine import Engine
+from oracle import corc
+from tests.golden.make_golden import load
+
+pytestmark = pytest.mark.gpu
+
+FIXTURES = ["hairpin64", "small_pbc", "mixed", "remd4"]
+
+
+def _sorted_rows(a):
+    return a[np.lexsort(a.T[::-1])] if len(a) else a
+
+
+def _pair_sets_equal(eng_pairs, ref_pairs):
+    return np.array_equal(_sorted_rows(eng_pairs), _sorted_rows(ref_pairs))
+
+
+def _engine(sysd, d=None, **kw):
+    eng = Engine(sysd, parity=True, **kw)
+    if d is not None:
+        for r in range(1, sysd["nrep"] + 1):
+            eng.set_rng_tape(r, d["tapes"][r - 1])
+        if sysd["nrep"] > 1:
+            eng.set_rng_tape(-1, d["tape_rep"])
+    return eng
+
+
+@pytest.mark.parametrize("name", FIXTURES)
+def test_lists_match_reference(name):
+    sysd, d = load(name)
+    print(f"[case] lists {name}", flush=True)
+    eng = _engine(sysd, d)
+    eng.neighbor_list(0)
+    L = eng.lists(1)
+    assert _pair_sets_equal(L["wca"], d["wca"])
+    assert _pair_sets_equal(L["ele"], d["ele"])
+    assert np.array_equal(L["bp"], d["bp"])
+    assert np.array_equal(L["bpcoef"], d["bpcoef"])
+
+
+@pytest.mark.parametrize("name", FIXTURES)
+def test_force_energy_bp_state(name):
+    sysd, d = load(name)
+    print(f"[case] force {name}", flush=True)
+    eng = _engine(sysd, d)
+    eng.neighbor_list(0)
+    f = eng.force(1, 1)
+    fr = d["force"]
+    assert np.abs(f - fr).max() / np.abs(fr).max() < 1e-10
+    b = eng.bp(1)
+    assert np.array_equal(b["status"], d["bp_status"].astype(np.uint8))
+    assert np.allclose(b["ene"], d["ene_bp"], rtol=1e-9, atol=1e-300)
+    e = eng.energy_sumup(1, float(sysd["temps"][0]), 0, 1)
+    assert np.allclose(e[1:], d["energies"][1:], rtol=1e-9, atol=1e-10)
+
+
+@pytest.mark.parametrize("name", FIXTURES)
+def test_trajectory_matches_reference(name):
+    """Same MT words as the reference run: initial velocities, initial
+    energies with the BP MC, nstep steps (rebuilds, MC, exchanges)."""
+    sysd, d = load(name)
+    print(f"[case] trajectory {name}", flush=True)
+    nex = int(d["nstep_exchange"])
+    eng = _engine(sysd, d, exchange_every=nex)
+    eng.init_velocities()
+    eng.neighbor_list(0)
+    for r in range(1, sysd["nrep"] + 1):
+        eng.energy_sumup(r, -1.0, 1, 0)
+    eng.step(int(d["nstep"]))
+    x, v, a = eng.get_state()
+    assert np.abs(x - d["md_xyz"]).max() < 1e-8
+    assert np.abs(v - d["md_velos"]).max() < 1e-9
+    assert np.abs(a - d["md_accels"]).max() < 1e-12
+    if sysd["nrep"] > 1:
+        assert np.array_equal(eng.labels(), d["md_rep2lab"][: sysd["nrep"]])
+
+
+def test_condensate_full_size_lists_and_forces():
+    """C2 at full size (200 x 47 nt, periodic 2,499 A box): cell-list
+    rebuild gives the oracle's pair sets bit-exactly; FP64 forces agree."""
+    s = configs.condensate_c2()
+    print("[case] C2 lists", flush=True)
+    eng = Engine(s, parity=True)
+    tape = corc.mt_words(77, 2_000_000)
+    eng.set_rng_tape(1, tape)
+    o = corc.Oracle(s)
+    o.set_tape(1, tape)
+    eng.neighbor_list(0)
+    o.neighbor_list(1)
+    Lg, Lo = eng.lists(1), o.lists(1)
+    assert _pair_sets_equal(Lg["wca"], Lo["wca"])
+    assert _pair_sets_equal(Lg["ele"], Lo["ele"])
+    assert np.array_equal(Lg["bp"], Lo["bp"])
+    fg, fo = eng.force(1, 1), o.force(1, 1)
+    assert np.abs(fg - fo).max() / np.abs(fo).max() < 1e-10
+    assert np.array_equal(eng.bp(1)["status"], o.bp_state(1)[0])
+
+
+def test_fp32_pair_arithmetic_error():
+    """Precision form: FP32 pair math, FP64 state and accumulation."""
+    s = configs.cag_system(60, 47, 2.0, 310.0, 5.0, seed=9)
+    print("[case] fp32", flush=True)
+    e64 = Engine(s, precision=0)
+    e32 = Engine(s, precision=1)
+    o = corc.Oracle(s)
+    o.neighbor_list(1)
+    fo = o.force(1, 0)
+    for e in (e64, e32):
+        e.neighbor_list(0)
+    f64 = e64.force(1, 0)
+    f32 = e32.force(1, 0)
+    den = np.maximum(np.linalg.norm(fo, axis=1), 1e-3)
+    rel32 = np.linalg.norm(f32 - fo, axis=1) / den
+    rel64 = np.linalg.norm(f64 - fo, axis=1) / den
+    print(f"fp32 max {rel32.max():.2e} rms {np.sqrt((rel32**2).mean()):.2e}; fp64 max {rel64.max():.2e}")
+    assert rel64.max() < 1e-10
+    assert rel32.max() < 1e-5
+
+
+def test_performance_mode_limit_rule():
+    """Counter-RNG limit MC per component: afterwards no nucleotide keeps
+    more than max_bp_per_nt pairs, and only pairs formed before the MC
+    survive."""
+    s = configs.condensate_c2()
+    print("[case] perf MC", flush=True)
+    eng = Engine(s, parity=False, seed=99)
+    eng.neighbor_list(0)
+    eng.force(1, 0)            # no MC: all formed pairs kept
+    pre = eng.bp(1)["status"].copy()
+    eng.force(1, 1)
+    b = eng.bp(1)
+    st = b["status"]
+    assert np.all(st <= pre)
+    cnt = np.zeros(s["xyz"].shape[1] + 1, int)
+    for row in b["bp"][st == 1]:
+        cnt[row[0]] += 1
+        cnt[row[1]] += 1
+    assert cnt.max() <= 1
+    # every nucleotide that had a formed pair before keeps >= 1 pair unless
+    # all its pairs went to partners that kept another one
+    assert st.sum() > 0.3 * pre.sum()
+
+
+def test_performance_mode_dynamics_stable():
+    s = configs.condensate_c2()
+    print("[case] perf steps", flush=True)
+    eng = Engine(s, parity=False, seed=7)
+    eng.init_velocities()
+    eng.neighbor_list(0)
+    eng.step(200)
+    x, v, a = eng.get_state()
+    assert np.all(np.isfinite(x)) and np.all(np.isfinite(v))
+    mass = np.array([{0: 328.212, 1: 305.164, 2: 344.212, 3: 304.182}[q] for q in s["seq"]])
+    kb = 1.380649e-23 * 6.02214076e23 / 4184.0
+    T = (mass[:, None] * v[0] ** 2).sum() / (3 * len(mass) * kb)
+    assert 250.0 < T < 380.0
+    e = eng.energy_sumup(1, -1.0, 0, 1)
+    assert np.all(np.isfinite(e))

@@ -1,0 +1,197 @@
+"""The C restatement (oracle/liboracle.so) pinned against the reference:
+golden fixtures written by the reference Fortran (tests/golden/), the live
+reference build (oracle/_ref) where present, and closed forms."""
+import math
+
+import numpy as np
+import pytest
+
+from sis_amd import configs
+from oracle import corc, refdrv
+from tests.golden.make_golden import load
+
+FIXTURES = ["hairpin64", "small_pbc", "mixed", "remd4"]
+
+
+def _oracle_from_fixture(name):
+    sysd, d = load(name)
+    o = corc.Oracle(sysd)
+    for r in range(1, sysd["nrep"] + 1):
+        o.set_tape(r, d["tapes"][r - 1])
+    if sysd["nrep"] > 1:
+        o.set_tape(-1, d["tape_rep"])
+    return sysd, d, o
+
+
+@pytest.mark.parametrize("name", FIXTURES)
+def test_lists_bit_exact(name):
+    sysd, d, o = _oracle_from_fixture(name)
+    for r in range(1, sysd["nrep"] + 1):
+        o.neighbor_list(r)
+    L = o.lists(1)
+    assert np.array_equal(L["wca"], d["wca"])
+    assert np.array_equal(L["ele"], d["ele"])
+    assert np.array_equal(L["bp"], d["bp"])          # same candidates, same order
+    assert np.array_equal(L["bpcoef"], d["bpcoef"])  # triplet dH/dS bit-exact
+
+
+@pytest.mark.parametrize("name", FIXTURES)
+def test_force_and_bp_state(name):
+    sysd, d, o = _oracle_from_fixture(name)
+    for r in range(1, sysd["nrep"] + 1):
+        o.neighbor_list(r)
+    f = o.force(1, 1)
+    fr = d["force"]
+    rel = np.abs(f - fr).max() / np.abs(fr).max()
+    assert rel < 1e-12
+    st, en = o.bp_state(1)
+    assert np.array_equal(st, d["bp_status"].astype(np.uint8))   # limit MC on the same words
+    assert np.allclose(en, d["ene_bp"], rtol=1e-12, atol=1e-300)
+    e = o.energy(1, float(sysd["temps"][0]), 0, 1)
+    assert np.allclose(e[1:], d["energies"][1:], rtol=1e-10, atol=1e-12)
+    assert np.allclose(o.ele_params(1), d["ele_params"], rtol=1e-14)
+
+
+@pytest.mark.parametrize("name", FIXTURES)
+def test_tables(name):
+    sysd, d, o = _oracle_from_fixture(name)
+    dh, ds, cut = o.tables()
+    assert np.array_equal(dh, d["bp3_dH"]) and np.array_equal(ds, d["bp3_dS"])
+    assert np.array_equal(cut, d["bp_cut"])
+    assert np.count_nonzero(dh) > 400   # 468 triplets (src/init_bp.F90:170-171)
+
+
+@pytest.mark.parametrize("name", FIXTURES)
+def test_md_trajectory(name):
+    """job_md for a few dozen steps with the reference's own MT streams:
+    noise, limit MC, rebuilds and (remd4) exchanges consumed in order."""
+    sysd, d, o = _oracle_from_fixture(name)
+    o.init_velocities()
+    o.md(int(d["nstep"]), int(d["nstep_exchange"]))
+    x, v, a = o.state()
+    assert np.abs(x - d["md_xyz"]).max() < 1e-10
+    assert np.abs(v - d["md_velos"]).max() < 1e-12
+    assert np.abs(a - d["md_accels"]).max() < 1e-14
+    if sysd["nrep"] > 1:
+        assert np.array_equal(o.labels(), d["md_rep2lab"][: sysd["nrep"]])
+
+
+def test_h4_index_rule_exercised():
+    """mixed has chains of 24 and 40 nt: some bp_mp(6) entries follow the
+    reference's comparison with the length of chain i
+    (src/neighbor_list.F90:212) rather than j+1."""
+    sysd, d = load("mixed")
+    bp = d["bp"]
+    assert np.any(bp[:, 5] != bp[:, 1] + 1)
+
+
+# ---------------- closed forms and invariants ----------------
+
+def test_mt19937_known_answers():
+    # first outputs of init_genrand(5489), the MT19937 reference values
+    w = corc.mt_words(5489, 5)
+    assert list(w) == [3499211612, 581869302, 3890346734, 3586334585, 545404204]
+    # 10000th output of the default-seeded generator (C++11 std::mt19937)
+    assert corc.mt_words(5489, 10000)[-1] == 4123659995
+
+
+def test_uniform_conversions_closed_form():
+    s = configs.hairpin64(1)
+    o = corc.Oracle(s)
+    words = np.array([0xFFFFFFFF, 0xFFFFFFFF, 0, 0, 0x80000000, 0x40000000] * 4, np.uint32)
+    o.set_tape(1, words)
+    d1 = o.genrand(1, 1, 3)
+    assert d1[0] == 1.0 and d1[1] == 0.0           # [0,1] closed interval
+    ia, ib = 0x80000000 >> 5, 0x40000000 >> 6
+    assert d1[2] == (ia * 67108864.0 + ib) / 9007199254740991.0
+    o.set_tape(1, words)
+    d3 = o.genrand(1, 3, 2)
+    assert 0.0 < d3[1] < d3[0] < 1.0               # (0,1) open interval
+    assert d3[1] == 0.5 / 4503599627370496.0
+
+
+def test_set_ele_closed_form():
+    """T = 300 K, I = 0.15 M: eps = 77.648, lB = 7.173 A, lambda_D = 7.836 A,
+    coefficient 1.5957 kcal A/mol (src/set_ele.F90:45-99)."""
+    s = configs.hairpin64(1)
+    o = corc.Oracle(s)
+    lam, coef, cut, eps = o.ele_params(1)
+    Tc = 300.0 - 273.15
+    assert eps == pytest.approx(87.740 - 0.40008 * Tc + 9.398e-4 * Tc**2 - 1.410e-6 * Tc**3, rel=1e-14)
+    e0, qe, kb, na = 8.8541878128e-12, 1.602176634e-19, 1.380649e-23, 6.02214076e23
+    lb = qe**2 / (4 * math.pi * e0 * eps * kb * 300.0) * 1e10
+    assert lb == pytest.approx(7.173, abs=1e-3)
+    lamD = math.sqrt(e0 * eps * kb * 300.0 / (2 * na * 1e3 * 0.15 * qe**2)) * 1e10
+    assert lam == pytest.approx(lamD, rel=1e-12)
+    assert coef == pytest.approx((4.38178046 / lb) ** 2 * lb * kb * 300.0 * na / 4184.0, rel=1e-12)
+    assert cut == 50.0
+
+
+def _energy_total(o, x):
+    o.set_xyz(x[None])
+    return o.energy(1, None, 0, 0)[0]
+
+
+def test_finite_difference_forces():
+    """job_check_force semantics (src/job_check_force.F90:46-82): central
+    differences of energy_sumup with h = 1e-3 against force, BP MC off."""
+    s = configs.small_pbc(3, 20, 90.0, seed=3)
+    s["params"]["nstep_bp_mc"] = 0
+    o = corc.Oracle(s)
+    o.neighbor_list(1)
+    x0 = s["xyz"][0].copy()
+    f = o.force(1, 0)
+    h = 1e-3
+    rng = np.random.default_rng(0)
+    worst = 0.0
+    for i in rng.choice(x0.shape[0], 12, replace=False):
+        for k in range(3):
+            xp = x0.copy(); xp[i, k] += h
+            xm = x0.copy(); xm[i, k] -= h
+            fd = -(_energy_total(o, xp) - _energy_total(o, xm)) / (2 * h)
+            worst = max(worst, abs(fd - f[i, k]) / max(1.0, abs(f[i, k])))
+    assert worst < 1e-4
+
+
+def test_bond_force_zero_at_r0_and_wca_zero_beyond_sigma():
+    s = configs.hairpin64(1)
+    s["params"].update(angl_reb=0, angl_k=0.0, dih_type=0, ele_on=0, bp_model=5)
+    n = s["xyz"].shape[1]
+    x = np.zeros((n, 3))
+    x[:, 0] = 5.9 * np.arange(n)                     # straight chain, bonds at r0
+    s["xyz"] = x[None]
+    o = corc.Oracle(s)
+    o.neighbor_list(1)
+    f = o.force(1, 0)
+    assert np.abs(f).max() < 1e-12                   # i,i+3 at 17.7 A > sigma
+
+
+def test_free_particle_velocity_decay():
+    """GJF-2GJ without force or noise: v(t+h) = a v(t) (src/job_md.F90:424-428)."""
+    s = configs.hairpin64(1)
+    o = corc.Oracle(s)
+    c = o.md_coef(1)
+    a = c[:, 0]
+    gamma_over_m = 6 * math.pi * 1.4115e-5 * math.sqrt(1e3 / 4184.0) * 6.02214076e23 * 1e-20 * 10.0
+    mass = np.array([{0: 328.212, 1: 305.164, 2: 344.212, 3: 304.182}[q] for q in s["seq"]])
+    c1 = 0.5 * 0.05 * gamma_over_m / mass
+    assert np.allclose(a, (1 - c1) / (1 + c1), rtol=1e-14)
+    assert np.allclose(c[:, 2], math.sqrt(1.0) * 0.05 / np.sqrt(1 + c1), rtol=1e-14)
+
+
+@pytest.mark.ref
+@pytest.mark.parametrize("seed", [21, 22])
+def test_live_reference_random_systems(seed):
+    """Fresh random periodic systems: oracle vs the reference build."""
+    s = configs.small_pbc(5, 26, 110.0, seed=seed, tempK=290.0 + seed)
+    ref = refdrv.run(s, [("tape", 1, 200000, "t"), ("nl", 1), ("lists", 1, "L"), ("force", 1, 1, "F"),
+                         ("bp_state", 1, "B")])
+    o = corc.Oracle(s)
+    o.set_tape(1, ref["t"])
+    o.neighbor_list(1)
+    L = o.lists(1)
+    for k in ("wca", "ele", "bp"):
+        assert np.array_equal(L[k], ref["L"][k])
+    f = o.force(1, 1)
+    assert np.abs(f - ref["F"]).max() / np.abs(ref["F"]).max() < 1e-12
+    assert np.array_equal(o.bp_state(1)[0], ref["B"][0].astype(np.uint8))
