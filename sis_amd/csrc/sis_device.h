@@ -39,7 +39,7 @@ struct Params {
   double wca_nl_cut2, bp_nl_cut2;
   int angl_reb, dih_type, ele_on, ele_excl_bond, ele_excl_angl;
   int bp_model, bp_min_loop, max_bp_per_nt, nstep_bp_mc, pbc;
-  int N, R, fix_h4, pad;
+  int N, R, fix_h4, nchains, maxlen, pad;
   BpPara bpp[7];  // 1..6 = GC AU GU CG UA UG
 };
 
@@ -55,6 +55,7 @@ __host__ __device__ inline bool topo_charged(const Topo &t) { return (t.flags >>
 // Device pointers of a context.
 struct Ptrs {
   const Topo *topo;          // [N]
+  const int *chain_id;       // [N] chain index of each bead
   const int *bp3_ctx;        // [N] packed (seq[i-1],seq[i],seq[i+1]) context for eligibility
   const int *bp3_hash;       // [4097] triplet hash -> table index (0 = none)
   const double *bp3_dH, *bp3_dS;  // [469]
@@ -68,6 +69,8 @@ struct Ptrs {
   int *bp_tmp, *n_bp_tmp;    // per-bead BP partners found by the search [R*N*maxbpb]
   int *bp_off;               // [R*(N+1)] exclusive scan of n_bp_tmp
   int *bp_mp;                // [R*maxbp*8]
+  unsigned long long *bp_key; // [maxbp] sort key of each candidate
+  int *bp_val;               // [maxbp]
   double *bp_coef;           // [R*maxbp*2]
   int *n_bp;                 // [R]
   int *inc, *n_inc;          // [R*maxinc*N], [R*N]

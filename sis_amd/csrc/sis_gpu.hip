@@ -1,6 +1,7 @@
 // sis_gpu.hip - host side of the MI355X engine: context set-up, list
 // management, the per-phase step loop and the C ABI of include/sis_gpu.h.
 #include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
 
 #include <algorithm>
 #include <cmath>
@@ -49,12 +50,19 @@ struct DevBuf {
   T *p = nullptr;
   size_t n = 0;
   void alloc(size_t count) {
-    if (p) HIPCHK(hipFree(p));
+    if (p) {
+      HIPCHK(hipDeviceSynchronize());
+      HIPCHK(hipFree(p));
+    }
     p = nullptr;
     n = count;
     if (count) {
+      // the engine works on a non-blocking stream: finish the zero fill
+      // before any kernel can touch the new buffer
+      HIPCHK(hipDeviceSynchronize());
       HIPCHK(hipMalloc(&p, count * sizeof(T)));
       HIPCHK(hipMemset(p, 0, count * sizeof(T)));
+      HIPCHK(hipDeviceSynchronize());
     }
   }
   void free_() {
@@ -107,7 +115,10 @@ struct sis_gpu_ctx {
   DevBuf<uint32_t> tape_d;
   DevBuf<int64_t> tape_off_d, tape_pos_d;
   DevBuf<double> noise_d, bm_cache_d, ene_part_d, ene_out_d, ecT_d, eCoef_d, eLam_d;
-  DevBuf<int> pers_sync_d;
+  DevBuf<int> pers_sync_d, chain_id_d, bp_val_d, bp_val2_d, bp_mp2_d;
+  DevBuf<unsigned long long> bp_key_d, bp_key2_d;
+  DevBuf<double> bp_coef2_d;
+  DevBuf<char> sort_tmp_d;
   int nbp_total = 0;
   int nblk_force = 0;
   int block = 256;
@@ -115,6 +126,9 @@ struct sis_gpu_ctx {
   Ptrs ptrs() {
     Ptrs d{};
     d.topo = topo_d.p;
+    d.chain_id = chain_id_d.p;
+    d.bp_key = bp_key_d.p;
+    d.bp_val = bp_val_d.p;
     d.bp3_hash = bp3_hash_d.p;
     d.bp3_dH = bp3_dH_d.p;
     d.bp3_dS = bp3_dS_d.p;
@@ -147,12 +161,14 @@ struct sis_gpu_ctx {
                          &comp_d, &flags_d, &cell_of_d, &cell_cnt_d, &cell_start_d,
                          &cell_beads_d, &cell_fill_d, &rebuild_flag_d, &mask_d, &mc_d,
                          &scan_bsum_d, &scan_total_d, &exc_cnt_d, &exc_off_d, &exc_fill_d,
-                         &exc_list_d, &ntlist_d, &bpseq_d, &pers_sync_d};
+                         &exc_list_d, &ntlist_d, &bpseq_d, &pers_sync_d, &chain_id_d,
+                         &bp_val_d, &bp_val2_d, &bp_mp2_d};
     for (auto *b : ib) b->free_();
     DevBuf<double> *db[] = {&bp3_dH_d, &bp3_dS_d, &md_a_d, &md_cf_d, &md_cx_d, &mass_d,
                             &md_cr_d, &tK_d, &ele_coef_d, &lambdaD_d, &ele_cut_d, &box_d,
                             &bp_coef_d, &ene_bp_d, &for_bp_d, &blk_top2_d, &noise_d,
-                            &bm_cache_d, &ene_part_d, &ene_out_d, &ecT_d, &eCoef_d, &eLam_d};
+                            &bm_cache_d, &ene_part_d, &ene_out_d, &ecT_d, &eCoef_d, &eLam_d,
+                            &bp_coef2_d};
     for (auto *b : db) b->free_();
     DevBuf<double4> *vb[] = {&pos_d, &vel_d, &acc_d, &move_d, &frc_d};
     for (auto *b : vb) b->free_();
@@ -162,6 +178,9 @@ struct sis_gpu_ctx {
     tape_d.free_();
     tape_off_d.free_();
     tape_pos_d.free_();
+    bp_key_d.free_();
+    bp_key2_d.free_();
+    sort_tmp_d.free_();
     if (ev0) (void)hipEventDestroy(ev0);
     if (ev1) (void)hipEventDestroy(ev1);
     if (stream) (void)hipStreamDestroy(stream);
@@ -305,6 +324,12 @@ void alloc_lists(sis_gpu_ctx *c) {
   c->ene_bp_d.alloc(c->D.maxbp);
   c->for_bp_d.alloc((size_t)c->D.maxbp * 18);
   c->bpseq_d.alloc((size_t)c->D.maxbp + c->R + 2);
+  c->bp_key_d.alloc(c->D.maxbp);
+  c->bp_key2_d.alloc(c->D.maxbp);
+  c->bp_val_d.alloc(c->D.maxbp);
+  c->bp_val2_d.alloc(c->D.maxbp);
+  c->bp_mp2_d.alloc((size_t)c->D.maxbp * 8);
+  c->bp_coef2_d.alloc((size_t)c->D.maxbp * 2);
 }
 
 void setup_cells(sis_gpu_ctx *c) {
@@ -388,11 +413,27 @@ void rebuild(sis_gpu_ctx *c, const std::vector<int> &mask) {
     scan_exclusive(c, c->n_bp_tmp_d.p, c->bp_off_d.p, (int64_t)R * N, nullptr);
     HIPCHK(hipMemsetAsync(c->n_inc_d.p, 0, (size_t)R * N * sizeof(int), c->stream));
     hipLaunchKernelGGL(k_bp_fill, grid_beads(N, R, B), dim3(B), 0, c->stream, c->P, D, nullptr);
-    hipLaunchKernelGGL(k_inc_sort, grid_beads(N, R, B), dim3(B), 0, c->stream, c->P, D, nullptr);
     int hflags[8];
     int total = 0;
-    HIPCHK(hipMemcpyAsync(hflags, c->flags_d.p, sizeof hflags, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipMemcpyAsync(&total, c->bp_off_d.p + (size_t)R * N, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (total > 0 && total <= c->D.maxbp) {
+      // candidates into the reference's list order, then their incidences
+      size_t tb = 0;
+      HIPCHK(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, c->bp_key_d.p, c->bp_key2_d.p, c->bp_val_d.p,
+                                                c->bp_val2_d.p, total, 0, 64, c->stream));
+      if (c->sort_tmp_d.n < tb) c->sort_tmp_d.alloc(tb);
+      HIPCHK(hipcub::DeviceRadixSort::SortPairs(c->sort_tmp_d.p, tb, c->bp_key_d.p, c->bp_key2_d.p, c->bp_val_d.p,
+                                                c->bp_val2_d.p, total, 0, 64, c->stream));
+      hipLaunchKernelGGL(k_bp_permute, dim3((total + B - 1) / B), dim3(B), 0, c->stream, D, total, c->bp_val2_d.p,
+                         c->bp_mp2_d.p, c->bp_coef2_d.p);
+      std::swap(c->bp_mp_d, c->bp_mp2_d);
+      std::swap(c->bp_coef_d, c->bp_coef2_d);
+      D = c->ptrs();
+      hipLaunchKernelGGL(k_bp_inc, dim3((total + B - 1) / B), dim3(B), 0, c->stream, c->P, D, total);
+    }
+    hipLaunchKernelGGL(k_inc_sort, grid_beads(N, R, B), dim3(B), 0, c->stream, c->P, D, nullptr);
+    HIPCHK(hipMemcpyAsync(hflags, c->flags_d.p, sizeof hflags, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
     bool grow = false;
     if (hflags[2] > c->D.maxnb) { c->D.maxnb = hflags[2] + hflags[2] / 4 + 8; grow = true; }
@@ -693,6 +734,20 @@ int sis_gpu_create(const sis_topo *topo, const sis_params *prm, const sis_option
         c->topo[imp] = Topo{imp - i + 1, n, i, (topo->moltype[ch] == 2 ? 1 : 0) | (sq << 4) | ((q ? 1 : 0) << 8)};
       }
     }
+    std::vector<int> cid(N);
+    int maxlen = 1;
+    for (int ch = 0, q = 0; ch < topo->nchains; ch++) {
+      maxlen = std::max(maxlen, (int)topo->nmp_chain[ch]);
+      for (int i = 0; i < topo->nmp_chain[ch]; i++) cid[q++] = ch;
+    }
+    {
+      double span = (double)R * topo->nchains * (double)topo->nchains * maxlen * (double)maxlen;
+      if (span > 9.0e18) throw ApiError{SIS_ERR_ARG, "system too large for the base-pair list key"};
+    }
+    c->chain_id_d.alloc(N);
+    HIPCHK(hipMemcpy(c->chain_id_d.p, cid.data(), N * sizeof(int), hipMemcpyHostToDevice));
+    c->P.nchains = topo->nchains;
+    c->P.maxlen = maxlen;
     c->topo_d.alloc(N);
     HIPCHK(hipMemcpy(c->topo_d.p, c->topo.data(), N * sizeof(Topo), hipMemcpyHostToDevice));
     // parameters

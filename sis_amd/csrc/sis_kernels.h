@@ -318,12 +318,34 @@ __global__ void k_bp_fill(Params P, Ptrs D, const int *mask) {
     dst[7] = r;
     D.bp_coef[2 * (size_t)gb] = D.bp3_dH[h];
     D.bp_coef[2 * (size_t)gb + 1] = D.bp3_dS[h];
-    for (int q = 0; q < 6; q++) {
-      size_t gq = (size_t)r * N + (mp[q] - 1);
-      int slot = atomicAdd(&D.n_inc[gq], 1);
-      if (slot < D.maxinc) D.inc[((size_t)r * D.maxinc + slot) * N + (mp[q] - 1)] = gb * 8 + q;
-      else { D.flags[0] = 1; atomicMax(&D.flags[6], slot + 1); }
-    }
+    // reference list order: (chain_i, chain_j, i, j)
+    // (src/neighbor_list.F90:99-114)
+    D.bp_key[gb] = (((unsigned long long)(r * (int64_t)P.nchains + D.chain_id[a]) * P.nchains +
+                     D.chain_id[b]) * P.maxlen + (i - 1)) * P.maxlen + (j - 1);
+    D.bp_val[gb] = gb;
+  }
+}
+// gather the candidate records into key order
+__global__ void k_bp_permute(Ptrs D, int total, const int *order, int *mp_out, double *coef_out) {
+  int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= total) return;
+  int o = order[p];
+  for (int q = 0; q < 8; q++) mp_out[(size_t)p * 8 + q] = D.bp_mp[(size_t)o * 8 + q];
+  coef_out[2 * (size_t)p] = D.bp_coef[2 * (size_t)o];
+  coef_out[2 * (size_t)p + 1] = D.bp_coef[2 * (size_t)o + 1];
+}
+// incidences of each candidate on its six beads
+__global__ void k_bp_inc(Params P, Ptrs D, int total) {
+  int gb = blockIdx.x * blockDim.x + threadIdx.x;
+  if (gb >= total) return;
+  const int *mp = D.bp_mp + (size_t)gb * 8;
+  int r = mp[7];
+  const int N = P.N;
+  for (int q = 0; q < 6; q++) {
+    size_t gq = (size_t)r * N + (mp[q] - 1);
+    int slot = atomicAdd(&D.n_inc[gq], 1);
+    if (slot < D.maxinc) D.inc[((size_t)r * D.maxinc + slot) * N + (mp[q] - 1)] = gb * 8 + q;
+    else { D.flags[0] = 1; atomicMax(&D.flags[6], slot + 1); }
   }
 }
 // deterministic incidence order: sort each bead's list by (candidate, role)
@@ -640,30 +662,29 @@ __device__ __forceinline__ V3<double> bead_force(const Params &P, const Ptrs &D,
   for (int p = 0; p < 2; p++) {
     int s = loc - p;
     if (!valid(s, 2)) continue;
-    V3<T> v = cvt<T>(pvec(P, box, x[at(s, 0)], x[at(s, 1)]));
-    V3<T> f[2];
-    bond_force<T>(P, v, f);
-    F = F + cvt<double>(f[p]);
+    V3<double> v = pvec(P, box, x[at(s, 0)], x[at(s, 1)]);
+    V3<double> f[2];
+    bond_force<double>(P, v, f);
+    F = F + f[p];
   }
   for (int p = 0; p < 3; p++) {
     int s = loc - p;
     if (!valid(s, 3)) continue;
     double4 p0 = x[at(s, 0)], p1 = x[at(s, 1)], p2 = x[at(s, 2)];
-    V3<T> v12 = cvt<T>(pvec(P, box, p0, p1)), v32 = cvt<T>(pvec(P, box, p2, p1));
-    V3<T> f[3];
-    angle_force<T>(P, v12, v32, f);
-    F = F + cvt<double>(f[p]);
+    V3<double> v12 = pvec(P, box, p0, p1), v32 = pvec(P, box, p2, p1);
+    V3<double> f[3];
+    angle_force<double>(P, v12, v32, f);
+    F = F + f[p];
   }
   if (P.dih_type) {
     for (int p = 0; p < 4; p++) {
       int s = loc - p;
       if (!valid(s, 4)) continue;
       double4 p0 = x[at(s, 0)], p1 = x[at(s, 1)], p2 = x[at(s, 2)], p3 = x[at(s, 3)];
-      V3<T> vij = cvt<T>(pvec(P, box, p0, p1)), vkj = cvt<T>(pvec(P, box, p2, p1)),
-            vkl = cvt<T>(pvec(P, box, p2, p3));
-      V3<T> f[4];
-      dih_force<T>(P, vij, vkj, vkl, f);
-      F = F + cvt<double>(f[p]);
+      V3<double> vij = pvec(P, box, p0, p1), vkj = pvec(P, box, p2, p1), vkl = pvec(P, box, p2, p3);
+      V3<double> f[4];
+      dih_force<double>(P, vij, vkj, vkl, f);
+      F = F + f[p];
     }
   }
   size_t g = (size_t)r * N + a;

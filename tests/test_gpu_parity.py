@@ -133,25 +133,60 @@ def test_fp32_pair_arithmetic_error():
 def test_performance_mode_limit_rule():
     """Counter-RNG limit MC per component: afterwards no nucleotide keeps
     more than max_bp_per_nt pairs, and only pairs formed before the MC
-    survive."""
+    survive (formed set taken from a run without the limit rule)."""
     s = configs.condensate_c2()
     print("[case] perf MC", flush=True)
+    s0 = dict(s)
+    s0["params"] = dict(s["params"], nstep_bp_mc=0)
+    eng0 = Engine(s0, parity=False, seed=99)
+    eng0.neighbor_list(0)
+    eng0.force(1, 0)
+    pre = eng0.bp(1)["status"].copy()
     eng = Engine(s, parity=False, seed=99)
     eng.neighbor_list(0)
-    eng.force(1, 0)            # no MC: all formed pairs kept
-    pre = eng.bp(1)["status"].copy()
     eng.force(1, 1)
     b = eng.bp(1)
     st = b["status"]
+    assert np.array_equal(b["bp"], eng0.bp(1)["bp"])
     assert np.all(st <= pre)
     cnt = np.zeros(s["xyz"].shape[1] + 1, int)
     for row in b["bp"][st == 1]:
         cnt[row[0]] += 1
         cnt[row[1]] += 1
     assert cnt.max() <= 1
-    # every nucleotide that had a formed pair before keeps >= 1 pair unless
-    # all its pairs went to partners that kept another one
-    assert st.sum() > 0.3 * pre.sum()
+
+
+def test_performance_mode_limit_distribution():
+    """The per-component limit MC samples the same distribution as the
+    reference's serial MC: survivor counts and per-candidate survival
+    frequencies over 160 independent draws on one C2 configuration."""
+    s = configs.condensate_c2()
+    print("[case] perf MC distribution", flush=True)
+    K = 160
+    o = corc.Oracle(s)
+    o.neighbor_list(1)
+    ser = []
+    for k in range(K):
+        o.set_tape(1, corc.mt_words(1000 + k, 120_000))
+        o.force(1, 1)
+        ser.append(o.bp_state(1)[0].astype(float))
+    ser = np.array(ser)
+    eng = Engine(s, parity=False)
+    eng.neighbor_list(0)
+    par = []
+    for k in range(K):
+        eng.set_rng(5000 + k)
+        eng.force(1, 1)
+        par.append(eng.bp(1)["status"].astype(float))
+    par = np.array(par)
+    a, b = ser.sum(1), par.sum(1)
+    se = np.sqrt(a.var() / K + b.var() / K)
+    print(f"survivors serial {a.mean():.2f}+-{a.std():.2f}  components {b.mean():.2f}+-{b.std():.2f}")
+    assert abs(a.mean() - b.mean()) < 4.0 * se + 1e-9
+    pa, pb = ser.mean(0), par.mean(0)
+    mask = (pa + pb) > 0
+    assert np.corrcoef(pa[mask], pb[mask])[0, 1] > 0.95
+    assert np.abs(pa - pb).max() < 0.25
 
 
 def test_performance_mode_dynamics_stable():
