@@ -173,7 +173,10 @@ int sis_gpu_get_labels(sis_gpu_ctx *ctx, int32_t *rep2lab);
  * [8] launch mode used, [9] workgroup size. */
 int sis_gpu_get_info(sis_gpu_ctx *ctx, int64_t *out, int32_t n);
 /* Device time (ms, HIP events on the context stream) of the last call of
- * a kind: 0 = step, 1 = rebuild, 2 = force. */
+ * a kind: 0 = step, 1 = rebuild, 2 = force; 3 = mean duration of one
+ * launch of the step's dominant kernel during the last sis_gpu_step call
+ * (the fused force + integrator kernel, or the persistent / batched step
+ * kernel), 4 = number of such launches. */
 double sis_gpu_last_kernel_ms(sis_gpu_ctx *ctx, int32_t which);
 
 #ifdef __cplusplus

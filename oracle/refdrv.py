@@ -126,6 +126,12 @@ def _worker(q, so, sysd, ops, seed, nstep_exchange):
                 x = np.zeros((nrep, nmp, 3))
                 lib.sisref_get_xyz(_dp(x))
                 out[op[-1]] = x
+            elif kind == "job_md_timed":
+                import time
+                e = np.zeros((nrep, 11))
+                t0 = time.perf_counter()
+                lib.sisref_job_md(C.c_int64(int(op[1])), _dp(e))
+                out[op[-1]] = time.perf_counter() - t0
             elif kind == "job_md":
                 e = np.zeros((nrep, 11))
                 lib.sisref_job_md(C.c_int64(int(op[1])), _dp(e))

@@ -63,6 +63,7 @@ struct Ptrs {
   double *md_cr;             // [R*N]
   double *tK, *ele_coef, *lambdaD, *ele_cut, *box;  // [R], box [R*3]
   double4 *pos, *vel, *acc, *move, *frc;  // [R*N]
+  double4 *pos_out;          // [R*N] positions after the update (double buffer)
   // lists
   int maxnb, maxw, maxbp, maxinc, maxbpb;
   int *nb_ele, *n_ele, *nb_wca, *n_wca;   // [R*maxnb*N], [R*N] ...

@@ -88,6 +88,29 @@ struct sis_gpu_ctx {
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   double last_ms[4] = {0, 0, 0, 0};
+  // HIP events around each launch of the dominant kernel of a step call
+  std::vector<hipEvent_t> tev;
+  size_t tev_used = 0;
+  double dom_ms = 0.0;
+  int64_t dom_launches = 0;
+  void tmark() {
+    if (tev_used >= tev.size()) {
+      hipEvent_t e;
+      HIPCHK(hipEventCreate(&e));
+      tev.push_back(e);
+    }
+    HIPCHK(hipEventRecord(tev[tev_used++], stream));
+  }
+  void tcollect() {
+    dom_ms = 0.0;
+    dom_launches = (int64_t)(tev_used / 2);
+    for (size_t k = 0; k + 1 < tev_used; k += 2) {
+      float ms = 0.f;
+      HIPCHK(hipEventElapsedTime(&ms, tev[k], tev[k + 1]));
+      dom_ms += ms;
+    }
+    tev_used = 0;
+  }
   // host copies
   std::vector<Topo> topo;
   std::vector<double> tK, ionic, mass, fric;
@@ -103,7 +126,7 @@ struct sis_gpu_ctx {
   DevBuf<int> bp3_hash_d;
   DevBuf<double> bp3_dH_d, bp3_dS_d, md_a_d, md_cf_d, md_cx_d, mass_d, md_cr_d;
   DevBuf<double> tK_d, ele_coef_d, lambdaD_d, ele_cut_d, box_d;
-  DevBuf<double4> pos_d, vel_d, acc_d, move_d, frc_d;
+  DevBuf<double4> pos_d, pos2_d, vel_d, acc_d, move_d, frc_d;
   DevBuf<int> nb_ele_d, n_ele_d, nb_wca_d, n_wca_d, bp_tmp_d, n_bp_tmp_d, bp_off_d, bp_mp_d,
       n_bp_d, inc_d, n_inc_d, nt_cnt_d, comp_d, flags_d;
   DevBuf<double> bp_coef_d, ene_bp_d, for_bp_d;
@@ -111,10 +134,11 @@ struct sis_gpu_ctx {
   DevBuf<int> cell_of_d, cell_cnt_d, cell_start_d, cell_beads_d, cell_fill_d;
   DevBuf<double> blk_top2_d;
   DevBuf<int> rebuild_flag_d, mask_d, mc_d, scan_bsum_d, scan_total_d;
-  DevBuf<int> exc_cnt_d, exc_off_d, exc_fill_d, exc_list_d, ntlist_d, bpseq_d;
+  DevBuf<int> exc_cnt_d, exc_off_d, exc_fill_d, exc_list_d, ntlist_d, bpseq_d, seg_d;
   DevBuf<uint32_t> tape_d;
   DevBuf<int64_t> tape_off_d, tape_pos_d;
   DevBuf<double> noise_d, bm_cache_d, ene_part_d, ene_out_d, ecT_d, eCoef_d, eLam_d;
+  DevBuf<unsigned long long> pers_bar_d;
   DevBuf<int> pers_sync_d, chain_id_d, bp_val_d, bp_val2_d, bp_mp2_d;
   DevBuf<unsigned long long> bp_key_d, bp_key2_d;
   DevBuf<double> bp_coef2_d;
@@ -136,7 +160,7 @@ struct sis_gpu_ctx {
     d.md_cr = md_cr_d.p;
     d.tK = tK_d.p; d.ele_coef = ele_coef_d.p; d.lambdaD = lambdaD_d.p; d.ele_cut = ele_cut_d.p;
     d.box = box_d.p;
-    d.pos = pos_d.p; d.vel = vel_d.p; d.acc = acc_d.p; d.move = move_d.p; d.frc = frc_d.p;
+    d.pos = pos_d.p; d.pos_out = pos2_d.p; d.vel = vel_d.p; d.acc = acc_d.p; d.move = move_d.p; d.frc = frc_d.p;
     d.maxnb = D.maxnb; d.maxw = D.maxw; d.maxbp = D.maxbp; d.maxinc = D.maxinc; d.maxbpb = D.maxbpb;
     d.nb_ele = nb_ele_d.p; d.n_ele = n_ele_d.p; d.nb_wca = nb_wca_d.p; d.n_wca = n_wca_d.p;
     d.bp_tmp = bp_tmp_d.p; d.n_bp_tmp = n_bp_tmp_d.p; d.bp_off = bp_off_d.p;
@@ -161,7 +185,7 @@ struct sis_gpu_ctx {
                          &comp_d, &flags_d, &cell_of_d, &cell_cnt_d, &cell_start_d,
                          &cell_beads_d, &cell_fill_d, &rebuild_flag_d, &mask_d, &mc_d,
                          &scan_bsum_d, &scan_total_d, &exc_cnt_d, &exc_off_d, &exc_fill_d,
-                         &exc_list_d, &ntlist_d, &bpseq_d, &pers_sync_d, &chain_id_d,
+                         &exc_list_d, &ntlist_d, &bpseq_d, &seg_d, &pers_sync_d, &chain_id_d,
                          &bp_val_d, &bp_val2_d, &bp_mp2_d};
     for (auto *b : ib) b->free_();
     DevBuf<double> *db[] = {&bp3_dH_d, &bp3_dS_d, &md_a_d, &md_cf_d, &md_cx_d, &mass_d,
@@ -170,7 +194,7 @@ struct sis_gpu_ctx {
                             &bm_cache_d, &ene_part_d, &ene_out_d, &ecT_d, &eCoef_d, &eLam_d,
                             &bp_coef2_d};
     for (auto *b : db) b->free_();
-    DevBuf<double4> *vb[] = {&pos_d, &vel_d, &acc_d, &move_d, &frc_d};
+    DevBuf<double4> *vb[] = {&pos_d, &pos2_d, &vel_d, &acc_d, &move_d, &frc_d};
     for (auto *b : vb) b->free_();
     topo_d.free_();
     bp_status_d.free_();
@@ -179,8 +203,10 @@ struct sis_gpu_ctx {
     tape_off_d.free_();
     tape_pos_d.free_();
     bp_key_d.free_();
+    pers_bar_d.free_();
     bp_key2_d.free_();
     sort_tmp_d.free_();
+    for (auto e : tev) (void)hipEventDestroy(e);
     if (ev0) (void)hipEventDestroy(ev0);
     if (ev1) (void)hipEventDestroy(ev1);
     if (stream) (void)hipStreamDestroy(stream);
@@ -482,30 +508,30 @@ void bp_phase(sis_gpu_ctx *c, const std::vector<int> &mc, const double *ecT_dev,
                        c->ntlist_d.p, c->bpseq_d.p, mask_dev);
   } else {
     if (c->nbp_total == 0) return;
-    hipLaunchKernelGGL(k_uf_init, grid_beads(N, R, B), dim3(B), 0, c->stream, c->P, D, c->exc_cnt_d.p,
+    size_t RN = (size_t)R * N;
+    if (c->seg_d.n < RN * SEGCAP) c->seg_d.alloc(RN * SEGCAP);
+    hipLaunchKernelGGL(k_comp_init, grid_beads(N, R, B), dim3(B), 0, c->stream, c->P, D, c->exc_cnt_d.p,
                        c->exc_fill_d.p, mask_dev);
     hipLaunchKernelGGL(k_uf_union, dim3((c->nbp_total + B - 1) / B), dim3(B), 0, c->stream, c->P, D,
                        c->nbp_total, c->mc_d.p, mask_dev);
-    hipLaunchKernelGGL(k_uf_count, grid_beads(N, R, B), dim3(B), 0, c->stream, c->P, D, c->exc_cnt_d.p,
-                       c->mc_d.p, mask_dev);
-    scan_exclusive(c, c->exc_cnt_d.p, c->exc_off_d.p, (int64_t)R * N, nullptr);
-    hipLaunchKernelGGL(k_uf_fill, grid_beads(N, R, B), dim3(B), 0, c->stream, c->P, D, c->exc_off_d.p,
-                       c->exc_fill_d.p, c->exc_list_d.p, c->mc_d.p, mask_dev);
-    hipLaunchKernelGGL(k_mc_comp, grid_beads(N, R, 64), dim3(64), 0, c->stream, c->P, D, c->exc_off_d.p,
-                       c->exc_cnt_d.p, c->exc_list_d.p, c->mc_d.p, ecT_dev, c->seed, c->istep, mask_dev);
+    hipLaunchKernelGGL(k_seg_fill, dim3((c->nbp_total + B - 1) / B), dim3(B), 0, c->stream, c->P, D,
+                       c->nbp_total, c->exc_cnt_d.p, c->seg_d.p, c->exc_fill_d.p, c->mc_d.p, mask_dev);
+    hipLaunchKernelGGL(k_mc_wave, dim3(N, R), dim3(64), 0, c->stream, c->P, D, c->exc_cnt_d.p, c->seg_d.p,
+                       c->exc_fill_d.p, c->mc_d.p, ecT_dev, c->seed, c->istep, mask_dev);
   }
 }
 
 void force_kernel(sis_gpu_ctx *c, int integrate, const int *mask_dev) {
   const int N = c->N, R = c->R;
-  const int B = 256;
-  dim3 g = grid_beads(N, R, B);
+  constexpr int B = 256, LPB = 8;
+  dim3 g((N + B / LPB - 1) / (B / LPB), R);
   c->nblk_force = g.x;
+  if ((int64_t)c->blk_top2_d.n < (int64_t)R * g.x * 2) c->blk_top2_d.alloc((size_t)R * g.x * 2);
   if (c->precision == 1)
-    hipLaunchKernelGGL((k_force_int<float, B>), g, dim3(B), 0, c->stream, c->P, c->ptrs(), integrate,
+    hipLaunchKernelGGL((k_force_lanes<float, LPB, B>), g, dim3(B), 0, c->stream, c->P, c->ptrs(), integrate,
                        c->parity, c->seed, c->istep, mask_dev);
   else
-    hipLaunchKernelGGL((k_force_int<double, B>), g, dim3(B), 0, c->stream, c->P, c->ptrs(), integrate,
+    hipLaunchKernelGGL((k_force_lanes<double, LPB, B>), g, dim3(B), 0, c->stream, c->P, c->ptrs(), integrate,
                        c->parity, c->seed, c->istep, mask_dev);
 }
 
@@ -514,6 +540,7 @@ void check_flags(sis_gpu_ctx *c) {
   HIPCHK(hipMemcpyAsync(f, c->flags_d.p, sizeof f, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
   if (f[1] == 2) throw ApiError{SIS_ERR_STATE, "limit MC found an excess nucleotide without pairs"};
+  if (f[5]) throw ApiError{SIS_ERR_CAPACITY, "a base-pair conflict component exceeds 256 formed pairs"};
   if (f[1]) throw ApiError{SIS_ERR_STATE, "parity random-number tape exhausted"};
 }
 
@@ -657,7 +684,10 @@ void step_phases(sis_gpu_ctx *c) {
   }
   if (c->parity)
     hipLaunchKernelGGL(k_noise_parity, dim3(1), dim3(1), 0, c->stream, c->P, c->ptrs(), 3 * N, 0, nullptr);
+  c->tmark();
   force_kernel(c, 1, nullptr);
+  c->tmark();
+  std::swap(c->pos_d, c->pos2_d);   // updated positions become current
   hipLaunchKernelGGL(k_skin, dim3(R), dim3(64), 0, c->stream, c->P, c->ptrs(), c->nblk_force);
   if (c->opt.exchange_every > 0 && R > 1 && c->istep % c->opt.exchange_every == 0) do_exchange(c, nullptr, nullptr);
 }
@@ -833,7 +863,7 @@ int sis_gpu_create(const sis_topo *topo, const sis_params *prm, const sis_option
     c->tK_d.alloc(R); c->ele_coef_d.alloc(R); c->lambdaD_d.alloc(R); c->ele_cut_d.alloc(R);
     c->box_d.alloc(3 * R);
     c->ecT_d.alloc(R); c->eCoef_d.alloc(R); c->eLam_d.alloc(R);
-    c->pos_d.alloc(RN); c->vel_d.alloc(RN); c->acc_d.alloc(RN); c->move_d.alloc(RN); c->frc_d.alloc(RN);
+    c->pos_d.alloc(RN); c->pos2_d.alloc(RN); c->vel_d.alloc(RN); c->acc_d.alloc(RN); c->move_d.alloc(RN); c->frc_d.alloc(RN);
     c->n_ele_d.alloc(RN); c->n_wca_d.alloc(RN); c->n_bp_tmp_d.alloc(RN); c->bp_off_d.alloc(RN + 1);
     c->n_bp_d.alloc(R); c->n_inc_d.alloc(RN); c->nt_cnt_d.alloc(RN); c->comp_d.alloc(RN);
     c->flags_d.alloc(8);
@@ -1182,6 +1212,7 @@ int sis_gpu_step(sis_gpu_ctx *c, int64_t nsteps, int64_t *rebuilds) {
   if (!c) return fail(SIS_ERR_ARG, "null context");
   if (nsteps < 0) return fail(SIS_ERR_ARG, "nsteps < 0");
   int64_t r0 = c->rebuilds;
+  c->tev_used = 0;
   HIPCHK(hipEventRecord(c->ev0, c->stream));
   if (!c->lists_valid) {
     std::vector<int> all(c->R, 1);
@@ -1201,6 +1232,7 @@ int sis_gpu_step(sis_gpu_ctx *c, int64_t nsteps, int64_t *rebuilds) {
   float ms = 0;
   HIPCHK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
   c->last_ms[0] = ms;
+  c->tcollect();
   check_flags(c);
   if (rebuilds) *rebuilds = c->rebuilds - r0;
   return SIS_OK;
@@ -1240,7 +1272,9 @@ int sis_gpu_get_info(sis_gpu_ctx *c, int64_t *out, int32_t n) {
 }
 
 double sis_gpu_last_kernel_ms(sis_gpu_ctx *c, int32_t which) {
-  if (!c || which < 0 || which > 3) return -1.0;
+  if (!c || which < 0 || which > 4) return -1.0;
+  if (which == 3) return c->dom_launches ? c->dom_ms / (double)c->dom_launches : 0.0;
+  if (which == 4) return (double)c->dom_launches;
   return c->last_ms[which];
 }
 

@@ -2,6 +2,7 @@
 // per component).  Included by sis_gpu.hip.
 #pragma once
 #include "sis_device.h"
+#include "sis_mc_wave.h"
 
 namespace sis {
 
@@ -593,6 +594,119 @@ __global__ void k_mc_comp(Params P, Ptrs D, const int *exc_off, const int *cnt_e
   }
 }
 
+// Performance-form components: union-find roots, pairs bucketed per root
+// (capacity SEGCAP per component), one wavefront per component with excess.
+constexpr int SEGCAP = 256;
+__global__ void k_comp_init(Params P, Ptrs D, int *seg_n, int *has_exc, const int *mask) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x, r = blockIdx.y;
+  if (i >= P.N || (mask && !mask[r])) return;
+  size_t g = (size_t)r * P.N + i;
+  D.comp[g] = i;
+  seg_n[g] = 0;
+  has_exc[g] = 0;
+}
+__global__ void k_seg_fill(Params P, Ptrs D, int nbp_total, int *seg_n, int *seg, int *has_exc,
+                           const int *mc, const int *mask) {
+  int gb = blockIdx.x * blockDim.x + threadIdx.x;
+  if (gb >= nbp_total || !D.bp_status[gb]) return;
+  const int *mp = D.bp_mp + (size_t)gb * 8;
+  int r = mp[7];
+  if ((mask && !mask[r]) || !mc[r]) return;
+  const int N = P.N;
+  int a = mp[0] - 1, b = mp[1] - 1;
+  int root = uf_find(D.comp + (size_t)r * N, a);
+  size_t gr = (size_t)r * N + root;
+  int slot = atomicAdd(&seg_n[gr], 1);
+  if (slot < SEGCAP) seg[gr * SEGCAP + slot] = gb;
+  if (D.nt_cnt[(size_t)r * N + a] > 0 || D.nt_cnt[(size_t)r * N + b] > 0) has_exc[gr] = 1;
+}
+// serial fallback for a component larger than one wave holds: the same
+// algorithm on global memory by one lane
+__device__ void mc_component_serial(const Params &P, const Ptrs &D, int r, int root, int *seg, int n,
+                                    double beta, uint64_t seed, int64_t step) {
+  const int N = P.N;
+  int *cnt = D.nt_cnt + (size_t)r * N;
+  for (int x = 1; x < n; x++) {
+    int v = seg[x], y = x - 1;
+    while (y >= 0 && seg[y] > v) { seg[y + 1] = seg[y]; y--; }
+    seg[y + 1] = v;
+  }
+  uint32_t draw = 0x10000u;
+  int sq[64];
+  for (int iter = 0; iter <= n; iter++) {
+    // active nucleotides = endpoints with count > 0, ascending id
+    int nact = 0, prev = -1;
+    while (true) {
+      int nxt = 0x7fffffff;
+      for (int t = 0; t < n; t++) {
+        const int *mp = D.bp_mp + (size_t)seg[t] * 8;
+        int a = mp[0] - 1, b = mp[1] - 1;
+        if (a > prev && a < nxt && cnt[a] > 0) nxt = a;
+        if (b > prev && b < nxt && cnt[b] > 0) nxt = b;
+      }
+      if (nxt == 0x7fffffff) break;
+      nact++;
+      prev = nxt;
+    }
+    if (nact == 0) return;
+    int pick = (int)ceil(philox_u(seed, r, step, root, draw++) * nact), ntd = -1;
+    prev = -1;
+    for (int c = 0; c < pick; c++) {
+      int nxt = 0x7fffffff;
+      for (int t = 0; t < n; t++) {
+        const int *mp = D.bp_mp + (size_t)seg[t] * 8;
+        int a = mp[0] - 1, b = mp[1] - 1;
+        if (a > prev && a < nxt && cnt[a] > 0) nxt = a;
+        if (b > prev && b < nxt && cnt[b] > 0) nxt = b;
+      }
+      prev = nxt;
+    }
+    ntd = prev;
+    int ns = 0;
+    for (int t = 0; t < n && ns < 64; t++) {
+      int gb = seg[t];
+      const int *mp = D.bp_mp + (size_t)gb * 8;
+      if (D.bp_status[gb] && (mp[0] - 1 == ntd || mp[1] - 1 == ntd)) sq[ns++] = gb;
+    }
+    if (ns == 0) { D.flags[1] = 2; return; }
+    for (int i = 0; i < ns; i++) {
+      int sw = (int)ceil(philox_u(seed, r, step, root, draw++) * ns) - 1;
+      int t = sq[i]; sq[i] = sq[sw]; sq[sw] = t;
+    }
+    int del = sq[0];
+    for (int i = 1; i < ns; i++) {
+      double ratio = exp((D.ene_bp[sq[i]] - D.ene_bp[del]) * beta);
+      if (philox_u(seed, r, step, root, draw++) < ratio) del = sq[i];
+    }
+    D.bp_status[del] = 0;
+    const int *mp = D.bp_mp + (size_t)del * 8;
+    cnt[mp[0] - 1] -= 1;
+    cnt[mp[1] - 1] -= 1;
+  }
+}
+__global__ void __launch_bounds__(64) k_mc_wave(Params P, Ptrs D, int *seg_n, int *seg,
+                                                const int *has_exc, const int *mc,
+                                                const double *ecT, uint64_t seed, int64_t step,
+                                                const int *mask) {
+  __shared__ int sseq[64];
+  int root = blockIdx.x, r = blockIdx.y;
+  if ((mask && !mask[r]) || !mc[r]) return;
+  size_t gr = (size_t)r * P.N + root;
+  if (!has_exc[gr]) return;
+  int n = seg_n[gr];
+  double tK = ecT ? ecT[r] : D.tK[r];
+  const double BOLTZ = 1.380649e-23 * ((1.0 / 4184.0) * 6.02214076e23);
+  double beta = 1.0 / (BOLTZ * tK);
+  if (n > SEGCAP) {
+    if (threadIdx.x == 0) D.flags[5] = 1;   // bucket overflow: reported to the host
+    return;
+  }
+  int *sg = seg + gr * SEGCAP;
+  if (!mc_component_wave(P, D, r, root, sg, n, beta, seed, step, sseq)) {
+    if (threadIdx.x == 0) mc_component_serial(P, D, r, root, sg, n, beta, seed, step);
+  }
+}
+
 // ---------------------------------------------------------------------
 // Parity noise: 3N (or `count`) normals per replica with the polar
 // Box-Muller of src/rnd_boxmuller.F90 on the replica's MT words, replicas
@@ -774,7 +888,7 @@ __global__ void __launch_bounds__(B) k_force_int(Params P, Ptrs D, int integrate
       D.frc[g] = double4{F.x, F.y, F.z, 0.0};
       mv.x += dx; mv.y += dy; mv.z += dz;
       D.move[g] = mv;
-      D.pos[g] = double4{p.x + dx, p.y + dy, p.z + dz, p.w};
+      D.pos_out[g] = double4{p.x + dx, p.y + dy, p.z + dz, p.w};
       d2 = dot_nc(mv.x, mv.y, mv.z, mv.x, mv.y, mv.z);
     }
   }
@@ -790,6 +904,153 @@ __global__ void __launch_bounds__(B) k_force_int(Params P, Ptrs D, int integrate
       double m2 = a1 > b1 ? (a2 > b1 ? a2 : b1) : (a1 > b2 ? a1 : b2);
       s1[threadIdx.x] = m1;
       s2[threadIdx.x] = m2;
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    D.blk_top2[((size_t)r * gridDim.x + blockIdx.x) * 2] = s1[0];
+    D.blk_top2[((size_t)r * gridDim.x + blockIdx.x) * 2 + 1] = s2[0];
+  }
+}
+
+// Lane-split owner-computes: LPB lanes share bead a; lane l takes items
+// l, l+LPB, ... of the bead's work list [bonded terms | WCA neighbours |
+// DH neighbours | BP incidences]; partial forces are reduced with lane
+// shuffles (FP64).
+template <typename T>
+__device__ __forceinline__ V3<double> bead_force_part(const Params &P, const Ptrs &D, int r, int a,
+                                                      int l, int LPB) {
+  const int N = P.N;
+  const double *box = D.box + 3 * r;
+  const double4 *x = D.pos + (size_t)r * N;
+  const double4 pa = x[a];
+  const Topo ta = D.topo[a];
+  const int loc = ta.loc, n = ta.len, st = ta.start;
+  const bool circ = topo_circ(ta);
+  const size_t g = (size_t)r * N + a;
+  const int nw = min(D.n_wca[g], D.maxw);
+  const int ne = P.ele_on ? min(D.n_ele[g], D.maxnb) : 0;
+  const int ni = min(D.n_inc[g], D.maxinc);
+  const int nbond = P.dih_type ? 9 : 5;
+  const int total = nbond + nw + ne + ni;
+  V3<double> F{0.0, 0.0, 0.0};
+  const T coef = (T)D.ele_coef[r], rc2 = (T)(D.ele_cut[r] * D.ele_cut[r]), rl = (T)(1.0 / D.lambdaD[r]);
+  for (int it = l; it < total; it += LPB) {
+    if (it < nbond) {
+      int m = it < 2 ? 2 : (it < 5 ? 3 : 4);
+      int p = it < 2 ? it : (it < 5 ? it - 2 : it - 5);
+      int s0 = loc - p;
+      bool ok;
+      if (circ) { ok = n >= m; if (s0 < 1) s0 += n; }
+      else ok = s0 >= 1 && s0 + m - 1 <= n;
+      if (!ok) continue;
+      int idx[4];
+      for (int k = 0; k < m; k++) { int q = s0 + k; if (q > n) q -= n; idx[k] = st + q - 1; }
+      if (m == 2) {
+        V3<double> f[2];
+        bond_force<double>(P, pvec(P, box, x[idx[0]], x[idx[1]]), f);
+        F = F + f[p];
+      } else if (m == 3) {
+        V3<double> f[3];
+        angle_force<double>(P, pvec(P, box, x[idx[0]], x[idx[1]]), pvec(P, box, x[idx[2]], x[idx[1]]), f);
+        F = F + f[p];
+      } else {
+        V3<double> f[4];
+        dih_force<double>(P, pvec(P, box, x[idx[0]], x[idx[1]]), pvec(P, box, x[idx[2]], x[idx[1]]),
+                          pvec(P, box, x[idx[2]], x[idx[3]]), f);
+        F = F + f[p];
+      }
+    } else if (it < nbond + nw) {
+      int b = D.nb_wca[((size_t)r * D.maxw + (it - nbond)) * N + a];
+      F = F + cvt<double>(wca_force<T>(P, cvt<T>(pvec(P, box, pa, x[b]))));
+    } else if (it < nbond + nw + ne) {
+      int b = D.nb_ele[((size_t)r * D.maxnb + (it - nbond - nw)) * N + a];
+      F = F + cvt<double>(dh_force<T>(coef, rc2, rl, cvt<T>(pvec(P, box, pa, x[b]))));
+    } else {
+      int e = D.inc[((size_t)r * D.maxinc + (it - nbond - nw - ne)) * N + a];
+      int gb = e >> 3, role = e & 7;
+      if (D.bp_status[gb]) {
+        const double *fo = D.for_bp + (size_t)gb * 18 + 3 * role;
+        F.x += fo[0]; F.y += fo[1]; F.z += fo[2];
+      }
+    }
+  }
+  return F;
+}
+
+// GJF-2GJ update of one bead (src/job_md.F90:415-444); returns |move|^2
+__device__ __forceinline__ double gjf_update(const Params &P, const Ptrs &D, int r, int a,
+                                             const V3<double> &F, int parity, uint64_t seed,
+                                             int64_t step) {
+  size_t g = (size_t)r * P.N + a;
+  double n0, n1, n2;
+  if (parity) {
+    const double *nz = D.noise + g * 3;
+    n0 = nz[0]; n1 = nz[1]; n2 = nz[2];
+  } else {
+    uint32_t k1 = (uint32_t)(seed >> 32) ^ (0x9e3779b9u * (uint32_t)(r + 1));
+    U4 o = philox(U4{(uint32_t)step, (uint32_t)(step >> 32), (uint32_t)a, 0x6e6f6973u}, (uint32_t)seed, k1);
+    U4 o2 = philox(U4{(uint32_t)step, (uint32_t)(step >> 32), (uint32_t)a, 0x6e6f6974u}, (uint32_t)seed, k1);
+    double u1 = u01(o.x, o.y), u2 = u01(o.z, o.w), u3 = u01(o2.x, o2.y), u4 = u01(o2.z, o2.w);
+    double rr = sqrt(-2.0 * log(u1)), r2 = sqrt(-2.0 * log(u3));
+    n0 = rr * cos(6.283185307179586 * u2);
+    n1 = rr * sin(6.283185307179586 * u2);
+    n2 = r2 * cos(6.283185307179586 * u4);
+  }
+  double cr = D.md_cr[g], ca = D.md_a[a], cf = D.md_cf[a], cx = D.md_cx[a];
+  double4 v = D.vel[g], ac = D.acc[g], p = D.pos[g], mv = D.move[g];
+  double b0 = cr * n0, b1 = cr * n1, b2 = cr * n2;
+  v.x = ca * v.x + cf * F.x + (ac.x + b0);
+  v.y = ca * v.y + cf * F.y + (ac.y + b1);
+  v.z = ca * v.z + cf * F.z + (ac.z + b2);
+  double dx = cx * v.x, dy = cx * v.y, dz = cx * v.z;
+  D.vel[g] = v;
+  D.acc[g] = double4{b0, b1, b2, 0.0};
+  D.frc[g] = double4{F.x, F.y, F.z, 0.0};
+  mv.x += dx; mv.y += dy; mv.z += dz;
+  D.move[g] = mv;
+  // other beads still read D.pos in this pass: write the next buffer
+  D.pos_out[g] = double4{p.x + dx, p.y + dy, p.z + dz, p.w};
+  return dot_nc(mv.x, mv.y, mv.z, mv.x, mv.y, mv.z);
+}
+
+template <int LPB>
+__device__ __forceinline__ V3<double> group_sum(V3<double> F) {
+#pragma unroll
+  for (int off = LPB / 2; off > 0; off >>= 1) {
+    F.x += __shfl_xor(F.x, off, LPB);
+    F.y += __shfl_xor(F.y, off, LPB);
+    F.z += __shfl_xor(F.z, off, LPB);
+  }
+  return F;
+}
+
+template <typename T, int LPB, int B>
+__global__ void __launch_bounds__(B) k_force_lanes(Params P, Ptrs D, int integrate, int parity,
+                                                  uint64_t seed, int64_t step, const int *mask) {
+  __shared__ double s1[B / LPB], s2[B / LPB];
+  const int grp = threadIdx.x / LPB, l = threadIdx.x % LPB;
+  const int a = blockIdx.x * (B / LPB) + grp, r = blockIdx.y;
+  const int N = P.N;
+  const bool act = a < N && !(mask && !mask[r]);
+  V3<double> F{0.0, 0.0, 0.0};
+  if (act) F = bead_force_part<T>(P, D, r, a, l, LPB);
+  F = group_sum<LPB>(F);
+  double d2 = 0.0;
+  if (act && l == 0) {
+    size_t g = (size_t)r * N + a;
+    D.nt_cnt[g] = -P.max_bp_per_nt;
+    if (!integrate) D.frc[g] = double4{F.x, F.y, F.z, 0.0};
+    else d2 = gjf_update(P, D, r, a, F, parity, seed, step);
+  }
+  if (!integrate) return;
+  if (l == 0) { s1[grp] = d2; s2[grp] = 0.0; }
+  __syncthreads();
+  for (int off = (B / LPB) / 2; off > 0; off >>= 1) {
+    if (threadIdx.x < off) {
+      double a1 = s1[threadIdx.x], a2 = s2[threadIdx.x], b1 = s1[threadIdx.x + off], b2 = s2[threadIdx.x + off];
+      s1[threadIdx.x] = a1 > b1 ? a1 : b1;
+      s2[threadIdx.x] = a1 > b1 ? (a2 > b1 ? a2 : b1) : (a1 > b2 ? a1 : b2);
     }
     __syncthreads();
   }
