@@ -1157,8 +1157,11 @@ int64_t orc_md(orc_sys *s, int64_t nsteps, int32_t nstep_exchange, double *energ
     int last_exchange = 0;
     for (int64_t st = 0; st < nsteps; st++) {
         s->istep++;
-        int bp_mc = 0;
-        if (s->p.nstep_bp_mc > 0 && s->istep % s->p.nstep_bp_mc == 0) bp_mc = 1;
+        /* one flag for the whole replica loop (src/job_md.F90:364-366): a
+         * rebuild in replica r also forces MC on replicas r+1.. of this step
+         * (src/job_md.F90:392-394) */
+        int mc = 0;
+        if (s->p.nstep_bp_mc > 0 && s->istep % s->p.nstep_bp_mc == 0) mc = 1;
         for (int r = 0; r < nr; r++) {
             double *x = s->xyz + (size_t)r * n * 3, *v = s->velos + (size_t)r * n * 3;
             double *ac = s->accels + (size_t)r * n * 3, *mv = s->move + (size_t)r * n * 3;
@@ -1168,7 +1171,6 @@ int64_t orc_md(orc_sys *s, int64_t nsteps, int32_t nstep_exchange, double *energ
                 if (d2 > d2max) { d2max2 = d2max; d2max = d2; }
                 else if (d2 > d2max2) d2max2 = d2;
             }
-            int mc = bp_mc;
             if (sqrt(d2max) + sqrt(d2max2) > s->p.nl_margin) {
                 orc_neighbor_list(s, r + 1);
                 memset(mv, 0, sizeof(double) * 3 * n);

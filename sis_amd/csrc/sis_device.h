@@ -64,6 +64,7 @@ struct Ptrs {
   double *tK, *ele_coef, *lambdaD, *ele_cut, *box;  // [R], box [R*3]
   double4 *pos, *vel, *acc, *move, *frc;  // [R*N]
   double4 *pos_out;          // [R*N] positions after the update (double buffer)
+  double4 *fbond;            // [R*N] bonded force of each bead (computed before the pair pass)
   // lists
   int maxnb, maxw, maxbp, maxinc, maxbpb;
   int *nb_ele, *n_ele, *nb_wca, *n_wca;   // [R*maxnb*N], [R*N] ...
@@ -221,6 +222,64 @@ __device__ __forceinline__ void dih_force(const Params &P, const V3<T> &vij, con
   f[1] = ((T)-1 + a) * fi + (-b) * fl;
   f[2] = (-a) * fi + ((T)-1 + b) * fl;
   f[3] = fl;
+}
+
+// Force on position p of a bonded term only (owner-computes).
+__device__ __forceinline__ V3<double> bond_force_p(const Params &P, const V3<double> &v, int p) {
+  double d = sqrt(dot(v, v));
+  double delta = d - P.bond_r0;
+  V3<double> fv = (P.bond_k * delta / d) * v;
+  return p == 0 ? -fv : fv;
+}
+__device__ __forceinline__ V3<double> angle_force_p(const Params &P, const V3<double> &v12,
+                                                    const V3<double> &v32, int p) {
+  V3<double> f1, f3;
+  if (P.angl_reb) {
+    double d12 = dot(v12, v12), d32 = dot(v32, v32), a1232 = sqrt(d12 * d32);
+    double cosine = dot(v12, v32) / a1232;
+    double co0 = P.angl_co0;
+    double den = 1.0 - cosine * cosine;
+    double ff = -P.angl_k * (cosine - co0) * (1.0 - cosine * co0) / (den * den);
+    f1 = ff * ((1.0 / a1232) * v32 - (cosine / d12) * v12);
+    f3 = ff * ((1.0 / a1232) * v12 - (cosine / d32) * v32);
+  } else {
+    V3<double> v21 = -v12;
+    double d21 = dot(v21, v21), d32 = dot(v32, v32), d2132 = dot(v32, v21);
+    double cosine = -d2132 / sqrt(d21 * d32);
+    if (cosine > 1.0) cosine = 1.0;
+    if (cosine < -1.0) cosine = -1.0;
+    double delta = acos(cosine) - P.angl_t0;
+    double t = d21 * d32 - d2132 * d2132;
+    if (t <= 1.0) t = 1.0;
+    double ff = P.angl_k * delta / sqrt(t);
+    V3<double> f21 = ff * ((d2132 / d21) * v21 - v32);
+    V3<double> f32 = ff * ((d2132 / d32) * v32 - v21);
+    f1 = -f21;
+    f3 = f32;
+  }
+  if (p == 0) return f1;
+  if (p == 2) return f3;
+  return -(f1 + f3);
+}
+__device__ __forceinline__ V3<double> dih_force_p(const Params &P, const V3<double> &vij,
+                                                  const V3<double> &vkj, const V3<double> &vkl, int p) {
+  V3<double> m = cross(vij, vkj), n = cross(vkj, vkl);
+  double akj2 = dot(vkj, vkj), akj = sqrt(akj2);
+  double dih = atan2(akj * dot(vij, n), dot(m, n));
+  double pre;
+  if (P.dih_type == 2) {
+    double dd = dih - P.dih_p0;
+    pre = (P.dih_k * P.dih_w) * dd * exp((-0.5 * P.dih_w) * (dd * dd)) * akj;
+  } else {
+    pre = -P.dih_k * sin(dih + P.dih_p0) * akj;
+  }
+  V3<double> fi = (-pre / dot(m, m)) * m;
+  V3<double> fl = (pre / dot(n, n)) * n;
+  if (p == 0) return fi;
+  if (p == 3) return fl;
+  double a = dot(vij, vkj) / akj2, b = dot(vkl, vkj) / akj2;
+  if (p == 1) return (-1.0 + a) * fi + (-b) * fl;
+  return (-a) * fi + (-1.0 + b) * fl;
 }
 
 // WCA (src/force_wca.F90:51-58): force on bead 1 of v12 = x1 - x2.

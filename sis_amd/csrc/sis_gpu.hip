@@ -126,11 +126,11 @@ struct sis_gpu_ctx {
   DevBuf<int> bp3_hash_d;
   DevBuf<double> bp3_dH_d, bp3_dS_d, md_a_d, md_cf_d, md_cx_d, mass_d, md_cr_d;
   DevBuf<double> tK_d, ele_coef_d, lambdaD_d, ele_cut_d, box_d;
-  DevBuf<double4> pos_d, pos2_d, vel_d, acc_d, move_d, frc_d;
+  DevBuf<double4> pos_d, pos2_d, vel_d, acc_d, move_d, frc_d, fbond_d;
   DevBuf<int> nb_ele_d, n_ele_d, nb_wca_d, n_wca_d, bp_tmp_d, n_bp_tmp_d, bp_off_d, bp_mp_d,
       n_bp_d, inc_d, n_inc_d, nt_cnt_d, comp_d, flags_d;
   DevBuf<double> bp_coef_d, ene_bp_d, for_bp_d;
-  DevBuf<uint8_t> bp_status_d, bp_status_mc_d;
+  DevBuf<uint8_t> bp_status_d, bp_status_mc_d, bp_status_keep_d;
   DevBuf<int> cell_of_d, cell_cnt_d, cell_start_d, cell_beads_d, cell_fill_d;
   DevBuf<double> blk_top2_d;
   DevBuf<int> rebuild_flag_d, mask_d, mc_d, scan_bsum_d, scan_total_d;
@@ -139,11 +139,18 @@ struct sis_gpu_ctx {
   DevBuf<int64_t> tape_off_d, tape_pos_d;
   DevBuf<double> noise_d, bm_cache_d, ene_part_d, ene_out_d, ecT_d, eCoef_d, eLam_d;
   DevBuf<unsigned long long> pers_bar_d;
+  int batch_capb = 0;
+  DevBuf<int> bb_mp_d, bb_done_d, bb_nbp_d, bb_rebuild_d, bb_todo_d, bb_off_d, bb_mcfirst_d, bb_nreb_d;
+  DevBuf<int64_t> bb_stepno_d;
+  DevBuf<double> bb_coef_d, bb_ene_d, bb_for_d;
+  DevBuf<uint8_t> bb_status_d, bb_status_mc_d;
   DevBuf<int> pers_sync_d, chain_id_d, bp_val_d, bp_val2_d, bp_mp2_d;
   DevBuf<unsigned long long> bp_key_d, bp_key2_d;
   DevBuf<double> bp_coef2_d;
   DevBuf<char> sort_tmp_d;
   int nbp_total = 0;
+  std::vector<int64_t> bp_rep_off;   // candidates of replica r: [off[r], off[r+1])
+  std::vector<int> rebuilt;          // replicas the last rebuild() rebuilt
   int nblk_force = 0;
   int block = 256;
 
@@ -160,7 +167,7 @@ struct sis_gpu_ctx {
     d.md_cr = md_cr_d.p;
     d.tK = tK_d.p; d.ele_coef = ele_coef_d.p; d.lambdaD = lambdaD_d.p; d.ele_cut = ele_cut_d.p;
     d.box = box_d.p;
-    d.pos = pos_d.p; d.pos_out = pos2_d.p; d.vel = vel_d.p; d.acc = acc_d.p; d.move = move_d.p; d.frc = frc_d.p;
+    d.pos = pos_d.p; d.pos_out = pos2_d.p; d.fbond = fbond_d.p; d.vel = vel_d.p; d.acc = acc_d.p; d.move = move_d.p; d.frc = frc_d.p;
     d.maxnb = D.maxnb; d.maxw = D.maxw; d.maxbp = D.maxbp; d.maxinc = D.maxinc; d.maxbpb = D.maxbpb;
     d.nb_ele = nb_ele_d.p; d.n_ele = n_ele_d.p; d.nb_wca = nb_wca_d.p; d.n_wca = n_wca_d.p;
     d.bp_tmp = bp_tmp_d.p; d.n_bp_tmp = n_bp_tmp_d.p; d.bp_off = bp_off_d.p;
@@ -194,16 +201,20 @@ struct sis_gpu_ctx {
                             &bm_cache_d, &ene_part_d, &ene_out_d, &ecT_d, &eCoef_d, &eLam_d,
                             &bp_coef2_d};
     for (auto *b : db) b->free_();
-    DevBuf<double4> *vb[] = {&pos_d, &pos2_d, &vel_d, &acc_d, &move_d, &frc_d};
+    DevBuf<double4> *vb[] = {&pos_d, &pos2_d, &vel_d, &acc_d, &move_d, &frc_d, &fbond_d};
     for (auto *b : vb) b->free_();
     topo_d.free_();
     bp_status_d.free_();
-    bp_status_mc_d.free_();
+    bp_status_mc_d.free_(); bp_status_keep_d.free_();
     tape_d.free_();
     tape_off_d.free_();
     tape_pos_d.free_();
     bp_key_d.free_();
     pers_bar_d.free_();
+    bb_mp_d.free_(); bb_done_d.free_(); bb_nbp_d.free_(); bb_rebuild_d.free_(); bb_todo_d.free_();
+    bb_off_d.free_(); bb_mcfirst_d.free_(); bb_nreb_d.free_();
+    bb_stepno_d.free_(); bb_coef_d.free_(); bb_ene_d.free_(); bb_for_d.free_();
+    bb_status_d.free_(); bb_status_mc_d.free_();
     bp_key2_d.free_();
     sort_tmp_d.free_();
     for (auto e : tev) (void)hipEventDestroy(e);
@@ -337,12 +348,17 @@ void scan_exclusive(sis_gpu_ctx *c, const int *in, int *out, int64_t n, int *tot
   (void)total_dev;
 }
 
+void alloc_bp(sis_gpu_ctx *c);
 void alloc_lists(sis_gpu_ctx *c) {
   size_t RN = (size_t)c->R * c->N;
   c->nb_ele_d.alloc(RN * c->D.maxnb);
   c->nb_wca_d.alloc(RN * c->D.maxw);
   c->bp_tmp_d.alloc(RN * c->D.maxbpb);
   c->inc_d.alloc(RN * c->D.maxinc);
+  alloc_bp(c);
+}
+// base-pair candidate arrays of the packed layout (contents are lost)
+void alloc_bp(sis_gpu_ctx *c) {
   c->bp_mp_d.alloc((size_t)c->D.maxbp * 8);
   c->bp_coef_d.alloc((size_t)c->D.maxbp * 2);
   c->bp_status_d.alloc(c->D.maxbp);
@@ -414,8 +430,19 @@ void set_mc(sis_gpu_ctx *c, const std::vector<int> &m) {
 }
 
 // neighbor_list for the replicas in `mask` (src/neighbor_list.F90).
-void rebuild(sis_gpu_ctx *c, const std::vector<int> &mask) {
+void rebuild(sis_gpu_ctx *c, const std::vector<int> &mask_in) {
   const int N = c->N, R = c->R, B = 256;
+  std::vector<int> mask = mask_in;
+  // Replicas that are not rebuilt keep their candidates, but the packed
+  // array moves them: keep their limit-MC state to carry it over.
+  const bool carry = R > 1 && c->P.nstep_bp_mc > 1 && (int64_t)c->bp_rep_off.size() == R + 1 &&
+                     c->nbp_total > 0;
+  if (carry) {
+    if ((int64_t)c->bp_status_keep_d.n < c->nbp_total) c->bp_status_keep_d.alloc(c->nbp_total);
+    HIPCHK(hipMemcpyAsync(c->bp_status_keep_d.p, c->bp_status_mc_d.p, c->nbp_total, hipMemcpyDeviceToDevice,
+                          c->stream));
+  }
+  const std::vector<int64_t> old_off = c->bp_rep_off;
   set_mask(c, mask);
   for (int attempt = 0; attempt < 8; attempt++) {
     HIPCHK(hipMemsetAsync(c->flags_d.p, 0, 8 * sizeof(int), c->stream));
@@ -475,12 +502,32 @@ void rebuild(sis_gpu_ctx *c, const std::vector<int> &mask) {
       c->nbp_total = total;
       hipLaunchKernelGGL(k_clear_move, grid_beads(N, R, B), dim3(B), 0, c->stream, c->P, c->ptrs(),
                          c->mask_d.p);
+      c->bp_rep_off.assign(R + 1, 0);
+      if (R == 1) {
+        c->bp_rep_off[1] = total;
+      } else if (c->P.nstep_bp_mc > 1) {
+        std::vector<int> off((size_t)R * N + 1);
+        HIPCHK(hipMemcpyAsync(off.data(), c->bp_off_d.p, off.size() * sizeof(int), hipMemcpyDeviceToHost,
+                              c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        for (int r = 0; r <= R; r++) c->bp_rep_off[r] = off[(size_t)r * N];
+        if (carry)
+          for (int r = 0; r < R; r++) {
+            if (mask[r]) continue;
+            int64_t n0 = old_off[r + 1] - old_off[r], n1 = c->bp_rep_off[r + 1] - c->bp_rep_off[r];
+            if (n0 != n1) throw ApiError{SIS_ERR_STATE, "candidate count of a kept replica changed"};
+            if (n1 > 0)
+              HIPCHK(hipMemcpyAsync(c->bp_status_mc_d.p + c->bp_rep_off[r], c->bp_status_keep_d.p + old_off[r],
+                                    n1, hipMemcpyDeviceToDevice, c->stream));
+          }
+      }
+      c->rebuilt = mask;
       c->lists_valid = true;
       return;
     }
     // lists of every replica are rebuilt after a resize
-    std::vector<int> all(R, 1);
-    set_mask(c, all);
+    std::fill(mask.begin(), mask.end(), 1);
+    set_mask(c, mask);
     alloc_lists(c);
   }
   throw ApiError{SIS_ERR_CAPACITY, "neighbour lists did not converge in size"};
@@ -527,6 +574,7 @@ void force_kernel(sis_gpu_ctx *c, int integrate, const int *mask_dev) {
   dim3 g((N + B / LPB - 1) / (B / LPB), R);
   c->nblk_force = g.x;
   if ((int64_t)c->blk_top2_d.n < (int64_t)R * g.x * 2) c->blk_top2_d.alloc((size_t)R * g.x * 2);
+  hipLaunchKernelGGL((k_bonded<LPB, B>), g, dim3(B), 0, c->stream, c->P, c->ptrs(), mask_dev);
   if (c->precision == 1)
     hipLaunchKernelGGL((k_force_lanes<float, LPB, B>), g, dim3(B), 0, c->stream, c->P, c->ptrs(), integrate,
                        c->parity, c->seed, c->istep, mask_dev);
@@ -539,6 +587,7 @@ void check_flags(sis_gpu_ctx *c) {
   int f[8];
   HIPCHK(hipMemcpyAsync(f, c->flags_d.p, sizeof f, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
+  if (f[7]) throw ApiError{SIS_ERR_STATE, "limit MC met a base-pair candidate outside the system"};
   if (f[1] == 2) throw ApiError{SIS_ERR_STATE, "limit MC found an excess nucleotide without pairs"};
   if (f[5]) throw ApiError{SIS_ERR_CAPACITY, "a base-pair conflict component exceeds 256 formed pairs"};
   if (f[1]) throw ApiError{SIS_ERR_STATE, "parity random-number tape exhausted"};
@@ -595,6 +644,7 @@ double rep_uniform(sis_gpu_ctx *c, uint64_t k) {
 void do_exchange(sis_gpu_ctx *c, double *replica_energies, int32_t *accepted) {
   const int R = c->R;
   std::vector<int> all(R, 1);
+  if (!c->lists_valid) rebuild(c, all);
   std::vector<double> e1((size_t)R * 16), e2((size_t)R * 16);
   int mcflag = c->P.nstep_bp_mc > 0 ? 1 : 0;
   energy_phase(c, all, c->tK, mcflag, e1.data());
@@ -671,16 +721,34 @@ void step_phases(sis_gpu_ctx *c) {
     HIPCHK(hipMemcpyAsync(flag.data(), c->rebuild_flag_d.p, R * sizeof(int), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
   }
-  bool any = false;
-  for (int r = 0; r < R; r++) if (flag[r]) { any = true; if (c->P.nstep_bp_mc > 0) mc[r] = 1; }
-  if (any) { rebuild(c, flag); c->rebuilds++; }
+  // A rebuild forces MC (src/job_md.F90:392-394).  The reference keeps one
+  // flag for its replica loop, so replicas after a rebuilding one also do
+  // MC in that step; parity mode follows it.  Otherwise the MC decision of
+  // a replica depends on that replica alone, so that results do not depend
+  // on which other replicas share the launch.
+  bool any = false, sticky = false;
+  for (int r = 0; r < R; r++) any = any || flag[r];
+  if (any) { rebuild(c, flag); c->rebuilds++; flag = c->rebuilt; }
+  for (int r = 0; r < R; r++) {
+    if (flag[r]) sticky = c->parity != 0;
+    if ((flag[r] || sticky) && c->P.nstep_bp_mc > 0) mc[r] = 1;
+  }
   std::vector<int> all(R, 1);
   set_mask(c, all);
   bp_phase(c, mc, nullptr, nullptr);
   if (c->P.nstep_bp_mc > 1) {
-    // bp_status_MC = bp_status after an MC step (src/job_md.F90:399-401)
-    for (int r = 0; r < R; r++) (void)r;
-    if (mc_all || any) HIPCHK(hipMemcpyAsync(c->bp_status_mc_d.p, c->bp_status_d.p, c->nbp_total, hipMemcpyDeviceToDevice, c->stream));
+    // bp_status_MC = bp_status of the replicas that did MC
+    // (src/job_md.F90:399-401); candidates are grouped by replica
+    if (mc_all) {
+      HIPCHK(hipMemcpyAsync(c->bp_status_mc_d.p, c->bp_status_d.p, c->nbp_total, hipMemcpyDeviceToDevice, c->stream));
+    } else if (any) {
+      for (int r = 0; r < R; r++) {
+        if (!mc[r]) continue;
+        int64_t b0 = c->bp_rep_off[r], b1 = c->bp_rep_off[r + 1];
+        if (b1 > b0)
+          HIPCHK(hipMemcpyAsync(c->bp_status_mc_d.p + b0, c->bp_status_d.p + b0, b1 - b0, hipMemcpyDeviceToDevice, c->stream));
+      }
+    }
   }
   if (c->parity)
     hipLaunchKernelGGL(k_noise_parity, dim3(1), dim3(1), 0, c->stream, c->P, c->ptrs(), 3 * N, 0, nullptr);
@@ -863,7 +931,7 @@ int sis_gpu_create(const sis_topo *topo, const sis_params *prm, const sis_option
     c->tK_d.alloc(R); c->ele_coef_d.alloc(R); c->lambdaD_d.alloc(R); c->ele_cut_d.alloc(R);
     c->box_d.alloc(3 * R);
     c->ecT_d.alloc(R); c->eCoef_d.alloc(R); c->eLam_d.alloc(R);
-    c->pos_d.alloc(RN); c->pos2_d.alloc(RN); c->vel_d.alloc(RN); c->acc_d.alloc(RN); c->move_d.alloc(RN); c->frc_d.alloc(RN);
+    c->pos_d.alloc(RN); c->pos2_d.alloc(RN); c->fbond_d.alloc(RN); c->vel_d.alloc(RN); c->acc_d.alloc(RN); c->move_d.alloc(RN); c->frc_d.alloc(RN);
     c->n_ele_d.alloc(RN); c->n_wca_d.alloc(RN); c->n_bp_tmp_d.alloc(RN); c->bp_off_d.alloc(RN + 1);
     c->n_bp_d.alloc(R); c->n_inc_d.alloc(RN); c->nt_cnt_d.alloc(RN); c->comp_d.alloc(RN);
     c->flags_d.alloc(8);

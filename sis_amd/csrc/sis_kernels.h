@@ -622,10 +622,14 @@ __global__ void k_seg_fill(Params P, Ptrs D, int nbp_total, int *seg_n, int *seg
 }
 // serial fallback for a component larger than one wave holds: the same
 // algorithm on global memory by one lane
-__device__ void mc_component_serial(const Params &P, const Ptrs &D, int r, int root, int *seg, int n,
+__device__ __noinline__ void mc_component_serial(const Params &P, const Ptrs &D, int r, int root, int *seg, int n,
                                     double beta, uint64_t seed, int64_t step) {
   const int N = P.N;
   int *cnt = D.nt_cnt + (size_t)r * N;
+  for (int t = 0; t < n; t++) {
+    const int *mp = D.bp_mp + (size_t)seg[t] * 8;
+    if (mp[0] < 1 || mp[0] > N || mp[1] < 1 || mp[1] > N) { D.flags[7] = 1; return; }
+  }
   for (int x = 1; x < n; x++) {
     int v = seg[x], y = x - 1;
     while (y >= 0 && seg[y] > v) { seg[y + 1] = seg[y]; y--; }
@@ -917,65 +921,78 @@ __global__ void __launch_bounds__(B) k_force_int(Params P, Ptrs D, int integrate
 // l, l+LPB, ... of the bead's work list [bonded terms | WCA neighbours |
 // DH neighbours | BP incidences]; partial forces are reduced with lane
 // shuffles (FP64).
+// Bonded terms of bead a (owner-computes): lane l of an LPB-lane group
+// takes terms l, l+LPB, ... of [2 bonds | 3 angles | 4 dihedrals]
+// (src/list_local.F90:24-104 enumerates the same terms).  FP64 always.
+__device__ __forceinline__ V3<double> bonded_part(const Params &P, const Ptrs &D, int r, int a,
+                                                  int l, int LPB) {
+  const int N = P.N;
+  const double *box = D.box + 3 * r;
+  const double4 *x = D.pos + (size_t)r * N;
+  const Topo ta = D.topo[a];
+  const int loc = ta.loc, n = ta.len, st = ta.start;
+  const bool circ = topo_circ(ta);
+  const int nbond = P.dih_type ? 9 : 5;
+  V3<double> F{0.0, 0.0, 0.0};
+  for (int it = l; it < nbond; it += LPB) {
+    const int m = it < 2 ? 2 : (it < 5 ? 3 : 4);
+    const int p = it < 2 ? it : (it < 5 ? it - 2 : it - 5);
+    int s0 = loc - p;
+    bool ok;
+    if (circ) { ok = n >= m; if (s0 < 1) s0 += n; }
+    else ok = s0 >= 1 && s0 + m - 1 <= n;
+    if (!ok) continue;
+    auto bead = [&](int k) { int q = s0 + k; if (q > n) q -= n; return st + q - 1; };
+    if (m == 2) {
+      F = F + bond_force_p(P, pvec(P, box, x[bead(0)], x[bead(1)]), p);
+    } else if (m == 3) {
+      const double4 p1 = x[bead(1)];
+      F = F + angle_force_p(P, pvec(P, box, x[bead(0)], p1), pvec(P, box, x[bead(2)], p1), p);
+    } else {
+      const double4 p1 = x[bead(1)], p2 = x[bead(2)];
+      F = F + dih_force_p(P, pvec(P, box, x[bead(0)], p1), pvec(P, box, p2, p1),
+                          pvec(P, box, p2, x[bead(3)]), p);
+    }
+  }
+  return F;
+}
+
+// Pair terms of bead a: WCA and DH over the full neighbour lists in T,
+// 6-bead BP forces gathered through the incidence list; FP64 sums.
 template <typename T>
-__device__ __forceinline__ V3<double> bead_force_part(const Params &P, const Ptrs &D, int r, int a,
-                                                      int l, int LPB) {
+__device__ __forceinline__ V3<double> pair_part(const Params &P, const Ptrs &D, int r, int a, int l,
+                                                int LPB) {
   const int N = P.N;
   const double *box = D.box + 3 * r;
   const double4 *x = D.pos + (size_t)r * N;
   const double4 pa = x[a];
-  const Topo ta = D.topo[a];
-  const int loc = ta.loc, n = ta.len, st = ta.start;
-  const bool circ = topo_circ(ta);
   const size_t g = (size_t)r * N + a;
   const int nw = min(D.n_wca[g], D.maxw);
   const int ne = P.ele_on ? min(D.n_ele[g], D.maxnb) : 0;
   const int ni = min(D.n_inc[g], D.maxinc);
-  const int nbond = P.dih_type ? 9 : 5;
-  const int total = nbond + nw + ne + ni;
   V3<double> F{0.0, 0.0, 0.0};
   const T coef = (T)D.ele_coef[r], rc2 = (T)(D.ele_cut[r] * D.ele_cut[r]), rl = (T)(1.0 / D.lambdaD[r]);
-  for (int it = l; it < total; it += LPB) {
-    if (it < nbond) {
-      int m = it < 2 ? 2 : (it < 5 ? 3 : 4);
-      int p = it < 2 ? it : (it < 5 ? it - 2 : it - 5);
-      int s0 = loc - p;
-      bool ok;
-      if (circ) { ok = n >= m; if (s0 < 1) s0 += n; }
-      else ok = s0 >= 1 && s0 + m - 1 <= n;
-      if (!ok) continue;
-      int idx[4];
-      for (int k = 0; k < m; k++) { int q = s0 + k; if (q > n) q -= n; idx[k] = st + q - 1; }
-      if (m == 2) {
-        V3<double> f[2];
-        bond_force<double>(P, pvec(P, box, x[idx[0]], x[idx[1]]), f);
-        F = F + f[p];
-      } else if (m == 3) {
-        V3<double> f[3];
-        angle_force<double>(P, pvec(P, box, x[idx[0]], x[idx[1]]), pvec(P, box, x[idx[2]], x[idx[1]]), f);
-        F = F + f[p];
-      } else {
-        V3<double> f[4];
-        dih_force<double>(P, pvec(P, box, x[idx[0]], x[idx[1]]), pvec(P, box, x[idx[2]], x[idx[1]]),
-                          pvec(P, box, x[idx[2]], x[idx[3]]), f);
-        F = F + f[p];
-      }
-    } else if (it < nbond + nw) {
-      int b = D.nb_wca[((size_t)r * D.maxw + (it - nbond)) * N + a];
-      F = F + cvt<double>(wca_force<T>(P, cvt<T>(pvec(P, box, pa, x[b]))));
-    } else if (it < nbond + nw + ne) {
-      int b = D.nb_ele[((size_t)r * D.maxnb + (it - nbond - nw)) * N + a];
-      F = F + cvt<double>(dh_force<T>(coef, rc2, rl, cvt<T>(pvec(P, box, pa, x[b]))));
-    } else {
-      int e = D.inc[((size_t)r * D.maxinc + (it - nbond - nw - ne)) * N + a];
-      int gb = e >> 3, role = e & 7;
-      if (D.bp_status[gb]) {
-        const double *fo = D.for_bp + (size_t)gb * 18 + 3 * role;
-        F.x += fo[0]; F.y += fo[1]; F.z += fo[2];
-      }
+  const int *nbw = D.nb_wca + (size_t)r * D.maxw * N + a;
+  for (int k = l; k < nw; k += LPB)
+    F = F + cvt<double>(wca_force<T>(P, cvt<T>(pvec(P, box, pa, x[nbw[(size_t)k * N]]))));
+  const int *nbe = D.nb_ele + (size_t)r * D.maxnb * N + a;
+  for (int k = l; k < ne; k += LPB)
+    F = F + cvt<double>(dh_force<T>(coef, rc2, rl, cvt<T>(pvec(P, box, pa, x[nbe[(size_t)k * N]]))));
+  const int *inc = D.inc + (size_t)r * D.maxinc * N + a;
+  for (int k = l; k < ni; k += LPB) {
+    int e = inc[(size_t)k * N], gb = e >> 3, role = e & 7;
+    if (D.bp_status[gb]) {
+      const double *fo = D.for_bp + (size_t)gb * 18 + 3 * role;
+      F.x += fo[0]; F.y += fo[1]; F.z += fo[2];
     }
   }
   return F;
+}
+
+template <typename T>
+__device__ __forceinline__ V3<double> bead_force_part(const Params &P, const Ptrs &D, int r, int a,
+                                                      int l, int LPB) {
+  return bonded_part(P, D, r, a, l, LPB) + pair_part<T>(P, D, r, a, l, LPB);
 }
 
 // GJF-2GJ update of one bead (src/job_md.F90:415-444); returns |move|^2
@@ -1025,6 +1042,18 @@ __device__ __forceinline__ V3<double> group_sum(V3<double> F) {
   return F;
 }
 
+// bonded forces per bead into D.fbond (FP64)
+template <int LPB, int B>
+__global__ void __launch_bounds__(B) k_bonded(Params P, Ptrs D, const int *mask) {
+  const int grp = threadIdx.x / LPB, l = threadIdx.x % LPB;
+  const int a = blockIdx.x * (B / LPB) + grp, r = blockIdx.y;
+  const bool act = a < P.N && !(mask && !mask[r]);
+  V3<double> F{0.0, 0.0, 0.0};
+  if (act) F = bonded_part(P, D, r, a, l, LPB);
+  F = group_sum<LPB>(F);
+  if (act && l == 0) D.fbond[(size_t)r * P.N + a] = double4{F.x, F.y, F.z, 0.0};
+}
+
 template <typename T, int LPB, int B>
 __global__ void __launch_bounds__(B) k_force_lanes(Params P, Ptrs D, int integrate, int parity,
                                                   uint64_t seed, int64_t step, const int *mask) {
@@ -1034,11 +1063,13 @@ __global__ void __launch_bounds__(B) k_force_lanes(Params P, Ptrs D, int integra
   const int N = P.N;
   const bool act = a < N && !(mask && !mask[r]);
   V3<double> F{0.0, 0.0, 0.0};
-  if (act) F = bead_force_part<T>(P, D, r, a, l, LPB);
+  if (act) F = pair_part<T>(P, D, r, a, l, LPB);
   F = group_sum<LPB>(F);
   double d2 = 0.0;
   if (act && l == 0) {
     size_t g = (size_t)r * N + a;
+    const double4 fb = D.fbond[g];
+    F.x += fb.x; F.y += fb.y; F.z += fb.z;
     D.nt_cnt[g] = -P.max_bp_per_nt;
     if (!integrate) D.frc[g] = double4{F.x, F.y, F.z, 0.0};
     else d2 = gjf_update(P, D, r, a, F, parity, seed, step);

@@ -15,8 +15,8 @@
 
 namespace sis {
 
-constexpr int MCW_ITEMS = 4;   // pairs per lane   -> 256 pairs per component
-constexpr int MCW_NODES = 8;   // nodes per lane   -> 512 nucleotides
+constexpr int MCW_ITEMS = 1;   // pairs per lane   -> 64 pairs per component
+constexpr int MCW_NODES = 2;   // nodes per lane   -> 128 nucleotides
 
 __device__ __forceinline__ int lane_id() { return __lane_id(); }
 
@@ -29,7 +29,7 @@ __device__ __forceinline__ double philox_u(uint64_t seed, int r, int64_t step, i
 // Resolve the component whose formed pairs are seg[0..n) (candidate ids,
 // any order).  Must be called by all 64 lanes of a wave with uniform
 // arguments.  Returns false if the component exceeds the wave capacity.
-__device__ bool mc_component_wave(const Params &P, const Ptrs &D, int r, int root, const int *seg,
+__device__ __noinline__ bool mc_component_wave(const Params &P, const Ptrs &D, int r, int root, const int *seg,
                                   int n, double beta, uint64_t seed, int64_t step, int *seq) {
   const int L = lane_id();
   const int N = P.N;
@@ -81,6 +81,18 @@ __device__ bool mc_component_wave(const Params &P, const Ptrs &D, int r, int roo
     } else {
       ia[k] = ib[k] = -1;
       en[k] = 0.0;
+    }
+  }
+  // a candidate naming a bead outside the system means corrupted MC state:
+  // report it (flags[7]) instead of indexing with it
+  {
+    bool bad = false;
+#pragma unroll
+    for (int k = 0; k < MCW_ITEMS; k++)
+      bad = bad || (alive[k] && (ia[k] < 0 || ia[k] >= N || ib[k] < 0 || ib[k] >= N));
+    if (__ballot(bad)) {
+      if (L == 0) D.flags[7] = 1;
+      return true;
     }
   }
   // ---- unique nucleotides in ascending id order, with their counts

@@ -111,6 +111,17 @@ __global__ void __launch_bounds__(B) k_persistent(Params P, Ptrs D, PersArgs A) 
         }
       }
     }
+    // bonded forces read only positions: computed here, added in phase D
+    {
+      const int grp = gtid / LPB, l = gtid % LPB, ngrp = gthreads / LPB;
+      for (int a0 = 0; a0 < N; a0 += ngrp) {
+        int a = a0 + grp;
+        V3<double> F{0.0, 0.0, 0.0};
+        if (a < N) F = bonded_part(P, D, 0, a, l, LPB);
+        F = group_sum<LPB>(F);
+        if (a < N && l == 0) D.fbond[a] = double4{F.x, F.y, F.z, 0.0};
+      }
+    }
     grid_barrier(A.bar, target, G);
     if (mc) {
       // ---- B: bucket formed pairs per component root
@@ -143,16 +154,21 @@ __global__ void __launch_bounds__(B) k_persistent(Params P, Ptrs D, PersArgs A) 
       }
       grid_barrier(A.bar, target, G);
     }
-    // ---- D: forces and the update; reset per-step MC state
+    // ---- D: forces and the update; reset per-step MC state.  The MC
+    // result is what later non-MC steps evaluate (src/job_md.F90:388-395).
+    if (mc)
+      for (int gb = gtid; gb < A.nbp_total; gb += gthreads) D.bp_status_mc[gb] = D.bp_status[gb];
     double m1 = 0.0, m2 = 0.0;
     {
       const int grp = gtid / LPB, l = gtid % LPB, ngrp = gthreads / LPB;
       for (int a0 = 0; a0 < N; a0 += ngrp) {
         int a = a0 + grp;
         V3<double> F{0.0, 0.0, 0.0};
-        if (a < N) F = bead_force_part<T>(P, D, 0, a, l, LPB);
+        if (a < N) F = pair_part<T>(P, D, 0, a, l, LPB);
         F = group_sum<LPB>(F);
         if (a < N && l == 0) {
+          const double4 fb = D.fbond[a];
+          F.x += fb.x; F.y += fb.y; F.z += fb.z;
           D.nt_cnt[a] = -P.max_bp_per_nt;
           D.comp[a] = a;
           A.seg_n[a] = 0;
@@ -206,6 +222,328 @@ __global__ void __launch_bounds__(B) k_persistent(Params P, Ptrs D, PersArgs A) 
     A.done[0] = (int)completed;
     A.done[1] = rebuild;
     A.done[2] = (pos_a == A.pos_b) ? 1 : 0;
+  }
+}
+
+}  // namespace sis
+
+namespace sis {
+
+// ---------------------------------------------------------------------
+// Batched step kernel (mode 3): one workgroup per replica of a small
+// single-chain system.  Each workgroup advances its replica through all
+// phases of job_md steps with workgroup barriers only, rebuilding its own
+// lists (all-pairs search) when its skin test fires.  Candidate ids are
+// r * CAPB + k, so the per-phase device functions index the same arrays.
+struct BatchArgs {
+  const int *todo;            // [R] steps to run this launch
+  const int64_t *stepno;      // [R] istep of each replica before the launch
+  uint64_t seed;
+  int *seg_n, *seg, *has_exc;
+  int *done;          // [R] steps completed per replica
+  int *nbp;           // [R] candidates per replica
+  int *rebuild;       // [R] rebuild before the first step
+  const int *mc_first; // [R] limit MC on the first step (lists rebuilt by the host)
+  int *nreb;          // [R] in-kernel rebuilds done
+  double4 *pos_b;
+  int capb;
+};
+
+// Base-pair candidate arrays of one layout.
+struct BpArrays {
+  int *mp;
+  double *coef;
+  uint8_t *status, *status_mc;
+  double *ene, *for_bp;
+};
+
+// Candidates move between the packed layout (replicas concatenated in the
+// reference's list order; replica r at rep_off[r]..rep_off[r+1]) and the
+// batched one (replica r at r * capb + k).  dir = 1: packed -> batched.
+__global__ void k_bp_relayout(BpArrays S, BpArrays T, const int *rep_off, int capb, int dir) {
+  const int r = blockIdx.y;
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  const int n = rep_off[r + 1] - rep_off[r];
+  if (k >= n) return;
+  const size_t pk = (size_t)rep_off[r] + k, bt = (size_t)r * capb + k;
+  const size_t src = dir ? pk : bt, dst = dir ? bt : pk;
+  for (int q = 0; q < 8; q++) T.mp[dst * 8 + q] = S.mp[src * 8 + q];
+  T.coef[2 * dst] = S.coef[2 * src];
+  T.coef[2 * dst + 1] = S.coef[2 * src + 1];
+  T.status[dst] = S.status[src];
+  T.status_mc[dst] = S.status_mc[src];
+  T.ene[dst] = S.ene[src];
+  for (int q = 0; q < 18; q++) T.for_bp[dst * 18 + q] = S.for_bp[src * 18 + q];
+}
+
+// Incidence entries (candidate * 8 + role) renumbered in place to the
+// other layout; every entry of a bead of replica r names a candidate of r.
+__global__ void k_inc_remap(Params P, Ptrs D, const int *rep_off, int capb, int dir) {
+  const int a = blockIdx.x * blockDim.x + threadIdx.x, r = blockIdx.y;
+  if (a >= P.N) return;
+  const int N = P.N;
+  const size_t g = (size_t)r * N + a;
+  const int n = min(D.n_inc[g], D.maxinc);
+  int *inc = D.inc + (size_t)r * D.maxinc * N + a;
+  for (int k = 0; k < n; k++) {
+    const int e = inc[(size_t)k * N], gb = e >> 3, role = e & 7;
+    const int ngb = dir ? r * capb + (gb - rep_off[r]) : rep_off[r] + (gb - r * capb);
+    inc[(size_t)k * N] = ngb * 8 + role;
+  }
+}
+
+template <int B>
+__device__ void batched_rebuild(const Params &P, Ptrs &D, const BatchArgs &A, int r, int *s_off,
+                                int *s_flag) {
+  const int N = P.N;
+  const int tid = threadIdx.x;
+  // pbc_wrap (src/pbc.F90:196-226)
+  if (P.pbc) {
+    for (int a = tid; a < N; a += B) {
+      double4 &p = D.pos[(size_t)r * N + a];
+      const double *box = D.box + 3 * r;
+      double v[3] = {p.x, p.y, p.z};
+      for (int k = 0; k < 3; k++) {
+        double L = box[k];
+        if (v[k] > L) v[k] = v[k] - L * ((double)(int)((v[k] - L) / L) + 1);
+        else if (v[k] < 0.0) v[k] = v[k] + L * ((double)(int)((0.0 - v[k]) / L) + 1);
+      }
+      p.x = v[0]; p.y = v[1]; p.z = v[2];
+    }
+  }
+  __syncthreads();
+  // all-pairs search with the criteria of src/neighbor_list.F90:123-223
+  double ec = D.ele_cut[r] + P.nl_margin;
+  double ele_cut2 = P.ele_on ? ec * ec : 0.0;
+  for (int a = tid; a < N; a += B) {
+    SearchOut o{0, 0, 0};
+    double4 pa = D.pos[(size_t)r * N + a];
+    for (int b = 0; b < N; b++) {
+      if (b == a) continue;
+      pair_test(P, D, r, a, b, pa, D.pos[(size_t)r * N + b], ele_cut2, o);
+    }
+    if (o.ne > D.maxnb || o.nw > D.maxw || o.nb > D.maxbpb) *s_flag = 1;
+    search_finish(P, D, r, a, o);
+    D.n_inc[(size_t)r * N + a] = 0;
+  }
+  __syncthreads();
+  // candidate offsets (bead order is the reference order for one chain)
+  if (tid == 0) {
+    int acc = 0;
+    for (int a = 0; a < N; a++) { s_off[a] = acc; acc += D.n_bp_tmp[(size_t)r * N + a]; }
+    s_off[N] = acc;
+    if (acc > A.capb) *s_flag = 1;
+    A.nbp[r] = acc < A.capb ? acc : A.capb;
+  }
+  __syncthreads();
+  for (int a = tid; a < N; a += B) {
+    size_t g = (size_t)r * N + a;
+    Topo ti = D.topo[a];
+    int nb = D.n_bp_tmp[g];
+    for (int k = 0; k < nb; k++) {
+      int lk = s_off[a] + k;
+      if (lk >= A.capb) break;
+      int gb = r * A.capb + lk;
+      int b = D.bp_tmp[g * D.maxbpb + k];
+      Topo tj = D.topo[b];
+      int i = ti.loc, j = tj.loc, imp = a + 1, jmp = b + 1;
+      int *dst = D.bp_mp + (size_t)gb * 8;
+      dst[0] = imp;
+      dst[1] = jmp;
+      dst[2] = (i == 1) ? imp + ti.len - 1 : imp - 1;
+      dst[3] = (j == 1) ? jmp + tj.len - 1 : jmp - 1;
+      dst[4] = (i == ti.len) ? imp + 1 - ti.len : imp + 1;
+      dst[5] = (j == tj.len) ? jmp + 1 - tj.len : jmp + 1;
+      int bt;
+      int h = bp_eligible(P, D, a, b, &bt);
+      dst[6] = bt;
+      dst[7] = r;
+      D.bp_coef[2 * (size_t)gb] = D.bp3_dH[h];
+      D.bp_coef[2 * (size_t)gb + 1] = D.bp3_dS[h];
+    }
+  }
+  __syncthreads();
+  int nbr = A.nbp[r];
+  for (int k = tid; k < nbr; k += B) {
+    int gb = r * A.capb + k;
+    const int *mp = D.bp_mp + (size_t)gb * 8;
+    for (int q = 0; q < 6; q++) {
+      size_t gq = (size_t)r * N + (mp[q] - 1);
+      int slot = atomicAdd(&D.n_inc[gq], 1);
+      if (slot < D.maxinc) D.inc[((size_t)r * D.maxinc + slot) * N + (mp[q] - 1)] = gb * 8 + q;
+      else *s_flag = 1;
+    }
+  }
+  __syncthreads();
+  for (int a = tid; a < N; a += B) {
+    size_t g = (size_t)r * N + a;
+    int n = min(D.n_inc[g], D.maxinc);
+    size_t base = (size_t)r * D.maxinc * N + a, st = N;
+    for (int x = 1; x < n; x++) {
+      int v = D.inc[base + x * st], y = x - 1;
+      while (y >= 0 && D.inc[base + y * st] > v) { D.inc[base + (y + 1) * st] = D.inc[base + y * st]; y--; }
+      D.inc[base + (y + 1) * st] = v;
+    }
+    D.move[g] = double4{0.0, 0.0, 0.0, 0.0};
+    D.nt_cnt[g] = -P.max_bp_per_nt;
+    D.comp[g] = a;
+    A.seg_n[g] = 0;
+    A.has_exc[g] = 0;
+  }
+  __syncthreads();
+}
+
+template <typename T, int LPB, int B>
+__global__ void __launch_bounds__(B) k_batched(Params P, Ptrs D, BatchArgs A) {
+  __shared__ double s1[B / LPB], s2[B / LPB];
+  __shared__ int sseq[B];
+  __shared__ int s_off[1025];
+  __shared__ int s_flag, s_exit;
+  const int r = blockIdx.x;
+  const int N = P.N;
+  const int tid = threadIdx.x;
+  const double BOLTZ = 1.380649e-23 * ((1.0 / 4184.0) * 6.02214076e23);
+  const double beta = 1.0 / (BOLTZ * D.tK[r]);
+  double4 *pos_a = D.pos, *pos_b = A.pos_b;
+  int *comp = D.comp + (size_t)r * N;
+  double4 *const pos_first = D.pos;
+  if (tid == 0) s_flag = 0;
+  // per-step limit-MC state starts reset whatever ran before this launch
+  for (int a = tid; a < N; a += B) {
+    const size_t g = (size_t)r * N + a;
+    D.nt_cnt[g] = -P.max_bp_per_nt;
+    comp[a] = a;
+    A.seg_n[g] = 0;
+    A.has_exc[g] = 0;
+  }
+  __syncthreads();
+  // the first step after a rebuild does the limit MC (src/job_md.F90:388-395)
+  int mc_next = A.mc_first[r];
+  int nreb = 0;
+  if (A.todo[r] > 0 && A.rebuild[r]) {
+    batched_rebuild<B>(P, D, A, r, s_off, &s_flag);
+    if (tid == 0) A.rebuild[r] = 0;
+    mc_next = 1;
+    nreb++;
+  }
+  int64_t completed = 0;
+  int overflow = s_flag;
+  const int64_t nsteps = A.todo[r], step0 = A.stepno[r];
+  for (int64_t s = 0; s < nsteps && !overflow; s++) {
+    const int64_t step = step0 + s + 1;
+    const int mc = P.nstep_bp_mc > 0 && ((step % P.nstep_bp_mc) == 0 || mc_next);
+    mc_next = 0;
+    D.pos = pos_a;
+    D.pos_out = pos_b;
+    const int nbr = A.nbp[r];
+    for (int k = tid; k < nbr; k += B) {
+      int gb = r * A.capb + k;
+      if (bp_eval_one<T>(P, D, gb, mc) && mc) {
+        const int *mp = D.bp_mp + (size_t)gb * 8;
+        int a = mp[0] - 1, b = mp[1] - 1;
+        while (true) {
+          a = uf_find(comp, a);
+          b = uf_find(comp, b);
+          if (a == b) break;
+          if (a > b) { int t = a; a = b; b = t; }
+          if (atomicCAS(&comp[b], b, a) == b) break;
+        }
+      }
+    }
+    {
+      const int grp = tid / LPB, l = tid % LPB, ngrp = B / LPB;
+      for (int a0 = 0; a0 < N; a0 += ngrp) {
+        int a = a0 + grp;
+        V3<double> F{0.0, 0.0, 0.0};
+        if (a < N) F = bonded_part(P, D, r, a, l, LPB);
+        F = group_sum<LPB>(F);
+        if (a < N && l == 0) D.fbond[(size_t)r * N + a] = double4{F.x, F.y, F.z, 0.0};
+      }
+    }
+    __syncthreads();
+    if (mc) {
+      for (int k = tid; k < nbr; k += B) {
+        int gb = r * A.capb + k;
+        if (!D.bp_status[gb]) continue;
+        const int *mp = D.bp_mp + (size_t)gb * 8;
+        int a = mp[0] - 1, b = mp[1] - 1;
+        int root = uf_find(comp, a);
+        size_t gr = (size_t)r * N + root;
+        int slot = atomicAdd(&A.seg_n[gr], 1);
+        if (slot < SEGCAP) A.seg[gr * SEGCAP + slot] = gb;
+        if (D.nt_cnt[(size_t)r * N + a] > 0 || D.nt_cnt[(size_t)r * N + b] > 0) A.has_exc[gr] = 1;
+      }
+      __syncthreads();
+      const int wave = tid / 64, nwaves = B / 64;
+      int *seq = sseq + wave * 64;
+      for (int root = wave; root < N; root += nwaves) {
+        size_t gr = (size_t)r * N + root;
+        if (!A.has_exc[gr]) continue;
+        int n = A.seg_n[gr];
+        if (n > SEGCAP) { if ((tid & 63) == 0) D.flags[5] = 1; continue; }
+        int *sg = A.seg + gr * SEGCAP;
+        if (!mc_component_wave(P, D, r, root, sg, n, beta, A.seed, step, seq))
+          if ((tid & 63) == 0) mc_component_serial(P, D, r, root, sg, n, beta, A.seed, step);
+      }
+      __syncthreads();
+    }
+    if (mc)
+      for (int k = tid; k < nbr; k += B) D.bp_status_mc[r * A.capb + k] = D.bp_status[r * A.capb + k];
+    double m1 = 0.0, m2 = 0.0;
+    {
+      const int grp = tid / LPB, l = tid % LPB, ngrp = B / LPB;
+      for (int a0 = 0; a0 < N; a0 += ngrp) {
+        int a = a0 + grp;
+        V3<double> F{0.0, 0.0, 0.0};
+        if (a < N) F = pair_part<T>(P, D, r, a, l, LPB);
+        F = group_sum<LPB>(F);
+        if (a < N && l == 0) {
+          size_t g = (size_t)r * N + a;
+          const double4 fb = D.fbond[g];
+          F.x += fb.x; F.y += fb.y; F.z += fb.z;
+          D.nt_cnt[g] = -P.max_bp_per_nt;
+          comp[a] = a;
+          A.seg_n[g] = 0;
+          A.has_exc[g] = 0;
+          double d2 = gjf_update(P, D, r, a, F, 0, A.seed, step);
+          if (d2 > m1) { m2 = m1; m1 = d2; }
+          else if (d2 > m2) m2 = d2;
+        }
+      }
+      if (l == 0) { s1[grp] = m1; s2[grp] = m2; }
+    }
+    __syncthreads();
+    for (int off = (B / LPB) / 2; off > 0; off >>= 1) {
+      if (tid < off) {
+        double a1 = s1[tid], a2 = s2[tid], b1 = s1[tid + off], b2 = s2[tid + off];
+        s1[tid] = a1 > b1 ? a1 : b1;
+        s2[tid] = a1 > b1 ? (a2 > b1 ? a2 : b1) : (a1 > b2 ? a1 : b2);
+      }
+      __syncthreads();
+    }
+    if (tid == 0) s_exit = (sqrt(s1[0]) + sqrt(s2[0]) > P.nl_margin) ? 1 : 0;
+    { double4 *t = pos_a; pos_a = pos_b; pos_b = t; }
+    __syncthreads();
+    completed = s + 1;
+    if (s_exit) {
+      if (s + 1 < nsteps) {
+        D.pos = pos_a;
+        batched_rebuild<B>(P, D, A, r, s_off, &s_flag);
+        overflow = s_flag;
+        mc_next = 1;
+        nreb++;
+      } else if (tid == 0) {
+        A.rebuild[r] = 1;   // rebuild at the start of the next launch
+      }
+    }
+  }
+  // the host keeps one position buffer for this form: copy back if needed
+  if (pos_a != pos_first)
+    for (int a = tid; a < N; a += B) pos_first[(size_t)r * N + a] = pos_a[(size_t)r * N + a];
+  if (tid == 0) {
+    A.done[r] = (int)completed;
+    A.nreb[r] = nreb;
+    if (overflow) D.flags[0] = 1;
   }
 }
 

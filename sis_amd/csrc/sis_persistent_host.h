@@ -2,8 +2,13 @@
 // Included by sis_gpu.hip after the per-phase driver.
 #pragma once
 
+bool batched_ok(sis_gpu_ctx *c) {
+  return !c->parity && c->R >= 2 && c->P.nchains == 1 && c->N <= 1024;
+}
+
 int choose_mode(sis_gpu_ctx *c) {
   if (c->parity) return 1;
+  if (batched_ok(c)) return 3;
   if (c->R == 1 && c->N >= 1024) return 2;
   return 1;
 }
@@ -58,6 +63,7 @@ void persistent_steps(sis_gpu_ctx *c, int64_t nsteps) {
                        c->exc_fill_d.p, nullptr);
     hipLaunchKernelGGL(k_reset_cnt, grid_beads(N, 1, 256), dim3(256), 0, c->stream, c->P, D, nullptr);
     HIPCHK(hipMemsetAsync(c->pers_bar_d.p, 0, sizeof(unsigned long long), c->stream));
+    HIPCHK(hipMemsetAsync(c->flags_d.p, 0, 8 * sizeof(int), c->stream));
     PersArgs A{};
     A.nsteps = left;
     A.step0 = c->istep;
@@ -78,6 +84,7 @@ void persistent_steps(sis_gpu_ctx *c, int64_t nsteps) {
     int done[4] = {0, 0, 0, 0};
     HIPCHK(hipMemcpyAsync(done, c->pers_sync_d.p, sizeof done, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
+    check_flags(c);
     if (done[0] <= 0) throw ApiError{SIS_ERR_STATE, "persistent kernel made no progress"};
     c->istep += done[0];
     left -= done[0];
@@ -89,6 +96,210 @@ void persistent_steps(sis_gpu_ctx *c, int64_t nsteps) {
   HIPCHK(hipStreamSynchronize(c->stream));
 }
 
+namespace {
+
+BpArrays packed_arrays(sis_gpu_ctx *c) {
+  return BpArrays{c->bp_mp_d.p, c->bp_coef_d.p, c->bp_status_d.p, c->bp_status_mc_d.p, c->ene_bp_d.p,
+                  c->for_bp_d.p};
+}
+BpArrays batched_arrays(sis_gpu_ctx *c) {
+  return BpArrays{c->bb_mp_d.p, c->bb_coef_d.p, c->bb_status_d.p, c->bb_status_mc_d.p, c->bb_ene_d.p,
+                  c->bb_for_d.p};
+}
+
+// candidate range of each replica in the packed layout
+std::vector<int> packed_rep_off(sis_gpu_ctx *c) {
+  const int N = c->N, R = c->R;
+  std::vector<int> off((size_t)R * N + 1), ro(R + 1);
+  HIPCHK(hipMemcpyAsync(off.data(), c->bp_off_d.p, off.size() * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  for (int r = 0; r <= R; r++) ro[r] = off[(size_t)r * N];
+  return ro;
+}
+
+// batched slots per replica for the largest replica list `mx`
+void ensure_batched_capacity(sis_gpu_ctx *c, int mx) {
+  const int R = c->R;
+  int capb = std::max(c->batch_capb, std::max(256, 2 * mx + 64));
+  if ((int64_t)R * capb * 8 >= (int64_t)INT32_MAX)
+    throw ApiError{SIS_ERR_CAPACITY, "batched base-pair candidate ids exceed 32 bits"};
+  if (capb != c->batch_capb || c->bb_mp_d.n < (size_t)R * capb * 8) {
+    c->batch_capb = capb;
+    size_t RC = (size_t)R * capb;
+    c->bb_mp_d.alloc(RC * 8);
+    c->bb_coef_d.alloc(RC * 2);
+    c->bb_status_d.alloc(RC);
+    c->bb_status_mc_d.alloc(RC);
+    c->bb_ene_d.alloc(RC);
+    c->bb_for_d.alloc(RC * 18);
+  }
+}
+
+// packed -> batched: candidates, incidences, pending rebuild flags
+void to_batched(sis_gpu_ctx *c) {
+  const int N = c->N, R = c->R;
+  std::vector<int> ro = packed_rep_off(c), cnt(R);
+  int mx = 0;
+  for (int r = 0; r < R; r++) { cnt[r] = ro[r + 1] - ro[r]; mx = std::max(mx, cnt[r]); }
+  ensure_batched_capacity(c, mx);
+  const int capb = c->batch_capb;
+  HIPCHK(hipMemcpyAsync(c->bb_off_d.p, ro.data(), (R + 1) * sizeof(int), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(c->bb_nbp_d.p, cnt.data(), R * sizeof(int), hipMemcpyHostToDevice, c->stream));
+  if (mx > 0)
+    hipLaunchKernelGGL(k_bp_relayout, dim3((mx + 255) / 256, R), dim3(256), 0, c->stream, packed_arrays(c),
+                       batched_arrays(c), c->bb_off_d.p, capb, 1);
+  hipLaunchKernelGGL(k_inc_remap, grid_beads(N, R, 256), dim3(256), 0, c->stream, c->P, c->ptrs(), c->bb_off_d.p,
+                     capb, 1);
+  HIPCHK(hipMemcpyAsync(c->bb_rebuild_d.p, c->rebuild_flag_d.p, R * sizeof(int), hipMemcpyDeviceToDevice,
+                        c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+}
+
+// batched -> packed (the layout every other entry point reads)
+void to_packed(sis_gpu_ctx *c) {
+  const int N = c->N, R = c->R;
+  const int capb = c->batch_capb;
+  std::vector<int> cnt(R), ro(R + 1, 0);
+  HIPCHK(hipMemcpyAsync(cnt.data(), c->bb_nbp_d.p, R * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  int mx = 0;
+  for (int r = 0; r < R; r++) { ro[r + 1] = ro[r] + cnt[r]; mx = std::max(mx, cnt[r]); }
+  const int total = ro[R];
+  if (total > c->D.maxbp) {
+    c->D.maxbp = total + total / 4 + 64;
+    alloc_bp(c);
+  }
+  HIPCHK(hipMemcpyAsync(c->bb_off_d.p, ro.data(), (R + 1) * sizeof(int), hipMemcpyHostToDevice, c->stream));
+  if (mx > 0)
+    hipLaunchKernelGGL(k_bp_relayout, dim3((mx + 255) / 256, R), dim3(256), 0, c->stream, batched_arrays(c),
+                       packed_arrays(c), c->bb_off_d.p, capb, 0);
+  hipLaunchKernelGGL(k_inc_remap, grid_beads(N, R, 256), dim3(256), 0, c->stream, c->P, c->ptrs(), c->bb_off_d.p,
+                     capb, 0);
+  // per-bead candidate offsets of the packed layout
+  scan_exclusive(c, c->n_bp_tmp_d.p, c->bp_off_d.p, (int64_t)R * N, nullptr);
+  HIPCHK(hipMemcpyAsync(c->rebuild_flag_d.p, c->bb_rebuild_d.p, R * sizeof(int), hipMemcpyDeviceToDevice,
+                        c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  c->nbp_total = total;
+  c->bp_rep_off.assign(ro.begin(), ro.end());
+  c->lists_valid = true;
+}
+
+// host rebuild of the replicas in `mask`; clears their pending flags and
+// returns which replicas the first step must run the limit MC on
+std::vector<int> host_rebuild(sis_gpu_ctx *c, const std::vector<int> &mask) {
+  const int R = c->R;
+  rebuild(c, mask);
+  c->rebuilds++;
+  std::vector<int> pend(R);
+  HIPCHK(hipMemcpy(pend.data(), c->rebuild_flag_d.p, R * sizeof(int), hipMemcpyDeviceToHost));
+  std::vector<int> mcf(R, 0);
+  for (int r = 0; r < R; r++)
+    if (c->rebuilt[r]) { pend[r] = 0; mcf[r] = c->P.nstep_bp_mc > 0 ? 1 : 0; }
+  HIPCHK(hipMemcpy(c->rebuild_flag_d.p, pend.data(), R * sizeof(int), hipMemcpyHostToDevice));
+  return mcf;
+}
+
+}  // namespace
+
+// Batched form: one workgroup per replica runs whole segments of steps
+// (up to the next exchange).  Lists move to the batched layout for the
+// segment and back afterwards, so skin counters, pending rebuilds and the
+// limit-MC state carry over exactly as in the per-phase form.
 void batched_steps(sis_gpu_ctx *c, int64_t nsteps) {
-  for (int64_t s = 0; s < nsteps; s++) step_phases(c);
+  if (!batched_ok(c)) {
+    for (int64_t s = 0; s < nsteps; s++) step_phases(c);
+    return;
+  }
+  const int N = c->N, R = c->R;
+  constexpr int B = 256;
+  const size_t RN = (size_t)R * N;
+  std::vector<int> all(R, 1), mcf(R, 0);
+  if (!c->lists_valid) mcf = host_rebuild(c, all);
+  if (c->seg_d.n < RN * SEGCAP) c->seg_d.alloc(RN * SEGCAP);
+  if (c->bb_done_d.n < (size_t)R) {
+    c->bb_done_d.alloc(R); c->bb_nbp_d.alloc(R); c->bb_rebuild_d.alloc(R); c->bb_todo_d.alloc(R);
+    c->bb_stepno_d.alloc(R); c->bb_off_d.alloc(R + 1); c->bb_mcfirst_d.alloc(R); c->bb_nreb_d.alloc(R);
+  }
+  const int lpb = N <= 32 ? 8 : (N <= 64 ? 4 : (N <= 128 ? 2 : 1));
+  const void *kfn;
+  if (c->precision == 1)
+    kfn = lpb == 8 ? (const void *)k_batched<float, 8, B> : lpb == 4 ? (const void *)k_batched<float, 4, B>
+        : lpb == 2 ? (const void *)k_batched<float, 2, B> : (const void *)k_batched<float, 1, B>;
+  else
+    kfn = lpb == 8 ? (const void *)k_batched<double, 8, B> : lpb == 4 ? (const void *)k_batched<double, 4, B>
+        : lpb == 2 ? (const void *)k_batched<double, 2, B> : (const void *)k_batched<double, 1, B>;
+  c->block = B;
+  int64_t left = nsteps;
+  const int ex = c->opt.exchange_every;
+  while (left > 0) {
+    int64_t seg = left;
+    if (ex > 0) seg = std::min<int64_t>(left, ex - (c->istep % ex));
+    std::vector<int> todo(R, (int)seg), done(R, 0);
+    std::vector<int64_t> stepno(R, c->istep);
+    to_batched(c);
+    for (int attempt = 0;; attempt++) {
+      HIPCHK(hipMemcpyAsync(c->bb_todo_d.p, todo.data(), R * sizeof(int), hipMemcpyHostToDevice, c->stream));
+      HIPCHK(hipMemcpyAsync(c->bb_stepno_d.p, stepno.data(), R * sizeof(int64_t), hipMemcpyHostToDevice, c->stream));
+      HIPCHK(hipMemcpyAsync(c->bb_mcfirst_d.p, mcf.data(), R * sizeof(int), hipMemcpyHostToDevice, c->stream));
+      HIPCHK(hipMemsetAsync(c->flags_d.p, 0, 8 * sizeof(int), c->stream));
+      BatchArgs A{};
+      A.todo = c->bb_todo_d.p;
+      A.stepno = c->bb_stepno_d.p;
+      A.seed = c->seed;
+      A.seg_n = c->exc_cnt_d.p;
+      A.seg = c->seg_d.p;
+      A.has_exc = c->exc_fill_d.p;
+      A.done = c->bb_done_d.p;
+      A.nbp = c->bb_nbp_d.p;
+      A.rebuild = c->bb_rebuild_d.p;
+      A.mc_first = c->bb_mcfirst_d.p;
+      A.nreb = c->bb_nreb_d.p;
+      A.pos_b = c->pos2_d.p;
+      A.capb = c->batch_capb;
+      Params P = c->P;
+      Ptrs D = c->ptrs();
+      D.bp_mp = c->bb_mp_d.p;
+      D.bp_coef = c->bb_coef_d.p;
+      D.bp_status = c->bb_status_d.p;
+      D.bp_status_mc = c->bb_status_mc_d.p;
+      D.ene_bp = c->bb_ene_d.p;
+      D.for_bp = c->bb_for_d.p;
+      void *args[] = {&P, &D, &A};
+      c->tmark();
+      HIPCHK(hipLaunchKernel(kfn, dim3(R), dim3(B), args, 0, c->stream));
+      c->tmark();
+      std::vector<int> d(R), nr(R), f(8);
+      HIPCHK(hipMemcpyAsync(d.data(), c->bb_done_d.p, R * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+      HIPCHK(hipMemcpyAsync(nr.data(), c->bb_nreb_d.p, R * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+      HIPCHK(hipMemcpyAsync(f.data(), c->flags_d.p, 8 * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+      HIPCHK(hipStreamSynchronize(c->stream));
+      if (f[7]) throw ApiError{SIS_ERR_STATE, "limit MC met a base-pair candidate outside the system"};
+      if (f[1] == 2) throw ApiError{SIS_ERR_STATE, "limit MC found an excess nucleotide without pairs"};
+      if (f[5]) throw ApiError{SIS_ERR_CAPACITY, "a base-pair conflict component exceeds 256 formed pairs"};
+      bool lag = false;
+      std::fill(mcf.begin(), mcf.end(), 0);
+      std::vector<int> lagm(R, 0);
+      for (int r = 0; r < R; r++) {
+        done[r] += d[r];
+        stepno[r] += d[r];
+        todo[r] = (int)(seg - done[r]);
+        c->rebuilds += nr[r];
+        if (todo[r] > 0) { lag = true; lagm[r] = 1; }
+      }
+      if (!lag) break;
+      if (attempt == 7) throw ApiError{SIS_ERR_CAPACITY, "batched lists did not converge in size"};
+      // an in-kernel rebuild outgrew a capacity: rebuild the lagging
+      // replicas on the per-phase path (which grows every list), then let
+      // them finish the segment
+      to_packed(c);
+      mcf = host_rebuild(c, lagm);
+      to_batched(c);
+    }
+    to_packed(c);
+    std::fill(mcf.begin(), mcf.end(), 0);
+    c->istep += seg;
+    left -= seg;
+    if (ex > 0 && c->istep % ex == 0) do_exchange(c, nullptr, nullptr);
+  }
 }

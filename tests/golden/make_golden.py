@@ -7,7 +7,7 @@ energies, base-pair state after the limit MC, and the state after a short
 job_md run.  Only the tape prefix that the run consumes is stored (counted
 by replaying the run in the C restatement).
 
-Run:  python tests/golden/make_golden.py
+Run:  python tests/golden/make_golden.py [name ...]
 """
 from __future__ import annotations
 
@@ -57,11 +57,22 @@ def remd_small(nrep=4, seed=5):
     return sysd
 
 
+def remd_mc3(nrep=4, seed=9):
+    """Limit MC every 3 steps with a 0.04 A skin: rebuilds land on non-MC
+    steps, exercising the replica loop's shared MC flag
+    (src/job_md.F90:364-401)."""
+    sysd = configs.remd_c3(nrep=nrep, n=40, seed=seed, margin=0.04)
+    sysd["params"] = dict(sysd["params"], nstep_bp_mc=3)
+    sysd["name"] = "remd4mc3"
+    return sysd
+
+
 SYSTEMS = {
     "hairpin64": (lambda: configs.hairpin64(1), 60, 0),
     "small_pbc": (lambda: configs.small_pbc(4, 30, 100.0, seed=5), 60, 0),
     "mixed": (mixed_chains, 40, 0),
     "remd4": (remd_small, 40, 10),
+    "remd4mc3": (remd_mc3, 40, 10),
 }
 
 
@@ -135,5 +146,7 @@ def load(name):
 
 
 if __name__ == "__main__":
-    for nm, (b, n, nex) in SYSTEMS.items():
+    names = sys.argv[1:] or list(SYSTEMS)
+    for nm in names:
+        b, n, nex = SYSTEMS[nm]
         make(nm, b, n, nex)

@@ -18,7 +18,7 @@ from tests.golden.make_golden import load
 
 pytestmark = pytest.mark.gpu
 
-FIXTURES = ["hairpin64", "small_pbc", "mixed", "remd4"]
+FIXTURES = ["hairpin64", "small_pbc", "mixed", "remd4", "remd4mc3"]
 
 
 def _sorted_rows(a):
@@ -207,13 +207,15 @@ def test_performance_mode_dynamics_stable():
     assert np.all(np.isfinite(e))
 
 
-@pytest.mark.parametrize("precision", [0, 1])
-def test_persistent_form_matches_phase_form(precision):
+@pytest.mark.parametrize("precision,mc_every", [(0, 1), (1, 1), (0, 3)])
+def test_persistent_form_matches_phase_form(precision, mc_every):
     """The persistent single-launch step (grid barriers between phases)
     gives the same trajectory as one launch per phase: same counter-RNG
-    streams, same summation order."""
+    streams, same summation order.  mc_every > 1 covers the steps between
+    limit-MC steps, which evaluate only the pairs the last MC kept."""
     s = configs.condensate_c2()
-    print(f"[case] persistent vs phases precision={precision}", flush=True)
+    s["params"] = dict(s["params"], nstep_bp_mc=mc_every)
+    print(f"[case] persistent vs phases precision={precision} nstep_bp_mc={mc_every}", flush=True)
     out = []
     for mode in ("phases", "persistent"):
         eng = Engine(s, parity=False, mode=mode, seed=31, precision=precision)
@@ -224,3 +226,37 @@ def test_persistent_form_matches_phase_form(precision):
         assert eng.info()["mode"] == (1 if mode == "phases" else 2)
     for a, b in zip(out[0], out[1]):
         assert np.abs(a - b).max() < 1e-9
+
+
+@pytest.mark.parametrize("mc_every", [1, 3])
+def test_batched_form_matches_phase_form(mc_every):
+    """Batched form (one workgroup per replica, in-workgroup rebuilds and
+    limit MC, host exchange at the boundaries) follows the per-phase form
+    on 8 temperature replicas of a 128-nt chain with exchanges every 10
+    steps: same exchange decisions, same trajectories to rounding."""
+    s = configs.remd_c3(nrep=8, n=128, seed=3)
+    s["params"] = dict(s["params"], nstep_bp_mc=mc_every)
+    print(f"[case] batched vs phases nstep_bp_mc={mc_every}", flush=True)
+    res = []
+    for mode in ("phases", "batched"):
+        eng = Engine(s, parity=False, mode=mode, seed=77, exchange_every=10)
+        eng.init_velocities()
+        eng.neighbor_list(0)
+        eng.step(40)
+        res.append((eng.get_state(), eng.labels(), eng.info()["mode"]))
+    assert res[0][2] == 1 and res[1][2] == 3
+    assert np.array_equal(res[0][1], res[1][1])
+    for a, b in zip(res[0][0], res[1][0]):
+        assert np.abs(a - b).max() < 1e-6
+
+
+def test_batched_dynamics_stable_fp32():
+    s = configs.remd_c3(nrep=32, n=128, seed=4)
+    print("[case] batched fp32", flush=True)
+    eng = Engine(s, parity=False, mode="batched", seed=5, exchange_every=100, precision=1)
+    eng.init_velocities()
+    eng.neighbor_list(0)
+    eng.step(300)
+    x, v, a = eng.get_state()
+    assert np.all(np.isfinite(x)) and np.all(np.isfinite(v))
+    assert eng.info()["istep"] == 300

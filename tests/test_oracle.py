@@ -10,7 +10,7 @@ from sis_amd import configs
 from oracle import corc, refdrv
 from tests.golden.make_golden import load
 
-FIXTURES = ["hairpin64", "small_pbc", "mixed", "remd4"]
+FIXTURES = ["hairpin64", "small_pbc", "mixed", "remd4", "remd4mc3"]
 
 
 def _oracle_from_fixture(name):
@@ -195,3 +195,28 @@ def test_live_reference_random_systems(seed):
     f = o.force(1, 1)
     assert np.abs(f - ref["F"]).max() / np.abs(ref["F"]).max() < 1e-12
     assert np.array_equal(o.bp_state(1)[0], ref["B"][0].astype(np.uint8))
+
+
+@pytest.mark.ref
+def test_live_reference_shared_mc_flag():
+    """Replica loop with limit MC every 3 steps and a 0.04 A skin, so that
+    rebuilds fall on non-MC steps: the reference's single flg_bp_MC makes a
+    rebuild in replica r force MC on the later replicas of that step
+    (src/job_md.F90:364-401).  Oracle trajectory vs the reference build."""
+    s = configs.remd_c3(nrep=4, n=40, seed=9, margin=0.04)
+    s["params"] = dict(s["params"], nstep_bp_mc=3)
+    nstep, nex = 40, 10
+    ops = [("tape", r, 400000, f"t{r}") for r in range(1, 5)] + [("tape", -1, 4096, "tr")]
+    ref0 = refdrv.run(s, ops, seed=99, nstep_exchange=nex)
+    ref1 = refdrv.run(s, [("job_md", nstep, "R")], seed=99, nstep_exchange=nex)["R"]
+    o = corc.Oracle(s)
+    for r in range(1, 5):
+        o.set_tape(r, ref0[f"t{r}"])
+    o.set_tape(-1, ref0["tr"])
+    o.init_velocities()
+    nreb, _ = o.md(nstep, nex)
+    assert nreb >= 4   # rebuilds happen inside the run
+    x, v, a = o.state()
+    assert np.abs(x - ref1["xyz"]).max() < 1e-10
+    assert np.abs(v - ref1["velos"]).max() < 1e-12
+    assert np.array_equal(o.labels(), ref1["rep2lab"][:4])
