@@ -8,6 +8,8 @@ trajectories driven by the same MT words within 1e-7 A (positions) and
 1e-6 A/tau (velocities) of the reference after the fixture's 40-60 steps
 (FP64 summation order differs, so differences grow with the step count); FP32 pair arithmetic within 1e-5 per-bead
 relative force error (|dF| / max(|F_ref|, 1e-3 kcal/mol/A))."""
+from pathlib import Path
+
 import numpy as np
 import pytest
 
@@ -260,3 +262,45 @@ def test_batched_dynamics_stable_fp32():
     x, v, a = eng.get_state()
     assert np.all(np.isfinite(x)) and np.all(np.isfinite(v))
     assert eng.info()["istep"] == 300
+
+
+def test_fortran_host_matches_c_abi(tmp_path):
+    """The Fortran host program (sis_amd/fortran/sis_gpu_host_demo, bound
+    through module sis_gpu_iface) gives the same forces, energies and
+    trajectory as the same calls made through ctypes."""
+    import subprocess
+    from sis_amd.abi import params_to_struct
+    exe = Path(__file__).resolve().parents[1] / "sis_amd" / "fortran" / "sis_gpu_host_demo"
+    if not exe.exists():
+        pytest.skip("Fortran host program not built (flang absent at build time)")
+    s = configs.small_pbc()
+    nsteps, seed = 50, 1234
+    nmp = int(np.sum(s["nmp_chain"]))
+    fin, fout = tmp_path / "in.bin", tmp_path / "out.bin"
+    with open(fin, "wb") as f:
+        f.write(bytes(params_to_struct(s["params"])))
+        f.write(np.array([s["nchains"], nmp, 1, 0], np.int32).tobytes())
+        for k in ("nmp_chain", "moltype", "seq"):
+            f.write(np.ascontiguousarray(s[k], np.int32).tobytes())
+        f.write(np.ascontiguousarray(s["xyz"], np.float64).tobytes())
+        f.write(np.array([nsteps, seed], np.int64).tobytes())
+    print("[case] fortran host demo", flush=True)
+    r = subprocess.run([str(exe), str(fin), str(fout)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    raw = fout.read_bytes()
+    nd = 3 * nmp + 11 + 3 * nmp
+    d = np.frombuffer(raw[: 8 * nd], np.float64)
+    f_ft, e_ft, x_ft = d[: 3 * nmp].reshape(nmp, 3), d[3 * nmp: 3 * nmp + 11], d[3 * nmp + 11:].reshape(nmp, 3)
+    nreb_ft = int(np.frombuffer(raw[8 * nd: 8 * nd + 8], np.int64)[0])
+
+    eng = Engine(s, parity=False, mode="auto", seed=seed)
+    eng.neighbor_list(0)
+    f_py = eng.force(1, 1)
+    e_py = eng.energy_sumup(1, -1.0, 1, 1)
+    eng.init_velocities()
+    nreb_py = eng.step(nsteps)
+    x_py = eng.get_state()[0][0]
+    assert np.abs(f_ft - f_py).max() <= 1e-12 * np.abs(f_py).max()
+    assert np.allclose(e_ft, e_py, rtol=1e-12, atol=1e-12)
+    assert nreb_ft == nreb_py
+    assert np.abs(x_ft - x_py).max() < 1e-9
