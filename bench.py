@@ -1,20 +1,31 @@
 #!/usr/bin/env python3
 """Benchmark of the SIS MD step on one MI355X.
 
-Default workload: 200 chains of 47-nt CAG repeat (9,400 beads) in a periodic
-2,499.1 A cube (1 mM nucleotides), WCA + Debye-Hueckel (150 mM, 50 A cutoff)
-+ triplet base pairing with the limit rule (one pair per nucleotide, MC
-every step), GJF-2GJ Langevin at 310 K, dt = 0.05, displacement-triggered
-neighbour-list rebuild with a 5 A skin, FP64.  One "step" is one pass of
-the job_md loop body over all beads.  Synthetic seeded coordinates.
+Default workload (the metric's "batched FP32" case): 256 temperature
+replicas of one 128-nt chain, geometric ladder 290-380 K, non-periodic,
+exchange every 1,000 steps; WCA + Debye-Hueckel (150 mM, 50 A cutoff) +
+triplet base pairing with the limit rule (MC every step); GJF-2GJ Langevin,
+dt = 0.05; displacement-triggered rebuild with a 5 A skin.  Batched form
+(one workgroup per replica), FP32 pair arithmetic with FP64 state and
+accumulation.  One "step" is one pass of the job_md loop body over every
+replica.  Synthetic seeded coordinates.
 
-Prints one JSON line (metric, value, roofline, cpu_baseline, ...).
+Beside the headline value the same JSON line carries:
+  plain_fp64   - the same chain as one job per launch sequence, FP64,
+                 per-phase form (the baseline the batched FP32 form is
+                 compared with), and the ratio;
+  fp32_force   - per-bead force error of FP32 pair arithmetic vs FP64 on
+                 the same lists (|dF| / max(|F|, 1e-3 kcal/mol/A));
+  serving      - p50/p99 latency and throughput of a Poisson stream of
+                 jobs through the job pool (steps scaled down, stated);
+  roofline     - algorithmic HBM bytes of the dominant kernel / its
+                 measured launch time;
+  cpu_baseline - the reference Fortran (OpenMP build) on this host.
 """
 from __future__ import annotations
 
 import argparse
 import json
-import math
 import os
 import sys
 import time
@@ -27,6 +38,7 @@ sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0
 TAU_FS = 48.888  # reduced time unit (g/mol, A, kcal/mol)
+METRIC = "bead-steps/s batched FP32 vs plain FP64 at force tol; p50/p99 job latency"
 
 
 def bytes_per_bead_step(p_wca, p_ele, p_bp):
@@ -36,13 +48,18 @@ def bytes_per_bead_step(p_wca, p_ele, p_bp):
     return 144.0 + 36.0 + 8.0 * p_wca + 8.0 * p_ele + 44.0 * p_bp
 
 
+def force_kernel_bytes(p_wca, p_ele):
+    """Algorithmic bytes of the per-phase pair/integrator kernel per bead:
+    state read + write and the WCA / DH pair indices."""
+    return 144.0 + 8.0 * p_wca + 8.0 * p_ele
+
+
 def load_traffic(workload, mode):
     f = ROOT / "profiles" / "traffic.json"
     if not f.exists():
         return None
     try:
-        d = json.loads(f.read_text())
-        return d.get(f"{workload}:{mode}")
+        return json.loads(f.read_text()).get(f"{workload}:{mode}")
     except Exception:
         return None
 
@@ -54,25 +71,157 @@ def cpu_baseline(sysd, steps, threads):
     from oracle import refdrv
     if not refdrv.available(omp=True):
         return None
-    t1 = refdrv.run(sysd, [("job_md_timed", steps, "t")], omp=True, threads=threads, timeout=900)["t"]
-    t0 = refdrv.run(sysd, [("job_md_timed", 0, "t")], omp=True, threads=threads, timeout=900)["t"]
+    nex = int(sysd.get("exchange_every", 0))
+    t1 = refdrv.run(sysd, [("job_md_timed", steps, "t")], omp=True, threads=threads, timeout=900,
+                    nstep_exchange=nex)["t"]
+    t0 = refdrv.run(sysd, [("job_md_timed", 0, "t")], omp=True, threads=threads, timeout=900,
+                    nstep_exchange=nex)["t"]
     dt = max(t1 - t0, 1e-9)
     n = int(np.sum(sysd["nmp_chain"])) * int(sysd["nrep"])
-    return dict(value=n * steps / dt, unit="bead-steps/s", cores=threads, kind="reference",
-                sample=f"{steps} job_md steps of the same system (set-up excluded), OMP_NUM_THREADS={threads}",
+    return dict(value=round(n * steps / dt, 1), unit="bead-steps/s", cores=threads, kind="reference",
+                sample=f"{steps} job_md steps of the same workload (set-up excluded), OMP_NUM_THREADS={threads}",
                 seconds=round(dt, 3))
+
+
+def single_replica(sysd, r=0):
+    s = dict(sysd)
+    s["nrep"] = 1
+    s["temps"] = np.asarray(sysd["temps"])[r: r + 1]
+    s["xyz"] = np.asarray(sysd["xyz"]).reshape(int(sysd["nrep"]), -1, 3)[r: r + 1]
+    s["params"] = dict(sysd["params"], tempK=float(s["temps"][0]))
+    return s
+
+
+def plain_fp64_rate(sysd1, steps, warmup):
+    """One job, FP64, per-phase form (one launch per phase per step)."""
+    from sis_amd.engine import Engine
+    eng = Engine(sysd1, precision=0, parity=False, mode="phases", seed=7)
+    eng.init_velocities()
+    eng.neighbor_list(0)
+    eng.step(warmup)
+    t0 = time.perf_counter()
+    eng.step(steps)
+    wall = time.perf_counter() - t0
+    n = int(np.sum(sysd1["nmp_chain"]))
+    eng.close()
+    return dict(value=round(n * steps / wall, 1), unit="bead-steps/s", steps=steps,
+                ms_per_step=round(1e3 * wall / steps, 4), form="phases", dtype="f64")
+
+
+def fp32_force_error(sysd):
+    """FP32 vs FP64 pair arithmetic on the same lists, every replica, limit
+    MC off so that every formed pair contributes."""
+    from sis_amd.engine import Engine
+    s = dict(sysd, params=dict(sysd["params"], nstep_bp_mc=0))
+    e64 = Engine(s, precision=0, mode="phases")
+    e32 = Engine(s, precision=1, mode="phases")
+    worst, sq, n, gnum, gden = 0.0, 0.0, 0, 0.0, 0.0
+    for e in (e64, e32):
+        e.neighbor_list(0)
+    for r in range(1, int(s["nrep"]) + 1):
+        f64 = e64.force(r, 0)
+        f32 = e32.force(r, 0)
+        d = np.linalg.norm(f32 - f64, axis=1)
+        rel = d / np.maximum(np.linalg.norm(f64, axis=1), 1e-3)
+        worst = max(worst, float(rel.max()))
+        sq += float((rel ** 2).sum())
+        n += rel.size
+        gnum += float((d ** 2).sum())
+        gden += float((f64 ** 2).sum())
+    e64.close()
+    e32.close()
+    return dict(max=float(f"{worst:.3e}"), rms=float(f"{np.sqrt(sq / n):.3e}"),
+                global_l2=float(f"{np.sqrt(gnum / max(gden, 1e-300)):.3e}"), tol=1e-5,
+                floor_kcal_mol_A=1e-3)
+
+
+def _draw_job(args):
+    seed, k, max_chains = args
+    from sis_amd import configs
+    return configs.serving_job(np.random.default_rng((seed, k)), max_chains=max_chains)
+
+
+def draw_serving_jobs(njobs, max_chains, seed=2026):
+    """Jobs drawn like the serving config (1-max_chains chains of 20-400 nt,
+    40-70% GC, 280-360 K, 50-500 mM, 0.5-5 mM, log-uniform 1e4-1e6 steps);
+    job k from its own stream (seed, k), in worker processes started before
+    this process touches the GPU."""
+    import multiprocessing as mp
+    nproc = max(1, min(16, os.cpu_count() or 1))
+    with mp.get_context("spawn").Pool(nproc) as wp:
+        return wp.map(_draw_job, [(seed, k, max_chains) for k in range(njobs)])
+
+
+def serving_run(drawn, step_scale, rate, max_chains, seed=2026, timeout=150.0):
+    """Poisson arrivals at `rate` of the drawn jobs, steps times step_scale,
+    alternately FP64 and FP32, through the job pool.  Prints progress every
+    few seconds; stops waiting after `timeout` s (reported as incomplete)."""
+    from sis_amd.serving import DONE, JobPool
+    njobs = len(drawn)
+    jobs = [(s, max(1, int(round(s["nstep"] * step_scale))), k % 2) for k, s in enumerate(drawn)]
+    rng = np.random.default_rng(seed)
+    gaps = rng.exponential(1.0 / rate, size=njobs)
+    arrivals = np.cumsum(gaps) - gaps[0]
+    nb = [int(np.sum(s["nmp_chain"])) for s, _, _ in jobs]
+    print(f"[bench] serving: {njobs} jobs, beads {min(nb)}-{max(nb)} (mean {int(np.mean(nb))}), steps "
+          f"{min(j[1] for j in jobs)}-{max(j[1] for j in jobs)}, rate {rate}/s", flush=True)
+    lat, ids = [], []
+    incomplete = 0
+    with JobPool() as pool:
+        t0 = time.perf_counter()
+        k = 0
+        pend = {}
+        t_print = 0.0
+        while k < njobs or pend:
+            now = time.perf_counter() - t0
+            if now - t_print > 3.0:
+                t_print = now
+                st_ = pool.stats()
+                print(f"[bench] serving t={now:.1f}s submitted {k} done {int(st_['done'])} running "
+                      f"{int(st_['running'])} queued {int(st_['queued'])} launches {int(st_['launches'])} "
+                      f"bead-steps {st_['bead_steps']:.3e}", flush=True)
+            if now > timeout:
+                incomplete = len(pend) + (njobs - k)
+                break
+            while k < njobs and arrivals[k] <= now:
+                s, nst, pr = jobs[k]
+                jid = pool.submit(s, nst, seed=1000 + k, precision=pr)
+                pend[jid] = (k, nst * int(np.sum(s["nmp_chain"])))
+                ids.append(jid)
+                k += 1
+            for jid in list(pend):
+                st, tm, _ = pool.poll(jid)
+                if st == DONE:
+                    lat.append(tm[2] - tm[0])
+                    pend.pop(jid)
+                    pool.forget(jid)
+            time.sleep(0.002)
+        wall = time.perf_counter() - t0
+        stt = pool.stats()
+    lat = np.array(lat) if lat else np.array([np.nan])
+    return dict(jobs=njobs, incomplete=incomplete, step_scale=step_scale, arrival_rate_per_s=rate,
+                max_chains=max_chains,
+                p50_s=round(float(np.percentile(lat, 50)), 4), p99_s=round(float(np.percentile(lat, 99)), 4),
+                bead_steps_per_s=round(stt["bead_steps"] / wall, 1), wall_s=round(wall, 2),
+                gpu_busy_frac=round(stt["gpu_s"] / wall, 3), launches=int(stt["launches"]),
+                mix="half FP32, half FP64 (alternating)")
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=4000)
-    ap.add_argument("--warmup", type=int, default=400)
-    ap.add_argument("--workload", default="c2", choices=["c2", "c3", "c4"])
+    ap.add_argument("--steps", type=int, default=None)
+    ap.add_argument("--warmup", type=int, default=None)
+    ap.add_argument("--workload", default="c3", choices=["c2", "c3", "c4"])
     ap.add_argument("--mode", default="auto", choices=["auto", "phases", "persistent", "batched"])
-    ap.add_argument("--precision", type=int, default=0)
-    ap.add_argument("--cpu-steps", type=int, default=300)
+    ap.add_argument("--precision", type=int, default=None)
+    ap.add_argument("--cpu-steps", type=int, default=None)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-extras", action="store_true", help="skip plain FP64, FP32 error and serving legs")
+    ap.add_argument("--serving-jobs", type=int, default=48)
+    ap.add_argument("--serving-scale", type=float, default=2e-3)
+    ap.add_argument("--serving-rate", type=float, default=40.0)
+    ap.add_argument("--serving-max-chains", type=int, default=64)
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -80,32 +229,45 @@ def main():
               "(replica / spatial sharding over several GPUs is a design note only).", file=sys.stderr)
         sys.exit(2)
 
+    drawn = None
+    if not args.no_extras and args.serving_jobs > 0:
+        t0 = time.perf_counter()
+        drawn = draw_serving_jobs(args.serving_jobs, args.serving_max_chains)
+        print(f"[bench] drew {len(drawn)} serving jobs in {time.perf_counter() - t0:.1f}s", flush=True)
+
     from sis_amd import configs
     from sis_amd.engine import Engine
 
     if args.workload == "c2":
         sysd = configs.condensate_c2()
-        wname = "C2: 200 x 47-nt CAG, 9,400 beads, PBC 2,499.1 A (1 mM), 150 mM, 310 K, FP64, single job"
+        wname = "C2: 200 x 47-nt CAG, 9,400 beads, PBC 2,499.1 A (1 mM), 150 mM, 310 K, single job"
+        steps, warm, prec, cpu_steps = 4000, 400, 0, 300
     elif args.workload == "c4":
         sysd = configs.condensate_c4()
         wname = "C4: 2,000 x 47-nt CAG, 94,000 beads, PBC 4,273.5 A (2 mM), 150 mM, 310 K"
+        steps, warm, prec, cpu_steps = 1000, 100, 1, 20
     else:
         sysd = configs.remd_c3()
-        wname = "C3: 256 T-replicas x 128 nt, 290-380 K, exchange every 1,000 steps"
+        wname = "C3: 256 T-replicas x 128 nt (50% GC), 290-380 K, non-periodic, exchange every 1,000 steps"
+        steps, warm, prec, cpu_steps = 20000, 2000, 1, 200
+    steps = args.steps if args.steps is not None else steps
+    warm = args.warmup if args.warmup is not None else warm
+    prec = args.precision if args.precision is not None else prec
+    cpu_steps = args.cpu_steps if args.cpu_steps is not None else cpu_steps
     nmp = int(np.sum(sysd["nmp_chain"]))
     nrep = int(sysd["nrep"])
-    print(f"[bench] workload {args.workload}: {nmp} beads x {nrep} replicas; mode {args.mode}", flush=True)
+    print(f"[bench] workload {args.workload}: {nmp} beads x {nrep} replicas; mode {args.mode}; "
+          f"precision {prec}", flush=True)
 
-    eng = Engine(sysd, precision=args.precision, parity=False, mode=args.mode, seed=2024,
+    eng = Engine(sysd, precision=prec, parity=False, mode=args.mode, seed=2024,
                  exchange_every=int(sysd.get("exchange_every", 0)))
     eng.init_velocities()
     eng.neighbor_list(0)
     t0 = time.perf_counter()
-    eng.step(args.warmup)
-    print(f"[bench] warmup {args.warmup} steps {time.perf_counter() - t0:.2f}s", flush=True)
-    info = eng.info()
+    eng.step(warm)
+    print(f"[bench] warmup {warm} steps {time.perf_counter() - t0:.2f}s", flush=True)
     t0 = time.perf_counter()
-    reb = eng.step(args.steps)          # returns after the device work completes
+    reb = eng.step(steps)          # returns after the device work completes
     wall = time.perf_counter() - t0
     dev_ms = eng.last_ms(0)
     dom_ms = eng.last_ms(3)
@@ -113,50 +275,66 @@ def main():
     info2 = eng.info()
     x, v, a = eng.get_state()
     assert np.all(np.isfinite(x)), "non-finite coordinates"
-    bead_steps = nmp * nrep * args.steps
+    eng.close()
+    print(f"[bench] timed {steps} steps {wall:.3f}s", flush=True)
+    bead_steps = nmp * nrep * steps
     value = bead_steps / wall
     p_wca = info2["nwca"] / nmp
     p_ele = info2["nele"] / nmp
     p_bp = info2["nbp"] / nmp
     B = bytes_per_bead_step(p_wca, p_ele, p_bp)
-    steps_per_launch = args.steps / max(dom_n, 1)
-    achieved = None
-    if dom_ms > 0:
-        achieved = B * nmp * nrep * steps_per_launch / (dom_ms * 1e-3) / 1e9
     mode_used = {1: "phases", 2: "persistent", 3: "batched"}.get(info2["mode"], str(info2["mode"]))
-    ns_day = (args.steps / wall) * 0.05 * TAU_FS * 86400 * 1e-6
+    steps_per_launch = steps / max(dom_n, 1)
+    if mode_used == "phases":
+        kname, kbytes = "k_force_lanes", force_kernel_bytes(p_wca, p_ele)
+    else:
+        kname, kbytes = {"persistent": "k_persistent", "batched": "k_batched"}[mode_used], B
+    ns_day = (steps / wall) * 0.05 * TAU_FS * 86400 * 1e-6
     out = {
-        "metric": "bead-steps/s batched FP32 vs plain FP64 at force tol; p50/p99 job latency",
+        "metric": METRIC,
         "value": round(value, 1),
         "unit": "bead-steps/s",
         "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": round(1e3 * wall / args.steps, 5),
+        "steps": steps,
+        "warmup": warm,
+        "ms_per_step": round(1e3 * wall / steps, 5),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if args.precision == 0 else "f32-pair/f64-state",
+        "dtype": "f64" if prec == 0 else "f32-pair/f64-state",
         "data": "synthetic (seeded random coils)",
-        "config": {"workload": wname, "beads": nmp, "replicas": nrep, "mode": mode_used,
+        "config": {"workload": wname, "beads": nmp, "replicas": nrep, "form": mode_used,
                    "rebuilds": int(reb), "p_wca": round(p_wca, 3), "p_ele": round(p_ele, 3),
                    "p_bp": round(p_bp, 3), "bytes_per_bead_step": round(B, 1),
-                   "ns_per_day": round(ns_day, 2), "device_ms": round(dev_ms, 3)},
+                   "ns_per_day_per_trajectory": round(ns_day, 2), "device_ms": round(dev_ms, 3)},
         "roofline": None,
         "cpu_baseline": None,
     }
-    if achieved is not None:
+    if dom_ms > 0:
+        achieved = kbytes * nmp * nrep * steps_per_launch / (dom_ms * 1e-3) / 1e9
         out["roofline"] = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
                            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
                            "traffic": load_traffic(args.workload, mode_used),
-                           "kernel": {"phases": "k_force_int", "persistent": "k_persistent_steps",
-                                      "batched": "k_batched_steps"}.get(mode_used, mode_used),
-                           "kernel_ms": round(dom_ms, 5), "launches": dom_n,
+                           "kernel": kname, "kernel_ms": round(dom_ms, 5), "launches": dom_n,
+                           "bytes_per_bead_step": round(kbytes, 1),
                            "bead_steps_per_launch": int(nmp * nrep * steps_per_launch)}
+    if not args.no_extras:
+        if nrep > 1:
+            s1 = single_replica(sysd)
+            pl = plain_fp64_rate(s1, steps=min(steps, 2000), warmup=200)
+            pl["workload"] = "replica 1 of the same workload alone"
+            out["plain_fp64"] = pl
+            out["ratio_vs_plain_fp64"] = round(value / pl["value"], 2)
+            print(f"[bench] plain fp64 {pl['value']:.3e}", flush=True)
+        out["fp32_force"] = fp32_force_error(sysd if nrep > 1 else sysd)
+        print(f"[bench] fp32 force error {out['fp32_force']}", flush=True)
+        if drawn:
+            out["serving"] = serving_run(drawn, args.serving_scale, args.serving_rate, args.serving_max_chains)
+            print(f"[bench] serving {out['serving']}", flush=True)
     if not args.no_cpu:
         threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(16, os.cpu_count() or 1)
         try:
-            out["cpu_baseline"] = cpu_baseline(sysd, args.cpu_steps, threads)
+            out["cpu_baseline"] = cpu_baseline(sysd, cpu_steps, threads)
         except Exception as exc:  # keep the GPU line even if the CPU leg fails
             print(f"[bench] cpu baseline failed: {exc}", file=sys.stderr)
     print(json.dumps(out), flush=True)

@@ -179,6 +179,35 @@ int sis_gpu_get_info(sis_gpu_ctx *ctx, int64_t *out, int32_t n);
  * kernel), 4 = number of such launches. */
 double sis_gpu_last_kernel_ms(sis_gpu_ctx *ctx, int32_t which);
 
+/* ---- Job pool: the serving form -------------------------------------
+ * No counterpart in the reference, where one job is one process.  Jobs
+ * (own topology, temperature, salt, box, steps and seed; the force field
+ * of the pool's first job) are submitted as they arrive.  A scheduler
+ * thread admits them in arrival order while their HBM footprint fits
+ * hbm_budget_bytes (0 = 80% of free memory).  It keeps their state
+ * resident and advances all resident jobs in shared launches, each job by
+ * about quantum_bead_steps per round (0 = 2e6).  A job runs the job_md loop
+ * body (src/job_md.F90:360-460) in performance mode; its result depends
+ * only on its own inputs, not on the other jobs in the pool. */
+enum { SIS_JOB_QUEUED = 0, SIS_JOB_RUNNING = 1, SIS_JOB_DONE = 2, SIS_JOB_FAILED = 3 };
+typedef struct sis_pool sis_pool;
+int sis_pool_create(uint64_t hbm_budget_bytes, int64_t quantum_bead_steps, int32_t max_active,
+                    sis_pool **pool);
+/* xyz (3, nmp) of one replica; prm->tempK, ionic_strength and box are the job's
+ * own.  precision as in sis_options. */
+int sis_pool_submit(sis_pool *pool, const sis_topo *topo, const sis_params *prm, const double *xyz,
+                    int32_t precision, int64_t nsteps, uint64_t seed, int64_t *job_id);
+/* state = SIS_JOB_*; times[5] (may be NULL) = submit, start, end (s since the pool
+ * was created, -1 if not yet), steps done, rebuilds; xyz (3, nmp) is filled once
+ * the job is done.  A failed job returns SIS_ERR_STATE with its message. */
+int sis_pool_poll(sis_pool *pool, int64_t job_id, int32_t *state, double *times, double *xyz);
+/* drop the record of a finished or failed job */
+int sis_pool_forget(sis_pool *pool, int64_t job_id);
+/* out[0..8]: queued, running, done, failed, bead-steps, GPU seconds in launches,
+ * launches, HBM bytes held by resident jobs, seconds since creation */
+int sis_pool_stats(sis_pool *pool, double *out, int32_t n);
+int sis_pool_destroy(sis_pool *pool);
+
 #ifdef __cplusplus
 }
 #endif

@@ -118,6 +118,8 @@ EXPORTED = [
     "sis_gpu_set_rng", "sis_gpu_set_rng_tape", "sis_gpu_init_velocities",
     "sis_gpu_step", "sis_gpu_exchange", "sis_gpu_get_labels",
     "sis_gpu_get_info", "sis_gpu_last_kernel_ms",
+    "sis_pool_create", "sis_pool_submit", "sis_pool_poll", "sis_pool_forget",
+    "sis_pool_stats", "sis_pool_destroy",
 ]
 
 
@@ -146,6 +148,12 @@ def _declare(lib: C.CDLL) -> None:
         "sis_gpu_get_labels": (i32, [vp, P(i32)]),
         "sis_gpu_get_info": (i32, [vp, P(i64), i32]),
         "sis_gpu_last_kernel_ms": (d, [vp, i32]),
+        "sis_pool_create": (i32, [C.c_uint64, i64, i32, P(vp)]),
+        "sis_pool_submit": (i32, [vp, P(SisTopo), P(SisParams), P(d), i32, i64, C.c_uint64, P(i64)]),
+        "sis_pool_poll": (i32, [vp, i64, P(i32), P(d), P(d)]),
+        "sis_pool_forget": (i32, [vp, i64]),
+        "sis_pool_stats": (i32, [vp, P(d), i32]),
+        "sis_pool_destroy": (i32, [vp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)

@@ -134,24 +134,22 @@ def _place_chains(rng, seqs, box, clash=10.0):
     """Random coils with start points uniform in the box; whole chains are
     rejected while any bead sits within `clash` of an earlier chain
     (minimum image)."""
+    from scipy.spatial import cKDTree
     coords = []
     allx = np.zeros((0, 3))
+    tree = None
     for s in seqs:
         for _t in range(1000):
             start = rng.uniform(0.0, box, size=3)
             x = random_coil(rng, len(s), start=start)
-            if len(allx):
-                ok = True
-                for p in x:
-                    d = allx - p
-                    d -= box * np.round(d / box)
-                    if (d * d).sum(1).min() < clash * clash:
-                        ok = False
-                        break
-                if not ok:
+            if tree is not None:
+                # nearest earlier bead under the minimum image
+                d, _ = tree.query(np.mod(x, box), k=1)
+                if d.min() < clash:
                     continue
             coords.append(x)
             allx = np.vstack([allx, x])
+            tree = cKDTree(np.mod(allx, box), boxsize=box)
             break
         else:
             raise RuntimeError("could not place chain")

@@ -301,6 +301,35 @@ __device__ __forceinline__ V3<T> dh_force(T coef, T rc2, T rlam, const V3<T> &v1
   return dv * v12;
 }
 
+// One BP angle term for the angle between a (= v12) and b, with
+// aa = |a|^2, bb = |b|^2, ab = a.b, nab = |a||b|.  Returns d = theta -
+// theta0 and Fa = pre (a - (ab/bb) b), Fb = pre (b - (ab/aa) a) with
+// pre = 2 k d / (|a||b| sin theta).  FP64 follows the reference's
+// expressions (src/force_bp_limit_triplet.F90:144-154).  FP32 takes theta,
+// |a||b| sin theta and the two perpendicular components from c = a x b
+// instead: the same quantities, but acos, the radicand and the
+// subtractions lose all accuracy (or leave their domains) for nearly
+// collinear beads at FP32 rounding.
+template <typename T>
+__device__ __forceinline__ T bp_angle(const V3<T> &a, const V3<T> &b, T aa, T bb, T ab, T nab, T theta0, T k,
+                                      V3<T> &Fa, V3<T> &Fb) {
+  T d;
+  if constexpr (sizeof(T) == 8) {
+    d = m_acos(ab / nab) - theta0;
+    const T pre = (T)2 * k * d / m_sqrt(bb * aa - ab * ab);
+    Fa = pre * (a - (ab / bb) * b);
+    Fb = pre * (b - (ab / aa) * a);
+  } else {
+    const V3<T> c = cross(a, b);
+    const T s = m_sqrt(dot(c, c));
+    d = m_atan2(s, ab) - theta0;
+    const T g = (T)2 * k * d / fmaxf(s, 1e-30f);
+    Fa = (g / bb) * cross(b, c);
+    Fb = (g / aa) * cross(c, a);
+  }
+  return d;
+}
+
 // ---------------------------------------------------------------------
 // 6-bead base-pair term (src/force_bp_limit_triplet.F90:100-243): returns
 // 0 if |r - r_bp| > cutoff_ddist, else u and the unscaled forces.
@@ -325,38 +354,31 @@ __device__ __forceinline__ int bp_geom(const BpPara &bp, const V3<double> &v12d,
   T d1313 = dot(v13, v13), d4242 = dot(v42, v42), d1213 = dot(v12, v13), d1242 = dot(v12, v42);
   T d1215 = dot(v12, v15), d1515 = dot(v15, v15), d1262 = dot(v12, v62), d6262 = dot(v62, v62);
   T o13 = d1213 / d1212, o42 = d1242 / d1212, o15 = d1215 / d1212, o62 = d1262 / d1212;
-  T cosine, pre;
+  V3<T> fa, fc;
+  T pre;
   // angle 3-1=2
-  cosine = d1213 / (m_sqrt(d1313) * a12);
-  d = m_acos(cosine) - (T)bp.angl_theta2;
-  pre = (T)2 * (T)bp.angl_k2 * d / m_sqrt(d1313 * d1212 - d1213 * d1213);
+  d = bp_angle<T>(v12, v13, d1212, d1313, d1213, m_sqrt(d1313) * a12, (T)bp.angl_theta2, (T)bp.angl_k2, fa, fc);
   u = u + (T)bp.angl_k2 * (d * d);
-  fi = pre * (v12 - (d1213 / d1313) * v13);
-  fk = pre * (v13 - o13 * v12);
+  fi = fa;
+  fk = fc;
   fb[0] = fb[0] - fi - fk; fb[1] = fb[1] + fk; fb[2] = fi;
   // angle 1=2-4
-  cosine = d1242 / (a12 * m_sqrt(d4242));
-  d = m_acos(cosine) - (T)bp.angl_theta1;
+  d = bp_angle<T>(v12, v42, d1212, d4242, d1242, a12 * m_sqrt(d4242), (T)bp.angl_theta1, (T)bp.angl_k1, fa, fc);
   u = u + (T)bp.angl_k1 * (d * d);
-  pre = (T)2 * (T)bp.angl_k1 * d / m_sqrt(d1212 * d4242 - d1242 * d1242);
-  fi = -pre * (v42 - o42 * v12);
-  fk = -pre * (v12 - (d1242 / d4242) * v42);
+  fi = -fc;
+  fk = -fa;
   fb[0] = fb[0] + fi; fb[1] = fb[1] - fi - fk; fb[3] = fk;
   // angle 5-1=2
-  cosine = d1215 / (m_sqrt(d1515) * a12);
-  d = m_acos(cosine) - (T)bp.angl_theta4;
+  d = bp_angle<T>(v12, v15, d1212, d1515, d1215, m_sqrt(d1515) * a12, (T)bp.angl_theta4, (T)bp.angl_k4, fa, fc);
   u = u + (T)bp.angl_k4 * (d * d);
-  pre = (T)2 * (T)bp.angl_k4 * d / m_sqrt(d1515 * d1212 - d1215 * d1215);
-  fi = pre * (v12 - (d1215 / d1515) * v15);
-  fk = pre * (v15 - o15 * v12);
+  fi = fa;
+  fk = fc;
   fb[0] = fb[0] - fi - fk; fb[1] = fb[1] + fk; fb[4] = fi;
   // angle 1=2-6
-  cosine = d1262 / (a12 * m_sqrt(d6262));
-  d = m_acos(cosine) - (T)bp.angl_theta3;
+  d = bp_angle<T>(v12, v62, d1212, d6262, d1262, a12 * m_sqrt(d6262), (T)bp.angl_theta3, (T)bp.angl_k3, fa, fc);
   u = u + (T)bp.angl_k3 * (d * d);
-  pre = (T)2 * (T)bp.angl_k3 * d / m_sqrt(d1212 * d6262 - d1262 * d1262);
-  fi = -pre * (v62 - o62 * v12);
-  fk = -pre * (v12 - (d1262 / d6262) * v62);
+  fi = -fc;
+  fk = -fa;
   fb[0] = fb[0] + fi; fb[1] = fb[1] - fi - fk; fb[5] = fk;
   // dihedral 4-2=1=3
   V3<T> m = cross(v42, v12), n = cross(v12, v13), fl;

@@ -1282,10 +1282,7 @@ int sis_gpu_step(sis_gpu_ctx *c, int64_t nsteps, int64_t *rebuilds) {
   int64_t r0 = c->rebuilds;
   c->tev_used = 0;
   HIPCHK(hipEventRecord(c->ev0, c->stream));
-  if (!c->lists_valid) {
-    std::vector<int> all(c->R, 1);
-    rebuild(c, all);
-  }
+  // each form rebuilds invalid lists itself and forces MC after it
   int mode = c->mode;
   if (mode == 0) mode = choose_mode(c);
   if (mode == 2 && !c->parity) {
@@ -1347,3 +1344,5 @@ double sis_gpu_last_kernel_ms(sis_gpu_ctx *c, int32_t which) {
 }
 
 }  // extern "C"
+
+#include "sis_pool_host.h"

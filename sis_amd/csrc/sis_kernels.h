@@ -622,8 +622,8 @@ __global__ void k_seg_fill(Params P, Ptrs D, int nbp_total, int *seg_n, int *seg
 }
 // serial fallback for a component larger than one wave holds: the same
 // algorithm on global memory by one lane
-__device__ __noinline__ void mc_component_serial(const Params &P, const Ptrs &D, int r, int root, int *seg, int n,
-                                    double beta, uint64_t seed, int64_t step) {
+__device__ __noinline__ void mc_component_serial(const Params &P, const Ptrs &D, int r, int rk, int root, int *seg,
+                                                 int n, double beta, uint64_t seed, int64_t step) {
   const int N = P.N;
   int *cnt = D.nt_cnt + (size_t)r * N;
   for (int t = 0; t < n; t++) {
@@ -653,7 +653,7 @@ __device__ __noinline__ void mc_component_serial(const Params &P, const Ptrs &D,
       prev = nxt;
     }
     if (nact == 0) return;
-    int pick = (int)ceil(philox_u(seed, r, step, root, draw++) * nact), ntd = -1;
+    int pick = (int)ceil(philox_u(seed, rk, step, root, draw++) * nact), ntd = -1;
     prev = -1;
     for (int c = 0; c < pick; c++) {
       int nxt = 0x7fffffff;
@@ -674,13 +674,13 @@ __device__ __noinline__ void mc_component_serial(const Params &P, const Ptrs &D,
     }
     if (ns == 0) { D.flags[1] = 2; return; }
     for (int i = 0; i < ns; i++) {
-      int sw = (int)ceil(philox_u(seed, r, step, root, draw++) * ns) - 1;
+      int sw = (int)ceil(philox_u(seed, rk, step, root, draw++) * ns) - 1;
       int t = sq[i]; sq[i] = sq[sw]; sq[sw] = t;
     }
     int del = sq[0];
     for (int i = 1; i < ns; i++) {
       double ratio = exp((D.ene_bp[sq[i]] - D.ene_bp[del]) * beta);
-      if (philox_u(seed, r, step, root, draw++) < ratio) del = sq[i];
+      if (philox_u(seed, rk, step, root, draw++) < ratio) del = sq[i];
     }
     D.bp_status[del] = 0;
     const int *mp = D.bp_mp + (size_t)del * 8;
@@ -706,8 +706,8 @@ __global__ void __launch_bounds__(64) k_mc_wave(Params P, Ptrs D, int *seg_n, in
     return;
   }
   int *sg = seg + gr * SEGCAP;
-  if (!mc_component_wave(P, D, r, root, sg, n, beta, seed, step, sseq)) {
-    if (threadIdx.x == 0) mc_component_serial(P, D, r, root, sg, n, beta, seed, step);
+  if (!mc_component_wave(P, D, r, r, root, sg, n, beta, seed, step, sseq)) {
+    if (threadIdx.x == 0) mc_component_serial(P, D, r, r, root, sg, n, beta, seed, step);
   }
 }
 
@@ -996,7 +996,8 @@ __device__ __forceinline__ V3<double> bead_force_part(const Params &P, const Ptr
 }
 
 // GJF-2GJ update of one bead (src/job_md.F90:415-444); returns |move|^2
-__device__ __forceinline__ double gjf_update(const Params &P, const Ptrs &D, int r, int a,
+// r indexes the per-replica arrays; rk keys the random stream.
+__device__ __forceinline__ double gjf_update(const Params &P, const Ptrs &D, int r, int rk, int a,
                                              const V3<double> &F, int parity, uint64_t seed,
                                              int64_t step) {
   size_t g = (size_t)r * P.N + a;
@@ -1005,7 +1006,7 @@ __device__ __forceinline__ double gjf_update(const Params &P, const Ptrs &D, int
     const double *nz = D.noise + g * 3;
     n0 = nz[0]; n1 = nz[1]; n2 = nz[2];
   } else {
-    uint32_t k1 = (uint32_t)(seed >> 32) ^ (0x9e3779b9u * (uint32_t)(r + 1));
+    uint32_t k1 = (uint32_t)(seed >> 32) ^ (0x9e3779b9u * (uint32_t)(rk + 1));
     U4 o = philox(U4{(uint32_t)step, (uint32_t)(step >> 32), (uint32_t)a, 0x6e6f6973u}, (uint32_t)seed, k1);
     U4 o2 = philox(U4{(uint32_t)step, (uint32_t)(step >> 32), (uint32_t)a, 0x6e6f6974u}, (uint32_t)seed, k1);
     double u1 = u01(o.x, o.y), u2 = u01(o.z, o.w), u3 = u01(o2.x, o2.y), u4 = u01(o2.z, o2.w);
@@ -1072,7 +1073,7 @@ __global__ void __launch_bounds__(B) k_force_lanes(Params P, Ptrs D, int integra
     F.x += fb.x; F.y += fb.y; F.z += fb.z;
     D.nt_cnt[g] = -P.max_bp_per_nt;
     if (!integrate) D.frc[g] = double4{F.x, F.y, F.z, 0.0};
-    else d2 = gjf_update(P, D, r, a, F, parity, seed, step);
+    else d2 = gjf_update(P, D, r, r, a, F, parity, seed, step);
   }
   if (!integrate) return;
   if (l == 0) { s1[grp] = d2; s2[grp] = 0.0; }

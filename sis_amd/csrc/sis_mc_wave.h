@@ -15,8 +15,8 @@
 
 namespace sis {
 
-constexpr int MCW_ITEMS = 1;   // pairs per lane   -> 64 pairs per component
-constexpr int MCW_NODES = 2;   // nodes per lane   -> 128 nucleotides
+constexpr int MCW_ITEMS = 4;   // pairs per lane   -> 256 pairs per component
+constexpr int MCW_NODES = 8;   // nodes per lane   -> 512 nucleotides
 
 __device__ __forceinline__ int lane_id() { return __lane_id(); }
 
@@ -29,8 +29,10 @@ __device__ __forceinline__ double philox_u(uint64_t seed, int r, int64_t step, i
 // Resolve the component whose formed pairs are seg[0..n) (candidate ids,
 // any order).  Must be called by all 64 lanes of a wave with uniform
 // arguments.  Returns false if the component exceeds the wave capacity.
-__device__ __noinline__ bool mc_component_wave(const Params &P, const Ptrs &D, int r, int root, const int *seg,
-                                  int n, double beta, uint64_t seed, int64_t step, int *seq) {
+// r indexes the per-replica arrays; rk keys the random stream.
+__device__ __noinline__ bool mc_component_wave(const Params &P, const Ptrs &D, int r, int rk, int root,
+                                               const int *seg, int n, double beta, uint64_t seed,
+                                               int64_t step, int *seq) {
   const int L = lane_id();
   const int N = P.N;
   if (n > 64 * MCW_ITEMS) return false;
@@ -181,7 +183,7 @@ __device__ __noinline__ bool mc_component_wave(const Params &P, const Ptrs &D, i
       nact += __popcll(am[q]);
     }
     if (nact == 0) break;
-    double u = philox_u(seed, r, step, root, draw++);
+    double u = philox_u(seed, rk, step, root, draw++);
     int pick = (int)ceil(u * nact);   // 1..nact
     int ntd = -1;
     {
@@ -223,7 +225,7 @@ __device__ __noinline__ bool mc_component_wave(const Params &P, const Ptrs &D, i
     if (ns > 64) ns = 64;
     // reference shuffle: for i = 1..n swap(i, ceil(rnd*n))
     for (int i = 0; i < ns; i++) {
-      double v = philox_u(seed, r, step, root, draw++);
+      double v = philox_u(seed, rk, step, root, draw++);
       int sw = (int)ceil(v * ns) - 1;
       int t = seq[i];
       seq[i] = seq[sw];
@@ -245,7 +247,7 @@ __device__ __noinline__ bool mc_component_wave(const Params &P, const Ptrs &D, i
     for (int i = 1; i < ns; i++) {
       double ej = ene_of(seq[i]);
       double ratio = exp((ej - edel) * beta);
-      double v = philox_u(seed, r, step, root, draw++);
+      double v = philox_u(seed, rk, step, root, draw++);
       if (v < ratio) { del = seq[i]; edel = ej; }
     }
     // delete: pair slot del, decrement its two nucleotides

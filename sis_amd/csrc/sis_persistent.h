@@ -64,7 +64,7 @@ __device__ __forceinline__ int bp_eval_one(const Params &P, const Ptrs &D, int g
   V3<T> fb[6];
   if (!bp_geom<T>(bp, v12, v13, v15, v42, v62, u, fb)) return 0;
   double e = bp.U0 * dG * exp(-(double)u);
-  if (e > P.bp_cutoff_energy) return 0;
+  if (!(e <= P.bp_cutoff_energy)) return 0;   // also rejects a NaN energy
   D.bp_status[gb] = 1;
   D.ene_bp[gb] = e;
   double *fo = D.for_bp + (size_t)gb * 18;
@@ -147,8 +147,8 @@ __global__ void __launch_bounds__(B) k_persistent(Params P, Ptrs D, PersArgs A) 
             continue;
           }
           int *sg = A.seg + (size_t)root * SEGCAP;
-          if (!mc_component_wave(P, D, 0, root, sg, n, beta, A.seed, step, seq)) {
-            if ((threadIdx.x & 63) == 0) mc_component_serial(P, D, 0, root, sg, n, beta, A.seed, step);
+          if (!mc_component_wave(P, D, 0, 0, root, sg, n, beta, A.seed, step, seq)) {
+            if ((threadIdx.x & 63) == 0) mc_component_serial(P, D, 0, 0, root, sg, n, beta, A.seed, step);
           }
         }
       }
@@ -173,7 +173,7 @@ __global__ void __launch_bounds__(B) k_persistent(Params P, Ptrs D, PersArgs A) 
           D.comp[a] = a;
           A.seg_n[a] = 0;
           A.has_exc[a] = 0;
-          double d2 = gjf_update(P, D, 0, a, F, 0, A.seed, step);
+          double d2 = gjf_update(P, D, 0, 0, a, F, 0, A.seed, step);
           if (d2 > m1) { m2 = m1; m1 = d2; }
           else if (d2 > m2) m2 = d2;
         }
@@ -230,23 +230,39 @@ __global__ void __launch_bounds__(B) k_persistent(Params P, Ptrs D, PersArgs A) 
 namespace sis {
 
 // ---------------------------------------------------------------------
-// Batched step kernel (mode 3): one workgroup per replica of a small
-// single-chain system.  Each workgroup advances its replica through all
-// phases of job_md steps with workgroup barriers only, rebuilding its own
-// lists (all-pairs search) when its skin test fires.  Candidate ids are
-// r * CAPB + k, so the per-phase device functions index the same arrays.
+// Batched form (mode 3) and the job pool.  One workgroup advances one
+// system through whole job_md steps with workgroup barriers only,
+// rebuilding its own lists when its skin test fires.  batched_body works
+// on replica r of (P, D): k_batched runs replica r = blockIdx.x of one
+// context; k_jobs runs one job per workgroup with r = 0 and (P, D)
+// pointing at that job's own arrays.  Candidate ids are r * capb + k.
+
+constexpr int EXC_LDS = 1024;   // listed roots with excess per system and step
+
+// Per-system launch arguments.
+struct SysIO {
+  int todo;          // steps to run in this launch
+  int mc_first;      // limit MC on the first step (lists rebuilt by the host)
+  int capb;          // candidate slots of this system
+  int lpb;           // lanes per bead: 1, 2, 4 or 8
+  int rk;            // key of the system's random streams
+  int64_t stepno;    // istep before the launch
+  uint64_t seed;
+  int *seg_n, *seg, *has_exc;   // MC scratch (replica-r indexing)
+  double4 *pos_b;               // second position buffer (replica-r indexing)
+  int *done, *nbp, *rebuild, *nreb;  // this system's slots
+};
+
+// Launch arguments of k_batched (arrays over replicas).
 struct BatchArgs {
-  const int *todo;            // [R] steps to run this launch
-  const int64_t *stepno;      // [R] istep of each replica before the launch
+  const int *todo;
+  const int64_t *stepno;
+  const int *mc_first;
   uint64_t seed;
   int *seg_n, *seg, *has_exc;
-  int *done;          // [R] steps completed per replica
-  int *nbp;           // [R] candidates per replica
-  int *rebuild;       // [R] rebuild before the first step
-  const int *mc_first; // [R] limit MC on the first step (lists rebuilt by the host)
-  int *nreb;          // [R] in-kernel rebuilds done
+  int *done, *nbp, *rebuild, *nreb;
   double4 *pos_b;
-  int capb;
+  int capb, lpb;
 };
 
 // Base-pair candidate arrays of one layout.
@@ -292,16 +308,30 @@ __global__ void k_inc_remap(Params P, Ptrs D, const int *rep_off, int capb, int 
   }
 }
 
+__device__ __forceinline__ V3<double> group_sum_rt(V3<double> F, int lpb) {
+  for (int off = lpb >> 1; off > 0; off >>= 1) {
+    F.x += __shfl_xor(F.x, off, lpb);
+    F.y += __shfl_xor(F.y, off, lpb);
+    F.z += __shfl_xor(F.z, off, lpb);
+  }
+  return F;
+}
+
+// neighbor_list for one system inside its workgroup: pbc_wrap, all-pairs
+// search over LDS position tiles with the criteria of
+// src/neighbor_list.F90:123-223, candidates in owner-bead order, their
+// incidences sorted per bead.  Sets *s_flag when a capacity is exceeded.
 template <int B>
-__device__ void batched_rebuild(const Params &P, Ptrs &D, const BatchArgs &A, int r, int *s_off,
-                                int *s_flag) {
+__device__ void batched_rebuild(const Params &P, const Ptrs &D, const SysIO &J, int r, int *s_scan,
+                                double4 *s_tile, int *s_flag) {
   const int N = P.N;
   const int tid = threadIdx.x;
+  const size_t base = (size_t)r * N;
   // pbc_wrap (src/pbc.F90:196-226)
   if (P.pbc) {
+    const double *box = D.box + 3 * r;
     for (int a = tid; a < N; a += B) {
-      double4 &p = D.pos[(size_t)r * N + a];
-      const double *box = D.box + 3 * r;
+      double4 &p = D.pos[base + a];
       double v[3] = {p.x, p.y, p.z};
       for (int k = 0; k < 3; k++) {
         double L = box[k];
@@ -312,41 +342,65 @@ __device__ void batched_rebuild(const Params &P, Ptrs &D, const BatchArgs &A, in
     }
   }
   __syncthreads();
-  // all-pairs search with the criteria of src/neighbor_list.F90:123-223
-  double ec = D.ele_cut[r] + P.nl_margin;
-  double ele_cut2 = P.ele_on ? ec * ec : 0.0;
-  for (int a = tid; a < N; a += B) {
+  const double ec = D.ele_cut[r] + P.nl_margin;
+  const double ele_cut2 = P.ele_on ? ec * ec : 0.0;
+  for (int a0 = 0; a0 < N; a0 += B) {
+    const int a = a0 + tid;
+    const bool act = a < N;
     SearchOut o{0, 0, 0};
-    double4 pa = D.pos[(size_t)r * N + a];
-    for (int b = 0; b < N; b++) {
-      if (b == a) continue;
-      pair_test(P, D, r, a, b, pa, D.pos[(size_t)r * N + b], ele_cut2, o);
+    const double4 pa = act ? D.pos[base + a] : double4{0.0, 0.0, 0.0, 0.0};
+    for (int t0 = 0; t0 < N; t0 += B) {
+      __syncthreads();
+      if (t0 + tid < N) s_tile[tid] = D.pos[base + t0 + tid];
+      __syncthreads();
+      if (act) {
+        const int nt = min(B, N - t0);
+        for (int k = 0; k < nt; k++) {
+          const int b = t0 + k;
+          if (b != a) pair_test(P, D, r, a, b, pa, s_tile[k], ele_cut2, o);
+        }
+      }
     }
-    if (o.ne > D.maxnb || o.nw > D.maxw || o.nb > D.maxbpb) *s_flag = 1;
-    search_finish(P, D, r, a, o);
-    D.n_inc[(size_t)r * N + a] = 0;
+    if (act) {
+      if (o.ne > D.maxnb || o.nw > D.maxw || o.nb > D.maxbpb) *s_flag = 1;
+      search_finish(P, D, r, a, o);
+      D.n_inc[base + a] = 0;
+    }
   }
   __syncthreads();
-  // candidate offsets (bead order is the reference order for one chain)
+  // candidate offsets: exclusive scan of the per-bead counts
+  int running = 0;
+  for (int a0 = 0; a0 < N; a0 += B) {
+    const int a = a0 + tid;
+    const int v = a < N ? D.n_bp_tmp[base + a] : 0;
+    s_scan[tid] = v;
+    __syncthreads();
+    for (int off = 1; off < B; off <<= 1) {
+      const int t = tid >= off ? s_scan[tid - off] : 0;
+      __syncthreads();
+      s_scan[tid] += t;
+      __syncthreads();
+    }
+    if (a < N) D.bp_off[base + a] = running + s_scan[tid] - v;
+    running += s_scan[B - 1];
+    __syncthreads();
+  }
+  const int nbr = min(running, J.capb);
   if (tid == 0) {
-    int acc = 0;
-    for (int a = 0; a < N; a++) { s_off[a] = acc; acc += D.n_bp_tmp[(size_t)r * N + a]; }
-    s_off[N] = acc;
-    if (acc > A.capb) *s_flag = 1;
-    A.nbp[r] = acc < A.capb ? acc : A.capb;
+    if (running > J.capb) *s_flag = 1;
+    *J.nbp = nbr;
   }
-  __syncthreads();
   for (int a = tid; a < N; a += B) {
-    size_t g = (size_t)r * N + a;
-    Topo ti = D.topo[a];
-    int nb = D.n_bp_tmp[g];
+    const size_t g = base + a;
+    const Topo ti = D.topo[a];
+    const int nb = D.n_bp_tmp[g], off = D.bp_off[g];
     for (int k = 0; k < nb; k++) {
-      int lk = s_off[a] + k;
-      if (lk >= A.capb) break;
-      int gb = r * A.capb + lk;
-      int b = D.bp_tmp[g * D.maxbpb + k];
-      Topo tj = D.topo[b];
-      int i = ti.loc, j = tj.loc, imp = a + 1, jmp = b + 1;
+      const int lk = off + k;
+      if (lk >= J.capb) break;
+      const int gb = r * J.capb + lk;
+      const int b = D.bp_tmp[g * D.maxbpb + k];
+      const Topo tj = D.topo[b];
+      const int i = ti.loc, j = tj.loc, imp = a + 1, jmp = b + 1;
       int *dst = D.bp_mp + (size_t)gb * 8;
       dst[0] = imp;
       dst[1] = jmp;
@@ -355,7 +409,7 @@ __device__ void batched_rebuild(const Params &P, Ptrs &D, const BatchArgs &A, in
       dst[4] = (i == ti.len) ? imp + 1 - ti.len : imp + 1;
       dst[5] = (j == tj.len) ? jmp + 1 - tj.len : jmp + 1;
       int bt;
-      int h = bp_eligible(P, D, a, b, &bt);
+      const int h = bp_eligible(P, D, a, b, &bt);
       dst[6] = bt;
       dst[7] = r;
       D.bp_coef[2 * (size_t)gb] = D.bp3_dH[h];
@@ -363,81 +417,86 @@ __device__ void batched_rebuild(const Params &P, Ptrs &D, const BatchArgs &A, in
     }
   }
   __syncthreads();
-  int nbr = A.nbp[r];
   for (int k = tid; k < nbr; k += B) {
-    int gb = r * A.capb + k;
+    const int gb = r * J.capb + k;
     const int *mp = D.bp_mp + (size_t)gb * 8;
     for (int q = 0; q < 6; q++) {
-      size_t gq = (size_t)r * N + (mp[q] - 1);
-      int slot = atomicAdd(&D.n_inc[gq], 1);
+      const size_t gq = base + (mp[q] - 1);
+      const int slot = atomicAdd(&D.n_inc[gq], 1);
       if (slot < D.maxinc) D.inc[((size_t)r * D.maxinc + slot) * N + (mp[q] - 1)] = gb * 8 + q;
       else *s_flag = 1;
     }
   }
   __syncthreads();
   for (int a = tid; a < N; a += B) {
-    size_t g = (size_t)r * N + a;
-    int n = min(D.n_inc[g], D.maxinc);
-    size_t base = (size_t)r * D.maxinc * N + a, st = N;
+    const size_t g = base + a;
+    const int n = min(D.n_inc[g], D.maxinc);
+    const size_t ib = (size_t)r * D.maxinc * N + a, st = N;
     for (int x = 1; x < n; x++) {
-      int v = D.inc[base + x * st], y = x - 1;
-      while (y >= 0 && D.inc[base + y * st] > v) { D.inc[base + (y + 1) * st] = D.inc[base + y * st]; y--; }
-      D.inc[base + (y + 1) * st] = v;
+      const int v = D.inc[ib + x * st];
+      int y = x - 1;
+      while (y >= 0 && D.inc[ib + y * st] > v) { D.inc[ib + (y + 1) * st] = D.inc[ib + y * st]; y--; }
+      D.inc[ib + (y + 1) * st] = v;
     }
     D.move[g] = double4{0.0, 0.0, 0.0, 0.0};
     D.nt_cnt[g] = -P.max_bp_per_nt;
     D.comp[g] = a;
-    A.seg_n[g] = 0;
-    A.has_exc[g] = 0;
+    J.seg_n[g] = 0;
+    J.has_exc[g] = 0;
   }
   __syncthreads();
 }
 
-template <typename T, int LPB, int B>
-__global__ void __launch_bounds__(B) k_batched(Params P, Ptrs D, BatchArgs A) {
-  __shared__ double s1[B / LPB], s2[B / LPB];
+// Whole job_md steps of system r (src/job_md.F90:374-460) for J.todo steps.
+template <typename T, int B>
+__device__ void batched_body(const Params &P, Ptrs &D, const SysIO &J, const int r) {
+  __shared__ double s1[B], s2[B];
   __shared__ int sseq[B];
-  __shared__ int s_off[1025];
+  __shared__ int s_scan[B];
+  __shared__ double4 s_tile[B];
   __shared__ int s_flag, s_exit;
-  const int r = blockIdx.x;
+  __shared__ int s_exc[EXC_LDS];   // component roots with excess (MC steps)
+  __shared__ int s_nexc;
   const int N = P.N;
   const int tid = threadIdx.x;
+  const size_t base = (size_t)r * N;
+  const int lpb = J.lpb, grp = tid / lpb, l = tid % lpb, ngrp = B / lpb;
   const double BOLTZ = 1.380649e-23 * ((1.0 / 4184.0) * 6.02214076e23);
   const double beta = 1.0 / (BOLTZ * D.tK[r]);
-  double4 *pos_a = D.pos, *pos_b = A.pos_b;
-  int *comp = D.comp + (size_t)r * N;
+  double4 *pos_a = D.pos, *pos_b = J.pos_b;
+  int *comp = D.comp + base;
   double4 *const pos_first = D.pos;
   if (tid == 0) s_flag = 0;
   // per-step limit-MC state starts reset whatever ran before this launch
   for (int a = tid; a < N; a += B) {
-    const size_t g = (size_t)r * N + a;
-    D.nt_cnt[g] = -P.max_bp_per_nt;
+    D.nt_cnt[base + a] = -P.max_bp_per_nt;
     comp[a] = a;
-    A.seg_n[g] = 0;
-    A.has_exc[g] = 0;
+    J.seg_n[base + a] = 0;
+    J.has_exc[base + a] = 0;
   }
   __syncthreads();
   // the first step after a rebuild does the limit MC (src/job_md.F90:388-395)
-  int mc_next = A.mc_first[r];
+  int mc_next = J.mc_first;
   int nreb = 0;
-  if (A.todo[r] > 0 && A.rebuild[r]) {
-    batched_rebuild<B>(P, D, A, r, s_off, &s_flag);
-    if (tid == 0) A.rebuild[r] = 0;
+  if (J.todo > 0 && *J.rebuild) {
+    batched_rebuild<B>(P, D, J, r, s_scan, s_tile, &s_flag);
+    if (tid == 0) *J.rebuild = 0;
     mc_next = 1;
     nreb++;
   }
   int64_t completed = 0;
   int overflow = s_flag;
-  const int64_t nsteps = A.todo[r], step0 = A.stepno[r];
+  const int64_t nsteps = J.todo, step0 = J.stepno;
   for (int64_t s = 0; s < nsteps && !overflow; s++) {
     const int64_t step = step0 + s + 1;
     const int mc = P.nstep_bp_mc > 0 && ((step % P.nstep_bp_mc) == 0 || mc_next);
     mc_next = 0;
     D.pos = pos_a;
     D.pos_out = pos_b;
-    const int nbr = A.nbp[r];
+    const int nbr = *J.nbp;
+    // ---- A: base pairs (+ union of formed pairs on MC steps), bonded forces
     for (int k = tid; k < nbr; k += B) {
-      int gb = r * A.capb + k;
+      const int gb = r * J.capb + k;
       if (bp_eval_one<T>(P, D, gb, mc) && mc) {
         const int *mp = D.bp_mp + (size_t)gb * 8;
         int a = mp[0] - 1, b = mp[1] - 1;
@@ -450,72 +509,78 @@ __global__ void __launch_bounds__(B) k_batched(Params P, Ptrs D, BatchArgs A) {
         }
       }
     }
-    {
-      const int grp = tid / LPB, l = tid % LPB, ngrp = B / LPB;
-      for (int a0 = 0; a0 < N; a0 += ngrp) {
-        int a = a0 + grp;
-        V3<double> F{0.0, 0.0, 0.0};
-        if (a < N) F = bonded_part(P, D, r, a, l, LPB);
-        F = group_sum<LPB>(F);
-        if (a < N && l == 0) D.fbond[(size_t)r * N + a] = double4{F.x, F.y, F.z, 0.0};
-      }
+    for (int a0 = 0; a0 < N; a0 += ngrp) {
+      const int a = a0 + grp;
+      V3<double> F{0.0, 0.0, 0.0};
+      if (a < N) F = bonded_part(P, D, r, a, l, lpb);
+      F = group_sum_rt(F, lpb);
+      if (a < N && l == 0) D.fbond[base + a] = double4{F.x, F.y, F.z, 0.0};
     }
+    if (tid == 0) s_nexc = 0;
     __syncthreads();
     if (mc) {
+      // ---- B: formed pairs bucketed by component root; roots with
+      // excess listed once
       for (int k = tid; k < nbr; k += B) {
-        int gb = r * A.capb + k;
+        const int gb = r * J.capb + k;
         if (!D.bp_status[gb]) continue;
         const int *mp = D.bp_mp + (size_t)gb * 8;
-        int a = mp[0] - 1, b = mp[1] - 1;
-        int root = uf_find(comp, a);
-        size_t gr = (size_t)r * N + root;
-        int slot = atomicAdd(&A.seg_n[gr], 1);
-        if (slot < SEGCAP) A.seg[gr * SEGCAP + slot] = gb;
-        if (D.nt_cnt[(size_t)r * N + a] > 0 || D.nt_cnt[(size_t)r * N + b] > 0) A.has_exc[gr] = 1;
-      }
-      __syncthreads();
-      const int wave = tid / 64, nwaves = B / 64;
-      int *seq = sseq + wave * 64;
-      for (int root = wave; root < N; root += nwaves) {
-        size_t gr = (size_t)r * N + root;
-        if (!A.has_exc[gr]) continue;
-        int n = A.seg_n[gr];
-        if (n > SEGCAP) { if ((tid & 63) == 0) D.flags[5] = 1; continue; }
-        int *sg = A.seg + gr * SEGCAP;
-        if (!mc_component_wave(P, D, r, root, sg, n, beta, A.seed, step, seq))
-          if ((tid & 63) == 0) mc_component_serial(P, D, r, root, sg, n, beta, A.seed, step);
-      }
-      __syncthreads();
-    }
-    if (mc)
-      for (int k = tid; k < nbr; k += B) D.bp_status_mc[r * A.capb + k] = D.bp_status[r * A.capb + k];
-    double m1 = 0.0, m2 = 0.0;
-    {
-      const int grp = tid / LPB, l = tid % LPB, ngrp = B / LPB;
-      for (int a0 = 0; a0 < N; a0 += ngrp) {
-        int a = a0 + grp;
-        V3<double> F{0.0, 0.0, 0.0};
-        if (a < N) F = pair_part<T>(P, D, r, a, l, LPB);
-        F = group_sum<LPB>(F);
-        if (a < N && l == 0) {
-          size_t g = (size_t)r * N + a;
-          const double4 fb = D.fbond[g];
-          F.x += fb.x; F.y += fb.y; F.z += fb.z;
-          D.nt_cnt[g] = -P.max_bp_per_nt;
-          comp[a] = a;
-          A.seg_n[g] = 0;
-          A.has_exc[g] = 0;
-          double d2 = gjf_update(P, D, r, a, F, 0, A.seed, step);
-          if (d2 > m1) { m2 = m1; m1 = d2; }
-          else if (d2 > m2) m2 = d2;
+        const int a = mp[0] - 1, b = mp[1] - 1;
+        const int root = uf_find(comp, a);
+        const size_t gr = base + root;
+        const int slot = atomicAdd(&J.seg_n[gr], 1);
+        if (slot < SEGCAP) J.seg[gr * SEGCAP + slot] = gb;
+        if ((D.nt_cnt[base + a] > 0 || D.nt_cnt[base + b] > 0) && atomicExch(&J.has_exc[gr], 1) == 0) {
+          const int i = atomicAdd(&s_nexc, 1);
+          if (i < EXC_LDS) s_exc[i] = root;
         }
       }
-      if (l == 0) { s1[grp] = m1; s2[grp] = m2; }
+      __syncthreads();
+      // ---- C: one wavefront per component with excess (components are
+      // disjoint and keyed by their root, so the order does not matter)
+      const int wave = tid / 64, nwaves = B / 64;
+      int *seq = sseq + wave * 64;
+      const int nexc = s_nexc;
+      const bool listed = nexc <= EXC_LDS;
+      for (int it = wave; it < (listed ? nexc : N); it += nwaves) {
+        const int root = listed ? s_exc[it] : it;
+        const size_t gr = base + root;
+        if (!J.has_exc[gr]) continue;
+        const int n = J.seg_n[gr];
+        if (n > SEGCAP) { if ((tid & 63) == 0) D.flags[5] = 1; continue; }
+        int *sg = J.seg + gr * SEGCAP;
+        if (!mc_component_wave(P, D, r, J.rk, root, sg, n, beta, J.seed, step, seq))
+          if ((tid & 63) == 0) mc_component_serial(P, D, r, J.rk, root, sg, n, beta, J.seed, step);
+      }
+      __syncthreads();
+      // the MC result is what later non-MC steps evaluate
+      for (int k = tid; k < nbr; k += B) D.bp_status_mc[r * J.capb + k] = D.bp_status[r * J.capb + k];
     }
+    // ---- D: pair forces + bonded, GJF-2GJ, per-step MC state reset
+    double m1 = 0.0, m2 = 0.0;
+    for (int a0 = 0; a0 < N; a0 += ngrp) {
+      const int a = a0 + grp;
+      V3<double> F{0.0, 0.0, 0.0};
+      if (a < N) F = pair_part<T>(P, D, r, a, l, lpb);
+      F = group_sum_rt(F, lpb);
+      if (a < N && l == 0) {
+        const size_t g = base + a;
+        const double4 fb = D.fbond[g];
+        F.x += fb.x; F.y += fb.y; F.z += fb.z;
+        D.nt_cnt[g] = -P.max_bp_per_nt;
+        comp[a] = a;
+        J.seg_n[g] = 0;
+        J.has_exc[g] = 0;
+        const double d2 = gjf_update(P, D, r, J.rk, a, F, 0, J.seed, step);
+        if (d2 > m1) { m2 = m1; m1 = d2; }
+        else if (d2 > m2) m2 = d2;
+      }
+    }
+    if (l == 0) { s1[grp] = m1; s2[grp] = m2; }
     __syncthreads();
-    for (int off = (B / LPB) / 2; off > 0; off >>= 1) {
+    for (int off = ngrp / 2; off > 0; off >>= 1) {
       if (tid < off) {
-        double a1 = s1[tid], a2 = s2[tid], b1 = s1[tid + off], b2 = s2[tid + off];
+        const double a1 = s1[tid], a2 = s2[tid], b1 = s1[tid + off], b2 = s2[tid + off];
         s1[tid] = a1 > b1 ? a1 : b1;
         s2[tid] = a1 > b1 ? (a2 > b1 ? a2 : b1) : (a1 > b2 ? a1 : b2);
       }
@@ -528,23 +593,109 @@ __global__ void __launch_bounds__(B) k_batched(Params P, Ptrs D, BatchArgs A) {
     if (s_exit) {
       if (s + 1 < nsteps) {
         D.pos = pos_a;
-        batched_rebuild<B>(P, D, A, r, s_off, &s_flag);
+        batched_rebuild<B>(P, D, J, r, s_scan, s_tile, &s_flag);
         overflow = s_flag;
         mc_next = 1;
         nreb++;
       } else if (tid == 0) {
-        A.rebuild[r] = 1;   // rebuild at the start of the next launch
+        *J.rebuild = 1;   // rebuild at the start of the next launch
       }
     }
   }
-  // the host keeps one position buffer for this form: copy back if needed
+  // callers keep one position buffer for this form: copy back if needed
   if (pos_a != pos_first)
-    for (int a = tid; a < N; a += B) pos_first[(size_t)r * N + a] = pos_a[(size_t)r * N + a];
+    for (int a = tid; a < N; a += B) pos_first[base + a] = pos_a[base + a];
   if (tid == 0) {
-    A.done[r] = (int)completed;
-    A.nreb[r] = nreb;
+    *J.done = (int)completed;
+    *J.nreb = nreb;
     if (overflow) D.flags[0] = 1;
   }
+}
+
+template <typename T, int B>
+__global__ void __launch_bounds__(B) k_batched(Params P, Ptrs D, BatchArgs A) {
+  const int r = blockIdx.x;
+  SysIO J;
+  J.todo = A.todo[r];
+  J.mc_first = A.mc_first[r];
+  J.capb = A.capb;
+  J.lpb = A.lpb;
+  J.rk = r;
+  J.stepno = A.stepno[r];
+  J.seed = A.seed;
+  J.seg_n = A.seg_n;
+  J.seg = A.seg;
+  J.has_exc = A.has_exc;
+  J.pos_b = A.pos_b;
+  J.done = A.done + r;
+  J.nbp = A.nbp + r;
+  J.rebuild = A.rebuild + r;
+  J.nreb = A.nreb + r;
+  Ptrs Dl = D;
+  batched_body<T, B>(P, Dl, J, r);
+}
+
+// One serving job: its sizes and the device arrays of its own context.
+struct JobView {
+  int N, nchains, maxlen, capb, lpb, rk, todo, mc_first;
+  int maxnb, maxw, maxinc, maxbpb;
+  int64_t stepno;
+  uint64_t seed;
+  const Topo *topo;
+  const int *chain_id, *bp3_ctx;
+  const double *md_a, *md_cf, *md_cx;
+  double *md_cr, *tK, *ele_coef, *lambdaD, *ele_cut, *box;
+  double4 *pos, *pos_b, *vel, *acc, *move, *frc, *fbond;
+  int *nb_ele, *n_ele, *nb_wca, *n_wca, *bp_tmp, *n_bp_tmp, *bp_off;
+  int *bp_mp;
+  double *bp_coef;
+  int *inc, *n_inc;
+  uint8_t *bp_status, *bp_status_mc;
+  double *ene_bp, *for_bp;
+  int *nt_cnt, *comp, *flags;
+  int *seg_n, *seg, *has_exc;
+  int *io;   // [0] steps done, [1] candidates, [2] rebuild pending, [3] rebuilds
+};
+
+// Job pool: workgroup j advances job j.  P carries the force field shared
+// by the jobs (sizes are the job's own); D0 carries the shared tables.
+template <typename T, int B>
+__global__ void __launch_bounds__(B) k_jobs(Params P, Ptrs D0, const JobView *__restrict__ jobs) {
+  const JobView &V = jobs[blockIdx.x];
+  Params Pj = P;
+  Pj.N = V.N;
+  Pj.R = 1;
+  Pj.nchains = V.nchains;
+  Pj.maxlen = V.maxlen;
+  Ptrs D = D0;
+  D.topo = V.topo; D.chain_id = V.chain_id; D.bp3_ctx = V.bp3_ctx;
+  D.md_a = V.md_a; D.md_cf = V.md_cf; D.md_cx = V.md_cx; D.md_cr = V.md_cr;
+  D.tK = V.tK; D.ele_coef = V.ele_coef; D.lambdaD = V.lambdaD; D.ele_cut = V.ele_cut; D.box = V.box;
+  D.pos = V.pos; D.vel = V.vel; D.acc = V.acc; D.move = V.move; D.frc = V.frc; D.fbond = V.fbond;
+  D.pos_out = V.pos_b;
+  D.maxnb = V.maxnb; D.maxw = V.maxw; D.maxinc = V.maxinc; D.maxbpb = V.maxbpb;
+  D.nb_ele = V.nb_ele; D.n_ele = V.n_ele; D.nb_wca = V.nb_wca; D.n_wca = V.n_wca;
+  D.bp_tmp = V.bp_tmp; D.n_bp_tmp = V.n_bp_tmp; D.bp_off = V.bp_off;
+  D.bp_mp = V.bp_mp; D.bp_coef = V.bp_coef; D.inc = V.inc; D.n_inc = V.n_inc;
+  D.bp_status = V.bp_status; D.bp_status_mc = V.bp_status_mc; D.ene_bp = V.ene_bp; D.for_bp = V.for_bp;
+  D.nt_cnt = V.nt_cnt; D.comp = V.comp; D.flags = V.flags;
+  SysIO J;
+  J.todo = V.todo;
+  J.mc_first = V.mc_first;
+  J.capb = V.capb;
+  J.lpb = V.lpb;
+  J.rk = V.rk;
+  J.stepno = V.stepno;
+  J.seed = V.seed;
+  J.seg_n = V.seg_n;
+  J.seg = V.seg;
+  J.has_exc = V.has_exc;
+  J.pos_b = V.pos_b;
+  J.done = V.io;
+  J.nbp = V.io + 1;
+  J.rebuild = V.io + 2;
+  J.nreb = V.io + 3;
+  batched_body<T, B>(Pj, D, J, 0);
 }
 
 }  // namespace sis

@@ -2,6 +2,9 @@
 // Included by sis_gpu.hip after the per-phase driver.
 #pragma once
 
+// lanes per bead of the one-workgroup forms: fill 256 threads
+int lanes_per_bead(int N) { return N <= 32 ? 8 : (N <= 64 ? 4 : (N <= 128 ? 2 : 1)); }
+
 bool batched_ok(sis_gpu_ctx *c) {
   return !c->parity && c->R >= 2 && c->P.nchains == 1 && c->N <= 1024;
 }
@@ -221,14 +224,8 @@ void batched_steps(sis_gpu_ctx *c, int64_t nsteps) {
     c->bb_done_d.alloc(R); c->bb_nbp_d.alloc(R); c->bb_rebuild_d.alloc(R); c->bb_todo_d.alloc(R);
     c->bb_stepno_d.alloc(R); c->bb_off_d.alloc(R + 1); c->bb_mcfirst_d.alloc(R); c->bb_nreb_d.alloc(R);
   }
-  const int lpb = N <= 32 ? 8 : (N <= 64 ? 4 : (N <= 128 ? 2 : 1));
-  const void *kfn;
-  if (c->precision == 1)
-    kfn = lpb == 8 ? (const void *)k_batched<float, 8, B> : lpb == 4 ? (const void *)k_batched<float, 4, B>
-        : lpb == 2 ? (const void *)k_batched<float, 2, B> : (const void *)k_batched<float, 1, B>;
-  else
-    kfn = lpb == 8 ? (const void *)k_batched<double, 8, B> : lpb == 4 ? (const void *)k_batched<double, 4, B>
-        : lpb == 2 ? (const void *)k_batched<double, 2, B> : (const void *)k_batched<double, 1, B>;
+  const int lpb = lanes_per_bead(N);
+  const void *kfn = c->precision == 1 ? (const void *)k_batched<float, B> : (const void *)k_batched<double, B>;
   c->block = B;
   int64_t left = nsteps;
   const int ex = c->opt.exchange_every;
@@ -257,6 +254,7 @@ void batched_steps(sis_gpu_ctx *c, int64_t nsteps) {
       A.nreb = c->bb_nreb_d.p;
       A.pos_b = c->pos2_d.p;
       A.capb = c->batch_capb;
+      A.lpb = lpb;
       Params P = c->P;
       Ptrs D = c->ptrs();
       D.bp_mp = c->bb_mp_d.p;

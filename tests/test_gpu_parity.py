@@ -304,3 +304,31 @@ def test_fortran_host_matches_c_abi(tmp_path):
     assert np.allclose(e_ft, e_py, rtol=1e-12, atol=1e-12)
     assert nreb_ft == nreb_py
     assert np.abs(x_ft - x_py).max() < 1e-9
+
+
+@pytest.mark.parametrize("delta", [1e-6, 1e-5, 1e-4, 1e-3])
+def test_fp32_nearly_collinear_base_pair(delta):
+    """A G-C candidate whose angle (i-1)-i-j is pi - delta: FP32 pair
+    arithmetic must stay finite and within the FP32 force tolerance of
+    FP64 (acos and the reference's radicand leave their domains at FP32
+    rounding here).  Limit MC off, so the candidate contributes."""
+    zig = lambda k: np.array([k * 5.9 * np.sin(np.pi / 3), (k % 2) * 5.9 * np.cos(np.pi / 3), 0.0])
+    A = np.array([zig(k) for k in range(6)])
+    u = (A[3] - A[2]) / np.linalg.norm(A[3] - A[2])
+    c, s_ = np.cos(delta), np.sin(delta)
+    w = np.array([c * u[0] - s_ * u[1], s_ * u[0] + c * u[1], 0.0])   # u turned by delta in the plane
+    j = A[3] + 13.7 * w
+    ez, ep = np.array([0.0, 0.0, 1.0]), np.cross(w, [0.0, 0.0, 1.0])
+    Bc = np.array([j + (k - 3) * 5.9 * np.sin(np.pi / 3) * ez + ((k % 2) - 1) * 2.95 * ep for k in range(6)])
+    p = configs.default_params(tempK=300.0, nl_margin=5.0, nstep_bp_mc=0)
+    s = configs._mk("collinear", ["GGGGGG", "CCCCCC"], np.vstack([A, Bc]), p)
+    print(f"[case] fp32 collinear delta={delta}", flush=True)
+    f = []
+    for prec in (0, 1):
+        e = Engine(s, precision=prec)
+        e.neighbor_list(0)
+        assert e.info()["nbp"] >= 1
+        f.append(e.force(1, 0))
+    assert np.all(np.isfinite(f[0])) and np.all(np.isfinite(f[1]))
+    rel = np.linalg.norm(f[1] - f[0], axis=1) / np.maximum(np.linalg.norm(f[0], axis=1), 1e-3)
+    assert rel.max() < 1e-5
