@@ -40,7 +40,6 @@ struct Params {
   int angl_reb, dih_type, ele_on, ele_excl_bond, ele_excl_angl;
   int bp_model, bp_min_loop, max_bp_per_nt, nstep_bp_mc, pbc;
   int N, R, fix_h4, nchains, maxlen, pad;
-  BpPara bpp[7];  // 1..6 = GC AU GU CG UA UG
 };
 
 // Per-bead topology word: chain start (replica-local 0-based), chain
@@ -59,6 +58,8 @@ struct Ptrs {
   const int *bp3_ctx;        // [N] packed (seq[i-1],seq[i],seq[i+1]) context for eligibility
   const int *bp3_hash;       // [4097] triplet hash -> table index (0 = none)
   const double *bp3_dH, *bp3_dS;  // [469]
+  const BpPara *bpp;         // [7] base-pair types 1..6 = GC AU GU CG UA UG (device memory:
+                             // indexed per candidate, so not part of the kernel arguments)
   const double *md_a, *md_cf, *md_cx, *mass;  // [N]
   double *md_cr;             // [R*N]
   double *tK, *ele_coef, *lambdaD, *ele_cut, *box;  // [R], box [R*3]

@@ -121,8 +121,10 @@ struct sis_gpu_ctx {
   std::vector<uint32_t> rep_tape;
   int64_t rep_tape_pos = 0;
   std::vector<int64_t> tape_off_h;
+  BpPara bpp_h[7] = {};   // base-pair types 1..6 (GC AU GU CG UA UG)
   // device buffers
   DevBuf<Topo> topo_d;
+  DevBuf<BpPara> bpp_d;
   DevBuf<int> bp3_hash_d;
   DevBuf<double> bp3_dH_d, bp3_dS_d, md_a_d, md_cf_d, md_cx_d, mass_d, md_cr_d;
   DevBuf<double> tK_d, ele_coef_d, lambdaD_d, ele_cut_d, box_d;
@@ -167,6 +169,7 @@ struct sis_gpu_ctx {
     d.md_cr = md_cr_d.p;
     d.tK = tK_d.p; d.ele_coef = ele_coef_d.p; d.lambdaD = lambdaD_d.p; d.ele_cut = ele_cut_d.p;
     d.box = box_d.p;
+    d.bpp = bpp_d.p;
     d.pos = pos_d.p; d.pos_out = pos2_d.p; d.fbond = fbond_d.p; d.vel = vel_d.p; d.acc = acc_d.p; d.move = move_d.p; d.frc = frc_d.p;
     d.maxnb = D.maxnb; d.maxw = D.maxw; d.maxbp = D.maxbp; d.maxinc = D.maxinc; d.maxbpb = D.maxbpb;
     d.nb_ele = nb_ele_d.p; d.n_ele = n_ele_d.p; d.nb_wca = nb_wca_d.p; d.n_wca = n_wca_d.p;
@@ -201,6 +204,7 @@ struct sis_gpu_ctx {
                             &bm_cache_d, &ene_part_d, &ene_out_d, &ecT_d, &eCoef_d, &eLam_d,
                             &bp_coef2_d};
     for (auto *b : db) b->free_();
+    bpp_d.free_();
     DevBuf<double4> *vb[] = {&pos_d, &pos2_d, &vel_d, &acc_d, &move_d, &frc_d, &fbond_d};
     for (auto *b : vb) b->free_();
     topo_d.free_();
@@ -867,19 +871,19 @@ int sis_gpu_create(const sis_topo *topo, const sis_params *prm, const sis_option
       const sis_bp_para &b = prm->bp[t];
       BpPara m;
       std::memcpy(&m, &b, sizeof m);
-      std::memcpy(&P.bpp[t + 1], &b, sizeof m);
+      std::memcpy(&c->bpp_h[t + 1], &b, sizeof m);
       BpPara mr = m;
       mr.angl_k1 = m.angl_k2; mr.angl_k2 = m.angl_k1; mr.angl_k3 = m.angl_k4; mr.angl_k4 = m.angl_k3;
-      P.bpp[t + 4] = mr;
+      c->bpp_h[t + 4] = mr;
     }
     if (std::fabs(prm->bp_cutoff_energy) <= 2.220446049250313e-16) {
       P.bp_cutoff_dist = 18.0;
-      for (int t = 1; t <= 6; t++) P.bpp[t].cutoff_ddist = P.bp_cutoff_dist - 13.8;
+      for (int t = 1; t <= 6; t++) c->bpp_h[t].cutoff_ddist = P.bp_cutoff_dist - 13.8;
     } else {
       P.bp_cutoff_dist = 0.0;
       for (int t = 1; t <= 6; t++) {
-        P.bpp[t].cutoff_ddist = std::sqrt(std::log(std::fabs(10.0 / prm->bp_cutoff_energy)) / P.bpp[t].bond_k);
-        P.bp_cutoff_dist = std::max(P.bp_cutoff_dist, P.bpp[t].bond_r + P.bpp[t].cutoff_ddist);
+        c->bpp_h[t].cutoff_ddist = std::sqrt(std::log(std::fabs(10.0 / prm->bp_cutoff_energy)) / c->bpp_h[t].bond_k);
+        P.bp_cutoff_dist = std::max(P.bp_cutoff_dist, c->bpp_h[t].bond_r + c->bpp_h[t].cutoff_ddist);
       }
     }
     P.wca_nl_cut2 = (P.wca_sigma + P.nl_margin) * (P.wca_sigma + P.nl_margin);
@@ -930,6 +934,8 @@ int sis_gpu_create(const sis_topo *topo, const sis_params *prm, const sis_option
     c->md_cr_d.alloc(RN);
     c->tK_d.alloc(R); c->ele_coef_d.alloc(R); c->lambdaD_d.alloc(R); c->ele_cut_d.alloc(R);
     c->box_d.alloc(3 * R);
+    c->bpp_d.alloc(7);
+    HIPCHK(hipMemcpy(c->bpp_d.p, c->bpp_h, sizeof c->bpp_h, hipMemcpyHostToDevice));
     c->ecT_d.alloc(R); c->eCoef_d.alloc(R); c->eLam_d.alloc(R);
     c->pos_d.alloc(RN); c->pos2_d.alloc(RN); c->fbond_d.alloc(RN); c->vel_d.alloc(RN); c->acc_d.alloc(RN); c->move_d.alloc(RN); c->frc_d.alloc(RN);
     c->n_ele_d.alloc(RN); c->n_wca_d.alloc(RN); c->n_bp_tmp_d.alloc(RN); c->bp_off_d.alloc(RN + 1);
@@ -1324,7 +1330,9 @@ int sis_gpu_get_labels(sis_gpu_ctx *c, int32_t *rep2lab) {
 int sis_gpu_get_info(sis_gpu_ctx *c, int64_t *out, int32_t n) {
   API_BEGIN
   if (!c || !out) return fail(SIS_ERR_ARG, "null argument");
-  int64_t v[10] = {c->N, c->R, c->istep, c->rebuilds, 0, 0, 0, c->D.maxnb, c->mode, c->block};
+  int mode_used = c->parity ? 1 : (c->mode ? c->mode : choose_mode(c));
+  if ((mode_used == 3 && !batched_ok(c)) || (mode_used == 2 && c->R != 1)) mode_used = 1;   // forms fall back
+  int64_t v[10] = {c->N, c->R, c->istep, c->rebuilds, 0, 0, 0, c->D.maxnb, mode_used, c->block};
   if (c->lists_valid) {
     int64_t k;
     if (sis_gpu_get_pair_count(c, 1, 0, &k) == SIS_OK) v[4] = k;

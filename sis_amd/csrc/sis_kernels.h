@@ -396,7 +396,7 @@ __global__ void k_bp_eval(Params P, Ptrs D, int nbp_total, const int *mc, const 
   V3<double> v12 = pvec(P, box, p1, p2), v13 = pvec(P, box, p1, x[mp[2] - 1]),
              v15 = pvec(P, box, p1, x[mp[4] - 1]), v42 = pvec(P, box, x[mp[3] - 1], p2),
              v62 = pvec(P, box, x[mp[5] - 1], p2);
-  const BpPara &bp = P.bpp[mp[6]];
+  const BpPara &bp = D.bpp[mp[6]];
   T u;
   V3<T> fb[6];
   if (!bp_geom<T>(bp, v12, v13, v15, v42, v62, u, fb)) return;
@@ -622,9 +622,16 @@ __global__ void k_seg_fill(Params P, Ptrs D, int nbp_total, int *seg_n, int *seg
 }
 // serial fallback for a component larger than one wave holds: the same
 // algorithm on global memory by one lane
-__device__ __noinline__ void mc_component_serial(const Params &P, const Ptrs &D, int r, int rk, int root, int *seg,
+// The arrays it touches come by value so that the callers' parameter
+// structs stay in registers.
+struct McArrays {
+  const int *bp_mp;
+  uint8_t *bp_status;
+  const double *ene_bp;
+  int *nt_cnt, *flags;
+};
+__device__ __noinline__ void mc_component_serial(const McArrays D, const int N, int r, int rk, int root, int *seg,
                                                  int n, double beta, uint64_t seed, int64_t step) {
-  const int N = P.N;
   int *cnt = D.nt_cnt + (size_t)r * N;
   for (int t = 0; t < n; t++) {
     const int *mp = D.bp_mp + (size_t)seg[t] * 8;
@@ -688,11 +695,13 @@ __device__ __noinline__ void mc_component_serial(const Params &P, const Ptrs &D,
     cnt[mp[1] - 1] -= 1;
   }
 }
+__device__ __forceinline__ McArrays mc_arrays(const Ptrs &D) {
+  return McArrays{D.bp_mp, D.bp_status, D.ene_bp, D.nt_cnt, D.flags};
+}
 __global__ void __launch_bounds__(64) k_mc_wave(Params P, Ptrs D, int *seg_n, int *seg,
                                                 const int *has_exc, const int *mc,
                                                 const double *ecT, uint64_t seed, int64_t step,
                                                 const int *mask) {
-  __shared__ int sseq[64];
   int root = blockIdx.x, r = blockIdx.y;
   if ((mask && !mask[r]) || !mc[r]) return;
   size_t gr = (size_t)r * P.N + root;
@@ -706,8 +715,14 @@ __global__ void __launch_bounds__(64) k_mc_wave(Params P, Ptrs D, int *seg_n, in
     return;
   }
   int *sg = seg + gr * SEGCAP;
-  if (!mc_component_wave(P, D, r, r, root, sg, n, beta, seed, step, sseq)) {
-    if (threadIdx.x == 0) mc_component_serial(P, D, r, r, root, sg, n, beta, seed, step);
+  __shared__ int ws[MCW_LDS];
+  if (n <= MCL_PAIRS) {   // small component: one lane, same result
+    if (threadIdx.x == 0 && !mc_component_lane(D, P.N, r, r, root, sg, n, beta, seed, step))
+      mc_component_serial(mc_arrays(D), P.N, r, r, root, sg, n, beta, seed, step);
+    return;
+  }
+  if (!mc_component_wave(P, D, r, r, root, sg, n, beta, seed, step, ws)) {
+    if (threadIdx.x == 0) mc_component_serial(mc_arrays(D), P.N, r, r, root, sg, n, beta, seed, step);
   }
 }
 
@@ -972,12 +987,39 @@ __device__ __forceinline__ V3<double> pair_part(const Params &P, const Ptrs &D, 
   const int ni = min(D.n_inc[g], D.maxinc);
   V3<double> F{0.0, 0.0, 0.0};
   const T coef = (T)D.ele_coef[r], rc2 = (T)(D.ele_cut[r] * D.ele_cut[r]), rl = (T)(1.0 / D.lambdaD[r]);
+  // neighbours in batches of PU: their index and position loads are all
+  // issued before the arithmetic; the summation order is unchanged
+  constexpr int PU = 4;
   const int *nbw = D.nb_wca + (size_t)r * D.maxw * N + a;
-  for (int k = l; k < nw; k += LPB)
-    F = F + cvt<double>(wca_force<T>(P, cvt<T>(pvec(P, box, pa, x[nbw[(size_t)k * N]]))));
+  for (int k0 = l; k0 < nw; k0 += PU * LPB) {
+    int id[PU];
+    double4 pb[PU];
+#pragma unroll
+    for (int u = 0; u < PU; u++) {
+      const int k = k0 + u * LPB;
+      id[u] = k < nw ? nbw[(size_t)k * N] : -1;
+    }
+#pragma unroll
+    for (int u = 0; u < PU; u++) pb[u] = x[id[u] >= 0 ? id[u] : a];
+#pragma unroll
+    for (int u = 0; u < PU; u++)
+      if (id[u] >= 0) F = F + cvt<double>(wca_force<T>(P, cvt<T>(pvec(P, box, pa, pb[u]))));
+  }
   const int *nbe = D.nb_ele + (size_t)r * D.maxnb * N + a;
-  for (int k = l; k < ne; k += LPB)
-    F = F + cvt<double>(dh_force<T>(coef, rc2, rl, cvt<T>(pvec(P, box, pa, x[nbe[(size_t)k * N]]))));
+  for (int k0 = l; k0 < ne; k0 += PU * LPB) {
+    int id[PU];
+    double4 pb[PU];
+#pragma unroll
+    for (int u = 0; u < PU; u++) {
+      const int k = k0 + u * LPB;
+      id[u] = k < ne ? nbe[(size_t)k * N] : -1;
+    }
+#pragma unroll
+    for (int u = 0; u < PU; u++) pb[u] = x[id[u] >= 0 ? id[u] : a];
+#pragma unroll
+    for (int u = 0; u < PU; u++)
+      if (id[u] >= 0) F = F + cvt<double>(dh_force<T>(coef, rc2, rl, cvt<T>(pvec(P, box, pa, pb[u]))));
+  }
   const int *inc = D.inc + (size_t)r * D.maxinc * N + a;
   for (int k = l; k < ni; k += LPB) {
     int e = inc[(size_t)k * N], gb = e >> 3, role = e & 7;

@@ -15,8 +15,9 @@
 
 namespace sis {
 
-constexpr int MCW_ITEMS = 4;   // pairs per lane   -> 256 pairs per component
-constexpr int MCW_NODES = 8;   // nodes per lane   -> 512 nucleotides
+constexpr int MCW_ITEMS = 2;   // pairs per lane   -> 128 pairs per component
+constexpr int MCW_NODES = 4;   // nodes per lane   -> 256 nucleotides
+constexpr int MCW_LDS = 6 * 64 * MCW_ITEMS + 64;   // ints of wave scratch (LDS) per wave
 
 __device__ __forceinline__ int lane_id() { return __lane_id(); }
 
@@ -26,154 +27,165 @@ __device__ __forceinline__ double philox_u(uint64_t seed, int r, int64_t step, i
   return u01(o.x, o.y);
 }
 
+// Broadcasts from a lane chosen by a wave-uniform index go through
+// v_readlane, and writes to one lane are a per-lane select: register
+// traffic, no LDS round trip.
+__device__ __forceinline__ int bcast(int v, int lane) { return __builtin_amdgcn_readlane(v, lane); }
+__device__ __forceinline__ double bcast(double v, int lane) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)b, lane), hi = __builtin_amdgcn_readlane((int)(b >> 32), lane);
+  return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned int)lo);
+}
+__device__ __forceinline__ int setlane(int v, int x, int lane) { return __lane_id() == lane ? x : v; }
+
+// Set bits of m below this lane (prefix population count).
+__device__ __forceinline__ int below(unsigned long long m) {
+  return __builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0));
+}
+// Orders LDS writes of this wave before its later LDS reads of other lanes' words.
+__device__ __forceinline__ void wave_lds_fence() { __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront"); }
+
 // Resolve the component whose formed pairs are seg[0..n) (candidate ids,
 // any order).  Must be called by all 64 lanes of a wave with uniform
-// arguments.  Returns false if the component exceeds the wave capacity.
-// r indexes the per-replica arrays; rk keys the random stream.
-__device__ __noinline__ bool mc_component_wave(const Params &P, const Ptrs &D, int r, int rk, int root,
-                                               const int *seg, int n, double beta, uint64_t seed,
-                                               int64_t step, int *seq) {
+// arguments; ws is MCW_LDS ints of LDS private to the wave.  Returns false
+// if the component exceeds the wave capacity.  r indexes the per-replica
+// arrays; rk keys the random stream.
+// Layout: pair slot t (candidate-id order) lives in lane t % 64, chunk
+// t / 64; nucleotide slot u (ascending id) in lane u % 64, chunk u / 64;
+// the pair sequence of one iteration holds entry m in lane m.
+__device__ __forceinline__ bool mc_component_wave(const Params &P, const Ptrs &D, int r, int rk, int root,
+                                                  const int *seg, int n, double beta, uint64_t seed,
+                                                  int64_t step, int *ws) {
+  constexpr int NONE = 0x7fffffff;
+  constexpr int NP = 64 * MCW_ITEMS, NE = 2 * NP;
   const int L = lane_id();
   const int N = P.N;
-  if (n > 64 * MCW_ITEMS) return false;
-  // ---- load pairs, order them by candidate id (= reference list order)
-  int gb[MCW_ITEMS], ia[MCW_ITEMS], ib[MCW_ITEMS];
-  double en[MCW_ITEMS];
-  bool alive[MCW_ITEMS];
+  // uniform by contract; said so to the compiler, so that loop counters
+  // and lane indices stay scalar
+  n = __builtin_amdgcn_readfirstlane(n);
+  root = __builtin_amdgcn_readfirstlane(root);
+  if (n > NP) return false;
+  const int nc = 2 * n;
+  int *sraw = ws, *ssort = ws + NP, *send = ws + 2 * NP, *sflag = ws + 2 * NP + NE, *sseq = ws + 2 * NP + 2 * NE;
+  // ---- pairs in candidate-id order (= reference list order): rank by
+  // comparing against all ids (4 per LDS read), then scatter by rank
+  const int n4 = (n + 3) & ~3;
   int raw[MCW_ITEMS];
 #pragma unroll
   for (int k = 0; k < MCW_ITEMS; k++) {
-    int t = L + 64 * k;
-    raw[k] = t < n ? seg[t] : 0x7fffffff;
+    const int t = L + 64 * k;
+    raw[k] = t < n ? seg[t] : NONE;
+    if (t < n4) sraw[t] = raw[k];
   }
-  // rank of each raw id among all n ids
-  int rank[MCW_ITEMS];
+  wave_lds_fence();
+  int rank[MCW_ITEMS] = {};
+  for (int j = 0; j < n4; j += 4) {
+    const int4 v = *reinterpret_cast<const int4 *>(sraw + j);
 #pragma unroll
-  for (int k = 0; k < MCW_ITEMS; k++) rank[k] = 0;
-  for (int c = 0; c < MCW_ITEMS; c++) {
-    if (64 * c >= n) break;
-    for (int s = 0; s < 64; s++) {
-      int v = __shfl(raw[c], s);
-#pragma unroll
-      for (int k = 0; k < MCW_ITEMS; k++) rank[k] += (v < raw[k]) ? 1 : 0;
-    }
+    for (int k = 0; k < MCW_ITEMS; k++)
+      rank[k] += (v.x < raw[k]) + (v.y < raw[k]) + (v.z < raw[k]) + (v.w < raw[k]);
   }
-  // sorted[t] = raw id with rank t: gather by searching ranks
 #pragma unroll
-  for (int k = 0; k < MCW_ITEMS; k++) gb[k] = 0x7fffffff;
-  for (int c = 0; c < MCW_ITEMS; c++) {
-    if (64 * c >= n) break;
-    for (int s = 0; s < 64; s++) {
-      int v = __shfl(raw[c], s), rk = __shfl(rank[c], s);
-      if (64 * c + s >= n) break;
-#pragma unroll
-      for (int k = 0; k < MCW_ITEMS; k++)
-        if (rk == L + 64 * k) gb[k] = v;
-    }
-  }
+  for (int k = 0; k < MCW_ITEMS; k++)
+    if (L + 64 * k < n) ssort[rank[k]] = raw[k];
+  wave_lds_fence();
+  int gb[MCW_ITEMS], ia[MCW_ITEMS], ib[MCW_ITEMS];
+  double en[MCW_ITEMS];
+  bool alive[MCW_ITEMS];
+  bool bad = false;
 #pragma unroll
   for (int k = 0; k < MCW_ITEMS; k++) {
-    int t = L + 64 * k;
+    const int t = L + 64 * k;
     alive[k] = t < n;
+    gb[k] = NONE;
+    ia[k] = ib[k] = -1;
+    en[k] = 0.0;
     if (alive[k]) {
+      gb[k] = ssort[t];
       const int *mp = D.bp_mp + (size_t)gb[k] * 8;
       ia[k] = mp[0] - 1;
       ib[k] = mp[1] - 1;
       en[k] = D.ene_bp[gb[k]];
-    } else {
-      ia[k] = ib[k] = -1;
-      en[k] = 0.0;
+      bad = bad || ia[k] < 0 || ia[k] >= N || ib[k] < 0 || ib[k] >= N;
     }
   }
   // a candidate naming a bead outside the system means corrupted MC state:
   // report it (flags[7]) instead of indexing with it
-  {
-    bool bad = false;
+  if (__ballot(bad)) {
+    if (L == 0) D.flags[7] = 1;
+    return true;
+  }
+  // ---- distinct nucleotides in ascending id order, with their counts.
+  // Endpoint c = 2t (+1) is nucleotide a (b) of pair slot t.
+  const int nc4 = (nc + 3) & ~3;
 #pragma unroll
-    for (int k = 0; k < MCW_ITEMS; k++)
-      bad = bad || (alive[k] && (ia[k] < 0 || ia[k] >= N || ib[k] < 0 || ib[k] >= N));
-    if (__ballot(bad)) {
-      if (L == 0) D.flags[7] = 1;
-      return true;
+  for (int k = 0; k < MCW_ITEMS; k++) {
+    const int t = L + 64 * k;
+    if (t < n) {
+      send[2 * t] = ia[k];
+      send[2 * t + 1] = ib[k];
+    } else if (2 * t < nc4) {
+      send[2 * t] = NONE;
+      send[2 * t + 1] = NONE;
     }
   }
-  // ---- unique nucleotides in ascending id order, with their counts
-  // candidate c = 2*t (+1) is endpoint a (b) of pair t
-  int nid[MCW_NODES];
-  int ncnt[MCW_NODES];
-  const int nc = 2 * n;
-  if (nc > 64 * MCW_NODES) return false;
-  int cand[MCW_NODES];
-#pragma unroll
-  for (int q = 0; q < MCW_NODES; q++) {
-    int c = L + 64 * q;
-    int t = (c < nc) ? (c >> 1) : 0;
-    int kk = t / 64, ll = t % 64;
-    int a = 0, b = 0;
-#pragma unroll
-    for (int k = 0; k < MCW_ITEMS; k++) {
-      int aa = __shfl(ia[k], ll), bb = __shfl(ib[k], ll);   // all lanes active
-      if (k == kk) { a = aa; b = bb; }
-    }
-    cand[q] = (c < nc) ? ((c & 1) ? b : a) : 0x7fffffff;
-  }
-  // first occurrence flag and rank among unique ids
+  wave_lds_fence();
+  // first occurrence of each endpoint value
+  int ev[MCW_NODES];
   bool first[MCW_NODES];
 #pragma unroll
-  for (int q = 0; q < MCW_NODES; q++) first[q] = (L + 64 * q) < nc;
-  for (int c = 0; c < MCW_NODES; c++) {
-    if (64 * c >= nc) break;
-    for (int s = 0; s < 64; s++) {
-      int v = __shfl(cand[c], s);
-      int pos = 64 * c + s;
+  for (int q = 0; q < MCW_NODES; q++) {
+    const int c = L + 64 * q;
+    ev[q] = c < nc ? send[c] : NONE;
+    first[q] = c < nc;
+  }
+  for (int j = 0; j < nc4; j += 4) {
+    const int4 v = *reinterpret_cast<const int4 *>(send + j);
 #pragma unroll
-      for (int q = 0; q < MCW_NODES; q++)
-        if (pos < L + 64 * q && v == cand[q]) first[q] = false;
+    for (int q = 0; q < MCW_NODES; q++) {
+      const int c = L + 64 * q;
+      if ((j < c && v.x == ev[q]) || (j + 1 < c && v.y == ev[q]) || (j + 2 < c && v.z == ev[q]) ||
+          (j + 3 < c && v.w == ev[q]))
+        first[q] = false;
     }
   }
-  int urank[MCW_NODES];
 #pragma unroll
-  for (int q = 0; q < MCW_NODES; q++) urank[q] = 0;
-  for (int c = 0; c < MCW_NODES; c++) {
-    if (64 * c >= nc) break;
-    for (int s = 0; s < 64; s++) {
-      int v = __shfl(cand[c], s);
-      int f = __shfl((int)first[c], s);
-#pragma unroll
-      for (int q = 0; q < MCW_NODES; q++)
-        if (f && v < cand[q]) urank[q]++;
-    }
+  for (int q = 0; q < MCW_NODES; q++) {
+    const int c = L + 64 * q;
+    if (c < nc4) sflag[c] = first[q] ? ev[q] : NONE;   // first occurrences only
   }
-  // place unique ids at their rank: node slot u (lane u%64, chunk u/64)
-#pragma unroll
-  for (int q = 0; q < MCW_NODES; q++) nid[q] = 0x7fffffff;
-  for (int c = 0; c < MCW_NODES; c++) {
-    if (64 * c >= nc) break;
-    for (int s = 0; s < 64; s++) {
-      int v = __shfl(cand[c], s), f = __shfl((int)first[c], s), rk = __shfl(urank[c], s);
-      if (f) {
-#pragma unroll
-        for (int q = 0; q < MCW_NODES; q++)
-          if (rk == L + 64 * q) nid[q] = v;
-      }
-    }
-  }
+  wave_lds_fence();
+  // rank among distinct values, then scatter
   int nn = 0;
-  {
-    int cntu = 0;
 #pragma unroll
-    for (int q = 0; q < MCW_NODES; q++) cntu += (nid[q] != 0x7fffffff) ? 1 : 0;
-    for (int off = 32; off > 0; off >>= 1) cntu += __shfl_xor(cntu, off);
-    nn = cntu;
+  for (int q = 0; q < MCW_NODES; q++) nn += __popcll(__ballot(first[q]));
+  int urank[MCW_NODES] = {};
+  for (int j = 0; j < nc4; j += 4) {
+    const int4 v = *reinterpret_cast<const int4 *>(sflag + j);
+#pragma unroll
+    for (int q = 0; q < MCW_NODES; q++)
+      urank[q] += (v.x < ev[q]) + (v.y < ev[q]) + (v.z < ev[q]) + (v.w < ev[q]);
   }
 #pragma unroll
   for (int q = 0; q < MCW_NODES; q++)
-    ncnt[q] = (nid[q] != 0x7fffffff) ? D.nt_cnt[(size_t)r * N + nid[q]] : 0;
-  // ---- the Monte Carlo loop
-  uint32_t draw = 0;
+    if (first[q]) send[urank[q]] = ev[q];   // endpoints no longer needed
+  wave_lds_fence();
+  int nid[MCW_NODES], ncnt[MCW_NODES];
+#pragma unroll
+  for (int q = 0; q < MCW_NODES; q++) {
+    const int u = L + 64 * q;
+    nid[q] = u < nn ? send[u] : NONE;
+    ncnt[q] = u < nn ? D.nt_cnt[(size_t)r * N + nid[q]] : 0;
+  }
+  // ---- the Monte Carlo loop.  Draws come from a window of 64 counters,
+  // lane i holding draw w0 + i and its logarithm (for the Metropolis test,
+  // log u < dE beta, the same test as u < exp(dE beta)); the window moves
+  // on when an iteration could run past it.
+  uint32_t draw = 0, w0 = 0;
+  double wu = philox_u(seed, rk, step, root, (uint32_t)L), wl = log(wu);
   const int nchunk_nodes = (nn + 63) / 64;
   const int nchunk_items = (n + 63) / 64;
-  for (int iter = 0; iter < 4 * 64 * MCW_ITEMS + 4; iter++) {
+  for (int iter = 0; iter < 4 * NP + 4; iter++) {
     // active nucleotides (count > 0) in ascending id order
     int nact = 0;
     unsigned long long am[MCW_NODES];
@@ -183,81 +195,88 @@ __device__ __noinline__ bool mc_component_wave(const Params &P, const Ptrs &D, i
       nact += __popcll(am[q]);
     }
     if (nact == 0) break;
-    double u = philox_u(seed, rk, step, root, draw++);
-    int pick = (int)ceil(u * nact);   // 1..nact
+    if (draw + 1 > w0 + 64u) {   // the pick draw must be in the window
+      w0 = draw;
+      wu = philox_u(seed, rk, step, root, w0 + (uint32_t)L);
+      wl = log(wu);
+    }
+    const int pick = (int)ceil(bcast(wu, (int)(draw - w0)) * nact);   // 1..nact
+    // the pick-th active nucleotide: its chunk, then the lane whose prefix
+    // count of active lanes is pick - 1
     int ntd = -1;
     {
       int acc = 0;
 #pragma unroll
       for (int q = 0; q < MCW_NODES; q++) {
-        int pc = __popcll(am[q]);
+        const int pc = __popcll(am[q]);
         if (ntd < 0 && acc + pc >= pick) {
-          int want = pick - acc;   // want-th set bit of am[q]
-          unsigned long long m = am[q];
-          for (int w = 1; w < want; w++) m &= m - 1;
-          int lanebit = __ffsll((long long)m) - 1;
-          ntd = __shfl(nid[q], lanebit);
+          const int want = pick - acc - 1;
+          const unsigned long long hit = __ballot(((am[q] >> L) & 1ull) && below(am[q]) == want);
+          ntd = bcast(nid[q], __ffsll((long long)hit) - 1);
         }
         acc += pc;
       }
     }
-    // formed pairs of ntd in list order
+    // formed pairs of ntd in list order: entry m of the sequence in lane m
     unsigned long long sm[MCW_ITEMS];
     int ns = 0;
 #pragma unroll
     for (int k = 0; k < MCW_ITEMS; k++) {
       sm[k] = (k < nchunk_items) ? __ballot(alive[k] && (ia[k] == ntd || ib[k] == ntd)) : 0ull;
+      const int m = ns + below(sm[k]);
+      if (((sm[k] >> L) & 1ull) && m < 64) sseq[m] = 64 * k + L;
       ns += __popcll(sm[k]);
     }
     if (ns == 0) return false;
-    // sequence of pair slots (slot = 64*k + lane); every lane holds the
-    // same values and writes the same LDS words
-    int m = 0;
-#pragma unroll
-    for (int k = 0; k < MCW_ITEMS; k++) {
-      unsigned long long b = sm[k];
-      while (b && m < 64) {
-        int l = __ffsll((long long)b) - 1;
-        seq[m++] = 64 * k + l;
-        b &= b - 1;
-      }
-    }
     if (ns > 64) ns = 64;
+    wave_lds_fence();
+    int seqv = L < ns ? sseq[L] : 0;
+    // draws used: 1 (pick) + ns (shuffle) + ns - 1 (Metropolis) = 2 ns;
+    // a window holding all of them starts at this iteration's pick draw
+    const bool fits = 2 * ns <= 64;
+    if (fits && draw + 2u * (uint32_t)ns > w0 + 64u) {
+      w0 = draw;
+      wu = philox_u(seed, rk, step, root, w0 + (uint32_t)L);
+      wl = log(wu);
+    }
+    const int o = (int)(draw - w0);
+    // u and log u of the draw `k` places after the pick draw
+    auto du = [&](int k) { return fits ? bcast(wu, o + k) : philox_u(seed, rk, step, root, draw + (uint32_t)k); };
+    auto dlog = [&](int k) { return fits ? bcast(wl, o + k) : log(philox_u(seed, rk, step, root, draw + (uint32_t)k)); };
+    // swap targets of the shuffle, lane i holding the i-th
+    const int swl = fits ? (int)ceil(__shfl(wu, (o + 1 + L) & 63) * ns) - 1 : 0;
     // reference shuffle: for i = 1..n swap(i, ceil(rnd*n))
     for (int i = 0; i < ns; i++) {
-      double v = philox_u(seed, rk, step, root, draw++);
-      int sw = (int)ceil(v * ns) - 1;
-      int t = seq[i];
-      seq[i] = seq[sw];
-      seq[sw] = t;
+      const int sw = fits ? bcast(swl, i) : (int)ceil(du(1 + i) * ns) - 1;
+      const int ti = bcast(seqv, i), ts = bcast(seqv, sw);
+      seqv = setlane(seqv, ts, i);
+      seqv = setlane(seqv, ti, sw);
     }
-    // energies of the sequence's pairs
+    // Metropolis sweep on the energies of the sequence's pairs
     auto ene_of = [&](int slot) {
-      int k = slot / 64, l = slot % 64;
-      double e = 0.0;
+      const int k = slot >> 6;
+      double x = en[0];
 #pragma unroll
-      for (int kk = 0; kk < MCW_ITEMS; kk++) {
-        double x = __shfl(en[kk], l);
-        if (kk == k) e = x;
-      }
-      return e;
+      for (int kk = 1; kk < MCW_ITEMS; kk++) x = (kk == k) ? en[kk] : x;
+      return bcast(x, slot & 63);
     };
-    int del = seq[0];
+    int del = bcast(seqv, 0);
     double edel = ene_of(del);
     for (int i = 1; i < ns; i++) {
-      double ej = ene_of(seq[i]);
-      double ratio = exp((ej - edel) * beta);
-      double v = philox_u(seed, rk, step, root, draw++);
-      if (v < ratio) { del = seq[i]; edel = ej; }
+      const int si = bcast(seqv, i);
+      const double ej = ene_of(si);
+      if (dlog(ns + i) < (ej - edel) * beta) { del = si; edel = ej; }
     }
+    draw += 2u * (uint32_t)ns;
     // delete: pair slot del, decrement its two nucleotides
-    int dk = del / 64, dl = del % 64;
-    int da = -1, db = -1;
+    const int dk = del >> 6, dl = del & 63;
+    int xa = ia[0], xb = ib[0];
 #pragma unroll
-    for (int k = 0; k < MCW_ITEMS; k++) {
-      int aa = __shfl(ia[k], dl), bb = __shfl(ib[k], dl);
-      if (k == dk) { da = aa; db = bb; }
+    for (int k = 1; k < MCW_ITEMS; k++) {
+      xa = (k == dk) ? ia[k] : xa;
+      xb = (k == dk) ? ib[k] : xb;
     }
+    const int da = bcast(xa, dl), db = bcast(xb, dl);
 #pragma unroll
     for (int k = 0; k < MCW_ITEMS; k++)
       if (k == dk && L == dl) alive[k] = false;
@@ -268,9 +287,159 @@ __device__ __noinline__ bool mc_component_wave(const Params &P, const Ptrs &D, i
   // write back the pairs deleted here
 #pragma unroll
   for (int k = 0; k < MCW_ITEMS; k++) {
-    int t = L + 64 * k;
+    const int t = L + 64 * k;
     if (t < n && !alive[k]) D.bp_status[gb[k]] = 0;
   }
+  return true;
+}
+
+// ---------------------------------------------------------------------
+// The same algorithm for a component of at most MCL_PAIRS pairs, run by
+// one lane with everything in registers: arrays are read through unrolled
+// selects and the pair sequence is packed in 4-bit fields, so no per-lane
+// array is indexed dynamically.  The draws (seed, rk, step, root, 0, 1, ...)
+// and every decision are those of mc_component_wave, so the two give
+// identical results; many small components resolve in parallel, one per
+// lane.  Returns false if the component is too large or inconsistent.
+constexpr int MCL_PAIRS = 4;
+
+template <int M>
+__device__ __forceinline__ int sel_i(const int (&a)[M], int i) {
+  int v = a[0];
+#pragma unroll
+  for (int k = 1; k < M; k++) v = (i == k) ? a[k] : v;
+  return v;
+}
+template <int M>
+__device__ __forceinline__ double sel_d(const double (&a)[M], int i) {
+  double v = a[0];
+#pragma unroll
+  for (int k = 1; k < M; k++) v = (i == k) ? a[k] : v;
+  return v;
+}
+
+__device__ __forceinline__ bool mc_component_lane(const Ptrs &D, int N, int r, int rk, int root, const int *seg,
+                                                  int n, double beta, uint64_t seed, int64_t step) {
+  constexpr int M = MCL_PAIRS, M2 = 2 * MCL_PAIRS;
+  constexpr int NONE = 0x7fffffff;
+  if (n > M) return false;
+  // candidate ids in ascending order (= list order)
+  int raw[M], g[M];
+#pragma unroll
+  for (int j = 0; j < M; j++) raw[j] = j < n ? seg[j] : NONE;
+#pragma unroll
+  for (int p = 0; p < M; p++) g[p] = NONE;
+#pragma unroll
+  for (int i = 0; i < M; i++) {
+    int rank = 0;
+#pragma unroll
+    for (int j = 0; j < M; j++) rank += raw[j] < raw[i] ? 1 : 0;
+#pragma unroll
+    for (int p = 0; p < M; p++)
+      if (i < n && rank == p) g[p] = raw[i];
+  }
+  int pa[M], pb[M];
+  double en[M];
+  bool bad = false;
+#pragma unroll
+  for (int j = 0; j < M; j++) {
+    pa[j] = pb[j] = -1;
+    en[j] = 0.0;
+    if (j < n) {
+      const int *mp = D.bp_mp + (size_t)g[j] * 8;
+      pa[j] = mp[0] - 1;
+      pb[j] = mp[1] - 1;
+      en[j] = D.ene_bp[g[j]];
+      bad = bad || pa[j] < 0 || pa[j] >= N || pb[j] < 0 || pb[j] >= N;
+    }
+  }
+  if (bad) {
+    D.flags[7] = 1;
+    return true;
+  }
+  // distinct nucleotides in ascending id order, with their counts
+  int e[M2];
+#pragma unroll
+  for (int j = 0; j < M; j++) {
+    e[2 * j] = j < n ? pa[j] : NONE;
+    e[2 * j + 1] = j < n ? pb[j] : NONE;
+  }
+  bool first[M2];
+#pragma unroll
+  for (int k = 0; k < M2; k++) {
+    bool f = e[k] != NONE;
+#pragma unroll
+    for (int j = 0; j < M2; j++)
+      if (j < k && e[j] == e[k]) f = false;
+    first[k] = f;
+  }
+  int node[M2], cnt[M2];
+#pragma unroll
+  for (int q = 0; q < M2; q++) node[q] = NONE;
+#pragma unroll
+  for (int k = 0; k < M2; k++) {
+    int ur = 0;
+#pragma unroll
+    for (int j = 0; j < M2; j++) ur += (first[j] && e[j] < e[k]) ? 1 : 0;
+#pragma unroll
+    for (int q = 0; q < M2; q++)
+      if (first[k] && ur == q) node[q] = e[k];
+  }
+#pragma unroll
+  for (int q = 0; q < M2; q++) cnt[q] = node[q] != NONE ? D.nt_cnt[(size_t)r * N + node[q]] : 0;
+  // the Monte Carlo loop
+  uint32_t alive = (n >= 32) ? 0xffffffffu : ((1u << n) - 1u);
+  uint32_t draw = 0;
+  for (int iter = 0; iter < 4 * M + 4; iter++) {
+    int nact = 0;
+#pragma unroll
+    for (int q = 0; q < M2; q++) nact += cnt[q] > 0 ? 1 : 0;
+    if (nact == 0) break;
+    const int pick = (int)ceil(philox_u(seed, rk, step, root, draw++) * nact);   // 1..nact
+    int ntd = -1, acc = 0;
+#pragma unroll
+    for (int q = 0; q < M2; q++)
+      if (cnt[q] > 0) {
+        acc++;
+        if (acc == pick) ntd = node[q];
+      }
+    // formed pairs of ntd in list order, packed 4 bits each
+    uint32_t seqp = 0;
+    int ns = 0;
+#pragma unroll
+    for (int j = 0; j < M; j++)
+      if (((alive >> j) & 1u) && (pa[j] == ntd || pb[j] == ntd)) {
+        seqp |= (uint32_t)j << (4 * ns);
+        ns++;
+      }
+    if (ns == 0) return false;
+    // reference shuffle: for i = 1..n swap(i, ceil(rnd*n))
+    for (int i = 0; i < ns; i++) {
+      const int sw = (int)ceil(philox_u(seed, rk, step, root, draw++) * ns) - 1;
+      const uint32_t ti = (seqp >> (4 * i)) & 15u, ts = (seqp >> (4 * sw)) & 15u;
+      seqp = (seqp & ~(15u << (4 * i))) | (ts << (4 * i));
+      seqp = (seqp & ~(15u << (4 * sw))) | (ti << (4 * sw));
+    }
+    // Metropolis sweep on the energies
+    int del = (int)(seqp & 15u);
+    double edel = sel_d(en, del);
+    for (int i = 1; i < ns; i++) {
+      const int j = (int)((seqp >> (4 * i)) & 15u);
+      const double ej = sel_d(en, j);
+      if (log(philox_u(seed, rk, step, root, draw++)) < (ej - edel) * beta) {
+        del = j;
+        edel = ej;
+      }
+    }
+    alive &= ~(1u << del);
+    const int da = sel_i(pa, del), db = sel_i(pb, del);
+#pragma unroll
+    for (int q = 0; q < M2; q++)
+      if (node[q] == da || node[q] == db) cnt[q] -= 1;
+  }
+#pragma unroll
+  for (int j = 0; j < M; j++)
+    if (j < n && !((alive >> j) & 1u)) D.bp_status[g[j]] = 0;
   return true;
 }
 

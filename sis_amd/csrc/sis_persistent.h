@@ -59,7 +59,7 @@ __device__ __forceinline__ int bp_eval_one(const Params &P, const Ptrs &D, int g
   V3<double> v12 = pvec(P, box, p1, p2), v13 = pvec(P, box, p1, x[mp[2] - 1]),
              v15 = pvec(P, box, p1, x[mp[4] - 1]), v42 = pvec(P, box, x[mp[3] - 1], p2),
              v62 = pvec(P, box, x[mp[5] - 1], p2);
-  const BpPara &bp = P.bpp[mp[6]];
+  const BpPara &bp = D.bpp[mp[6]];
   T u;
   V3<T> fb[6];
   if (!bp_geom<T>(bp, v12, v13, v15, v42, v62, u, fb)) return 0;
@@ -81,8 +81,8 @@ __device__ __forceinline__ int bp_eval_one(const Params &P, const Ptrs &D, int g
 template <typename T, int LPB, int B>
 __global__ void __launch_bounds__(B) k_persistent(Params P, Ptrs D, PersArgs A) {
   __shared__ double s1[B / LPB], s2[B / LPB];
-  __shared__ int sseq[B];   // one 64-entry sequence per wave
   __shared__ int s_exit;
+  __shared__ int s_mcws[(B / 64) * MCW_LDS];   // wave MC scratch, one slice per wave
   const int N = P.N;
   const unsigned G = gridDim.x;
   const int gtid = blockIdx.x * B + threadIdx.x, gthreads = G * B;
@@ -135,20 +135,28 @@ __global__ void __launch_bounds__(B) k_persistent(Params P, Ptrs D, PersArgs A) 
         if (D.nt_cnt[a] > 0 || D.nt_cnt[b] > 0) A.has_exc[root] = 1;
       }
       grid_barrier(A.bar, target, G);
-      // ---- C: one wavefront per component with excess
+      // ---- C: small components one lane each, larger ones one wavefront each
       {
+        for (int root = gtid; root < N; root += gthreads) {
+          if (!A.has_exc[root]) continue;
+          const int n = A.seg_n[root];
+          if (n > MCL_PAIRS) continue;
+          int *sg = A.seg + (size_t)root * SEGCAP;
+          if (!mc_component_lane(D, N, 0, 0, root, sg, n, beta, A.seed, step))
+            mc_component_serial(mc_arrays(D), P.N, 0, 0, root, sg, n, beta, A.seed, step);
+        }
         const int wave = gtid / 64, nwaves = gthreads / 64;
-        int *seq = sseq + (threadIdx.x / 64) * 64;
         for (int root = wave; root < N; root += nwaves) {
           if (!A.has_exc[root]) continue;
           int n = A.seg_n[root];
+          if (n <= MCL_PAIRS) continue;
           if (n > SEGCAP) {
             if ((threadIdx.x & 63) == 0) D.flags[5] = 1;
             continue;
           }
           int *sg = A.seg + (size_t)root * SEGCAP;
-          if (!mc_component_wave(P, D, 0, 0, root, sg, n, beta, A.seed, step, seq)) {
-            if ((threadIdx.x & 63) == 0) mc_component_serial(P, D, 0, 0, root, sg, n, beta, A.seed, step);
+          if (!mc_component_wave(P, D, 0, 0, root, sg, n, beta, A.seed, step, s_mcws + (threadIdx.x / 64) * MCW_LDS)) {
+            if ((threadIdx.x & 63) == 0) mc_component_serial(mc_arrays(D), P.N, 0, 0, root, sg, n, beta, A.seed, step);
           }
         }
       }
@@ -322,7 +330,7 @@ __device__ __forceinline__ V3<double> group_sum_rt(V3<double> F, int lpb) {
 // src/neighbor_list.F90:123-223, candidates in owner-bead order, their
 // incidences sorted per bead.  Sets *s_flag when a capacity is exceeded.
 template <int B>
-__device__ void batched_rebuild(const Params &P, const Ptrs &D, const SysIO &J, int r, int *s_scan,
+__device__ __forceinline__ void batched_rebuild(const Params &P, const Ptrs &D, const SysIO &J, int r, int *s_scan,
                                 double4 *s_tile, int *s_flag) {
   const int N = P.N;
   const int tid = threadIdx.x;
@@ -449,23 +457,21 @@ __device__ void batched_rebuild(const Params &P, const Ptrs &D, const SysIO &J, 
 
 // Whole job_md steps of system r (src/job_md.F90:374-460) for J.todo steps.
 template <typename T, int B>
-__device__ void batched_body(const Params &P, Ptrs &D, const SysIO &J, const int r) {
+__device__ __forceinline__ void batched_body(const Params &P, const Ptrs &D, const SysIO &J, const int r) {
   __shared__ double s1[B], s2[B];
-  __shared__ int sseq[B];
   __shared__ int s_scan[B];
   __shared__ double4 s_tile[B];
   __shared__ int s_flag, s_exit;
   __shared__ int s_exc[EXC_LDS];   // component roots with excess (MC steps)
   __shared__ int s_nexc;
+  __shared__ int s_mcws[(B / 64) * MCW_LDS];   // wave MC scratch, one slice per wave
   const int N = P.N;
   const int tid = threadIdx.x;
   const size_t base = (size_t)r * N;
   const int lpb = J.lpb, grp = tid / lpb, l = tid % lpb, ngrp = B / lpb;
   const double BOLTZ = 1.380649e-23 * ((1.0 / 4184.0) * 6.02214076e23);
   const double beta = 1.0 / (BOLTZ * D.tK[r]);
-  double4 *pos_a = D.pos, *pos_b = J.pos_b;
   int *comp = D.comp + base;
-  double4 *const pos_first = D.pos;
   if (tid == 0) s_flag = 0;
   // per-step limit-MC state starts reset whatever ran before this launch
   for (int a = tid; a < N; a += B) {
@@ -491,8 +497,6 @@ __device__ void batched_body(const Params &P, Ptrs &D, const SysIO &J, const int
     const int64_t step = step0 + s + 1;
     const int mc = P.nstep_bp_mc > 0 && ((step % P.nstep_bp_mc) == 0 || mc_next);
     mc_next = 0;
-    D.pos = pos_a;
-    D.pos_out = pos_b;
     const int nbr = *J.nbp;
     // ---- A: base pairs (+ union of formed pairs on MC steps), bonded forces
     for (int k = tid; k < nbr; k += B) {
@@ -539,18 +543,30 @@ __device__ void batched_body(const Params &P, Ptrs &D, const SysIO &J, const int
       // ---- C: one wavefront per component with excess (components are
       // disjoint and keyed by their root, so the order does not matter)
       const int wave = tid / 64, nwaves = B / 64;
-      int *seq = sseq + wave * 64;
       const int nexc = s_nexc;
       const bool listed = nexc <= EXC_LDS;
+      // small components: one lane each
+      for (int it = tid; it < (listed ? nexc : N); it += B) {
+        const int root = listed ? s_exc[it] : it;
+        const size_t gr = base + root;
+        if (!J.has_exc[gr]) continue;
+        const int n = J.seg_n[gr];
+        if (n > MCL_PAIRS) continue;
+        int *sg = J.seg + gr * SEGCAP;
+        if (!mc_component_lane(D, N, r, J.rk, root, sg, n, beta, J.seed, step))
+          mc_component_serial(mc_arrays(D), P.N, r, J.rk, root, sg, n, beta, J.seed, step);
+      }
+      // larger ones: one wavefront each
       for (int it = wave; it < (listed ? nexc : N); it += nwaves) {
         const int root = listed ? s_exc[it] : it;
         const size_t gr = base + root;
         if (!J.has_exc[gr]) continue;
         const int n = J.seg_n[gr];
+        if (n <= MCL_PAIRS) continue;
         if (n > SEGCAP) { if ((tid & 63) == 0) D.flags[5] = 1; continue; }
         int *sg = J.seg + gr * SEGCAP;
-        if (!mc_component_wave(P, D, r, J.rk, root, sg, n, beta, J.seed, step, seq))
-          if ((tid & 63) == 0) mc_component_serial(P, D, r, J.rk, root, sg, n, beta, J.seed, step);
+        if (!mc_component_wave(P, D, r, J.rk, root, sg, n, beta, J.seed, step, s_mcws + wave * MCW_LDS))
+          if ((tid & 63) == 0) mc_component_serial(mc_arrays(D), P.N, r, J.rk, root, sg, n, beta, J.seed, step);
       }
       __syncthreads();
       // the MC result is what later non-MC steps evaluate
@@ -587,12 +603,13 @@ __device__ void batched_body(const Params &P, Ptrs &D, const SysIO &J, const int
       __syncthreads();
     }
     if (tid == 0) s_exit = (sqrt(s1[0]) + sqrt(s2[0]) > P.nl_margin) ? 1 : 0;
-    { double4 *t = pos_a; pos_a = pos_b; pos_b = t; }
+    // the updated positions become current (every reader of D.pos in
+    // this step has passed the barriers above)
+    for (int a = tid; a < N; a += B) D.pos[base + a] = D.pos_out[base + a];
     __syncthreads();
     completed = s + 1;
     if (s_exit) {
       if (s + 1 < nsteps) {
-        D.pos = pos_a;
         batched_rebuild<B>(P, D, J, r, s_scan, s_tile, &s_flag);
         overflow = s_flag;
         mc_next = 1;
@@ -602,9 +619,6 @@ __device__ void batched_body(const Params &P, Ptrs &D, const SysIO &J, const int
       }
     }
   }
-  // callers keep one position buffer for this form: copy back if needed
-  if (pos_a != pos_first)
-    for (int a = tid; a < N; a += B) pos_first[base + a] = pos_a[base + a];
   if (tid == 0) {
     *J.done = (int)completed;
     *J.nreb = nreb;
@@ -631,8 +645,7 @@ __global__ void __launch_bounds__(B) k_batched(Params P, Ptrs D, BatchArgs A) {
   J.nbp = A.nbp + r;
   J.rebuild = A.rebuild + r;
   J.nreb = A.nreb + r;
-  Ptrs Dl = D;
-  batched_body<T, B>(P, Dl, J, r);
+  batched_body<T, B>(P, D, J, r);   // D.pos_out is the second position buffer
 }
 
 // One serving job: its sizes and the device arrays of its own context.

@@ -70,6 +70,7 @@ struct sis_pool {
   size_t bytes_used = 0;
   bool have_P0 = false;
   Params P0{};
+  BpPara bpp0[7] = {};
   hipStream_t stream[2] = {nullptr, nullptr};
   DevBuf<int> io_d, flags_d;
   DevBuf<JobView> views_d[2];
@@ -129,8 +130,12 @@ bool job_admit(sis_pool *p, PoolJob &j) {
   sis_gpu_ctx *c = nullptr;
   if (sis_gpu_create(&t, &j.prm, &o, &c) != SIS_OK) { job_fail(p, j, g_err); return false; }
   j.c = c;
-  if (!p->have_P0) { p->P0 = c->P; p->have_P0 = true; }
-  if (!same_force_field(p->P0, c->P)) {
+  if (!p->have_P0) {
+    p->P0 = c->P;
+    std::memcpy(p->bpp0, c->bpp_h, sizeof p->bpp0);
+    p->have_P0 = true;
+  }
+  if (!same_force_field(p->P0, c->P) || std::memcmp(p->bpp0, c->bpp_h, sizeof p->bpp0) != 0) {
     job_fail(p, j, "job force field differs from the pool's");
     job_release(p, j);
     return false;
