@@ -168,11 +168,22 @@ def run(sysd: dict, ops: list, seed: int = 12345, omp: bool = False, nstep_excha
         raise RuntimeError(f"{so} not built (make -C oracle ref)")
     ctx = mp.get_context("spawn")
     q = ctx.Queue()
-    env_old = os.environ.get("OMP_NUM_THREADS")
+    env_old = {k: os.environ.get(k) for k in ("OMP_NUM_THREADS", "OMP_STACKSIZE")}
     os.environ["OMP_NUM_THREADS"] = str(threads)
+    # the reference keeps per-replica work arrays as stack automatics
+    # (src/job_md.F90:40-44, src/force.F90:20): the child gets the largest
+    # stack the hard limit allows, and large OpenMP thread stacks
+    os.environ.setdefault("OMP_STACKSIZE", "1G")
+    import resource
+    stack_old = resource.getrlimit(resource.RLIMIT_STACK)
+    try:
+        resource.setrlimit(resource.RLIMIT_STACK, (stack_old[1], stack_old[1]))
+    except (ValueError, OSError):
+        pass
     try:
         pr = ctx.Process(target=_worker, args=(q, so, sysd, ops, seed, nstep_exchange))
         pr.start()
+        resource.setrlimit(resource.RLIMIT_STACK, stack_old)
         import queue
         import time
         t0 = time.time()
@@ -190,10 +201,15 @@ def run(sysd: dict, ops: list, seed: int = 12345, omp: bool = False, nstep_excha
                     break
         pr.join(timeout=30)
     finally:
-        if env_old is None:
-            os.environ.pop("OMP_NUM_THREADS", None)
-        else:
-            os.environ["OMP_NUM_THREADS"] = env_old
+        try:
+            resource.setrlimit(resource.RLIMIT_STACK, stack_old)
+        except (ValueError, OSError):
+            pass
+        for k, v in env_old.items():
+            if v is None:
+                os.environ.pop(k, None)
+            else:
+                os.environ[k] = v
     if status != "ok":
         raise RuntimeError("reference run failed:\n" + payload)
     return payload

@@ -715,15 +715,15 @@ __global__ void __launch_bounds__(64) k_mc_wave(Params P, Ptrs D, int *seg_n, in
     return;
   }
   int *sg = seg + gr * SEGCAP;
-  __shared__ int ws[MCW_LDS];
+  __shared__ int ws[MCB_LDS];
   if (n <= MCL_PAIRS) {   // small component: one lane, same result
     if (threadIdx.x == 0 && !mc_component_lane(D, P.N, r, r, root, sg, n, beta, seed, step))
       mc_component_serial(mc_arrays(D), P.N, r, r, root, sg, n, beta, seed, step);
     return;
   }
-  if (!mc_component_wave(P, D, r, r, root, sg, n, beta, seed, step, ws)) {
-    if (threadIdx.x == 0) mc_component_serial(mc_arrays(D), P.N, r, r, root, sg, n, beta, seed, step);
-  }
+  const bool ok = n <= 64 * MCW_ITEMS ? mc_component_wave(P, D, r, r, root, sg, n, beta, seed, step, ws)
+                                      : mc_component_wave<MCB_ITEMS, MCB_NODES>(P, D, r, r, root, sg, n, beta, seed, step, ws);
+  if (!ok && threadIdx.x == 0) mc_component_serial(mc_arrays(D), P.N, r, r, root, sg, n, beta, seed, step);
 }
 
 // ---------------------------------------------------------------------

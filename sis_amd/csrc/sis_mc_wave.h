@@ -17,7 +17,12 @@ namespace sis {
 
 constexpr int MCW_ITEMS = 2;   // pairs per lane   -> 128 pairs per component
 constexpr int MCW_NODES = 4;   // nodes per lane   -> 256 nucleotides
-constexpr int MCW_LDS = 6 * 64 * MCW_ITEMS + 64;   // ints of wave scratch (LDS) per wave
+// ints of wave scratch (LDS) for `items` pairs per lane
+constexpr int mcw_lds(int items) { return 6 * 64 * items + 64; }
+constexpr int MCW_LDS = mcw_lds(MCW_ITEMS);
+// the large form: components of up to 256 pairs (the bucket capacity)
+constexpr int MCB_ITEMS = 4, MCB_NODES = 8;
+constexpr int MCB_LDS = mcw_lds(MCB_ITEMS);
 
 __device__ __forceinline__ int lane_id() { return __lane_id(); }
 
@@ -47,17 +52,19 @@ __device__ __forceinline__ void wave_lds_fence() { __builtin_amdgcn_fence(__ATOM
 
 // Resolve the component whose formed pairs are seg[0..n) (candidate ids,
 // any order).  Must be called by all 64 lanes of a wave with uniform
-// arguments; ws is MCW_LDS ints of LDS private to the wave.  Returns false
-// if the component exceeds the wave capacity.  r indexes the per-replica
+// arguments; ws is mcw_lds(IT) ints of LDS private to the wave.  Returns
+// false if the component exceeds the capacity of 64*IT pairs.  The result
+// does not depend on IT (same draws, same decisions).  r indexes the per-replica
 // arrays; rk keys the random stream.
 // Layout: pair slot t (candidate-id order) lives in lane t % 64, chunk
 // t / 64; nucleotide slot u (ascending id) in lane u % 64, chunk u / 64;
 // the pair sequence of one iteration holds entry m in lane m.
+template <int IT = MCW_ITEMS, int ND = MCW_NODES>
 __device__ __forceinline__ bool mc_component_wave(const Params &P, const Ptrs &D, int r, int rk, int root,
                                                   const int *seg, int n, double beta, uint64_t seed,
                                                   int64_t step, int *ws) {
   constexpr int NONE = 0x7fffffff;
-  constexpr int NP = 64 * MCW_ITEMS, NE = 2 * NP;
+  constexpr int NP = 64 * IT, NE = 2 * NP;
   const int L = lane_id();
   const int N = P.N;
   // uniform by contract; said so to the compiler, so that loop counters
@@ -70,31 +77,31 @@ __device__ __forceinline__ bool mc_component_wave(const Params &P, const Ptrs &D
   // ---- pairs in candidate-id order (= reference list order): rank by
   // comparing against all ids (4 per LDS read), then scatter by rank
   const int n4 = (n + 3) & ~3;
-  int raw[MCW_ITEMS];
+  int raw[IT];
 #pragma unroll
-  for (int k = 0; k < MCW_ITEMS; k++) {
+  for (int k = 0; k < IT; k++) {
     const int t = L + 64 * k;
     raw[k] = t < n ? seg[t] : NONE;
     if (t < n4) sraw[t] = raw[k];
   }
   wave_lds_fence();
-  int rank[MCW_ITEMS] = {};
+  int rank[IT] = {};
   for (int j = 0; j < n4; j += 4) {
     const int4 v = *reinterpret_cast<const int4 *>(sraw + j);
 #pragma unroll
-    for (int k = 0; k < MCW_ITEMS; k++)
+    for (int k = 0; k < IT; k++)
       rank[k] += (v.x < raw[k]) + (v.y < raw[k]) + (v.z < raw[k]) + (v.w < raw[k]);
   }
 #pragma unroll
-  for (int k = 0; k < MCW_ITEMS; k++)
+  for (int k = 0; k < IT; k++)
     if (L + 64 * k < n) ssort[rank[k]] = raw[k];
   wave_lds_fence();
-  int gb[MCW_ITEMS], ia[MCW_ITEMS], ib[MCW_ITEMS];
-  double en[MCW_ITEMS];
-  bool alive[MCW_ITEMS];
+  int gb[IT], ia[IT], ib[IT];
+  double en[IT];
+  bool alive[IT];
   bool bad = false;
 #pragma unroll
-  for (int k = 0; k < MCW_ITEMS; k++) {
+  for (int k = 0; k < IT; k++) {
     const int t = L + 64 * k;
     alive[k] = t < n;
     gb[k] = NONE;
@@ -119,7 +126,7 @@ __device__ __forceinline__ bool mc_component_wave(const Params &P, const Ptrs &D
   // Endpoint c = 2t (+1) is nucleotide a (b) of pair slot t.
   const int nc4 = (nc + 3) & ~3;
 #pragma unroll
-  for (int k = 0; k < MCW_ITEMS; k++) {
+  for (int k = 0; k < IT; k++) {
     const int t = L + 64 * k;
     if (t < n) {
       send[2 * t] = ia[k];
@@ -131,10 +138,10 @@ __device__ __forceinline__ bool mc_component_wave(const Params &P, const Ptrs &D
   }
   wave_lds_fence();
   // first occurrence of each endpoint value
-  int ev[MCW_NODES];
-  bool first[MCW_NODES];
+  int ev[ND];
+  bool first[ND];
 #pragma unroll
-  for (int q = 0; q < MCW_NODES; q++) {
+  for (int q = 0; q < ND; q++) {
     const int c = L + 64 * q;
     ev[q] = c < nc ? send[c] : NONE;
     first[q] = c < nc;
@@ -142,7 +149,7 @@ __device__ __forceinline__ bool mc_component_wave(const Params &P, const Ptrs &D
   for (int j = 0; j < nc4; j += 4) {
     const int4 v = *reinterpret_cast<const int4 *>(send + j);
 #pragma unroll
-    for (int q = 0; q < MCW_NODES; q++) {
+    for (int q = 0; q < ND; q++) {
       const int c = L + 64 * q;
       if ((j < c && v.x == ev[q]) || (j + 1 < c && v.y == ev[q]) || (j + 2 < c && v.z == ev[q]) ||
           (j + 3 < c && v.w == ev[q]))
@@ -150,7 +157,7 @@ __device__ __forceinline__ bool mc_component_wave(const Params &P, const Ptrs &D
     }
   }
 #pragma unroll
-  for (int q = 0; q < MCW_NODES; q++) {
+  for (int q = 0; q < ND; q++) {
     const int c = L + 64 * q;
     if (c < nc4) sflag[c] = first[q] ? ev[q] : NONE;   // first occurrences only
   }
@@ -158,21 +165,21 @@ __device__ __forceinline__ bool mc_component_wave(const Params &P, const Ptrs &D
   // rank among distinct values, then scatter
   int nn = 0;
 #pragma unroll
-  for (int q = 0; q < MCW_NODES; q++) nn += __popcll(__ballot(first[q]));
-  int urank[MCW_NODES] = {};
+  for (int q = 0; q < ND; q++) nn += __popcll(__ballot(first[q]));
+  int urank[ND] = {};
   for (int j = 0; j < nc4; j += 4) {
     const int4 v = *reinterpret_cast<const int4 *>(sflag + j);
 #pragma unroll
-    for (int q = 0; q < MCW_NODES; q++)
+    for (int q = 0; q < ND; q++)
       urank[q] += (v.x < ev[q]) + (v.y < ev[q]) + (v.z < ev[q]) + (v.w < ev[q]);
   }
 #pragma unroll
-  for (int q = 0; q < MCW_NODES; q++)
+  for (int q = 0; q < ND; q++)
     if (first[q]) send[urank[q]] = ev[q];   // endpoints no longer needed
   wave_lds_fence();
-  int nid[MCW_NODES], ncnt[MCW_NODES];
+  int nid[ND], ncnt[ND];
 #pragma unroll
-  for (int q = 0; q < MCW_NODES; q++) {
+  for (int q = 0; q < ND; q++) {
     const int u = L + 64 * q;
     nid[q] = u < nn ? send[u] : NONE;
     ncnt[q] = u < nn ? D.nt_cnt[(size_t)r * N + nid[q]] : 0;
@@ -188,9 +195,9 @@ __device__ __forceinline__ bool mc_component_wave(const Params &P, const Ptrs &D
   for (int iter = 0; iter < 4 * NP + 4; iter++) {
     // active nucleotides (count > 0) in ascending id order
     int nact = 0;
-    unsigned long long am[MCW_NODES];
+    unsigned long long am[ND];
 #pragma unroll
-    for (int q = 0; q < MCW_NODES; q++) {
+    for (int q = 0; q < ND; q++) {
       am[q] = (q < nchunk_nodes) ? __ballot(ncnt[q] > 0) : 0ull;
       nact += __popcll(am[q]);
     }
@@ -207,7 +214,7 @@ __device__ __forceinline__ bool mc_component_wave(const Params &P, const Ptrs &D
     {
       int acc = 0;
 #pragma unroll
-      for (int q = 0; q < MCW_NODES; q++) {
+      for (int q = 0; q < ND; q++) {
         const int pc = __popcll(am[q]);
         if (ntd < 0 && acc + pc >= pick) {
           const int want = pick - acc - 1;
@@ -218,10 +225,10 @@ __device__ __forceinline__ bool mc_component_wave(const Params &P, const Ptrs &D
       }
     }
     // formed pairs of ntd in list order: entry m of the sequence in lane m
-    unsigned long long sm[MCW_ITEMS];
+    unsigned long long sm[IT];
     int ns = 0;
 #pragma unroll
-    for (int k = 0; k < MCW_ITEMS; k++) {
+    for (int k = 0; k < IT; k++) {
       sm[k] = (k < nchunk_items) ? __ballot(alive[k] && (ia[k] == ntd || ib[k] == ntd)) : 0ull;
       const int m = ns + below(sm[k]);
       if (((sm[k] >> L) & 1ull) && m < 64) sseq[m] = 64 * k + L;
@@ -257,7 +264,7 @@ __device__ __forceinline__ bool mc_component_wave(const Params &P, const Ptrs &D
       const int k = slot >> 6;
       double x = en[0];
 #pragma unroll
-      for (int kk = 1; kk < MCW_ITEMS; kk++) x = (kk == k) ? en[kk] : x;
+      for (int kk = 1; kk < IT; kk++) x = (kk == k) ? en[kk] : x;
       return bcast(x, slot & 63);
     };
     int del = bcast(seqv, 0);
@@ -272,21 +279,21 @@ __device__ __forceinline__ bool mc_component_wave(const Params &P, const Ptrs &D
     const int dk = del >> 6, dl = del & 63;
     int xa = ia[0], xb = ib[0];
 #pragma unroll
-    for (int k = 1; k < MCW_ITEMS; k++) {
+    for (int k = 1; k < IT; k++) {
       xa = (k == dk) ? ia[k] : xa;
       xb = (k == dk) ? ib[k] : xb;
     }
     const int da = bcast(xa, dl), db = bcast(xb, dl);
 #pragma unroll
-    for (int k = 0; k < MCW_ITEMS; k++)
+    for (int k = 0; k < IT; k++)
       if (k == dk && L == dl) alive[k] = false;
 #pragma unroll
-    for (int q = 0; q < MCW_NODES; q++)
+    for (int q = 0; q < ND; q++)
       if (nid[q] == da || nid[q] == db) ncnt[q] -= 1;
   }
   // write back the pairs deleted here
 #pragma unroll
-  for (int k = 0; k < MCW_ITEMS; k++) {
+  for (int k = 0; k < IT; k++) {
     const int t = L + 64 * k;
     if (t < n && !alive[k]) D.bp_status[gb[k]] = 0;
   }

@@ -81,8 +81,10 @@ __device__ __forceinline__ int bp_eval_one(const Params &P, const Ptrs &D, int g
 template <typename T, int LPB, int B>
 __global__ void __launch_bounds__(B) k_persistent(Params P, Ptrs D, PersArgs A) {
   __shared__ double s1[B / LPB], s2[B / LPB];
-  __shared__ int s_exit;
+  __shared__ int s_exit, s_big;
   __shared__ int s_mcws[(B / 64) * MCW_LDS];   // wave MC scratch, one slice per wave
+  static_assert((B / 64) * MCW_LDS >= MCB_LDS, "large-component MC needs the block's whole scratch");
+  if (threadIdx.x == 0) s_big = 0;
   const int N = P.N;
   const unsigned G = gridDim.x;
   const int gtid = blockIdx.x * B + threadIdx.x, gthreads = G * B;
@@ -154,10 +156,39 @@ __global__ void __launch_bounds__(B) k_persistent(Params P, Ptrs D, PersArgs A) 
             if ((threadIdx.x & 63) == 0) D.flags[5] = 1;
             continue;
           }
+          if (n > 64 * MCW_ITEMS) { s_big = 1; continue; }
           int *sg = A.seg + (size_t)root * SEGCAP;
           if (!mc_component_wave(P, D, 0, 0, root, sg, n, beta, A.seed, step, s_mcws + (threadIdx.x / 64) * MCW_LDS)) {
             if ((threadIdx.x & 63) == 0) mc_component_serial(mc_arrays(D), P.N, 0, 0, root, sg, n, beta, A.seed, step);
           }
+        }
+        // components above one wave's scratch: wave 0 of the block takes
+        // those its waves met, with the whole block's scratch
+        __syncthreads();
+        if (s_big) {
+          if (threadIdx.x < 64) {
+            const int L = threadIdx.x;
+            for (int lw = 0; lw < B / 64; lw++) {
+              const int w0 = blockIdx.x * (B / 64) + lw;
+              for (int k0 = 0; w0 + k0 * nwaves < N; k0 += 64) {
+                const int root = w0 + (k0 + L) * nwaves;
+                bool big = false;
+                if (root < N) {
+                  const int n = A.seg_n[root];
+                  big = A.has_exc[root] && n > 64 * MCW_ITEMS && n <= SEGCAP;
+                }
+                for (unsigned long long m = __ballot(big); m; m &= m - 1) {
+                  const int rt = bcast(root, __ffsll((long long)m) - 1);
+                  const int n = A.seg_n[rt];
+                  int *sg = A.seg + (size_t)rt * SEGCAP;
+                  if (!mc_component_wave<MCB_ITEMS, MCB_NODES>(P, D, 0, 0, rt, sg, n, beta, A.seed, step, s_mcws))
+                    if (L == 0) mc_component_serial(mc_arrays(D), P.N, 0, 0, rt, sg, n, beta, A.seed, step);
+                }
+              }
+            }
+          }
+          __syncthreads();
+          if (threadIdx.x == 0) s_big = 0;
         }
       }
       grid_barrier(A.bar, target, G);
@@ -463,8 +494,9 @@ __device__ __forceinline__ void batched_body(const Params &P, const Ptrs &D, con
   __shared__ double4 s_tile[B];
   __shared__ int s_flag, s_exit;
   __shared__ int s_exc[EXC_LDS];   // component roots with excess (MC steps)
-  __shared__ int s_nexc;
+  __shared__ int s_nexc, s_big;
   __shared__ int s_mcws[(B / 64) * MCW_LDS];   // wave MC scratch, one slice per wave
+  static_assert((B / 64) * MCW_LDS >= MCB_LDS, "large-component MC needs the block's whole scratch");
   const int N = P.N;
   const int tid = threadIdx.x;
   const size_t base = (size_t)r * N;
@@ -520,7 +552,7 @@ __device__ __forceinline__ void batched_body(const Params &P, const Ptrs &D, con
       F = group_sum_rt(F, lpb);
       if (a < N && l == 0) D.fbond[base + a] = double4{F.x, F.y, F.z, 0.0};
     }
-    if (tid == 0) s_nexc = 0;
+    if (tid == 0) { s_nexc = 0; s_big = 0; }
     __syncthreads();
     if (mc) {
       // ---- B: formed pairs bucketed by component root; roots with
@@ -564,11 +596,38 @@ __device__ __forceinline__ void batched_body(const Params &P, const Ptrs &D, con
         const int n = J.seg_n[gr];
         if (n <= MCL_PAIRS) continue;
         if (n > SEGCAP) { if ((tid & 63) == 0) D.flags[5] = 1; continue; }
+        if (n > 64 * MCW_ITEMS) { s_big = 1; continue; }
         int *sg = J.seg + gr * SEGCAP;
         if (!mc_component_wave(P, D, r, J.rk, root, sg, n, beta, J.seed, step, s_mcws + wave * MCW_LDS))
           if ((tid & 63) == 0) mc_component_serial(mc_arrays(D), P.N, r, J.rk, root, sg, n, beta, J.seed, step);
       }
       __syncthreads();
+      // components above one wave's scratch: wave 0 with the whole block's
+      if (s_big) {
+        if (wave == 0) {
+          const int cnt = listed ? nexc : N;
+          for (int i0 = 0; i0 < cnt; i0 += 64) {
+            const int it = i0 + (tid & 63);
+            int root = 0;
+            bool big = false;
+            if (it < cnt) {
+              root = listed ? s_exc[it] : it;
+              const size_t gr = base + root;
+              const int n = J.seg_n[gr];
+              big = J.has_exc[gr] && n > 64 * MCW_ITEMS && n <= SEGCAP;
+            }
+            for (unsigned long long m = __ballot(big); m; m &= m - 1) {
+              const int rt = bcast(root, __ffsll((long long)m) - 1);
+              const size_t gr = base + rt;
+              const int n = J.seg_n[gr];
+              int *sg = J.seg + gr * SEGCAP;
+              if (!mc_component_wave<MCB_ITEMS, MCB_NODES>(P, D, r, J.rk, rt, sg, n, beta, J.seed, step, s_mcws))
+                if ((tid & 63) == 0) mc_component_serial(mc_arrays(D), P.N, r, J.rk, rt, sg, n, beta, J.seed, step);
+            }
+          }
+        }
+        __syncthreads();
+      }
       // the MC result is what later non-MC steps evaluate
       for (int k = tid; k < nbr; k += B) D.bp_status_mc[r * J.capb + k] = D.bp_status[r * J.capb + k];
     }

@@ -3,13 +3,18 @@
 // submitted as they arrive.  A scheduler thread admits them in arrival
 // order while their HBM footprint fits the budget, keeps each job's state,
 // lists and random-stream position resident in its own single-replica
-// context, and advances every resident job in shared launches of k_jobs
-// (one workgroup per job, one launch per precision class).  Each round
-// gives a job about `quantum` bead-steps, so a round takes about the same
-// time for every job size and new arrivals join at the next round.  A
-// job's trajectory depends only on its own inputs: its RNG streams are
-// keyed by its seed, its step chunks by its own size, and rounds continue
-// it exactly (lists, pending rebuilds and MC state carry over).
+// context, and advances every resident job each round:
+//   - jobs of up to POOL_WG_MAX beads in shared launches of k_jobs (one
+//     workgroup per job, one launch per precision class);
+//   - larger jobs, which one compute unit would serve too slowly, on the
+//     per-phase path over the whole GPU, several at once from worker
+//     threads, overlapping the shared launches.
+// Each round gives a job about `quantum` bead-steps, so a round takes
+// about the same time for every job size and new arrivals join at the
+// next round.  A job's trajectory depends only on its own inputs: its path
+// follows from its size, its RNG streams are keyed by its seed, its step
+// chunks by its size, and rounds continue it exactly (lists, pending
+// rebuilds and MC state carry over).
 // Included by sis_gpu.hip after the batched form's driver.
 #pragma once
 #include <chrono>
@@ -21,6 +26,9 @@
 #include <unordered_map>
 
 namespace {
+
+constexpr int POOL_WG_MAX = 2048;     // largest job run by one workgroup
+constexpr int POOL_LARGE_PAR = 8;     // large jobs stepped at once
 
 struct PoolJob {
   int64_t id = 0;
@@ -200,23 +208,27 @@ JobView job_view(sis_pool *p, PoolJob &j) {
 // one round: every resident job advances by its chunk
 void pool_round(sis_pool *p) {
   std::vector<JobView> views[2];
-  std::vector<PoolJob *> members[2];
+  std::vector<PoolJob *> members[2], large;
   for (auto &jp : p->active) {
     PoolJob &j = *jp;
     j.todo = std::min<int64_t>(j.nsteps - j.done, std::max<int64_t>(1, p->quantum / std::max(j.nmp, 1)));
+    if (j.nmp > POOL_WG_MAX) {
+      large.push_back(&j);
+      continue;
+    }
     const int k = j.precision == 1 ? 1 : 0;
     views[k].push_back(job_view(p, j));
     members[k].push_back(&j);
   }
   HIPCHK(hipMemsetAsync(p->flags_d.p, 0, p->flags_d.n * sizeof(int), p->stream[0]));
   HIPCHK(hipStreamSynchronize(p->stream[0]));
-  const Ptrs D0 = p->active.front()->c->ptrs();   // the shared base-pair tables
   auto t0 = std::chrono::steady_clock::now();
   int nl = 0;
   for (int k = 0; k < 2; k++) {
     if (views[k].empty()) continue;
     HIPCHK(hipMemcpyAsync(p->views_d[k].p, views[k].data(), views[k].size() * sizeof(JobView),
                           hipMemcpyHostToDevice, p->stream[k]));
+    const Ptrs D0 = members[k].front()->c->ptrs();   // the shared base-pair tables
     if (k == 1)
       hipLaunchKernelGGL((k_jobs<float, 256>), dim3((unsigned)views[k].size()), dim3(256), 0, p->stream[k], p->P0, D0,
                          (const JobView *)p->views_d[k].p);
@@ -224,6 +236,20 @@ void pool_round(sis_pool *p) {
       hipLaunchKernelGGL((k_jobs<double, 256>), dim3((unsigned)views[k].size()), dim3(256), 0, p->stream[k], p->P0,
                          D0, (const JobView *)p->views_d[k].p);
     HIPCHK(hipGetLastError());
+    nl++;
+  }
+  // large jobs: whole-GPU per-phase steps, POOL_LARGE_PAR at a time, while
+  // the shared launches run
+  std::vector<std::string> lerr(large.size());
+  std::vector<int64_t> lreb(large.size(), 0);
+  for (size_t b = 0; b < large.size(); b += POOL_LARGE_PAR) {
+    std::vector<std::thread> th;
+    for (size_t i = b; i < std::min(large.size(), b + POOL_LARGE_PAR); i++)
+      th.emplace_back([p, &large, &lerr, &lreb, i]() {
+        if (hipSetDevice(p->device) != hipSuccess) { lerr[i] = "hipSetDevice failed"; return; }
+        if (sis_gpu_step(large[i]->c, large[i]->todo, &lreb[i]) != SIS_OK) lerr[i] = sis_gpu_last_error();
+      });
+    for (auto &t : th) t.join();
     nl++;
   }
   for (int k = 0; k < 2; k++) HIPCHK(hipStreamSynchronize(p->stream[k]));
@@ -261,6 +287,13 @@ void pool_round(sis_pool *p) {
         }
       }
     }
+  for (size_t i = 0; i < large.size(); i++) {
+    PoolJob &j = *large[i];
+    if (!lerr[i].empty()) { job_fail(p, j, lerr[i]); continue; }
+    j.done += j.todo;
+    j.rebuilds += lreb[i];
+    bs += (double)j.todo * j.nmp;
+  }
   std::lock_guard<std::mutex> lk(p->mu);
   p->bead_steps += bs;
   p->gpu_s += dt;

@@ -252,6 +252,63 @@ def test_batched_form_matches_phase_form(mc_every):
         assert np.abs(a - b).max() < 1e-6
 
 
+def _largest_formed_component(eng, irep=1):
+    """Pairs in the largest connected set of candidates with negative
+    energy (formed before the limit rule; pairs sharing a nucleotide are
+    connected), from the engine's last evaluation."""
+    b = eng.bp(irep)
+    mp = b["bp"][b["ene"] < 0][:, :2]
+    par = {}
+
+    def find(x):
+        while par.setdefault(x, x) != x:
+            par[x] = par[par[x]]
+            x = par[x]
+        return x
+    for a, c in mp:
+        ra, rc = find(int(a)), find(int(c))
+        if ra != rc:
+            par[ra] = rc
+    cnt = {}
+    for a, _ in mp:
+        r = find(int(a))
+        cnt[r] = cnt.get(r, 0) + 1
+    return max(cnt.values()) if cnt else 0
+
+
+def _limit_holds(eng, irep=1):
+    """After the limit MC no nucleotide has more formed pairs than allowed."""
+    b = eng.bp(irep)
+    ends = b["bp"][b["status"] > 0][:, :2].ravel()
+    return ends.size == 0 or np.bincount(ends).max() <= eng.sysd["params"].get("max_bp_per_nt", 1)
+
+
+@pytest.mark.parametrize("form", ["persistent", "batched"])
+def test_large_components_forms_match_phases(form):
+    """Limit MC on components above one wavefront's capacity (more than 128
+    formed pairs; a 1,024-nt chain forms components of about 150): the
+    large form (one wavefront with the workgroup's whole scratch) in the
+    persistent and batched kernels against the per-phase form."""
+    nrep = 1 if form == "persistent" else 4
+    s = configs.remd_c3(nrep=nrep, n=1024, seed=3)
+    print(f"[case] large components {form}", flush=True)
+    res = []
+    for mode in ("phases", form):
+        eng = Engine(s, parity=False, mode=mode, seed=5, exchange_every=0 if nrep == 1 else 10)
+        eng.init_velocities()
+        eng.neighbor_list(0)
+        eng.step(60)
+        assert eng.info()["mode"] == {"phases": 1, "persistent": 2, "batched": 3}[mode]
+        big = max(_largest_formed_component(eng, k + 1) for k in range(nrep))
+        assert 128 < big <= 256, big
+        assert all(_limit_holds(eng, k + 1) for k in range(nrep))
+        res.append((eng.get_state(), eng.labels()))
+    assert np.array_equal(res[0][1], res[1][1])
+    tol = 1e-9 if form == "persistent" else 1e-6
+    for a, b in zip(res[0][0], res[1][0]):
+        assert np.abs(a - b).max() < tol
+
+
 def test_batched_dynamics_stable_fp32():
     s = configs.remd_c3(nrep=32, n=128, seed=4)
     print("[case] batched fp32", flush=True)
