@@ -263,7 +263,7 @@ def main():
     else:
         sysd = configs.remd_c3()
         wname = "C3: 256 T-replicas x 128 nt (50% GC), 290-380 K, non-periodic, exchange every 1,000 steps"
-        steps, warm, prec, cpu_steps = 20000, 2000, 1, 2000
+        steps, warm, prec, cpu_steps = 20000, 2000, 1, 5000
     steps = args.steps if args.steps is not None else steps
     warm = args.warmup if args.warmup is not None else warm
     prec = args.precision if args.precision is not None else prec
