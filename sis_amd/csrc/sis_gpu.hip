@@ -137,6 +137,7 @@ struct sis_gpu_ctx {
   DevBuf<double> blk_top2_d;
   DevBuf<int> rebuild_flag_d, mask_d, mc_d, scan_bsum_d, scan_total_d;
   DevBuf<int> exc_cnt_d, exc_off_d, exc_fill_d, exc_list_d, ntlist_d, bpseq_d, seg_d;
+  DevBuf<int> huge_d;   // per replica: a conflict component above the bucket capacity this step
   DevBuf<uint32_t> tape_d;
   DevBuf<int64_t> tape_off_d, tape_pos_d;
   DevBuf<double> noise_d, bm_cache_d, ene_part_d, ene_out_d, ecT_d, eCoef_d, eLam_d;
@@ -195,7 +196,7 @@ struct sis_gpu_ctx {
                          &comp_d, &flags_d, &cell_of_d, &cell_cnt_d, &cell_start_d,
                          &cell_beads_d, &cell_fill_d, &rebuild_flag_d, &mask_d, &mc_d,
                          &scan_bsum_d, &scan_total_d, &exc_cnt_d, &exc_off_d, &exc_fill_d,
-                         &exc_list_d, &ntlist_d, &bpseq_d, &seg_d, &pers_sync_d, &chain_id_d,
+                         &exc_list_d, &ntlist_d, &bpseq_d, &seg_d, &huge_d, &pers_sync_d, &chain_id_d,
                          &bp_val_d, &bp_val2_d, &bp_mp2_d};
     for (auto *b : ib) b->free_();
     DevBuf<double> *db[] = {&bp3_dH_d, &bp3_dS_d, &md_a_d, &md_cf_d, &md_cx_d, &mass_d,
@@ -568,7 +569,9 @@ void bp_phase(sis_gpu_ctx *c, const std::vector<int> &mc, const double *ecT_dev,
     hipLaunchKernelGGL(k_seg_fill, dim3((c->nbp_total + B - 1) / B), dim3(B), 0, c->stream, c->P, D,
                        c->nbp_total, c->exc_cnt_d.p, c->seg_d.p, c->exc_fill_d.p, c->mc_d.p, mask_dev);
     hipLaunchKernelGGL(k_mc_wave, dim3(N, R), dim3(64), 0, c->stream, c->P, D, c->exc_cnt_d.p, c->seg_d.p,
-                       c->exc_fill_d.p, c->mc_d.p, ecT_dev, c->seed, c->istep, mask_dev);
+                       c->exc_fill_d.p, c->mc_d.p, ecT_dev, c->seed, c->istep, mask_dev, c->huge_d.p);
+    hipLaunchKernelGGL(k_mc_huge, dim3(R), dim3(64), 0, c->stream, c->P, D, c->exc_cnt_d.p, c->seg_d.p,
+                       c->exc_fill_d.p, c->huge_d.p, ecT_dev, c->seed, c->istep);
   }
 }
 
@@ -593,7 +596,7 @@ void check_flags(sis_gpu_ctx *c) {
   HIPCHK(hipStreamSynchronize(c->stream));
   if (f[7]) throw ApiError{SIS_ERR_STATE, "limit MC met a base-pair candidate outside the system"};
   if (f[1] == 2) throw ApiError{SIS_ERR_STATE, "limit MC found an excess nucleotide without pairs"};
-  if (f[5]) throw ApiError{SIS_ERR_CAPACITY, "a base-pair conflict component exceeds 256 formed pairs"};
+  if (f[5]) throw ApiError{SIS_ERR_CAPACITY, "a base-pair conflict component outgrew the limit-MC scratch"};
   if (f[1]) throw ApiError{SIS_ERR_STATE, "parity random-number tape exhausted"};
 }
 
@@ -942,7 +945,7 @@ int sis_gpu_create(const sis_topo *topo, const sis_params *prm, const sis_option
     c->n_bp_d.alloc(R); c->n_inc_d.alloc(RN); c->nt_cnt_d.alloc(RN); c->comp_d.alloc(RN);
     c->flags_d.alloc(8);
     c->blk_top2_d.alloc((size_t)R * ((N + 255) / 256 + 1) * 2);
-    c->rebuild_flag_d.alloc(R); c->mask_d.alloc(R); c->mc_d.alloc(R);
+    c->rebuild_flag_d.alloc(R); c->mask_d.alloc(R); c->mc_d.alloc(R); c->huge_d.alloc(R);
     c->scan_total_d.alloc(1);
     c->exc_cnt_d.alloc(RN); c->exc_off_d.alloc(RN + 1); c->exc_fill_d.alloc(RN); c->exc_list_d.alloc(RN);
     c->ntlist_d.alloc((size_t)R * (N + 1));

@@ -698,10 +698,218 @@ __device__ __noinline__ void mc_component_serial(const McArrays D, const int N, 
 __device__ __forceinline__ McArrays mc_arrays(const Ptrs &D) {
   return McArrays{D.bp_mp, D.bp_status, D.ene_bp, D.nt_cnt, D.flags};
 }
+
+// Scratch (ints) that mc_component_huge needs for a replica with nc
+// candidates and N beads.
+__device__ __forceinline__ size_t mc_huge_ints(int nc, int N) {
+  const size_t pc = (size_t)((nc + 1) & ~1);
+  const size_t nw = (size_t)(N + 31) / 32;
+  return 8 * pc + ((nw + (size_t)N + 1) & ~(size_t)1) + 2 * (size_t)((N + 63) / 64);
+}
+
+// A component of any size (those above the bucket capacity SEGCAP): one
+// wavefront, the state of the sweep in global scratch.  The sweep is that
+// of mc_component_wave (an excess nucleotide picked uniformly in ascending
+// id order, its formed pairs in list order, the reference's shuffle and
+// Metropolis sweep, one deletion per iteration), on its own draws:
+//   - the component's formed pairs come from a scan of the replica's
+//     candidates [g0, g1) in order (list order kept), keeping those whose
+//     union-find root is `root`;
+//   - its nucleotides are listed in ascending id order through a bitmap
+//     over bead ids;
+//   - active nucleotides are 64-bit masks, updated on each deletion;
+//   - each iteration's sequence is a ballot scan over the pairs, staged
+//     through sseq (64 ints of LDS).
+// comp is the replica's union-find array; scr holds cap ints.  Returns
+// false if cap is too small.  Must be called by all 64 lanes of a wave
+// with uniform arguments.
+__device__ __noinline__ bool mc_component_huge(const McArrays D, const int N, int r, int rk, int root, int *comp,
+                                               int g0, int g1, double beta, uint64_t seed, int64_t step, int *scr,
+                                               size_t cap, int *sseq) {
+  const int L = lane_id();
+  root = __builtin_amdgcn_readfirstlane(root);
+  const int nc = g1 - g0;
+  if (mc_huge_ints(nc, N) > cap) return false;
+  const size_t pc = (size_t)((nc + 1) & ~1);
+  const int nw = (N + 31) / 32;
+  int *hp = scr, *ha = hp + pc, *hb = ha + pc, *hua = hb + pc, *hub = hua + pc, *hal = hub + pc;
+  double *he = reinterpret_cast<double *>(hal + pc);
+  uint32_t *bm = reinterpret_cast<uint32_t *>(he + pc);
+  int *hn = reinterpret_cast<int *>(bm + nw);
+  unsigned long long *am = reinterpret_cast<unsigned long long *>(hn + ((nw + N + 1) & ~1) - nw);
+  int *cnt = D.nt_cnt + (size_t)r * N;
+  // ---- the component's formed pairs, in candidate order
+  int n = 0;
+  bool bad = false;
+  for (int g = g0; g < g1; g += 64) {
+    const int gb = g + L;
+    bool in = false;
+    int a = 0, b = 0;
+    if (gb < g1 && D.bp_status[gb]) {
+      const int *mp = D.bp_mp + (size_t)gb * 8;
+      a = mp[0] - 1;
+      b = mp[1] - 1;
+      if (a < 0 || a >= N || b < 0 || b >= N) bad = true;
+      else in = uf_find(comp, a) == root;
+    }
+    const unsigned long long m = __ballot(in);
+    if (in) {
+      const int t = n + below(m);
+      hp[t] = gb; ha[t] = a; hb[t] = b; hua[t] = 0; hub[t] = 0; hal[t] = 1;
+      he[t] = D.ene_bp[gb];
+    }
+    n += __popcll(m);
+  }
+  if (__ballot(bad)) {
+    if (L == 0) D.flags[7] = 1;
+    return true;
+  }
+  // ---- its nucleotides in ascending id order
+  for (int w = L; w < nw; w += 64) bm[w] = 0u;
+  __threadfence();
+  for (int t = L; t < n; t += 64) {
+    atomicOr(&bm[ha[t] >> 5], 1u << (ha[t] & 31));
+    atomicOr(&bm[hb[t] >> 5], 1u << (hb[t] & 31));
+  }
+  __threadfence();
+  int U = 0;
+  for (int w0 = 0; w0 < nw; w0 += 64) {
+    const int w = w0 + L;
+    const uint32_t bits = w < nw ? bm[w] : 0u;
+    const int c = __popc(bits), inc = wave_incl_scan(c);
+    int k = U + inc - c;
+    for (uint32_t x = bits; x; x &= x - 1) hn[k++] = 32 * w + __ffs(x) - 1;
+    U += bcast(inc, 63);
+  }
+  __threadfence();
+  // node index of each endpoint, then the active masks
+  for (int t = L; t < n; t += 64) {
+    int v = ha[t], lo = 0, hi = U;
+    while (lo < hi) { const int mid = (lo + hi) >> 1; if (hn[mid] < v) lo = mid + 1; else hi = mid; }
+    hua[t] = lo;
+    v = hb[t]; lo = 0; hi = U;
+    while (lo < hi) { const int mid = (lo + hi) >> 1; if (hn[mid] < v) lo = mid + 1; else hi = mid; }
+    hub[t] = lo;
+  }
+  const int nuw = (U + 63) / 64;
+  for (int u0 = 0; u0 < U; u0 += 64) {
+    const int u = u0 + L;
+    const unsigned long long m = __ballot(u < U && cnt[hn[u < U ? u : 0]] > 0);
+    if (L == 0) am[u0 >> 6] = m;
+  }
+  __threadfence();
+  // ---- the sweep: one deletion per iteration
+  uint32_t draw = 0;
+  for (int iter = 0; iter <= n; iter++) {
+    int nact = 0;
+    for (int w0 = 0; w0 < nuw; w0 += 64) {
+      const int w = w0 + L;
+      nact += bcast(wave_incl_scan(w < nuw ? __popcll(am[w]) : 0), 63);
+    }
+    if (nact == 0) break;
+    const int pick = __builtin_amdgcn_readfirstlane((int)ceil(philox_u(seed, rk, step, root, draw) * nact));
+    int ntu = -1, acc = 0;
+    for (int w0 = 0; w0 < nuw && ntu < 0; w0 += 64) {
+      const int w = w0 + L;
+      const unsigned long long m = w < nuw ? am[w] : 0ull;
+      const int c = __popcll(m), inc = wave_incl_scan(c), tot = bcast(inc, 63);
+      if (acc + tot >= pick) {
+        const int want = pick - acc - 1;
+        const unsigned long long sel = __ballot(c > 0 && want >= inc - c && want < inc);
+        const int ln = __ffsll((long long)sel) - 1;
+        if (ln < 0) break;
+        const unsigned long long mw = bcast64(m, ln);
+        const int k = want - (bcast(inc, ln) - bcast(c, ln));
+        const unsigned long long hit = __ballot(((mw >> L) & 1ull) && below(mw) == k);
+        ntu = 64 * (w0 + ln) + __ffsll((long long)hit) - 1;
+      }
+      acc += tot;
+    }
+    ntu = __builtin_amdgcn_readfirstlane(ntu);
+    if (ntu < 0 || ntu >= U) {   // masks and counts disagree: corrupted state
+      if (L == 0) D.flags[7] = 1;
+      return true;
+    }
+    const int ntd = __builtin_amdgcn_readfirstlane(hn[ntu]);
+    // formed pairs of ntd in list order (the first 64)
+    int ns = 0;
+    for (int t0 = 0; t0 < n && ns < 64; t0 += 64) {
+      const int t = t0 + L;
+      const bool hit = t < n && hal[t] && (ha[t] == ntd || hb[t] == ntd);
+      const unsigned long long m = __ballot(hit);
+      if (hit) {
+        const int k = ns + below(m);
+        if (k < 64) sseq[k] = t;
+      }
+      ns += __popcll(m);
+    }
+    if (ns == 0) {
+      if (L == 0) D.flags[1] = 2;
+      return true;
+    }
+    if (ns > 64) ns = 64;
+    wave_lds_fence();
+    int seqv = L < ns ? sseq[L] : 0;
+    // draws after the pick: shuffle 1..ns, Metropolis ns+1..2ns-1
+    const int swl = L < ns ? (int)ceil(philox_u(seed, rk, step, root, draw + 1u + (uint32_t)L) * ns) - 1 : 0;
+    const double lgl = (L >= 1 && L < ns) ? log(philox_u(seed, rk, step, root, draw + (uint32_t)(ns + L))) : 0.0;
+    for (int i = 0; i < ns; i++) {
+      const int sw = bcast(swl, i);
+      const int ti = bcast(seqv, i), ts = bcast(seqv, sw);
+      seqv = setlane(seqv, ts, i);
+      seqv = setlane(seqv, ti, sw);
+    }
+    const double ev = L < ns ? he[seqv] : 0.0;
+    int del = bcast(seqv, 0);
+    double edel = bcast(ev, 0);
+    for (int i = 1; i < ns; i++) {
+      const double ej = bcast(ev, i);
+      if (bcast(lgl, i) < (ej - edel) * beta) { del = bcast(seqv, i); edel = ej; }
+    }
+    draw += 2u * (uint32_t)ns;
+    if (L == 0) {
+      hal[del] = 0;
+      D.bp_status[hp[del]] = 0;
+      if (atomicSub(&cnt[ha[del]], 1) <= 1) atomicAnd(&am[hua[del] >> 6], ~(1ull << (hua[del] & 63)));
+      if (atomicSub(&cnt[hb[del]], 1) <= 1) atomicAnd(&am[hub[del] >> 6], ~(1ull << (hub[del] & 63)));
+    }
+    __threadfence();
+  }
+  return true;
+}
+
+// Per-phase form: components above SEGCAP pairs, one wavefront per
+// replica after k_mc_wave (which marks the replicas that have them).
+__global__ void __launch_bounds__(64) k_mc_huge(Params P, Ptrs D, const int *seg_n, int *seg, const int *has_exc,
+                                                int *huge, const double *ecT, uint64_t seed, int64_t step) {
+  const int r = blockIdx.x, L = threadIdx.x, N = P.N;
+  if (!huge[r]) return;
+  __shared__ int sseq[64];
+  const double tK = ecT ? ecT[r] : D.tK[r];
+  const double BOLTZ = 1.380649e-23 * ((1.0 / 4184.0) * 6.02214076e23);
+  const double beta = 1.0 / (BOLTZ * tK);
+  const int g0 = D.bp_off[(size_t)r * N], g1 = D.bp_off[(size_t)(r + 1) * N];
+  int *scr = seg + (size_t)r * N * SEGCAP;
+  for (int u0 = 0; u0 < N; u0 += 64) {
+    const int root = u0 + L;
+    bool h = false;
+    if (root < N) {
+      const size_t gr = (size_t)r * N + root;
+      h = has_exc[gr] && seg_n[gr] > SEGCAP;
+    }
+    for (unsigned long long m = __ballot(h); m; m &= m - 1) {
+      const int rt = bcast(root, __ffsll((long long)m) - 1);
+      if (!mc_component_huge(mc_arrays(D), N, r, r, rt, D.comp + (size_t)r * N, g0, g1, beta, seed, step, scr,
+                             (size_t)N * SEGCAP, sseq))
+        if (L == 0) D.flags[5] = 1;
+    }
+  }
+  if (L == 0) huge[r] = 0;
+}
 __global__ void __launch_bounds__(64) k_mc_wave(Params P, Ptrs D, int *seg_n, int *seg,
                                                 const int *has_exc, const int *mc,
                                                 const double *ecT, uint64_t seed, int64_t step,
-                                                const int *mask) {
+                                                const int *mask, int *huge) {
   int root = blockIdx.x, r = blockIdx.y;
   if ((mask && !mask[r]) || !mc[r]) return;
   size_t gr = (size_t)r * P.N + root;
@@ -710,8 +918,8 @@ __global__ void __launch_bounds__(64) k_mc_wave(Params P, Ptrs D, int *seg_n, in
   double tK = ecT ? ecT[r] : D.tK[r];
   const double BOLTZ = 1.380649e-23 * ((1.0 / 4184.0) * 6.02214076e23);
   double beta = 1.0 / (BOLTZ * tK);
-  if (n > SEGCAP) {
-    if (threadIdx.x == 0) D.flags[5] = 1;   // bucket overflow: reported to the host
+  if (n > SEGCAP) {   // above the bucket capacity: k_mc_huge takes it
+    if (threadIdx.x == 0) huge[r] = 1;
     return;
   }
   int *sg = seg + gr * SEGCAP;

@@ -42,6 +42,21 @@ __device__ __forceinline__ double bcast(double v, int lane) {
   return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned int)lo);
 }
 __device__ __forceinline__ int setlane(int v, int x, int lane) { return __lane_id() == lane ? x : v; }
+__device__ __forceinline__ unsigned long long bcast64(unsigned long long v, int lane) {
+  const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)v, lane);
+  const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(v >> 32), lane);
+  return ((unsigned long long)hi << 32) | lo;
+}
+// Inclusive prefix sum over the 64 lanes.
+__device__ __forceinline__ int wave_incl_scan(int v) {
+  const int L = __lane_id();
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const int t = __shfl_up(v, d);
+    if (L >= d) v += t;
+  }
+  return v;
+}
 
 // Set bits of m below this lane (prefix population count).
 __device__ __forceinline__ int below(unsigned long long m) {
@@ -65,6 +80,7 @@ __device__ __forceinline__ bool mc_component_wave(const Params &P, const Ptrs &D
                                                   int64_t step, int *ws) {
   constexpr int NONE = 0x7fffffff;
   constexpr int NP = 64 * IT, NE = 2 * NP;
+  static_assert(NP <= 256, "pair slots are packed in 8 bits");
   const int L = lane_id();
   const int N = P.N;
   // uniform by contract; said so to the compiler, so that loop counters
@@ -224,42 +240,17 @@ __device__ __forceinline__ bool mc_component_wave(const Params &P, const Ptrs &D
         acc += pc;
       }
     }
-    // formed pairs of ntd in list order: entry m of the sequence in lane m
+    // formed pairs of ntd in list order
     unsigned long long sm[IT];
     int ns = 0;
 #pragma unroll
     for (int k = 0; k < IT; k++) {
       sm[k] = (k < nchunk_items) ? __ballot(alive[k] && (ia[k] == ntd || ib[k] == ntd)) : 0ull;
-      const int m = ns + below(sm[k]);
-      if (((sm[k] >> L) & 1ull) && m < 64) sseq[m] = 64 * k + L;
       ns += __popcll(sm[k]);
     }
     if (ns == 0) return false;
     if (ns > 64) ns = 64;
-    wave_lds_fence();
-    int seqv = L < ns ? sseq[L] : 0;
-    // draws used: 1 (pick) + ns (shuffle) + ns - 1 (Metropolis) = 2 ns;
-    // a window holding all of them starts at this iteration's pick draw
-    const bool fits = 2 * ns <= 64;
-    if (fits && draw + 2u * (uint32_t)ns > w0 + 64u) {
-      w0 = draw;
-      wu = philox_u(seed, rk, step, root, w0 + (uint32_t)L);
-      wl = log(wu);
-    }
-    const int o = (int)(draw - w0);
-    // u and log u of the draw `k` places after the pick draw
-    auto du = [&](int k) { return fits ? bcast(wu, o + k) : philox_u(seed, rk, step, root, draw + (uint32_t)k); };
-    auto dlog = [&](int k) { return fits ? bcast(wl, o + k) : log(philox_u(seed, rk, step, root, draw + (uint32_t)k)); };
-    // swap targets of the shuffle, lane i holding the i-th
-    const int swl = fits ? (int)ceil(__shfl(wu, (o + 1 + L) & 63) * ns) - 1 : 0;
-    // reference shuffle: for i = 1..n swap(i, ceil(rnd*n))
-    for (int i = 0; i < ns; i++) {
-      const int sw = fits ? bcast(swl, i) : (int)ceil(du(1 + i) * ns) - 1;
-      const int ti = bcast(seqv, i), ts = bcast(seqv, sw);
-      seqv = setlane(seqv, ts, i);
-      seqv = setlane(seqv, ti, sw);
-    }
-    // Metropolis sweep on the energies of the sequence's pairs
+    // energy of pair slot `slot` (wave-uniform)
     auto ene_of = [&](int slot) {
       const int k = slot >> 6;
       double x = en[0];
@@ -267,12 +258,79 @@ __device__ __forceinline__ bool mc_component_wave(const Params &P, const Ptrs &D
       for (int kk = 1; kk < IT; kk++) x = (kk == k) ? en[kk] : x;
       return bcast(x, slot & 63);
     };
-    int del = bcast(seqv, 0);
-    double edel = ene_of(del);
-    for (int i = 1; i < ns; i++) {
-      const int si = bcast(seqv, i);
-      const double ej = ene_of(si);
-      if (dlog(ns + i) < (ej - edel) * beta) { del = si; edel = ej; }
+    int del;
+    double edel;
+    if (ns <= 8) {
+      // short sequence (the usual case): kept in scalar registers, entry m
+      // in bits 8m..8m+7 (slots < 256); the window holds the 2 ns draws
+      uint64_t sp = 0;
+      int m = 0;
+#pragma unroll
+      for (int k = 0; k < IT; k++)
+        for (unsigned long long mk = sm[k]; mk; mk &= mk - 1) {
+          sp |= (uint64_t)(64 * k + __builtin_ctzll(mk)) << (8 * m);
+          m++;
+        }
+      if (draw + 2u * (uint32_t)ns > w0 + 64u) {
+        w0 = draw;
+        wu = philox_u(seed, rk, step, root, w0 + (uint32_t)L);
+        wl = log(wu);
+      }
+      const int o = (int)(draw - w0);
+      // swap target of a shuffle draw held by this lane
+      const int swv = (int)ceil(wu * ns) - 1;
+      // reference shuffle: for i = 1..n swap(i, ceil(rnd*n))
+      for (int i = 0; i < ns; i++) {
+        const int sw = bcast(swv, o + 1 + i);
+        const uint64_t ti = (sp >> (8 * i)) & 0xffull, ts = (sp >> (8 * sw)) & 0xffull;
+        sp = (sp & ~(0xffull << (8 * i))) | (ts << (8 * i));
+        sp = (sp & ~(0xffull << (8 * sw))) | (ti << (8 * sw));
+      }
+      // Metropolis sweep on the energies of the sequence's pairs
+      del = (int)(sp & 0xffull);
+      edel = ene_of(del);
+      for (int i = 1; i < ns; i++) {
+        const int si = (int)((sp >> (8 * i)) & 0xffull);
+        const double ej = ene_of(si);
+        if (bcast(wl, o + ns + i) < (ej - edel) * beta) { del = si; edel = ej; }
+      }
+    } else {
+      // long sequence: entry m in lane m (through LDS)
+#pragma unroll
+      for (int k = 0, m0 = 0; k < IT; k++) {
+        const int m = m0 + below(sm[k]);
+        if (((sm[k] >> L) & 1ull) && m < 64) sseq[m] = 64 * k + L;
+        m0 += __popcll(sm[k]);
+      }
+      wave_lds_fence();
+      int seqv = L < ns ? sseq[L] : 0;
+      // draws used: 1 (pick) + ns (shuffle) + ns - 1 (Metropolis) = 2 ns;
+      // a window holding all of them starts at this iteration's pick draw
+      const bool fits = 2 * ns <= 64;
+      if (fits && draw + 2u * (uint32_t)ns > w0 + 64u) {
+        w0 = draw;
+        wu = philox_u(seed, rk, step, root, w0 + (uint32_t)L);
+        wl = log(wu);
+      }
+      const int o = (int)(draw - w0);
+      // u and log u of the draw `k` places after the pick draw
+      auto du = [&](int k) { return fits ? bcast(wu, o + k) : philox_u(seed, rk, step, root, draw + (uint32_t)k); };
+      auto dlog = [&](int k) { return fits ? bcast(wl, o + k) : log(philox_u(seed, rk, step, root, draw + (uint32_t)k)); };
+      // swap targets of the shuffle, lane i holding the i-th
+      const int swl = fits ? (int)ceil(__shfl(wu, (o + 1 + L) & 63) * ns) - 1 : 0;
+      for (int i = 0; i < ns; i++) {
+        const int sw = fits ? bcast(swl, i) : (int)ceil(du(1 + i) * ns) - 1;
+        const int ti = bcast(seqv, i), ts = bcast(seqv, sw);
+        seqv = setlane(seqv, ts, i);
+        seqv = setlane(seqv, ti, sw);
+      }
+      del = bcast(seqv, 0);
+      edel = ene_of(del);
+      for (int i = 1; i < ns; i++) {
+        const int si = bcast(seqv, i);
+        const double ej = ene_of(si);
+        if (dlog(ns + i) < (ej - edel) * beta) { del = si; edel = ej; }
+      }
     }
     draw += 2u * (uint32_t)ns;
     // delete: pair slot del, decrement its two nucleotides

@@ -21,6 +21,7 @@ struct PersArgs {
   int *done;                // [0] steps completed, [1] rebuild requested, [2] buffer parity
   double4 *pos_b;           // second position buffer
   int nbp_total, force_mc_first;
+  int *huge;                // a conflict component above the bucket capacity this step
 };
 
 // Grid-wide barrier on a monotonic counter.  Agent-scope release before
@@ -152,8 +153,8 @@ __global__ void __launch_bounds__(B) k_persistent(Params P, Ptrs D, PersArgs A) 
           if (!A.has_exc[root]) continue;
           int n = A.seg_n[root];
           if (n <= MCL_PAIRS) continue;
-          if (n > SEGCAP) {
-            if ((threadIdx.x & 63) == 0) D.flags[5] = 1;
+          if (n > SEGCAP) {   // resolved after the phase by mc_component_huge
+            if ((threadIdx.x & 63) == 0) *A.huge = 1;
             continue;
           }
           if (n > 64 * MCW_ITEMS) { s_big = 1; continue; }
@@ -192,6 +193,26 @@ __global__ void __launch_bounds__(B) k_persistent(Params P, Ptrs D, PersArgs A) 
         }
       }
       grid_barrier(A.bar, target, G);
+      // components above the bucket capacity: wave 0 of block 0, then a
+      // barrier (every block reads the same flag: it is written before the
+      // barrier above and cleared after the one below)
+      if (__hip_atomic_load(A.huge, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+        if (blockIdx.x == 0 && threadIdx.x < 64) {
+          const int L = threadIdx.x;
+          for (int u0 = 0; u0 < N; u0 += 64) {
+            const int root = u0 + L;
+            const bool h = root < N && A.has_exc[root] && A.seg_n[root] > SEGCAP;
+            for (unsigned long long m = __ballot(h); m; m &= m - 1) {
+              const int rt = bcast(root, __ffsll((long long)m) - 1);
+              if (!mc_component_huge(mc_arrays(D), N, 0, 0, rt, D.comp, 0, A.nbp_total, beta, A.seed, step, A.seg,
+                                     (size_t)N * SEGCAP, s_mcws))
+                if (L == 0) D.flags[5] = 1;
+            }
+          }
+        }
+        grid_barrier(A.bar, target, G);
+        if (blockIdx.x == 0 && threadIdx.x == 0) *A.huge = 0;
+      }
     }
     // ---- D: forces and the update; reset per-step MC state.  The MC
     // result is what later non-MC steps evaluate (src/job_md.F90:388-395).
@@ -277,6 +298,7 @@ namespace sis {
 // pointing at that job's own arrays.  Candidate ids are r * capb + k.
 
 constexpr int EXC_LDS = 1024;   // listed roots with excess per system and step
+constexpr int MCW_LIST = 64;    // wavefront-MC components scheduled largest first per step
 
 // Per-system launch arguments.
 struct SysIO {
@@ -494,7 +516,8 @@ __device__ __forceinline__ void batched_body(const Params &P, const Ptrs &D, con
   __shared__ double4 s_tile[B];
   __shared__ int s_flag, s_exit;
   __shared__ int s_exc[EXC_LDS];   // component roots with excess (MC steps)
-  __shared__ int s_nexc, s_big;
+  __shared__ int s_nexc, s_big, s_nw, s_next, s_huge;
+  __shared__ int s_wl[MCW_LIST];   // wavefront-MC components of this step: size << 21 | root
   __shared__ int s_mcws[(B / 64) * MCW_LDS];   // wave MC scratch, one slice per wave
   static_assert((B / 64) * MCW_LDS >= MCB_LDS, "large-component MC needs the block's whole scratch");
   const int N = P.N;
@@ -552,7 +575,7 @@ __device__ __forceinline__ void batched_body(const Params &P, const Ptrs &D, con
       F = group_sum_rt(F, lpb);
       if (a < N && l == 0) D.fbond[base + a] = double4{F.x, F.y, F.z, 0.0};
     }
-    if (tid == 0) { s_nexc = 0; s_big = 0; }
+    if (tid == 0) { s_nexc = 0; s_big = 0; s_nw = 0; s_next = 0; s_huge = 0; }
     __syncthreads();
     if (mc) {
       // ---- B: formed pairs bucketed by component root; roots with
@@ -577,29 +600,62 @@ __device__ __forceinline__ void batched_body(const Params &P, const Ptrs &D, con
       const int wave = tid / 64, nwaves = B / 64;
       const int nexc = s_nexc;
       const bool listed = nexc <= EXC_LDS;
-      // small components: one lane each
+      // small components one lane each; the others are classified: the
+      // wavefront ones listed with their size, the large ones flagged
       for (int it = tid; it < (listed ? nexc : N); it += B) {
         const int root = listed ? s_exc[it] : it;
         const size_t gr = base + root;
         if (!J.has_exc[gr]) continue;
         const int n = J.seg_n[gr];
-        if (n > MCL_PAIRS) continue;
+        if (n > SEGCAP) { s_huge = 1; continue; }
+        if (n > 64 * MCW_ITEMS) { s_big = 1; continue; }
+        if (n > MCL_PAIRS) {
+          const int q = atomicAdd(&s_nw, 1);
+          if (q < MCW_LIST) s_wl[q] = (n << 21) | root;   // root < 2^21 (N <= 2048 here)
+          continue;
+        }
         int *sg = J.seg + gr * SEGCAP;
         if (!mc_component_lane(D, N, r, J.rk, root, sg, n, beta, J.seed, step))
           mc_component_serial(mc_arrays(D), P.N, r, J.rk, root, sg, n, beta, J.seed, step);
       }
-      // larger ones: one wavefront each
-      for (int it = wave; it < (listed ? nexc : N); it += nwaves) {
-        const int root = listed ? s_exc[it] : it;
-        const size_t gr = base + root;
-        if (!J.has_exc[gr]) continue;
-        const int n = J.seg_n[gr];
-        if (n <= MCL_PAIRS) continue;
-        if (n > SEGCAP) { if ((tid & 63) == 0) D.flags[5] = 1; continue; }
-        if (n > 64 * MCW_ITEMS) { s_big = 1; continue; }
-        int *sg = J.seg + gr * SEGCAP;
-        if (!mc_component_wave(P, D, r, J.rk, root, sg, n, beta, J.seed, step, s_mcws + wave * MCW_LDS))
-          if ((tid & 63) == 0) mc_component_serial(mc_arrays(D), P.N, r, J.rk, root, sg, n, beta, J.seed, step);
+      __syncthreads();
+      const int nw = s_nw;
+      if (nw <= MCW_LIST) {
+        // largest first (rank by size, then root: keys are distinct), then
+        // handed out one at a time: the longest wave is close to
+        // max(largest component, total / waves)
+        if (wave == 0) {
+          const int L = tid & 63;
+          const int key = L < nw ? s_wl[L] : 0;
+          int rank = 0;
+          for (int j = 0; j < nw; j++) rank += s_wl[j] > key;
+          wave_lds_fence();
+          if (L < nw) s_wl[rank] = key;
+        }
+        __syncthreads();
+        for (;;) {
+          int k = 0;
+          if ((tid & 63) == 0) k = atomicAdd(&s_next, 1);
+          k = bcast(k, 0);
+          if (k >= nw) break;
+          const int key = s_wl[k];
+          const int root = key & 0x1fffff, n = key >> 21;
+          int *sg = J.seg + (base + root) * SEGCAP;
+          if (!mc_component_wave(P, D, r, J.rk, root, sg, n, beta, J.seed, step, s_mcws + wave * MCW_LDS))
+            if ((tid & 63) == 0) mc_component_serial(mc_arrays(D), P.N, r, J.rk, root, sg, n, beta, J.seed, step);
+        }
+      } else {
+        // more than the list holds: static round robin over the roots
+        for (int it = wave; it < (listed ? nexc : N); it += nwaves) {
+          const int root = listed ? s_exc[it] : it;
+          const size_t gr = base + root;
+          if (!J.has_exc[gr]) continue;
+          const int n = J.seg_n[gr];
+          if (n <= MCL_PAIRS || n > 64 * MCW_ITEMS) continue;
+          int *sg = J.seg + gr * SEGCAP;
+          if (!mc_component_wave(P, D, r, J.rk, root, sg, n, beta, J.seed, step, s_mcws + wave * MCW_LDS))
+            if ((tid & 63) == 0) mc_component_serial(mc_arrays(D), P.N, r, J.rk, root, sg, n, beta, J.seed, step);
+        }
       }
       __syncthreads();
       // components above one wave's scratch: wave 0 with the whole block's
@@ -623,6 +679,30 @@ __device__ __forceinline__ void batched_body(const Params &P, const Ptrs &D, con
               int *sg = J.seg + gr * SEGCAP;
               if (!mc_component_wave<MCB_ITEMS, MCB_NODES>(P, D, r, J.rk, rt, sg, n, beta, J.seed, step, s_mcws))
                 if ((tid & 63) == 0) mc_component_serial(mc_arrays(D), P.N, r, J.rk, rt, sg, n, beta, J.seed, step);
+            }
+          }
+        }
+        __syncthreads();
+      }
+      // components above the bucket capacity: wave 0, global scratch (the
+      // replica's bucket region, free once the passes above are done)
+      if (s_huge) {
+        if (wave == 0) {
+          const int cnt = listed ? nexc : N;
+          for (int i0 = 0; i0 < cnt; i0 += 64) {
+            const int it = i0 + (tid & 63);
+            int root = 0;
+            bool huge = false;
+            if (it < cnt) {
+              root = listed ? s_exc[it] : it;
+              const size_t gr = base + root;
+              huge = J.has_exc[gr] && J.seg_n[gr] > SEGCAP;
+            }
+            for (unsigned long long m = __ballot(huge); m; m &= m - 1) {
+              const int rt = bcast(root, __ffsll((long long)m) - 1);
+              if (!mc_component_huge(mc_arrays(D), N, r, J.rk, rt, comp, r * J.capb, r * J.capb + nbr, beta, J.seed,
+                                     step, J.seg + base * SEGCAP, (size_t)N * SEGCAP, s_mcws))
+                if ((tid & 63) == 0) D.flags[5] = 1;
             }
           }
         }

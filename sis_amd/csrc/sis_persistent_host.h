@@ -79,6 +79,7 @@ void persistent_steps(sis_gpu_ctx *c, int64_t nsteps) {
     A.pos_b = c->pos2_d.p;
     A.nbp_total = c->nbp_total;
     A.force_mc_first = force_mc;
+    A.huge = c->huge_d.p;
     Params P = c->P;
     void *args[] = {&P, &D, &A};
     c->tmark();

@@ -270,7 +270,7 @@ void pool_round(sis_pool *p) {
       j.mc_first = 0;
       if (f[7]) { job_fail(p, j, "limit MC met a base-pair candidate outside the system"); continue; }
       if (f[1] == 2) { job_fail(p, j, "limit MC found an excess nucleotide without pairs"); continue; }
-      if (f[5]) { job_fail(p, j, "a base-pair conflict component exceeds 256 formed pairs"); continue; }
+      if (f[5]) { job_fail(p, j, "a base-pair conflict component outgrew the limit-MC scratch"); continue; }
       if (d < j.todo) {
         // an in-kernel rebuild outgrew a capacity: rebuild on the host,
         // which grows the lists, and carry on next round

@@ -309,6 +309,31 @@ def test_large_components_forms_match_phases(form):
         assert np.abs(a - b).max() < tol
 
 
+def test_components_above_bucket_capacity():
+    """Components of more than 256 formed pairs (one replica of eight
+    1,024-nt chains reaches about 300 after 200 steps) are resolved by the
+    unbounded wavefront path (global scratch) in the per-phase and batched
+    forms: the limit holds, and both forms give the same exchanges and
+    trajectories."""
+    s = configs.remd_c3(nrep=8, n=1024, seed=3)
+    print("[case] components above the bucket capacity", flush=True)
+    res = []
+    for mode in ("phases", "batched"):
+        eng = Engine(s, parity=False, mode=mode, seed=9, exchange_every=50)
+        eng.init_velocities()
+        eng.neighbor_list(0)
+        big = 0
+        for _ in range(6):
+            eng.step(50)
+            big = max([big] + [_largest_formed_component(eng, k + 1) for k in range(8)])
+            assert all(_limit_holds(eng, k + 1) for k in range(8))
+        assert big > 256, big
+        res.append((eng.get_state(), eng.labels()))
+    assert np.array_equal(res[0][1], res[1][1])
+    for a, b in zip(res[0][0], res[1][0]):
+        assert np.abs(a - b).max() < 1e-6
+
+
 def test_batched_dynamics_stable_fp32():
     s = configs.remd_c3(nrep=32, n=128, seed=4)
     print("[case] batched fp32", flush=True)
