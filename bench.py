@@ -135,6 +135,35 @@ def fp32_force_error(sysd):
                 floor_kcal_mol_A=1e-3)
 
 
+def ensemble_agreement(nrep=64, n=128, tempK=330.0, steps=(2000, 4000, 6000), seed=3):
+    """Ensembles of FP32 and FP64 pair arithmetic: nrep independent copies
+    of the C3 chain at one temperature, batched form, same start and
+    different noise; at each sample time, two-sample KS tests over the
+    replicas on Rg, E_total and the paired fraction.  Reports the smallest
+    p-value and every test."""
+    from sis_amd import configs, observables
+    from sis_amd.engine import Engine
+    s = configs.remd_c3(nrep=nrep, n=n, seed=seed)
+    s["temps"] = np.full(nrep, tempK)
+    s["params"] = dict(s["params"], tempK=tempK)
+    engs = [Engine(s, precision=p, mode="batched", seed=100 + p) for p in (0, 1)]
+    for e in engs:
+        e.init_velocities()
+        e.neighbor_list(0)
+    tests, done = {}, 0
+    for t in steps:
+        samp = []
+        for e in engs:
+            e.step(t - done)
+            samp.append(observables.sample(e))
+        done = t
+        tests[str(t)] = observables.ks_compare(samp[0], samp[1])
+    for e in engs:
+        e.close()
+    pmin = min(v["p"] for d in tests.values() for v in d.values())
+    return dict(replicas=nrep, tempK=tempK, steps=list(steps), min_p=pmin, tests=tests)
+
+
 def _draw_job(args):
     seed, k, max_chains = args
     from sis_amd import configs
@@ -340,8 +369,11 @@ def main():
             out["plain_fp64"] = pl
             out["ratio_vs_plain_fp64"] = round(value / pl["value"], 2)
             print(f"[bench] plain fp64 {pl['value']:.3e}", flush=True)
-        out["fp32_force"] = fp32_force_error(sysd if nrep > 1 else sysd)
+        out["fp32_force"] = fp32_force_error(sysd)
         print(f"[bench] fp32 force error {out['fp32_force']}", flush=True)
+        if args.workload == "c3":
+            out["fp32_ensemble"] = ensemble_agreement()
+            print(f"[bench] fp32 vs fp64 ensembles: min KS p {out['fp32_ensemble']['min_p']}", flush=True)
         if drawn:
             out["serving"] = serving_run(drawn, args.serving_scale, args.serving_rate, args.serving_max_chains)
             print(f"[bench] serving {out['serving']}", flush=True)

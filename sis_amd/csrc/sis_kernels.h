@@ -929,7 +929,7 @@ __global__ void __launch_bounds__(64) k_mc_wave(Params P, Ptrs D, int *seg_n, in
       mc_component_serial(mc_arrays(D), P.N, r, r, root, sg, n, beta, seed, step);
     return;
   }
-  const bool ok = n <= 64 * MCW_ITEMS ? mc_component_wave(P, D, r, r, root, sg, n, beta, seed, step, ws)
+  const bool ok = n <= 64 * MCW_ITEMS ? mc_component_wave_any(P, D, r, r, root, sg, n, beta, seed, step, ws)
                                       : mc_component_wave<MCB_ITEMS, MCB_NODES>(P, D, r, r, root, sg, n, beta, seed, step, ws);
   if (!ok && threadIdx.x == 0) mc_component_serial(mc_arrays(D), P.N, r, r, root, sg, n, beta, seed, step);
 }
@@ -1148,10 +1148,10 @@ __global__ void __launch_bounds__(B) k_force_int(Params P, Ptrs D, int integrate
 // takes terms l, l+LPB, ... of [2 bonds | 3 angles | 4 dihedrals]
 // (src/list_local.F90:24-104 enumerates the same terms).  FP64 always.
 __device__ __forceinline__ V3<double> bonded_part(const Params &P, const Ptrs &D, int r, int a,
-                                                  int l, int LPB) {
+                                                  int l, int LPB, const double4 *xr = nullptr) {
   const int N = P.N;
   const double *box = D.box + 3 * r;
-  const double4 *x = D.pos + (size_t)r * N;
+  const double4 *x = xr ? xr : D.pos + (size_t)r * N;
   const Topo ta = D.topo[a];
   const int loc = ta.loc, n = ta.len, st = ta.start;
   const bool circ = topo_circ(ta);
@@ -1184,10 +1184,10 @@ __device__ __forceinline__ V3<double> bonded_part(const Params &P, const Ptrs &D
 // 6-bead BP forces gathered through the incidence list; FP64 sums.
 template <typename T>
 __device__ __forceinline__ V3<double> pair_part(const Params &P, const Ptrs &D, int r, int a, int l,
-                                                int LPB) {
+                                                int LPB, const double4 *xr = nullptr) {
   const int N = P.N;
   const double *box = D.box + 3 * r;
-  const double4 *x = D.pos + (size_t)r * N;
+  const double4 *x = xr ? xr : D.pos + (size_t)r * N;
   const double4 pa = x[a];
   const size_t g = (size_t)r * N + a;
   const int nw = min(D.n_wca[g], D.maxw);

@@ -359,6 +359,155 @@ __device__ __forceinline__ bool mc_component_wave(const Params &P, const Ptrs &D
 }
 
 // ---------------------------------------------------------------------
+// The same sweep for a component of at most 64 pairs (the usual size),
+// with one pair per lane and the bookkeeping in scalar registers: the live
+// pairs are a 64-bit mask, the nucleotides two slots per lane, the sequence
+// of up to 8 pairs packed in a 64-bit scalar.  A single wavefront runs this
+// serially, so its cost is its instruction count; this form needs about a
+// third of the general one's.  Draws and decisions are those of
+// mc_component_wave.  Returns false (having written nothing) if the
+// component is larger, a nucleotide has more than 8 formed pairs, or the
+// state is inconsistent; the caller then runs the general form from the
+// start, which reproduces the same result.
+__device__ __forceinline__ bool mc_component_wave64(const Params &P, const Ptrs &D, int r, int rk, int root,
+                                                    const int *seg, int n, double beta, uint64_t seed, int64_t step,
+                                                    int *ws) {
+  constexpr int NONE = 0x7fffffff;
+  const int L = lane_id();
+  const int N = P.N;
+  n = __builtin_amdgcn_readfirstlane(n);
+  root = __builtin_amdgcn_readfirstlane(root);
+  if (n > 64) return false;
+  int *sraw = ws, *ssort = ws + 64, *send = ws + 128, *sflag = ws + 256;   // 384 ints
+  // ---- pairs in candidate-id order: rank by comparison, scatter
+  const int n4 = (n + 3) & ~3;
+  const int raw = L < n ? seg[L] : NONE;
+  if (L < n4) sraw[L] = raw;
+  wave_lds_fence();
+  int rank = 0;
+  for (int j = 0; j < n4; j += 4) {
+    const int4 v = *reinterpret_cast<const int4 *>(sraw + j);
+    rank += (v.x < raw) + (v.y < raw) + (v.z < raw) + (v.w < raw);
+  }
+  if (L < n) ssort[rank] = raw;
+  wave_lds_fence();
+  int gb = NONE, ia = -1, ib = -1;
+  double en = 0.0;
+  bool bad = false;
+  if (L < n) {
+    gb = ssort[L];
+    const int *mp = D.bp_mp + (size_t)gb * 8;
+    ia = mp[0] - 1;
+    ib = mp[1] - 1;
+    en = D.ene_bp[gb];
+    bad = ia < 0 || ia >= N || ib < 0 || ib >= N;
+  }
+  if (__ballot(bad)) return false;
+  // ---- distinct nucleotides in ascending id order (slots L and 64 + L)
+  const int nc = 2 * n, nc4 = (nc + 3) & ~3;
+  if (L < n) {
+    send[2 * L] = ia;
+    send[2 * L + 1] = ib;
+  } else if (2 * L < nc4) {
+    send[2 * L] = NONE;
+    send[2 * L + 1] = NONE;
+  }
+  wave_lds_fence();
+  const int c1 = 64 + L;
+  const int ev0 = L < nc ? send[L] : NONE, ev1 = c1 < nc ? send[c1] : NONE;
+  bool f0 = L < nc, f1 = c1 < nc;
+  for (int j = 0; j < nc4; j += 4) {
+    const int4 v = *reinterpret_cast<const int4 *>(send + j);
+    if ((j < L && v.x == ev0) || (j + 1 < L && v.y == ev0) || (j + 2 < L && v.z == ev0) || (j + 3 < L && v.w == ev0))
+      f0 = false;
+    if ((j < c1 && v.x == ev1) || (j + 1 < c1 && v.y == ev1) || (j + 2 < c1 && v.z == ev1) ||
+        (j + 3 < c1 && v.w == ev1))
+      f1 = false;
+  }
+  if (L < nc4) sflag[L] = f0 ? ev0 : NONE;
+  if (c1 < nc4) sflag[c1] = f1 ? ev1 : NONE;
+  wave_lds_fence();
+  const int nn = __popcll(__ballot(f0)) + __popcll(__ballot(f1));
+  int u0 = 0, u1 = 0;
+  for (int j = 0; j < nc4; j += 4) {
+    const int4 v = *reinterpret_cast<const int4 *>(sflag + j);
+    u0 += (v.x < ev0) + (v.y < ev0) + (v.z < ev0) + (v.w < ev0);
+    u1 += (v.x < ev1) + (v.y < ev1) + (v.z < ev1) + (v.w < ev1);
+  }
+  if (f0) send[u0] = ev0;
+  if (f1) send[u1] = ev1;
+  wave_lds_fence();
+  const int nid0 = L < nn ? send[L] : NONE, nid1 = c1 < nn ? send[c1] : NONE;
+  int cnt0 = L < nn ? D.nt_cnt[(size_t)r * N + nid0] : 0;
+  int cnt1 = c1 < nn ? D.nt_cnt[(size_t)r * N + nid1] : 0;
+  // ---- the sweep
+  unsigned long long live = __ballot(L < n);
+  uint32_t draw = 0, w0 = 0;
+  double wu = philox_u(seed, rk, step, root, (uint32_t)L), wl = log(wu);
+  for (int iter = 0; iter < 4 * 64 + 4; iter++) {
+    const unsigned long long a0 = __ballot(cnt0 > 0), a1 = __ballot(cnt1 > 0);
+    const int p0 = __popcll(a0), nact = p0 + __popcll(a1);
+    if (nact == 0) break;
+    if (draw + 1 > w0 + 64u) {
+      w0 = draw;
+      wu = philox_u(seed, rk, step, root, w0 + (uint32_t)L);
+      wl = log(wu);
+    }
+    const int pick = (int)ceil(bcast(wu, (int)(draw - w0)) * nact);   // 1..nact
+    const bool upper = pick > p0;
+    const unsigned long long am = upper ? a1 : a0;
+    const int want = upper ? pick - p0 - 1 : pick - 1;
+    const int ln = __ffsll((long long)__ballot(((am >> L) & 1ull) && below(am) == want)) - 1;
+    const int ntd = upper ? bcast(nid1, ln) : bcast(nid0, ln);
+    unsigned long long sm = __ballot(ia == ntd || ib == ntd) & live;
+    const int ns = __popcll(sm);
+    if (ns == 0 || ns > 8) return false;
+    uint64_t sp = 0;
+    for (int m = 0; m < ns; m++) {
+      sp |= (uint64_t)__builtin_ctzll(sm) << (8 * m);
+      sm &= sm - 1;
+    }
+    if (draw + 2u * (uint32_t)ns > w0 + 64u) {
+      w0 = draw;
+      wu = philox_u(seed, rk, step, root, w0 + (uint32_t)L);
+      wl = log(wu);
+    }
+    const int o = (int)(draw - w0);
+    const int swv = (int)ceil(wu * ns) - 1;
+    for (int i = 0; i < ns; i++) {
+      const int sw = bcast(swv, o + 1 + i);
+      const uint64_t ti = (sp >> (8 * i)) & 0xffull, ts = (sp >> (8 * sw)) & 0xffull;
+      sp = (sp & ~(0xffull << (8 * i))) | (ts << (8 * i));
+      sp = (sp & ~(0xffull << (8 * sw))) | (ti << (8 * sw));
+    }
+    int del = (int)(sp & 0xffull);
+    double edel = bcast(en, del);
+    for (int i = 1; i < ns; i++) {
+      const int si = (int)((sp >> (8 * i)) & 0xffull);
+      const double ej = bcast(en, si);
+      if (bcast(wl, o + ns + i) < (ej - edel) * beta) { del = si; edel = ej; }
+    }
+    draw += 2u * (uint32_t)ns;
+    live &= ~(1ull << del);
+    const int da = bcast(ia, del), db = bcast(ib, del);
+    cnt0 -= (nid0 == da) + (nid0 == db);
+    cnt1 -= (nid1 == da) + (nid1 == db);
+  }
+  if (L < n && !((live >> L) & 1ull)) D.bp_status[gb] = 0;
+  return true;
+}
+
+// The wavefront forms by size: 64 pairs or fewer in the scalar form, up
+// to 64 * MCW_ITEMS in the general one (also the fallback of the scalar
+// form).  ws holds MCW_LDS ints.
+__device__ __forceinline__ bool mc_component_wave_any(const Params &P, const Ptrs &D, int r, int rk, int root,
+                                                      const int *seg, int n, double beta, uint64_t seed,
+                                                      int64_t step, int *ws) {
+  if (n <= 64 && mc_component_wave64(P, D, r, rk, root, seg, n, beta, seed, step, ws)) return true;
+  return mc_component_wave(P, D, r, rk, root, seg, n, beta, seed, step, ws);
+}
+
+// ---------------------------------------------------------------------
 // The same algorithm for a component of at most MCL_PAIRS pairs, run by
 // one lane with everything in registers: arrays are read through unrolled
 // selects and the pair sequence is packed in 4-bit fields, so no per-lane

@@ -44,7 +44,8 @@ __device__ __forceinline__ void grid_barrier(unsigned long long *bar, unsigned l
 // one candidate of phase A (src/force_bp_limit_triplet.F90:85-256);
 // returns 1 if formed
 template <typename T>
-__device__ __forceinline__ int bp_eval_one(const Params &P, const Ptrs &D, int gb, int mc) {
+__device__ __forceinline__ int bp_eval_one(const Params &P, const Ptrs &D, int gb, int mc,
+                                           const double4 *xr = nullptr) {
   const int *mp = D.bp_mp + (size_t)gb * 8;
   const int r = mp[7];
   const int N = P.N;
@@ -55,7 +56,7 @@ __device__ __forceinline__ int bp_eval_one(const Params &P, const Ptrs &D, int g
   double dG = D.bp_coef[2 * (size_t)gb] - tK * D.bp_coef[2 * (size_t)gb + 1];
   if (dG >= 0.0) return 0;
   const double *box = D.box + 3 * r;
-  const double4 *x = D.pos + (size_t)r * N;
+  const double4 *x = xr ? xr : D.pos + (size_t)r * N;
   double4 p1 = x[mp[0] - 1], p2 = x[mp[1] - 1];
   V3<double> v12 = pvec(P, box, p1, p2), v13 = pvec(P, box, p1, x[mp[2] - 1]),
              v15 = pvec(P, box, p1, x[mp[4] - 1]), v42 = pvec(P, box, x[mp[3] - 1], p2),
@@ -159,7 +160,7 @@ __global__ void __launch_bounds__(B) k_persistent(Params P, Ptrs D, PersArgs A) 
           }
           if (n > 64 * MCW_ITEMS) { s_big = 1; continue; }
           int *sg = A.seg + (size_t)root * SEGCAP;
-          if (!mc_component_wave(P, D, 0, 0, root, sg, n, beta, A.seed, step, s_mcws + (threadIdx.x / 64) * MCW_LDS)) {
+          if (!mc_component_wave_any(P, D, 0, 0, root, sg, n, beta, A.seed, step, s_mcws + (threadIdx.x / 64) * MCW_LDS)) {
             if ((threadIdx.x & 63) == 0) mc_component_serial(mc_arrays(D), P.N, 0, 0, root, sg, n, beta, A.seed, step);
           }
         }
@@ -299,6 +300,9 @@ namespace sis {
 
 constexpr int EXC_LDS = 1024;   // listed roots with excess per system and step
 constexpr int MCW_LIST = 64;    // wavefront-MC components scheduled largest first per step
+// Positions of systems up to this many beads are read from an LDS copy in
+// the one-workgroup forms (32 KB of dynamic LDS beside about 31 KB static).
+constexpr int BATCH_X_LDS = 1024;
 
 // Per-system launch arguments.
 struct SysIO {
@@ -312,6 +316,7 @@ struct SysIO {
   int *seg_n, *seg, *has_exc;   // MC scratch (replica-r indexing)
   double4 *pos_b;               // second position buffer (replica-r indexing)
   int *done, *nbp, *rebuild, *nreb;  // this system's slots
+  int xlds;                     // beads whose positions fit the launch's dynamic LDS
 };
 
 // Launch arguments of k_batched (arrays over replicas).
@@ -324,6 +329,7 @@ struct BatchArgs {
   int *done, *nbp, *rebuild, *nreb;
   double4 *pos_b;
   int capb, lpb;
+  int xlds;   // beads whose positions fit the dynamic LDS of the launch
 };
 
 // Base-pair candidate arrays of one layout.
@@ -527,6 +533,16 @@ __device__ __forceinline__ void batched_body(const Params &P, const Ptrs &D, con
   const double BOLTZ = 1.380649e-23 * ((1.0 / 4184.0) * 6.02214076e23);
   const double beta = 1.0 / (BOLTZ * D.tK[r]);
   int *comp = D.comp + base;
+  // the step reads positions from an LDS copy when they fit (refreshed
+  // after every update and every rebuild, which may wrap them)
+  extern __shared__ double4 s_x[];
+  const bool xl = N <= J.xlds;
+  const double4 *xr = xl ? s_x : D.pos + base;
+  auto stage_x = [&]() {
+    if (xl)
+      for (int a = tid; a < N; a += B) s_x[a] = D.pos[base + a];
+    __syncthreads();
+  };
   if (tid == 0) s_flag = 0;
   // per-step limit-MC state starts reset whatever ran before this launch
   for (int a = tid; a < N; a += B) {
@@ -545,6 +561,7 @@ __device__ __forceinline__ void batched_body(const Params &P, const Ptrs &D, con
     mc_next = 1;
     nreb++;
   }
+  stage_x();
   int64_t completed = 0;
   int overflow = s_flag;
   const int64_t nsteps = J.todo, step0 = J.stepno;
@@ -556,7 +573,7 @@ __device__ __forceinline__ void batched_body(const Params &P, const Ptrs &D, con
     // ---- A: base pairs (+ union of formed pairs on MC steps), bonded forces
     for (int k = tid; k < nbr; k += B) {
       const int gb = r * J.capb + k;
-      if (bp_eval_one<T>(P, D, gb, mc) && mc) {
+      if (bp_eval_one<T>(P, D, gb, mc, xr) && mc) {
         const int *mp = D.bp_mp + (size_t)gb * 8;
         int a = mp[0] - 1, b = mp[1] - 1;
         while (true) {
@@ -571,7 +588,7 @@ __device__ __forceinline__ void batched_body(const Params &P, const Ptrs &D, con
     for (int a0 = 0; a0 < N; a0 += ngrp) {
       const int a = a0 + grp;
       V3<double> F{0.0, 0.0, 0.0};
-      if (a < N) F = bonded_part(P, D, r, a, l, lpb);
+      if (a < N) F = bonded_part(P, D, r, a, l, lpb, xr);
       F = group_sum_rt(F, lpb);
       if (a < N && l == 0) D.fbond[base + a] = double4{F.x, F.y, F.z, 0.0};
     }
@@ -641,7 +658,7 @@ __device__ __forceinline__ void batched_body(const Params &P, const Ptrs &D, con
           const int key = s_wl[k];
           const int root = key & 0x1fffff, n = key >> 21;
           int *sg = J.seg + (base + root) * SEGCAP;
-          if (!mc_component_wave(P, D, r, J.rk, root, sg, n, beta, J.seed, step, s_mcws + wave * MCW_LDS))
+          if (!mc_component_wave_any(P, D, r, J.rk, root, sg, n, beta, J.seed, step, s_mcws + wave * MCW_LDS))
             if ((tid & 63) == 0) mc_component_serial(mc_arrays(D), P.N, r, J.rk, root, sg, n, beta, J.seed, step);
         }
       } else {
@@ -653,7 +670,7 @@ __device__ __forceinline__ void batched_body(const Params &P, const Ptrs &D, con
           const int n = J.seg_n[gr];
           if (n <= MCL_PAIRS || n > 64 * MCW_ITEMS) continue;
           int *sg = J.seg + gr * SEGCAP;
-          if (!mc_component_wave(P, D, r, J.rk, root, sg, n, beta, J.seed, step, s_mcws + wave * MCW_LDS))
+          if (!mc_component_wave_any(P, D, r, J.rk, root, sg, n, beta, J.seed, step, s_mcws + wave * MCW_LDS))
             if ((tid & 63) == 0) mc_component_serial(mc_arrays(D), P.N, r, J.rk, root, sg, n, beta, J.seed, step);
         }
       }
@@ -716,7 +733,7 @@ __device__ __forceinline__ void batched_body(const Params &P, const Ptrs &D, con
     for (int a0 = 0; a0 < N; a0 += ngrp) {
       const int a = a0 + grp;
       V3<double> F{0.0, 0.0, 0.0};
-      if (a < N) F = pair_part<T>(P, D, r, a, l, lpb);
+      if (a < N) F = pair_part<T>(P, D, r, a, l, lpb, xr);
       F = group_sum_rt(F, lpb);
       if (a < N && l == 0) {
         const size_t g = base + a;
@@ -744,7 +761,11 @@ __device__ __forceinline__ void batched_body(const Params &P, const Ptrs &D, con
     if (tid == 0) s_exit = (sqrt(s1[0]) + sqrt(s2[0]) > P.nl_margin) ? 1 : 0;
     // the updated positions become current (every reader of D.pos in
     // this step has passed the barriers above)
-    for (int a = tid; a < N; a += B) D.pos[base + a] = D.pos_out[base + a];
+    for (int a = tid; a < N; a += B) {
+      const double4 xn = D.pos_out[base + a];
+      D.pos[base + a] = xn;
+      if (xl) s_x[a] = xn;
+    }
     __syncthreads();
     completed = s + 1;
     if (s_exit) {
@@ -753,6 +774,7 @@ __device__ __forceinline__ void batched_body(const Params &P, const Ptrs &D, con
         overflow = s_flag;
         mc_next = 1;
         nreb++;
+        stage_x();
       } else if (tid == 0) {
         *J.rebuild = 1;   // rebuild at the start of the next launch
       }
@@ -780,6 +802,7 @@ __global__ void __launch_bounds__(B) k_batched(Params P, Ptrs D, BatchArgs A) {
   J.seg = A.seg;
   J.has_exc = A.has_exc;
   J.pos_b = A.pos_b;
+  J.xlds = A.xlds;
   J.done = A.done + r;
   J.nbp = A.nbp + r;
   J.rebuild = A.rebuild + r;
@@ -812,7 +835,7 @@ struct JobView {
 // Job pool: workgroup j advances job j.  P carries the force field shared
 // by the jobs (sizes are the job's own); D0 carries the shared tables.
 template <typename T, int B>
-__global__ void __launch_bounds__(B) k_jobs(Params P, Ptrs D0, const JobView *__restrict__ jobs) {
+__global__ void __launch_bounds__(B) k_jobs(Params P, Ptrs D0, const JobView *__restrict__ jobs, int xlds) {
   const JobView &V = jobs[blockIdx.x];
   Params Pj = P;
   Pj.N = V.N;
@@ -843,6 +866,7 @@ __global__ void __launch_bounds__(B) k_jobs(Params P, Ptrs D0, const JobView *__
   J.seg = V.seg;
   J.has_exc = V.has_exc;
   J.pos_b = V.pos_b;
+  J.xlds = xlds;
   J.done = V.io;
   J.nbp = V.io + 1;
   J.rebuild = V.io + 2;

@@ -256,6 +256,7 @@ void batched_steps(sis_gpu_ctx *c, int64_t nsteps) {
       A.pos_b = c->pos2_d.p;
       A.capb = c->batch_capb;
       A.lpb = lpb;
+      A.xlds = N <= BATCH_X_LDS ? N : 0;
       Params P = c->P;
       Ptrs D = c->ptrs();
       D.bp_mp = c->bb_mp_d.p;
@@ -266,7 +267,7 @@ void batched_steps(sis_gpu_ctx *c, int64_t nsteps) {
       D.for_bp = c->bb_for_d.p;
       void *args[] = {&P, &D, &A};
       c->tmark();
-      HIPCHK(hipLaunchKernel(kfn, dim3(R), dim3(B), args, 0, c->stream));
+      HIPCHK(hipLaunchKernel(kfn, dim3(R), dim3(B), args, (size_t)A.xlds * sizeof(double4), c->stream));
       c->tmark();
       std::vector<int> d(R), nr(R), f(8);
       HIPCHK(hipMemcpyAsync(d.data(), c->bb_done_d.p, R * sizeof(int), hipMemcpyDeviceToHost, c->stream));

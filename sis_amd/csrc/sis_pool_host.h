@@ -229,12 +229,17 @@ void pool_round(sis_pool *p) {
     HIPCHK(hipMemcpyAsync(p->views_d[k].p, views[k].data(), views[k].size() * sizeof(JobView),
                           hipMemcpyHostToDevice, p->stream[k]));
     const Ptrs D0 = members[k].front()->c->ptrs();   // the shared base-pair tables
+    // dynamic LDS for the positions of the largest member that fits
+    int xlds = 0;
+    for (PoolJob *jp : members[k])
+      if (jp->nmp <= BATCH_X_LDS) xlds = std::max(xlds, jp->nmp);
+    const size_t dyn = (size_t)xlds * sizeof(double4);
     if (k == 1)
-      hipLaunchKernelGGL((k_jobs<float, 256>), dim3((unsigned)views[k].size()), dim3(256), 0, p->stream[k], p->P0, D0,
-                         (const JobView *)p->views_d[k].p);
+      hipLaunchKernelGGL((k_jobs<float, 256>), dim3((unsigned)views[k].size()), dim3(256), dyn, p->stream[k], p->P0,
+                         D0, (const JobView *)p->views_d[k].p, xlds);
     else
-      hipLaunchKernelGGL((k_jobs<double, 256>), dim3((unsigned)views[k].size()), dim3(256), 0, p->stream[k], p->P0,
-                         D0, (const JobView *)p->views_d[k].p);
+      hipLaunchKernelGGL((k_jobs<double, 256>), dim3((unsigned)views[k].size()), dim3(256), dyn, p->stream[k],
+                         p->P0, D0, (const JobView *)p->views_d[k].p, xlds);
     HIPCHK(hipGetLastError());
     nl++;
   }

@@ -334,6 +334,44 @@ def test_components_above_bucket_capacity():
         assert np.abs(a - b).max() < 1e-6
 
 
+def test_components_above_bucket_capacity_persistent():
+    """The same unbounded path inside the persistent kernel (wave 0 of
+    block 0 between two grid barriers): a 2,048-nt chain (60% GC) whose
+    largest formed component holds about 320 pairs; persistent and
+    per-phase forms agree."""
+    rng = np.random.default_rng(11)
+    seq = configs.random_seq(rng, 2048, 0.6)
+    s = configs._mk("chain2048", [seq], configs.random_coil(rng, 2048),
+                    configs.default_params(tempK=300.0, nl_margin=5.0))
+    print("[case] components above the bucket capacity, persistent", flush=True)
+    out = []
+    for mode in ("phases", "persistent"):
+        eng = Engine(s, parity=False, mode=mode, seed=9)
+        eng.init_velocities()
+        eng.neighbor_list(0)
+        eng.step(100)
+        assert eng.info()["mode"] == (1 if mode == "phases" else 2)
+        assert _largest_formed_component(eng) > 256
+        assert _limit_holds(eng)
+        out.append(eng.get_state()[0])
+    assert np.abs(out[0] - out[1]).max() < 1e-9
+
+
+def test_fp32_and_fp64_ensembles_agree():
+    """64 independent copies of the 128-nt chain at 330 K, batched form:
+    FP32 and FP64 pair arithmetic give the same distributions of Rg, total
+    potential energy and paired fraction at 2,000, 4,000 and 6,000 steps
+    (two-sample KS over replicas, every p-value above 0.001)."""
+    import importlib.util
+    spec = importlib.util.spec_from_file_location("bench", Path(__file__).resolve().parents[1] / "bench.py")
+    bench = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(bench)
+    print("[case] fp32 vs fp64 ensembles", flush=True)
+    res = bench.ensemble_agreement()
+    print(res, flush=True)
+    assert res["min_p"] > 1e-3, res
+
+
 def test_batched_dynamics_stable_fp32():
     s = configs.remd_c3(nrep=32, n=128, seed=4)
     print("[case] batched fp32", flush=True)
