@@ -1182,9 +1182,12 @@ __device__ __forceinline__ V3<double> bonded_part(const Params &P, const Ptrs &D
 
 // Pair terms of bead a: WCA and DH over the full neighbour lists in T,
 // 6-bead BP forces gathered through the incidence list; FP64 sums.
+// Pair forces on bead a, lane l of LPB: WCA and Debye-Hueckel neighbours
+// (pair_nb), then the formed base pairs it takes part in (pair_bp), both
+// accumulated onto F in that order.  pair_part is the two in sequence.
 template <typename T>
-__device__ __forceinline__ V3<double> pair_part(const Params &P, const Ptrs &D, int r, int a, int l,
-                                                int LPB, const double4 *xr = nullptr) {
+__device__ __forceinline__ V3<double> pair_nb(const Params &P, const Ptrs &D, int r, int a, int l, int LPB,
+                                              const double4 *xr, V3<double> F) {
   const int N = P.N;
   const double *box = D.box + 3 * r;
   const double4 *x = xr ? xr : D.pos + (size_t)r * N;
@@ -1192,8 +1195,6 @@ __device__ __forceinline__ V3<double> pair_part(const Params &P, const Ptrs &D, 
   const size_t g = (size_t)r * N + a;
   const int nw = min(D.n_wca[g], D.maxw);
   const int ne = P.ele_on ? min(D.n_ele[g], D.maxnb) : 0;
-  const int ni = min(D.n_inc[g], D.maxinc);
-  V3<double> F{0.0, 0.0, 0.0};
   const T coef = (T)D.ele_coef[r], rc2 = (T)(D.ele_cut[r] * D.ele_cut[r]), rl = (T)(1.0 / D.lambdaD[r]);
   // neighbours in batches of PU: their index and position loads are all
   // issued before the arithmetic; the summation order is unchanged
@@ -1228,6 +1229,14 @@ __device__ __forceinline__ V3<double> pair_part(const Params &P, const Ptrs &D, 
     for (int u = 0; u < PU; u++)
       if (id[u] >= 0) F = F + cvt<double>(dh_force<T>(coef, rc2, rl, cvt<T>(pvec(P, box, pa, pb[u]))));
   }
+  return F;
+}
+
+__device__ __forceinline__ V3<double> pair_bp(const Params &P, const Ptrs &D, int r, int a, int l, int LPB,
+                                              V3<double> F) {
+  const int N = P.N;
+  const size_t g = (size_t)r * N + a;
+  const int ni = min(D.n_inc[g], D.maxinc);
   const int *inc = D.inc + (size_t)r * D.maxinc * N + a;
   for (int k = l; k < ni; k += LPB) {
     int e = inc[(size_t)k * N], gb = e >> 3, role = e & 7;
@@ -1237,6 +1246,12 @@ __device__ __forceinline__ V3<double> pair_part(const Params &P, const Ptrs &D, 
     }
   }
   return F;
+}
+
+template <typename T>
+__device__ __forceinline__ V3<double> pair_part(const Params &P, const Ptrs &D, int r, int a, int l,
+                                                int LPB, const double4 *xr = nullptr) {
+  return pair_bp(P, D, r, a, l, LPB, pair_nb<T>(P, D, r, a, l, LPB, xr, V3<double>{0.0, 0.0, 0.0}));
 }
 
 template <typename T>

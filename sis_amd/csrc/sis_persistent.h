@@ -303,6 +303,10 @@ constexpr int MCW_LIST = 64;    // wavefront-MC components scheduled largest fir
 // Positions of systems up to this many beads are read from an LDS copy in
 // the one-workgroup forms (32 KB of dynamic LDS beside about 31 KB static).
 constexpr int BATCH_X_LDS = 1024;
+// Per-lane partial pair forces (bead count x lanes per bead) kept in LDS
+// beside the positions, 16 KB at most.
+constexpr int BATCH_FP_LDS = 512;
+__host__ __device__ inline int lanes_per_bead(int N) { return N <= 32 ? 8 : (N <= 64 ? 4 : (N <= 128 ? 2 : 1)); }
 
 // Per-system launch arguments.
 struct SysIO {
@@ -317,6 +321,7 @@ struct SysIO {
   double4 *pos_b;               // second position buffer (replica-r indexing)
   int *done, *nbp, *rebuild, *nreb;  // this system's slots
   int xlds;                     // beads whose positions fit the launch's dynamic LDS
+  int fplds;                    // per-lane partial forces that fit after them
 };
 
 // Launch arguments of k_batched (arrays over replicas).
@@ -330,6 +335,7 @@ struct BatchArgs {
   double4 *pos_b;
   int capb, lpb;
   int xlds;   // beads whose positions fit the dynamic LDS of the launch
+  int fplds;  // per-lane partial forces that fit after them
 };
 
 // Base-pair candidate arrays of one layout.
@@ -538,6 +544,10 @@ __device__ __forceinline__ void batched_body(const Params &P, const Ptrs &D, con
   extern __shared__ double4 s_x[];
   const bool xl = N <= J.xlds;
   const double4 *xr = xl ? s_x : D.pos + base;
+  // per-lane partial pair forces (WCA + Debye-Hueckel) follow the positions
+  // in the dynamic LDS when they fit; the work loop of phase C fills them
+  double4 *s_fp = s_x + J.xlds;
+  const bool ovl = xl && N * lpb <= J.fplds;
   auto stage_x = [&]() {
     if (xl)
       for (int a = tid; a < N; a += B) s_x[a] = D.pos[base + a];
@@ -570,7 +580,8 @@ __device__ __forceinline__ void batched_body(const Params &P, const Ptrs &D, con
     const int mc = P.nstep_bp_mc > 0 && ((step % P.nstep_bp_mc) == 0 || mc_next);
     mc_next = 0;
     const int nbr = *J.nbp;
-    // ---- A: base pairs (+ union of formed pairs on MC steps), bonded forces
+    // ---- A: base pairs (+ union of formed pairs on MC steps); bonded
+    // forces here unless the work loop below takes them
     for (int k = tid; k < nbr; k += B) {
       const int gb = r * J.capb + k;
       if (bp_eval_one<T>(P, D, gb, mc, xr) && mc) {
@@ -585,15 +596,17 @@ __device__ __forceinline__ void batched_body(const Params &P, const Ptrs &D, con
         }
       }
     }
-    for (int a0 = 0; a0 < N; a0 += ngrp) {
-      const int a = a0 + grp;
-      V3<double> F{0.0, 0.0, 0.0};
-      if (a < N) F = bonded_part(P, D, r, a, l, lpb, xr);
-      F = group_sum_rt(F, lpb);
-      if (a < N && l == 0) D.fbond[base + a] = double4{F.x, F.y, F.z, 0.0};
-    }
+    if (!ovl)
+      for (int a0 = 0; a0 < N; a0 += ngrp) {
+        const int a = a0 + grp;
+        V3<double> F{0.0, 0.0, 0.0};
+        if (a < N) F = bonded_part(P, D, r, a, l, lpb, xr);
+        F = group_sum_rt(F, lpb);
+        if (a < N && l == 0) D.fbond[base + a] = double4{F.x, F.y, F.z, 0.0};
+      }
     if (tid == 0) { s_nexc = 0; s_big = 0; s_nw = 0; s_next = 0; s_huge = 0; }
     __syncthreads();
+    int nw = 0;
     if (mc) {
       // ---- B: formed pairs bucketed by component root; roots with
       // excess listed once
@@ -636,11 +649,9 @@ __device__ __forceinline__ void batched_body(const Params &P, const Ptrs &D, con
           mc_component_serial(mc_arrays(D), P.N, r, J.rk, root, sg, n, beta, J.seed, step);
       }
       __syncthreads();
-      const int nw = s_nw;
+      nw = s_nw;
       if (nw <= MCW_LIST) {
-        // largest first (rank by size, then root: keys are distinct), then
-        // handed out one at a time: the longest wave is close to
-        // max(largest component, total / waves)
+        // largest first (rank by size, then root: keys are distinct)
         if (wave == 0) {
           const int L = tid & 63;
           const int key = L < nw ? s_wl[L] : 0;
@@ -650,18 +661,48 @@ __device__ __forceinline__ void batched_body(const Params &P, const Ptrs &D, con
           if (L < nw) s_wl[rank] = key;
         }
         __syncthreads();
-        for (;;) {
-          int k = 0;
-          if ((tid & 63) == 0) k = atomicAdd(&s_next, 1);
-          k = bcast(k, 0);
-          if (k >= nw) break;
+      }
+    }
+    // ---- C: one work list for the waves, handed out one item at a time:
+    // the wavefront-MC components (largest first), then, when the partial
+    // forces fit in LDS, the bonded and WCA/Debye-Hueckel forces of groups
+    // of beads (they do not depend on the MC).  The longest wave is then
+    // close to the largest component, with the force work in the shadow
+    // of the MC.
+    {
+      const int wave = tid / 64, L = tid & 63;
+      const int nmc = (mc && nw <= MCW_LIST) ? nw : 0;
+      const int bpw = 64 / lpb, nchunk = ovl ? (N + bpw - 1) / bpw : 0;
+      for (;;) {
+        int k = 0;
+        if (L == 0) k = atomicAdd(&s_next, 1);
+        k = bcast(k, 0);
+        if (k >= nmc + nchunk) break;
+        if (k < nmc) {
           const int key = s_wl[k];
           const int root = key & 0x1fffff, n = key >> 21;
           int *sg = J.seg + (base + root) * SEGCAP;
           if (!mc_component_wave_any(P, D, r, J.rk, root, sg, n, beta, J.seed, step, s_mcws + wave * MCW_LDS))
-            if ((tid & 63) == 0) mc_component_serial(mc_arrays(D), P.N, r, J.rk, root, sg, n, beta, J.seed, step);
+            if (L == 0) mc_component_serial(mc_arrays(D), P.N, r, J.rk, root, sg, n, beta, J.seed, step);
+          continue;
         }
-      } else {
+        const int a = (k - nmc) * bpw + L / lpb, lw = L % lpb;
+        V3<double> F{0.0, 0.0, 0.0};
+        if (a < N) {
+          const V3<double> Fp = pair_nb<T>(P, D, r, a, lw, lpb, xr, V3<double>{0.0, 0.0, 0.0});
+          s_fp[(size_t)a * lpb + lw] = double4{Fp.x, Fp.y, Fp.z, 0.0};
+          F = bonded_part(P, D, r, a, lw, lpb, xr);
+        }
+        F = group_sum_rt(F, lpb);
+        if (a < N && lw == 0) D.fbond[base + a] = double4{F.x, F.y, F.z, 0.0};
+      }
+    }
+    __syncthreads();
+    if (mc) {
+      const int wave = tid / 64, nwaves = B / 64;
+      const int nexc = s_nexc;
+      const bool listed = nexc <= EXC_LDS;
+      if (nw > MCW_LIST) {
         // more than the list holds: static round robin over the roots
         for (int it = wave; it < (listed ? nexc : N); it += nwaves) {
           const int root = listed ? s_exc[it] : it;
@@ -673,8 +714,8 @@ __device__ __forceinline__ void batched_body(const Params &P, const Ptrs &D, con
           if (!mc_component_wave_any(P, D, r, J.rk, root, sg, n, beta, J.seed, step, s_mcws + wave * MCW_LDS))
             if ((tid & 63) == 0) mc_component_serial(mc_arrays(D), P.N, r, J.rk, root, sg, n, beta, J.seed, step);
         }
+        __syncthreads();
       }
-      __syncthreads();
       // components above one wave's scratch: wave 0 with the whole block's
       if (s_big) {
         if (wave == 0) {
@@ -728,12 +769,21 @@ __device__ __forceinline__ void batched_body(const Params &P, const Ptrs &D, con
       // the MC result is what later non-MC steps evaluate
       for (int k = tid; k < nbr; k += B) D.bp_status_mc[r * J.capb + k] = D.bp_status[r * J.capb + k];
     }
-    // ---- D: pair forces + bonded, GJF-2GJ, per-step MC state reset
+    // ---- D: base-pair forces onto the partial pair forces (or the whole
+    // pair sum when those did not fit in LDS), bonded, GJF-2GJ, per-step
+    // MC state reset
     double m1 = 0.0, m2 = 0.0;
     for (int a0 = 0; a0 < N; a0 += ngrp) {
       const int a = a0 + grp;
       V3<double> F{0.0, 0.0, 0.0};
-      if (a < N) F = pair_part<T>(P, D, r, a, l, lpb, xr);
+      if (a < N) {
+        if (ovl) {
+          const double4 fp = s_fp[(size_t)a * lpb + l];
+          F = pair_bp(P, D, r, a, l, lpb, V3<double>{fp.x, fp.y, fp.z});
+        } else {
+          F = pair_part<T>(P, D, r, a, l, lpb, xr);
+        }
+      }
       F = group_sum_rt(F, lpb);
       if (a < N && l == 0) {
         const size_t g = base + a;
@@ -803,6 +853,7 @@ __global__ void __launch_bounds__(B) k_batched(Params P, Ptrs D, BatchArgs A) {
   J.has_exc = A.has_exc;
   J.pos_b = A.pos_b;
   J.xlds = A.xlds;
+  J.fplds = A.fplds;
   J.done = A.done + r;
   J.nbp = A.nbp + r;
   J.rebuild = A.rebuild + r;
@@ -835,7 +886,8 @@ struct JobView {
 // Job pool: workgroup j advances job j.  P carries the force field shared
 // by the jobs (sizes are the job's own); D0 carries the shared tables.
 template <typename T, int B>
-__global__ void __launch_bounds__(B) k_jobs(Params P, Ptrs D0, const JobView *__restrict__ jobs, int xlds) {
+__global__ void __launch_bounds__(B) k_jobs(Params P, Ptrs D0, const JobView *__restrict__ jobs, int xlds,
+                                            int fplds) {
   const JobView &V = jobs[blockIdx.x];
   Params Pj = P;
   Pj.N = V.N;
@@ -867,6 +919,7 @@ __global__ void __launch_bounds__(B) k_jobs(Params P, Ptrs D0, const JobView *__
   J.has_exc = V.has_exc;
   J.pos_b = V.pos_b;
   J.xlds = xlds;
+  J.fplds = fplds;
   J.done = V.io;
   J.nbp = V.io + 1;
   J.rebuild = V.io + 2;

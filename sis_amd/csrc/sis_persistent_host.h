@@ -2,8 +2,6 @@
 // Included by sis_gpu.hip after the per-phase driver.
 #pragma once
 
-// lanes per bead of the one-workgroup forms: fill 256 threads
-int lanes_per_bead(int N) { return N <= 32 ? 8 : (N <= 64 ? 4 : (N <= 128 ? 2 : 1)); }
 
 bool batched_ok(sis_gpu_ctx *c) {
   return !c->parity && c->R >= 2 && c->P.nchains == 1 && c->N <= 1024;
@@ -257,6 +255,7 @@ void batched_steps(sis_gpu_ctx *c, int64_t nsteps) {
       A.capb = c->batch_capb;
       A.lpb = lpb;
       A.xlds = N <= BATCH_X_LDS ? N : 0;
+      A.fplds = (A.xlds && N * lpb <= BATCH_FP_LDS) ? N * lpb : 0;
       Params P = c->P;
       Ptrs D = c->ptrs();
       D.bp_mp = c->bb_mp_d.p;
@@ -267,7 +266,7 @@ void batched_steps(sis_gpu_ctx *c, int64_t nsteps) {
       D.for_bp = c->bb_for_d.p;
       void *args[] = {&P, &D, &A};
       c->tmark();
-      HIPCHK(hipLaunchKernel(kfn, dim3(R), dim3(B), args, (size_t)A.xlds * sizeof(double4), c->stream));
+      HIPCHK(hipLaunchKernel(kfn, dim3(R), dim3(B), args, (size_t)(A.xlds + A.fplds) * sizeof(double4), c->stream));
       c->tmark();
       std::vector<int> d(R), nr(R), f(8);
       HIPCHK(hipMemcpyAsync(d.data(), c->bb_done_d.p, R * sizeof(int), hipMemcpyDeviceToHost, c->stream));

@@ -230,16 +230,20 @@ void pool_round(sis_pool *p) {
                           hipMemcpyHostToDevice, p->stream[k]));
     const Ptrs D0 = members[k].front()->c->ptrs();   // the shared base-pair tables
     // dynamic LDS for the positions of the largest member that fits
-    int xlds = 0;
+    int xlds = 0, fplds = 0;
     for (PoolJob *jp : members[k])
-      if (jp->nmp <= BATCH_X_LDS) xlds = std::max(xlds, jp->nmp);
-    const size_t dyn = (size_t)xlds * sizeof(double4);
+      if (jp->nmp <= BATCH_X_LDS) {
+        xlds = std::max(xlds, jp->nmp);
+        const int fp = jp->nmp * lanes_per_bead(jp->nmp);
+        if (fp <= BATCH_FP_LDS) fplds = std::max(fplds, fp);
+      }
+    const size_t dyn = (size_t)(xlds + fplds) * sizeof(double4);
     if (k == 1)
       hipLaunchKernelGGL((k_jobs<float, 256>), dim3((unsigned)views[k].size()), dim3(256), dyn, p->stream[k], p->P0,
-                         D0, (const JobView *)p->views_d[k].p, xlds);
+                         D0, (const JobView *)p->views_d[k].p, xlds, fplds);
     else
       hipLaunchKernelGGL((k_jobs<double, 256>), dim3((unsigned)views[k].size()), dim3(256), dyn, p->stream[k],
-                         p->P0, D0, (const JobView *)p->views_d[k].p, xlds);
+                         p->P0, D0, (const JobView *)p->views_d[k].p, xlds, fplds);
     HIPCHK(hipGetLastError());
     nl++;
   }
