@@ -288,7 +288,10 @@ template <typename T>
 __device__ __forceinline__ V3<T> wca_force(const Params &P, const V3<T> &v12) {
   T d2 = dot(v12, v12);
   if (d2 >= (T)P.wca_sig2) return V3<T>{0, 0, 0};
-  T s = (T)P.wca_sig2 / d2;
+  // FP32: hardware reciprocal (1 ulp) instead of the correctly rounded divide
+  T s;
+  if constexpr (sizeof(T) == 4) s = (T)P.wca_sig2 * __builtin_amdgcn_rcpf(d2);
+  else s = (T)P.wca_sig2 / d2;
   T s2 = s * s, s4 = s2 * s2, s7 = s4 * s2 * s;
   return ((T)P.wca_coef * (s7 - s4)) * v12;
 }
@@ -297,7 +300,15 @@ template <typename T>
 __device__ __forceinline__ V3<T> dh_force(T coef, T rc2, T rlam, const V3<T> &v12) {
   T d2 = dot(v12, v12);
   if (d2 > rc2) return V3<T>{0, 0, 0};
-  T d1 = m_sqrt(d2), r1 = (T)1 / d1;
+  // FP32: hardware reciprocal square root (1 ulp) gives 1/d and d = d2/d
+  T d1, r1;
+  if constexpr (sizeof(T) == 4) {
+    r1 = __builtin_amdgcn_rsqf(d2);
+    d1 = d2 * r1;
+  } else {
+    d1 = m_sqrt(d2);
+    r1 = (T)1 / d1;
+  }
   T dv = coef * r1 * r1 * (r1 + rlam) * m_exp(-d1 * rlam);
   return dv * v12;
 }
