@@ -168,6 +168,29 @@ int sis_gpu_exchange(sis_gpu_ctx *ctx, double *replica_energies, int32_t *accept
 /* rep2lab(1:nrep) (src/var_replica.F90:49). */
 int sis_gpu_get_labels(sis_gpu_ctx *ctx, int32_t *rep2lab);
 
+/* Trajectory output in the reference's DCD layout (file_dcd of
+ * src/dcd.F90: write_header :287-355, write_onestep :538-596), one file
+ * per replica as job_md opens them (src/job_md.F90:294-303).  The unit
+ * cell blocks are written when the system is periodic; fix_com_origin > 0
+ * subtracts the centre of geometry as write_onestep does.  With
+ * nstep_save > 0, sis_gpu_step writes a frame of every replica after each
+ * step whose count is a multiple of nstep_save (the save block of
+ * src/job_md.F90:485-508): the positions are copied to pinned host memory
+ * on the engine's stream and written by a background thread while the
+ * next steps run.  sis_gpu_dcd_write writes the current state now (the
+ * initial frame of src/job_md.F90:336-339) and returns when it is on disk;
+ * sis_gpu_dcd_close waits for queued frames and closes the files (also
+ * done by sis_gpu_destroy).  paths holds nrep file names. */
+int sis_gpu_dcd_open(sis_gpu_ctx *ctx, const char *const *paths, int32_t fix_com_origin, int64_t nstep_save);
+int sis_gpu_dcd_write(sis_gpu_ctx *ctx);
+int sis_gpu_dcd_close(sis_gpu_ctx *ctx);
+
+/* The same DCD writer on host arrays: nframes frames of xyz(3, nmp)
+ * (Fortran layout, frames consecutive); box[3] is used when unitcell != 0.
+ * Needs no GPU. */
+int sis_dcd_write_file(const char *path, int32_t nmp, int32_t nframes, const double *xyz, int32_t unitcell,
+                       const double *box, int32_t fix_com_origin);
+
 /* Counters: out[0] nmp, [1] nrep, [2] istep, [3] rebuilds, [4] nwca,
  * [5] nele, [6] nbp (replica 1), [7] neighbour slots per bead,
  * [8] launch mode used, [9] workgroup size. */

@@ -510,4 +510,33 @@ contains
       rep2lab_out(1:nrep_all) = rep2lab(1:nrep_all)
    end subroutine sisref_replica_labels
 
+   ! Write nframe frames of xyz(3, nmp, nframe) to a DCD file with the
+   ! reference's own writer (file_dcd of src/dcd.F90: write_header,
+   ! write_onestep), for byte-level comparison with other writers.
+   subroutine sisref_write_dcd(path, plen, nmp_in, nframe, xyz, unitcell, box, fix_com) &
+                               bind(C, name="sisref_write_dcd")
+      use dcd, only : file_dcd, DCD_OPEN_MODE
+      character(kind=c_char), intent(in) :: path(*)
+      integer(c_int), value :: plen, nmp_in, nframe, unitcell, fix_com
+      real(c_double), intent(in) :: xyz(3, nmp_in, *)
+      real(c_double), intent(in) :: box(3)
+      type(file_dcd) :: f
+      character(len=:), allocatable :: fname
+      integer :: k
+      allocate(character(len=plen) :: fname)
+      do k = 1, plen
+         fname(k:k) = path(k)
+      enddo
+      f = file_dcd(93, fname, DCD_OPEN_MODE%WRITE)
+      if (unitcell /= 0) then
+         f%flg_unitcell = .True.
+         f%box(:) = box(:)
+      endif
+      call f%write_header(nmp_in)
+      do k = 1, nframe
+         call f%write_onestep(nmp_in, xyz(:, :, k), fix_com)
+      enddo
+      call f%close()
+   end subroutine sisref_write_dcd
+
 end module sisref_driver

@@ -213,3 +213,32 @@ def run(sysd: dict, ops: list, seed: int = 12345, omp: bool = False, nstep_excha
     if status != "ok":
         raise RuntimeError("reference run failed:\n" + payload)
     return payload
+
+
+def _dcd_worker(q, so, path, xyz, box, fix_com):
+    try:
+        lib = C.CDLL(str(so))
+        x = np.ascontiguousarray(xyz, np.float64)
+        b = np.ascontiguousarray(box if box is not None else np.zeros(3), np.float64)
+        p = str(path).encode()
+        lib.sisref_write_dcd(C.c_char_p(p), C.c_int(len(p)), C.c_int(x.shape[1]), C.c_int(x.shape[0]), _dp(x),
+                             C.c_int(1 if box is not None else 0), _dp(b), C.c_int(int(fix_com)))
+        q.put(("ok", None))
+    except Exception as e:  # pragma: no cover - reported to the caller
+        q.put(("err", repr(e)))
+
+
+def write_dcd(path, xyz_frames, box=None, fix_com=0, timeout=60):
+    """The reference's DCD writer (file_dcd, src/dcd.F90) on xyz_frames
+    (nframes, nmp, 3), run in a child process."""
+    import multiprocessing as mp
+    ctx = mp.get_context("spawn")
+    q = ctx.Queue()
+    pr = ctx.Process(target=_dcd_worker, args=(q, str(REF_SO), str(path), np.asarray(xyz_frames), box, fix_com))
+    pr.start()
+    try:
+        st, msg = q.get(timeout=timeout)
+    finally:
+        pr.join(timeout=10)
+    if st != "ok":
+        raise RuntimeError(f"reference DCD writer failed: {msg}")

@@ -120,6 +120,7 @@ EXPORTED = [
     "sis_gpu_get_info", "sis_gpu_last_kernel_ms",
     "sis_pool_create", "sis_pool_submit", "sis_pool_poll", "sis_pool_forget",
     "sis_pool_stats", "sis_pool_destroy",
+    "sis_gpu_dcd_open", "sis_gpu_dcd_write", "sis_gpu_dcd_close", "sis_dcd_write_file",
 ]
 
 
@@ -154,6 +155,10 @@ def _declare(lib: C.CDLL) -> None:
         "sis_pool_forget": (i32, [vp, i64]),
         "sis_pool_stats": (i32, [vp, P(d), i32]),
         "sis_pool_destroy": (i32, [vp]),
+        "sis_gpu_dcd_open": (i32, [vp, P(C.c_char_p), i32, i64]),
+        "sis_gpu_dcd_write": (i32, [vp]),
+        "sis_gpu_dcd_close": (i32, [vp]),
+        "sis_dcd_write_file": (i32, [C.c_char_p, i32, i32, P(d), i32, P(d), i32]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)

@@ -74,7 +74,10 @@ struct DevBuf {
 
 }  // namespace
 
+struct DcdOut;
+
 struct sis_gpu_ctx {
+  DcdOut *dcd = nullptr;   // trajectory output (sis_gpu_dcd_open), or none
   Params P{};
   Ptrs D{};
   sis_options opt{};
@@ -781,6 +784,8 @@ void step_phases(sis_gpu_ctx *c) {
   catch (const ApiError &e) { return fail(e.code, e.msg); }          \
   catch (const std::exception &e) { return fail(SIS_ERR_ARG, e.what()); }
 
+#include "sis_output_host.h"
+
 extern "C" {
 
 const char *sis_gpu_last_error(void) { return g_err.c_str(); }
@@ -975,6 +980,7 @@ int sis_gpu_create(const sis_topo *topo, const sis_params *prm, const sis_option
 
 int sis_gpu_destroy(sis_gpu_ctx *c) {
   API_BEGIN
+  if (c) dcd_shutdown(c);
   delete c;
   return SIS_OK;
   API_END
@@ -1294,12 +1300,28 @@ int sis_gpu_step(sis_gpu_ctx *c, int64_t nsteps, int64_t *rebuilds) {
   // each form rebuilds invalid lists itself and forces MC after it
   int mode = c->mode;
   if (mode == 0) mode = choose_mode(c);
-  if (mode == 2 && !c->parity) {
-    persistent_steps(c, nsteps);
-  } else if (mode == 3 && !c->parity) {
-    batched_steps(c, nsteps);
+  auto run = [&](int64_t n) {
+    if (mode == 2 && !c->parity) {
+      persistent_steps(c, n);
+    } else if (mode == 3 && !c->parity) {
+      batched_steps(c, n);
+    } else {
+      for (int64_t s = 0; s < n; s++) step_phases(c);
+    }
+  };
+  if (c->dcd && c->dcd->nstep_save > 0) {
+    // segments end at the save steps (istep a multiple of nstep_save,
+    // src/job_md.F90:485-508); each frame is copied out and written while
+    // the next segment runs
+    const int64_t every = c->dcd->nstep_save;
+    for (int64_t left = nsteps; left > 0;) {
+      const int64_t k = std::min(left, every - c->istep % every);
+      run(k);
+      left -= k;
+      if (c->istep % every == 0) dcd_enqueue(c);
+    }
   } else {
-    for (int64_t s = 0; s < nsteps; s++) step_phases(c);
+    run(nsteps);
   }
   HIPCHK(hipEventRecord(c->ev1, c->stream));
   HIPCHK(hipEventSynchronize(c->ev1));

@@ -1,0 +1,273 @@
+// Trajectory output: DCD files in the reference's layout, fed from the
+// device without stalling the step loop.  Included by sis_gpu.hip after the
+// context and the step drivers.
+//
+// File layout (src/dcd.F90:287-355 write_header, :538-596 write_onestep;
+// the file is opened form='unformatted', access='stream', :115-116, so the
+// block sizes are written explicitly and no record markers are added):
+//   int32 84, "CORD", 20 int32 control words (all 0 but word 11 = 1 when
+//   unit-cell blocks follow and word 20 = 24), int32 84;
+//   int32 84, int32 1 (titles), 80 blanks, int32 84;
+//   int32 4, int32 natom, int32 4;
+//   per frame: [int32 48, six float64 (box1, 0, box2, 0, 0, box3), int32 48]
+//   when the box is present, then for x, y and z in turn: int32 4N, N
+//   float32 (coordinate minus the centre of geometry when fix_com_origin),
+//   int32 4N.
+//
+// Device feed: at a save step the positions of all replicas are copied
+// into one of two pinned host buffers on the engine's stream; a writer
+// thread waits for the copy and writes one frame per replica file, so the
+// next segment of steps runs while the frames are written.
+#pragma once
+
+#include <condition_variable>
+#include <deque>
+#include <mutex>
+#include <thread>
+
+namespace {
+
+void dcd_put(FILE *f, const void *p, size_t n) {
+  if (fwrite(p, 1, n, f) != n) throw ApiError{SIS_ERR_ARG, "DCD write failed"};
+}
+void dcd_put_i32(FILE *f, int32_t v) { dcd_put(f, &v, 4); }
+
+void dcd_header(FILE *f, int32_t natom, bool unitcell) {
+  int32_t control[20] = {0};
+  control[10] = unitcell ? 1 : 0;
+  control[19] = 24;
+  dcd_put_i32(f, 84);
+  dcd_put(f, "CORD", 4);
+  dcd_put(f, control, sizeof control);
+  dcd_put_i32(f, 84);
+  char title[80];
+  memset(title, ' ', sizeof title);
+  dcd_put_i32(f, 4 + 80);
+  dcd_put_i32(f, 1);
+  dcd_put(f, title, sizeof title);
+  dcd_put_i32(f, 4 + 80);
+  dcd_put_i32(f, 4);
+  dcd_put_i32(f, natom);
+  dcd_put_i32(f, 4);
+  fflush(f);
+}
+
+// One frame; xyz(k, a) at xyz[stride * a + k] (stride 3: the reference's
+// xyz(3, nmp); stride 4: the device's double4 layout).
+void dcd_frame(FILE *f, int nmp, const double *xyz, int stride, bool unitcell, const double box[3], int fix_com,
+               std::vector<float> &tmp) {
+  if (unitcell) {
+    const double cell[6] = {box[0], 0.0, box[1], 0.0, 0.0, box[2]};
+    dcd_put_i32(f, 48);
+    dcd_put(f, cell, sizeof cell);
+    dcd_put_i32(f, 48);
+  }
+  double com[3] = {0.0, 0.0, 0.0};
+  if (fix_com > 0) {
+    for (int k = 0; k < 3; k++) {
+      double s = 0.0;
+      for (int a = 0; a < nmp; a++) s += xyz[(size_t)stride * a + k];
+      com[k] = s / (double)nmp;
+    }
+  }
+  tmp.resize(nmp);
+  const int32_t num = nmp * 4;
+  for (int k = 0; k < 3; k++) {
+    for (int a = 0; a < nmp; a++) tmp[a] = (float)(xyz[(size_t)stride * a + k] - com[k]);
+    dcd_put_i32(f, num);
+    dcd_put(f, tmp.data(), (size_t)nmp * 4);
+    dcd_put_i32(f, num);
+  }
+  fflush(f);
+}
+
+}  // namespace
+
+struct DcdOut {
+  std::vector<FILE *> f;
+  int fix_com = 0;
+  int64_t nstep_save = 0;
+  bool unitcell = false;
+  double box[3] = {0, 0, 0};
+  int device = 0;
+  // two pinned staging buffers of R*N double4
+  double4 *buf[2] = {nullptr, nullptr};
+  hipEvent_t ev[2] = {nullptr, nullptr};
+  bool busy[2] = {false, false};
+  int next = 0;
+  std::thread th;
+  std::mutex mu;
+  std::condition_variable cv;
+  std::deque<int> q;
+  bool stop = false;
+  std::string err;
+  int64_t frames = 0;
+};
+
+namespace {
+
+void dcd_writer_loop(sis_gpu_ctx *c, DcdOut *o) {
+  (void)hipSetDevice(o->device);
+  std::vector<float> tmp;
+  for (;;) {
+    int b;
+    {
+      std::unique_lock<std::mutex> lk(o->mu);
+      o->cv.wait(lk, [o] { return o->stop || !o->q.empty(); });
+      if (o->q.empty()) return;
+      b = o->q.front();
+      o->q.pop_front();
+    }
+    std::string e;
+    if (hipEventSynchronize(o->ev[b]) != hipSuccess) e = "DCD staging copy failed";
+    try {
+      if (e.empty())
+        for (int r = 0; r < c->R; r++)
+          dcd_frame(o->f[r], c->N, reinterpret_cast<const double *>(o->buf[b] + (size_t)r * c->N), 4, o->unitcell,
+                    o->box, o->fix_com, tmp);
+    } catch (const ApiError &ex) {
+      e = ex.msg;
+    }
+    std::lock_guard<std::mutex> lk(o->mu);
+    if (!e.empty() && o->err.empty()) o->err = e;
+    o->busy[b] = false;
+    o->frames++;
+    o->cv.notify_all();
+  }
+}
+
+// queue the current positions of every replica as one frame each
+void dcd_enqueue(sis_gpu_ctx *c) {
+  DcdOut *o = c->dcd;
+  int b;
+  {
+    std::unique_lock<std::mutex> lk(o->mu);
+    b = o->next;
+    o->cv.wait(lk, [o, b] { return !o->busy[b]; });
+    if (!o->err.empty()) throw ApiError{SIS_ERR_ARG, o->err};
+    o->busy[b] = true;
+    o->next ^= 1;
+  }
+  const size_t RN = (size_t)c->R * c->N;
+  HIPCHK(hipMemcpyAsync(o->buf[b], c->pos_d.p, RN * sizeof(double4), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipEventRecord(o->ev[b], c->stream));
+  std::lock_guard<std::mutex> lk(o->mu);
+  o->q.push_back(b);
+  o->cv.notify_all();
+}
+
+// wait for queued frames; rethrows a writer failure
+void dcd_drain(DcdOut *o) {
+  std::unique_lock<std::mutex> lk(o->mu);
+  o->cv.wait(lk, [o] { return o->q.empty() && !o->busy[0] && !o->busy[1]; });
+  if (!o->err.empty()) {
+    const std::string e = o->err;
+    o->err.clear();
+    throw ApiError{SIS_ERR_ARG, e};
+  }
+}
+
+void dcd_shutdown(sis_gpu_ctx *c) {
+  DcdOut *o = c->dcd;
+  if (!o) return;
+  {
+    std::lock_guard<std::mutex> lk(o->mu);
+    o->stop = true;
+    o->cv.notify_all();
+  }
+  if (o->th.joinable()) o->th.join();
+  for (FILE *f : o->f)
+    if (f) fclose(f);
+  for (int b = 0; b < 2; b++) {
+    if (o->buf[b]) (void)hipHostFree(o->buf[b]);
+    if (o->ev[b]) (void)hipEventDestroy(o->ev[b]);
+  }
+  delete o;
+  c->dcd = nullptr;
+}
+
+}  // namespace
+
+extern "C" {
+
+int sis_dcd_write_file(const char *path, int32_t nmp, int32_t nframes, const double *xyz, int32_t unitcell,
+                       const double *box, int32_t fix_com_origin) {
+  API_BEGIN
+  if (!path || nmp <= 0 || nframes < 0 || (nframes > 0 && !xyz) || (unitcell && !box))
+    return fail(SIS_ERR_ARG, "invalid argument");
+  FILE *f = fopen(path, "wb");
+  if (!f) return fail(SIS_ERR_ARG, std::string("cannot open ") + path);
+  try {
+    dcd_header(f, nmp, unitcell != 0);
+    std::vector<float> tmp;
+    for (int k = 0; k < nframes; k++)
+      dcd_frame(f, nmp, xyz + (size_t)3 * nmp * k, 3, unitcell != 0, box, fix_com_origin, tmp);
+  } catch (...) {
+    fclose(f);
+    throw;
+  }
+  fclose(f);
+  return SIS_OK;
+  API_END
+}
+
+int sis_gpu_dcd_open(sis_gpu_ctx *c, const char *const *paths, int32_t fix_com_origin, int64_t nstep_save) {
+  API_BEGIN
+  if (!c || !paths || nstep_save < 0) return fail(SIS_ERR_ARG, "invalid argument");
+  if (c->dcd) return fail(SIS_ERR_STATE, "trajectory output already open");
+  DcdOut *o = new DcdOut();
+  c->dcd = o;
+  try {
+    o->fix_com = fix_com_origin;
+    o->nstep_save = nstep_save;
+    o->unitcell = c->prm.pbc != 0;
+    for (int k = 0; k < 3; k++) o->box[k] = c->prm.box[k];
+    HIPCHK(hipGetDevice(&o->device));
+    const size_t RN = (size_t)c->R * c->N;
+    for (int b = 0; b < 2; b++) {
+      HIPCHK(hipHostMalloc((void **)&o->buf[b], RN * sizeof(double4), hipHostMallocDefault));
+      HIPCHK(hipEventCreateWithFlags(&o->ev[b], hipEventDisableTiming));
+    }
+    for (int r = 0; r < c->R; r++) {
+      if (!paths[r]) throw ApiError{SIS_ERR_ARG, "null DCD path"};
+      FILE *f = fopen(paths[r], "wb");
+      if (!f) throw ApiError{SIS_ERR_ARG, std::string("cannot open ") + paths[r]};
+      o->f.push_back(f);
+      dcd_header(f, c->N, o->unitcell);
+    }
+    o->th = std::thread(dcd_writer_loop, c, o);
+  } catch (...) {
+    dcd_shutdown(c);
+    throw;
+  }
+  return SIS_OK;
+  API_END
+}
+
+int sis_gpu_dcd_write(sis_gpu_ctx *c) {
+  API_BEGIN
+  if (!c) return fail(SIS_ERR_ARG, "null context");
+  if (!c->dcd) return fail(SIS_ERR_STATE, "trajectory output not open");
+  dcd_enqueue(c);
+  dcd_drain(c->dcd);
+  return SIS_OK;
+  API_END
+}
+
+int sis_gpu_dcd_close(sis_gpu_ctx *c) {
+  API_BEGIN
+  if (!c) return fail(SIS_ERR_ARG, "null context");
+  if (!c->dcd) return SIS_OK;
+  std::string e;
+  try {
+    dcd_drain(c->dcd);
+  } catch (const ApiError &ex) {
+    e = ex.msg;
+  }
+  dcd_shutdown(c);
+  if (!e.empty()) return fail(SIS_ERR_ARG, e);
+  return SIS_OK;
+  API_END
+}
+
+}  // extern "C"

@@ -177,6 +177,17 @@ class Engine:
         _check(lib().sis_gpu_exchange(self.h, abi.dptr(e), abi.iptr(acc)))
         return e, acc
 
+    # ---- trajectory output (DCD, one file per replica) ----
+    def dcd_open(self, paths, fix_com_origin: int = 0, nstep_save: int = 0):
+        arr = (C.c_char_p * self.nrep)(*[str(x).encode() for x in paths])
+        _check(lib().sis_gpu_dcd_open(self.h, arr, int(fix_com_origin), int(nstep_save)))
+
+    def dcd_write(self):
+        _check(lib().sis_gpu_dcd_write(self.h))
+
+    def dcd_close(self):
+        _check(lib().sis_gpu_dcd_close(self.h))
+
     def labels(self) -> np.ndarray:
         lab = np.zeros(self.nrep, np.int32)
         _check(lib().sis_gpu_get_labels(self.h, abi.iptr(lab)))
@@ -190,3 +201,13 @@ class Engine:
 
     def last_ms(self, which: int = 0) -> float:
         return float(lib().sis_gpu_last_kernel_ms(self.h, which))
+
+
+def write_dcd_file(path, xyz_frames, box=None, fix_com_origin: int = 0):
+    """Host-side DCD writer of the library (the format of src/dcd.F90):
+    xyz_frames (nframes, nmp, 3); box (3,) adds the unit-cell blocks."""
+    x = np.ascontiguousarray(np.asarray(xyz_frames, np.float64))
+    nfr, nmp = x.shape[0], x.shape[1]
+    b = np.ascontiguousarray(np.asarray(box if box is not None else [0.0, 0.0, 0.0], np.float64))
+    _check(lib().sis_dcd_write_file(str(path).encode(), nmp, nfr, abi.dptr(x), 1 if box is not None else 0,
+                                    abi.dptr(b), int(fix_com_origin)))

@@ -1,0 +1,55 @@
+"""Trajectory output (DCD) of the library against the reference's own
+writer (file_dcd of src/dcd.F90, compiled from the reference sources into
+oracle/_ref): byte-identical files on the same frames.  CPU only: the host
+writer needs no GPU."""
+import numpy as np
+import pytest
+
+from oracle import refdrv
+from sis_amd.engine import write_dcd_file
+
+needs_ref = pytest.mark.skipif(not refdrv.available(), reason="reference build (oracle/_ref) absent")
+
+
+def _frames(seed, nframes, nmp, scale=300.0):
+    rng = np.random.default_rng(seed)
+    return rng.uniform(-scale, scale, size=(nframes, nmp, 3))
+
+
+@needs_ref
+@pytest.mark.parametrize("box,fix_com", [(None, 0), (None, 1), ((250.0, 260.5, 1.0e3 / 3.0), 0),
+                                         ((2499.1, 2499.1, 2499.1), 1)])
+def test_dcd_bytes_match_reference_writer(tmp_path, box, fix_com):
+    x = _frames(7, 5, 129)
+    ours, ref = tmp_path / "ours.dcd", tmp_path / "ref.dcd"
+    write_dcd_file(ours, x, box=box, fix_com_origin=fix_com)
+    refdrv.write_dcd(ref, x, box=None if box is None else np.array(box), fix_com=fix_com)
+    a, b = ours.read_bytes(), ref.read_bytes()
+    assert len(a) == len(b)
+    assert a == b
+
+
+def test_dcd_layout_known_answer(tmp_path):
+    """Header and one frame decoded field by field (the layout of
+    src/dcd.F90:287-355,538-596), without the reference build."""
+    x = np.array([[[1.0, 2.0, 3.0], [4.0, 5.0, 6.5]]])
+    p = tmp_path / "a.dcd"
+    write_dcd_file(p, x, box=(10.0, 20.0, 30.0), fix_com_origin=1)
+    raw = p.read_bytes()
+    i32 = lambda o: int(np.frombuffer(raw, np.int32, 1, o)[0])  # noqa: E731
+    assert i32(0) == 84 and raw[4:8] == b"CORD"
+    ctrl = np.frombuffer(raw, np.int32, 20, 8)
+    assert ctrl[10] == 1 and ctrl[19] == 24 and ctrl[:10].sum() == 0
+    assert i32(88) == 84 and i32(92) == 84 and i32(96) == 1 and raw[100:180] == b" " * 80
+    assert i32(180) == 84 and i32(184) == 4 and i32(188) == 2 and i32(192) == 4
+    o = 196
+    assert i32(o) == 48
+    assert np.array_equal(np.frombuffer(raw, np.float64, 6, o + 4), [10.0, 0.0, 20.0, 0.0, 0.0, 30.0])
+    o += 56
+    com = x[0].mean(axis=0)
+    for k in range(3):
+        assert i32(o) == 8
+        assert np.array_equal(np.frombuffer(raw, np.float32, 2, o + 4), (x[0, :, k] - com[k]).astype(np.float32))
+        assert i32(o + 12) == 8
+        o += 16
+    assert o == len(raw)
