@@ -173,6 +173,26 @@ module sis_gpu_iface
          type(c_ptr), value :: ctx, rep2lab
       end function
 
+      ! trajectory output (DCD, src/dcd.F90 layout); paths is an array of
+      ! nrep c_ptr to NUL-terminated file names
+      integer(c_int) function sis_gpu_dcd_open(ctx, paths, fix_com_origin, nstep_save) &
+            bind(C, name="sis_gpu_dcd_open")
+         import :: c_int, c_ptr, c_int32_t, c_int64_t
+         type(c_ptr), value :: ctx, paths
+         integer(c_int32_t), value :: fix_com_origin
+         integer(c_int64_t), value :: nstep_save
+      end function
+
+      integer(c_int) function sis_gpu_dcd_write(ctx) bind(C, name="sis_gpu_dcd_write")
+         import :: c_int, c_ptr
+         type(c_ptr), value :: ctx
+      end function
+
+      integer(c_int) function sis_gpu_dcd_close(ctx) bind(C, name="sis_gpu_dcd_close")
+         import :: c_int, c_ptr
+         type(c_ptr), value :: ctx
+      end function
+
       integer(c_int) function sis_gpu_get_info(ctx, out, n) bind(C, name="sis_gpu_get_info")
          import :: c_int, c_ptr, c_int32_t
          type(c_ptr), value :: ctx, out
