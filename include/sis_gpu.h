@@ -168,6 +168,17 @@ int sis_gpu_exchange(sis_gpu_ctx *ctx, double *replica_energies, int32_t *accept
 /* rep2lab(1:nrep) (src/var_replica.F90:49). */
 int sis_gpu_get_labels(sis_gpu_ctx *ctx, int32_t *rep2lab);
 
+/* Restart: the STEP and REPLICA blocks of a restart file
+ * (src/write_rst.F90:40-60, read back by src/read_rst.F90).  set_step
+ * sets istep, which keys the performance-mode random streams, so a run
+ * restarted from sis_gpu_get_state / get_labels / get_info continues the
+ * same streams.  set_labels takes rep2lab(1:nrep) after
+ * sis_gpu_set_rep_conditions gave the temperature of each label; each
+ * replica takes its label's temperature, and the exchange pair table
+ * starts from its first half as the reference's does after a restart. */
+int sis_gpu_set_step(sis_gpu_ctx *ctx, int64_t istep);
+int sis_gpu_set_labels(sis_gpu_ctx *ctx, const int32_t *rep2lab);
+
 /* Trajectory output in the reference's DCD layout (file_dcd of
  * src/dcd.F90: write_header :287-355, write_onestep :538-596), one file
  * per replica as job_md opens them (src/job_md.F90:294-303).  The unit

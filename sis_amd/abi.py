@@ -121,6 +121,7 @@ EXPORTED = [
     "sis_pool_create", "sis_pool_submit", "sis_pool_poll", "sis_pool_forget",
     "sis_pool_stats", "sis_pool_destroy",
     "sis_gpu_dcd_open", "sis_gpu_dcd_write", "sis_gpu_dcd_close", "sis_dcd_write_file",
+    "sis_gpu_set_step", "sis_gpu_set_labels",
 ]
 
 
@@ -157,6 +158,8 @@ def _declare(lib: C.CDLL) -> None:
         "sis_pool_destroy": (i32, [vp]),
         "sis_gpu_dcd_open": (i32, [vp, P(C.c_char_p), i32, i64]),
         "sis_gpu_dcd_write": (i32, [vp]),
+        "sis_gpu_set_step": (i32, [vp, i64]),
+        "sis_gpu_set_labels": (i32, [vp, P(i32)]),
         "sis_gpu_dcd_close": (i32, [vp]),
         "sis_dcd_write_file": (i32, [C.c_char_p, i32, i32, P(d), i32, P(d), i32]),
     }

@@ -1344,6 +1344,40 @@ int sis_gpu_exchange(sis_gpu_ctx *c, double *replica_energies, int32_t *accepted
   API_END
 }
 
+int sis_gpu_set_step(sis_gpu_ctx *c, int64_t istep) {
+  API_BEGIN
+  if (!c) return fail(SIS_ERR_ARG, "null context");
+  if (istep < 0) return fail(SIS_ERR_ARG, "istep < 0");
+  c->istep = istep;
+  return SIS_OK;
+  API_END
+}
+
+int sis_gpu_set_labels(sis_gpu_ctx *c, const int32_t *rep2lab) {
+  API_BEGIN
+  if (!c || !rep2lab) return fail(SIS_ERR_ARG, "null argument");
+  const int R = c->R;
+  std::vector<int> seen(R + 1, 0);
+  for (int r = 0; r < R; r++) {
+    if (rep2lab[r] < 1 || rep2lab[r] > R || seen[rep2lab[r]]) return fail(SIS_ERR_ARG, "labels are not a permutation of 1..nrep");
+    seen[rep2lab[r]] = 1;
+  }
+  for (int r = 0; r < R; r++) {
+    c->rep2lab[r + 1] = rep2lab[r];
+    c->lab2rep[rep2lab[r]] = r + 1;
+    c->tK[r] = c->lab2val[rep2lab[r]];
+  }
+  // the pair table restarts from its first half, as var_replica's idx_pair
+  // does on a restarted run (it is not in the restart file,
+  // src/var_replica.F90:85, src/write_rst.F90:40-52)
+  c->idx_pair = 0;
+  upload_conditions(c);
+  setup_cells(c);
+  c->lists_valid = false;
+  return SIS_OK;
+  API_END
+}
+
 int sis_gpu_get_labels(sis_gpu_ctx *c, int32_t *rep2lab) {
   API_BEGIN
   if (!c || !rep2lab) return fail(SIS_ERR_ARG, "null argument");

@@ -188,6 +188,14 @@ class Engine:
     def dcd_close(self):
         _check(lib().sis_gpu_dcd_close(self.h))
 
+    # ---- restart ----
+    def set_step(self, istep: int):
+        _check(lib().sis_gpu_set_step(self.h, int(istep)))
+
+    def set_labels(self, rep2lab):
+        lab = np.ascontiguousarray(rep2lab, np.int32)
+        _check(lib().sis_gpu_set_labels(self.h, abi.iptr(lab)))
+
     def labels(self) -> np.ndarray:
         lab = np.zeros(self.nrep, np.int32)
         _check(lib().sis_gpu_get_labels(self.h, abi.iptr(lab)))

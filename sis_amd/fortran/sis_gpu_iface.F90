@@ -193,6 +193,17 @@ module sis_gpu_iface
          type(c_ptr), value :: ctx
       end function
 
+      integer(c_int) function sis_gpu_set_step(ctx, istep) bind(C, name="sis_gpu_set_step")
+         import :: c_int, c_ptr, c_int64_t
+         type(c_ptr), value :: ctx
+         integer(c_int64_t), value :: istep
+      end function
+
+      integer(c_int) function sis_gpu_set_labels(ctx, rep2lab) bind(C, name="sis_gpu_set_labels")
+         import :: c_int, c_ptr
+         type(c_ptr), value :: ctx, rep2lab
+      end function
+
       integer(c_int) function sis_gpu_get_info(ctx, out, n) bind(C, name="sis_gpu_get_info")
          import :: c_int, c_ptr, c_int32_t
          type(c_ptr), value :: ctx, out

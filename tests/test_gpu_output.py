@@ -44,3 +44,36 @@ def test_dcd_frames_during_steps(tmp_path, case):
         want = tmp_path / f"host_{r}.dcd"
         write_dcd_file(want, frames[:, r], box=box, fix_com_origin=fix_com)
         assert paths[r].read_bytes() == want.read_bytes(), r
+
+
+@pytest.mark.parametrize("mode", ["batched", "phases"])
+def test_restart_continues_the_run(mode):
+    """A run stopped after two exchanges and restarted in a new context from
+    its state, step count and labels (the restart-file blocks) follows the
+    uninterrupted run: same exchanges, trajectories equal to rounding (the
+    restarted context rebuilds its lists at the restart, which changes the
+    neighbour order and hence the FP64 summation order)."""
+    s = configs.remd_c3(nrep=8, n=128, seed=3)
+    print(f"[case] restart {mode}", flush=True)
+    kw = dict(mode=mode, seed=77, exchange_every=20, precision=0)
+    a = Engine(s, **kw)
+    a.init_velocities()
+    a.neighbor_list(0)
+    a.step(80)
+    xa, va, aa = a.get_state()
+    b = Engine(s, **kw)
+    b.init_velocities()
+    b.neighbor_list(0)
+    b.step(40)
+    x, v, acc = b.get_state()
+    lab, istep = b.labels(), b.info()["istep"]
+    c = Engine(s, **kw)
+    c.set_labels(lab)
+    c.set_state(x, v, acc)
+    c.set_step(istep)
+    c.step(40)
+    xc, vc, _ = c.get_state()
+    assert c.info()["istep"] == 80
+    assert np.array_equal(a.labels(), c.labels())
+    assert np.abs(xa - xc).max() < 1e-6
+    assert np.abs(va - vc).max() < 1e-5
