@@ -196,6 +196,25 @@ int sis_gpu_dcd_open(sis_gpu_ctx *ctx, const char *const *paths, int32_t fix_com
 int sis_gpu_dcd_write(sis_gpu_ctx *ctx);
 int sis_gpu_dcd_close(sis_gpu_ctx *ctx);
 
+/* Base-pair output in the layout of src/write_bp.F90 (header of
+ * src/init_out_files.F90:102-104): kind 0 = .bp (formed pairs with
+ * E <= -kT, src/write_bp.F90:28-51), kind 1 = .bpall (all formed pairs,
+ * :53-82); records (int16 i, int16 j, float32 E) in list order and a
+ * (0, 0, 0.0) terminator per save step.  With nstep_save > 0 sis_gpu_step
+ * writes every replica's records at the save steps, before that step's
+ * exchange (the save block precedes the exchange block in
+ * src/job_md.F90:485-570); sis_gpu_bp_write writes the current state (the
+ * initial write_bp of src/job_md.F90:336), which is that of the last
+ * force or energy evaluation. */
+int sis_gpu_bp_open(sis_gpu_ctx *ctx, const char *const *paths, int32_t kind, int64_t nstep_save);
+int sis_gpu_bp_write(sis_gpu_ctx *ctx);
+int sis_gpu_bp_close(sis_gpu_ctx *ctx);
+
+/* The same writer on host arrays: nsets save steps of n candidates each,
+ * ij (2, n) 1-based bead ids, status (n) and ene (n) per set. */
+int sis_bp_write_file(const char *path, int64_t n, const int32_t *ij, const uint8_t *status, const double *ene,
+                      int32_t nsets, double tempK, int32_t kind);
+
 /* The same DCD writer on host arrays: nframes frames of xyz(3, nmp)
  * (Fortran layout, frames consecutive); box[3] is used when unitcell != 0.
  * Needs no GPU. */

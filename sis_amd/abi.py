@@ -122,6 +122,7 @@ EXPORTED = [
     "sis_pool_stats", "sis_pool_destroy",
     "sis_gpu_dcd_open", "sis_gpu_dcd_write", "sis_gpu_dcd_close", "sis_dcd_write_file",
     "sis_gpu_set_step", "sis_gpu_set_labels",
+    "sis_gpu_bp_open", "sis_gpu_bp_write", "sis_gpu_bp_close", "sis_bp_write_file",
 ]
 
 
@@ -159,6 +160,10 @@ def _declare(lib: C.CDLL) -> None:
         "sis_gpu_dcd_open": (i32, [vp, P(C.c_char_p), i32, i64]),
         "sis_gpu_dcd_write": (i32, [vp]),
         "sis_gpu_set_step": (i32, [vp, i64]),
+        "sis_gpu_bp_open": (i32, [vp, P(C.c_char_p), i32, i64]),
+        "sis_gpu_bp_write": (i32, [vp]),
+        "sis_gpu_bp_close": (i32, [vp]),
+        "sis_bp_write_file": (i32, [C.c_char_p, i64, P(i32), P(C.c_uint8), P(d), i32, d, i32]),
         "sis_gpu_set_labels": (i32, [vp, P(i32)]),
         "sis_gpu_dcd_close": (i32, [vp]),
         "sis_dcd_write_file": (i32, [C.c_char_p, i32, i32, P(d), i32, P(d), i32]),

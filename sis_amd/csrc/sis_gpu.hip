@@ -75,9 +75,11 @@ struct DevBuf {
 }  // namespace
 
 struct DcdOut;
+struct BpOut;
 
 struct sis_gpu_ctx {
   DcdOut *dcd = nullptr;   // trajectory output (sis_gpu_dcd_open), or none
+  BpOut *bpo = nullptr;    // base-pair output (sis_gpu_bp_open), or none
   Params P{};
   Ptrs D{};
   sis_options opt{};
@@ -767,7 +769,6 @@ void step_phases(sis_gpu_ctx *c) {
   c->tmark();
   std::swap(c->pos_d, c->pos2_d);   // updated positions become current
   hipLaunchKernelGGL(k_skin, dim3(R), dim3(64), 0, c->stream, c->P, c->ptrs(), c->nblk_force);
-  if (c->opt.exchange_every > 0 && R > 1 && c->istep % c->opt.exchange_every == 0) do_exchange(c, nullptr, nullptr);
 }
 
 }  // namespace
@@ -980,7 +981,10 @@ int sis_gpu_create(const sis_topo *topo, const sis_params *prm, const sis_option
 
 int sis_gpu_destroy(sis_gpu_ctx *c) {
   API_BEGIN
-  if (c) dcd_shutdown(c);
+  if (c) {
+    dcd_shutdown(c);
+    bp_shutdown(c);
+  }
   delete c;
   return SIS_OK;
   API_END
@@ -1309,19 +1313,22 @@ int sis_gpu_step(sis_gpu_ctx *c, int64_t nsteps, int64_t *rebuilds) {
       for (int64_t s = 0; s < n; s++) step_phases(c);
     }
   };
-  if (c->dcd && c->dcd->nstep_save > 0) {
-    // segments end at the save steps (istep a multiple of nstep_save,
-    // src/job_md.F90:485-508); each frame is copied out and written while
-    // the next segment runs
-    const int64_t every = c->dcd->nstep_save;
-    for (int64_t left = nsteps; left > 0;) {
-      const int64_t k = std::min(left, every - c->istep % every);
-      run(k);
-      left -= k;
-      if (c->istep % every == 0) dcd_enqueue(c);
-    }
-  } else {
-    run(nsteps);
+  // segments end at save steps and exchange steps; at such a step the
+  // frames are queued first, then the replicas exchange, the order of the
+  // save and exchange blocks of src/job_md.F90:485-570
+  const int64_t ex = (c->opt.exchange_every > 0 && c->R > 1) ? c->opt.exchange_every : 0;
+  const int64_t sv = (c->dcd && c->dcd->nstep_save > 0) ? c->dcd->nstep_save : 0;
+  const int64_t sb = (c->bpo && c->bpo->nstep_save > 0) ? c->bpo->nstep_save : 0;
+  for (int64_t left = nsteps; left > 0;) {
+    int64_t k = left;
+    if (ex) k = std::min(k, ex - c->istep % ex);
+    if (sv) k = std::min(k, sv - c->istep % sv);
+    if (sb) k = std::min(k, sb - c->istep % sb);
+    run(k);
+    left -= k;
+    if (sv && c->istep % sv == 0) dcd_enqueue(c);
+    if (sb && c->istep % sb == 0) bp_write_now(c);
+    if (ex && c->istep % ex == 0) do_exchange(c, nullptr, nullptr);
   }
   HIPCHK(hipEventRecord(c->ev1, c->stream));
   HIPCHK(hipEventSynchronize(c->ev1));

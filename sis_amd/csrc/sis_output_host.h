@@ -188,7 +188,137 @@ void dcd_shutdown(sis_gpu_ctx *c) {
 
 }  // namespace
 
+// Base-pair output (src/write_bp.F90:23-51 and :53-82; header written at
+// open by src/init_out_files.F90:102-104, stream access): per save step,
+// records (int16 i, int16 j, float32 E) of the formed pairs in list order,
+// those with E <= -kT for .bp (kind 0) or all for .bpall (kind 1), then a
+// (0, 0, 0.0) terminator.
+struct BpOut {
+  std::vector<FILE *> f;
+  int kind = 0;
+  int64_t nstep_save = 0;
+};
+
+namespace {
+
+void bp_header(FILE *f) {
+  dcd_put_i32(f, 2);
+  dcd_put_i32(f, 4);
+  fflush(f);
+}
+
+void bp_records(FILE *f, int64_t n, const int32_t *ij, int ij_stride, const uint8_t *status, const double *ene,
+                double kT, int kind) {
+  for (int64_t k = 0; k < n; k++) {
+    if (!status[k]) continue;
+    if (kind == 0 && !(ene[k] <= -kT)) continue;
+    const int16_t i = (int16_t)ij[(size_t)ij_stride * k], j = (int16_t)ij[(size_t)ij_stride * k + 1];
+    const float e = (float)ene[k];
+    dcd_put(f, &i, 2);
+    dcd_put(f, &j, 2);
+    dcd_put(f, &e, 4);
+  }
+  const int16_t z = 0;
+  const float ez = 0.0f;
+  dcd_put(f, &z, 2);
+  dcd_put(f, &z, 2);
+  dcd_put(f, &ez, 4);
+  fflush(f);
+}
+
+// one save step of every replica, from the packed candidate arrays
+void bp_write_now(sis_gpu_ctx *c) {
+  BpOut *o = c->bpo;
+  const size_t nb = c->nbp_total;
+  std::vector<int32_t> mp(nb * 8);
+  std::vector<uint8_t> st(nb);
+  std::vector<double> en(nb);
+  if (nb) {
+    HIPCHK(hipMemcpyAsync(mp.data(), c->bp_mp_d.p, nb * 8 * sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpyAsync(st.data(), c->bp_status_d.p, nb, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpyAsync(en.data(), c->ene_bp_d.p, nb * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  }
+  HIPCHK(hipStreamSynchronize(c->stream));
+  // candidates of replica r: [off[r N], off[(r + 1) N]) of the per-bead offsets
+  const std::vector<int> ro = packed_rep_off(c);
+  for (int r = 0; r < c->R; r++) {
+    const int64_t b0 = ro[r], b1 = ro[r + 1];
+    bp_records(o->f[r], b1 - b0, mp.data() + (size_t)b0 * 8, 8, st.data() + b0, en.data() + b0,
+               c->tK[r] * KB_KCAL, o->kind);
+  }
+}
+
+void bp_shutdown(sis_gpu_ctx *c) {
+  if (!c->bpo) return;
+  for (FILE *f : c->bpo->f)
+    if (f) fclose(f);
+  delete c->bpo;
+  c->bpo = nullptr;
+}
+
+}  // namespace
+
 extern "C" {
+
+int sis_bp_write_file(const char *path, int64_t n, const int32_t *ij, const uint8_t *status, const double *ene,
+                      int32_t nsets, double tempK, int32_t kind) {
+  API_BEGIN
+  if (!path || n < 0 || nsets < 0 || (n > 0 && (!ij || !status || !ene)) || (kind != 0 && kind != 1))
+    return fail(SIS_ERR_ARG, "invalid argument");
+  FILE *f = fopen(path, "wb");
+  if (!f) return fail(SIS_ERR_ARG, std::string("cannot open ") + path);
+  try {
+    bp_header(f);
+    for (int s = 0; s < nsets; s++)
+      bp_records(f, n, ij + (size_t)2 * n * s, 2, status + (size_t)n * s, ene + (size_t)n * s, tempK * KB_KCAL, kind);
+  } catch (...) {
+    fclose(f);
+    throw;
+  }
+  fclose(f);
+  return SIS_OK;
+  API_END
+}
+
+int sis_gpu_bp_open(sis_gpu_ctx *c, const char *const *paths, int32_t kind, int64_t nstep_save) {
+  API_BEGIN
+  if (!c || !paths || nstep_save < 0 || (kind != 0 && kind != 1)) return fail(SIS_ERR_ARG, "invalid argument");
+  if (c->bpo) return fail(SIS_ERR_STATE, "base-pair output already open");
+  c->bpo = new BpOut();
+  try {
+    c->bpo->kind = kind;
+    c->bpo->nstep_save = nstep_save;
+    for (int r = 0; r < c->R; r++) {
+      if (!paths[r]) throw ApiError{SIS_ERR_ARG, "null path"};
+      FILE *f = fopen(paths[r], "wb");
+      if (!f) throw ApiError{SIS_ERR_ARG, std::string("cannot open ") + paths[r]};
+      c->bpo->f.push_back(f);
+      bp_header(f);
+    }
+  } catch (...) {
+    bp_shutdown(c);
+    throw;
+  }
+  return SIS_OK;
+  API_END
+}
+
+int sis_gpu_bp_write(sis_gpu_ctx *c) {
+  API_BEGIN
+  if (!c) return fail(SIS_ERR_ARG, "null context");
+  if (!c->bpo) return fail(SIS_ERR_STATE, "base-pair output not open");
+  bp_write_now(c);
+  return SIS_OK;
+  API_END
+}
+
+int sis_gpu_bp_close(sis_gpu_ctx *c) {
+  API_BEGIN
+  if (!c) return fail(SIS_ERR_ARG, "null context");
+  bp_shutdown(c);
+  return SIS_OK;
+  API_END
+}
 
 int sis_dcd_write_file(const char *path, int32_t nmp, int32_t nframes, const double *xyz, int32_t unitcell,
                        const double *box, int32_t fix_com_origin) {

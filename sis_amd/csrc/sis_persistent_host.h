@@ -299,6 +299,5 @@ void batched_steps(sis_gpu_ctx *c, int64_t nsteps) {
     std::fill(mcf.begin(), mcf.end(), 0);
     c->istep += seg;
     left -= seg;
-    if (ex > 0 && c->istep % ex == 0) do_exchange(c, nullptr, nullptr);
   }
 }

@@ -188,6 +188,16 @@ class Engine:
     def dcd_close(self):
         _check(lib().sis_gpu_dcd_close(self.h))
 
+    def bp_open(self, paths, kind: int = 0, nstep_save: int = 0):
+        arr = (C.c_char_p * self.nrep)(*[str(x).encode() for x in paths])
+        _check(lib().sis_gpu_bp_open(self.h, arr, int(kind), int(nstep_save)))
+
+    def bp_write(self):
+        _check(lib().sis_gpu_bp_write(self.h))
+
+    def bp_close(self):
+        _check(lib().sis_gpu_bp_close(self.h))
+
     # ---- restart ----
     def set_step(self, istep: int):
         _check(lib().sis_gpu_set_step(self.h, int(istep)))
@@ -219,3 +229,17 @@ def write_dcd_file(path, xyz_frames, box=None, fix_com_origin: int = 0):
     b = np.ascontiguousarray(np.asarray(box if box is not None else [0.0, 0.0, 0.0], np.float64))
     _check(lib().sis_dcd_write_file(str(path).encode(), nmp, nfr, abi.dptr(x), 1 if box is not None else 0,
                                     abi.dptr(b), int(fix_com_origin)))
+
+
+def write_bp_file(path, sets, tempK: float, kind: int = 0):
+    """Host-side base-pair writer of the library (src/write_bp.F90 layout):
+    sets is a list of (ij (n, 2) 1-based, status (n,), ene (n,)) of equal n."""
+    n = len(sets[0][1]) if sets else 0
+    ij = np.ascontiguousarray(np.concatenate([np.asarray(s[0], np.int32).reshape(n, 2) for s in sets]) if sets
+                              else np.zeros((0, 2), np.int32))
+    st = np.ascontiguousarray(np.concatenate([np.asarray(s[1], np.uint8) for s in sets]) if sets
+                              else np.zeros(0, np.uint8))
+    en = np.ascontiguousarray(np.concatenate([np.asarray(s[2], np.float64) for s in sets]) if sets
+                              else np.zeros(0))
+    _check(lib().sis_bp_write_file(str(path).encode(), n, abi.iptr(ij), abi.u8ptr(st), abi.dptr(en), len(sets),
+                                   float(tempK), int(kind)))

@@ -193,6 +193,24 @@ module sis_gpu_iface
          type(c_ptr), value :: ctx
       end function
 
+      ! base-pair output (src/write_bp.F90 layout); kind 0 = .bp, 1 = .bpall
+      integer(c_int) function sis_gpu_bp_open(ctx, paths, kind, nstep_save) bind(C, name="sis_gpu_bp_open")
+         import :: c_int, c_ptr, c_int32_t, c_int64_t
+         type(c_ptr), value :: ctx, paths
+         integer(c_int32_t), value :: kind
+         integer(c_int64_t), value :: nstep_save
+      end function
+
+      integer(c_int) function sis_gpu_bp_write(ctx) bind(C, name="sis_gpu_bp_write")
+         import :: c_int, c_ptr
+         type(c_ptr), value :: ctx
+      end function
+
+      integer(c_int) function sis_gpu_bp_close(ctx) bind(C, name="sis_gpu_bp_close")
+         import :: c_int, c_ptr
+         type(c_ptr), value :: ctx
+      end function
+
       integer(c_int) function sis_gpu_set_step(ctx, istep) bind(C, name="sis_gpu_set_step")
          import :: c_int, c_ptr, c_int64_t
          type(c_ptr), value :: ctx

@@ -77,3 +77,52 @@ def test_restart_continues_the_run(mode):
     assert np.array_equal(a.labels(), c.labels())
     assert np.abs(xa - xc).max() < 1e-6
     assert np.abs(va - vc).max() < 1e-5
+
+
+@pytest.mark.parametrize("case", ["batched", "pbc_phases"])
+def test_bp_records_during_steps(tmp_path, case):
+    """Base-pair records (.bpall: every formed pair; the E <= -kT filter of
+    .bp is checked on the host writer) written at the save steps equal the
+    formed pairs, in list order, read back from an identical run between
+    shorter sis_gpu_step calls; the first set follows an energy
+    evaluation, as the initial write_bp does."""
+    from tests.test_output import _read_bp
+    if case == "batched":
+        s = configs.remd_c3(nrep=4, n=128, seed=3)
+        kw = dict(mode="batched", precision=1)
+    else:
+        s = configs.small_pbc()
+        kw = dict(mode="phases", precision=0)
+    R = int(s["nrep"])
+    print(f"[case] bp output {case}", flush=True)
+    eng = Engine(s, seed=43, **kw)
+    eng.init_velocities()
+    eng.neighbor_list(0)
+    for r in range(R):
+        eng.energy_sumup(r + 1)
+    paths = [tmp_path / f"dev_{r}.bp" for r in range(R)]
+    eng.bp_open(paths, kind=1, nstep_save=25)
+    eng.bp_write()
+    eng.step(60)
+    eng.step(40)
+    eng.bp_close()
+    ref = Engine(s, seed=43, **kw)
+    ref.init_velocities()
+    ref.neighbor_list(0)
+    for r in range(R):
+        ref.energy_sumup(r + 1)
+    want = [[] for _ in range(R)]
+    for k in range(5):
+        if k:
+            ref.step(25)
+        for r in range(R):
+            b = ref.bp(r + 1)
+            keep = b["status"] > 0
+            want[r].append([(int(i), int(j), float(np.float32(e)))
+                            for (i, j), e in zip(b["bp"][keep][:, :2], b["ene"][keep])])
+    total = 0
+    for r in range(R):
+        got = _read_bp(paths[r])
+        assert got == want[r], r
+        total += sum(len(x) for x in got)
+    assert total > 0

@@ -53,3 +53,36 @@ def test_dcd_layout_known_answer(tmp_path):
         assert i32(o + 12) == 8
         o += 16
     assert o == len(raw)
+
+
+def _read_bp(path):
+    raw = open(path, "rb").read()
+    assert np.frombuffer(raw, np.int32, 2, 0).tolist() == [2, 4]
+    rec = np.frombuffer(raw[8:], dtype=[("i", "<i2"), ("j", "<i2"), ("e", "<f4")])
+    sets, cur = [], []
+    for r in rec:
+        if r["i"] == 0 and r["j"] == 0 and r["e"] == 0.0:
+            sets.append(cur)
+            cur = []
+        else:
+            cur.append((int(r["i"]), int(r["j"]), float(r["e"])))
+    assert not cur
+    return sets
+
+
+def test_bp_layout_known_answer(tmp_path):
+    """Records of src/write_bp.F90: formed pairs in list order, E <= -kT for
+    .bp, every formed pair for .bpall, float32 energies, a zero terminator
+    per save step."""
+    from sis_amd.engine import write_bp_file
+    kT = 300.0 * 1.380649e-23 * ((1.0 / 4184.0) * 6.02214076e23)
+    ij = np.array([[3, 40], [5, 38], [7, 36], [9, 34]])
+    st = np.array([1, 1, 0, 1], np.uint8)
+    en = np.array([-2.0 * kT, -0.5 * kT, -3.0, -kT])
+    p0, p1 = tmp_path / "a.bp", tmp_path / "a.bpall"
+    write_bp_file(p0, [(ij, st, en), (ij, 1 - st, en)], 300.0, kind=0)
+    write_bp_file(p1, [(ij, st, en)], 300.0, kind=1)
+    s0 = _read_bp(p0)
+    f32 = lambda x: float(np.float32(x))  # noqa: E731
+    assert s0 == [[(3, 40, f32(en[0])), (9, 34, f32(en[3]))], [(7, 36, f32(-3.0))]]
+    assert _read_bp(p1) == [[(3, 40, f32(en[0])), (5, 38, f32(en[1])), (9, 34, f32(en[3]))]]
