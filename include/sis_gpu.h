@@ -179,6 +179,11 @@ int sis_gpu_get_labels(sis_gpu_ctx *ctx, int32_t *rep2lab);
 int sis_gpu_set_step(sis_gpu_ctx *ctx, int64_t istep);
 int sis_gpu_set_labels(sis_gpu_ctx *ctx, const int32_t *rep2lab);
 
+/* energy_kinetic (src/energy_kinetic.F90:1-31): 0.5 sum m |v|^2 of
+ * replica irep (1-based), or of every replica into ekin[nrep] when
+ * irep = 0. */
+int sis_gpu_kinetic(sis_gpu_ctx *ctx, int32_t irep, double *ekin);
+
 /* Trajectory output in the reference's DCD layout (file_dcd of
  * src/dcd.F90: write_header :287-355, write_onestep :538-596), one file
  * per replica as job_md opens them (src/job_md.F90:294-303).  The unit

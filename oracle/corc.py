@@ -50,6 +50,8 @@ def lib():
         L.orc_mt_words.argtypes = [C.c_uint32, i64, P(C.c_uint32)]
         L.orc_genrand.argtypes = [vp, i32, i32, i64, P(d)]
         L.orc_init_velocities.argtypes = [vp]
+        L.orc_kinetic.restype = d
+        L.orc_kinetic.argtypes = [vp, i32]
         L.orc_md.restype = i64
         L.orc_md.argtypes = [vp, i64, i32, P(d)]
         L.orc_labels.argtypes = [vp, P(i32)]
@@ -198,6 +200,9 @@ class Oracle:
         o = np.zeros(n)
         lib().orc_genrand(self.h, irep, kind, n, _d(o))
         return o
+
+    def kinetic(self, irep=1):
+        return float(lib().orc_kinetic(self.h, irep))
 
     def init_velocities(self):
         lib().orc_init_velocities(self.h)

@@ -314,6 +314,44 @@ contains
       accels_out(:,:,:) = accels(:,:,:)
    end subroutine sisref_get_state
 
+   subroutine sisref_set_velos(velos_in) bind(C, name="sisref_set_velos")
+      use var_state, only : velos
+      use var_top, only : nmp
+      use var_replica, only : nrep_proc
+      real(c_double), intent(in) :: velos_in(3, nmp, nrep_proc)
+      if (.not. allocated(velos)) allocate(velos(3, nmp, nrep_proc))   ! job_md allocates it otherwise
+      velos(:,:,:) = velos_in(:,:,:)
+   end subroutine sisref_set_velos
+
+   ! energy_kinetic needs var_top's mass, which job_md fills
+   ! (src/job_md.F90:116-133); set it the same way here
+   subroutine sisref_kinetic(irep, ekin) bind(C, name="sisref_kinetic")
+      use const, only : PREC
+      use const_idx, only : SEQT
+      use var_top, only : nmp, mass, seq, lmp_mp, ichain_mp
+      integer(c_int), value :: irep
+      real(c_double), intent(out) :: ekin
+      integer :: imp
+      if (.not. allocated(mass)) allocate(mass(nmp))
+      if (.true.) then
+         do imp = 1, nmp
+            select case (seq(lmp_mp(imp), ichain_mp(imp)))
+               case (SEQT%A)
+                  mass(imp) = 328.212_PREC
+               case (SEQT%G)
+                  mass(imp) = 344.212_PREC
+               case (SEQT%C)
+                  mass(imp) = 304.182_PREC
+               case (SEQT%U)
+                  mass(imp) = 305.164_PREC
+               case default
+                  mass(imp) = 320.443_PREC
+            endselect
+         enddo
+      endif
+      call energy_kinetic(irep, ekin)
+   end subroutine sisref_kinetic
+
    subroutine sisref_neighbor_list(irep) bind(C, name="sisref_neighbor_list")
       integer(c_int), value :: irep
       call neighbor_list(irep)

@@ -88,6 +88,18 @@ def _worker(q, so, sysd, ops, seed, nstep_exchange):
                 lib.sisref_energy(C.c_int(op[1]), C.c_double(op[2]), C.c_int(op[3]),
                                   C.c_int(op[4]), _dp(e))
                 out[op[-1]] = e
+            elif kind == "set_velos":
+                v = np.ascontiguousarray(op[1], np.float64)
+                lib.sisref_set_velos(_dp(v))
+            elif kind == "get_velos":
+                vo = np.zeros((nrep, nmp, 3))
+                ao = np.zeros((nrep, nmp, 3))
+                lib.sisref_get_state(_dp(vo), _dp(ao))
+                out[op[-1]] = vo
+            elif kind == "kinetic":
+                ek = C.c_double(0.0)
+                lib.sisref_kinetic(C.c_int(op[1]), C.byref(ek))
+                out[op[-1]] = ek.value
             elif kind == "ele":
                 e = np.zeros(4, np.float64)
                 lib.sisref_ele_params(C.c_int(op[1]), _dp(e))

@@ -1060,6 +1060,16 @@ void orc_md_coef(orc_sys *s, int32_t irep, double *coef4) {
 }
 
 /* initial velocities and accelerations (src/job_md.F90:157-176) */
+/* energy_kinetic (src/energy_kinetic.F90:16-29): 0.5 sum_i m_i |v_i|^2,
+ * beads in order. */
+double orc_kinetic(orc_sys *s, int32_t irep1) {
+    const double *v = s->velos + (size_t)(irep1 - 1) * s->nmp * 3;
+    double e = 0.0;
+    for (int i = 0; i < s->nmp; i++)
+        e += s->mass[i] * (v[3 * i] * v[3 * i] + v[3 * i + 1] * v[3 * i + 1] + v[3 * i + 2] * v[3 * i + 2]);
+    return 0.5 * e;
+}
+
 void orc_init_velocities(orc_sys *s) {
     for (int r = 0; r < s->nrep; r++) {
         double tK = s->rep[r].tK;

@@ -13,6 +13,7 @@ orc_sys *orc_create(const sis_topo *topo, const sis_params *prm, int32_t nrep,
 void orc_destroy(orc_sys *s);
 void orc_set_xyz(orc_sys *s, const double *xyz);           /* (nrep, nmp, 3) */
 void orc_get_state(orc_sys *s, double *xyz, double *velos, double *accels);
+double orc_kinetic(orc_sys *s, int32_t irep1);
 void orc_set_state(orc_sys *s, const double *xyz, const double *velos, const double *accels);
 void orc_neighbor_list(orc_sys *s, int32_t irep);            /* 1-based irep */
 int64_t orc_count(orc_sys *s, int32_t irep, int32_t kind);   /* 0 wca 1 ele 2 bp */

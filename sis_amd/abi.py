@@ -123,6 +123,7 @@ EXPORTED = [
     "sis_gpu_dcd_open", "sis_gpu_dcd_write", "sis_gpu_dcd_close", "sis_dcd_write_file",
     "sis_gpu_set_step", "sis_gpu_set_labels",
     "sis_gpu_bp_open", "sis_gpu_bp_write", "sis_gpu_bp_close", "sis_bp_write_file",
+    "sis_gpu_kinetic",
 ]
 
 
@@ -160,6 +161,7 @@ def _declare(lib: C.CDLL) -> None:
         "sis_gpu_dcd_open": (i32, [vp, P(C.c_char_p), i32, i64]),
         "sis_gpu_dcd_write": (i32, [vp]),
         "sis_gpu_set_step": (i32, [vp, i64]),
+        "sis_gpu_kinetic": (i32, [vp, i32, P(d)]),
         "sis_gpu_bp_open": (i32, [vp, P(C.c_char_p), i32, i64]),
         "sis_gpu_bp_write": (i32, [vp]),
         "sis_gpu_bp_close": (i32, [vp]),

@@ -143,6 +143,7 @@ struct sis_gpu_ctx {
   DevBuf<int> rebuild_flag_d, mask_d, mc_d, scan_bsum_d, scan_total_d;
   DevBuf<int> exc_cnt_d, exc_off_d, exc_fill_d, exc_list_d, ntlist_d, bpseq_d, seg_d;
   DevBuf<int> huge_d;   // per replica: a conflict component above the bucket capacity this step
+  DevBuf<double> kin_d; // per replica kinetic energy (sis_gpu_kinetic)
   DevBuf<uint32_t> tape_d;
   DevBuf<int64_t> tape_off_d, tape_pos_d;
   DevBuf<double> noise_d, bm_cache_d, ene_part_d, ene_out_d, ecT_d, eCoef_d, eLam_d;
@@ -204,7 +205,7 @@ struct sis_gpu_ctx {
                          &exc_list_d, &ntlist_d, &bpseq_d, &seg_d, &huge_d, &pers_sync_d, &chain_id_d,
                          &bp_val_d, &bp_val2_d, &bp_mp2_d};
     for (auto *b : ib) b->free_();
-    DevBuf<double> *db[] = {&bp3_dH_d, &bp3_dS_d, &md_a_d, &md_cf_d, &md_cx_d, &mass_d,
+    DevBuf<double> *db[] = {&kin_d, &bp3_dH_d, &bp3_dS_d, &md_a_d, &md_cf_d, &md_cx_d, &mass_d,
                             &md_cr_d, &tK_d, &ele_coef_d, &lambdaD_d, &ele_cut_d, &box_d,
                             &bp_coef_d, &ene_bp_d, &for_bp_d, &blk_top2_d, &noise_d,
                             &bm_cache_d, &ene_part_d, &ene_out_d, &ecT_d, &eCoef_d, &eLam_d,
@@ -603,6 +604,15 @@ void check_flags(sis_gpu_ctx *c) {
   if (f[1] == 2) throw ApiError{SIS_ERR_STATE, "limit MC found an excess nucleotide without pairs"};
   if (f[5]) throw ApiError{SIS_ERR_CAPACITY, "a base-pair conflict component outgrew the limit-MC scratch"};
   if (f[1]) throw ApiError{SIS_ERR_STATE, "parity random-number tape exhausted"};
+}
+
+// kinetic energy of every replica into e[R]
+void kinetic_all(sis_gpu_ctx *c, double *e) {
+  const int R = c->R;
+  if (c->kin_d.n < (size_t)R) c->kin_d.alloc(R);
+  hipLaunchKernelGGL(k_kinetic<256>, dim3(R), dim3(256), 0, c->stream, c->P, c->ptrs(), c->kin_d.p);
+  HIPCHK(hipMemcpyAsync(e, c->kin_d.p, R * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
 }
 
 // energy_sumup for replica mask at conditions (ecT, coef, lam per replica)
@@ -1105,6 +1115,18 @@ int sis_gpu_energy(sis_gpu_ctx *c, int32_t irep, double tempK, int32_t flags, do
   std::vector<double> out((size_t)c->R * 16);
   energy_phase(c, m, T, flags, out.data());
   for (int k = 0; k < 11; k++) energies[k] = out[(size_t)(irep - 1) * 16 + k];
+  return SIS_OK;
+  API_END
+}
+
+int sis_gpu_kinetic(sis_gpu_ctx *c, int32_t irep, double *ekin) {
+  API_BEGIN
+  if (!c || !ekin) return fail(SIS_ERR_ARG, "null argument");
+  if (irep < 0 || irep > c->R) return fail(SIS_ERR_ARG, "irep out of range");
+  std::vector<double> e(c->R);
+  kinetic_all(c, e.data());
+  if (irep == 0) std::copy(e.begin(), e.end(), ekin);
+  else *ekin = e[irep - 1];
   return SIS_OK;
   API_END
 }

@@ -1515,6 +1515,26 @@ __global__ void __launch_bounds__(B) k_energy(Params P, Ptrs D, const double *ec
   if (threadIdx.x == 0)
     for (int k = 1; k < 7; k++) partial[((size_t)r * gridDim.x + blockIdx.x) * 8 + k] = s[k][0];
 }
+// energy_kinetic (src/energy_kinetic.F90:16-29): 0.5 sum m |v|^2 of
+// replica blockIdx.x, one workgroup per replica, FP64 tree sum.
+template <int B>
+__global__ void __launch_bounds__(B) k_kinetic(Params P, Ptrs D, double *out) {
+  __shared__ double s[B];
+  const int r = blockIdx.x, N = P.N;
+  double e = 0.0;
+  for (int a = threadIdx.x; a < N; a += B) {
+    const double4 v = D.vel[(size_t)r * N + a];
+    e += D.mass[a] * (v.x * v.x + v.y * v.y + v.z * v.z);
+  }
+  s[threadIdx.x] = e;
+  __syncthreads();
+  for (int off = B / 2; off > 0; off >>= 1) {
+    if (threadIdx.x < off) s[threadIdx.x] += s[threadIdx.x + off];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[r] = 0.5 * s[0];
+}
+
 __global__ void k_energy_reduce(const double *partial, int nblk, double *out) {
   int r = blockIdx.x;
   if (threadIdx.x != 0) return;

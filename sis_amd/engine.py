@@ -127,6 +127,12 @@ class Engine:
         _check(lib().sis_gpu_energy(self.h, irep, float(tempK), flags, abi.dptr(e)))
         return e
 
+    def kinetic(self, irep: int = 0):
+        """energy_kinetic of replica irep (1-based), or of all replicas (0)."""
+        out = np.zeros(max(self.nrep, 1))
+        _check(lib().sis_gpu_kinetic(self.h, int(irep), abi.dptr(out)))
+        return out if irep == 0 else float(out[0])
+
     def pairs(self, irep: int = 1, kind: int = 0) -> np.ndarray:
         n = C.c_int64()
         _check(lib().sis_gpu_get_pair_count(self.h, irep, kind, C.byref(n)))

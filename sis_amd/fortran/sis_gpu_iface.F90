@@ -211,6 +211,12 @@ module sis_gpu_iface
          type(c_ptr), value :: ctx
       end function
 
+      integer(c_int) function sis_gpu_kinetic(ctx, irep, ekin) bind(C, name="sis_gpu_kinetic")
+         import :: c_int, c_ptr, c_int32_t
+         type(c_ptr), value :: ctx, ekin
+         integer(c_int32_t), value :: irep
+      end function
+
       integer(c_int) function sis_gpu_set_step(ctx, istep) bind(C, name="sis_gpu_set_step")
          import :: c_int, c_ptr, c_int64_t
          type(c_ptr), value :: ctx

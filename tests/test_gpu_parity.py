@@ -372,6 +372,23 @@ def test_fp32_and_fp64_ensembles_agree():
     assert res["min_p"] > 1e-3, res
 
 
+def test_kinetic_energy_matches_oracle():
+    """sis_gpu_kinetic (energy_kinetic) against the oracle on random
+    velocities of three replicas, rel <= 1e-12 (tree vs sequential sum)."""
+    s = configs.remd_c3(nrep=3, n=50, seed=5)
+    v = np.random.default_rng(12).normal(0.0, 2.0, size=(3, 50, 3))
+    print("[case] kinetic energy", flush=True)
+    eng = Engine(s, parity=False, mode="phases")
+    eng.set_state(velos=v)
+    o = corc.Oracle(s)
+    o.set_state(velos=v)
+    ek = eng.kinetic(0)
+    for r in range(3):
+        want = o.kinetic(r + 1)
+        assert abs(ek[r] - want) <= 1e-12 * want
+        assert abs(eng.kinetic(r + 1) - want) <= 1e-12 * want
+
+
 def test_batched_dynamics_stable_fp32():
     s = configs.remd_c3(nrep=32, n=128, seed=4)
     print("[case] batched fp32", flush=True)

@@ -220,3 +220,34 @@ def test_live_reference_shared_mc_flag():
     assert np.abs(x - ref1["xyz"]).max() < 1e-10
     assert np.abs(v - ref1["velos"]).max() < 1e-12
     assert np.array_equal(o.labels(), ref1["rep2lab"][:4])
+
+
+@pytest.mark.ref
+def test_live_reference_kinetic_energy():
+    """energy_kinetic: oracle against the reference build on random
+    velocities of every replica (masses from the sequence)."""
+    s = configs.remd_c3(nrep=3, n=50, seed=5)
+    rng = np.random.default_rng(12)
+    v = rng.normal(0.0, 2.0, size=(3, 50, 3))
+    ref = refdrv.run(s, [("set_velos", v)] + [("kinetic", r, f"k{r}") for r in (1, 2, 3)])
+    o = corc.Oracle(s)
+    o.set_state(velos=v)
+    for r in (1, 2, 3):
+        assert abs(o.kinetic(r) - ref[f"k{r}"]) <= 1e-12 * ref[f"k{r}"]
+
+
+def test_kinetic_energy_closed_form():
+    """0.5 sum m v^2 with the masses of the sequence (A, U, G, C differ):
+    oracle against numpy, and zero for zero velocities."""
+    s = configs.remd_c3(nrep=2, n=30, seed=6)
+    o = corc.Oracle(s)
+    rng = np.random.default_rng(3)
+    v = rng.normal(size=(2, 30, 3))
+    o.set_state(velos=v)
+    # masses per nucleotide type, src/job_md.F90:116-133 (A, U, G, C codes 0-3)
+    m = np.array([328.212, 305.164, 344.212, 304.182])[np.asarray(s["seq"])]
+    for r in (1, 2):
+        want = 0.5 * float(np.sum(m[:, None] * v[r - 1] ** 2))
+        assert abs(o.kinetic(r) - want) <= 1e-12 * want
+    o.set_state(velos=np.zeros_like(v))
+    assert o.kinetic(1) == 0.0
