@@ -215,6 +215,22 @@ int sis_gpu_bp_open(sis_gpu_ctx *ctx, const char *const *paths, int32_t kind, in
 int sis_gpu_bp_write(sis_gpu_ctx *ctx);
 int sis_gpu_bp_close(sis_gpu_ctx *ctx);
 
+/* Energy lines (.out) in the format of src/write_out.F90: the header of
+ * write_out_header and, per save step, istep, the replica label (REMD),
+ * T, Ekin and energies(0:5) [and Eele] with the edit descriptors i12, i4,
+ * f6.2 and g13.6.  With nstep_save > 0 sis_gpu_step writes a line per
+ * replica at the save steps: labels, T and Ekin before that step's
+ * exchange, energies at the replica's own temperature (with a fresh limit
+ * MC on exchange steps, reusing the step's base-pair state otherwise, as
+ * src/job_md.F90:474-510 does). */
+int sis_gpu_out_open(sis_gpu_ctx *ctx, const char *const *paths, int64_t nstep_save);
+int sis_gpu_out_close(sis_gpu_ctx *ctx);
+
+/* The same line writer on host arrays: energies (11, nline). */
+int sis_out_write_file(const char *path, int32_t nline, const int64_t *isteps, const int32_t *labels,
+                       const double *temps, const double *ekin, const double *energies, int32_t replica,
+                       int32_t ele);
+
 /* The same writer on host arrays: nsets save steps of n candidates each,
  * ij (2, n) 1-based bead ids, status (n) and ene (n) per set. */
 int sis_bp_write_file(const char *path, int64_t n, const int32_t *ij, const uint8_t *status, const double *ene,

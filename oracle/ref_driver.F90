@@ -577,4 +577,48 @@ contains
       call f%close()
    end subroutine sisref_write_dcd
 
+   ! One .out file through the reference's write_out_header and write_out
+   ! (src/write_out.F90), for nline lines of (istep, label, tK, Ekin,
+   ! energies(0:10)); used to compare other writers byte by byte.
+   subroutine sisref_write_out(path, plen, nline, isteps, labels, temps, ekin, evals, replica, ele) &
+                               bind(C, name="sisref_write_out")
+      use, intrinsic :: iso_fortran_env, only : INT64
+      use const_idx, only : ENE
+      use var_io, only : hdl_out
+      use var_state, only : energies, Ekinetic
+      use var_replica, only : flg_replica
+      use var_potential, only : flg_ele, flg_stage, flg_twz, flg_bias_rg, flg_restraint
+      character(kind=c_char), intent(in) :: path(*)
+      integer(c_int), value :: plen, nline, replica, ele
+      integer(c_int64_t), intent(in) :: isteps(nline)
+      integer(c_int), intent(in) :: labels(nline)
+      real(c_double), intent(in) :: temps(nline), ekin(nline), evals(0:10, nline)
+      character(len=:), allocatable :: fname
+      integer :: k
+      integer(INT64) :: istep_out
+      allocate(character(len=plen) :: fname)
+      do k = 1, plen
+         fname(k:k) = path(k)
+      enddo
+      if (.not. allocated(hdl_out)) allocate(hdl_out(1))
+      if (.not. allocated(energies)) allocate(energies(0:ENE%MAX, 1))
+      if (.not. allocated(Ekinetic)) allocate(Ekinetic(1))
+      flg_replica = replica /= 0
+      flg_ele = ele /= 0
+      flg_stage = .False.
+      flg_twz = .False.
+      flg_bias_rg = .False.
+      flg_restraint = .False.
+      hdl_out(1) = 94
+      open(hdl_out(1), file=fname, status='replace', action='write', form='formatted')
+      call write_out_header(1)
+      do k = 1, nline
+         energies(0:10, 1) = evals(0:10, k)
+         Ekinetic(1) = ekin(k)
+         istep_out = isteps(k)
+         call write_out(1, istep_out, labels(k), temps(k))
+      enddo
+      close(hdl_out(1))
+   end subroutine sisref_write_out
+
 end module sisref_driver

@@ -254,3 +254,37 @@ def write_dcd(path, xyz_frames, box=None, fix_com=0, timeout=60):
         pr.join(timeout=10)
     if st != "ok":
         raise RuntimeError(f"reference DCD writer failed: {msg}")
+
+
+def _out_worker(q, so, path, isteps, labels, temps, ekin, ene, replica, ele):
+    try:
+        lib = C.CDLL(str(so))
+        p = str(path).encode()
+        n = len(isteps)
+        ist = np.ascontiguousarray(isteps, np.int64)
+        lab = np.ascontiguousarray(labels, np.int32)
+        tk = np.ascontiguousarray(temps, np.float64)
+        ek = np.ascontiguousarray(ekin, np.float64)
+        en = np.ascontiguousarray(ene, np.float64)   # (n, 11) = Fortran (0:10, n)
+        lib.sisref_write_out(C.c_char_p(p), C.c_int(len(p)), C.c_int(n), ist.ctypes.data_as(C.POINTER(C.c_int64)),
+                             _ip(lab), _dp(tk), _dp(ek), _dp(en), C.c_int(int(replica)), C.c_int(int(ele)))
+        q.put(("ok", None))
+    except Exception as e:  # pragma: no cover - reported to the caller
+        q.put(("err", repr(e)))
+
+
+def write_out(path, isteps, labels, temps, ekin, ene, replica=False, ele=True, timeout=60):
+    """The reference's .out writer (write_out_header + write_out,
+    src/write_out.F90) on the given lines, run in a child process."""
+    import multiprocessing as mp
+    ctx = mp.get_context("spawn")
+    q = ctx.Queue()
+    pr = ctx.Process(target=_out_worker, args=(q, str(REF_SO), str(path), np.asarray(isteps), np.asarray(labels),
+                                               np.asarray(temps), np.asarray(ekin), np.asarray(ene), replica, ele))
+    pr.start()
+    try:
+        st, msg = q.get(timeout=timeout)
+    finally:
+        pr.join(timeout=10)
+    if st != "ok":
+        raise RuntimeError(f"reference .out writer failed: {msg}")

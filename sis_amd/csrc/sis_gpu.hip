@@ -76,10 +76,12 @@ struct DevBuf {
 
 struct DcdOut;
 struct BpOut;
+struct OutFile;
 
 struct sis_gpu_ctx {
   DcdOut *dcd = nullptr;   // trajectory output (sis_gpu_dcd_open), or none
   BpOut *bpo = nullptr;    // base-pair output (sis_gpu_bp_open), or none
+  OutFile *outf = nullptr; // energy lines (sis_gpu_out_open), or none
   Params P{};
   Ptrs D{};
   sis_options opt{};
@@ -663,7 +665,7 @@ double rep_uniform(sis_gpu_ctx *c, uint64_t k) {
 
 // energy_replica + replica_exchange (src/energy_replica.F90:47-156,
 // src/replica_exchange.F90:35-154) for a one-dimensional temperature ladder.
-void do_exchange(sis_gpu_ctx *c, double *replica_energies, int32_t *accepted) {
+void do_exchange(sis_gpu_ctx *c, double *replica_energies, int32_t *accepted, double *own_energies = nullptr) {
   const int R = c->R;
   std::vector<int> all(R, 1);
   if (!c->lists_valid) rebuild(c, all);
@@ -690,6 +692,7 @@ void do_exchange(sis_gpu_ctx *c, double *replica_energies, int32_t *accepted) {
     HIPCHK(hipMemcpyAsync(c->bp_status_d.p, st.data(), nb, hipMemcpyHostToDevice, c->stream));
     HIPCHK(hipMemcpyAsync(c->ene_bp_d.p, en.data(), nb * sizeof(double), hipMemcpyHostToDevice, c->stream));
   }
+  if (own_energies) std::copy(e1.begin(), e1.end(), own_energies);
   std::vector<double> E(2 * (R + 1), 0.0);
   for (int r = 0; r < R; r++) {
     E[2 * (r + 1)] = e1[(size_t)r * 16];
@@ -994,6 +997,7 @@ int sis_gpu_destroy(sis_gpu_ctx *c) {
   if (c) {
     dcd_shutdown(c);
     bp_shutdown(c);
+    out_shutdown(c);
   }
   delete c;
   return SIS_OK;
@@ -1341,16 +1345,39 @@ int sis_gpu_step(sis_gpu_ctx *c, int64_t nsteps, int64_t *rebuilds) {
   const int64_t ex = (c->opt.exchange_every > 0 && c->R > 1) ? c->opt.exchange_every : 0;
   const int64_t sv = (c->dcd && c->dcd->nstep_save > 0) ? c->dcd->nstep_save : 0;
   const int64_t sb = (c->bpo && c->bpo->nstep_save > 0) ? c->bpo->nstep_save : 0;
+  const int64_t so = (c->outf && c->outf->nstep_save > 0) ? c->outf->nstep_save : 0;
   for (int64_t left = nsteps; left > 0;) {
     int64_t k = left;
     if (ex) k = std::min(k, ex - c->istep % ex);
     if (sv) k = std::min(k, sv - c->istep % sv);
     if (sb) k = std::min(k, sb - c->istep % sb);
+    if (so) k = std::min(k, so - c->istep % so);
     run(k);
     left -= k;
+    const bool exch = ex && c->istep % ex == 0;
     if (sv && c->istep % sv == 0) dcd_enqueue(c);
     if (sb && c->istep % sb == 0) bp_write_now(c);
-    if (ex && c->istep % ex == 0) do_exchange(c, nullptr, nullptr);
+    if (so && c->istep % so == 0) {
+      // energy lines: labels, temperatures and Ekin before the exchange;
+      // the energies of an exchange step are its own-condition energies
+      // (fresh limit MC, src/job_md.F90:474-481), otherwise the step's
+      // base-pair state is reused (flg_bp_energy)
+      const int R = c->R;
+      std::vector<int> lab(R);
+      for (int r = 0; r < R; r++) lab[r] = c->rep2lab[r + 1];
+      const std::vector<double> temps = c->tK;
+      std::vector<double> ekin(R), e16((size_t)R * 16);
+      kinetic_all(c, ekin.data());
+      if (exch) {
+        do_exchange(c, nullptr, nullptr, e16.data());
+      } else {
+        std::vector<int> all(R, 1);
+        energy_phase(c, all, c->tK, (c->P.nstep_bp_mc > 0 ? 1 : 0) | 2, e16.data());
+      }
+      out_lines(c, e16.data(), ekin.data(), lab, temps);
+      continue;
+    }
+    if (exch) do_exchange(c, nullptr, nullptr);
   }
   HIPCHK(hipEventRecord(c->ev1, c->stream));
   HIPCHK(hipEventSynchronize(c->ev1));

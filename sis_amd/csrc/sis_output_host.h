@@ -401,3 +401,171 @@ int sis_gpu_dcd_close(sis_gpu_ctx *c) {
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------
+// Energy lines (.out): write_out_header and write_out of
+// src/write_out.F90:1-116.  The line format is
+//   (i12, 1x, [i4, 1x,] f6.2, 7(1x, g13.6)) [+ (1x, g13.6) for Eele]
+// with Ekin and energies(0:5) in the seven fields; the Fortran G, F and I
+// edit descriptors are reproduced below (round to nearest).
+struct OutFile {
+  std::vector<FILE *> f;
+  int64_t nstep_save = 0;
+  bool replica = false, ele = false;
+};
+
+namespace {
+
+std::string pad_left(const std::string &s, int w) {
+  return (int)s.size() >= w ? s : std::string(w - s.size(), ' ') + s;
+}
+
+// Gw.d output editing (no exponent width): F editing with 4 trailing
+// blanks when 0.1 <= |x| < 10^d after rounding to d significant digits,
+// else Ew.d (0.dddddd E+ee, or 0.dddddd+eee beyond two exponent digits).
+std::string fortran_g(double x, int w, int d) {
+  char buf[96];
+  if (std::isnan(x)) return pad_left("NaN", w);
+  if (std::isinf(x)) return pad_left(x < 0 ? "-Infinity" : "Infinity", w);
+  const bool neg = std::signbit(x);
+  const double a = std::fabs(x);
+  if (a == 0.0) {
+    std::string f = std::string(neg ? "-" : "") + "0." + std::string(d - 1, '0');
+    return pad_left(f, w - 4) + "    ";
+  }
+  snprintf(buf, sizeof buf, "%.*e", d - 1, a);
+  const char *ep = strchr(buf, 'e');
+  const int e10 = atoi(ep + 1);
+  const int k = e10 + 1;   // 10^(k-1) <= |x| rounded < 10^k
+  if (k >= 0 && k <= d) {
+    char fb[96];
+    snprintf(fb, sizeof fb, "%#*.*f", w - 4, d - k, x);
+    return std::string(fb) + "    ";
+  }
+  std::string digits;
+  for (const char *q = buf; q < ep; q++)
+    if (*q != '.') digits += *q;
+  const int ee = e10 + 1;
+  char eb[16];
+  if (std::abs(ee) <= 99) snprintf(eb, sizeof eb, "E%c%02d", ee < 0 ? '-' : '+', std::abs(ee));
+  else snprintf(eb, sizeof eb, "%c%03d", ee < 0 ? '-' : '+', std::abs(ee));
+  return pad_left(std::string(neg ? "-" : "") + "0." + digits + eb, w);
+}
+
+void out_header(FILE *f, bool replica, bool ele) {
+  std::string h;
+  int icol;
+  if (replica) {
+    h = "#(1)nframe   (2)R (3)T   (4)Ekin       (5)Epot       (6)Ebond     "
+        " (7)Eangl      (8)Edih       (9)Ebp        (10)Eexv     ";
+    icol = 10;
+  } else {
+    h = "#(1)nframe   (2)T   (3)Ekin       (4)Epot       (5)Ebond     "
+        " (6)Eangl      (7)Edih       (8)Ebp        (9)Eexv      ";
+    icol = 9;
+  }
+  if (ele) {
+    char b[32];
+    snprintf(b, sizeof b, " (%2d)Eele     ", icol + 1);
+    h += b;
+  }
+  h += "\n";
+  dcd_put(f, h.data(), h.size());
+  fflush(f);
+}
+
+void out_line(FILE *f, int64_t istep, int label, double tK, double ekin, const double *e, bool replica, bool ele) {
+  char b[64];
+  std::string s;
+  snprintf(b, sizeof b, "%12lld", (long long)istep);
+  s += b;
+  if (replica) {
+    snprintf(b, sizeof b, " %4d", label);
+    s += b;
+  }
+  snprintf(b, sizeof b, " %6.2f", tK);
+  s += b;
+  s += " " + fortran_g(ekin, 13, 6);
+  for (int k = 0; k <= 5; k++) s += " " + fortran_g(e[k], 13, 6);
+  if (ele) s += " " + fortran_g(e[6], 13, 6);
+  s += "\n";
+  dcd_put(f, s.data(), s.size());
+  fflush(f);
+}
+
+void out_shutdown(sis_gpu_ctx *c) {
+  if (!c->outf) return;
+  for (FILE *f : c->outf->f)
+    if (f) fclose(f);
+  delete c->outf;
+  c->outf = nullptr;
+}
+
+// energies (R x 16) and kinetic energies of every replica at the current
+// step, labels and temperatures as they are now
+void out_lines(sis_gpu_ctx *c, const double *e16, const double *ekin, const std::vector<int> &labels,
+               const std::vector<double> &temps) {
+  for (int r = 0; r < c->R; r++)
+    out_line(c->outf->f[r], c->istep, labels[r], temps[r], ekin[r], e16 + (size_t)r * 16, c->outf->replica,
+             c->outf->ele);
+}
+
+}  // namespace
+
+extern "C" {
+
+int sis_out_write_file(const char *path, int32_t nline, const int64_t *isteps, const int32_t *labels,
+                       const double *temps, const double *ekin, const double *energies, int32_t replica,
+                       int32_t ele) {
+  API_BEGIN
+  if (!path || nline < 0 || (nline > 0 && (!isteps || !temps || !ekin || !energies || (replica && !labels))))
+    return fail(SIS_ERR_ARG, "invalid argument");
+  FILE *f = fopen(path, "wb");
+  if (!f) return fail(SIS_ERR_ARG, std::string("cannot open ") + path);
+  try {
+    out_header(f, replica != 0, ele != 0);
+    for (int k = 0; k < nline; k++)
+      out_line(f, isteps[k], replica ? labels[k] : 0, temps[k], ekin[k], energies + (size_t)11 * k, replica != 0,
+               ele != 0);
+  } catch (...) {
+    fclose(f);
+    throw;
+  }
+  fclose(f);
+  return SIS_OK;
+  API_END
+}
+
+int sis_gpu_out_open(sis_gpu_ctx *c, const char *const *paths, int64_t nstep_save) {
+  API_BEGIN
+  if (!c || !paths || nstep_save < 0) return fail(SIS_ERR_ARG, "invalid argument");
+  if (c->outf) return fail(SIS_ERR_STATE, "energy output already open");
+  c->outf = new OutFile();
+  try {
+    c->outf->nstep_save = nstep_save;
+    c->outf->replica = c->R > 1;
+    c->outf->ele = c->P.ele_on != 0;
+    for (int r = 0; r < c->R; r++) {
+      if (!paths[r]) throw ApiError{SIS_ERR_ARG, "null path"};
+      FILE *f = fopen(paths[r], "wb");
+      if (!f) throw ApiError{SIS_ERR_ARG, std::string("cannot open ") + paths[r]};
+      c->outf->f.push_back(f);
+      out_header(f, c->outf->replica, c->outf->ele);
+    }
+  } catch (...) {
+    out_shutdown(c);
+    throw;
+  }
+  return SIS_OK;
+  API_END
+}
+
+int sis_gpu_out_close(sis_gpu_ctx *c) {
+  API_BEGIN
+  if (!c) return fail(SIS_ERR_ARG, "null context");
+  out_shutdown(c);
+  return SIS_OK;
+  API_END
+}
+
+}  // extern "C"

@@ -204,6 +204,13 @@ class Engine:
     def bp_close(self):
         _check(lib().sis_gpu_bp_close(self.h))
 
+    def out_open(self, paths, nstep_save: int):
+        arr = (C.c_char_p * self.nrep)(*[str(x).encode() for x in paths])
+        _check(lib().sis_gpu_out_open(self.h, arr, int(nstep_save)))
+
+    def out_close(self):
+        _check(lib().sis_gpu_out_close(self.h))
+
     # ---- restart ----
     def set_step(self, istep: int):
         _check(lib().sis_gpu_set_step(self.h, int(istep)))
@@ -249,3 +256,16 @@ def write_bp_file(path, sets, tempK: float, kind: int = 0):
                               else np.zeros(0))
     _check(lib().sis_bp_write_file(str(path).encode(), n, abi.iptr(ij), abi.u8ptr(st), abi.dptr(en), len(sets),
                                    float(tempK), int(kind)))
+
+
+def write_out_file(path, isteps, labels, temps, ekin, energies, replica: bool, ele: bool):
+    """Host-side .out line writer of the library (src/write_out.F90 format);
+    energies (nline, 11)."""
+    ist = np.ascontiguousarray(isteps, np.int64)
+    lab = np.ascontiguousarray(labels, np.int32)
+    tk = np.ascontiguousarray(temps, np.float64)
+    ek = np.ascontiguousarray(ekin, np.float64)
+    en = np.ascontiguousarray(energies, np.float64)
+    _check(lib().sis_out_write_file(str(path).encode(), len(ist), ist.ctypes.data_as(C.POINTER(C.c_int64)),
+                                    abi.iptr(lab), abi.dptr(tk), abi.dptr(ek), abi.dptr(en), int(bool(replica)),
+                                    int(bool(ele))))

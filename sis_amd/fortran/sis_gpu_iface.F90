@@ -211,6 +211,18 @@ module sis_gpu_iface
          type(c_ptr), value :: ctx
       end function
 
+      ! energy lines (.out, src/write_out.F90 format)
+      integer(c_int) function sis_gpu_out_open(ctx, paths, nstep_save) bind(C, name="sis_gpu_out_open")
+         import :: c_int, c_ptr, c_int64_t
+         type(c_ptr), value :: ctx, paths
+         integer(c_int64_t), value :: nstep_save
+      end function
+
+      integer(c_int) function sis_gpu_out_close(ctx) bind(C, name="sis_gpu_out_close")
+         import :: c_int, c_ptr
+         type(c_ptr), value :: ctx
+      end function
+
       integer(c_int) function sis_gpu_kinetic(ctx, irep, ekin) bind(C, name="sis_gpu_kinetic")
          import :: c_int, c_ptr, c_int32_t
          type(c_ptr), value :: ctx, ekin

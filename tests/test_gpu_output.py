@@ -126,3 +126,45 @@ def test_bp_records_during_steps(tmp_path, case):
         assert got == want[r], r
         total += sum(len(x) for x in got)
     assert total > 0
+
+
+@pytest.mark.parametrize("case", ["batched", "pbc_phases"])
+def test_out_lines_during_steps(tmp_path, case):
+    """Energy lines written at the save steps equal the host line writer
+    applied to the kinetic and potential energies (step's base-pair state,
+    sis_gpu_energy with bp_energy_valid) of an identical run read between
+    shorter sis_gpu_step calls."""
+    from sis_amd.engine import write_out_file
+    if case == "batched":
+        s = configs.remd_c3(nrep=4, n=128, seed=3)
+        kw = dict(mode="batched", precision=1)
+    else:
+        s = configs.small_pbc()
+        kw = dict(mode="phases", precision=0)
+    R = int(s["nrep"])
+    print(f"[case] out lines {case}", flush=True)
+    eng = Engine(s, seed=47, **kw)
+    eng.init_velocities()
+    eng.neighbor_list(0)
+    paths = [tmp_path / f"dev_{r}.out" for r in range(R)]
+    eng.out_open(paths, nstep_save=25)
+    eng.step(60)
+    eng.step(40)
+    eng.out_close()
+    ref = Engine(s, seed=47, **kw)
+    ref.init_velocities()
+    ref.neighbor_list(0)
+    rows = [[] for _ in range(R)]
+    for k in range(4):
+        ref.step(25)
+        ek = ref.kinetic(0)
+        lab = ref.labels()
+        for r in range(R):
+            e = ref.energy_sumup(r + 1, bp_mc=1, bp_energy_valid=1)
+            rows[r].append((25 * (k + 1), lab[r], float(s["temps"][lab[r] - 1]), ek[r], e))
+    ele = bool(s["params"].get("ele_on", 1))
+    for r in range(R):
+        want = tmp_path / f"host_{r}.out"
+        st, lb, tk, ek, en = zip(*rows[r])
+        write_out_file(want, st, lb, tk, ek, np.array(en), replica=R > 1, ele=ele)
+        assert paths[r].read_text() == want.read_text(), r

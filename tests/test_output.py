@@ -86,3 +86,38 @@ def test_bp_layout_known_answer(tmp_path):
     f32 = lambda x: float(np.float32(x))  # noqa: E731
     assert s0 == [[(3, 40, f32(en[0])), (9, 34, f32(en[3]))], [(7, 36, f32(-3.0))]]
     assert _read_bp(p1) == [[(3, 40, f32(en[0])), (5, 38, f32(en[1])), (9, 34, f32(en[3]))]]
+
+
+def _out_cases():
+    edge = [0.0, -0.0, 1e-8, -1e-8, 0.0999999, 0.09999996, 0.1, 0.123456789, 1.0, 0.9999996, 9.9999996,
+            99999.94, 999999.4, 999999.6, 1e6, -1e6, 123456.789, -0.001234, 3.14159265, -42.0, 1e-101,
+            1.23e120, -7.5e-5, 0.05, 0.5, 5.0, 50.0, 500.0, 5000.0, 50000.0, 500000.0, 5e6, 12.3456749999]
+    rng = np.random.default_rng(5)
+    rand = list(rng.normal(size=40) * 10.0 ** rng.integers(-9, 9, size=40))
+    return np.array(edge + rand)
+
+
+@needs_ref
+@pytest.mark.parametrize("replica,ele", [(True, True), (False, True), (False, False)])
+def test_out_lines_match_reference_writer(tmp_path, replica, ele):
+    """Energy lines (i12, i4, f6.2, g13.6 fields) byte-identical to the
+    reference's write_out_header + write_out on edge values (zero, the
+    F/E switch points of G editing, three-digit exponents) and random
+    magnitudes from 1e-9 to 1e9."""
+    from sis_amd.engine import write_out_file
+    v = _out_cases()
+    n = len(v)
+    rng = np.random.default_rng(9)
+    ene = v[:, None] * rng.uniform(-2.0, 2.0, size=(n, 11))
+    ene[:, 0] = v
+    ekin = np.abs(v[::-1]) * 1.5
+    temps = rng.uniform(280.0, 380.0, size=n)
+    isteps = np.arange(n, dtype=np.int64) * 1000
+    labels = np.arange(1, n + 1) % 256 + 1
+    ours, ref = tmp_path / "ours.out", tmp_path / "ref.out"
+    write_out_file(ours, isteps, labels, temps, ekin, ene, replica, ele)
+    refdrv.write_out(ref, isteps, labels, temps, ekin, ene, replica=replica, ele=ele)
+    a, b = ours.read_text().splitlines(), ref.read_text().splitlines()
+    assert len(a) == len(b)
+    for x, y in zip(a, b):
+        assert x == y
