@@ -226,6 +226,21 @@ int sis_gpu_bp_close(sis_gpu_ctx *ctx);
 int sis_gpu_out_open(sis_gpu_ctx *ctx, const char *const *paths, int64_t nstep_save);
 int sis_gpu_out_close(sis_gpu_ctx *ctx);
 
+/* job_dcd (src/job_dcd.F90:59-80): each frame of the DCD file at path
+ * (read as src/dcd.F90 read_onestep does, float32 to double) becomes the
+ * coordinates of replica 1; the lists are rebuilt and the energies
+ * evaluated with the limit MC when nstep_bp_MC > 0.  energies (11,
+ * max_frames) receives them, *nframes the frame count; with out_path the
+ * .out lines of job_dcd are written (frame number, T, Ekin = 0). */
+int sis_gpu_dcd_energies(sis_gpu_ctx *ctx, const char *path, const char *out_path, int64_t max_frames,
+                         double *energies, int64_t *nframes);
+
+/* DCD reader on the host: up to max_frames frames into xyz (3, nmp,
+ * max_frames); *nframes = frames in the file; box (3) = the last unit
+ * cell, or -1 when the file has none. */
+int sis_dcd_read_file(const char *path, int32_t nmp, int64_t max_frames, double *xyz, double *box,
+                      int64_t *nframes);
+
 /* The same line writer on host arrays: energies (11, nline). */
 int sis_out_write_file(const char *path, int32_t nline, const int64_t *isteps, const int32_t *labels,
                        const double *temps, const double *ekin, const double *energies, int32_t replica,

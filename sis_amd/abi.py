@@ -124,6 +124,7 @@ EXPORTED = [
     "sis_gpu_set_step", "sis_gpu_set_labels",
     "sis_gpu_bp_open", "sis_gpu_bp_write", "sis_gpu_bp_close", "sis_bp_write_file",
     "sis_gpu_kinetic", "sis_gpu_out_open", "sis_gpu_out_close", "sis_out_write_file",
+    "sis_gpu_dcd_energies", "sis_dcd_read_file",
 ]
 
 
@@ -164,6 +165,8 @@ def _declare(lib: C.CDLL) -> None:
         "sis_gpu_kinetic": (i32, [vp, i32, P(d)]),
         "sis_gpu_out_open": (i32, [vp, P(C.c_char_p), i64]),
         "sis_gpu_out_close": (i32, [vp]),
+        "sis_gpu_dcd_energies": (i32, [vp, C.c_char_p, C.c_char_p, i64, P(d), P(i64)]),
+        "sis_dcd_read_file": (i32, [C.c_char_p, i32, i64, P(d), P(d), P(i64)]),
         "sis_out_write_file": (i32, [C.c_char_p, i32, P(i64), P(i32), P(d), P(d), P(d), i32, i32]),
         "sis_gpu_bp_open": (i32, [vp, P(C.c_char_p), i32, i64]),
         "sis_gpu_bp_write": (i32, [vp]),

@@ -569,3 +569,142 @@ int sis_gpu_out_close(sis_gpu_ctx *c) {
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------
+// Reading a DCD file (read_header / read_onestep of src/dcd.F90:146-536)
+// and the energy-only analysis of job_dcd (src/job_dcd.F90:59-80): every
+// frame becomes the coordinates of replica 1, the lists are rebuilt, the
+// energies evaluated (limit MC when nstep_bp_MC > 0), and a .out line
+// written with frame number, label 1, T and Ekin = 0 (job_dcd computes no
+// kinetic energy).
+namespace {
+
+struct DcdIn {
+  FILE *f = nullptr;
+  bool unitcell = false;
+  int32_t natom = 0;
+  ~DcdIn() {
+    if (f) fclose(f);
+  }
+};
+
+bool dcd_get(FILE *f, void *p, size_t n) { return fread(p, 1, n, f) == n; }
+int32_t dcd_get_i32(FILE *f) {
+  int32_t v;
+  if (!dcd_get(f, &v, 4)) throw ApiError{SIS_ERR_ARG, "truncated DCD file"};
+  return v;
+}
+void dcd_skip_block(FILE *f) {
+  const int32_t n = dcd_get_i32(f);
+  if (n < 0 || fseek(f, n, SEEK_CUR) != 0 || dcd_get_i32(f) != n) throw ApiError{SIS_ERR_ARG, "bad DCD block"};
+}
+
+void dcd_open_read(DcdIn &d, const char *path) {
+  d.f = fopen(path, "rb");
+  if (!d.f) throw ApiError{SIS_ERR_ARG, std::string("cannot open ") + path};
+  int32_t n = dcd_get_i32(d.f);
+  char cord[4];
+  int32_t control[20];
+  if (n != 84 || !dcd_get(d.f, cord, 4) || memcmp(cord, "CORD", 4) != 0 || !dcd_get(d.f, control, sizeof control) ||
+      dcd_get_i32(d.f) != 84)
+    throw ApiError{SIS_ERR_ARG, "not a DCD file"};
+  d.unitcell = control[10] == 1;
+  dcd_skip_block(d.f);   // titles
+  n = dcd_get_i32(d.f);
+  if (n != 4) throw ApiError{SIS_ERR_ARG, "bad DCD atom block"};
+  d.natom = dcd_get_i32(d.f);
+  if (dcd_get_i32(d.f) != 4) throw ApiError{SIS_ERR_ARG, "bad DCD atom block"};
+}
+
+// next frame into xyz (natom x 3, double); false at the end of the file
+bool dcd_read_frame(DcdIn &d, std::vector<double> &xyz, double box[3]) {
+  int32_t n;
+  if (!dcd_get(d.f, &n, 4)) return false;
+  if (d.unitcell) {
+    double cell[6];
+    if (n != 48 || !dcd_get(d.f, cell, sizeof cell) || dcd_get_i32(d.f) != 48)
+      throw ApiError{SIS_ERR_ARG, "bad DCD unit-cell block"};
+    box[0] = cell[0];
+    box[1] = cell[2];
+    box[2] = cell[5];
+    n = dcd_get_i32(d.f);
+  }
+  std::vector<float> c((size_t)d.natom);
+  xyz.assign((size_t)3 * d.natom, 0.0);
+  for (int k = 0; k < 3; k++) {
+    if (k > 0) n = dcd_get_i32(d.f);
+    if (n != 4 * d.natom || !dcd_get(d.f, c.data(), c.size() * 4) || dcd_get_i32(d.f) != n)
+      throw ApiError{SIS_ERR_ARG, "bad DCD coordinate block"};
+    for (int a = 0; a < d.natom; a++) xyz[(size_t)3 * a + k] = (double)c[a];
+  }
+  return true;
+}
+
+}  // namespace
+
+extern "C" {
+
+int sis_dcd_read_file(const char *path, int32_t nmp, int64_t max_frames, double *xyz, double *box,
+                      int64_t *nframes) {
+  API_BEGIN
+  if (!path || !nframes || max_frames < 0 || (max_frames > 0 && !xyz)) return fail(SIS_ERR_ARG, "invalid argument");
+  DcdIn d;
+  dcd_open_read(d, path);
+  if (d.natom != nmp) return fail(SIS_ERR_ARG, "atom count of the DCD file differs");
+  std::vector<double> x;
+  double b[3] = {-1.0, -1.0, -1.0};
+  int64_t k = 0;
+  while (dcd_read_frame(d, x, b)) {
+    if (k < max_frames) std::copy(x.begin(), x.end(), xyz + (size_t)3 * nmp * k);
+    k++;
+  }
+  if (box) std::copy(b, b + 3, box);
+  *nframes = k;
+  return SIS_OK;
+  API_END
+}
+
+int sis_gpu_dcd_energies(sis_gpu_ctx *c, const char *path, const char *out_path, int64_t max_frames,
+                         double *energies, int64_t *nframes) {
+  API_BEGIN
+  if (!c || !path || !nframes || max_frames < 0 || (max_frames > 0 && !energies))
+    return fail(SIS_ERR_ARG, "invalid argument");
+  DcdIn d;
+  dcd_open_read(d, path);
+  if (d.natom != c->N) return fail(SIS_ERR_ARG, "atom count of the DCD file differs from the system");
+  FILE *fo = nullptr;
+  if (out_path) {
+    fo = fopen(out_path, "wb");
+    if (!fo) return fail(SIS_ERR_ARG, std::string("cannot open ") + out_path);
+  }
+  try {
+    if (fo) out_header(fo, false, c->P.ele_on != 0);
+    const int N = c->N, R = c->R;
+    std::vector<double> x;
+    std::vector<double4> h((size_t)N);
+    double b[3];
+    std::vector<int> m(R, 0);
+    m[0] = 1;
+    std::vector<double> e16((size_t)R * 16);
+    int64_t k = 0;
+    while (dcd_read_frame(d, x, b)) {
+      for (int a = 0; a < N; a++) h[a] = double4{x[(size_t)3 * a], x[(size_t)3 * a + 1], x[(size_t)3 * a + 2], 0.0};
+      HIPCHK(hipMemcpyAsync(c->pos_d.p, h.data(), (size_t)N * sizeof(double4), hipMemcpyHostToDevice, c->stream));
+      c->lists_valid = false;
+      rebuild(c, m);
+      energy_phase(c, m, c->tK, c->P.nstep_bp_mc > 0 ? 1 : 0, e16.data());
+      if (k < max_frames) std::copy(e16.begin(), e16.begin() + 11, energies + (size_t)11 * k);
+      k++;
+      if (fo) out_line(fo, k, 1, c->tK[0], 0.0, e16.data(), false, c->P.ele_on != 0);
+    }
+    *nframes = k;
+  } catch (...) {
+    if (fo) fclose(fo);
+    throw;
+  }
+  if (fo) fclose(fo);
+  return SIS_OK;
+  API_END
+}
+
+}  // extern "C"

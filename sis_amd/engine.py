@@ -211,6 +211,15 @@ class Engine:
     def out_close(self):
         _check(lib().sis_gpu_out_close(self.h))
 
+    def dcd_energies(self, path, out_path=None, max_frames: int = 100000):
+        """job_dcd: energies (nframes, 11) of the frames of a DCD file."""
+        e = np.zeros((max_frames, 11))
+        n = C.c_int64()
+        _check(lib().sis_gpu_dcd_energies(self.h, str(path).encode(),
+                                          None if out_path is None else str(out_path).encode(), int(max_frames),
+                                          abi.dptr(e), C.byref(n)))
+        return e[: min(n.value, max_frames)].copy()
+
     # ---- restart ----
     def set_step(self, istep: int):
         _check(lib().sis_gpu_set_step(self.h, int(istep)))
@@ -269,3 +278,14 @@ def write_out_file(path, isteps, labels, temps, ekin, energies, replica: bool, e
     _check(lib().sis_out_write_file(str(path).encode(), len(ist), ist.ctypes.data_as(C.POINTER(C.c_int64)),
                                     abi.iptr(lab), abi.dptr(tk), abi.dptr(ek), abi.dptr(en), int(bool(replica)),
                                     int(bool(ele))))
+
+
+def read_dcd_file(path, nmp: int, max_frames: int = 100000):
+    """Host-side DCD reader of the library: (frames (n, nmp, 3), box (3,) or
+    -1 when the file has no unit cell)."""
+    x = np.zeros((max_frames, nmp, 3))
+    b = np.zeros(3)
+    n = C.c_int64()
+    _check(lib().sis_dcd_read_file(str(path).encode(), int(nmp), int(max_frames), abi.dptr(x), abi.dptr(b),
+                                   C.byref(n)))
+    return x[: min(n.value, max_frames)].copy(), b

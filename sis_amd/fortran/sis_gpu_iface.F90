@@ -223,6 +223,14 @@ module sis_gpu_iface
          type(c_ptr), value :: ctx
       end function
 
+      ! job_dcd: energies of the frames of a DCD file
+      integer(c_int) function sis_gpu_dcd_energies(ctx, path, out_path, max_frames, energies, nframes) &
+            bind(C, name="sis_gpu_dcd_energies")
+         import :: c_int, c_ptr, c_int64_t
+         type(c_ptr), value :: ctx, path, out_path, energies, nframes
+         integer(c_int64_t), value :: max_frames
+      end function
+
       integer(c_int) function sis_gpu_kinetic(ctx, irep, ekin) bind(C, name="sis_gpu_kinetic")
          import :: c_int, c_ptr, c_int32_t
          type(c_ptr), value :: ctx, ekin

@@ -168,3 +168,36 @@ def test_out_lines_during_steps(tmp_path, case):
         st, lb, tk, ek, en = zip(*rows[r])
         write_out_file(want, st, lb, tk, ek, np.array(en), replica=R > 1, ele=ele)
         assert paths[r].read_text() == want.read_text(), r
+
+
+def test_dcd_energy_analysis(tmp_path):
+    """job_dcd: energies of the frames of a DCD file equal those of the same
+    (float32-rounded) coordinates set by hand, and the .out lines are those
+    of the host writer (frame number, T, Ekin = 0)."""
+    from sis_amd.engine import write_dcd_file, write_out_file
+    s = configs.small_pbc()
+    print("[case] dcd energies", flush=True)
+    eng = Engine(s, seed=5, mode="phases")
+    eng.init_velocities()
+    eng.neighbor_list(0)
+    frames = []
+    for _ in range(4):
+        eng.step(30)
+        frames.append(eng.get_state()[0][0])
+    frames = np.array(frames)
+    p = tmp_path / "t.dcd"
+    write_dcd_file(p, frames, box=s["params"]["box"])
+    ana = Engine(s, seed=5, mode="phases")
+    e = ana.dcd_energies(p, out_path=tmp_path / "t.out")
+    assert e.shape == (4, 11)
+    chk = Engine(s, seed=5, mode="phases")
+    want = []
+    for f in frames.astype(np.float32).astype(np.float64):
+        chk.set_state(f[None])
+        chk.neighbor_list(0)
+        want.append(chk.energy_sumup(1))
+    assert np.array_equal(e, np.array(want))
+    host = tmp_path / "h.out"
+    write_out_file(host, np.arange(1, 5), np.ones(4), np.full(4, s["params"]["tempK"]), np.zeros(4), e,
+                   replica=False, ele=True)
+    assert (tmp_path / "t.out").read_text() == host.read_text()

@@ -121,3 +121,18 @@ def test_out_lines_match_reference_writer(tmp_path, replica, ele):
     assert len(a) == len(b)
     for x, y in zip(a, b):
         assert x == y
+
+
+@needs_ref
+@pytest.mark.parametrize("box", [None, (300.0, 310.0, 320.0)])
+def test_dcd_reader_round_trip_of_reference_file(tmp_path, box):
+    """Frames written by the reference's writer read back by the library's
+    reader (read_onestep of src/dcd.F90): float32 values and the unit cell."""
+    from sis_amd.engine import read_dcd_file
+    x = _frames(3, 4, 77)
+    p = tmp_path / "ref.dcd"
+    refdrv.write_dcd(p, x, box=None if box is None else np.array(box), fix_com=0)
+    got, b = read_dcd_file(p, 77)
+    assert got.shape == x.shape
+    assert np.array_equal(got, x.astype(np.float32).astype(np.float64))
+    assert np.array_equal(b, [-1.0, -1.0, -1.0] if box is None else box)
