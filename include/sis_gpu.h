@@ -235,6 +235,13 @@ int sis_gpu_out_close(sis_gpu_ctx *ctx);
 int sis_gpu_dcd_energies(sis_gpu_ctx *ctx, const char *path, const char *out_path, int64_t max_frames,
                          double *energies, int64_t *nframes);
 
+/* job_check_force (src/job_check_force.F90:46-82): forces of replica irep
+ * (f_force, 3 x nmp) against central differences of its total energy with
+ * step h (f_energy, 3 x nmp); the limit MC as the run is configured.  With
+ * out_path the per-bead lines of job_check_force are written. */
+int sis_gpu_check_force(sis_gpu_ctx *ctx, int32_t irep, double h, const char *out_path, double *f_energy,
+                        double *f_force);
+
 /* DCD reader on the host: up to max_frames frames into xyz (3, nmp,
  * max_frames); *nframes = frames in the file; box (3) = the last unit
  * cell, or -1 when the file has none. */

@@ -124,7 +124,7 @@ EXPORTED = [
     "sis_gpu_set_step", "sis_gpu_set_labels",
     "sis_gpu_bp_open", "sis_gpu_bp_write", "sis_gpu_bp_close", "sis_bp_write_file",
     "sis_gpu_kinetic", "sis_gpu_out_open", "sis_gpu_out_close", "sis_out_write_file",
-    "sis_gpu_dcd_energies", "sis_dcd_read_file",
+    "sis_gpu_dcd_energies", "sis_dcd_read_file", "sis_gpu_check_force",
 ]
 
 
@@ -166,6 +166,7 @@ def _declare(lib: C.CDLL) -> None:
         "sis_gpu_out_open": (i32, [vp, P(C.c_char_p), i64]),
         "sis_gpu_out_close": (i32, [vp]),
         "sis_gpu_dcd_energies": (i32, [vp, C.c_char_p, C.c_char_p, i64, P(d), P(i64)]),
+        "sis_gpu_check_force": (i32, [vp, i32, d, C.c_char_p, P(d), P(d)]),
         "sis_dcd_read_file": (i32, [C.c_char_p, i32, i64, P(d), P(d), P(i64)]),
         "sis_out_write_file": (i32, [C.c_char_p, i32, P(i64), P(i32), P(d), P(d), P(d), i32, i32]),
         "sis_gpu_bp_open": (i32, [vp, P(C.c_char_p), i32, i64]),

@@ -708,3 +708,72 @@ int sis_gpu_dcd_energies(sis_gpu_ctx *c, const char *path, const char *out_path,
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------
+// job_check_force (src/job_check_force.F90:46-82) on the device: forces of
+// replica irep against central differences of its total energy with step
+// h along every coordinate (the lists stay valid: h is far below the
+// skin), with the limit MC as the run is configured.  With out_path the
+// per-bead lines of job_check_force are written.
+extern "C" int sis_gpu_check_force(sis_gpu_ctx *c, int32_t irep, double h, const char *out_path, double *f_energy,
+                                   double *f_force) {
+  API_BEGIN
+  if (!c) return fail(SIS_ERR_ARG, "null context");
+  if (irep < 1 || irep > c->R || !(h > 0.0)) return fail(SIS_ERR_ARG, "invalid argument");
+  if (!c->lists_valid) return fail(SIS_ERR_STATE, "check_force called before rebuild");
+  const int N = c->N;
+  const int mc = c->P.nstep_bp_mc > 0 ? 1 : 0;
+  std::vector<double> F((size_t)3 * N), fe((size_t)3 * N);
+  if (sis_gpu_force(c, irep, mc, F.data()) != SIS_OK) return SIS_ERR_STATE;
+  double4 *xd = c->pos_d.p + (size_t)(irep - 1) * N;
+  std::vector<double4> x((size_t)N);
+  HIPCHK(hipMemcpyAsync(x.data(), xd, (size_t)N * sizeof(double4), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  double e[11];
+  auto energy_at = [&](int a, int k, double v) {
+    double4 p = x[a];
+    (k == 0 ? p.x : (k == 1 ? p.y : p.z)) = v;
+    HIPCHK(hipMemcpyAsync(xd + a, &p, sizeof p, hipMemcpyHostToDevice, c->stream));
+    if (sis_gpu_energy(c, irep, -1.0, mc, e) != SIS_OK) throw ApiError{SIS_ERR_STATE, g_err};
+    return e[0];
+  };
+  for (int a = 0; a < N; a++) {
+    for (int k = 0; k < 3; k++) {
+      const double x0 = k == 0 ? x[a].x : (k == 1 ? x[a].y : x[a].z);
+      const double ep = energy_at(a, k, x0 + h);
+      const double em = energy_at(a, k, x0 - h);
+      fe[(size_t)3 * a + k] = -(ep - em) / (2.0 * h);
+    }
+    HIPCHK(hipMemcpyAsync(xd + a, &x[a], sizeof(double4), hipMemcpyHostToDevice, c->stream));
+  }
+  HIPCHK(hipStreamSynchronize(c->stream));
+  if (out_path) {
+    FILE *f = fopen(out_path, "wb");
+    if (!f) return fail(SIS_ERR_ARG, std::string("cannot open ") + out_path);
+    char b[64];
+    for (int a = 0; a < N; a++) {
+      std::string s;
+      snprintf(b, sizeof b, "imp=%6d  f_energy", a + 1);
+      s += b;
+      for (int k = 0; k < 3; k++) s += " " + fortran_g(fe[(size_t)3 * a + k], 11, 4);
+      s += "  f_force=";
+      for (int k = 0; k < 3; k++) s += " " + fortran_g(F[(size_t)3 * a + k], 11, 4);
+      snprintf(b, sizeof b, "\nf_energy - f_force%6d", a + 1);
+      s += b;
+      for (int k = 0; k < 3; k++) {
+        snprintf(b, sizeof b, "%11.4f", fe[(size_t)3 * a + k] - F[(size_t)3 * a + k]);
+        s += b;
+      }
+      s += "\n";
+      if (fwrite(s.data(), 1, s.size(), f) != s.size()) {
+        fclose(f);
+        return fail(SIS_ERR_ARG, "write failed");
+      }
+    }
+    fclose(f);
+  }
+  if (f_energy) std::copy(fe.begin(), fe.end(), f_energy);
+  if (f_force) std::copy(F.begin(), F.end(), f_force);
+  return SIS_OK;
+  API_END
+}

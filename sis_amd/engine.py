@@ -220,6 +220,15 @@ class Engine:
                                           abi.dptr(e), C.byref(n)))
         return e[: min(n.value, max_frames)].copy()
 
+    def check_force(self, irep: int = 1, h: float = 1e-3, out_path=None):
+        """job_check_force: (f_energy, f_force), each (nmp, 3)."""
+        fe = np.zeros((self.nmp, 3))
+        ff = np.zeros((self.nmp, 3))
+        _check(lib().sis_gpu_check_force(self.h, int(irep), float(h),
+                                         None if out_path is None else str(out_path).encode(), abi.dptr(fe),
+                                         abi.dptr(ff)))
+        return fe, ff
+
     # ---- restart ----
     def set_step(self, istep: int):
         _check(lib().sis_gpu_set_step(self.h, int(istep)))

@@ -223,6 +223,15 @@ module sis_gpu_iface
          type(c_ptr), value :: ctx
       end function
 
+      ! job_check_force: forces against central differences of the energy
+      integer(c_int) function sis_gpu_check_force(ctx, irep, h, out_path, f_energy, f_force) &
+            bind(C, name="sis_gpu_check_force")
+         import :: c_int, c_ptr, c_int32_t, c_double
+         type(c_ptr), value :: ctx, out_path, f_energy, f_force
+         integer(c_int32_t), value :: irep
+         real(c_double), value :: h
+      end function
+
       ! job_dcd: energies of the frames of a DCD file
       integer(c_int) function sis_gpu_dcd_energies(ctx, path, out_path, max_frames, energies, nframes) &
             bind(C, name="sis_gpu_dcd_energies")

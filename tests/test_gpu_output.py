@@ -201,3 +201,21 @@ def test_dcd_energy_analysis(tmp_path):
     write_out_file(host, np.arange(1, 5), np.ones(4), np.full(4, s["params"]["tempK"]), np.zeros(4), e,
                    replica=False, ele=True)
     assert (tmp_path / "t.out").read_text() == host.read_text()
+
+
+def test_check_force_on_device(tmp_path):
+    """job_check_force on the device (h = 1e-3, limit MC off as in the
+    oracle's own finite-difference pin): central differences of the total
+    energy agree with the forces to 1e-4 relative (floor 1 kcal/mol/A),
+    and the report has the reference's two lines per bead."""
+    s = configs.small_pbc(3, 20, 90.0, seed=3)
+    s["params"] = dict(s["params"], nstep_bp_mc=0)
+    print("[case] check_force", flush=True)
+    eng = Engine(s, mode="phases")
+    eng.neighbor_list(0)
+    fe, ff = eng.check_force(1, 1e-3, out_path=tmp_path / "cf.out")
+    rel = np.abs(fe - ff) / np.maximum(1.0, np.abs(ff))
+    assert rel.max() < 1e-4, rel.max()
+    lines = (tmp_path / "cf.out").read_text().splitlines()
+    assert len(lines) == 2 * ff.shape[0]
+    assert lines[0].startswith("imp=     1  f_energy") and lines[1].startswith("f_energy - f_force     1")
