@@ -70,7 +70,9 @@ typedef struct sis_params {
     int32_t dih_type;          /* 0: none, 1: cosine, 2: exp (Gaussian) */
     int32_t ele_on, ele_cutoff_type, ele_excl_bond, ele_excl_angl, dummy_has_charge;
     int32_t bp_model, bp_min_loop, max_bp_per_nt, nstep_bp_mc, nnend;
-    int32_t pbc, pad;
+    int32_t pbc;
+    int32_t integrator;        /* INTGRT (src/const_idx.F90:51-57): 1 GJF-2GJ Langevin,
+                                  2 Ermak-McCammon Brownian; 0 means 1 */
 } sis_params;
 
 /* Topology: what read_fasta / init_sequence build (src/var_top.F90). */
@@ -156,7 +158,7 @@ int sis_gpu_init_velocities(sis_gpu_ctx *ctx);
 
 /* nsteps of the job_md loop body for every replica: skin test and
  * rebuild, force (with the BP limit MC every nstep_bp_mc steps), noise,
- * GJF-2GJ update (src/job_md.F90:360-460), and replica exchange every
+ * GJF-2GJ or Ermak-McCammon update (src/job_md.F90:360-460), and replica exchange every
  * exchange_every steps (src/job_md.F90:462-570).  *rebuilds (may be NULL)
  * returns the number of neighbour-list rebuilds done. */
 int sis_gpu_step(sis_gpu_ctx *ctx, int64_t nsteps, int64_t *rebuilds);

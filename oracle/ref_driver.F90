@@ -49,7 +49,7 @@ module sisref_driver
       integer(c_int32_t) :: angl_reb, dih_type
       integer(c_int32_t) :: ele_on, ele_cutoff_type, ele_excl_bond, ele_excl_angl, dummy_has_charge
       integer(c_int32_t) :: bp_model, bp_min_loop, max_bp_per_nt, nstep_bp_mc, nnend
-      integer(c_int32_t) :: pbc, pad
+      integer(c_int32_t) :: pbc, integrator
    end type c_params
 
 contains
@@ -134,6 +134,7 @@ contains
       kT = BOLTZ_KCAL_MOL * tempK
       rng_seed = rng_seed_in
       integrator = INTGRT%LD_GJF2GJ
+      if (p%integrator == INTGRT%BD_EM) integrator = INTGRT%BD_EM
       viscosity_Pas = p%viscosity_pas
       dt = p%dt
       nstep = 0

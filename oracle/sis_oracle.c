@@ -962,10 +962,13 @@ orc_sys *orc_create(const sis_topo *topo, const sis_params *prm, int32_t nrep, c
     s->xyz = calloc(n3, sizeof(double)); s->velos = calloc(n3, sizeof(double));
     s->accels = calloc(n3, sizeof(double)); s->move = calloc(n3, sizeof(double));
     s->rep = calloc(nrep, sizeof(rep_t));
-    /* friction and masses (src/job_md.F90:94-133) */
+    /* friction and masses (src/job_md.F90:94-133): the viscosity in
+     * internal units differs between the Langevin and the Brownian
+     * integrator (lines 98-100 and 106-108) */
     s->mass = malloc(sizeof(double) * s->nmp);
     s->fric = malloc(sizeof(double) * s->nmp);
-    double v = prm->viscosity_pas * sqrt(1.0e3 / KCAL2JOUL) * N_AVO * 1.0e-20;
+    double v = prm->integrator == 2 ? (JOUL2KCAL_MOL * 1.0e-18) * prm->viscosity_pas
+                                    : prm->viscosity_pas * sqrt(1.0e3 / KCAL2JOUL) * N_AVO * 1.0e-20;
     for (int i = 0; i < s->nmp; i++) { s->mass[i] = md_mass(s->seq[i]); s->fric[i] = 6.0 * PI_ * v * 10.0; }
     /* replicas: labels, values and the odd/even pair table of a
      * one-dimensional temperature ladder (src/init_replica.F90:70-125,
@@ -1045,8 +1048,17 @@ void orc_genrand(orc_sys *s, int32_t irep, int32_t kind, int64_t n, double *out)
 void orc_labels(orc_sys *s, int32_t *rep2lab) { for (int k = 1; k <= s->nrep; k++) rep2lab[k - 1] = s->rep2lab[k]; }
 
 /* set_md_coef (src/job_md.F90:700-767): a, sqrt(b)h/m, sqrt(b)h, and the
- * per-replica noise amplitude sqrt(b)/2m*sqrt(2 gamma kT h) */
+ * per-replica noise amplitude sqrt(b)/2m*sqrt(2 gamma kT h); for the
+ * Brownian integrator (lines 725-731, 755-761) a = h/gamma and the noise
+ * amplitude is sqrt(2 kT h / gamma), cf = cx = 0 */
 static void md_coef(orc_sys *s, int i, double tK, double *a, double *cf, double *cx, double *cr) {
+    if (s->p.integrator == 2) {
+        *a = s->p.dt / s->fric[i];
+        *cf = 0.0;
+        *cx = 0.0;
+        *cr = sqrt(2.0 * BOLTZ_KCAL_MOL * tK * s->p.dt / s->fric[i]);
+        return;
+    }
     double c1 = 0.5 * s->p.dt * s->fric[i] / s->mass[i];
     double c2 = sqrt(1.0 / (1.0 + c1));
     *a = (1.0 - c1) / (1.0 + c1);
@@ -1193,6 +1205,15 @@ int64_t orc_md(orc_sys *s, int64_t nsteps, int32_t nstep_exchange, double *energ
             for (int i = 0; i < n; i++) {
                 double a, cf, cx, cr;
                 md_coef(s, i, s->rep[r].tK, &a, &cf, &cx, &cr);
+                if (s->p.integrator == 2) {
+                    /* Ermak-McCammon (src/job_md.F90:446-456) */
+                    for (int k = 0; k < 3; k++) {
+                        double dx = a * forces[3 * i + k] + cr * rnd[3 * i + k];
+                        x[3 * i + k] = x[3 * i + k] + dx;
+                        mv[3 * i + k] = mv[3 * i + k] + dx;
+                    }
+                    continue;
+                }
                 for (int k = 0; k < 3; k++) {
                     double apre = cr * rnd[3 * i + k];
                     v[3 * i + k] = a * v[3 * i + k] + cf * forces[3 * i + k] + (ac[3 * i + k] + apre);

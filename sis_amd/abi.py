@@ -61,7 +61,7 @@ class SisParams(C.Structure):
         ("dummy_has_charge", C.c_int32),
         ("bp_model", C.c_int32), ("bp_min_loop", C.c_int32),
         ("max_bp_per_nt", C.c_int32), ("nstep_bp_mc", C.c_int32), ("nnend", C.c_int32),
-        ("pbc", C.c_int32), ("pad", C.c_int32),
+        ("pbc", C.c_int32), ("integrator", C.c_int32),
     ]
 
 

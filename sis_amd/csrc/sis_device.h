@@ -39,7 +39,8 @@ struct Params {
   double wca_nl_cut2, bp_nl_cut2;
   int angl_reb, dih_type, ele_on, ele_excl_bond, ele_excl_angl;
   int bp_model, bp_min_loop, max_bp_per_nt, nstep_bp_mc, pbc;
-  int N, R, fix_h4, nchains, maxlen, pad;
+  int N, R, fix_h4, nchains, maxlen;
+  int bd;  // 1: Ermak-McCammon Brownian update in place of GJF-2GJ
 };
 
 // Per-bead topology word: chain start (replica-local 0-based), chain

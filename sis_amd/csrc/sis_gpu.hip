@@ -419,6 +419,10 @@ void upload_conditions(sis_gpu_ctx *c) {
   for (int r = 0; r < R; r++) {
     host_set_ele(c->prm, c->tK[r], c->ionic[r], coef[r], lam[r], cut[r]);
     for (int i = 0; i < N; i++) {
+      if (c->P.bd) {  // src/job_md.F90:755-761
+        cr[(size_t)r * N + i] = std::sqrt(2.0 * KB_KCAL * c->tK[r] * c->prm.dt / c->fric[i]);
+        continue;
+      }
       double c1 = 0.5 * c->prm.dt * c->fric[i] / c->mass[i];
       double c2 = std::sqrt(1.0 / (1.0 + c1));
       cr[(size_t)r * N + i] =
@@ -813,6 +817,8 @@ int sis_gpu_create(const sis_topo *topo, const sis_params *prm, const sis_option
   if (prm->bp_model != 5 && prm->bp_model != 0)
     return fail(SIS_ERR_UNSUPPORTED, "only base-pair model 5 (triplet) is implemented on the GPU");
   if (prm->max_bp_per_nt < 1) return fail(SIS_ERR_ARG, "max_bp_per_nt must be >= 1");
+  if (prm->integrator < 0 || prm->integrator > 2)
+    return fail(SIS_ERR_ARG, "integrator must be 1 (GJF-2GJ) or 2 (Ermak-McCammon)");
   sis_options o{};
   if (opt) o = *opt;
   if (o.nrep <= 0) o.nrep = 1;
@@ -889,6 +895,7 @@ int sis_gpu_create(const sis_topo *topo, const sis_params *prm, const sis_option
     P.bp_model = prm->bp_model; P.bp_min_loop = prm->bp_min_loop;
     P.max_bp_per_nt = prm->max_bp_per_nt; P.nstep_bp_mc = prm->nstep_bp_mc; P.pbc = prm->pbc;
     P.N = N; P.R = R; P.fix_h4 = 0;
+    P.bd = prm->integrator == 2;
     for (int t = 0; t < 3; t++) {
       const sis_bp_para &b = prm->bp[t];
       BpPara m;
@@ -920,14 +927,22 @@ int sis_gpu_create(const sis_topo *topo, const sis_params *prm, const sis_option
     HIPCHK(hipMemcpy(c->bp3_hash_d.p, hash.data(), hash.size() * sizeof(int), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(c->bp3_dH_d.p, dH.data(), dH.size() * sizeof(double), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(c->bp3_dS_d.p, dS.data(), dS.size() * sizeof(double), hipMemcpyHostToDevice));
-    // dynamics coefficients (src/job_md.F90:94-133, 700-767)
+    // dynamics coefficients (src/job_md.F90:94-133, 700-767); the Brownian
+    // integrator takes the viscosity in other units (lines 102-108) and
+    // keeps only a = h/gamma (lines 725-731)
     c->mass.resize(N);
     c->fric.resize(N);
-    double v = prm->viscosity_pas * std::sqrt(1.0e3 / 4184.0) * 6.02214076e23 * 1.0e-20;
+    double v = P.bd ? ((1.0 / 4184.0) * 6.02214076e23 * 1.0e-18) * prm->viscosity_pas
+                    : prm->viscosity_pas * std::sqrt(1.0e3 / 4184.0) * 6.02214076e23 * 1.0e-20;
     std::vector<double> ma(N), mf(N), mx(N);
     for (int i = 0; i < N; i++) {
       c->mass[i] = md_mass(topo->seq[i]);
       c->fric[i] = 6.0 * PI_D * v * 10.0;
+      if (P.bd) {
+        ma[i] = prm->dt / c->fric[i];
+        mf[i] = mx[i] = 0.0;
+        continue;
+      }
       double c1 = 0.5 * prm->dt * c->fric[i] / c->mass[i];
       double c2 = std::sqrt(1.0 / (1.0 + c1));
       ma[i] = (1.0 - c1) / (1.0 + c1);

@@ -1069,6 +1069,40 @@ __device__ __forceinline__ V3<double> bead_force(const Params &P, const Ptrs &D,
   return F;
 }
 
+// GJF-2GJ update of one bead (src/job_md.F90:415-444), or the
+// Ermak-McCammon Brownian step (lines 446-456) when P.bd; g = r*N + a.
+// Returns |move|^2.
+__device__ __forceinline__ double move_bead(const Params &P, const Ptrs &D, size_t g, int a,
+                                            const V3<double> &F, double n0, double n1, double n2) {
+  const double cr = D.md_cr[g], ca = D.md_a[a];
+  const double4 p = D.pos[g];
+  double4 mv = D.move[g];
+  double dx, dy, dz;
+  if (P.bd) {
+    // dx = (h/gamma) F + sqrt(2 kT h/gamma) xi; velocities are not used
+    dx = ca * F.x + cr * n0;
+    dy = ca * F.y + cr * n1;
+    dz = ca * F.z + cr * n2;
+  } else {
+    const double cf = D.md_cf[a], cx = D.md_cx[a];
+    double4 v = D.vel[g];
+    const double4 ac = D.acc[g];
+    const double b0 = cr * n0, b1 = cr * n1, b2 = cr * n2;
+    v.x = ca * v.x + cf * F.x + (ac.x + b0);
+    v.y = ca * v.y + cf * F.y + (ac.y + b1);
+    v.z = ca * v.z + cf * F.z + (ac.z + b2);
+    dx = cx * v.x; dy = cx * v.y; dz = cx * v.z;
+    D.vel[g] = v;
+    D.acc[g] = double4{b0, b1, b2, 0.0};
+  }
+  D.frc[g] = double4{F.x, F.y, F.z, 0.0};
+  mv.x += dx; mv.y += dy; mv.z += dz;
+  D.move[g] = mv;
+  // other beads still read D.pos in this pass: write the next buffer
+  D.pos_out[g] = double4{p.x + dx, p.y + dy, p.z + dz, p.w};
+  return dot_nc(mv.x, mv.y, mv.z, mv.x, mv.y, mv.z);
+}
+
 // Force (and optionally the GJF-2GJ update, src/job_md.F90:415-444) per
 // bead; block-level top-2 of the accumulated displacement for the skin
 // test (src/job_md.F90:376-386).
@@ -1102,21 +1136,7 @@ __global__ void __launch_bounds__(B) k_force_int(Params P, Ptrs D, int integrate
         n1 = rr * sin(6.283185307179586 * u2);
         n2 = r2 * cos(6.283185307179586 * u4);
       }
-      double cr = D.md_cr[g], ca = D.md_a[a], cf = D.md_cf[a], cx = D.md_cx[a];
-      double4 v = D.vel[g], ac = D.acc[g], p = D.pos[g], mv = D.move[g];
-      double b0 = cr * n0, b1 = cr * n1, b2 = cr * n2;
-      v.x = ca * v.x + cf * F.x + (ac.x + b0);
-      v.y = ca * v.y + cf * F.y + (ac.y + b1);
-      v.z = ca * v.z + cf * F.z + (ac.z + b2);
-      ac.x = b0; ac.y = b1; ac.z = b2;
-      double dx = cx * v.x, dy = cx * v.y, dz = cx * v.z;
-      D.vel[g] = v;
-      D.acc[g] = ac;
-      D.frc[g] = double4{F.x, F.y, F.z, 0.0};
-      mv.x += dx; mv.y += dy; mv.z += dz;
-      D.move[g] = mv;
-      D.pos_out[g] = double4{p.x + dx, p.y + dy, p.z + dz, p.w};
-      d2 = dot_nc(mv.x, mv.y, mv.z, mv.x, mv.y, mv.z);
+      d2 = move_bead(P, D, g, a, F, n0, n1, n2);
     }
   }
   if (!integrate) return;
@@ -1260,8 +1280,9 @@ __device__ __forceinline__ V3<double> bead_force_part(const Params &P, const Ptr
   return bonded_part(P, D, r, a, l, LPB) + pair_part<T>(P, D, r, a, l, LPB);
 }
 
-// GJF-2GJ update of one bead (src/job_md.F90:415-444); returns |move|^2
-// r indexes the per-replica arrays; rk keys the random stream.
+// One bead's update with its noise draws (the injected MT words in the
+// parity form, else the counter RNG); r indexes the per-replica arrays,
+// rk keys the random stream.
 __device__ __forceinline__ double gjf_update(const Params &P, const Ptrs &D, int r, int rk, int a,
                                              const V3<double> &F, int parity, uint64_t seed,
                                              int64_t step) {
@@ -1280,21 +1301,7 @@ __device__ __forceinline__ double gjf_update(const Params &P, const Ptrs &D, int
     n1 = rr * sin(6.283185307179586 * u2);
     n2 = r2 * cos(6.283185307179586 * u4);
   }
-  double cr = D.md_cr[g], ca = D.md_a[a], cf = D.md_cf[a], cx = D.md_cx[a];
-  double4 v = D.vel[g], ac = D.acc[g], p = D.pos[g], mv = D.move[g];
-  double b0 = cr * n0, b1 = cr * n1, b2 = cr * n2;
-  v.x = ca * v.x + cf * F.x + (ac.x + b0);
-  v.y = ca * v.y + cf * F.y + (ac.y + b1);
-  v.z = ca * v.z + cf * F.z + (ac.z + b2);
-  double dx = cx * v.x, dy = cx * v.y, dz = cx * v.z;
-  D.vel[g] = v;
-  D.acc[g] = double4{b0, b1, b2, 0.0};
-  D.frc[g] = double4{F.x, F.y, F.z, 0.0};
-  mv.x += dx; mv.y += dy; mv.z += dz;
-  D.move[g] = mv;
-  // other beads still read D.pos in this pass: write the next buffer
-  D.pos_out[g] = double4{p.x + dx, p.y + dy, p.z + dz, p.w};
-  return dot_nc(mv.x, mv.y, mv.z, mv.x, mv.y, mv.z);
+  return move_bead(P, D, g, a, F, n0, n1, n2);
 }
 
 template <int LPB>

@@ -95,7 +95,6 @@ bool same_force_field(Params a, Params b) {
   a.R = b.R = 0;
   a.nchains = b.nchains = 0;
   a.maxlen = b.maxlen = 0;
-  a.pad = b.pad = 0;
   return std::memcmp(&a, &b, sizeof(Params)) == 0;
 }
 

@@ -57,6 +57,7 @@ def default_params(**over) -> dict:
         bp=copy.deepcopy(_BP),
         tempK=300.0, dt=0.05, viscosity_pas=1.4115e-5, nl_margin=10.0,
         pbc=0, box=[0.0, 0.0, 0.0],
+        integrator=1,            # 1 GJF-2GJ Langevin, 2 Ermak-McCammon Brownian
     )
     p.update(over)
     return p

@@ -36,7 +36,7 @@ module sis_gpu_iface
       integer(c_int32_t) :: angl_reb, dih_type
       integer(c_int32_t) :: ele_on, ele_cutoff_type, ele_excl_bond, ele_excl_angl, dummy_has_charge
       integer(c_int32_t) :: bp_model, bp_min_loop, max_bp_per_nt, nstep_bp_mc, nnend
-      integer(c_int32_t) :: pbc, pad
+      integer(c_int32_t) :: pbc, integrator
    end type sis_params
 
    type, bind(C) :: sis_topo

@@ -67,12 +67,23 @@ def remd_mc3(nrep=4, seed=9):
     return sysd
 
 
+def remd_bd(nrep=4, seed=4):
+    """Ermak-McCammon Brownian dynamics (src/job_md.F90:446-456) on a
+    temperature ladder, dt 0.005 ps, a 1.5 A skin: rebuilds and exchanges
+    inside the run."""
+    sysd = configs.remd_c3(nrep=nrep, n=30, seed=seed, margin=1.5)
+    sysd["params"] = dict(sysd["params"], integrator=2, dt=0.005)
+    sysd["name"] = "remd4bd"
+    return sysd
+
+
 SYSTEMS = {
     "hairpin64": (lambda: configs.hairpin64(1), 60, 0),
     "small_pbc": (lambda: configs.small_pbc(4, 30, 100.0, seed=5), 60, 0),
     "mixed": (mixed_chains, 40, 0),
     "remd4": (remd_small, 40, 10),
     "remd4mc3": (remd_mc3, 40, 10),
+    "remd4bd": (remd_bd, 40, 10),
 }
 
 

@@ -20,7 +20,7 @@ from tests.golden.make_golden import load
 
 pytestmark = pytest.mark.gpu
 
-FIXTURES = ["hairpin64", "small_pbc", "mixed", "remd4", "remd4mc3"]
+FIXTURES = ["hairpin64", "small_pbc", "mixed", "remd4", "remd4mc3", "remd4bd"]
 
 
 def _sorted_rows(a):
@@ -250,6 +250,37 @@ def test_batched_form_matches_phase_form(mc_every):
     assert np.array_equal(res[0][1], res[1][1])
     for a, b in zip(res[0][0], res[1][0]):
         assert np.abs(a - b).max() < 1e-6
+
+
+@pytest.mark.parametrize("form", ["batched", "persistent"])
+def test_brownian_forms_match_phase_form(form):
+    """Ermak-McCammon steps (src/job_md.F90:446-456) in the batched form
+    (8 replicas of a 128-nt chain, exchanges every 10 steps) and the
+    persistent form (one 2,000-nt condensate replica) follow the per-phase
+    form on the same counter-RNG streams.  Velocities are not integrated:
+    without exchanges (which rescale them) they stay as initialised."""
+    if form == "batched":
+        s = configs.remd_c3(nrep=8, n=128, seed=3)
+        kw = dict(exchange_every=10)
+    else:
+        s = configs.condensate_c2()
+        kw = {}
+    s["params"] = dict(s["params"], integrator=2, dt=0.005)
+    print(f"[case] brownian {form} vs phases", flush=True)
+    res = []
+    for mode in ("phases", form):
+        eng = Engine(s, parity=False, mode=mode, seed=41, **kw)
+        eng.init_velocities()
+        v0 = eng.get_state()[1]
+        eng.neighbor_list(0)
+        eng.step(60)
+        st = eng.get_state()
+        if not kw:
+            assert np.array_equal(st[1], v0)
+        assert np.all(np.isfinite(st[0]))
+        res.append((st, eng.labels()))
+    assert np.array_equal(res[0][1], res[1][1])
+    assert np.abs(res[0][0][0] - res[1][0][0]).max() < 1e-6
 
 
 def _largest_formed_component(eng, irep=1):

@@ -10,7 +10,7 @@ from sis_amd import configs
 from oracle import corc, refdrv
 from tests.golden.make_golden import load
 
-FIXTURES = ["hairpin64", "small_pbc", "mixed", "remd4", "remd4mc3"]
+FIXTURES = ["hairpin64", "small_pbc", "mixed", "remd4", "remd4mc3", "remd4bd"]
 
 
 def _oracle_from_fixture(name):
@@ -177,6 +177,51 @@ def test_free_particle_velocity_decay():
     c1 = 0.5 * 0.05 * gamma_over_m / mass
     assert np.allclose(a, (1 - c1) / (1 + c1), rtol=1e-14)
     assert np.allclose(c[:, 2], math.sqrt(1.0) * 0.05 / np.sqrt(1 + c1), rtol=1e-14)
+
+
+def test_brownian_coefficients_stokes_einstein():
+    """Ermak-McCammon (src/job_md.F90:102-108, 725-731, 755-761): the drift
+    per unit force h/gamma and the noise amplitude sqrt(2 D h) obey the
+    fluctuation-dissipation relation cr^2 / (2 a) = kT, and the diffusion
+    constant is the Stokes-Einstein one, kB T / (6 pi eta R) with R = 10 A,
+    computed here in SI units and converted to A^2/ps."""
+    s = configs.remd_c3(nrep=2, n=20, seed=1)
+    s["params"] = dict(s["params"], integrator=2, dt=0.004)
+    o = corc.Oracle(s)
+    kb_si, na = 1.380649e-23, 6.02214076e23
+    for r in (1, 2):
+        tK = float(s["temps"][r - 1])
+        c = o.md_coef(r)
+        a, cr = c[:, 0], c[:, 3]
+        kT = kb_si * tK * na / 4184.0                      # kcal/mol
+        assert np.allclose(cr ** 2 / (2 * a), kT, rtol=1e-13)
+        d_si = kb_si * tK / (6 * math.pi * 1.4115e-5 * 10e-10)   # m^2/s
+        assert np.allclose(cr ** 2 / (2 * 0.004), d_si * 1e8, rtol=1e-12)
+        assert np.all(c[:, 1] == 0) and np.all(c[:, 2] == 0)
+
+
+@pytest.mark.ref
+def test_live_reference_brownian_random_system():
+    """Brownian dynamics on a fresh 4-replica ladder with a 0.5 A skin
+    (a rebuild on nearly every step) and exchanges every 10 steps: oracle
+    trajectory vs the reference build."""
+    s = configs.remd_c3(nrep=4, n=30, seed=14, margin=0.5)
+    s["params"] = dict(s["params"], integrator=2, dt=0.005)
+    nstep, nex = 40, 10
+    ops = [("tape", r, 400000, f"t{r}") for r in range(1, 5)] + [("tape", -1, 4096, "tr")]
+    ref0 = refdrv.run(s, ops, seed=7, nstep_exchange=nex)
+    ref1 = refdrv.run(s, [("job_md", nstep, "R")], seed=7, nstep_exchange=nex)["R"]
+    o = corc.Oracle(s)
+    for r in range(1, 5):
+        o.set_tape(r, ref0[f"t{r}"])
+    o.set_tape(-1, ref0["tr"])
+    o.init_velocities()
+    nreb, _ = o.md(nstep, nex)
+    assert nreb >= 40
+    x, v, a = o.state()
+    assert np.abs(x - ref1["xyz"]).max() < 1e-10
+    assert np.abs(v - ref1["velos"]).max() == 0.0   # velocities are not integrated
+    assert np.array_equal(o.labels(), ref1["rep2lab"][:4])
 
 
 @pytest.mark.ref
