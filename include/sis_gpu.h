@@ -228,6 +228,25 @@ int sis_gpu_bp_close(sis_gpu_ctx *ctx);
 int sis_gpu_out_open(sis_gpu_ctx *ctx, const char *const *paths, int64_t nstep_save);
 int sis_gpu_out_close(sis_gpu_ctx *ctx);
 
+/* Restart files (.rst) in the block format of write_rst
+ * (src/write_rst.F90:25-145; block ids src/const_idx.F90:72-85), one per
+ * replica: REPLICA (replica 1's file, nrep > 1), STEP, XYZ, VELO, ACCEL,
+ * PBC (replica 1's file, periodic systems) and block 11, the counter-RNG
+ * seed of the performance mode (uint64; readers that do not ask for it
+ * skip it, src/read_rst.F90:48-53).  The parity form's MT streams belong
+ * to the caller, so the device-fed files hold no PRNG / PRNGREP blocks
+ * (read_rst then reports them absent, src/read_rst.F90:640-660).  With
+ * nstep_save_rst > 0, sis_gpu_step rewrites the files after each step whose
+ * count is a multiple of it, after that step's exchange
+ * (src/job_md.F90:664-666); sis_gpu_rst_write writes them now.
+ * sis_gpu_rst_read is read_rst for a restarted run (src/read_rst.F90, called
+ * from src/job_md.F90:149-150,202, src/init_replica.F90:148): it restores
+ * labels, state and step count; the lists are rebuilt by the next step. */
+int sis_gpu_rst_open(sis_gpu_ctx *ctx, const char *const *paths, int64_t nstep_save_rst);
+int sis_gpu_rst_write(sis_gpu_ctx *ctx);
+int sis_gpu_rst_close(sis_gpu_ctx *ctx);
+int sis_gpu_rst_read(sis_gpu_ctx *ctx, const char *const *paths);
+
 /* job_dcd (src/job_dcd.F90:59-80): each frame of the DCD file at path
  * (read as src/dcd.F90 read_onestep does, float32 to double) becomes the
  * coordinates of replica 1; the lists are rebuilt and the energies
@@ -249,6 +268,21 @@ int sis_gpu_check_force(sis_gpu_ctx *ctx, int32_t irep, double h, const char *ou
  * cell, or -1 when the file has none. */
 int sis_dcd_read_file(const char *path, int32_t nmp, int64_t max_frames, double *xyz, double *box,
                       int64_t *nframes);
+
+/* The restart writer on host arrays for replica grep: xyz, velos, accels
+ * (3, nmp); rep2lab [nrep] (replica 1 with nrep > 1); mts_rep / mts the
+ * 643-word mt_stream states of the PRNGREP / PRNG blocks, box[3] (PBC
+ * block) and seed (block 11) may each be NULL to leave the block out.
+ * Needs no GPU. */
+int sis_rst_write_file(const char *path, int32_t grep, int32_t nrep, int32_t nmp, int64_t istep,
+                       const int32_t *rep2lab, const double *xyz, const double *velos, const double *accels,
+                       const uint32_t *mts_rep, const uint32_t *mts, const double *box, const uint64_t *seed);
+
+/* Parse a restart file: *blocks gets bit (id-1) for every block present;
+ * each output may be NULL.  Needs no GPU. */
+int sis_rst_read_file(const char *path, int32_t nmp, int32_t nrep, int32_t *blocks, int32_t *grep,
+                      int64_t *istep, int32_t *rep2lab, double *xyz, double *velos, double *accels,
+                      double *box, uint64_t *seed);
 
 /* The same line writer on host arrays: energies (11, nline). */
 int sis_out_write_file(const char *path, int32_t nline, const int64_t *isteps, const int32_t *labels,

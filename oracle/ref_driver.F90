@@ -543,6 +543,35 @@ contains
       energies_out(:, 1:nrep_proc) = energies(:, 1:nrep_proc)
    end subroutine sisref_job_md
 
+   ! Write every replica's restart file with the reference's own writer
+   ! (src/write_rst.F90): replica irep goes to <prefix><irep, i4.4>.rst.
+   subroutine sisref_write_rst(prefix, plen) bind(C, name="sisref_write_rst")
+      use var_io, only : hdl_rst, cfile_rst
+      use var_replica, only : nrep_proc
+      character(kind=c_char), intent(in) :: prefix(*)
+      integer(c_int), value :: plen
+      character(len=:), allocatable :: pre
+      character(len=4) :: tag
+      integer :: irep, k
+      interface
+         subroutine write_rst()
+         end subroutine write_rst
+      end interface
+      allocate(character(len=plen) :: pre)
+      do k = 1, plen
+         pre(k:k) = prefix(k)
+      enddo
+      if (allocated(hdl_rst)) deallocate(hdl_rst)
+      if (allocated(cfile_rst)) deallocate(cfile_rst)
+      allocate(hdl_rst(nrep_proc), cfile_rst(nrep_proc))
+      do irep = 1, nrep_proc
+         write(tag, '(i4.4)') irep
+         hdl_rst(irep) = 700 + irep
+         cfile_rst(irep) = pre // tag // '.rst'
+      enddo
+      call write_rst()
+   end subroutine sisref_write_rst
+
    subroutine sisref_replica_labels(rep2lab_out) bind(C, name="sisref_replica_labels")
       use var_replica, only : rep2lab, nrep_all
       integer(c_int), intent(out) :: rep2lab_out(*)

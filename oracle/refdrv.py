@@ -144,6 +144,9 @@ def _worker(q, so, sysd, ops, seed, nstep_exchange):
                 t0 = time.perf_counter()
                 lib.sisref_job_md(C.c_int64(int(op[1])), _dp(e))
                 out[op[-1]] = time.perf_counter() - t0
+            elif kind == "write_rst":
+                pre = str(op[1]).encode()
+                lib.sisref_write_rst(C.c_char_p(pre), C.c_int(len(pre)))
             elif kind == "job_md":
                 e = np.zeros((nrep, 11))
                 lib.sisref_job_md(C.c_int64(int(op[1])), _dp(e))

@@ -77,11 +77,13 @@ struct DevBuf {
 struct DcdOut;
 struct BpOut;
 struct OutFile;
+struct RstOut;
 
 struct sis_gpu_ctx {
   DcdOut *dcd = nullptr;   // trajectory output (sis_gpu_dcd_open), or none
   BpOut *bpo = nullptr;    // base-pair output (sis_gpu_bp_open), or none
   OutFile *outf = nullptr; // energy lines (sis_gpu_out_open), or none
+  RstOut *rst = nullptr;   // restart files (sis_gpu_rst_open), or none
   Params P{};
   Ptrs D{};
   sis_options opt{};
@@ -1013,6 +1015,7 @@ int sis_gpu_destroy(sis_gpu_ctx *c) {
     dcd_shutdown(c);
     bp_shutdown(c);
     out_shutdown(c);
+    rst_shutdown(c);
   }
   delete c;
   return SIS_OK;
@@ -1356,17 +1359,20 @@ int sis_gpu_step(sis_gpu_ctx *c, int64_t nsteps, int64_t *rebuilds) {
   };
   // segments end at save steps and exchange steps; at such a step the
   // frames are queued first, then the replicas exchange, the order of the
-  // save and exchange blocks of src/job_md.F90:485-570
+  // save and exchange blocks of src/job_md.F90:485-570; restart files are
+  // written last (src/job_md.F90:664-666)
   const int64_t ex = (c->opt.exchange_every > 0 && c->R > 1) ? c->opt.exchange_every : 0;
   const int64_t sv = (c->dcd && c->dcd->nstep_save > 0) ? c->dcd->nstep_save : 0;
   const int64_t sb = (c->bpo && c->bpo->nstep_save > 0) ? c->bpo->nstep_save : 0;
   const int64_t so = (c->outf && c->outf->nstep_save > 0) ? c->outf->nstep_save : 0;
+  const int64_t rs = (c->rst && c->rst->nstep_save > 0) ? c->rst->nstep_save : 0;
   for (int64_t left = nsteps; left > 0;) {
     int64_t k = left;
     if (ex) k = std::min(k, ex - c->istep % ex);
     if (sv) k = std::min(k, sv - c->istep % sv);
     if (sb) k = std::min(k, sb - c->istep % sb);
     if (so) k = std::min(k, so - c->istep % so);
+    if (rs) k = std::min(k, rs - c->istep % rs);
     run(k);
     left -= k;
     const bool exch = ex && c->istep % ex == 0;
@@ -1390,9 +1396,10 @@ int sis_gpu_step(sis_gpu_ctx *c, int64_t nsteps, int64_t *rebuilds) {
         energy_phase(c, all, c->tK, (c->P.nstep_bp_mc > 0 ? 1 : 0) | 2, e16.data());
       }
       out_lines(c, e16.data(), ekin.data(), lab, temps);
-      continue;
+    } else if (exch) {
+      do_exchange(c, nullptr, nullptr);
     }
-    if (exch) do_exchange(c, nullptr, nullptr);
+    if (rs && c->istep % rs == 0) rst_write_all(c, c->rst->paths);
   }
   HIPCHK(hipEventRecord(c->ev1, c->stream));
   HIPCHK(hipEventSynchronize(c->ev1));

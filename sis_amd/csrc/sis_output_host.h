@@ -777,3 +777,303 @@ extern "C" int sis_gpu_check_force(sis_gpu_ctx *c, int32_t irep, double h, const
   return SIS_OK;
   API_END
 }
+
+// ---------------------------------------------------------------------------
+// Restart files (.rst): the blocks of src/write_rst.F90:25-145 in stream
+// access, each an int32 block id, an int32 byte size and the block's fields
+// (ids src/const_idx.F90:72-85).  Id 11 is this engine's own block: the
+// counter-RNG seed (uint64) of the performance mode; the reference reader
+// skips blocks it does not ask for (src/read_rst.F90:48-53).
+// ---------------------------------------------------------------------------
+
+struct RstOut {
+  std::vector<std::string> paths;
+  int64_t nstep_save = 0;
+};
+
+namespace {
+
+enum : int32_t {
+  RST_STEP = 1, RST_ANNEAL = 2, RST_XYZ = 3, RST_VELO = 4, RST_ACCEL = 5, RST_REPLICA = 6,
+  RST_PRNG = 7, RST_PRNGREP = 8, RST_TWZ = 9, RST_PBC = 10, RST_SEED = 11
+};
+constexpr int32_t MTS_WORDS = 19 + 624;   // mt_stream's saved state (src/const.F90:40)
+
+struct RstRec {
+  int32_t grep = 1, nrep = 1, nmp = 0;
+  int64_t istep = 0;
+  const int32_t *rep2lab = nullptr;       // [nrep], written to replica 1's file when nrep > 1
+  const double *xyz = nullptr, *vel = nullptr, *acc = nullptr;   // (3, nmp)
+  const uint32_t *mts_rep = nullptr;      // [MTS_WORDS] or none
+  const uint32_t *mts = nullptr;          // [MTS_WORDS] or none
+  const double *box = nullptr;            // [3] or none (no PBC block)
+  const uint64_t *seed = nullptr;         // or none
+};
+
+struct ByteSink {
+  std::vector<char> b;
+  template <typename T> void put(T v) {
+    const char *p = reinterpret_cast<const char *>(&v);
+    b.insert(b.end(), p, p + sizeof v);
+  }
+  void raw(const void *p, size_t n) { b.insert(b.end(), (const char *)p, (const char *)p + n); }
+};
+
+std::vector<char> rst_bytes(const RstRec &r) {
+  ByteSink s;
+  const int32_t n3 = 3 * r.nmp;
+  if (r.grep == 1 && r.nrep > 1) {
+    s.put<int32_t>(RST_REPLICA);
+    s.put<int32_t>(4 * (1 + 2 * r.nrep));
+    s.put<int32_t>(r.nrep);
+    for (int i = 1; i <= r.nrep; i++) { s.put<int32_t>(i); s.put<int32_t>(r.rep2lab[i - 1]); }
+  }
+  s.put<int32_t>(RST_STEP);
+  s.put<int32_t>(4 + 8);
+  s.put<int32_t>(1);          // istep_sim: reserved, always 1
+  s.put<int64_t>(r.istep);
+  const double *arr[3] = {r.xyz, r.vel, r.acc};
+  const int32_t ids[3] = {RST_XYZ, RST_VELO, RST_ACCEL};
+  for (int k = 0; k < 3; k++) {
+    s.put<int32_t>(ids[k]);
+    s.put<int32_t>(4 * 2 + 8 * n3);
+    s.put<int32_t>(r.grep);
+    s.put<int32_t>(r.nmp);
+    s.raw(arr[k], sizeof(double) * n3);
+  }
+  if (r.grep == 1 && r.nrep > 1 && r.mts_rep) {
+    s.put<int32_t>(RST_PRNGREP);
+    s.put<int32_t>(4 * MTS_WORDS);
+    s.raw(r.mts_rep, 4 * MTS_WORDS);
+  }
+  if (r.mts) {
+    s.put<int32_t>(RST_PRNG);
+    s.put<int32_t>(4 + 4 * MTS_WORDS);
+    s.put<int32_t>(r.grep);
+    s.raw(r.mts, 4 * MTS_WORDS);
+  }
+  if (r.grep == 1 && r.box) {
+    s.put<int32_t>(RST_PBC);
+    s.put<int32_t>(8 * 3);
+    s.raw(r.box, 8 * 3);
+  }
+  if (r.seed) {
+    s.put<int32_t>(RST_SEED);
+    s.put<int32_t>(8);
+    s.put<uint64_t>(*r.seed);
+  }
+  return s.b;
+}
+
+void write_whole(const std::string &path, const std::vector<char> &b) {
+  FILE *f = fopen(path.c_str(), "wb");
+  if (!f) throw ApiError{SIS_ERR_ARG, "cannot open " + path};
+  const bool ok = fwrite(b.data(), 1, b.size(), f) == b.size();
+  if (fclose(f) != 0 || !ok) throw ApiError{SIS_ERR_ARG, "write failed: " + path};
+}
+
+// Parsed blocks of one restart file.
+struct RstIn {
+  int32_t found = 0;      // bit (id - 1) set for every block seen
+  int32_t grep = 0, nrep = 0;
+  int64_t istep = 0;
+  uint64_t seed = 0;
+  std::vector<int32_t> rep2lab;
+  std::vector<double> xyz, vel, acc;
+  double box[3] = {0, 0, 0};
+};
+
+RstIn rst_parse(const std::string &path, int32_t nmp) {
+  FILE *f = fopen(path.c_str(), "rb");
+  if (!f) throw ApiError{SIS_ERR_ARG, "cannot open " + path};
+  std::vector<char> b;
+  char tmp[65536];
+  size_t got;
+  while ((got = fread(tmp, 1, sizeof tmp, f)) > 0) b.insert(b.end(), tmp, tmp + got);
+  fclose(f);
+  RstIn o;
+  size_t p = 0;
+  auto need = [&](size_t n) {
+    if (p + n > b.size()) throw ApiError{SIS_ERR_ARG, "truncated restart file: " + path};
+  };
+  auto get32 = [&]() { need(4); int32_t v; std::memcpy(&v, b.data() + p, 4); p += 4; return v; };
+  while (p < b.size()) {
+    const int32_t id = get32(), size = get32();
+    if (size < 0) throw ApiError{SIS_ERR_ARG, "bad block size in " + path};
+    need((size_t)size);
+    const size_t end = p + (size_t)size;
+    if (id == RST_STEP && size == 12) {
+      std::memcpy(&o.istep, b.data() + p + 4, 8);
+    } else if ((id == RST_XYZ || id == RST_VELO || id == RST_ACCEL) && size >= 8) {
+      int32_t g, n;
+      std::memcpy(&g, b.data() + p, 4);
+      std::memcpy(&n, b.data() + p + 4, 4);
+      if (n != nmp || (size_t)size != 8 + (size_t)24 * n)
+        throw ApiError{SIS_ERR_ARG, "nmp is not consistent in the restart file " + path};
+      o.grep = g;
+      std::vector<double> &dst = id == RST_XYZ ? o.xyz : id == RST_VELO ? o.vel : o.acc;
+      dst.resize((size_t)3 * n);
+      std::memcpy(dst.data(), b.data() + p + 8, (size_t)24 * n);
+    } else if (id == RST_REPLICA && size >= 4) {
+      std::memcpy(&o.nrep, b.data() + p, 4);
+      if (o.nrep < 1 || (size_t)size != 4 + (size_t)8 * o.nrep)
+        throw ApiError{SIS_ERR_ARG, "bad REPLICA block in " + path};
+      o.rep2lab.assign(o.nrep, 0);
+      for (int k = 0; k < o.nrep; k++) {
+        int32_t i, m;
+        std::memcpy(&i, b.data() + p + 4 + 8 * k, 4);
+        std::memcpy(&m, b.data() + p + 8 + 8 * k, 4);
+        if (i < 1 || i > o.nrep) throw ApiError{SIS_ERR_ARG, "bad REPLICA block in " + path};
+        o.rep2lab[i - 1] = m;
+      }
+    } else if (id == RST_PBC && size == 24) {
+      std::memcpy(o.box, b.data() + p, 24);
+    } else if (id == RST_SEED && size == 8) {
+      std::memcpy(&o.seed, b.data() + p, 8);
+    }
+    if (id >= 1 && id <= 31) o.found |= 1 << (id - 1);
+    p = end;
+  }
+  return o;
+}
+
+// one file per replica from the device state (replica r's file holds grep r+1)
+void rst_write_all(sis_gpu_ctx *c, const std::vector<std::string> &paths) {
+  HIPCHK(hipStreamSynchronize(c->stream));
+  const size_t RN = (size_t)c->R * c->N;
+  std::vector<double> x(3 * RN), v(3 * RN), a(3 * RN);
+  std::vector<double4> h(RN);
+  auto down = [&](std::vector<double> &dst, double4 *src) {
+    HIPCHK(hipMemcpy(h.data(), src, RN * sizeof(double4), hipMemcpyDeviceToHost));
+    for (size_t g = 0; g < RN; g++) { dst[3 * g] = h[g].x; dst[3 * g + 1] = h[g].y; dst[3 * g + 2] = h[g].z; }
+  };
+  down(x, c->pos_d.p);
+  down(v, c->vel_d.p);
+  down(a, c->acc_d.p);
+  std::vector<int32_t> lab(c->R);
+  for (int r = 0; r < c->R; r++) lab[r] = c->rep2lab[r + 1];
+  const uint64_t seed = c->opt.seed;
+  for (int r = 0; r < c->R; r++) {
+    RstRec q;
+    q.grep = r + 1; q.nrep = c->R; q.nmp = c->N; q.istep = c->istep; q.rep2lab = lab.data();
+    q.xyz = x.data() + (size_t)3 * c->N * r;
+    q.vel = v.data() + (size_t)3 * c->N * r;
+    q.acc = a.data() + (size_t)3 * c->N * r;
+    q.box = c->prm.pbc ? c->prm.box : nullptr;
+    q.seed = c->parity ? nullptr : &seed;   // the parity form's MT state belongs to the caller
+    write_whole(paths[r], rst_bytes(q));
+  }
+}
+
+void rst_shutdown(sis_gpu_ctx *c) {
+  delete c->rst;
+  c->rst = nullptr;
+}
+
+}  // namespace
+
+int sis_rst_write_file(const char *path, int32_t grep, int32_t nrep, int32_t nmp, int64_t istep,
+                       const int32_t *rep2lab, const double *xyz, const double *velos, const double *accels,
+                       const uint32_t *mts_rep, const uint32_t *mts, const double *box, const uint64_t *seed) {
+  API_BEGIN
+  if (!path || !xyz || !velos || !accels || nmp < 1 || nrep < 1 || grep < 1 || grep > nrep ||
+      (grep == 1 && nrep > 1 && !rep2lab))
+    return fail(SIS_ERR_ARG, "invalid argument");
+  RstRec q;
+  q.grep = grep; q.nrep = nrep; q.nmp = nmp; q.istep = istep; q.rep2lab = rep2lab;
+  q.xyz = xyz; q.vel = velos; q.acc = accels; q.mts_rep = mts_rep; q.mts = mts; q.box = box; q.seed = seed;
+  write_whole(path, rst_bytes(q));
+  return SIS_OK;
+  API_END
+}
+
+int sis_rst_read_file(const char *path, int32_t nmp, int32_t nrep, int32_t *blocks, int32_t *grep,
+                      int64_t *istep, int32_t *rep2lab, double *xyz, double *velos, double *accels,
+                      double *box, uint64_t *seed) {
+  API_BEGIN
+  if (!path || nmp < 1 || nrep < 1) return fail(SIS_ERR_ARG, "invalid argument");
+  RstIn o = rst_parse(path, nmp);
+  if ((o.found & (1 << (RST_REPLICA - 1))) && o.nrep != nrep)
+    return fail(SIS_ERR_ARG, "nrep is not consistent in the restart file");
+  if (blocks) *blocks = o.found;
+  if (grep) *grep = o.grep;
+  if (istep) *istep = o.istep;
+  if (rep2lab && !o.rep2lab.empty()) std::copy(o.rep2lab.begin(), o.rep2lab.end(), rep2lab);
+  if (xyz && !o.xyz.empty()) std::copy(o.xyz.begin(), o.xyz.end(), xyz);
+  if (velos && !o.vel.empty()) std::copy(o.vel.begin(), o.vel.end(), velos);
+  if (accels && !o.acc.empty()) std::copy(o.acc.begin(), o.acc.end(), accels);
+  if (box) std::copy(o.box, o.box + 3, box);
+  if (seed) *seed = o.seed;
+  return SIS_OK;
+  API_END
+}
+
+int sis_gpu_rst_open(sis_gpu_ctx *c, const char *const *paths, int64_t nstep_save_rst) {
+  API_BEGIN
+  if (!c || !paths || nstep_save_rst < 0) return fail(SIS_ERR_ARG, "invalid argument");
+  if (c->rst) return fail(SIS_ERR_STATE, "restart output already open");
+  RstOut *o = new RstOut();
+  o->nstep_save = nstep_save_rst;
+  for (int r = 0; r < c->R; r++) {
+    if (!paths[r]) { delete o; return fail(SIS_ERR_ARG, "null path"); }
+    o->paths.emplace_back(paths[r]);
+  }
+  c->rst = o;
+  return SIS_OK;
+  API_END
+}
+
+int sis_gpu_rst_write(sis_gpu_ctx *c) {
+  API_BEGIN
+  if (!c) return fail(SIS_ERR_ARG, "null context");
+  if (!c->rst) return fail(SIS_ERR_STATE, "restart output not open");
+  rst_write_all(c, c->rst->paths);
+  return SIS_OK;
+  API_END
+}
+
+int sis_gpu_rst_close(sis_gpu_ctx *c) {
+  API_BEGIN
+  if (!c) return fail(SIS_ERR_ARG, "null context");
+  rst_shutdown(c);
+  return SIS_OK;
+  API_END
+}
+
+int sis_gpu_rst_read(sis_gpu_ctx *c, const char *const *paths) {
+  API_BEGIN
+  if (!c || !paths) return fail(SIS_ERR_ARG, "null argument");
+  const int R = c->R, N = c->N;
+  const size_t n3 = (size_t)3 * N;
+  std::vector<double> x(n3 * R), v(n3 * R), a(n3 * R);
+  std::vector<int> have(R, 0);
+  int64_t istep = -1;
+  std::vector<int32_t> lab;
+  for (int r = 0; r < R; r++) {
+    if (!paths[r]) return fail(SIS_ERR_ARG, "null path");
+    RstIn o = rst_parse(paths[r], N);
+    const int need = (1 << (RST_XYZ - 1)) | (1 << (RST_VELO - 1)) | (1 << (RST_ACCEL - 1));
+    if ((o.found & need) != need) return fail(SIS_ERR_ARG, std::string("XYZ, VELO or ACCEL block missing in ") + paths[r]);
+    if (o.grep < 1 || o.grep > R || have[o.grep - 1]) return fail(SIS_ERR_ARG, "replica ids in the restart files are not 1..nrep");
+    have[o.grep - 1] = 1;
+    std::copy(o.xyz.begin(), o.xyz.end(), x.begin() + n3 * (o.grep - 1));
+    std::copy(o.vel.begin(), o.vel.end(), v.begin() + n3 * (o.grep - 1));
+    std::copy(o.acc.begin(), o.acc.end(), a.begin() + n3 * (o.grep - 1));
+    if (o.found & (1 << (RST_STEP - 1))) istep = o.istep;
+    if (!o.rep2lab.empty()) {
+      if (o.nrep != R) return fail(SIS_ERR_ARG, "nrep is not consistent in the restart file");
+      lab = o.rep2lab;
+    }
+    if ((o.found & (1 << (RST_PBC - 1))) && c->prm.pbc &&
+        (o.box[0] != c->prm.box[0] || o.box[1] != c->prm.box[1] || o.box[2] != c->prm.box[2]))
+      return fail(SIS_ERR_UNSUPPORTED, "the restart file's box differs from the context's (box changes are not supported)");
+  }
+  if (istep < 0) return fail(SIS_ERR_ARG, "STEP block missing");
+  if (R > 1 && lab.empty()) return fail(SIS_ERR_ARG, "REPLICA block missing");
+  int rc;
+  if (R > 1 && (rc = sis_gpu_set_labels(c, lab.data())) != SIS_OK) return rc;
+  if ((rc = sis_gpu_set_state(c, x.data(), v.data(), a.data())) != SIS_OK) return rc;
+  if ((rc = sis_gpu_set_step(c, istep)) != SIS_OK) return rc;
+  return SIS_OK;
+  API_END
+}

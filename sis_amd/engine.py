@@ -211,6 +211,22 @@ class Engine:
     def out_close(self):
         _check(lib().sis_gpu_out_close(self.h))
 
+    def rst_open(self, paths, nstep_save_rst: int = 0):
+        """Restart files (src/write_rst.F90 blocks), one per replica."""
+        arr = (C.c_char_p * self.nrep)(*[str(x).encode() for x in paths])
+        _check(lib().sis_gpu_rst_open(self.h, arr, int(nstep_save_rst)))
+
+    def rst_write(self):
+        _check(lib().sis_gpu_rst_write(self.h))
+
+    def rst_close(self):
+        _check(lib().sis_gpu_rst_close(self.h))
+
+    def rst_read(self, paths):
+        """read_rst for a restarted run: labels, state and step count."""
+        arr = (C.c_char_p * self.nrep)(*[str(x).encode() for x in paths])
+        _check(lib().sis_gpu_rst_read(self.h, arr))
+
     def dcd_energies(self, path, out_path=None, max_frames: int = 100000):
         """job_dcd: energies (nframes, 11) of the frames of a DCD file."""
         e = np.zeros((max_frames, 11))
@@ -287,6 +303,42 @@ def write_out_file(path, isteps, labels, temps, ekin, energies, replica: bool, e
     _check(lib().sis_out_write_file(str(path).encode(), len(ist), ist.ctypes.data_as(C.POINTER(C.c_int64)),
                                     abi.iptr(lab), abi.dptr(tk), abi.dptr(ek), abi.dptr(en), int(bool(replica)),
                                     int(bool(ele))))
+
+
+def write_rst_file(path, grep, nrep, istep, xyz, velos, accels, rep2lab=None, mts_rep=None, mts=None,
+                   box=None, seed=None):
+    """Host-side restart writer of the library for one replica; xyz,
+    velos, accels (nmp, 3); mts_rep / mts 643 uint32 words or None."""
+    x = np.ascontiguousarray(xyz, np.float64)
+    v = np.ascontiguousarray(velos, np.float64)
+    a = np.ascontiguousarray(accels, np.float64)
+    lab = np.ascontiguousarray(rep2lab if rep2lab is not None else np.zeros(1), np.int32)
+    u32 = C.POINTER(C.c_uint32)
+    mr = np.ascontiguousarray(mts_rep, np.uint32) if mts_rep is not None else None
+    ms = np.ascontiguousarray(mts, np.uint32) if mts is not None else None
+    bx = np.ascontiguousarray(box, np.float64) if box is not None else None
+    sd = C.c_uint64(int(seed)) if seed is not None else None
+    _check(lib().sis_rst_write_file(
+        str(path).encode(), int(grep), int(nrep), x.shape[0], int(istep), abi.iptr(lab), abi.dptr(x), abi.dptr(v),
+        abi.dptr(a), mr.ctypes.data_as(u32) if mr is not None else None,
+        ms.ctypes.data_as(u32) if ms is not None else None, abi.dptr(bx) if bx is not None else None,
+        C.byref(sd) if sd is not None else None))
+
+
+def read_rst_file(path, nmp: int, nrep: int):
+    """Host-side restart reader of the library: dict of the blocks found."""
+    blocks, grep = C.c_int32(), C.c_int32()
+    istep = C.c_int64()
+    lab = np.zeros(nrep, np.int32)
+    x, v, a = np.zeros((nmp, 3)), np.zeros((nmp, 3)), np.zeros((nmp, 3))
+    box = np.zeros(3)
+    seed = C.c_uint64()
+    _check(lib().sis_rst_read_file(str(path).encode(), int(nmp), int(nrep), C.byref(blocks), C.byref(grep),
+                                   C.byref(istep), abi.iptr(lab), abi.dptr(x), abi.dptr(v), abi.dptr(a),
+                                   abi.dptr(box), C.byref(seed)))
+    ids = [k + 1 for k in range(31) if blocks.value >> k & 1]
+    return dict(blocks=ids, grep=grep.value, istep=istep.value, rep2lab=lab, xyz=x, velos=v, accels=a, box=box,
+                seed=seed.value)
 
 
 def read_dcd_file(path, nmp: int, max_frames: int = 100000):

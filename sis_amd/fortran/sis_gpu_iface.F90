@@ -223,6 +223,28 @@ module sis_gpu_iface
          type(c_ptr), value :: ctx
       end function
 
+      ! restart files (.rst, src/write_rst.F90 blocks) and read_rst
+      integer(c_int) function sis_gpu_rst_open(ctx, paths, nstep_save_rst) bind(C, name="sis_gpu_rst_open")
+         import :: c_int, c_ptr, c_int64_t
+         type(c_ptr), value :: ctx, paths
+         integer(c_int64_t), value :: nstep_save_rst
+      end function
+
+      integer(c_int) function sis_gpu_rst_write(ctx) bind(C, name="sis_gpu_rst_write")
+         import :: c_int, c_ptr
+         type(c_ptr), value :: ctx
+      end function
+
+      integer(c_int) function sis_gpu_rst_close(ctx) bind(C, name="sis_gpu_rst_close")
+         import :: c_int, c_ptr
+         type(c_ptr), value :: ctx
+      end function
+
+      integer(c_int) function sis_gpu_rst_read(ctx, paths) bind(C, name="sis_gpu_rst_read")
+         import :: c_int, c_ptr
+         type(c_ptr), value :: ctx, paths
+      end function
+
       ! job_check_force: forces against central differences of the energy
       integer(c_int) function sis_gpu_check_force(ctx, irep, h, out_path, f_energy, f_force) &
             bind(C, name="sis_gpu_check_force")

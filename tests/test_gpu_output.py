@@ -79,6 +79,57 @@ def test_restart_continues_the_run(mode):
     assert np.abs(va - vc).max() < 1e-5
 
 
+@pytest.mark.parametrize("mode", ["batched", "phases"])
+def test_restart_files_continue_the_run(tmp_path, mode):
+    """Restart files written by sis_gpu_step every 40 steps (after that
+    step's exchange, src/job_md.F90:664-666) hold the state, labels and step
+    of the run at step 40 exactly; a new context that reads them
+    (sis_gpu_rst_read) starts from that state and, after 60 more steps,
+    follows the uninterrupted run (same exchanges; trajectories equal to
+    rounding, since the lists are rebuilt at the restart).  The restart
+    falls after an even number of exchanges: the exchange pair table is not
+    in the file and restarts from its first half (src/var_replica.F90:85),
+    as the uninterrupted run's does after an even count."""
+    from sis_amd.engine import read_rst_file
+    s = configs.remd_c3(nrep=8, n=128, seed=3)
+    R, N = int(s["nrep"]), int(np.sum(s["nmp_chain"]))
+    print(f"[case] restart files {mode}", flush=True)
+    kw = dict(mode=mode, seed=77, exchange_every=20, precision=0)
+    a = Engine(s, **kw)
+    a.init_velocities()
+    a.neighbor_list(0)
+    a.step(100)
+    xa, va, _ = a.get_state()
+    b = Engine(s, **kw)
+    b.init_velocities()
+    b.neighbor_list(0)
+    paths = [tmp_path / f"run_{r + 1:04d}.rst" for r in range(R)]
+    b.rst_open(paths, nstep_save_rst=40)
+    b.step(25)
+    b.step(15)
+    xb, vb, ab = b.get_state()
+    lab = b.labels()
+    b.step(30)    # no multiple of 40 in 41..70: the files stay those of step 40
+    b.rst_close()
+    for r in range(R):
+        f = read_rst_file(paths[r], N, R)
+        assert f["istep"] == 40 and f["grep"] == r + 1 and f["seed"] == 77
+        assert np.array_equal(f["xyz"], xb[r]) and np.array_equal(f["velos"], vb[r])
+        assert np.array_equal(f["accels"], ab[r])
+        assert f["blocks"] == ([1, 3, 4, 5, 6, 11] if r == 0 else [1, 3, 4, 5, 11])
+    assert np.array_equal(read_rst_file(paths[0], N, R)["rep2lab"], lab)
+    c = Engine(s, **kw)
+    c.rst_read(paths)
+    assert c.info()["istep"] == 40 and np.array_equal(c.labels(), lab)
+    xc0, vc0, ac0 = c.get_state()
+    assert np.array_equal(xc0, xb) and np.array_equal(vc0, vb) and np.array_equal(ac0, ab)
+    c.step(60)
+    xc, vc, _ = c.get_state()
+    assert np.array_equal(a.labels(), c.labels())
+    assert np.abs(xa - xc).max() < 1e-6
+    assert np.abs(va - vc).max() < 1e-5
+
+
 @pytest.mark.parametrize("case", ["batched", "pbc_phases"])
 def test_bp_records_during_steps(tmp_path, case):
     """Base-pair records (.bpall: every formed pair; the E <= -kT filter of

@@ -136,3 +136,79 @@ def test_dcd_reader_round_trip_of_reference_file(tmp_path, box):
     assert got.shape == x.shape
     assert np.array_equal(got, x.astype(np.float32).astype(np.float64))
     assert np.array_equal(b, [-1.0, -1.0, -1.0] if box is None else box)
+
+
+def _rst_blocks(raw):
+    """(id, size, offset of the fields) of each block, walking by size as
+    read_rst's skip does (src/read_rst.F90:48-53)."""
+    out, p = [], 0
+    while p < len(raw):
+        i, n = np.frombuffer(raw, np.int32, 2, p)
+        out.append((int(i), int(n), p + 8))
+        p += 8 + int(n)
+    assert p == len(raw)
+    return out
+
+
+@needs_ref
+@pytest.mark.parametrize("case", ["remd3", "pbc"])
+def test_rst_bytes_match_reference_writer(tmp_path, case):
+    """Restart files of write_rst (src/write_rst.F90:25-145) after 30 steps
+    of the reference's job_md: the library's writer, given the same state,
+    labels, step and the reference's MT states (opaque 643-word blocks),
+    writes the same bytes; the library's reader returns that state."""
+    from sis_amd import configs
+    from sis_amd.engine import read_rst_file, write_rst_file
+    s = configs.remd_c3(nrep=3, n=20, seed=2) if case == "remd3" else configs.small_pbc(3, 10, 80.0, seed=3)
+    pre = str(tmp_path / "ref")
+    r = refdrv.run(s, [("job_md", 30, "R"), ("write_rst", pre)], seed=5, nstep_exchange=10)["R"]
+    nrep, nmp = int(s["nrep"]), int(np.sum(s["nmp_chain"]))
+    for g in range(1, nrep + 1):
+        raw = (tmp_path / f"ref{g:04d}.rst").read_bytes()
+        blocks = {i: (n, o) for i, n, o in _rst_blocks(raw)}
+        mts_rep = np.frombuffer(raw, np.uint32, 643, blocks[8][1]) if 8 in blocks else None
+        mts = np.frombuffer(raw, np.uint32, 643, blocks[7][1] + 4)
+        box = np.array(s["params"]["box"]) if s["params"]["pbc"] else None
+        ours = tmp_path / f"ours{g}.rst"
+        write_rst_file(ours, g, nrep, 30, r["xyz"][g - 1], r["velos"][g - 1], r["accels"][g - 1],
+                       rep2lab=r["rep2lab"][:nrep], mts_rep=mts_rep, mts=mts, box=box)
+        assert ours.read_bytes() == raw
+        back = read_rst_file(tmp_path / f"ref{g:04d}.rst", nmp, nrep)
+        assert back["istep"] == 30 and back["grep"] == g
+        assert np.array_equal(back["xyz"], r["xyz"][g - 1])
+        assert np.array_equal(back["velos"], r["velos"][g - 1])
+        assert np.array_equal(back["accels"], r["accels"][g - 1])
+        if g == 1 and nrep > 1:
+            assert np.array_equal(back["rep2lab"], r["rep2lab"][:nrep])
+        if box is not None and g == 1:
+            assert np.array_equal(back["box"], box)
+
+
+def test_rst_layout_known_answer(tmp_path):
+    """Block ids, sizes and fields of a restart file decoded by hand (the
+    layout of src/write_rst.F90:40-145): REPLICA, STEP, XYZ, VELO, ACCEL,
+    PBC and the counter-RNG seed block 11; the reader returns them."""
+    from sis_amd.engine import read_rst_file, write_rst_file
+    x = np.arange(6, dtype=float).reshape(2, 3) + 0.5
+    v, a = -x, 2 * x
+    p = tmp_path / "k.rst"
+    write_rst_file(p, 1, 2, 123456789012, x, v, a, rep2lab=[2, 1], box=[10.0, 20.0, 30.0], seed=2**63 + 5)
+    raw = p.read_bytes()
+    bl = _rst_blocks(raw)
+    assert [(i, n) for i, n, _ in bl] == [(6, 20), (1, 12), (3, 56), (4, 56), (5, 56), (10, 24), (11, 8)]
+    i32 = lambda o, k=1: np.frombuffer(raw, np.int32, k, o)  # noqa: E731
+    assert list(i32(bl[0][2], 5)) == [2, 1, 2, 2, 1]                       # nrep, (i, rep2lab(i))
+    assert i32(bl[1][2])[0] == 1 and np.frombuffer(raw, np.int64, 1, bl[1][2] + 4)[0] == 123456789012
+    for (i, n, o), arr in zip(bl[2:5], (x, v, a)):
+        assert list(i32(o, 2)) == [1, 2]                                   # grep, nmp
+        assert np.array_equal(np.frombuffer(raw, np.float64, 6, o + 8), arr.ravel())
+    assert list(np.frombuffer(raw, np.float64, 3, bl[5][2])) == [10.0, 20.0, 30.0]
+    assert np.frombuffer(raw, np.uint64, 1, bl[6][2])[0] == 2**63 + 5
+    back = read_rst_file(p, 2, 2)
+    assert back["blocks"] == [1, 3, 4, 5, 6, 10, 11] and back["istep"] == 123456789012
+    assert list(back["rep2lab"]) == [2, 1] and back["seed"] == 2**63 + 5
+    assert np.array_equal(back["velos"], v)
+    # replica 2's file carries no REPLICA and no PBC block
+    q = tmp_path / "k2.rst"
+    write_rst_file(q, 2, 2, 7, x, v, a, box=[10.0, 20.0, 30.0])
+    assert [i for i, _, _ in _rst_blocks(q.read_bytes())] == [1, 3, 4, 5]
