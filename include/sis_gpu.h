@@ -170,6 +170,41 @@ int sis_gpu_exchange(sis_gpu_ctx *ctx, double *replica_energies, int32_t *accept
 /* rep2lab(1:nrep) (src/var_replica.F90:49). */
 int sis_gpu_get_labels(sis_gpu_ctx *ctx, int32_t *rep2lab);
 
+/* External terms beside the bonded and pair potentials
+ * (src/force.F90:62-75, src/energy_sumup.F90:46-53): the wall at z = 0
+ * (force_stage / energy_stage, src/force_stage.F90, src/energy_stage.F90),
+ * optical tweezers with constant-force pairs and force ramps
+ * (src/force_tweezers.F90, src/energy_tweezers.F90; the ramp anchors move
+ * as init + velo * istep), the Rg bias (src/force_rg.F90,
+ * src/energy_rg.F90) and sigmoid restraints (src/force_restraint.F90,
+ * src/energy_restraint.F90).  Arrays are in the reference's layout with
+ * 1-based bead ids; energies go to ENE STAGE, TWZ, RG and REST (7-10). */
+typedef struct sis_external {
+    int32_t stage;                /* flg_stage */
+    int32_t n_dcf;                /* ntwz_DCF */
+    int32_t n_fr;                 /* ntwz_FR */
+    int32_t rg;                   /* 0 off, else bias_rg_pott (POTT: 1 harmonic, 2 flat-bottomed) */
+    int32_t n_rest;               /* nrest_sigb */
+    int32_t pad;
+    double stage_sigma, stage_eps;
+    double rg_k;                  /* bias_rg_k */
+    const double *rg_0;           /* bias_rg_0 [nrep] */
+    const int32_t *dcf_pairs;     /* twz_DCF_pairs (2, n_dcf) */
+    const double *dcf_forces;     /* twz_DCF_forces (3, n_dcf, nrep) */
+    const int32_t *fr_pairs;      /* twz_FR_pairs (2, n_fr) */
+    const double *fr_k;           /* twz_FR_k (2, n_fr) */
+    const double *fr_init;        /* twz_FR_init (3, 2, n_fr) */
+    const double *fr_velo;        /* twz_FR_velo (3, 2, n_fr) */
+    const int32_t *rest_id;       /* rest_sigb_id (3, n_rest): bead, reference bead, 1 sigmoid / 2 sigmoid-to-bead */
+    const double *rest_rcut;      /* rest_sigb_rcut (n_rest) */
+    const double *rest_para;      /* rest_sigb_para (3, n_rest): epsilon, r_bound, delta */
+} sis_external;
+
+/* Set (or with NULL / all terms off, clear) the external terms.  They run
+ * in the per-phase form; a context with external terms steps in that form
+ * whatever mode it was created with. */
+int sis_gpu_set_external(sis_gpu_ctx *ctx, const sis_external *ext);
+
 /* Restart: the STEP and REPLICA blocks of a restart file
  * (src/write_rst.F90:40-60, read back by src/read_rst.F90).  set_step
  * sets istep, which keys the performance-mode random streams, so a run

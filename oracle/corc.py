@@ -12,6 +12,7 @@ import numpy as np
 HERE = Path(__file__).resolve().parent
 SO = HERE / "liboracle.so"
 sys.path.insert(0, str(HERE.parent))
+from sis_amd import abi  # noqa: E402
 from sis_amd.abi import SisParams, SisTopo, params_to_struct  # noqa: E402
 
 _lib = None
@@ -43,6 +44,8 @@ def lib():
         L.orc_bp_pair.argtypes = [vp, i32, i32, P(i32), P(i32)]
         L.orc_ele_params.argtypes = [vp, i32, P(d)]
         L.orc_md_coef.argtypes = [vp, i32, P(d)]
+        L.orc_set_external.argtypes = [vp, P(abi.SisExternal)]
+        L.orc_set_step.argtypes = [vp, i64]
         L.orc_set_tape.argtypes = [vp, i32, P(C.c_uint32), i64]
         L.orc_tape_pos.restype = i64
         L.orc_tape_pos.argtypes = [vp, i32]
@@ -177,6 +180,18 @@ class Oracle:
         e = np.zeros(4)
         lib().orc_ele_params(self.h, irep, _d(e))
         return e
+
+    def set_external(self, ext):
+        """External terms (abi.external_to_struct dict), None clears."""
+        if ext is None:
+            lib().orc_set_external(self.h, None)
+            return
+        s, keep = abi.external_to_struct(ext, self.nrep)
+        lib().orc_set_external(self.h, C.byref(s))
+        del keep
+
+    def set_step(self, istep):
+        lib().orc_set_step(self.h, int(istep))
 
     def md_coef(self, irep=1):
         c = np.zeros((self.nmp, 4))

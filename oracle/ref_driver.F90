@@ -427,7 +427,8 @@ contains
 
    ! energies(0:10) as ENE indices (src/const_idx.F90:21)
    subroutine sisref_energy(irep, tempk_in, bp_mc, bp_energy_valid, energies) bind(C, name="sisref_energy")
-      use var_state, only : flg_bp_MC, flg_bp_energy
+      use var_state, only : flg_bp_MC, flg_bp_energy, rg
+      use var_replica, only : nrep_proc
       integer(c_int), value :: irep, bp_mc, bp_energy_valid
       real(c_double), value :: tempk_in
       real(c_double), intent(out) :: energies(0:10)
@@ -440,10 +441,75 @@ contains
             real(PREC), intent(out) :: energies(0:ENE%MAX)
          end subroutine energy_sumup
       end interface
+      logical :: own_rg
       flg_bp_MC = (bp_mc /= 0)
       flg_bp_energy = (bp_energy_valid /= 0)
+      ! energy_rg stores Rg in var_state's rg, which job_md allocates
+      own_rg = .not. allocated(rg)
+      if (own_rg) allocate(rg(nrep_proc))
       call energy_sumup(irep, tempk_in, energies)
+      if (own_rg) deallocate(rg)
    end subroutine sisref_energy
+
+   ! The external terms, set as read_input / init_tweezers / init_restraint
+   ! leave them (src/var_potential.F90:115-163); istep keys the force-ramp
+   ! anchors of force_tweezers.
+   subroutine sisref_set_external(stage, sigma, eps, ndcf, dcf_pairs, dcf_forces, nfr, fr_pairs, fr_k, &
+                                  fr_init, fr_velo, rgpot, rgk, rg0, nrest, rest_id, rest_rcut, rest_para, &
+                                  istep_in) bind(C, name="sisref_set_external")
+      use var_potential, only : flg_stage, stage_sigma, stage_eps, flg_twz, ntwz_DCF, twz_DCF_pairs, &
+                                twz_DCF_forces, ntwz_FR, twz_FR_pairs, twz_FR_k, twz_FR_init, twz_FR_velo, &
+                                flg_bias_rg, bias_rg_pott, bias_rg_k, bias_rg_0, flg_restraint, flg_rest_sigb, &
+                                nrest_sigb, rest_sigb_id, rest_sigb_rcut, rest_sigb_para
+      use var_io, only : flg_out_twz
+      use var_state, only : istep
+      use var_replica, only : nrep_proc
+      integer(c_int), value :: stage, ndcf, nfr, rgpot, nrest
+      real(c_double), value :: sigma, eps, rgk
+      integer(c_int64_t), value :: istep_in
+      integer(c_int), intent(in) :: dcf_pairs(2, *), fr_pairs(2, *), rest_id(3, *)
+      real(c_double), intent(in) :: dcf_forces(3, max(ndcf, 1), *), fr_k(2, *), fr_init(3, 2, *), &
+                                    fr_velo(3, 2, *), rg0(*), rest_rcut(*), rest_para(3, *)
+      flg_stage = (stage /= 0)
+      stage_sigma = sigma
+      stage_eps = eps
+      flg_out_twz = .False.
+      ntwz_DCF = ndcf
+      ntwz_FR = nfr
+      flg_twz = (ndcf > 0 .or. nfr > 0)
+      if (allocated(twz_DCF_pairs)) deallocate(twz_DCF_pairs, twz_DCF_forces)
+      allocate(twz_DCF_pairs(2, max(ndcf, 1)), twz_DCF_forces(3, max(ndcf, 1), nrep_proc))
+      if (ndcf > 0) then
+         twz_DCF_pairs(:, 1:ndcf) = dcf_pairs(:, 1:ndcf)
+         twz_DCF_forces(:, 1:ndcf, 1:nrep_proc) = dcf_forces(:, 1:ndcf, 1:nrep_proc)
+      endif
+      if (allocated(twz_FR_pairs)) deallocate(twz_FR_pairs, twz_FR_k, twz_FR_init, twz_FR_velo)
+      allocate(twz_FR_pairs(2, max(nfr, 1)), twz_FR_k(2, max(nfr, 1)), twz_FR_init(3, 2, max(nfr, 1)), &
+               twz_FR_velo(3, 2, max(nfr, 1)))
+      if (nfr > 0) then
+         twz_FR_pairs(:, 1:nfr) = fr_pairs(:, 1:nfr)
+         twz_FR_k(:, 1:nfr) = fr_k(:, 1:nfr)
+         twz_FR_init(:, :, 1:nfr) = fr_init(:, :, 1:nfr)
+         twz_FR_velo(:, :, 1:nfr) = fr_velo(:, :, 1:nfr)
+      endif
+      flg_bias_rg = (rgpot /= 0)
+      bias_rg_pott = rgpot
+      bias_rg_k = rgk
+      if (allocated(bias_rg_0)) deallocate(bias_rg_0)
+      allocate(bias_rg_0(nrep_proc))
+      if (rgpot /= 0) bias_rg_0(1:nrep_proc) = rg0(1:nrep_proc)
+      nrest_sigb = nrest
+      flg_rest_sigb = (nrest > 0)
+      flg_restraint = (nrest > 0)
+      if (allocated(rest_sigb_id)) deallocate(rest_sigb_id, rest_sigb_rcut, rest_sigb_para)
+      allocate(rest_sigb_id(3, max(nrest, 1)), rest_sigb_rcut(max(nrest, 1)), rest_sigb_para(3, max(nrest, 1)))
+      if (nrest > 0) then
+         rest_sigb_id(:, 1:nrest) = rest_id(:, 1:nrest)
+         rest_sigb_rcut(1:nrest) = rest_rcut(1:nrest)
+         rest_sigb_para(:, 1:nrest) = rest_para(:, 1:nrest)
+      endif
+      istep = istep_in
+   end subroutine sisref_set_external
 
    subroutine sisref_ele_params(irep, out) bind(C, name="sisref_ele_params")
       use var_state, only : lambdaD, diele

@@ -144,6 +144,27 @@ def _worker(q, so, sysd, ops, seed, nstep_exchange):
                 t0 = time.perf_counter()
                 lib.sisref_job_md(C.c_int64(int(op[1])), _dp(e))
                 out[op[-1]] = time.perf_counter() - t0
+            elif kind == "external":
+                # op[1]: external-term dict (sis_amd.abi.external_to_struct keys), op[2]: istep
+                ext = op[1]
+                i32a = lambda a, n: np.ascontiguousarray(a if a is not None else np.zeros(n), np.int32)  # noqa: E731
+                f64a = lambda a, n: np.ascontiguousarray(a if a is not None else np.zeros(n), np.float64)  # noqa: E731
+                dp_ = i32a(ext.get("dcf_pairs"), 2)
+                nd = len(dp_) if ext.get("dcf_pairs") is not None else 0
+                df = f64a(ext.get("dcf_forces"), 3 * max(nrep, 1))
+                fp = i32a(ext.get("fr_pairs"), 2)
+                nf = len(fp) if ext.get("fr_pairs") is not None else 0
+                fk, fi, fv = f64a(ext.get("fr_k"), 2), f64a(ext.get("fr_init"), 6), f64a(ext.get("fr_velo"), 6)
+                rg0 = f64a(np.broadcast_to(np.asarray(ext.get("rg_0", 0.0), float), (nrep,)), nrep)
+                ri = i32a(ext.get("rest_id"), 3)
+                nr = len(ri) if ext.get("rest_id") is not None else 0
+                rc, rp = f64a(ext.get("rest_rcut"), 1), f64a(ext.get("rest_para"), 3)
+                lib.sisref_set_external(
+                    C.c_int(int(bool(ext.get("stage", 0)))), C.c_double(float(ext.get("stage_sigma", 10.0))),
+                    C.c_double(float(ext.get("stage_eps", 1.2))), C.c_int(nd), _ip(dp_), _dp(df), C.c_int(nf),
+                    _ip(fp), _dp(fk), _dp(fi), _dp(fv), C.c_int(int(ext.get("rg", 0))),
+                    C.c_double(float(ext.get("rg_k", 0.0))), _dp(rg0), C.c_int(nr), _ip(ri), _dp(rc), _dp(rp),
+                    C.c_int64(int(op[2])))
             elif kind == "write_rst":
                 pre = str(op[1]).encode()
                 lib.sisref_write_rst(C.c_char_p(pre), C.c_int(len(pre)))

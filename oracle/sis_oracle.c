@@ -71,6 +71,11 @@ struct orc_sys {
     tape_t tape_rep;
     int64_t istep;
     int flg_bp_energy;
+    /* external terms (copies of the caller's arrays) */
+    int x_on;
+    sis_external x;
+    int32_t *x_dcf_pairs, *x_fr_pairs, *x_rest_id;
+    double *x_rg0, *x_dcf_f, *x_fr_k, *x_fr_init, *x_fr_velo, *x_rest_rcut, *x_rest_para;
 };
 
 /* ---------------- PBC (src/pbc.F90:140-165, 196-226) ---------------- */
@@ -750,6 +755,112 @@ static void force_bp(orc_sys *s, int irep, int bp_mc, const double *x, double *f
 }
 
 /* force (src/force.F90:27-69), single thread */
+/* centre of mass and Rg (src/force_rg.F90:17-27, src/energy_rg.F90:17-27) */
+static double rg_of(const orc_sys *s, const double *x, double *cm) {
+    int n = s->nmp;
+    for (int k = 0; k < 3; k++) {
+        double t = 0.0;
+        for (int i = 0; i < n; i++) t += x[3 * i + k];
+        cm[k] = t / n;
+    }
+    double s2 = 0.0;
+    for (int i = 0; i < n; i++) {
+        double v[3] = {x[3 * i] - cm[0], x[3 * i + 1] - cm[1], x[3 * i + 2] - cm[2]};
+        s2 = s2 + dot3(v, v);
+    }
+    return sqrt(s2 / n);
+}
+
+/* external terms after the pair and bonded sum, in the reference's order
+ * (src/force.F90:62-75): stage inside the sum, then tweezers, Rg bias and
+ * restraints added to the forces */
+static void force_external(orc_sys *s, int irep, const double *x, double *f) {
+    const sis_external *e = &s->x;
+    if (e->stage) {   /* src/force_stage.F90:17-30 */
+        double coef = 24.0 * e->stage_eps / e->stage_sigma;
+        for (int i = 0; i < s->nmp; i++) {
+            double z = x[3 * i + 2];
+            if (z < 2.220446049250313e-16) z = 2.220446049250313e-16;
+            z = e->stage_sigma / z;
+            f[3 * i + 2] = f[3 * i + 2] + coef * (2 * pow(z, 13) - pow(z, 7));
+        }
+    }
+    for (int k = 0; k < e->n_dcf; k++) {   /* src/force_tweezers.F90:30-36 */
+        int i = s->x_dcf_pairs[2 * k] - 1, j = s->x_dcf_pairs[2 * k + 1] - 1;
+        const double *fv = s->x_dcf_f + 3 * ((size_t)irep * e->n_dcf + k);
+        for (int c = 0; c < 3; c++) { f[3 * i + c] += fv[c]; f[3 * j + c] -= fv[c]; }
+    }
+    for (int k = 0; k < e->n_fr; k++) {   /* src/force_tweezers.F90:38-48 */
+        for (int end = 0; end < 2; end++) {
+            int i = s->x_fr_pairs[2 * k + end] - 1;
+            double p[3], v[3];
+            for (int c = 0; c < 3; c++) p[c] = s->x_fr_init[6 * k + 3 * end + c] + s->x_fr_velo[6 * k + 3 * end + c] * (double)s->istep;
+            pbc_vec_d(s, p, F3(x, i), v);
+            for (int c = 0; c < 3; c++) f[3 * i + c] = f[3 * i + c] + s->x_fr_k[2 * k + end] * v[c];
+        }
+    }
+    if (e->rg) {   /* src/force_rg.F90:17-39 */
+        double cm[3], rg = rg_of(s, x, cm), r0 = s->x_rg0[irep];
+        if (!(e->rg == 2 && rg <= r0)) {
+            double factor = -e->rg_k * (rg - r0) / (rg * s->nmp);
+            for (int i = 0; i < s->nmp; i++)
+                for (int c = 0; c < 3; c++) f[3 * i + c] = f[3 * i + c] + factor * (x[3 * i + c] - cm[c]);
+        }
+    }
+    for (int k = 0; k < e->n_rest; k++) {   /* src/force_restraint.F90:18-40 */
+        int i = s->x_rest_id[3 * k] - 1, j = s->x_rest_id[3 * k + 1] - 1, ty = s->x_rest_id[3 * k + 2];
+        double v[3];
+        pbc_vec_d(s, F3(x, i), F3(x, j), v);
+        double r = sqrt(dot3(v, v));
+        if (r >= s->x_rest_rcut[k]) continue;
+        double eps = s->x_rest_para[3 * k], d = s->x_rest_para[3 * k + 1], sg = s->x_rest_para[3 * k + 2];
+        double t = tanh((r - d) / sg);
+        double coef = 0.5 * eps / sg * (1.0 - t * t) / r;
+        for (int c = 0; c < 3; c++) f[3 * i + c] = f[3 * i + c] - coef * v[c];
+        if (ty == 1) for (int c = 0; c < 3; c++) f[3 * j + c] = f[3 * j + c] + coef * v[c];
+    }
+}
+
+/* energies of the external terms into E[7..10] (src/energy_stage.F90,
+ * energy_tweezers.F90, energy_rg.F90, energy_restraint.F90); returns -1
+ * where energy_stage stops the run (a bead below 0.85 stage_sigma) */
+static int energy_external(orc_sys *s, int irep, const double *x, double *E) {
+    const sis_external *e = &s->x;
+    if (e->stage) {
+        double es = 0.0;
+        for (int i = 0; i < s->nmp; i++) if (x[3 * i + 2] < 0.85 * e->stage_sigma) return -1;
+        for (int i = 0; i < s->nmp; i++) {
+            double z = e->stage_sigma / x[3 * i + 2];
+            es = es + 4 * e->stage_eps * (pow(z, 12) - pow(z, 6));
+        }
+        E[7] = es;
+    }
+    if (e->n_dcf > 0 || e->n_fr > 0) {
+        double et = 0.0;
+        for (int k = 0; k < e->n_dcf; k++) {
+            int i = s->x_dcf_pairs[2 * k] - 1, j = s->x_dcf_pairs[2 * k + 1] - 1;
+            double v[3];
+            pbc_vec_d(s, F3(x, i), F3(x, j), v);
+            et = et - dot3(v, s->x_dcf_f + 3 * ((size_t)irep * e->n_dcf + k));
+        }
+        E[8] = et;
+    }
+    if (e->rg) {
+        double cm[3], rg = rg_of(s, x, cm), r0 = s->x_rg0[irep];
+        E[9] = (e->rg == 2 && rg <= r0) ? 0.0 : 0.5 * e->rg_k * (rg - r0) * (rg - r0);
+    }
+    for (int k = 0; k < e->n_rest; k++) {
+        int i = s->x_rest_id[3 * k] - 1, j = s->x_rest_id[3 * k + 1] - 1;
+        double v[3];
+        pbc_vec_d(s, F3(x, i), F3(x, j), v);
+        double r = sqrt(dot3(v, v));
+        if (r >= s->x_rest_rcut[k]) continue;
+        double eps = s->x_rest_para[3 * k], d = s->x_rest_para[3 * k + 1], sg = s->x_rest_para[3 * k + 2];
+        E[10] = E[10] - 0.5 * eps * (1.0 - tanh((r - d) / sg));
+    }
+    return 0;
+}
+
 void orc_force(orc_sys *s, int32_t irep1, int32_t bp_mc, double *forces) {
     int irep = irep1 - 1;
     const double *x = s->xyz + (size_t)irep * s->nmp * 3;
@@ -760,6 +871,41 @@ void orc_force(orc_sys *s, int32_t irep1, int32_t bp_mc, double *forces) {
     force_bp(s, irep, bp_mc, x, forces);
     force_wca(s, irep, x, forces);
     if (s->p.ele_on) force_ele(s, irep, x, forces);
+    if (s->x_on) force_external(s, irep, x, forces);
+}
+
+static void *dup_n(const void *src, size_t bytes) {
+    if (!src || bytes == 0) return NULL;
+    void *p = malloc(bytes);
+    memcpy(p, src, bytes);
+    return p;
+}
+
+static void free_external(orc_sys *s) {
+    free(s->x_dcf_pairs); free(s->x_fr_pairs); free(s->x_rest_id); free(s->x_rg0); free(s->x_dcf_f);
+    free(s->x_fr_k); free(s->x_fr_init); free(s->x_fr_velo); free(s->x_rest_rcut); free(s->x_rest_para);
+    s->x_dcf_pairs = s->x_fr_pairs = s->x_rest_id = NULL;
+    s->x_rg0 = s->x_dcf_f = s->x_fr_k = s->x_fr_init = s->x_fr_velo = s->x_rest_rcut = s->x_rest_para = NULL;
+    s->x_on = 0;
+}
+
+void orc_set_step(orc_sys *s, int64_t istep) { s->istep = istep; }
+
+void orc_set_external(orc_sys *s, const sis_external *e) {
+    free_external(s);
+    if (!e) return;
+    s->x = *e;
+    s->x_on = 1;
+    s->x_dcf_pairs = dup_n(e->dcf_pairs, sizeof(int32_t) * 2 * e->n_dcf);
+    s->x_dcf_f = dup_n(e->dcf_forces, sizeof(double) * 3 * e->n_dcf * s->nrep);
+    s->x_fr_pairs = dup_n(e->fr_pairs, sizeof(int32_t) * 2 * e->n_fr);
+    s->x_fr_k = dup_n(e->fr_k, sizeof(double) * 2 * e->n_fr);
+    s->x_fr_init = dup_n(e->fr_init, sizeof(double) * 6 * e->n_fr);
+    s->x_fr_velo = dup_n(e->fr_velo, sizeof(double) * 6 * e->n_fr);
+    s->x_rg0 = dup_n(e->rg_0, sizeof(double) * s->nrep);
+    s->x_rest_id = dup_n(e->rest_id, sizeof(int32_t) * 3 * e->n_rest);
+    s->x_rest_rcut = dup_n(e->rest_rcut, sizeof(double) * e->n_rest);
+    s->x_rest_para = dup_n(e->rest_para, sizeof(double) * 3 * e->n_rest);
 }
 
 /* ---------------- energies (src/energy_*.F90) ---------------- */
@@ -908,6 +1054,10 @@ void orc_energy(orc_sys *s, int32_t irep1, double tK, int32_t bp_mc, int32_t bp_
         }
         E[6] = e;
     }
+    if (s->x_on && energy_external(s, irep, x, E) < 0) {
+        for (int k = 0; k < 11; k++) E[k] = NAN;
+        return;
+    }
     double tot = 0.0;
     for (int k = 1; k <= 10; k++) tot += E[k];
     E[0] = tot;
@@ -999,6 +1149,7 @@ void orc_set_temps(orc_sys *s, const double *temps) {
 
 void orc_destroy(orc_sys *s) {
     if (!s) return;
+    free_external(s);
     for (int r = 0; r < s->nrep; r++) {
         free(s->rep[r].wca.a); free(s->rep[r].ele.a); free(s->rep[r].bp.a); free(s->rep[r].bpcoef.a);
         free(s->rep[r].bp_status); free(s->rep[r].bp_status_mc); free(s->rep[r].ene_bp);

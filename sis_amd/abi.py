@@ -65,6 +65,74 @@ class SisParams(C.Structure):
     ]
 
 
+class SisExternal(C.Structure):
+    """struct sis_external: stage wall, tweezers, Rg bias, restraints."""
+    _fields_ = [
+        ("stage", C.c_int32), ("n_dcf", C.c_int32), ("n_fr", C.c_int32), ("rg", C.c_int32),
+        ("n_rest", C.c_int32), ("pad", C.c_int32),
+        ("stage_sigma", C.c_double), ("stage_eps", C.c_double), ("rg_k", C.c_double),
+        ("rg_0", C.POINTER(C.c_double)),
+        ("dcf_pairs", C.POINTER(C.c_int32)), ("dcf_forces", C.POINTER(C.c_double)),
+        ("fr_pairs", C.POINTER(C.c_int32)), ("fr_k", C.POINTER(C.c_double)),
+        ("fr_init", C.POINTER(C.c_double)), ("fr_velo", C.POINTER(C.c_double)),
+        ("rest_id", C.POINTER(C.c_int32)), ("rest_rcut", C.POINTER(C.c_double)),
+        ("rest_para", C.POINTER(C.c_double)),
+    ]
+
+
+def external_to_struct(ext: dict, nrep: int):
+    """Fill a SisExternal from a dict: stage, stage_sigma, stage_eps;
+    dcf_pairs (n, 2) 1-based, dcf_forces (nrep, n, 3); fr_pairs (n, 2),
+    fr_k (n, 2), fr_init / fr_velo (n, 2, 3); rg (0 off, 1 harmonic,
+    2 flat-bottomed), rg_k, rg_0 (nrep); rest_id (n, 3), rest_rcut (n),
+    rest_para (n, 3).  Returns (struct, arrays to keep alive)."""
+    s = SisExternal()
+    keep = []
+
+    def arr(name, dtype, shape_tail):
+        a = ext.get(name)
+        if a is None:
+            return None, 0
+        a = np.ascontiguousarray(a, dtype)
+        keep.append(a)
+        return a, (a.shape[0] if a.ndim else 0)
+
+    s.stage = int(bool(ext.get("stage", 0)))
+    s.stage_sigma = float(ext.get("stage_sigma", 10.0))
+    s.stage_eps = float(ext.get("stage_eps", 1.2))
+    a, n = arr("dcf_pairs", np.int32, 2)
+    if a is not None:
+        s.n_dcf = n
+        s.dcf_pairs = _ptr(a, C.c_int32)
+        f = np.ascontiguousarray(ext["dcf_forces"], np.float64).reshape(nrep, n, 3)
+        keep.append(f)
+        s.dcf_forces = _ptr(f, C.c_double)
+    a, n = arr("fr_pairs", np.int32, 2)
+    if a is not None:
+        s.n_fr = n
+        s.fr_pairs = _ptr(a, C.c_int32)
+        for nm, shp in (("fr_k", (n, 2)), ("fr_init", (n, 2, 3)), ("fr_velo", (n, 2, 3))):
+            b = np.ascontiguousarray(ext[nm], np.float64).reshape(shp)
+            keep.append(b)
+            setattr(s, nm, _ptr(b, C.c_double))
+    s.rg = int(ext.get("rg", 0))
+    if s.rg:
+        s.rg_k = float(ext["rg_k"])
+        r0 = np.ascontiguousarray(np.broadcast_to(np.asarray(ext["rg_0"], np.float64), (nrep,)))
+        keep.append(r0)
+        s.rg_0 = _ptr(r0, C.c_double)
+    a, n = arr("rest_id", np.int32, 3)
+    if a is not None:
+        s.n_rest = n
+        s.rest_id = _ptr(a, C.c_int32)
+        rc = np.ascontiguousarray(ext["rest_rcut"], np.float64).reshape(n)
+        rp = np.ascontiguousarray(ext["rest_para"], np.float64).reshape(n, 3)
+        keep += [rc, rp]
+        s.rest_rcut = _ptr(rc, C.c_double)
+        s.rest_para = _ptr(rp, C.c_double)
+    return s, keep
+
+
 def params_to_struct(p: dict) -> SisParams:
     """Fill a SisParams from a flat dict (see forcefield.default_params)."""
     s = SisParams()
@@ -126,7 +194,7 @@ EXPORTED = [
     "sis_gpu_kinetic", "sis_gpu_out_open", "sis_gpu_out_close", "sis_out_write_file",
     "sis_gpu_dcd_energies", "sis_dcd_read_file", "sis_gpu_check_force",
     "sis_gpu_rst_open", "sis_gpu_rst_write", "sis_gpu_rst_close", "sis_gpu_rst_read",
-    "sis_rst_write_file", "sis_rst_read_file",
+    "sis_rst_write_file", "sis_rst_read_file", "sis_gpu_set_external",
 ]
 
 
@@ -176,6 +244,7 @@ def _declare(lib: C.CDLL) -> None:
         "sis_gpu_bp_close": (i32, [vp]),
         "sis_bp_write_file": (i32, [C.c_char_p, i64, P(i32), P(C.c_uint8), P(d), i32, d, i32]),
         "sis_gpu_set_labels": (i32, [vp, P(i32)]),
+        "sis_gpu_set_external": (i32, [vp, P(SisExternal)]),
         "sis_gpu_rst_open": (i32, [vp, P(C.c_char_p), i64]),
         "sis_gpu_rst_write": (i32, [vp]),
         "sis_gpu_rst_close": (i32, [vp]),

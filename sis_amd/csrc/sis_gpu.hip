@@ -14,6 +14,7 @@
 #include "sis_device.h"
 #include "sis_kernels.h"
 #include "sis_persistent.h"
+#include "sis_external.h"
 
 using namespace sis;
 
@@ -84,6 +85,13 @@ struct sis_gpu_ctx {
   BpOut *bpo = nullptr;    // base-pair output (sis_gpu_bp_open), or none
   OutFile *outf = nullptr; // energy lines (sis_gpu_out_open), or none
   RstOut *rst = nullptr;   // restart files (sis_gpu_rst_open), or none
+  // external terms (sis_gpu_set_external)
+  bool ext_on = false;
+  ExtDev X{};
+  DevBuf<double> x_rg0_d, x_dcf_f_d, x_fr_k_d, x_fr_init_d, x_fr_velo_d, x_rest_rcut_d, x_rest_para_d,
+      x_rg_stat_d, x_ene_d;
+  DevBuf<int> x_term_off_d, x_dcf_pairs_d, x_fr_pairs_d, x_rest_id_d, x_err_d;
+  DevBuf<int2> x_terms_d;
   Params P{};
   Ptrs D{};
   sis_options opt{};
@@ -216,6 +224,12 @@ struct sis_gpu_ctx {
                             &bp_coef2_d};
     for (auto *b : db) b->free_();
     bpp_d.free_();
+    DevBuf<double> *xb[] = {&x_rg0_d, &x_dcf_f_d, &x_fr_k_d, &x_fr_init_d, &x_fr_velo_d, &x_rest_rcut_d,
+                            &x_rest_para_d, &x_rg_stat_d, &x_ene_d};
+    for (auto *b : xb) b->free_();
+    DevBuf<int> *xi[] = {&x_term_off_d, &x_dcf_pairs_d, &x_fr_pairs_d, &x_rest_id_d, &x_err_d};
+    for (auto *b : xi) b->free_();
+    x_terms_d.free_();
     DevBuf<double4> *vb[] = {&pos_d, &pos2_d, &vel_d, &acc_d, &move_d, &frc_d, &fbond_d};
     for (auto *b : vb) b->free_();
     topo_d.free_();
@@ -596,6 +610,11 @@ void force_kernel(sis_gpu_ctx *c, int integrate, const int *mask_dev) {
   c->nblk_force = g.x;
   if ((int64_t)c->blk_top2_d.n < (int64_t)R * g.x * 2) c->blk_top2_d.alloc((size_t)R * g.x * 2);
   hipLaunchKernelGGL((k_bonded<LPB, B>), g, dim3(B), 0, c->stream, c->P, c->ptrs(), mask_dev);
+  if (c->ext_on) {
+    if (c->X.rg) hipLaunchKernelGGL(k_rg_stats<256>, dim3(R), dim3(256), 0, c->stream, c->P, c->ptrs(), c->X);
+    hipLaunchKernelGGL(k_external<256>, grid_beads(N, R, 256), dim3(256), 0, c->stream, c->P, c->ptrs(), c->X,
+                       c->istep, mask_dev);
+  }
   if (c->precision == 1)
     hipLaunchKernelGGL((k_force_lanes<float, LPB, B>), g, dim3(B), 0, c->stream, c->P, c->ptrs(), integrate,
                        c->parity, c->seed, c->istep, mask_dev);
@@ -646,10 +665,26 @@ void energy_phase(sis_gpu_ctx *c, const std::vector<int> &mask, const std::vecto
   if ((int64_t)c->ene_part_d.n < (int64_t)R * g.x * 8) c->ene_part_d.alloc((size_t)R * g.x * 8);
   hipLaunchKernelGGL(k_energy<B>, g, dim3(B), 0, c->stream, c->P, c->ptrs(), c->ecT_d.p, c->eCoef_d.p,
                      c->eLam_d.p, c->ene_part_d.p, c->mask_d.p, flags & 1);
-  hipLaunchKernelGGL(k_energy_reduce, dim3(R), dim3(1), 0, c->stream, c->ene_part_d.p, (int)g.x, c->ene_out_d.p);
+  const double *ext = nullptr;
+  if (c->ext_on) {
+    if (c->X.rg) hipLaunchKernelGGL(k_rg_stats<256>, dim3(R), dim3(256), 0, c->stream, c->P, c->ptrs(), c->X);
+    hipLaunchKernelGGL(k_external_energy<256>, dim3(R), dim3(256), 0, c->stream, c->P, c->ptrs(), c->X);
+    ext = c->x_ene_d.p;
+  }
+  hipLaunchKernelGGL(k_energy_reduce, dim3(R), dim3(1), 0, c->stream, c->ene_part_d.p, (int)g.x, c->ene_out_d.p,
+                     ext);
   HIPCHK(hipMemcpyAsync(out, c->ene_out_d.p, (size_t)R * 16 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
   check_flags(c);
+  if (c->ext_on && c->X.stage) {
+    int err = 0;
+    HIPCHK(hipMemcpy(&err, c->x_err_d.p, sizeof err, hipMemcpyDeviceToHost));
+    if (err) {
+      HIPCHK(hipMemset(c->x_err_d.p, 0, sizeof err));
+      // src/energy_stage.F90:23-27 stops the run here
+      throw ApiError{SIS_ERR_STATE, "Z-coordinate < 0.85 * stage_sigma in energy_stage"};
+    }
+  }
 }
 
 int get_pair(sis_gpu_ctx *c, int lab) { return c->pair_tb[c->idx_pair * (c->R + 2) + lab]; }
@@ -1431,6 +1466,93 @@ int sis_gpu_set_step(sis_gpu_ctx *c, int64_t istep) {
   API_END
 }
 
+int sis_gpu_set_external(sis_gpu_ctx *c, const sis_external *e) {
+  API_BEGIN
+  if (!c) return fail(SIS_ERR_ARG, "null context");
+  const int N = c->N, R = c->R;
+  if (!e || (!e->stage && e->n_dcf == 0 && e->n_fr == 0 && !e->rg && e->n_rest == 0)) {
+    c->ext_on = false;
+    c->X = ExtDev{};
+    return SIS_OK;
+  }
+  if (e->n_dcf < 0 || e->n_fr < 0 || e->n_rest < 0 || e->rg < 0 || e->rg > 2)
+    return fail(SIS_ERR_ARG, "invalid external-term counts");
+  if (e->stage && !(e->stage_sigma > 0.0)) return fail(SIS_ERR_ARG, "stage_sigma must be positive");
+  if ((e->n_dcf && (!e->dcf_pairs || !e->dcf_forces)) ||
+      (e->n_fr && (!e->fr_pairs || !e->fr_k || !e->fr_init || !e->fr_velo)) || (e->rg && !e->rg_0) ||
+      (e->n_rest && (!e->rest_id || !e->rest_rcut || !e->rest_para)))
+    return fail(SIS_ERR_ARG, "null external-term array");
+  auto bead = [&](int32_t b) {
+    if (b < 1 || b > N) throw ApiError{SIS_ERR_ARG, "external-term bead index out of range"};
+    return b - 1;
+  };
+  // per-bead lists of term ends, in the order the reference adds them
+  // (tweezers: constant-force pairs, then ramps; then restraints)
+  std::vector<std::vector<int2>> per(N);
+  std::vector<int> dcfp(2 * e->n_dcf), frp(2 * e->n_fr), rid(3 * e->n_rest);
+  for (int k = 0; k < e->n_dcf; k++) {
+    dcfp[2 * k] = bead(e->dcf_pairs[2 * k]);
+    dcfp[2 * k + 1] = bead(e->dcf_pairs[2 * k + 1]);
+    per[dcfp[2 * k]].push_back(int2{XT_DCF1, k});
+    per[dcfp[2 * k + 1]].push_back(int2{XT_DCF2, k});
+  }
+  for (int k = 0; k < e->n_fr; k++) {
+    frp[2 * k] = bead(e->fr_pairs[2 * k]);
+    frp[2 * k + 1] = bead(e->fr_pairs[2 * k + 1]);
+    per[frp[2 * k]].push_back(int2{XT_FR1, k});
+    per[frp[2 * k + 1]].push_back(int2{XT_FR2, k});
+  }
+  for (int k = 0; k < e->n_rest; k++) {
+    const int ty = e->rest_id[3 * k + 2];
+    if (ty != 1 && ty != 2) return fail(SIS_ERR_ARG, "restraint type must be 1 (sigmoid) or 2 (sigmoid-to-bead)");
+    if (!(e->rest_para[3 * k + 2] != 0.0)) return fail(SIS_ERR_ARG, "restraint width must be non-zero");
+    rid[3 * k] = bead(e->rest_id[3 * k]);
+    rid[3 * k + 1] = bead(e->rest_id[3 * k + 1]);
+    rid[3 * k + 2] = ty;
+    per[rid[3 * k]].push_back(int2{XT_REST_I, k});
+    if (ty == 1) per[rid[3 * k + 1]].push_back(int2{XT_REST_J, k});
+  }
+  std::vector<int> off(N + 1, 0);
+  std::vector<int2> terms;
+  for (int a = 0; a < N; a++) {
+    off[a] = (int)terms.size();
+    terms.insert(terms.end(), per[a].begin(), per[a].end());
+  }
+  off[N] = (int)terms.size();
+  auto up = [&](auto &buf, const auto *src, size_t n) {
+    buf.alloc(std::max<size_t>(n, 1));
+    if (n) HIPCHK(hipMemcpy(buf.p, src, n * sizeof(*src), hipMemcpyHostToDevice));
+    return buf.p;
+  };
+  ExtDev X{};
+  X.stage = e->stage != 0; X.rg = e->rg; X.ndcf = e->n_dcf; X.nfr = e->n_fr; X.nrest = e->n_rest;
+  X.stage_sigma = e->stage_sigma; X.stage_eps = e->stage_eps; X.rg_k = e->rg_k;
+  std::vector<double> rg0(R, 0.0);
+  if (e->rg) std::copy(e->rg_0, e->rg_0 + R, rg0.begin());
+  X.rg0 = up(c->x_rg0_d, rg0.data(), (size_t)R);
+  X.term_off = up(c->x_term_off_d, off.data(), off.size());
+  X.terms = up(c->x_terms_d, terms.data(), terms.size());
+  X.dcf_pairs = up(c->x_dcf_pairs_d, dcfp.data(), dcfp.size());
+  X.dcf_f = up(c->x_dcf_f_d, e->dcf_forces, (size_t)3 * e->n_dcf * R);
+  X.fr_pairs = up(c->x_fr_pairs_d, frp.data(), frp.size());
+  X.fr_k = up(c->x_fr_k_d, e->fr_k, (size_t)2 * e->n_fr);
+  X.fr_init = up(c->x_fr_init_d, e->fr_init, (size_t)6 * e->n_fr);
+  X.fr_velo = up(c->x_fr_velo_d, e->fr_velo, (size_t)6 * e->n_fr);
+  X.rest_id = up(c->x_rest_id_d, rid.data(), rid.size());
+  X.rest_rcut = up(c->x_rest_rcut_d, e->rest_rcut, (size_t)e->n_rest);
+  X.rest_para = up(c->x_rest_para_d, e->rest_para, (size_t)3 * e->n_rest);
+  c->x_rg_stat_d.alloc((size_t)4 * R);
+  c->x_ene_d.alloc((size_t)4 * R);
+  c->x_err_d.alloc(1);
+  X.rg_stat = c->x_rg_stat_d.p;
+  X.ene = c->x_ene_d.p;
+  X.err = c->x_err_d.p;
+  c->X = X;
+  c->ext_on = true;
+  return SIS_OK;
+  API_END
+}
+
 int sis_gpu_set_labels(sis_gpu_ctx *c, const int32_t *rep2lab) {
   API_BEGIN
   if (!c || !rep2lab) return fail(SIS_ERR_ARG, "null argument");
@@ -1468,7 +1590,7 @@ int sis_gpu_get_info(sis_gpu_ctx *c, int64_t *out, int32_t n) {
   API_BEGIN
   if (!c || !out) return fail(SIS_ERR_ARG, "null argument");
   int mode_used = c->parity ? 1 : (c->mode ? c->mode : choose_mode(c));
-  if ((mode_used == 3 && !batched_ok(c)) || (mode_used == 2 && c->R != 1)) mode_used = 1;   // forms fall back
+  if ((mode_used == 3 && !batched_ok(c)) || (mode_used == 2 && (c->R != 1 || c->ext_on))) mode_used = 1;   // forms fall back
   int64_t v[10] = {c->N, c->R, c->istep, c->rebuilds, 0, 0, 0, c->D.maxnb, mode_used, c->block};
   if (c->lists_valid) {
     int64_t k;

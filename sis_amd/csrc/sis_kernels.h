@@ -1542,12 +1542,15 @@ __global__ void __launch_bounds__(B) k_kinetic(Params P, Ptrs D, double *out) {
   if (threadIdx.x == 0) out[r] = 0.5 * s[0];
 }
 
-__global__ void k_energy_reduce(const double *partial, int nblk, double *out) {
+// ext: stage, tweezers, Rg and restraint energies [4*R], or null
+__global__ void k_energy_reduce(const double *partial, int nblk, double *out, const double *ext) {
   int r = blockIdx.x;
   if (threadIdx.x != 0) return;
   double e[11] = {0};
   for (int b = 0; b < nblk; b++)
     for (int k = 1; k < 7; k++) e[k] += partial[((size_t)r * nblk + b) * 8 + k];
+  if (ext)
+    for (int k = 0; k < 4; k++) e[7 + k] = ext[4 * r + k];
   double t = 0.0;
   for (int k = 1; k <= 10; k++) t += e[k];
   e[0] = t;

@@ -4,13 +4,13 @@
 
 
 bool batched_ok(sis_gpu_ctx *c) {
-  return !c->parity && c->R >= 2 && c->P.nchains == 1 && c->N <= 1024;
+  return !c->parity && !c->ext_on && c->R >= 2 && c->P.nchains == 1 && c->N <= 1024;
 }
 
 int choose_mode(sis_gpu_ctx *c) {
   if (c->parity) return 1;
   if (batched_ok(c)) return 3;
-  if (c->R == 1 && c->N >= 1024) return 2;
+  if (c->R == 1 && c->N >= 1024 && !c->ext_on) return 2;
   return 1;
 }
 
@@ -32,7 +32,7 @@ int persistent_grid(sis_gpu_ctx *c, const void *kfn, int B) {
 }  // namespace
 
 void persistent_steps(sis_gpu_ctx *c, int64_t nsteps) {
-  if (c->R != 1) {
+  if (c->R != 1 || c->ext_on) {   // external terms run in the per-phase form
     for (int64_t s = 0; s < nsteps; s++) step_phases(c);
     return;
   }

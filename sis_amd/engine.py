@@ -211,6 +211,16 @@ class Engine:
     def out_close(self):
         _check(lib().sis_gpu_out_close(self.h))
 
+    def set_external(self, ext: dict | None):
+        """Stage wall, tweezers, Rg bias and restraints (see
+        abi.external_to_struct); None clears them."""
+        if ext is None:
+            _check(lib().sis_gpu_set_external(self.h, None))
+            return
+        s, keep = abi.external_to_struct(ext, self.nrep)
+        _check(lib().sis_gpu_set_external(self.h, C.byref(s)))
+        del keep
+
     def rst_open(self, paths, nstep_save_rst: int = 0):
         """Restart files (src/write_rst.F90 blocks), one per replica."""
         arr = (C.c_char_p * self.nrep)(*[str(x).encode() for x in paths])

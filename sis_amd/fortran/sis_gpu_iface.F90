@@ -51,6 +51,15 @@ module sis_gpu_iface
       integer(c_int64_t) :: seed
    end type sis_options
 
+   ! external terms (src/force.F90:62-75): c_loc of the reference's arrays
+   ! (twz_DCF_pairs, twz_DCF_forces, ..., rest_sigb_para), c_null_ptr when unused
+   type, bind(C) :: sis_external
+      integer(c_int32_t) :: stage, n_dcf, n_fr, rg, n_rest, pad
+      real(c_double) :: stage_sigma, stage_eps, rg_k
+      type(c_ptr) :: rg_0, dcf_pairs, dcf_forces, fr_pairs, fr_k, fr_init, fr_velo
+      type(c_ptr) :: rest_id, rest_rcut, rest_para
+   end type sis_external
+
    interface
       integer(c_int) function sis_gpu_create(topo, prm, opt, ctx) bind(C, name="sis_gpu_create")
          import :: c_int, c_ptr, sis_topo, sis_params, sis_options
@@ -221,6 +230,13 @@ module sis_gpu_iface
       integer(c_int) function sis_gpu_out_close(ctx) bind(C, name="sis_gpu_out_close")
          import :: c_int, c_ptr
          type(c_ptr), value :: ctx
+      end function
+
+      ! stage wall, tweezers, Rg bias, restraints
+      integer(c_int) function sis_gpu_set_external(ctx, ext) bind(C, name="sis_gpu_set_external")
+         import :: c_int, c_ptr, sis_external
+         type(c_ptr), value :: ctx
+         type(sis_external), intent(in) :: ext
       end function
 
       ! restart files (.rst, src/write_rst.F90 blocks) and read_rst

@@ -77,6 +77,34 @@ def remd_bd(nrep=4, seed=4):
     return sysd
 
 
+def remd_external(nrep=2, seed=6):
+    """Every external term of src/force.F90:62-75 on a two-replica ladder
+    lifted above the stage wall: stage, two constant-force tweezer pairs, a
+    force ramp, the Rg bias (harmonic, per-replica target) and two sigmoid
+    restraints (one two-sided, one to a fixed bead), near their bounds so
+    that their forces are not small."""
+    sysd = configs.remd_c3(nrep=nrep, n=40, seed=seed)
+    x = np.asarray(sysd["xyz"], float).copy()
+    x[..., 2] += 20.0 - x[..., 2].min()
+    sysd["xyz"] = x
+    n = 40
+    x0 = x.reshape(nrep, n, 3)[0]
+    rng = np.random.default_rng(seed)
+    d = lambda i, j: float(np.linalg.norm(x0[i - 1] - x0[j - 1]))  # noqa: E731
+    sysd["external"] = dict(
+        stage=1, stage_sigma=10.0, stage_eps=1.2,
+        dcf_pairs=np.array([[1, n], [3, n - 4]]), dcf_forces=rng.normal(0, 2.0, size=(nrep, 2, 3)),
+        fr_pairs=np.array([[2, n - 1]]), fr_k=np.array([[0.5, 0.7]]),
+        fr_init=(x0[[1, n - 2]] + rng.normal(0, 3.0, size=(2, 3)))[None],
+        fr_velo=rng.normal(0, 0.01, size=(1, 2, 3)),
+        rg=1, rg_k=0.8, rg_0=np.linspace(5.0, 12.0, nrep),
+        rest_id=np.array([[5, n - 10, 1], [7, n - 15, 2]]),
+        rest_rcut=np.array([d(5, n - 10) + 5.0, d(7, n - 15) + 5.0]),
+        rest_para=np.array([[2.0, d(5, n - 10) - 1.0, 3.0], [1.5, d(7, n - 15) + 0.5, 2.0]]))
+    sysd["name"] = "remd2ext"
+    return sysd
+
+
 SYSTEMS = {
     "hairpin64": (lambda: configs.hairpin64(1), 60, 0),
     "small_pbc": (lambda: configs.small_pbc(4, 30, 100.0, seed=5), 60, 0),
@@ -84,6 +112,7 @@ SYSTEMS = {
     "remd4": (remd_small, 40, 10),
     "remd4mc3": (remd_mc3, 40, 10),
     "remd4bd": (remd_bd, 40, 10),
+    "remd2ext": (remd_external, 30, 10),
 }
 
 
@@ -94,13 +123,17 @@ def make(name, builder, nstep, nex, seed=4321):
     ops = [("tape", r, big, f"tape{r}") for r in range(1, nrep + 1)]
     if nrep > 1:
         ops.append(("tape", -1, 4096, "tape_rep"))
-    ops += [("nl", r) for r in range(1, nrep + 1)]
+    ext = s.get("external")
+    pre = [("external", ext, 0)] if ext else []
+    ops = pre + ops + [("nl", r) for r in range(1, nrep + 1)]
     ops += [("lists", 1, "L"), ("force", 1, 1, "F"), ("bp_state", 1, "B"),
             ("energy", 1, float(s["temps"][0]), 0, 1, "E"), ("ele", 1, "ele"), ("bp_tables", "T")]
     ref0 = refdrv.run(s, ops, seed=seed, nstep_exchange=nex)
-    ref1 = refdrv.run(s, [("job_md", nstep, "R")], seed=seed, nstep_exchange=nex)
+    ref1 = refdrv.run(s, pre + [("job_md", nstep, "R")], seed=seed, nstep_exchange=nex)
     # count the words the static part + the run consume, using the oracle
     o = corc.Oracle(s)
+    if ext:
+        o.set_external(ext)
     for r in range(1, nrep + 1):
         o.set_tape(r, ref0[f"tape{r}"])
     if nrep > 1:
@@ -110,6 +143,8 @@ def make(name, builder, nstep, nex, seed=4321):
     o.force(1, 1)
     used_static = [o.tape_pos(r) for r in range(1, nrep + 1)]
     o2 = corc.Oracle(s)
+    if ext:
+        o2.set_external(ext)
     for r in range(1, nrep + 1):
         o2.set_tape(r, ref0[f"tape{r}"])
     if nrep > 1:
@@ -135,6 +170,8 @@ def make(name, builder, nstep, nex, seed=4321):
     out["param_names"] = np.array(sorted(scal))
     out["param_values"] = np.array([float(scal[k]) for k in sorted(scal)])
     out["box"] = np.array(p["box"], float)
+    for k, v in (ext or {}).items():
+        out["ext_" + k] = np.asarray(v)
     np.savez_compressed(HERE / f"{name}.npz", **out)
     print(name, "tape words kept", keep, "lists", {k: len(ref0["L"][k]) for k in ("wca", "ele", "bp")})
 
@@ -153,6 +190,13 @@ def load(name):
                 seq=d["seq"], xyz=d["xyz"], params=p, nrep=int(d["nrep"]), temps=d["temps"],
                 nstep=int(d["nstep"]), seqs=seqs)
     p["box"] = [float(b) for b in d["box"]]
+    ext = {k[4:]: d[k] for k in d if k.startswith("ext_")}
+    if ext:
+        for k in ("stage", "rg"):
+            ext[k] = int(ext[k])
+        for k in ("stage_sigma", "stage_eps", "rg_k"):
+            ext[k] = float(ext[k])
+        sysd["external"] = ext
     return sysd, d
 
 

@@ -20,7 +20,7 @@ from tests.golden.make_golden import load
 
 pytestmark = pytest.mark.gpu
 
-FIXTURES = ["hairpin64", "small_pbc", "mixed", "remd4", "remd4mc3", "remd4bd"]
+FIXTURES = ["hairpin64", "small_pbc", "mixed", "remd4", "remd4mc3", "remd4bd", "remd2ext"]
 
 
 def _sorted_rows(a):
@@ -33,6 +33,8 @@ def _pair_sets_equal(eng_pairs, ref_pairs):
 
 def _engine(sysd, d=None, **kw):
     eng = Engine(sysd, parity=True, **kw)
+    if "external" in sysd:
+        eng.set_external(sysd["external"])
     if d is not None:
         for r in range(1, sysd["nrep"] + 1):
             eng.set_rng_tape(r, d["tapes"][r - 1])
@@ -281,6 +283,31 @@ def test_brownian_forms_match_phase_form(form):
         res.append((st, eng.labels()))
     assert np.array_equal(res[0][1], res[1][1])
     assert np.abs(res[0][0][0] - res[1][0][0]).max() < 1e-6
+
+
+def test_external_terms_in_performance_mode():
+    """External terms (stage, tweezers, Rg bias, restraints) with the counter
+    RNG: a context created for the batched form steps in the per-phase
+    form (the terms are not in the batched kernel), so its run equals a
+    per-phase context's exactly; the run stays above the stage wall and its
+    external energies stay finite."""
+    from tests.golden.make_golden import remd_external
+    s = remd_external()
+    print("[case] external terms, performance mode", flush=True)
+    res = []
+    for mode in ("phases", "batched"):
+        eng = Engine(s, parity=False, mode=mode, seed=5, exchange_every=10)
+        eng.set_external(s["external"])
+        assert eng.info()["mode"] == 1
+        eng.init_velocities()
+        eng.neighbor_list(0)
+        eng.step(200)
+        x = eng.get_state()[0]
+        e = eng.energy_sumup(1, float(s["temps"][0]), 0, 1)
+        assert np.all(np.isfinite(x)) and x[..., 2].min() > 8.5
+        assert np.all(np.isfinite(e)) and np.all(e[7:] != 0.0)
+        res.append((x, eng.labels()))
+    assert np.array_equal(res[0][0], res[1][0]) and np.array_equal(res[0][1], res[1][1])
 
 
 def _largest_formed_component(eng, irep=1):

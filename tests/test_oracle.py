@@ -10,12 +10,14 @@ from sis_amd import configs
 from oracle import corc, refdrv
 from tests.golden.make_golden import load
 
-FIXTURES = ["hairpin64", "small_pbc", "mixed", "remd4", "remd4mc3", "remd4bd"]
+FIXTURES = ["hairpin64", "small_pbc", "mixed", "remd4", "remd4mc3", "remd4bd", "remd2ext"]
 
 
 def _oracle_from_fixture(name):
     sysd, d = load(name)
     o = corc.Oracle(sysd)
+    if "external" in sysd:
+        o.set_external(sysd["external"])
     for r in range(1, sysd["nrep"] + 1):
         o.set_tape(r, d["tapes"][r - 1])
     if sysd["nrep"] > 1:
@@ -222,6 +224,67 @@ def test_live_reference_brownian_random_system():
     assert np.abs(x - ref1["xyz"]).max() < 1e-10
     assert np.abs(v - ref1["velos"]).max() == 0.0   # velocities are not integrated
     assert np.array_equal(o.labels(), ref1["rep2lab"][:4])
+
+
+def test_external_terms_are_energy_gradients():
+    """Stage wall, constant-force tweezers, Rg bias and sigmoid restraints
+    (src/force_stage.F90 ... src/force_restraint.F90): the force they add
+    is minus the central-difference gradient of the energy they add
+    (src/energy_stage.F90 ... src/energy_restraint.F90).  The force ramp
+    has no energy term in the reference and is left out here."""
+    from tests.golden.make_golden import remd_external
+    s = remd_external()
+    ext = dict(s["external"])
+    for k in ("fr_pairs", "fr_k", "fr_init", "fr_velo"):
+        ext.pop(k)
+    o0, o1 = corc.Oracle(s), corc.Oracle(s)
+    o1.set_external(ext)
+    x = np.asarray(s["xyz"], float).reshape(2, 40, 3)
+    tK = float(s["temps"][0])
+
+    def eext(o, xx):
+        o.set_xyz(xx)
+        return o.energy(1, tK, 0, 0)[7:].sum()
+
+    for o in (o0, o1):
+        o.neighbor_list(1)
+    f = o1.force(1, 0) - o0.force(1, 0)
+    h = 1e-5
+    rng = np.random.default_rng(3)
+    for a in list(range(0, 8)) + list(rng.choice(40, 6, replace=False)):
+        for c in range(3):
+            xp, xm = x.copy(), x.copy()
+            xp[0, a, c] += h
+            xm[0, a, c] -= h
+            g = (eext(o1, xp) - eext(o1, xm)) / (2 * h)
+            assert abs(f[a, c] + g) <= 1e-5 * max(1.0, abs(g)), (a, c, f[a, c], -g)
+
+
+@pytest.mark.ref
+def test_live_reference_external_terms_periodic():
+    """Tweezers (constant force and a ramp at step 40), flat-bottomed Rg bias
+    and restraints across the periodic boundary: oracle forces and energies
+    vs the reference build."""
+    from tests.golden.make_golden import remd_external
+    sp = configs.small_pbc(3, 12, 80.0, seed=4)
+    n = 36
+    ext = dict(remd_external()["external"])
+    ext.update(stage=0, rg=2, rg_0=[6.0], dcf_forces=np.asarray(ext["dcf_forces"])[:1],
+               dcf_pairs=[[1, n], [3, n - 4]], fr_pairs=[[2, n - 1]],
+               rest_id=[[5, n - 10, 1], [7, n - 15, 2]], rest_rcut=[60.0, 60.0],
+               rest_para=[[2.0, 20.0, 3.0], [1.5, 25.0, 2.0]])
+    ops = [("external", ext, 40), ("nl", 1), ("force", 1, 0, "F"),
+           ("energy", 1, float(sp["temps"][0]), 0, 1, "E")]
+    ref = refdrv.run(sp, ops)
+    o = corc.Oracle(sp)
+    o.set_external(ext)
+    o.set_step(40)
+    o.neighbor_list(1)
+    f = o.force(1, 0)
+    e = o.energy(1, float(sp["temps"][0]), 0, 1)
+    assert np.abs(f - ref["F"]).max() / np.abs(ref["F"]).max() < 1e-12
+    assert np.allclose(e, ref["E"], rtol=1e-12, atol=1e-12)
+    assert np.all(e[8:] != 0.0)
 
 
 @pytest.mark.ref
