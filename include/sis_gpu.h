@@ -205,6 +205,15 @@ typedef struct sis_external {
  * whatever mode it was created with. */
 int sis_gpu_set_external(sis_gpu_ctx *ctx, const sis_external *ext);
 
+/* Secondary-structure restriction (read_ss, src/read_ss.F90, applied by
+ * init_bp, src/init_bp.F90:177-209): only the listed pairs (npair pairs of
+ * 1-based bead ids, either order, as a CT, BPSEQ or BPL file gives them)
+ * may form, after read_ss's filters (chain ends, minimum loop, isolated
+ * pairs for models 3-5, non-canonical pairs).  npair < 0 lifts the
+ * restriction.  The lists are rebuilt by the next step.  The distance bias
+ * of flg_bias_ss (src/force_bp_limit_triplet.F90:104-110) is not covered. */
+int sis_gpu_set_ss(sis_gpu_ctx *ctx, int32_t npair, const int32_t *pairs);
+
 /* Restart: the STEP and REPLICA blocks of a restart file
  * (src/write_rst.F90:40-60, read back by src/read_rst.F90).  set_step
  * sets istep, which keys the performance-mode random streams, so a run

@@ -46,6 +46,8 @@ def lib():
         L.orc_md_coef.argtypes = [vp, i32, P(d)]
         L.orc_set_external.argtypes = [vp, P(abi.SisExternal)]
         L.orc_set_step.argtypes = [vp, i64]
+        L.orc_set_ss.restype = i32
+        L.orc_set_ss.argtypes = [vp, i32, P(i32)]
         L.orc_set_tape.argtypes = [vp, i32, P(C.c_uint32), i64]
         L.orc_tape_pos.restype = i64
         L.orc_tape_pos.argtypes = [vp, i32]
@@ -189,6 +191,14 @@ class Oracle:
         s, keep = abi.external_to_struct(ext, self.nrep)
         lib().orc_set_external(self.h, C.byref(s))
         del keep
+
+    def set_ss(self, pairs):
+        """Restrict base pairs to those of a CT/BPSEQ/BPL file, (n, 2)
+        1-based bead ids; None clears.  Returns the number kept."""
+        if pairs is None:
+            return lib().orc_set_ss(self.h, -1, None)
+        p = np.ascontiguousarray(pairs, np.int32).reshape(-1, 2)
+        return int(lib().orc_set_ss(self.h, len(p), p.ctypes.data_as(C.POINTER(C.c_int32))))
 
     def set_step(self, istep):
         lib().orc_set_step(self.h, int(istep))

@@ -511,6 +511,30 @@ contains
       istep = istep_in
    end subroutine sisref_set_external
 
+   ! Re-run init_bp with a base-pair list (BPL) file, as a run with
+   ! [Files.In] bpl does (src/init_bp.F90:177-181 -> src/read_ss.F90).
+   subroutine sisref_set_ss(path, plen) bind(C, name="sisref_set_ss")
+      use var_io, only : flg_in_bpl, cfile_bpl_in
+      use var_potential, only : bp_map, bp3_map
+      character(kind=c_char), intent(in) :: path(*)
+      integer(c_int), value :: plen
+      integer :: k
+      character(len=:), allocatable :: name
+      interface
+         subroutine init_bp()
+         end subroutine init_bp
+      end interface
+      allocate(character(len=plen) :: name)
+      do k = 1, plen
+         name(k:k) = path(k)
+      enddo
+      cfile_bpl_in = name
+      flg_in_bpl = .True.
+      if (allocated(bp_map)) deallocate(bp_map)
+      if (allocated(bp3_map)) deallocate(bp3_map)
+      call init_bp()
+   end subroutine sisref_set_ss
+
    subroutine sisref_ele_params(irep, out) bind(C, name="sisref_ele_params")
       use var_state, only : lambdaD, diele
       use var_potential, only : ele_coef, ele_cutoff

@@ -165,6 +165,13 @@ def _worker(q, so, sysd, ops, seed, nstep_exchange):
                     _ip(fp), _dp(fk), _dp(fi), _dp(fv), C.c_int(int(ext.get("rg", 0))),
                     C.c_double(float(ext.get("rg_k", 0.0))), _dp(rg0), C.c_int(nr), _ip(ri), _dp(rc), _dp(rp),
                     C.c_int64(int(op[2])))
+            elif kind == "ss":
+                # op[1]: (n, 2) 1-based bead pairs, written as a BPL file
+                bpl = os.path.join(tmp, "ss.bpl")
+                with open(bpl, "w") as fh:
+                    for a, b in np.asarray(op[1], int).reshape(-1, 2):
+                        fh.write(f"{a} {b}\n")
+                lib.sisref_set_ss(C.c_char_p(bpl.encode()), C.c_int(len(bpl)))
             elif kind == "write_rst":
                 pre = str(op[1]).encode()
                 lib.sisref_write_rst(C.c_char_p(pre), C.c_int(len(pre)))

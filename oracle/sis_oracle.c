@@ -75,6 +75,7 @@ struct orc_sys {
     int x_on;
     sis_external x;
     int32_t *x_dcf_pairs, *x_fr_pairs, *x_rest_id;
+    uint8_t *ss_map;          /* [nmp*nmp] pairs allowed by a CT/BPSEQ/BPL file, or NULL */
     double *x_rg0, *x_dcf_f, *x_fr_k, *x_fr_init, *x_fr_velo, *x_rest_rcut, *x_rest_para;
 };
 
@@ -200,11 +201,13 @@ static void init_bp_tables(orc_sys *s) {
 }
 
 /* Base-pair eligibility of beads imp < jmp (0-based) and its triplet index:
- * the body of the pair loop of src/init_bp.F90:184-313 for models 3/4/5
- * without secondary-structure files.  Returns 0 when not eligible. */
+ * the body of the pair loop of src/init_bp.F90:184-313 for models 3/4/5;
+ * with a secondary-structure file only the pairs it allows are considered
+ * (lines 206-209).  Returns 0 when not eligible. */
 void orc_bp_pair(orc_sys *s, int32_t imp, int32_t jmp, int32_t *bptype, int32_t *bp3) {
     *bptype = 0; *bp3 = 0;
     if (imp >= jmp) return;
+    if (s->ss_map && !s->ss_map[(size_t)imp * s->nmp + jmp]) return;
     int ich = s->chain_of[imp], jch = s->chain_of[jmp];
     int i = s->lmp[imp], j = s->lmp[jmp];                 /* 1-based local */
     int ni = s->nmp_chain[ich], nj = s->nmp_chain[jch];
@@ -891,6 +894,41 @@ static void free_external(orc_sys *s) {
 
 void orc_set_step(orc_sys *s, int64_t istep) { s->istep = istep; }
 
+/* The pairs of a CT/BPSEQ/BPL file (1-based bead ids, either order) as
+ * read_ss keeps them (src/read_ss.F90:126-205): ends of a chain, pairs
+ * closer than the minimum loop, isolated pairs (models 3-5) and
+ * non-canonical pairs are dropped.  npair < 0 clears the restriction.
+ * Returns the number of pairs kept. */
+int32_t orc_set_ss(orc_sys *s, int32_t npair, const int32_t *pairs) {
+    free(s->ss_map);
+    s->ss_map = NULL;
+    if (npair < 0) return 0;
+    int n = s->nmp, kept = 0;
+    s->ss_map = calloc((size_t)n * n, 1);
+    for (int k = 0; k < npair; k++) {
+        int imp = pairs[2 * k], jmp = pairs[2 * k + 1];
+        if (jmp == 0) continue;
+        if (jmp < imp) { int t = imp; imp = jmp; jmp = t; }
+        imp--; jmp--;
+        int ich = s->chain_of[imp], jch = s->chain_of[jmp];
+        int i = s->lmp[imp], j = s->lmp[jmp];
+        if (i == 1 || i == s->nmp_chain[ich]) continue;
+        if (j == 1 || j == s->nmp_chain[jch]) continue;
+        if (ich == jch && i + s->p.bp_min_loop >= j) continue;
+        const int *si = s->seq + s->chain_start[ich] - 1, *sj = s->seq + s->chain_start[jch] - 1;
+        int m = s->p.bp_model;
+        if ((m == 3 || m == 4 || m == 5) && !is_complement(si[i - 1], sj[j + 1]) && !is_complement(si[i + 1], sj[j - 1]))
+            continue;
+        int a = si[i], b = sj[j];
+        int canon = (a == G_ && b == C_) || (a == C_ && b == G_) || (a == A_ && b == U_) || (a == U_ && b == A_) ||
+                    (a == G_ && b == U_) || (a == U_ && b == G_);
+        if (!canon) continue;
+        s->ss_map[(size_t)imp * n + jmp] = s->ss_map[(size_t)jmp * n + imp] = 1;
+        kept++;
+    }
+    return kept;
+}
+
 void orc_set_external(orc_sys *s, const sis_external *e) {
     free_external(s);
     if (!e) return;
@@ -1150,6 +1188,7 @@ void orc_set_temps(orc_sys *s, const double *temps) {
 void orc_destroy(orc_sys *s) {
     if (!s) return;
     free_external(s);
+    free(s->ss_map);
     for (int r = 0; r < s->nrep; r++) {
         free(s->rep[r].wca.a); free(s->rep[r].ele.a); free(s->rep[r].bp.a); free(s->rep[r].bpcoef.a);
         free(s->rep[r].bp_status); free(s->rep[r].bp_status_mc); free(s->rep[r].ene_bp);

@@ -40,4 +40,5 @@ void orc_set_temps(orc_sys *s, const double *temps);
 int64_t orc_mc_iterations(void);
 void orc_set_external(orc_sys *s, const sis_external *ext);   /* NULL clears */
 void orc_set_step(orc_sys *s, int64_t istep);
+int32_t orc_set_ss(orc_sys *s, int32_t npair, const int32_t *pairs);   /* (2, npair) 1-based; npair < 0 clears */
 #endif

@@ -194,7 +194,7 @@ EXPORTED = [
     "sis_gpu_kinetic", "sis_gpu_out_open", "sis_gpu_out_close", "sis_out_write_file",
     "sis_gpu_dcd_energies", "sis_dcd_read_file", "sis_gpu_check_force",
     "sis_gpu_rst_open", "sis_gpu_rst_write", "sis_gpu_rst_close", "sis_gpu_rst_read",
-    "sis_rst_write_file", "sis_rst_read_file", "sis_gpu_set_external",
+    "sis_rst_write_file", "sis_rst_read_file", "sis_gpu_set_external", "sis_gpu_set_ss",
 ]
 
 
@@ -245,6 +245,7 @@ def _declare(lib: C.CDLL) -> None:
         "sis_bp_write_file": (i32, [C.c_char_p, i64, P(i32), P(C.c_uint8), P(d), i32, d, i32]),
         "sis_gpu_set_labels": (i32, [vp, P(i32)]),
         "sis_gpu_set_external": (i32, [vp, P(SisExternal)]),
+        "sis_gpu_set_ss": (i32, [vp, i32, P(i32)]),
         "sis_gpu_rst_open": (i32, [vp, P(C.c_char_p), i64]),
         "sis_gpu_rst_write": (i32, [vp]),
         "sis_gpu_rst_close": (i32, [vp]),

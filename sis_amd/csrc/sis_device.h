@@ -98,6 +98,9 @@ struct Ptrs {
   double *bm_cache;          // [2] stored flag, value (process-global cache)
   // energies
   double *ene_acc;           // [R*16]
+  // pairs allowed by a secondary-structure file, or null: per bead i the
+  // sorted partners j > i in ss_partner[ss_off[i] .. ss_off[i+1])
+  const int *ss_off, *ss_partner;
 };
 
 // ---------------------------------------------------------------------

@@ -85,6 +85,9 @@ struct sis_gpu_ctx {
   BpOut *bpo = nullptr;    // base-pair output (sis_gpu_bp_open), or none
   OutFile *outf = nullptr; // energy lines (sis_gpu_out_open), or none
   RstOut *rst = nullptr;   // restart files (sis_gpu_rst_open), or none
+  // secondary-structure restriction (sis_gpu_set_ss)
+  bool ss_on = false;
+  DevBuf<int> ss_off_d, ss_partner_d;
   // external terms (sis_gpu_set_external)
   bool ext_on = false;
   ExtDev X{};
@@ -205,6 +208,8 @@ struct sis_gpu_ctx {
     d.tape = tape_d.p; d.tape_off = tape_off_d.p; d.tape_pos = tape_pos_d.p;
     d.noise = noise_d.p; d.bm_cache = bm_cache_d.p;
     d.ene_acc = ene_out_d.p;
+    d.ss_off = ss_on ? ss_off_d.p : nullptr;
+    d.ss_partner = ss_on ? ss_partner_d.p : nullptr;
     return d;
   }
   ~sis_gpu_ctx() {
@@ -227,7 +232,7 @@ struct sis_gpu_ctx {
     DevBuf<double> *xb[] = {&x_rg0_d, &x_dcf_f_d, &x_fr_k_d, &x_fr_init_d, &x_fr_velo_d, &x_rest_rcut_d,
                             &x_rest_para_d, &x_rg_stat_d, &x_ene_d};
     for (auto *b : xb) b->free_();
-    DevBuf<int> *xi[] = {&x_term_off_d, &x_dcf_pairs_d, &x_fr_pairs_d, &x_rest_id_d, &x_err_d};
+    DevBuf<int> *xi[] = {&x_term_off_d, &x_dcf_pairs_d, &x_fr_pairs_d, &x_rest_id_d, &x_err_d, &ss_off_d, &ss_partner_d};
     for (auto *b : xi) b->free_();
     x_terms_d.free_();
     DevBuf<double4> *vb[] = {&pos_d, &pos2_d, &vel_d, &acc_d, &move_d, &frc_d, &fbond_d};
@@ -1462,6 +1467,58 @@ int sis_gpu_set_step(sis_gpu_ctx *c, int64_t istep) {
   if (!c) return fail(SIS_ERR_ARG, "null context");
   if (istep < 0) return fail(SIS_ERR_ARG, "istep < 0");
   c->istep = istep;
+  return SIS_OK;
+  API_END
+}
+
+int sis_gpu_set_ss(sis_gpu_ctx *c, int32_t npair, const int32_t *pairs) {
+  API_BEGIN
+  if (!c) return fail(SIS_ERR_ARG, "null context");
+  const int N = c->N;
+  if (npair < 0) {
+    c->ss_on = false;
+    c->lists_valid = false;
+    return SIS_OK;
+  }
+  if (npair > 0 && !pairs) return fail(SIS_ERR_ARG, "null pairs");
+  // the filters of read_ss (src/read_ss.F90:126-205) on the topology
+  auto sq = [&](int a) { return topo_seq(c->topo[a]); };
+  auto comp = [](int a, int b) {
+    return (a == SQ_A && b == SQ_U) || (a == SQ_U && (b == SQ_A || b == SQ_G)) ||
+           (a == SQ_G && (b == SQ_C || b == SQ_U)) || (a == SQ_C && b == SQ_G);
+  };
+  std::vector<std::vector<int>> part(N);
+  for (int k = 0; k < npair; k++) {
+    int imp = pairs[2 * k], jmp = pairs[2 * k + 1];
+    if (jmp == 0) continue;
+    if (imp < 1 || imp > N || jmp < 1 || jmp > N) return fail(SIS_ERR_ARG, "pair bead index out of range");
+    if (jmp < imp) std::swap(imp, jmp);
+    if (imp == jmp) continue;
+    const int a = imp - 1, b = jmp - 1;
+    const Topo &ta = c->topo[a], &tb = c->topo[b];
+    if (ta.loc == 1 || ta.loc == ta.len || tb.loc == 1 || tb.loc == tb.len) continue;   // chain ends
+    if (ta.start == tb.start && ta.loc + c->P.bp_min_loop >= tb.loc) continue;         // minimum loop
+    if (c->P.bp_model >= 3 && !comp(sq(a - 1), sq(b + 1)) && !comp(sq(a + 1), sq(b - 1))) continue;   // isolated
+    const int x = sq(a), y = sq(b);
+    const bool canon = (x == SQ_G && (y == SQ_C || y == SQ_U)) || (x == SQ_C && y == SQ_G) ||
+                       (x == SQ_A && y == SQ_U) || (x == SQ_U && (y == SQ_A || y == SQ_G));
+    if (!canon) continue;
+    part[a].push_back(b);
+  }
+  std::vector<int> off(N + 1, 0), flat;
+  for (int a = 0; a < N; a++) {
+    std::sort(part[a].begin(), part[a].end());
+    part[a].erase(std::unique(part[a].begin(), part[a].end()), part[a].end());
+    off[a] = (int)flat.size();
+    flat.insert(flat.end(), part[a].begin(), part[a].end());
+  }
+  off[N] = (int)flat.size();
+  c->ss_off_d.alloc(N + 1);
+  HIPCHK(hipMemcpy(c->ss_off_d.p, off.data(), (N + 1) * sizeof(int), hipMemcpyHostToDevice));
+  c->ss_partner_d.alloc(std::max<size_t>(flat.size(), 1));
+  if (!flat.empty()) HIPCHK(hipMemcpy(c->ss_partner_d.p, flat.data(), flat.size() * sizeof(int), hipMemcpyHostToDevice));
+  c->ss_on = true;
+  c->lists_valid = false;
   return SIS_OK;
   API_END
 }

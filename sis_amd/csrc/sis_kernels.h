@@ -25,8 +25,9 @@ __global__ void k_wrap(Params P, Ptrs D, const int *mask) {
 // ---------------------------------------------------------------------
 // Base-pair eligibility of replica-local beads i < j (0-based) and the
 // triplet-table index: the pair rules of src/init_bp.F90:184-313 (models
-// 3/4/5, no secondary-structure input).  Returns table index (0 = not
-// eligible); *bpt receives the BPT type.
+// 3/4/5); with a secondary-structure file only its pairs are considered
+// (lines 206-209).  Returns table index (0 = not eligible); *bpt receives
+// the BPT type.
 __device__ __forceinline__ int is_comp(int a, int b) {
   return (a == SQ_A && b == SQ_U) || (a == SQ_U && (b == SQ_A || b == SQ_G)) ||
          (a == SQ_G && (b == SQ_C || b == SQ_U)) || (a == SQ_C && b == SQ_G);
@@ -37,6 +38,15 @@ __device__ __forceinline__ int hkey(int w, int x, int u, int z, int y, int v) {
 }
 __device__ __forceinline__ int bp_eligible(const Params &P, const Ptrs &D, int i, int j, int *bpt) {
   *bpt = 0;
+  if (D.ss_off) {
+    const int a = i < j ? i : j, b = i < j ? j : i;
+    int lo = D.ss_off[a], hi = D.ss_off[a + 1];
+    while (lo < hi) {
+      const int m = (lo + hi) >> 1;
+      if (D.ss_partner[m] < b) lo = m + 1; else hi = m;
+    }
+    if (lo == D.ss_off[a + 1] || D.ss_partner[lo] != b) return 0;
+  }
   Topo ti = D.topo[i], tj = D.topo[j];
   int li = ti.loc, lj = tj.loc, ni = ti.len, nj = tj.len;
   bool ic = topo_circ(ti), jc = topo_circ(tj);

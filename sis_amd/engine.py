@@ -221,6 +221,15 @@ class Engine:
         _check(lib().sis_gpu_set_external(self.h, C.byref(s)))
         del keep
 
+    def set_ss(self, pairs):
+        """Restrict base pairs to a CT/BPSEQ/BPL file's pairs, (n, 2)
+        1-based bead ids; None lifts the restriction."""
+        if pairs is None:
+            _check(lib().sis_gpu_set_ss(self.h, -1, None))
+            return
+        p = np.ascontiguousarray(pairs, np.int32).reshape(-1, 2)
+        _check(lib().sis_gpu_set_ss(self.h, len(p), abi.iptr(p)))
+
     def rst_open(self, paths, nstep_save_rst: int = 0):
         """Restart files (src/write_rst.F90 blocks), one per replica."""
         arr = (C.c_char_p * self.nrep)(*[str(x).encode() for x in paths])

@@ -232,6 +232,14 @@ module sis_gpu_iface
          type(c_ptr), value :: ctx
       end function
 
+      ! secondary-structure restriction (read_ss pairs)
+      integer(c_int) function sis_gpu_set_ss(ctx, npair, pairs) bind(C, name="sis_gpu_set_ss")
+         import :: c_int, c_ptr, c_int32_t
+         type(c_ptr), value :: ctx
+         integer(c_int32_t), value :: npair
+         integer(c_int32_t), intent(in) :: pairs(2, *)
+      end function
+
       ! stage wall, tweezers, Rg bias, restraints
       integer(c_int) function sis_gpu_set_external(ctx, ext) bind(C, name="sis_gpu_set_external")
          import :: c_int, c_ptr, sis_external

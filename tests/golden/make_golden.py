@@ -105,6 +105,43 @@ def remd_external(nrep=2, seed=6):
     return sysd
 
 
+def ss_pairs_for(sysd, seed=3, keep=0.4):
+    """A base-pair list for a secondary-structure restriction: a random
+    share of the pairs init_bp would allow, plus pairs read_ss drops (chain
+    ends, too short a loop, non-canonical, repeated, reversed order)."""
+    import ctypes as C
+    o = corc.Oracle(sysd)
+    n = o.nmp
+    ok = []
+    for i in range(n):
+        for j in range(i + 1, n):
+            bt, b3 = C.c_int32(), C.c_int32()
+            corc.lib().orc_bp_pair(o.h, i, j, C.byref(bt), C.byref(b3))
+            if b3.value:
+                ok.append((i + 1, j + 1))
+    rng = np.random.default_rng(seed)
+    pick = [p for p in ok if rng.random() < keep]
+    junk = [(1, n // 2), (n // 2, n), (10, 11), (5, 5 + 3)] + [(b, a) for a, b in pick[:5]] + pick[:3]
+    seq = np.asarray(sysd["seq"])
+    for a in range(2, n - 1):
+        for b in range(a + 5, n):
+            if {int(seq[a - 1]), int(seq[b - 1])} in ({0, 0}, {2, 0}, {3, 1}):   # AA, GA, CU
+                junk.append((a, b))
+                break
+        if len(junk) > 16:
+            break
+    return np.array(pick + junk, np.int32)
+
+
+def hairpin_ss():
+    """hairpin64 restricted to a base-pair list (src/read_ss.F90 with a BPL
+    file, src/init_bp.F90:177-209)."""
+    sysd = configs.hairpin64(1)
+    sysd["ss"] = ss_pairs_for(sysd)
+    sysd["name"] = "hairpin_ss"
+    return sysd
+
+
 SYSTEMS = {
     "hairpin64": (lambda: configs.hairpin64(1), 60, 0),
     "small_pbc": (lambda: configs.small_pbc(4, 30, 100.0, seed=5), 60, 0),
@@ -113,6 +150,7 @@ SYSTEMS = {
     "remd4mc3": (remd_mc3, 40, 10),
     "remd4bd": (remd_bd, 40, 10),
     "remd2ext": (remd_external, 30, 10),
+    "hairpin_ss": (hairpin_ss, 60, 0),
 }
 
 
@@ -125,6 +163,8 @@ def make(name, builder, nstep, nex, seed=4321):
         ops.append(("tape", -1, 4096, "tape_rep"))
     ext = s.get("external")
     pre = [("external", ext, 0)] if ext else []
+    if s.get("ss") is not None:
+        pre.append(("ss", s["ss"]))
     ops = pre + ops + [("nl", r) for r in range(1, nrep + 1)]
     ops += [("lists", 1, "L"), ("force", 1, 1, "F"), ("bp_state", 1, "B"),
             ("energy", 1, float(s["temps"][0]), 0, 1, "E"), ("ele", 1, "ele"), ("bp_tables", "T")]
@@ -134,6 +174,8 @@ def make(name, builder, nstep, nex, seed=4321):
     o = corc.Oracle(s)
     if ext:
         o.set_external(ext)
+    if s.get("ss") is not None:
+        o.set_ss(s["ss"])
     for r in range(1, nrep + 1):
         o.set_tape(r, ref0[f"tape{r}"])
     if nrep > 1:
@@ -145,6 +187,8 @@ def make(name, builder, nstep, nex, seed=4321):
     o2 = corc.Oracle(s)
     if ext:
         o2.set_external(ext)
+    if s.get("ss") is not None:
+        o2.set_ss(s["ss"])
     for r in range(1, nrep + 1):
         o2.set_tape(r, ref0[f"tape{r}"])
     if nrep > 1:
@@ -172,6 +216,8 @@ def make(name, builder, nstep, nex, seed=4321):
     out["box"] = np.array(p["box"], float)
     for k, v in (ext or {}).items():
         out["ext_" + k] = np.asarray(v)
+    if s.get("ss") is not None:
+        out["ss_pairs"] = np.asarray(s["ss"], np.int32)
     np.savez_compressed(HERE / f"{name}.npz", **out)
     print(name, "tape words kept", keep, "lists", {k: len(ref0["L"][k]) for k in ("wca", "ele", "bp")})
 
@@ -197,6 +243,8 @@ def load(name):
         for k in ("stage_sigma", "stage_eps", "rg_k"):
             ext[k] = float(ext[k])
         sysd["external"] = ext
+    if "ss_pairs" in d:
+        sysd["ss"] = d["ss_pairs"]
     return sysd, d
 
 

@@ -20,7 +20,7 @@ from tests.golden.make_golden import load
 
 pytestmark = pytest.mark.gpu
 
-FIXTURES = ["hairpin64", "small_pbc", "mixed", "remd4", "remd4mc3", "remd4bd", "remd2ext"]
+FIXTURES = ["hairpin64", "small_pbc", "mixed", "remd4", "remd4mc3", "remd4bd", "remd2ext", "hairpin_ss"]
 
 
 def _sorted_rows(a):
@@ -35,6 +35,8 @@ def _engine(sysd, d=None, **kw):
     eng = Engine(sysd, parity=True, **kw)
     if "external" in sysd:
         eng.set_external(sysd["external"])
+    if "ss" in sysd:
+        eng.set_ss(sysd["ss"])
     if d is not None:
         for r in range(1, sysd["nrep"] + 1):
             eng.set_rng_tape(r, d["tapes"][r - 1])
@@ -283,6 +285,32 @@ def test_brownian_forms_match_phase_form(form):
         res.append((st, eng.labels()))
     assert np.array_equal(res[0][1], res[1][1])
     assert np.abs(res[0][0][0] - res[1][0][0]).max() < 1e-6
+
+
+def test_ss_restriction_batched_form():
+    """A secondary-structure restriction in the batched form (8 replicas of a
+    128-nt chain): the run follows the per-phase form, and every candidate
+    pair is one the list allows."""
+    from tests.golden.make_golden import ss_pairs_for
+    s = configs.remd_c3(nrep=8, n=128, seed=3)
+    pairs = ss_pairs_for(s, seed=2, keep=0.5)
+    allowed = {(min(a, b), max(a, b)) for a, b in pairs}
+    print("[case] ss restriction batched vs phases", flush=True)
+    res = []
+    for mode in ("phases", "batched"):
+        eng = Engine(s, parity=False, mode=mode, seed=19, exchange_every=10)
+        eng.set_ss(pairs)
+        eng.init_velocities()
+        eng.neighbor_list(0)
+        eng.step(60)
+        assert eng.info()["mode"] == (1 if mode == "phases" else 3)
+        for r in range(1, 9):
+            bp = eng.bp(r)["bp"]
+            assert len(bp) > 0 and all((int(i), int(j)) in allowed for i, j in bp[:, :2])
+        res.append((eng.get_state(), eng.labels()))
+    assert np.array_equal(res[0][1], res[1][1])
+    for a, b in zip(res[0][0], res[1][0]):
+        assert np.abs(a - b).max() < 1e-6
 
 
 def test_external_terms_in_performance_mode():

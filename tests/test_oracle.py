@@ -10,7 +10,7 @@ from sis_amd import configs
 from oracle import corc, refdrv
 from tests.golden.make_golden import load
 
-FIXTURES = ["hairpin64", "small_pbc", "mixed", "remd4", "remd4mc3", "remd4bd", "remd2ext"]
+FIXTURES = ["hairpin64", "small_pbc", "mixed", "remd4", "remd4mc3", "remd4bd", "remd2ext", "hairpin_ss"]
 
 
 def _oracle_from_fixture(name):
@@ -18,6 +18,8 @@ def _oracle_from_fixture(name):
     o = corc.Oracle(sysd)
     if "external" in sysd:
         o.set_external(sysd["external"])
+    if "ss" in sysd:
+        o.set_ss(sysd["ss"])
     for r in range(1, sysd["nrep"] + 1):
         o.set_tape(r, d["tapes"][r - 1])
     if sysd["nrep"] > 1:
@@ -258,6 +260,32 @@ def test_external_terms_are_energy_gradients():
             xm[0, a, c] -= h
             g = (eext(o1, xp) - eext(o1, xm)) / (2 * h)
             assert abs(f[a, c] + g) <= 1e-5 * max(1.0, abs(g)), (a, c, f[a, c], -g)
+
+
+@pytest.mark.ref
+def test_live_reference_ss_restriction():
+    """A base-pair list on two chains (inter- and intra-chain pairs, plus
+    pairs read_ss drops): the oracle's eligibility map equals the
+    reference's bp3_map after read_ss + init_bp (src/read_ss.F90,
+    src/init_bp.F90:177-313), and the BP candidate lists agree."""
+    import ctypes as C
+    from tests.golden.make_golden import mixed_chains, ss_pairs_for
+    s = mixed_chains()
+    pairs = ss_pairs_for(s, seed=5, keep=0.3)
+    ref = refdrv.run(s, [("ss", pairs), ("bp_maps", "M"), ("nl", 1), ("lists", 1, "L")])
+    o = corc.Oracle(s)
+    o.set_ss(pairs)
+    n = o.nmp
+    elig = np.zeros((n, n), np.int32)
+    for i in range(n):
+        for j in range(i + 1, n):
+            bt, b3 = C.c_int32(), C.c_int32()
+            corc.lib().orc_bp_pair(o.h, i, j, C.byref(bt), C.byref(b3))
+            elig[i, j] = b3.value
+    assert np.array_equal(elig, np.triu(ref["M"][1], 1))
+    assert 0 < np.count_nonzero(elig) < len(pairs)
+    o.neighbor_list(1)
+    assert np.array_equal(o.lists(1)["bp"], ref["L"]["bp"])
 
 
 @pytest.mark.ref
