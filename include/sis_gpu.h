@@ -205,6 +205,18 @@ typedef struct sis_external {
  * whatever mode it was created with. */
 int sis_gpu_set_external(sis_gpu_ctx *ctx, const sis_external *ext);
 
+/* Simulated annealing (read_anneal's schedule, src/read_anneal.F90;
+ * src/main.F90:90-97, src/job_md.F90:207-230,600-612): entry k sets the
+ * temperature temps[k] from step steps[k] on (steps increasing; the first
+ * entry is the starting temperature).  As in the reference, the noise
+ * amplitude, the base-pair dG and the energies follow the schedule, the
+ * electrostatics stay at the first entry's temperature and the force's
+ * limit MC keeps the kT of sis_params.tempK.  Call it after
+ * sis_gpu_set_state / sis_gpu_rst_read and before sis_gpu_init_velocities;
+ * n <= 0 turns it off.  Not with temperature exchange (nrep > 1 and
+ * exchange_every > 0). */
+int sis_gpu_set_anneal(sis_gpu_ctx *ctx, int32_t n, const int64_t *steps, const double *temps);
+
 /* Secondary-structure restriction (read_ss, src/read_ss.F90, applied by
  * init_bp, src/init_bp.F90:177-209): only the listed pairs (npair pairs of
  * 1-based bead ids, either order, as a CT, BPSEQ or BPL file gives them)

@@ -46,6 +46,7 @@ def lib():
         L.orc_md_coef.argtypes = [vp, i32, P(d)]
         L.orc_set_external.argtypes = [vp, P(abi.SisExternal)]
         L.orc_set_step.argtypes = [vp, i64]
+        L.orc_set_anneal.argtypes = [vp, i32, P(i64), P(d)]
         L.orc_set_ss.restype = i32
         L.orc_set_ss.argtypes = [vp, i32, P(i32)]
         L.orc_set_tape.argtypes = [vp, i32, P(C.c_uint32), i64]
@@ -191,6 +192,11 @@ class Oracle:
         s, keep = abi.external_to_struct(ext, self.nrep)
         lib().orc_set_external(self.h, C.byref(s))
         del keep
+
+    def set_anneal(self, steps, temps):
+        st = np.ascontiguousarray(steps, np.int64)
+        tk = np.ascontiguousarray(temps, np.float64)
+        lib().orc_set_anneal(self.h, len(st), st.ctypes.data_as(C.POINTER(C.c_int64)), _d(tk))
 
     def set_ss(self, pairs):
         """Restrict base pairs to those of a CT/BPSEQ/BPL file, (n, 2)

@@ -535,6 +535,39 @@ contains
       call init_bp()
    end subroutine sisref_set_ss
 
+   ! An annealing schedule as read_anneal leaves it, with what main does
+   ! next (src/main.F90:90-97): tempK = the first entry, and init_ele's
+   ! set_ele at that temperature; kT keeps the input temperature's value.
+   subroutine sisref_set_anneal(n, steps, temps) bind(C, name="sisref_set_anneal")
+      use var_state, only : opt_anneal, nanneal, anneal_step, anneal_tempK, tempK, ionic_strength
+      use var_replica, only : nrep_proc
+      integer(c_int), value :: n
+      integer(c_int64_t), intent(in) :: steps(n)
+      real(c_double), intent(in) :: temps(n)
+      integer :: irep
+      interface
+         subroutine set_ele(irep, tempk, ionstr, out_lb, out_Zp)
+            use const, only : PREC
+            integer, intent(in) :: irep
+            real(PREC), intent(in) :: tempk
+            real(PREC), intent(in) :: ionstr
+            real(PREC), intent(out), optional :: out_lb
+            real(PREC), intent(out), optional :: out_Zp
+         end subroutine set_ele
+      end interface
+      opt_anneal = 1
+      nanneal = n
+      if (allocated(anneal_step)) deallocate(anneal_step)
+      if (allocated(anneal_tempK)) deallocate(anneal_tempK)
+      allocate(anneal_step(n), anneal_tempK(n))
+      anneal_step(:) = steps(:)
+      anneal_tempK(:) = temps(:)
+      tempK = anneal_tempK(1)
+      do irep = 1, nrep_proc
+         call set_ele(irep, tempK, ionic_strength)
+      enddo
+   end subroutine sisref_set_anneal
+
    subroutine sisref_ele_params(irep, out) bind(C, name="sisref_ele_params")
       use var_state, only : lambdaD, diele
       use var_potential, only : ele_coef, ele_cutoff

@@ -165,6 +165,10 @@ def _worker(q, so, sysd, ops, seed, nstep_exchange):
                     _ip(fp), _dp(fk), _dp(fi), _dp(fv), C.c_int(int(ext.get("rg", 0))),
                     C.c_double(float(ext.get("rg_k", 0.0))), _dp(rg0), C.c_int(nr), _ip(ri), _dp(rc), _dp(rp),
                     C.c_int64(int(op[2])))
+            elif kind == "anneal":
+                st = np.ascontiguousarray(op[1], np.int64)
+                tk = np.ascontiguousarray(op[2], np.float64)
+                lib.sisref_set_anneal(C.c_int(len(st)), st.ctypes.data_as(C.POINTER(C.c_int64)), _dp(tk))
             elif kind == "ss":
                 # op[1]: (n, 2) 1-based bead pairs, written as a BPL file
                 bpl = os.path.join(tmp, "ss.bpl")

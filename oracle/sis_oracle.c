@@ -76,6 +76,9 @@ struct orc_sys {
     sis_external x;
     int32_t *x_dcf_pairs, *x_fr_pairs, *x_rest_id;
     uint8_t *ss_map;          /* [nmp*nmp] pairs allowed by a CT/BPSEQ/BPL file, or NULL */
+    /* annealing: schedule, current entry (1-based), next switch step, and
+     * the kT of the force's limit MC (var_state's kT, never updated) */
+    int an_n, ianneal; int64_t *an_step; double *an_T; int64_t an_next; double mc_kT;
     double *x_rg0, *x_dcf_f, *x_fr_k, *x_fr_init, *x_fr_velo, *x_rest_rcut, *x_rest_para;
 };
 
@@ -725,7 +728,7 @@ static void force_bp(orc_sys *s, int irep, int bp_mc, const double *x, double *f
      * src/force_bp_triplet.F90 (no MC, skips candidates not in bp_status_MC) */
     rep_t *r = &s->rep[irep];
     int64_t nbp = r->bp.n / 7;
-    double tK = r->tK, beta = 1.0 / (BOLTZ_KCAL_MOL * tK);
+    double tK = r->tK, beta = s->mc_kT > 0.0 ? 1.0 / s->mc_kT : 1.0 / (BOLTZ_KCAL_MOL * tK);
     double (*fbp)[6][3] = calloc(nbp + 1, sizeof(double[6][3]));
     int *nt_excess = malloc(sizeof(int) * s->nmp);
     for (int i = 0; i < s->nmp; i++) nt_excess[i] = -s->p.max_bp_per_nt;
@@ -899,6 +902,25 @@ void orc_set_step(orc_sys *s, int64_t istep) { s->istep = istep; }
  * closer than the minimum loop, isolated pairs (models 3-5) and
  * non-canonical pairs are dropped.  npair < 0 clears the restriction.
  * Returns the number of pairs kept. */
+/* Annealing (src/main.F90:90-97, src/job_md.F90:207-230,600-612): the
+ * electrostatics are set once at the first temperature, the force's MC
+ * keeps the input kT, and the replicas' temperature follows the schedule. */
+void orc_set_anneal(orc_sys *s, int32_t n, const int64_t *steps, const double *temps) {
+    free(s->an_step); free(s->an_T);
+    s->an_step = NULL; s->an_T = NULL; s->an_n = 0; s->an_next = 0; s->mc_kT = 0.0;
+    if (n <= 0) return;
+    s->an_n = n;
+    s->an_step = malloc(sizeof(int64_t) * n); memcpy(s->an_step, steps, sizeof(int64_t) * n);
+    s->an_T = malloc(sizeof(double) * n); memcpy(s->an_T, temps, sizeof(double) * n);
+    s->ianneal = s->istep >= steps[n - 1] ? n : 1;
+    s->an_next = s->ianneal < n ? steps[s->ianneal] : 0;
+    for (int r = 0; r < s->nrep; r++) {
+        set_ele(s, r, temps[0], s->p.ionic_strength);
+        s->rep[r].tK = temps[s->ianneal - 1];
+    }
+    s->mc_kT = BOLTZ_KCAL_MOL * s->p.tempK;
+}
+
 int32_t orc_set_ss(orc_sys *s, int32_t npair, const int32_t *pairs) {
     free(s->ss_map);
     s->ss_map = NULL;
@@ -1189,6 +1211,7 @@ void orc_destroy(orc_sys *s) {
     if (!s) return;
     free_external(s);
     free(s->ss_map);
+    free(s->an_step); free(s->an_T);
     for (int r = 0; r < s->nrep; r++) {
         free(s->rep[r].wca.a); free(s->rep[r].ele.a); free(s->rep[r].bp.a); free(s->rep[r].bpcoef.a);
         free(s->rep[r].bp_status); free(s->rep[r].bp_status_mc); free(s->rep[r].ene_bp);
@@ -1418,6 +1441,11 @@ int64_t orc_md(orc_sys *s, int64_t nsteps, int32_t nstep_exchange, double *energ
         if (nr > 1 && nstep_exchange > 0 && s->istep % nstep_exchange == 0) {
             exchange(s, energies_out);
             last_exchange = 1;
+        }
+        if (s->an_n > 0 && s->an_next > 0 && s->istep + 1 == s->an_next) {
+            s->ianneal++;
+            for (int r = 0; r < nr; r++) s->rep[r].tK = s->an_T[s->ianneal - 1];
+            s->an_next = s->ianneal < s->an_n ? s->an_step[s->ianneal] : 0;
         }
     }
     /* energies of the last step as written at a save step: the BP state of

@@ -37,11 +37,21 @@ struct Params {
   double wca_sigma, wca_eps, wca_sig2, wca_coef;
   double bp_cutoff_energy, bp_cutoff_dist, nl_margin;
   double wca_nl_cut2, bp_nl_cut2;
+  double mc_kT;   // > 0: kT of the limit-MC acceptance when it is not the replica's (annealing)
   int angl_reb, dih_type, ele_on, ele_excl_bond, ele_excl_angl;
   int bp_model, bp_min_loop, max_bp_per_nt, nstep_bp_mc, pbc;
   int N, R, fix_h4, nchains, maxlen;
   int bd;  // 1: Ermak-McCammon Brownian update in place of GJF-2GJ
 };
+
+// 1/kT of the limit-MC acceptance.  The force's MC takes var_state's kT,
+// which an annealed run never updates (src/force_bp_limit_triplet.F90:61-67);
+// energy_sumup's MC takes the temperature it is given
+// (src/energy_bp_limit_triplet.F90:37-39).
+__device__ __forceinline__ double mc_beta(const Params &P, double tK, bool force_side) {
+  const double BOLTZ = 1.380649e-23 * ((1.0 / 4184.0) * 6.02214076e23);
+  return (force_side && P.mc_kT > 0.0) ? 1.0 / P.mc_kT : 1.0 / (BOLTZ * tK);
+}
 
 // Per-bead topology word: chain start (replica-local 0-based), chain
 // length, local index (1-based), packed flags.

@@ -85,6 +85,14 @@ struct sis_gpu_ctx {
   BpOut *bpo = nullptr;    // base-pair output (sis_gpu_bp_open), or none
   OutFile *outf = nullptr; // energy lines (sis_gpu_out_open), or none
   RstOut *rst = nullptr;   // restart files (sis_gpu_rst_open), or none
+  // annealing schedule (sis_gpu_set_anneal): temperatures from step
+  // an_step[k]; ele_T holds the temperature the electrostatics were set at
+  bool anneal_on = false;
+  std::vector<int64_t> an_step;
+  std::vector<double> an_T, ele_T;
+  int ianneal = 0;            // 1-based index of the current schedule entry
+  int64_t an_next = 0;        // step whose start takes the next entry, 0 = none
+  int rst_ianneal = 0;        // ANNEAL block of a restart file read, or 0
   // secondary-structure restriction (sis_gpu_set_ss)
   bool ss_on = false;
   DevBuf<int> ss_off_d, ss_partner_d;
@@ -414,7 +422,7 @@ void setup_cells(sis_gpu_ctx *c) {
   double rcut = std::max({c->P.wca_sigma, c->P.bp_cutoff_dist, 0.0}) + c->P.nl_margin;
   for (int r = 0; r < c->R; r++) {
     double ec = 0.0, lam, coef;
-    host_set_ele(c->prm, c->tK[r], c->ionic[r], coef, lam, ec);
+    host_set_ele(c->prm, c->anneal_on ? c->ele_T[r] : c->tK[r], c->ionic[r], coef, lam, ec);
     if (c->prm.ele_on) rcut = std::max(rcut, ec + c->P.nl_margin);
   }
   int nc[3];
@@ -438,7 +446,7 @@ void upload_conditions(sis_gpu_ctx *c) {
   int R = c->R, N = c->N;
   std::vector<double> coef(R), lam(R), cut(R), cr((size_t)R * N), box(3 * R);
   for (int r = 0; r < R; r++) {
-    host_set_ele(c->prm, c->tK[r], c->ionic[r], coef[r], lam[r], cut[r]);
+    host_set_ele(c->prm, c->anneal_on ? c->ele_T[r] : c->tK[r], c->ionic[r], coef[r], lam[r], cut[r]);
     for (int i = 0; i < N; i++) {
       if (c->P.bd) {  // src/job_md.F90:755-761
         cr[(size_t)r * N + i] = std::sqrt(2.0 * KB_KCAL * c->tK[r] * c->prm.dt / c->fric[i]);
@@ -655,7 +663,9 @@ void energy_phase(sis_gpu_ctx *c, const std::vector<int> &mask, const std::vecto
   std::vector<double> coef(R), lam(R);
   for (int r = 0; r < R; r++) {
     double cut;
-    host_set_ele(c->prm, T[r], c->ionic[r], coef[r], lam[r], cut);
+    // an annealed run keeps the electrostatics of its first temperature
+    // (set_ele runs once, src/main.F90:90-97 before init_ele)
+    host_set_ele(c->prm, c->anneal_on ? c->ele_T[r] : T[r], c->ionic[r], coef[r], lam[r], cut);
   }
   HIPCHK(hipMemcpyAsync(c->ecT_d.p, T.data(), R * sizeof(double), hipMemcpyHostToDevice, c->stream));
   HIPCHK(hipMemcpyAsync(c->eCoef_d.p, coef.data(), R * sizeof(double), hipMemcpyHostToDevice, c->stream));
@@ -1407,12 +1417,16 @@ int sis_gpu_step(sis_gpu_ctx *c, int64_t nsteps, int64_t *rebuilds) {
   const int64_t so = (c->outf && c->outf->nstep_save > 0) ? c->outf->nstep_save : 0;
   const int64_t rs = (c->rst && c->rst->nstep_save > 0) ? c->rst->nstep_save : 0;
   for (int64_t left = nsteps; left > 0;) {
+    // the next annealing entry is taken at the end of step an_next - 1
+    // (src/job_md.F90:600-612)
+    const bool an = c->anneal_on && c->an_next > 0 && c->an_next - 1 > c->istep;
     int64_t k = left;
     if (ex) k = std::min(k, ex - c->istep % ex);
     if (sv) k = std::min(k, sv - c->istep % sv);
     if (sb) k = std::min(k, sb - c->istep % sb);
     if (so) k = std::min(k, so - c->istep % so);
     if (rs) k = std::min(k, rs - c->istep % rs);
+    if (an) k = std::min(k, c->an_next - 1 - c->istep);
     run(k);
     left -= k;
     const bool exch = ex && c->istep % ex == 0;
@@ -1438,6 +1452,13 @@ int sis_gpu_step(sis_gpu_ctx *c, int64_t nsteps, int64_t *rebuilds) {
       out_lines(c, e16.data(), ekin.data(), lab, temps);
     } else if (exch) {
       do_exchange(c, nullptr, nullptr);
+    }
+    if (an && c->istep + 1 == c->an_next) {
+      const int n = (int)c->an_T.size();
+      c->ianneal++;
+      std::fill(c->tK.begin(), c->tK.end(), c->an_T[c->ianneal - 1]);
+      c->an_next = c->ianneal < n ? c->an_step[c->ianneal] : 0;
+      upload_conditions(c);
     }
     if (rs && c->istep % rs == 0) rst_write_all(c, c->rst->paths);
   }
@@ -1467,6 +1488,46 @@ int sis_gpu_set_step(sis_gpu_ctx *c, int64_t istep) {
   if (!c) return fail(SIS_ERR_ARG, "null context");
   if (istep < 0) return fail(SIS_ERR_ARG, "istep < 0");
   c->istep = istep;
+  return SIS_OK;
+  API_END
+}
+
+int sis_gpu_set_anneal(sis_gpu_ctx *c, int32_t n, const int64_t *steps, const double *temps) {
+  API_BEGIN
+  if (!c) return fail(SIS_ERR_ARG, "null context");
+  if (n <= 0) {
+    if (c->anneal_on) {
+      c->anneal_on = false;
+      c->P.mc_kT = 0.0;
+      upload_conditions(c);
+    }
+    return SIS_OK;
+  }
+  if (!steps || !temps) return fail(SIS_ERR_ARG, "null schedule");
+  for (int k = 0; k < n; k++) {
+    if (!(temps[k] > 0.0)) return fail(SIS_ERR_ARG, "annealing temperature must be positive");
+    if (k > 0 && steps[k] <= steps[k - 1]) return fail(SIS_ERR_ARG, "annealing steps must increase");
+  }
+  if (c->R > 1 && c->opt.exchange_every > 0)
+    return fail(SIS_ERR_UNSUPPORTED, "annealing with temperature exchange");
+  c->an_step.assign(steps, steps + n);
+  c->an_T.assign(temps, temps + n);
+  // job_md's set-up (src/job_md.F90:207-230): the entry of the current step
+  // (from the restart file's ANNEAL block on a restarted run), the first
+  // entry's temperature for the electrostatics, and the force's MC at the
+  // input temperature's kT
+  int ia = 1;
+  if (c->istep >= c->an_step[n - 1]) ia = n;
+  else if (c->rst_ianneal > 0) ia = std::min(c->rst_ianneal, n);
+  c->ianneal = ia;
+  c->an_next = ia < n ? c->an_step[ia] : 0;
+  c->ele_T.assign(c->R, c->an_T[0]);
+  std::fill(c->tK.begin(), c->tK.end(), c->an_T[ia - 1]);
+  c->P.mc_kT = KB_KCAL * c->prm.tempK;
+  c->anneal_on = true;
+  upload_conditions(c);
+  setup_cells(c);
+  c->lists_valid = false;
   return SIS_OK;
   API_END
 }

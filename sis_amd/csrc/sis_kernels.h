@@ -438,8 +438,7 @@ __global__ void k_mc_serial(Params P, Ptrs D, const int *mc, const double *ecT, 
   if (!mc[r]) return;
   const int N = P.N;
   double tK = ecT ? ecT[r] : D.tK[r];
-  const double BOLTZ = 1.380649e-23 * ((1.0 / 4184.0) * 6.02214076e23);
-  double beta = 1.0 / (BOLTZ * tK);
+  double beta = mc_beta(P, tK, ecT == nullptr);
   int b0 = D.bp_off[(size_t)r * N], nb = D.bp_off[(size_t)(r + 1) * N] - b0;
   const uint32_t *w = D.tape + D.tape_off[r];
   int64_t pos = D.tape_pos[r], lim = D.tape_off[r + 1] - D.tape_off[r];
@@ -560,8 +559,7 @@ __global__ void k_mc_comp(Params P, Ptrs D, const int *exc_off, const int *cnt_e
     E[y + 1] = v;
   }
   double tK = ecT ? ecT[r] : D.tK[r];
-  const double BOLTZ = 1.380649e-23 * ((1.0 / 4184.0) * 6.02214076e23);
-  double beta = 1.0 / (BOLTZ * tK);
+  double beta = mc_beta(P, tK, ecT == nullptr);
   int *cnt = D.nt_cnt + (size_t)r * N;
   uint32_t k0 = (uint32_t)seed, k1 = (uint32_t)(seed >> 32) ^ (0x5bd1e995u * (uint32_t)(r + 1));
   uint32_t draw = 0;
@@ -896,8 +894,7 @@ __global__ void __launch_bounds__(64) k_mc_huge(Params P, Ptrs D, const int *seg
   if (!huge[r]) return;
   __shared__ int sseq[64];
   const double tK = ecT ? ecT[r] : D.tK[r];
-  const double BOLTZ = 1.380649e-23 * ((1.0 / 4184.0) * 6.02214076e23);
-  const double beta = 1.0 / (BOLTZ * tK);
+  const double beta = mc_beta(P, tK, ecT == nullptr);
   const int g0 = D.bp_off[(size_t)r * N], g1 = D.bp_off[(size_t)(r + 1) * N];
   int *scr = seg + (size_t)r * N * SEGCAP;
   for (int u0 = 0; u0 < N; u0 += 64) {
@@ -926,8 +923,7 @@ __global__ void __launch_bounds__(64) k_mc_wave(Params P, Ptrs D, int *seg_n, in
   if (!has_exc[gr]) return;
   int n = seg_n[gr];
   double tK = ecT ? ecT[r] : D.tK[r];
-  const double BOLTZ = 1.380649e-23 * ((1.0 / 4184.0) * 6.02214076e23);
-  double beta = 1.0 / (BOLTZ * tK);
+  double beta = mc_beta(P, tK, ecT == nullptr);
   if (n > SEGCAP) {   // above the bucket capacity: k_mc_huge takes it
     if (threadIdx.x == 0) huge[r] = 1;
     return;

@@ -808,6 +808,7 @@ struct RstRec {
   const uint32_t *mts = nullptr;          // [MTS_WORDS] or none
   const double *box = nullptr;            // [3] or none (no PBC block)
   const uint64_t *seed = nullptr;         // or none
+  int32_t ianneal = 0;                    // ANNEAL block (current schedule entry), 0 = none
 };
 
 struct ByteSink {
@@ -832,6 +833,11 @@ std::vector<char> rst_bytes(const RstRec &r) {
   s.put<int32_t>(4 + 8);
   s.put<int32_t>(1);          // istep_sim: reserved, always 1
   s.put<int64_t>(r.istep);
+  if (r.ianneal > 0) {
+    s.put<int32_t>(RST_ANNEAL);
+    s.put<int32_t>(4);
+    s.put<int32_t>(r.ianneal);
+  }
   const double *arr[3] = {r.xyz, r.vel, r.acc};
   const int32_t ids[3] = {RST_XYZ, RST_VELO, RST_ACCEL};
   for (int k = 0; k < 3; k++) {
@@ -875,7 +881,7 @@ void write_whole(const std::string &path, const std::vector<char> &b) {
 // Parsed blocks of one restart file.
 struct RstIn {
   int32_t found = 0;      // bit (id - 1) set for every block seen
-  int32_t grep = 0, nrep = 0;
+  int32_t grep = 0, nrep = 0, ianneal = 0;
   int64_t istep = 0;
   uint64_t seed = 0;
   std::vector<int32_t> rep2lab;
@@ -926,6 +932,8 @@ RstIn rst_parse(const std::string &path, int32_t nmp) {
         if (i < 1 || i > o.nrep) throw ApiError{SIS_ERR_ARG, "bad REPLICA block in " + path};
         o.rep2lab[i - 1] = m;
       }
+    } else if (id == RST_ANNEAL && size == 4) {
+      std::memcpy(&o.ianneal, b.data() + p, 4);
     } else if (id == RST_PBC && size == 24) {
       std::memcpy(o.box, b.data() + p, 24);
     } else if (id == RST_SEED && size == 8) {
@@ -961,6 +969,8 @@ void rst_write_all(sis_gpu_ctx *c, const std::vector<std::string> &paths) {
     q.acc = a.data() + (size_t)3 * c->N * r;
     q.box = c->prm.pbc ? c->prm.box : nullptr;
     q.seed = c->parity ? nullptr : &seed;   // the parity form's MT state belongs to the caller
+    // written while a schedule entry remains (opt_anneal > 0, src/write_rst.F90:62-68)
+    q.ianneal = (c->anneal_on && c->an_next > 0) ? c->ianneal : 0;
     write_whole(paths[r], rst_bytes(q));
   }
 }
@@ -1060,6 +1070,7 @@ int sis_gpu_rst_read(sis_gpu_ctx *c, const char *const *paths) {
     std::copy(o.vel.begin(), o.vel.end(), v.begin() + n3 * (o.grep - 1));
     std::copy(o.acc.begin(), o.acc.end(), a.begin() + n3 * (o.grep - 1));
     if (o.found & (1 << (RST_STEP - 1))) istep = o.istep;
+    if (o.found & (1 << (RST_ANNEAL - 1))) c->rst_ianneal = o.ianneal;
     if (!o.rep2lab.empty()) {
       if (o.nrep != R) return fail(SIS_ERR_ARG, "nrep is not consistent in the restart file");
       lab = o.rep2lab;

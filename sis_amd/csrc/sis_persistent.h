@@ -91,8 +91,7 @@ __global__ void __launch_bounds__(B) k_persistent(Params P, Ptrs D, PersArgs A) 
   const unsigned G = gridDim.x;
   const int gtid = blockIdx.x * B + threadIdx.x, gthreads = G * B;
   unsigned long long target = 0;
-  const double BOLTZ = 1.380649e-23 * ((1.0 / 4184.0) * 6.02214076e23);
-  const double beta = 1.0 / (BOLTZ * D.tK[0]);
+  const double beta = mc_beta(P, D.tK[0], true);
   double4 *pos_a = D.pos, *pos_b = A.pos_b;
   int64_t completed = 0;
   int rebuild = 0;
@@ -536,8 +535,7 @@ __device__ __forceinline__ void batched_body(const Params &P, const Ptrs &D, con
   const int tid = threadIdx.x;
   const size_t base = (size_t)r * N;
   const int lpb = J.lpb, grp = tid / lpb, l = tid % lpb, ngrp = B / lpb;
-  const double BOLTZ = 1.380649e-23 * ((1.0 / 4184.0) * 6.02214076e23);
-  const double beta = 1.0 / (BOLTZ * D.tK[r]);
+  const double beta = mc_beta(P, D.tK[r], true);
   int *comp = D.comp + base;
   // the step reads positions from an LDS copy when they fit (refreshed
   // after every update and every rebuild, which may wrap them)

@@ -221,6 +221,13 @@ class Engine:
         _check(lib().sis_gpu_set_external(self.h, C.byref(s)))
         del keep
 
+    def set_anneal(self, steps, temps):
+        """Annealing schedule (read_anneal): temps[k] from step steps[k];
+        empty turns it off."""
+        st = np.ascontiguousarray(steps, np.int64)
+        tk = np.ascontiguousarray(temps, np.float64)
+        _check(lib().sis_gpu_set_anneal(self.h, len(st), st.ctypes.data_as(C.POINTER(C.c_int64)), abi.dptr(tk)))
+
     def set_ss(self, pairs):
         """Restrict base pairs to a CT/BPSEQ/BPL file's pairs, (n, 2)
         1-based bead ids; None lifts the restriction."""

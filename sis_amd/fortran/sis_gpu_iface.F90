@@ -232,6 +232,15 @@ module sis_gpu_iface
          type(c_ptr), value :: ctx
       end function
 
+      ! annealing schedule (read_anneal)
+      integer(c_int) function sis_gpu_set_anneal(ctx, n, steps, temps) bind(C, name="sis_gpu_set_anneal")
+         import :: c_int, c_ptr, c_int32_t, c_int64_t, c_double
+         type(c_ptr), value :: ctx
+         integer(c_int32_t), value :: n
+         integer(c_int64_t), intent(in) :: steps(*)
+         real(c_double), intent(in) :: temps(*)
+      end function
+
       ! secondary-structure restriction (read_ss pairs)
       integer(c_int) function sis_gpu_set_ss(ctx, npair, pairs) bind(C, name="sis_gpu_set_ss")
          import :: c_int, c_ptr, c_int32_t

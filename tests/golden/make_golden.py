@@ -142,6 +142,20 @@ def hairpin_ss():
     return sysd
 
 
+def hairpin_anneal():
+    """hairpin64 annealed 300 -> 290 -> 280 K (steps 0, 100, 200) with the
+    input temperature at 900 K: the force's limit MC keeps the input kT
+    (src/force_bp_limit_triplet.F90:61-67, kT set once by read_input), the
+    electrostatics the first entry's (src/main.F90:90-97); over 300 steps
+    the trajectory depends on both."""
+    sysd = configs.hairpin64(1)
+    sysd["params"] = dict(sysd["params"], tempK=900.0)
+    sysd["temps"] = np.array([900.0])
+    sysd["anneal"] = (np.array([0, 100, 200], np.int64), np.array([300.0, 290.0, 280.0]))
+    sysd["name"] = "hairpin_anneal"
+    return sysd
+
+
 SYSTEMS = {
     "hairpin64": (lambda: configs.hairpin64(1), 60, 0),
     "small_pbc": (lambda: configs.small_pbc(4, 30, 100.0, seed=5), 60, 0),
@@ -151,6 +165,7 @@ SYSTEMS = {
     "remd4bd": (remd_bd, 40, 10),
     "remd2ext": (remd_external, 30, 10),
     "hairpin_ss": (hairpin_ss, 60, 0),
+    "hairpin_anneal": (hairpin_anneal, 300, 0),
 }
 
 
@@ -165,6 +180,8 @@ def make(name, builder, nstep, nex, seed=4321):
     pre = [("external", ext, 0)] if ext else []
     if s.get("ss") is not None:
         pre.append(("ss", s["ss"]))
+    if s.get("anneal") is not None:
+        pre.append(("anneal", s["anneal"][0], s["anneal"][1]))
     ops = pre + ops + [("nl", r) for r in range(1, nrep + 1)]
     ops += [("lists", 1, "L"), ("force", 1, 1, "F"), ("bp_state", 1, "B"),
             ("energy", 1, float(s["temps"][0]), 0, 1, "E"), ("ele", 1, "ele"), ("bp_tables", "T")]
@@ -176,6 +193,8 @@ def make(name, builder, nstep, nex, seed=4321):
         o.set_external(ext)
     if s.get("ss") is not None:
         o.set_ss(s["ss"])
+    if s.get("anneal") is not None:
+        o.set_anneal(*s["anneal"])
     for r in range(1, nrep + 1):
         o.set_tape(r, ref0[f"tape{r}"])
     if nrep > 1:
@@ -189,6 +208,8 @@ def make(name, builder, nstep, nex, seed=4321):
         o2.set_external(ext)
     if s.get("ss") is not None:
         o2.set_ss(s["ss"])
+    if s.get("anneal") is not None:
+        o2.set_anneal(*s["anneal"])
     for r in range(1, nrep + 1):
         o2.set_tape(r, ref0[f"tape{r}"])
     if nrep > 1:
@@ -218,6 +239,9 @@ def make(name, builder, nstep, nex, seed=4321):
         out["ext_" + k] = np.asarray(v)
     if s.get("ss") is not None:
         out["ss_pairs"] = np.asarray(s["ss"], np.int32)
+    if s.get("anneal") is not None:
+        out["anneal_steps"] = np.asarray(s["anneal"][0], np.int64)
+        out["anneal_temps"] = np.asarray(s["anneal"][1], np.float64)
     np.savez_compressed(HERE / f"{name}.npz", **out)
     print(name, "tape words kept", keep, "lists", {k: len(ref0["L"][k]) for k in ("wca", "ele", "bp")})
 
@@ -245,6 +269,8 @@ def load(name):
         sysd["external"] = ext
     if "ss_pairs" in d:
         sysd["ss"] = d["ss_pairs"]
+    if "anneal_steps" in d:
+        sysd["anneal"] = (d["anneal_steps"], d["anneal_temps"])
     return sysd, d
 
 

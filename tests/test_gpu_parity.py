@@ -20,7 +20,7 @@ from tests.golden.make_golden import load
 
 pytestmark = pytest.mark.gpu
 
-FIXTURES = ["hairpin64", "small_pbc", "mixed", "remd4", "remd4mc3", "remd4bd", "remd2ext", "hairpin_ss"]
+FIXTURES = ["hairpin64", "small_pbc", "mixed", "remd4", "remd4mc3", "remd4bd", "remd2ext", "hairpin_ss", "hairpin_anneal"]
 
 
 def _sorted_rows(a):
@@ -37,6 +37,8 @@ def _engine(sysd, d=None, **kw):
         eng.set_external(sysd["external"])
     if "ss" in sysd:
         eng.set_ss(sysd["ss"])
+    if "anneal" in sysd:
+        eng.set_anneal(*sysd["anneal"])
     if d is not None:
         for r in range(1, sysd["nrep"] + 1):
             eng.set_rng_tape(r, d["tapes"][r - 1])
@@ -285,6 +287,26 @@ def test_brownian_forms_match_phase_form(form):
         res.append((st, eng.labels()))
     assert np.array_equal(res[0][1], res[1][1])
     assert np.abs(res[0][0][0] - res[1][0][0]).max() < 1e-6
+
+
+def test_anneal_batched_form():
+    """An annealing schedule (temperatures switched at steps 20 and 40,
+    force MC at the input kT) in the batched form follows the per-phase
+    form on 4 replicas of a 128-nt chain without exchange."""
+    s = configs.remd_c3(nrep=4, n=128, seed=3)
+    print("[case] anneal batched vs phases", flush=True)
+    res = []
+    for mode in ("phases", "batched"):
+        eng = Engine(s, parity=False, mode=mode, seed=23, exchange_every=0)
+        eng.set_anneal([0, 20, 40], [360.0, 330.0, 300.0])
+        eng.init_velocities()
+        eng.neighbor_list(0)
+        eng.step(30)
+        eng.step(30)
+        assert eng.info()["mode"] == (1 if mode == "phases" else 3)
+        res.append(eng.get_state())
+    for a, b in zip(res[0], res[1]):
+        assert np.abs(a - b).max() < 1e-6
 
 
 def test_ss_restriction_batched_form():

@@ -10,7 +10,7 @@ from sis_amd import configs
 from oracle import corc, refdrv
 from tests.golden.make_golden import load
 
-FIXTURES = ["hairpin64", "small_pbc", "mixed", "remd4", "remd4mc3", "remd4bd", "remd2ext", "hairpin_ss"]
+FIXTURES = ["hairpin64", "small_pbc", "mixed", "remd4", "remd4mc3", "remd4bd", "remd2ext", "hairpin_ss", "hairpin_anneal"]
 
 
 def _oracle_from_fixture(name):
@@ -20,6 +20,8 @@ def _oracle_from_fixture(name):
         o.set_external(sysd["external"])
     if "ss" in sysd:
         o.set_ss(sysd["ss"])
+    if "anneal" in sysd:
+        o.set_anneal(*sysd["anneal"])
     for r in range(1, sysd["nrep"] + 1):
         o.set_tape(r, d["tapes"][r - 1])
     if sysd["nrep"] > 1:
@@ -260,6 +262,31 @@ def test_external_terms_are_energy_gradients():
             xm[0, a, c] -= h
             g = (eext(o1, xp) - eext(o1, xm)) / (2 * h)
             assert abs(f[a, c] + g) <= 1e-5 * max(1.0, abs(g)), (a, c, f[a, c], -g)
+
+
+@pytest.mark.ref
+def test_live_reference_anneal_keeps_input_kT():
+    """Annealing (src/job_md.F90:207-230,600-612): the same schedule from two
+    input temperatures.  Only the force's limit MC sees the input
+    temperature (its kT is never updated), so the two 400-step runs differ;
+    the oracle follows the reference build in both."""
+    xs = []
+    for tin in (300.0, 900.0):
+        s = configs.hairpin64(1)
+        s["params"] = dict(s["params"], tempK=tin)
+        s["temps"] = np.array([tin])
+        steps, temps = [0, 150, 300], [300.0, 290.0, 280.0]
+        ref0 = refdrv.run(s, [("tape", 1, 3000000, "t")], seed=99)
+        ref1 = refdrv.run(s, [("anneal", steps, temps), ("job_md", 400, "R")], seed=99)["R"]
+        o = corc.Oracle(s)
+        o.set_tape(1, ref0["t"])
+        o.set_anneal(steps, temps)
+        o.init_velocities()
+        o.md(400, 0)
+        x = o.state()[0]
+        assert np.abs(x - ref1["xyz"]).max() < 1e-10
+        xs.append(x)
+    assert np.abs(xs[0] - xs[1]).max() > 0.1
 
 
 @pytest.mark.ref
