@@ -205,6 +205,15 @@ typedef struct sis_external {
  * whatever mode it was created with. */
 int sis_gpu_set_external(sis_gpu_ctx *ctx, const sis_external *ext);
 
+/* Box resizing ([PBC_box] resize_step / resize_change / resize_target;
+ * pbc_resize, src/pbc.F90:79-138, applied by src/job_md.F90:576-596):
+ * every resize_step steps each axis becomes original + change * count
+ * (original = the box when this is called), clamped at target (NULL: no
+ * target) until every axis reaches it; then every replica's lists are
+ * rebuilt and later DCD frames and restart files carry the new box.
+ * resize_step <= 0 turns it off.  Needs a periodic box. */
+int sis_gpu_set_box_resize(sis_gpu_ctx *ctx, int64_t resize_step, const double *change, const double *target);
+
 /* Simulated annealing (read_anneal's schedule, src/read_anneal.F90;
  * src/main.F90:90-97, src/job_md.F90:207-230,600-612): entry k sets the
  * temperature temps[k] from step steps[k] on (steps increasing; the first
@@ -296,8 +305,9 @@ int sis_gpu_out_close(sis_gpu_ctx *ctx);
  * count is a multiple of it, after that step's exchange
  * (src/job_md.F90:664-666); sis_gpu_rst_write writes them now.
  * sis_gpu_rst_read is read_rst for a restarted run (src/read_rst.F90, called
- * from src/job_md.F90:149-150,202, src/init_replica.F90:148): it restores
- * labels, state and step count; the lists are rebuilt by the next step. */
+ * from src/job_md.F90:149-150,202, src/init_replica.F90:148, src/pbc.F90:35-45):
+ * it restores labels, state, step count and the PBC block's box; the lists
+ * are rebuilt by the next step. */
 int sis_gpu_rst_open(sis_gpu_ctx *ctx, const char *const *paths, int64_t nstep_save_rst);
 int sis_gpu_rst_write(sis_gpu_ctx *ctx);
 int sis_gpu_rst_close(sis_gpu_ctx *ctx);

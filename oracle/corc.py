@@ -47,6 +47,8 @@ def lib():
         L.orc_set_external.argtypes = [vp, P(abi.SisExternal)]
         L.orc_set_step.argtypes = [vp, i64]
         L.orc_set_anneal.argtypes = [vp, i32, P(i64), P(d)]
+        L.orc_set_box_resize.argtypes = [vp, i64, P(d), P(d)]
+        L.orc_get_box.argtypes = [vp, P(d)]
         L.orc_set_ss.restype = i32
         L.orc_set_ss.argtypes = [vp, i32, P(i32)]
         L.orc_set_tape.argtypes = [vp, i32, P(C.c_uint32), i64]
@@ -192,6 +194,16 @@ class Oracle:
         s, keep = abi.external_to_struct(ext, self.nrep)
         lib().orc_set_external(self.h, C.byref(s))
         del keep
+
+    def set_box_resize(self, resize_step, change, target=None):
+        ch = np.ascontiguousarray(change, np.float64)
+        tg = np.ascontiguousarray(target, np.float64) if target is not None else None
+        lib().orc_set_box_resize(self.h, int(resize_step), _d(ch), _d(tg) if tg is not None else None)
+
+    def box(self):
+        b = np.zeros(3)
+        lib().orc_get_box(self.h, _d(b))
+        return b
 
     def set_anneal(self, steps, temps):
         st = np.ascontiguousarray(steps, np.int64)

@@ -568,6 +568,29 @@ contains
       enddo
    end subroutine sisref_set_anneal
 
+   ! [PBC_box] resize settings as read_input and init_pbc leave them
+   ! (src/read_input.F90:821-870, src/pbc.F90:51-52)
+   subroutine sisref_set_box_resize(step, change, has_target, target) bind(C, name="sisref_set_box_resize")
+      use pbc, only : flg_pbc_resize, pbc_resize_step, pbc_resize_change, flg_pbc_resize_target, &
+                      pbc_resize_target, pbc_resize_count, pbc_box_original, pbc_box
+      integer(c_int64_t), value :: step
+      integer(c_int), value :: has_target
+      real(c_double), intent(in) :: change(3), target(3)
+      flg_pbc_resize = .True.
+      pbc_resize_step = step
+      pbc_resize_change(:) = change(:)
+      flg_pbc_resize_target = (has_target /= 0)
+      pbc_resize_target(:) = target(:)
+      pbc_resize_count(:) = 0
+      pbc_box_original(:) = pbc_box(:)
+   end subroutine sisref_set_box_resize
+
+   subroutine sisref_get_box(box) bind(C, name="sisref_get_box")
+      use pbc, only : pbc_box
+      real(c_double), intent(out) :: box(3)
+      box(:) = pbc_box(:)
+   end subroutine sisref_get_box
+
    subroutine sisref_ele_params(irep, out) bind(C, name="sisref_ele_params")
       use var_state, only : lambdaD, diele
       use var_potential, only : ele_coef, ele_cutoff

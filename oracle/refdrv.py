@@ -165,6 +165,14 @@ def _worker(q, so, sysd, ops, seed, nstep_exchange):
                     _ip(fp), _dp(fk), _dp(fi), _dp(fv), C.c_int(int(ext.get("rg", 0))),
                     C.c_double(float(ext.get("rg_k", 0.0))), _dp(rg0), C.c_int(nr), _ip(ri), _dp(rc), _dp(rp),
                     C.c_int64(int(op[2])))
+            elif kind == "resize":
+                ch = np.ascontiguousarray(op[2], np.float64)
+                tg = np.ascontiguousarray(op[3] if op[3] is not None else np.zeros(3), np.float64)
+                lib.sisref_set_box_resize(C.c_int64(int(op[1])), _dp(ch), C.c_int(int(op[3] is not None)), _dp(tg))
+            elif kind == "box":
+                b = np.zeros(3)
+                lib.sisref_get_box(_dp(b))
+                out[op[-1]] = b
             elif kind == "anneal":
                 st = np.ascontiguousarray(op[1], np.int64)
                 tk = np.ascontiguousarray(op[2], np.float64)

@@ -79,6 +79,8 @@ struct orc_sys {
     /* annealing: schedule, current entry (1-based), next switch step, and
      * the kT of the force's limit MC (var_state's kT, never updated) */
     int an_n, ianneal; int64_t *an_step; double *an_T; int64_t an_next; double mc_kT;
+    /* box resizing (src/pbc.F90:79-138) */
+    int rz_on, rz_has_target; int64_t rz_step, rz_count[3]; double rz_change[3], rz_target[3], rz_orig[3];
     double *x_rg0, *x_dcf_f, *x_fr_k, *x_fr_init, *x_fr_velo, *x_rest_rcut, *x_rest_para;
 };
 
@@ -896,12 +898,46 @@ static void free_external(orc_sys *s) {
 }
 
 void orc_set_step(orc_sys *s, int64_t istep) { s->istep = istep; }
+void orc_get_box(orc_sys *s, double *box) { for (int k = 0; k < 3; k++) box[k] = s->p.box[k]; }
 
 /* The pairs of a CT/BPSEQ/BPL file (1-based bead ids, either order) as
  * read_ss keeps them (src/read_ss.F90:126-205): ends of a chain, pairs
  * closer than the minimum loop, isolated pairs (models 3-5) and
  * non-canonical pairs are dropped.  npair < 0 clears the restriction.
  * Returns the number of pairs kept. */
+/* [PBC_box] resizing: pbc_resize (src/pbc.F90:79-138) every resize_step
+ * steps; target may be NULL */
+void orc_set_box_resize(orc_sys *s, int64_t resize_step, const double *change, const double *target) {
+    s->rz_on = resize_step > 0;
+    s->rz_step = resize_step;
+    s->rz_has_target = target != NULL;
+    for (int k = 0; k < 3; k++) {
+        s->rz_change[k] = change ? change[k] : 0.0;
+        s->rz_target[k] = target ? target[k] : 0.0;
+        s->rz_orig[k] = s->p.box[k];
+        s->rz_count[k] = 0;
+    }
+}
+
+static void pbc_resize(orc_sys *s) {
+    const double SMALL = 2.220446049250313e-16;
+    int reach[3] = {0, 0, 0};
+    for (int i = 0; i < 3; i++) {
+        if (fabs(s->rz_change[i]) < SMALL) { reach[i] = 1; continue; }
+        if (s->rz_has_target && fabs(s->p.box[i] - s->rz_target[i]) < SMALL) { reach[i] = 1; continue; }
+        s->rz_count[i]++;
+        double x = s->rz_orig[i] + s->rz_change[i] * s->rz_count[i];
+        if (s->rz_change[i] < 0.0) {
+            if (s->rz_has_target && x - s->rz_target[i] < SMALL) { s->p.box[i] = s->rz_target[i]; reach[i] = 1; }
+            else s->p.box[i] = x;
+        } else {
+            if (s->rz_has_target && x - s->rz_target[i] > SMALL) { s->p.box[i] = s->rz_target[i]; reach[i] = 1; }
+            else s->p.box[i] = x;
+        }
+    }
+    if (reach[0] && reach[1] && reach[2]) s->rz_on = 0;
+}
+
 /* Annealing (src/main.F90:90-97, src/job_md.F90:207-230,600-612): the
  * electrostatics are set once at the first temperature, the force's MC
  * keeps the input kT, and the replicas' temperature follows the schedule. */
@@ -1442,17 +1478,26 @@ int64_t orc_md(orc_sys *s, int64_t nsteps, int32_t nstep_exchange, double *energ
             exchange(s, energies_out);
             last_exchange = 1;
         }
+        /* energies of the last step as written at a save step, before the
+         * resize block: the BP state of the last force call is reused
+         * (src/job_md.F90:474-481) */
+        if (st == nsteps - 1 && !last_exchange && energies_out) {
+            int mc_last = s->p.nstep_bp_mc > 0 && s->istep % s->p.nstep_bp_mc == 0;
+            for (int r = 1; r <= nr; r++) orc_energy(s, r, s->rep[r - 1].tK, mc_last, 1, energies_out + 11 * (r - 1));
+        }
+        /* resize block (src/job_md.F90:576-596) */
+        if (s->rz_on && s->istep % s->rz_step == 0) {
+            pbc_resize(s);
+            for (int r = 0; r < nr; r++) {
+                orc_neighbor_list(s, r + 1);
+                memset(s->move + (size_t)r * n * 3, 0, sizeof(double) * 3 * n);
+            }
+        }
         if (s->an_n > 0 && s->an_next > 0 && s->istep + 1 == s->an_next) {
             s->ianneal++;
             for (int r = 0; r < nr; r++) s->rep[r].tK = s->an_T[s->ianneal - 1];
             s->an_next = s->ianneal < s->an_n ? s->an_step[s->ianneal] : 0;
         }
-    }
-    /* energies of the last step as written at a save step: the BP state of
-     * the last force call is reused (src/job_md.F90:474-481) */
-    if (nsteps > 0 && !last_exchange && energies_out) {
-        int mc_last = s->p.nstep_bp_mc > 0 && s->istep % s->p.nstep_bp_mc == 0;
-        for (int r = 1; r <= nr; r++) orc_energy(s, r, s->rep[r - 1].tK, mc_last, 1, energies_out + 11 * (r - 1));
     }
     free(forces); free(rnd);
     return rebuilds;

@@ -93,6 +93,11 @@ struct sis_gpu_ctx {
   int ianneal = 0;            // 1-based index of the current schedule entry
   int64_t an_next = 0;        // step whose start takes the next entry, 0 = none
   int rst_ianneal = 0;        // ANNEAL block of a restart file read, or 0
+  // box resizing (sis_gpu_set_box_resize; src/pbc.F90:79-138)
+  bool rz_on = false, rz_has_target = false;
+  int64_t rz_step = 0;
+  int64_t rz_count[3] = {0, 0, 0};
+  double rz_change[3] = {0, 0, 0}, rz_target[3] = {0, 0, 0}, rz_orig[3] = {0, 0, 0};
   // secondary-structure restriction (sis_gpu_set_ss)
   bool ss_on = false;
   DevBuf<int> ss_off_d, ss_partner_d;
@@ -468,6 +473,27 @@ void upload_conditions(sis_gpu_ctx *c) {
   HIPCHK(hipStreamSynchronize(c->stream));
 }
 
+// pbc_resize (src/pbc.F90:79-138): each axis moves to original + change x
+// count, clamped at the target, until every axis has reached it
+void pbc_resize_host(sis_gpu_ctx *c) {
+  const double SMALL = 2.220446049250313e-16;   // epsilon(PI), src/const_phys.F90:15
+  bool reach[3] = {false, false, false};
+  for (int i = 0; i < 3; i++) {
+    if (std::fabs(c->rz_change[i]) < SMALL) { reach[i] = true; continue; }
+    if (c->rz_has_target && std::fabs(c->prm.box[i] - c->rz_target[i]) < SMALL) { reach[i] = true; continue; }
+    c->rz_count[i]++;
+    const double x = c->rz_orig[i] + c->rz_change[i] * (double)c->rz_count[i];
+    if (c->rz_change[i] < 0.0) {
+      if (c->rz_has_target && x - c->rz_target[i] < SMALL) { c->prm.box[i] = c->rz_target[i]; reach[i] = true; }
+      else c->prm.box[i] = x;
+    } else {
+      if (c->rz_has_target && x - c->rz_target[i] > SMALL) { c->prm.box[i] = c->rz_target[i]; reach[i] = true; }
+      else c->prm.box[i] = x;
+    }
+  }
+  if (reach[0] && reach[1] && reach[2]) c->rz_on = false;
+}
+
 void set_mask(sis_gpu_ctx *c, const std::vector<int> &m) {
   HIPCHK(hipMemcpyAsync(c->mask_d.p, m.data(), m.size() * sizeof(int), hipMemcpyHostToDevice, c->stream));
 }
@@ -789,6 +815,21 @@ void do_exchange(sis_gpu_ctx *c, double *replica_energies, int32_t *accepted, do
 
 // One job_md step for every replica (src/job_md.F90:362-460), one launch
 // per phase.
+void dcd_set_box(sis_gpu_ctx *c);   // sis_output_host.h
+
+// The resize block of src/job_md.F90:576-596: new box, then every
+// replica's lists rebuilt and displacements zeroed; the next step's skin
+// test starts from zero and does not force the limit MC
+void box_resize_now(sis_gpu_ctx *c) {
+  pbc_resize_host(c);
+  dcd_set_box(c);
+  upload_conditions(c);
+  setup_cells(c);
+  std::vector<int> all(c->R, 1);
+  rebuild(c, all);
+  HIPCHK(hipMemsetAsync(c->rebuild_flag_d.p, 0, c->R * sizeof(int), c->stream));
+}
+
 void step_phases(sis_gpu_ctx *c) {
   const int N = c->N, R = c->R;
   c->istep++;
@@ -1417,6 +1458,7 @@ int sis_gpu_step(sis_gpu_ctx *c, int64_t nsteps, int64_t *rebuilds) {
   const int64_t so = (c->outf && c->outf->nstep_save > 0) ? c->outf->nstep_save : 0;
   const int64_t rs = (c->rst && c->rst->nstep_save > 0) ? c->rst->nstep_save : 0;
   for (int64_t left = nsteps; left > 0;) {
+    const int64_t rz = c->rz_on ? c->rz_step : 0;
     // the next annealing entry is taken at the end of step an_next - 1
     // (src/job_md.F90:600-612)
     const bool an = c->anneal_on && c->an_next > 0 && c->an_next - 1 > c->istep;
@@ -1427,6 +1469,7 @@ int sis_gpu_step(sis_gpu_ctx *c, int64_t nsteps, int64_t *rebuilds) {
     if (so) k = std::min(k, so - c->istep % so);
     if (rs) k = std::min(k, rs - c->istep % rs);
     if (an) k = std::min(k, c->an_next - 1 - c->istep);
+    if (rz) k = std::min(k, rz - c->istep % rz);
     run(k);
     left -= k;
     const bool exch = ex && c->istep % ex == 0;
@@ -1453,6 +1496,7 @@ int sis_gpu_step(sis_gpu_ctx *c, int64_t nsteps, int64_t *rebuilds) {
     } else if (exch) {
       do_exchange(c, nullptr, nullptr);
     }
+    if (rz && c->istep % rz == 0) box_resize_now(c);
     if (an && c->istep + 1 == c->an_next) {
       const int n = (int)c->an_T.size();
       c->ianneal++;
@@ -1488,6 +1532,25 @@ int sis_gpu_set_step(sis_gpu_ctx *c, int64_t istep) {
   if (!c) return fail(SIS_ERR_ARG, "null context");
   if (istep < 0) return fail(SIS_ERR_ARG, "istep < 0");
   c->istep = istep;
+  return SIS_OK;
+  API_END
+}
+
+int sis_gpu_set_box_resize(sis_gpu_ctx *c, int64_t resize_step, const double *change, const double *target) {
+  API_BEGIN
+  if (!c) return fail(SIS_ERR_ARG, "null context");
+  if (resize_step <= 0) { c->rz_on = false; return SIS_OK; }
+  if (!c->prm.pbc) return fail(SIS_ERR_ARG, "box resizing needs a periodic box");
+  if (!change) return fail(SIS_ERR_ARG, "null change");
+  for (int k = 0; k < 3; k++) {
+    c->rz_change[k] = change[k];
+    c->rz_target[k] = target ? target[k] : 0.0;
+    c->rz_orig[k] = c->prm.box[k];   // init_pbc: the box at the start (after a restart's PBC block)
+    c->rz_count[k] = 0;
+  }
+  c->rz_has_target = target != nullptr;
+  c->rz_step = resize_step;
+  c->rz_on = true;
   return SIS_OK;
   API_END
 }

@@ -88,7 +88,8 @@ struct DcdOut {
   int fix_com = 0;
   int64_t nstep_save = 0;
   bool unitcell = false;
-  double box[3] = {0, 0, 0};
+  double box[3] = {0, 0, 0};      // current unit cell (follows box resizes)
+  double sbox[2][3] = {};          // unit cell of the frame in each staging buffer
   int device = 0;
   // two pinned staging buffers of R*N double4
   double4 *buf[2] = {nullptr, nullptr};
@@ -124,7 +125,7 @@ void dcd_writer_loop(sis_gpu_ctx *c, DcdOut *o) {
       if (e.empty())
         for (int r = 0; r < c->R; r++)
           dcd_frame(o->f[r], c->N, reinterpret_cast<const double *>(o->buf[b] + (size_t)r * c->N), 4, o->unitcell,
-                    o->box, o->fix_com, tmp);
+                    o->sbox[b], o->fix_com, tmp);
     } catch (const ApiError &ex) {
       e = ex.msg;
     }
@@ -147,6 +148,7 @@ void dcd_enqueue(sis_gpu_ctx *c) {
     if (!o->err.empty()) throw ApiError{SIS_ERR_ARG, o->err};
     o->busy[b] = true;
     o->next ^= 1;
+    for (int k = 0; k < 3; k++) o->sbox[b][k] = o->box[k];
   }
   const size_t RN = (size_t)c->R * c->N;
   HIPCHK(hipMemcpyAsync(o->buf[b], c->pos_d.p, RN * sizeof(double4), hipMemcpyDeviceToHost, c->stream));
@@ -154,6 +156,14 @@ void dcd_enqueue(sis_gpu_ctx *c) {
   std::lock_guard<std::mutex> lk(o->mu);
   o->q.push_back(b);
   o->cv.notify_all();
+}
+
+// later frames carry the context's current box (fdcd%box after a resize,
+// src/job_md.F90:582)
+void dcd_set_box(sis_gpu_ctx *c) {
+  if (!c->dcd) return;
+  std::lock_guard<std::mutex> lk(c->dcd->mu);
+  for (int k = 0; k < 3; k++) c->dcd->box[k] = c->prm.box[k];
 }
 
 // wait for queued frames; rethrows a writer failure
@@ -1058,6 +1068,7 @@ int sis_gpu_rst_read(sis_gpu_ctx *c, const char *const *paths) {
   std::vector<double> x(n3 * R), v(n3 * R), a(n3 * R);
   std::vector<int> have(R, 0);
   int64_t istep = -1;
+  double new_box[3] = {c->prm.box[0], c->prm.box[1], c->prm.box[2]};
   std::vector<int32_t> lab;
   for (int r = 0; r < R; r++) {
     if (!paths[r]) return fail(SIS_ERR_ARG, "null path");
@@ -1075,12 +1086,18 @@ int sis_gpu_rst_read(sis_gpu_ctx *c, const char *const *paths) {
       if (o.nrep != R) return fail(SIS_ERR_ARG, "nrep is not consistent in the restart file");
       lab = o.rep2lab;
     }
-    if ((o.found & (1 << (RST_PBC - 1))) && c->prm.pbc &&
-        (o.box[0] != c->prm.box[0] || o.box[1] != c->prm.box[1] || o.box[2] != c->prm.box[2]))
-      return fail(SIS_ERR_UNSUPPORTED, "the restart file's box differs from the context's (box changes are not supported)");
+    if ((o.found & (1 << (RST_PBC - 1))) && c->prm.pbc) std::copy(o.box, o.box + 3, new_box);
   }
   if (istep < 0) return fail(SIS_ERR_ARG, "STEP block missing");
   if (R > 1 && lab.empty()) return fail(SIS_ERR_ARG, "REPLICA block missing");
+  // the PBC block's box replaces the input box (init_pbc, src/pbc.F90:35-45)
+  if (new_box[0] != c->prm.box[0] || new_box[1] != c->prm.box[1] || new_box[2] != c->prm.box[2]) {
+    std::copy(new_box, new_box + 3, c->prm.box);
+    dcd_set_box(c);
+    upload_conditions(c);
+    setup_cells(c);
+    c->lists_valid = false;
+  }
   int rc;
   if (R > 1 && (rc = sis_gpu_set_labels(c, lab.data())) != SIS_OK) return rc;
   if ((rc = sis_gpu_set_state(c, x.data(), v.data(), a.data())) != SIS_OK) return rc;

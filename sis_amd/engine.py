@@ -228,6 +228,14 @@ class Engine:
         tk = np.ascontiguousarray(temps, np.float64)
         _check(lib().sis_gpu_set_anneal(self.h, len(st), st.ctypes.data_as(C.POINTER(C.c_int64)), abi.dptr(tk)))
 
+    def set_box_resize(self, resize_step: int, change, target=None):
+        """[PBC_box] resizing: every resize_step steps the box moves by
+        change per axis, up to target (None: no target)."""
+        ch = np.ascontiguousarray(change, np.float64)
+        tg = np.ascontiguousarray(target, np.float64) if target is not None else None
+        _check(lib().sis_gpu_set_box_resize(self.h, int(resize_step), abi.dptr(ch),
+                                            abi.dptr(tg) if tg is not None else None))
+
     def set_ss(self, pairs):
         """Restrict base pairs to a CT/BPSEQ/BPL file's pairs, (n, 2)
         1-based bead ids; None lifts the restriction."""

@@ -232,6 +232,14 @@ module sis_gpu_iface
          type(c_ptr), value :: ctx
       end function
 
+      ! box resizing ([PBC_box] resize_step / resize_change / resize_target)
+      integer(c_int) function sis_gpu_set_box_resize(ctx, resize_step, change, target) &
+            bind(C, name="sis_gpu_set_box_resize")
+         import :: c_int, c_ptr, c_int64_t
+         type(c_ptr), value :: ctx, change, target
+         integer(c_int64_t), value :: resize_step
+      end function
+
       ! annealing schedule (read_anneal)
       integer(c_int) function sis_gpu_set_anneal(ctx, n, steps, temps) bind(C, name="sis_gpu_set_anneal")
          import :: c_int, c_ptr, c_int32_t, c_int64_t, c_double

@@ -156,6 +156,16 @@ def hairpin_anneal():
     return sysd
 
 
+def pbc_resize():
+    """small_pbc with [PBC_box] resizing every 10 steps: x and y shrink,
+    z grows, each until its target (x reaches it at step 40, y after the
+    run, z at step 40) (src/pbc.F90:79-138, src/job_md.F90:576-596)."""
+    sysd = configs.small_pbc(4, 30, 100.0, seed=5)
+    sysd["resize"] = (10, np.array([-0.5, -0.4, 0.3]), np.array([98.0, 97.0, 101.0]))
+    sysd["name"] = "pbc_resize"
+    return sysd
+
+
 SYSTEMS = {
     "hairpin64": (lambda: configs.hairpin64(1), 60, 0),
     "small_pbc": (lambda: configs.small_pbc(4, 30, 100.0, seed=5), 60, 0),
@@ -166,6 +176,7 @@ SYSTEMS = {
     "remd2ext": (remd_external, 30, 10),
     "hairpin_ss": (hairpin_ss, 60, 0),
     "hairpin_anneal": (hairpin_anneal, 300, 0),
+    "pbc_resize": (pbc_resize, 60, 0),
 }
 
 
@@ -182,6 +193,8 @@ def make(name, builder, nstep, nex, seed=4321):
         pre.append(("ss", s["ss"]))
     if s.get("anneal") is not None:
         pre.append(("anneal", s["anneal"][0], s["anneal"][1]))
+    if s.get("resize") is not None:
+        pre.append(("resize",) + tuple(s["resize"]))
     ops = pre + ops + [("nl", r) for r in range(1, nrep + 1)]
     ops += [("lists", 1, "L"), ("force", 1, 1, "F"), ("bp_state", 1, "B"),
             ("energy", 1, float(s["temps"][0]), 0, 1, "E"), ("ele", 1, "ele"), ("bp_tables", "T")]
@@ -195,6 +208,8 @@ def make(name, builder, nstep, nex, seed=4321):
         o.set_ss(s["ss"])
     if s.get("anneal") is not None:
         o.set_anneal(*s["anneal"])
+    if s.get("resize") is not None:
+        o.set_box_resize(*s["resize"])
     for r in range(1, nrep + 1):
         o.set_tape(r, ref0[f"tape{r}"])
     if nrep > 1:
@@ -210,6 +225,8 @@ def make(name, builder, nstep, nex, seed=4321):
         o2.set_ss(s["ss"])
     if s.get("anneal") is not None:
         o2.set_anneal(*s["anneal"])
+    if s.get("resize") is not None:
+        o2.set_box_resize(*s["resize"])
     for r in range(1, nrep + 1):
         o2.set_tape(r, ref0[f"tape{r}"])
     if nrep > 1:
@@ -239,6 +256,10 @@ def make(name, builder, nstep, nex, seed=4321):
         out["ext_" + k] = np.asarray(v)
     if s.get("ss") is not None:
         out["ss_pairs"] = np.asarray(s["ss"], np.int32)
+    if s.get("resize") is not None:
+        out["resize_step"] = np.int64(s["resize"][0])
+        out["resize_change"] = np.asarray(s["resize"][1], np.float64)
+        out["resize_target"] = np.asarray(s["resize"][2], np.float64)
     if s.get("anneal") is not None:
         out["anneal_steps"] = np.asarray(s["anneal"][0], np.int64)
         out["anneal_temps"] = np.asarray(s["anneal"][1], np.float64)
@@ -269,6 +290,8 @@ def load(name):
         sysd["external"] = ext
     if "ss_pairs" in d:
         sysd["ss"] = d["ss_pairs"]
+    if "resize_step" in d:
+        sysd["resize"] = (int(d["resize_step"]), d["resize_change"], d["resize_target"])
     if "anneal_steps" in d:
         sysd["anneal"] = (d["anneal_steps"], d["anneal_temps"])
     return sysd, d

@@ -20,7 +20,7 @@ from tests.golden.make_golden import load
 
 pytestmark = pytest.mark.gpu
 
-FIXTURES = ["hairpin64", "small_pbc", "mixed", "remd4", "remd4mc3", "remd4bd", "remd2ext", "hairpin_ss", "hairpin_anneal"]
+FIXTURES = ["hairpin64", "small_pbc", "mixed", "remd4", "remd4mc3", "remd4bd", "remd2ext", "hairpin_ss", "hairpin_anneal", "pbc_resize"]
 
 
 def _sorted_rows(a):
@@ -39,6 +39,8 @@ def _engine(sysd, d=None, **kw):
         eng.set_ss(sysd["ss"])
     if "anneal" in sysd:
         eng.set_anneal(*sysd["anneal"])
+    if "resize" in sysd:
+        eng.set_box_resize(*sysd["resize"])
     if d is not None:
         for r in range(1, sysd["nrep"] + 1):
             eng.set_rng_tape(r, d["tapes"][r - 1])
@@ -287,6 +289,35 @@ def test_brownian_forms_match_phase_form(form):
         res.append((st, eng.labels()))
     assert np.array_equal(res[0][1], res[1][1])
     assert np.abs(res[0][0][0] - res[1][0][0]).max() < 1e-6
+
+
+def test_box_resize_persistent_form(tmp_path):
+    """Box resizing every 20 steps (x shrinks 1 A until 2,496 A, z grows)
+    on the 2,000-nt condensate: the persistent form follows the per-phase
+    form, and the last DCD frame carries the box the schedule gives
+    (original + change x count, clamped at the target)."""
+    from sis_amd.engine import read_dcd_file
+    s = configs.condensate_c2()
+    b0 = np.array(s["params"]["box"], float)
+    ch, tg = np.array([-1.0, 0.0, 0.5]), b0 + np.array([-3.0, 0.0, 10.0])
+    print("[case] box resize persistent vs phases", flush=True)
+    res = []
+    for mode in ("phases", "persistent"):
+        eng = Engine(s, parity=False, mode=mode, seed=29)
+        eng.set_box_resize(20, ch, tg)
+        eng.init_velocities()
+        eng.neighbor_list(0)
+        path = tmp_path / f"{mode}.dcd"
+        eng.dcd_open([path], nstep_save=20)
+        eng.step(100)
+        eng.dcd_close()
+        frames, box = read_dcd_file(path, int(np.sum(s["nmp_chain"])))
+        assert len(frames) == 5
+        want = np.minimum(np.maximum(b0 + ch * 5, np.minimum(b0, tg)), np.maximum(b0, tg))
+        assert np.allclose(box, want, rtol=0, atol=1e-9), (box, want)
+        res.append(eng.get_state())
+    for a, b in zip(res[0], res[1]):
+        assert np.abs(a - b).max() < 1e-6
 
 
 def test_anneal_batched_form():

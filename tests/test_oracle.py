@@ -10,7 +10,7 @@ from sis_amd import configs
 from oracle import corc, refdrv
 from tests.golden.make_golden import load
 
-FIXTURES = ["hairpin64", "small_pbc", "mixed", "remd4", "remd4mc3", "remd4bd", "remd2ext", "hairpin_ss", "hairpin_anneal"]
+FIXTURES = ["hairpin64", "small_pbc", "mixed", "remd4", "remd4mc3", "remd4bd", "remd2ext", "hairpin_ss", "hairpin_anneal", "pbc_resize"]
 
 
 def _oracle_from_fixture(name):
@@ -22,6 +22,8 @@ def _oracle_from_fixture(name):
         o.set_ss(sysd["ss"])
     if "anneal" in sysd:
         o.set_anneal(*sysd["anneal"])
+    if "resize" in sysd:
+        o.set_box_resize(*sysd["resize"])
     for r in range(1, sysd["nrep"] + 1):
         o.set_tape(r, d["tapes"][r - 1])
     if sysd["nrep"] > 1:
