@@ -292,6 +292,9 @@ int sis_gpu_bp_close(sis_gpu_ctx *ctx);
  * src/job_md.F90:474-510 does). */
 int sis_gpu_out_open(sis_gpu_ctx *ctx, const char *const *paths, int64_t nstep_save);
 int sis_gpu_out_close(sis_gpu_ctx *ctx);
+/* The lines of the current state now: the initial write_out of
+ * src/job_md.F90:280-291,331-333 (energies with a fresh limit MC). */
+int sis_gpu_out_write(sis_gpu_ctx *ctx);
 
 /* Restart files (.rst) in the block format of write_rst
  * (src/write_rst.F90:25-145; block ids src/const_idx.F90:72-85), one per

@@ -195,7 +195,7 @@ EXPORTED = [
     "sis_gpu_dcd_energies", "sis_dcd_read_file", "sis_gpu_check_force",
     "sis_gpu_rst_open", "sis_gpu_rst_write", "sis_gpu_rst_close", "sis_gpu_rst_read",
     "sis_rst_write_file", "sis_rst_read_file", "sis_gpu_set_external", "sis_gpu_set_ss",
-    "sis_gpu_set_anneal", "sis_gpu_set_box_resize",
+    "sis_gpu_set_anneal", "sis_gpu_set_box_resize", "sis_gpu_out_write",
 ]
 
 
@@ -236,6 +236,7 @@ def _declare(lib: C.CDLL) -> None:
         "sis_gpu_kinetic": (i32, [vp, i32, P(d)]),
         "sis_gpu_out_open": (i32, [vp, P(C.c_char_p), i64]),
         "sis_gpu_out_close": (i32, [vp]),
+        "sis_gpu_out_write": (i32, [vp]),
         "sis_gpu_dcd_energies": (i32, [vp, C.c_char_p, C.c_char_p, i64, P(d), P(i64)]),
         "sis_gpu_check_force": (i32, [vp, i32, d, C.c_char_p, P(d), P(d)]),
         "sis_dcd_read_file": (i32, [C.c_char_p, i32, i64, P(d), P(d), P(i64)]),

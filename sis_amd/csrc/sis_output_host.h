@@ -570,6 +570,24 @@ int sis_gpu_out_open(sis_gpu_ctx *c, const char *const *paths, int64_t nstep_sav
   API_END
 }
 
+int sis_gpu_out_write(sis_gpu_ctx *c) {
+  API_BEGIN
+  if (!c) return fail(SIS_ERR_ARG, "null context");
+  if (!c->outf) return fail(SIS_ERR_STATE, "energy output not open");
+  if (!c->lists_valid) return fail(SIS_ERR_STATE, "energy line written before rebuild");
+  // the initial state (src/job_md.F90:280-291,331-333): energies at each
+  // replica's temperature with a fresh limit MC, and Ekin
+  const int R = c->R;
+  std::vector<int> lab(R), all(R, 1);
+  for (int r = 0; r < R; r++) lab[r] = c->rep2lab[r + 1];
+  std::vector<double> ekin(R), e16((size_t)R * 16);
+  kinetic_all(c, ekin.data());
+  energy_phase(c, all, c->tK, c->P.nstep_bp_mc > 0 ? 1 : 0, e16.data());
+  out_lines(c, e16.data(), ekin.data(), lab, c->tK);
+  return SIS_OK;
+  API_END
+}
+
 int sis_gpu_out_close(sis_gpu_ctx *c) {
   API_BEGIN
   if (!c) return fail(SIS_ERR_ARG, "null context");

@@ -208,6 +208,10 @@ class Engine:
         arr = (C.c_char_p * self.nrep)(*[str(x).encode() for x in paths])
         _check(lib().sis_gpu_out_open(self.h, arr, int(nstep_save)))
 
+    def out_write(self):
+        """Energy lines of the current state now (the initial line)."""
+        _check(lib().sis_gpu_out_write(self.h))
+
     def out_close(self):
         _check(lib().sis_gpu_out_close(self.h))
 

@@ -227,6 +227,11 @@ module sis_gpu_iface
          integer(c_int64_t), value :: nstep_save
       end function
 
+      integer(c_int) function sis_gpu_out_write(ctx) bind(C, name="sis_gpu_out_write")
+         import :: c_int, c_ptr
+         type(c_ptr), value :: ctx
+      end function
+
       integer(c_int) function sis_gpu_out_close(ctx) bind(C, name="sis_gpu_out_close")
          import :: c_int, c_ptr
          type(c_ptr), value :: ctx

@@ -181,10 +181,11 @@ def test_bp_records_during_steps(tmp_path, case):
 
 @pytest.mark.parametrize("case", ["batched", "pbc_phases"])
 def test_out_lines_during_steps(tmp_path, case):
-    """Energy lines written at the save steps equal the host line writer
-    applied to the kinetic and potential energies (step's base-pair state,
-    sis_gpu_energy with bp_energy_valid) of an identical run read between
-    shorter sis_gpu_step calls."""
+    """The initial line (sis_gpu_out_write: energies with a fresh limit MC)
+    and the energy lines written at the save steps equal the host line
+    writer applied to the kinetic and potential energies (step's base-pair
+    state, sis_gpu_energy with bp_energy_valid) of an identical run read
+    between shorter sis_gpu_step calls."""
     from sis_amd.engine import write_out_file
     if case == "batched":
         s = configs.remd_c3(nrep=4, n=128, seed=3)
@@ -199,6 +200,7 @@ def test_out_lines_during_steps(tmp_path, case):
     eng.neighbor_list(0)
     paths = [tmp_path / f"dev_{r}.out" for r in range(R)]
     eng.out_open(paths, nstep_save=25)
+    eng.out_write()
     eng.step(60)
     eng.step(40)
     eng.out_close()
@@ -206,6 +208,10 @@ def test_out_lines_during_steps(tmp_path, case):
     ref.init_velocities()
     ref.neighbor_list(0)
     rows = [[] for _ in range(R)]
+    ek0, lab0 = ref.kinetic(0), ref.labels()
+    for r in range(R):
+        e = ref.energy_sumup(r + 1, bp_mc=1, bp_energy_valid=0)
+        rows[r].append((0, lab0[r], float(s["temps"][lab0[r] - 1]), ek0[r], e))
     for k in range(4):
         ref.step(25)
         ek = ref.kinetic(0)

@@ -295,7 +295,9 @@ def test_box_resize_persistent_form(tmp_path):
     """Box resizing every 20 steps (x shrinks 1 A until 2,496 A, z grows)
     on the 2,000-nt condensate: the persistent form follows the per-phase
     form, and the last DCD frame carries the box the schedule gives
-    (original + change x count, clamped at the target)."""
+    (original + change x count, clamped at the target).  The frame of step
+    100 is saved before that step's resize (the save block precedes the
+    resize block, src/job_md.F90:485-596): it has the box of 4 resizes."""
     from sis_amd.engine import read_dcd_file
     s = configs.condensate_c2()
     b0 = np.array(s["params"]["box"], float)
@@ -313,7 +315,7 @@ def test_box_resize_persistent_form(tmp_path):
         eng.dcd_close()
         frames, box = read_dcd_file(path, int(np.sum(s["nmp_chain"])))
         assert len(frames) == 5
-        want = np.minimum(np.maximum(b0 + ch * 5, np.minimum(b0, tg)), np.maximum(b0, tg))
+        want = np.minimum(np.maximum(b0 + ch * 4, np.minimum(b0, tg)), np.maximum(b0, tg))
         assert np.allclose(box, want, rtol=0, atol=1e-9), (box, want)
         res.append(eng.get_state())
     for a, b in zip(res[0], res[1]):
