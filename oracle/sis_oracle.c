@@ -160,7 +160,8 @@ static void init_bp_tables(orc_sys *s) {
     memset(s->bp3_dH, 0, sizeof s->bp3_dH);
     memset(s->bp3_dS, 0, sizeof s->bp3_dS);
     int h = 0;
-    for (int x = 0; x <= 2; x++)
+    /* the table exists for model 5 only (src/init_bp.F90:39) */
+    for (int x = 0; x <= 2 && p->bp_model == 5; x++)
         for (int y = x + 1; y <= 3; y++) {
             if (!is_complement(x, y)) continue;
             for (int w = 0; w <= 3; w++)
@@ -398,8 +399,10 @@ void orc_neighbor_list(orc_sys *s, int32_t irep1) {
                          * (src/neighbor_list.F90:212) */
                         ipush(&r->bp, (j == ni) ? jmp + 1 + 1 - nj : jmp + 1 + 1);
                         ipush(&r->bp, bt);
-                        dpush(&r->bpcoef, s->bp3_dH[b3]);
-                        dpush(&r->bpcoef, s->bp3_dS[b3]);
+                        /* the triplet table is zero for models other than 5
+                         * (src/init_bp.F90:39, src/neighbor_list.F90:220-221) */
+                        dpush(&r->bpcoef, s->p.bp_model == 5 ? s->bp3_dH[b3] : 0.0);
+                        dpush(&r->bpcoef, s->p.bp_model == 5 ? s->bp3_dS[b3] : 0.0);
                     }
                 }
             }
@@ -725,9 +728,18 @@ static void bp_limit_mc(orc_sys *s, int irep, int *nt_excess, double beta) {
     free(bp_seq);
 }
 
+/* prefactor of a candidate's energy: dG = dH - T dS of its triplet for
+ * model 5 (src/force_bp_triplet.F90:90-97), 1 for models 1/3, whose
+ * energy is U0 exp(-u) (src/force_bp.F90:203) */
+static double bp_prefactor(const orc_sys *s, const rep_t *r, int64_t b, double tK) {
+    if (s->p.bp_model != 5) return 1.0;
+    return r->bpcoef.a[2 * b] - tK * r->bpcoef.a[2 * b + 1];
+}
+
 static void force_bp(orc_sys *s, int irep, int bp_mc, const double *x, double *f) {
     /* models 4/5: src/force_bp_limit_triplet.F90 (bp_mc) and
-     * src/force_bp_triplet.F90 (no MC, skips candidates not in bp_status_MC) */
+     * src/force_bp_triplet.F90 (no MC, skips candidates not in bp_status_MC);
+     * models 1/3 without MC: src/force_bp.F90 */
     rep_t *r = &s->rep[irep];
     int64_t nbp = r->bp.n / 7;
     double tK = r->tK, beta = s->mc_kT > 0.0 ? 1.0 / s->mc_kT : 1.0 / (BOLTZ_KCAL_MOL * tK);
@@ -738,8 +750,8 @@ static void force_bp(orc_sys *s, int irep, int bp_mc, const double *x, double *f
     for (int64_t b = 0; b < nbp; b++) {
         if (!bp_mc && s->p.nstep_bp_mc > 0 && !r->bp_status_mc[b]) continue;
         const int32_t *mp = r->bp.a + 7 * b;
-        double dG = r->bpcoef.a[2 * b] - tK * r->bpcoef.a[2 * b + 1];
-        if (dG >= 0.0) continue;
+        double dG = bp_prefactor(s, r, b, tK);
+        if (s->p.bp_model == 5 && dG >= 0.0) continue;
         double u, fb[6][3];
         if (!bp_geom(s, x, mp, &u, fb)) continue;
         double e = s->bpp[mp[6]].U0 * dG * exp(-u);
@@ -871,6 +883,7 @@ static int energy_external(orc_sys *s, int irep, const double *x, double *E) {
 
 void orc_force(orc_sys *s, int32_t irep1, int32_t bp_mc, double *forces) {
     int irep = irep1 - 1;
+    if (s->p.bp_model != 5) bp_mc = 0;   /* models 1/3 run without the limit MC */
     const double *x = s->xyz + (size_t)irep * s->nmp * 3;
     memset(forces, 0, sizeof(double) * 3 * s->nmp);
     force_bond(s, x, forces);
@@ -1035,8 +1048,8 @@ static double energy_bp(orc_sys *s, int irep, double tK, int bp_mc, const double
             if (!bp_mc && s->p.nstep_bp_mc > 0 && !r->bp_status_mc[b]) continue;
             const int32_t *mp = r->bp.a + 7 * b;
             const sis_bp_para *bpp = &s->bpp[mp[6]];
-            double dG = r->bpcoef.a[2 * b] - tK * r->bpcoef.a[2 * b + 1];
-            if (dG >= 0.0) continue;
+            double dG = bp_prefactor(s, r, b, tK);
+            if (s->p.bp_model == 5 && dG >= 0.0) continue;
             double v[3];
             pbc_vec_d(s, F3(x, mp[0] - 1), F3(x, mp[1] - 1), v);
             double d = sqrt(dot3(v, v)) - bpp->bond_r;
@@ -1070,6 +1083,7 @@ static double energy_bp(orc_sys *s, int irep, double tK, int bp_mc, const double
 
 void orc_energy(orc_sys *s, int32_t irep1, double tK, int32_t bp_mc, int32_t bp_energy_valid, double *E) {
     int irep = irep1 - 1;
+    if (s->p.bp_model != 5) bp_mc = 0;
     rep_t *r = &s->rep[irep];
     const double *x = s->xyz + (size_t)irep * s->nmp * 3;
     for (int k = 0; k < 11; k++) E[k] = 0.0;

@@ -907,8 +907,11 @@ int sis_gpu_create(const sis_topo *topo, const sis_params *prm, const sis_option
   if (!topo || !prm || !out) return fail(SIS_ERR_ARG, "null argument");
   *out = nullptr;
   if (topo->nchains <= 0 || topo->nmp <= 0) return fail(SIS_ERR_ARG, "empty topology");
-  if (prm->bp_model != 5 && prm->bp_model != 0)
-    return fail(SIS_ERR_UNSUPPORTED, "only base-pair model 5 (triplet) is implemented on the GPU");
+  // model 5 (triplet), or models 1/3 without the limit MC: their MC form
+  // reads the base-pair type from the wrong list slot and model 4 never
+  // forms a pair (src/force_bp_limit.F90:62, src/init_bp.F90:39)
+  if (prm->bp_model != 5 && prm->bp_model != 0 && !((prm->bp_model == 1 || prm->bp_model == 3) && prm->nstep_bp_mc == 0))
+    return fail(SIS_ERR_UNSUPPORTED, "base-pair model 5, or models 1 and 3 with nstep_bp_mc = 0");
   if (prm->max_bp_per_nt < 1) return fail(SIS_ERR_ARG, "max_bp_per_nt must be >= 1");
   if (prm->integrator < 0 || prm->integrator > 2)
     return fail(SIS_ERR_ARG, "integrator must be 1 (GJF-2GJ) or 2 (Ermak-McCammon)");

@@ -327,8 +327,10 @@ __global__ void k_bp_fill(Params P, Ptrs D, const int *mask) {
     int *dst = D.bp_mp + (size_t)gb * 8;
     for (int q = 0; q < 7; q++) dst[q] = mp[q];
     dst[7] = r;
-    D.bp_coef[2 * (size_t)gb] = D.bp3_dH[h];
-    D.bp_coef[2 * (size_t)gb + 1] = D.bp3_dS[h];
+    // the triplet table exists for model 5 only, other models read zeros
+    // (src/init_bp.F90:39, src/neighbor_list.F90:220-221)
+    D.bp_coef[2 * (size_t)gb] = P.bp_model == 5 ? D.bp3_dH[h] : 0.0;
+    D.bp_coef[2 * (size_t)gb + 1] = P.bp_model == 5 ? D.bp3_dS[h] : 0.0;
     // reference list order: (chain_i, chain_j, i, j)
     // (src/neighbor_list.F90:99-114)
     D.bp_key[gb] = (((unsigned long long)(r * (int64_t)P.nchains + D.chain_id[a]) * P.nchains +
@@ -398,8 +400,13 @@ __global__ void k_bp_eval(Params P, Ptrs D, int nbp_total, const int *mc, const 
   D.ene_bp[gb] = 0.0;
   if (!mc[r] && P.nstep_bp_mc > 0 && !D.bp_status_mc[gb]) return;
   double tK = ecT ? ecT[r] : D.tK[r];
-  double dG = D.bp_coef[2 * (size_t)gb] - tK * D.bp_coef[2 * (size_t)gb + 1];
-  if (dG >= 0.0) return;
+  // model 5: dG = dH - T dS of the triplet; models 1/3: U0 exp(-u) alone
+  // (src/force_bp.F90:203)
+  double dG = 1.0;
+  if (P.bp_model == 5) {
+    dG = D.bp_coef[2 * (size_t)gb] - tK * D.bp_coef[2 * (size_t)gb + 1];
+    if (dG >= 0.0) return;
+  }
   const double *box = D.box + 3 * r;
   const double4 *x = D.pos + (size_t)r * N;
   double4 p1 = x[mp[0] - 1], p2 = x[mp[1] - 1];

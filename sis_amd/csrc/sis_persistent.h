@@ -53,8 +53,11 @@ __device__ __forceinline__ int bp_eval_one(const Params &P, const Ptrs &D, int g
   D.ene_bp[gb] = 0.0;
   if (!mc && P.nstep_bp_mc > 0 && !D.bp_status_mc[gb]) return 0;
   double tK = D.tK[r];
-  double dG = D.bp_coef[2 * (size_t)gb] - tK * D.bp_coef[2 * (size_t)gb + 1];
-  if (dG >= 0.0) return 0;
+  double dG = 1.0;   // models 1/3: U0 exp(-u) (src/force_bp.F90:203)
+  if (P.bp_model == 5) {
+    dG = D.bp_coef[2 * (size_t)gb] - tK * D.bp_coef[2 * (size_t)gb + 1];
+    if (dG >= 0.0) return 0;
+  }
   const double *box = D.box + 3 * r;
   const double4 *x = xr ? xr : D.pos + (size_t)r * N;
   double4 p1 = x[mp[0] - 1], p2 = x[mp[1] - 1];
@@ -484,8 +487,8 @@ __device__ __forceinline__ void batched_rebuild(const Params &P, const Ptrs &D, 
       const int h = bp_eligible(P, D, a, b, &bt);
       dst[6] = bt;
       dst[7] = r;
-      D.bp_coef[2 * (size_t)gb] = D.bp3_dH[h];
-      D.bp_coef[2 * (size_t)gb + 1] = D.bp3_dS[h];
+      D.bp_coef[2 * (size_t)gb] = P.bp_model == 5 ? D.bp3_dH[h] : 0.0;
+      D.bp_coef[2 * (size_t)gb + 1] = P.bp_model == 5 ? D.bp3_dS[h] : 0.0;
     }
   }
   __syncthreads();

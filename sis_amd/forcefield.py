@@ -39,6 +39,26 @@ _BP = {
 }
 
 
+def _nht22_bp(u0):
+    # para/rna_NHT22.ff:14-50; the file sets no angl_k3/k4 terms
+    return dict(U0=u0, bond_k=3.0, bond_r=13.8, angl_k1=1.5, angl_k2=1.5, angl_k3=0.0, angl_k4=0.0,
+                angl_theta1=1.8326, angl_theta2=0.9425, angl_theta3=0.0, angl_theta4=0.0,
+                dihd_k1=0.5, dihd_k2=0.5, dihd_phi1=1.8326, dihd_phi2=1.1345)
+
+
+def nht22_params(**over) -> dict:
+    """The NHT22 force field (para/rna_NHT22.ff): harmonic bond and angle,
+    no dihedral, base-pair model 1 (U0 exp(-u), no limit MC), WCA; with the
+    electrostatics of check/test_DCD_CAG47/input_DCD_CAG47.toml (295.15 K,
+    0.15 M, 50 A cutoff)."""
+    p = default_params(tempK=295.15, nl_margin=5.0, bond_k=15.0, bond_r0=5.9, angl_reb=0, angl_k=10.0,
+                       angl_t0=2.618, dih_type=0, wca_sigma=10.0, wca_eps=2.0, ionic_strength=0.15,
+                       ele_cutoff_type=1, ele_cutoff=50.0, bp_model=1, nstep_bp_mc=0, bp_cutoff_energy=-0.01,
+                       bp={"GC": _nht22_bp(-5.0), "AU": _nht22_bp(-3.33333), "GU": _nht22_bp(-3.33333)})
+    p.update(over)
+    return p
+
+
 def default_params(**over) -> dict:
     """Flat parameter dict matching struct sis_params."""
     from .abi import NNT_ORDER

@@ -166,6 +166,21 @@ def pbc_resize():
     return sysd
 
 
+CAG47_DIR = HERE / "cag47"   # data of the reference's check/test_DCD_CAG47 (OpenMM-written DCD)
+
+
+def cag47_system(frame=50, model=1):
+    """CAG x 47 (141 nt) from frame `frame` of CAG47.dcd with the NHT22
+    force field (para/rna_NHT22.ff): base-pair model 1 without limit MC."""
+    from sis_amd.engine import read_dcd_file
+    from sis_amd.forcefield import nht22_params
+    seq = "".join(l.strip() for l in open(CAG47_DIR / "CAG47.fasta") if not l.startswith(">"))
+    frames, _ = read_dcd_file(CAG47_DIR / "CAG47.dcd", len(seq))
+    sysd = configs._mk("CAG47", [seq], frames[frame], nht22_params(bp_model=model))
+    sysd["name"] = "cag47_m1"
+    return sysd
+
+
 SYSTEMS = {
     "hairpin64": (lambda: configs.hairpin64(1), 60, 0),
     "small_pbc": (lambda: configs.small_pbc(4, 30, 100.0, seed=5), 60, 0),
@@ -177,6 +192,7 @@ SYSTEMS = {
     "hairpin_ss": (hairpin_ss, 60, 0),
     "hairpin_anneal": (hairpin_anneal, 300, 0),
     "pbc_resize": (pbc_resize, 60, 0),
+    "cag47_m1": (cag47_system, 40, 0),
 }
 
 
@@ -196,7 +212,8 @@ def make(name, builder, nstep, nex, seed=4321):
     if s.get("resize") is not None:
         pre.append(("resize",) + tuple(s["resize"]))
     ops = pre + ops + [("nl", r) for r in range(1, nrep + 1)]
-    ops += [("lists", 1, "L"), ("force", 1, 1, "F"), ("bp_state", 1, "B"),
+    fmc = 1 if s["params"]["bp_model"] == 5 else 0   # models 1/3 run without the limit MC
+    ops += [("lists", 1, "L"), ("force", 1, fmc, "F"), ("bp_state", 1, "B"),
             ("energy", 1, float(s["temps"][0]), 0, 1, "E"), ("ele", 1, "ele"), ("bp_tables", "T")]
     ref0 = refdrv.run(s, ops, seed=seed, nstep_exchange=nex)
     ref1 = refdrv.run(s, pre + [("job_md", nstep, "R")], seed=seed, nstep_exchange=nex)
@@ -216,7 +233,7 @@ def make(name, builder, nstep, nex, seed=4321):
         o.set_tape(-1, ref0["tape_rep"])
     for r in range(1, nrep + 1):
         o.neighbor_list(r)
-    o.force(1, 1)
+    o.force(1, fmc)
     used_static = [o.tape_pos(r) for r in range(1, nrep + 1)]
     o2 = corc.Oracle(s)
     if ext:
@@ -252,6 +269,9 @@ def make(name, builder, nstep, nex, seed=4321):
     out["param_names"] = np.array(sorted(scal))
     out["param_values"] = np.array([float(scal[k]) for k in sorted(scal)])
     out["box"] = np.array(p["box"], float)
+    # per-type base-pair parameters (GC, AU, GU) in BP_FIELDS order
+    from sis_amd.abi import BP_FIELDS
+    out["bp_params"] = np.array([[float(p["bp"][t].get(f, 0.0)) for f in BP_FIELDS] for t in ("GC", "AU", "GU")])
     for k, v in (ext or {}).items():
         out["ext_" + k] = np.asarray(v)
     if s.get("ss") is not None:
@@ -281,6 +301,9 @@ def load(name):
                 seq=d["seq"], xyz=d["xyz"], params=p, nrep=int(d["nrep"]), temps=d["temps"],
                 nstep=int(d["nstep"]), seqs=seqs)
     p["box"] = [float(b) for b in d["box"]]
+    if "bp_params" in d:
+        from sis_amd.abi import BP_FIELDS
+        p["bp"] = {t: {f: float(v) for f, v in zip(BP_FIELDS, row)} for t, row in zip(("GC", "AU", "GU"), d["bp_params"])}
     ext = {k[4:]: d[k] for k in d if k.startswith("ext_")}
     if ext:
         for k in ("stage", "rg"):

@@ -227,6 +227,28 @@ def test_out_lines_during_steps(tmp_path, case):
         assert paths[r].read_text() == want.read_text(), r
 
 
+def test_cag47_dcd_energies_on_device():
+    """job_dcd on the device over the 100 OpenMM frames of CAG x 47 (NHT22,
+    model-1 base pairs): per-frame energies equal the oracle's (1e-9) and
+    the OpenMM columns within the tolerances of the oracle's own test."""
+    from oracle import corc
+    from tests.golden.make_golden import CAG47_DIR
+    from tests.test_oracle import CAG47_COLS, cag47_frames_and_reference
+    s, frames, ref = cag47_frames_and_reference()
+    print("[case] cag47 dcd energies", flush=True)
+    eng = Engine(s, mode="phases", precision=0)
+    E = eng.dcd_energies(CAG47_DIR / "CAG47.dcd")
+    assert E.shape == (100, 11)
+    o = corc.Oracle(s)
+    for k in (0, 37, 99):
+        o.set_xyz(frames[k][None])
+        o.neighbor_list(1)
+        eo = o.energy(1, 295.15, 0, 0)
+        assert np.allclose(E[k], eo, rtol=1e-9, atol=1e-9), k
+    for name, ci, ei, rtol, atol in CAG47_COLS:
+        assert np.allclose(E[:, ei], ref[:, ci], rtol=rtol, atol=atol), name
+
+
 def test_dcd_energy_analysis(tmp_path):
     """job_dcd: energies of the frames of a DCD file equal those of the same
     (float32-rounded) coordinates set by hand, and the .out lines are those

@@ -10,7 +10,7 @@ from sis_amd import configs
 from oracle import corc, refdrv
 from tests.golden.make_golden import load
 
-FIXTURES = ["hairpin64", "small_pbc", "mixed", "remd4", "remd4mc3", "remd4bd", "remd2ext", "hairpin_ss", "hairpin_anneal", "pbc_resize"]
+FIXTURES = ["hairpin64", "small_pbc", "mixed", "remd4", "remd4mc3", "remd4bd", "remd2ext", "hairpin_ss", "hairpin_anneal", "pbc_resize", "cag47_m1"]
 
 
 def _oracle_from_fixture(name):
@@ -66,7 +66,8 @@ def test_tables(name):
     dh, ds, cut = o.tables()
     assert np.array_equal(dh, d["bp3_dH"]) and np.array_equal(ds, d["bp3_dS"])
     assert np.array_equal(cut, d["bp_cut"])
-    assert np.count_nonzero(dh) > 400   # 468 triplets (src/init_bp.F90:170-171)
+    if sysd["params"]["bp_model"] == 5:
+        assert np.count_nonzero(dh) > 400   # 468 triplets (src/init_bp.F90:170-171)
 
 
 @pytest.mark.parametrize("name", FIXTURES)
@@ -264,6 +265,69 @@ def test_external_terms_are_energy_gradients():
             xm[0, a, c] -= h
             g = (eext(o1, xp) - eext(o1, xm)) / (2 * h)
             assert abs(f[a, c] + g) <= 1e-5 * max(1.0, abs(g)), (a, c, f[a, c], -g)
+
+
+# OpenMM energies of check/test_DCD_CAG47 (tests/golden/cag47): bond, angle
+# and WCA agree to float32-coordinate rounding; the electrostatic energy
+# differs by < 0.5% (the run used the alternative dielectric of
+# -D_HTN_CONSISTENT, src/set_ele.F90:29-44, or its own constants); the
+# base-pair column comes from a different base-pair model than the stale
+# input selects and is not compared.
+CAG47_COLS = (("bond", 1, 1, 1e-4, 0.0), ("angl", 2, 2, 1e-4, 0.0), ("exv", 3, 5, 0.0, 1e-3),
+              ("ele", 4, 6, 1e-2, 0.0))
+
+
+def cag47_frames_and_reference():
+    from sis_amd.engine import read_dcd_file
+    from tests.golden.make_golden import CAG47_DIR, cag47_system
+    s = cag47_system()
+    frames, _ = read_dcd_file(CAG47_DIR / "CAG47.dcd", 141)
+    return s, frames, np.loadtxt(CAG47_DIR / "CAG47.energy.out")
+
+
+def test_cag47_openmm_energies():
+    """job_dcd over the 100 OpenMM frames of CAG x 47 with the NHT22 force
+    field (harmonic angle, model-1 base pairs): the oracle's energies
+    against the OpenMM column of each term."""
+    s, frames, ref = cag47_frames_and_reference()
+    o = corc.Oracle(s)
+    E = []
+    for x in frames:
+        o.set_xyz(x[None])
+        o.neighbor_list(1)
+        E.append(o.energy(1, 295.15, 0, 0))
+    E = np.array(E)
+    for name, ci, ei, rtol, atol in CAG47_COLS:
+        assert np.allclose(E[:, ei], ref[:, ci], rtol=rtol, atol=atol), name
+    assert np.all(E[:, 4] < 0)   # base pairs form in every frame
+
+
+@pytest.mark.ref
+@pytest.mark.parametrize("model", [1, 3])
+def test_live_reference_base_pair_models_1_and_3(model):
+    """Base-pair models 1 and 3 without the limit MC (src/force_bp.F90,
+    src/energy_bp.F90; model 3 adds the isolated-pair rule of
+    src/init_bp.F90): lists, forces, energies and a 40-step trajectory of
+    the oracle against the reference build."""
+    from tests.golden.make_golden import cag47_system
+    s = cag47_system(frame=50, model=model)
+    ops = [("tape", 1, 400000, "t"), ("nl", 1), ("lists", 1, "L"), ("force", 1, 0, "F"),
+           ("energy", 1, 295.15, 0, 0, "E")]
+    ref = refdrv.run(s, ops, seed=7)
+    ref1 = refdrv.run(s, [("job_md", 40, "R")], seed=7)["R"]
+    o = corc.Oracle(s)
+    o.set_tape(1, ref["t"])
+    o.neighbor_list(1)
+    assert np.array_equal(o.lists(1)["bp"], ref["L"]["bp"])
+    f = o.force(1, 0)
+    assert np.abs(f - ref["F"]).max() / np.abs(ref["F"]).max() < 1e-12
+    e = o.energy(1, 295.15, 0, 0)
+    assert np.allclose(e, ref["E"], rtol=1e-12, atol=1e-12) and e[4] < 0
+    o2 = corc.Oracle(s)
+    o2.set_tape(1, ref["t"])
+    o2.init_velocities()
+    o2.md(40, 0)
+    assert np.abs(o2.state()[0] - ref1["xyz"]).max() < 1e-10
 
 
 @pytest.mark.ref
