@@ -214,6 +214,13 @@ int sis_gpu_set_external(sis_gpu_ctx *ctx, const sis_external *ext);
  * resize_step <= 0 turns it off.  Needs a periodic box. */
 int sis_gpu_set_box_resize(sis_gpu_ctx *ctx, int64_t resize_step, const double *change, const double *target);
 
+/* Timed Rg bias (read_timed_bias_rg, src/read_timed_bias_rg.F90;
+ * src/job_md.F90:231-265,614-627): entry i sets the Rg-bias constant k[i]
+ * and target r0[i] of every replica from step steps[i] on.  Needs the Rg
+ * bias of sis_gpu_set_external; a later sis_gpu_set_external drops the
+ * schedule, so call this after it.  n <= 0 stops it. */
+int sis_gpu_set_rg_schedule(sis_gpu_ctx *ctx, int32_t n, const int64_t *steps, const double *k, const double *r0);
+
 /* Simulated annealing (read_anneal's schedule, src/read_anneal.F90;
  * src/main.F90:90-97, src/job_md.F90:207-230,600-612): entry k sets the
  * temperature temps[k] from step steps[k] on (steps increasing; the first

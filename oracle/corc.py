@@ -47,6 +47,7 @@ def lib():
         L.orc_set_external.argtypes = [vp, P(abi.SisExternal)]
         L.orc_set_step.argtypes = [vp, i64]
         L.orc_set_anneal.argtypes = [vp, i32, P(i64), P(d)]
+        L.orc_set_rg_schedule.argtypes = [vp, i32, P(i64), P(d), P(d)]
         L.orc_set_box_resize.argtypes = [vp, i64, P(d), P(d)]
         L.orc_get_box.argtypes = [vp, P(d)]
         L.orc_set_ss.restype = i32
@@ -204,6 +205,12 @@ class Oracle:
         b = np.zeros(3)
         lib().orc_get_box(self.h, _d(b))
         return b
+
+    def set_rg_schedule(self, steps, k, r0):
+        st = np.ascontiguousarray(steps, np.int64)
+        kk = np.ascontiguousarray(k, np.float64)
+        rr = np.ascontiguousarray(r0, np.float64)
+        lib().orc_set_rg_schedule(self.h, len(st), st.ctypes.data_as(C.POINTER(C.c_int64)), _d(kk), _d(rr))
 
     def set_anneal(self, steps, temps):
         st = np.ascontiguousarray(steps, np.int64)

@@ -591,6 +591,23 @@ contains
       box(:) = pbc_box(:)
    end subroutine sisref_get_box
 
+   ! A timed Rg-bias schedule file for job_md's read_timed_bias_rg
+   ! (src/job_md.F90:231-265): flg_timed_bias_rg and the file name.
+   subroutine sisref_set_rg_schedule(path, plen) bind(C, name="sisref_set_rg_schedule")
+      use var_io, only : cfile_bias_rg_in
+      use var_potential, only : flg_timed_bias_rg
+      character(kind=c_char), intent(in) :: path(*)
+      integer(c_int), value :: plen
+      character(len=:), allocatable :: name
+      integer :: k
+      allocate(character(len=plen) :: name)
+      do k = 1, plen
+         name(k:k) = path(k)
+      enddo
+      cfile_bias_rg_in = name
+      flg_timed_bias_rg = .True.
+   end subroutine sisref_set_rg_schedule
+
    subroutine sisref_ele_params(irep, out) bind(C, name="sisref_ele_params")
       use var_state, only : lambdaD, diele
       use var_potential, only : ele_coef, ele_cutoff

@@ -165,6 +165,12 @@ def _worker(q, so, sysd, ops, seed, nstep_exchange):
                     _ip(fp), _dp(fk), _dp(fi), _dp(fv), C.c_int(int(ext.get("rg", 0))),
                     C.c_double(float(ext.get("rg_k", 0.0))), _dp(rg0), C.c_int(nr), _ip(ri), _dp(rc), _dp(rp),
                     C.c_int64(int(op[2])))
+            elif kind == "rg_schedule":
+                fn = os.path.join(tmp, "rg.sched")
+                with open(fn, "w") as fh:
+                    for st_, k_, r_ in zip(op[1], op[2], op[3]):
+                        fh.write(f"{int(st_)} {float(k_)!r} {float(r_)!r}\n")
+                lib.sisref_set_rg_schedule(C.c_char_p(fn.encode()), C.c_int(len(fn)))
             elif kind == "resize":
                 ch = np.ascontiguousarray(op[2], np.float64)
                 tg = np.ascontiguousarray(op[3] if op[3] is not None else np.zeros(3), np.float64)

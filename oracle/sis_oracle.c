@@ -79,6 +79,8 @@ struct orc_sys {
     /* annealing: schedule, current entry (1-based), next switch step, and
      * the kT of the force's limit MC (var_state's kT, never updated) */
     int an_n, ianneal; int64_t *an_step; double *an_T; int64_t an_next; double mc_kT;
+    /* timed Rg bias (src/job_md.F90:231-265,614-627) */
+    int rgs_n, irgs; int64_t *rgs_step, rgs_next; double *rgs_k, *rgs_0;
     /* box resizing (src/pbc.F90:79-138) */
     int rz_on, rz_has_target; int64_t rz_step, rz_count[3]; double rz_change[3], rz_target[3], rz_orig[3];
     double *x_rg0, *x_dcf_f, *x_fr_k, *x_fr_init, *x_fr_velo, *x_rest_rcut, *x_rest_para;
@@ -918,6 +920,29 @@ void orc_get_box(orc_sys *s, double *box) { for (int k = 0; k < 3; k++) box[k] =
  * closer than the minimum loop, isolated pairs (models 3-5) and
  * non-canonical pairs are dropped.  npair < 0 clears the restriction.
  * Returns the number of pairs kept. */
+static void rg_schedule_apply(orc_sys *s) {
+    s->x.rg_k = s->rgs_k[s->irgs - 1];
+    for (int r = 0; r < s->nrep; r++) s->x_rg0[r] = s->rgs_0[s->irgs - 1];
+    s->rgs_next = s->irgs < s->rgs_n ? s->rgs_step[s->irgs] : 0;
+}
+
+/* Timed Rg bias (read_timed_bias_rg; job_md's set-up and its switch at the
+ * end of step next - 1).  Needs the Rg bias of orc_set_external. */
+void orc_set_rg_schedule(orc_sys *s, int32_t n, const int64_t *steps, const double *k, const double *r0) {
+    free(s->rgs_step); free(s->rgs_k); free(s->rgs_0);
+    s->rgs_step = NULL; s->rgs_k = NULL; s->rgs_0 = NULL; s->rgs_n = 0; s->rgs_next = 0;
+    if (n <= 0 || !s->x_on || !s->x.rg) return;
+    s->rgs_n = n;
+    s->rgs_step = malloc(sizeof(int64_t) * n); memcpy(s->rgs_step, steps, sizeof(int64_t) * n);
+    s->rgs_k = malloc(sizeof(double) * n); memcpy(s->rgs_k, k, sizeof(double) * n);
+    s->rgs_0 = malloc(sizeof(double) * n); memcpy(s->rgs_0, r0, sizeof(double) * n);
+    int i = 1;
+    if (s->istep >= steps[n - 1]) i = n;
+    else while (i < n && steps[i] <= s->istep) i++;
+    s->irgs = i;
+    rg_schedule_apply(s);
+}
+
 /* [PBC_box] resizing: pbc_resize (src/pbc.F90:79-138) every resize_step
  * steps; target may be NULL */
 void orc_set_box_resize(orc_sys *s, int64_t resize_step, const double *change, const double *target) {
@@ -1262,6 +1287,7 @@ void orc_destroy(orc_sys *s) {
     free_external(s);
     free(s->ss_map);
     free(s->an_step); free(s->an_T);
+    free(s->rgs_step); free(s->rgs_k); free(s->rgs_0);
     for (int r = 0; r < s->nrep; r++) {
         free(s->rep[r].wca.a); free(s->rep[r].ele.a); free(s->rep[r].bp.a); free(s->rep[r].bpcoef.a);
         free(s->rep[r].bp_status); free(s->rep[r].bp_status_mc); free(s->rep[r].ene_bp);
@@ -1511,6 +1537,10 @@ int64_t orc_md(orc_sys *s, int64_t nsteps, int32_t nstep_exchange, double *energ
             s->ianneal++;
             for (int r = 0; r < nr; r++) s->rep[r].tK = s->an_T[s->ianneal - 1];
             s->an_next = s->ianneal < s->an_n ? s->an_step[s->ianneal] : 0;
+        }
+        if (s->rgs_n > 0 && s->rgs_next > 0 && s->istep + 1 == s->rgs_next) {
+            s->irgs++;
+            rg_schedule_apply(s);
         }
     }
     free(forces); free(rnd);

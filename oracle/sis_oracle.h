@@ -42,6 +42,7 @@ void orc_set_external(orc_sys *s, const sis_external *ext);   /* NULL clears */
 void orc_set_step(orc_sys *s, int64_t istep);
 void orc_get_box(orc_sys *s, double *box);
 void orc_set_box_resize(orc_sys *s, int64_t resize_step, const double *change, const double *target);
+void orc_set_rg_schedule(orc_sys *s, int32_t n, const int64_t *steps, const double *k, const double *r0);
 void orc_set_anneal(orc_sys *s, int32_t n, const int64_t *steps, const double *temps);
 int32_t orc_set_ss(orc_sys *s, int32_t npair, const int32_t *pairs);   /* (2, npair) 1-based; npair < 0 clears */
 #endif

@@ -93,6 +93,11 @@ struct sis_gpu_ctx {
   int ianneal = 0;            // 1-based index of the current schedule entry
   int64_t an_next = 0;        // step whose start takes the next entry, 0 = none
   int rst_ianneal = 0;        // ANNEAL block of a restart file read, or 0
+  // timed Rg-bias schedule (sis_gpu_set_rg_schedule)
+  std::vector<int64_t> rgs_step;
+  std::vector<double> rgs_k, rgs_0;
+  int irgs = 0;
+  int64_t rgs_next = 0;       // step whose start takes the next entry, 0 = none
   // box resizing (sis_gpu_set_box_resize; src/pbc.F90:79-138)
   bool rz_on = false, rz_has_target = false;
   int64_t rz_step = 0;
@@ -817,6 +822,17 @@ void do_exchange(sis_gpu_ctx *c, double *replica_energies, int32_t *accepted, do
 // per phase.
 void dcd_set_box(sis_gpu_ctx *c);   // sis_output_host.h
 
+// current entry of the timed Rg bias: k and the target of every replica
+void rg_schedule_apply(sis_gpu_ctx *c) {
+  const int n = (int)c->rgs_step.size();
+  c->X.rg_k = c->rgs_k[c->irgs - 1];
+  std::vector<double> r0(c->R, c->rgs_0[c->irgs - 1]);
+  // on the engine's stream: the steps already queued read the old targets
+  HIPCHK(hipMemcpyAsync(c->x_rg0_d.p, r0.data(), c->R * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  c->rgs_next = c->irgs < n ? c->rgs_step[c->irgs] : 0;
+}
+
 // The resize block of src/job_md.F90:576-596: new box, then every
 // replica's lists rebuilt and displacements zeroed; the next step's skin
 // test starts from zero and does not force the limit MC
@@ -1465,6 +1481,7 @@ int sis_gpu_step(sis_gpu_ctx *c, int64_t nsteps, int64_t *rebuilds) {
     // the next annealing entry is taken at the end of step an_next - 1
     // (src/job_md.F90:600-612)
     const bool an = c->anneal_on && c->an_next > 0 && c->an_next - 1 > c->istep;
+    const bool rgs = c->ext_on && c->X.rg && c->rgs_next > 0 && c->rgs_next - 1 > c->istep;
     int64_t k = left;
     if (ex) k = std::min(k, ex - c->istep % ex);
     if (sv) k = std::min(k, sv - c->istep % sv);
@@ -1473,6 +1490,7 @@ int sis_gpu_step(sis_gpu_ctx *c, int64_t nsteps, int64_t *rebuilds) {
     if (rs) k = std::min(k, rs - c->istep % rs);
     if (an) k = std::min(k, c->an_next - 1 - c->istep);
     if (rz) k = std::min(k, rz - c->istep % rz);
+    if (rgs) k = std::min(k, c->rgs_next - 1 - c->istep);
     run(k);
     left -= k;
     const bool exch = ex && c->istep % ex == 0;
@@ -1507,6 +1525,10 @@ int sis_gpu_step(sis_gpu_ctx *c, int64_t nsteps, int64_t *rebuilds) {
       c->an_next = c->ianneal < n ? c->an_step[c->ianneal] : 0;
       upload_conditions(c);
     }
+    if (rgs && c->istep + 1 == c->rgs_next) {   // src/job_md.F90:617-627
+      c->irgs++;
+      rg_schedule_apply(c);
+    }
     if (rs && c->istep % rs == 0) rst_write_all(c, c->rst->paths);
   }
   HIPCHK(hipEventRecord(c->ev1, c->stream));
@@ -1535,6 +1557,30 @@ int sis_gpu_set_step(sis_gpu_ctx *c, int64_t istep) {
   if (!c) return fail(SIS_ERR_ARG, "null context");
   if (istep < 0) return fail(SIS_ERR_ARG, "istep < 0");
   c->istep = istep;
+  return SIS_OK;
+  API_END
+}
+
+int sis_gpu_set_rg_schedule(sis_gpu_ctx *c, int32_t n, const int64_t *steps, const double *k, const double *r0) {
+  API_BEGIN
+  if (!c) return fail(SIS_ERR_ARG, "null context");
+  if (n <= 0) { c->rgs_next = 0; c->rgs_step.clear(); return SIS_OK; }
+  if (!steps || !k || !r0) return fail(SIS_ERR_ARG, "null schedule");
+  if (!(c->ext_on && c->X.rg)) return fail(SIS_ERR_STATE, "the Rg bias is not set (sis_gpu_set_external)");
+  for (int i = 0; i < n; i++) {
+    // read_timed_bias_rg's checks (src/read_timed_bias_rg.F90:31-37)
+    if (steps[i] < 0 || k[i] < 0.0 || !(r0[i] > 0.0)) return fail(SIS_ERR_ARG, "invalid Rg-bias schedule entry");
+    if (i > 0 && steps[i] <= steps[i - 1]) return fail(SIS_ERR_ARG, "Rg-bias schedule steps must increase");
+  }
+  c->rgs_step.assign(steps, steps + n);
+  c->rgs_k.assign(k, k + n);
+  c->rgs_0.assign(r0, r0 + n);
+  // job_md's set-up (src/job_md.F90:231-265): the entry of the current step
+  int i = 1;
+  if (c->istep >= steps[n - 1]) i = n;
+  else while (i < n && steps[i] <= c->istep) i++;
+  c->irgs = i;
+  rg_schedule_apply(c);
   return SIS_OK;
   API_END
 }
@@ -1654,6 +1700,8 @@ int sis_gpu_set_external(sis_gpu_ctx *c, const sis_external *e) {
   API_BEGIN
   if (!c) return fail(SIS_ERR_ARG, "null context");
   const int N = c->N, R = c->R;
+  c->rgs_next = 0;   // a timed Rg bias is set again after the terms
+  c->rgs_step.clear();
   if (!e || (!e->stage && e->n_dcf == 0 && e->n_fr == 0 && !e->rg && e->n_rest == 0)) {
     c->ext_on = false;
     c->X = ExtDev{};

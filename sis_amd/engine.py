@@ -232,6 +232,14 @@ class Engine:
         tk = np.ascontiguousarray(temps, np.float64)
         _check(lib().sis_gpu_set_anneal(self.h, len(st), st.ctypes.data_as(C.POINTER(C.c_int64)), abi.dptr(tk)))
 
+    def set_rg_schedule(self, steps, k, r0):
+        """Timed Rg bias: (k[i], r0[i]) from step steps[i] on."""
+        st = np.ascontiguousarray(steps, np.int64)
+        kk = np.ascontiguousarray(k, np.float64)
+        rr = np.ascontiguousarray(r0, np.float64)
+        _check(lib().sis_gpu_set_rg_schedule(self.h, len(st), st.ctypes.data_as(C.POINTER(C.c_int64)),
+                                             abi.dptr(kk), abi.dptr(rr)))
+
     def set_box_resize(self, resize_step: int, change, target=None):
         """[PBC_box] resizing: every resize_step steps the box moves by
         change per axis, up to target (None: no target)."""

@@ -237,6 +237,15 @@ module sis_gpu_iface
          type(c_ptr), value :: ctx
       end function
 
+      ! timed Rg bias (read_timed_bias_rg)
+      integer(c_int) function sis_gpu_set_rg_schedule(ctx, n, steps, k, r0) bind(C, name="sis_gpu_set_rg_schedule")
+         import :: c_int, c_ptr, c_int32_t, c_int64_t, c_double
+         type(c_ptr), value :: ctx
+         integer(c_int32_t), value :: n
+         integer(c_int64_t), intent(in) :: steps(*)
+         real(c_double), intent(in) :: k(*), r0(*)
+      end function
+
       ! box resizing ([PBC_box] resize_step / resize_change / resize_target)
       integer(c_int) function sis_gpu_set_box_resize(ctx, resize_step, change, target) &
             bind(C, name="sis_gpu_set_box_resize")

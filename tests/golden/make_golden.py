@@ -181,6 +181,19 @@ def cag47_system(frame=50, model=1):
     return sysd
 
 
+def remd_rg_schedule():
+    """remd2ext with a timed Rg bias (read_timed_bias_rg; k and target
+    switch at steps 10 and 20, src/job_md.F90:614-627)."""
+    sysd = remd_external()
+    sysd["rg_schedule"] = (np.array([0, 10, 20], np.int64), np.array([0.8, 1.5, 0.3]),
+                           np.array([12.0, 9.0, 14.0]))
+    # the static part of the fixture sees the terms before job_md applies
+    # the schedule's first entry: start from that entry
+    sysd["external"] = dict(sysd["external"], rg_k=0.8, rg_0=np.array([12.0, 12.0]))
+    sysd["name"] = "remd2rgs"
+    return sysd
+
+
 SYSTEMS = {
     "hairpin64": (lambda: configs.hairpin64(1), 60, 0),
     "small_pbc": (lambda: configs.small_pbc(4, 30, 100.0, seed=5), 60, 0),
@@ -193,6 +206,7 @@ SYSTEMS = {
     "hairpin_anneal": (hairpin_anneal, 300, 0),
     "pbc_resize": (pbc_resize, 60, 0),
     "cag47_m1": (cag47_system, 40, 0),
+    "remd2rgs": (remd_rg_schedule, 30, 10),
 }
 
 
@@ -205,6 +219,8 @@ def make(name, builder, nstep, nex, seed=4321):
         ops.append(("tape", -1, 4096, "tape_rep"))
     ext = s.get("external")
     pre = [("external", ext, 0)] if ext else []
+    if s.get("rg_schedule") is not None:
+        pre.append(("rg_schedule",) + tuple(s["rg_schedule"]))
     if s.get("ss") is not None:
         pre.append(("ss", s["ss"]))
     if s.get("anneal") is not None:
@@ -221,6 +237,8 @@ def make(name, builder, nstep, nex, seed=4321):
     o = corc.Oracle(s)
     if ext:
         o.set_external(ext)
+    if s.get("rg_schedule") is not None:
+        o.set_rg_schedule(*s["rg_schedule"])
     if s.get("ss") is not None:
         o.set_ss(s["ss"])
     if s.get("anneal") is not None:
@@ -238,6 +256,8 @@ def make(name, builder, nstep, nex, seed=4321):
     o2 = corc.Oracle(s)
     if ext:
         o2.set_external(ext)
+    if s.get("rg_schedule") is not None:
+        o2.set_rg_schedule(*s["rg_schedule"])
     if s.get("ss") is not None:
         o2.set_ss(s["ss"])
     if s.get("anneal") is not None:
@@ -276,6 +296,8 @@ def make(name, builder, nstep, nex, seed=4321):
         out["ext_" + k] = np.asarray(v)
     if s.get("ss") is not None:
         out["ss_pairs"] = np.asarray(s["ss"], np.int32)
+    if s.get("rg_schedule") is not None:
+        out["rgs_steps"], out["rgs_k"], out["rgs_0"] = [np.asarray(a) for a in s["rg_schedule"]]
     if s.get("resize") is not None:
         out["resize_step"] = np.int64(s["resize"][0])
         out["resize_change"] = np.asarray(s["resize"][1], np.float64)
@@ -313,6 +335,8 @@ def load(name):
         sysd["external"] = ext
     if "ss_pairs" in d:
         sysd["ss"] = d["ss_pairs"]
+    if "rgs_steps" in d:
+        sysd["rg_schedule"] = (d["rgs_steps"], d["rgs_k"], d["rgs_0"])
     if "resize_step" in d:
         sysd["resize"] = (int(d["resize_step"]), d["resize_change"], d["resize_target"])
     if "anneal_steps" in d:
