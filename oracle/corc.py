@@ -47,6 +47,7 @@ def lib():
         L.orc_set_external.argtypes = [vp, P(abi.SisExternal)]
         L.orc_set_step.argtypes = [vp, i64]
         L.orc_set_anneal.argtypes = [vp, i32, P(i64), P(d)]
+        L.orc_set_ionic.argtypes = [vp, P(d)]
         L.orc_set_rg_schedule.argtypes = [vp, i32, P(i64), P(d), P(d)]
         L.orc_set_box_resize.argtypes = [vp, i64, P(d), P(d)]
         L.orc_get_box.argtypes = [vp, P(d)]
@@ -211,6 +212,11 @@ class Oracle:
         kk = np.ascontiguousarray(k, np.float64)
         rr = np.ascontiguousarray(r0, np.float64)
         lib().orc_set_rg_schedule(self.h, len(st), st.ctypes.data_as(C.POINTER(C.c_int64)), _d(kk), _d(rr))
+
+    def set_ionic(self, ionic):
+        """Per-label ionic strengths (REMD over REPT%ION)."""
+        self._ion = np.ascontiguousarray(ionic, np.float64)
+        lib().orc_set_ionic(self.h, _d(self._ion))
 
     def set_anneal(self, steps, temps):
         st = np.ascontiguousarray(steps, np.int64)

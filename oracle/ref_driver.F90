@@ -52,7 +52,21 @@ module sisref_driver
       integer(c_int32_t) :: pbc, integrator
    end type c_params
 
+   ! REMD over the ionic strength instead of the temperature: set by
+   ! sisref_remd_ion before sisref_setup
+   logical, save :: remd_ion = .False.
+   real(c_double), allocatable, save :: ion_values(:)
+
 contains
+
+   subroutine sisref_remd_ion(n, values) bind(C, name="sisref_remd_ion")
+      integer(c_int), value :: n
+      real(c_double), intent(in) :: values(n)
+      remd_ion = .True.
+      if (allocated(ion_values)) deallocate(ion_values)
+      allocate(ion_values(n))
+      ion_values(:) = values(:)
+   end subroutine sisref_remd_ion
 
    subroutine copy_bp(src, dst)
       use var_potential, only : basepair_parameters
@@ -178,12 +192,20 @@ contains
       if (nrep_in > 1) then
          flg_replica = .True.
          flg_exchange = .True.
-         flg_repvar(REPT%TEMP) = .True.
-         nrep(REPT%TEMP) = nrep_in
          replica_values(:,:) = 0.0_PREC
-         do irep = 1, nrep_in
-            replica_values(irep, REPT%TEMP) = rep_temps(irep)
-         enddo
+         if (remd_ion) then      ! [Replica] ion_strength ladder
+            flg_repvar(REPT%ION) = .True.
+            nrep(REPT%ION) = nrep_in
+            do irep = 1, nrep_in
+               replica_values(irep, REPT%ION) = ion_values(irep)
+            enddo
+         else
+            flg_repvar(REPT%TEMP) = .True.
+            nrep(REPT%TEMP) = nrep_in
+            do irep = 1, nrep_in
+               replica_values(irep, REPT%TEMP) = rep_temps(irep)
+            enddo
+         endif
          nstep_rep_exchange = nstep_rep_exchange_in
          nstep_rep_save = nstep_rep_exchange_in
       else

@@ -41,6 +41,9 @@ def _setup(lib, sysd, prefix, seed, nstep_exchange=0):
     seq = np.ascontiguousarray(sysd["seq"], np.int32)
     temps = np.ascontiguousarray(sysd["temps"], np.float64)
     xyz = np.ascontiguousarray(sysd["xyz"], np.float64)
+    if sysd.get("remd_ion") is not None:   # ladder over the ionic strength
+        ion = np.ascontiguousarray(sysd["remd_ion"], np.float64)
+        lib.sisref_remd_ion(C.c_int(len(ion)), _dp(ion))
     lib.sisref_setup(C.byref(p), C.c_int(int(sysd["nchains"])), _ip(nmp_chain), _ip(molt),
                      _ip(seq), C.c_int(int(sysd["nrep"])), _dp(temps), C.c_int(int(seed)),
                      _dp(xyz), prefix.encode() + b"\0", C.c_int64(int(nstep_exchange)))

@@ -49,6 +49,7 @@ typedef struct {
     int have_status_mc;
     double lambdaD, ele_coef, ele_cutoff, diele;
     double tK;               /* temperature of this replica */
+    double ionic;            /* ionic strength of this replica */
     tape_t tape;
 } rep_t;
 
@@ -68,6 +69,7 @@ struct orc_sys {
     int bm_stored; double bm_value;
     /* replica exchange */
     int *rep2lab, *lab2rep; double *lab2val; int idx_pair; int *pair_tb; /* [2][nrep+1] */
+    double *lab2ion;         /* ionic strength of each label (REMD over REPT%ION) */
     tape_t tape_rep;
     int64_t istep;
     int flg_bp_energy;
@@ -989,7 +991,7 @@ void orc_set_anneal(orc_sys *s, int32_t n, const int64_t *steps, const double *t
     s->ianneal = s->istep >= steps[n - 1] ? n : 1;
     s->an_next = s->ianneal < n ? steps[s->ianneal] : 0;
     for (int r = 0; r < s->nrep; r++) {
-        set_ele(s, r, temps[0], s->p.ionic_strength);
+        set_ele(s, r, temps[0], s->rep[r].ionic);
         s->rep[r].tK = temps[s->ianneal - 1];
     }
     s->mc_kT = BOLTZ_KCAL_MOL * s->p.tempK;
@@ -1261,6 +1263,8 @@ orc_sys *orc_create(const sis_topo *topo, const sis_params *prm, int32_t nrep, c
     s->rep2lab = malloc(sizeof(int) * (nrep + 1));
     s->lab2rep = malloc(sizeof(int) * (nrep + 1));
     s->lab2val = malloc(sizeof(double) * (nrep + 1));
+    s->lab2ion = malloc(sizeof(double) * (nrep + 1));
+    for (int k = 0; k <= nrep; k++) s->lab2ion[k] = prm->ionic_strength;
     s->pair_tb = calloc(2 * (nrep + 2), sizeof(int));
     for (int k = 1; k <= nrep; k++) { s->rep2lab[k] = k; s->lab2rep[k] = k; s->lab2val[k] = temps ? temps[k - 1] : prm->tempK; }
     for (int par = 0; par < 2; par++) {
@@ -1273,13 +1277,25 @@ orc_sys *orc_create(const sis_topo *topo, const sis_params *prm, int32_t nrep, c
     s->idx_pair = 0;
     for (int r = 0; r < nrep; r++) {
         s->rep[r].tK = (nrep > 1) ? s->lab2val[s->rep2lab[r + 1]] : prm->tempK;
-        set_ele(s, r, s->rep[r].tK, prm->ionic_strength);
+        s->rep[r].ionic = prm->ionic_strength;
+        set_ele(s, r, s->rep[r].tK, s->rep[r].ionic);
     }
     return s;
 }
 
 void orc_set_temps(orc_sys *s, const double *temps) {
-    for (int r = 0; r < s->nrep; r++) { s->rep[r].tK = temps[r]; set_ele(s, r, temps[r], s->p.ionic_strength); }
+    for (int r = 0; r < s->nrep; r++) { s->rep[r].tK = temps[r]; set_ele(s, r, temps[r], s->rep[r].ionic); }
+}
+
+/* REMD over the ionic strength (REPT%ION): label k carries ionic[k-1];
+ * each replica takes its label's (src/init_ele.F90, energy_replica and
+ * replica_exchange treat it like the temperature) */
+void orc_set_ionic(orc_sys *s, const double *ionic) {
+    for (int k = 1; k <= s->nrep; k++) s->lab2ion[k] = ionic[k - 1];
+    for (int r = 0; r < s->nrep; r++) {
+        s->rep[r].ionic = s->lab2ion[s->rep2lab[r + 1]];
+        set_ele(s, r, s->rep[r].tK, s->rep[r].ionic);
+    }
 }
 
 void orc_destroy(orc_sys *s) {
@@ -1295,7 +1311,7 @@ void orc_destroy(orc_sys *s) {
     free(s->rep); free(s->nmp_chain); free(s->moltype); free(s->chain_start); free(s->seq);
     free(s->chain_of); free(s->lmp); free(s->has_charge); free(s->xyz); free(s->velos);
     free(s->accels); free(s->move); free(s->mass); free(s->fric); free(s->rep2lab);
-    free(s->lab2rep); free(s->lab2val); free(s->pair_tb);
+    free(s->lab2rep); free(s->lab2val); free(s->lab2ion); free(s->pair_tb);
     free(s);
 }
 
@@ -1411,10 +1427,10 @@ static void exchange(orc_sys *s, double *energies_out) {
         if (opp == 0 || opp > n) E[2 * (r + 1) + 1] = 0.0;
         else {
             double tk_own = rr->tK, t_opp = s->lab2val[opp];
-            set_ele(s, r, t_opp, s->p.ionic_strength);
+            set_ele(s, r, t_opp, s->lab2ion[opp]);   /* src/energy_replica.F90:96-112 */
             orc_energy(s, r + 1, t_opp, s->p.nstep_bp_mc > 0, 0, e);
             E[2 * (r + 1) + 1] = e[0];
-            set_ele(s, r, tk_own, s->p.ionic_strength);
+            set_ele(s, r, tk_own, rr->ionic);
         }
         memcpy(rr->bp_status, st, nbp + 1); memcpy(rr->ene_bp, en, (nbp + 1) * sizeof(double));
         free(st); free(en);
@@ -1441,7 +1457,11 @@ static void exchange(orc_sys *s, double *energies_out) {
         }
     }
     s->idx_pair = (s->idx_pair + 1) % 2;
-    for (int r = 0; r < n; r++) { s->rep[r].tK = s->lab2val[s->rep2lab[r + 1]]; set_ele(s, r, s->rep[r].tK, s->p.ionic_strength); }
+    for (int r = 0; r < n; r++) {
+        s->rep[r].tK = s->lab2val[s->rep2lab[r + 1]];
+        s->rep[r].ionic = s->lab2ion[s->rep2lab[r + 1]];
+        set_ele(s, r, s->rep[r].tK, s->rep[r].ionic);
+    }
     free(rnd); free(E);
 }
 

@@ -154,6 +154,7 @@ struct sis_gpu_ctx {
   std::vector<double> tK, ionic, mass, fric;
   std::vector<int> rep2lab, lab2rep;
   std::vector<double> lab2val;
+  std::vector<double> lab2ion;   // ionic strength of each label (REMD over the ionic strength)
   int idx_pair = 0;
   std::vector<int> pair_tb;  // [2][R+2]
   std::vector<uint32_t> rep_tape;
@@ -688,7 +689,7 @@ void kinetic_all(sis_gpu_ctx *c, double *e) {
 
 // energy_sumup for replica mask at conditions (ecT, coef, lam per replica)
 void energy_phase(sis_gpu_ctx *c, const std::vector<int> &mask, const std::vector<double> &T,
-                  int flags, double *out /* R*16 */) {
+                  int flags, double *out /* R*16 */, const std::vector<double> *ionic = nullptr) {
   const int N = c->N, R = c->R;
   const int B = 256;
   std::vector<double> coef(R), lam(R);
@@ -696,7 +697,8 @@ void energy_phase(sis_gpu_ctx *c, const std::vector<int> &mask, const std::vecto
     double cut;
     // an annealed run keeps the electrostatics of its first temperature
     // (set_ele runs once, src/main.F90:90-97 before init_ele)
-    host_set_ele(c->prm, c->anneal_on ? c->ele_T[r] : T[r], c->ionic[r], coef[r], lam[r], cut);
+    host_set_ele(c->prm, c->anneal_on ? c->ele_T[r] : T[r], ionic ? (*ionic)[r] : c->ionic[r], coef[r], lam[r],
+                 cut);
   }
   HIPCHK(hipMemcpyAsync(c->ecT_d.p, T.data(), R * sizeof(double), hipMemcpyHostToDevice, c->stream));
   HIPCHK(hipMemcpyAsync(c->eCoef_d.p, coef.data(), R * sizeof(double), hipMemcpyHostToDevice, c->stream));
@@ -760,12 +762,14 @@ void do_exchange(sis_gpu_ctx *c, double *replica_energies, int32_t *accepted, do
   int mcflag = c->P.nstep_bp_mc > 0 ? 1 : 0;
   energy_phase(c, all, c->tK, mcflag, e1.data());
   // counterpart conditions; the BP state of the own-condition pass is kept
-  std::vector<double> Topp(R);
+  // (temperature and ionic strength of the partner's label,
+  // src/energy_replica.F90:90-113)
+  std::vector<double> Topp(R), Iopp(R);
   std::vector<int> has(R, 0);
   for (int r = 0; r < R; r++) {
     int lab = c->rep2lab[r + 1], opp = get_pair(c, lab);
-    if (opp > 0 && opp <= R) { Topp[r] = c->lab2val[opp]; has[r] = 1; }
-    else Topp[r] = c->tK[r];
+    if (opp > 0 && opp <= R) { Topp[r] = c->lab2val[opp]; Iopp[r] = c->lab2ion[opp]; has[r] = 1; }
+    else { Topp[r] = c->tK[r]; Iopp[r] = c->ionic[r]; }
   }
   size_t nb = c->nbp_total;
   std::vector<uint8_t> st(nb);
@@ -774,7 +778,7 @@ void do_exchange(sis_gpu_ctx *c, double *replica_energies, int32_t *accepted, do
     HIPCHK(hipMemcpyAsync(st.data(), c->bp_status_d.p, nb, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipMemcpyAsync(en.data(), c->ene_bp_d.p, nb * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   }
-  energy_phase(c, has, Topp, mcflag, e2.data());
+  energy_phase(c, has, Topp, mcflag, e2.data(), &Iopp);
   if (nb) {
     HIPCHK(hipMemcpyAsync(c->bp_status_d.p, st.data(), nb, hipMemcpyHostToDevice, c->stream));
     HIPCHK(hipMemcpyAsync(c->ene_bp_d.p, en.data(), nb * sizeof(double), hipMemcpyHostToDevice, c->stream));
@@ -813,7 +817,12 @@ void do_exchange(sis_gpu_ctx *c, double *replica_energies, int32_t *accepted, do
     }
   }
   c->idx_pair = (c->idx_pair + 1) % 2;
-  for (int r = 0; r < R; r++) c->tK[r] = c->lab2val[c->rep2lab[r + 1]];
+  for (int r = 0; r < R; r++) {
+    c->tK[r] = c->lab2val[c->rep2lab[r + 1]];
+    c->ionic[r] = c->lab2ion[c->rep2lab[r + 1]];
+  }
+  // an exchange permutes the conditions, so the largest electrostatic
+  // cutoff and the cell grid stay as they are
   upload_conditions(c);
   (void)nacc;
 }
@@ -1071,6 +1080,7 @@ int sis_gpu_create(const sis_topo *topo, const sis_params *prm, const sis_option
     c->ionic.assign(R, prm->ionic_strength);
     c->rep2lab.resize(R + 1); c->lab2rep.resize(R + 1); c->lab2val.resize(R + 1);
     for (int k = 1; k <= R; k++) { c->rep2lab[k] = k; c->lab2rep[k] = k; c->lab2val[k] = prm->tempK; }
+    c->lab2ion.assign(R + 1, prm->ionic_strength);
     c->pair_tb.assign(2 * (R + 2), 0);
     for (int par = 0; par < 2; par++) {
       int *tb = c->pair_tb.data() + par * (R + 2);
@@ -1179,6 +1189,7 @@ int sis_gpu_set_rep_conditions(sis_gpu_ctx *c, const double *tempK, const double
     if (ionic) {
       if (!(ionic[r] > 0.0)) return fail(SIS_ERR_ARG, "ionic strength must be positive");
       c->ionic[r] = ionic[r];
+      c->lab2ion[c->rep2lab[r + 1]] = ionic[r];   // the label carries it through exchanges
     }
   }
   upload_conditions(c);
@@ -1798,6 +1809,7 @@ int sis_gpu_set_labels(sis_gpu_ctx *c, const int32_t *rep2lab) {
     c->rep2lab[r + 1] = rep2lab[r];
     c->lab2rep[rep2lab[r]] = r + 1;
     c->tK[r] = c->lab2val[rep2lab[r]];
+    c->ionic[r] = c->lab2ion[rep2lab[r]];
   }
   // the pair table restarts from its first half, as var_replica's idx_pair
   // does on a restarted run (it is not in the restart file,

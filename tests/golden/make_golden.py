@@ -194,6 +194,16 @@ def remd_rg_schedule():
     return sysd
 
 
+def remd_ion():
+    """Replica exchange over the ionic strength (REPT%ION: 0.05-0.4 M at one
+    temperature; src/energy_replica.F90:90-113, src/replica_exchange.F90)."""
+    sysd = configs.remd_c3(nrep=4, n=40, seed=9)
+    sysd["temps"] = np.full(4, float(sysd["params"]["tempK"]))
+    sysd["remd_ion"] = np.array([0.05, 0.1, 0.2, 0.4])
+    sysd["name"] = "remd4ion"
+    return sysd
+
+
 SYSTEMS = {
     "hairpin64": (lambda: configs.hairpin64(1), 60, 0),
     "small_pbc": (lambda: configs.small_pbc(4, 30, 100.0, seed=5), 60, 0),
@@ -207,6 +217,7 @@ SYSTEMS = {
     "pbc_resize": (pbc_resize, 60, 0),
     "cag47_m1": (cag47_system, 40, 0),
     "remd2rgs": (remd_rg_schedule, 30, 10),
+    "remd4ion": (remd_ion, 40, 10),
 }
 
 
@@ -235,6 +246,8 @@ def make(name, builder, nstep, nex, seed=4321):
     ref1 = refdrv.run(s, pre + [("job_md", nstep, "R")], seed=seed, nstep_exchange=nex)
     # count the words the static part + the run consume, using the oracle
     o = corc.Oracle(s)
+    if s.get("remd_ion") is not None:
+        o.set_ionic(s["remd_ion"])
     if ext:
         o.set_external(ext)
     if s.get("rg_schedule") is not None:
@@ -254,6 +267,8 @@ def make(name, builder, nstep, nex, seed=4321):
     o.force(1, fmc)
     used_static = [o.tape_pos(r) for r in range(1, nrep + 1)]
     o2 = corc.Oracle(s)
+    if s.get("remd_ion") is not None:
+        o2.set_ionic(s["remd_ion"])
     if ext:
         o2.set_external(ext)
     if s.get("rg_schedule") is not None:
@@ -296,6 +311,8 @@ def make(name, builder, nstep, nex, seed=4321):
         out["ext_" + k] = np.asarray(v)
     if s.get("ss") is not None:
         out["ss_pairs"] = np.asarray(s["ss"], np.int32)
+    if s.get("remd_ion") is not None:
+        out["remd_ion"] = np.asarray(s["remd_ion"], np.float64)
     if s.get("rg_schedule") is not None:
         out["rgs_steps"], out["rgs_k"], out["rgs_0"] = [np.asarray(a) for a in s["rg_schedule"]]
     if s.get("resize") is not None:
@@ -335,6 +352,8 @@ def load(name):
         sysd["external"] = ext
     if "ss_pairs" in d:
         sysd["ss"] = d["ss_pairs"]
+    if "remd_ion" in d:
+        sysd["remd_ion"] = d["remd_ion"]
     if "rgs_steps" in d:
         sysd["rg_schedule"] = (d["rgs_steps"], d["rgs_k"], d["rgs_0"])
     if "resize_step" in d:

@@ -20,7 +20,7 @@ from tests.golden.make_golden import load
 
 pytestmark = pytest.mark.gpu
 
-FIXTURES = ["hairpin64", "small_pbc", "mixed", "remd4", "remd4mc3", "remd4bd", "remd2ext", "hairpin_ss", "hairpin_anneal", "pbc_resize", "cag47_m1", "remd2rgs"]
+FIXTURES = ["hairpin64", "small_pbc", "mixed", "remd4", "remd4mc3", "remd4bd", "remd2ext", "hairpin_ss", "hairpin_anneal", "pbc_resize", "cag47_m1", "remd2rgs", "remd4ion"]
 
 
 def _sorted_rows(a):
@@ -33,6 +33,8 @@ def _pair_sets_equal(eng_pairs, ref_pairs):
 
 def _engine(sysd, d=None, **kw):
     eng = Engine(sysd, parity=True, **kw)
+    if "remd_ion" in sysd:
+        eng.set_rep_conditions(None, sysd["remd_ion"])
     if "external" in sysd:
         eng.set_external(sysd["external"])
     if "rg_schedule" in sysd:
@@ -321,6 +323,27 @@ def test_box_resize_persistent_form(tmp_path):
         assert np.allclose(box, want, rtol=0, atol=1e-9), (box, want)
         res.append(eng.get_state())
     for a, b in zip(res[0], res[1]):
+        assert np.abs(a - b).max() < 1e-6
+
+
+def test_ion_exchange_batched_form():
+    """Replica exchange over the ionic strength (8 replicas of a 128-nt
+    chain at one temperature, 0.02-0.6 M): the batched form follows the
+    per-phase form, with the same exchanges."""
+    s = configs.remd_c3(nrep=8, n=128, seed=3)
+    s["temps"] = np.full(8, float(s["params"]["tempK"]))
+    ion = np.geomspace(0.02, 0.6, 8)
+    print("[case] ion exchange batched vs phases", flush=True)
+    res = []
+    for mode in ("phases", "batched"):
+        eng = Engine(s, parity=False, mode=mode, seed=31, exchange_every=10)
+        eng.set_rep_conditions(None, ion)
+        eng.init_velocities()
+        eng.neighbor_list(0)
+        eng.step(60)
+        res.append((eng.get_state(), eng.labels()))
+    assert np.array_equal(res[0][1], res[1][1]) and not np.array_equal(res[0][1], np.arange(1, 9))
+    for a, b in zip(res[0][0], res[1][0]):
         assert np.abs(a - b).max() < 1e-6
 
 

@@ -10,12 +10,14 @@ from sis_amd import configs
 from oracle import corc, refdrv
 from tests.golden.make_golden import load
 
-FIXTURES = ["hairpin64", "small_pbc", "mixed", "remd4", "remd4mc3", "remd4bd", "remd2ext", "hairpin_ss", "hairpin_anneal", "pbc_resize", "cag47_m1", "remd2rgs"]
+FIXTURES = ["hairpin64", "small_pbc", "mixed", "remd4", "remd4mc3", "remd4bd", "remd2ext", "hairpin_ss", "hairpin_anneal", "pbc_resize", "cag47_m1", "remd2rgs", "remd4ion"]
 
 
 def _oracle_from_fixture(name):
     sysd, d = load(name)
     o = corc.Oracle(sysd)
+    if "remd_ion" in sysd:
+        o.set_ionic(sysd["remd_ion"])
     if "external" in sysd:
         o.set_external(sysd["external"])
     if "rg_schedule" in sysd:
