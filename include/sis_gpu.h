@@ -214,6 +214,13 @@ int sis_gpu_set_external(sis_gpu_ctx *ctx, const sis_external *ext);
  * resize_step <= 0 turns it off.  Needs a periodic box. */
 int sis_gpu_set_box_resize(sis_gpu_ctx *ctx, int64_t resize_step, const double *change, const double *target);
 
+/* Replica exchange over the Rg-bias target (REPT%RG; src/init_bias_rg.F90:13-16,
+ * src/energy_replica.F90:107-109, src/job_md.F90:564-566): rg0[nrep] are
+ * the targets of the replicas' current labels and travel with them
+ * through exchanges.  Needs the Rg bias of sis_gpu_set_external (which
+ * drops the ladder again); NULL stops it. */
+int sis_gpu_set_rep_rg(sis_gpu_ctx *ctx, const double *rg0);
+
 /* Timed Rg bias (read_timed_bias_rg, src/read_timed_bias_rg.F90;
  * src/job_md.F90:231-265,614-627): entry i sets the Rg-bias constant k[i]
  * and target r0[i] of every replica from step steps[i] on.  Needs the Rg

@@ -48,6 +48,7 @@ def lib():
         L.orc_set_step.argtypes = [vp, i64]
         L.orc_set_anneal.argtypes = [vp, i32, P(i64), P(d)]
         L.orc_set_ionic.argtypes = [vp, P(d)]
+        L.orc_set_rep_rg.argtypes = [vp, P(d)]
         L.orc_set_rg_schedule.argtypes = [vp, i32, P(i64), P(d), P(d)]
         L.orc_set_box_resize.argtypes = [vp, i64, P(d), P(d)]
         L.orc_get_box.argtypes = [vp, P(d)]
@@ -217,6 +218,11 @@ class Oracle:
         """Per-label ionic strengths (REMD over REPT%ION)."""
         self._ion = np.ascontiguousarray(ionic, np.float64)
         lib().orc_set_ionic(self.h, _d(self._ion))
+
+    def set_rep_rg(self, rg0):
+        """Per-label Rg-bias targets (REMD over REPT%RG)."""
+        self._rg = np.ascontiguousarray(rg0, np.float64)
+        lib().orc_set_rep_rg(self.h, _d(self._rg))
 
     def set_anneal(self, steps, temps):
         st = np.ascontiguousarray(steps, np.int64)

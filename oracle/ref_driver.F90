@@ -52,21 +52,21 @@ module sisref_driver
       integer(c_int32_t) :: pbc, integrator
    end type c_params
 
-   ! REMD over the ionic strength instead of the temperature: set by
-   ! sisref_remd_ion before sisref_setup
-   logical, save :: remd_ion = .False.
-   real(c_double), allocatable, save :: ion_values(:)
+   ! REMD over another variable than the temperature (REPT id: 3 ionic
+   ! strength, 4 Rg target): set by sisref_remd_var before sisref_setup
+   integer, save :: remd_kind = 0
+   real(c_double), allocatable, save :: remd_values(:)
 
 contains
 
-   subroutine sisref_remd_ion(n, values) bind(C, name="sisref_remd_ion")
-      integer(c_int), value :: n
+   subroutine sisref_remd_var(kind, n, values) bind(C, name="sisref_remd_var")
+      integer(c_int), value :: kind, n
       real(c_double), intent(in) :: values(n)
-      remd_ion = .True.
-      if (allocated(ion_values)) deallocate(ion_values)
-      allocate(ion_values(n))
-      ion_values(:) = values(:)
-   end subroutine sisref_remd_ion
+      remd_kind = kind
+      if (allocated(remd_values)) deallocate(remd_values)
+      allocate(remd_values(n))
+      remd_values(:) = values(:)
+   end subroutine sisref_remd_var
 
    subroutine copy_bp(src, dst)
       use var_potential, only : basepair_parameters
@@ -193,11 +193,11 @@ contains
          flg_replica = .True.
          flg_exchange = .True.
          replica_values(:,:) = 0.0_PREC
-         if (remd_ion) then      ! [Replica] ion_strength ladder
-            flg_repvar(REPT%ION) = .True.
-            nrep(REPT%ION) = nrep_in
+         if (remd_kind == REPT%ION .or. remd_kind == REPT%RG) then   ! [Replica] ion_strength / rg ladder
+            flg_repvar(remd_kind) = .True.
+            nrep(remd_kind) = nrep_in
             do irep = 1, nrep_in
-               replica_values(irep, REPT%ION) = ion_values(irep)
+               replica_values(irep, remd_kind) = remd_values(irep)
             enddo
          else
             flg_repvar(REPT%TEMP) = .True.

@@ -41,9 +41,10 @@ def _setup(lib, sysd, prefix, seed, nstep_exchange=0):
     seq = np.ascontiguousarray(sysd["seq"], np.int32)
     temps = np.ascontiguousarray(sysd["temps"], np.float64)
     xyz = np.ascontiguousarray(sysd["xyz"], np.float64)
-    if sysd.get("remd_ion") is not None:   # ladder over the ionic strength
-        ion = np.ascontiguousarray(sysd["remd_ion"], np.float64)
-        lib.sisref_remd_ion(C.c_int(len(ion)), _dp(ion))
+    for key, kind in (("remd_ion", 3), ("remd_rg", 4)):   # ladders over REPT%ION / REPT%RG
+        if sysd.get(key) is not None:
+            vals = np.ascontiguousarray(sysd[key], np.float64)
+            lib.sisref_remd_var(C.c_int(kind), C.c_int(len(vals)), _dp(vals))
     lib.sisref_setup(C.byref(p), C.c_int(int(sysd["nchains"])), _ip(nmp_chain), _ip(molt),
                      _ip(seq), C.c_int(int(sysd["nrep"])), _dp(temps), C.c_int(int(seed)),
                      _dp(xyz), prefix.encode() + b"\0", C.c_int64(int(nstep_exchange)))

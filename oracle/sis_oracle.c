@@ -70,6 +70,8 @@ struct orc_sys {
     /* replica exchange */
     int *rep2lab, *lab2rep; double *lab2val; int idx_pair; int *pair_tb; /* [2][nrep+1] */
     double *lab2ion;         /* ionic strength of each label (REMD over REPT%ION) */
+    double *lab2rg;          /* Rg-bias target of each label (REMD over REPT%RG) */
+    int rg_remd;             /* the Rg targets travel with the labels */
     tape_t tape_rep;
     int64_t istep;
     int flg_bp_energy;
@@ -925,6 +927,7 @@ void orc_get_box(orc_sys *s, double *box) { for (int k = 0; k < 3; k++) box[k] =
 static void rg_schedule_apply(orc_sys *s) {
     s->x.rg_k = s->rgs_k[s->irgs - 1];
     for (int r = 0; r < s->nrep; r++) s->x_rg0[r] = s->rgs_0[s->irgs - 1];
+    for (int k = 1; k <= s->nrep; k++) s->lab2rg[k] = s->rgs_0[s->irgs - 1];
     s->rgs_next = s->irgs < s->rgs_n ? s->rgs_step[s->irgs] : 0;
 }
 
@@ -1042,6 +1045,15 @@ void orc_set_external(orc_sys *s, const sis_external *e) {
     s->x_rest_id = dup_n(e->rest_id, sizeof(int32_t) * 3 * e->n_rest);
     s->x_rest_rcut = dup_n(e->rest_rcut, sizeof(double) * e->n_rest);
     s->x_rest_para = dup_n(e->rest_para, sizeof(double) * 3 * e->n_rest);
+    s->rg_remd = 0;
+}
+
+/* REMD over the Rg-bias target (REPT%RG): rg0 of each replica's label
+ * (init_bias_rg, src/init_bias_rg.F90:13-16); needs the Rg bias */
+void orc_set_rep_rg(orc_sys *s, const double *rg0) {
+    if (!s->x_on || !s->x.rg) return;
+    for (int r = 0; r < s->nrep; r++) { s->lab2rg[s->rep2lab[r + 1]] = rg0[r]; s->x_rg0[r] = rg0[r]; }
+    s->rg_remd = 1;
 }
 
 /* ---------------- energies (src/energy_*.F90) ---------------- */
@@ -1264,6 +1276,7 @@ orc_sys *orc_create(const sis_topo *topo, const sis_params *prm, int32_t nrep, c
     s->lab2rep = malloc(sizeof(int) * (nrep + 1));
     s->lab2val = malloc(sizeof(double) * (nrep + 1));
     s->lab2ion = malloc(sizeof(double) * (nrep + 1));
+    s->lab2rg = calloc(nrep + 1, sizeof(double));
     for (int k = 0; k <= nrep; k++) s->lab2ion[k] = prm->ionic_strength;
     s->pair_tb = calloc(2 * (nrep + 2), sizeof(int));
     for (int k = 1; k <= nrep; k++) { s->rep2lab[k] = k; s->lab2rep[k] = k; s->lab2val[k] = temps ? temps[k - 1] : prm->tempK; }
@@ -1311,7 +1324,7 @@ void orc_destroy(orc_sys *s) {
     free(s->rep); free(s->nmp_chain); free(s->moltype); free(s->chain_start); free(s->seq);
     free(s->chain_of); free(s->lmp); free(s->has_charge); free(s->xyz); free(s->velos);
     free(s->accels); free(s->move); free(s->mass); free(s->fric); free(s->rep2lab);
-    free(s->lab2rep); free(s->lab2val); free(s->lab2ion); free(s->pair_tb);
+    free(s->lab2rep); free(s->lab2val); free(s->lab2ion); free(s->lab2rg); free(s->pair_tb);
     free(s);
 }
 
@@ -1427,10 +1440,14 @@ static void exchange(orc_sys *s, double *energies_out) {
         if (opp == 0 || opp > n) E[2 * (r + 1) + 1] = 0.0;
         else {
             double tk_own = rr->tK, t_opp = s->lab2val[opp];
+            int rgl = s->x_on && s->x.rg && s->rg_remd;
+            double rg_own = rgl ? s->x_rg0[r] : 0.0;
             set_ele(s, r, t_opp, s->lab2ion[opp]);   /* src/energy_replica.F90:96-112 */
+            if (rgl) s->x_rg0[r] = s->lab2rg[opp];
             orc_energy(s, r + 1, t_opp, s->p.nstep_bp_mc > 0, 0, e);
             E[2 * (r + 1) + 1] = e[0];
             set_ele(s, r, tk_own, rr->ionic);
+            if (rgl) s->x_rg0[r] = rg_own;
         }
         memcpy(rr->bp_status, st, nbp + 1); memcpy(rr->ene_bp, en, (nbp + 1) * sizeof(double));
         free(st); free(en);
@@ -1461,6 +1478,7 @@ static void exchange(orc_sys *s, double *energies_out) {
         s->rep[r].tK = s->lab2val[s->rep2lab[r + 1]];
         s->rep[r].ionic = s->lab2ion[s->rep2lab[r + 1]];
         set_ele(s, r, s->rep[r].tK, s->rep[r].ionic);
+        if (s->x_on && s->x.rg && s->rg_remd) s->x_rg0[r] = s->lab2rg[s->rep2lab[r + 1]];   /* src/job_md.F90:564-566 */
     }
     free(rnd); free(E);
 }

@@ -109,7 +109,7 @@ struct sis_gpu_ctx {
   // external terms (sis_gpu_set_external)
   bool ext_on = false;
   ExtDev X{};
-  DevBuf<double> x_rg0_d, x_dcf_f_d, x_fr_k_d, x_fr_init_d, x_fr_velo_d, x_rest_rcut_d, x_rest_para_d,
+  DevBuf<double> x_rg0_d, x_rg0_opp_d, x_dcf_f_d, x_fr_k_d, x_fr_init_d, x_fr_velo_d, x_rest_rcut_d, x_rest_para_d,
       x_rg_stat_d, x_ene_d;
   DevBuf<int> x_term_off_d, x_dcf_pairs_d, x_fr_pairs_d, x_rest_id_d, x_err_d;
   DevBuf<int2> x_terms_d;
@@ -155,6 +155,8 @@ struct sis_gpu_ctx {
   std::vector<int> rep2lab, lab2rep;
   std::vector<double> lab2val;
   std::vector<double> lab2ion;   // ionic strength of each label (REMD over the ionic strength)
+  std::vector<double> lab2rg;    // Rg-bias target of each label (REMD over the Rg target)
+  bool rg_remd = false;          // the Rg targets travel with the labels (sis_gpu_set_rep_rg)
   int idx_pair = 0;
   std::vector<int> pair_tb;  // [2][R+2]
   std::vector<uint32_t> rep_tape;
@@ -248,7 +250,7 @@ struct sis_gpu_ctx {
                             &bp_coef2_d};
     for (auto *b : db) b->free_();
     bpp_d.free_();
-    DevBuf<double> *xb[] = {&x_rg0_d, &x_dcf_f_d, &x_fr_k_d, &x_fr_init_d, &x_fr_velo_d, &x_rest_rcut_d,
+    DevBuf<double> *xb[] = {&x_rg0_d, &x_rg0_opp_d, &x_dcf_f_d, &x_fr_k_d, &x_fr_init_d, &x_fr_velo_d, &x_rest_rcut_d,
                             &x_rest_para_d, &x_rg_stat_d, &x_ene_d};
     for (auto *b : xb) b->free_();
     DevBuf<int> *xi[] = {&x_term_off_d, &x_dcf_pairs_d, &x_fr_pairs_d, &x_rest_id_d, &x_err_d, &ss_off_d, &ss_partner_d};
@@ -689,7 +691,8 @@ void kinetic_all(sis_gpu_ctx *c, double *e) {
 
 // energy_sumup for replica mask at conditions (ecT, coef, lam per replica)
 void energy_phase(sis_gpu_ctx *c, const std::vector<int> &mask, const std::vector<double> &T,
-                  int flags, double *out /* R*16 */, const std::vector<double> *ionic = nullptr) {
+                  int flags, double *out /* R*16 */, const std::vector<double> *ionic = nullptr,
+                  const std::vector<double> *rg0 = nullptr) {
   const int N = c->N, R = c->R;
   const int B = 256;
   std::vector<double> coef(R), lam(R);
@@ -716,7 +719,13 @@ void energy_phase(sis_gpu_ctx *c, const std::vector<int> &mask, const std::vecto
   const double *ext = nullptr;
   if (c->ext_on) {
     if (c->X.rg) hipLaunchKernelGGL(k_rg_stats<256>, dim3(R), dim3(256), 0, c->stream, c->P, c->ptrs(), c->X);
-    hipLaunchKernelGGL(k_external_energy<256>, dim3(R), dim3(256), 0, c->stream, c->P, c->ptrs(), c->X);
+    ExtDev X = c->X;
+    if (rg0 && X.rg) {   // Rg targets of other labels (src/energy_replica.F90:107-109)
+      if (c->x_rg0_opp_d.n < (size_t)R) c->x_rg0_opp_d.alloc(R);
+      HIPCHK(hipMemcpyAsync(c->x_rg0_opp_d.p, rg0->data(), R * sizeof(double), hipMemcpyHostToDevice, c->stream));
+      X.rg0 = c->x_rg0_opp_d.p;
+    }
+    hipLaunchKernelGGL(k_external_energy<256>, dim3(R), dim3(256), 0, c->stream, c->P, c->ptrs(), X);
     ext = c->x_ene_d.p;
   }
   hipLaunchKernelGGL(k_energy_reduce, dim3(R), dim3(1), 0, c->stream, c->ene_part_d.p, (int)g.x, c->ene_out_d.p,
@@ -764,12 +773,12 @@ void do_exchange(sis_gpu_ctx *c, double *replica_energies, int32_t *accepted, do
   // counterpart conditions; the BP state of the own-condition pass is kept
   // (temperature and ionic strength of the partner's label,
   // src/energy_replica.F90:90-113)
-  std::vector<double> Topp(R), Iopp(R);
+  std::vector<double> Topp(R), Iopp(R), Gopp(R);
   std::vector<int> has(R, 0);
   for (int r = 0; r < R; r++) {
     int lab = c->rep2lab[r + 1], opp = get_pair(c, lab);
-    if (opp > 0 && opp <= R) { Topp[r] = c->lab2val[opp]; Iopp[r] = c->lab2ion[opp]; has[r] = 1; }
-    else { Topp[r] = c->tK[r]; Iopp[r] = c->ionic[r]; }
+    if (opp > 0 && opp <= R) { Topp[r] = c->lab2val[opp]; Iopp[r] = c->lab2ion[opp]; Gopp[r] = c->lab2rg[opp]; has[r] = 1; }
+    else { Topp[r] = c->tK[r]; Iopp[r] = c->ionic[r]; Gopp[r] = c->lab2rg[lab]; }
   }
   size_t nb = c->nbp_total;
   std::vector<uint8_t> st(nb);
@@ -778,7 +787,7 @@ void do_exchange(sis_gpu_ctx *c, double *replica_energies, int32_t *accepted, do
     HIPCHK(hipMemcpyAsync(st.data(), c->bp_status_d.p, nb, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipMemcpyAsync(en.data(), c->ene_bp_d.p, nb * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   }
-  energy_phase(c, has, Topp, mcflag, e2.data(), &Iopp);
+  energy_phase(c, has, Topp, mcflag, e2.data(), &Iopp, c->rg_remd ? &Gopp : nullptr);
   if (nb) {
     HIPCHK(hipMemcpyAsync(c->bp_status_d.p, st.data(), nb, hipMemcpyHostToDevice, c->stream));
     HIPCHK(hipMemcpyAsync(c->ene_bp_d.p, en.data(), nb * sizeof(double), hipMemcpyHostToDevice, c->stream));
@@ -821,6 +830,12 @@ void do_exchange(sis_gpu_ctx *c, double *replica_energies, int32_t *accepted, do
     c->tK[r] = c->lab2val[c->rep2lab[r + 1]];
     c->ionic[r] = c->lab2ion[c->rep2lab[r + 1]];
   }
+  if (c->ext_on && c->X.rg && c->rg_remd) {   // src/job_md.F90:564-566
+    std::vector<double> g(R);
+    for (int r = 0; r < R; r++) g[r] = c->lab2rg[c->rep2lab[r + 1]];
+    HIPCHK(hipMemcpyAsync(c->x_rg0_d.p, g.data(), R * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+  }
   // an exchange permutes the conditions, so the largest electrostatic
   // cutoff and the cell grid stay as they are
   upload_conditions(c);
@@ -836,6 +851,7 @@ void rg_schedule_apply(sis_gpu_ctx *c) {
   const int n = (int)c->rgs_step.size();
   c->X.rg_k = c->rgs_k[c->irgs - 1];
   std::vector<double> r0(c->R, c->rgs_0[c->irgs - 1]);
+  std::fill(c->lab2rg.begin(), c->lab2rg.end(), c->rgs_0[c->irgs - 1]);
   // on the engine's stream: the steps already queued read the old targets
   HIPCHK(hipMemcpyAsync(c->x_rg0_d.p, r0.data(), c->R * sizeof(double), hipMemcpyHostToDevice, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
@@ -1081,6 +1097,7 @@ int sis_gpu_create(const sis_topo *topo, const sis_params *prm, const sis_option
     c->rep2lab.resize(R + 1); c->lab2rep.resize(R + 1); c->lab2val.resize(R + 1);
     for (int k = 1; k <= R; k++) { c->rep2lab[k] = k; c->lab2rep[k] = k; c->lab2val[k] = prm->tempK; }
     c->lab2ion.assign(R + 1, prm->ionic_strength);
+    c->lab2rg.assign(R + 1, 0.0);
     c->pair_tb.assign(2 * (R + 2), 0);
     for (int par = 0; par < 2; par++) {
       int *tb = c->pair_tb.data() + par * (R + 2);
@@ -1572,6 +1589,23 @@ int sis_gpu_set_step(sis_gpu_ctx *c, int64_t istep) {
   API_END
 }
 
+int sis_gpu_set_rep_rg(sis_gpu_ctx *c, const double *rg0) {
+  API_BEGIN
+  if (!c) return fail(SIS_ERR_ARG, "null context");
+  if (!rg0) { c->rg_remd = false; return SIS_OK; }
+  if (!(c->ext_on && c->X.rg)) return fail(SIS_ERR_STATE, "the Rg bias is not set (sis_gpu_set_external)");
+  const int R = c->R;
+  for (int r = 0; r < R; r++) {
+    if (!(rg0[r] > 0.0)) return fail(SIS_ERR_ARG, "Rg target must be positive");
+    c->lab2rg[c->rep2lab[r + 1]] = rg0[r];
+  }
+  HIPCHK(hipMemcpyAsync(c->x_rg0_d.p, rg0, R * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  c->rg_remd = true;
+  return SIS_OK;
+  API_END
+}
+
 int sis_gpu_set_rg_schedule(sis_gpu_ctx *c, int32_t n, const int64_t *steps, const double *k, const double *r0) {
   API_BEGIN
   if (!c) return fail(SIS_ERR_ARG, "null context");
@@ -1772,6 +1806,7 @@ int sis_gpu_set_external(sis_gpu_ctx *c, const sis_external *e) {
   X.stage_sigma = e->stage_sigma; X.stage_eps = e->stage_eps; X.rg_k = e->rg_k;
   std::vector<double> rg0(R, 0.0);
   if (e->rg) std::copy(e->rg_0, e->rg_0 + R, rg0.begin());
+  c->rg_remd = false;   // a ladder over the Rg target is set again after the terms
   X.rg0 = up(c->x_rg0_d, rg0.data(), (size_t)R);
   X.term_off = up(c->x_term_off_d, off.data(), off.size());
   X.terms = up(c->x_terms_d, terms.data(), terms.size());
@@ -1810,6 +1845,12 @@ int sis_gpu_set_labels(sis_gpu_ctx *c, const int32_t *rep2lab) {
     c->lab2rep[rep2lab[r]] = r + 1;
     c->tK[r] = c->lab2val[rep2lab[r]];
     c->ionic[r] = c->lab2ion[rep2lab[r]];
+  }
+  if (c->ext_on && c->X.rg && c->rg_remd) {
+    std::vector<double> g(R);
+    for (int r = 0; r < R; r++) g[r] = c->lab2rg[rep2lab[r]];
+    HIPCHK(hipMemcpyAsync(c->x_rg0_d.p, g.data(), R * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
   }
   // the pair table restarts from its first half, as var_replica's idx_pair
   // does on a restarted run (it is not in the restart file,

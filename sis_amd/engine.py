@@ -232,6 +232,14 @@ class Engine:
         tk = np.ascontiguousarray(temps, np.float64)
         _check(lib().sis_gpu_set_anneal(self.h, len(st), st.ctypes.data_as(C.POINTER(C.c_int64)), abi.dptr(tk)))
 
+    def set_rep_rg(self, rg0):
+        """Replica exchange over the Rg-bias target: rg0 per replica's label."""
+        if rg0 is None:
+            _check(lib().sis_gpu_set_rep_rg(self.h, None))
+            return
+        g = np.ascontiguousarray(rg0, np.float64)
+        _check(lib().sis_gpu_set_rep_rg(self.h, abi.dptr(g)))
+
     def set_rg_schedule(self, steps, k, r0):
         """Timed Rg bias: (k[i], r0[i]) from step steps[i] on."""
         st = np.ascontiguousarray(steps, np.int64)

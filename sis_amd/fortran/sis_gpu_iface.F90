@@ -237,6 +237,12 @@ module sis_gpu_iface
          type(c_ptr), value :: ctx
       end function
 
+      ! replica exchange over the Rg-bias target
+      integer(c_int) function sis_gpu_set_rep_rg(ctx, rg0) bind(C, name="sis_gpu_set_rep_rg")
+         import :: c_int, c_ptr
+         type(c_ptr), value :: ctx, rg0
+      end function
+
       ! timed Rg bias (read_timed_bias_rg)
       integer(c_int) function sis_gpu_set_rg_schedule(ctx, n, steps, k, r0) bind(C, name="sis_gpu_set_rg_schedule")
          import :: c_int, c_ptr, c_int32_t, c_int64_t, c_double

@@ -204,6 +204,18 @@ def remd_ion():
     return sysd
 
 
+def remd_rg():
+    """Replica exchange over the Rg-bias target (REPT%RG: 6-12 A at one
+    temperature; src/energy_replica.F90:107-109,143-145,
+    src/job_md.F90:564-566) with the other external terms of remd2ext."""
+    sysd = remd_external(nrep=4)
+    sysd["temps"] = np.full(4, float(sysd["params"]["tempK"]))
+    sysd["remd_rg"] = np.array([6.0, 8.0, 10.0, 12.0])
+    sysd["external"] = dict(sysd["external"], rg_0=sysd["remd_rg"].copy())
+    sysd["name"] = "remd4rg"
+    return sysd
+
+
 SYSTEMS = {
     "hairpin64": (lambda: configs.hairpin64(1), 60, 0),
     "small_pbc": (lambda: configs.small_pbc(4, 30, 100.0, seed=5), 60, 0),
@@ -218,6 +230,7 @@ SYSTEMS = {
     "cag47_m1": (cag47_system, 40, 0),
     "remd2rgs": (remd_rg_schedule, 30, 10),
     "remd4ion": (remd_ion, 40, 10),
+    "remd4rg": (remd_rg, 40, 10),
 }
 
 
@@ -252,6 +265,8 @@ def make(name, builder, nstep, nex, seed=4321):
         o.set_external(ext)
     if s.get("rg_schedule") is not None:
         o.set_rg_schedule(*s["rg_schedule"])
+    if s.get("remd_rg") is not None:
+        o.set_rep_rg(s["remd_rg"])
     if s.get("ss") is not None:
         o.set_ss(s["ss"])
     if s.get("anneal") is not None:
@@ -273,6 +288,8 @@ def make(name, builder, nstep, nex, seed=4321):
         o2.set_external(ext)
     if s.get("rg_schedule") is not None:
         o2.set_rg_schedule(*s["rg_schedule"])
+    if s.get("remd_rg") is not None:
+        o2.set_rep_rg(s["remd_rg"])
     if s.get("ss") is not None:
         o2.set_ss(s["ss"])
     if s.get("anneal") is not None:
@@ -313,6 +330,8 @@ def make(name, builder, nstep, nex, seed=4321):
         out["ss_pairs"] = np.asarray(s["ss"], np.int32)
     if s.get("remd_ion") is not None:
         out["remd_ion"] = np.asarray(s["remd_ion"], np.float64)
+    if s.get("remd_rg") is not None:
+        out["remd_rg"] = np.asarray(s["remd_rg"], np.float64)
     if s.get("rg_schedule") is not None:
         out["rgs_steps"], out["rgs_k"], out["rgs_0"] = [np.asarray(a) for a in s["rg_schedule"]]
     if s.get("resize") is not None:
@@ -354,6 +373,8 @@ def load(name):
         sysd["ss"] = d["ss_pairs"]
     if "remd_ion" in d:
         sysd["remd_ion"] = d["remd_ion"]
+    if "remd_rg" in d:
+        sysd["remd_rg"] = d["remd_rg"]
     if "rgs_steps" in d:
         sysd["rg_schedule"] = (d["rgs_steps"], d["rgs_k"], d["rgs_0"])
     if "resize_step" in d:
