@@ -221,6 +221,13 @@ int sis_gpu_set_box_resize(sis_gpu_ctx *ctx, int64_t resize_step, const double *
  * drops the ladder again); NULL stops it. */
 int sis_gpu_set_rep_rg(sis_gpu_ctx *ctx, const double *rg0);
 
+/* Replica exchange over the tweezers' constant force (REPT%TWZDCF,
+ * f-REMD; src/init_tweezers.F90:24-40, src/energy_replica.F90:103-105,
+ * src/job_md.F90:559-562): one constant-force pair whose force is
+ * k[label] times the normalised direction[3]; k travels with the labels.
+ * NULL k stops it. */
+int sis_gpu_set_rep_dcf(sis_gpu_ctx *ctx, const double *k, const double *direction);
+
 /* Timed Rg bias (read_timed_bias_rg, src/read_timed_bias_rg.F90;
  * src/job_md.F90:231-265,614-627): entry i sets the Rg-bias constant k[i]
  * and target r0[i] of every replica from step steps[i] on.  Needs the Rg

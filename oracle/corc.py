@@ -49,6 +49,7 @@ def lib():
         L.orc_set_anneal.argtypes = [vp, i32, P(i64), P(d)]
         L.orc_set_ionic.argtypes = [vp, P(d)]
         L.orc_set_rep_rg.argtypes = [vp, P(d)]
+        L.orc_set_rep_dcf.argtypes = [vp, P(d), P(d)]
         L.orc_set_rg_schedule.argtypes = [vp, i32, P(i64), P(d), P(d)]
         L.orc_set_box_resize.argtypes = [vp, i64, P(d), P(d)]
         L.orc_get_box.argtypes = [vp, P(d)]
@@ -218,6 +219,12 @@ class Oracle:
         """Per-label ionic strengths (REMD over REPT%ION)."""
         self._ion = np.ascontiguousarray(ionic, np.float64)
         lib().orc_set_ionic(self.h, _d(self._ion))
+
+    def set_rep_dcf(self, k, direction):
+        """f-REMD: force magnitude per label along the normalised direction."""
+        self._dk = np.ascontiguousarray(k, np.float64)
+        self._dd = np.ascontiguousarray(direction, np.float64)
+        lib().orc_set_rep_dcf(self.h, _d(self._dk), _d(self._dd))
 
     def set_rep_rg(self, rg0):
         """Per-label Rg-bias targets (REMD over REPT%RG)."""

@@ -193,7 +193,8 @@ contains
          flg_replica = .True.
          flg_exchange = .True.
          replica_values(:,:) = 0.0_PREC
-         if (remd_kind == REPT%ION .or. remd_kind == REPT%RG) then   ! [Replica] ion_strength / rg ladder
+         if (remd_kind == REPT%ION .or. remd_kind == REPT%RG .or. remd_kind == REPT%TWZDCF) then
+            ! [Replica] ion_strength / rg / force ladder
             flg_repvar(remd_kind) = .True.
             nrep(remd_kind) = nrep_in
             do irep = 1, nrep_in
@@ -629,6 +630,22 @@ contains
       cfile_bias_rg_in = name
       flg_timed_bias_rg = .True.
    end subroutine sisref_set_rg_schedule
+
+   ! f-REMD: the normalised direction of the one constant-force pair and
+   ! its per-replica forces, as init_tweezers sets them (src/init_tweezers.F90:24-40)
+   subroutine sisref_set_dcf_remd(dir) bind(C, name="sisref_set_dcf_remd")
+      use var_potential, only : twz_DCF_direction, twz_DCF_forces
+      use var_replica, only : nrep_proc, rep2val, irep2grep
+      use const_idx, only : REPT
+      real(c_double), intent(in) :: dir(3)
+      integer :: irep
+      if (allocated(twz_DCF_direction)) deallocate(twz_DCF_direction)
+      allocate(twz_DCF_direction(3, 1))
+      twz_DCF_direction(:, 1) = dir(:) / norm2(dir(:))
+      do irep = 1, nrep_proc
+         twz_DCF_forces(:, 1, irep) = rep2val(irep2grep(irep), REPT%TWZDCF) * twz_DCF_direction(:, 1)
+      enddo
+   end subroutine sisref_set_dcf_remd
 
    subroutine sisref_ele_params(irep, out) bind(C, name="sisref_ele_params")
       use var_state, only : lambdaD, diele

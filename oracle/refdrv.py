@@ -41,7 +41,7 @@ def _setup(lib, sysd, prefix, seed, nstep_exchange=0):
     seq = np.ascontiguousarray(sysd["seq"], np.int32)
     temps = np.ascontiguousarray(sysd["temps"], np.float64)
     xyz = np.ascontiguousarray(sysd["xyz"], np.float64)
-    for key, kind in (("remd_ion", 3), ("remd_rg", 4)):   # ladders over REPT%ION / REPT%RG
+    for key, kind in (("remd_ion", 3), ("remd_rg", 4), ("remd_dcf", 2)):   # ladders over REPT%ION / RG / TWZDCF
         if sysd.get(key) is not None:
             vals = np.ascontiguousarray(sysd[key], np.float64)
             lib.sisref_remd_var(C.c_int(kind), C.c_int(len(vals)), _dp(vals))
@@ -169,6 +169,9 @@ def _worker(q, so, sysd, ops, seed, nstep_exchange):
                     _ip(fp), _dp(fk), _dp(fi), _dp(fv), C.c_int(int(ext.get("rg", 0))),
                     C.c_double(float(ext.get("rg_k", 0.0))), _dp(rg0), C.c_int(nr), _ip(ri), _dp(rc), _dp(rp),
                     C.c_int64(int(op[2])))
+            elif kind == "dcf_remd":
+                dd = np.ascontiguousarray(op[1], np.float64)
+                lib.sisref_set_dcf_remd(_dp(dd))
             elif kind == "rg_schedule":
                 fn = os.path.join(tmp, "rg.sched")
                 with open(fn, "w") as fh:

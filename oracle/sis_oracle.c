@@ -72,6 +72,7 @@ struct orc_sys {
     double *lab2ion;         /* ionic strength of each label (REMD over REPT%ION) */
     double *lab2rg;          /* Rg-bias target of each label (REMD over REPT%RG) */
     int rg_remd;             /* the Rg targets travel with the labels */
+    double *lab2dcf; double dcf_dir[3]; int dcf_remd;   /* f-REMD (REPT%TWZDCF) */
     tape_t tape_rep;
     int64_t istep;
     int flg_bp_energy;
@@ -1046,6 +1047,21 @@ void orc_set_external(orc_sys *s, const sis_external *e) {
     s->x_rest_rcut = dup_n(e->rest_rcut, sizeof(double) * e->n_rest);
     s->x_rest_para = dup_n(e->rest_para, sizeof(double) * 3 * e->n_rest);
     s->rg_remd = 0;
+    s->dcf_remd = 0;
+}
+
+static void dcf_set_forces(orc_sys *s, int r, double k) {
+    for (int c = 0; c < 3; c++) s->x_dcf_f[3 * r + c] = k * s->dcf_dir[c];
+}
+
+/* f-REMD over the one constant-force pair (src/init_tweezers.F90:24-40):
+ * force = k[label] x normalised direction */
+void orc_set_rep_dcf(orc_sys *s, const double *k, const double *dir) {
+    if (!s->x_on || s->x.n_dcf != 1) return;
+    double n = sqrt(dir[0] * dir[0] + dir[1] * dir[1] + dir[2] * dir[2]);
+    for (int c = 0; c < 3; c++) s->dcf_dir[c] = dir[c] / n;
+    for (int r = 0; r < s->nrep; r++) { s->lab2dcf[s->rep2lab[r + 1]] = k[r]; dcf_set_forces(s, r, k[r]); }
+    s->dcf_remd = 1;
 }
 
 /* REMD over the Rg-bias target (REPT%RG): rg0 of each replica's label
@@ -1277,6 +1293,7 @@ orc_sys *orc_create(const sis_topo *topo, const sis_params *prm, int32_t nrep, c
     s->lab2val = malloc(sizeof(double) * (nrep + 1));
     s->lab2ion = malloc(sizeof(double) * (nrep + 1));
     s->lab2rg = calloc(nrep + 1, sizeof(double));
+    s->lab2dcf = calloc(nrep + 1, sizeof(double));
     for (int k = 0; k <= nrep; k++) s->lab2ion[k] = prm->ionic_strength;
     s->pair_tb = calloc(2 * (nrep + 2), sizeof(int));
     for (int k = 1; k <= nrep; k++) { s->rep2lab[k] = k; s->lab2rep[k] = k; s->lab2val[k] = temps ? temps[k - 1] : prm->tempK; }
@@ -1324,7 +1341,7 @@ void orc_destroy(orc_sys *s) {
     free(s->rep); free(s->nmp_chain); free(s->moltype); free(s->chain_start); free(s->seq);
     free(s->chain_of); free(s->lmp); free(s->has_charge); free(s->xyz); free(s->velos);
     free(s->accels); free(s->move); free(s->mass); free(s->fric); free(s->rep2lab);
-    free(s->lab2rep); free(s->lab2val); free(s->lab2ion); free(s->lab2rg); free(s->pair_tb);
+    free(s->lab2rep); free(s->lab2val); free(s->lab2ion); free(s->lab2rg); free(s->lab2dcf); free(s->pair_tb);
     free(s);
 }
 
@@ -1444,10 +1461,13 @@ static void exchange(orc_sys *s, double *energies_out) {
             double rg_own = rgl ? s->x_rg0[r] : 0.0;
             set_ele(s, r, t_opp, s->lab2ion[opp]);   /* src/energy_replica.F90:96-112 */
             if (rgl) s->x_rg0[r] = s->lab2rg[opp];
+            int fl = s->x_on && s->dcf_remd;
+            if (fl) dcf_set_forces(s, r, s->lab2dcf[opp]);
             orc_energy(s, r + 1, t_opp, s->p.nstep_bp_mc > 0, 0, e);
             E[2 * (r + 1) + 1] = e[0];
             set_ele(s, r, tk_own, rr->ionic);
             if (rgl) s->x_rg0[r] = rg_own;
+            if (fl) dcf_set_forces(s, r, s->lab2dcf[lab]);
         }
         memcpy(rr->bp_status, st, nbp + 1); memcpy(rr->ene_bp, en, (nbp + 1) * sizeof(double));
         free(st); free(en);
@@ -1479,6 +1499,7 @@ static void exchange(orc_sys *s, double *energies_out) {
         s->rep[r].ionic = s->lab2ion[s->rep2lab[r + 1]];
         set_ele(s, r, s->rep[r].tK, s->rep[r].ionic);
         if (s->x_on && s->x.rg && s->rg_remd) s->x_rg0[r] = s->lab2rg[s->rep2lab[r + 1]];   /* src/job_md.F90:564-566 */
+        if (s->x_on && s->dcf_remd) dcf_set_forces(s, r, s->lab2dcf[s->rep2lab[r + 1]]);   /* :559-562 */
     }
     free(rnd); free(E);
 }

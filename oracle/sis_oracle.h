@@ -38,7 +38,8 @@ int64_t orc_md(orc_sys *s, int64_t nsteps, int32_t nstep_exchange, double *energ
 void orc_labels(orc_sys *s, int32_t *rep2lab);
 void orc_set_temps(orc_sys *s, const double *temps);
 void orc_set_ionic(orc_sys *s, const double *ionic);
-void orc_set_rep_rg(orc_sys *s, const double *rg0);    /* per label: REMD over the Rg target */   /* per label: REMD over the ionic strength */
+void orc_set_rep_rg(orc_sys *s, const double *rg0);
+void orc_set_rep_dcf(orc_sys *s, const double *k, const double *dir);   /* f-REMD */    /* per label: REMD over the Rg target */   /* per label: REMD over the ionic strength */
 int64_t orc_mc_iterations(void);
 void orc_set_external(orc_sys *s, const sis_external *ext);   /* NULL clears */
 void orc_set_step(orc_sys *s, int64_t istep);

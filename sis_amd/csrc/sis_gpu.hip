@@ -109,7 +109,7 @@ struct sis_gpu_ctx {
   // external terms (sis_gpu_set_external)
   bool ext_on = false;
   ExtDev X{};
-  DevBuf<double> x_rg0_d, x_rg0_opp_d, x_dcf_f_d, x_fr_k_d, x_fr_init_d, x_fr_velo_d, x_rest_rcut_d, x_rest_para_d,
+  DevBuf<double> x_rg0_d, x_rg0_opp_d, x_dcf_opp_d, x_dcf_f_d, x_fr_k_d, x_fr_init_d, x_fr_velo_d, x_rest_rcut_d, x_rest_para_d,
       x_rg_stat_d, x_ene_d;
   DevBuf<int> x_term_off_d, x_dcf_pairs_d, x_fr_pairs_d, x_rest_id_d, x_err_d;
   DevBuf<int2> x_terms_d;
@@ -157,6 +157,9 @@ struct sis_gpu_ctx {
   std::vector<double> lab2ion;   // ionic strength of each label (REMD over the ionic strength)
   std::vector<double> lab2rg;    // Rg-bias target of each label (REMD over the Rg target)
   bool rg_remd = false;          // the Rg targets travel with the labels (sis_gpu_set_rep_rg)
+  std::vector<double> lab2dcf;   // constant-force magnitude of each label (REMD over the force)
+  double dcf_dir[3] = {0, 0, 0}; // unit direction of the one constant-force pair
+  bool dcf_remd = false;
   int idx_pair = 0;
   std::vector<int> pair_tb;  // [2][R+2]
   std::vector<uint32_t> rep_tape;
@@ -250,7 +253,7 @@ struct sis_gpu_ctx {
                             &bp_coef2_d};
     for (auto *b : db) b->free_();
     bpp_d.free_();
-    DevBuf<double> *xb[] = {&x_rg0_d, &x_rg0_opp_d, &x_dcf_f_d, &x_fr_k_d, &x_fr_init_d, &x_fr_velo_d, &x_rest_rcut_d,
+    DevBuf<double> *xb[] = {&x_rg0_d, &x_rg0_opp_d, &x_dcf_opp_d, &x_dcf_f_d, &x_fr_k_d, &x_fr_init_d, &x_fr_velo_d, &x_rest_rcut_d,
                             &x_rest_para_d, &x_rg_stat_d, &x_ene_d};
     for (auto *b : xb) b->free_();
     DevBuf<int> *xi[] = {&x_term_off_d, &x_dcf_pairs_d, &x_fr_pairs_d, &x_rest_id_d, &x_err_d, &ss_off_d, &ss_partner_d};
@@ -692,7 +695,7 @@ void kinetic_all(sis_gpu_ctx *c, double *e) {
 // energy_sumup for replica mask at conditions (ecT, coef, lam per replica)
 void energy_phase(sis_gpu_ctx *c, const std::vector<int> &mask, const std::vector<double> &T,
                   int flags, double *out /* R*16 */, const std::vector<double> *ionic = nullptr,
-                  const std::vector<double> *rg0 = nullptr) {
+                  const std::vector<double> *rg0 = nullptr, const std::vector<double> *dcf_k = nullptr) {
   const int N = c->N, R = c->R;
   const int B = 256;
   std::vector<double> coef(R), lam(R);
@@ -724,6 +727,16 @@ void energy_phase(sis_gpu_ctx *c, const std::vector<int> &mask, const std::vecto
       if (c->x_rg0_opp_d.n < (size_t)R) c->x_rg0_opp_d.alloc(R);
       HIPCHK(hipMemcpyAsync(c->x_rg0_opp_d.p, rg0->data(), R * sizeof(double), hipMemcpyHostToDevice, c->stream));
       X.rg0 = c->x_rg0_opp_d.p;
+    }
+    std::vector<double> fo;
+    if (dcf_k && X.ndcf == 1) {   // forces of other labels (src/energy_replica.F90:103-105)
+      fo.resize((size_t)3 * R);
+      for (int r = 0; r < R; r++)
+        for (int k = 0; k < 3; k++) fo[3 * r + k] = (*dcf_k)[r] * c->dcf_dir[k];
+      if (c->x_dcf_opp_d.n < fo.size()) c->x_dcf_opp_d.alloc(fo.size());
+      HIPCHK(hipMemcpyAsync(c->x_dcf_opp_d.p, fo.data(), fo.size() * sizeof(double), hipMemcpyHostToDevice,
+                            c->stream));
+      X.dcf_f = c->x_dcf_opp_d.p;
     }
     hipLaunchKernelGGL(k_external_energy<256>, dim3(R), dim3(256), 0, c->stream, c->P, c->ptrs(), X);
     ext = c->x_ene_d.p;
@@ -761,6 +774,8 @@ double rep_uniform(sis_gpu_ctx *c, uint64_t k) {
   return ((double)m + 0.5) * (1.0 / 9007199254740992.0);
 }
 
+void dcf_upload(sis_gpu_ctx *c);
+
 // energy_replica + replica_exchange (src/energy_replica.F90:47-156,
 // src/replica_exchange.F90:35-154) for a one-dimensional temperature ladder.
 void do_exchange(sis_gpu_ctx *c, double *replica_energies, int32_t *accepted, double *own_energies = nullptr) {
@@ -773,12 +788,16 @@ void do_exchange(sis_gpu_ctx *c, double *replica_energies, int32_t *accepted, do
   // counterpart conditions; the BP state of the own-condition pass is kept
   // (temperature and ionic strength of the partner's label,
   // src/energy_replica.F90:90-113)
-  std::vector<double> Topp(R), Iopp(R), Gopp(R);
+  std::vector<double> Topp(R), Iopp(R), Gopp(R), Fopp(R);
   std::vector<int> has(R, 0);
   for (int r = 0; r < R; r++) {
     int lab = c->rep2lab[r + 1], opp = get_pair(c, lab);
-    if (opp > 0 && opp <= R) { Topp[r] = c->lab2val[opp]; Iopp[r] = c->lab2ion[opp]; Gopp[r] = c->lab2rg[opp]; has[r] = 1; }
-    else { Topp[r] = c->tK[r]; Iopp[r] = c->ionic[r]; Gopp[r] = c->lab2rg[lab]; }
+    if (opp > 0 && opp <= R) {
+      Topp[r] = c->lab2val[opp]; Iopp[r] = c->lab2ion[opp]; Gopp[r] = c->lab2rg[opp]; Fopp[r] = c->lab2dcf[opp];
+      has[r] = 1;
+    } else {
+      Topp[r] = c->tK[r]; Iopp[r] = c->ionic[r]; Gopp[r] = c->lab2rg[lab]; Fopp[r] = c->lab2dcf[lab];
+    }
   }
   size_t nb = c->nbp_total;
   std::vector<uint8_t> st(nb);
@@ -787,7 +806,8 @@ void do_exchange(sis_gpu_ctx *c, double *replica_energies, int32_t *accepted, do
     HIPCHK(hipMemcpyAsync(st.data(), c->bp_status_d.p, nb, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipMemcpyAsync(en.data(), c->ene_bp_d.p, nb * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   }
-  energy_phase(c, has, Topp, mcflag, e2.data(), &Iopp, c->rg_remd ? &Gopp : nullptr);
+  energy_phase(c, has, Topp, mcflag, e2.data(), &Iopp, c->rg_remd ? &Gopp : nullptr,
+               c->dcf_remd ? &Fopp : nullptr);
   if (nb) {
     HIPCHK(hipMemcpyAsync(c->bp_status_d.p, st.data(), nb, hipMemcpyHostToDevice, c->stream));
     HIPCHK(hipMemcpyAsync(c->ene_bp_d.p, en.data(), nb * sizeof(double), hipMemcpyHostToDevice, c->stream));
@@ -836,6 +856,7 @@ void do_exchange(sis_gpu_ctx *c, double *replica_energies, int32_t *accepted, do
     HIPCHK(hipMemcpyAsync(c->x_rg0_d.p, g.data(), R * sizeof(double), hipMemcpyHostToDevice, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
   }
+  if (c->ext_on && c->dcf_remd) dcf_upload(c);   // src/job_md.F90:559-562
   // an exchange permutes the conditions, so the largest electrostatic
   // cutoff and the cell grid stay as they are
   upload_conditions(c);
@@ -845,6 +866,16 @@ void do_exchange(sis_gpu_ctx *c, double *replica_energies, int32_t *accepted, do
 // One job_md step for every replica (src/job_md.F90:362-460), one launch
 // per phase.
 void dcd_set_box(sis_gpu_ctx *c);   // sis_output_host.h
+
+// constant forces of every replica from its label's magnitude
+void dcf_upload(sis_gpu_ctx *c) {
+  const int R = c->R;
+  std::vector<double> f((size_t)3 * R);
+  for (int r = 0; r < R; r++)
+    for (int k = 0; k < 3; k++) f[3 * r + k] = c->lab2dcf[c->rep2lab[r + 1]] * c->dcf_dir[k];
+  HIPCHK(hipMemcpyAsync(c->x_dcf_f_d.p, f.data(), f.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+}
 
 // current entry of the timed Rg bias: k and the target of every replica
 void rg_schedule_apply(sis_gpu_ctx *c) {
@@ -1098,6 +1129,7 @@ int sis_gpu_create(const sis_topo *topo, const sis_params *prm, const sis_option
     for (int k = 1; k <= R; k++) { c->rep2lab[k] = k; c->lab2rep[k] = k; c->lab2val[k] = prm->tempK; }
     c->lab2ion.assign(R + 1, prm->ionic_strength);
     c->lab2rg.assign(R + 1, 0.0);
+    c->lab2dcf.assign(R + 1, 0.0);
     c->pair_tb.assign(2 * (R + 2), 0);
     for (int par = 0; par < 2; par++) {
       int *tb = c->pair_tb.data() + par * (R + 2);
@@ -1589,6 +1621,22 @@ int sis_gpu_set_step(sis_gpu_ctx *c, int64_t istep) {
   API_END
 }
 
+int sis_gpu_set_rep_dcf(sis_gpu_ctx *c, const double *k, const double *direction) {
+  API_BEGIN
+  if (!c) return fail(SIS_ERR_ARG, "null context");
+  if (!k) { c->dcf_remd = false; return SIS_OK; }
+  if (!direction) return fail(SIS_ERR_ARG, "null direction");
+  if (!(c->ext_on && c->X.ndcf == 1)) return fail(SIS_ERR_STATE, "force exchange needs exactly one constant-force pair");
+  const double n = std::sqrt(direction[0] * direction[0] + direction[1] * direction[1] + direction[2] * direction[2]);
+  if (!(n > 0.0)) return fail(SIS_ERR_ARG, "zero force direction");
+  for (int i = 0; i < 3; i++) c->dcf_dir[i] = direction[i] / n;   // src/init_tweezers.F90:32
+  for (int r = 0; r < c->R; r++) c->lab2dcf[c->rep2lab[r + 1]] = k[r];
+  c->dcf_remd = true;
+  dcf_upload(c);
+  return SIS_OK;
+  API_END
+}
+
 int sis_gpu_set_rep_rg(sis_gpu_ctx *c, const double *rg0) {
   API_BEGIN
   if (!c) return fail(SIS_ERR_ARG, "null context");
@@ -1806,7 +1854,8 @@ int sis_gpu_set_external(sis_gpu_ctx *c, const sis_external *e) {
   X.stage_sigma = e->stage_sigma; X.stage_eps = e->stage_eps; X.rg_k = e->rg_k;
   std::vector<double> rg0(R, 0.0);
   if (e->rg) std::copy(e->rg_0, e->rg_0 + R, rg0.begin());
-  c->rg_remd = false;   // a ladder over the Rg target is set again after the terms
+  c->rg_remd = false;   // ladders over the Rg target or the force are set again after the terms
+  c->dcf_remd = false;
   X.rg0 = up(c->x_rg0_d, rg0.data(), (size_t)R);
   X.term_off = up(c->x_term_off_d, off.data(), off.size());
   X.terms = up(c->x_terms_d, terms.data(), terms.size());
@@ -1852,6 +1901,7 @@ int sis_gpu_set_labels(sis_gpu_ctx *c, const int32_t *rep2lab) {
     HIPCHK(hipMemcpyAsync(c->x_rg0_d.p, g.data(), R * sizeof(double), hipMemcpyHostToDevice, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
   }
+  if (c->ext_on && c->dcf_remd) dcf_upload(c);
   // the pair table restarts from its first half, as var_replica's idx_pair
   // does on a restarted run (it is not in the restart file,
   // src/var_replica.F90:85, src/write_rst.F90:40-52)

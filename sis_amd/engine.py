@@ -232,6 +232,13 @@ class Engine:
         tk = np.ascontiguousarray(temps, np.float64)
         _check(lib().sis_gpu_set_anneal(self.h, len(st), st.ctypes.data_as(C.POINTER(C.c_int64)), abi.dptr(tk)))
 
+    def set_rep_dcf(self, k, direction):
+        """f-REMD: constant-force magnitude k per replica's label along the
+        normalised direction of the one constant-force pair."""
+        kk = np.ascontiguousarray(k, np.float64)
+        dd = np.ascontiguousarray(direction, np.float64)
+        _check(lib().sis_gpu_set_rep_dcf(self.h, abi.dptr(kk), abi.dptr(dd)))
+
     def set_rep_rg(self, rg0):
         """Replica exchange over the Rg-bias target: rg0 per replica's label."""
         if rg0 is None:

@@ -237,6 +237,12 @@ module sis_gpu_iface
          type(c_ptr), value :: ctx
       end function
 
+      ! replica exchange over the tweezers' constant force
+      integer(c_int) function sis_gpu_set_rep_dcf(ctx, k, direction) bind(C, name="sis_gpu_set_rep_dcf")
+         import :: c_int, c_ptr
+         type(c_ptr), value :: ctx, k, direction
+      end function
+
       ! replica exchange over the Rg-bias target
       integer(c_int) function sis_gpu_set_rep_rg(ctx, rg0) bind(C, name="sis_gpu_set_rep_rg")
          import :: c_int, c_ptr

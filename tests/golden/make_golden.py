@@ -216,6 +216,23 @@ def remd_rg():
     return sysd
 
 
+def remd_force():
+    """Replica exchange over the tweezers' constant force (f-REMD,
+    REPT%TWZDCF: one pulled pair, 2-8 kcal/mol/A along one direction, at one
+    temperature; src/init_tweezers.F90:24-40, src/energy_replica.F90:103-105)."""
+    sysd = remd_external(nrep=4)
+    sysd["temps"] = np.full(4, float(sysd["params"]["tempK"]))
+    ext = dict(sysd["external"])
+    ext["dcf_pairs"] = ext["dcf_pairs"][:1]
+    sysd["remd_dcf"] = np.array([2.0, 4.0, 6.0, 8.0])
+    sysd["dcf_dir"] = np.array([0.3, -0.5, 0.8])
+    u = sysd["dcf_dir"] / np.linalg.norm(sysd["dcf_dir"])
+    ext["dcf_forces"] = sysd["remd_dcf"][:, None, None] * u[None, None, :]
+    sysd["external"] = ext
+    sysd["name"] = "remd4dcf"
+    return sysd
+
+
 SYSTEMS = {
     "hairpin64": (lambda: configs.hairpin64(1), 60, 0),
     "small_pbc": (lambda: configs.small_pbc(4, 30, 100.0, seed=5), 60, 0),
@@ -231,6 +248,7 @@ SYSTEMS = {
     "remd2rgs": (remd_rg_schedule, 30, 10),
     "remd4ion": (remd_ion, 40, 10),
     "remd4rg": (remd_rg, 40, 10),
+    "remd4dcf": (remd_force, 40, 10),
 }
 
 
@@ -245,6 +263,8 @@ def make(name, builder, nstep, nex, seed=4321):
     pre = [("external", ext, 0)] if ext else []
     if s.get("rg_schedule") is not None:
         pre.append(("rg_schedule",) + tuple(s["rg_schedule"]))
+    if s.get("remd_dcf") is not None:
+        pre.append(("dcf_remd", s["dcf_dir"]))
     if s.get("ss") is not None:
         pre.append(("ss", s["ss"]))
     if s.get("anneal") is not None:
@@ -267,6 +287,8 @@ def make(name, builder, nstep, nex, seed=4321):
         o.set_rg_schedule(*s["rg_schedule"])
     if s.get("remd_rg") is not None:
         o.set_rep_rg(s["remd_rg"])
+    if s.get("remd_dcf") is not None:
+        o.set_rep_dcf(s["remd_dcf"], s["dcf_dir"])
     if s.get("ss") is not None:
         o.set_ss(s["ss"])
     if s.get("anneal") is not None:
@@ -290,6 +312,8 @@ def make(name, builder, nstep, nex, seed=4321):
         o2.set_rg_schedule(*s["rg_schedule"])
     if s.get("remd_rg") is not None:
         o2.set_rep_rg(s["remd_rg"])
+    if s.get("remd_dcf") is not None:
+        o2.set_rep_dcf(s["remd_dcf"], s["dcf_dir"])
     if s.get("ss") is not None:
         o2.set_ss(s["ss"])
     if s.get("anneal") is not None:
@@ -332,6 +356,9 @@ def make(name, builder, nstep, nex, seed=4321):
         out["remd_ion"] = np.asarray(s["remd_ion"], np.float64)
     if s.get("remd_rg") is not None:
         out["remd_rg"] = np.asarray(s["remd_rg"], np.float64)
+    if s.get("remd_dcf") is not None:
+        out["remd_dcf"] = np.asarray(s["remd_dcf"], np.float64)
+        out["dcf_dir"] = np.asarray(s["dcf_dir"], np.float64)
     if s.get("rg_schedule") is not None:
         out["rgs_steps"], out["rgs_k"], out["rgs_0"] = [np.asarray(a) for a in s["rg_schedule"]]
     if s.get("resize") is not None:
@@ -375,6 +402,8 @@ def load(name):
         sysd["remd_ion"] = d["remd_ion"]
     if "remd_rg" in d:
         sysd["remd_rg"] = d["remd_rg"]
+    if "remd_dcf" in d:
+        sysd["remd_dcf"], sysd["dcf_dir"] = d["remd_dcf"], d["dcf_dir"]
     if "rgs_steps" in d:
         sysd["rg_schedule"] = (d["rgs_steps"], d["rgs_k"], d["rgs_0"])
     if "resize_step" in d:

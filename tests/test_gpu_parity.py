@@ -20,7 +20,7 @@ from tests.golden.make_golden import load
 
 pytestmark = pytest.mark.gpu
 
-FIXTURES = ["hairpin64", "small_pbc", "mixed", "remd4", "remd4mc3", "remd4bd", "remd2ext", "hairpin_ss", "hairpin_anneal", "pbc_resize", "cag47_m1", "remd2rgs", "remd4ion", "remd4rg"]
+FIXTURES = ["hairpin64", "small_pbc", "mixed", "remd4", "remd4mc3", "remd4bd", "remd2ext", "hairpin_ss", "hairpin_anneal", "pbc_resize", "cag47_m1", "remd2rgs", "remd4ion", "remd4rg", "remd4dcf"]
 
 
 def _sorted_rows(a):
@@ -41,6 +41,8 @@ def _engine(sysd, d=None, **kw):
         eng.set_rg_schedule(*sysd["rg_schedule"])
     if "remd_rg" in sysd:
         eng.set_rep_rg(sysd["remd_rg"])
+    if "remd_dcf" in sysd:
+        eng.set_rep_dcf(sysd["remd_dcf"], sysd["dcf_dir"])
     if "ss" in sysd:
         eng.set_ss(sysd["ss"])
     if "anneal" in sysd:

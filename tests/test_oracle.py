@@ -10,7 +10,7 @@ from sis_amd import configs
 from oracle import corc, refdrv
 from tests.golden.make_golden import load
 
-FIXTURES = ["hairpin64", "small_pbc", "mixed", "remd4", "remd4mc3", "remd4bd", "remd2ext", "hairpin_ss", "hairpin_anneal", "pbc_resize", "cag47_m1", "remd2rgs", "remd4ion", "remd4rg"]
+FIXTURES = ["hairpin64", "small_pbc", "mixed", "remd4", "remd4mc3", "remd4bd", "remd2ext", "hairpin_ss", "hairpin_anneal", "pbc_resize", "cag47_m1", "remd2rgs", "remd4ion", "remd4rg", "remd4dcf"]
 
 
 def _oracle_from_fixture(name):
@@ -24,6 +24,8 @@ def _oracle_from_fixture(name):
         o.set_rg_schedule(*sysd["rg_schedule"])
     if "remd_rg" in sysd:
         o.set_rep_rg(sysd["remd_rg"])
+    if "remd_dcf" in sysd:
+        o.set_rep_dcf(sysd["remd_dcf"], sysd["dcf_dir"])
     if "ss" in sysd:
         o.set_ss(sysd["ss"])
     if "anneal" in sysd:
