@@ -317,6 +317,15 @@ int sis_gpu_out_close(sis_gpu_ctx *ctx);
  * src/job_md.F90:280-291,331-333 (energies with a fresh limit MC). */
 int sis_gpu_out_write(sis_gpu_ctx *ctx);
 
+/* Tweezers' force-ramp lines (.twz, src/init_out_files.F90:133-139;
+ * written by force_tweezers, src/force_tweezers.F90:50-59): one file, its
+ * header, then at every step whose count is a multiple of nstep_save one
+ * line per replica and force-ramp pair (step, pair, trap distance, bead
+ * distance, bead-trap distances, forces in pN) from the positions at the
+ * start of that step. */
+int sis_gpu_twz_open(sis_gpu_ctx *ctx, const char *path, int64_t nstep_save);
+int sis_gpu_twz_close(sis_gpu_ctx *ctx);
+
 /* Restart files (.rst) in the block format of write_rst
  * (src/write_rst.F90:25-145; block ids src/const_idx.F90:72-85), one per
  * replica: REPLICA (replica 1's file, nrep > 1), STEP, XYZ, VELO, ACCEL,

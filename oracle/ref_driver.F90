@@ -55,6 +55,7 @@ module sisref_driver
    ! REMD over another variable than the temperature (REPT id: 3 ionic
    ! strength, 4 Rg target): set by sisref_remd_var before sisref_setup
    integer, save :: remd_kind = 0
+   integer(c_int64_t), save :: twz_nsave = 0   ! nstep_save of a job_md run that writes .twz lines
    real(c_double), allocatable, save :: remd_values(:)
 
 contains
@@ -633,6 +634,28 @@ contains
 
    ! f-REMD: the normalised direction of the one constant-force pair and
    ! its per-replica forces, as init_tweezers sets them (src/init_tweezers.F90:24-40)
+   ! Let force_tweezers write its .twz lines (src/force_tweezers.F90:50-59)
+   ! into path, with the header of src/init_out_files.F90:133-139, every
+   ! nsave steps of the job_md run that follows.
+   subroutine sisref_open_twz(path, plen, nsave) bind(C, name="sisref_open_twz")
+      use var_io, only : flg_out_twz, hdl_twz
+      use var_state, only : nstep_save
+      character(kind=c_char), intent(in) :: path(*)
+      integer(c_int), value :: plen
+      integer(c_int64_t), value :: nsave
+      character(len=:), allocatable :: name
+      integer :: k
+      allocate(character(len=plen) :: name)
+      do k = 1, plen
+         name(k:k) = path(k)
+      enddo
+      hdl_twz = 95
+      open(hdl_twz, file=name, status='replace', action='write', form='formatted')
+      write(hdl_twz, '(a)') '#  (1)step  (2)id (3)d[traps]   (4)d[beads]   (5)d[t1-b1]   (6)d[t2-b2]   (7)f[b1]      (8)f[b2]'
+      flg_out_twz = .True.
+      twz_nsave = nsave
+   end subroutine sisref_open_twz
+
    subroutine sisref_set_dcf_remd(dir) bind(C, name="sisref_set_dcf_remd")
       use var_potential, only : twz_DCF_direction, twz_DCF_forces
       use var_replica, only : nrep_proc, rep2val, irep2grep
@@ -741,6 +764,7 @@ contains
       real(c_double), intent(out) :: energies_out(0:10, *)
       nstep = nstep_in
       nstep_save = max(nstep_in, 1_c_int64_t)
+      if (twz_nsave > 0) nstep_save = twz_nsave
       call job_md()
       energies_out(:, 1:nrep_proc) = energies(:, 1:nrep_proc)
    end subroutine sisref_job_md

@@ -169,6 +169,9 @@ def _worker(q, so, sysd, ops, seed, nstep_exchange):
                     _ip(fp), _dp(fk), _dp(fi), _dp(fv), C.c_int(int(ext.get("rg", 0))),
                     C.c_double(float(ext.get("rg_k", 0.0))), _dp(rg0), C.c_int(nr), _ip(ri), _dp(rc), _dp(rp),
                     C.c_int64(int(op[2])))
+            elif kind == "twz":
+                fn = str(op[1]).encode()
+                lib.sisref_open_twz(C.c_char_p(fn), C.c_int(len(fn)), C.c_int64(int(op[2])))
             elif kind == "dcf_remd":
                 dd = np.ascontiguousarray(op[1], np.float64)
                 lib.sisref_set_dcf_remd(_dp(dd))

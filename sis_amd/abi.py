@@ -197,6 +197,7 @@ EXPORTED = [
     "sis_rst_write_file", "sis_rst_read_file", "sis_gpu_set_external", "sis_gpu_set_ss",
     "sis_gpu_set_anneal", "sis_gpu_set_box_resize", "sis_gpu_out_write",
     "sis_gpu_set_rg_schedule", "sis_gpu_set_rep_rg", "sis_gpu_set_rep_dcf",
+    "sis_gpu_twz_open", "sis_gpu_twz_close",
 ]
 
 
@@ -254,6 +255,8 @@ def _declare(lib: C.CDLL) -> None:
         "sis_gpu_set_rg_schedule": (i32, [vp, i32, P(i64), P(d), P(d)]),
         "sis_gpu_set_rep_rg": (i32, [vp, P(d)]),
         "sis_gpu_set_rep_dcf": (i32, [vp, P(d), P(d)]),
+        "sis_gpu_twz_open": (i32, [vp, C.c_char_p, i64]),
+        "sis_gpu_twz_close": (i32, [vp]),
         "sis_gpu_rst_open": (i32, [vp, P(C.c_char_p), i64]),
         "sis_gpu_rst_write": (i32, [vp]),
         "sis_gpu_rst_close": (i32, [vp]),

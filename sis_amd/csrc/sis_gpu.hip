@@ -79,12 +79,14 @@ struct DcdOut;
 struct BpOut;
 struct OutFile;
 struct RstOut;
+struct TwzOut;
 
 struct sis_gpu_ctx {
   DcdOut *dcd = nullptr;   // trajectory output (sis_gpu_dcd_open), or none
   BpOut *bpo = nullptr;    // base-pair output (sis_gpu_bp_open), or none
   OutFile *outf = nullptr; // energy lines (sis_gpu_out_open), or none
   RstOut *rst = nullptr;   // restart files (sis_gpu_rst_open), or none
+  TwzOut *twz = nullptr;   // tweezers' force-ramp lines (sis_gpu_twz_open), or none
   // annealing schedule (sis_gpu_set_anneal): temperatures from step
   // an_step[k]; ele_T holds the temperature the electrostatics were set at
   bool anneal_on = false;
@@ -1185,6 +1187,7 @@ int sis_gpu_destroy(sis_gpu_ctx *c) {
     bp_shutdown(c);
     out_shutdown(c);
     rst_shutdown(c);
+    twz_shutdown(c);
   }
   delete c;
   return SIS_OK;
@@ -1538,6 +1541,10 @@ int sis_gpu_step(sis_gpu_ctx *c, int64_t nsteps, int64_t *rebuilds) {
   const int64_t rs = (c->rst && c->rst->nstep_save > 0) ? c->rst->nstep_save : 0;
   for (int64_t left = nsteps; left > 0;) {
     const int64_t rz = c->rz_on ? c->rz_step : 0;
+    // .twz lines are taken before a step whose count is a multiple of the
+    // cadence (the force of that step writes them)
+    const int64_t tw = (c->twz && c->ext_on && c->X.nfr > 0) ? c->twz->nstep_save : 0;
+    if (tw && (c->istep + 1) % tw == 0) twz_lines(c, c->istep + 1);
     // the next annealing entry is taken at the end of step an_next - 1
     // (src/job_md.F90:600-612)
     const bool an = c->anneal_on && c->an_next > 0 && c->an_next - 1 > c->istep;
@@ -1551,6 +1558,10 @@ int sis_gpu_step(sis_gpu_ctx *c, int64_t nsteps, int64_t *rebuilds) {
     if (an) k = std::min(k, c->an_next - 1 - c->istep);
     if (rz) k = std::min(k, rz - c->istep % rz);
     if (rgs) k = std::min(k, c->rgs_next - 1 - c->istep);
+    if (tw) {   // up to the step before the next multiple
+      const int64_t m = (c->istep + 1) % tw;
+      k = std::min(k, m == 0 ? tw : tw - m);
+    }
     run(k);
     left -= k;
     const bool exch = ex && c->istep % ex == 0;

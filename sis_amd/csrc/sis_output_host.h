@@ -570,6 +570,97 @@ int sis_gpu_out_open(sis_gpu_ctx *c, const char *const *paths, int64_t nstep_sav
   API_END
 }
 
+// Tweezers' force-ramp lines (.twz; src/init_out_files.F90:133-139,
+// src/force_tweezers.F90:50-59): written from the positions at the start
+// of every step whose count is a multiple of nstep_save, for every replica
+// in turn, into one file.
+struct TwzOut {
+  FILE *f = nullptr;
+  int64_t nstep_save = 0;
+};
+
+namespace {
+
+void twz_lines(sis_gpu_ctx *c, int64_t istep) {
+  const int R = c->R, N = c->N, nfr = c->X.nfr;
+  if (!c->twz || !c->twz->f || nfr <= 0) return;
+  HIPCHK(hipStreamSynchronize(c->stream));
+  std::vector<double4> h((size_t)R * N);
+  HIPCHK(hipMemcpy(h.data(), c->pos_d.p, h.size() * sizeof(double4), hipMemcpyDeviceToHost));
+  std::vector<int> pairs(2 * nfr);
+  std::vector<double> kk(2 * nfr), init(6 * nfr), velo(6 * nfr);
+  HIPCHK(hipMemcpy(pairs.data(), c->X.fr_pairs, pairs.size() * sizeof(int), hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(kk.data(), c->X.fr_k, kk.size() * sizeof(double), hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(init.data(), c->X.fr_init, init.size() * sizeof(double), hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(velo.data(), c->X.fr_velo, velo.size() * sizeof(double), hipMemcpyDeviceToHost));
+  const double pN = (1.0 / 4184.0) * 6.02214076e23 * 1.0e-22;   // JOUL2KCAL_MOL * 1e-22
+  auto norm = [](const double *v) { return std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]); };
+  auto pvec = [&](const double *a, const double *b, double *v) {   // pbc_vec_d (src/pbc.F90:140-165)
+    for (int k = 0; k < 3; k++) {
+      v[k] = a[k] - b[k];
+      if (c->prm.pbc) {
+        const double L = c->prm.box[k], hL = 0.5 * L;
+        if (v[k] > hL) v[k] -= L; else if (v[k] < -hL) v[k] += L;
+      }
+    }
+  };
+  std::string s;
+  char b[64];
+  for (int r = 0; r < R; r++)
+    for (int q = 0; q < nfr; q++) {
+      const double4 x1 = h[(size_t)r * N + pairs[2 * q]], x2 = h[(size_t)r * N + pairs[2 * q + 1]];
+      const double b1[3] = {x1.x, x1.y, x1.z}, b2[3] = {x2.x, x2.y, x2.z};
+      double p1[3], p2[3], v1[3], v2[3], d12[3], dp[3];
+      for (int k = 0; k < 3; k++) {
+        p1[k] = init[6 * q + k] + velo[6 * q + k] * (double)istep;
+        p2[k] = init[6 * q + 3 + k] + velo[6 * q + 3 + k] * (double)istep;
+        d12[k] = b2[k] - b1[k];
+        dp[k] = p2[k] - p1[k];
+      }
+      pvec(p1, b1, v1);
+      pvec(p2, b2, v2);
+      snprintf(b, sizeof b, "%13lld %3d", (long long)istep, q + 1);
+      s += b;
+      const double vals[6] = {norm(dp), norm(d12), norm(v1), norm(v2), kk[2 * q] * norm(v1) / pN,
+                              kk[2 * q + 1] * norm(v2) / pN};
+      for (double v : vals) s += " " + fortran_g(v, 15, 8);
+      s += "\n";
+    }
+  if (fwrite(s.data(), 1, s.size(), c->twz->f) != s.size()) throw ApiError{SIS_ERR_ARG, "write failed (.twz)"};
+}
+
+void twz_shutdown(sis_gpu_ctx *c) {
+  if (!c->twz) return;
+  if (c->twz->f) fclose(c->twz->f);
+  delete c->twz;
+  c->twz = nullptr;
+}
+
+}  // namespace
+
+int sis_gpu_twz_open(sis_gpu_ctx *c, const char *path, int64_t nstep_save) {
+  API_BEGIN
+  if (!c || !path || nstep_save <= 0) return fail(SIS_ERR_ARG, "invalid argument");
+  if (c->twz) return fail(SIS_ERR_STATE, "tweezers output already open");
+  FILE *f = fopen(path, "wb");
+  if (!f) return fail(SIS_ERR_ARG, std::string("cannot open ") + path);
+  const char *hdr = "#  (1)step  (2)id (3)d[traps]   (4)d[beads]   (5)d[t1-b1]   (6)d[t2-b2]   (7)f[b1]      (8)f[b2]\n";
+  fputs(hdr, f);
+  c->twz = new TwzOut();
+  c->twz->f = f;
+  c->twz->nstep_save = nstep_save;
+  return SIS_OK;
+  API_END
+}
+
+int sis_gpu_twz_close(sis_gpu_ctx *c) {
+  API_BEGIN
+  if (!c) return fail(SIS_ERR_ARG, "null context");
+  twz_shutdown(c);
+  return SIS_OK;
+  API_END
+}
+
 int sis_gpu_out_write(sis_gpu_ctx *c) {
   API_BEGIN
   if (!c) return fail(SIS_ERR_ARG, "null context");

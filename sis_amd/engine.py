@@ -232,6 +232,13 @@ class Engine:
         tk = np.ascontiguousarray(temps, np.float64)
         _check(lib().sis_gpu_set_anneal(self.h, len(st), st.ctypes.data_as(C.POINTER(C.c_int64)), abi.dptr(tk)))
 
+    def twz_open(self, path, nstep_save: int):
+        """Tweezers' force-ramp lines (.twz) every nstep_save steps."""
+        _check(lib().sis_gpu_twz_open(self.h, str(path).encode(), int(nstep_save)))
+
+    def twz_close(self):
+        _check(lib().sis_gpu_twz_close(self.h))
+
     def set_rep_dcf(self, k, direction):
         """f-REMD: constant-force magnitude k per replica's label along the
         normalised direction of the one constant-force pair."""

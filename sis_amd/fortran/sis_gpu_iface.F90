@@ -237,6 +237,19 @@ module sis_gpu_iface
          type(c_ptr), value :: ctx
       end function
 
+      ! tweezers' force-ramp lines (.twz)
+      integer(c_int) function sis_gpu_twz_open(ctx, path, nstep_save) bind(C, name="sis_gpu_twz_open")
+         import :: c_int, c_ptr, c_int64_t, c_char
+         type(c_ptr), value :: ctx
+         character(kind=c_char), intent(in) :: path(*)
+         integer(c_int64_t), value :: nstep_save
+      end function
+
+      integer(c_int) function sis_gpu_twz_close(ctx) bind(C, name="sis_gpu_twz_close")
+         import :: c_int, c_ptr
+         type(c_ptr), value :: ctx
+      end function
+
       ! replica exchange over the tweezers' constant force
       integer(c_int) function sis_gpu_set_rep_dcf(ctx, k, direction) bind(C, name="sis_gpu_set_rep_dcf")
          import :: c_int, c_ptr

@@ -276,7 +276,14 @@ def make(name, builder, nstep, nex, seed=4321):
     ops += [("lists", 1, "L"), ("force", 1, fmc, "F"), ("bp_state", 1, "B"),
             ("energy", 1, float(s["temps"][0]), 0, 1, "E"), ("ele", 1, "ele"), ("bp_tables", "T")]
     ref0 = refdrv.run(s, ops, seed=seed, nstep_exchange=nex)
-    ref1 = refdrv.run(s, pre + [("job_md", nstep, "R")], seed=seed, nstep_exchange=nex)
+    twz = None
+    if ext and ext.get("fr_pairs") is not None:   # the reference's .twz lines of the run, every 10 steps
+        import tempfile
+        twz = Path(tempfile.mkdtemp()) / "run.twz"
+        pre1 = pre + [("twz", twz, 10)]
+    else:
+        pre1 = pre
+    ref1 = refdrv.run(s, pre1 + [("job_md", nstep, "R")], seed=seed, nstep_exchange=nex)
     # count the words the static part + the run consume, using the oracle
     o = corc.Oracle(s)
     if s.get("remd_ion") is not None:
@@ -352,6 +359,8 @@ def make(name, builder, nstep, nex, seed=4321):
         out["ext_" + k] = np.asarray(v)
     if s.get("ss") is not None:
         out["ss_pairs"] = np.asarray(s["ss"], np.int32)
+    if twz is not None:
+        out["twz_text"] = np.array(twz.read_text())
     if s.get("remd_ion") is not None:
         out["remd_ion"] = np.asarray(s["remd_ion"], np.float64)
     if s.get("remd_rg") is not None:

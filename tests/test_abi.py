@@ -18,7 +18,7 @@ ROOT = Path(__file__).resolve().parent.parent
 
 def _declared():
     txt = (ROOT / "include" / "sis_gpu.h").read_text()
-    return sorted(set(re.findall(r"\b(sis_(?:gpu|pool|dcd|bp|out|rst)_[a-z_]+)\s*\(", txt)))
+    return sorted(set(re.findall(r"\b(sis_(?:gpu|pool|dcd|bp|out|rst|twz)_[a-z_]+)\s*\(", txt)))
 
 
 def test_library_exports_every_declared_symbol():

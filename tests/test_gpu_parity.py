@@ -107,6 +107,34 @@ def test_trajectory_matches_reference(name):
         assert np.array_equal(eng.labels(), d["md_rep2lab"][: sysd["nrep"]])
 
 
+def test_twz_lines_match_reference(tmp_path):
+    """Tweezers' force-ramp lines (.twz) written during the remd2ext run
+    (every 10 steps, from the positions at the start of the step, both
+    replicas) against the reference's own lines of the same run: header and
+    layout identical (i13, i3, g15.8 fields), values within 1e-6 relative
+    (the trajectories agree to 1e-7 A)."""
+    sysd, d = load("remd2ext")
+    print("[case] twz remd2ext", flush=True)
+    eng = _engine(sysd, d, exchange_every=int(d["nstep_exchange"]))
+    path = tmp_path / "dev.twz"
+    eng.twz_open(path, 10)
+    eng.init_velocities()
+    eng.neighbor_list(0)
+    for r in range(1, sysd["nrep"] + 1):
+        eng.energy_sumup(r, -1.0, 1, 0)
+    eng.step(int(d["nstep"]))
+    eng.twz_close()
+    got = path.read_text().splitlines()
+    want = str(d["twz_text"]).splitlines()
+    assert len(got) == len(want) == 1 + 3 * sysd["nrep"]
+    assert got[0] == want[0]
+    for g, w in zip(got[1:], want[1:]):
+        assert len(g) == len(w)
+        gs, ws = g.split(), w.split()
+        assert gs[:2] == ws[:2]
+        assert np.allclose(np.array(gs[2:], float), np.array(ws[2:], float), rtol=1e-6, atol=1e-6)
+
+
 def test_condensate_full_size_lists_and_forces():
     """C2 at full size (200 x 47 nt, periodic 2,499 A box): cell-list
     rebuild gives the oracle's pair sets bit-exactly; FP64 forces agree."""
