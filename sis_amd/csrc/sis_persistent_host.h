@@ -275,7 +275,7 @@ void batched_steps(sis_gpu_ctx *c, int64_t nsteps) {
       HIPCHK(hipStreamSynchronize(c->stream));
       if (f[7]) throw ApiError{SIS_ERR_STATE, "limit MC met a base-pair candidate outside the system"};
       if (f[1] == 2) throw ApiError{SIS_ERR_STATE, "limit MC found an excess nucleotide without pairs"};
-      if (f[5]) throw ApiError{SIS_ERR_CAPACITY, "a base-pair conflict component exceeds 256 formed pairs"};
+      if (f[5]) throw ApiError{SIS_ERR_CAPACITY, "a base-pair conflict component outgrew the limit-MC scratch"};
       bool lag = false;
       std::fill(mcf.begin(), mcf.end(), 0);
       std::vector<int> lagm(R, 0);
