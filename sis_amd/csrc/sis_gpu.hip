@@ -1230,6 +1230,17 @@ int sis_gpu_get_state(sis_gpu_ctx *c, double *xyz, double *velos, double *accels
 int sis_gpu_set_rep_conditions(sis_gpu_ctx *c, const double *tempK, const double *ionic) {
   API_BEGIN
   if (!c) return fail(SIS_ERR_ARG, "null context");
+  if (tempK) {
+    // new temperatures reset the labels to the identity: the values the
+    // labels carry are re-keyed to the replicas that hold them now
+    const int R = c->R;
+    std::vector<double> ion(R), rg(R), dcf(R);
+    for (int r = 0; r < R; r++) {
+      const int lab = c->rep2lab[r + 1];
+      ion[r] = c->lab2ion[lab]; rg[r] = c->lab2rg[lab]; dcf[r] = c->lab2dcf[lab];
+    }
+    for (int r = 0; r < R; r++) { c->lab2ion[r + 1] = ion[r]; c->lab2rg[r + 1] = rg[r]; c->lab2dcf[r + 1] = dcf[r]; }
+  }
   for (int r = 0; r < c->R; r++) {
     if (tempK) {
       if (!(tempK[r] > 0.0)) return fail(SIS_ERR_ARG, "temperature must be positive");
