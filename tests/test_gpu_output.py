@@ -130,6 +130,47 @@ def test_restart_files_continue_the_run(tmp_path, mode):
     assert np.abs(va - vc).max() < 1e-5
 
 
+def test_restart_files_carry_anneal_and_box(tmp_path):
+    """An annealed run with box resizing, stopped at step 50 and restarted
+    from its restart files: the ANNEAL block (entry 2 of the 0/30/70
+    schedule) and the PBC block (the resized box) bring the new context to
+    the uninterrupted run's conditions, which it then follows to step 100
+    (src/job_md.F90:207-230, src/pbc.F90:35-45)."""
+    from sis_amd.engine import read_rst_file
+    s = configs.small_pbc(4, 30, 100.0, seed=5)
+    N = int(np.sum(s["nmp_chain"]))
+    sched = ([0, 30, 70], [330.0, 310.0, 290.0])
+    print("[case] restart with anneal and box resize", flush=True)
+
+    def fresh():
+        e = Engine(s, mode="phases", seed=13, precision=0)
+        e.set_anneal(*sched)
+        e.set_box_resize(25, [-0.5, 0.0, 0.5])
+        return e
+
+    a = fresh()
+    a.init_velocities()
+    a.neighbor_list(0)
+    a.step(100)
+    b = fresh()
+    b.init_velocities()
+    b.neighbor_list(0)
+    path = tmp_path / "run_0001.rst"
+    b.rst_open([path], nstep_save_rst=50)
+    b.step(50)
+    b.rst_close()
+    f = read_rst_file(path, N, 1)
+    assert 2 in f["blocks"] and 10 in f["blocks"] and f["istep"] == 50
+    assert np.allclose(f["box"], [99.0, 100.0, 101.0])   # resized at steps 25 and 50
+    c = Engine(s, mode="phases", seed=13, precision=0)
+    c.rst_read([path])
+    c.set_anneal(*sched)                         # entry 2 from the ANNEAL block
+    c.set_box_resize(25, [-0.5, 0.0, 0.5])       # counts from the restart's box
+    c.step(50)
+    xa, xc = a.get_state()[0], c.get_state()[0]
+    assert np.abs(xa - xc).max() < 1e-6
+
+
 @pytest.mark.parametrize("case", ["batched", "pbc_phases"])
 def test_bp_records_during_steps(tmp_path, case):
     """Base-pair records (.bpall: every formed pair; the E <= -kT filter of
