@@ -214,7 +214,8 @@ void batched_steps(sis_gpu_ctx *c, int64_t nsteps) {
     return;
   }
   const int N = c->N, R = c->R;
-  constexpr int B = 256;
+  // 256 threads (4 waves) per replica by default; 512 (8 waves) on request
+  const int B = c->opt.block == 512 ? 512 : 256;
   const size_t RN = (size_t)R * N;
   std::vector<int> all(R, 1), mcf(R, 0);
   if (!c->lists_valid) mcf = host_rebuild(c, all);
@@ -224,7 +225,8 @@ void batched_steps(sis_gpu_ctx *c, int64_t nsteps) {
     c->bb_stepno_d.alloc(R); c->bb_off_d.alloc(R + 1); c->bb_mcfirst_d.alloc(R); c->bb_nreb_d.alloc(R);
   }
   const int lpb = lanes_per_bead(N);
-  const void *kfn = c->precision == 1 ? (const void *)k_batched<float, B> : (const void *)k_batched<double, B>;
+  const void *kfn = B == 512 ? (c->precision == 1 ? (const void *)k_batched<float, 512> : (const void *)k_batched<double, 512>)
+                             : (c->precision == 1 ? (const void *)k_batched<float, 256> : (const void *)k_batched<double, 256>);
   c->block = B;
   int64_t left = nsteps;
   const int ex = c->opt.exchange_every;

@@ -358,6 +358,23 @@ def test_box_resize_persistent_form(tmp_path):
         assert np.abs(a - b).max() < 1e-6
 
 
+def test_batched_form_512_threads():
+    """The batched kernel at 512 threads (8 waves) per replica, an option of
+    sis_options.block, follows the per-phase form like the default 256."""
+    s = configs.remd_c3(nrep=8, n=128, seed=3)
+    print("[case] batched 512 threads vs phases", flush=True)
+    res = []
+    for mode, blk in (("phases", 0), ("batched", 512)):
+        eng = Engine(s, parity=False, mode=mode, seed=77, exchange_every=10, block=blk)
+        eng.init_velocities()
+        eng.neighbor_list(0)
+        eng.step(40)
+        res.append((eng.get_state(), eng.labels(), eng.info()["block"]))
+    assert res[1][2] == 512 and np.array_equal(res[0][1], res[1][1])
+    for a, b in zip(res[0][0], res[1][0]):
+        assert np.abs(a - b).max() < 1e-6
+
+
 def test_ion_exchange_batched_form():
     """Replica exchange over the ionic strength (8 replicas of a 128-nt
     chain at one temperature, 0.02-0.6 M): the batched form follows the
