@@ -48,6 +48,7 @@ def lib():
         L.orc_set_step.argtypes = [vp, i64]
         L.orc_set_anneal.argtypes = [vp, i32, P(i64), P(d)]
         L.orc_set_ionic.argtypes = [vp, P(d)]
+        L.orc_plant.argtypes = [vp, i32, i32]
         L.orc_set_rep_rg.argtypes = [vp, P(d)]
         L.orc_set_rep_dcf.argtypes = [vp, P(d), P(d)]
         L.orc_set_rg_schedule.argtypes = [vp, i32, P(i64), P(d), P(d)]
@@ -214,6 +215,11 @@ class Oracle:
         kk = np.ascontiguousarray(k, np.float64)
         rr = np.ascontiguousarray(r0, np.float64)
         lib().orc_set_rg_schedule(self.h, len(st), st.ctypes.data_as(C.POINTER(C.c_int64)), _d(kk), _d(rr))
+
+    def plant(self, which: int, on: bool = True):
+        """Switch on a planted mistake (1: set_ele moves the DH cutoff at
+        once), for tests that show a fixture catches it."""
+        lib().orc_plant(self.h, which, int(on))
 
     def set_ionic(self, ionic):
         """Per-label ionic strengths (REMD over REPT%ION)."""

@@ -76,6 +76,8 @@ struct orc_sys {
     tape_t tape_rep;
     int64_t istep;
     int flg_bp_energy;
+    /* planted mistakes, for tests that show a fixture catches them (orc_plant) */
+    int plant_cut_at_set_ele;
     /* external terms (copies of the caller's arrays) */
     int x_on;
     sis_external x;
@@ -277,8 +279,24 @@ static void set_ele(orc_sys *s, int irep, double tempk, double ionstr) {
     r->ele_coef = r->ele_coef * (Zp * Zp);
     r->lambdaD = 1.0e10 * sqrt((1.0e-3 * EPS0 * r->diele * BOLTZ_J) / (2.0 * N_AVO * (ELE * ELE))) *
                  sqrt(tempk / ionstr);
+    if (s->plant_cut_at_set_ele && s->p.ele_cutoff_type == 2) r->ele_cutoff = s->p.ele_cutoff * r->lambdaD;
+}
+
+/* Test hook: which = 1 makes set_ele move the cutoff at once (not the
+ * reference's behaviour; the planted mistake the ion/temperature ladder
+ * fixtures must catch). */
+void orc_plant(orc_sys *s, int32_t which, int32_t on) {
+    if (which == 1) s->plant_cut_at_set_ele = on;
+}
+
+/* set_ele at set-up, followed by init_ele's cutoff (src/init_ele.F90:73-101).
+ * Later calls of set_ele (exchange, energy_replica) leave ele_cutoff alone:
+ * it changes only in neighbor_list (src/neighbor_list.F90:80-82). */
+static void init_ele(orc_sys *s, int irep, double tempk, double ionstr) {
+    rep_t *r = &s->rep[irep];
+    set_ele(s, irep, tempk, ionstr);
     if (s->p.ele_cutoff_type == 1) r->ele_cutoff = s->p.ele_cutoff;
-    else r->ele_cutoff = s->p.ele_cutoff * r->lambdaD;   /* src/init_ele.F90:99-101 */
+    else r->ele_cutoff = s->p.ele_cutoff * r->lambdaD;
 }
 
 /* ---------------- MT19937 words and conversions ---------------- */
@@ -352,6 +370,7 @@ void orc_neighbor_list(orc_sys *s, int32_t irep1) {
     r->wca.n = r->ele.n = r->bp.n = r->bpcoef.n = 0;
     const double margin = s->p.nl_margin;
     double ele_nl_cut2 = 0.0;
+    if (s->p.ele_on && s->p.ele_cutoff_type == 2) r->ele_cutoff = s->p.ele_cutoff * r->lambdaD;
     if (s->p.ele_on) ele_nl_cut2 = (r->ele_cutoff + margin) * (r->ele_cutoff + margin);
     double wca_nl_cut2 = (s->p.wca_sigma + margin) * (s->p.wca_sigma + margin);
     double bp_nl_cut2 = (s->bp_cutoff_dist + margin) * (s->bp_cutoff_dist + margin);
@@ -995,7 +1014,7 @@ void orc_set_anneal(orc_sys *s, int32_t n, const int64_t *steps, const double *t
     s->ianneal = s->istep >= steps[n - 1] ? n : 1;
     s->an_next = s->ianneal < n ? steps[s->ianneal] : 0;
     for (int r = 0; r < s->nrep; r++) {
-        set_ele(s, r, temps[0], s->rep[r].ionic);
+        init_ele(s, r, temps[0], s->rep[r].ionic);
         s->rep[r].tK = temps[s->ianneal - 1];
     }
     s->mc_kT = BOLTZ_KCAL_MOL * s->p.tempK;
@@ -1308,13 +1327,13 @@ orc_sys *orc_create(const sis_topo *topo, const sis_params *prm, int32_t nrep, c
     for (int r = 0; r < nrep; r++) {
         s->rep[r].tK = (nrep > 1) ? s->lab2val[s->rep2lab[r + 1]] : prm->tempK;
         s->rep[r].ionic = prm->ionic_strength;
-        set_ele(s, r, s->rep[r].tK, s->rep[r].ionic);
+        init_ele(s, r, s->rep[r].tK, s->rep[r].ionic);
     }
     return s;
 }
 
 void orc_set_temps(orc_sys *s, const double *temps) {
-    for (int r = 0; r < s->nrep; r++) { s->rep[r].tK = temps[r]; set_ele(s, r, temps[r], s->rep[r].ionic); }
+    for (int r = 0; r < s->nrep; r++) { s->rep[r].tK = temps[r]; init_ele(s, r, temps[r], s->rep[r].ionic); }
 }
 
 /* REMD over the ionic strength (REPT%ION): label k carries ionic[k-1];
@@ -1324,7 +1343,7 @@ void orc_set_ionic(orc_sys *s, const double *ionic) {
     for (int k = 1; k <= s->nrep; k++) s->lab2ion[k] = ionic[k - 1];
     for (int r = 0; r < s->nrep; r++) {
         s->rep[r].ionic = s->lab2ion[s->rep2lab[r + 1]];
-        set_ele(s, r, s->rep[r].tK, s->rep[r].ionic);
+        init_ele(s, r, s->rep[r].tK, s->rep[r].ionic);
     }
 }
 

@@ -48,4 +48,5 @@ void orc_set_box_resize(orc_sys *s, int64_t resize_step, const double *change, c
 void orc_set_rg_schedule(orc_sys *s, int32_t n, const int64_t *steps, const double *k, const double *r0);
 void orc_set_anneal(orc_sys *s, int32_t n, const int64_t *steps, const double *temps);
 int32_t orc_set_ss(orc_sys *s, int32_t npair, const int32_t *pairs);   /* (2, npair) 1-based; npair < 0 clears */
+void orc_plant(orc_sys *s, int32_t which, int32_t on);   /* test hook: planted mistakes */
 #endif

@@ -115,6 +115,24 @@ def _mk(name, seqs, xyz, params, nrep=1, temps=None, nstep=1000):
                 nrep=nrep, temps=temps, nstep=nstep)
 
 
+def read_xyz_coords(path) -> np.ndarray:
+    """Coordinates of an XYZ file as the reference reads xyz_ini: a count
+    line, a comment line, then one `name x y z` line per bead."""
+    lines = open(path).read().splitlines()
+    n = int(lines[0].split()[0])
+    return np.array([[float(v) for v in l.split()[1:4]] for l in lines[2:2 + n]], np.float64)
+
+
+def read_pdb_coords(path) -> np.ndarray:
+    """Coordinates of the ATOM/HETATM records of a one-bead-per-nucleotide
+    PDB file (columns 31-54), in file order."""
+    out = []
+    for l in open(path):
+        if l.startswith(("ATOM", "HETATM")):
+            out.append([float(l[30:38]), float(l[38:46]), float(l[46:54])])
+    return np.array(out, np.float64)
+
+
 def hairpin64(seed: int = 1) -> dict:
     """C1: one 64-nt hairpin chain: 5-nt A/U flank, 24-bp stem (60% GC),
     8-nt loop, reverse-complement stem, 3-nt flank (5+24+8+24+3 = 64),

@@ -38,10 +38,12 @@ struct Params {
   double bp_cutoff_energy, bp_cutoff_dist, nl_margin;
   double wca_nl_cut2, bp_nl_cut2;
   double mc_kT;   // > 0: kT of the limit-MC acceptance when it is not the replica's (annealing)
+  double ele_cut_inp;   // [Electrostatic] cutoff as read (A, or Debye lengths with cutoff_type 2)
   int angl_reb, dih_type, ele_on, ele_excl_bond, ele_excl_angl;
   int bp_model, bp_min_loop, max_bp_per_nt, nstep_bp_mc, pbc;
   int N, R, fix_h4, nchains, maxlen;
   int bd;  // 1: Ermak-McCammon Brownian update in place of GJF-2GJ
+  int ele_cut_type;   // 1: fixed cutoff; 2: cutoff = ele_cut_inp * lambdaD
 };
 
 // 1/kT of the limit-MC acceptance.  The force's MC takes var_state's kT,
@@ -112,6 +114,16 @@ struct Ptrs {
   // sorted partners j > i in ss_partner[ss_off[i] .. ss_off[i+1])
   const int *ss_off, *ss_partner;
 };
+
+// Debye-Hueckel cutoff a list rebuild of replica r takes.  With cutoff_type 2
+// it follows the replica's current Debye length (src/neighbor_list.F90:80-82);
+// the forces and energies between rebuilds keep the value of the last
+// rebuild (ele_cut), also after an exchange has changed lambdaD
+// (set_ele leaves ele_cutoff alone, src/job_md.F90:556).
+__device__ __forceinline__ double rebuild_ele_cut(const Params &P, const Ptrs &D, int r) {
+#pragma clang fp contract(off)
+  return P.ele_cut_type == 2 ? P.ele_cut_inp * D.lambdaD[r] : D.ele_cut[r];
+}
 
 // ---------------------------------------------------------------------
 // minimum image (src/pbc.F90:140-165): one conditional shift per axis.

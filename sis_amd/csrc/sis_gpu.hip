@@ -460,7 +460,11 @@ void setup_cells(sis_gpu_ctx *c) {
 }
 
 // Per-replica conditions: set_ele and set_md_coef (src/job_md.F90:700-767).
-void upload_conditions(sis_gpu_ctx *c) {
+// set_cut: also the electrostatic cutoff, as init_ele sets it at set-up
+// (src/init_ele.F90:73-101).  After an exchange set_ele changes lambdaD and
+// the coefficient only; the cutoff follows at the replica's next rebuild
+// (src/job_md.F90:556, src/neighbor_list.F90:80-82).
+void upload_conditions(sis_gpu_ctx *c, bool set_cut = true) {
   int R = c->R, N = c->N;
   std::vector<double> coef(R), lam(R), cut(R), cr((size_t)R * N), box(3 * R);
   for (int r = 0; r < R; r++) {
@@ -480,7 +484,8 @@ void upload_conditions(sis_gpu_ctx *c) {
   HIPCHK(hipMemcpyAsync(c->tK_d.p, c->tK.data(), R * sizeof(double), hipMemcpyHostToDevice, c->stream));
   HIPCHK(hipMemcpyAsync(c->ele_coef_d.p, coef.data(), R * sizeof(double), hipMemcpyHostToDevice, c->stream));
   HIPCHK(hipMemcpyAsync(c->lambdaD_d.p, lam.data(), R * sizeof(double), hipMemcpyHostToDevice, c->stream));
-  HIPCHK(hipMemcpyAsync(c->ele_cut_d.p, cut.data(), R * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  if (set_cut)
+    HIPCHK(hipMemcpyAsync(c->ele_cut_d.p, cut.data(), R * sizeof(double), hipMemcpyHostToDevice, c->stream));
   HIPCHK(hipMemcpyAsync(c->md_cr_d.p, cr.data(), cr.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
   HIPCHK(hipMemcpyAsync(c->box_d.p, box.data(), box.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
@@ -860,8 +865,9 @@ void do_exchange(sis_gpu_ctx *c, double *replica_energies, int32_t *accepted, do
   }
   if (c->ext_on && c->dcf_remd) dcf_upload(c);   // src/job_md.F90:559-562
   // an exchange permutes the conditions, so the largest electrostatic
-  // cutoff and the cell grid stay as they are
-  upload_conditions(c);
+  // cutoff and the cell grid stay as they are; each replica keeps its
+  // cutoff until its next rebuild
+  upload_conditions(c, false);
   (void)nacc;
 }
 
@@ -1061,6 +1067,7 @@ int sis_gpu_create(const sis_topo *topo, const sis_params *prm, const sis_option
     P.bp_cutoff_energy = prm->bp_cutoff_energy;
     P.nl_margin = prm->nl_margin;
     P.angl_reb = prm->angl_reb; P.dih_type = prm->dih_type; P.ele_on = prm->ele_on;
+    P.ele_cut_type = prm->ele_cutoff_type; P.ele_cut_inp = prm->ele_cutoff;
     P.ele_excl_bond = prm->ele_excl_bond; P.ele_excl_angl = prm->ele_excl_angl;
     P.bp_model = prm->bp_model; P.bp_min_loop = prm->bp_min_loop;
     P.max_bp_per_nt = prm->max_bp_per_nt; P.nstep_bp_mc = prm->nstep_bp_mc; P.pbc = prm->pbc;
@@ -1230,6 +1237,11 @@ int sis_gpu_get_state(sis_gpu_ctx *c, double *xyz, double *velos, double *accels
 int sis_gpu_set_rep_conditions(sis_gpu_ctx *c, const double *tempK, const double *ionic) {
   API_BEGIN
   if (!c) return fail(SIS_ERR_ARG, "null context");
+  // every value is checked before any state changes
+  for (int r = 0; r < c->R; r++) {
+    if (tempK && !(tempK[r] > 0.0)) return fail(SIS_ERR_ARG, "temperature must be positive");
+    if (ionic && !(ionic[r] > 0.0)) return fail(SIS_ERR_ARG, "ionic strength must be positive");
+  }
   if (tempK) {
     // new temperatures reset the labels to the identity: the values the
     // labels carry are re-keyed to the replicas that hold them now
@@ -1243,14 +1255,12 @@ int sis_gpu_set_rep_conditions(sis_gpu_ctx *c, const double *tempK, const double
   }
   for (int r = 0; r < c->R; r++) {
     if (tempK) {
-      if (!(tempK[r] > 0.0)) return fail(SIS_ERR_ARG, "temperature must be positive");
       c->tK[r] = tempK[r];
       c->lab2val[r + 1] = tempK[r];
       c->rep2lab[r + 1] = r + 1;
       c->lab2rep[r + 1] = r + 1;
     }
     if (ionic) {
-      if (!(ionic[r] > 0.0)) return fail(SIS_ERR_ARG, "ionic strength must be positive");
       c->ionic[r] = ionic[r];
       c->lab2ion[c->rep2lab[r + 1]] = ionic[r];   // the label carries it through exchanges
     }

@@ -8,9 +8,13 @@ namespace sis {
 
 // ---------------------------------------------------------------------
 // pbc_wrap (src/pbc.F90:196-226) for replicas with mask[r] != 0.
+// pbc_wrap of the replicas being rebuilt (src/pbc.F90:196-226), and their
+// electrostatic cutoff for the new lists (src/neighbor_list.F90:80-82)
 __global__ void k_wrap(Params P, Ptrs D, const int *mask) {
   int i = blockIdx.x * blockDim.x + threadIdx.x, r = blockIdx.y;
-  if (i >= P.N || !P.pbc || (mask && !mask[r])) return;
+  if (mask && !mask[r]) return;
+  if (i == 0) D.ele_cut[r] = rebuild_ele_cut(P, D, r);
+  if (i >= P.N || !P.pbc) return;
   const double *box = D.box + 3 * r;
   double4 &p = D.pos[(size_t)r * P.N + i];
   double v[3] = {p.x, p.y, p.z};

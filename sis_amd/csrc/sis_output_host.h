@@ -1027,9 +1027,15 @@ RstIn rst_parse(const std::string &path, int32_t nmp) {
     if (size < 0) throw ApiError{SIS_ERR_ARG, "bad block size in " + path};
     need((size_t)size);
     const size_t end = p + (size_t)size;
-    if (id == RST_STEP && size == 12) {
+    // a block this reader knows must have its size; others (the MT state
+    // blocks) are skipped
+    auto bad = [&](const char *blk) { throw ApiError{SIS_ERR_ARG, std::string("bad ") + blk + " block in " + path}; };
+    bool parsed = true;
+    if (id == RST_STEP) {
+      if (size != 12) bad("STEP");
       std::memcpy(&o.istep, b.data() + p + 4, 8);
-    } else if ((id == RST_XYZ || id == RST_VELO || id == RST_ACCEL) && size >= 8) {
+    } else if (id == RST_XYZ || id == RST_VELO || id == RST_ACCEL) {
+      if (size < 8) bad("XYZ/VELO/ACCEL");
       int32_t g, n;
       std::memcpy(&g, b.data() + p, 4);
       std::memcpy(&n, b.data() + p + 4, 4);
@@ -1039,7 +1045,8 @@ RstIn rst_parse(const std::string &path, int32_t nmp) {
       std::vector<double> &dst = id == RST_XYZ ? o.xyz : id == RST_VELO ? o.vel : o.acc;
       dst.resize((size_t)3 * n);
       std::memcpy(dst.data(), b.data() + p + 8, (size_t)24 * n);
-    } else if (id == RST_REPLICA && size >= 4) {
+    } else if (id == RST_REPLICA) {
+      if (size < 4) bad("REPLICA");
       std::memcpy(&o.nrep, b.data() + p, 4);
       if (o.nrep < 1 || (size_t)size != 4 + (size_t)8 * o.nrep)
         throw ApiError{SIS_ERR_ARG, "bad REPLICA block in " + path};
@@ -1051,14 +1058,19 @@ RstIn rst_parse(const std::string &path, int32_t nmp) {
         if (i < 1 || i > o.nrep) throw ApiError{SIS_ERR_ARG, "bad REPLICA block in " + path};
         o.rep2lab[i - 1] = m;
       }
-    } else if (id == RST_ANNEAL && size == 4) {
+    } else if (id == RST_ANNEAL) {
+      if (size != 4) bad("ANNEAL");
       std::memcpy(&o.ianneal, b.data() + p, 4);
-    } else if (id == RST_PBC && size == 24) {
+    } else if (id == RST_PBC) {
+      if (size != 24) bad("PBC");
       std::memcpy(o.box, b.data() + p, 24);
-    } else if (id == RST_SEED && size == 8) {
+    } else if (id == RST_SEED) {
+      if (size != 8) bad("SEED");
       std::memcpy(&o.seed, b.data() + p, 8);
+    } else {
+      parsed = false;
     }
-    if (id >= 1 && id <= 31) o.found |= 1 << (id - 1);
+    if (parsed) o.found |= 1 << (id - 1);
     p = end;
   }
   return o;
@@ -1179,6 +1191,8 @@ int sis_gpu_rst_read(sis_gpu_ctx *c, const char *const *paths) {
   int64_t istep = -1;
   double new_box[3] = {c->prm.box[0], c->prm.box[1], c->prm.box[2]};
   std::vector<int32_t> lab;
+  int ianneal = -1;
+  // every file is read and checked before the context changes
   for (int r = 0; r < R; r++) {
     if (!paths[r]) return fail(SIS_ERR_ARG, "null path");
     RstIn o = rst_parse(paths[r], N);
@@ -1190,7 +1204,7 @@ int sis_gpu_rst_read(sis_gpu_ctx *c, const char *const *paths) {
     std::copy(o.vel.begin(), o.vel.end(), v.begin() + n3 * (o.grep - 1));
     std::copy(o.acc.begin(), o.acc.end(), a.begin() + n3 * (o.grep - 1));
     if (o.found & (1 << (RST_STEP - 1))) istep = o.istep;
-    if (o.found & (1 << (RST_ANNEAL - 1))) c->rst_ianneal = o.ianneal;
+    if (o.found & (1 << (RST_ANNEAL - 1))) ianneal = o.ianneal;
     if (!o.rep2lab.empty()) {
       if (o.nrep != R) return fail(SIS_ERR_ARG, "nrep is not consistent in the restart file");
       lab = o.rep2lab;
@@ -1199,6 +1213,16 @@ int sis_gpu_rst_read(sis_gpu_ctx *c, const char *const *paths) {
   }
   if (istep < 0) return fail(SIS_ERR_ARG, "STEP block missing");
   if (R > 1 && lab.empty()) return fail(SIS_ERR_ARG, "REPLICA block missing");
+  if (R > 1) {   // labels: a permutation of 1..nrep
+    std::vector<int> seen(R + 1, 0);
+    for (int r = 0; r < R; r++) {
+      if (lab[r] < 1 || lab[r] > R || seen[lab[r]]) return fail(SIS_ERR_ARG, "bad labels in the REPLICA block");
+      seen[lab[r]] = 1;
+    }
+  }
+  if (!(new_box[0] > 0.0 && new_box[1] > 0.0 && new_box[2] > 0.0) && c->prm.pbc)
+    return fail(SIS_ERR_ARG, "bad box in the PBC block");
+  if (ianneal >= 0) c->rst_ianneal = ianneal;
   // the PBC block's box replaces the input box (init_pbc, src/pbc.F90:35-45)
   if (new_box[0] != c->prm.box[0] || new_box[1] != c->prm.box[1] || new_box[2] != c->prm.box[2]) {
     std::copy(new_box, new_box + 3, c->prm.box);

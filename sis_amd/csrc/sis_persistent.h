@@ -416,8 +416,11 @@ __device__ __forceinline__ void batched_rebuild(const Params &P, const Ptrs &D, 
       p.x = v[0]; p.y = v[1]; p.z = v[2];
     }
   }
+  // the electrostatic cutoff of the new lists (src/neighbor_list.F90:80-82)
+  const double ecut = rebuild_ele_cut(P, D, r);
   __syncthreads();
-  const double ec = D.ele_cut[r] + P.nl_margin;
+  if (tid == 0) D.ele_cut[r] = ecut;
+  const double ec = ecut + P.nl_margin;
   const double ele_cut2 = P.ele_on ? ec * ec : 0.0;
   for (int a0 = 0; a0 < N; a0 += B) {
     const int a = a0 + tid;

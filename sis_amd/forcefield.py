@@ -4,8 +4,15 @@ Values are data taken from the reference's force-field files:
   * bond / angle_ReB / dihedral_exp / basepair blocks: check/sis_dih_ReB_NN.ff:3-85
     (bond k and r0 overridden to 15.0 and 5.9 for the benchmark systems)
   * nearest-neighbour dH / dS (Zuber et al. 2022): sis.ff:125-169
-  * electrostatics defaults: check/input_T2S2HP.toml:34-39 (cutoff type 1, 50 A)
+  * electrostatics of the benchmark systems: ionic strength and length per
+    charge of check/input_T2S2HP.toml:33-38; cutoff type 1 at 50 A, the
+    cutoff BASELINE's condensate configs state (that input file itself uses
+    cutoff type 2, 10 Debye lengths; read_ff below plus a [Electrostatic]
+    override reproduces it)
 Units: Angstrom, kcal/mol, K, reduced time (tau = 48.888 fs).
+
+read_ff() turns a reference force-field file (TOML, src/read_force_field.F90)
+into the same flat parameter dict.
 """
 from __future__ import annotations
 
@@ -79,5 +86,70 @@ def default_params(**over) -> dict:
         pbc=0, box=[0.0, 0.0, 0.0],
         integrator=1,            # 1 GJF-2GJ Langevin, 2 Ermak-McCammon Brownian
     )
+    p.update(over)
+    return p
+
+
+def _lower_keys(t):
+    if isinstance(t, dict):
+        return {str(k).lower(): _lower_keys(v) for k, v in t.items()}
+    return t
+
+
+def read_ff(path, **over) -> dict:
+    """Parameters of a reference force-field file, as src/read_force_field.F90
+    reads it: [potential.bond] k/r0; [potential.angle] (harmonic) or
+    [potential.angle_ReB] k/a0; [potential.dihedral] (cosine: k, phi0) or
+    [potential.dihedral_exp] (k, w, phi0), neither meaning no dihedral term;
+    [potential.basepair] cutoff_energy, coef_dG, dH0, dS0, NNend, dHend0,
+    dSend0 and the GC/AU/GU blocks; [potential.wca] sigma/epsilon; [NN.dH] and
+    [NN.dS] by stack name; [NNend.dH]/[NNend.dS] when NNend is true.  Keys are
+    matched case-insensitively, as the reference's TOML reader does.  Block
+    fields a file leaves out are 0.  Everything the file does not set (run
+    conditions, electrostatics, base-pair model) comes from default_params()
+    and `over`."""
+    import tomli
+    from .abi import NNT_ORDER, NNEND_ORDER, BP_FIELDS
+    with open(path, "rb") as fh:
+        t = _lower_keys(tomli.load(fh))
+    pot = t["potential"]
+    p = default_params()
+    b = pot["bond"]
+    p.update(bond_k=float(b["k"]), bond_r0=float(b["r0"]))
+    if "angle" in pot and "angle_reb" in pot:   # src/read_force_field.F90:85-105
+        raise ValueError("[potential.angle] and [potential.angle_ReB] cannot be specified together")
+    if "angle" in pot:
+        p.update(angl_reb=0, angl_k=float(pot["angle"]["k"]), angl_t0=float(pot["angle"]["a0"]))
+    elif "angle_reb" in pot:
+        p.update(angl_reb=1, angl_k=float(pot["angle_reb"]["k"]), angl_t0=float(pot["angle_reb"]["a0"]))
+    if "dihedral" in pot and "dihedral_exp" in pot:   # :133-137
+        raise ValueError("[dihedral] and [dihedral_exp] cannot be specified together")
+    if "dihedral" in pot:
+        d = pot["dihedral"]
+        p.update(dih_type=1, dih_k=float(d["k"]), dih_w=0.0, dih_p0=float(d["phi0"]))
+    elif "dihedral_exp" in pot:
+        d = pot["dihedral_exp"]
+        p.update(dih_type=2, dih_k=float(d["k"]), dih_w=float(d["w"]), dih_p0=float(d["phi0"]))
+    else:
+        p.update(dih_type=0)
+    bp = pot["basepair"]
+    p.update(bp_cutoff_energy=float(bp.get("cutoff_energy", 0.0)), bp_coef_dG=float(bp.get("coef_dg", 1.0)),
+             bp_dH0=float(bp.get("dh0", 0.0)), bp_dS0=float(bp.get("ds0", 0.0)),
+             nnend=int(bool(bp.get("nnend", False))),
+             bp_dHend0=float(bp.get("dhend0", 0.0)), bp_dSend0=float(bp.get("dsend0", 0.0)))
+    p["bp"] = {}
+    for ty in ("GC", "AU", "GU"):
+        blk = bp.get(ty.lower(), {})
+        p["bp"][ty] = {f: float(blk.get(f.lower(), 0.0)) for f in BP_FIELDS if f != "cutoff_ddist"}
+    w = pot["wca"]
+    p.update(wca_sigma=float(w["sigma"]), wca_eps=float(w["epsilon"]))
+    nn = t.get("nn", {})
+    if "dh" in nn and "ds" in nn:
+        p["nn_dH"] = [float(nn["dh"][k.lower()]) for k in NNT_ORDER]
+        p["nn_dS"] = [float(nn["ds"][k.lower()]) for k in NNT_ORDER]
+    if p["nnend"]:
+        ne = t["nnend"]
+        p["nnend_dH"] = [float(ne["dh"][k.lower()]) for k in NNEND_ORDER]
+        p["nnend_dS"] = [float(ne["ds"][k.lower()]) for k in NNEND_ORDER]
     p.update(over)
     return p

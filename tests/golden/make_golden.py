@@ -233,6 +233,74 @@ def remd_force():
     return sysd
 
 
+REF_DIR = Path("/root/reference")   # the reference's own example inputs (read here only, when writing fixtures)
+
+
+def _t2s2hp(coords="folded"):
+    """The reference's T2S2HP hairpin (27 nt, check/T2S2HP.fasta) at the
+    coordinates of check/T2S2HP_folded.pdb or check/T2S2HP_random.xyz."""
+    seq = "".join(l.strip() for l in open(REF_DIR / "check/T2S2HP.fasta") if not l.startswith(">"))
+    if coords == "folded":
+        x = configs.read_pdb_coords(REF_DIR / "check/T2S2HP_folded.pdb")
+    else:
+        x = configs.read_xyz_coords(REF_DIR / "check/T2S2HP_random.xyz")
+    assert len(x) == len(seq)
+    return seq, x
+
+
+def t2s2hp_sisff():
+    """The reference's template force field sis.ff (harmonic angle, cosine
+    dihedral, sis.ff:8-18) on T2S2HP from its folded structure, with the
+    electrostatics of check/input_T2S2HP.toml:33-38 (cutoff type 2, 10 Debye
+    lengths, 0.15 M, 300 K, 10 A margin, viscosity 1e-5 Pa s).  sis.ff's
+    negative U0 make it a model-1/3 file (E = U0 exp(-u)); base-pair model 3
+    without the limit MC (src/force_bp.F90)."""
+    from sis_amd.forcefield import read_ff
+    seq, x = _t2s2hp("folded")
+    p = read_ff(REF_DIR / "sis.ff", tempK=300.0, ionic_strength=0.15, ele_cutoff_type=2, ele_cutoff=10.0,
+                nl_margin=10.0, viscosity_pas=1e-5, dt=0.05, bp_model=3, bp_min_loop=3, nstep_bp_mc=0)
+    sysd = configs._mk("t2s2hp_sisff", [seq], x, p)
+    return sysd
+
+
+def t2s2hp_ion8(cutoff=10.0, name="t2s2hp_ion8"):
+    """Replica exchange over the ionic strength as in
+    check/T2S2HP_REMD_C/input_REMD.toml: 8 replicas at 0.02-0.80 M, 300 K,
+    DH cutoff type 2 at 10 Debye lengths, exchange every 10 steps, base-pair
+    model 5 without the limit MC (nstep_MC = 0), from T2S2HP_folded.pdb.
+    Force field check/para.ff (NNend = true, para.ff:24; that input's
+    htv23-19.ff is not in the reference).  The margin is 1 A instead of the
+    file's 10 so that the 100 steps hold rebuilds between exchanges: each
+    exchange changes lambdaD at once and the cutoff only at the replica's
+    next rebuild (src/job_md.F90:556, src/neighbor_list.F90:80-82)."""
+    from sis_amd.forcefield import read_ff
+    seq, x = _t2s2hp("folded")
+    p = read_ff(REF_DIR / "check/para.ff", tempK=300.0, ionic_strength=0.15, ele_cutoff_type=2,
+                ele_cutoff=cutoff, nl_margin=1.0, viscosity_pas=1e-5, dt=0.05, bp_model=5, bp_min_loop=3,
+                max_bp_per_nt=1, nstep_bp_mc=0)
+    sysd = configs._mk(name, [seq], x, p, nrep=8, temps=np.full(8, 300.0))
+    sysd["remd_ion"] = np.array([0.02, 0.05, 0.10, 0.15, 0.20, 0.30, 0.50, 0.80])
+    return sysd
+
+
+def t2s2hp_temp8(cutoff=10.0, name="t2s2hp_temp8"):
+    """Replica exchange over the temperature as in
+    check/T2S2HP_REMD_T/input_REMD.toml: 8 replicas at 293.15-363.15 K,
+    0.15 M, DH cutoff type 2 at 10 Debye lengths, exchange every 10 steps,
+    from check/T2S2HP_random.xyz with check/sis_dih_ReB_NN.ff, the force
+    field of check/input_T2S2HP.toml.  The limit MC runs every step
+    (input_T2S2HP.toml's nstep_MC = 1) and the margin is 1 A, so that
+    exchanges fall between rebuilds."""
+    from sis_amd.forcefield import read_ff
+    seq, x = _t2s2hp("random")
+    p = read_ff(REF_DIR / "check/sis_dih_ReB_NN.ff", tempK=300.0, ionic_strength=0.15, ele_cutoff_type=2,
+                ele_cutoff=cutoff, nl_margin=1.0, viscosity_pas=1e-5, dt=0.05, bp_model=5, bp_min_loop=3,
+                max_bp_per_nt=1, nstep_bp_mc=1)
+    temps = np.array([293.15, 303.15, 313.15, 323.15, 333.15, 343.15, 353.15, 363.15])
+    sysd = configs._mk(name, [seq], x, p, nrep=8, temps=temps)
+    return sysd
+
+
 SYSTEMS = {
     "hairpin64": (lambda: configs.hairpin64(1), 60, 0),
     "small_pbc": (lambda: configs.small_pbc(4, 30, 100.0, seed=5), 60, 0),
@@ -249,7 +317,23 @@ SYSTEMS = {
     "remd4ion": (remd_ion, 40, 10),
     "remd4rg": (remd_rg, 40, 10),
     "remd4dcf": (remd_force, 40, 10),
+    "t2s2hp_sisff": (t2s2hp_sisff, 200, 0),
+    "t2s2hp_ion8": (t2s2hp_ion8, 100, 10),
+    "t2s2hp_temp8": (t2s2hp_temp8, 100, 10),
+    # the same ladders with the cutoff at 3 and 2 Debye lengths, where the
+    # cutoff's timing moves the trajectory by far more than the tolerances
+    "t2s2hp_ion8c3": (lambda: t2s2hp_ion8(3.0, "t2s2hp_ion8c3"), 100, 10),
+    "t2s2hp_temp8c2": (lambda: t2s2hp_temp8(2.0, "t2s2hp_temp8c2"), 100, 10),
 }
+LIST_PARAMS = ("nn_dH", "nn_dS", "nnend_dH", "nnend_dS")
+
+
+def force_mc(sysd) -> int:
+    """Whether the fixture's single force evaluation runs the limit MC: as
+    job_md would (flg_bp_MC only with nstep_MC > 0, src/job_md.F90:270-273,
+    364-367); models 1/3 run without it."""
+    p = sysd["params"]
+    return 1 if p["bp_model"] == 5 and p["nstep_bp_mc"] > 0 else 0
 
 
 def make(name, builder, nstep, nex, seed=4321):
@@ -272,7 +356,7 @@ def make(name, builder, nstep, nex, seed=4321):
     if s.get("resize") is not None:
         pre.append(("resize",) + tuple(s["resize"]))
     ops = pre + ops + [("nl", r) for r in range(1, nrep + 1)]
-    fmc = 1 if s["params"]["bp_model"] == 5 else 0   # models 1/3 run without the limit MC
+    fmc = force_mc(s)
     ops += [("lists", 1, "L"), ("force", 1, fmc, "F"), ("bp_state", 1, "B"),
             ("energy", 1, float(s["temps"][0]), 0, 1, "E"), ("ele", 1, "ele"), ("bp_tables", "T")]
     ref0 = refdrv.run(s, ops, seed=seed, nstep_exchange=nex)
@@ -352,6 +436,8 @@ def make(name, builder, nstep, nex, seed=4321):
     out["param_names"] = np.array(sorted(scal))
     out["param_values"] = np.array([float(scal[k]) for k in sorted(scal)])
     out["box"] = np.array(p["box"], float)
+    for k in LIST_PARAMS:
+        out["param_" + k] = np.asarray(p[k], np.float64)
     # per-type base-pair parameters (GC, AU, GU) in BP_FIELDS order
     from sis_amd.abi import BP_FIELDS
     out["bp_params"] = np.array([[float(p["bp"][t].get(f, 0.0)) for f in BP_FIELDS] for t in ("GC", "AU", "GU")])
@@ -395,6 +481,9 @@ def load(name):
                 seq=d["seq"], xyz=d["xyz"], params=p, nrep=int(d["nrep"]), temps=d["temps"],
                 nstep=int(d["nstep"]), seqs=seqs)
     p["box"] = [float(b) for b in d["box"]]
+    for k in LIST_PARAMS:
+        if "param_" + k in d:
+            p[k] = [float(v) for v in d["param_" + k]]
     if "bp_params" in d:
         from sis_amd.abi import BP_FIELDS
         p["bp"] = {t: {f: float(v) for f, v in zip(BP_FIELDS, row)} for t, row in zip(("GC", "AU", "GU"), d["bp_params"])}

@@ -171,6 +171,50 @@ def test_restart_files_carry_anneal_and_box(tmp_path):
     assert np.abs(xa - xc).max() < 1e-6
 
 
+def test_restart_file_with_bad_block_is_refused(tmp_path):
+    """A restart file whose STEP block has the wrong size, or whose REPLICA
+    block labels are not a permutation, is refused, and the context keeps
+    its state, step and labels (nothing is applied before every file has
+    been read and checked)."""
+    from sis_amd.engine import SisGpuError
+    import struct
+    s = configs.remd_c3(nrep=2, n=30, seed=3)
+    print("[case] restart file with a bad block", flush=True)
+    e = Engine(s, mode="phases", seed=3, precision=0)
+    e.init_velocities()
+    e.neighbor_list(0)
+    e.step(20)
+    paths = [tmp_path / "r_0001.rst", tmp_path / "r_0002.rst"]
+    e.rst_open(paths)
+    e.rst_write()
+    e.rst_close()
+    x0, v0, _ = e.get_state()
+    lab0 = e.labels().copy()
+    good = [p.read_bytes() for p in paths]
+
+    def blocks(b):
+        out, p = [], 0
+        while p < len(b):
+            i, n = struct.unpack_from("<ii", b, p)
+            out.append((i, b[p + 8:p + 8 + n]))
+            p += 8 + n
+        return out
+
+    def join(bl):
+        return b"".join(struct.pack("<ii", i, len(v)) + v for i, v in bl)
+
+    bad_step = join([(i, v + b"\0\0\0\0") if i == 1 else (i, v) for i, v in blocks(good[0])])
+    rep = [(i, v) for i, v in blocks(good[0])]
+    bad_lab = join([(i, struct.pack("<iiiii", 2, 1, 1, 2, 1)) if i == 6 else (i, v) for i, v in rep])
+    for bad in (bad_step, bad_lab):
+        paths[0].write_bytes(bad)
+        with pytest.raises(SisGpuError):
+            e.rst_read(paths)
+        x1, v1, _ = e.get_state()
+        assert np.array_equal(x0, x1) and np.array_equal(v0, v1)
+        assert np.array_equal(e.labels(), lab0) and e.info()["istep"] == 20
+
+
 @pytest.mark.parametrize("case", ["batched", "pbc_phases"])
 def test_bp_records_during_steps(tmp_path, case):
     """Base-pair records (.bpall: every formed pair; the E <= -kT filter of

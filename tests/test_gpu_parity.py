@@ -16,11 +16,12 @@ import pytest
 from sis_amd import configs
 from sis_amd.engine import Engine
 from oracle import corc
-from tests.golden.make_golden import load
+from tests.golden.make_golden import force_mc, load
 
 pytestmark = pytest.mark.gpu
 
-FIXTURES = ["hairpin64", "small_pbc", "mixed", "remd4", "remd4mc3", "remd4bd", "remd2ext", "hairpin_ss", "hairpin_anneal", "pbc_resize", "cag47_m1", "remd2rgs", "remd4ion", "remd4rg", "remd4dcf"]
+FIXTURES = ["hairpin64", "small_pbc", "mixed", "remd4", "remd4mc3", "remd4bd", "remd2ext", "hairpin_ss", "hairpin_anneal", "pbc_resize", "cag47_m1", "remd2rgs", "remd4ion", "remd4rg", "remd4dcf",
+            "t2s2hp_sisff", "t2s2hp_ion8", "t2s2hp_temp8", "t2s2hp_ion8c3", "t2s2hp_temp8c2"]
 
 
 def _sorted_rows(a):
@@ -76,7 +77,7 @@ def test_force_energy_bp_state(name):
     print(f"[case] force {name}", flush=True)
     eng = _engine(sysd, d)
     eng.neighbor_list(0)
-    f = eng.force(1, 1)
+    f = eng.force(1, force_mc(sysd))
     fr = d["force"]
     assert np.abs(f - fr).max() / np.abs(fr).max() < 1e-10
     b = eng.bp(1)
@@ -684,3 +685,38 @@ def test_fp32_nearly_collinear_base_pair(delta):
     assert np.all(np.isfinite(f[0])) and np.all(np.isfinite(f[1]))
     rel = np.linalg.norm(f[1] - f[0], axis=1) / np.maximum(np.linalg.norm(f[0], axis=1), 1e-3)
     assert rel.max() < 1e-5
+
+
+def test_new_temperatures_keep_each_replicas_ionic_strength():
+    """After exchanges on the ionic-strength ladder (remd4ion) the labels are
+    permuted; new temperatures reset the labels to the identity, and each
+    replica keeps the ionic strength its label carried: its energies equal
+    those of a fresh context given those temperatures and ionic strengths
+    directly.  A non-positive value is refused before anything changes."""
+    sysd, d = load("remd4ion")
+    print("[case] set_rep_conditions after ion exchanges", flush=True)
+    eng = _engine(sysd, d, exchange_every=int(d["nstep_exchange"]))
+    eng.init_velocities()
+    eng.neighbor_list(0)
+    for r in range(1, sysd["nrep"] + 1):
+        eng.energy_sumup(r, -1.0, 1, 0)
+    eng.step(int(d["nstep"]))
+    lab = eng.labels()
+    assert not np.array_equal(lab, np.arange(1, sysd["nrep"] + 1))   # exchanges happened
+    ion = np.asarray(sysd["remd_ion"])[lab - 1]
+    x, v, a = eng.get_state()
+    with pytest.raises(Exception):
+        eng.set_rep_conditions([300.0, -1.0, 300.0, 300.0])
+    assert np.array_equal(eng.labels(), lab)
+    newT = np.array([301.0, 302.0, 303.0, 304.0])
+    eng.set_rep_conditions(newT)
+    assert np.array_equal(eng.labels(), np.arange(1, sysd["nrep"] + 1))
+    eng.neighbor_list(0)
+    ref = Engine(dict(sysd, temps=newT), parity=True)
+    ref.set_rep_conditions(newT, ion)
+    ref.set_state(x, v, a)
+    ref.neighbor_list(0)
+    for r in range(1, sysd["nrep"] + 1):
+        e1 = eng.energy_sumup(r, -1.0, 0, 0)
+        e2 = ref.energy_sumup(r, -1.0, 0, 0)
+        assert np.allclose(e1, e2, rtol=1e-12, atol=1e-12), r

@@ -8,9 +8,10 @@ import pytest
 
 from sis_amd import configs
 from oracle import corc, refdrv
-from tests.golden.make_golden import load
+from tests.golden.make_golden import force_mc, load
 
-FIXTURES = ["hairpin64", "small_pbc", "mixed", "remd4", "remd4mc3", "remd4bd", "remd2ext", "hairpin_ss", "hairpin_anneal", "pbc_resize", "cag47_m1", "remd2rgs", "remd4ion", "remd4rg", "remd4dcf"]
+FIXTURES = ["hairpin64", "small_pbc", "mixed", "remd4", "remd4mc3", "remd4bd", "remd2ext", "hairpin_ss", "hairpin_anneal", "pbc_resize", "cag47_m1", "remd2rgs", "remd4ion", "remd4rg", "remd4dcf",
+            "t2s2hp_sisff", "t2s2hp_ion8", "t2s2hp_temp8", "t2s2hp_ion8c3", "t2s2hp_temp8c2"]
 
 
 def _oracle_from_fixture(name):
@@ -56,7 +57,7 @@ def test_force_and_bp_state(name):
     sysd, d, o = _oracle_from_fixture(name)
     for r in range(1, sysd["nrep"] + 1):
         o.neighbor_list(r)
-    f = o.force(1, 1)
+    f = o.force(1, force_mc(sysd))
     fr = d["force"]
     rel = np.abs(f - fr).max() / np.abs(fr).max()
     assert rel < 1e-12
@@ -91,6 +92,63 @@ def test_md_trajectory(name):
     assert np.abs(a - d["md_accels"]).max() < 1e-14
     if sysd["nrep"] > 1:
         assert np.array_equal(o.labels(), d["md_rep2lab"][: sysd["nrep"]])
+
+
+@pytest.mark.parametrize("name", ["t2s2hp_ion8c3", "t2s2hp_temp8c2"])
+def test_cutoff_follows_rebuilds_not_exchanges(name):
+    """DH cutoff type 2 under an ionic-strength or temperature ladder: the
+    reference's set_ele after an exchange changes lambdaD and the
+    coefficient, while ele_cutoff moves only at the replica's next
+    neighbor_list (src/job_md.F90:556, src/neighbor_list.F90:80-82).  The
+    fixture's run has exchanges between rebuilds, so an oracle with the
+    cutoff moved at set_ele (planted) leaves the reference trajectory."""
+    sysd, d, o = _oracle_from_fixture(name)
+    o.init_velocities()
+    nreb, _ = o.md(int(d["nstep"]), int(d["nstep_exchange"]))
+    assert nreb >= 1
+    assert np.abs(o.state()[0] - d["md_xyz"]).max() < 1e-10
+    sysd, d, o = _oracle_from_fixture(name)
+    o.plant(1)
+    o.init_velocities()
+    o.md(int(d["nstep"]), int(d["nstep_exchange"]))
+    assert np.abs(o.state()[0] - d["md_xyz"]).max() > 1e-5
+
+
+def test_fixtures_use_reference_default_physics():
+    """The T2S2HP fixtures carry the reference's own example physics: the
+    cosine dihedral and harmonic angle of sis.ff, DH cutoff type 2 at 10
+    Debye lengths, NNend corrections (check/para.ff), base-pair model 3, and
+    both REMD ladders of check/T2S2HP_REMD_C/T."""
+    p = {n: load(n)[0]["params"] for n in ("t2s2hp_sisff", "t2s2hp_ion8", "t2s2hp_temp8")}
+    assert p["t2s2hp_sisff"]["dih_type"] == 1 and p["t2s2hp_sisff"]["angl_reb"] == 0
+    assert p["t2s2hp_sisff"]["bp_model"] == 3
+    assert all(q["ele_cutoff_type"] == 2 and q["ele_cutoff"] == 10.0 for q in p.values())
+    assert p["t2s2hp_ion8"]["nnend"] == 1 and any(v != 0.0 for v in p["t2s2hp_ion8"]["nnend_dH"])
+    d = load("t2s2hp_ion8")[1]
+    # the NNend corrections change the triplet table the reference built
+    sysd = load("t2s2hp_ion8")[0]
+    q = dict(sysd["params"], nnend=0)
+    o = corc.Oracle(dict(sysd, params=q))
+    dh, _, _ = o.tables()
+    assert not np.array_equal(dh, d["bp3_dH"])
+
+
+@pytest.mark.skipif(not __import__("os").path.exists("/root/reference/check/sis_dih_ReB_NN.ff"),
+                    reason="reference force-field files not present")
+def test_read_ff_matches_benchmark_parameters():
+    """read_ff on check/sis_dih_ReB_NN.ff gives the benchmark force field
+    (default_params) except the bond, which the benchmark systems set to
+    k 15, r0 5.9; on sis.ff the harmonic angle and cosine dihedral."""
+    from sis_amd.forcefield import default_params, read_ff
+    d = default_params()
+    p = read_ff("/root/reference/check/sis_dih_ReB_NN.ff")
+    diff = {k for k in d if d[k] != p[k]}
+    assert diff == {"bond_k", "bond_r0"}
+    q = read_ff("/root/reference/sis.ff")
+    assert (q["angl_reb"], q["angl_k"], q["angl_t0"]) == (0, 10.0, 2.643)
+    assert (q["dih_type"], q["dih_k"], q["dih_p0"]) == (1, 5.0, 2.861)
+    n = read_ff("/root/reference/check/para.ff")
+    assert n["nnend"] == 1 and n["nnend_dH"][0] == 4.36
 
 
 def test_h4_index_rule_exercised():
