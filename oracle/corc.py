@@ -49,6 +49,8 @@ def lib():
         L.orc_set_anneal.argtypes = [vp, i32, P(i64), P(d)]
         L.orc_set_ionic.argtypes = [vp, P(d)]
         L.orc_plant.argtypes = [vp, i32, i32]
+        L.orc_set_counter_rng.argtypes = [vp, C.c_uint64, i32]
+        L.orc_philox.argtypes = [P(C.c_uint32), P(C.c_uint32), P(C.c_uint32)]
         L.orc_set_rep_rg.argtypes = [vp, P(d)]
         L.orc_set_rep_dcf.argtypes = [vp, P(d), P(d)]
         L.orc_set_rg_schedule.argtypes = [vp, i32, P(i64), P(d), P(d)]
@@ -221,6 +223,12 @@ class Oracle:
         once), for tests that show a fixture catches it."""
         lib().orc_plant(self.h, which, int(on))
 
+    def set_counter_rng(self, seed: int, on: bool = True):
+        """Draw the noise, initial velocities and exchange uniforms from the
+        engine's performance-form counter RNG (Philox4x32-10 keyed by seed,
+        replica, step and bead) instead of the MT tapes."""
+        lib().orc_set_counter_rng(self.h, int(seed), int(on))
+
     def set_ionic(self, ionic):
         """Per-label ionic strengths (REMD over REPT%ION)."""
         self._ion = np.ascontiguousarray(ionic, np.float64)
@@ -291,3 +299,14 @@ class Oracle:
         lab = np.zeros(self.nrep, np.int32)
         lib().orc_labels(self.h, _i(lab))
         return lab
+
+
+def philox4x32_10(ctr, key):
+    """Philox4x32-10 block (the oracle's restatement) of a 4-word counter
+    and a 2-word key."""
+    c = np.ascontiguousarray(ctr, np.uint32)
+    k = np.ascontiguousarray(key, np.uint32)
+    out = np.zeros(4, np.uint32)
+    P = C.POINTER(C.c_uint32)
+    lib().orc_philox(c.ctypes.data_as(P), k.ctypes.data_as(P), out.ctypes.data_as(P))
+    return out

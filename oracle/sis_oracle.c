@@ -78,6 +78,8 @@ struct orc_sys {
     int flg_bp_energy;
     /* planted mistakes, for tests that show a fixture catches them (orc_plant) */
     int plant_cut_at_set_ele;
+    /* counter-based random numbers in place of the MT tapes (orc_set_counter_rng) */
+    int ctr_on; uint64_t ctr_seed;
     /* external terms (copies of the caller's arrays) */
     int x_on;
     sis_external x;
@@ -298,6 +300,71 @@ static void init_ele(orc_sys *s, int irep, double tempk, double ionstr) {
     if (s->p.ele_cutoff_type == 1) r->ele_cutoff = s->p.ele_cutoff;
     else r->ele_cutoff = s->p.ele_cutoff * r->lambdaD;
 }
+
+/* ---------------- counter-based random numbers ----------------
+ * The engine's performance form draws its normals and exchange uniforms
+ * from Philox4x32-10 (J. K. Salmon, M. A. Moraes, R. O. Dror, D. E. Shaw,
+ * "Parallel random numbers: as easy as 1, 2, 3", SC'11): ten rounds of the
+ * 4x32 S-P network with multipliers 0xD2511F53 / 0xCD9E8D57 and Weyl key
+ * increments 0x9E3779B9 / 0xBB67AE85.  Restated here from that paper so the
+ * oracle can run the same trajectories; it is not part of the reference. */
+void orc_philox(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]) {
+    uint32_t c0 = ctr[0], c1 = ctr[1], c2 = ctr[2], c3 = ctr[3], k0 = key[0], k1 = key[1];
+    for (int round = 0; round < 10; round++) {
+        uint64_t a = (uint64_t)0xD2511F53u * c0, b = (uint64_t)0xCD9E8D57u * c2;
+        uint32_t n0 = (uint32_t)(b >> 32) ^ c1 ^ k0, n1 = (uint32_t)b;
+        uint32_t n2 = (uint32_t)(a >> 32) ^ c3 ^ k1, n3 = (uint32_t)a;
+        c0 = n0; c1 = n1; c2 = n2; c3 = n3;
+        k0 += 0x9E3779B9u; k1 += 0xBB67AE85u;
+    }
+    out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
+}
+/* 53-bit uniform in (0,1) from two words: bits 31..0 of a, then the top
+ * 21 bits of b, centred in their interval */
+static double ctr_u01(uint32_t a, uint32_t b) {
+    uint64_t m = (((uint64_t)a << 21) ^ (uint64_t)(b >> 11)) & ((1ull << 53) - 1);
+    return ((double)m + 0.5) * (1.0 / 9007199254740992.0);
+}
+/* three normals for bead `a` from two Philox blocks (counter words c0,c1
+ * and tags t0,t1 in the last word): the Box-Muller transform of four
+ * uniforms, the pair's sine left over */
+static void ctr_normals3(uint32_t c0, uint32_t c1, uint32_t a, uint32_t t0, uint32_t t1,
+                         const uint32_t key[2], int init, double n[3]) {
+    uint32_t o[4], o2[4];
+    uint32_t q[4] = {c0, c1, a, t0}, q2[4] = {init ? c0 - 1u : c0, c1, a, t1};
+    orc_philox(q, key, o);
+    orc_philox(q2, key, o2);
+    double u1 = ctr_u01(o[0], o[1]), u2 = ctr_u01(o[2], o[3]), u3 = ctr_u01(o2[0], o2[1]), u4 = ctr_u01(o2[2], o2[3]);
+    double rr = sqrt(-2.0 * log(u1)), r2 = sqrt(-2.0 * log(u3));
+    n[0] = rr * cos(6.283185307179586 * u2);
+    n[1] = rr * sin(6.283185307179586 * u2);
+    n[2] = r2 * cos(6.283185307179586 * u4);
+}
+static void ctr_key(orc_sys *s, int rk, uint32_t key[2]) {
+    key[0] = (uint32_t)s->ctr_seed;
+    key[1] = (uint32_t)(s->ctr_seed >> 32) ^ (0x9e3779b9u * (uint32_t)(rk + 1));
+}
+/* normals of replica r, bead a at step istep (the engine's GJF noise) */
+static void ctr_step_normals(orc_sys *s, int r, int a, double n[3]) {
+    uint32_t key[2];
+    ctr_key(s, r, key);
+    ctr_normals3((uint32_t)s->istep, (uint32_t)(s->istep >> 32), (uint32_t)a, 0x6e6f6973u, 0x6e6f6974u, key, 0, n);
+}
+/* initial velocities (which 0) and noise terms (which 1) */
+static void ctr_init_normals(orc_sys *s, int r, int a, int which, double n[3]) {
+    uint32_t key[2];
+    ctr_key(s, r, key);
+    ctr_normals3(0xffffffffu, (uint32_t)which, (uint32_t)a, 0x696e6974u, 0x696e6974u, key, 1, n);
+}
+/* the k-th exchange uniform of step istep (0-based pair counter) */
+static double ctr_exchange_uniform(orc_sys *s, int k) {
+    uint32_t o[4];
+    uint32_t q[4] = {(uint32_t)s->istep, (uint32_t)(s->istep >> 32), (uint32_t)k, 0x72657863u};
+    uint32_t key[2] = {(uint32_t)s->ctr_seed, (uint32_t)(s->ctr_seed >> 32) ^ 0x1234567u};
+    orc_philox(q, key, o);
+    return ctr_u01(o[0], o[1]);
+}
+void orc_set_counter_rng(orc_sys *s, uint64_t seed, int32_t on) { s->ctr_on = on; s->ctr_seed = seed; }
 
 /* ---------------- MT19937 words and conversions ---------------- */
 /* init_genrand (mt_stream.F90:419-455) and genrand_int32 with tempering
@@ -1440,6 +1507,17 @@ void orc_init_velocities(orc_sys *s) {
     for (int r = 0; r < s->nrep; r++) {
         double tK = s->rep[r].tK;
         double *v = s->velos + (size_t)r * s->nmp * 3, *ac = s->accels + (size_t)r * s->nmp * 3;
+        if (s->ctr_on) {   /* the engine's counter-based draws, same formulas */
+            for (int i = 0; i < s->nmp; i++) {
+                double nv[3], na[3], a, cf, cx, cr;
+                ctr_init_normals(s, r, i, 0, nv);
+                ctr_init_normals(s, r, i, 1, na);
+                double c1 = sqrt(tK * BOLTZ_KCAL_MOL / s->mass[i]);
+                md_coef(s, i, tK, &a, &cf, &cx, &cr);
+                for (int k = 0; k < 3; k++) { v[3 * i + k] = c1 * nv[k]; ac[3 * i + k] = cr * na[k]; }
+            }
+            continue;
+        }
         for (int i = 0; i < s->nmp; i++) {
             double c1 = sqrt(tK * BOLTZ_KCAL_MOL / s->mass[i]);
             for (int k = 0; k < 3; k++) v[3 * i + k] = c1 * boxmuller(s, r);
@@ -1493,7 +1571,10 @@ static void exchange(orc_sys *s, double *energies_out) {
     }
     double *rnd = malloc(sizeof(double) * (n + 1));
     int nr = 0;
-    for (int i = 1; i <= n; i += 2) rnd[++nr] = gen_d1(&s->tape_rep);
+    for (int i = 1; i <= n; i += 2) {
+        double u = s->ctr_on ? ctr_exchange_uniform(s, nr) : gen_d1(&s->tape_rep);
+        rnd[++nr] = u;
+    }
     nr = 0;
     for (int li = 1; li <= n; li++) {
         int lj = get_pair(s, li);
@@ -1568,7 +1649,10 @@ int64_t orc_md(orc_sys *s, int64_t nsteps, int32_t nstep_exchange, double *energ
             }
             orc_force(s, r + 1, mc, forces);
             if (mc) memcpy(s->rep[r].bp_status_mc, s->rep[r].bp_status, s->rep[r].bp.n / 7 + 1);
-            for (int i = 0; i < 3 * n; i++) rnd[i] = boxmuller(s, r);
+            if (s->ctr_on)
+                for (int i = 0; i < n; i++) ctr_step_normals(s, r, i, rnd + 3 * i);
+            else
+                for (int i = 0; i < 3 * n; i++) rnd[i] = boxmuller(s, r);
             for (int i = 0; i < n; i++) {
                 double a, cf, cx, cr;
                 md_coef(s, i, s->rep[r].tK, &a, &cf, &cx, &cr);

@@ -49,4 +49,8 @@ void orc_set_rg_schedule(orc_sys *s, int32_t n, const int64_t *steps, const doub
 void orc_set_anneal(orc_sys *s, int32_t n, const int64_t *steps, const double *temps);
 int32_t orc_set_ss(orc_sys *s, int32_t npair, const int32_t *pairs);   /* (2, npair) 1-based; npair < 0 clears */
 void orc_plant(orc_sys *s, int32_t which, int32_t on);   /* test hook: planted mistakes */
+/* counter-based random numbers of the engine's performance form in place of
+ * the MT tapes (Philox4x32-10 keyed by seed, replica, step, bead) */
+void orc_set_counter_rng(orc_sys *s, uint64_t seed, int32_t on);
+void orc_philox(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
 #endif

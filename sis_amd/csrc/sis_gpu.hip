@@ -829,7 +829,10 @@ void do_exchange(sis_gpu_ctx *c, double *replica_energies, int32_t *accepted, do
     for (int r = 0; r < R; r++) { replica_energies[2 * r] = E[2 * (r + 1)]; replica_energies[2 * r + 1] = E[2 * (r + 1) + 1]; }
   std::vector<double> rnd(R + 2);
   int nr = 0;
-  for (int i = 1; i <= R; i += 2) rnd[++nr] = rep_uniform(c, nr);
+  for (int i = 1; i <= R; i += 2) {   // uniform k (0-based) of the step for pair k+1
+    const double u = rep_uniform(c, nr);
+    rnd[++nr] = u;
+  }
   nr = 0;
   int nacc = 0;
   for (int li = 1; li <= R; li++) {
@@ -1300,8 +1303,13 @@ int sis_gpu_force(sis_gpu_ctx *c, int32_t irep, int32_t bp_mc, double *forces) {
   mc[irep - 1] = bp_mc ? 1 : 0;
   set_mask(c, m);
   HIPCHK(hipEventRecord(c->ev0, c->stream));
-  bp_phase(c, mc, nullptr, c->mask_d.p);
-  force_kernel(c, 0, c->mask_d.p);
+  if (c->mode == 3 && batched_ok(c)) {
+    // a context created for the batched form evaluates with that kernel
+    batched_force(c, irep - 1, bp_mc);
+  } else {
+    bp_phase(c, mc, nullptr, c->mask_d.p);
+    force_kernel(c, 0, c->mask_d.p);
+  }
   HIPCHK(hipEventRecord(c->ev1, c->stream));
   if (forces) {
     std::vector<double4> h(c->N);

@@ -324,6 +324,7 @@ struct SysIO {
   int *done, *nbp, *rebuild, *nreb;  // this system's slots
   int xlds;                     // beads whose positions fit the launch's dynamic LDS
   int fplds;                    // per-lane partial forces that fit after them
+  int force_only;               // evaluate the forces of the current positions into D.frc, no update
 };
 
 // Launch arguments of k_batched (arrays over replicas).
@@ -338,6 +339,7 @@ struct BatchArgs {
   int capb, lpb;
   int xlds;   // beads whose positions fit the dynamic LDS of the launch
   int fplds;  // per-lane partial forces that fit after them
+  int force_only;   // one force evaluation (limit MC if mc_first) into D.frc, no update
 };
 
 // Base-pair candidate arrays of one layout.
@@ -569,7 +571,7 @@ __device__ __forceinline__ void batched_body(const Params &P, const Ptrs &D, con
   // the first step after a rebuild does the limit MC (src/job_md.F90:388-395)
   int mc_next = J.mc_first;
   int nreb = 0;
-  if (J.todo > 0 && *J.rebuild) {
+  if (J.todo > 0 && *J.rebuild && !J.force_only) {
     batched_rebuild<B>(P, D, J, r, s_scan, s_tile, &s_flag);
     if (tid == 0) *J.rebuild = 0;
     mc_next = 1;
@@ -581,7 +583,8 @@ __device__ __forceinline__ void batched_body(const Params &P, const Ptrs &D, con
   const int64_t nsteps = J.todo, step0 = J.stepno;
   for (int64_t s = 0; s < nsteps && !overflow; s++) {
     const int64_t step = step0 + s + 1;
-    const int mc = P.nstep_bp_mc > 0 && ((step % P.nstep_bp_mc) == 0 || mc_next);
+    const int mc = J.force_only ? (P.nstep_bp_mc > 0 && J.mc_first)
+                                : P.nstep_bp_mc > 0 && ((step % P.nstep_bp_mc) == 0 || mc_next);
     mc_next = 0;
     const int nbr = *J.nbp;
     // ---- A: base pairs (+ union of formed pairs on MC steps); bonded
@@ -797,10 +800,18 @@ __device__ __forceinline__ void batched_body(const Params &P, const Ptrs &D, con
         comp[a] = a;
         J.seg_n[g] = 0;
         J.has_exc[g] = 0;
-        const double d2 = gjf_update(P, D, r, J.rk, a, F, 0, J.seed, step);
-        if (d2 > m1) { m2 = m1; m1 = d2; }
-        else if (d2 > m2) m2 = d2;
+        if (J.force_only) {
+          D.frc[g] = double4{F.x, F.y, F.z, 0.0};
+        } else {
+          const double d2 = gjf_update(P, D, r, J.rk, a, F, 0, J.seed, step);
+          if (d2 > m1) { m2 = m1; m1 = d2; }
+          else if (d2 > m2) m2 = d2;
+        }
       }
+    }
+    if (J.force_only) {   // forces of the current positions only: no update, no skin test
+      __syncthreads();
+      break;
     }
     if (l == 0) { s1[grp] = m1; s2[grp] = m2; }
     __syncthreads();
@@ -858,6 +869,7 @@ __global__ void __launch_bounds__(B) k_batched(Params P, Ptrs D, BatchArgs A) {
   J.pos_b = A.pos_b;
   J.xlds = A.xlds;
   J.fplds = A.fplds;
+  J.force_only = A.force_only;
   J.done = A.done + r;
   J.nbp = A.nbp + r;
   J.rebuild = A.rebuild + r;
@@ -924,6 +936,7 @@ __global__ void __launch_bounds__(B) k_jobs(Params P, Ptrs D0, const JobView *__
   J.pos_b = V.pos_b;
   J.xlds = xlds;
   J.fplds = fplds;
+  J.force_only = 0;
   J.done = V.io;
   J.nbp = V.io + 1;
   J.rebuild = V.io + 2;

@@ -204,6 +204,86 @@ std::vector<int> host_rebuild(sis_gpu_ctx *c, const std::vector<int> &mask) {
 
 }  // namespace
 
+namespace {
+
+void ensure_batched_io(sis_gpu_ctx *c) {
+  const int R = c->R;
+  const size_t RN = (size_t)R * c->N;
+  if (c->seg_d.n < RN * SEGCAP) c->seg_d.alloc(RN * SEGCAP);
+  if (c->bb_done_d.n < (size_t)R) {
+    c->bb_done_d.alloc(R); c->bb_nbp_d.alloc(R); c->bb_rebuild_d.alloc(R); c->bb_todo_d.alloc(R);
+    c->bb_stepno_d.alloc(R); c->bb_off_d.alloc(R + 1); c->bb_mcfirst_d.alloc(R); c->bb_nreb_d.alloc(R);
+  }
+}
+
+const void *batched_kernel(sis_gpu_ctx *c, int B) {
+  return B == 512 ? (c->precision == 1 ? (const void *)k_batched<float, 512> : (const void *)k_batched<double, 512>)
+                  : (c->precision == 1 ? (const void *)k_batched<float, 256> : (const void *)k_batched<double, 256>);
+}
+
+// arguments of one k_batched launch over the batched layout
+BatchArgs batched_args(sis_gpu_ctx *c, int lpb) {
+  const int N = c->N;
+  BatchArgs A{};
+  A.todo = c->bb_todo_d.p;
+  A.stepno = c->bb_stepno_d.p;
+  A.seed = c->seed;
+  A.seg_n = c->exc_cnt_d.p;
+  A.seg = c->seg_d.p;
+  A.has_exc = c->exc_fill_d.p;
+  A.done = c->bb_done_d.p;
+  A.nbp = c->bb_nbp_d.p;
+  A.rebuild = c->bb_rebuild_d.p;
+  A.mc_first = c->bb_mcfirst_d.p;
+  A.nreb = c->bb_nreb_d.p;
+  A.pos_b = c->pos2_d.p;
+  A.capb = c->batch_capb;
+  A.lpb = lpb;
+  A.xlds = N <= BATCH_X_LDS ? N : 0;
+  A.fplds = (A.xlds && N * lpb <= BATCH_FP_LDS) ? N * lpb : 0;
+  A.force_only = 0;
+  return A;
+}
+Ptrs batched_ptrs(sis_gpu_ctx *c) {
+  Ptrs D = c->ptrs();
+  D.bp_mp = c->bb_mp_d.p;
+  D.bp_coef = c->bb_coef_d.p;
+  D.bp_status = c->bb_status_d.p;
+  D.bp_status_mc = c->bb_status_mc_d.p;
+  D.ene_bp = c->bb_ene_d.p;
+  D.for_bp = c->bb_for_d.p;
+  return D;
+}
+
+}  // namespace
+
+// The forces of replica irep0's current positions from the batched kernel
+// itself (k_batched in force-only mode: its base-pair evaluation, limit MC
+// when bp_mc, pair and bonded arithmetic in the context's precision), into
+// frc_d.  Lists must be valid; nothing moves.
+void batched_force(sis_gpu_ctx *c, int irep0, int bp_mc) {
+  const int R = c->R;
+  const int B = c->opt.block == 512 ? 512 : 256;
+  ensure_batched_io(c);
+  to_batched(c);
+  std::vector<int> todo(R, 0), mcf(R, 0);
+  std::vector<int64_t> stepno(R, c->istep);
+  todo[irep0] = 1;
+  mcf[irep0] = bp_mc ? 1 : 0;
+  HIPCHK(hipMemcpyAsync(c->bb_todo_d.p, todo.data(), R * sizeof(int), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(c->bb_stepno_d.p, stepno.data(), R * sizeof(int64_t), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(c->bb_mcfirst_d.p, mcf.data(), R * sizeof(int), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemsetAsync(c->flags_d.p, 0, 8 * sizeof(int), c->stream));
+  BatchArgs A = batched_args(c, lanes_per_bead(c->N));
+  A.force_only = 1;
+  Params P = c->P;
+  Ptrs D = batched_ptrs(c);
+  void *args[] = {&P, &D, &A};
+  HIPCHK(hipLaunchKernel(batched_kernel(c, B), dim3(R), dim3(B), args, (size_t)(A.xlds + A.fplds) * sizeof(double4),
+                         c->stream));
+  to_packed(c);
+}
+
 // Batched form: one workgroup per replica runs whole segments of steps
 // (up to the next exchange).  Lists move to the batched layout for the
 // segment and back afterwards, so skin counters, pending rebuilds and the
@@ -216,17 +296,11 @@ void batched_steps(sis_gpu_ctx *c, int64_t nsteps) {
   const int N = c->N, R = c->R;
   // 256 threads (4 waves) per replica by default; 512 (8 waves) on request
   const int B = c->opt.block == 512 ? 512 : 256;
-  const size_t RN = (size_t)R * N;
   std::vector<int> all(R, 1), mcf(R, 0);
   if (!c->lists_valid) mcf = host_rebuild(c, all);
-  if (c->seg_d.n < RN * SEGCAP) c->seg_d.alloc(RN * SEGCAP);
-  if (c->bb_done_d.n < (size_t)R) {
-    c->bb_done_d.alloc(R); c->bb_nbp_d.alloc(R); c->bb_rebuild_d.alloc(R); c->bb_todo_d.alloc(R);
-    c->bb_stepno_d.alloc(R); c->bb_off_d.alloc(R + 1); c->bb_mcfirst_d.alloc(R); c->bb_nreb_d.alloc(R);
-  }
+  ensure_batched_io(c);
   const int lpb = lanes_per_bead(N);
-  const void *kfn = B == 512 ? (c->precision == 1 ? (const void *)k_batched<float, 512> : (const void *)k_batched<double, 512>)
-                             : (c->precision == 1 ? (const void *)k_batched<float, 256> : (const void *)k_batched<double, 256>);
+  const void *kfn = batched_kernel(c, B);
   c->block = B;
   int64_t left = nsteps;
   const int ex = c->opt.exchange_every;
@@ -241,31 +315,9 @@ void batched_steps(sis_gpu_ctx *c, int64_t nsteps) {
       HIPCHK(hipMemcpyAsync(c->bb_stepno_d.p, stepno.data(), R * sizeof(int64_t), hipMemcpyHostToDevice, c->stream));
       HIPCHK(hipMemcpyAsync(c->bb_mcfirst_d.p, mcf.data(), R * sizeof(int), hipMemcpyHostToDevice, c->stream));
       HIPCHK(hipMemsetAsync(c->flags_d.p, 0, 8 * sizeof(int), c->stream));
-      BatchArgs A{};
-      A.todo = c->bb_todo_d.p;
-      A.stepno = c->bb_stepno_d.p;
-      A.seed = c->seed;
-      A.seg_n = c->exc_cnt_d.p;
-      A.seg = c->seg_d.p;
-      A.has_exc = c->exc_fill_d.p;
-      A.done = c->bb_done_d.p;
-      A.nbp = c->bb_nbp_d.p;
-      A.rebuild = c->bb_rebuild_d.p;
-      A.mc_first = c->bb_mcfirst_d.p;
-      A.nreb = c->bb_nreb_d.p;
-      A.pos_b = c->pos2_d.p;
-      A.capb = c->batch_capb;
-      A.lpb = lpb;
-      A.xlds = N <= BATCH_X_LDS ? N : 0;
-      A.fplds = (A.xlds && N * lpb <= BATCH_FP_LDS) ? N * lpb : 0;
+      BatchArgs A = batched_args(c, lpb);
       Params P = c->P;
-      Ptrs D = c->ptrs();
-      D.bp_mp = c->bb_mp_d.p;
-      D.bp_coef = c->bb_coef_d.p;
-      D.bp_status = c->bb_status_d.p;
-      D.bp_status_mc = c->bb_status_mc_d.p;
-      D.ene_bp = c->bb_ene_d.p;
-      D.for_bp = c->bb_for_d.p;
+      Ptrs D = batched_ptrs(c);
       void *args[] = {&P, &D, &A};
       c->tmark();
       HIPCHK(hipLaunchKernel(kfn, dim3(R), dim3(B), args, (size_t)(A.xlds + A.fplds) * sizeof(double4), c->stream));

@@ -1,0 +1,173 @@
+"""The performance forms against the oracle, through the C ABI: the batched
+kernel (k_batched, one workgroup per replica), the persistent kernel
+(k_persistent, one cooperative launch per rebuild interval) and the job
+pool's shared launch (k_jobs), in FP64 and FP32 pair arithmetic.
+
+The oracle draws the same counter-based random numbers as the engine's
+performance form (Philox4x32-10 keyed by seed, replica, step and bead,
+restated in oracle/sis_oracle.c and pinned by its known-answer vectors), so
+initial velocities, noise, rebuilds and exchange decisions are the same
+calculation on both sides.  The limit MC is off (nstep_MC = 0): its
+performance form resolves conflicts per component with its own draws, and
+its distribution is tested in test_gpu_parity.py.
+
+Bars: FP64 positions and velocities within 1e-9 (A, A/tau) of the oracle
+after 100-200 steps with in-kernel rebuilds and exchanges, labels equal;
+FP32 pair arithmetic within 1e-5; forces from the batched kernel itself,
+per-bead relative error |dF| / max(|F_ref|, 1e-3 kcal/mol/A): FP64 <= 1e-10,
+FP32 <= 1e-5 (the north star's bound)."""
+import time
+
+import numpy as np
+import pytest
+
+from sis_amd import configs
+from sis_amd.engine import Engine
+from sis_amd.serving import DONE, JobPool
+from oracle import corc
+
+pytestmark = pytest.mark.gpu
+
+SEED = 4242
+TOL = {0: 1e-9, 1: 1e-5}
+
+
+def _oracle_run(s, seed, nstep, nex):
+    o = corc.Oracle(s)
+    o.set_counter_rng(seed)
+    o.init_velocities()
+    nreb, _ = o.md(nstep, nex)
+    return o, nreb
+
+
+def _c3(nrep, margin=1.0):
+    s = configs.remd_c3(nrep=nrep, n=128, seed=3, margin=margin)
+    s["params"] = dict(s["params"], nstep_bp_mc=0)
+    return s
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+@pytest.mark.parametrize("nrep,nstep,nex", [(16, 200, 50), (256, 100, 50)])
+def test_batched_trajectory_matches_oracle(prec, nrep, nstep, nex):
+    """C3's chain (128 nt, 290-380 K ladder) with a 1 A skin: in-kernel
+    rebuilds and exchanges every 50 steps; 256 replicas is C3's own ladder
+    at the reference's per-dimension maximum."""
+    s = _c3(nrep)
+    print(f"[case] batched prec {prec} R {nrep}", flush=True)
+    e = Engine(s, mode="batched", precision=prec, seed=SEED, exchange_every=nex)
+    e.init_velocities()
+    e.neighbor_list(0)
+    e.step(nstep)
+    assert e.info()["mode"] == 3
+    o, nreb = _oracle_run(s, SEED, nstep, nex)
+    assert nreb > 0 and e.info()["rebuilds"] > 0
+    x, v, _ = e.get_state()
+    xo, vo, _ = o.state()
+    assert np.abs(x - xo).max() < TOL[prec]
+    assert np.abs(v - vo).max() < TOL[prec]
+    assert np.array_equal(e.labels(), o.labels())
+    assert not np.array_equal(o.labels(), np.arange(1, nrep + 1))   # exchanges were accepted
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+def test_batched_kernel_forces_match_oracle(prec):
+    """Forces of the batched kernel's own arithmetic (force-only launch) on
+    replicas 1 and 5 of C3's chain."""
+    s = configs.remd_c3(nrep=8, n=128, seed=3)
+    print(f"[case] batched forces prec {prec}", flush=True)
+    e = Engine(s, mode="batched", precision=prec, seed=SEED)
+    e.neighbor_list(0)
+    o = corc.Oracle(s)
+    for r in range(1, 9):
+        o.neighbor_list(r)
+    for r in (1, 5):
+        fe, fo = e.force(r, 0), o.force(r, 0)
+        rel = np.linalg.norm(fe - fo, axis=1) / np.maximum(np.linalg.norm(fo, axis=1), 1e-3)
+        assert rel.max() <= (1e-10 if prec == 0 else 1e-5), rel.max()
+
+
+def test_batched_kernel_limit_mc_step():
+    """One force evaluation of the batched kernel with the limit MC: the
+    energy of every pair formed by energy equals the oracle's (the MC keeps
+    it for the pairs it deletes), the surviving pairs are a subset
+    of those formed by energy, and no nucleotide is left above
+    max_bp_per_nt (src/force_bp_limit_triplet.F90:96-330)."""
+    s = configs.remd_c3(nrep=4, n=128, seed=3)
+    print("[case] batched limit MC step", flush=True)
+    e = Engine(s, mode="batched", precision=0, seed=SEED)
+    e.neighbor_list(0)
+    o = corc.Oracle(s)
+    for r in range(1, 5):
+        o.neighbor_list(r)
+    mp = s["params"]["max_bp_per_nt"]
+    deleted = 0
+    for r in range(1, 5):
+        e.force(r, 1)
+        o.force(r, 0)
+        be = e.bp(r)
+        st_o, en_o = o.bp_state(r)
+        lst = e.lists(r)["bp"]
+        assert np.array_equal(lst, o.lists(r)["bp"])
+        fo = st_o.astype(bool)
+        assert np.allclose(be["ene"][fo], en_o[fo], rtol=1e-10, atol=1e-12)   # kept for deleted pairs too
+        st_e = be["status"].astype(bool)
+        assert np.all(~st_e | fo)
+        cnt = np.zeros(s["xyz"].shape[1] + 1, int)
+        for (i, j) in lst[st_e][:, :2]:
+            cnt[i] += 1
+            cnt[j] += 1
+        assert cnt.max() <= mp
+        deleted += int(st_o.astype(bool).sum() - st_e.sum())
+    assert deleted > 0   # the limit had work to do
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+def test_persistent_trajectory_matches_oracle(prec):
+    """One periodic condensate (40 x 47-nt CAG, 1,880 beads, 1 mM, cell
+    list) in the persistent kernel, 1 A skin so that rebuilds end launches."""
+    s = configs.cag_system(40, 47, 1.0, 310.0, margin=1.0)
+    s["params"] = dict(s["params"], nstep_bp_mc=0)
+    print(f"[case] persistent prec {prec}", flush=True)
+    e = Engine(s, mode="persistent", precision=prec, seed=SEED)
+    e.init_velocities()
+    e.neighbor_list(0)
+    e.step(80)
+    assert e.info()["mode"] == 2
+    o, nreb = _oracle_run(s, SEED, 80, 0)
+    assert nreb > 0 and e.info()["rebuilds"] > 0
+    x, v, _ = e.get_state()
+    xo, vo, _ = o.state()
+    assert np.abs(x - xo).max() < TOL[prec]
+    assert np.abs(v - vo).max() < TOL[prec]
+
+
+def _job(seed, nch, length, box, tK=300.0, ion=0.15):
+    rng = np.random.default_rng(seed)
+    seqs = [configs.random_seq(rng, length, 0.55) for _ in range(nch)]
+    x = configs._place_chains(rng, seqs, box, clash=6.0)
+    p = configs.default_params(tempK=tK, ionic_strength=ion, nl_margin=1.0, pbc=1, box=[box] * 3, nstep_bp_mc=0)
+    return configs._mk("pool_job", seqs, x, p)
+
+
+def test_pool_jobs_match_oracle():
+    """Jobs of the serving pool's shared launch (k_jobs), one per
+    precision, each against the oracle seeded with the job's seed."""
+    jobs = [(_job(1, 4, 40, 160.0), 200, 11, 0), (_job(2, 2, 90, 200.0, tK=340.0, ion=0.3), 150, 12, 1)]
+    print("[case] pool jobs vs oracle", flush=True)
+    with JobPool(quantum_bead_steps=20000) as pool:
+        ids = [pool.submit(j, n, seed=sd, precision=pr) for j, n, sd, pr in jobs]
+        res = {}
+        t0 = time.time()
+        while len(res) < len(ids):
+            assert time.time() - t0 < 120, "pool did not finish"
+            for i in ids:
+                if i not in res:
+                    st, tm, x = pool.poll(i, want_xyz=True)
+                    if st == DONE:
+                        res[i] = (x, tm)
+            time.sleep(0.01)
+    for (j, n, sd, pr), i in zip(jobs, ids):
+        o, _ = _oracle_run(j, sd, n, 0)
+        x, tm = res[i]
+        assert tm[4] > 0   # rebuilds inside the job
+        assert np.abs(x - o.state()[0][0]).max() < TOL[pr]

@@ -151,6 +151,36 @@ def test_read_ff_matches_benchmark_parameters():
     assert n["nnend"] == 1 and n["nnend_dH"][0] == 4.36
 
 
+def test_philox_known_answers():
+    """Philox4x32-10 restatement against the known-answer vectors published
+    with the Random123 library of Salmon et al. (SC'11): zero counter and
+    key, all-ones counter and key, and the pi-digits counter/key."""
+    kat = [
+        ([0, 0, 0, 0], [0, 0], [0x6627e8d5, 0xe169c58d, 0xbc57ac4c, 0x9b00dbd8]),
+        ([0xffffffff] * 4, [0xffffffff] * 2, [0x408f276d, 0x41c83b0e, 0xa20bc7c6, 0x6d5451fd]),
+        ([0x243f6a88, 0x85a308d3, 0x13198a2e, 0x03707344], [0xa4093822, 0x299f31d0],
+         [0xd16cfe09, 0x94fdcceb, 0x5001e420, 0x24126ea1]),
+    ]
+    for ctr, key, want in kat:
+        assert [int(v) for v in corc.philox4x32_10(ctr, key)] == want
+
+
+def test_counter_rng_normals_are_standard():
+    """The counter-RNG normals the oracle draws for the engine's performance
+    form: mean 0, variance 1 per component over many beads and steps, and
+    the initial velocities at the equipartition temperature."""
+    s = configs.remd_c3(nrep=2, n=400, seed=3)
+    s["params"] = dict(s["params"], nstep_bp_mc=0)
+    o = corc.Oracle(s)
+    o.set_counter_rng(1234)
+    o.init_velocities()
+    _, v, _ = o.state()
+    m = np.array([{0: 328.212, 1: 305.164, 2: 344.212, 3: 304.182}[int(q)] for q in s["seq"]])
+    kT = 0.0019872042586 * np.asarray(s["temps"])
+    z = v / np.sqrt(kT[:, None, None] / m[None, :, None])
+    assert abs(z.mean()) < 0.05 and abs(z.var() - 1.0) < 0.05
+
+
 def test_h4_index_rule_exercised():
     """mixed has chains of 24 and 40 nt: some bp_mp(6) entries follow the
     reference's comparison with the length of chain i
