@@ -96,7 +96,9 @@ def test_batched_kernel_limit_mc_step():
     print("[case] batched limit MC step", flush=True)
     e = Engine(s, mode="batched", precision=0, seed=SEED)
     e.neighbor_list(0)
-    o = corc.Oracle(s)
+    # the oracle without the MC: every candidate below the energy cutoff is
+    # formed (src/force_bp_triplet.F90 with nstep_MC = 0)
+    o = corc.Oracle(dict(s, params=dict(s["params"], nstep_bp_mc=0)))
     for r in range(1, 5):
         o.neighbor_list(r)
     mp = s["params"]["max_bp_per_nt"]
