@@ -133,6 +133,21 @@ int sis_gpu_force(sis_gpu_ctx *ctx, int32_t irep, int32_t bp_mc, double *forces)
 int sis_gpu_energy(sis_gpu_ctx *ctx, int32_t irep, double tempK, int32_t flags,
                    double *energies);
 
+/* sis_gpu_energy without any device allocation.  No counterpart in the
+ * reference: for callers that cannot allocate (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Same
+ * kernels and summation order as sis_gpu_energy; its partial sums, output
+ * row and per-call conditions live in a static device array of
+ * SIS_NOALLOC_SCRATCH_BYTES per device instead of library allocations.
+ * Needs (8*ceil(nmp/256) + 16 + nrep) doubles + 2*nrep ints of it;
+ * larger contexts get SIS_ERR_CAPACITY.  flags: bit1 as in
+ * sis_gpu_energy; bit0 (the limit MC) is refused.  The context's device
+ * must be current.  Every call on a device shares that device's array:
+ * calls that overlap on one device race. */
+#define SIS_NOALLOC_SCRATCH_BYTES 262144
+int sis_gpu_energy_noalloc(sis_gpu_ctx *ctx, int32_t irep, double tempK, int32_t flags,
+                           double *energies);
+
 /* Parity dumps of the interaction lists.  kind 0 = wca_mp, 1 = ele_mp
  * (2 ints per pair, i < j), 2 = bp_mp (7 ints per candidate, list order
  * of src/neighbor_list.F90:99-114). */

@@ -7,6 +7,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -203,6 +204,7 @@ struct sis_gpu_ctx {
   std::vector<int> rebuilt;          // replicas the last rebuild() rebuilt
   int nblk_force = 0;
   int block = 256;
+  int dev = 0;   // the HIP device current when the context was created
 
   Ptrs ptrs() {
     Ptrs d{};
@@ -725,7 +727,7 @@ void energy_phase(sis_gpu_ctx *c, const std::vector<int> &mask, const std::vecto
   dim3 g = grid_beads(N, R, B);
   if ((int64_t)c->ene_part_d.n < (int64_t)R * g.x * 8) c->ene_part_d.alloc((size_t)R * g.x * 8);
   hipLaunchKernelGGL(k_energy<B>, g, dim3(B), 0, c->stream, c->P, c->ptrs(), c->ecT_d.p, c->eCoef_d.p,
-                     c->eLam_d.p, c->ene_part_d.p, c->mask_d.p, flags & 1);
+                     c->eLam_d.p, c->ene_part_d.p, c->mask_d.p, flags & 1, -1, 0.0, 0.0);
   const double *ext = nullptr;
   if (c->ext_on) {
     if (c->X.rg) hipLaunchKernelGGL(k_rg_stats<256>, dim3(R), dim3(256), 0, c->stream, c->P, c->ptrs(), c->X);
@@ -749,7 +751,7 @@ void energy_phase(sis_gpu_ctx *c, const std::vector<int> &mask, const std::vecto
     ext = c->x_ene_d.p;
   }
   hipLaunchKernelGGL(k_energy_reduce, dim3(R), dim3(1), 0, c->stream, c->ene_part_d.p, (int)g.x, c->ene_out_d.p,
-                     ext);
+                     ext, -1);
   HIPCHK(hipMemcpyAsync(out, c->ene_out_d.p, (size_t)R * 16 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
   check_flags(c);
@@ -980,6 +982,49 @@ void step_phases(sis_gpu_ctx *c) {
 
 #include "sis_output_host.h"
 
+// Device memory of sis_gpu_energy_noalloc: per-block partial sums, the
+// output row and the call's conditions, mask and MC flags.  Each device
+// the library runs on holds its own copy until the process ends; calls
+// that overlap on one device share it and race.
+static __device__ __align__(256) unsigned char scratch[SIS_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+
+// Address of each device's `scratch` (the array's name in host code is a
+// stand-in, not a device address), resolved once per device with that
+// device current.
+constexpr int SCRATCH_MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[SCRATCH_MAX_DEVICES] = {};
+
+void resolve_scratch(int dev) {
+  if (dev < 0 || dev >= SCRATCH_MAX_DEVICES) throw ApiError{SIS_ERR_HIP, "device id beyond the scratch table"};
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (g_scratch_ptr[dev]) return;
+  void *p = nullptr;
+  HIPCHK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+  if (!p) throw ApiError{SIS_ERR_HIP, "hipGetSymbolAddress gave no address for the scratch array"};
+  g_scratch_ptr[dev] = p;
+}
+
+void *scratch_of(int dev) {
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  return (dev >= 0 && dev < SCRATCH_MAX_DEVICES) ? g_scratch_ptr[dev] : nullptr;
+}
+
+// conditions of one no-allocation energy call: temperature of every
+// replica, the mask and MC flags selecting replica r0 (a kernel, not
+// hipMemsetAsync, so that the call can sit in a captured graph)
+__global__ void k_noalloc_setup(double *ecT, int *mask, int *mc, int R, int r0, double T) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= R) return;
+  ecT[r] = T;
+  mask[r] = r == r0 ? 1 : 0;
+  mc[r] = 0;
+}
+
+}  // namespace
+
 extern "C" {
 
 const char *sis_gpu_last_error(void) { return g_err.c_str(); }
@@ -1012,6 +1057,8 @@ int sis_gpu_create(const sis_topo *topo, const sis_params *prm, const sis_option
   if (dev_count < 1) return fail(SIS_ERR_HIP, "no HIP device");
   auto *c = new sis_gpu_ctx();
   try {
+    HIPCHK(hipGetDevice(&c->dev));
+    resolve_scratch(c->dev);   // sis_gpu_energy_noalloc's scratch on this device
     c->opt = o;
     c->prm = *prm;
     c->N = topo->nmp;
@@ -1339,6 +1386,65 @@ int sis_gpu_energy(sis_gpu_ctx *c, int32_t irep, double tempK, int32_t flags, do
   std::vector<double> out((size_t)c->R * 16);
   energy_phase(c, m, T, flags, out.data());
   for (int k = 0; k < 11; k++) energies[k] = out[(size_t)(irep - 1) * 16 + k];
+  return SIS_OK;
+  API_END
+}
+
+int sis_gpu_energy_noalloc(sis_gpu_ctx *c, int32_t irep, double tempK, int32_t flags, double *energies) {
+  API_BEGIN
+  if (!c || !energies) return fail(SIS_ERR_ARG, "null argument");
+  if (irep < 1 || irep > c->R) return fail(SIS_ERR_ARG, "irep out of range");
+  if (flags & ~3) return fail(SIS_ERR_ARG, "unknown flags");
+  if (flags & 1) return fail(SIS_ERR_ARG, "the limit MC needs scratch that sis_gpu_energy_noalloc does not have; use sis_gpu_energy");
+  const int N = c->N, R = c->R, B = 256;
+  const int nblk = (N + B - 1) / B;
+  // partial sums [nblk*8], output [16], temperatures [R] (doubles), then
+  // mask [R] and MC flags [R] (ints)
+  const size_t need = ((size_t)nblk * 8 + 16 + (size_t)R) * sizeof(double) + (size_t)2 * R * sizeof(int);
+  if (need > (size_t)SIS_NOALLOC_SCRATCH_BYTES)
+    return fail(SIS_ERR_CAPACITY, "sis_gpu_energy_noalloc needs " + std::to_string(need) +
+                                      " bytes of scratch for this context; the maximum is " +
+                                      std::to_string((size_t)SIS_NOALLOC_SCRATCH_BYTES));
+  if (!c->lists_valid) return fail(SIS_ERR_STATE, "energy called before rebuild");
+  int dev = -1;
+  HIPCHK(hipGetDevice(&dev));
+  if (dev != c->dev) return fail(SIS_ERR_STATE, "the context's device must be current");
+  void *base = scratch_of(dev);
+  if (!base) return fail(SIS_ERR_STATE, "scratch of this device not resolved");
+  double *partial = static_cast<double *>(base);
+  double *out = partial + (size_t)nblk * 8;
+  double *ecT = out + 16;
+  int *mask = reinterpret_cast<int *>(ecT + R);
+  int *mc = mask + R;
+  const int r0 = irep - 1;
+  const double T = tempK > 0 ? tempK : c->tK[r0];
+  double coef, lam, cut;
+  host_set_ele(c->prm, c->anneal_on ? c->ele_T[r0] : T, c->ionic[r0], coef, lam, cut);
+  hipLaunchKernelGGL(k_noalloc_setup, dim3((R + 255) / 256), dim3(256), 0, c->stream, ecT, mask, mc, R, r0, T);
+  Ptrs D = c->ptrs();
+  if (!(flags & 2)) {   // base-pair energies at T without the limit MC
+    hipLaunchKernelGGL(k_reset_cnt, grid_beads(N, R, B), dim3(B), 0, c->stream, c->P, D, mask);
+    if (c->nbp_total > 0) {
+      if (c->precision == 1)
+        hipLaunchKernelGGL(k_bp_eval<float>, dim3((c->nbp_total + B - 1) / B), dim3(B), 0, c->stream, c->P, D,
+                           c->nbp_total, mc, ecT, mask);
+      else
+        hipLaunchKernelGGL(k_bp_eval<double>, dim3((c->nbp_total + B - 1) / B), dim3(B), 0, c->stream, c->P, D,
+                           c->nbp_total, mc, ecT, mask);
+    }
+  }
+  hipLaunchKernelGGL(k_energy<B>, dim3(nblk, 1), dim3(B), 0, c->stream, c->P, D, ecT, nullptr, nullptr, partial,
+                     mask, 0, r0, coef, lam);
+  const double *ext = nullptr;
+  if (c->ext_on) {   // external terms into their own buffer (made by sis_gpu_set_external)
+    if (c->X.rg) hipLaunchKernelGGL(k_rg_stats<256>, dim3(R), dim3(256), 0, c->stream, c->P, D, c->X);
+    hipLaunchKernelGGL(k_external_energy<256>, dim3(R), dim3(256), 0, c->stream, c->P, D, c->X);
+    ext = c->x_ene_d.p;
+  }
+  hipLaunchKernelGGL(k_energy_reduce, dim3(1), dim3(1), 0, c->stream, partial, nblk, out, ext, r0);
+  HIPCHK(hipMemcpyAsync(energies, out, 11 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  check_flags(c);
   return SIS_OK;
   API_END
 }

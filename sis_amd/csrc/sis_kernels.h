@@ -1446,13 +1446,17 @@ __global__ void k_scale_vel(Params P, Ptrs D, int r, double f, int divide) {
 // Energies (src/energy_sumup.F90 and the terms): per-block partial sums,
 // reduced in a fixed order.  Terms: 1 bond, 2 angle, 3 dihedral, 4 BP,
 // 5 WCA, 6 DH.  Each term is counted once (first bead of a bonded term,
-// lower bead of a pair, candidate owner for BP).
+// lower bead of a pair, candidate owner for BP).  r_only >= 0: one
+// replica, grid.y = 1, its DH coefficient and Debye length by value
+// (coef1, lam1) and its partial sums in slot 0 of `partial`.
 template <int B>
 __global__ void __launch_bounds__(B) k_energy(Params P, Ptrs D, const double *ecT,
                                               const double *eCoef, const double *eLam,
-                                              double *partial, const int *mask, int bp_masked) {
+                                              double *partial, const int *mask, int bp_masked,
+                                              int r_only, double coef1, double lam1) {
   __shared__ double s[7][B];
-  int a = blockIdx.x * B + threadIdx.x, r = blockIdx.y;
+  int a = blockIdx.x * B + threadIdx.x, r = r_only >= 0 ? r_only : blockIdx.y;
+  const int slot = r_only >= 0 ? 0 : r;
   const int N = P.N;
   double e[7] = {0, 0, 0, 0, 0, 0, 0};
   if (a < N && !(mask && !mask[r])) {
@@ -1516,8 +1520,8 @@ __global__ void __launch_bounds__(B) k_energy(Params P, Ptrs D, const double *ec
     if (P.ele_on) {
       int ne = min(D.n_ele[g], D.maxnb);
       const int *nb = D.nb_ele + (size_t)r * D.maxnb * N + a;
-      double coef = eCoef ? eCoef[r] : D.ele_coef[r];
-      double rc = 1.0 / (eLam ? eLam[r] : D.lambdaD[r]);
+      double coef = r_only >= 0 ? coef1 : (eCoef ? eCoef[r] : D.ele_coef[r]);
+      double rc = 1.0 / (r_only >= 0 ? lam1 : (eLam ? eLam[r] : D.lambdaD[r]));
       for (int k = 0; k < ne; k++) {
         int b = nb[(size_t)k * N];
         if (b < a) continue;
@@ -1537,7 +1541,7 @@ __global__ void __launch_bounds__(B) k_energy(Params P, Ptrs D, const double *ec
     __syncthreads();
   }
   if (threadIdx.x == 0)
-    for (int k = 1; k < 7; k++) partial[((size_t)r * gridDim.x + blockIdx.x) * 8 + k] = s[k][0];
+    for (int k = 1; k < 7; k++) partial[((size_t)slot * gridDim.x + blockIdx.x) * 8 + k] = s[k][0];
 }
 // energy_kinetic (src/energy_kinetic.F90:16-29): 0.5 sum m |v|^2 of
 // replica blockIdx.x, one workgroup per replica, FP64 tree sum.
@@ -1559,19 +1563,20 @@ __global__ void __launch_bounds__(B) k_kinetic(Params P, Ptrs D, double *out) {
   if (threadIdx.x == 0) out[r] = 0.5 * s[0];
 }
 
-// ext: stage, tweezers, Rg and restraint energies [4*R], or null
-__global__ void k_energy_reduce(const double *partial, int nblk, double *out, const double *ext) {
-  int r = blockIdx.x;
+// ext: stage, tweezers, Rg and restraint energies [4*R], or null.
+// r_only >= 0: that replica alone, partial sums and output in slot 0.
+__global__ void k_energy_reduce(const double *partial, int nblk, double *out, const double *ext, int r_only) {
+  const int r = r_only >= 0 ? r_only : blockIdx.x, slot = r_only >= 0 ? 0 : r;
   if (threadIdx.x != 0) return;
   double e[11] = {0};
   for (int b = 0; b < nblk; b++)
-    for (int k = 1; k < 7; k++) e[k] += partial[((size_t)r * nblk + b) * 8 + k];
+    for (int k = 1; k < 7; k++) e[k] += partial[((size_t)slot * nblk + b) * 8 + k];
   if (ext)
     for (int k = 0; k < 4; k++) e[7 + k] = ext[4 * r + k];
   double t = 0.0;
   for (int k = 1; k <= 10; k++) t += e[k];
   e[0] = t;
-  for (int k = 0; k < 11; k++) out[(size_t)r * 16 + k] = e[k];
+  for (int k = 0; k < 11; k++) out[(size_t)slot * 16 + k] = e[k];
 }
 
 }  // namespace sis

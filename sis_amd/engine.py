@@ -127,6 +127,14 @@ class Engine:
         _check(lib().sis_gpu_energy(self.h, irep, float(tempK), flags, abi.dptr(e)))
         return e
 
+    def energy_noalloc(self, irep: int = 1, tempK: float = -1.0, bp_energy_valid: int = 1) -> np.ndarray:
+        """energy_sumup without device allocation (sis_gpu_energy_noalloc);
+        the limit MC is not available there."""
+        e = np.zeros(11)
+        flags = 2 if bp_energy_valid else 0
+        _check(lib().sis_gpu_energy_noalloc(self.h, irep, float(tempK), flags, abi.dptr(e)))
+        return e
+
     def kinetic(self, irep: int = 0):
         """energy_kinetic of replica irep (1-based), or of all replicas (0)."""
         out = np.zeros(max(self.nrep, 1))

@@ -119,6 +119,17 @@ module sis_gpu_iface
          real(c_double), intent(out) :: energies(0:10)
       end function
 
+      ! energy_sumup without device allocation (no reference counterpart;
+      ! for callers that cannot allocate): flags bit1 only
+      integer(c_int) function sis_gpu_energy_noalloc(ctx, irep, tempK, flags, energies) &
+            bind(C, name="sis_gpu_energy_noalloc")
+         import :: c_int, c_ptr, c_int32_t, c_double
+         type(c_ptr), value :: ctx
+         integer(c_int32_t), value :: irep, flags
+         real(c_double), value :: tempK
+         real(c_double), intent(out) :: energies(0:10)
+      end function
+
       integer(c_int) function sis_gpu_get_pair_count(ctx, irep, kind, n) bind(C, name="sis_gpu_get_pair_count")
          import :: c_int, c_ptr, c_int32_t, c_int64_t
          type(c_ptr), value :: ctx
