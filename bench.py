@@ -55,11 +55,15 @@ def force_kernel_bytes(p_wca, p_ele):
 
 
 def load_traffic(workload, mode):
+    """Measured HBM bytes per bead-step of the dominant kernel (FETCH_SIZE +
+    WRITE_SIZE, calibrated; profiles/traffic.json names the summary each
+    figure comes from), or None."""
     f = ROOT / "profiles" / "traffic.json"
     if not f.exists():
         return None
     try:
-        return json.loads(f.read_text()).get(f"{workload}:{mode}")
+        e = json.loads(f.read_text()).get(f"{workload}:{mode}")
+        return float(e["bytes_per_bead_step"]) if e else None
     except Exception:
         return None
 
@@ -108,13 +112,15 @@ def plain_fp64_rate(sysd1, steps, warmup):
                 ms_per_step=round(1e3 * wall / steps, 4), form="phases", dtype="f64")
 
 
-def fp32_force_error(sysd):
+def fp32_force_error(sysd, mode="phases"):
     """FP32 vs FP64 pair arithmetic on the same lists, every replica, limit
-    MC off so that every formed pair contributes."""
+    MC off so that every formed pair contributes.  mode "batched": the
+    forces come from the batched kernel itself (its force-only launch),
+    whose FP64 form the tests pin against the oracle."""
     from sis_amd.engine import Engine
     s = dict(sysd, params=dict(sysd["params"], nstep_bp_mc=0))
-    e64 = Engine(s, precision=0, mode="phases")
-    e32 = Engine(s, precision=1, mode="phases")
+    e64 = Engine(s, precision=0, mode=mode)
+    e32 = Engine(s, precision=1, mode=mode)
     worst, sq, n, gnum, gden = 0.0, 0.0, 0, 0.0, 0.0
     for e in (e64, e32):
         e.neighbor_list(0)
@@ -132,7 +138,48 @@ def fp32_force_error(sysd):
     e32.close()
     return dict(max=float(f"{worst:.3e}"), rms=float(f"{np.sqrt(sq / n):.3e}"),
                 global_l2=float(f"{np.sqrt(gnum / max(gden, 1e-300)):.3e}"), tol=1e-5,
-                floor_kcal_mol_A=1e-3)
+                floor_kcal_mol_A=1e-3, kernel={"batched": "k_batched", "phases": "k_force_lanes"}[mode])
+
+
+def precision_report(sysd, steps, nsamples=5, seed=2025):
+    """The precision config's report: an FP32-pair run and an FP64 run of
+    the same system from the same seed (same noise), persistent form, for
+    `steps` steps; at nsamples evenly spaced times in the second half,
+    per-chain Rg and paired fraction (2,000 values each) and E_total.
+    Returns the KS tests of the pooled per-chain distributions and the
+    relative difference of the mean total energies."""
+    from scipy.stats import ks_2samp
+    from sis_amd import observables
+    from sis_amd.engine import Engine
+    engs = [Engine(sysd, precision=p, mode="persistent", seed=seed) for p in (0, 1)]
+    for e in engs:
+        e.init_velocities()
+        e.neighbor_list(0)
+    half = steps // 2
+    every = max(1, half // nsamples)
+    samples = [[], []]
+    t0 = time.perf_counter()
+    for i, e in enumerate(engs):
+        e.step(half)
+        for k in range(nsamples):
+            e.step(every)
+            samples[i].append(observables.chain_sample(e))
+            print(f"[bench] c4 report: precision {i} sample {k + 1}/{nsamples} "
+                  f"({time.perf_counter() - t0:.0f}s)", flush=True)
+    for e in engs:
+        e.close()
+    out = dict(steps=half + every * nsamples, samples=nsamples, seed=seed, form="persistent")
+    for key in ("rg", "bp_fraction"):
+        a = np.concatenate([q[key] for q in samples[0]])
+        b = np.concatenate([q[key] for q in samples[1]])
+        r = ks_2samp(a, b)
+        out[key] = dict(D=round(float(r.statistic), 4), p=round(float(r.pvalue), 4),
+                        mean_f64=round(float(a.mean()), 4), mean_f32=round(float(b.mean()), 4), n=int(a.size))
+    e64 = np.array([q["e_total"] for q in samples[0]])
+    e32 = np.array([q["e_total"] for q in samples[1]])
+    out["e_total"] = dict(mean_f64=round(float(e64.mean()), 3), mean_f32=round(float(e32.mean()), 3),
+                          rel_diff=float(f"{abs(e32.mean() - e64.mean()) / abs(e64.mean()):.3e}"))
+    return out
 
 
 def ensemble_agreement(nrep=64, n=128, tempK=330.0, steps=(2000, 4000, 6000), seed=3):
@@ -241,7 +288,9 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=None)
-    ap.add_argument("--workload", default="c3", choices=["c2", "c3", "c4"])
+    ap.add_argument("--workload", default="c3", choices=["c2", "c3", "c4", "c5"])
+    ap.add_argument("--report-steps", type=int, default=200000,
+                    help="c4: steps of the FP32-vs-FP64 distribution report")
     ap.add_argument("--mode", default="auto", choices=["auto", "phases", "persistent", "batched"])
     ap.add_argument("--precision", type=int, default=None)
     ap.add_argument("--cpu-steps", type=int, default=None)
@@ -272,8 +321,12 @@ def main():
               "(replica / spatial sharding over several GPUs is a design note only).", file=sys.stderr)
         sys.exit(2)
 
+    if args.workload == "c5":
+        serving_main(args)
+        return
+
     drawn = None
-    if not args.no_extras and args.serving_jobs > 0:
+    if not args.no_extras and args.serving_jobs > 0 and args.workload == "c3":
         t0 = time.perf_counter()
         drawn = draw_serving_jobs(args.serving_jobs, args.serving_max_chains)
         print(f"[bench] drew {len(drawn)} serving jobs in {time.perf_counter() - t0:.1f}s", flush=True)
@@ -355,9 +408,12 @@ def main():
     }
     if dom_ms > 0:
         achieved = kbytes * nmp * nrep * steps_per_launch / (dom_ms * 1e-3) / 1e9
+        tb = load_traffic(args.workload, mode_used)
         out["roofline"] = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
                            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
-                           "traffic": load_traffic(args.workload, mode_used),
+                           # counter bytes per bead-step x this run's bead-steps per launch
+                           "traffic": None if tb is None else int(tb * nmp * nrep * steps_per_launch),
+                           "traffic_bytes_per_bead_step": tb,
                            "kernel": kname, "kernel_ms": round(dom_ms, 5), "launches": dom_n,
                            "bytes_per_bead_step": round(kbytes, 1),
                            "bead_steps_per_launch": int(nmp * nrep * steps_per_launch)}
@@ -369,8 +425,11 @@ def main():
             out["plain_fp64"] = pl
             out["ratio_vs_plain_fp64"] = round(value / pl["value"], 2)
             print(f"[bench] plain fp64 {pl['value']:.3e}", flush=True)
-        out["fp32_force"] = fp32_force_error(sysd)
+        out["fp32_force"] = fp32_force_error(sysd, mode="batched" if mode_used == "batched" else "phases")
         print(f"[bench] fp32 force error {out['fp32_force']}", flush=True)
+        if args.workload == "c4":
+            out["precision_report"] = precision_report(sysd, args.report_steps)
+            print(f"[bench] c4 precision report {out['precision_report']}", flush=True)
         if args.workload == "c3":
             out["fp32_ensemble"] = ensemble_agreement()
             print(f"[bench] fp32 vs fp64 ensembles: min KS p {out['fp32_ensemble']['min_p']}", flush=True)
@@ -393,6 +452,27 @@ def main():
                 out["cpu_baseline"]["sample"] += cnote
         except Exception as exc:  # keep the GPU line even if the CPU leg fails
             print(f"[bench] cpu baseline failed: {exc}", file=sys.stderr)
+    print(json.dumps(out), flush=True)
+
+
+def serving_main(args):
+    """The serving config as its own line: args.serving_jobs jobs (2,000 by
+    default here) drawn as the config states, Poisson arrivals at a fixed
+    rate, steps scaled by args.serving_scale (stated in the line), half
+    FP32 and half FP64; value = aggregate bead-steps/s, with p50/p99."""
+    njobs = args.serving_jobs if args.serving_jobs != 48 else 2000
+    t0 = time.perf_counter()
+    drawn = draw_serving_jobs(njobs, args.serving_max_chains)
+    print(f"[bench] drew {len(drawn)} serving jobs in {time.perf_counter() - t0:.1f}s", flush=True)
+    sv = serving_run(drawn, args.serving_scale, args.serving_rate, args.serving_max_chains, timeout=480.0)
+    out = {
+        "metric": METRIC, "value": sv["bead_steps_per_s"], "unit": "bead-steps/s", "n_gpus": 1,
+        "steps": None, "warmup": 0, "ms_per_step": None, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "mixed f64 / f32-pair", "data": "synthetic (seeded random jobs)",
+        "config": {"workload": f"C5: {njobs} jobs, Poisson {args.serving_rate}/s, 1-{args.serving_max_chains} "
+                               f"chains x 20-400 nt, steps x {args.serving_scale}"},
+        "serving": sv, "roofline": None, "cpu_baseline": None,
+    }
     print(json.dumps(out), flush=True)
 
 
