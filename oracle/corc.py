@@ -50,6 +50,7 @@ def lib():
         L.orc_set_ionic.argtypes = [vp, P(d)]
         L.orc_plant.argtypes = [vp, i32, i32]
         L.orc_set_counter_rng.argtypes = [vp, C.c_uint64, i32]
+        L.orc_set_nl_brute.argtypes = [vp, i32]
         L.orc_philox.argtypes = [P(C.c_uint32), P(C.c_uint32), P(C.c_uint32)]
         L.orc_set_rep_rg.argtypes = [vp, P(d)]
         L.orc_set_rep_dcf.argtypes = [vp, P(d), P(d)]
@@ -228,6 +229,11 @@ class Oracle:
         engine's performance-form counter RNG (Philox4x32-10 keyed by seed,
         replica, step and bead) instead of the MT tapes."""
         lib().orc_set_counter_rng(self.h, int(seed), int(on))
+
+    def set_nl_brute(self, on: bool = True):
+        """Neighbour lists by the reference's O(N^2) loops even where the
+        cell-list enumeration would be used (large periodic systems)."""
+        lib().orc_set_nl_brute(self.h, int(on))
 
     def set_ionic(self, ionic):
         """Per-label ionic strengths (REMD over REPT%ION)."""

@@ -80,6 +80,7 @@ struct orc_sys {
     int plant_cut_at_set_ele;
     /* counter-based random numbers in place of the MT tapes (orc_set_counter_rng) */
     int ctr_on; uint64_t ctr_seed;
+    int nl_brute;            /* 1: always the O(N^2) loops (orc_set_nl_brute) */
     /* external terms (copies of the caller's arrays) */
     int x_on;
     sis_external x;
@@ -429,6 +430,142 @@ static double boxmuller(orc_sys *s, int irep) {
     return s->bm_value;
 }
 
+/* One (imp, jmp) pair of neighbor_list's loops, chain ich position i,
+ * chain jch position j (1-based), in the reference's loop body
+ * (src/neighbor_list.F90:119-223). */
+typedef struct { double wca2, ele2, bp2; } nl_cuts;
+static void nl_pair(orc_sys *s, rep_t *r, const double *x, const nl_cuts *c, int ich, int jch, int i, int j) {
+    int imp = s->chain_start[ich] + i - 1, jmp = s->chain_start[jch] + j - 1;
+    int iseq = s->seq[imp], jseq = s->seq[jmp];
+    double v[3];
+    pbc_vec_d(s, x + 3 * imp, x + 3 * jmp, v);
+    double d2 = dot3(v, v);
+    if (iseq != D_ && jseq != D_ && d2 <= c->wca2) {
+        int add = 1;
+        if (ich == jch) {
+            if (i + 2 >= j) add = 0;
+            else if (s->moltype[ich] == 2 && j + 2 >= i + s->nmp_chain[ich]) add = 0;
+        }
+        if (add) { ipush(&r->wca, imp + 1); ipush(&r->wca, jmp + 1); }
+    }
+    if (s->p.ele_on && d2 <= c->ele2 && s->has_charge[imp] && s->has_charge[jmp]) {
+        int add = 1;
+        if (ich == jch) {
+            int n = s->nmp_chain[ich];
+            if (j == i + 1 && s->p.ele_excl_bond) add = 0;
+            if (j == i + 2 && s->p.ele_excl_angl) add = 0;
+            if (s->moltype[ich] == 2) {
+                if (i == 1) {
+                    if (j == n && s->p.ele_excl_bond) add = 0;
+                    if (j == n - 1 && s->p.ele_excl_angl) add = 0;
+                } else if (i == 2) {
+                    if (j == n && s->p.ele_excl_angl) add = 0;
+                }
+            }
+        }
+        if (add) { ipush(&r->ele, imp + 1); ipush(&r->ele, jmp + 1); }
+    }
+    if (d2 > c->bp2) return;
+    int bt, b3;
+    orc_bp_pair(s, imp, jmp, &bt, &b3);
+    if (b3 > 0) {
+        int ni = s->nmp_chain[ich], nj = s->nmp_chain[jch];
+        ipush(&r->bp, imp + 1);
+        ipush(&r->bp, jmp + 1);
+        ipush(&r->bp, (i == 1) ? imp + 1 + ni - 1 : imp + 1 - 1);
+        ipush(&r->bp, (j == 1) ? jmp + 1 + nj - 1 : jmp + 1 - 1);
+        ipush(&r->bp, (i == ni) ? imp + 1 + 1 - ni : imp + 1 + 1);
+        /* the reference compares j with nmp_chain(ichain) here
+         * (src/neighbor_list.F90:212) */
+        ipush(&r->bp, (j == ni) ? jmp + 1 + 1 - nj : jmp + 1 + 1);
+        ipush(&r->bp, bt);
+        /* the triplet table is zero for models other than 5
+         * (src/init_bp.F90:39, src/neighbor_list.F90:220-221) */
+        dpush(&r->bpcoef, s->p.bp_model == 5 ? s->bp3_dH[b3] : 0.0);
+        dpush(&r->bpcoef, s->p.bp_model == 5 ? s->bp3_dS[b3] : 0.0);
+    }
+}
+
+typedef struct { int ich, jch, a, b; } nl_cand;
+/* the loops' order: chain of the first bead, chain of the second, then
+ * the positions (bead ids grow along a chain) */
+static int cmp_cand(const void *p, const void *q) {
+    const nl_cand *x = p, *y = q;
+    if (x->ich != y->ich) return x->ich < y->ich ? -1 : 1;
+    if (x->jch != y->jch) return x->jch < y->jch ? -1 : 1;
+    if (x->a != y->a) return x->a < y->a ? -1 : 1;
+    return (x->b > y->b) - (x->b < y->b);
+}
+
+/* Cell-list enumeration of the same pairs for large periodic systems: the
+ * reference's O(N^2) loops visit every pair in (ichain, jchain >= ichain,
+ * i, j) order and test distances; pairs farther apart than the largest
+ * list cutoff add nothing, so the candidates found through cells of at
+ * least that width (27 neighbours, minimum image) are sorted into the
+ * loops' order and given the same loop body.  The result equals the
+ * brute-force lists (pinned by test_oracle.py on periodic systems). */
+static int nl_cells(orc_sys *s, rep_t *r, const double *x, const nl_cuts *c) {
+    double cut2 = c->wca2;
+    if (c->ele2 > cut2) cut2 = c->ele2;
+    if (c->bp2 > cut2) cut2 = c->bp2;
+    const double cut = sqrt(cut2);
+    int nc[3];
+    for (int k = 0; k < 3; k++) {
+        nc[k] = (int)floor(s->p.box[k] / cut);
+        if (nc[k] < 3) return 0;
+        if (nc[k] > 256) nc[k] = 256;
+    }
+    const int n = s->nmp;
+    const int ncell = nc[0] * nc[1] * nc[2];
+    int *cell = malloc(sizeof(int) * n), *cnt = calloc(ncell + 1, sizeof(int)), *beads = malloc(sizeof(int) * n);
+    for (int a = 0; a < n; a++) {
+        int q[3];
+        for (int k = 0; k < 3; k++) {
+            q[k] = (int)(x[3 * a + k] * nc[k] / s->p.box[k]);
+            if (q[k] < 0) q[k] = 0;
+            if (q[k] >= nc[k]) q[k] = nc[k] - 1;
+        }
+        cell[a] = (q[2] * nc[1] + q[1]) * nc[0] + q[0];
+        cnt[cell[a] + 1]++;
+    }
+    for (int k = 0; k < ncell; k++) cnt[k + 1] += cnt[k];
+    int *fill = malloc(sizeof(int) * ncell);
+    memcpy(fill, cnt, sizeof(int) * ncell);
+    for (int a = 0; a < n; a++) beads[fill[cell[a]]++] = a;
+    free(fill);
+    /* candidate pairs, each once: the loops take a pair (a, b) with a in
+     * the earlier chain, or earlier in the same chain, i.e. a < b */
+    int64_t cap = 1 << 20, m = 0;
+    nl_cand *key = malloc(sizeof(nl_cand) * cap);
+    for (int a = 0; a < n; a++) {
+        int cx = cell[a] % nc[0], cy = (cell[a] / nc[0]) % nc[1], cz = cell[a] / (nc[0] * nc[1]);
+        for (int dz = -1; dz <= 1; dz++)
+            for (int dy = -1; dy <= 1; dy++)
+                for (int dx = -1; dx <= 1; dx++) {
+                    int qx = (cx + dx + nc[0]) % nc[0], qy = (cy + dy + nc[1]) % nc[1], qz = (cz + dz + nc[2]) % nc[2];
+                    int q = (qz * nc[1] + qy) * nc[0] + qx;
+                    for (int t = cnt[q]; t < cnt[q + 1]; t++) {
+                        int b = beads[t];
+                        if (b <= a) continue;
+                        double v[3];
+                        pbc_vec_d(s, x + 3 * a, x + 3 * b, v);
+                        if (dot3(v, v) > cut2) continue;
+                        if (m == cap) { cap *= 2; key = realloc(key, sizeof(nl_cand) * cap); }
+                        key[m].ich = s->chain_of[a]; key[m].jch = s->chain_of[b];
+                        key[m].a = a; key[m].b = b;
+                        m++;
+                    }
+                }
+    }
+    /* with at least 3 cells per axis the 27 neighbour cells are distinct,
+     * so every pair is found once */
+    qsort(key, m, sizeof(nl_cand), cmp_cand);
+    for (int64_t k = 0; k < m; k++)
+        nl_pair(s, r, x, c, key[k].ich, key[k].jch, s->lmp[key[k].a], s->lmp[key[k].b]);
+    free(key); free(cell); free(cnt); free(beads);
+    return 1;
+}
+
 /* ---------------- neighbour lists (src/neighbor_list.F90:84-233) ---------------- */
 void orc_neighbor_list(orc_sys *s, int32_t irep1) {
     int irep = irep1 - 1;
@@ -436,69 +573,21 @@ void orc_neighbor_list(orc_sys *s, int32_t irep1) {
     if (s->p.pbc) pbc_wrap(s, irep);
     r->wca.n = r->ele.n = r->bp.n = r->bpcoef.n = 0;
     const double margin = s->p.nl_margin;
-    double ele_nl_cut2 = 0.0;
+    nl_cuts c;
+    c.ele2 = 0.0;
     if (s->p.ele_on && s->p.ele_cutoff_type == 2) r->ele_cutoff = s->p.ele_cutoff * r->lambdaD;
-    if (s->p.ele_on) ele_nl_cut2 = (r->ele_cutoff + margin) * (r->ele_cutoff + margin);
-    double wca_nl_cut2 = (s->p.wca_sigma + margin) * (s->p.wca_sigma + margin);
-    double bp_nl_cut2 = (s->bp_cutoff_dist + margin) * (s->bp_cutoff_dist + margin);
+    if (s->p.ele_on) c.ele2 = (r->ele_cutoff + margin) * (r->ele_cutoff + margin);
+    c.wca2 = (s->p.wca_sigma + margin) * (s->p.wca_sigma + margin);
+    c.bp2 = (s->bp_cutoff_dist + margin) * (s->bp_cutoff_dist + margin);
     const double *x = s->xyz + (size_t)irep * s->nmp * 3;
+    if (s->p.pbc && s->nmp >= 4096 && !s->nl_brute && nl_cells(s, r, x, &c)) goto done;
     for (int ich = 0; ich < s->nchains; ich++)
         for (int jch = ich; jch < s->nchains; jch++)
             for (int i = 1; i <= s->nmp_chain[ich]; i++) {
-                int imp = s->chain_start[ich] + i - 1;
-                int iseq = s->seq[imp];
                 int j_start = (ich == jch) ? i + 1 : 1;
-                for (int j = j_start; j <= s->nmp_chain[jch]; j++) {
-                    int jmp = s->chain_start[jch] + j - 1;
-                    int jseq = s->seq[jmp];
-                    double v[3];
-                    pbc_vec_d(s, x + 3 * imp, x + 3 * jmp, v);
-                    double d2 = dot3(v, v);
-                    if (iseq != D_ && jseq != D_ && d2 <= wca_nl_cut2) {
-                        int add = 1;
-                        if (ich == jch) {
-                            if (i + 2 >= j) add = 0;
-                            else if (s->moltype[ich] == 2 && j + 2 >= i + s->nmp_chain[ich]) add = 0;
-                        }
-                        if (add) { ipush(&r->wca, imp + 1); ipush(&r->wca, jmp + 1); }
-                    }
-                    if (s->p.ele_on && d2 <= ele_nl_cut2 && s->has_charge[imp] && s->has_charge[jmp]) {
-                        int add = 1;
-                        if (ich == jch) {
-                            int n = s->nmp_chain[ich];
-                            if (j == i + 1 && s->p.ele_excl_bond) add = 0;
-                            if (j == i + 2 && s->p.ele_excl_angl) add = 0;
-                            if (s->moltype[ich] == 2) {
-                                if (i == 1) {
-                                    if (j == n && s->p.ele_excl_bond) add = 0;
-                                    if (j == n - 1 && s->p.ele_excl_angl) add = 0;
-                                } else if (i == 2) {
-                                    if (j == n && s->p.ele_excl_angl) add = 0;
-                                }
-                            }
-                        }
-                        if (add) { ipush(&r->ele, imp + 1); ipush(&r->ele, jmp + 1); }
-                    }
-                    int bt, b3;
-                    orc_bp_pair(s, imp, jmp, &bt, &b3);
-                    if (b3 > 0 && d2 <= bp_nl_cut2) {
-                        int ni = s->nmp_chain[ich], nj = s->nmp_chain[jch];
-                        ipush(&r->bp, imp + 1);
-                        ipush(&r->bp, jmp + 1);
-                        ipush(&r->bp, (i == 1) ? imp + 1 + ni - 1 : imp + 1 - 1);
-                        ipush(&r->bp, (j == 1) ? jmp + 1 + nj - 1 : jmp + 1 - 1);
-                        ipush(&r->bp, (i == ni) ? imp + 1 + 1 - ni : imp + 1 + 1);
-                        /* the reference compares j with nmp_chain(ichain) here
-                         * (src/neighbor_list.F90:212) */
-                        ipush(&r->bp, (j == ni) ? jmp + 1 + 1 - nj : jmp + 1 + 1);
-                        ipush(&r->bp, bt);
-                        /* the triplet table is zero for models other than 5
-                         * (src/init_bp.F90:39, src/neighbor_list.F90:220-221) */
-                        dpush(&r->bpcoef, s->p.bp_model == 5 ? s->bp3_dH[b3] : 0.0);
-                        dpush(&r->bpcoef, s->p.bp_model == 5 ? s->bp3_dS[b3] : 0.0);
-                    }
-                }
+                for (int j = j_start; j <= s->nmp_chain[jch]; j++) nl_pair(s, r, x, &c, ich, jch, i, j);
             }
+done:;
     int64_t nbp = r->bp.n / 7;
     r->bp_status = realloc(r->bp_status, nbp + 1);
     r->ene_bp = realloc(r->ene_bp, (nbp + 1) * sizeof(double));
@@ -508,6 +597,8 @@ void orc_neighbor_list(orc_sys *s, int32_t irep1) {
     memset(r->bp_status_mc, 0, nbp + 1);
     r->have_status_mc = 0;
 }
+
+void orc_set_nl_brute(orc_sys *s, int32_t on) { s->nl_brute = on; }
 
 int64_t orc_count(orc_sys *s, int32_t irep, int32_t kind) {
     rep_t *r = &s->rep[irep - 1];

@@ -154,20 +154,30 @@ def _place_chains(rng, seqs, box, clash=10.0):
     (minimum image)."""
     from scipy.spatial import cKDTree
     coords = []
-    allx = np.zeros((0, 3))
     tree = None
+    n_tree = 0          # beads of the first chains, held in the tree
+    recent = []         # wrapped beads of the chains placed since the tree was built
     for s in seqs:
         for _t in range(1000):
             start = rng.uniform(0.0, box, size=3)
             x = random_coil(rng, len(s), start=start)
-            if tree is not None:
-                # nearest earlier bead under the minimum image
-                d, _ = tree.query(np.mod(x, box), k=1)
-                if d.min() < clash:
+            xw = np.mod(x, box)
+            # nearest earlier bead under the minimum image: the tree's beads,
+            # then the recent ones directly
+            if tree is not None and tree.query(xw, k=1)[0].min() < clash:
+                continue
+            if recent:
+                r = np.vstack(recent)
+                d = xw[:, None, :] - r[None, :, :]
+                d -= box * np.round(d / box)
+                if np.sqrt((d ** 2).sum(axis=2)).min() < clash:
                     continue
             coords.append(x)
-            allx = np.vstack([allx, x])
-            tree = cKDTree(np.mod(allx, box), boxsize=box)
+            recent.append(xw)
+            if sum(len(q) for q in recent) > 2048:
+                n_tree += sum(len(q) for q in recent)
+                recent = []
+                tree = cKDTree(np.mod(np.vstack(coords), box), boxsize=box)
             break
         else:
             raise RuntimeError("could not place chain")

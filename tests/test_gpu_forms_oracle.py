@@ -173,3 +173,57 @@ def test_pool_jobs_match_oracle():
         x, tm = res[i]
         assert tm[4] > 0   # rebuilds inside the job
         assert np.abs(x - o.state()[0][0]).max() < TOL[pr]
+
+
+@pytest.fixture(scope="module")
+def c4_system():
+    s = configs.condensate_c4()
+    s["params"] = dict(s["params"], nstep_bp_mc=0)
+    return s
+
+
+def test_c4_full_size_lists_and_forces(c4_system):
+    """C4 at full size (2,000 x 47-nt CAG, 94,000 beads, 4,273.5 A box):
+    the cell-list rebuild gives the oracle's lists (WCA / DH pair sets,
+    base-pair candidates in the reference's order); FP64 forces within
+    1e-10 of the oracle's; FP32 pair arithmetic within the 1e-5 per-bead
+    bound (|dF| / max(|F_ref|, 1e-3 kcal/mol/A))."""
+    s = c4_system
+    print("[case] C4 lists and forces", flush=True)
+    o = corc.Oracle(s)
+    o.neighbor_list(1)
+    fo = o.force(1, 0)
+    Lo = o.lists(1)
+    e = Engine(s, mode="phases", precision=0, seed=SEED)
+    e.neighbor_list(0)
+    L = e.lists(1)
+    for k in ("wca", "ele"):
+        a, b = L[k], Lo[k]
+        assert np.array_equal(a[np.lexsort(a.T[::-1])], b[np.lexsort(b.T[::-1])]), k
+    assert np.array_equal(L["bp"], Lo["bp"]) and np.array_equal(L["bpcoef"], Lo["bpcoef"])
+    f64 = e.force(1, 0)
+    assert np.abs(f64 - fo).max() / np.abs(fo).max() < 1e-10
+    e32 = Engine(s, mode="phases", precision=1, seed=SEED)
+    e32.neighbor_list(0)
+    f32 = e32.force(1, 0)
+    rel = np.linalg.norm(f32 - fo, axis=1) / np.maximum(np.linalg.norm(fo, axis=1), 1e-3)
+    print(f"C4 FP32 force error vs oracle: max {rel.max():.3e} rms {np.sqrt((rel ** 2).mean()):.3e}", flush=True)
+    assert rel.max() <= 1e-5
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+def test_c4_persistent_trajectory_matches_oracle(c4_system, prec):
+    """C4 in the persistent kernel for 20 steps against the oracle on the
+    same counter-RNG draws."""
+    s = c4_system
+    print(f"[case] C4 persistent prec {prec}", flush=True)
+    e = Engine(s, mode="persistent", precision=prec, seed=SEED)
+    e.init_velocities()
+    e.neighbor_list(0)
+    e.step(20)
+    assert e.info()["mode"] == 2
+    o, _ = _oracle_run(s, SEED, 20, 0)
+    x, v, _ = e.get_state()
+    xo, vo, _ = o.state()
+    assert np.abs(x - xo).max() < TOL[prec]
+    assert np.abs(v - vo).max() < TOL[prec]

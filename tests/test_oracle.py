@@ -181,6 +181,36 @@ def test_counter_rng_normals_are_standard():
     assert abs(z.mean()) < 0.05 and abs(z.var() - 1.0) < 0.05
 
 
+@pytest.mark.parametrize("which", ["c2", "mixed_lengths"])
+def test_cell_enumeration_equals_reference_loops(which):
+    """The oracle's cell-list enumeration of large periodic systems gives
+    the O(N^2) loops' lists exactly: same pairs, same order (chain i, chain
+    j, i, j), same base-pair candidates and triplet coefficients, and the
+    same forces.  C2 at full size (9,400 beads), and 110 chains of 20-69 nt
+    in a 230 A box (4 cells per axis at the 55 A DH list cutoff)."""
+    if which == "c2":
+        s = configs.condensate_c2()
+    else:
+        rng = np.random.default_rng(4)
+        seqs = [configs.random_seq(rng, int(n), 0.6) for n in rng.integers(20, 70, size=110)]
+        box = 230.0
+        x = configs._place_chains(rng, seqs, box, clash=4.0)
+        s = configs._mk("mixed_pbc", seqs, x, configs.default_params(pbc=1, box=[box] * 3, nl_margin=5.0))
+    s["params"] = dict(s["params"], nstep_bp_mc=0)
+    assert int(np.sum(s["nmp_chain"])) >= 4096
+    out = []
+    for brute in (True, False):
+        o = corc.Oracle(s)
+        o.set_nl_brute(brute)
+        o.neighbor_list(1)
+        out.append((o.lists(1), o.force(1, 0)))
+    (la, fa), (lb, fb) = out
+    for k in ("wca", "ele", "bp", "bpcoef"):
+        assert np.array_equal(la[k], lb[k]), k
+    assert len(la["ele"]) > 1000 and len(la["bp"]) > 10
+    assert np.array_equal(fa, fb)
+
+
 def test_h4_index_rule_exercised():
     """mixed has chains of 24 and 40 nt: some bp_mp(6) entries follow the
     reference's comparison with the length of chain i
