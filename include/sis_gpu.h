@@ -91,12 +91,18 @@ typedef struct sis_options {
     int32_t precision;       /* 0: FP64 everywhere; 1: FP32 pair arithmetic, FP64 state */
     int32_t parity;          /* 1: consume injected MT19937 words as the reference does */
     int32_t mode;            /* 0 auto, 1 one launch per phase, 2 persistent, 3 batched */
-    int32_t block;           /* threads per workgroup, 0 = tuned default */
+    int32_t block;           /* batched form: threads per replica's workgroup, 128, 256 or 512
+                                (0 = 256); other values are refused */
     int32_t max_nbr;         /* neighbour slots per bead, 0 = grow on demand */
     int32_t rebuild_every;   /* 0: displacement trigger (src/job_md.F90:376-395) */
     int32_t exchange_every;  /* replica exchange cadence in steps, 0 = none */
     uint64_t seed;           /* counter-RNG seed for the performance mode */
+    int32_t grid;            /* persistent form: workgroups of the cooperative launch
+                                (0 = as many as fit on the device) */
+    int32_t flags;           /* bit0 SIS_OPT_HOST_EXCHANGE: batched runs exchange on the
+                                host between segments (no device exchange, no graph) */
 } sis_options;
+#define SIS_OPT_HOST_EXCHANGE 1
 
 typedef struct sis_gpu_ctx sis_gpu_ctx;
 
@@ -416,7 +422,9 @@ int sis_dcd_write_file(const char *path, int32_t nmp, int32_t nframes, const dou
 
 /* Counters: out[0] nmp, [1] nrep, [2] istep, [3] rebuilds, [4] nwca,
  * [5] nele, [6] nbp (replica 1), [7] neighbour slots per bead,
- * [8] launch mode used, [9] workgroup size. */
+ * [8] launch mode used, [9] workgroup size (batched form: threads per
+ * replica; persistent form: workgroups), [10] exchanges done on the
+ * device. */
 int sis_gpu_get_info(sis_gpu_ctx *ctx, int64_t *out, int32_t n);
 /* Device time (ms, HIP events on the context stream) of the last call of
  * a kind: 0 = step, 1 = rebuild, 2 = force; 3 = mean duration of one

@@ -294,9 +294,11 @@ class SisOptions(C.Structure):
         ("precision", C.c_int32),      # 0 = FP64 pair math, 1 = FP32 pair math
         ("parity", C.c_int32),         # 1 = consume an injected MT word tape
         ("mode", C.c_int32),           # 0 = auto, 1 = per-kernel, 2 = persistent, 3 = batched
-        ("block", C.c_int32),          # threads per workgroup (0 = auto)
+        ("block", C.c_int32),          # batched form: threads per replica (0 = 256; 128/256/512)
         ("max_nbr", C.c_int32),        # neighbour slots per bead (0 = auto)
         ("rebuild_every", C.c_int32),  # 0 = displacement trigger as the reference
         ("exchange_every", C.c_int32),  # replica exchange cadence (0 = none)
         ("seed", C.c_uint64),
+        ("grid", C.c_int32),           # persistent form: workgroups (0 = as many as fit)
+        ("flags", C.c_int32),          # bit0: batched exchanges on the host
     ]

@@ -15,6 +15,7 @@
 #include "sis_device.h"
 #include "sis_kernels.h"
 #include "sis_persistent.h"
+#include "sis_exchange.h"
 #include "sis_external.h"
 
 using namespace sis;
@@ -205,6 +206,13 @@ struct sis_gpu_ctx {
   int nblk_force = 0;
   int block = 256;
   int dev = 0;   // the HIP device current when the context was created
+  // device exchange of the batched form (sis_exchange.h)
+  DevBuf<int> dx_rep2lab_d, dx_lab2rep_d, dx_pair_tb_d, dx_idx_d, dx_has_d, dx_vsdiv_d, dx_nexch_d, dx_acc_d;
+  DevBuf<double> dx_labT_d, dx_labCoef_d, dx_labLam_d, dx_labCr_d, dx_oppT_d, dx_oppCoef_d, dx_oppLam_d,
+      dx_energy_d, dx_vsf_d;
+  hipGraphExec_t dx_graph = nullptr;
+  int dx_graph_key[4] = {0, 0, 0, 0};   // block, precision, exchange cadence, dynamic LDS of the captured graph
+  int64_t dev_exchanges = 0;            // exchanges done on the device
 
   Ptrs ptrs() {
     Ptrs d{};
@@ -279,6 +287,13 @@ struct sis_gpu_ctx {
     bb_status_d.free_(); bb_status_mc_d.free_();
     bp_key2_d.free_();
     sort_tmp_d.free_();
+    DevBuf<int> *dxi[] = {&dx_rep2lab_d, &dx_lab2rep_d, &dx_pair_tb_d, &dx_idx_d, &dx_has_d, &dx_vsdiv_d,
+                          &dx_nexch_d, &dx_acc_d};
+    for (auto *b : dxi) b->free_();
+    DevBuf<double> *dxd[] = {&dx_labT_d, &dx_labCoef_d, &dx_labLam_d, &dx_labCr_d, &dx_oppT_d, &dx_oppCoef_d,
+                             &dx_oppLam_d, &dx_energy_d, &dx_vsf_d};
+    for (auto *b : dxd) b->free_();
+    if (dx_graph) (void)hipGraphExecDestroy(dx_graph);
     for (auto e : tev) (void)hipEventDestroy(e);
     if (ev0) (void)hipEventDestroy(ev0);
     if (ev1) (void)hipEventDestroy(ev1);
@@ -1046,6 +1061,10 @@ int sis_gpu_create(const sis_topo *topo, const sis_params *prm, const sis_option
   sis_options o{};
   if (opt) o = *opt;
   if (o.nrep <= 0) o.nrep = 1;
+  if (o.block != 0 && o.block != 128 && o.block != 256 && o.block != 512)
+    return fail(SIS_ERR_ARG, "block: threads per replica of the batched form, 128, 256 or 512 (0 = 256)");
+  if (o.grid < 0) return fail(SIS_ERR_ARG, "grid must be >= 0");
+  if (o.flags & ~SIS_OPT_HOST_EXCHANGE) return fail(SIS_ERR_ARG, "unknown option flags");
   int sum = 0;
   for (int c = 0; c < topo->nchains; c++) {
     if (topo->nmp_chain[c] < 1) return fail(SIS_ERR_ARG, "chain of zero length");
@@ -1674,7 +1693,15 @@ int sis_gpu_step(sis_gpu_ctx *c, int64_t nsteps, int64_t *rebuilds) {
   const int64_t sb = (c->bpo && c->bpo->nstep_save > 0) ? c->bpo->nstep_save : 0;
   const int64_t so = (c->outf && c->outf->nstep_save > 0) ? c->outf->nstep_save : 0;
   const int64_t rs = (c->rst && c->rst->nstep_save > 0) ? c->rst->nstep_save : 0;
+  // whole segments of a batched replica-exchange run with nothing else at
+  // their ends: exchange on the device, segments replayed as a HIP graph
+  const bool dev_ex = mode == 3 && !c->parity && ex && device_exchange_ok(c) && !sv && !sb && !so && !rs &&
+                      !c->rz_on && !c->twz;
   for (int64_t left = nsteps; left > 0;) {
+    if (dev_ex && c->istep % ex == 0 && left >= ex) {
+      left -= batched_device_run(c, left / ex);
+      continue;
+    }
     const int64_t rz = c->rz_on ? c->rz_step : 0;
     // .twz lines are taken before a step whose count is a multiple of the
     // cadence (the force of that step writes them)
@@ -2072,14 +2099,14 @@ int sis_gpu_get_info(sis_gpu_ctx *c, int64_t *out, int32_t n) {
   if (!c || !out) return fail(SIS_ERR_ARG, "null argument");
   int mode_used = c->parity ? 1 : (c->mode ? c->mode : choose_mode(c));
   if ((mode_used == 3 && !batched_ok(c)) || (mode_used == 2 && (c->R != 1 || c->ext_on))) mode_used = 1;   // forms fall back
-  int64_t v[10] = {c->N, c->R, c->istep, c->rebuilds, 0, 0, 0, c->D.maxnb, mode_used, c->block};
+  int64_t v[11] = {c->N, c->R, c->istep, c->rebuilds, 0, 0, 0, c->D.maxnb, mode_used, c->block, c->dev_exchanges};
   if (c->lists_valid) {
     int64_t k;
     if (sis_gpu_get_pair_count(c, 1, 0, &k) == SIS_OK) v[4] = k;
     if (sis_gpu_get_pair_count(c, 1, 1, &k) == SIS_OK) v[5] = k;
     if (sis_gpu_get_pair_count(c, 1, 2, &k) == SIS_OK) v[6] = k;
   }
-  for (int k = 0; k < n && k < 10; k++) out[k] = v[k];
+  for (int k = 0; k < n && k < 11; k++) out[k] = v[k];
   return SIS_OK;
   API_END
 }

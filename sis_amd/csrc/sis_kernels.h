@@ -1443,12 +1443,87 @@ __global__ void k_scale_vel(Params P, Ptrs D, int r, double f, int divide) {
 }
 
 // ---------------------------------------------------------------------
-// Energies (src/energy_sumup.F90 and the terms): per-block partial sums,
-// reduced in a fixed order.  Terms: 1 bond, 2 angle, 3 dihedral, 4 BP,
-// 5 WCA, 6 DH.  Each term is counted once (first bead of a bonded term,
-// lower bead of a pair, candidate owner for BP).  r_only >= 0: one
-// replica, grid.y = 1, its DH coefficient and Debye length by value
-// (coef1, lam1) and its partial sums in slot 0 of `partial`.
+// Energy terms of bead a of replica r into e[1..6] (src/energy_sumup.F90
+// and the terms): 1 bond, 2 angle, 3 dihedral, 4 BP, 5 WCA, 6 DH.  Each
+// term is counted once: on the first bead of a bonded term, the lower bead
+// of a pair, the owner (role 0) of a base-pair candidate.  coef / rlam: the
+// DH coefficient and inverse Debye length to use; bp_masked: sum only the
+// pairs that survived the limit MC (src/energy_bp_limit_triplet.F90:195),
+// else all formed ones (src/energy_bp_triplet.F90:88).
+__device__ __forceinline__ void bead_energy(const Params &P, const Ptrs &D, int r, int a, double coef,
+                                            double rc, int bp_masked, const double4 *x, double e[7]) {
+  const int N = P.N;
+  const double *box = D.box + 3 * r;
+  Topo ta = D.topo[a];
+  int loc = ta.loc, n = ta.len, st = ta.start;
+  bool circ = topo_circ(ta);
+  auto at = [&](int k) { int l = loc + k; if (l > n) l -= n; return st + l - 1; };
+  // bond starting at this bead
+  if (circ || loc + 1 <= n) {
+    V3<double> v = pvec(P, box, x[at(0)], x[at(1)]);
+    double d = sqrt(dot(v, v));
+    e[1] += 0.5 * P.bond_k * ((d - P.bond_r0) * (d - P.bond_r0));
+  }
+  if ((circ && n >= 3) || loc + 2 <= n) {
+    V3<double> v12 = pvec(P, box, x[at(0)], x[at(1)]), v32 = pvec(P, box, x[at(2)], x[at(1)]);
+    double co = dot(v32, v12) / sqrt(dot(v12, v12) * dot(v32, v32));
+    if (P.angl_reb) {
+      e[2] += 0.5 * P.angl_k * ((co - P.angl_co0) * (co - P.angl_co0)) / (1.0 - co * co);
+    } else {
+      if (co > 1.0) co = 1.0; else if (co < -1.0) co = -1.0;
+      double t = acos(co);
+      e[2] += 0.5 * P.angl_k * ((t - P.angl_t0) * (t - P.angl_t0));
+    }
+  }
+  if (P.dih_type && ((circ && n >= 4) || loc + 3 <= n)) {
+    V3<double> v12 = pvec(P, box, x[at(1)], x[at(0)]), v23 = pvec(P, box, x[at(2)], x[at(1)]),
+               v34 = pvec(P, box, x[at(3)], x[at(2)]);
+    V3<double> c1 = cross(v12, v23), c2 = cross(v23, v34);
+    double dih = atan2(sqrt(dot(v23, v23)) * dot(v12, c2), dot(c1, c2));
+    if (P.dih_type == 2) e[3] -= P.dih_k * exp(-0.5 * P.dih_w * ((dih - P.dih_p0) * (dih - P.dih_p0)));
+    else e[3] += P.dih_k * (1.0 + cos(dih + P.dih_p0));
+  }
+  size_t g = (size_t)r * N + a;
+  {
+    int ni = min(D.n_inc[g], D.maxinc);
+    const int *inc = D.inc + (size_t)r * D.maxinc * N + a;
+    for (int k = 0; k < ni; k++) {
+      int q = inc[(size_t)k * N];
+      if ((q & 7) == 0 && (D.bp_status[q >> 3] || !bp_masked)) e[4] += D.ene_bp[q >> 3];
+    }
+  }
+  {
+    int nw = min(D.n_wca[g], D.maxw);
+    const int *nb = D.nb_wca + (size_t)r * D.maxw * N + a;
+    for (int k = 0; k < nw; k++) {
+      int b = nb[(size_t)k * N];
+      if (b < a) continue;
+      V3<double> v = pvec(P, box, x[a], x[b]);
+      double d = sqrt(dot(v, v));
+      if (d >= P.wca_sigma) continue;
+      d = P.wca_sigma / d;
+      double d6 = d * d * d * d * d * d;
+      e[5] += P.wca_eps * (d6 * d6 - 2 * d6 + 1.0);
+    }
+  }
+  if (P.ele_on) {
+    int ne = min(D.n_ele[g], D.maxnb);
+    const int *nb = D.nb_ele + (size_t)r * D.maxnb * N + a;
+    for (int k = 0; k < ne; k++) {
+      int b = nb[(size_t)k * N];
+      if (b < a) continue;
+      V3<double> v = pvec(P, box, x[a], x[b]);
+      double dist = sqrt(dot(v, v));
+      if (dist > D.ele_cut[r]) continue;
+      e[6] += coef / dist * exp(-dist * rc);
+    }
+  }
+}
+
+// Energies (src/energy_sumup.F90): per-block partial sums of bead_energy,
+// reduced in a fixed order.  r_only >= 0: one replica, grid.y = 1, its DH
+// coefficient and Debye length by value (coef1, lam1) and its partial
+// sums in slot 0 of `partial`.
 template <int B>
 __global__ void __launch_bounds__(B) k_energy(Params P, Ptrs D, const double *ecT,
                                               const double *eCoef, const double *eLam,
@@ -1460,77 +1535,9 @@ __global__ void __launch_bounds__(B) k_energy(Params P, Ptrs D, const double *ec
   const int N = P.N;
   double e[7] = {0, 0, 0, 0, 0, 0, 0};
   if (a < N && !(mask && !mask[r])) {
-    const double *box = D.box + 3 * r;
-    const double4 *x = D.pos + (size_t)r * N;
-    Topo ta = D.topo[a];
-    int loc = ta.loc, n = ta.len, st = ta.start;
-    bool circ = topo_circ(ta);
-    auto at = [&](int k) { int l = loc + k; if (l > n) l -= n; return st + l - 1; };
-    // bond starting at this bead
-    if (circ || loc + 1 <= n) {
-      V3<double> v = pvec(P, box, x[at(0)], x[at(1)]);
-      double d = sqrt(dot(v, v));
-      e[1] += 0.5 * P.bond_k * ((d - P.bond_r0) * (d - P.bond_r0));
-    }
-    if ((circ && n >= 3) || loc + 2 <= n) {
-      V3<double> v12 = pvec(P, box, x[at(0)], x[at(1)]), v32 = pvec(P, box, x[at(2)], x[at(1)]);
-      double co = dot(v32, v12) / sqrt(dot(v12, v12) * dot(v32, v32));
-      if (P.angl_reb) {
-        e[2] += 0.5 * P.angl_k * ((co - P.angl_co0) * (co - P.angl_co0)) / (1.0 - co * co);
-      } else {
-        if (co > 1.0) co = 1.0; else if (co < -1.0) co = -1.0;
-        double t = acos(co);
-        e[2] += 0.5 * P.angl_k * ((t - P.angl_t0) * (t - P.angl_t0));
-      }
-    }
-    if (P.dih_type && ((circ && n >= 4) || loc + 3 <= n)) {
-      V3<double> v12 = pvec(P, box, x[at(1)], x[at(0)]), v23 = pvec(P, box, x[at(2)], x[at(1)]),
-                 v34 = pvec(P, box, x[at(3)], x[at(2)]);
-      V3<double> c1 = cross(v12, v23), c2 = cross(v23, v34);
-      double dih = atan2(sqrt(dot(v23, v23)) * dot(v12, c2), dot(c1, c2));
-      if (P.dih_type == 2) e[3] -= P.dih_k * exp(-0.5 * P.dih_w * ((dih - P.dih_p0) * (dih - P.dih_p0)));
-      else e[3] += P.dih_k * (1.0 + cos(dih + P.dih_p0));
-    }
-    size_t g = (size_t)r * N + a;
-    // BP: candidates owned by this bead (role 0)
-    {
-      int ni = min(D.n_inc[g], D.maxinc);
-      const int *inc = D.inc + (size_t)r * D.maxinc * N + a;
-      for (int k = 0; k < ni; k++) {
-        int q = inc[(size_t)k * N];
-        // masked sum after the limit MC (src/energy_bp_limit_triplet.F90:195),
-        // plain sum otherwise (src/energy_bp_triplet.F90:88)
-        if ((q & 7) == 0 && (D.bp_status[q >> 3] || !bp_masked)) e[4] += D.ene_bp[q >> 3];
-      }
-    }
-    {
-      int nw = min(D.n_wca[g], D.maxw);
-      const int *nb = D.nb_wca + (size_t)r * D.maxw * N + a;
-      for (int k = 0; k < nw; k++) {
-        int b = nb[(size_t)k * N];
-        if (b < a) continue;
-        V3<double> v = pvec(P, box, x[a], x[b]);
-        double d = sqrt(dot(v, v));
-        if (d >= P.wca_sigma) continue;
-        d = P.wca_sigma / d;
-        double d6 = d * d * d * d * d * d;
-        e[5] += P.wca_eps * (d6 * d6 - 2 * d6 + 1.0);
-      }
-    }
-    if (P.ele_on) {
-      int ne = min(D.n_ele[g], D.maxnb);
-      const int *nb = D.nb_ele + (size_t)r * D.maxnb * N + a;
-      double coef = r_only >= 0 ? coef1 : (eCoef ? eCoef[r] : D.ele_coef[r]);
-      double rc = 1.0 / (r_only >= 0 ? lam1 : (eLam ? eLam[r] : D.lambdaD[r]));
-      for (int k = 0; k < ne; k++) {
-        int b = nb[(size_t)k * N];
-        if (b < a) continue;
-        V3<double> v = pvec(P, box, x[a], x[b]);
-        double dist = sqrt(dot(v, v));
-        if (dist > D.ele_cut[r]) continue;
-        e[6] += coef / dist * exp(-dist * rc);
-      }
-    }
+    double coef = r_only >= 0 ? coef1 : (eCoef ? eCoef[r] : D.ele_coef[r]);
+    double rc = 1.0 / (r_only >= 0 ? lam1 : (eLam ? eLam[r] : D.lambdaD[r]));
+    bead_energy(P, D, r, a, coef, rc, bp_masked, D.pos + (size_t)r * N, e);
   }
   (void)ecT;
   for (int k = 1; k < 7; k++) s[k][threadIdx.x] = e[k];

@@ -325,13 +325,22 @@ struct SysIO {
   int xlds;                     // beads whose positions fit the launch's dynamic LDS
   int fplds;                    // per-lane partial forces that fit after them
   int force_only;               // evaluate the forces of the current positions into D.frc, no update
+  // energy mode: one evaluation of energy_sumup at this system's D.tK (BP
+  // terms and the limit MC at that temperature), DH coefficient e_coef and
+  // Debye length e_lam, into energy_out[0..15]; nothing moves, the MC
+  // state of the steps (bp_status_mc) is left alone
+  int energy_mode;
+  double e_coef, e_lam;
+  double *energy_out;
+  int64_t *stepno_out;          // istep after the launch (device-driven segments), or null
+  int *mc_first_out;            // cleared after the launch, or null
 };
 
 // Launch arguments of k_batched (arrays over replicas).
 struct BatchArgs {
   const int *todo;
-  const int64_t *stepno;
-  const int *mc_first;
+  int64_t *stepno;
+  int *mc_first;
   uint64_t seed;
   int *seg_n, *seg, *has_exc;
   int *done, *nbp, *rebuild, *nreb;
@@ -340,6 +349,14 @@ struct BatchArgs {
   int xlds;   // beads whose positions fit the dynamic LDS of the launch
   int fplds;  // per-lane partial forces that fit after them
   int force_only;   // one force evaluation (limit MC if mc_first) into D.frc, no update
+  // energy mode (one energy_sumup per replica, see SysIO): DH coefficient
+  // and Debye length per replica (null: the replica's own), output
+  // energy[r * 2 + energy_slot], and replicas to skip (their output 0)
+  int energy_mode, energy_slot;
+  const double *e_coef, *e_lam;
+  const int *e_use;
+  double *energy;
+  int write_back;   // write istep and clear mc_first on the device after the launch
 };
 
 // Base-pair candidate arrays of one layout.
@@ -571,7 +588,7 @@ __device__ __forceinline__ void batched_body(const Params &P, const Ptrs &D, con
   // the first step after a rebuild does the limit MC (src/job_md.F90:388-395)
   int mc_next = J.mc_first;
   int nreb = 0;
-  if (J.todo > 0 && *J.rebuild && !J.force_only) {
+  if (J.todo > 0 && *J.rebuild && !J.force_only && !J.energy_mode) {
     batched_rebuild<B>(P, D, J, r, s_scan, s_tile, &s_flag);
     if (tid == 0) *J.rebuild = 0;
     mc_next = 1;
@@ -582,9 +599,12 @@ __device__ __forceinline__ void batched_body(const Params &P, const Ptrs &D, con
   int overflow = s_flag;
   const int64_t nsteps = J.todo, step0 = J.stepno;
   for (int64_t s = 0; s < nsteps && !overflow; s++) {
-    const int64_t step = step0 + s + 1;
-    const int mc = J.force_only ? (P.nstep_bp_mc > 0 && J.mc_first)
-                                : P.nstep_bp_mc > 0 && ((step % P.nstep_bp_mc) == 0 || mc_next);
+    // the energy evaluation keys its MC draws by the current step, as the
+    // host's energy_sumup after that step does
+    const int64_t step = J.energy_mode ? step0 : step0 + s + 1;
+    const int mc = J.energy_mode ? P.nstep_bp_mc > 0
+                   : J.force_only ? (P.nstep_bp_mc > 0 && J.mc_first)
+                                  : P.nstep_bp_mc > 0 && ((step % P.nstep_bp_mc) == 0 || mc_next);
     mc_next = 0;
     const int nbr = *J.nbp;
     // ---- A: base pairs (+ union of formed pairs on MC steps); bonded
@@ -774,7 +794,42 @@ __device__ __forceinline__ void batched_body(const Params &P, const Ptrs &D, con
         __syncthreads();
       }
       // the MC result is what later non-MC steps evaluate
-      for (int k = tid; k < nbr; k += B) D.bp_status_mc[r * J.capb + k] = D.bp_status[r * J.capb + k];
+      if (!J.energy_mode)
+        for (int k = tid; k < nbr; k += B) D.bp_status_mc[r * J.capb + k] = D.bp_status[r * J.capb + k];
+    }
+    if (J.energy_mode) {
+      // energy_sumup: per-bead terms summed as k_energy and k_energy_reduce
+      // do (blocks of 256 beads, tree sum, blocks in order), so the result
+      // equals the per-phase evaluation's
+      // (128 threads: each holds beads t and t + 128, the tree's first level)
+      static_assert(B == 128 || B >= 256, "energy sums need 128 or at least 256 threads");
+      constexpr int H = B < 256 ? B : 256;
+      __syncthreads();
+      double acc[7] = {0, 0, 0, 0, 0, 0, 0};
+      for (int a0 = 0; a0 < N; a0 += 256) {
+        double e[7] = {0, 0, 0, 0, 0, 0, 0}, e2[7] = {0, 0, 0, 0, 0, 0, 0};
+        if (tid < H && a0 + tid < N) bead_energy(P, D, r, a0 + tid, J.e_coef, 1.0 / J.e_lam, mc, xr, e);
+        if (B < 256 && tid < H && a0 + tid + 128 < N)
+          bead_energy(P, D, r, a0 + tid + 128, J.e_coef, 1.0 / J.e_lam, mc, xr, e2);
+        for (int k = 1; k < 7; k++) {
+          if (tid < H) s1[tid] = B < 256 ? e[k] + e2[k] : e[k];
+          __syncthreads();
+          for (int off = H / 2; off > 0; off >>= 1) {
+            if (tid < off) s1[tid] += s1[tid + off];
+            __syncthreads();
+          }
+          if (tid == 0) acc[k] += s1[0];
+          __syncthreads();
+        }
+      }
+      if (tid == 0) {
+        double t = 0.0;
+        for (int k = 1; k <= 6; k++) t += acc[k];
+        acc[0] = t;
+        for (int k = 0; k < 7; k++) J.energy_out[k] = acc[k];
+        for (int k = 7; k < 16; k++) J.energy_out[k] = 0.0;
+      }
+      break;
     }
     // ---- D: base-pair forces onto the partial pair forces (or the whole
     // pair sum when those did not fit in LDS), bonded, GJF-2GJ, per-step
@@ -845,10 +900,12 @@ __device__ __forceinline__ void batched_body(const Params &P, const Ptrs &D, con
       }
     }
   }
-  if (tid == 0) {
+  if (tid == 0 && !J.energy_mode && !J.force_only) {
     *J.done = (int)completed;
-    *J.nreb = nreb;
+    *J.nreb = J.stepno_out ? *J.nreb + nreb : nreb;   // device-driven segments add up
     if (overflow) D.flags[0] = 1;
+    if (J.stepno_out) *J.stepno_out = step0 + completed;
+    if (J.mc_first_out) *J.mc_first_out = 0;
   }
 }
 
@@ -874,6 +931,21 @@ __global__ void __launch_bounds__(B) k_batched(Params P, Ptrs D, BatchArgs A) {
   J.nbp = A.nbp + r;
   J.rebuild = A.rebuild + r;
   J.nreb = A.nreb + r;
+  J.energy_mode = A.energy_mode;
+  J.e_coef = A.e_coef ? A.e_coef[r] : D.ele_coef[r];
+  J.e_lam = A.e_lam ? A.e_lam[r] : D.lambdaD[r];
+  J.energy_out = A.energy ? A.energy + ((size_t)r * 2 + A.energy_slot) * 16 : nullptr;
+  J.stepno_out = A.write_back ? A.stepno + r : nullptr;
+  J.mc_first_out = A.write_back ? A.mc_first + r : nullptr;
+  if (A.energy_mode || A.write_back) {
+    // a device-driven sequence: a capacity overflow earlier in it stops
+    // every later launch (the host finishes the sequence on its own path)
+    if (D.flags[0]) return;
+    if (A.energy_mode && A.e_use && !A.e_use[r]) {
+      if (threadIdx.x < 16) J.energy_out[threadIdx.x] = 0.0;
+      return;
+    }
+  }
   batched_body<T, B>(P, D, J, r);   // D.pos_out is the second position buffer
 }
 
@@ -937,6 +1009,12 @@ __global__ void __launch_bounds__(B) k_jobs(Params P, Ptrs D0, const JobView *__
   J.xlds = xlds;
   J.fplds = fplds;
   J.force_only = 0;
+  J.energy_mode = 0;
+  J.e_coef = 0.0;
+  J.e_lam = 1.0;
+  J.energy_out = nullptr;
+  J.stepno_out = nullptr;
+  J.mc_first_out = nullptr;
   J.done = V.io;
   J.nbp = V.io + 1;
   J.rebuild = V.io + 2;

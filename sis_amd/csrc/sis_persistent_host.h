@@ -25,7 +25,7 @@ int persistent_grid(sis_gpu_ctx *c, const void *kfn, int B) {
   int want = (c->N * 8 + B - 1) / B;               // one 8-lane group per bead
   int cap = per * ncu;                               // co-resident limit
   int g = std::min(want, cap);
-  if (c->opt.block > 0) g = std::min(cap, std::max(1, c->opt.block));
+  if (c->opt.grid > 0) g = std::min(cap, c->opt.grid);
   return std::max(g, 1);
 }
 
@@ -216,9 +216,14 @@ void ensure_batched_io(sis_gpu_ctx *c) {
   }
 }
 
+// threads per replica's workgroup: 256 unless the options ask for 128 or 512
+int batched_block(sis_gpu_ctx *c) { return c->opt.block == 128 || c->opt.block == 512 ? c->opt.block : 256; }
+
 const void *batched_kernel(sis_gpu_ctx *c, int B) {
-  return B == 512 ? (c->precision == 1 ? (const void *)k_batched<float, 512> : (const void *)k_batched<double, 512>)
-                  : (c->precision == 1 ? (const void *)k_batched<float, 256> : (const void *)k_batched<double, 256>);
+  const bool f = c->precision == 1;
+  if (B == 512) return f ? (const void *)k_batched<float, 512> : (const void *)k_batched<double, 512>;
+  if (B == 128) return f ? (const void *)k_batched<float, 128> : (const void *)k_batched<double, 128>;
+  return f ? (const void *)k_batched<float, 256> : (const void *)k_batched<double, 256>;
 }
 
 // arguments of one k_batched launch over the batched layout
@@ -263,7 +268,7 @@ Ptrs batched_ptrs(sis_gpu_ctx *c) {
 // frc_d.  Lists must be valid; nothing moves.
 void batched_force(sis_gpu_ctx *c, int irep0, int bp_mc) {
   const int R = c->R;
-  const int B = c->opt.block == 512 ? 512 : 256;
+  const int B = batched_block(c);
   ensure_batched_io(c);
   to_batched(c);
   std::vector<int> todo(R, 0), mcf(R, 0);
@@ -288,70 +293,247 @@ void batched_force(sis_gpu_ctx *c, int irep0, int bp_mc) {
 // (up to the next exchange).  Lists move to the batched layout for the
 // segment and back afterwards, so skin counters, pending rebuilds and the
 // limit-MC state carry over exactly as in the per-phase form.
+namespace {
+
+// Runs the batched kernel over the batched layout until every replica has
+// done its `todo` steps (from `stepno`), growing lists through host
+// rebuilds when an in-kernel rebuild outgrew a capacity.  mcf: replicas
+// whose first step does the limit MC.
+void batched_segment(sis_gpu_ctx *c, int64_t seg, std::vector<int> todo, std::vector<int64_t> stepno,
+                     std::vector<int> mcf) {
+  const int N = c->N, R = c->R;
+  const int B = batched_block(c);
+  const int lpb = lanes_per_bead(N);
+  const void *kfn = batched_kernel(c, B);
+  std::vector<int> done(R);
+  for (int r = 0; r < R; r++) done[r] = (int)(seg - todo[r]);
+  for (int attempt = 0;; attempt++) {
+    HIPCHK(hipMemcpyAsync(c->bb_todo_d.p, todo.data(), R * sizeof(int), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(c->bb_stepno_d.p, stepno.data(), R * sizeof(int64_t), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(c->bb_mcfirst_d.p, mcf.data(), R * sizeof(int), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemsetAsync(c->flags_d.p, 0, 8 * sizeof(int), c->stream));
+    BatchArgs A = batched_args(c, lpb);
+    Params P = c->P;
+    Ptrs D = batched_ptrs(c);
+    void *args[] = {&P, &D, &A};
+    c->tmark();
+    HIPCHK(hipLaunchKernel(kfn, dim3(R), dim3(B), args, (size_t)(A.xlds + A.fplds) * sizeof(double4), c->stream));
+    c->tmark();
+    std::vector<int> d(R), nr(R), f(8);
+    HIPCHK(hipMemcpyAsync(d.data(), c->bb_done_d.p, R * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpyAsync(nr.data(), c->bb_nreb_d.p, R * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpyAsync(f.data(), c->flags_d.p, 8 * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (f[7]) throw ApiError{SIS_ERR_STATE, "limit MC met a base-pair candidate outside the system"};
+    if (f[1] == 2) throw ApiError{SIS_ERR_STATE, "limit MC found an excess nucleotide without pairs"};
+    if (f[5]) throw ApiError{SIS_ERR_CAPACITY, "a base-pair conflict component outgrew the limit-MC scratch"};
+    bool lag = false;
+    std::fill(mcf.begin(), mcf.end(), 0);
+    std::vector<int> lagm(R, 0);
+    for (int r = 0; r < R; r++) {
+      done[r] += d[r];
+      stepno[r] += d[r];
+      todo[r] = (int)(seg - done[r]);
+      c->rebuilds += nr[r];
+      if (todo[r] > 0) { lag = true; lagm[r] = 1; }
+    }
+    if (!lag) break;
+    if (attempt == 7) throw ApiError{SIS_ERR_CAPACITY, "batched lists did not converge in size"};
+    // an in-kernel rebuild outgrew a capacity: rebuild the lagging
+    // replicas on the per-phase path (which grows every list), then let
+    // them finish the segment
+    to_packed(c);
+    mcf = host_rebuild(c, lagm);
+    to_batched(c);
+  }
+}
+
+// noise amplitude c_r of bead i at temperature T (set_md_coef,
+// src/job_md.F90:700-767); the formula upload_conditions applies
+double md_cr_at(sis_gpu_ctx *c, double T, int i) {
+  if (c->P.bd) return std::sqrt(2.0 * KB_KCAL * T * c->prm.dt / c->fric[i]);
+  const double c1 = 0.5 * c->prm.dt * c->fric[i] / c->mass[i];
+  const double c2 = std::sqrt(1.0 / (1.0 + c1));
+  return 0.5 * c2 / c->mass[i] * std::sqrt(2.0 * c->fric[i] * KB_KCAL * T * c->prm.dt);
+}
+
+}  // namespace
+
+// Batched form: one workgroup per replica runs whole segments of steps
+// (up to the next exchange).  Lists move to the batched layout for the
+// segment and back afterwards, so skin counters, pending rebuilds and the
+// limit-MC state carry over exactly as in the per-phase form.
 void batched_steps(sis_gpu_ctx *c, int64_t nsteps) {
   if (!batched_ok(c)) {
     for (int64_t s = 0; s < nsteps; s++) step_phases(c);
     return;
   }
-  const int N = c->N, R = c->R;
-  // 256 threads (4 waves) per replica by default; 512 (8 waves) on request
-  const int B = c->opt.block == 512 ? 512 : 256;
+  const int R = c->R;
   std::vector<int> all(R, 1), mcf(R, 0);
   if (!c->lists_valid) mcf = host_rebuild(c, all);
   ensure_batched_io(c);
-  const int lpb = lanes_per_bead(N);
-  const void *kfn = batched_kernel(c, B);
-  c->block = B;
+  c->block = batched_block(c);
   int64_t left = nsteps;
   const int ex = c->opt.exchange_every;
   while (left > 0) {
     int64_t seg = left;
     if (ex > 0) seg = std::min<int64_t>(left, ex - (c->istep % ex));
-    std::vector<int> todo(R, (int)seg), done(R, 0);
-    std::vector<int64_t> stepno(R, c->istep);
     to_batched(c);
-    for (int attempt = 0;; attempt++) {
-      HIPCHK(hipMemcpyAsync(c->bb_todo_d.p, todo.data(), R * sizeof(int), hipMemcpyHostToDevice, c->stream));
-      HIPCHK(hipMemcpyAsync(c->bb_stepno_d.p, stepno.data(), R * sizeof(int64_t), hipMemcpyHostToDevice, c->stream));
-      HIPCHK(hipMemcpyAsync(c->bb_mcfirst_d.p, mcf.data(), R * sizeof(int), hipMemcpyHostToDevice, c->stream));
-      HIPCHK(hipMemsetAsync(c->flags_d.p, 0, 8 * sizeof(int), c->stream));
-      BatchArgs A = batched_args(c, lpb);
-      Params P = c->P;
-      Ptrs D = batched_ptrs(c);
-      void *args[] = {&P, &D, &A};
-      c->tmark();
-      HIPCHK(hipLaunchKernel(kfn, dim3(R), dim3(B), args, (size_t)(A.xlds + A.fplds) * sizeof(double4), c->stream));
-      c->tmark();
-      std::vector<int> d(R), nr(R), f(8);
-      HIPCHK(hipMemcpyAsync(d.data(), c->bb_done_d.p, R * sizeof(int), hipMemcpyDeviceToHost, c->stream));
-      HIPCHK(hipMemcpyAsync(nr.data(), c->bb_nreb_d.p, R * sizeof(int), hipMemcpyDeviceToHost, c->stream));
-      HIPCHK(hipMemcpyAsync(f.data(), c->flags_d.p, 8 * sizeof(int), hipMemcpyDeviceToHost, c->stream));
-      HIPCHK(hipStreamSynchronize(c->stream));
-      if (f[7]) throw ApiError{SIS_ERR_STATE, "limit MC met a base-pair candidate outside the system"};
-      if (f[1] == 2) throw ApiError{SIS_ERR_STATE, "limit MC found an excess nucleotide without pairs"};
-      if (f[5]) throw ApiError{SIS_ERR_CAPACITY, "a base-pair conflict component outgrew the limit-MC scratch"};
-      bool lag = false;
-      std::fill(mcf.begin(), mcf.end(), 0);
-      std::vector<int> lagm(R, 0);
-      for (int r = 0; r < R; r++) {
-        done[r] += d[r];
-        stepno[r] += d[r];
-        todo[r] = (int)(seg - done[r]);
-        c->rebuilds += nr[r];
-        if (todo[r] > 0) { lag = true; lagm[r] = 1; }
-      }
-      if (!lag) break;
-      if (attempt == 7) throw ApiError{SIS_ERR_CAPACITY, "batched lists did not converge in size"};
-      // an in-kernel rebuild outgrew a capacity: rebuild the lagging
-      // replicas on the per-phase path (which grows every list), then let
-      // them finish the segment
-      to_packed(c);
-      mcf = host_rebuild(c, lagm);
-      to_batched(c);
-    }
+    batched_segment(c, seg, std::vector<int>(R, (int)seg), std::vector<int64_t>(R, c->istep), mcf);
     to_packed(c);
     std::fill(mcf.begin(), mcf.end(), 0);
     c->istep += seg;
     left -= seg;
   }
+}
+
+// Whether whole batched segments can run with the exchange on the device:
+// a temperature or ionic-strength ladder of the batched form, nothing
+// else happening at segment ends.
+bool device_exchange_ok(sis_gpu_ctx *c) {
+  return batched_ok(c) && c->opt.exchange_every > 0 && c->R > 1 && !(c->opt.flags & SIS_OPT_HOST_EXCHANGE) &&
+         !c->anneal_on && !c->ext_on && !c->rg_remd && !c->dcf_remd;
+}
+
+// nseg whole segments of the batched form with energy_replica and
+// replica_exchange on the device (sis_exchange.h): one segment = the
+// steps launch, the counterpart and own-condition energy launches, the
+// Metropolis sweep and the rescale, captured once as a HIP graph and
+// replayed nseg times with no host synchronisation in between.  Returns
+// the steps done (nseg * cadence).  If an in-kernel rebuild outgrows a
+// capacity, the rest of the graph does nothing; the host finishes that
+// segment and its exchange on its own path and returns early.
+int64_t batched_device_run(sis_gpu_ctx *c, int64_t nseg) {
+  const int N = c->N, R = c->R, ex = c->opt.exchange_every;
+  const int B = batched_block(c);
+  const int lpb = lanes_per_bead(N);
+  std::vector<int> all(R, 1), mcf(R, 0);
+  if (!c->lists_valid) mcf = host_rebuild(c, all);
+  ensure_batched_io(c);
+  c->block = B;
+  // label tables from the host's own formulas
+  auto need = [](auto &b, size_t n) { if (b.n < n) b.alloc(n); };
+  need(c->dx_rep2lab_d, R + 1); need(c->dx_lab2rep_d, R + 1); need(c->dx_pair_tb_d, 2 * (size_t)(R + 2));
+  need(c->dx_idx_d, 1); need(c->dx_has_d, R); need(c->dx_vsdiv_d, R); need(c->dx_nexch_d, 1);
+  need(c->dx_acc_d, R / 2 + 1);
+  need(c->dx_labT_d, R + 1); need(c->dx_labCoef_d, R + 1); need(c->dx_labLam_d, R + 1);
+  need(c->dx_labCr_d, (size_t)(R + 1) * N); need(c->dx_oppT_d, R); need(c->dx_oppCoef_d, R);
+  need(c->dx_oppLam_d, R); need(c->dx_energy_d, (size_t)R * 32); need(c->dx_vsf_d, R);
+  std::vector<double> lT(R + 1, 0.0), lC(R + 1, 0.0), lL(R + 1, 1.0), lCr((size_t)(R + 1) * N, 0.0);
+  for (int L = 1; L <= R; L++) {
+    double cut;
+    lT[L] = c->lab2val[L];
+    host_set_ele(c->prm, lT[L], c->lab2ion[L], lC[L], lL[L], cut);
+    for (int i = 0; i < N; i++) lCr[(size_t)L * N + i] = md_cr_at(c, lT[L], i);
+  }
+  const int idx0 = c->idx_pair, zero = 0;
+  HIPCHK(hipMemcpyAsync(c->dx_labT_d.p, lT.data(), lT.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(c->dx_labCoef_d.p, lC.data(), lC.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(c->dx_labLam_d.p, lL.data(), lL.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(c->dx_labCr_d.p, lCr.data(), lCr.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(c->dx_rep2lab_d.p, c->rep2lab.data(), (R + 1) * sizeof(int), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(c->dx_lab2rep_d.p, c->lab2rep.data(), (R + 1) * sizeof(int), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(c->dx_pair_tb_d.p, c->pair_tb.data(), c->pair_tb.size() * sizeof(int), hipMemcpyHostToDevice,
+                        c->stream));
+  HIPCHK(hipMemcpyAsync(c->dx_idx_d.p, &idx0, sizeof(int), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(c->dx_nexch_d.p, &zero, sizeof(int), hipMemcpyHostToDevice, c->stream));
+  to_batched(c);
+  std::vector<int> todo(R, ex);
+  std::vector<int64_t> stepno(R, c->istep);
+  HIPCHK(hipMemcpyAsync(c->bb_todo_d.p, todo.data(), R * sizeof(int), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(c->bb_stepno_d.p, stepno.data(), R * sizeof(int64_t), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(c->bb_mcfirst_d.p, mcf.data(), R * sizeof(int), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemsetAsync(c->bb_nreb_d.p, 0, R * sizeof(int), c->stream));
+  HIPCHK(hipMemsetAsync(c->flags_d.p, 0, 8 * sizeof(int), c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));   // the host tables above are released after this
+  ExchDev X{};
+  X.R = R; X.N = N;
+  X.rep2lab = c->dx_rep2lab_d.p; X.lab2rep = c->dx_lab2rep_d.p; X.pair_tb = c->dx_pair_tb_d.p;
+  X.idx_pair = c->dx_idx_d.p;
+  X.lab_T = c->dx_labT_d.p; X.lab_coef = c->dx_labCoef_d.p; X.lab_lam = c->dx_labLam_d.p; X.lab_cr = c->dx_labCr_d.p;
+  X.opp_T = c->dx_oppT_d.p; X.opp_coef = c->dx_oppCoef_d.p; X.opp_lam = c->dx_oppLam_d.p; X.has_opp = c->dx_has_d.p;
+  X.energy = c->dx_energy_d.p; X.vs_f = c->dx_vsf_d.p; X.vs_div = c->dx_vsdiv_d.p;
+  X.stepno = c->bb_stepno_d.p; X.accepted = c->dx_acc_d.p; X.nexch = c->dx_nexch_d.p;
+  X.seed = c->seed; X.flags = c->flags_d.p;
+  // one segment, captured (no memset or host copy inside: the per-segment
+  // state lives on the device)
+  const void *kfn = batched_kernel(c, B);
+  Params P = c->P;
+  BatchArgs As = batched_args(c, lpb);
+  As.write_back = 1;
+  BatchArgs Ao = As, Aw = As;
+  Ao.write_back = 0; Ao.energy_mode = 1; Ao.energy_slot = 1; Ao.energy = c->dx_energy_d.p;
+  Ao.e_coef = c->dx_oppCoef_d.p; Ao.e_lam = c->dx_oppLam_d.p; Ao.e_use = c->dx_has_d.p;
+  Aw.write_back = 0; Aw.energy_mode = 1; Aw.energy_slot = 0; Aw.energy = c->dx_energy_d.p;
+  Ptrs Db = batched_ptrs(c), Dopp = Db, Dp = c->ptrs();
+  Dopp.tK = c->dx_oppT_d.p;   // the energies at the partner's temperature
+  const size_t lds = (size_t)(As.xlds + As.fplds) * sizeof(double4);
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+  {
+    void *a1[] = {&P, &Db, &As};
+    HIPCHK(hipLaunchKernel(kfn, dim3(R), dim3(B), a1, lds, c->stream));
+    hipLaunchKernelGGL(k_exchange_prep, dim3((R + 255) / 256), dim3(256), 0, c->stream, X, Dp);
+    void *a2[] = {&P, &Dopp, &Ao};
+    HIPCHK(hipLaunchKernel(kfn, dim3(R), dim3(B), a2, lds, c->stream));
+    void *a3[] = {&P, &Db, &Aw};
+    HIPCHK(hipLaunchKernel(kfn, dim3(R), dim3(B), a3, lds, c->stream));
+    hipLaunchKernelGGL(k_exchange_decide, dim3(1), dim3(64), 0, c->stream, X);
+    hipLaunchKernelGGL(k_exchange_apply, dim3((N + 255) / 256, R), dim3(256), 0, c->stream, X, Dp);
+  }
+  HIPCHK(hipStreamEndCapture(c->stream, &graph));
+  HIPCHK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+  HIPCHK(hipGraphDestroy(graph));
+  for (int64_t sgi = 0; sgi < nseg; sgi++) {
+    c->tmark();
+    HIPCHK(hipGraphLaunch(exec, c->stream));
+    c->tmark();
+  }
+  HIPCHK(hipStreamSynchronize(c->stream));
+  HIPCHK(hipGraphExecDestroy(exec));
+  std::vector<int> f(8), nr(R), lab(R + 1), l2r(R + 1);
+  int idx = 0, nexch = 0;
+  HIPCHK(hipMemcpy(f.data(), c->flags_d.p, 8 * sizeof(int), hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(nr.data(), c->bb_nreb_d.p, R * sizeof(int), hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(lab.data(), c->dx_rep2lab_d.p, (R + 1) * sizeof(int), hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(l2r.data(), c->dx_lab2rep_d.p, (R + 1) * sizeof(int), hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(&idx, c->dx_idx_d.p, sizeof(int), hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(&nexch, c->dx_nexch_d.p, sizeof(int), hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(stepno.data(), c->bb_stepno_d.p, R * sizeof(int64_t), hipMemcpyDeviceToHost));
+  if (f[7]) throw ApiError{SIS_ERR_STATE, "limit MC met a base-pair candidate outside the system"};
+  if (f[1] == 2) throw ApiError{SIS_ERR_STATE, "limit MC found an excess nucleotide without pairs"};
+  if (f[5]) throw ApiError{SIS_ERR_CAPACITY, "a base-pair conflict component outgrew the limit-MC scratch"};
+  for (int r = 0; r < R; r++) c->rebuilds += nr[r];
+  // the host's copies of the labels and the conditions they carry
+  for (int k = 0; k <= R; k++) { c->rep2lab[k] = lab[k]; c->lab2rep[k] = l2r[k]; }
+  for (int r = 0; r < R; r++) {
+    c->tK[r] = c->lab2val[c->rep2lab[r + 1]];
+    c->ionic[r] = c->lab2ion[c->rep2lab[r + 1]];
+  }
+  c->idx_pair = idx;
+  c->dev_exchanges += nexch;
+  const int64_t istep0 = c->istep;
+  if (!f[0]) {
+    to_packed(c);
+    c->istep = istep0 + nseg * ex;
+    return nseg * ex;
+  }
+  // a capacity overflow in segment `nexch`: every later launch did
+  // nothing.  Finish that segment on the host path, then its exchange.
+  const int64_t seg_end = istep0 + (int64_t)(nexch + 1) * ex;
+  std::vector<int> lagm(R, 0);
+  for (int r = 0; r < R; r++) {
+    todo[r] = (int)(seg_end - stepno[r]);
+    lagm[r] = todo[r] > 0 ? 1 : 0;
+  }
+  HIPCHK(hipMemset(c->flags_d.p, 0, 8 * sizeof(int)));
+  to_packed(c);
+  mcf = host_rebuild(c, lagm);
+  to_batched(c);
+  batched_segment(c, ex, todo, stepno, mcf);
+  to_packed(c);
+  c->istep = seg_end;
+  do_exchange(c, nullptr, nullptr);
+  return (int64_t)(nexch + 1) * ex;
 }

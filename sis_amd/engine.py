@@ -44,7 +44,7 @@ def _check(rc: int):
 class Engine:
     def __init__(self, sysd: dict, precision: int = 0, parity: bool = False, mode: str | int = "phases",
                  seed: int = 20241, exchange_every: int = 0, rebuild_every: int = 0, block: int = 0,
-                 max_nbr: int = 0):
+                 max_nbr: int = 0, grid: int = 0, host_exchange: bool = False):
         self.sysd = sysd
         self.nmp = int(np.sum(sysd["nmp_chain"]))
         self.nrep = int(sysd["nrep"])
@@ -73,6 +73,8 @@ class Engine:
         o.rebuild_every = rebuild_every
         o.exchange_every = exchange_every
         o.seed = seed
+        o.grid = grid
+        o.flags = 1 if host_exchange else 0
         self.opt = o
         h = C.c_void_p()
         _check(lib().sis_gpu_create(C.byref(t), C.byref(self.prm), C.byref(o), C.byref(h)))
@@ -335,9 +337,10 @@ class Engine:
         return lab
 
     def info(self) -> dict:
-        v = np.zeros(10, np.int64)
-        _check(lib().sis_gpu_get_info(self.h, abi.i64ptr(v), 10))
-        keys = ["nmp", "nrep", "istep", "rebuilds", "nwca", "nele", "nbp", "max_nbr", "mode", "block"]
+        v = np.zeros(11, np.int64)
+        _check(lib().sis_gpu_get_info(self.h, abi.i64ptr(v), 11))
+        keys = ["nmp", "nrep", "istep", "rebuilds", "nwca", "nele", "nbp", "max_nbr", "mode", "block",
+                "device_exchanges"]
         return dict(zip(keys, [int(x) for x in v]))
 
     def last_ms(self, which: int = 0) -> float:

@@ -49,6 +49,7 @@ module sis_gpu_iface
       integer(c_int32_t) :: nrep, precision, parity, mode, block, max_nbr
       integer(c_int32_t) :: rebuild_every, exchange_every
       integer(c_int64_t) :: seed
+      integer(c_int32_t) :: grid, flags
    end type sis_options
 
    ! external terms (src/force.F90:62-75): c_loc of the reference's arrays

@@ -33,7 +33,7 @@ def test_library_exports_every_declared_symbol():
 def test_struct_layouts():
     assert C.sizeof(abi.SisBpPara) == 16 * 8
     assert C.sizeof(abi.SisParams) == (18 + 42 + 12 + 48 + 4 + 3) * 8 + 14 * 4
-    assert C.sizeof(abi.SisOptions) == 8 * 4 + 8
+    assert C.sizeof(abi.SisOptions) == 8 * 4 + 8 + 2 * 4
 
 
 @pytest.mark.skipif(not Path("/opt/rocm/llvm/bin/flang").exists(), reason="flang absent")

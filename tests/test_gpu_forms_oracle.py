@@ -227,3 +227,35 @@ def test_c4_persistent_trajectory_matches_oracle(c4_system, prec):
     xo, vo, _ = o.state()
     assert np.abs(x - xo).max() < TOL[prec]
     assert np.abs(v - vo).max() < TOL[prec]
+
+
+@pytest.mark.parametrize("ladder", ["temperature", "ionic"])
+@pytest.mark.parametrize("block", [256, 128])
+def test_device_exchange_equals_host_exchange(ladder, block):
+    """Batched replica exchange done on the device (energies at both
+    conditions in the batched kernel with the limit MC, Metropolis sweep
+    and rescale on the device, segments replayed as a HIP graph) against
+    the host's sweep between segments: same labels and the same
+    trajectories, on a temperature and on an ionic-strength ladder, MC
+    every step, 1 A skin (in-kernel rebuilds)."""
+    s = configs.remd_c3(nrep=16, n=128, seed=3, margin=1.0)
+    if ladder == "ionic":
+        s["temps"] = np.full(16, 300.0)
+    print(f"[case] device vs host exchange {ladder} block {block}", flush=True)
+    res = []
+    for host in (True, False):
+        e = Engine(s, mode="batched", precision=0, seed=SEED, exchange_every=50, block=block, host_exchange=host)
+        if ladder == "ionic":
+            e.set_rep_conditions(None, np.geomspace(0.02, 0.8, 16))
+        e.init_velocities()
+        e.neighbor_list(0)
+        e.step(300)
+        info = e.info()
+        assert info["device_exchanges"] == (0 if host else 6) and info["block"] == block
+        res.append((e.get_state(), e.labels(), info["rebuilds"]))
+    (xa, va, aa), la, ra = res[0]
+    (xb, vb, ab), lb, rb = res[1]
+    assert np.array_equal(la, lb)
+    assert not np.array_equal(la, np.arange(1, 17))
+    assert np.abs(xa - xb).max() < 1e-10 and np.abs(va - vb).max() < 1e-10
+    assert ra == rb
