@@ -296,6 +296,7 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=None)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extras", action="store_true", help="skip plain FP64, FP32 error and serving legs")
+    ap.add_argument("--no-tune", action="store_true", help="skip the autotuner (defaults: 256 threads, input skin)")
     ap.add_argument("--serving-jobs", type=int, default=48)
     ap.add_argument("--serving-scale", type=float, default=2e-3)
     ap.add_argument("--serving-rate", type=float, default=40.0)
@@ -359,6 +360,12 @@ def main():
                  exchange_every=int(sysd.get("exchange_every", 0)))
     eng.init_velocities()
     eng.neighbor_list(0)
+    tune = None
+    if not args.no_tune and args.workload in ("c3", "c4"):
+        t0 = time.perf_counter()
+        tune = eng.autotune((3.0, 5.0, 8.0), nsteps=200 if args.workload == "c3" else 100)
+        print(f"[bench] autotune {time.perf_counter() - t0:.1f}s: shape {tune['shape']} skin {tune['skin']} A; "
+              + ", ".join(f"{t['shape']}/{t['skin']:g}: {t['ms_per_step']:.4f} ms" for t in tune["trials"]), flush=True)
     t0 = time.perf_counter()
     eng.step(warm)
     print(f"[bench] warmup {warm} steps {time.perf_counter() - t0:.2f}s", flush=True)
@@ -402,7 +409,10 @@ def main():
         "config": {"workload": wname, "beads": nmp, "replicas": nrep, "form": mode_used,
                    "rebuilds": int(reb), "p_wca": round(p_wca, 3), "p_ele": round(p_ele, 3),
                    "p_bp": round(p_bp, 3), "bytes_per_bead_step": round(B, 1),
-                   "ns_per_day_per_trajectory": round(ns_day, 2), "device_ms": round(dev_ms, 3)},
+                   "ns_per_day_per_trajectory": round(ns_day, 2), "device_ms": round(dev_ms, 3),
+                   "autotune": None if tune is None else {"shape": tune["shape"], "skin_A": tune["skin"],
+                                                         "trials": tune["trials"]},
+                   "device_exchanges": info2.get("device_exchanges", 0)},
         "roofline": None,
         "cpu_baseline": None,
     }

@@ -420,6 +420,18 @@ int sis_bp_write_file(const char *path, int64_t n, const int32_t *ij, const uint
 int sis_dcd_write_file(const char *path, int32_t nmp, int32_t nframes, const double *xyz, int32_t unitcell,
                        const double *box, int32_t fix_com_origin);
 
+/* Project-written autotuner (no reference counterpart).  From the current
+ * state, runs nsteps steps of the context's form for every launch shape
+ * (batched form: 128, 256 and 512 threads per replica; persistent form:
+ * 1 and 2 workgroups per compute unit and as many as fit) and every
+ * neighbour-list skin skins[0..nskin) (A), restoring the state before each
+ * trial, and keeps the fastest shape and skin (the skin moves rebuilds,
+ * not forces).  The state is restored at the end and the lists are
+ * rebuilt at the next step.  report (may be NULL): [0] chosen shape,
+ * [1] chosen skin, then per trial (shape, skin, ms per step);
+ * 2 + 9 * nskin doubles. */
+int sis_gpu_autotune(sis_gpu_ctx *ctx, const double *skins, int32_t nskin, int64_t nsteps, double *report);
+
 /* Counters: out[0] nmp, [1] nrep, [2] istep, [3] rebuilds, [4] nwca,
  * [5] nele, [6] nbp (replica 1), [7] neighbour slots per bead,
  * [8] launch mode used, [9] workgroup size (batched form: threads per

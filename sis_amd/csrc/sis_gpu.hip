@@ -1776,6 +1776,15 @@ int sis_gpu_step(sis_gpu_ctx *c, int64_t nsteps, int64_t *rebuilds) {
   API_END
 }
 
+int sis_gpu_autotune(sis_gpu_ctx *c, const double *skins, int32_t nskin, int64_t nsteps, double *report) {
+  API_BEGIN
+  if (!c || !skins || nskin < 1 || nsteps < 1) return fail(SIS_ERR_ARG, "invalid argument");
+  if (c->parity) return fail(SIS_ERR_UNSUPPORTED, "the autotuner runs the performance forms");
+  autotune(c, skins, nskin, nsteps, report);
+  return SIS_OK;
+  API_END
+}
+
 int sis_gpu_exchange(sis_gpu_ctx *c, double *replica_energies, int32_t *accepted) {
   API_BEGIN
   if (!c) return fail(SIS_ERR_ARG, "null context");

@@ -537,3 +537,101 @@ int64_t batched_device_run(sis_gpu_ctx *c, int64_t nseg) {
   do_exchange(c, nullptr, nullptr);
   return (int64_t)(nexch + 1) * ex;
 }
+
+namespace {
+
+// neighbour-list skin of the context (neighbor_list_margin): the list
+// cutoffs and the cell grid follow it
+void apply_margin(sis_gpu_ctx *c, double m) {
+  c->prm.nl_margin = m;
+  c->P.nl_margin = m;
+  c->P.wca_nl_cut2 = (c->P.wca_sigma + m) * (c->P.wca_sigma + m);
+  if (c->P.bp_model != 0) c->P.bp_nl_cut2 = (c->P.bp_cutoff_dist + m) * (c->P.bp_cutoff_dist + m);
+  setup_cells(c);
+  c->lists_valid = false;
+}
+
+}  // namespace
+
+// Autotuner of the launch shape and the rebuild skin: from the current
+// state, times nsteps steps of the context's form at every candidate
+// (batched: threads per replica 128 / 256 / 512; persistent: 1, 2 and as
+// many workgroups per compute unit as fit) times every skin, restoring
+// the state before each trial and at the end, and keeps the fastest.
+// The skin changes when lists are rebuilt, not the forces (every term
+// applies its own cutoff), so the choice is a matter of speed only.
+void autotune(sis_gpu_ctx *c, const double *skins, int nskin, int64_t nsteps, double *report) {
+  int mode = c->mode ? c->mode : choose_mode(c);
+  const bool batched = mode == 3 && batched_ok(c);
+  const bool pers = mode == 2 && c->R == 1 && !c->ext_on;
+  if (!batched && !pers) throw ApiError{SIS_ERR_UNSUPPORTED, "the autotuner covers the batched and persistent forms"};
+  const int R = c->R, N = c->N;
+  const size_t RN = (size_t)R * N;
+  // saved state: coordinates, velocities, noise terms, step, labels
+  std::vector<double> x(3 * RN), v(3 * RN), a(3 * RN);
+  {
+    std::vector<double4> h(RN);
+    auto down = [&](std::vector<double> &dst, DevBuf<double4> &src) {
+      HIPCHK(hipMemcpy(h.data(), src.p, RN * sizeof(double4), hipMemcpyDeviceToHost));
+      for (size_t g = 0; g < RN; g++) { dst[3 * g] = h[g].x; dst[3 * g + 1] = h[g].y; dst[3 * g + 2] = h[g].z; }
+    };
+    down(x, c->pos_d); down(v, c->vel_d); down(a, c->acc_d);
+  }
+  const int64_t istep0 = c->istep, reb0 = c->rebuilds, dex0 = c->dev_exchanges;
+  const std::vector<int> lab0(c->rep2lab.begin() + 1, c->rep2lab.end());
+  const int idx0 = c->idx_pair;
+  const double m0 = c->prm.nl_margin;
+  const int block0 = c->opt.block, grid0 = c->opt.grid;
+  auto restore = [&]() {
+    if (R > 1) { std::vector<int32_t> l(lab0.begin(), lab0.end()); sis_gpu_set_labels(c, l.data()); }
+    c->idx_pair = idx0;
+    sis_gpu_set_state(c, x.data(), v.data(), a.data());
+    c->istep = istep0;
+    c->lists_valid = false;
+  };
+  std::vector<int> shapes;
+  if (batched) {
+    shapes = {128, 256, 512};
+  } else {
+    int dev = 0, ncu = 0;
+    HIPCHK(hipGetDevice(&dev));
+    HIPCHK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+    shapes = {ncu, 2 * ncu, 0};
+  }
+  double best = 1e300;
+  int best_shape = batched ? 256 : 0;
+  double best_skin = m0;
+  int k = 0;
+  for (int sh : shapes) {
+    for (int q = 0; q < nskin; q++) {
+      if (!(skins[q] > 0.0)) throw ApiError{SIS_ERR_ARG, "skin must be positive"};
+      if (batched) c->opt.block = sh; else c->opt.grid = sh;
+      apply_margin(c, skins[q]);
+      restore();
+      HIPCHK(hipStreamSynchronize(c->stream));
+      HIPCHK(hipEventRecord(c->ev0, c->stream));
+      int64_t rb = 0;
+      int rc = sis_gpu_step(c, nsteps, &rb);
+      if (rc != SIS_OK) throw ApiError{rc, std::string("autotune trial failed: ") + g_err};
+      HIPCHK(hipEventRecord(c->ev1, c->stream));
+      HIPCHK(hipEventSynchronize(c->ev1));
+      float ms = 0.f;
+      HIPCHK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+      const double per = ms / (double)nsteps;
+      if (report) {
+        report[2 + 3 * k] = batched ? sh : c->block;
+        report[3 + 3 * k] = skins[q];
+        report[4 + 3 * k] = per;
+      }
+      k++;
+      if (per < best) { best = per; best_shape = sh; best_skin = skins[q]; }
+    }
+  }
+  if (batched) c->opt.block = best_shape; else c->opt.grid = best_shape;
+  (void)block0; (void)grid0;
+  apply_margin(c, best_skin);
+  restore();
+  c->rebuilds = reb0;
+  c->dev_exchanges = dex0;
+  if (report) { report[0] = batched ? best_shape : best_shape; report[1] = best_skin; }
+}

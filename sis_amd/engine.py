@@ -137,6 +137,16 @@ class Engine:
         _check(lib().sis_gpu_energy_noalloc(self.h, irep, float(tempK), flags, abi.dptr(e)))
         return e
 
+    def autotune(self, skins=(3.0, 5.0, 8.0), nsteps: int = 200) -> dict:
+        """sis_gpu_autotune: times every launch shape x skin from the
+        current state and keeps the fastest (state restored)."""
+        sk = np.ascontiguousarray(skins, np.float64)
+        rep = np.zeros(2 + 9 * len(sk))
+        _check(lib().sis_gpu_autotune(self.h, abi.dptr(sk), len(sk), int(nsteps), abi.dptr(rep)))
+        trials = [dict(shape=int(rep[2 + 3 * k]), skin=float(rep[3 + 3 * k]), ms_per_step=round(float(rep[4 + 3 * k]), 5))
+                  for k in range(3 * len(sk))]
+        return dict(shape=int(rep[0]), skin=float(rep[1]), trials=trials)
+
     def kinetic(self, irep: int = 0):
         """energy_kinetic of replica irep (1-based), or of all replicas (0)."""
         out = np.zeros(max(self.nrep, 1))

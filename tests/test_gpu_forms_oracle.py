@@ -259,3 +259,30 @@ def test_device_exchange_equals_host_exchange(ladder, block):
     assert not np.array_equal(la, np.arange(1, 17))
     assert np.abs(xa - xb).max() < 1e-10 and np.abs(va - vb).max() < 1e-10
     assert ra == rb
+
+
+def test_autotuner_sweeps_and_restores():
+    """sis_gpu_autotune on a batched ladder: 3 workgroup sizes x 3 skins
+    timed, the fastest kept, the state (coordinates, velocities, step,
+    labels) restored; the run then matches a fresh context created with
+    the chosen block and skin."""
+    s = configs.remd_c3(nrep=16, n=128, seed=3)
+    print("[case] autotuner", flush=True)
+    e = Engine(s, mode="batched", precision=1, seed=SEED, exchange_every=50)
+    e.init_velocities()
+    e.neighbor_list(0)
+    x0, v0, a0 = e.get_state()
+    rep = e.autotune((3.0, 5.0, 8.0), nsteps=100)
+    assert len(rep["trials"]) == 9 and all(t["ms_per_step"] > 0 for t in rep["trials"])
+    assert rep["shape"] in (128, 256, 512) and rep["skin"] in (3.0, 5.0, 8.0)
+    x1, v1, a1 = e.get_state()
+    assert np.array_equal(x0, x1) and np.array_equal(v0, v1) and np.array_equal(a0, a1)
+    assert e.info()["istep"] == 0
+    e.step(100)
+    s2 = dict(s, params=dict(s["params"], nl_margin=rep["skin"]))
+    f = Engine(s2, mode="batched", precision=1, seed=SEED, exchange_every=50, block=rep["shape"])
+    f.set_state(x0, v0, a0)
+    f.neighbor_list(0)
+    f.step(100)
+    assert np.abs(e.get_state()[0] - f.get_state()[0]).max() < 1e-9
+    assert np.array_equal(e.labels(), f.labels())
