@@ -100,9 +100,12 @@ typedef struct sis_options {
     int32_t grid;            /* persistent form: workgroups of the cooperative launch
                                 (0 = as many as fit on the device) */
     int32_t flags;           /* bit0 SIS_OPT_HOST_EXCHANGE: batched runs exchange on the
-                                host between segments (no device exchange, no graph) */
+                                host between segments (no device exchange, no graph);
+                                bit1 SIS_OPT_NO_GRAPH: the per-phase form launches every
+                                step from the host instead of replaying a captured graph */
 } sis_options;
 #define SIS_OPT_HOST_EXCHANGE 1
+#define SIS_OPT_NO_GRAPH 2
 
 typedef struct sis_gpu_ctx sis_gpu_ctx;
 

@@ -206,6 +206,9 @@ struct sis_gpu_ctx {
   int nblk_force = 0;
   int block = 256;
   int dev = 0;   // the HIP device current when the context was created
+  // replayed per-phase steps (phases_graph_steps)
+  DevBuf<int64_t> g_step_d;
+  DevBuf<int> g_int_d;
   // device exchange of the batched form (sis_exchange.h)
   DevBuf<int> dx_rep2lab_d, dx_lab2rep_d, dx_pair_tb_d, dx_idx_d, dx_has_d, dx_vsdiv_d, dx_nexch_d, dx_acc_d;
   DevBuf<double> dx_labT_d, dx_labCoef_d, dx_labLam_d, dx_labCr_d, dx_oppT_d, dx_oppCoef_d, dx_oppLam_d,
@@ -294,6 +297,8 @@ struct sis_gpu_ctx {
                              &dx_oppLam_d, &dx_energy_d, &dx_vsf_d};
     for (auto *b : dxd) b->free_();
     if (dx_graph) (void)hipGraphExecDestroy(dx_graph);
+    g_step_d.free_();
+    g_int_d.free_();
     for (auto e : tev) (void)hipEventDestroy(e);
     if (ev0) (void)hipEventDestroy(ev0);
     if (ev1) (void)hipEventDestroy(ev1);
@@ -641,9 +646,13 @@ void rebuild(sis_gpu_ctx *c, const std::vector<int> &mask_in) {
 }
 
 // Base-pair evaluation + limit MC (force_bp_limit_triplet / force_bp_triplet).
-void bp_phase(sis_gpu_ctx *c, const std::vector<int> &mc, const double *ecT_dev, const int *mask_dev) {
+// mc: limit-MC flags per replica, uploaded here; null when the flags are
+// already on the device (replayed graph steps, whose step number is
+// step_dev's)
+void bp_phase(sis_gpu_ctx *c, const std::vector<int> *mc, const double *ecT_dev, const int *mask_dev,
+              const int64_t *step_dev = nullptr) {
   const int N = c->N, R = c->R, B = 256;
-  set_mc(c, mc);
+  if (mc) set_mc(c, *mc);
   Ptrs D = c->ptrs();
   hipLaunchKernelGGL(k_reset_cnt, grid_beads(N, R, B), dim3(B), 0, c->stream, c->P, D, mask_dev);
   if (c->nbp_total > 0) {
@@ -654,8 +663,9 @@ void bp_phase(sis_gpu_ctx *c, const std::vector<int> &mc, const double *ecT_dev,
       hipLaunchKernelGGL(k_bp_eval<double>, dim3((c->nbp_total + B - 1) / B), dim3(B), 0, c->stream, c->P, D,
                          c->nbp_total, c->mc_d.p, ecT_dev, mask_dev);
   }
-  bool any_mc = false;
-  for (int r = 0; r < R; r++) any_mc |= mc[r] != 0;
+  bool any_mc = !mc;
+  if (mc)
+    for (int r = 0; r < R; r++) any_mc |= (*mc)[r] != 0;
   if (!any_mc || c->P.nstep_bp_mc <= 0) return;
   if (c->parity) {
     hipLaunchKernelGGL(k_mc_serial, dim3(R), dim3(64), 0, c->stream, c->P, D, c->mc_d.p, ecT_dev,
@@ -671,13 +681,13 @@ void bp_phase(sis_gpu_ctx *c, const std::vector<int> &mc, const double *ecT_dev,
     hipLaunchKernelGGL(k_seg_fill, dim3((c->nbp_total + B - 1) / B), dim3(B), 0, c->stream, c->P, D,
                        c->nbp_total, c->exc_cnt_d.p, c->seg_d.p, c->exc_fill_d.p, c->mc_d.p, mask_dev);
     hipLaunchKernelGGL(k_mc_wave, dim3(N, R), dim3(64), 0, c->stream, c->P, D, c->exc_cnt_d.p, c->seg_d.p,
-                       c->exc_fill_d.p, c->mc_d.p, ecT_dev, c->seed, c->istep, mask_dev, c->huge_d.p);
+                       c->exc_fill_d.p, c->mc_d.p, ecT_dev, c->seed, c->istep, mask_dev, c->huge_d.p, step_dev);
     hipLaunchKernelGGL(k_mc_huge, dim3(R), dim3(64), 0, c->stream, c->P, D, c->exc_cnt_d.p, c->seg_d.p,
-                       c->exc_fill_d.p, c->huge_d.p, ecT_dev, c->seed, c->istep);
+                       c->exc_fill_d.p, c->huge_d.p, ecT_dev, c->seed, c->istep, step_dev);
   }
 }
 
-void force_kernel(sis_gpu_ctx *c, int integrate, const int *mask_dev) {
+void force_kernel(sis_gpu_ctx *c, int integrate, const int *mask_dev, const int64_t *step_dev = nullptr) {
   const int N = c->N, R = c->R;
   constexpr int B = 256, LPB = 8;
   dim3 g((N + B / LPB - 1) / (B / LPB), R);
@@ -691,10 +701,10 @@ void force_kernel(sis_gpu_ctx *c, int integrate, const int *mask_dev) {
   }
   if (c->precision == 1)
     hipLaunchKernelGGL((k_force_lanes<float, LPB, B>), g, dim3(B), 0, c->stream, c->P, c->ptrs(), integrate,
-                       c->parity, c->seed, c->istep, mask_dev);
+                       c->parity, c->seed, c->istep, mask_dev, step_dev);
   else
     hipLaunchKernelGGL((k_force_lanes<double, LPB, B>), g, dim3(B), 0, c->stream, c->P, c->ptrs(), integrate,
-                       c->parity, c->seed, c->istep, mask_dev);
+                       c->parity, c->seed, c->istep, mask_dev, step_dev);
 }
 
 void check_flags(sis_gpu_ctx *c) {
@@ -737,7 +747,7 @@ void energy_phase(sis_gpu_ctx *c, const std::vector<int> &mask, const std::vecto
   if (!(flags & 2)) {
     std::vector<int> mc(R, 0);
     for (int r = 0; r < R; r++) mc[r] = (flags & 1) && mask[r];
-    bp_phase(c, mc, c->ecT_d.p, c->mask_d.p);
+    bp_phase(c, &mc, c->ecT_d.p, c->mask_d.p);
   }
   dim3 g = grid_beads(N, R, B);
   if ((int64_t)c->ene_part_d.n < (int64_t)R * g.x * 8) c->ene_part_d.alloc((size_t)R * g.x * 8);
@@ -957,7 +967,7 @@ void step_phases(sis_gpu_ctx *c) {
   }
   std::vector<int> all(R, 1);
   set_mask(c, all);
-  bp_phase(c, mc, nullptr, nullptr);
+  bp_phase(c, &mc, nullptr, nullptr);
   if (c->P.nstep_bp_mc > 1) {
     // bp_status_MC = bp_status of the replicas that did MC
     // (src/job_md.F90:399-401); candidates are grouped by replica
@@ -978,7 +988,99 @@ void step_phases(sis_gpu_ctx *c) {
   force_kernel(c, 1, nullptr);
   c->tmark();
   std::swap(c->pos_d, c->pos2_d);   // updated positions become current
-  hipLaunchKernelGGL(k_skin, dim3(R), dim3(64), 0, c->stream, c->P, c->ptrs(), c->nblk_force);
+  hipLaunchKernelGGL(k_skin<256>, dim3(R), dim3(256), 0, c->stream, c->P, c->ptrs(), c->nblk_force, GStep{});
+}
+
+// Whether the per-phase form can replay its steps from a captured graph:
+// the performance form without external terms or a fixed rebuild cadence.
+bool graph_steps_ok(sis_gpu_ctx *c) {
+  return !c->parity && !c->ext_on && c->opt.rebuild_every == 0 && !(c->opt.flags & SIS_OPT_NO_GRAPH);
+}
+
+// The per-phase form's steps (step_phases) replayed from a HIP graph of
+// GRAPH_STEPS steps, with no host round trip per step: the step number and
+// the limit-MC flags come from k_gstep_begin, and k_skin halts the steps
+// that follow when a replica needs a rebuild (or the requested count is
+// reached).  The host then rebuilds, captures again (rebuilds resize the
+// launches) and replays on.  Same kernels, same arithmetic and random
+// streams as step_phases.
+constexpr int GRAPH_STEPS = 16;   // even: the positions end in the buffer they started in
+
+void phases_graph_steps(sis_gpu_ctx *c, int64_t nsteps) {
+  const int R = c->R;
+  if (c->g_step_d.n < 3) c->g_step_d.alloc(3);
+  if (c->g_int_d.n < (size_t)(1 + 2 * R)) c->g_int_d.alloc(1 + 2 * R);
+  GStep G{};
+  G.istep = c->g_step_d.p;
+  G.istep_cur = c->g_step_d.p + 1;
+  G.stop = c->g_step_d.p + 2;
+  G.halt = c->g_int_d.p;
+  G.active = c->g_int_d.p + 1;
+  G.force_mc = c->g_int_d.p + 1 + R;
+  G.mc = c->mc_d.p;
+  std::vector<int> fmc(R, 0), flag(R, 0);
+  const int64_t stop = c->istep + nsteps;
+  while (c->istep < stop) {
+    // a rebuild due from the last skin test, or lists not built yet
+    // (src/job_md.F90:376-395); the rebuilt replicas do the limit MC first
+    if (!c->lists_valid) {
+      std::fill(flag.begin(), flag.end(), 1);
+    } else {
+      HIPCHK(hipMemcpy(flag.data(), c->rebuild_flag_d.p, R * sizeof(int), hipMemcpyDeviceToHost));
+    }
+    bool any = false;
+    for (int r = 0; r < R; r++) any = any || flag[r];
+    if (any) {
+      rebuild(c, flag);
+      c->rebuilds++;
+      for (int r = 0; r < R; r++) fmc[r] = (c->rebuilt[r] && c->P.nstep_bp_mc > 0) ? 1 : fmc[r];
+    }
+    const int64_t st[3] = {c->istep, c->istep, stop};
+    const int zero = 0;
+    HIPCHK(hipMemcpyAsync(c->g_step_d.p, st, sizeof st, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(c->g_int_d.p, &zero, sizeof zero, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(G.force_mc, fmc.data(), R * sizeof(int), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    std::fill(fmc.begin(), fmc.end(), 0);
+    // buffers the captured kernels use are allocated before the capture
+    const int nblk = (c->N + 31) / 32;   // force_kernel: 32 beads per workgroup
+    if ((int64_t)c->blk_top2_d.n < (int64_t)R * nblk * 2) c->blk_top2_d.alloc((size_t)R * nblk * 2);
+    if (c->P.nstep_bp_mc > 0 && c->seg_d.n < (size_t)R * c->N * SEGCAP) c->seg_d.alloc((size_t)R * c->N * SEGCAP);
+    // capture GRAPH_STEPS steps from the current position buffer
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    for (int j = 0; j < GRAPH_STEPS; j++) {
+      hipLaunchKernelGGL(k_gstep_begin, dim3((R + 255) / 256), dim3(256), 0, c->stream, c->P, G);
+      bp_phase(c, nullptr, nullptr, G.active, G.istep_cur);
+      if (c->P.nstep_bp_mc > 1 && c->nbp_total > 0)
+        hipLaunchKernelGGL(k_mc_keep, dim3((c->nbp_total + 255) / 256), dim3(256), 0, c->stream, c->ptrs(),
+                           (int)c->nbp_total, G.mc, G.active);
+      force_kernel(c, 1, G.active, G.istep_cur);
+      std::swap(c->pos_d, c->pos2_d);
+      hipLaunchKernelGGL(k_skin<256>, dim3(R), dim3(256), 0, c->stream, c->P, c->ptrs(), c->nblk_force, G);
+    }
+    HIPCHK(hipStreamEndCapture(c->stream, &graph));
+    HIPCHK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+    HIPCHK(hipGraphDestroy(graph));
+    // replay until a halt
+    int halt = 0;
+    int64_t now = c->istep;
+    while (!halt) {
+      HIPCHK(hipGraphLaunch(exec, c->stream));
+      HIPCHK(hipMemcpyAsync(&halt, c->g_int_d.p, sizeof halt, hipMemcpyDeviceToHost, c->stream));
+      HIPCHK(hipMemcpyAsync(&now, c->g_step_d.p, sizeof now, hipMemcpyDeviceToHost, c->stream));
+      HIPCHK(hipStreamSynchronize(c->stream));
+      if (!halt && now - c->istep != (int64_t)GRAPH_STEPS)
+        throw ApiError{SIS_ERR_STATE, "graph steps did not advance"};
+      if (!halt) c->istep = now;
+    }
+    HIPCHK(hipGraphExecDestroy(exec));
+    // an odd number of steps in the last replay leaves the positions in
+    // the other buffer
+    if ((now - c->istep) % 2) std::swap(c->pos_d, c->pos2_d);
+    c->istep = now;
+  }
 }
 
 }  // namespace
@@ -1064,7 +1166,7 @@ int sis_gpu_create(const sis_topo *topo, const sis_params *prm, const sis_option
   if (o.block != 0 && o.block != 128 && o.block != 256 && o.block != 512)
     return fail(SIS_ERR_ARG, "block: threads per replica of the batched form, 128, 256 or 512 (0 = 256)");
   if (o.grid < 0) return fail(SIS_ERR_ARG, "grid must be >= 0");
-  if (o.flags & ~SIS_OPT_HOST_EXCHANGE) return fail(SIS_ERR_ARG, "unknown option flags");
+  if (o.flags & ~(SIS_OPT_HOST_EXCHANGE | SIS_OPT_NO_GRAPH)) return fail(SIS_ERR_ARG, "unknown option flags");
   int sum = 0;
   for (int c = 0; c < topo->nchains; c++) {
     if (topo->nmp_chain[c] < 1) return fail(SIS_ERR_ARG, "chain of zero length");
@@ -1373,7 +1475,7 @@ int sis_gpu_force(sis_gpu_ctx *c, int32_t irep, int32_t bp_mc, double *forces) {
     // a context created for the batched form evaluates with that kernel
     batched_force(c, irep - 1, bp_mc);
   } else {
-    bp_phase(c, mc, nullptr, c->mask_d.p);
+    bp_phase(c, &mc, nullptr, c->mask_d.p);
     force_kernel(c, 0, c->mask_d.p);
   }
   HIPCHK(hipEventRecord(c->ev1, c->stream));
@@ -1680,6 +1782,8 @@ int sis_gpu_step(sis_gpu_ctx *c, int64_t nsteps, int64_t *rebuilds) {
       persistent_steps(c, n);
     } else if (mode == 3 && !c->parity) {
       batched_steps(c, n);
+    } else if (graph_steps_ok(c)) {
+      phases_graph_steps(c, n);
     } else {
       for (int64_t s = 0; s < n; s++) step_phases(c);
     }

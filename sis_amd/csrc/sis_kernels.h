@@ -900,9 +900,11 @@ __device__ __noinline__ bool mc_component_huge(const McArrays D, const int N, in
 // Per-phase form: components above SEGCAP pairs, one wavefront per
 // replica after k_mc_wave (which marks the replicas that have them).
 __global__ void __launch_bounds__(64) k_mc_huge(Params P, Ptrs D, const int *seg_n, int *seg, const int *has_exc,
-                                                int *huge, const double *ecT, uint64_t seed, int64_t step) {
+                                                int *huge, const double *ecT, uint64_t seed, int64_t step0,
+                                                const int64_t *step_dev) {
   const int r = blockIdx.x, L = threadIdx.x, N = P.N;
   if (!huge[r]) return;
+  const int64_t step = step_dev ? *step_dev : step0;   // graph replays read the step from the device
   __shared__ int sseq[64];
   const double tK = ecT ? ecT[r] : D.tK[r];
   const double beta = mc_beta(P, tK, ecT == nullptr);
@@ -926,10 +928,11 @@ __global__ void __launch_bounds__(64) k_mc_huge(Params P, Ptrs D, const int *seg
 }
 __global__ void __launch_bounds__(64) k_mc_wave(Params P, Ptrs D, int *seg_n, int *seg,
                                                 const int *has_exc, const int *mc,
-                                                const double *ecT, uint64_t seed, int64_t step,
-                                                const int *mask, int *huge) {
+                                                const double *ecT, uint64_t seed, int64_t step0,
+                                                const int *mask, int *huge, const int64_t *step_dev) {
   int root = blockIdx.x, r = blockIdx.y;
   if ((mask && !mask[r]) || !mc[r]) return;
+  const int64_t step = step_dev ? *step_dev : step0;   // graph replays read the step from the device
   size_t gr = (size_t)r * P.N + root;
   if (!has_exc[gr]) return;
   int n = seg_n[gr];
@@ -1346,8 +1349,10 @@ __global__ void __launch_bounds__(B) k_bonded(Params P, Ptrs D, const int *mask)
 
 template <typename T, int LPB, int B>
 __global__ void __launch_bounds__(B) k_force_lanes(Params P, Ptrs D, int integrate, int parity,
-                                                  uint64_t seed, int64_t step, const int *mask) {
+                                                  uint64_t seed, int64_t step0, const int *mask,
+                                                  const int64_t *step_dev) {
   __shared__ double s1[B / LPB], s2[B / LPB];
+  const int64_t step = step_dev ? *step_dev : step0;   // graph replays read the step from the device
   const int grp = threadIdx.x / LPB, l = threadIdx.x % LPB;
   const int a = blockIdx.x * (B / LPB) + grp, r = blockIdx.y;
   const int N = P.N;
@@ -1381,19 +1386,86 @@ __global__ void __launch_bounds__(B) k_force_lanes(Params P, Ptrs D, int integra
   }
 }
 
-// skin test per replica: sqrt(d2max) + sqrt(d2max_2nd) > nl_margin
-__global__ void k_skin(Params P, Ptrs D, int nblk) {
-  int r = blockIdx.x;
-  if (threadIdx.x != 0) return;
+// top two of (a1 >= a2) and (b1 >= b2): the two largest values of the four
+__device__ __forceinline__ void top2_merge(double &m1, double &m2, double b1, double b2) {
+  const double n1 = m1 > b1 ? m1 : b1;
+  const double n2 = m1 > b1 ? (m2 > b1 ? m2 : b1) : (m1 > b2 ? m1 : b2);
+  m1 = n1;
+  m2 = n2;
+}
+
+// Steps replayed from a captured graph (the per-phase form without a host
+// round trip per step): the step each launch works on, when it stops, and
+// the rebuild it stopped for.
+struct GStep {
+  int64_t *istep;        // [1] last completed step
+  int64_t *istep_cur;    // [1] step of the current launch sequence
+  const int64_t *stop;   // [1] stop once this step is completed
+  int *halt;             // [1] 1: the next steps do nothing (a rebuild or the stop is due)
+  int *active;           // [R] mask of the step's kernels (0 after a halt)
+  int *force_mc;         // [R] limit MC on the next step (after a rebuild)
+  int *mc;               // [R] limit MC on the current step
+};
+
+// skin test per replica (src/job_md.F90:376-395): sqrt(d2max) +
+// sqrt(d2max_2nd) > nl_margin over the per-block top-2 displacements,
+// reduced by a workgroup (the top two are the same values in any order).
+// With a graph step: a replica needing a rebuild halts the steps that
+// follow, and the step is committed.
+template <int B>
+__global__ void __launch_bounds__(B) k_skin(Params P, Ptrs D, int nblk, GStep G) {
+  __shared__ double s1[B], s2[B];
+  const int r = blockIdx.x, t = threadIdx.x;
+  if (G.halt && !G.active[r]) return;
   double m1 = 0.0, m2 = 0.0;
-  for (int b = 0; b < nblk; b++) {
-    double a1 = D.blk_top2[((size_t)r * nblk + b) * 2], a2 = D.blk_top2[((size_t)r * nblk + b) * 2 + 1];
-    double n1 = m1 > a1 ? m1 : a1;
-    double n2 = m1 > a1 ? (m2 > a1 ? m2 : a1) : (m1 > a2 ? m1 : a2);
-    m1 = n1;
-    m2 = n2;
+  for (int b = t; b < nblk; b += B)
+    top2_merge(m1, m2, D.blk_top2[((size_t)r * nblk + b) * 2], D.blk_top2[((size_t)r * nblk + b) * 2 + 1]);
+  s1[t] = m1;
+  s2[t] = m2;
+  __syncthreads();
+  for (int off = B / 2; off > 0; off >>= 1) {
+    if (t < off) {
+      double a1 = s1[t], a2 = s2[t];
+      top2_merge(a1, a2, s1[t + off], s2[t + off]);
+      s1[t] = a1;
+      s2[t] = a2;
+    }
+    __syncthreads();
   }
-  D.rebuild_flag[r] = (sqrt(m1) + sqrt(m2) > P.nl_margin) ? 1 : 0;
+  if (t != 0) return;
+  const int f = (sqrt(s1[0]) + sqrt(s2[0]) > P.nl_margin) ? 1 : 0;
+  D.rebuild_flag[r] = f;
+  if (G.halt) {
+    if (f) atomicOr(G.halt, 1);
+    if (r == 0) {
+      *G.istep = *G.istep_cur;
+      if (*G.istep_cur >= *G.stop) atomicOr(G.halt, 1);
+    }
+  }
+}
+
+// start of a replayed step: nothing if halted; else the step number and
+// the limit-MC flags (every nstep_MC steps and after a rebuild,
+// src/job_md.F90:364-394)
+__global__ void k_gstep_begin(Params P, GStep G) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= P.R) return;
+  const bool halted = *G.halt != 0;
+  G.active[r] = halted ? 0 : 1;
+  if (halted) return;
+  const int64_t s = *G.istep + 1;
+  if (r == 0) *G.istep_cur = s;
+  G.mc[r] = P.nstep_bp_mc > 0 && ((s % P.nstep_bp_mc) == 0 || G.force_mc[r]) ? 1 : 0;
+  G.force_mc[r] = 0;
+}
+
+// bp_status_MC = bp_status for the replicas that did the MC this step
+// (src/job_md.F90:399-401)
+__global__ void k_mc_keep(Ptrs D, int nbp_total, const int *mc, const int *mask) {
+  const int gb = blockIdx.x * blockDim.x + threadIdx.x;
+  if (gb >= nbp_total) return;
+  const int r = D.bp_mp[(size_t)gb * 8 + 7];
+  if (mask[r] && mc[r]) D.bp_status_mc[gb] = D.bp_status[gb];
 }
 
 __global__ void k_clear_move(Params P, Ptrs D, const int *mask) {

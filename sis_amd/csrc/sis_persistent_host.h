@@ -7,10 +7,14 @@ bool batched_ok(sis_gpu_ctx *c) {
   return !c->parity && !c->ext_on && c->R >= 2 && c->P.nchains == 1 && c->N <= 1024;
 }
 
+// The form a context runs when none is asked for: the batched form for
+// many replicas of a small chain, else the per-phase kernels (replayed
+// from captured graphs in the performance form).  The persistent form
+// stays available on request; on one large system the replayed per-phase
+// kernels are faster (each at its own occupancy, no grid barriers).
 int choose_mode(sis_gpu_ctx *c) {
   if (c->parity) return 1;
   if (batched_ok(c)) return 3;
-  if (c->R == 1 && c->N >= 1024 && !c->ext_on) return 2;
   return 1;
 }
 

@@ -134,6 +134,9 @@ bool job_admit(sis_pool *p, PoolJob &j) {
   o.precision = j.precision;
   o.mode = 1;
   o.seed = j.seed;
+  // the pool's worker threads launch steps while its scheduler thread
+  // allocates and frees other jobs' memory: no stream capture there
+  o.flags = SIS_OPT_NO_GRAPH;
   sis_gpu_ctx *c = nullptr;
   if (sis_gpu_create(&t, &j.prm, &o, &c) != SIS_OK) { job_fail(p, j, g_err); return false; }
   j.c = c;

@@ -44,7 +44,7 @@ def _check(rc: int):
 class Engine:
     def __init__(self, sysd: dict, precision: int = 0, parity: bool = False, mode: str | int = "phases",
                  seed: int = 20241, exchange_every: int = 0, rebuild_every: int = 0, block: int = 0,
-                 max_nbr: int = 0, grid: int = 0, host_exchange: bool = False):
+                 max_nbr: int = 0, grid: int = 0, host_exchange: bool = False, no_graph: bool = False):
         self.sysd = sysd
         self.nmp = int(np.sum(sysd["nmp_chain"]))
         self.nrep = int(sysd["nrep"])
@@ -74,7 +74,7 @@ class Engine:
         o.exchange_every = exchange_every
         o.seed = seed
         o.grid = grid
-        o.flags = 1 if host_exchange else 0
+        o.flags = (1 if host_exchange else 0) | (2 if no_graph else 0)
         self.opt = o
         h = C.c_void_p()
         _check(lib().sis_gpu_create(C.byref(t), C.byref(self.prm), C.byref(o), C.byref(h)))

@@ -47,18 +47,20 @@ def _c3(nrep, margin=1.0):
 
 
 @pytest.mark.parametrize("prec", [0, 1])
-@pytest.mark.parametrize("nrep,nstep,nex", [(16, 200, 50), (256, 100, 50)])
-def test_batched_trajectory_matches_oracle(prec, nrep, nstep, nex):
+@pytest.mark.parametrize("nrep,nstep,nex,mode", [(16, 200, 50, "batched"), (256, 100, 50, "batched"),
+                                                 (16, 200, 50, "phases")])
+def test_batched_trajectory_matches_oracle(prec, nrep, nstep, nex, mode):
     """C3's chain (128 nt, 290-380 K ladder) with a 1 A skin: in-kernel
     rebuilds and exchanges every 50 steps; 256 replicas is C3's own ladder
-    at the reference's per-dimension maximum."""
+    at the reference's per-dimension maximum.  mode "phases": the per-phase
+    kernels replayed from captured graphs, rebuilds between replays."""
     s = _c3(nrep)
-    print(f"[case] batched prec {prec} R {nrep}", flush=True)
-    e = Engine(s, mode="batched", precision=prec, seed=SEED, exchange_every=nex)
+    print(f"[case] {mode} prec {prec} R {nrep}", flush=True)
+    e = Engine(s, mode=mode, precision=prec, seed=SEED, exchange_every=nex)
     e.init_velocities()
     e.neighbor_list(0)
     e.step(nstep)
-    assert e.info()["mode"] == 3
+    assert e.info()["mode"] == {"batched": 3, "phases": 1}[mode]
     o, nreb = _oracle_run(s, SEED, nstep, nex)
     assert nreb > 0 and e.info()["rebuilds"] > 0
     x, v, _ = e.get_state()
@@ -286,3 +288,36 @@ def test_autotuner_sweeps_and_restores():
     f.step(100)
     assert np.abs(e.get_state()[0] - f.get_state()[0]).max() < 1e-9
     assert np.array_equal(e.labels(), f.labels())
+
+
+@pytest.mark.parametrize("case", ["chain_mc1", "ladder_mc3", "pbc_cells"])
+def test_graph_steps_equal_host_steps(case):
+    """The per-phase form replayed from captured graphs (device step count,
+    MC flags and skin test, halting for rebuilds) gives the same state as
+    the same kernels launched step by step from the host: one chain with MC
+    every step (0.3 A skin), a 4-replica ladder with MC every 3 steps and
+    exchanges, and a periodic condensate on the cell list (1 A skins), so
+    that rebuilds fall inside the replays."""
+    if case == "chain_mc1":
+        s, nex = configs.remd_c3(nrep=1, n=128, seed=3, margin=0.3), 0
+    elif case == "ladder_mc3":
+        s, nex = configs.remd_c3(nrep=4, n=128, seed=3, margin=1.0), 25
+        s["params"] = dict(s["params"], nstep_bp_mc=3)
+    else:
+        s, nex = configs.cag_system(40, 47, 1.0, 310.0, margin=1.0), 0
+    print(f"[case] graph steps {case}", flush=True)
+    out = []
+    for no_graph in (True, False):
+        e = Engine(s, mode="phases", precision=1, seed=SEED, exchange_every=nex, no_graph=no_graph)
+        e.init_velocities()
+        e.neighbor_list(0)
+        e.step(37)
+        e.step(163)
+        out.append((e.get_state(), e.labels() if s["nrep"] > 1 else None, e.info()["rebuilds"], e.bp(1)["status"]))
+    (xa, va, aa), la, ra, ba = out[0]
+    (xb, vb, ab), lb, rb, bb = out[1]
+    assert ra == rb and ra > 2
+    assert np.array_equal(xa, xb) and np.array_equal(va, vb) and np.array_equal(aa, ab)
+    assert np.array_equal(ba, bb)
+    if la is not None:
+        assert np.array_equal(la, lb)
