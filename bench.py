@@ -378,6 +378,17 @@ def main():
     info2 = eng.info()
     x, v, a = eng.get_state()
     assert np.all(np.isfinite(x)), "non-finite coordinates"
+    mode_used0 = {1: "phases", 2: "persistent", 3: "batched"}.get(info2["mode"], "")
+    if mode_used0 == "phases" and dom_n == 0:
+        # replayed graphs carry no per-kernel events: time the pair /
+        # integrator kernel on host-launched steps of the same state
+        pe = Engine(sysd, precision=prec, parity=False, mode="phases", seed=2024, no_graph=True,
+                    exchange_every=int(sysd.get("exchange_every", 0)))
+        pe.set_state(x, v, a)
+        pe.neighbor_list(0)
+        pe.step(min(steps, 50))
+        dom_ms, dom_n = pe.last_ms(3), int(pe.last_ms(4))
+        pe.close()
     eng.close()
     print(f"[bench] timed {steps} steps {wall:.3f}s", flush=True)
     bead_steps = nmp * nrep * steps
@@ -387,7 +398,9 @@ def main():
     p_bp = info2["nbp"] / nmp
     B = bytes_per_bead_step(p_wca, p_ele, p_bp)
     mode_used = {1: "phases", 2: "persistent", 3: "batched"}.get(info2["mode"], str(info2["mode"]))
-    steps_per_launch = steps / max(dom_n, 1)
+    # one k_force_lanes launch per step in the per-phase form; the batched
+    # and persistent kernels run whole segments per launch
+    steps_per_launch = 1 if mode_used == "phases" else steps / max(dom_n, 1)
     if mode_used == "phases":
         kname, kbytes = "k_force_lanes", force_kernel_bytes(p_wca, p_ele)
     else:

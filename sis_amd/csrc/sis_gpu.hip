@@ -650,61 +650,77 @@ void rebuild(sis_gpu_ctx *c, const std::vector<int> &mask_in) {
 // already on the device (replayed graph steps, whose step number is
 // step_dev's)
 void bp_phase(sis_gpu_ctx *c, const std::vector<int> *mc, const double *ecT_dev, const int *mask_dev,
-              const int64_t *step_dev = nullptr) {
+              const int64_t *step_dev = nullptr, bool resets_done = false) {
   const int N = c->N, R = c->R, B = 256;
   if (mc) set_mc(c, *mc);
   Ptrs D = c->ptrs();
-  hipLaunchKernelGGL(k_reset_cnt, grid_beads(N, R, B), dim3(B), 0, c->stream, c->P, D, mask_dev);
-  if (c->nbp_total > 0) {
-    if (c->precision == 1)
-      hipLaunchKernelGGL(k_bp_eval<float>, dim3((c->nbp_total + B - 1) / B), dim3(B), 0, c->stream, c->P, D,
-                         c->nbp_total, c->mc_d.p, ecT_dev, mask_dev);
-    else
-      hipLaunchKernelGGL(k_bp_eval<double>, dim3((c->nbp_total + B - 1) / B), dim3(B), 0, c->stream, c->P, D,
-                         c->nbp_total, c->mc_d.p, ecT_dev, mask_dev);
-  }
   bool any_mc = !mc;
   if (mc)
     for (int r = 0; r < R; r++) any_mc |= (*mc)[r] != 0;
-  if (!any_mc || c->P.nstep_bp_mc <= 0) return;
+  any_mc = any_mc && c->P.nstep_bp_mc > 0;
+  // performance-form MC: counts, union-find and buckets reset in one pass,
+  // the union done by the evaluation itself, the components with excess
+  // listed for k_mc_list
+  const bool perf_mc = any_mc && !c->parity;
+  const size_t RN = (size_t)R * N;
+  if (perf_mc) {
+    if (c->seg_d.n < RN * SEGCAP) c->seg_d.alloc(RN * SEGCAP);
+    if (c->exc_list_d.n < RN + 1) c->exc_list_d.alloc(RN + 1);
+    if (!resets_done)
+      hipLaunchKernelGGL(k_mc_reset, grid_beads(N, R, B), dim3(B), 0, c->stream, c->P, D, c->exc_cnt_d.p,
+                         c->exc_fill_d.p, c->exc_list_d.p, mask_dev);
+  } else if (!resets_done) {
+    hipLaunchKernelGGL(k_reset_cnt, grid_beads(N, R, B), dim3(B), 0, c->stream, c->P, D, mask_dev);
+  }
+  if (c->nbp_total > 0) {
+    if (c->precision == 1)
+      hipLaunchKernelGGL(k_bp_eval<float>, dim3((c->nbp_total + B - 1) / B), dim3(B), 0, c->stream, c->P, D,
+                         c->nbp_total, c->mc_d.p, ecT_dev, mask_dev, perf_mc ? 1 : 0);
+    else
+      hipLaunchKernelGGL(k_bp_eval<double>, dim3((c->nbp_total + B - 1) / B), dim3(B), 0, c->stream, c->P, D,
+                         c->nbp_total, c->mc_d.p, ecT_dev, mask_dev, perf_mc ? 1 : 0);
+  }
+  if (!any_mc) return;
   if (c->parity) {
     hipLaunchKernelGGL(k_mc_serial, dim3(R), dim3(64), 0, c->stream, c->P, D, c->mc_d.p, ecT_dev,
                        c->ntlist_d.p, c->bpseq_d.p, mask_dev);
   } else {
     if (c->nbp_total == 0) return;
-    size_t RN = (size_t)R * N;
-    if (c->seg_d.n < RN * SEGCAP) c->seg_d.alloc(RN * SEGCAP);
-    hipLaunchKernelGGL(k_comp_init, grid_beads(N, R, B), dim3(B), 0, c->stream, c->P, D, c->exc_cnt_d.p,
-                       c->exc_fill_d.p, mask_dev);
-    hipLaunchKernelGGL(k_uf_union, dim3((c->nbp_total + B - 1) / B), dim3(B), 0, c->stream, c->P, D,
-                       c->nbp_total, c->mc_d.p, mask_dev);
     hipLaunchKernelGGL(k_seg_fill, dim3((c->nbp_total + B - 1) / B), dim3(B), 0, c->stream, c->P, D,
-                       c->nbp_total, c->exc_cnt_d.p, c->seg_d.p, c->exc_fill_d.p, c->mc_d.p, mask_dev);
-    hipLaunchKernelGGL(k_mc_wave, dim3(N, R), dim3(64), 0, c->stream, c->P, D, c->exc_cnt_d.p, c->seg_d.p,
-                       c->exc_fill_d.p, c->mc_d.p, ecT_dev, c->seed, c->istep, mask_dev, c->huge_d.p, step_dev);
+                       c->nbp_total, c->exc_cnt_d.p, c->seg_d.p, c->exc_fill_d.p, c->mc_d.p, mask_dev,
+                       c->exc_list_d.p);
+    const int nlist = (int)std::min<int64_t>((int64_t)std::max<int64_t>(RN, 1), 2048);
+    hipLaunchKernelGGL(k_mc_list, dim3(nlist), dim3(64), 0, c->stream, c->P, D, c->exc_cnt_d.p, c->seg_d.p,
+                       c->exc_list_d.p, ecT_dev, c->seed, c->istep, c->huge_d.p, step_dev);
     hipLaunchKernelGGL(k_mc_huge, dim3(R), dim3(64), 0, c->stream, c->P, D, c->exc_cnt_d.p, c->seg_d.p,
                        c->exc_fill_d.p, c->huge_d.p, ecT_dev, c->seed, c->istep, step_dev);
   }
 }
 
-void force_kernel(sis_gpu_ctx *c, int integrate, const int *mask_dev, const int64_t *step_dev = nullptr) {
+// mark: time the pair/integrator kernel with the context's event pairs
+void force_kernel(sis_gpu_ctx *c, int integrate, const int *mask_dev, const int64_t *step_dev = nullptr,
+                  bool mark = false) {
   const int N = c->N, R = c->R;
   constexpr int B = 256, LPB = 8;
   dim3 g((N + B / LPB - 1) / (B / LPB), R);
   c->nblk_force = g.x;
   if ((int64_t)c->blk_top2_d.n < (int64_t)R * g.x * 2) c->blk_top2_d.alloc((size_t)R * g.x * 2);
-  hipLaunchKernelGGL((k_bonded<LPB, B>), g, dim3(B), 0, c->stream, c->P, c->ptrs(), mask_dev);
+  // bonded forces in the pair kernel, unless external terms add to them
+  const int inl = c->ext_on ? 0 : 1;
+  if (!inl) hipLaunchKernelGGL((k_bonded<LPB, B>), g, dim3(B), 0, c->stream, c->P, c->ptrs(), mask_dev);
   if (c->ext_on) {
     if (c->X.rg) hipLaunchKernelGGL(k_rg_stats<256>, dim3(R), dim3(256), 0, c->stream, c->P, c->ptrs(), c->X);
     hipLaunchKernelGGL(k_external<256>, grid_beads(N, R, 256), dim3(256), 0, c->stream, c->P, c->ptrs(), c->X,
                        c->istep, mask_dev);
   }
+  if (mark) c->tmark();
   if (c->precision == 1)
     hipLaunchKernelGGL((k_force_lanes<float, LPB, B>), g, dim3(B), 0, c->stream, c->P, c->ptrs(), integrate,
-                       c->parity, c->seed, c->istep, mask_dev, step_dev);
+                       c->parity, c->seed, c->istep, mask_dev, step_dev, inl);
   else
     hipLaunchKernelGGL((k_force_lanes<double, LPB, B>), g, dim3(B), 0, c->stream, c->P, c->ptrs(), integrate,
-                       c->parity, c->seed, c->istep, mask_dev, step_dev);
+                       c->parity, c->seed, c->istep, mask_dev, step_dev, inl);
+  if (mark) c->tmark();
 }
 
 void check_flags(sis_gpu_ctx *c) {
@@ -984,9 +1000,7 @@ void step_phases(sis_gpu_ctx *c) {
   }
   if (c->parity)
     hipLaunchKernelGGL(k_noise_parity, dim3(1), dim3(1), 0, c->stream, c->P, c->ptrs(), 3 * N, 0, nullptr);
-  c->tmark();
-  force_kernel(c, 1, nullptr);
-  c->tmark();
+  force_kernel(c, 1, nullptr, nullptr, true);
   std::swap(c->pos_d, c->pos2_d);   // updated positions become current
   hipLaunchKernelGGL(k_skin<256>, dim3(R), dim3(256), 0, c->stream, c->P, c->ptrs(), c->nblk_force, GStep{});
 }
@@ -1009,7 +1023,10 @@ constexpr int GRAPH_STEPS = 16;   // even: the positions end in the buffer they 
 void phases_graph_steps(sis_gpu_ctx *c, int64_t nsteps) {
   const int R = c->R;
   if (c->g_step_d.n < 3) c->g_step_d.alloc(3);
-  if (c->g_int_d.n < (size_t)(1 + 2 * R)) c->g_int_d.alloc(1 + 2 * R);
+  if (c->g_int_d.n < (size_t)(1 + 3 * R)) {
+    c->g_int_d.alloc(1 + 3 * R);
+    HIPCHK(hipMemset(c->g_int_d.p, 0, (1 + 3 * (size_t)R) * sizeof(int)));
+  }
   GStep G{};
   G.istep = c->g_step_d.p;
   G.istep_cur = c->g_step_d.p + 1;
@@ -1046,13 +1063,17 @@ void phases_graph_steps(sis_gpu_ctx *c, int64_t nsteps) {
     const int nblk = (c->N + 31) / 32;   // force_kernel: 32 beads per workgroup
     if ((int64_t)c->blk_top2_d.n < (int64_t)R * nblk * 2) c->blk_top2_d.alloc((size_t)R * nblk * 2);
     if (c->P.nstep_bp_mc > 0 && c->seg_d.n < (size_t)R * c->N * SEGCAP) c->seg_d.alloc((size_t)R * c->N * SEGCAP);
+    if (c->P.nstep_bp_mc > 0 && c->exc_list_d.n < (size_t)R * c->N + 1) c->exc_list_d.alloc((size_t)R * c->N + 1);
     // capture GRAPH_STEPS steps from the current position buffer
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    const bool perf_mc = c->P.nstep_bp_mc > 0;
     for (int j = 0; j < GRAPH_STEPS; j++) {
-      hipLaunchKernelGGL(k_gstep_begin, dim3((R + 255) / 256), dim3(256), 0, c->stream, c->P, G);
-      bp_phase(c, nullptr, nullptr, G.active, G.istep_cur);
+      // begin + resets, base pairs (+ MC), pair forces + update, skin test
+      hipLaunchKernelGGL(k_gstep_reset, grid_beads(c->N, R, 256), dim3(256), 0, c->stream, c->P, c->ptrs(), G,
+                         c->exc_cnt_d.p, c->exc_fill_d.p, perf_mc ? c->exc_list_d.p : nullptr, perf_mc ? 1 : 0);
+      bp_phase(c, nullptr, nullptr, G.active, G.istep_cur, true);
       if (c->P.nstep_bp_mc > 1 && c->nbp_total > 0)
         hipLaunchKernelGGL(k_mc_keep, dim3((c->nbp_total + 255) / 256), dim3(256), 0, c->stream, c->ptrs(),
                            (int)c->nbp_total, G.mc, G.active);
@@ -1332,7 +1353,7 @@ int sis_gpu_create(const sis_topo *topo, const sis_params *prm, const sis_option
     c->blk_top2_d.alloc((size_t)R * ((N + 255) / 256 + 1) * 2);
     c->rebuild_flag_d.alloc(R); c->mask_d.alloc(R); c->mc_d.alloc(R); c->huge_d.alloc(R);
     c->scan_total_d.alloc(1);
-    c->exc_cnt_d.alloc(RN); c->exc_off_d.alloc(RN + 1); c->exc_fill_d.alloc(RN); c->exc_list_d.alloc(RN);
+    c->exc_cnt_d.alloc(RN); c->exc_off_d.alloc(RN + 1); c->exc_fill_d.alloc(RN); c->exc_list_d.alloc(RN + 1);   // exc_list: [0] count, then the components with excess
     c->ntlist_d.alloc((size_t)R * (N + 1));
     c->noise_d.alloc(RN * 3);
     c->bm_cache_d.alloc(2);
@@ -1548,10 +1569,10 @@ int sis_gpu_energy_noalloc(sis_gpu_ctx *c, int32_t irep, double tempK, int32_t f
     if (c->nbp_total > 0) {
       if (c->precision == 1)
         hipLaunchKernelGGL(k_bp_eval<float>, dim3((c->nbp_total + B - 1) / B), dim3(B), 0, c->stream, c->P, D,
-                           c->nbp_total, mc, ecT, mask);
+                           c->nbp_total, mc, ecT, mask, 0);
       else
         hipLaunchKernelGGL(k_bp_eval<double>, dim3((c->nbp_total + B - 1) / B), dim3(B), 0, c->stream, c->P, D,
-                           c->nbp_total, mc, ecT, mask);
+                           c->nbp_total, mc, ecT, mask, 0);
     }
   }
   hipLaunchKernelGGL(k_energy<B>, dim3(nblk, 1), dim3(B), 0, c->stream, c->P, D, ecT, nullptr, nullptr, partial,

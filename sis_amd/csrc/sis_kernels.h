@@ -387,13 +387,23 @@ __global__ void k_reset_cnt(Params P, Ptrs D, const int *mask) {
   D.nt_cnt[(size_t)r * P.N + i] = -P.max_bp_per_nt;
 }
 
+__device__ __forceinline__ int uf_find(int *p, int x) {
+  while (true) {
+    int q = p[x];
+    if (q == x) return x;
+    int qq = p[q];
+    if (qq != q) atomicCAS(&p[x], q, qq);
+    x = q;
+  }
+}
+
 // ---------------------------------------------------------------------
 // BP evaluation per candidate (src/force_bp_limit_triplet.F90:85-256 and
 // src/force_bp_triplet.F90).  mc[r]: MC step for replica r.  ecT (may be
 // null): temperature override per replica (energy at another condition).
 template <typename T>
 __global__ void k_bp_eval(Params P, Ptrs D, int nbp_total, const int *mc, const double *ecT,
-                          const int *mask) {
+                          const int *mask, int do_union) {
   int gb = blockIdx.x * blockDim.x + threadIdx.x;
   if (gb >= nbp_total) return;
   const int *mp = D.bp_mp + (size_t)gb * 8;
@@ -433,6 +443,19 @@ __global__ void k_bp_eval(Params P, Ptrs D, int nbp_total, const int *mc, const 
     }
     atomicAdd(&D.nt_cnt[(size_t)r * N + mp[0] - 1], 1);
     atomicAdd(&D.nt_cnt[(size_t)r * N + mp[1] - 1], 1);
+    // performance-form MC: join the pair's nucleotides into one conflict
+    // component (union-find, larger root linked under the smaller)
+    if (do_union && mc[r]) {
+      int *pc = D.comp + (size_t)r * N;
+      int ua = mp[0] - 1, ub = mp[1] - 1;
+      while (true) {
+        ua = uf_find(pc, ua);
+        ub = uf_find(pc, ub);
+        if (ua == ub) break;
+        if (ua > ub) { int t = ua; ua = ub; ub = t; }
+        if (atomicCAS(&pc[ub], ub, ua) == ub) break;
+      }
+    }
   }
 }
 
@@ -500,15 +523,6 @@ __global__ void k_mc_serial(Params P, Ptrs D, const int *mc, const double *ecT, 
 // the uniform choice among all excess nucleotides restricted to one
 // component is uniform within it.  Each component is therefore resolved
 // independently by one thread with its own counter-RNG stream.
-__device__ __forceinline__ int uf_find(int *p, int x) {
-  while (true) {
-    int q = p[x];
-    if (q == x) return x;
-    int qq = p[q];
-    if (qq != q) atomicCAS(&p[x], q, qq);
-    x = q;
-  }
-}
 __global__ void k_uf_init(Params P, Ptrs D, int *cnt_exc, int *fill, const int *mask) {
   int i = blockIdx.x * blockDim.x + threadIdx.x, r = blockIdx.y;
   if (i >= P.N || (mask && !mask[r])) return;
@@ -624,8 +638,21 @@ __global__ void k_comp_init(Params P, Ptrs D, int *seg_n, int *has_exc, const in
   seg_n[g] = 0;
   has_exc[g] = 0;
 }
+// the per-step state of the performance-form MC in one pass: formed-pair
+// counts (k_reset_cnt), union-find parents and buckets (k_comp_init), and
+// the list of components with excess (its count in exc_list[0])
+__global__ void k_mc_reset(Params P, Ptrs D, int *seg_n, int *has_exc, int *exc_list, const int *mask) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x, r = blockIdx.y;
+  if (i == 0 && r == 0) exc_list[0] = 0;
+  if (i >= P.N || (mask && !mask[r])) return;
+  size_t g = (size_t)r * P.N + i;
+  D.nt_cnt[g] = -P.max_bp_per_nt;
+  D.comp[g] = i;
+  seg_n[g] = 0;
+  has_exc[g] = 0;
+}
 __global__ void k_seg_fill(Params P, Ptrs D, int nbp_total, int *seg_n, int *seg, int *has_exc,
-                           const int *mc, const int *mask) {
+                           const int *mc, const int *mask, int *exc_list) {
   int gb = blockIdx.x * blockDim.x + threadIdx.x;
   if (gb >= nbp_total || !D.bp_status[gb]) return;
   const int *mp = D.bp_mp + (size_t)gb * 8;
@@ -637,7 +664,10 @@ __global__ void k_seg_fill(Params P, Ptrs D, int nbp_total, int *seg_n, int *seg
   size_t gr = (size_t)r * N + root;
   int slot = atomicAdd(&seg_n[gr], 1);
   if (slot < SEGCAP) seg[gr * SEGCAP + slot] = gb;
-  if (D.nt_cnt[(size_t)r * N + a] > 0 || D.nt_cnt[(size_t)r * N + b] > 0) has_exc[gr] = 1;
+  if (D.nt_cnt[(size_t)r * N + a] > 0 || D.nt_cnt[(size_t)r * N + b] > 0) {
+    // listed once (exc_list: [0] count, then replica * N + root)
+    if (atomicExch(&has_exc[gr], 1) == 0 && exc_list) exc_list[1 + atomicAdd(&exc_list[0], 1)] = (int)gr;
+  }
 }
 // serial fallback for a component larger than one wave holds: the same
 // algorithm on global memory by one lane
@@ -926,6 +956,31 @@ __global__ void __launch_bounds__(64) k_mc_huge(Params P, Ptrs D, const int *seg
   }
   if (L == 0) huge[r] = 0;
 }
+// one conflict component (root of replica r) resolved by a 64-thread
+// block: one lane, one wavefront, or the serial fallback; components above
+// the bucket capacity are flagged for k_mc_huge
+__device__ __forceinline__ void mc_one_component(const Params &P, const Ptrs &D, int r, int root, int *seg_n,
+                                                 int *seg, const double *ecT, uint64_t seed, int64_t step,
+                                                 int *huge, int *ws) {
+  size_t gr = (size_t)r * P.N + root;
+  int n = seg_n[gr];
+  double tK = ecT ? ecT[r] : D.tK[r];
+  double beta = mc_beta(P, tK, ecT == nullptr);
+  if (n > SEGCAP) {   // above the bucket capacity: k_mc_huge takes it
+    if (threadIdx.x == 0) huge[r] = 1;
+    return;
+  }
+  int *sg = seg + gr * SEGCAP;
+  if (n <= MCL_PAIRS) {   // small component: one lane, same result
+    if (threadIdx.x == 0 && !mc_component_lane(D, P.N, r, r, root, sg, n, beta, seed, step))
+      mc_component_serial(mc_arrays(D), P.N, r, r, root, sg, n, beta, seed, step);
+    return;
+  }
+  const bool ok = n <= 64 * MCW_ITEMS ? mc_component_wave_any(P, D, r, r, root, sg, n, beta, seed, step, ws)
+                                      : mc_component_wave<MCB_ITEMS, MCB_NODES>(P, D, r, r, root, sg, n, beta, seed, step, ws);
+  if (!ok && threadIdx.x == 0) mc_component_serial(mc_arrays(D), P.N, r, r, root, sg, n, beta, seed, step);
+}
+
 __global__ void __launch_bounds__(64) k_mc_wave(Params P, Ptrs D, int *seg_n, int *seg,
                                                 const int *has_exc, const int *mc,
                                                 const double *ecT, uint64_t seed, int64_t step0,
@@ -935,23 +990,24 @@ __global__ void __launch_bounds__(64) k_mc_wave(Params P, Ptrs D, int *seg_n, in
   const int64_t step = step_dev ? *step_dev : step0;   // graph replays read the step from the device
   size_t gr = (size_t)r * P.N + root;
   if (!has_exc[gr]) return;
-  int n = seg_n[gr];
-  double tK = ecT ? ecT[r] : D.tK[r];
-  double beta = mc_beta(P, tK, ecT == nullptr);
-  if (n > SEGCAP) {   // above the bucket capacity: k_mc_huge takes it
-    if (threadIdx.x == 0) huge[r] = 1;
-    return;
-  }
-  int *sg = seg + gr * SEGCAP;
   __shared__ int ws[MCB_LDS];
-  if (n <= MCL_PAIRS) {   // small component: one lane, same result
-    if (threadIdx.x == 0 && !mc_component_lane(D, P.N, r, r, root, sg, n, beta, seed, step))
-      mc_component_serial(mc_arrays(D), P.N, r, r, root, sg, n, beta, seed, step);
-    return;
+  mc_one_component(P, D, r, root, seg_n, seg, ecT, seed, step, huge, ws);
+}
+
+// The components listed by k_seg_fill (exc_list: [0] count, then
+// replica * N + root), a grid-stride loop of 64-thread blocks: the same
+// per-component work as k_mc_wave without a block per nucleotide.
+__global__ void __launch_bounds__(64) k_mc_list(Params P, Ptrs D, int *seg_n, int *seg, const int *exc_list,
+                                                const double *ecT, uint64_t seed, int64_t step0, int *huge,
+                                                const int64_t *step_dev) {
+  const int64_t step = step_dev ? *step_dev : step0;
+  const int n = exc_list[0];
+  __shared__ int ws[MCB_LDS];
+  for (int k = blockIdx.x; k < n; k += gridDim.x) {
+    const int gr = exc_list[1 + k];
+    const int r = gr / P.N, root = gr - r * P.N;
+    mc_one_component(P, D, r, root, seg_n, seg, ecT, seed, step, huge, ws);
   }
-  const bool ok = n <= 64 * MCW_ITEMS ? mc_component_wave_any(P, D, r, r, root, sg, n, beta, seed, step, ws)
-                                      : mc_component_wave<MCB_ITEMS, MCB_NODES>(P, D, r, r, root, sg, n, beta, seed, step, ws);
-  if (!ok && threadIdx.x == 0) mc_component_serial(mc_arrays(D), P.N, r, r, root, sg, n, beta, seed, step);
 }
 
 // ---------------------------------------------------------------------
@@ -1347,45 +1403,6 @@ __global__ void __launch_bounds__(B) k_bonded(Params P, Ptrs D, const int *mask)
   if (act && l == 0) D.fbond[(size_t)r * P.N + a] = double4{F.x, F.y, F.z, 0.0};
 }
 
-template <typename T, int LPB, int B>
-__global__ void __launch_bounds__(B) k_force_lanes(Params P, Ptrs D, int integrate, int parity,
-                                                  uint64_t seed, int64_t step0, const int *mask,
-                                                  const int64_t *step_dev) {
-  __shared__ double s1[B / LPB], s2[B / LPB];
-  const int64_t step = step_dev ? *step_dev : step0;   // graph replays read the step from the device
-  const int grp = threadIdx.x / LPB, l = threadIdx.x % LPB;
-  const int a = blockIdx.x * (B / LPB) + grp, r = blockIdx.y;
-  const int N = P.N;
-  const bool act = a < N && !(mask && !mask[r]);
-  V3<double> F{0.0, 0.0, 0.0};
-  if (act) F = pair_part<T>(P, D, r, a, l, LPB);
-  F = group_sum<LPB>(F);
-  double d2 = 0.0;
-  if (act && l == 0) {
-    size_t g = (size_t)r * N + a;
-    const double4 fb = D.fbond[g];
-    F.x += fb.x; F.y += fb.y; F.z += fb.z;
-    D.nt_cnt[g] = -P.max_bp_per_nt;
-    if (!integrate) D.frc[g] = double4{F.x, F.y, F.z, 0.0};
-    else d2 = gjf_update(P, D, r, r, a, F, parity, seed, step);
-  }
-  if (!integrate) return;
-  if (l == 0) { s1[grp] = d2; s2[grp] = 0.0; }
-  __syncthreads();
-  for (int off = (B / LPB) / 2; off > 0; off >>= 1) {
-    if (threadIdx.x < off) {
-      double a1 = s1[threadIdx.x], a2 = s2[threadIdx.x], b1 = s1[threadIdx.x + off], b2 = s2[threadIdx.x + off];
-      s1[threadIdx.x] = a1 > b1 ? a1 : b1;
-      s2[threadIdx.x] = a1 > b1 ? (a2 > b1 ? a2 : b1) : (a1 > b2 ? a1 : b2);
-    }
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) {
-    D.blk_top2[((size_t)r * gridDim.x + blockIdx.x) * 2] = s1[0];
-    D.blk_top2[((size_t)r * gridDim.x + blockIdx.x) * 2 + 1] = s2[0];
-  }
-}
-
 // top two of (a1 >= a2) and (b1 >= b2): the two largest values of the four
 __device__ __forceinline__ void top2_merge(double &m1, double &m2, double b1, double b2) {
   const double n1 = m1 > b1 ? m1 : b1;
@@ -1407,16 +1424,13 @@ struct GStep {
   int *mc;               // [R] limit MC on the current step
 };
 
-// skin test per replica (src/job_md.F90:376-395): sqrt(d2max) +
-// sqrt(d2max_2nd) > nl_margin over the per-block top-2 displacements,
-// reduced by a workgroup (the top two are the same values in any order).
-// With a graph step: a replica needing a rebuild halts the steps that
-// follow, and the step is committed.
+// the skin decision of replica r from its nblk block maxima, reduced by the
+// calling workgroup (B threads, s1/s2 of B doubles); with a graph step
+// also the halt and the step's commit.  Called by every thread.
 template <int B>
-__global__ void __launch_bounds__(B) k_skin(Params P, Ptrs D, int nblk, GStep G) {
-  __shared__ double s1[B], s2[B];
-  const int r = blockIdx.x, t = threadIdx.x;
-  if (G.halt && !G.active[r]) return;
+__device__ __forceinline__ void skin_decide(const Params &P, const Ptrs &D, int r, int nblk, const GStep &G,
+                                            double *s1, double *s2) {
+  const int t = threadIdx.x;
   double m1 = 0.0, m2 = 0.0;
   for (int b = t; b < nblk; b += B)
     top2_merge(m1, m2, D.blk_top2[((size_t)r * nblk + b) * 2], D.blk_top2[((size_t)r * nblk + b) * 2 + 1]);
@@ -1444,6 +1458,65 @@ __global__ void __launch_bounds__(B) k_skin(Params P, Ptrs D, int nblk, GStep G)
   }
 }
 
+template <typename T, int LPB, int B>
+__global__ void __launch_bounds__(B) k_force_lanes(Params P, Ptrs D, int integrate, int parity,
+                                                  uint64_t seed, int64_t step0, const int *mask,
+                                                  const int64_t *step_dev, int bonded_inline) {
+  __shared__ double s1[B / LPB], s2[B / LPB];
+  const int64_t step = step_dev ? *step_dev : step0;   // graph replays read the step from the device
+  const int grp = threadIdx.x / LPB, l = threadIdx.x % LPB;
+  const int a = blockIdx.x * (B / LPB) + grp, r = blockIdx.y;
+  const int N = P.N;
+  const bool act = a < N && !(mask && !mask[r]);
+  // bonded forces here (bonded_inline) or from k_bonded's D.fbond (which
+  // external terms add to): the same lanes and sums either way
+  V3<double> Fb{0.0, 0.0, 0.0};
+  if (bonded_inline) {
+    if (act) Fb = bonded_part(P, D, r, a, l, LPB);
+    Fb = group_sum<LPB>(Fb);
+  }
+  V3<double> F{0.0, 0.0, 0.0};
+  if (act) F = pair_part<T>(P, D, r, a, l, LPB);
+  F = group_sum<LPB>(F);
+  double d2 = 0.0;
+  if (act && l == 0) {
+    size_t g = (size_t)r * N + a;
+    const double4 fb = bonded_inline ? double4{Fb.x, Fb.y, Fb.z, 0.0} : D.fbond[g];
+    F.x += fb.x; F.y += fb.y; F.z += fb.z;
+    D.nt_cnt[g] = -P.max_bp_per_nt;
+    if (!integrate) D.frc[g] = double4{F.x, F.y, F.z, 0.0};
+    else d2 = gjf_update(P, D, r, r, a, F, parity, seed, step);
+  }
+  if (!integrate) return;
+  if (l == 0) { s1[grp] = d2; s2[grp] = 0.0; }
+  __syncthreads();
+  for (int off = (B / LPB) / 2; off > 0; off >>= 1) {
+    if (threadIdx.x < off) {
+      double a1 = s1[threadIdx.x], a2 = s2[threadIdx.x], b1 = s1[threadIdx.x + off], b2 = s2[threadIdx.x + off];
+      s1[threadIdx.x] = a1 > b1 ? a1 : b1;
+      s2[threadIdx.x] = a1 > b1 ? (a2 > b1 ? a2 : b1) : (a1 > b2 ? a1 : b2);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    D.blk_top2[((size_t)r * gridDim.x + blockIdx.x) * 2] = s1[0];
+    D.blk_top2[((size_t)r * gridDim.x + blockIdx.x) * 2 + 1] = s2[0];
+  }
+}
+
+// skin test per replica (src/job_md.F90:376-395): sqrt(d2max) +
+// sqrt(d2max_2nd) > nl_margin over the per-block top-2 displacements,
+// reduced by a workgroup (the top two are the same values in any order).
+// With a graph step: a replica needing a rebuild halts the steps that
+// follow, and the step is committed.
+template <int B>
+__global__ void __launch_bounds__(B) k_skin(Params P, Ptrs D, int nblk, GStep G) {
+  __shared__ double s1[B], s2[B];
+  const int r = blockIdx.x;
+  if (G.halt && !G.active[r]) return;
+  skin_decide<B>(P, D, r, nblk, G, s1, s2);
+}
+
 // start of a replayed step: nothing if halted; else the step number and
 // the limit-MC flags (every nstep_MC steps and after a rebuild,
 // src/job_md.F90:364-394)
@@ -1457,6 +1530,33 @@ __global__ void k_gstep_begin(Params P, GStep G) {
   if (r == 0) *G.istep_cur = s;
   G.mc[r] = P.nstep_bp_mc > 0 && ((s % P.nstep_bp_mc) == 0 || G.force_mc[r]) ? 1 : 0;
   G.force_mc[r] = 0;
+}
+
+// start of a replayed step and its resets in one pass: the step number,
+// limit-MC flags and mask (as k_gstep_begin), then for an active step the
+// formed-pair counts and, for the performance-form MC, union-find parents,
+// buckets and the list of components with excess (as k_mc_reset)
+__global__ void k_gstep_reset(Params P, Ptrs D, GStep G, int *seg_n, int *has_exc, int *exc_list, int mc_reset) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x, r = blockIdx.y;
+  const bool halted = *G.halt != 0;
+  if (i == 0) {
+    G.active[r] = halted ? 0 : 1;
+    if (!halted) {
+      const int64_t s = *G.istep + 1;
+      if (r == 0) *G.istep_cur = s;
+      G.mc[r] = P.nstep_bp_mc > 0 && ((s % P.nstep_bp_mc) == 0 || G.force_mc[r]) ? 1 : 0;
+      G.force_mc[r] = 0;
+    }
+    if (r == 0 && exc_list) exc_list[0] = 0;
+  }
+  if (halted || i >= P.N) return;
+  const size_t g = (size_t)r * P.N + i;
+  D.nt_cnt[g] = -P.max_bp_per_nt;
+  if (mc_reset) {
+    D.comp[g] = i;
+    seg_n[g] = 0;
+    has_exc[g] = 0;
+  }
 }
 
 // bp_status_MC = bp_status for the replicas that did the MC this step
