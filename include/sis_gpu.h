@@ -132,6 +132,17 @@ int sis_gpu_set_rep_conditions(sis_gpu_ctx *ctx, const double *tempK, const doub
  * (src/neighbor_list.F90:1-233, called from src/job_md.F90:197,389,584). */
 int sis_gpu_rebuild(sis_gpu_ctx *ctx, int32_t irep);
 
+/* The lists of the jobs other than MD (src/main.F90:144-154: list_bp,
+ * list_exv, list_ele; src/list_bp.F90:41-93, src/list_exv.F90:19-64,
+ * src/list_ele.F90:18-65) for every replica: no circular wrap in the
+ * exclusions, list_ele's bond / angle exclusions on every chain pair,
+ * base-pair candidates with jmp >= imp + bp_min_loop + 1 over the whole
+ * system in (imp, jmp) order.  Force and energy calls then use them; the
+ * next step rebuilds the neighbour lists of the MD job.  Pairs beyond the
+ * neighbour cutoffs are not listed (they have no energy, force or formed
+ * base pair). */
+int sis_gpu_analysis_lists(sis_gpu_ctx *ctx);
+
 /* force(irep, forces) (src/force.F90:1-77); bp_mc = flg_bp_MC.
  * forces is (3, nmp) or NULL (keep on device). */
 int sis_gpu_force(sis_gpu_ctx *ctx, int32_t irep, int32_t bp_mc, double *forces);
@@ -372,7 +383,8 @@ int sis_gpu_rst_read(sis_gpu_ctx *ctx, const char *const *paths);
 
 /* job_dcd (src/job_dcd.F90:59-80): each frame of the DCD file at path
  * (read as src/dcd.F90 read_onestep does, float32 to double) becomes the
- * coordinates of replica 1; the lists are rebuilt and the energies
+ * coordinates of replica 1; the lists of sis_gpu_analysis_lists are built
+ * and the energies
  * evaluated with the limit MC when nstep_bp_MC > 0.  energies (11,
  * max_frames) receives them, *nframes the frame count; with out_path the
  * .out lines of job_dcd are written (frame number, T, Ekin = 0). */
@@ -381,7 +393,8 @@ int sis_gpu_dcd_energies(sis_gpu_ctx *ctx, const char *path, const char *out_pat
 
 /* job_check_force (src/job_check_force.F90:46-82): forces of replica irep
  * (f_force, 3 x nmp) against central differences of its total energy with
- * step h (f_energy, 3 x nmp); the limit MC as the run is configured.  With
+ * step h (f_energy, 3 x nmp), on the lists of sis_gpu_analysis_lists; the
+ * limit MC as the run is configured.  With
  * out_path the per-bead lines of job_check_force are written. */
 int sis_gpu_check_force(sis_gpu_ctx *ctx, int32_t irep, double h, const char *out_path, double *f_energy,
                         double *f_force);

@@ -51,6 +51,7 @@ def lib():
         L.orc_plant.argtypes = [vp, i32, i32]
         L.orc_set_counter_rng.argtypes = [vp, C.c_uint64, i32]
         L.orc_set_nl_brute.argtypes = [vp, i32]
+        L.orc_analysis_lists.argtypes = [vp, i32]
         L.orc_philox.argtypes = [P(C.c_uint32), P(C.c_uint32), P(C.c_uint32)]
         L.orc_set_rep_rg.argtypes = [vp, P(d)]
         L.orc_set_rep_dcf.argtypes = [vp, P(d), P(d)]
@@ -234,6 +235,11 @@ class Oracle:
         """Neighbour lists by the reference's O(N^2) loops even where the
         cell-list enumeration would be used (large periodic systems)."""
         lib().orc_set_nl_brute(self.h, int(on))
+
+    def analysis_lists(self, irep: int = 1):
+        """The cutoff-free lists of the DCD / CHECK_FORCE jobs (list_bp,
+        list_exv, list_ele) in place of neighbor_list."""
+        lib().orc_analysis_lists(self.h, irep)
 
     def set_ionic(self, ionic):
         """Per-label ionic strengths (REMD over REPT%ION)."""

@@ -382,6 +382,22 @@ contains
       call neighbor_list(irep)
    end subroutine sisref_neighbor_list
 
+   ! The cutoff-free lists of the DCD and CHECK_FORCE jobs
+   ! (src/main.F90:144-154): list_bp, list_exv, list_ele in place of
+   ! neighbor_list.  list_bp sizes nbp by count(bp3_map /= 0)
+   ! (src/list_bp.F90:18,28), which counts both triangles of the symmetric
+   ! map of models 1/3 and the cross-chain pairs its jmp >= imp +
+   ! bp_min_loop + 1 loop skips; the entries it leaves zero would be read at
+   ! index 0 by the energy and force routines, so nbp is cut to the entries
+   ! it filled.
+   subroutine sisref_analysis_lists() bind(C, name="sisref_analysis_lists")
+      use var_potential, only : flg_ele, nbp, bp_mp
+      call list_bp()
+      nbp(:) = count(bp_mp(1, :, 1) > 0)
+      call list_exv()
+      if (flg_ele) call list_ele()
+   end subroutine sisref_analysis_lists
+
    subroutine sisref_counts(irep, counts) bind(C, name="sisref_counts")
       use var_potential, only : nwca, nele, nbp, flg_ele
       integer(c_int), value :: irep
@@ -409,7 +425,12 @@ contains
       endif
       do k = 1, nbp(irep)
          bp(:,k) = bp_mp(:,k,irep)
-         bpcoef(:,k) = bp_coef(:,k,irep)
+         ! list_bp allocates bp_coef for model 5 only (src/list_bp.F90:36-39)
+         if (allocated(bp_coef)) then
+            bpcoef(:,k) = bp_coef(:,k,irep)
+         else
+            bpcoef(:,k) = 0.0_c_double
+         endif
       enddo
    end subroutine sisref_lists
 

@@ -65,6 +65,8 @@ def _worker(q, so, sysd, ops, seed, nstep_exchange):
             kind = op[0]
             if kind == "nl":
                 lib.sisref_neighbor_list(C.c_int(op[1]))
+            elif kind == "analysis_lists":
+                lib.sisref_analysis_lists()
             elif kind == "lists":
                 irep = op[1]
                 cnt = np.zeros(3, np.int32)
@@ -227,7 +229,7 @@ def run(sysd: dict, ops: list, seed: int = 12345, omp: bool = False, nstep_excha
     """Run `ops` against a fresh reference process set up with `sysd`.
 
     ops entries (last element is the result key where one is produced):
-      ("nl", irep) ; ("lists", irep, key) ; ("force", irep, bp_mc, key)
+      ("nl", irep) ; ("analysis_lists",) ; ("lists", irep, key) ; ("force", irep, bp_mc, key)
       ("bp_state", irep, key) ; ("energy", irep, tempK, bp_mc, bp_energy_valid, key)
       ("ele", irep, key) ; ("bp_tables", key) ; ("bp_maps", key)
       ("tape", irep, n, key) ; ("genrand", irep, kind, n, key)

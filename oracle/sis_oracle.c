@@ -600,6 +600,75 @@ done:;
 
 void orc_set_nl_brute(orc_sys *s, int32_t on) { s->nl_brute = on; }
 
+/* The cutoff-free lists of the DCD and CHECK_FORCE jobs, for replica irep:
+ * list_exv (src/list_exv.F90:19-64: j >= i + 3 within a chain, no circular
+ * wrap rule), list_ele (src/list_ele.F90:18-65: bond / angle exclusions
+ * without the circular wrap) in (chain i, chain j, i, j) order, and
+ * list_bp (src/list_bp.F90:41-93) in global (imp, jmp) order with
+ * jmp >= imp + min_loop + 1 over the whole system (also across chains),
+ * the neighbours' wrap taken only for circular chains, each with the
+ * bp3_map type and triplet coefficients. */
+void orc_analysis_lists(orc_sys *s, int32_t irep1) {
+    rep_t *r = &s->rep[irep1 - 1];
+    r->wca.n = r->ele.n = r->bp.n = r->bpcoef.n = 0;
+    /* init_ele's cutoff (src/init_ele.F90:72-104) */
+    if (s->p.ele_on && s->p.ele_cutoff_type == 2) r->ele_cutoff = s->p.ele_cutoff * r->lambdaD;
+    for (int ich = 0; ich < s->nchains; ich++)
+        for (int jch = ich; jch < s->nchains; jch++)
+            for (int i = 1; i <= s->nmp_chain[ich]; i++) {
+                int imp = s->chain_start[ich] + i - 1;
+                if (s->seq[imp] == D_) continue;
+                for (int j = (ich == jch) ? i + 3 : 1; j <= s->nmp_chain[jch]; j++) {
+                    int jmp = s->chain_start[jch] + j - 1;
+                    if (s->seq[jmp] == D_) continue;
+                    ipush(&r->wca, imp + 1); ipush(&r->wca, jmp + 1);
+                }
+            }
+    if (s->p.ele_on)
+        for (int ich = 0; ich < s->nchains; ich++)
+            for (int jch = ich; jch < s->nchains; jch++)
+                for (int i = 1; i <= s->nmp_chain[ich]; i++) {
+                    int imp = s->chain_start[ich] + i - 1;
+                    if (!s->has_charge[imp]) continue;
+                    for (int j = (ich == jch) ? i + 1 : 1; j <= s->nmp_chain[jch]; j++) {
+                        /* the reference tests j against i for every chain pair
+                         * (src/list_ele.F90:45-46) */
+                        if (j == i + 1 && s->p.ele_excl_bond) continue;
+                        if (j == i + 2 && s->p.ele_excl_angl) continue;
+                        int jmp = s->chain_start[jch] + j - 1;
+                        if (!s->has_charge[jmp]) continue;
+                        ipush(&r->ele, imp + 1); ipush(&r->ele, jmp + 1);
+                    }
+                }
+    for (int imp = 0; imp < s->nmp; imp++)
+        for (int jmp = imp + s->p.bp_min_loop + 1; jmp < s->nmp; jmp++) {
+            int bt, b3;
+            orc_bp_pair(s, imp, jmp, &bt, &b3);
+            if (b3 == 0) continue;
+            int ich = s->chain_of[imp], jch = s->chain_of[jmp];
+            int ni = s->nmp_chain[ich], nj = s->nmp_chain[jch];
+            int ic = s->moltype[ich] == 2, jc = s->moltype[jch] == 2;
+            int li = s->lmp[imp], lj = s->lmp[jmp];
+            ipush(&r->bp, imp + 1);
+            ipush(&r->bp, jmp + 1);
+            ipush(&r->bp, (ic && li == 1) ? imp + 1 + ni - 1 : imp + 1 - 1);
+            ipush(&r->bp, (jc && lj == 1) ? jmp + 1 + nj - 1 : jmp + 1 - 1);
+            ipush(&r->bp, (ic && li == ni) ? imp + 1 - ni + 1 : imp + 1 + 1);
+            ipush(&r->bp, (jc && lj == nj) ? jmp + 1 - nj + 1 : jmp + 1 + 1);
+            ipush(&r->bp, bt);
+            dpush(&r->bpcoef, s->p.bp_model == 5 ? s->bp3_dH[b3] : 0.0);
+            dpush(&r->bpcoef, s->p.bp_model == 5 ? s->bp3_dS[b3] : 0.0);
+        }
+    int64_t nbp = r->bp.n / 7;
+    r->bp_status = realloc(r->bp_status, nbp + 1);
+    r->ene_bp = realloc(r->ene_bp, (nbp + 1) * sizeof(double));
+    memset(r->bp_status, 0, nbp + 1);
+    memset(r->ene_bp, 0, (nbp + 1) * sizeof(double));
+    r->bp_status_mc = realloc(r->bp_status_mc, nbp + 1);
+    memset(r->bp_status_mc, 0, nbp + 1);
+    r->have_status_mc = 0;
+}
+
 int64_t orc_count(orc_sys *s, int32_t irep, int32_t kind) {
     rep_t *r = &s->rep[irep - 1];
     return kind == 0 ? r->wca.n / 2 : kind == 1 ? r->ele.n / 2 : r->bp.n / 7;

@@ -54,4 +54,5 @@ void orc_plant(orc_sys *s, int32_t which, int32_t on);   /* test hook: planted m
 void orc_set_counter_rng(orc_sys *s, uint64_t seed, int32_t on);
 void orc_philox(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
 void orc_set_nl_brute(orc_sys *s, int32_t on);   /* 1: neighbour lists by the O(N^2) loops only */
+void orc_analysis_lists(orc_sys *s, int32_t irep);   /* list_bp / list_exv / list_ele (DCD, CHECK_FORCE jobs) */
 #endif

@@ -181,7 +181,7 @@ u8ptr = lambda a: _ptr(a, C.c_uint8)  # noqa: E731
 EXPORTED = [
     "sis_gpu_create", "sis_gpu_destroy", "sis_gpu_last_error",
     "sis_gpu_set_state", "sis_gpu_get_state", "sis_gpu_set_rep_conditions",
-    "sis_gpu_rebuild", "sis_gpu_force", "sis_gpu_energy", "sis_gpu_energy_noalloc", "sis_gpu_autotune",
+    "sis_gpu_rebuild", "sis_gpu_analysis_lists", "sis_gpu_force", "sis_gpu_energy", "sis_gpu_energy_noalloc", "sis_gpu_autotune",
     "sis_gpu_get_pair_count", "sis_gpu_get_pairs", "sis_gpu_get_bp",
     "sis_gpu_set_rng", "sis_gpu_set_rng_tape", "sis_gpu_init_velocities",
     "sis_gpu_step", "sis_gpu_exchange", "sis_gpu_get_labels",
@@ -213,6 +213,7 @@ def _declare(lib: C.CDLL) -> None:
         "sis_gpu_get_state": (i32, [vp, P(d), P(d), P(d)]),
         "sis_gpu_set_rep_conditions": (i32, [vp, P(d), P(d)]),
         "sis_gpu_rebuild": (i32, [vp, i32]),
+        "sis_gpu_analysis_lists": (i32, [vp]),
         "sis_gpu_force": (i32, [vp, i32, i32, P(d)]),
         "sis_gpu_energy": (i32, [vp, i32, d, i32, P(d)]),
         "sis_gpu_energy_noalloc": (i32, [vp, i32, d, i32, P(d)]),

@@ -44,6 +44,10 @@ struct Params {
   int N, R, fix_h4, nchains, maxlen;
   int bd;  // 1: Ermak-McCammon Brownian update in place of GJF-2GJ
   int ele_cut_type;   // 1: fixed cutoff; 2: cutoff = ele_cut_inp * lambdaD
+  // 0: pair rules of the MD job's neighbour list (src/neighbor_list.F90);
+  // 1: those of the cutoff-free lists of the DCD and CHECK_FORCE jobs
+  // (src/list_exv.F90, src/list_ele.F90, src/list_bp.F90)
+  int list_rule;
 };
 
 // 1/kT of the limit-MC acceptance.  The force's MC takes var_state's kT,

@@ -126,6 +126,10 @@ struct sis_gpu_ctx {
   uint64_t seed = 0;
   int64_t istep = 0, rebuilds = 0;
   bool lists_valid = false;
+  // the lists follow the rules of the DCD / CHECK_FORCE jobs' list_bp,
+  // list_exv and list_ele (P.list_rule = 1 at their build), not those of
+  // the MD job's neighbour list; a step rebuilds them first
+  bool lists_analysis = false;
   bool use_cells = false;
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -635,6 +639,7 @@ void rebuild(sis_gpu_ctx *c, const std::vector<int> &mask_in) {
       }
       c->rebuilt = mask;
       c->lists_valid = true;
+      c->lists_analysis = c->P.list_rule != 0;
       return;
     }
     // lists of every replica are rebuilt after a resize
@@ -643,6 +648,26 @@ void rebuild(sis_gpu_ctx *c, const std::vector<int> &mask_in) {
     alloc_lists(c);
   }
   throw ApiError{SIS_ERR_CAPACITY, "neighbour lists did not converge in size"};
+}
+
+// The cutoff-free lists of the DCD and CHECK_FORCE jobs (src/main.F90:144-154:
+// list_bp, list_exv, list_ele) for every replica at the positions held.
+// Pairs beyond the neighbour cutoffs are left out: none of them has an
+// energy, a force or a formed base pair, so only the rules of the three
+// routines change (no circular wrap in the exclusions, list_ele's bond and
+// angle exclusions on every chain pair, base-pair candidates with
+// jmp >= imp + bp_min_loop + 1 over the system, in (imp, jmp) order).
+void analysis_rebuild(sis_gpu_ctx *c) {
+  c->P.list_rule = 1;
+  c->lists_valid = false;
+  try {
+    rebuild(c, std::vector<int>(c->R, 1));
+  } catch (...) {
+    c->P.list_rule = 0;
+    c->lists_valid = false;
+    throw;
+  }
+  c->P.list_rule = 0;
 }
 
 // Base-pair evaluation + limit MC (force_bp_limit_triplet / force_bp_triplet).
@@ -1470,7 +1495,7 @@ int sis_gpu_rebuild(sis_gpu_ctx *c, int32_t irep) {
   if (irep < 0 || irep > c->R) return fail(SIS_ERR_ARG, "irep out of range");
   std::vector<int> m(c->R, irep == 0 ? 1 : 0);
   if (irep > 0) m[irep - 1] = 1;
-  if (!c->lists_valid) std::fill(m.begin(), m.end(), 1);
+  if (!c->lists_valid || c->lists_analysis) std::fill(m.begin(), m.end(), 1);
   HIPCHK(hipEventRecord(c->ev0, c->stream));
   rebuild(c, m);
   HIPCHK(hipEventRecord(c->ev1, c->stream));
@@ -1478,6 +1503,14 @@ int sis_gpu_rebuild(sis_gpu_ctx *c, int32_t irep) {
   float ms = 0;
   HIPCHK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
   c->last_ms[1] = ms;
+  return SIS_OK;
+  API_END
+}
+
+int sis_gpu_analysis_lists(sis_gpu_ctx *c) {
+  API_BEGIN
+  if (!c) return fail(SIS_ERR_ARG, "null context");
+  analysis_rebuild(c);
   return SIS_OK;
   API_END
 }
@@ -1492,8 +1525,9 @@ int sis_gpu_force(sis_gpu_ctx *c, int32_t irep, int32_t bp_mc, double *forces) {
   mc[irep - 1] = bp_mc ? 1 : 0;
   set_mask(c, m);
   HIPCHK(hipEventRecord(c->ev0, c->stream));
-  if (c->mode == 3 && batched_ok(c)) {
+  if (c->mode == 3 && batched_ok(c) && !c->lists_analysis) {
     // a context created for the batched form evaluates with that kernel
+    // (which builds the MD job's lists itself)
     batched_force(c, irep - 1, bp_mc);
   } else {
     bp_phase(c, &mc, nullptr, c->mask_d.p);
@@ -1794,6 +1828,7 @@ int sis_gpu_step(sis_gpu_ctx *c, int64_t nsteps, int64_t *rebuilds) {
   if (nsteps < 0) return fail(SIS_ERR_ARG, "nsteps < 0");
   int64_t r0 = c->rebuilds;
   c->tev_used = 0;
+  if (c->lists_analysis) c->lists_valid = false;
   HIPCHK(hipEventRecord(c->ev0, c->stream));
   // each form rebuilds invalid lists itself and forces MC after it
   int mode = c->mode;
@@ -1914,6 +1949,7 @@ int sis_gpu_exchange(sis_gpu_ctx *c, double *replica_energies, int32_t *accepted
   API_BEGIN
   if (!c) return fail(SIS_ERR_ARG, "null context");
   if (!c->lists_valid) return fail(SIS_ERR_STATE, "exchange called before rebuild");
+  if (c->lists_analysis) rebuild(c, std::vector<int>(c->R, 1));
   do_exchange(c, replica_energies, accepted);
   return SIS_OK;
   API_END

@@ -110,12 +110,15 @@ __device__ __forceinline__ void pair_test(const Params &P, const Ptrs &D, int r,
   bool same = tl.start == th.start;
   int il = tl.loc, jl = th.loc, n = tl.len;
   bool circ = topo_circ(tl);
+  // the analysis lists take no circular wrap into their exclusions
+  // (src/list_exv.F90:34-38, src/list_ele.F90:35-46)
+  const bool nl_rule = P.list_rule == 0;
   // WCA
   if (topo_seq(tl) != SQ_D && topo_seq(th) != SQ_D && d2 <= P.wca_nl_cut2) {
     bool add = true;
     if (same) {
       if (il + 2 >= jl) add = false;
-      else if (circ && jl + 2 >= il + n) add = false;
+      else if (nl_rule && circ && jl + 2 >= il + n) add = false;
     }
     if (add) {
       if (o.nw < D.maxw) D.nb_wca[((size_t)r * D.maxw + o.nw) * N + a] = b;
@@ -125,9 +128,13 @@ __device__ __forceinline__ void pair_test(const Params &P, const Ptrs &D, int r,
   // Debye-Hueckel
   if (P.ele_on && d2 <= ele_cut2 && topo_charged(tl) && topo_charged(th)) {
     bool add = true;
-    if (same) {
+    // list_ele tests the chain-local indices of every chain pair, the
+    // neighbour list those of one chain only
+    if (same || !nl_rule) {
       if (jl == il + 1 && P.ele_excl_bond) add = false;
       if (jl == il + 2 && P.ele_excl_angl) add = false;
+    }
+    if (same && nl_rule) {
       if (circ) {
         if (il == 1) {
           if (jl == n && P.ele_excl_bond) add = false;
@@ -142,8 +149,11 @@ __device__ __forceinline__ void pair_test(const Params &P, const Ptrs &D, int r,
       o.ne++;
     }
   }
-  // base-pair candidates, owned by the lower bead
-  if (a < b && P.bp_model > 0 && d2 <= P.bp_nl_cut2) {
+  // base-pair candidates, owned by the lower bead; list_bp starts jmp at
+  // imp + bp_min_loop + 1 over the whole system (src/list_bp.F90:41-42).
+  // Candidates beyond the neighbour cutoff are left out in both rules:
+  // they can form no pair, so no energy, force or limit-MC choice sees them.
+  if (a < b && P.bp_model > 0 && d2 <= P.bp_nl_cut2 && (nl_rule || b >= a + P.bp_min_loop + 1)) {
     int bt;
     if (bp_eligible(P, D, a, b, &bt) > 0) {
       if (o.nb < D.maxbpb) D.bp_tmp[((size_t)r * N + a) * D.maxbpb + o.nb] = b;
@@ -317,14 +327,24 @@ __global__ void k_bp_fill(Params P, Ptrs D, const int *mask) {
     int mp[7];
     mp[0] = imp;
     mp[1] = jmp;
-    mp[2] = (i == 1) ? imp + ti.len - 1 : imp - 1;
-    mp[3] = (j == 1) ? jmp + tj.len - 1 : jmp - 1;
-    mp[4] = (i == ti.len) ? imp + 1 - ti.len : imp + 1;
-    // the reference compares j with the length of chain i here
-    // (src/neighbor_list.F90:212); kept unless it leaves the system
-    int m6 = (j == ti.len) ? jmp + 1 - tj.len : jmp + 1;
-    if (P.fix_h4 || m6 < 1 || m6 > N) m6 = (j == tj.len) ? jmp + 1 - tj.len : jmp + 1;
-    mp[5] = m6;
+    if (P.list_rule == 0) {
+      mp[2] = (i == 1) ? imp + ti.len - 1 : imp - 1;
+      mp[3] = (j == 1) ? jmp + tj.len - 1 : jmp - 1;
+      mp[4] = (i == ti.len) ? imp + 1 - ti.len : imp + 1;
+      // the reference compares j with the length of chain i here
+      // (src/neighbor_list.F90:212); kept unless it leaves the system
+      int m6 = (j == ti.len) ? jmp + 1 - tj.len : jmp + 1;
+      if (P.fix_h4 || m6 < 1 || m6 > N) m6 = (j == tj.len) ? jmp + 1 - tj.len : jmp + 1;
+      mp[5] = m6;
+    } else {
+      // list_bp wraps the neighbours of circular chains only and takes
+      // each bead's own chain length (src/list_bp.F90:60-82)
+      const bool ic = topo_circ(ti), jc = topo_circ(tj);
+      mp[2] = (ic && i == 1) ? imp + ti.len - 1 : imp - 1;
+      mp[3] = (jc && j == 1) ? jmp + tj.len - 1 : jmp - 1;
+      mp[4] = (ic && i == ti.len) ? imp + 1 - ti.len : imp + 1;
+      mp[5] = (jc && j == tj.len) ? jmp + 1 - tj.len : jmp + 1;
+    }
     int bt;
     int h = bp_eligible(P, D, a, b, &bt);
     mp[6] = bt;
@@ -335,10 +355,14 @@ __global__ void k_bp_fill(Params P, Ptrs D, const int *mask) {
     // (src/init_bp.F90:39, src/neighbor_list.F90:220-221)
     D.bp_coef[2 * (size_t)gb] = P.bp_model == 5 ? D.bp3_dH[h] : 0.0;
     D.bp_coef[2 * (size_t)gb + 1] = P.bp_model == 5 ? D.bp3_dS[h] : 0.0;
-    // reference list order: (chain_i, chain_j, i, j)
-    // (src/neighbor_list.F90:99-114)
-    D.bp_key[gb] = (((unsigned long long)(r * (int64_t)P.nchains + D.chain_id[a]) * P.nchains +
-                     D.chain_id[b]) * P.maxlen + (i - 1)) * P.maxlen + (j - 1);
+    // reference list order: (chain_i, chain_j, i, j) for the neighbour list
+    // (src/neighbor_list.F90:99-114), (imp, jmp) for list_bp
+    // (src/list_bp.F90:41-42)
+    if (P.list_rule == 0)
+      D.bp_key[gb] = (((unsigned long long)(r * (int64_t)P.nchains + D.chain_id[a]) * P.nchains +
+                       D.chain_id[b]) * P.maxlen + (i - 1)) * P.maxlen + (j - 1);
+    else
+      D.bp_key[gb] = ((unsigned long long)r * N + a) * N + b;
     D.bp_val[gb] = gb;
   }
 }

@@ -809,8 +809,8 @@ int sis_gpu_dcd_energies(sis_gpu_ctx *c, const char *path, const char *out_path,
     while (dcd_read_frame(d, x, b)) {
       for (int a = 0; a < N; a++) h[a] = double4{x[(size_t)3 * a], x[(size_t)3 * a + 1], x[(size_t)3 * a + 2], 0.0};
       HIPCHK(hipMemcpyAsync(c->pos_d.p, h.data(), (size_t)N * sizeof(double4), hipMemcpyHostToDevice, c->stream));
-      c->lists_valid = false;
-      rebuild(c, m);
+      // the DCD job's lists (src/main.F90:144-154)
+      analysis_rebuild(c);
       energy_phase(c, m, c->tK, c->P.nstep_bp_mc > 0 ? 1 : 0, e16.data());
       if (k < max_frames) std::copy(e16.begin(), e16.begin() + 11, energies + (size_t)11 * k);
       k++;
@@ -839,9 +839,11 @@ extern "C" int sis_gpu_check_force(sis_gpu_ctx *c, int32_t irep, double h, const
   API_BEGIN
   if (!c) return fail(SIS_ERR_ARG, "null context");
   if (irep < 1 || irep > c->R || !(h > 0.0)) return fail(SIS_ERR_ARG, "invalid argument");
-  if (!c->lists_valid) return fail(SIS_ERR_STATE, "check_force called before rebuild");
   const int N = c->N;
   const int mc = c->P.nstep_bp_mc > 0 ? 1 : 0;
+  // the CHECK_FORCE job's lists (src/main.F90:144-154); a later step
+  // rebuilds the MD lists
+  analysis_rebuild(c);
   std::vector<double> F((size_t)3 * N), fe((size_t)3 * N);
   if (sis_gpu_force(c, irep, mc, F.data()) != SIS_OK) return SIS_ERR_STATE;
   double4 *xd = c->pos_d.p + (size_t)(irep - 1) * N;

@@ -117,6 +117,11 @@ class Engine:
     def neighbor_list(self, irep: int = 0):
         _check(lib().sis_gpu_rebuild(self.h, irep))
 
+    def analysis_lists(self):
+        """The lists of the DCD / CHECK_FORCE jobs (list_bp, list_exv,
+        list_ele) for every replica; the next step rebuilds the MD lists."""
+        _check(lib().sis_gpu_analysis_lists(self.h))
+
     def force(self, irep: int = 1, bp_mc: int = 1) -> np.ndarray:
         f = np.zeros((self.nmp, 3))
         _check(lib().sis_gpu_force(self.h, irep, bp_mc, abi.dptr(f)))

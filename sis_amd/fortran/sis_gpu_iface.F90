@@ -103,6 +103,13 @@ module sis_gpu_iface
          integer(c_int32_t), value :: irep
       end function
 
+      ! replaces: call list_bp(); call list_exv(); if (flg_ele) call list_ele()
+      !           (src/main.F90:144-154, jobs other than MD)
+      integer(c_int) function sis_gpu_analysis_lists(ctx) bind(C, name="sis_gpu_analysis_lists")
+         import :: c_int, c_ptr
+         type(c_ptr), value :: ctx
+      end function
+
       ! replaces: call force(irep, forces)
       integer(c_int) function sis_gpu_force(ctx, irep, bp_mc, forces) bind(C, name="sis_gpu_force")
          import :: c_int, c_ptr, c_int32_t

@@ -511,8 +511,98 @@ def load(name):
     return sysd, d
 
 
+def dcd_split(noise=0.5, nframes=6, seed=17):
+    """A two-chain duplex with a circular chain beside it, for the DCD job's
+    lists: T2S2HP_folded.pdb cut in its loop into chains of 13 and 14 nt
+    (the stem pairs become pairs between the chains), and a flat circular
+    chain of 16 nt 30 A away.  Force field check/sis_dih_ReB_NN.ff with the
+    electrostatics and limit MC of check/input_T2S2HP.toml (cutoff type 2,
+    10 Debye lengths, 0.15 M, nstep_MC = 1).  Frames: the folded
+    coordinates, then the same with N(0, noise^2) A displacements, stored
+    as float32 as a DCD file holds them."""
+    from sis_amd.forcefield import read_ff
+    seq, x = _t2s2hp("folded")
+    rng = np.random.default_rng(seed)
+    n = 16
+    ring_seq = configs.random_seq(rng, n, 0.6)
+    R = 5.9 / (2 * np.sin(np.pi / n))
+    th = 2 * np.pi * np.arange(n) / n
+    ring = np.stack([R * np.cos(th), R * np.sin(th), np.zeros(n)], axis=1) + x.mean(0) + np.array([30.0, 0.0, 0.0])
+    p = read_ff(REF_DIR / "check/sis_dih_ReB_NN.ff", tempK=300.0, ionic_strength=0.15, ele_cutoff_type=2,
+                ele_cutoff=10.0, nl_margin=10.0, viscosity_pas=1e-5, dt=0.05, bp_model=5, bp_min_loop=3,
+                max_bp_per_nt=1, nstep_bp_mc=1)
+    xyz = np.vstack([x, ring])
+    sysd = configs._mk("dcd_split", [seq[:13], seq[13:], ring_seq], xyz, p)
+    sysd["moltype"][2] = MOLT_CIRCRNA
+    frames = [xyz] + [xyz + rng.normal(0.0, noise, xyz.shape) for _ in range(nframes - 1)]
+    return sysd, np.array(frames).astype(np.float32)
+
+
+def make_dcd(name="dcd_split", seed=4321):
+    """job_dcd on the reference build: the lists of list_bp / list_exv /
+    list_ele (src/main.F90:144-154) and, per frame, energy_sumup with the
+    limit MC on replica 1's stream (src/job_dcd.F90:59-80) and the base-pair
+    state it leaves."""
+    s, frames = dcd_split()
+    T = float(s["temps"][0])
+    ops = [("tape", 1, 400_000, "t"), ("analysis_lists",), ("lists", 1, "L")]
+    for k, f in enumerate(frames):
+        ops += [("set_xyz", f.astype(np.float64)[None]), ("energy", 1, T, 1, 0, f"E{k}"), ("bp_state", 1, f"B{k}")]
+    ref = refdrv.run(s, ops, seed=seed)
+    o = corc.Oracle(s)
+    o.set_tape(1, ref["t"])
+    o.analysis_lists(1)
+    for f in frames:
+        o.set_xyz(f.astype(np.float64)[None])
+        o.energy(1, T, 1, 0)
+    keep = o.tape_pos(1) + 64
+    st = [ref[f"B{k}"][0].astype(np.uint8) for k in range(len(frames))]
+    out = dict(
+        nchains=s["nchains"], nmp_chain=s["nmp_chain"], moltype=s["moltype"], seq=s["seq"],
+        xyz=s["xyz"], temps=s["temps"], nrep=1, nstep=0, nstep_exchange=0,
+        tapes=ref["t"][:keep][None], tape_rep=np.zeros(1, np.uint32), frames=frames,
+        wca=ref["L"]["wca"], ele=ref["L"]["ele"], bp=ref["L"]["bp"], bpcoef=ref["L"]["bpcoef"],
+        energies=np.array([ref[f"E{k}"] for k in range(len(frames))]),
+        bp_status=np.array(st), ene_bp=np.array([ref[f"B{k}"][1] for k in range(len(frames))]),
+    )
+    p = s["params"]
+    scal = {k: v for k, v in p.items() if isinstance(v, (int, float))}
+    out["param_names"] = np.array(sorted(scal))
+    out["param_values"] = np.array([float(scal[k]) for k in sorted(scal)])
+    out["box"] = np.array(p["box"], float)
+    for k in LIST_PARAMS:
+        out["param_" + k] = np.asarray(p[k], np.float64)
+    from sis_amd.abi import BP_FIELDS
+    out["bp_params"] = np.array([[float(p["bp"][t].get(f, 0.0)) for f in BP_FIELDS] for t in ("GC", "AU", "GU")])
+    np.savez_compressed(HERE / f"{name}.npz", **out)
+    print(name, "tape words kept", keep, "lists", {k: len(ref["L"][k]) for k in ("wca", "ele", "bp")},
+          "formed", [int(x.sum()) for x in st])
+
+
+def make_cag47_dcd(name="cag47_dcd"):
+    """The reference build's job_dcd energies over the 100 frames of
+    check/test_DCD_CAG47/CAG47.dcd (NHT22, base-pair model 1, no limit MC):
+    list_bp / list_exv / list_ele, then energy_sumup per frame."""
+    from sis_amd.engine import read_dcd_file
+    s = cag47_system()
+    frames, _ = read_dcd_file(CAG47_DIR / "CAG47.dcd", 141)
+    ops = [("analysis_lists",)]
+    for k, f in enumerate(frames):
+        ops += [("set_xyz", f[None]), ("energy", 1, 295.15, 0, 0, f"E{k}")]
+    ref = refdrv.run(s, ops, seed=123)
+    E = np.array([ref[f"E{k}"] for k in range(len(frames))])
+    np.savez_compressed(HERE / f"{name}.npz", energies=E)
+    print(name, "E_bp range", E[:, 4].min(), E[:, 4].max())
+
+
 if __name__ == "__main__":
-    names = sys.argv[1:] or list(SYSTEMS)
+    names = sys.argv[1:] or list(SYSTEMS) + ["dcd_split", "cag47_dcd"]
     for nm in names:
+        if nm == "dcd_split":
+            make_dcd()
+            continue
+        if nm == "cag47_dcd":
+            make_cag47_dcd()
+            continue
         b, n, nex = SYSTEMS[nm]
         make(nm, b, n, nex)

@@ -2,6 +2,8 @@
 staging buffers, background writer) are byte-identical to the host writer
 applied to state snapshots of the same trajectory taken between shorter
 sis_gpu_step calls."""
+from pathlib import Path
+
 import numpy as np
 import pytest
 
@@ -325,13 +327,55 @@ def test_cag47_dcd_energies_on_device():
     E = eng.dcd_energies(CAG47_DIR / "CAG47.dcd")
     assert E.shape == (100, 11)
     o = corc.Oracle(s)
+    o.analysis_lists(1)
     for k in (0, 37, 99):
         o.set_xyz(frames[k][None])
-        o.neighbor_list(1)
         eo = o.energy(1, 295.15, 0, 0)
         assert np.allclose(E[k], eo, rtol=1e-9, atol=1e-9), k
     for name, ci, ei, rtol, atol in CAG47_COLS:
         assert np.allclose(E[:, ei], ref[:, ci], rtol=rtol, atol=atol), name
+    # every column, E_bp included, against the reference build's job_dcd
+    want = np.load(Path(__file__).parent / "golden" / "cag47_dcd.npz")["energies"]
+    assert np.allclose(E, want, rtol=1e-9, atol=1e-9)
+
+
+def test_dcd_split_on_device(tmp_path):
+    """job_dcd with the limit MC on a duplex of two chains and a circular
+    chain (the dcd_split fixture, written by the reference build): per-frame
+    energies within 1e-10 of the reference's on the same MT words; the
+    base-pair candidates are the reference's list_bp entries within the
+    neighbour cutoff, in list_bp's (imp, jmp) order; the pairs formed after
+    the last frame's limit MC are the reference's."""
+    from sis_amd.engine import write_dcd_file
+    from tests.golden.make_golden import load
+    sysd, d = load("dcd_split")
+    print("[case] dcd_split energies", flush=True)
+    eng = Engine(sysd, parity=True, mode="phases")
+    eng.set_rng_tape(1, d["tapes"][0])
+    p = tmp_path / "split.dcd"
+    write_dcd_file(p, d["frames"].astype(np.float64))
+    E = eng.dcd_energies(p)
+    assert E.shape == (len(d["frames"]), 11)
+    assert np.allclose(E, d["energies"], rtol=1e-10, atol=1e-10)
+    b = eng.bp(1)
+    ref_rows = [tuple(r) for r in d["bp"]]
+    pos = [ref_rows.index(tuple(r)) for r in b["bp"]]
+    assert pos == sorted(pos)   # a subsequence of list_bp's order
+    assert np.allclose(b["bpcoef"], d["bpcoef"][pos])
+    formed = {ref_rows[i][:2] for i in np.flatnonzero(d["bp_status"][-1])}
+    assert {tuple(r[:2]) for r in b["bp"][b["status"] > 0]} == formed and formed
+    # list_exv pairs the circular chain's ends (beads 28 and 43), the MD
+    # job's neighbour list does not; those lists come back with the next step
+    ends = (28, 43)
+    assert ends in {tuple(r) for r in eng.pairs(1, 0)}
+    eng.neighbor_list(0)
+    assert ends not in {tuple(r) for r in eng.pairs(1, 0)}
+    run = Engine(sysd, seed=3, mode="phases")
+    run.analysis_lists()
+    assert ends in {tuple(r) for r in run.pairs(1, 0)}
+    run.init_velocities()
+    run.step(1)
+    assert ends not in {tuple(r) for r in run.pairs(1, 0)}
 
 
 def test_dcd_energy_analysis(tmp_path):
@@ -358,7 +402,7 @@ def test_dcd_energy_analysis(tmp_path):
     want = []
     for f in frames.astype(np.float32).astype(np.float64):
         chk.set_state(f[None])
-        chk.neighbor_list(0)
+        chk.analysis_lists()
         want.append(chk.energy_sumup(1))
     assert np.array_equal(e, np.array(want))
     host = tmp_path / "h.out"

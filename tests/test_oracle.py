@@ -2,6 +2,7 @@
 golden fixtures written by the reference Fortran (tests/golden/), the live
 reference build (oracle/_ref) where present, and closed forms."""
 import math
+from pathlib import Path
 
 import numpy as np
 import pytest
@@ -418,14 +419,76 @@ def test_cag47_openmm_energies():
     s, frames, ref = cag47_frames_and_reference()
     o = corc.Oracle(s)
     E = []
+    o.analysis_lists(1)
     for x in frames:
         o.set_xyz(x[None])
-        o.neighbor_list(1)
         E.append(o.energy(1, 295.15, 0, 0))
     E = np.array(E)
     for name, ci, ei, rtol, atol in CAG47_COLS:
         assert np.allclose(E[:, ei], ref[:, ci], rtol=rtol, atol=atol), name
     assert np.all(E[:, 4] < 0)   # base pairs form in every frame
+    # every column, the base-pair one included, against the reference
+    # build's own job_dcd over the same frames
+    want = np.load(Path(__file__).parent / "golden" / "cag47_dcd.npz")["energies"]
+    assert np.allclose(E, want, rtol=1e-12, atol=1e-12)
+
+
+def test_dcd_split_fixture():
+    """job_dcd's lists (list_exv, list_ele, list_bp; src/main.F90:144-154)
+    and per-frame energies with the limit MC on a duplex of two chains and
+    a circular chain: the oracle against the reference build's output for
+    the same frames and MT words.  The lists of the MD job give other
+    energies here (WCA between a circular chain's ends, list_ele's local
+    index exclusions between chains), so the fixture tells the two rules
+    apart."""
+    sysd, d = load("dcd_split")
+    T = float(sysd["temps"][0])
+    o = corc.Oracle(sysd)
+    o.set_tape(1, d["tapes"][0])
+    o.analysis_lists(1)
+    L = o.lists(1)
+    for k in ("wca", "ele", "bp"):
+        assert np.array_equal(L[k], d[k]), k
+    assert np.array_equal(L["bpcoef"], d["bpcoef"])
+    for k, f in enumerate(d["frames"]):
+        o.set_xyz(f.astype(np.float64)[None])
+        e = o.energy(1, T, 1, 0)
+        assert np.allclose(e, d["energies"][k], rtol=1e-12, atol=1e-12), k
+        assert np.array_equal(o.bp_state(1)[0], d["bp_status"][k]), k
+    assert d["bp_status"].sum(axis=1).min() > 0
+    md = corc.Oracle(sysd)
+    md.set_tape(1, d["tapes"][0])
+    md.set_xyz(d["frames"][0].astype(np.float64)[None])
+    md.neighbor_list(1)
+    e_md = md.energy(1, T, 1, 0)
+    for col in (5, 6):   # exv, ele
+        assert abs(e_md[col] - d["energies"][0][col]) > 1e-3, col
+
+
+@pytest.mark.ref
+@pytest.mark.parametrize("seed", [11, 12, 13])
+def test_live_reference_analysis_lists(seed):
+    """list_exv / list_ele / list_bp of the oracle against the reference
+    build on systems of a circular chain and two linear chains of unequal
+    lengths (the same builder as the mixed fixture), base-pair models 5 and
+    1, and energies with the limit MC on the same MT words."""
+    from tests.golden.make_golden import mixed_chains
+    for model in (5, 1):
+        s = mixed_chains(seed)
+        s["params"] = dict(s["params"], bp_model=model)
+        ops = [("tape", 1, 100000, "t"), ("analysis_lists",), ("lists", 1, "L"),
+               ("energy", 1, 300.0, int(model == 5), 0, "E"), ("bp_state", 1, "B")]
+        ref = refdrv.run(s, ops, seed=5)
+        o = corc.Oracle(s)
+        o.set_tape(1, ref["t"])
+        o.analysis_lists(1)
+        L = o.lists(1)
+        for k in ("wca", "ele", "bp"):
+            assert np.array_equal(L[k], ref["L"][k]), (model, k)
+        assert np.array_equal(L["bpcoef"], ref["L"]["bpcoef"])
+        e = o.energy(1, 300.0, int(model == 5), 0)
+        assert np.allclose(e, ref["E"], rtol=1e-12, atol=1e-12), model
+        assert np.array_equal(o.bp_state(1)[0], ref["B"][0].astype(np.uint8))
 
 
 @pytest.mark.ref
