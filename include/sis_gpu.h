@@ -202,6 +202,17 @@ int sis_gpu_step(sis_gpu_ctx *ctx, int64_t nsteps, int64_t *rebuilds);
  * src/replica_exchange.F90); replica_energies(2, nrep) out (may be NULL),
  * accepted[nrep/2] out (may be NULL). */
 int sis_gpu_exchange(sis_gpu_ctx *ctx, double *replica_energies, int32_t *accepted);
+
+/* Replicas on a grid of ndim dimensions with nvalues[d] values along each
+ * (src/init_replica.F90:39-93: the variables with ladders in REPT order,
+ * temperature, force, ionic strength, Rg; nrep = the product; label l's
+ * values laid out with the first dimension slowest, as lab2val).  Builds
+ * exchange_pair_tb (src/var_replica.F90:162-259): exchanges take the odd
+ * pairs along each dimension in turn, then the even ones, and start again
+ * (set_forward, :104-117).  Without this call the replicas are one ladder
+ * (ndim = 1).  The labels' conditions are set with
+ * sis_gpu_set_rep_conditions / sis_gpu_set_rep_rg / sis_gpu_set_rep_dcf. */
+int sis_gpu_set_rep_dims(sis_gpu_ctx *ctx, int32_t ndim, const int32_t *nvalues);
 /* rep2lab(1:nrep) (src/var_replica.F90:49). */
 int sis_gpu_get_labels(sis_gpu_ctx *ctx, int32_t *rep2lab);
 

@@ -52,6 +52,8 @@ def lib():
         L.orc_set_counter_rng.argtypes = [vp, C.c_uint64, i32]
         L.orc_set_nl_brute.argtypes = [vp, i32]
         L.orc_analysis_lists.argtypes = [vp, i32]
+        L.orc_set_rep_dims.argtypes = [vp, i32, P(i32)]
+        L.orc_pair_table.argtypes = [vp, P(i32)]
         L.orc_philox.argtypes = [P(C.c_uint32), P(C.c_uint32), P(C.c_uint32)]
         L.orc_set_rep_rg.argtypes = [vp, P(d)]
         L.orc_set_rep_dcf.argtypes = [vp, P(d), P(d)]
@@ -240,6 +242,18 @@ class Oracle:
         """The cutoff-free lists of the DCD / CHECK_FORCE jobs (list_bp,
         list_exv, list_ele) in place of neighbor_list."""
         lib().orc_analysis_lists(self.h, irep)
+
+    def set_rep_dims(self, nvalues):
+        """Replica grid of len(nvalues) dimensions (the reference's REPT
+        order of the variables that have ladders)."""
+        self._dims = np.ascontiguousarray(nvalues, np.int32)
+        lib().orc_set_rep_dims(self.h, len(self._dims), self._dims.ctypes.data_as(C.POINTER(C.c_int32)))
+
+    def pair_table(self):
+        nset = 2 * len(getattr(self, "_dims", [0]))
+        out = np.zeros((nset, self.nrep + 2), np.int32)
+        lib().orc_pair_table(self.h, out.ctypes.data_as(C.POINTER(C.c_int32)))
+        return out
 
     def set_ionic(self, ionic):
         """Per-label ionic strengths (REMD over REPT%ION)."""

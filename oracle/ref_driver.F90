@@ -55,6 +55,9 @@ module sisref_driver
    ! REMD over another variable than the temperature (REPT id: 3 ionic
    ! strength, 4 Rg target): set by sisref_remd_var before sisref_setup
    integer, save :: remd_kind = 0
+   ! a temperature dimension beside remd_kind's (a two-dimensional grid):
+   ! set by sisref_remd_temps before sisref_setup
+   real(c_double), allocatable, save :: remd_temps(:)
    integer(c_int64_t), save :: twz_nsave = 0   ! nstep_save of a job_md run that writes .twz lines
    real(c_double), allocatable, save :: remd_values(:)
 
@@ -68,6 +71,14 @@ contains
       allocate(remd_values(n))
       remd_values(:) = values(:)
    end subroutine sisref_remd_var
+
+   subroutine sisref_remd_temps(n, values) bind(C, name="sisref_remd_temps")
+      integer(c_int), value :: n
+      real(c_double), intent(in) :: values(n)
+      if (allocated(remd_temps)) deallocate(remd_temps)
+      allocate(remd_temps(n))
+      remd_temps(:) = values(:)
+   end subroutine sisref_remd_temps
 
    subroutine copy_bp(src, dst)
       use var_potential, only : basepair_parameters
@@ -194,7 +205,16 @@ contains
          flg_replica = .True.
          flg_exchange = .True.
          replica_values(:,:) = 0.0_PREC
-         if (remd_kind == REPT%ION .or. remd_kind == REPT%RG .or. remd_kind == REPT%TWZDCF) then
+         if (allocated(remd_temps)) then
+            ! [Replica] nrep_temp x the other variable's ladder
+            flg_repvar(REPT%TEMP) = .True.
+            nrep(REPT%TEMP) = size(remd_temps)
+            replica_values(1:size(remd_temps), REPT%TEMP) = remd_temps(:)
+            flg_repvar(remd_kind) = .True.
+            nrep(remd_kind) = size(remd_values)
+            replica_values(1:size(remd_values), remd_kind) = remd_values(:)
+            if (nrep(REPT%TEMP) * nrep(remd_kind) /= nrep_in) error stop "sisref: grid size /= nrep"
+         else if (remd_kind == REPT%ION .or. remd_kind == REPT%RG .or. remd_kind == REPT%TWZDCF) then
             ! [Replica] ion_strength / rg / force ladder
             flg_repvar(remd_kind) = .True.
             nrep(remd_kind) = nrep_in

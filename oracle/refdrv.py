@@ -41,7 +41,15 @@ def _setup(lib, sysd, prefix, seed, nstep_exchange=0):
     seq = np.ascontiguousarray(sysd["seq"], np.int32)
     temps = np.ascontiguousarray(sysd["temps"], np.float64)
     xyz = np.ascontiguousarray(sysd["xyz"], np.float64)
+    grid = sysd.get("remd_grid")   # {"temps": [nT], "ion": [nI]}: a temperature x ionic-strength grid
+    if grid is not None:
+        tv = np.ascontiguousarray(grid["temps"], np.float64)
+        iv = np.ascontiguousarray(grid["ion"], np.float64)
+        lib.sisref_remd_temps(C.c_int(len(tv)), _dp(tv))
+        lib.sisref_remd_var(C.c_int(3), C.c_int(len(iv)), _dp(iv))
     for key, kind in (("remd_ion", 3), ("remd_rg", 4), ("remd_dcf", 2)):   # ladders over REPT%ION / RG / TWZDCF
+        if grid is not None:
+            break
         if sysd.get(key) is not None:
             vals = np.ascontiguousarray(sysd[key], np.float64)
             lib.sisref_remd_var(C.c_int(kind), C.c_int(len(vals)), _dp(vals))

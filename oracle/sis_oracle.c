@@ -68,7 +68,8 @@ struct orc_sys {
     /* rnd_boxmuller process-global cache (src/rnd_boxmuller.F90:13-15) */
     int bm_stored; double bm_value;
     /* replica exchange */
-    int *rep2lab, *lab2rep; double *lab2val; int idx_pair; int *pair_tb; /* [2][nrep+1] */
+    int *rep2lab, *lab2rep; double *lab2val; int idx_pair; int *pair_tb; /* [pair_sets][nrep+2] */
+    int pair_sets;           /* 2 x ndim_replica */
     double *lab2ion;         /* ionic strength of each label (REMD over REPT%ION) */
     double *lab2rg;          /* Rg-bias target of each label (REMD over REPT%RG) */
     int rg_remd;             /* the Rg targets travel with the labels */
@@ -1541,16 +1542,9 @@ orc_sys *orc_create(const sis_topo *topo, const sis_params *prm, int32_t nrep, c
     s->lab2rg = calloc(nrep + 1, sizeof(double));
     s->lab2dcf = calloc(nrep + 1, sizeof(double));
     for (int k = 0; k <= nrep; k++) s->lab2ion[k] = prm->ionic_strength;
-    s->pair_tb = calloc(2 * (nrep + 2), sizeof(int));
     for (int k = 1; k <= nrep; k++) { s->rep2lab[k] = k; s->lab2rep[k] = k; s->lab2val[k] = temps ? temps[k - 1] : prm->tempK; }
-    for (int par = 0; par < 2; par++) {
-        int *tb = s->pair_tb + par * (nrep + 2);
-        int iset = 0;
-        if (par == 1) tb[++iset] = 0;
-        for (int ir = (par == 0 ? 1 : 2); ir <= nrep - 1; ir += 2) { iset++; tb[iset] = iset + 1; iset++; tb[iset] = iset - 1; }
-        while (iset < nrep) tb[++iset] = 0;
-    }
-    s->idx_pair = 0;
+    s->pair_tb = NULL;
+    { int32_t one = nrep; orc_set_rep_dims(s, 1, &one); }
     for (int r = 0; r < nrep; r++) {
         s->rep[r].tK = (nrep > 1) ? s->lab2val[s->rep2lab[r + 1]] : prm->tempK;
         s->rep[r].ionic = prm->ionic_strength;
@@ -1566,6 +1560,54 @@ void orc_set_temps(orc_sys *s, const double *temps) {
 /* REMD over the ionic strength (REPT%ION): label k carries ionic[k-1];
  * each replica takes its label's (src/init_ele.F90, energy_replica and
  * replica_exchange treat it like the temperature) */
+/* make_replica_exchange_pair_tb (src/var_replica.F90:162-259) for ndim
+ * replica dimensions of nv[d] values each (labels laid out as
+ * src/init_replica.F90:66-93 does, the first dimension slowest).  Table
+ * icounter = (iparity - 1) * ndim + idimn pairs labels n_continue apart
+ * along dimension idimn, odd pairs first. */
+void orc_set_rep_dims(orc_sys *s, int32_t ndim, const int32_t *nv) {
+    int R = s->nrep;
+    free(s->pair_tb);
+    s->pair_tb = calloc((size_t)2 * ndim * (R + 2), sizeof(int));
+    s->pair_sets = 2 * ndim;
+    int icounter = 0;
+    for (int iparity = 1; iparity <= 2; iparity++) {
+        int n_division = 1;
+        for (int idimn = 1; idimn <= ndim; idimn++) {
+            icounter++;
+            int *tb = s->pair_tb + (size_t)(icounter - 1) * (R + 2);
+            int nrep_v = nv[idimn - 1];
+            int n_division_pre = n_division;
+            n_division *= nrep_v;
+            int n_continue = R / n_division;
+            int iset = 0;
+            for (int icycle = 1; icycle <= n_division_pre; icycle++) {
+                int post_zero = 0, start;
+                if (iparity == 1) {
+                    if (nrep_v % 2 == 1) post_zero = 1;
+                    start = 1;
+                } else {
+                    for (int ic = 1; ic <= n_continue; ic++) tb[++iset] = 0;
+                    if (nrep_v % 2 == 0) post_zero = 1;
+                    start = 2;
+                }
+                for (int irep = start; irep <= nrep_v - 1; irep += 2)
+                    for (int ic = 1; ic <= 2 * n_continue; ic++) {
+                        iset++;
+                        tb[iset] = (ic <= n_continue) ? iset + n_continue : iset - n_continue;
+                    }
+                if (post_zero)
+                    for (int ic = 1; ic <= n_continue; ic++) tb[++iset] = 0;
+            }
+        }
+    }
+    s->idx_pair = 0;
+}
+
+void orc_pair_table(orc_sys *s, int32_t *out) {
+    memcpy(out, s->pair_tb, sizeof(int) * (size_t)s->pair_sets * (s->nrep + 2));
+}
+
 void orc_set_ionic(orc_sys *s, const double *ionic) {
     for (int k = 1; k <= s->nrep; k++) s->lab2ion[k] = ionic[k - 1];
     for (int r = 0; r < s->nrep; r++) {
@@ -1753,7 +1795,7 @@ static void exchange(orc_sys *s, double *energies_out) {
             for (int k = 0; k < 3 * s->nmp; k++) { vi[k] = sf * vi[k]; vj[k] = vj[k] / sf; }
         }
     }
-    s->idx_pair = (s->idx_pair + 1) % 2;
+    s->idx_pair = (s->idx_pair + 1) % s->pair_sets;   /* set_forward, src/var_replica.F90:104-117 */
     for (int r = 0; r < n; r++) {
         s->rep[r].tK = s->lab2val[s->rep2lab[r + 1]];
         s->rep[r].ionic = s->lab2ion[s->rep2lab[r + 1]];

@@ -55,4 +55,6 @@ void orc_set_counter_rng(orc_sys *s, uint64_t seed, int32_t on);
 void orc_philox(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
 void orc_set_nl_brute(orc_sys *s, int32_t on);   /* 1: neighbour lists by the O(N^2) loops only */
 void orc_analysis_lists(orc_sys *s, int32_t irep);   /* list_bp / list_exv / list_ele (DCD, CHECK_FORCE jobs) */
+void orc_set_rep_dims(orc_sys *s, int32_t ndim, const int32_t *nvalues);   /* exchange_pair_tb of a replica grid */
+void orc_pair_table(orc_sys *s, int32_t *out);   /* [2 ndim][nrep + 2] */
 #endif

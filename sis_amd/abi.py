@@ -184,7 +184,7 @@ EXPORTED = [
     "sis_gpu_rebuild", "sis_gpu_analysis_lists", "sis_gpu_force", "sis_gpu_energy", "sis_gpu_energy_noalloc", "sis_gpu_autotune",
     "sis_gpu_get_pair_count", "sis_gpu_get_pairs", "sis_gpu_get_bp",
     "sis_gpu_set_rng", "sis_gpu_set_rng_tape", "sis_gpu_init_velocities",
-    "sis_gpu_step", "sis_gpu_exchange", "sis_gpu_get_labels",
+    "sis_gpu_step", "sis_gpu_exchange", "sis_gpu_set_rep_dims", "sis_gpu_get_labels",
     "sis_gpu_get_info", "sis_gpu_last_kernel_ms",
     "sis_pool_create", "sis_pool_submit", "sis_pool_poll", "sis_pool_forget",
     "sis_pool_stats", "sis_pool_destroy",
@@ -226,6 +226,7 @@ def _declare(lib: C.CDLL) -> None:
         "sis_gpu_init_velocities": (i32, [vp]),
         "sis_gpu_step": (i32, [vp, i64, P(i64)]),
         "sis_gpu_exchange": (i32, [vp, P(d), P(i32)]),
+        "sis_gpu_set_rep_dims": (i32, [vp, i32, P(i32)]),
         "sis_gpu_get_labels": (i32, [vp, P(i32)]),
         "sis_gpu_get_info": (i32, [vp, P(i64), i32]),
         "sis_gpu_last_kernel_ms": (d, [vp, i32]),

@@ -1,8 +1,8 @@
 // On-device replica exchange for the batched form: energy_replica and
 // replica_exchange (src/energy_replica.F90:47-156,
-// src/replica_exchange.F90:35-154, src/job_md.F90:536-567) for a
-// one-dimensional temperature or ionic-strength ladder, with no host round
-// trip.  The conditions every label carries (temperature, DH coefficient
+// src/replica_exchange.F90:35-154, src/job_md.F90:536-567) over labels
+// that carry a temperature and an ionic strength (ladders of either, or a
+// grid of both), with no host round trip.  The conditions every label carries (temperature, DH coefficient
 // and Debye length, the noise amplitudes md_cr) are tabulated by the host
 // once, from the same formulas the host path evaluates per exchange, so
 // both paths give the same numbers.  Included by sis_gpu.hip only.
@@ -13,8 +13,9 @@ namespace sis {
 struct ExchDev {
   int R, N;
   int *rep2lab, *lab2rep;     // [R+1], 1-based as var_replica's tables
-  const int *pair_tb;         // [2][R+2]
-  int *idx_pair;              // [1] which half of the pair table is next
+  const int *pair_tb;         // [pair_sets][R+2]
+  int *idx_pair;              // [1] which set of the pair table is next
+  int pair_sets;              // 2 x the number of replica dimensions
   const double *lab_T, *lab_coef, *lab_lam;   // [R+1]
   const double *lab_cr;       // [(R+1) * N]
   double *opp_T, *opp_coef, *opp_lam;          // [R] counterpart conditions of this exchange
@@ -76,7 +77,7 @@ __global__ void k_exchange_decide(ExchDev X) {
       X.vs_f[rj - 1] = sf; X.vs_div[rj - 1] = 1;
     }
   }
-  X.idx_pair[0] = (X.idx_pair[0] + 1) % 2;
+  X.idx_pair[0] = (X.idx_pair[0] + 1) % X.pair_sets;
   X.nexch[0] += 1;
 }
 

@@ -169,7 +169,8 @@ struct sis_gpu_ctx {
   double dcf_dir[3] = {0, 0, 0}; // unit direction of the one constant-force pair
   bool dcf_remd = false;
   int idx_pair = 0;
-  std::vector<int> pair_tb;  // [2][R+2]
+  int pair_sets = 2;         // 2 x the number of replica dimensions
+  std::vector<int> pair_tb;  // [pair_sets][R+2]
   std::vector<uint32_t> rep_tape;
   int64_t rep_tape_pos = 0;
   std::vector<int64_t> tape_off_h;
@@ -834,6 +835,40 @@ void energy_phase(sis_gpu_ctx *c, const std::vector<int> &mask, const std::vecto
 
 int get_pair(sis_gpu_ctx *c, int lab) { return c->pair_tb[c->idx_pair * (c->R + 2) + lab]; }
 
+// exchange_pair_tb (src/var_replica.F90:162-259) of a grid of replicas with
+// dims[d] values along dimension d (R = their product; labels laid out as
+// init_replica does, the first dimension slowest): set parity * ndim + d
+// pairs, along dimension d, labels l and l + n (n = R / (dims[0] ...
+// dims[d])) from the first value on (odd parity) or the second (even);
+// 0 marks a label without a partner.  Sets are used in that order.
+std::vector<int> make_pair_tb(int R, const std::vector<int> &dims) {
+  const int nd = (int)dims.size();
+  std::vector<int> tb((size_t)2 * nd * (R + 2), 0);
+  for (int par = 0; par < 2; par++) {
+    int ndiv = 1;
+    for (int d = 0; d < nd; d++) {
+      int *t = tb.data() + (size_t)(par * nd + d) * (R + 2);
+      const int nv = dims[d], ndiv_pre = ndiv;
+      ndiv *= nv;
+      const int n = R / ndiv;
+      int iset = 0;
+      for (int cyc = 0; cyc < ndiv_pre; cyc++) {
+        if (par == 1)
+          for (int k = 0; k < n; k++) t[++iset] = 0;
+        for (int v = par == 0 ? 1 : 2; v <= nv - 1; v += 2)
+          for (int k = 1; k <= 2 * n; k++) {
+            ++iset;
+            t[iset] = k <= n ? iset + n : iset - n;
+          }
+        // the last value has no partner when the pairs leave it out
+        if ((par == 0) == (nv % 2 == 1))
+          for (int k = 0; k < n; k++) t[++iset] = 0;
+      }
+    }
+  }
+  return tb;
+}
+
 double rep_uniform(sis_gpu_ctx *c, uint64_t k) {
   if (c->parity) {
     if (c->rep_tape_pos + 2 > (int64_t)c->rep_tape.size())
@@ -852,7 +887,7 @@ double rep_uniform(sis_gpu_ctx *c, uint64_t k) {
 void dcf_upload(sis_gpu_ctx *c);
 
 // energy_replica + replica_exchange (src/energy_replica.F90:47-156,
-// src/replica_exchange.F90:35-154) for a one-dimensional temperature ladder.
+// src/replica_exchange.F90:35-154) with the current set of the pair table.
 void do_exchange(sis_gpu_ctx *c, double *replica_energies, int32_t *accepted, double *own_energies = nullptr) {
   const int R = c->R;
   std::vector<int> all(R, 1);
@@ -923,7 +958,7 @@ void do_exchange(sis_gpu_ctx *c, double *replica_energies, int32_t *accepted, do
       hipLaunchKernelGGL(k_scale_vel, dim3((c->N + B - 1) / B), dim3(B), 0, c->stream, c->P, c->ptrs(), rj - 1, sf, 1);
     }
   }
-  c->idx_pair = (c->idx_pair + 1) % 2;
+  c->idx_pair = (c->idx_pair + 1) % c->pair_sets;   // set_forward (src/var_replica.F90:104-117)
   for (int r = 0; r < R; r++) {
     c->tK[r] = c->lab2val[c->rep2lab[r + 1]];
     c->ionic[r] = c->lab2ion[c->rep2lab[r + 1]];
@@ -1356,14 +1391,8 @@ int sis_gpu_create(const sis_topo *topo, const sis_params *prm, const sis_option
     c->lab2ion.assign(R + 1, prm->ionic_strength);
     c->lab2rg.assign(R + 1, 0.0);
     c->lab2dcf.assign(R + 1, 0.0);
-    c->pair_tb.assign(2 * (R + 2), 0);
-    for (int par = 0; par < 2; par++) {
-      int *tb = c->pair_tb.data() + par * (R + 2);
-      int iset = 0;
-      if (par == 1) tb[++iset] = 0;
-      for (int ir = (par == 0 ? 1 : 2); ir <= R - 1; ir += 2) { iset++; tb[iset] = iset + 1; iset++; tb[iset] = iset - 1; }
-      while (iset < R) tb[++iset] = 0;
-    }
+    c->pair_tb = make_pair_tb(R, std::vector<int>{R});
+    c->pair_sets = 2;
     size_t RN = (size_t)R * N;
     c->md_cr_d.alloc(RN);
     c->tK_d.alloc(R); c->ele_coef_d.alloc(R); c->lambdaD_d.alloc(R); c->ele_cut_d.alloc(R);
@@ -1941,6 +1970,25 @@ int sis_gpu_autotune(sis_gpu_ctx *c, const double *skins, int32_t nskin, int64_t
   if (!c || !skins || nskin < 1 || nsteps < 1) return fail(SIS_ERR_ARG, "invalid argument");
   if (c->parity) return fail(SIS_ERR_UNSUPPORTED, "the autotuner runs the performance forms");
   autotune(c, skins, nskin, nsteps, report);
+  return SIS_OK;
+  API_END
+}
+
+int sis_gpu_set_rep_dims(sis_gpu_ctx *c, int32_t ndim, const int32_t *nvalues) {
+  API_BEGIN
+  if (!c || !nvalues) return fail(SIS_ERR_ARG, "null argument");
+  if (ndim < 1 || ndim > 4) return fail(SIS_ERR_ARG, "ndim must be 1..4");
+  int64_t prod = 1;
+  std::vector<int> dims(ndim);
+  for (int d = 0; d < ndim; d++) {
+    if (nvalues[d] < 1) return fail(SIS_ERR_ARG, "a dimension needs at least one value");
+    dims[d] = nvalues[d];
+    prod *= nvalues[d];
+  }
+  if (prod != c->R) return fail(SIS_ERR_ARG, "the product of the dimensions differs from nrep");
+  c->pair_tb = make_pair_tb(c->R, dims);
+  c->pair_sets = 2 * ndim;
+  c->idx_pair = 0;
   return SIS_OK;
   API_END
 }

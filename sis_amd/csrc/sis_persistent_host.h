@@ -417,7 +417,7 @@ int64_t batched_device_run(sis_gpu_ctx *c, int64_t nseg) {
   c->block = B;
   // label tables from the host's own formulas
   auto need = [](auto &b, size_t n) { if (b.n < n) b.alloc(n); };
-  need(c->dx_rep2lab_d, R + 1); need(c->dx_lab2rep_d, R + 1); need(c->dx_pair_tb_d, 2 * (size_t)(R + 2));
+  need(c->dx_rep2lab_d, R + 1); need(c->dx_lab2rep_d, R + 1); need(c->dx_pair_tb_d, c->pair_tb.size());
   need(c->dx_idx_d, 1); need(c->dx_has_d, R); need(c->dx_vsdiv_d, R); need(c->dx_nexch_d, 1);
   need(c->dx_acc_d, R / 2 + 1);
   need(c->dx_labT_d, R + 1); need(c->dx_labCoef_d, R + 1); need(c->dx_labLam_d, R + 1);
@@ -453,7 +453,7 @@ int64_t batched_device_run(sis_gpu_ctx *c, int64_t nseg) {
   ExchDev X{};
   X.R = R; X.N = N;
   X.rep2lab = c->dx_rep2lab_d.p; X.lab2rep = c->dx_lab2rep_d.p; X.pair_tb = c->dx_pair_tb_d.p;
-  X.idx_pair = c->dx_idx_d.p;
+  X.idx_pair = c->dx_idx_d.p; X.pair_sets = c->pair_sets;
   X.lab_T = c->dx_labT_d.p; X.lab_coef = c->dx_labCoef_d.p; X.lab_lam = c->dx_labLam_d.p; X.lab_cr = c->dx_labCr_d.p;
   X.opp_T = c->dx_oppT_d.p; X.opp_coef = c->dx_oppCoef_d.p; X.opp_lam = c->dx_oppLam_d.p; X.has_opp = c->dx_has_d.p;
   X.energy = c->dx_energy_d.p; X.vs_f = c->dx_vsf_d.p; X.vs_div = c->dx_vsdiv_d.p;

@@ -113,6 +113,12 @@ class Engine:
         _check(lib().sis_gpu_set_rep_conditions(self.h, abi.dptr(t) if t is not None else None,
                                                 abi.dptr(i) if i is not None else None))
 
+    def set_rep_dims(self, nvalues):
+        """Replicas on a grid (REPT order of the variables with ladders,
+        first dimension slowest): the pair table of each dimension in turn."""
+        v = np.ascontiguousarray(nvalues, np.int32)
+        _check(lib().sis_gpu_set_rep_dims(self.h, len(v), abi.iptr(v)))
+
     # ---- hot path ----
     def neighbor_list(self, irep: int = 0):
         _check(lib().sis_gpu_rebuild(self.h, irep))

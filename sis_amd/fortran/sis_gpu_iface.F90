@@ -196,6 +196,15 @@ module sis_gpu_iface
          type(c_ptr), value :: ctx, replica_energies, accepted
       end function
 
+      ! replaces: make_replica_exchange_pair_tb (src/var_replica.F90:162-259)
+      ! for ndim_replica > 1; nvalues = nrep(ivar) of the variables with ladders
+      integer(c_int) function sis_gpu_set_rep_dims(ctx, ndim, nvalues) bind(C, name="sis_gpu_set_rep_dims")
+         import :: c_int, c_ptr, c_int32_t
+         type(c_ptr), value :: ctx
+         integer(c_int32_t), value :: ndim
+         integer(c_int32_t), intent(in) :: nvalues(*)
+      end function
+
       integer(c_int) function sis_gpu_get_labels(ctx, rep2lab) bind(C, name="sis_gpu_get_labels")
          import :: c_int, c_ptr
          type(c_ptr), value :: ctx, rep2lab

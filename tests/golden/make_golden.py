@@ -233,6 +233,22 @@ def remd_force():
     return sysd
 
 
+def remd_grid():
+    """Two-dimensional replica exchange, temperature x ionic strength
+    (src/init_replica.F90:66-93 label layout, the pair table of
+    src/var_replica.F90:162-259: sets T-odd, ION-odd, T-even, ION-even in
+    turn): 2 temperatures x 3 ionic strengths, exchange every 10 steps."""
+    temps_t = np.array([300.0, 304.0])
+    ion = np.array([0.05, 0.15, 0.4])
+    sysd = configs.remd_c3(nrep=6, n=30, seed=12)
+    sysd["temps"] = np.repeat(temps_t, len(ion))
+    sysd["remd_ion"] = np.tile(ion, len(temps_t))
+    sysd["remd_grid"] = dict(temps=temps_t, ion=ion)
+    sysd["remd_dims"] = np.array([len(temps_t), len(ion)], np.int32)
+    sysd["name"] = "remd2x3"
+    return sysd
+
+
 REF_DIR = Path("/root/reference")   # the reference's own example inputs (read here only, when writing fixtures)
 
 
@@ -324,6 +340,7 @@ SYSTEMS = {
     # cutoff's timing moves the trajectory by far more than the tolerances
     "t2s2hp_ion8c3": (lambda: t2s2hp_ion8(3.0, "t2s2hp_ion8c3"), 100, 10),
     "t2s2hp_temp8c2": (lambda: t2s2hp_temp8(2.0, "t2s2hp_temp8c2"), 100, 10),
+    "remd2x3": (remd_grid, 80, 10),
 }
 LIST_PARAMS = ("nn_dH", "nn_dS", "nnend_dH", "nnend_dS")
 
@@ -370,6 +387,8 @@ def make(name, builder, nstep, nex, seed=4321):
     ref1 = refdrv.run(s, pre1 + [("job_md", nstep, "R")], seed=seed, nstep_exchange=nex)
     # count the words the static part + the run consume, using the oracle
     o = corc.Oracle(s)
+    if s.get("remd_dims") is not None:
+        o.set_rep_dims(s["remd_dims"])
     if s.get("remd_ion") is not None:
         o.set_ionic(s["remd_ion"])
     if ext:
@@ -395,6 +414,8 @@ def make(name, builder, nstep, nex, seed=4321):
     o.force(1, fmc)
     used_static = [o.tape_pos(r) for r in range(1, nrep + 1)]
     o2 = corc.Oracle(s)
+    if s.get("remd_dims") is not None:
+        o2.set_rep_dims(s["remd_dims"])
     if s.get("remd_ion") is not None:
         o2.set_ionic(s["remd_ion"])
     if ext:
@@ -449,6 +470,10 @@ def make(name, builder, nstep, nex, seed=4321):
         out["twz_text"] = np.array(twz.read_text())
     if s.get("remd_ion") is not None:
         out["remd_ion"] = np.asarray(s["remd_ion"], np.float64)
+    if s.get("remd_dims") is not None:
+        out["remd_dims"] = np.asarray(s["remd_dims"], np.int32)
+        out["grid_temps"] = np.asarray(s["remd_grid"]["temps"], np.float64)
+        out["grid_ion"] = np.asarray(s["remd_grid"]["ion"], np.float64)
     if s.get("remd_rg") is not None:
         out["remd_rg"] = np.asarray(s["remd_rg"], np.float64)
     if s.get("remd_dcf") is not None:
@@ -498,6 +523,9 @@ def load(name):
         sysd["ss"] = d["ss_pairs"]
     if "remd_ion" in d:
         sysd["remd_ion"] = d["remd_ion"]
+    if "remd_dims" in d:
+        sysd["remd_dims"] = d["remd_dims"]
+        sysd["remd_grid"] = dict(temps=d["grid_temps"], ion=d["grid_ion"])
     if "remd_rg" in d:
         sysd["remd_rg"] = d["remd_rg"]
     if "remd_dcf" in d:

@@ -231,7 +231,7 @@ def test_c4_persistent_trajectory_matches_oracle(c4_system, prec):
     assert np.abs(v - vo).max() < TOL[prec]
 
 
-@pytest.mark.parametrize("ladder", ["temperature", "ionic"])
+@pytest.mark.parametrize("ladder", ["temperature", "ionic", "grid"])
 @pytest.mark.parametrize("block", [256, 128])
 def test_device_exchange_equals_host_exchange(ladder, block):
     """Batched replica exchange done on the device (energies at both
@@ -239,16 +239,23 @@ def test_device_exchange_equals_host_exchange(ladder, block):
     and rescale on the device, segments replayed as a HIP graph) against
     the host's sweep between segments: same labels and the same
     trajectories, on a temperature and on an ionic-strength ladder, MC
-    every step, 1 A skin (in-kernel rebuilds)."""
+    every step, 1 A skin (in-kernel rebuilds); and on a grid of 4
+    temperatures x 4 ionic strengths (the pair table of each dimension in
+    turn)."""
     s = configs.remd_c3(nrep=16, n=128, seed=3, margin=1.0)
     if ladder == "ionic":
         s["temps"] = np.full(16, 300.0)
+    if ladder == "grid":
+        s["temps"] = np.repeat([290.0, 310.0, 335.0, 365.0], 4)
     print(f"[case] device vs host exchange {ladder} block {block}", flush=True)
     res = []
     for host in (True, False):
         e = Engine(s, mode="batched", precision=0, seed=SEED, exchange_every=50, block=block, host_exchange=host)
         if ladder == "ionic":
             e.set_rep_conditions(None, np.geomspace(0.02, 0.8, 16))
+        if ladder == "grid":
+            e.set_rep_dims([4, 4])
+            e.set_rep_conditions(None, np.tile([0.02, 0.08, 0.2, 0.8], 4))
         e.init_velocities()
         e.neighbor_list(0)
         e.step(300)

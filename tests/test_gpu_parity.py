@@ -21,7 +21,7 @@ from tests.golden.make_golden import force_mc, load
 pytestmark = pytest.mark.gpu
 
 FIXTURES = ["hairpin64", "small_pbc", "mixed", "remd4", "remd4mc3", "remd4bd", "remd2ext", "hairpin_ss", "hairpin_anneal", "pbc_resize", "cag47_m1", "remd2rgs", "remd4ion", "remd4rg", "remd4dcf",
-            "t2s2hp_sisff", "t2s2hp_ion8", "t2s2hp_temp8", "t2s2hp_ion8c3", "t2s2hp_temp8c2"]
+            "t2s2hp_sisff", "t2s2hp_ion8", "t2s2hp_temp8", "t2s2hp_ion8c3", "t2s2hp_temp8c2", "remd2x3"]
 
 
 def _sorted_rows(a):
@@ -34,6 +34,8 @@ def _pair_sets_equal(eng_pairs, ref_pairs):
 
 def _engine(sysd, d=None, **kw):
     eng = Engine(sysd, parity=True, **kw)
+    if "remd_dims" in sysd:
+        eng.set_rep_dims(sysd["remd_dims"])
     if "remd_ion" in sysd:
         eng.set_rep_conditions(None, sysd["remd_ion"])
     if "external" in sysd:

@@ -12,12 +12,14 @@ from oracle import corc, refdrv
 from tests.golden.make_golden import force_mc, load
 
 FIXTURES = ["hairpin64", "small_pbc", "mixed", "remd4", "remd4mc3", "remd4bd", "remd2ext", "hairpin_ss", "hairpin_anneal", "pbc_resize", "cag47_m1", "remd2rgs", "remd4ion", "remd4rg", "remd4dcf",
-            "t2s2hp_sisff", "t2s2hp_ion8", "t2s2hp_temp8", "t2s2hp_ion8c3", "t2s2hp_temp8c2"]
+            "t2s2hp_sisff", "t2s2hp_ion8", "t2s2hp_temp8", "t2s2hp_ion8c3", "t2s2hp_temp8c2", "remd2x3"]
 
 
 def _oracle_from_fixture(name):
     sysd, d = load(name)
     o = corc.Oracle(sysd)
+    if "remd_dims" in sysd:
+        o.set_rep_dims(sysd["remd_dims"])
     if "remd_ion" in sysd:
         o.set_ionic(sysd["remd_ion"])
     if "external" in sysd:
@@ -431,6 +433,46 @@ def test_cag47_openmm_energies():
     # build's own job_dcd over the same frames
     want = np.load(Path(__file__).parent / "golden" / "cag47_dcd.npz")["energies"]
     assert np.allclose(E, want, rtol=1e-12, atol=1e-12)
+
+
+def _grid_pairs(dims):
+    """Exchange partners of every label on a replica grid, set by set,
+    from the grid coordinates alone: set (parity, d) moves along dimension
+    d, pairing values (1,2), (3,4), ... for the odd parity and (2,3),
+    (4,5), ... for the even one; labels count with the first dimension
+    slowest.  0 = no partner."""
+    dims = list(dims)
+    R = int(np.prod(dims))
+    idx = np.array(np.unravel_index(np.arange(R), dims)).T   # 0-based values
+    sets = []
+    for par in (0, 1):
+        for d, n in enumerate(dims):
+            row = np.zeros(R + 2, np.int32)
+            for lab in range(R):
+                v = idx[lab, d] + 1
+                if par == 0:
+                    w = v + 1 if v % 2 == 1 else v - 1
+                else:
+                    w = v + 1 if v % 2 == 0 else v - 1
+                if par == 1 and v == 1:
+                    w = 0
+                if w < 1 or w > n:
+                    continue
+                j = idx[lab].copy()
+                j[d] = w - 1
+                row[lab + 1] = np.ravel_multi_index(tuple(j), dims) + 1
+            sets.append(row)
+    return np.array(sets)
+
+
+@pytest.mark.parametrize("dims", [[6], [5], [2, 3], [3, 2], [4, 4], [3, 5, 2], [2, 2, 2]])
+def test_replica_grid_pair_table(dims):
+    """make_replica_exchange_pair_tb (src/var_replica.F90:162-259) of the
+    oracle against the partners the grid coordinates give."""
+    s = configs.remd_c3(nrep=int(np.prod(dims)), n=12, seed=3)
+    o = corc.Oracle(s)
+    o.set_rep_dims(dims)
+    assert np.array_equal(o.pair_table(), _grid_pairs(dims))
 
 
 def test_dcd_split_fixture():
