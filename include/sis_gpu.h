@@ -123,6 +123,13 @@ int sis_gpu_set_state(sis_gpu_ctx *ctx, const double *xyz, const double *velos,
                       const double *accels);
 int sis_gpu_get_state(sis_gpu_ctx *ctx, double *xyz, double *velos, double *accels);
 
+/* Coordinates (3, nmp) of replica irep, as the reference's force /
+ * energy_sumup read them from var_state's xyz(:,:,irep) when its own loop
+ * integrates on the host (src/job_md.F90:397,290).  Unlike
+ * sis_gpu_set_state the lists are kept: the caller rebuilds them when
+ * its own skin test says so (src/job_md.F90:372-395). */
+int sis_gpu_set_positions(sis_gpu_ctx *ctx, int32_t irep, const double *xyz);
+
 /* Per-replica temperature [nrep] and ionic strength [nrep] (NULL keeps the
  * current value); re-derives set_ele and set_md_coef per replica
  * (src/job_md.F90:536-567). */
@@ -186,6 +193,12 @@ int sis_gpu_get_bp(sis_gpu_ctx *ctx, int32_t irep, int32_t *bp_mp, double *bp_co
  * mt_stream.F90:708-775); irep = -1 gives the exchange stream mts_rep. */
 int sis_gpu_set_rng(sis_gpu_ctx *ctx, uint64_t seed);
 int sis_gpu_set_rng_tape(sis_gpu_ctx *ctx, int32_t irep, const uint32_t *words, int64_t n);
+
+/* Parity mode: words of replica irep's tape used so far (irep = -1: the
+ * exchange tape), so that a caller can advance its own mts(irep) stream
+ * by what the limit MC drew (genrand_double1 / genrand_double3 in
+ * src/force_bp_limit_triplet.F90, src/energy_bp_limit_triplet.F90). */
+int sis_gpu_get_rng_tape_pos(sis_gpu_ctx *ctx, int32_t irep, int64_t *pos);
 
 /* Maxwell-Boltzmann velocities and initial GJF noise terms
  * (src/job_md.F90:157-176). */

@@ -17,6 +17,9 @@ import numpy as np
 HERE = Path(__file__).resolve().parent
 REF_SO = HERE / "_ref" / "libsisref.so"
 REF_OMP_SO = HERE / "_ref" / "libsisref_omp.so"
+# the serial build with force / neighbor_list / energy_sumup replaced by the
+# drop-in layer on sis_amd/libsis_gpu.so (oracle/Makefile target dropin)
+REF_GPU_SO = HERE / "_ref" / "libsisref_gpu.so"
 
 
 def available(omp: bool = False) -> bool:
@@ -233,7 +236,7 @@ def _worker(q, so, sysd, ops, seed, nstep_exchange):
 
 
 def run(sysd: dict, ops: list, seed: int = 12345, omp: bool = False, nstep_exchange: int = 0,
-        timeout: float = 600.0, threads: int = 1) -> dict:
+        timeout: float = 600.0, threads: int = 1, dropin: bool = False) -> dict:
     """Run `ops` against a fresh reference process set up with `sysd`.
 
     ops entries (last element is the result key where one is produced):
@@ -244,9 +247,9 @@ def run(sysd: dict, ops: list, seed: int = 12345, omp: bool = False, nstep_excha
       ("boxmuller", irep, n, key) ; ("set_xyz", xyz) ; ("get_xyz", key)
       ("job_md", nstep, key)
     """
-    so = REF_OMP_SO if omp else REF_SO
+    so = REF_GPU_SO if dropin else (REF_OMP_SO if omp else REF_SO)
     if not so.exists():
-        raise RuntimeError(f"{so} not built (make -C oracle ref)")
+        raise RuntimeError(f"{so} not built (make -C oracle ref dropin)")
     ctx = mp.get_context("spawn")
     q = ctx.Queue()
     env_old = {k: os.environ.get(k) for k in ("OMP_NUM_THREADS", "OMP_STACKSIZE")}

@@ -180,7 +180,7 @@ u8ptr = lambda a: _ptr(a, C.c_uint8)  # noqa: E731
 
 EXPORTED = [
     "sis_gpu_create", "sis_gpu_destroy", "sis_gpu_last_error",
-    "sis_gpu_set_state", "sis_gpu_get_state", "sis_gpu_set_rep_conditions",
+    "sis_gpu_set_state", "sis_gpu_get_state", "sis_gpu_set_positions", "sis_gpu_get_rng_tape_pos", "sis_gpu_set_rep_conditions",
     "sis_gpu_rebuild", "sis_gpu_analysis_lists", "sis_gpu_force", "sis_gpu_energy", "sis_gpu_energy_noalloc", "sis_gpu_autotune",
     "sis_gpu_get_pair_count", "sis_gpu_get_pairs", "sis_gpu_get_bp",
     "sis_gpu_set_rng", "sis_gpu_set_rng_tape", "sis_gpu_init_velocities",
@@ -211,6 +211,8 @@ def _declare(lib: C.CDLL) -> None:
         "sis_gpu_last_error": (C.c_char_p, []),
         "sis_gpu_set_state": (i32, [vp, P(d), P(d), P(d)]),
         "sis_gpu_get_state": (i32, [vp, P(d), P(d), P(d)]),
+        "sis_gpu_set_positions": (i32, [vp, i32, P(d)]),
+        "sis_gpu_get_rng_tape_pos": (i32, [vp, i32, P(i64)]),
         "sis_gpu_set_rep_conditions": (i32, [vp, P(d), P(d)]),
         "sis_gpu_rebuild": (i32, [vp, i32]),
         "sis_gpu_analysis_lists": (i32, [vp]),

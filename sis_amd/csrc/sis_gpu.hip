@@ -1463,6 +1463,35 @@ int sis_gpu_set_state(sis_gpu_ctx *c, const double *xyz, const double *velos, co
   API_END
 }
 
+int sis_gpu_set_positions(sis_gpu_ctx *c, int32_t irep, const double *xyz) {
+  API_BEGIN
+  if (!c || !xyz) return fail(SIS_ERR_ARG, "null argument");
+  if (irep < 1 || irep > c->R) return fail(SIS_ERR_ARG, "irep out of range");
+  const int N = c->N;
+  std::vector<double4> h((size_t)N);
+  for (int i = 0; i < N; i++) h[i] = double4{xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2], 0.0};
+  HIPCHK(hipMemcpyAsync(c->pos_d.p + (size_t)(irep - 1) * N, h.data(), (size_t)N * sizeof(double4),
+                        hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return SIS_OK;
+  API_END
+}
+
+int sis_gpu_get_rng_tape_pos(sis_gpu_ctx *c, int32_t irep, int64_t *pos) {
+  API_BEGIN
+  if (!c || !pos) return fail(SIS_ERR_ARG, "null argument");
+  if (!c->parity) return fail(SIS_ERR_STATE, "context was not created in parity mode");
+  if (irep == -1) {
+    *pos = c->rep_tape_pos;
+    return SIS_OK;
+  }
+  if (irep < 1 || irep > c->R) return fail(SIS_ERR_ARG, "irep out of range");
+  HIPCHK(hipStreamSynchronize(c->stream));
+  HIPCHK(hipMemcpy(pos, c->tape_pos_d.p + (irep - 1), sizeof(int64_t), hipMemcpyDeviceToHost));
+  return SIS_OK;
+  API_END
+}
+
 int sis_gpu_get_state(sis_gpu_ctx *c, double *xyz, double *velos, double *accels) {
   API_BEGIN
   if (!c) return fail(SIS_ERR_ARG, "null context");

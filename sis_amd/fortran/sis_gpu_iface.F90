@@ -90,6 +90,15 @@ module sis_gpu_iface
          type(c_ptr), value :: ctx, xyz, velos, accels
       end function
 
+      ! coordinates of one replica, lists kept (the caller's own skin test
+      ! decides rebuilds, src/job_md.F90:372-395)
+      integer(c_int) function sis_gpu_set_positions(ctx, irep, xyz) bind(C, name="sis_gpu_set_positions")
+         import :: c_int, c_ptr, c_int32_t
+         type(c_ptr), value :: ctx
+         integer(c_int32_t), value :: irep
+         type(c_ptr), value :: xyz
+      end function
+
       integer(c_int) function sis_gpu_set_rep_conditions(ctx, tempK, ionic) &
             bind(C, name="sis_gpu_set_rep_conditions")
          import :: c_int, c_ptr
@@ -176,6 +185,14 @@ module sis_gpu_iface
          integer(c_int32_t), value :: irep
          type(c_ptr), value :: words
          integer(c_int64_t), value :: n
+      end function
+
+      ! words of the tape used so far: the caller advances mts(irep) by them
+      integer(c_int) function sis_gpu_get_rng_tape_pos(ctx, irep, pos) bind(C, name="sis_gpu_get_rng_tape_pos")
+         import :: c_int, c_ptr, c_int32_t, c_int64_t
+         type(c_ptr), value :: ctx
+         integer(c_int32_t), value :: irep
+         integer(c_int64_t), intent(out) :: pos
       end function
 
       integer(c_int) function sis_gpu_init_velocities(ctx) bind(C, name="sis_gpu_init_velocities")

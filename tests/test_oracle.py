@@ -508,6 +508,19 @@ def test_dcd_split_fixture():
 
 
 @pytest.mark.ref
+def test_dropin_refuses_what_it_does_not_cover():
+    """The drop-in layer under the reference's job_md stops with a message,
+    before any device call, on a setting it does not cover (here the
+    limit MC every 3 steps, whose bp_status_MC copies job_md keeps on the
+    host)."""
+    if not refdrv.REF_GPU_SO.exists():
+        pytest.skip("oracle/_ref/libsisref_gpu.so not built")
+    sysd, d = load("remd4mc3")
+    with pytest.raises(RuntimeError, match="exited with code 1"):
+        refdrv.run(sysd, [("job_md", 2, "R")], seed=4321, nstep_exchange=10, dropin=True, timeout=120)
+
+
+@pytest.mark.ref
 @pytest.mark.parametrize("seed", [11, 12, 13])
 def test_live_reference_analysis_lists(seed):
     """list_exv / list_ele / list_bp of the oracle against the reference
