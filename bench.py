@@ -68,23 +68,70 @@ def load_traffic(workload, mode):
         return None
 
 
-def cpu_baseline(sysd, steps, threads):
+def cpu_baseline(sysd, steps, threads, t_setup=None):
     """Reference Fortran (oracle/_ref, OpenMP build) timed on this host:
     job_md for `steps` steps minus a 0-step run (set-up, initial lists and
-    energies), i.e. the time-loop cost only."""
+    energies), i.e. the time-loop cost only.  t_setup: that 0-step time
+    when already measured."""
     from oracle import refdrv
     if not refdrv.available(omp=True):
         return None
     nex = int(sysd.get("exchange_every", 0))
     t1 = refdrv.run(sysd, [("job_md_timed", steps, "t")], omp=True, threads=threads, timeout=900,
                     nstep_exchange=nex)["t"]
-    t0 = refdrv.run(sysd, [("job_md_timed", 0, "t")], omp=True, threads=threads, timeout=900,
-                    nstep_exchange=nex)["t"]
-    dt = max(t1 - t0, 1e-9)
+    if t_setup is None:
+        t_setup = refdrv.run(sysd, [("job_md_timed", 0, "t")], omp=True, threads=threads, timeout=900,
+                             nstep_exchange=nex)["t"]
+    dt = max(t1 - t_setup, 1e-9)
     n = int(np.sum(sysd["nmp_chain"])) * int(sysd["nrep"])
     return dict(value=round(n * steps / dt, 1), unit="bead-steps/s", cores=threads, kind="reference",
                 sample=f"{steps} job_md steps of the same workload (set-up excluded), OMP_NUM_THREADS={threads}",
-                seconds=round(dt, 3))
+                seconds=round(dt, 3), setup_s=round(t_setup, 3))
+
+
+def host_cores():
+    """(cores this process may use, cores of one socket among them): from
+    the affinity mask and /proc/cpuinfo's physical ids; at most 16, the
+    host share of one GPU."""
+    cpus = sorted(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else list(range(os.cpu_count() or 1))
+    sock = {}
+    try:
+        cur = None
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("processor"):
+                cur = int(line.split(":")[1])
+            elif line.startswith("physical id") and cur is not None:
+                sock[cur] = int(line.split(":")[1])
+    except OSError:
+        pass
+    per = {}
+    for c in cpus:
+        per.setdefault(sock.get(c, 0), []).append(c)
+    allc = min(16, len(cpus))
+    one = min(allc, max(len(v) for v in per.values()) if per else allc)
+    return allc, one
+
+
+def cpu_baseline_legs(sysd, steps):
+    """The reference at 1 core, one socket and all cores of this host's
+    share (the 1-core leg runs 1/8 of the steps); the headline figure is
+    the all-core leg."""
+    allc, sock = host_cores()
+    legs, t_setup = [], None
+    for th in sorted({1, sock, allc}):
+        st = max(100, steps // 8) if th == 1 else steps
+        r = cpu_baseline(sysd, st, th, t_setup)
+        if r is None:
+            return None
+        t_setup = r["setup_s"]
+        legs.append(dict(threads=th, value=r["value"], steps=st, seconds=r["seconds"]))
+        print(f"[bench] cpu baseline {th} thread(s): {r['value']:.3e} bead-steps/s", flush=True)
+    out = dict(r)
+    out["legs"] = legs
+    out["host"] = f"{allc} cores usable, {sock} per socket"
+    out["mpi"] = ("not measured: no MPI in this image, so the reference's replica-parallel MPI build "
+                  "(PAR_MPI, one replica set per rank) cannot be built or run")
+    return out
 
 
 def single_replica(sysd, r=0):
@@ -460,7 +507,6 @@ def main():
             out["serving"] = serving_run(drawn, args.serving_scale, args.serving_rate, args.serving_max_chains)
             print(f"[bench] serving {out['serving']}", flush=True)
     if not args.no_cpu:
-        threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(16, os.cpu_count() or 1)
         # the reference loops over replicas one after another, so its rate per
         # bead does not depend on the replica count: the 256-replica workload
         # is sampled with 32 of its replicas (same chain, same 290-380 K span;
@@ -470,7 +516,7 @@ def main():
             csys = configs.remd_c3(nrep=32)
             cnote = "; sample: 32 of the 256 replicas (same chain and 290-380 K ladder span)"
         try:
-            out["cpu_baseline"] = cpu_baseline(csys, cpu_steps, threads)
+            out["cpu_baseline"] = cpu_baseline_legs(csys, cpu_steps)
             if out["cpu_baseline"] and cnote:
                 out["cpu_baseline"]["sample"] += cnote
         except Exception as exc:  # keep the GPU line even if the CPU leg fails

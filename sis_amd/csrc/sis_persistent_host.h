@@ -560,15 +560,18 @@ void apply_margin(sis_gpu_ctx *c, double m) {
 // Autotuner of the launch shape and the rebuild skin: from the current
 // state, times nsteps steps of the context's form at every candidate
 // (batched: threads per replica 128 / 256 / 512; persistent: 1, 2 and as
-// many workgroups per compute unit as fit) times every skin, restoring
-// the state before each trial and at the end, and keeps the fastest.
+// many workgroups per compute unit as fit; per-phase graph steps: their
+// one launch shape, shape 0) times every skin, restoring the state before
+// each trial and at the end, and keeps the fastest.
 // The skin changes when lists are rebuilt, not the forces (every term
 // applies its own cutoff), so the choice is a matter of speed only.
 void autotune(sis_gpu_ctx *c, const double *skins, int nskin, int64_t nsteps, double *report) {
   int mode = c->mode ? c->mode : choose_mode(c);
   const bool batched = mode == 3 && batched_ok(c);
   const bool pers = mode == 2 && c->R == 1 && !c->ext_on;
-  if (!batched && !pers) throw ApiError{SIS_ERR_UNSUPPORTED, "the autotuner covers the batched and persistent forms"};
+  const bool phases = !batched && !pers && mode == 1 && graph_steps_ok(c);
+  if (!batched && !pers && !phases)
+    throw ApiError{SIS_ERR_UNSUPPORTED, "the autotuner covers the batched, persistent and per-phase graph forms"};
   const int R = c->R, N = c->N;
   const size_t RN = (size_t)R * N;
   // saved state: coordinates, velocities, noise terms, step, labels
@@ -596,6 +599,8 @@ void autotune(sis_gpu_ctx *c, const double *skins, int nskin, int64_t nsteps, do
   std::vector<int> shapes;
   if (batched) {
     shapes = {128, 256, 512};
+  } else if (phases) {
+    shapes = {0};
   } else {
     int dev = 0, ncu = 0;
     HIPCHK(hipGetDevice(&dev));
@@ -609,7 +614,7 @@ void autotune(sis_gpu_ctx *c, const double *skins, int nskin, int64_t nsteps, do
   for (int sh : shapes) {
     for (int q = 0; q < nskin; q++) {
       if (!(skins[q] > 0.0)) throw ApiError{SIS_ERR_ARG, "skin must be positive"};
-      if (batched) c->opt.block = sh; else c->opt.grid = sh;
+      if (batched) c->opt.block = sh; else if (pers) c->opt.grid = sh;
       apply_margin(c, skins[q]);
       restore();
       HIPCHK(hipStreamSynchronize(c->stream));
@@ -623,7 +628,7 @@ void autotune(sis_gpu_ctx *c, const double *skins, int nskin, int64_t nsteps, do
       HIPCHK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
       const double per = ms / (double)nsteps;
       if (report) {
-        report[2 + 3 * k] = batched ? sh : c->block;
+        report[2 + 3 * k] = batched ? sh : (pers ? c->block : 0);
         report[3 + 3 * k] = skins[q];
         report[4 + 3 * k] = per;
       }
@@ -631,7 +636,7 @@ void autotune(sis_gpu_ctx *c, const double *skins, int nskin, int64_t nsteps, do
       if (per < best) { best = per; best_shape = sh; best_skin = skins[q]; }
     }
   }
-  if (batched) c->opt.block = best_shape; else c->opt.grid = best_shape;
+  if (batched) c->opt.block = best_shape; else if (pers) c->opt.grid = best_shape;
   (void)block0; (void)grid0;
   apply_margin(c, best_skin);
   restore();

@@ -155,7 +155,7 @@ class Engine:
         rep = np.zeros(2 + 9 * len(sk))
         _check(lib().sis_gpu_autotune(self.h, abi.dptr(sk), len(sk), int(nsteps), abi.dptr(rep)))
         trials = [dict(shape=int(rep[2 + 3 * k]), skin=float(rep[3 + 3 * k]), ms_per_step=round(float(rep[4 + 3 * k]), 5))
-                  for k in range(3 * len(sk))]
+                  for k in range(3 * len(sk)) if rep[3 + 3 * k] > 0]   # forms with one shape fill nskin trials
         return dict(shape=int(rep[0]), skin=float(rep[1]), trials=trials)
 
     def kinetic(self, irep: int = 0):

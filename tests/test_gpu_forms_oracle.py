@@ -297,6 +297,31 @@ def test_autotuner_sweeps_and_restores():
     assert np.array_equal(e.labels(), f.labels())
 
 
+def test_autotuner_per_phase_graph_form():
+    """sis_gpu_autotune on the per-phase form with graph-replayed steps (a
+    periodic condensate on the cell list): one trial per skin, shape 0, the
+    state restored; the run then matches a fresh context with the chosen
+    skin."""
+    s = configs.small_pbc(6, 40, 150.0, seed=8)
+    print("[case] autotuner per-phase", flush=True)
+    e = Engine(s, mode="phases", precision=1, seed=SEED)
+    e.init_velocities()
+    e.neighbor_list(0)
+    x0, v0, a0 = e.get_state()
+    rep = e.autotune((2.0, 4.0), nsteps=64)
+    assert [t["skin"] for t in rep["trials"]] == [2.0, 4.0] and all(t["shape"] == 0 for t in rep["trials"])
+    assert rep["shape"] == 0 and rep["skin"] in (2.0, 4.0)
+    x1, v1, a1 = e.get_state()
+    assert np.array_equal(x0, x1) and np.array_equal(v0, v1) and np.array_equal(a0, a1)
+    e.step(64)
+    s2 = dict(s, params=dict(s["params"], nl_margin=rep["skin"]))
+    f = Engine(s2, mode="phases", precision=1, seed=SEED)
+    f.set_state(x0, v0, a0)
+    f.neighbor_list(0)
+    f.step(64)
+    assert np.abs(e.get_state()[0] - f.get_state()[0]).max() < 1e-9
+
+
 @pytest.mark.parametrize("case", ["chain_mc1", "ladder_mc3", "pbc_cells"])
 def test_graph_steps_equal_host_steps(case):
     """The per-phase form replayed from captured graphs (device step count,
