@@ -464,7 +464,8 @@ int sis_dcd_write_file(const char *path, int32_t nmp, int32_t nframes, const dou
  * state, runs nsteps steps of the context's form for every launch shape
  * (batched form: 128, 256 and 512 threads per replica; persistent form:
  * 1 and 2 workgroups per compute unit and as many as fit; per-phase form
- * with graph-replayed steps: its one shape, reported as 0) and every
+ * with device-driven batches of steps: 0 replayed from a captured HIP
+ * graph, 1 the same launches issued directly) and every
  * neighbour-list skin skins[0..nskin) (A), restoring the state before each
  * trial, and keeps the fastest shape and skin (the skin moves rebuilds,
  * not forces).  The state is restored at the end and the lists are

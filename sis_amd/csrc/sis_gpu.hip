@@ -83,6 +83,15 @@ struct OutFile;
 struct RstOut;
 struct TwzOut;
 
+// What captured per-phase steps take by value: their graph is replayed only
+// while all of it is unchanged.
+struct PhaseGraphKey {
+  Params P;
+  Ptrs D;
+  const void *pos;
+  int nblk, block;
+};
+
 struct sis_gpu_ctx {
   DcdOut *dcd = nullptr;   // trajectory output (sis_gpu_dcd_open), or none
   BpOut *bpo = nullptr;    // base-pair output (sis_gpu_bp_open), or none
@@ -219,6 +228,11 @@ struct sis_gpu_ctx {
   DevBuf<double> dx_labT_d, dx_labCoef_d, dx_labLam_d, dx_labCr_d, dx_oppT_d, dx_oppCoef_d, dx_oppLam_d,
       dx_energy_d, dx_vsf_d;
   hipGraphExec_t dx_graph = nullptr;
+  // the per-phase form's captured steps (phases_graph_steps), one graph
+  // for each of the two position buffers a replay can start from
+  hipGraphExec_t pg_exec[2] = {nullptr, nullptr};
+  bool pg_eager = false;   // the same batches of steps issued directly instead (the autotuner's choice)
+  PhaseGraphKey pg_key[2]{};
   int dx_graph_key[4] = {0, 0, 0, 0};   // block, precision, exchange cadence, dynamic LDS of the captured graph
   int64_t dev_exchanges = 0;            // exchanges done on the device
 
@@ -302,6 +316,8 @@ struct sis_gpu_ctx {
                              &dx_oppLam_d, &dx_energy_d, &dx_vsf_d};
     for (auto *b : dxd) b->free_();
     if (dx_graph) (void)hipGraphExecDestroy(dx_graph);
+    for (auto &e : pg_exec)
+      if (e) (void)hipGraphExecDestroy(e);
     g_step_d.free_();
     g_int_d.free_();
     for (auto e : tev) (void)hipEventDestroy(e);
@@ -597,9 +613,11 @@ void rebuild(sis_gpu_ctx *c, const std::vector<int> &mask_in) {
                                                 c->bp_val2_d.p, total, 0, 64, c->stream));
       hipLaunchKernelGGL(k_bp_permute, dim3((total + B - 1) / B), dim3(B), 0, c->stream, D, total, c->bp_val2_d.p,
                          c->bp_mp2_d.p, c->bp_coef2_d.p);
-      std::swap(c->bp_mp_d, c->bp_mp2_d);
-      std::swap(c->bp_coef_d, c->bp_coef2_d);
-      D = c->ptrs();
+      // back into the same buffers: captured step graphs keep their pointers
+      HIPCHK(hipMemcpyAsync(c->bp_mp_d.p, c->bp_mp2_d.p, (size_t)total * 8 * sizeof(int), hipMemcpyDeviceToDevice,
+                            c->stream));
+      HIPCHK(hipMemcpyAsync(c->bp_coef_d.p, c->bp_coef2_d.p, (size_t)total * 2 * sizeof(double),
+                            hipMemcpyDeviceToDevice, c->stream));
       hipLaunchKernelGGL(k_bp_inc, dim3((total + B - 1) / B), dim3(B), 0, c->stream, c->P, D, total);
     }
     hipLaunchKernelGGL(k_inc_sort, grid_beads(N, R, B), dim3(B), 0, c->stream, c->P, D, nullptr);
@@ -698,22 +716,27 @@ void bp_phase(sis_gpu_ctx *c, const std::vector<int> *mc, const double *ecT_dev,
   } else if (!resets_done) {
     hipLaunchKernelGGL(k_reset_cnt, grid_beads(N, R, B), dim3(B), 0, c->stream, c->P, D, mask_dev);
   }
-  if (c->nbp_total > 0) {
+  // captured steps launch over the candidate capacity and read the count
+  // on the device, so that their graph stays valid across rebuilds
+  const bool dev_count = step_dev != nullptr;
+  const int nbp_arg = dev_count ? -1 : (int)c->nbp_total;
+  const int64_t nbp_grid = dev_count ? c->D.maxbp : c->nbp_total;
+  if (nbp_grid > 0) {
     if (c->precision == 1)
-      hipLaunchKernelGGL(k_bp_eval<float>, dim3((c->nbp_total + B - 1) / B), dim3(B), 0, c->stream, c->P, D,
-                         c->nbp_total, c->mc_d.p, ecT_dev, mask_dev, perf_mc ? 1 : 0);
+      hipLaunchKernelGGL(k_bp_eval<float>, dim3((nbp_grid + B - 1) / B), dim3(B), 0, c->stream, c->P, D,
+                         nbp_arg, c->mc_d.p, ecT_dev, mask_dev, perf_mc ? 1 : 0);
     else
-      hipLaunchKernelGGL(k_bp_eval<double>, dim3((c->nbp_total + B - 1) / B), dim3(B), 0, c->stream, c->P, D,
-                         c->nbp_total, c->mc_d.p, ecT_dev, mask_dev, perf_mc ? 1 : 0);
+      hipLaunchKernelGGL(k_bp_eval<double>, dim3((nbp_grid + B - 1) / B), dim3(B), 0, c->stream, c->P, D,
+                         nbp_arg, c->mc_d.p, ecT_dev, mask_dev, perf_mc ? 1 : 0);
   }
   if (!any_mc) return;
   if (c->parity) {
     hipLaunchKernelGGL(k_mc_serial, dim3(R), dim3(64), 0, c->stream, c->P, D, c->mc_d.p, ecT_dev,
                        c->ntlist_d.p, c->bpseq_d.p, mask_dev);
   } else {
-    if (c->nbp_total == 0) return;
-    hipLaunchKernelGGL(k_seg_fill, dim3((c->nbp_total + B - 1) / B), dim3(B), 0, c->stream, c->P, D,
-                       c->nbp_total, c->exc_cnt_d.p, c->seg_d.p, c->exc_fill_d.p, c->mc_d.p, mask_dev,
+    if (nbp_grid == 0) return;
+    hipLaunchKernelGGL(k_seg_fill, dim3((nbp_grid + B - 1) / B), dim3(B), 0, c->stream, c->P, D,
+                       nbp_arg, c->exc_cnt_d.p, c->seg_d.p, c->exc_fill_d.p, c->mc_d.p, mask_dev,
                        c->exc_list_d.p);
     const int nlist = (int)std::min<int64_t>((int64_t)std::max<int64_t>(RN, 1), 2048);
     hipLaunchKernelGGL(k_mc_list, dim3(nlist), dim3(64), 0, c->stream, c->P, D, c->exc_cnt_d.p, c->seg_d.p,
@@ -1080,6 +1103,27 @@ bool graph_steps_ok(sis_gpu_ctx *c) {
 // streams as step_phases.
 constexpr int GRAPH_STEPS = 16;   // even: the positions end in the buffer they started in
 
+// Runs batches of GRAPH_STEPS steps (launch() issues one batch: a captured
+// graph's replay or the same launches issued directly) until a replica
+// halts them (a rebuild is due) or the requested step is reached.
+template <typename F>
+void run_phase_batches(sis_gpu_ctx *c, F &&launch) {
+  int halt = 0;
+  int64_t now = c->istep;
+  while (!halt) {
+    launch();
+    HIPCHK(hipMemcpyAsync(&halt, c->g_int_d.p, sizeof halt, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpyAsync(&now, c->g_step_d.p, sizeof now, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (!halt && now - c->istep != (int64_t)GRAPH_STEPS) throw ApiError{SIS_ERR_STATE, "graph steps did not advance"};
+    if (!halt) c->istep = now;
+  }
+  // an odd number of steps in the last replay leaves the positions in the
+  // other buffer
+  if ((now - c->istep) % 2) std::swap(c->pos_d, c->pos2_d);
+  c->istep = now;
+}
+
 void phases_graph_steps(sis_gpu_ctx *c, int64_t nsteps) {
   const int R = c->R;
   if (c->g_step_d.n < 3) c->g_step_d.alloc(3);
@@ -1124,43 +1168,70 @@ void phases_graph_steps(sis_gpu_ctx *c, int64_t nsteps) {
     if ((int64_t)c->blk_top2_d.n < (int64_t)R * nblk * 2) c->blk_top2_d.alloc((size_t)R * nblk * 2);
     if (c->P.nstep_bp_mc > 0 && c->seg_d.n < (size_t)R * c->N * SEGCAP) c->seg_d.alloc((size_t)R * c->N * SEGCAP);
     if (c->P.nstep_bp_mc > 0 && c->exc_list_d.n < (size_t)R * c->N + 1) c->exc_list_d.alloc((size_t)R * c->N + 1);
-    // capture GRAPH_STEPS steps from the current position buffer
+    // GRAPH_STEPS steps from the current position buffer, captured once
+    // and replayed until something the launches take by value changes
+    // (parameters, buffer addresses and capacities, the current buffer)
+    PhaseGraphKey key;
+    std::memset(&key, 0, sizeof key);
+    key.P = c->P;
+    key.D = c->ptrs();
+    key.pos = c->pos_d.p;
+    key.nblk = c->nblk_force;
+    key.block = c->block;
+    const bool perf_mc = c->P.nstep_bp_mc > 0;
+    // one batch of steps: begin + resets, base pairs (+ MC), pair forces +
+    // update, skin test; the host's buffer swaps return to the start
+    auto launch_steps = [&]() {
+      for (int j = 0; j < GRAPH_STEPS; j++) {
+        hipLaunchKernelGGL(k_gstep_reset, grid_beads(c->N, R, 256), dim3(256), 0, c->stream, c->P, c->ptrs(), G,
+                           c->exc_cnt_d.p, c->exc_fill_d.p, perf_mc ? c->exc_list_d.p : nullptr, perf_mc ? 1 : 0);
+        bp_phase(c, nullptr, nullptr, G.active, G.istep_cur, true);
+        if (c->P.nstep_bp_mc > 1 && c->D.maxbp > 0)
+          hipLaunchKernelGGL(k_mc_keep, dim3((c->D.maxbp + 255) / 256), dim3(256), 0, c->stream, c->P, c->ptrs(),
+                             -1, G.mc, G.active);
+        force_kernel(c, 1, G.active, G.istep_cur);
+        std::swap(c->pos_d, c->pos2_d);
+        hipLaunchKernelGGL(k_skin<256>, dim3(R), dim3(256), 0, c->stream, c->P, c->ptrs(), c->nblk_force, G);
+      }
+    };
+    // issued directly: the host's launch cost overlaps the device's work,
+    // which pays for large systems (a graph launch costs more per node
+    // than a kernel launch here)
+    if (c->pg_eager) {
+      run_phase_batches(c, launch_steps);
+      continue;
+    }
+    int slot = -1;
+    for (int q = 0; q < 2; q++)
+      if (c->pg_exec[q] && std::memcmp(&key, &c->pg_key[q], sizeof key) == 0) slot = q;
+    if (slot >= 0) {
+      hipGraphExec_t ex = c->pg_exec[slot];
+      run_phase_batches(c, [&]() { HIPCHK(hipGraphLaunch(ex, c->stream)); });
+      continue;
+    }
+    // the slot of the other buffer's graph is kept when it is still valid
+    slot = 0;
+    if (c->pg_exec[0]) {
+      PhaseGraphKey k0 = c->pg_key[0];
+      k0.pos = key.pos;
+      k0.D.pos = key.D.pos;
+      k0.D.pos_out = key.D.pos_out;
+      if (std::memcmp(&k0, &key, sizeof key) == 0) slot = 1;
+    }
+    if (c->pg_exec[slot]) {
+      HIPCHK(hipGraphExecDestroy(c->pg_exec[slot]));
+      c->pg_exec[slot] = nullptr;
+    }
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-    const bool perf_mc = c->P.nstep_bp_mc > 0;
-    for (int j = 0; j < GRAPH_STEPS; j++) {
-      // begin + resets, base pairs (+ MC), pair forces + update, skin test
-      hipLaunchKernelGGL(k_gstep_reset, grid_beads(c->N, R, 256), dim3(256), 0, c->stream, c->P, c->ptrs(), G,
-                         c->exc_cnt_d.p, c->exc_fill_d.p, perf_mc ? c->exc_list_d.p : nullptr, perf_mc ? 1 : 0);
-      bp_phase(c, nullptr, nullptr, G.active, G.istep_cur, true);
-      if (c->P.nstep_bp_mc > 1 && c->nbp_total > 0)
-        hipLaunchKernelGGL(k_mc_keep, dim3((c->nbp_total + 255) / 256), dim3(256), 0, c->stream, c->ptrs(),
-                           (int)c->nbp_total, G.mc, G.active);
-      force_kernel(c, 1, G.active, G.istep_cur);
-      std::swap(c->pos_d, c->pos2_d);
-      hipLaunchKernelGGL(k_skin<256>, dim3(R), dim3(256), 0, c->stream, c->P, c->ptrs(), c->nblk_force, G);
-    }
+    launch_steps();
     HIPCHK(hipStreamEndCapture(c->stream, &graph));
     HIPCHK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
     HIPCHK(hipGraphDestroy(graph));
-    // replay until a halt
-    int halt = 0;
-    int64_t now = c->istep;
-    while (!halt) {
-      HIPCHK(hipGraphLaunch(exec, c->stream));
-      HIPCHK(hipMemcpyAsync(&halt, c->g_int_d.p, sizeof halt, hipMemcpyDeviceToHost, c->stream));
-      HIPCHK(hipMemcpyAsync(&now, c->g_step_d.p, sizeof now, hipMemcpyDeviceToHost, c->stream));
-      HIPCHK(hipStreamSynchronize(c->stream));
-      if (!halt && now - c->istep != (int64_t)GRAPH_STEPS)
-        throw ApiError{SIS_ERR_STATE, "graph steps did not advance"};
-      if (!halt) c->istep = now;
-    }
-    HIPCHK(hipGraphExecDestroy(exec));
-    // an odd number of steps in the last replay leaves the positions in
-    // the other buffer
-    if ((now - c->istep) % 2) std::swap(c->pos_d, c->pos2_d);
-    c->istep = now;
+    c->pg_exec[slot] = exec;
+    c->pg_key[slot] = key;
+    run_phase_batches(c, [&]() { HIPCHK(hipGraphLaunch(exec, c->stream)); });
   }
 }
 

@@ -422,6 +422,12 @@ __device__ __forceinline__ int uf_find(int *p, int x) {
 }
 
 // ---------------------------------------------------------------------
+// nbp_total < 0: the count of the current lists, read from the device (the
+// end of the candidates' scan; graph replays launch over the capacity)
+__device__ __forceinline__ int bp_count(const Params &P, const Ptrs &D, int nbp_total) {
+  return nbp_total >= 0 ? nbp_total : D.bp_off[(size_t)P.R * P.N];
+}
+
 // BP evaluation per candidate (src/force_bp_limit_triplet.F90:85-256 and
 // src/force_bp_triplet.F90).  mc[r]: MC step for replica r.  ecT (may be
 // null): temperature override per replica (energy at another condition).
@@ -429,7 +435,7 @@ template <typename T>
 __global__ void k_bp_eval(Params P, Ptrs D, int nbp_total, const int *mc, const double *ecT,
                           const int *mask, int do_union) {
   int gb = blockIdx.x * blockDim.x + threadIdx.x;
-  if (gb >= nbp_total) return;
+  if (gb >= bp_count(P, D, nbp_total)) return;
   const int *mp = D.bp_mp + (size_t)gb * 8;
   int r = mp[7];
   if (mask && !mask[r]) return;
@@ -678,7 +684,7 @@ __global__ void k_mc_reset(Params P, Ptrs D, int *seg_n, int *has_exc, int *exc_
 __global__ void k_seg_fill(Params P, Ptrs D, int nbp_total, int *seg_n, int *seg, int *has_exc,
                            const int *mc, const int *mask, int *exc_list) {
   int gb = blockIdx.x * blockDim.x + threadIdx.x;
-  if (gb >= nbp_total || !D.bp_status[gb]) return;
+  if (gb >= bp_count(P, D, nbp_total) || !D.bp_status[gb]) return;
   const int *mp = D.bp_mp + (size_t)gb * 8;
   int r = mp[7];
   if ((mask && !mask[r]) || !mc[r]) return;
@@ -1585,9 +1591,9 @@ __global__ void k_gstep_reset(Params P, Ptrs D, GStep G, int *seg_n, int *has_ex
 
 // bp_status_MC = bp_status for the replicas that did the MC this step
 // (src/job_md.F90:399-401)
-__global__ void k_mc_keep(Ptrs D, int nbp_total, const int *mc, const int *mask) {
+__global__ void k_mc_keep(Params P, Ptrs D, int nbp_total, const int *mc, const int *mask) {
   const int gb = blockIdx.x * blockDim.x + threadIdx.x;
-  if (gb >= nbp_total) return;
+  if (gb >= bp_count(P, D, nbp_total)) return;
   const int r = D.bp_mp[(size_t)gb * 8 + 7];
   if (mask[r] && mc[r]) D.bp_status_mc[gb] = D.bp_status[gb];
 }

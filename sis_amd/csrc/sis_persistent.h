@@ -324,6 +324,7 @@ struct SysIO {
   int *done, *nbp, *rebuild, *nreb;  // this system's slots
   int xlds;                     // beads whose positions fit the launch's dynamic LDS
   int fplds;                    // per-lane partial forces that fit after them
+  int nblds;                    // neighbour-list entries that fit after those (0: lists read from HBM)
   int force_only;               // evaluate the forces of the current positions into D.frc, no update
   // energy mode: one evaluation of energy_sumup at this system's D.tK (BP
   // terms and the limit MC at that temperature), DH coefficient e_coef and
@@ -348,6 +349,7 @@ struct BatchArgs {
   int capb, lpb;
   int xlds;   // beads whose positions fit the dynamic LDS of the launch
   int fplds;  // per-lane partial forces that fit after them
+  int nblds;  // neighbour-list entries that fit after those (0: lists read from HBM)
   int force_only;   // one force evaluation (limit MC if mc_first) into D.frc, no update
   // energy mode (one energy_sumup per replica, see SysIO): DH coefficient
   // and Debye length per replica (null: the replica's own), output
@@ -544,6 +546,29 @@ __device__ __forceinline__ void batched_rebuild(const Params &P, const Ptrs &D, 
   __syncthreads();
 }
 
+// Dynamic LDS of one batched launch: positions, per-lane partial forces,
+// then (nblds > 0) per-bead list offsets and counts and the list entries.
+__host__ __device__ inline size_t batched_lds_bytes(int xlds, int fplds, int nblds, int N) {
+  return (size_t)(xlds + fplds) * 16 * 2 + (nblds > 0 ? (size_t)(2 * N + nblds) * sizeof(int) : 0);
+}
+
+// pair_nb with the replica's lists staged in LDS: bead a's WCA entries at
+// s_nb[off ..), its Debye-Hueckel entries right after them (cnt = wca
+// count << 16 | DH count).  Same pairs in the same order as pair_nb.
+template <typename T>
+__device__ __forceinline__ V3<double> pair_nb_lds(const Params &P, const Ptrs &D, int r, int a, int l, int LPB,
+                                                  const double4 *x, const int *s_nb, int off, int cnt) {
+  const double *box = D.box + 3 * r;
+  const double4 pa = x[a];
+  const int nw = cnt >> 16, ne = P.ele_on ? (cnt & 0xffff) : 0;
+  const T coef = (T)D.ele_coef[r], rc2 = (T)(D.ele_cut[r] * D.ele_cut[r]), rl = (T)(1.0 / D.lambdaD[r]);
+  V3<double> F{0.0, 0.0, 0.0};
+  for (int k = l; k < nw; k += LPB) F = F + cvt<double>(wca_force<T>(P, cvt<T>(pvec(P, box, pa, x[s_nb[off + k]]))));
+  const int *e = s_nb + off + nw;
+  for (int k = l; k < ne; k += LPB) F = F + cvt<double>(dh_force<T>(coef, rc2, rl, cvt<T>(pvec(P, box, pa, x[e[k]]))));
+  return F;
+}
+
 // Whole job_md steps of system r (src/job_md.F90:374-460) for J.todo steps.
 template <typename T, int B>
 __device__ __forceinline__ void batched_body(const Params &P, const Ptrs &D, const SysIO &J, const int r) {
@@ -576,6 +601,54 @@ __device__ __forceinline__ void batched_body(const Params &P, const Ptrs &D, con
       for (int a = tid; a < N; a += B) s_x[a] = D.pos[base + a];
     __syncthreads();
   };
+  // the replica's WCA and Debye-Hueckel lists in LDS after the partial
+  // forces, when they fit (read by every step until the next rebuild):
+  // bead a's entries from s_noff[a], counts packed in s_ncnt[a]
+  __shared__ int s_nbok;
+  int *s_ncnt = reinterpret_cast<int *>(s_x + J.xlds + J.fplds);
+  int *s_noff = s_ncnt + N, *s_nb = s_noff + N;
+  auto stage_lists = [&]() {
+    int running = 0;
+    if (xl && J.nblds > 0) {
+      for (int a0 = 0; a0 < N; a0 += B) {
+        const int a = a0 + tid;
+        int cw = 0, ce = 0;
+        if (a < N) {
+          cw = min(D.n_wca[base + a], D.maxw);
+          ce = P.ele_on ? min(D.n_ele[base + a], D.maxnb) : 0;
+        }
+        const int v = cw + ce;
+        s_scan[tid] = v;
+        __syncthreads();
+        for (int off = 1; off < B; off <<= 1) {
+          const int t = tid >= off ? s_scan[tid - off] : 0;
+          __syncthreads();
+          s_scan[tid] += t;
+          __syncthreads();
+        }
+        if (a < N) {
+          s_noff[a] = running + s_scan[tid] - v;
+          s_ncnt[a] = (cw << 16) | ce;
+        }
+        running += s_scan[B - 1];
+        __syncthreads();
+      }
+    }
+    const bool ok = xl && J.nblds > 0 && running <= J.nblds;
+    if (ok) {
+      // slot k of every bead in turn: coalesced reads of the k-major lists
+      const int *nbw = D.nb_wca + (size_t)r * D.maxw * N, *nbe = D.nb_ele + (size_t)r * D.maxnb * N;
+      const int kmax = max(D.maxw, P.ele_on ? D.maxnb : 0);
+      for (int i = tid; i < kmax * N; i += B) {
+        const int k = i / N, a = i - k * N;
+        const int c = s_ncnt[a], cw = c >> 16, ce = c & 0xffff;
+        if (k < cw) s_nb[s_noff[a] + k] = nbw[(size_t)k * N + a];
+        if (k < ce) s_nb[s_noff[a] + cw + k] = nbe[(size_t)k * N + a];
+      }
+    }
+    if (tid == 0) s_nbok = ok ? 1 : 0;
+    __syncthreads();
+  };
   if (tid == 0) s_flag = 0;
   // per-step limit-MC state starts reset whatever ran before this launch
   for (int a = tid; a < N; a += B) {
@@ -595,6 +668,7 @@ __device__ __forceinline__ void batched_body(const Params &P, const Ptrs &D, con
     nreb++;
   }
   stage_x();
+  stage_lists();
   int64_t completed = 0;
   int overflow = s_flag;
   const int64_t nsteps = J.todo, step0 = J.stepno;
@@ -716,7 +790,8 @@ __device__ __forceinline__ void batched_body(const Params &P, const Ptrs &D, con
         const int a = (k - nmc) * bpw + L / lpb, lw = L % lpb;
         V3<double> F{0.0, 0.0, 0.0};
         if (a < N) {
-          const V3<double> Fp = pair_nb<T>(P, D, r, a, lw, lpb, xr, V3<double>{0.0, 0.0, 0.0});
+          const V3<double> Fp = s_nbok ? pair_nb_lds<T>(P, D, r, a, lw, lpb, s_x, s_nb, s_noff[a], s_ncnt[a])
+                                       : pair_nb<T>(P, D, r, a, lw, lpb, xr, V3<double>{0.0, 0.0, 0.0});
           s_fp[(size_t)a * lpb + lw] = double4{Fp.x, Fp.y, Fp.z, 0.0};
           F = bonded_part(P, D, r, a, lw, lpb, xr);
         }
@@ -842,6 +917,8 @@ __device__ __forceinline__ void batched_body(const Params &P, const Ptrs &D, con
         if (ovl) {
           const double4 fp = s_fp[(size_t)a * lpb + l];
           F = pair_bp(P, D, r, a, l, lpb, V3<double>{fp.x, fp.y, fp.z});
+        } else if (s_nbok) {
+          F = pair_bp(P, D, r, a, l, lpb, pair_nb_lds<T>(P, D, r, a, l, lpb, s_x, s_nb, s_noff[a], s_ncnt[a]));
         } else {
           F = pair_part<T>(P, D, r, a, l, lpb, xr);
         }
@@ -895,6 +972,7 @@ __device__ __forceinline__ void batched_body(const Params &P, const Ptrs &D, con
         mc_next = 1;
         nreb++;
         stage_x();
+        stage_lists();
       } else if (tid == 0) {
         *J.rebuild = 1;   // rebuild at the start of the next launch
       }
@@ -926,6 +1004,7 @@ __global__ void __launch_bounds__(B) k_batched(Params P, Ptrs D, BatchArgs A) {
   J.pos_b = A.pos_b;
   J.xlds = A.xlds;
   J.fplds = A.fplds;
+  J.nblds = A.nblds;
   J.force_only = A.force_only;
   J.done = A.done + r;
   J.nbp = A.nbp + r;
@@ -1008,6 +1087,7 @@ __global__ void __launch_bounds__(B) k_jobs(Params P, Ptrs D0, const JobView *__
   J.pos_b = V.pos_b;
   J.xlds = xlds;
   J.fplds = fplds;
+  J.nblds = 0;
   J.force_only = 0;
   J.energy_mode = 0;
   J.e_coef = 0.0;

@@ -230,6 +230,21 @@ const void *batched_kernel(sis_gpu_ctx *c, int B) {
   return f ? (const void *)k_batched<float, 256> : (const void *)k_batched<double, 256>;
 }
 
+// Dynamic LDS a k_batched launch may take beyond its static arrays: the
+// device's per-workgroup maximum (160 KB on gfx950) less the kernel's
+// static LDS.  The kernel is told it may use that much.
+size_t batched_lds_cap(sis_gpu_ctx *c, int B) {
+  const void *fn = batched_kernel(c, B);
+  int dev = 0, maxb = 0;
+  HIPCHK(hipGetDevice(&dev));
+  HIPCHK(hipDeviceGetAttribute(&maxb, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+  hipFuncAttributes fa{};
+  HIPCHK(hipFuncGetAttributes(&fa, fn));
+  const size_t cap = (size_t)maxb > fa.sharedSizeBytes ? (size_t)maxb - fa.sharedSizeBytes : 0;
+  (void)hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)cap);
+  return cap;
+}
+
 // arguments of one k_batched launch over the batched layout
 BatchArgs batched_args(sis_gpu_ctx *c, int lpb) {
   const int N = c->N;
@@ -250,6 +265,15 @@ BatchArgs batched_args(sis_gpu_ctx *c, int lpb) {
   A.lpb = lpb;
   A.xlds = N <= BATCH_X_LDS ? N : 0;
   A.fplds = (A.xlds && N * lpb <= BATCH_FP_LDS) ? N * lpb : 0;
+  // the WCA and Debye-Hueckel lists (at most their slot capacity) when
+  // they fit beside the positions and partial forces
+  A.nblds = 0;
+  if (A.xlds) {
+    const size_t cap = batched_lds_cap(c, batched_block(c));
+    const size_t used = batched_lds_bytes(A.xlds, A.fplds, 0, N) + (size_t)2 * N * sizeof(int);
+    const size_t want = (size_t)N * (c->D.maxw + (c->P.ele_on ? c->D.maxnb : 0));
+    if (cap > used) A.nblds = (int)std::min(want, (cap - used) / sizeof(int));
+  }
   A.force_only = 0;
   return A;
 }
@@ -288,8 +312,8 @@ void batched_force(sis_gpu_ctx *c, int irep0, int bp_mc) {
   Params P = c->P;
   Ptrs D = batched_ptrs(c);
   void *args[] = {&P, &D, &A};
-  HIPCHK(hipLaunchKernel(batched_kernel(c, B), dim3(R), dim3(B), args, (size_t)(A.xlds + A.fplds) * sizeof(double4),
-                         c->stream));
+  HIPCHK(hipLaunchKernel(batched_kernel(c, B), dim3(R), dim3(B), args,
+                         batched_lds_bytes(A.xlds, A.fplds, A.nblds, c->N), c->stream));
   to_packed(c);
 }
 
@@ -321,7 +345,7 @@ void batched_segment(sis_gpu_ctx *c, int64_t seg, std::vector<int> todo, std::ve
     Ptrs D = batched_ptrs(c);
     void *args[] = {&P, &D, &A};
     c->tmark();
-    HIPCHK(hipLaunchKernel(kfn, dim3(R), dim3(B), args, (size_t)(A.xlds + A.fplds) * sizeof(double4), c->stream));
+    HIPCHK(hipLaunchKernel(kfn, dim3(R), dim3(B), args, batched_lds_bytes(A.xlds, A.fplds, A.nblds, N), c->stream));
     c->tmark();
     std::vector<int> d(R), nr(R), f(8);
     HIPCHK(hipMemcpyAsync(d.data(), c->bb_done_d.p, R * sizeof(int), hipMemcpyDeviceToHost, c->stream));
@@ -471,7 +495,7 @@ int64_t batched_device_run(sis_gpu_ctx *c, int64_t nseg) {
   Aw.write_back = 0; Aw.energy_mode = 1; Aw.energy_slot = 0; Aw.energy = c->dx_energy_d.p;
   Ptrs Db = batched_ptrs(c), Dopp = Db, Dp = c->ptrs();
   Dopp.tK = c->dx_oppT_d.p;   // the energies at the partner's temperature
-  const size_t lds = (size_t)(As.xlds + As.fplds) * sizeof(double4);
+  const size_t lds = batched_lds_bytes(As.xlds, As.fplds, As.nblds, N);
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
@@ -560,9 +584,10 @@ void apply_margin(sis_gpu_ctx *c, double m) {
 // Autotuner of the launch shape and the rebuild skin: from the current
 // state, times nsteps steps of the context's form at every candidate
 // (batched: threads per replica 128 / 256 / 512; persistent: 1, 2 and as
-// many workgroups per compute unit as fit; per-phase graph steps: their
-// one launch shape, shape 0) times every skin, restoring the state before
-// each trial and at the end, and keeps the fastest.
+// many workgroups per compute unit as fit; per-phase steps in batches:
+// 0 replayed from a captured graph, 1 issued directly) times every skin,
+// restoring the state before each trial and at the end, and keeps the
+// fastest.
 // The skin changes when lists are rebuilt, not the forces (every term
 // applies its own cutoff), so the choice is a matter of speed only.
 void autotune(sis_gpu_ctx *c, const double *skins, int nskin, int64_t nsteps, double *report) {
@@ -600,7 +625,7 @@ void autotune(sis_gpu_ctx *c, const double *skins, int nskin, int64_t nsteps, do
   if (batched) {
     shapes = {128, 256, 512};
   } else if (phases) {
-    shapes = {0};
+    shapes = {0, 1};   // batches of steps replayed from a graph, or issued directly
   } else {
     int dev = 0, ncu = 0;
     HIPCHK(hipGetDevice(&dev));
@@ -614,7 +639,7 @@ void autotune(sis_gpu_ctx *c, const double *skins, int nskin, int64_t nsteps, do
   for (int sh : shapes) {
     for (int q = 0; q < nskin; q++) {
       if (!(skins[q] > 0.0)) throw ApiError{SIS_ERR_ARG, "skin must be positive"};
-      if (batched) c->opt.block = sh; else if (pers) c->opt.grid = sh;
+      if (batched) c->opt.block = sh; else if (pers) c->opt.grid = sh; else c->pg_eager = sh == 1;
       apply_margin(c, skins[q]);
       restore();
       HIPCHK(hipStreamSynchronize(c->stream));
@@ -628,7 +653,7 @@ void autotune(sis_gpu_ctx *c, const double *skins, int nskin, int64_t nsteps, do
       HIPCHK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
       const double per = ms / (double)nsteps;
       if (report) {
-        report[2 + 3 * k] = batched ? sh : (pers ? c->block : 0);
+        report[2 + 3 * k] = batched ? sh : (pers ? c->block : sh);
         report[3 + 3 * k] = skins[q];
         report[4 + 3 * k] = per;
       }
@@ -636,7 +661,7 @@ void autotune(sis_gpu_ctx *c, const double *skins, int nskin, int64_t nsteps, do
       if (per < best) { best = per; best_shape = sh; best_skin = skins[q]; }
     }
   }
-  if (batched) c->opt.block = best_shape; else if (pers) c->opt.grid = best_shape;
+  if (batched) c->opt.block = best_shape; else if (pers) c->opt.grid = best_shape; else c->pg_eager = best_shape == 1;
   (void)block0; (void)grid0;
   apply_margin(c, best_skin);
   restore();

@@ -298,10 +298,11 @@ def test_autotuner_sweeps_and_restores():
 
 
 def test_autotuner_per_phase_graph_form():
-    """sis_gpu_autotune on the per-phase form with graph-replayed steps (a
-    periodic condensate on the cell list): one trial per skin, shape 0, the
-    state restored; the run then matches a fresh context with the chosen
-    skin."""
+    """sis_gpu_autotune on the per-phase form with device-driven batches of
+    steps (a periodic condensate on the cell list): batches replayed from a
+    graph (shape 0) or issued directly (shape 1) at every skin, the state
+    restored; the run then matches a fresh context with the chosen skin,
+    whichever way its steps are issued."""
     s = configs.small_pbc(6, 40, 150.0, seed=8)
     print("[case] autotuner per-phase", flush=True)
     e = Engine(s, mode="phases", precision=1, seed=SEED)
@@ -309,8 +310,8 @@ def test_autotuner_per_phase_graph_form():
     e.neighbor_list(0)
     x0, v0, a0 = e.get_state()
     rep = e.autotune((2.0, 4.0), nsteps=64)
-    assert [t["skin"] for t in rep["trials"]] == [2.0, 4.0] and all(t["shape"] == 0 for t in rep["trials"])
-    assert rep["shape"] == 0 and rep["skin"] in (2.0, 4.0)
+    assert [(t["shape"], t["skin"]) for t in rep["trials"]] == [(0, 2.0), (0, 4.0), (1, 2.0), (1, 4.0)]
+    assert rep["shape"] in (0, 1) and rep["skin"] in (2.0, 4.0)
     x1, v1, a1 = e.get_state()
     assert np.array_equal(x0, x1) and np.array_equal(v0, v1) and np.array_equal(a0, a1)
     e.step(64)
