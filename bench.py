@@ -119,7 +119,7 @@ def cpu_baseline_legs(sysd, steps):
     allc, sock = host_cores()
     legs, t_setup = [], None
     for th in sorted({1, sock, allc}):
-        st = max(100, steps // 8) if th == 1 else steps
+        st = max(1, steps // 8) if th == 1 else steps
         r = cpu_baseline(sysd, st, th, t_setup)
         if r is None:
             return None
@@ -190,7 +190,8 @@ def fp32_force_error(sysd, mode="phases"):
 
 def precision_report(sysd, steps, nsamples=5, seed=2025):
     """The precision config's report: an FP32-pair run and an FP64 run of
-    the same system from the same seed (same noise), persistent form, for
+    the same system from the same seed (same noise), per-phase form (the
+    form C4 runs), for
     `steps` steps; at nsamples evenly spaced times in the second half,
     per-chain Rg and paired fraction (2,000 values each) and E_total.
     Returns the KS tests of the pooled per-chain distributions and the
@@ -198,7 +199,7 @@ def precision_report(sysd, steps, nsamples=5, seed=2025):
     from scipy.stats import ks_2samp
     from sis_amd import observables
     from sis_amd.engine import Engine
-    engs = [Engine(sysd, precision=p, mode="persistent", seed=seed) for p in (0, 1)]
+    engs = [Engine(sysd, precision=p, mode="phases", seed=seed) for p in (0, 1)]
     for e in engs:
         e.init_velocities()
         e.neighbor_list(0)
@@ -215,7 +216,7 @@ def precision_report(sysd, steps, nsamples=5, seed=2025):
                   f"({time.perf_counter() - t0:.0f}s)", flush=True)
     for e in engs:
         e.close()
-    out = dict(steps=half + every * nsamples, samples=nsamples, seed=seed, form="persistent")
+    out = dict(steps=half + every * nsamples, samples=nsamples, seed=seed, form="phases")
     for key in ("rg", "bp_fraction"):
         a = np.concatenate([q[key] for q in samples[0]])
         b = np.concatenate([q[key] for q in samples[1]])
@@ -408,7 +409,7 @@ def main():
     eng.init_velocities()
     eng.neighbor_list(0)
     tune = None
-    if not args.no_tune and args.workload in ("c3", "c4"):
+    if not args.no_tune and args.workload in ("c2", "c3", "c4"):
         t0 = time.perf_counter()
         tune = eng.autotune((3.0, 5.0, 8.0), nsteps=200 if args.workload == "c3" else 100)
         print(f"[bench] autotune {time.perf_counter() - t0:.1f}s: shape {tune['shape']} skin {tune['skin']} A; "
@@ -488,6 +489,23 @@ def main():
                            "bytes_per_bead_step": round(kbytes, 1),
                            "bead_steps_per_launch": int(nmp * nrep * steps_per_launch)}
     if not args.no_extras:
+        if nrep > 1 and mode_used == "batched" and prec == 1:
+            # the same batched form with FP64 pair arithmetic: the effect of
+            # FP32 alone
+            s64 = dict(sysd, params=dict(sysd["params"], nl_margin=tune["skin"])) if tune else sysd
+            bf = Engine(s64, precision=0, parity=False, mode="batched", seed=2024,
+                        exchange_every=int(sysd.get("exchange_every", 0)), block=tune["shape"] if tune else 0)
+            bf.init_velocities()
+            bf.neighbor_list(0)
+            bf.step(1000)
+            n64 = min(steps, 5000)
+            t0 = time.perf_counter()
+            bf.step(n64)
+            w64 = time.perf_counter() - t0
+            bf.close()
+            out["batched_fp64"] = dict(value=round(nmp * nrep * n64 / w64, 1), unit="bead-steps/s", steps=n64,
+                                       ms_per_step=round(1e3 * w64 / n64, 5), form="batched", dtype="f64")
+            print(f"[bench] batched fp64 {out['batched_fp64']['value']:.3e}", flush=True)
         if nrep > 1:
             s1 = single_replica(sysd)
             pl = plain_fp64_rate(s1, steps=min(steps, 2000), warmup=200)
@@ -506,7 +524,12 @@ def main():
         if drawn:
             out["serving"] = serving_run(drawn, args.serving_scale, args.serving_rate, args.serving_max_chains)
             print(f"[bench] serving {out['serving']}", flush=True)
-    if not args.no_cpu:
+    if not args.no_cpu and args.workload == "c4" and args.cpu_steps is None:
+        # the reference's set-up alone (an all-pairs neighbour list over
+        # 94,000 beads) exceeds a bounded sample; run it with --cpu-steps
+        out["cpu_baseline_note"] = ("not run by default for C4: the reference's all-pairs neighbour list over "
+                                    "94,000 beads alone exceeds a bounded sample (use --cpu-steps)")
+    elif not args.no_cpu:
         # the reference loops over replicas one after another, so its rate per
         # bead does not depend on the replica count: the 256-replica workload
         # is sampled with 32 of its replicas (same chain, same 290-380 K span;
