@@ -362,9 +362,11 @@ __device__ __forceinline__ bool mc_component_wave(const Params &P, const Ptrs &D
 // The same sweep for a component of at most 64 pairs (the usual size),
 // with one pair per lane and the bookkeeping in scalar registers: the live
 // pairs are a 64-bit mask, the nucleotides two slots per lane, the sequence
-// of up to 8 pairs packed in a 64-bit scalar.  A single wavefront runs this
-// serially, so its cost is its instruction count; this form needs about a
-// third of the general one's.  Draws and decisions are those of
+// of up to 8 pairs held as a 4-bit permutation of their ranks.  A single
+// wavefront runs this serially, so its cost is the latency of its chain of
+// lane reads and ballots (each about 25-30 cycles on gfx950); the common
+// two-pair case resolves its swap sweep from one ballot of the block of
+// draws taken at each refill.  Draws and decisions are those of
 // mc_component_wave.  Returns false (having written nothing) if the
 // component is larger, a nucleotide has more than 8 formed pairs, or the
 // state is inconsistent; the caller then runs the general form from the
@@ -444,6 +446,8 @@ __device__ __forceinline__ bool mc_component_wave64(const Params &P, const Ptrs 
   unsigned long long live = __ballot(L < n);
   uint32_t draw = 0, w0 = 0;
   double wu = philox_u(seed, rk, step, root, (uint32_t)L), wl = log(wu);
+  // draws above one half: the swap target of a two-pair sequence is 1 there
+  unsigned long long g2 = __ballot(wu > 0.5);
   for (int iter = 0; iter < 4 * 64 + 4; iter++) {
     const unsigned long long a0 = __ballot(cnt0 > 0), a1 = __ballot(cnt1 > 0);
     const int p0 = __popcll(a0), nact = p0 + __popcll(a1);
@@ -452,40 +456,53 @@ __device__ __forceinline__ bool mc_component_wave64(const Params &P, const Ptrs 
       w0 = draw;
       wu = philox_u(seed, rk, step, root, w0 + (uint32_t)L);
       wl = log(wu);
+      g2 = __ballot(wu > 0.5);
     }
-    const int pick = (int)ceil(bcast(wu, (int)(draw - w0)) * nact);   // 1..nact
+    const int pick = bcast((int)ceil(wu * nact), (int)(draw - w0));   // 1..nact
     const bool upper = pick > p0;
     const unsigned long long am = upper ? a1 : a0;
     const int want = upper ? pick - p0 - 1 : pick - 1;
-    const int ln = __ffsll((long long)__ballot(((am >> L) & 1ull) && below(am) == want)) - 1;
-    const int ntd = upper ? bcast(nid1, ln) : bcast(nid0, ln);
-    unsigned long long sm = __ballot(ia == ntd || ib == ntd) & live;
+    const int ln = __builtin_ctzll(__ballot(below(am) == want) & am);
+    const int ntd = bcast(upper ? nid1 : nid0, ln);
+    const unsigned long long sm = (__ballot(ia == ntd) | __ballot(ib == ntd)) & live;
     const int ns = __popcll(sm);
     if (ns == 0 || ns > 8) return false;
-    uint64_t sp = 0;
-    for (int m = 0; m < ns; m++) {
-      sp |= (uint64_t)__builtin_ctzll(sm) << (8 * m);
-      sm &= sm - 1;
-    }
     if (draw + 2u * (uint32_t)ns > w0 + 64u) {
       w0 = draw;
       wu = philox_u(seed, rk, step, root, w0 + (uint32_t)L);
       wl = log(wu);
+      g2 = __ballot(wu > 0.5);
     }
     const int o = (int)(draw - w0);
-    const int swv = (int)ceil(wu * ns) - 1;
-    for (int i = 0; i < ns; i++) {
-      const int sw = bcast(swv, o + 1 + i);
-      const uint64_t ti = (sp >> (8 * i)) & 0xffull, ts = (sp >> (8 * sw)) & 0xffull;
-      sp = (sp & ~(0xffull << (8 * i))) | (ts << (8 * i));
-      sp = (sp & ~(0xffull << (8 * sw))) | (ti << (8 * sw));
-    }
-    int del = (int)(sp & 0xffull);
-    double edel = bcast(en, del);
-    for (int i = 1; i < ns; i++) {
-      const int si = (int)((sp >> (8 * i)) & 0xffull);
-      const double ej = bcast(en, si);
-      if (bcast(wl, o + ns + i) < (ej - edel) * beta) { del = si; edel = ej; }
+    int del;
+    if (ns == 2) {
+      // two pairs: the swap sweep leaves them reversed exactly when both
+      // swap draws fall on the same side of one half
+      const int la = __builtin_ctzll(sm), lb = 63 - __builtin_clzll(sm);
+      const unsigned t = (unsigned)(g2 >> (o + 1)) & 3u;
+      const bool flip = t == 0u || t == 3u;
+      const int s0 = flip ? lb : la, s1 = flip ? la : lb;
+      const double e0 = bcast(en, s0), w = bcast(wl, o + 3);
+      const bool acc = (__ballot(w < (en - e0) * beta) >> s1) & 1ull;
+      del = acc ? s1 : s0;
+    } else {
+      const int swv = (int)ceil(wu * ns) - 1;
+      // ntd's pairs in list order are the set bits of sm in lane order (rank
+      // 0..ns-1); perm holds the rank at each sequence slot, 4 bits a slot
+      uint32_t perm = 0x76543210u;
+      for (int i = 0; i < ns; i++) {
+        const int sw = bcast(swv, o + 1 + i);
+        const uint32_t x = ((perm >> (4 * i)) ^ (perm >> (4 * sw))) & 15u;
+        perm ^= (x << (4 * i)) | (x << (4 * sw));
+      }
+      const int rkl = below(sm);
+      del = __builtin_ctzll(__ballot(rkl == (int)(perm & 15u)) & sm);
+      double edel = bcast(en, del);
+      for (int i = 1; i < ns; i++) {
+        const int si = __builtin_ctzll(__ballot(rkl == (int)((perm >> (4 * i)) & 15u)) & sm);
+        const double ej = bcast(en, si);
+        if (bcast(wl, o + ns + i) < (ej - edel) * beta) { del = si; edel = ej; }
+      }
     }
     draw += 2u * (uint32_t)ns;
     live &= ~(1ull << del);
