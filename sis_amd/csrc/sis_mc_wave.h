@@ -206,8 +206,14 @@ __device__ __forceinline__ bool mc_component_wave(const Params &P, const Ptrs &D
   // on when an iteration could run past it.
   uint32_t draw = 0, w0 = 0;
   double wu = philox_u(seed, rk, step, root, (uint32_t)L), wl = log(wu);
+  // draws above one half: the swap target of a two-pair sequence is 1 there
+  unsigned long long g2 = __ballot(wu > 0.5);
   const int nchunk_nodes = (nn + 63) / 64;
   const int nchunk_items = (n + 63) / 64;
+  // live pairs, one 64-bit mask per chunk of pair slots
+  unsigned long long live[IT];
+#pragma unroll
+  for (int k = 0; k < IT; k++) live[k] = __ballot(alive[k]);
   for (int iter = 0; iter < 4 * NP + 4; iter++) {
     // active nucleotides (count > 0) in ascending id order
     int nact = 0;
@@ -222,30 +228,35 @@ __device__ __forceinline__ bool mc_component_wave(const Params &P, const Ptrs &D
       w0 = draw;
       wu = philox_u(seed, rk, step, root, w0 + (uint32_t)L);
       wl = log(wu);
+      g2 = __ballot(wu > 0.5);
     }
-    const int pick = (int)ceil(bcast(wu, (int)(draw - w0)) * nact);   // 1..nact
-    // the pick-th active nucleotide: its chunk, then the lane whose prefix
-    // count of active lanes is pick - 1
-    int ntd = -1;
+    const int pick = bcast((int)ceil(wu * nact), (int)(draw - w0));   // 1..nact
+    // the pick-th active nucleotide: its chunk (scalar prefix counts),
+    // then the lane whose prefix count of active lanes is pick - 1
+    int want = pick - 1;
+    unsigned long long amq = am[0];
+    int nidq = nid[0];
     {
       int acc = 0;
+      bool found = false;
 #pragma unroll
       for (int q = 0; q < ND; q++) {
         const int pc = __popcll(am[q]);
-        if (ntd < 0 && acc + pc >= pick) {
-          const int want = pick - acc - 1;
-          const unsigned long long hit = __ballot(((am[q] >> L) & 1ull) && below(am[q]) == want);
-          ntd = bcast(nid[q], __ffsll((long long)hit) - 1);
-        }
+        const bool here = !found && acc + pc >= pick;
+        want = here ? pick - acc - 1 : want;
+        amq = here ? am[q] : amq;
+        nidq = here ? nid[q] : nidq;
+        found = found || here;
         acc += pc;
       }
     }
+    const int ntd = bcast(nidq, __builtin_ctzll(__ballot(below(amq) == want) & amq));
     // formed pairs of ntd in list order
     unsigned long long sm[IT];
     int ns = 0;
 #pragma unroll
     for (int k = 0; k < IT; k++) {
-      sm[k] = (k < nchunk_items) ? __ballot(alive[k] && (ia[k] == ntd || ib[k] == ntd)) : 0ull;
+      sm[k] = (k < nchunk_items) ? ((__ballot(ia[k] == ntd) | __ballot(ib[k] == ntd)) & live[k]) : 0ull;
       ns += __popcll(sm[k]);
     }
     if (ns == 0) return false;
@@ -259,10 +270,36 @@ __device__ __forceinline__ bool mc_component_wave(const Params &P, const Ptrs &D
       return bcast(x, slot & 63);
     };
     int del;
-    double edel;
-    if (ns <= 8) {
-      // short sequence (the usual case): kept in scalar registers, entry m
-      // in bits 8m..8m+7 (slots < 256); the window holds the 2 ns draws
+    if (ns == 2) {
+      // two pairs: the swap sweep leaves them reversed exactly when both
+      // swap draws fall on the same side of one half
+      if (draw + 4u > w0 + 64u) {
+        w0 = draw;
+        wu = philox_u(seed, rk, step, root, w0 + (uint32_t)L);
+        wl = log(wu);
+        g2 = __ballot(wu > 0.5);
+      }
+      const int o = (int)(draw - w0);
+      int la = -1, lb = -1;
+#pragma unroll
+      for (int k = IT - 1; k >= 0; k--)
+        if (sm[k]) la = 64 * k + __builtin_ctzll(sm[k]);
+#pragma unroll
+      for (int k = 0; k < IT; k++)
+        if (sm[k]) lb = 64 * k + 63 - __builtin_clzll(sm[k]);
+      const unsigned t = (unsigned)(g2 >> (o + 1)) & 3u;
+      const bool flip = t == 0u || t == 3u;
+      const int s0 = flip ? lb : la, s1 = flip ? la : lb;
+      const double e0 = ene_of(s0), w = bcast(wl, o + 3);
+      const int k1 = s1 >> 6;
+      double e1v = en[0];
+#pragma unroll
+      for (int kk = 1; kk < IT; kk++) e1v = (kk == k1) ? en[kk] : e1v;
+      const bool accd = (__ballot(w < (e1v - e0) * beta) >> (s1 & 63)) & 1ull;
+      del = accd ? s1 : s0;
+    } else if (ns <= 8) {
+      // short sequence: kept in scalar registers, entry m in bits
+      // 8m..8m+7 (slots < 256); the window holds the 2 ns draws
       uint64_t sp = 0;
       int m = 0;
 #pragma unroll
@@ -275,6 +312,7 @@ __device__ __forceinline__ bool mc_component_wave(const Params &P, const Ptrs &D
         w0 = draw;
         wu = philox_u(seed, rk, step, root, w0 + (uint32_t)L);
         wl = log(wu);
+        g2 = __ballot(wu > 0.5);
       }
       const int o = (int)(draw - w0);
       // swap target of a shuffle draw held by this lane
@@ -288,7 +326,7 @@ __device__ __forceinline__ bool mc_component_wave(const Params &P, const Ptrs &D
       }
       // Metropolis sweep on the energies of the sequence's pairs
       del = (int)(sp & 0xffull);
-      edel = ene_of(del);
+      double edel = ene_of(del);
       for (int i = 1; i < ns; i++) {
         const int si = (int)((sp >> (8 * i)) & 0xffull);
         const double ej = ene_of(si);
@@ -311,6 +349,7 @@ __device__ __forceinline__ bool mc_component_wave(const Params &P, const Ptrs &D
         w0 = draw;
         wu = philox_u(seed, rk, step, root, w0 + (uint32_t)L);
         wl = log(wu);
+        g2 = __ballot(wu > 0.5);
       }
       const int o = (int)(draw - w0);
       // u and log u of the draw `k` places after the pick draw
@@ -325,7 +364,7 @@ __device__ __forceinline__ bool mc_component_wave(const Params &P, const Ptrs &D
         seqv = setlane(seqv, ti, sw);
       }
       del = bcast(seqv, 0);
-      edel = ene_of(del);
+      double edel = ene_of(del);
       for (int i = 1; i < ns; i++) {
         const int si = bcast(seqv, i);
         const double ej = ene_of(si);
@@ -344,11 +383,12 @@ __device__ __forceinline__ bool mc_component_wave(const Params &P, const Ptrs &D
     const int da = bcast(xa, dl), db = bcast(xb, dl);
 #pragma unroll
     for (int k = 0; k < IT; k++)
-      if (k == dk && L == dl) alive[k] = false;
+      if (k == dk) live[k] &= ~(1ull << dl);
 #pragma unroll
-    for (int q = 0; q < ND; q++)
-      if (nid[q] == da || nid[q] == db) ncnt[q] -= 1;
+    for (int q = 0; q < ND; q++) ncnt[q] -= (nid[q] == da) + (nid[q] == db);
   }
+#pragma unroll
+  for (int k = 0; k < IT; k++) alive[k] = (live[k] >> L) & 1ull;
   // write back the pairs deleted here
 #pragma unroll
   for (int k = 0; k < IT; k++) {
