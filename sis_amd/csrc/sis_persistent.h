@@ -707,7 +707,8 @@ __device__ __forceinline__ void batched_body(const Params &P, const Ptrs &D, con
       }
     if (tid == 0) { s_nexc = 0; s_big = 0; s_nw = 0; s_next = 0; s_huge = 0; }
     __syncthreads();
-    int nw = 0;
+    int nw = 0, nexc = 0;
+    bool listed = true;
     if (mc) {
       // ---- B: formed pairs bucketed by component root; roots with
       // excess listed once
@@ -726,13 +727,13 @@ __device__ __forceinline__ void batched_body(const Params &P, const Ptrs &D, con
         }
       }
       __syncthreads();
-      // ---- C: one wavefront per component with excess (components are
-      // disjoint and keyed by their root, so the order does not matter)
-      const int wave = tid / 64, nwaves = B / 64;
-      const int nexc = s_nexc;
-      const bool listed = nexc <= EXC_LDS;
-      // small components one lane each; the others are classified: the
-      // wavefront ones listed with their size, the large ones flagged
+      // ---- C: components with excess classified by size (components are
+      // disjoint and keyed by their root, so the order does not matter):
+      // the wavefront ones listed with their size, the large ones flagged;
+      // the small ones (one lane each) are work-list items below
+      const int wave = tid / 64;
+      nexc = s_nexc;
+      listed = nexc <= EXC_LDS;
       for (int it = tid; it < (listed ? nexc : N); it += B) {
         const int root = listed ? s_exc[it] : it;
         const size_t gr = base + root;
@@ -743,11 +744,7 @@ __device__ __forceinline__ void batched_body(const Params &P, const Ptrs &D, con
         if (n > MCL_PAIRS) {
           const int q = atomicAdd(&s_nw, 1);
           if (q < MCW_LIST) s_wl[q] = (n << 21) | root;   // root < 2^21 (N <= 2048 here)
-          continue;
         }
-        int *sg = J.seg + gr * SEGCAP;
-        if (!mc_component_lane(D, N, r, J.rk, root, sg, n, beta, J.seed, step))
-          mc_component_serial(mc_arrays(D), P.N, r, J.rk, root, sg, n, beta, J.seed, step);
       }
       __syncthreads();
       nw = s_nw;
@@ -765,20 +762,22 @@ __device__ __forceinline__ void batched_body(const Params &P, const Ptrs &D, con
       }
     }
     // ---- C: one work list for the waves, handed out one item at a time:
-    // the wavefront-MC components (largest first), then, when the partial
-    // forces fit in LDS, the bonded and WCA/Debye-Hueckel forces of groups
-    // of beads (they do not depend on the MC).  The longest wave is then
-    // close to the largest component, with the force work in the shadow
-    // of the MC.
+    // the wavefront-MC components (largest first), then the small
+    // components, one lane each, 64 listed roots an item, then, when the
+    // partial forces fit in LDS, the bonded and WCA/Debye-Hueckel forces of
+    // groups of beads (they do not depend on the MC).  The longest wave is
+    // then close to the largest component, with the rest in its shadow.
     {
       const int wave = tid / 64, L = tid & 63;
       const int nmc = (mc && nw <= MCW_LIST) ? nw : 0;
+      const int nroot = listed ? nexc : N;
+      const int nlane = mc ? (nroot + 63) / 64 : 0;
       const int bpw = 64 / lpb, nchunk = ovl ? (N + bpw - 1) / bpw : 0;
       for (;;) {
         int k = 0;
         if (L == 0) k = atomicAdd(&s_next, 1);
         k = bcast(k, 0);
-        if (k >= nmc + nchunk) break;
+        if (k >= nmc + nlane + nchunk) break;
         if (k < nmc) {
           const int key = s_wl[k];
           const int root = key & 0x1fffff, n = key >> 21;
@@ -787,7 +786,23 @@ __device__ __forceinline__ void batched_body(const Params &P, const Ptrs &D, con
             if (L == 0) mc_component_serial(mc_arrays(D), P.N, r, J.rk, root, sg, n, beta, J.seed, step);
           continue;
         }
-        const int a = (k - nmc) * bpw + L / lpb, lw = L % lpb;
+        if (k < nmc + nlane) {
+          const int it = (k - nmc) * 64 + L;
+          if (it < nroot) {
+            const int root = listed ? s_exc[it] : it;
+            const size_t gr = base + root;
+            if (J.has_exc[gr]) {
+              const int n = J.seg_n[gr];
+              if (n <= MCL_PAIRS) {
+                int *sg = J.seg + gr * SEGCAP;
+                if (!mc_component_lane(D, N, r, J.rk, root, sg, n, beta, J.seed, step))
+                  mc_component_serial(mc_arrays(D), P.N, r, J.rk, root, sg, n, beta, J.seed, step);
+              }
+            }
+          }
+          continue;
+        }
+        const int a = (k - nmc - nlane) * bpw + L / lpb, lw = L % lpb;
         V3<double> F{0.0, 0.0, 0.0};
         if (a < N) {
           const V3<double> Fp = s_nbok ? pair_nb_lds<T>(P, D, r, a, lw, lpb, s_x, s_nb, s_noff[a], s_ncnt[a])
@@ -802,8 +817,6 @@ __device__ __forceinline__ void batched_body(const Params &P, const Ptrs &D, con
     __syncthreads();
     if (mc) {
       const int wave = tid / 64, nwaves = B / 64;
-      const int nexc = s_nexc;
-      const bool listed = nexc <= EXC_LDS;
       if (nw > MCW_LIST) {
         // more than the list holds: static round robin over the roots
         for (int it = wave; it < (listed ? nexc : N); it += nwaves) {
