@@ -65,141 +65,16 @@ __device__ __forceinline__ int below(unsigned long long m) {
 // Orders LDS writes of this wave before its later LDS reads of other lanes' words.
 __device__ __forceinline__ void wave_lds_fence() { __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront"); }
 
-// Resolve the component whose formed pairs are seg[0..n) (candidate ids,
-// any order).  Must be called by all 64 lanes of a wave with uniform
-// arguments; ws is mcw_lds(IT) ints of LDS private to the wave.  Returns
-// false if the component exceeds the capacity of 64*IT pairs.  The result
-// does not depend on IT (same draws, same decisions).  r indexes the per-replica
-// arrays; rk keys the random stream.
-// Layout: pair slot t (candidate-id order) lives in lane t % 64, chunk
-// t / 64; nucleotide slot u (ascending id) in lane u % 64, chunk u / 64;
-// the pair sequence of one iteration holds entry m in lane m.
-template <int IT = MCW_ITEMS, int ND = MCW_NODES>
-__device__ __forceinline__ bool mc_component_wave(const Params &P, const Ptrs &D, int r, int rk, int root,
-                                                  const int *seg, int n, double beta, uint64_t seed,
-                                                  int64_t step, int *ws) {
-  constexpr int NONE = 0x7fffffff;
-  constexpr int NP = 64 * IT, NE = 2 * NP;
-  static_assert(NP <= 256, "pair slots are packed in 8 bits");
+// The Monte Carlo loop of mc_component_wave over its pair slots (IT chunks
+// of 64) and distinct nucleotides (ND chunks of 64), then the write-back of
+// the deleted pairs.  Uniform arguments; all 64 lanes call it.
+template <int IT, int ND>
+__device__ __forceinline__ bool mcg_sweep(const Params &P, const Ptrs &D, int rk, int root, int n, int nn,
+                                          double beta, uint64_t seed, int64_t step, int *sseq, const int (&gb)[IT],
+                                          const int (&ia)[IT], const int (&ib)[IT], const double (&en)[IT],
+                                          bool (&alive)[IT], const int (&nid)[ND], int (&ncnt)[ND]) {
+  constexpr int NP = 64 * IT;
   const int L = lane_id();
-  const int N = P.N;
-  // uniform by contract; said so to the compiler, so that loop counters
-  // and lane indices stay scalar
-  n = __builtin_amdgcn_readfirstlane(n);
-  root = __builtin_amdgcn_readfirstlane(root);
-  if (n > NP) return false;
-  const int nc = 2 * n;
-  int *sraw = ws, *ssort = ws + NP, *send = ws + 2 * NP, *sflag = ws + 2 * NP + NE, *sseq = ws + 2 * NP + 2 * NE;
-  // ---- pairs in candidate-id order (= reference list order): rank by
-  // comparing against all ids (4 per LDS read), then scatter by rank
-  const int n4 = (n + 3) & ~3;
-  int raw[IT];
-#pragma unroll
-  for (int k = 0; k < IT; k++) {
-    const int t = L + 64 * k;
-    raw[k] = t < n ? seg[t] : NONE;
-    if (t < n4) sraw[t] = raw[k];
-  }
-  wave_lds_fence();
-  int rank[IT] = {};
-  for (int j = 0; j < n4; j += 4) {
-    const int4 v = *reinterpret_cast<const int4 *>(sraw + j);
-#pragma unroll
-    for (int k = 0; k < IT; k++)
-      rank[k] += (v.x < raw[k]) + (v.y < raw[k]) + (v.z < raw[k]) + (v.w < raw[k]);
-  }
-#pragma unroll
-  for (int k = 0; k < IT; k++)
-    if (L + 64 * k < n) ssort[rank[k]] = raw[k];
-  wave_lds_fence();
-  int gb[IT], ia[IT], ib[IT];
-  double en[IT];
-  bool alive[IT];
-  bool bad = false;
-#pragma unroll
-  for (int k = 0; k < IT; k++) {
-    const int t = L + 64 * k;
-    alive[k] = t < n;
-    gb[k] = NONE;
-    ia[k] = ib[k] = -1;
-    en[k] = 0.0;
-    if (alive[k]) {
-      gb[k] = ssort[t];
-      const int *mp = D.bp_mp + (size_t)gb[k] * 8;
-      ia[k] = mp[0] - 1;
-      ib[k] = mp[1] - 1;
-      en[k] = D.ene_bp[gb[k]];
-      bad = bad || ia[k] < 0 || ia[k] >= N || ib[k] < 0 || ib[k] >= N;
-    }
-  }
-  // a candidate naming a bead outside the system means corrupted MC state:
-  // report it (flags[7]) instead of indexing with it
-  if (__ballot(bad)) {
-    if (L == 0) D.flags[7] = 1;
-    return true;
-  }
-  // ---- distinct nucleotides in ascending id order, with their counts.
-  // Endpoint c = 2t (+1) is nucleotide a (b) of pair slot t.
-  const int nc4 = (nc + 3) & ~3;
-#pragma unroll
-  for (int k = 0; k < IT; k++) {
-    const int t = L + 64 * k;
-    if (t < n) {
-      send[2 * t] = ia[k];
-      send[2 * t + 1] = ib[k];
-    } else if (2 * t < nc4) {
-      send[2 * t] = NONE;
-      send[2 * t + 1] = NONE;
-    }
-  }
-  wave_lds_fence();
-  // first occurrence of each endpoint value
-  int ev[ND];
-  bool first[ND];
-#pragma unroll
-  for (int q = 0; q < ND; q++) {
-    const int c = L + 64 * q;
-    ev[q] = c < nc ? send[c] : NONE;
-    first[q] = c < nc;
-  }
-  for (int j = 0; j < nc4; j += 4) {
-    const int4 v = *reinterpret_cast<const int4 *>(send + j);
-#pragma unroll
-    for (int q = 0; q < ND; q++) {
-      const int c = L + 64 * q;
-      if ((j < c && v.x == ev[q]) || (j + 1 < c && v.y == ev[q]) || (j + 2 < c && v.z == ev[q]) ||
-          (j + 3 < c && v.w == ev[q]))
-        first[q] = false;
-    }
-  }
-#pragma unroll
-  for (int q = 0; q < ND; q++) {
-    const int c = L + 64 * q;
-    if (c < nc4) sflag[c] = first[q] ? ev[q] : NONE;   // first occurrences only
-  }
-  wave_lds_fence();
-  // rank among distinct values, then scatter
-  int nn = 0;
-#pragma unroll
-  for (int q = 0; q < ND; q++) nn += __popcll(__ballot(first[q]));
-  int urank[ND] = {};
-  for (int j = 0; j < nc4; j += 4) {
-    const int4 v = *reinterpret_cast<const int4 *>(sflag + j);
-#pragma unroll
-    for (int q = 0; q < ND; q++)
-      urank[q] += (v.x < ev[q]) + (v.y < ev[q]) + (v.z < ev[q]) + (v.w < ev[q]);
-  }
-#pragma unroll
-  for (int q = 0; q < ND; q++)
-    if (first[q]) send[urank[q]] = ev[q];   // endpoints no longer needed
-  wave_lds_fence();
-  int nid[ND], ncnt[ND];
-#pragma unroll
-  for (int q = 0; q < ND; q++) {
-    const int u = L + 64 * q;
-    nid[q] = u < nn ? send[u] : NONE;
-    ncnt[q] = u < nn ? D.nt_cnt[(size_t)r * N + nid[q]] : 0;
-  }
   // ---- the Monte Carlo loop.  Draws come from a window of 64 counters,
   // lane i holding draw w0 + i and its logarithm (for the Metropolis test,
   // log u < dE beta, the same test as u < exp(dE beta)); the window moves
@@ -396,6 +271,151 @@ __device__ __forceinline__ bool mc_component_wave(const Params &P, const Ptrs &D
     if (t < n && !alive[k]) D.bp_status[gb[k]] = 0;
   }
   return true;
+}
+
+// Resolve the component whose formed pairs are seg[0..n) (candidate ids,
+// any order).  Must be called by all 64 lanes of a wave with uniform
+// arguments; ws is mcw_lds(IT) ints of LDS private to the wave.  Returns
+// false if the component exceeds the capacity of 64*IT pairs.  The result
+// does not depend on IT (same draws, same decisions).  r indexes the per-replica
+// arrays; rk keys the random stream.
+// Layout: pair slot t (candidate-id order) lives in lane t % 64, chunk
+// t / 64; nucleotide slot u (ascending id) in lane u % 64, chunk u / 64;
+// the pair sequence of one iteration holds entry m in lane m.
+template <int IT = MCW_ITEMS, int ND = MCW_NODES>
+__device__ __forceinline__ bool mc_component_wave(const Params &P, const Ptrs &D, int r, int rk, int root,
+                                                  const int *seg, int n, double beta, uint64_t seed,
+                                                  int64_t step, int *ws) {
+  constexpr int NONE = 0x7fffffff;
+  constexpr int NP = 64 * IT, NE = 2 * NP;
+  static_assert(NP <= 256, "pair slots are packed in 8 bits");
+  const int L = lane_id();
+  const int N = P.N;
+  // uniform by contract; said so to the compiler, so that loop counters
+  // and lane indices stay scalar
+  n = __builtin_amdgcn_readfirstlane(n);
+  root = __builtin_amdgcn_readfirstlane(root);
+  if (n > NP) return false;
+  const int nc = 2 * n;
+  int *sraw = ws, *ssort = ws + NP, *send = ws + 2 * NP, *sflag = ws + 2 * NP + NE, *sseq = ws + 2 * NP + 2 * NE;
+  // ---- pairs in candidate-id order (= reference list order): rank by
+  // comparing against all ids (4 per LDS read), then scatter by rank
+  const int n4 = (n + 3) & ~3;
+  int raw[IT];
+#pragma unroll
+  for (int k = 0; k < IT; k++) {
+    const int t = L + 64 * k;
+    raw[k] = t < n ? seg[t] : NONE;
+    if (t < n4) sraw[t] = raw[k];
+  }
+  wave_lds_fence();
+  int rank[IT] = {};
+  for (int j = 0; j < n4; j += 4) {
+    const int4 v = *reinterpret_cast<const int4 *>(sraw + j);
+#pragma unroll
+    for (int k = 0; k < IT; k++)
+      rank[k] += (v.x < raw[k]) + (v.y < raw[k]) + (v.z < raw[k]) + (v.w < raw[k]);
+  }
+#pragma unroll
+  for (int k = 0; k < IT; k++)
+    if (L + 64 * k < n) ssort[rank[k]] = raw[k];
+  wave_lds_fence();
+  int gb[IT], ia[IT], ib[IT];
+  double en[IT];
+  bool alive[IT];
+  bool bad = false;
+#pragma unroll
+  for (int k = 0; k < IT; k++) {
+    const int t = L + 64 * k;
+    alive[k] = t < n;
+    gb[k] = NONE;
+    ia[k] = ib[k] = -1;
+    en[k] = 0.0;
+    if (alive[k]) {
+      gb[k] = ssort[t];
+      const int *mp = D.bp_mp + (size_t)gb[k] * 8;
+      ia[k] = mp[0] - 1;
+      ib[k] = mp[1] - 1;
+      en[k] = D.ene_bp[gb[k]];
+      bad = bad || ia[k] < 0 || ia[k] >= N || ib[k] < 0 || ib[k] >= N;
+    }
+  }
+  // a candidate naming a bead outside the system means corrupted MC state:
+  // report it (flags[7]) instead of indexing with it
+  if (__ballot(bad)) {
+    if (L == 0) D.flags[7] = 1;
+    return true;
+  }
+  // ---- distinct nucleotides in ascending id order, with their counts.
+  // Endpoint c = 2t (+1) is nucleotide a (b) of pair slot t.
+  const int nc4 = (nc + 3) & ~3;
+#pragma unroll
+  for (int k = 0; k < IT; k++) {
+    const int t = L + 64 * k;
+    if (t < n) {
+      send[2 * t] = ia[k];
+      send[2 * t + 1] = ib[k];
+    } else if (2 * t < nc4) {
+      send[2 * t] = NONE;
+      send[2 * t + 1] = NONE;
+    }
+  }
+  wave_lds_fence();
+  // first occurrence of each endpoint value
+  int ev[ND];
+  bool first[ND];
+#pragma unroll
+  for (int q = 0; q < ND; q++) {
+    const int c = L + 64 * q;
+    ev[q] = c < nc ? send[c] : NONE;
+    first[q] = c < nc;
+  }
+  for (int j = 0; j < nc4; j += 4) {
+    const int4 v = *reinterpret_cast<const int4 *>(send + j);
+#pragma unroll
+    for (int q = 0; q < ND; q++) {
+      const int c = L + 64 * q;
+      if ((j < c && v.x == ev[q]) || (j + 1 < c && v.y == ev[q]) || (j + 2 < c && v.z == ev[q]) ||
+          (j + 3 < c && v.w == ev[q]))
+        first[q] = false;
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < ND; q++) {
+    const int c = L + 64 * q;
+    if (c < nc4) sflag[c] = first[q] ? ev[q] : NONE;   // first occurrences only
+  }
+  wave_lds_fence();
+  // rank among distinct values, then scatter
+  int nn = 0;
+#pragma unroll
+  for (int q = 0; q < ND; q++) nn += __popcll(__ballot(first[q]));
+  int urank[ND] = {};
+  for (int j = 0; j < nc4; j += 4) {
+    const int4 v = *reinterpret_cast<const int4 *>(sflag + j);
+#pragma unroll
+    for (int q = 0; q < ND; q++)
+      urank[q] += (v.x < ev[q]) + (v.y < ev[q]) + (v.z < ev[q]) + (v.w < ev[q]);
+  }
+#pragma unroll
+  for (int q = 0; q < ND; q++)
+    if (first[q]) send[urank[q]] = ev[q];   // endpoints no longer needed
+  wave_lds_fence();
+  int nid[ND], ncnt[ND];
+#pragma unroll
+  for (int q = 0; q < ND; q++) {
+    const int u = L + 64 * q;
+    nid[q] = u < nn ? send[u] : NONE;
+    ncnt[q] = u < nn ? D.nt_cnt[(size_t)r * N + nid[q]] : 0;
+  }
+  // components with at most 128 distinct nucleotides (nearly all) run the
+  // sweep with two nucleotide chunks
+  if (ND > 2 && nn <= 128) {
+    const int nid2[2] = {nid[0], nid[1]};
+    int ncnt2[2] = {ncnt[0], ncnt[1]};
+    return mcg_sweep<IT, 2>(P, D, rk, root, n, nn, beta, seed, step, sseq, gb, ia, ib, en, alive, nid2, ncnt2);
+  }
+  return mcg_sweep<IT, ND>(P, D, rk, root, n, nn, beta, seed, step, sseq, gb, ia, ib, en, alive, nid, ncnt);
 }
 
 // ---------------------------------------------------------------------
