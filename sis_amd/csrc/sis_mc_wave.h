@@ -408,12 +408,18 @@ __device__ __forceinline__ bool mc_component_wave(const Params &P, const Ptrs &D
     nid[q] = u < nn ? send[u] : NONE;
     ncnt[q] = u < nn ? D.nt_cnt[(size_t)r * N + nid[q]] : 0;
   }
-  // components with at most 128 distinct nucleotides (nearly all) run the
-  // sweep with two nucleotide chunks
+  // the sweep runs over as few nucleotide chunks as the component needs:
+  // two for at most 128 distinct nucleotides (nearly all components), four
+  // for at most 256
   if (ND > 2 && nn <= 128) {
     const int nid2[2] = {nid[0], nid[1]};
     int ncnt2[2] = {ncnt[0], ncnt[1]};
     return mcg_sweep<IT, 2>(P, D, rk, root, n, nn, beta, seed, step, sseq, gb, ia, ib, en, alive, nid2, ncnt2);
+  }
+  if (ND > 4 && nn <= 256) {
+    const int nid4[4] = {nid[0], nid[1], nid[2], nid[3]};
+    int ncnt4[4] = {ncnt[0], ncnt[1], ncnt[2], ncnt[3]};
+    return mcg_sweep<IT, 4>(P, D, rk, root, n, nn, beta, seed, step, sseq, gb, ia, ib, en, alive, nid4, ncnt4);
   }
   return mcg_sweep<IT, ND>(P, D, rk, root, n, nn, beta, seed, step, sseq, gb, ia, ib, en, alive, nid, ncnt);
 }
