@@ -325,6 +325,7 @@ struct SysIO {
   int xlds;                     // beads whose positions fit the launch's dynamic LDS
   int fplds;                    // per-lane partial forces that fit after them
   int nblds;                    // neighbour-list entries that fit after those (0: lists read from HBM)
+  int stlds;                    // 1: the dynamics state lives in LDS for the launch (after the partial forces)
   int force_only;               // evaluate the forces of the current positions into D.frc, no update
   // energy mode: one evaluation of energy_sumup at this system's D.tK (BP
   // terms and the limit MC at that temperature), DH coefficient e_coef and
@@ -350,6 +351,8 @@ struct BatchArgs {
   int xlds;   // beads whose positions fit the dynamic LDS of the launch
   int fplds;  // per-lane partial forces that fit after them
   int nblds;  // neighbour-list entries that fit after those (0: lists read from HBM)
+  int stlds;  // 1: velocities, noise terms, displacements, forces, next positions and bonded
+              // forces in LDS for the launch, written back at its end
   int force_only;   // one force evaluation (limit MC if mc_first) into D.frc, no update
   // energy mode (one energy_sumup per replica, see SysIO): DH coefficient
   // and Debye length per replica (null: the replica's own), output
@@ -547,9 +550,12 @@ __device__ __forceinline__ void batched_rebuild(const Params &P, const Ptrs &D, 
 }
 
 // Dynamic LDS of one batched launch: positions, per-lane partial forces,
-// then (nblds > 0) per-bead list offsets and counts and the list entries.
-__host__ __device__ inline size_t batched_lds_bytes(int xlds, int fplds, int nblds, int N) {
-  return (size_t)(xlds + fplds) * 16 * 2 + (nblds > 0 ? (size_t)(2 * N + nblds) * sizeof(int) : 0);
+// (stlds) six arrays of N double4 of dynamics state, then (nblds > 0)
+// per-bead list offsets and counts and the list entries.
+constexpr int BATCH_ST_ARRAYS = 6;
+__host__ __device__ inline size_t batched_lds_bytes(int xlds, int fplds, int nblds, int N, int stlds = 0) {
+  return (size_t)(xlds + fplds + (stlds ? BATCH_ST_ARRAYS * N : 0)) * 16 * 2 +
+         (nblds > 0 ? (size_t)(2 * N + nblds) * sizeof(int) : 0);
 }
 
 // pair_nb with the replica's lists staged in LDS: bead a's WCA entries at
@@ -601,11 +607,53 @@ __device__ __forceinline__ void batched_body(const Params &P, const Ptrs &D, con
       for (int a = tid; a < N; a += B) s_x[a] = D.pos[base + a];
     __syncthreads();
   };
+  // the dynamics state of the replica in LDS for the whole launch when it
+  // fits (MD launches only): velocities, noise terms, accumulated
+  // displacements, forces, the next positions and the bonded forces.  DS is
+  // D with those arrays (and the current positions, s_x) pointing into LDS,
+  // indexed by bead alone (replica slot rs = 0, offset sb = 0; the one
+  // per-replica global array the update reads, md_cr, is shifted to this
+  // replica instead).  LDS pointers are never offset below their base: a
+  // flat pointer into LDS keeps its offset in the low 32 bits.  The global
+  // arrays are read at the start and written back before a rebuild and at
+  // the end.
+  const bool st = xl && J.stlds && !J.energy_mode && !J.force_only;
+  double4 *s_st = s_x + J.xlds + J.fplds;
+  Ptrs DS = D;
+  size_t sb = base;
+  int rs = r;
+  if (st) {
+    DS.vel = s_st;
+    DS.acc = s_st + N;
+    DS.move = s_st + 2 * N;
+    DS.frc = s_st + 3 * N;
+    DS.pos_out = s_st + 4 * N;
+    DS.fbond = s_st + 5 * N;
+    DS.pos = s_x;
+    DS.md_cr = D.md_cr + base;
+    sb = 0;
+    rs = 0;
+  }
+  auto state_in = [&]() {   // global -> LDS (positions are staged by stage_x)
+    if (st)
+      for (int a = tid; a < N; a += B) {
+        s_st[a] = D.vel[base + a];
+        s_st[N + a] = D.acc[base + a];
+        s_st[2 * N + a] = D.move[base + a];
+      }
+  };
+  auto state_out = [&]() {   // LDS -> global: positions and displacements (for a rebuild)
+    if (st)
+      for (int a = tid; a < N; a += B) {
+        D.pos[base + a] = s_x[a];
+        D.move[base + a] = s_st[2 * N + a];
+      }
+  };
   // the replica's WCA and Debye-Hueckel lists in LDS after the partial
-  // forces, when they fit (read by every step until the next rebuild):
-  // bead a's entries from s_noff[a], counts packed in s_ncnt[a]
+  // forces (and the state), when they fit (read by every step until the
+  // next rebuild): bead a's entries from s_noff[a], counts packed in s_ncnt[a]
   __shared__ int s_nbok;
-  int *s_ncnt = reinterpret_cast<int *>(s_x + J.xlds + J.fplds);
+  int *s_ncnt = reinterpret_cast<int *>(s_x + J.xlds + J.fplds + (J.stlds ? BATCH_ST_ARRAYS * N : 0));
   int *s_noff = s_ncnt + N, *s_nb = s_noff + N;
   auto stage_lists = [&]() {
     int running = 0;
@@ -668,6 +716,7 @@ __device__ __forceinline__ void batched_body(const Params &P, const Ptrs &D, con
     nreb++;
   }
   stage_x();
+  state_in();
   stage_lists();
   int64_t completed = 0;
   int overflow = s_flag;
@@ -703,7 +752,7 @@ __device__ __forceinline__ void batched_body(const Params &P, const Ptrs &D, con
         V3<double> F{0.0, 0.0, 0.0};
         if (a < N) F = bonded_part(P, D, r, a, l, lpb, xr);
         F = group_sum_rt(F, lpb);
-        if (a < N && l == 0) D.fbond[base + a] = double4{F.x, F.y, F.z, 0.0};
+        if (a < N && l == 0) DS.fbond[sb + a] = double4{F.x, F.y, F.z, 0.0};
       }
     if (tid == 0) { s_nexc = 0; s_big = 0; s_nw = 0; s_next = 0; s_huge = 0; }
     __syncthreads();
@@ -811,7 +860,7 @@ __device__ __forceinline__ void batched_body(const Params &P, const Ptrs &D, con
           F = bonded_part(P, D, r, a, lw, lpb, xr);
         }
         F = group_sum_rt(F, lpb);
-        if (a < N && lw == 0) D.fbond[base + a] = double4{F.x, F.y, F.z, 0.0};
+        if (a < N && lw == 0) DS.fbond[sb + a] = double4{F.x, F.y, F.z, 0.0};
       }
     }
     __syncthreads();
@@ -939,7 +988,7 @@ __device__ __forceinline__ void batched_body(const Params &P, const Ptrs &D, con
       F = group_sum_rt(F, lpb);
       if (a < N && l == 0) {
         const size_t g = base + a;
-        const double4 fb = D.fbond[g];
+        const double4 fb = DS.fbond[sb + a];
         F.x += fb.x; F.y += fb.y; F.z += fb.z;
         D.nt_cnt[g] = -P.max_bp_per_nt;
         comp[a] = a;
@@ -948,7 +997,7 @@ __device__ __forceinline__ void batched_body(const Params &P, const Ptrs &D, con
         if (J.force_only) {
           D.frc[g] = double4{F.x, F.y, F.z, 0.0};
         } else {
-          const double d2 = gjf_update(P, D, r, J.rk, a, F, 0, J.seed, step);
+          const double d2 = gjf_update(P, DS, rs, J.rk, a, F, 0, J.seed, step);
           if (d2 > m1) { m2 = m1; m1 = d2; }
           else if (d2 > m2) m2 = d2;
         }
@@ -972,25 +1021,43 @@ __device__ __forceinline__ void batched_body(const Params &P, const Ptrs &D, con
     // the updated positions become current (every reader of D.pos in
     // this step has passed the barriers above)
     for (int a = tid; a < N; a += B) {
-      const double4 xn = D.pos_out[base + a];
-      D.pos[base + a] = xn;
-      if (xl) s_x[a] = xn;
+      if (st) {
+        s_x[a] = s_st[4 * N + a];
+      } else {
+        const double4 xn = D.pos_out[base + a];
+        D.pos[base + a] = xn;
+        if (xl) s_x[a] = xn;
+      }
     }
     __syncthreads();
     completed = s + 1;
     if (s_exit) {
       if (s + 1 < nsteps) {
+        // the rebuild works on the global positions and displacements
+        state_out();
+        __syncthreads();
         batched_rebuild<B>(P, D, J, r, s_scan, s_tile, &s_flag);
         overflow = s_flag;
         mc_next = 1;
         nreb++;
         stage_x();
+        if (st)
+          for (int a = tid; a < N; a += B) s_st[2 * N + a] = D.move[base + a];
         stage_lists();
       } else if (tid == 0) {
         *J.rebuild = 1;   // rebuild at the start of the next launch
       }
     }
   }
+  // the state back to HBM (forces only once a step has produced them)
+  if (st)
+    for (int a = tid; a < N; a += B) {
+      D.pos[base + a] = s_x[a];
+      D.vel[base + a] = s_st[a];
+      D.acc[base + a] = s_st[N + a];
+      D.move[base + a] = s_st[2 * N + a];
+      if (completed > 0) D.frc[base + a] = s_st[3 * N + a];
+    }
   if (tid == 0 && !J.energy_mode && !J.force_only) {
     *J.done = (int)completed;
     *J.nreb = J.stepno_out ? *J.nreb + nreb : nreb;   // device-driven segments add up
@@ -1018,6 +1085,7 @@ __global__ void __launch_bounds__(B) k_batched(Params P, Ptrs D, BatchArgs A) {
   J.xlds = A.xlds;
   J.fplds = A.fplds;
   J.nblds = A.nblds;
+  J.stlds = A.stlds;
   J.force_only = A.force_only;
   J.done = A.done + r;
   J.nbp = A.nbp + r;
@@ -1101,6 +1169,7 @@ __global__ void __launch_bounds__(B) k_jobs(Params P, Ptrs D0, const JobView *__
   J.xlds = xlds;
   J.fplds = fplds;
   J.nblds = 0;
+  J.stlds = 0;
   J.force_only = 0;
   J.energy_mode = 0;
   J.e_coef = 0.0;

@@ -265,12 +265,15 @@ BatchArgs batched_args(sis_gpu_ctx *c, int lpb) {
   A.lpb = lpb;
   A.xlds = N <= BATCH_X_LDS ? N : 0;
   A.fplds = (A.xlds && N * lpb <= BATCH_FP_LDS) ? N * lpb : 0;
-  // the WCA and Debye-Hueckel lists (at most their slot capacity) when
-  // they fit beside the positions and partial forces
+  // the dynamics state for the launch, then the WCA and Debye-Hueckel
+  // lists (at most their slot capacity), when they fit beside the positions
+  // and partial forces
   A.nblds = 0;
+  A.stlds = 0;
   if (A.xlds) {
     const size_t cap = batched_lds_cap(c, batched_block(c));
-    const size_t used = batched_lds_bytes(A.xlds, A.fplds, 0, N) + (size_t)2 * N * sizeof(int);
+    if (batched_lds_bytes(A.xlds, A.fplds, 0, N, 1) <= cap) A.stlds = 1;
+    const size_t used = batched_lds_bytes(A.xlds, A.fplds, 0, N, A.stlds) + (size_t)2 * N * sizeof(int);
     const size_t want = (size_t)N * (c->D.maxw + (c->P.ele_on ? c->D.maxnb : 0));
     if (cap > used) A.nblds = (int)std::min(want, (cap - used) / sizeof(int));
   }
@@ -313,7 +316,7 @@ void batched_force(sis_gpu_ctx *c, int irep0, int bp_mc) {
   Ptrs D = batched_ptrs(c);
   void *args[] = {&P, &D, &A};
   HIPCHK(hipLaunchKernel(batched_kernel(c, B), dim3(R), dim3(B), args,
-                         batched_lds_bytes(A.xlds, A.fplds, A.nblds, c->N), c->stream));
+                         batched_lds_bytes(A.xlds, A.fplds, A.nblds, c->N, A.stlds), c->stream));
   to_packed(c);
 }
 
@@ -345,7 +348,8 @@ void batched_segment(sis_gpu_ctx *c, int64_t seg, std::vector<int> todo, std::ve
     Ptrs D = batched_ptrs(c);
     void *args[] = {&P, &D, &A};
     c->tmark();
-    HIPCHK(hipLaunchKernel(kfn, dim3(R), dim3(B), args, batched_lds_bytes(A.xlds, A.fplds, A.nblds, N), c->stream));
+    HIPCHK(hipLaunchKernel(kfn, dim3(R), dim3(B), args, batched_lds_bytes(A.xlds, A.fplds, A.nblds, N, A.stlds),
+                           c->stream));
     c->tmark();
     std::vector<int> d(R), nr(R), f(8);
     HIPCHK(hipMemcpyAsync(d.data(), c->bb_done_d.p, R * sizeof(int), hipMemcpyDeviceToHost, c->stream));
@@ -495,7 +499,7 @@ int64_t batched_device_run(sis_gpu_ctx *c, int64_t nseg) {
   Aw.write_back = 0; Aw.energy_mode = 1; Aw.energy_slot = 0; Aw.energy = c->dx_energy_d.p;
   Ptrs Db = batched_ptrs(c), Dopp = Db, Dp = c->ptrs();
   Dopp.tK = c->dx_oppT_d.p;   // the energies at the partner's temperature
-  const size_t lds = batched_lds_bytes(As.xlds, As.fplds, As.nblds, N);
+  const size_t lds = batched_lds_bytes(As.xlds, As.fplds, As.nblds, N, As.stlds);
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
