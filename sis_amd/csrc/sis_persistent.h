@@ -45,8 +45,8 @@ __device__ __forceinline__ void grid_barrier(unsigned long long *bar, unsigned l
 // returns 1 if formed
 template <typename T>
 __device__ __forceinline__ int bp_eval_one(const Params &P, const Ptrs &D, int gb, int mc,
-                                           const double4 *xr = nullptr) {
-  const int *mp = D.bp_mp + (size_t)gb * 8;
+                                           const double4 *xr = nullptr, const int *mpl = nullptr) {
+  const int *mp = mpl ? mpl : D.bp_mp + (size_t)gb * 8;   // mpl: the same record staged in LDS
   const int r = mp[7];
   const int N = P.N;
   D.bp_status[gb] = 0;
@@ -326,6 +326,7 @@ struct SysIO {
   int fplds;                    // per-lane partial forces that fit after them
   int nblds;                    // neighbour-list entries that fit after those (0: lists read from HBM)
   int stlds;                    // 1: the dynamics state lives in LDS for the launch (after the partial forces)
+  int mplds;                    // ints of LDS for the candidate records after that (0: none)
   int force_only;               // evaluate the forces of the current positions into D.frc, no update
   // energy mode: one evaluation of energy_sumup at this system's D.tK (BP
   // terms and the limit MC at that temperature), DH coefficient e_coef and
@@ -353,6 +354,7 @@ struct BatchArgs {
   int nblds;  // neighbour-list entries that fit after those (0: lists read from HBM)
   int stlds;  // 1: velocities, noise terms, displacements, forces, next positions and bonded
               // forces in LDS for the launch, written back at its end
+  int mplds;  // ints of LDS for the replica's candidate records (8 each) after that, or 0
   int force_only;   // one force evaluation (limit MC if mc_first) into D.frc, no update
   // energy mode (one energy_sumup per replica, see SysIO): DH coefficient
   // and Debye length per replica (null: the replica's own), output
@@ -550,11 +552,14 @@ __device__ __forceinline__ void batched_rebuild(const Params &P, const Ptrs &D, 
 }
 
 // Dynamic LDS of one batched launch: positions, per-lane partial forces,
-// (stlds) six arrays of N double4 of dynamics state, then (nblds > 0)
+// (stlds) six arrays of N double4 of dynamics state and three of N ints of
+// limit-MC scratch, mplds ints of candidate records, then (nblds > 0)
 // per-bead list offsets and counts and the list entries.
-constexpr int BATCH_ST_ARRAYS = 6;
-__host__ __device__ inline size_t batched_lds_bytes(int xlds, int fplds, int nblds, int N, int stlds = 0) {
+constexpr int BATCH_ST_ARRAYS = 6, BATCH_ST_INTS = 3;
+__host__ __device__ inline size_t batched_lds_bytes(int xlds, int fplds, int nblds, int N, int stlds = 0,
+                                                    int mplds = 0) {
   return (size_t)(xlds + fplds + (stlds ? BATCH_ST_ARRAYS * N : 0)) * 16 * 2 +
+         (size_t)((stlds ? BATCH_ST_INTS * N : 0) + mplds) * sizeof(int) +
          (nblds > 0 ? (size_t)(2 * N + nblds) * sizeof(int) : 0);
 }
 
@@ -592,7 +597,6 @@ __device__ __forceinline__ void batched_body(const Params &P, const Ptrs &D, con
   const size_t base = (size_t)r * N;
   const int lpb = J.lpb, grp = tid / lpb, l = tid % lpb, ngrp = B / lpb;
   const double beta = mc_beta(P, D.tK[r], true);
-  int *comp = D.comp + base;
   // the step reads positions from an LDS copy when they fit (refreshed
   // after every update and every rebuild, which may wrap them)
   extern __shared__ double4 s_x[];
@@ -619,6 +623,12 @@ __device__ __forceinline__ void batched_body(const Params &P, const Ptrs &D, con
   // the end.
   const bool st = xl && J.stlds && !J.energy_mode && !J.force_only;
   double4 *s_st = s_x + J.xlds + J.fplds;
+  // with the state, the per-step limit-MC scratch of the replica (component
+  // parents, bucket sizes, excess flags; N ints each) lives in LDS too
+  int *s_sti = reinterpret_cast<int *>(s_st + BATCH_ST_ARRAYS * N);
+  int *comp = st ? s_sti : D.comp + base;
+  int *segn = st ? s_sti + N : J.seg_n + base;
+  int *hexc = st ? s_sti + 2 * N : J.has_exc + base;
   Ptrs DS = D;
   size_t sb = base;
   int rs = r;
@@ -649,11 +659,15 @@ __device__ __forceinline__ void batched_body(const Params &P, const Ptrs &D, con
         D.move[base + a] = s_st[2 * N + a];
       }
   };
-  // the replica's WCA and Debye-Hueckel lists in LDS after the partial
-  // forces (and the state), when they fit (read by every step until the
-  // next rebuild): bead a's entries from s_noff[a], counts packed in s_ncnt[a]
+  // the replica's base-pair candidate records (bp_mp, 8 ints each) in LDS
+  // after the state when they fit (s_mpok; they change only at rebuilds)
+  __shared__ int s_mpok;
+  int *s_mp = J.stlds ? s_sti + BATCH_ST_INTS * N : reinterpret_cast<int *>(s_st);
+  // the replica's WCA and Debye-Hueckel lists in LDS after those, when they
+  // fit (read by every step until the next rebuild): bead a's entries from
+  // s_noff[a], counts packed in s_ncnt[a]
   __shared__ int s_nbok;
-  int *s_ncnt = reinterpret_cast<int *>(s_x + J.xlds + J.fplds + (J.stlds ? BATCH_ST_ARRAYS * N : 0));
+  int *s_ncnt = s_mp + J.mplds;
   int *s_noff = s_ncnt + N, *s_nb = s_noff + N;
   auto stage_lists = [&]() {
     int running = 0;
@@ -694,7 +708,16 @@ __device__ __forceinline__ void batched_body(const Params &P, const Ptrs &D, con
         if (k < ce) s_nb[s_noff[a] + cw + k] = nbe[(size_t)k * N + a];
       }
     }
-    if (tid == 0) s_nbok = ok ? 1 : 0;
+    const int nbc = *J.nbp;
+    const bool mok = J.mplds > 0 && nbc * 8 <= J.mplds;
+    if (mok) {
+      const int *src = D.bp_mp + (size_t)r * J.capb * 8;
+      for (int i = tid; i < nbc * 8; i += B) s_mp[i] = src[i];
+    }
+    if (tid == 0) {
+      s_nbok = ok ? 1 : 0;
+      s_mpok = mok ? 1 : 0;
+    }
     __syncthreads();
   };
   if (tid == 0) s_flag = 0;
@@ -702,8 +725,8 @@ __device__ __forceinline__ void batched_body(const Params &P, const Ptrs &D, con
   for (int a = tid; a < N; a += B) {
     D.nt_cnt[base + a] = -P.max_bp_per_nt;
     comp[a] = a;
-    J.seg_n[base + a] = 0;
-    J.has_exc[base + a] = 0;
+    segn[a] = 0;
+    hexc[a] = 0;
   }
   __syncthreads();
   // the first step after a rebuild does the limit MC (src/job_md.F90:388-395)
@@ -734,8 +757,9 @@ __device__ __forceinline__ void batched_body(const Params &P, const Ptrs &D, con
     // forces here unless the work loop below takes them
     for (int k = tid; k < nbr; k += B) {
       const int gb = r * J.capb + k;
-      if (bp_eval_one<T>(P, D, gb, mc, xr) && mc) {
-        const int *mp = D.bp_mp + (size_t)gb * 8;
+      const int *mpl = s_mpok ? s_mp + 8 * k : nullptr;
+      if (bp_eval_one<T>(P, D, gb, mc, xr, mpl) && mc) {
+        const int *mp = mpl ? mpl : D.bp_mp + (size_t)gb * 8;
         int a = mp[0] - 1, b = mp[1] - 1;
         while (true) {
           a = uf_find(comp, a);
@@ -764,13 +788,13 @@ __device__ __forceinline__ void batched_body(const Params &P, const Ptrs &D, con
       for (int k = tid; k < nbr; k += B) {
         const int gb = r * J.capb + k;
         if (!D.bp_status[gb]) continue;
-        const int *mp = D.bp_mp + (size_t)gb * 8;
+        const int *mp = s_mpok ? s_mp + 8 * k : D.bp_mp + (size_t)gb * 8;
         const int a = mp[0] - 1, b = mp[1] - 1;
         const int root = uf_find(comp, a);
         const size_t gr = base + root;
-        const int slot = atomicAdd(&J.seg_n[gr], 1);
+        const int slot = atomicAdd(&segn[root], 1);
         if (slot < SEGCAP) J.seg[gr * SEGCAP + slot] = gb;
-        if ((D.nt_cnt[base + a] > 0 || D.nt_cnt[base + b] > 0) && atomicExch(&J.has_exc[gr], 1) == 0) {
+        if ((D.nt_cnt[base + a] > 0 || D.nt_cnt[base + b] > 0) && atomicExch(&hexc[root], 1) == 0) {
           const int i = atomicAdd(&s_nexc, 1);
           if (i < EXC_LDS) s_exc[i] = root;
         }
@@ -785,9 +809,8 @@ __device__ __forceinline__ void batched_body(const Params &P, const Ptrs &D, con
       listed = nexc <= EXC_LDS;
       for (int it = tid; it < (listed ? nexc : N); it += B) {
         const int root = listed ? s_exc[it] : it;
-        const size_t gr = base + root;
-        if (!J.has_exc[gr]) continue;
-        const int n = J.seg_n[gr];
+        if (!hexc[root]) continue;
+        const int n = segn[root];
         if (n > SEGCAP) { s_huge = 1; continue; }
         if (n > 64 * MCW_ITEMS) { s_big = 1; continue; }
         if (n > MCL_PAIRS) {
@@ -840,8 +863,8 @@ __device__ __forceinline__ void batched_body(const Params &P, const Ptrs &D, con
           if (it < nroot) {
             const int root = listed ? s_exc[it] : it;
             const size_t gr = base + root;
-            if (J.has_exc[gr]) {
-              const int n = J.seg_n[gr];
+            if (hexc[root]) {
+              const int n = segn[root];
               if (n <= MCL_PAIRS) {
                 int *sg = J.seg + gr * SEGCAP;
                 if (!mc_component_lane(D, N, r, J.rk, root, sg, n, beta, J.seed, step))
@@ -871,8 +894,8 @@ __device__ __forceinline__ void batched_body(const Params &P, const Ptrs &D, con
         for (int it = wave; it < (listed ? nexc : N); it += nwaves) {
           const int root = listed ? s_exc[it] : it;
           const size_t gr = base + root;
-          if (!J.has_exc[gr]) continue;
-          const int n = J.seg_n[gr];
+          if (!hexc[root]) continue;
+          const int n = segn[root];
           if (n <= MCL_PAIRS || n > 64 * MCW_ITEMS) continue;
           int *sg = J.seg + gr * SEGCAP;
           if (!mc_component_wave_any(P, D, r, J.rk, root, sg, n, beta, J.seed, step, s_mcws + wave * MCW_LDS))
@@ -890,14 +913,13 @@ __device__ __forceinline__ void batched_body(const Params &P, const Ptrs &D, con
             bool big = false;
             if (it < cnt) {
               root = listed ? s_exc[it] : it;
-              const size_t gr = base + root;
-              const int n = J.seg_n[gr];
-              big = J.has_exc[gr] && n > 64 * MCW_ITEMS && n <= SEGCAP;
+              const int n = segn[root];
+              big = hexc[root] && n > 64 * MCW_ITEMS && n <= SEGCAP;
             }
             for (unsigned long long m = __ballot(big); m; m &= m - 1) {
               const int rt = bcast(root, __ffsll((long long)m) - 1);
               const size_t gr = base + rt;
-              const int n = J.seg_n[gr];
+              const int n = segn[rt];
               int *sg = J.seg + gr * SEGCAP;
               if (!mc_component_wave<MCB_ITEMS, MCB_NODES>(P, D, r, J.rk, rt, sg, n, beta, J.seed, step, s_mcws))
                 if ((tid & 63) == 0) mc_component_serial(mc_arrays(D), P.N, r, J.rk, rt, sg, n, beta, J.seed, step);
@@ -917,8 +939,7 @@ __device__ __forceinline__ void batched_body(const Params &P, const Ptrs &D, con
             bool huge = false;
             if (it < cnt) {
               root = listed ? s_exc[it] : it;
-              const size_t gr = base + root;
-              huge = J.has_exc[gr] && J.seg_n[gr] > SEGCAP;
+              huge = hexc[root] && segn[root] > SEGCAP;
             }
             for (unsigned long long m = __ballot(huge); m; m &= m - 1) {
               const int rt = bcast(root, __ffsll((long long)m) - 1);
@@ -992,8 +1013,8 @@ __device__ __forceinline__ void batched_body(const Params &P, const Ptrs &D, con
         F.x += fb.x; F.y += fb.y; F.z += fb.z;
         D.nt_cnt[g] = -P.max_bp_per_nt;
         comp[a] = a;
-        J.seg_n[g] = 0;
-        J.has_exc[g] = 0;
+        segn[a] = 0;
+        hexc[a] = 0;
         if (J.force_only) {
           D.frc[g] = double4{F.x, F.y, F.z, 0.0};
         } else {
@@ -1086,6 +1107,7 @@ __global__ void __launch_bounds__(B) k_batched(Params P, Ptrs D, BatchArgs A) {
   J.fplds = A.fplds;
   J.nblds = A.nblds;
   J.stlds = A.stlds;
+  J.mplds = A.mplds;
   J.force_only = A.force_only;
   J.done = A.done + r;
   J.nbp = A.nbp + r;
@@ -1170,6 +1192,7 @@ __global__ void __launch_bounds__(B) k_jobs(Params P, Ptrs D0, const JobView *__
   J.fplds = fplds;
   J.nblds = 0;
   J.stlds = 0;
+  J.mplds = 0;
   J.force_only = 0;
   J.energy_mode = 0;
   J.e_coef = 0.0;

@@ -270,10 +270,15 @@ BatchArgs batched_args(sis_gpu_ctx *c, int lpb) {
   // and partial forces
   A.nblds = 0;
   A.stlds = 0;
+  A.mplds = 0;
   if (A.xlds) {
     const size_t cap = batched_lds_cap(c, batched_block(c));
     if (batched_lds_bytes(A.xlds, A.fplds, 0, N, 1) <= cap) A.stlds = 1;
-    const size_t used = batched_lds_bytes(A.xlds, A.fplds, 0, N, A.stlds) + (size_t)2 * N * sizeof(int);
+    // the candidate records (up to the slot capacity)
+    const size_t mpw = (size_t)c->batch_capb * 8;
+    if (batched_lds_bytes(A.xlds, A.fplds, 0, N, A.stlds, (int)mpw) + (size_t)2 * N * sizeof(int) <= cap)
+      A.mplds = (int)mpw;
+    const size_t used = batched_lds_bytes(A.xlds, A.fplds, 0, N, A.stlds, A.mplds) + (size_t)2 * N * sizeof(int);
     const size_t want = (size_t)N * (c->D.maxw + (c->P.ele_on ? c->D.maxnb : 0));
     if (cap > used) A.nblds = (int)std::min(want, (cap - used) / sizeof(int));
   }
@@ -316,7 +321,7 @@ void batched_force(sis_gpu_ctx *c, int irep0, int bp_mc) {
   Ptrs D = batched_ptrs(c);
   void *args[] = {&P, &D, &A};
   HIPCHK(hipLaunchKernel(batched_kernel(c, B), dim3(R), dim3(B), args,
-                         batched_lds_bytes(A.xlds, A.fplds, A.nblds, c->N, A.stlds), c->stream));
+                         batched_lds_bytes(A.xlds, A.fplds, A.nblds, c->N, A.stlds, A.mplds), c->stream));
   to_packed(c);
 }
 
@@ -348,7 +353,7 @@ void batched_segment(sis_gpu_ctx *c, int64_t seg, std::vector<int> todo, std::ve
     Ptrs D = batched_ptrs(c);
     void *args[] = {&P, &D, &A};
     c->tmark();
-    HIPCHK(hipLaunchKernel(kfn, dim3(R), dim3(B), args, batched_lds_bytes(A.xlds, A.fplds, A.nblds, N, A.stlds),
+    HIPCHK(hipLaunchKernel(kfn, dim3(R), dim3(B), args, batched_lds_bytes(A.xlds, A.fplds, A.nblds, N, A.stlds, A.mplds),
                            c->stream));
     c->tmark();
     std::vector<int> d(R), nr(R), f(8);
@@ -499,7 +504,7 @@ int64_t batched_device_run(sis_gpu_ctx *c, int64_t nseg) {
   Aw.write_back = 0; Aw.energy_mode = 1; Aw.energy_slot = 0; Aw.energy = c->dx_energy_d.p;
   Ptrs Db = batched_ptrs(c), Dopp = Db, Dp = c->ptrs();
   Dopp.tK = c->dx_oppT_d.p;   // the energies at the partner's temperature
-  const size_t lds = batched_lds_bytes(As.xlds, As.fplds, As.nblds, N, As.stlds);
+  const size_t lds = batched_lds_bytes(As.xlds, As.fplds, As.nblds, N, As.stlds, As.mplds);
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
