@@ -45,7 +45,8 @@ __device__ __forceinline__ void grid_barrier(unsigned long long *bar, unsigned l
 // returns 1 if formed
 template <typename T>
 __device__ __forceinline__ int bp_eval_one(const Params &P, const Ptrs &D, int gb, int mc,
-                                           const double4 *xr = nullptr, const int *mpl = nullptr) {
+                                           const double4 *xr = nullptr, const int *mpl = nullptr,
+                                           int *ntc = nullptr) {
   const int *mp = mpl ? mpl : D.bp_mp + (size_t)gb * 8;   // mpl: the same record staged in LDS
   const int r = mp[7];
   const int N = P.N;
@@ -78,8 +79,9 @@ __device__ __forceinline__ int bp_eval_one(const Params &P, const Ptrs &D, int g
     fo[3 * q + 1] = e * (double)fb[q].y;
     fo[3 * q + 2] = e * (double)fb[q].z;
   }
-  atomicAdd(&D.nt_cnt[(size_t)r * N + mp[0] - 1], 1);
-  atomicAdd(&D.nt_cnt[(size_t)r * N + mp[1] - 1], 1);
+  int *cnt = ntc ? ntc : D.nt_cnt + (size_t)r * N;   // ntc: this replica's counts in LDS
+  atomicAdd(&cnt[mp[0] - 1], 1);
+  atomicAdd(&cnt[mp[1] - 1], 1);
   return 1;
 }
 
@@ -552,10 +554,10 @@ __device__ __forceinline__ void batched_rebuild(const Params &P, const Ptrs &D, 
 }
 
 // Dynamic LDS of one batched launch: positions, per-lane partial forces,
-// (stlds) six arrays of N double4 of dynamics state and three of N ints of
+// (stlds) six arrays of N double4 of dynamics state and four of N ints of
 // limit-MC scratch, mplds ints of candidate records, then (nblds > 0)
 // per-bead list offsets and counts and the list entries.
-constexpr int BATCH_ST_ARRAYS = 6, BATCH_ST_INTS = 3;
+constexpr int BATCH_ST_ARRAYS = 6, BATCH_ST_INTS = 4;
 __host__ __device__ inline size_t batched_lds_bytes(int xlds, int fplds, int nblds, int N, int stlds = 0,
                                                     int mplds = 0) {
   return (size_t)(xlds + fplds + (stlds ? BATCH_ST_ARRAYS * N : 0)) * 16 * 2 +
@@ -624,11 +626,21 @@ __device__ __forceinline__ void batched_body(const Params &P, const Ptrs &D, con
   const bool st = xl && J.stlds && !J.energy_mode && !J.force_only;
   double4 *s_st = s_x + J.xlds + J.fplds;
   // with the state, the per-step limit-MC scratch of the replica (component
-  // parents, bucket sizes, excess flags; N ints each) lives in LDS too
+  // parents, bucket sizes, excess flags, per-nucleotide pair counts; N ints
+  // each) lives in LDS too.  The MC routines index the counts by replica:
+  // they get DM, D with the counts pointing at the LDS row, and replica
+  // slot rm = 0.
   int *s_sti = reinterpret_cast<int *>(s_st + BATCH_ST_ARRAYS * N);
   int *comp = st ? s_sti : D.comp + base;
   int *segn = st ? s_sti + N : J.seg_n + base;
   int *hexc = st ? s_sti + 2 * N : J.has_exc + base;
+  int *ntc = st ? s_sti + 3 * N : D.nt_cnt + base;
+  Ptrs DM = D;
+  int rm = r;
+  if (st) {
+    DM.nt_cnt = ntc;
+    rm = 0;
+  }
   Ptrs DS = D;
   size_t sb = base;
   int rs = r;
@@ -723,7 +735,7 @@ __device__ __forceinline__ void batched_body(const Params &P, const Ptrs &D, con
   if (tid == 0) s_flag = 0;
   // per-step limit-MC state starts reset whatever ran before this launch
   for (int a = tid; a < N; a += B) {
-    D.nt_cnt[base + a] = -P.max_bp_per_nt;
+    ntc[a] = -P.max_bp_per_nt;
     comp[a] = a;
     segn[a] = 0;
     hexc[a] = 0;
@@ -758,7 +770,7 @@ __device__ __forceinline__ void batched_body(const Params &P, const Ptrs &D, con
     for (int k = tid; k < nbr; k += B) {
       const int gb = r * J.capb + k;
       const int *mpl = s_mpok ? s_mp + 8 * k : nullptr;
-      if (bp_eval_one<T>(P, D, gb, mc, xr, mpl) && mc) {
+      if (bp_eval_one<T>(P, D, gb, mc, xr, mpl, st ? ntc : nullptr) && mc) {
         const int *mp = mpl ? mpl : D.bp_mp + (size_t)gb * 8;
         int a = mp[0] - 1, b = mp[1] - 1;
         while (true) {
@@ -794,7 +806,7 @@ __device__ __forceinline__ void batched_body(const Params &P, const Ptrs &D, con
         const size_t gr = base + root;
         const int slot = atomicAdd(&segn[root], 1);
         if (slot < SEGCAP) J.seg[gr * SEGCAP + slot] = gb;
-        if ((D.nt_cnt[base + a] > 0 || D.nt_cnt[base + b] > 0) && atomicExch(&hexc[root], 1) == 0) {
+        if ((ntc[a] > 0 || ntc[b] > 0) && atomicExch(&hexc[root], 1) == 0) {
           const int i = atomicAdd(&s_nexc, 1);
           if (i < EXC_LDS) s_exc[i] = root;
         }
@@ -854,8 +866,8 @@ __device__ __forceinline__ void batched_body(const Params &P, const Ptrs &D, con
           const int key = s_wl[k];
           const int root = key & 0x1fffff, n = key >> 21;
           int *sg = J.seg + (base + root) * SEGCAP;
-          if (!mc_component_wave_any(P, D, r, J.rk, root, sg, n, beta, J.seed, step, s_mcws + wave * MCW_LDS))
-            if (L == 0) mc_component_serial(mc_arrays(D), P.N, r, J.rk, root, sg, n, beta, J.seed, step);
+          if (!mc_component_wave_any(P, DM, rm, J.rk, root, sg, n, beta, J.seed, step, s_mcws + wave * MCW_LDS))
+            if (L == 0) mc_component_serial(mc_arrays(DM), P.N, rm, J.rk, root, sg, n, beta, J.seed, step);
           continue;
         }
         if (k < nmc + nlane) {
@@ -867,8 +879,8 @@ __device__ __forceinline__ void batched_body(const Params &P, const Ptrs &D, con
               const int n = segn[root];
               if (n <= MCL_PAIRS) {
                 int *sg = J.seg + gr * SEGCAP;
-                if (!mc_component_lane(D, N, r, J.rk, root, sg, n, beta, J.seed, step))
-                  mc_component_serial(mc_arrays(D), P.N, r, J.rk, root, sg, n, beta, J.seed, step);
+                if (!mc_component_lane(DM, N, rm, J.rk, root, sg, n, beta, J.seed, step))
+                  mc_component_serial(mc_arrays(DM), P.N, rm, J.rk, root, sg, n, beta, J.seed, step);
               }
             }
           }
@@ -898,8 +910,8 @@ __device__ __forceinline__ void batched_body(const Params &P, const Ptrs &D, con
           const int n = segn[root];
           if (n <= MCL_PAIRS || n > 64 * MCW_ITEMS) continue;
           int *sg = J.seg + gr * SEGCAP;
-          if (!mc_component_wave_any(P, D, r, J.rk, root, sg, n, beta, J.seed, step, s_mcws + wave * MCW_LDS))
-            if ((tid & 63) == 0) mc_component_serial(mc_arrays(D), P.N, r, J.rk, root, sg, n, beta, J.seed, step);
+          if (!mc_component_wave_any(P, DM, rm, J.rk, root, sg, n, beta, J.seed, step, s_mcws + wave * MCW_LDS))
+            if ((tid & 63) == 0) mc_component_serial(mc_arrays(DM), P.N, rm, J.rk, root, sg, n, beta, J.seed, step);
         }
         __syncthreads();
       }
@@ -921,8 +933,8 @@ __device__ __forceinline__ void batched_body(const Params &P, const Ptrs &D, con
               const size_t gr = base + rt;
               const int n = segn[rt];
               int *sg = J.seg + gr * SEGCAP;
-              if (!mc_component_wave<MCB_ITEMS, MCB_NODES>(P, D, r, J.rk, rt, sg, n, beta, J.seed, step, s_mcws))
-                if ((tid & 63) == 0) mc_component_serial(mc_arrays(D), P.N, r, J.rk, rt, sg, n, beta, J.seed, step);
+              if (!mc_component_wave<MCB_ITEMS, MCB_NODES>(P, DM, rm, J.rk, rt, sg, n, beta, J.seed, step, s_mcws))
+                if ((tid & 63) == 0) mc_component_serial(mc_arrays(DM), P.N, rm, J.rk, rt, sg, n, beta, J.seed, step);
             }
           }
         }
@@ -943,7 +955,7 @@ __device__ __forceinline__ void batched_body(const Params &P, const Ptrs &D, con
             }
             for (unsigned long long m = __ballot(huge); m; m &= m - 1) {
               const int rt = bcast(root, __ffsll((long long)m) - 1);
-              if (!mc_component_huge(mc_arrays(D), N, r, J.rk, rt, comp, r * J.capb, r * J.capb + nbr, beta, J.seed,
+              if (!mc_component_huge(mc_arrays(DM), N, rm, J.rk, rt, comp, r * J.capb, r * J.capb + nbr, beta, J.seed,
                                      step, J.seg + base * SEGCAP, (size_t)N * SEGCAP, s_mcws))
                 if ((tid & 63) == 0) D.flags[5] = 1;
             }
@@ -1011,7 +1023,7 @@ __device__ __forceinline__ void batched_body(const Params &P, const Ptrs &D, con
         const size_t g = base + a;
         const double4 fb = DS.fbond[sb + a];
         F.x += fb.x; F.y += fb.y; F.z += fb.z;
-        D.nt_cnt[g] = -P.max_bp_per_nt;
+        ntc[a] = -P.max_bp_per_nt;
         comp[a] = a;
         segn[a] = 0;
         hexc[a] = 0;
