@@ -738,7 +738,10 @@ void bp_phase(sis_gpu_ctx *c, const std::vector<int> *mc, const double *ecT_dev,
     hipLaunchKernelGGL(k_seg_fill, dim3((nbp_grid + B - 1) / B), dim3(B), 0, c->stream, c->P, D,
                        nbp_arg, c->exc_cnt_d.p, c->seg_d.p, c->exc_fill_d.p, c->mc_d.p, mask_dev,
                        c->exc_list_d.p);
-    const int nlist = (int)std::min<int64_t>((int64_t)std::max<int64_t>(RN, 1), 2048);
+    // one block per listed component as far as the device holds them at
+    // once (C4 lists about 3,400 a step): a block that takes two in turn
+    // can start the longest deletion chain late
+    const int nlist = (int)std::min<int64_t>((int64_t)std::max<int64_t>(RN, 1), 8192);
     hipLaunchKernelGGL(k_mc_list, dim3(nlist), dim3(64), 0, c->stream, c->P, D, c->exc_cnt_d.p, c->seg_d.p,
                        c->exc_list_d.p, ecT_dev, c->seed, c->istep, c->huge_d.p, step_dev);
     hipLaunchKernelGGL(k_mc_huge, dim3(R), dim3(64), 0, c->stream, c->P, D, c->exc_cnt_d.p, c->seg_d.p,
