@@ -195,9 +195,8 @@ __device__ __forceinline__ bool mcg_sweep(const Params &P, const Ptrs &D, int rk
       // reference shuffle: for i = 1..n swap(i, ceil(rnd*n))
       for (int i = 0; i < ns; i++) {
         const int sw = bcast(swv, o + 1 + i);
-        const uint64_t ti = (sp >> (8 * i)) & 0xffull, ts = (sp >> (8 * sw)) & 0xffull;
-        sp = (sp & ~(0xffull << (8 * i))) | (ts << (8 * i));
-        sp = (sp & ~(0xffull << (8 * sw))) | (ti << (8 * sw));
+        const uint64_t x = ((sp >> (8 * i)) ^ (sp >> (8 * sw))) & 0xffull;   // swap entries i and sw
+        sp ^= (x << (8 * i)) | (x << (8 * sw));
       }
       // Metropolis sweep on the energies of the sequence's pairs
       del = (int)(sp & 0xffull);
