@@ -1194,7 +1194,12 @@ void phases_graph_steps(sis_gpu_ctx *c, int64_t nsteps) {
                              -1, G.mc, G.active);
         force_kernel(c, 1, G.active, G.istep_cur);
         std::swap(c->pos_d, c->pos2_d);
-        hipLaunchKernelGGL(k_skin<256>, dim3(R), dim3(256), 0, c->stream, c->P, c->ptrs(), c->nblk_force, G);
+        // a large system's thousands of block top-2 values over 1,024
+        // threads (C4: 1.3% a step); a few hundred over 256
+        if (c->nblk_force > 1024)
+          hipLaunchKernelGGL(k_skin<1024>, dim3(R), dim3(1024), 0, c->stream, c->P, c->ptrs(), c->nblk_force, G);
+        else
+          hipLaunchKernelGGL(k_skin<256>, dim3(R), dim3(256), 0, c->stream, c->P, c->ptrs(), c->nblk_force, G);
       }
     };
     // issued directly: the host's launch cost overlaps the device's work,
