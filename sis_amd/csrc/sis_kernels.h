@@ -690,11 +690,14 @@ __global__ void k_seg_fill(Params P, Ptrs D, int nbp_total, int *seg_n, int *seg
   if ((mask && !mask[r]) || !mc[r]) return;
   const int N = P.N;
   int a = mp[0] - 1, b = mp[1] - 1;
+  // the counts (final since k_bp_eval) are loaded before the dependent
+  // chain of the root walk and the bucket atomic, not after it
+  const bool exc = D.nt_cnt[(size_t)r * N + a] > 0 || D.nt_cnt[(size_t)r * N + b] > 0;
   int root = uf_find(D.comp + (size_t)r * N, a);
   size_t gr = (size_t)r * N + root;
   int slot = atomicAdd(&seg_n[gr], 1);
   if (slot < SEGCAP) seg[gr * SEGCAP + slot] = gb;
-  if (D.nt_cnt[(size_t)r * N + a] > 0 || D.nt_cnt[(size_t)r * N + b] > 0) {
+  if (exc) {
     // listed once (exc_list: [0] count, then replica * N + root)
     if (atomicExch(&has_exc[gr], 1) == 0 && exc_list) exc_list[1 + atomicAdd(&exc_list[0], 1)] = (int)gr;
   }
