@@ -1040,17 +1040,30 @@ __device__ __forceinline__ void batched_body(const Params &P, const Ptrs &D, con
       __syncthreads();
       break;
     }
-    if (l == 0) { s1[grp] = m1; s2[grp] = m2; }
-    __syncthreads();
-    for (int off = ngrp / 2; off > 0; off >>= 1) {
-      if (tid < off) {
-        const double a1 = s1[tid], a2 = s2[tid], b1 = s1[tid + off], b2 = s2[tid + off];
-        s1[tid] = a1 > b1 ? a1 : b1;
-        s2[tid] = a1 > b1 ? (a2 > b1 ? a2 : b1) : (a1 > b2 ? a1 : b2);
+    // the replica's two largest displacements: within each wave by lane
+    // shuffles (lanes other than a group's first hold zeros), then over the
+    // waves; the top two of a set do not depend on the order of the merges
+    {
+      double a1 = m1, a2 = m2;
+      for (int off = 32; off > 0; off >>= 1) {
+        const double b1 = __shfl_xor(a1, off), b2 = __shfl_xor(a2, off);
+        const double n2 = a1 > b1 ? (a2 > b1 ? a2 : b1) : (a1 > b2 ? a1 : b2);
+        a1 = a1 > b1 ? a1 : b1;
+        a2 = n2;
       }
-      __syncthreads();
+      if ((tid & 63) == 0) { s1[tid >> 6] = a1; s2[tid >> 6] = a2; }
     }
-    if (tid == 0) s_exit = (sqrt(s1[0]) + sqrt(s2[0]) > P.nl_margin) ? 1 : 0;
+    __syncthreads();
+    if (tid == 0) {
+      double a1 = s1[0], a2 = s2[0];
+      for (int w = 1; w < B / 64; w++) {
+        const double b1 = s1[w], b2 = s2[w];
+        const double n2 = a1 > b1 ? (a2 > b1 ? a2 : b1) : (a1 > b2 ? a1 : b2);
+        a1 = a1 > b1 ? a1 : b1;
+        a2 = n2;
+      }
+      s_exit = (sqrt(a1) + sqrt(a2) > P.nl_margin) ? 1 : 0;
+    }
     // the updated positions become current (every reader of D.pos in
     // this step has passed the barriers above)
     for (int a = tid; a < N; a += B) {
