@@ -222,12 +222,6 @@ void ensure_batched_io(sis_gpu_ctx *c) {
 
 // threads per replica's workgroup: 256 unless the options ask for 128 or 512
 int batched_block(sis_gpu_ctx *c) { return c->opt.block == 128 || c->opt.block == 512 ? c->opt.block : 256; }
-// lanes per bead of a batched launch: with 512 threads a 65-128-bead
-// replica takes four (every bead group busy in the per-bead phases; 1.3%
-// faster on a 128-nt chain, 2.8% slower with 256 threads)
-int batched_lpb(sis_gpu_ctx *c) {
-  return (c->N > 64 && c->N <= 128 && batched_block(c) >= 512) ? 4 : lanes_per_bead(c->N);
-}
 
 const void *batched_kernel(sis_gpu_ctx *c, int B) {
   const bool f = c->precision == 1;
@@ -321,7 +315,7 @@ void batched_force(sis_gpu_ctx *c, int irep0, int bp_mc) {
   HIPCHK(hipMemcpyAsync(c->bb_stepno_d.p, stepno.data(), R * sizeof(int64_t), hipMemcpyHostToDevice, c->stream));
   HIPCHK(hipMemcpyAsync(c->bb_mcfirst_d.p, mcf.data(), R * sizeof(int), hipMemcpyHostToDevice, c->stream));
   HIPCHK(hipMemsetAsync(c->flags_d.p, 0, 8 * sizeof(int), c->stream));
-  BatchArgs A = batched_args(c, batched_lpb(c));
+  BatchArgs A = batched_args(c, lanes_per_bead(c->N));
   A.force_only = 1;
   Params P = c->P;
   Ptrs D = batched_ptrs(c);
@@ -345,7 +339,7 @@ void batched_segment(sis_gpu_ctx *c, int64_t seg, std::vector<int> todo, std::ve
                      std::vector<int> mcf) {
   const int N = c->N, R = c->R;
   const int B = batched_block(c);
-  const int lpb = batched_lpb(c);
+  const int lpb = lanes_per_bead(N);
   const void *kfn = batched_kernel(c, B);
   std::vector<int> done(R);
   for (int r = 0; r < R; r++) done[r] = (int)(seg - todo[r]);
@@ -449,7 +443,7 @@ bool device_exchange_ok(sis_gpu_ctx *c) {
 int64_t batched_device_run(sis_gpu_ctx *c, int64_t nseg) {
   const int N = c->N, R = c->R, ex = c->opt.exchange_every;
   const int B = batched_block(c);
-  const int lpb = batched_lpb(c);
+  const int lpb = lanes_per_bead(N);
   std::vector<int> all(R, 1), mcf(R, 0);
   if (!c->lists_valid) mcf = host_rebuild(c, all);
   ensure_batched_io(c);
