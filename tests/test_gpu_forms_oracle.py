@@ -284,6 +284,14 @@ def test_autotuner_sweeps_and_restores():
     rep = e.autotune((3.0, 5.0, 8.0), nsteps=100)
     assert len(rep["trials"]) == 9 and all(t["ms_per_step"] > 0 for t in rep["trials"])
     assert rep["shape"] in (128, 256, 512) and rep["skin"] in (3.0, 5.0, 8.0)
+    # the narrowest shape whose fastest skin is within 3% of the fastest
+    # trial (times are reported rounded to 1e-5 ms)
+    best = min(t["ms_per_step"] for t in rep["trials"])
+    fastest = {sh: min(t["ms_per_step"] for t in rep["trials"] if t["shape"] == sh) for sh in (128, 256, 512)}
+    assert fastest[rep["shape"]] <= best * 1.03 + 2e-5
+    assert all(fastest[sh] > best * 1.03 - 2e-5 for sh in (128, 256, 512) if sh < rep["shape"])
+    chosen = [t for t in rep["trials"] if t["shape"] == rep["shape"] and t["skin"] == rep["skin"]]
+    assert len(chosen) == 1 and chosen[0]["ms_per_step"] <= fastest[rep["shape"]] + 2e-5
     x1, v1, a1 = e.get_state()
     assert np.array_equal(x0, x1) and np.array_equal(v0, v1) and np.array_equal(a0, a1)
     assert e.info()["istep"] == 0
