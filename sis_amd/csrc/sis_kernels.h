@@ -1491,6 +1491,16 @@ __device__ __forceinline__ void skin_decide(const Params &P, const Ptrs &D, int 
   }
 }
 
+// Workgroups are dispatched to the eight XCDs in turn (linear ID mod 8),
+// each XCD with its own L2.  xcd_tile maps a linear workgroup ID to a tile
+// so that XCD x takes the x-th contiguous slice of the n tiles: the
+// position and list gathers of one XCD's workgroups then fall mostly on
+// beads its own L2 already holds.  One-to-one for any n.
+__device__ __forceinline__ int xcd_tile(int id, int n) {
+  const int q = n >> 3, rem = n & 7, x = id & 7;
+  return x * q + min(x, rem) + (id >> 3);
+}
+
 template <typename T, int LPB, int B>
 __global__ void __launch_bounds__(B) k_force_lanes(Params P, Ptrs D, int integrate, int parity,
                                                   uint64_t seed, int64_t step0, const int *mask,
@@ -1498,7 +1508,10 @@ __global__ void __launch_bounds__(B) k_force_lanes(Params P, Ptrs D, int integra
   __shared__ double s1[B / LPB], s2[B / LPB];
   const int64_t step = step_dev ? *step_dev : step0;   // graph replays read the step from the device
   const int grp = threadIdx.x / LPB, l = threadIdx.x % LPB;
-  const int a = blockIdx.x * (B / LPB) + grp, r = blockIdx.y;
+  // tile (bx, r) of this workgroup: grid x = tiles of a replica, y = replicas
+  const int tile = xcd_tile(blockIdx.y * gridDim.x + blockIdx.x, gridDim.x * gridDim.y);
+  const int bx = tile % gridDim.x, r = tile / gridDim.x;
+  const int a = bx * (B / LPB) + grp;
   const int N = P.N;
   const bool act = a < N && !(mask && !mask[r]);
   // bonded forces here (bonded_inline) or from k_bonded's D.fbond (which
@@ -1532,8 +1545,8 @@ __global__ void __launch_bounds__(B) k_force_lanes(Params P, Ptrs D, int integra
     __syncthreads();
   }
   if (threadIdx.x == 0) {
-    D.blk_top2[((size_t)r * gridDim.x + blockIdx.x) * 2] = s1[0];
-    D.blk_top2[((size_t)r * gridDim.x + blockIdx.x) * 2 + 1] = s2[0];
+    D.blk_top2[((size_t)r * gridDim.x + bx) * 2] = s1[0];
+    D.blk_top2[((size_t)r * gridDim.x + bx) * 2 + 1] = s2[0];
   }
 }
 
