@@ -622,7 +622,6 @@ void autotune(sis_gpu_ctx *c, const double *skins, int nskin, int64_t nsteps, do
   const std::vector<int> lab0(c->rep2lab.begin() + 1, c->rep2lab.end());
   const int idx0 = c->idx_pair;
   const double m0 = c->prm.nl_margin;
-  const int block0 = c->opt.block, grid0 = c->opt.grid;
   auto restore = [&]() {
     if (R > 1) { std::vector<int32_t> l(lab0.begin(), lab0.end()); sis_gpu_set_labels(c, l.data()); }
     c->idx_pair = idx0;
@@ -687,10 +686,9 @@ void autotune(sis_gpu_ctx *c, const double *skins, int nskin, int64_t nsteps, do
         break;
       }
   if (batched) c->opt.block = best_shape; else if (pers) c->opt.grid = best_shape; else c->pg_eager = best_shape == 1;
-  (void)block0; (void)grid0;
   apply_margin(c, best_skin);
   restore();
   c->rebuilds = reb0;
   c->dev_exchanges = dex0;
-  if (report) { report[0] = batched ? best_shape : best_shape; report[1] = best_skin; }
+  if (report) { report[0] = best_shape; report[1] = best_skin; }
 }
