@@ -749,12 +749,17 @@ void bp_phase(sis_gpu_ctx *c, const std::vector<int> *mc, const double *ecT_dev,
   }
 }
 
+// launch shape of the pair/integrator kernel: FORCE_B threads per
+// workgroup, FORCE_LPB lanes per bead; force_blocks = workgroups per replica
+constexpr int FORCE_B = 256, FORCE_LPB = 8;
+inline int force_blocks(int N) { return (N + FORCE_B / FORCE_LPB - 1) / (FORCE_B / FORCE_LPB); }
+
 // mark: time the pair/integrator kernel with the context's event pairs
 void force_kernel(sis_gpu_ctx *c, int integrate, const int *mask_dev, const int64_t *step_dev = nullptr,
                   bool mark = false) {
   const int N = c->N, R = c->R;
-  constexpr int B = 256, LPB = 8;
-  dim3 g((N + B / LPB - 1) / (B / LPB), R);
+  constexpr int B = FORCE_B, LPB = FORCE_LPB;
+  dim3 g(force_blocks(N), R);
   c->nblk_force = g.x;
   if ((int64_t)c->blk_top2_d.n < (int64_t)R * g.x * 2) c->blk_top2_d.alloc((size_t)R * g.x * 2);
   // bonded forces in the pair kernel, unless external terms add to them
@@ -1167,7 +1172,8 @@ void phases_graph_steps(sis_gpu_ctx *c, int64_t nsteps) {
     HIPCHK(hipStreamSynchronize(c->stream));
     std::fill(fmc.begin(), fmc.end(), 0);
     // buffers the captured kernels use are allocated before the capture
-    const int nblk = (c->N + 31) / 32;   // force_kernel: 32 beads per workgroup
+    const int nblk = force_blocks(c->N);
+    c->nblk_force = nblk;
     if ((int64_t)c->blk_top2_d.n < (int64_t)R * nblk * 2) c->blk_top2_d.alloc((size_t)R * nblk * 2);
     if (c->P.nstep_bp_mc > 0 && c->seg_d.n < (size_t)R * c->N * SEGCAP) c->seg_d.alloc((size_t)R * c->N * SEGCAP);
     if (c->P.nstep_bp_mc > 0 && c->exc_list_d.n < (size_t)R * c->N + 1) c->exc_list_d.alloc((size_t)R * c->N + 1);
