@@ -6,7 +6,7 @@ Values are data taken from the reference's force-field files:
   * nearest-neighbour dH / dS (Zuber et al. 2022): sis.ff:125-169
   * electrostatics of the benchmark systems: ionic strength and length per
     charge of check/input_T2S2HP.toml:33-38; cutoff type 1 at 50 A, the
-    cutoff BASELINE's condensate configs state (that input file itself uses
+    cutoff of the condensate benchmark workloads (that input file itself uses
     cutoff type 2, 10 Debye lengths; read_ff below plus a [Electrostatic]
     override reproduces it)
 Units: Angstrom, kcal/mol, K, reduced time (tau = 48.888 fs).
